@@ -1,0 +1,273 @@
+"""Face-based polyhedral meshes: synthetic generators and native polyMesh I/O.
+
+The in-memory layout mirrors the reference's ``geometry`` module globals
+(``src/mesh/geometry.f90:19-83``) but is 0-based:
+
+* ``points``     (nNodes, 3) float64
+* ``nnodes``     (nFaces,)   int32   number of vertices of each face
+* ``node``       (nFaces, NOMAX) int32, 0-based vertex ids, padded with -1
+* ``owner``      (nFaces,)   int32   0-based owner cell of every face
+* ``neighbour``  (nInner,)   int32   0-based neighbour cell of every inner face
+* patches: ``names``, ``types``, ``nfaces``, ``start_face`` where
+  ``start_face`` is the 0-based id of the patch's first face, which is the
+  same number the reference stores as ``startFace`` (faces of patch ``ib``
+  are ``startFace(ib)+1 .. startFace(ib)+nfaces(ib)`` in 1-based numbering,
+  ``geometry.f90:78``/``laplacian.f90:78``).
+
+Faces are in the OpenFOAM upper-triangular order the reference's meshes use:
+inner faces first, sorted by owner then neighbour with owner < neighbour,
+then boundary faces patch by patch.
+
+Synthetic meshes: ``hex_mesh`` builds the unit cube split into N^3 hexahedra,
+cell id ``i + N*j + N*N*k``, node id ``i + (N+1)*j + (N+1)^2*k``, patches in
+the order x-, x+, y-, y+, z-, z+ with N^2 faces each.  Interior nodes can be
+jittered by a seeded uniform offset of +-``jitter*h`` per coordinate;
+boundary nodes stay on the cube faces.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .rng import uniform01
+
+NOMAX = 6  # max vertices per face, geometry.f90:40
+
+# Boundary-type codes used across the C-ABI (text names as in polyMesh/boundary,
+# geometry.f90:234-261 and updateBoundary.f90:31-44).
+BCTYPE_CODES = {
+    "wall": 1,
+    "symmetry": 2,
+    "inlet": 3,
+    "outlet": 4,
+    "pressure": 5,
+    "empty": 6,
+    "periodic": 7,
+    "wallQFlux": 8,
+}
+
+
+@dataclass
+class Mesh:
+    points: np.ndarray
+    nnodes: np.ndarray
+    node: np.ndarray
+    owner: np.ndarray
+    neighbour: np.ndarray
+    n_cells: int
+    names: list = field(default_factory=list)
+    types: list = field(default_factory=list)
+    nfaces: np.ndarray = None
+    start_face: np.ndarray = None
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.points.shape[0])
+
+    @property
+    def n_faces(self) -> int:
+        return int(self.owner.shape[0])
+
+    @property
+    def n_inner(self) -> int:
+        return int(self.neighbour.shape[0])
+
+    @property
+    def n_bnd(self) -> int:
+        return self.n_faces - self.n_inner
+
+    @property
+    def n_total(self) -> int:
+        return self.n_cells + self.n_bnd
+
+    @property
+    def n_patches(self) -> int:
+        return len(self.names)
+
+    @property
+    def bctype_codes(self) -> np.ndarray:
+        return np.array([BCTYPE_CODES[t] for t in self.types], dtype=np.int32)
+
+    def patch_of_bface(self) -> np.ndarray:
+        """Patch index of every boundary face (length nBnd)."""
+        out = np.empty(self.n_bnd, dtype=np.int32)
+        for ib in range(self.n_patches):
+            s = int(self.start_face[ib]) - self.n_inner
+            out[s:s + int(self.nfaces[ib])] = ib
+        return out
+
+    def check(self) -> None:
+        """Structural sanity checks mirroring geometry.f90:276-279."""
+        assert self.node.shape == (self.n_faces, NOMAX)
+        assert self.nnodes.min() >= 3 and self.nnodes.max() <= NOMAX
+        assert int(np.sum(self.nfaces)) == self.n_bnd
+        assert self.owner.min() >= 0 and self.owner.max() < self.n_cells
+        if self.n_inner:
+            assert self.neighbour.min() >= 0 and self.neighbour.max() < self.n_cells
+            assert np.all(self.neighbour != self.owner[: self.n_inner])
+
+
+def hex_mesh(N: int, jitter: float = 0.0, seed: int = 2,
+             types=("wall",) * 6) -> Mesh:
+    """Unit cube, N^3 hexahedra, optional interior-node jitter (fraction of h)."""
+    N = int(N)
+    M = N + 1
+    h = 1.0 / N
+    ii, jj, kk = np.meshgrid(np.arange(M), np.arange(M), np.arange(M), indexing="ij")
+    # node id = i + M*j + M*M*k  -> flatten in (k, j, i) order
+    ii = ii.transpose(2, 1, 0).reshape(-1)
+    jj = jj.transpose(2, 1, 0).reshape(-1)
+    kk = kk.transpose(2, 1, 0).reshape(-1)
+    # Lattice coordinates i/N (correctly rounded; planes 0 and 1 exact).
+    pts = np.stack([ii / N, jj / N, kk / N], axis=1).astype(np.float64)
+    if jitter:
+        nid = np.arange(M ** 3, dtype=np.uint64)
+        interior = (ii > 0) & (ii < N) & (jj > 0) & (jj < N) & (kk > 0) & (kk < N)
+        for c in range(3):
+            u = uniform01(seed, 100 + c, nid)
+            off = (2.0 * u - 1.0) * (jitter * h)
+            pts[:, c] = np.where(interior, pts[:, c] + off, pts[:, c])
+
+    def nid3(i, j, k):
+        return (i + M * j + M * M * k).astype(np.int32)
+
+    ci, cj, ck = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    ci = ci.transpose(2, 1, 0).reshape(-1)
+    cj = cj.transpose(2, 1, 0).reshape(-1)
+    ck = ck.transpose(2, 1, 0).reshape(-1)
+    cell = (ci + N * cj + N * N * ck).astype(np.int64)
+    ncell = N ** 3
+
+    # Inner faces in upper-triangular order: for every owner cell, its
+    # higher-numbered neighbours ascending: +x (c+1), +y (c+N), +z (c+N^2).
+    fx_own = cell[ci < N - 1]
+    fy_own = cell[cj < N - 1]
+    fz_own = cell[ck < N - 1]
+    # per-cell face slots: 3 possible, build sort key owner*3 + dir
+    own = np.concatenate([fx_own, fy_own, fz_own])
+    dirn = np.concatenate([np.zeros(fx_own.size, np.int64), np.ones(fy_own.size, np.int64),
+                           np.full(fz_own.size, 2, np.int64)])
+    order = np.argsort(own * 3 + dirn, kind="stable")
+    own = own[order]
+    dirn = dirn[order]
+    oi = own % N
+    oj = (own // N) % N
+    ok = own // (N * N)
+    nb = own + np.where(dirn == 0, 1, np.where(dirn == 1, N, N * N))
+
+    def quad_x(i, j, k):  # face at x=i, normal +x
+        return np.stack([nid3(i, j, k), nid3(i, j + 1, k), nid3(i, j + 1, k + 1), nid3(i, j, k + 1)], 1)
+
+    def quad_y(i, j, k):  # face at y=j, normal +y
+        return np.stack([nid3(i, j, k), nid3(i, j, k + 1), nid3(i + 1, j, k + 1), nid3(i + 1, j, k)], 1)
+
+    def quad_z(i, j, k):  # face at z=k, normal +z
+        return np.stack([nid3(i, j, k), nid3(i + 1, j, k), nid3(i + 1, j + 1, k), nid3(i, j + 1, k)], 1)
+
+    inner_nodes = np.empty((own.size, 4), np.int32)
+    mx = dirn == 0
+    my = dirn == 1
+    mz = dirn == 2
+    inner_nodes[mx] = quad_x(oi[mx] + 1, oj[mx], ok[mx])
+    inner_nodes[my] = quad_y(oi[my], oj[my] + 1, ok[my])
+    inner_nodes[mz] = quad_z(oi[mz], oj[mz], ok[mz] + 1)
+
+    # Boundary patches x-, x+, y-, y+, z-, z+ ; faces ordered by owner cell id.
+    bnodes = []
+    bown = []
+    names = ["xmin", "xmax", "ymin", "ymax", "zmin", "zmax"]
+    for p in range(6):
+        axis = p // 2
+        hi = p % 2
+        sel = [ci, cj, ck][axis] == (N - 1 if hi else 0)
+        c = cell[sel]
+        i, j, k = ci[sel], cj[sel], ck[sel]
+        if axis == 0:
+            q = quad_x(i + hi, j, k)
+        elif axis == 1:
+            q = quad_y(i, j + hi, k)
+        else:
+            q = quad_z(i, j, k + hi)
+        if not hi:  # outward normal is -axis: reverse vertex order
+            q = q[:, ::-1]
+        bnodes.append(q)
+        bown.append(c)
+    nb_faces = np.full(6, N * N, dtype=np.int32)
+    n_inner = own.size
+    start = n_inner + np.concatenate([[0], np.cumsum(nb_faces)[:-1]]).astype(np.int32)
+
+    allq = np.concatenate([inner_nodes] + bnodes, axis=0)
+    nf = allq.shape[0]
+    node = np.full((nf, NOMAX), -1, dtype=np.int32)
+    node[:, :4] = allq
+    nnodes = np.full(nf, 4, dtype=np.int32)
+    owner = np.concatenate([own] + bown).astype(np.int32)
+    m = Mesh(points=pts, nnodes=nnodes, node=node, owner=owner,
+             neighbour=nb.astype(np.int32), n_cells=ncell, names=names,
+             types=list(types), nfaces=nb_faces, start_face=start.astype(np.int32))
+    m.check()
+    return m
+
+
+def cell_centres_lattice(N: int) -> np.ndarray:
+    """Centres of the undistorted lattice cells (for analytic test fields)."""
+    c = (np.arange(N) + 0.5) / N
+    x = np.tile(c, N * N)
+    y = np.tile(np.repeat(c, N), N)
+    z = np.repeat(c, N * N)
+    return np.stack([x, y, z], 1)
+
+
+# --------------------------------------------------------------------------
+# Native polyMesh format (geometry.f90:118-406): 1-based text files
+# points / faces / owner / neighbour / boundary / size.
+# --------------------------------------------------------------------------
+
+def write_native(mesh: Mesh, case_dir: str) -> None:
+    d = os.path.join(case_dir, "polyMesh")
+    os.makedirs(d, exist_ok=True)
+    np.savetxt(os.path.join(d, "points"), mesh.points, fmt="%.17g")
+    with open(os.path.join(d, "faces"), "w") as f:
+        for k in range(mesh.n_faces):
+            nn = int(mesh.nnodes[k])
+            f.write(str(nn) + " " + " ".join(str(int(v) + 1) for v in mesh.node[k, :nn]) + "\n")
+    np.savetxt(os.path.join(d, "owner"), mesh.owner + 1, fmt="%d")
+    np.savetxt(os.path.join(d, "neighbour"), mesh.neighbour + 1, fmt="%d")
+    with open(os.path.join(d, "boundary"), "w") as f:
+        f.write("# bcname bctype nFaces startFace\n")
+        for ib in range(mesh.n_patches):
+            f.write(f"{mesh.names[ib]} {mesh.types[ib]} {int(mesh.nfaces[ib])} {int(mesh.start_face[ib])}\n")
+    with open(os.path.join(d, "size"), "w") as f:
+        for v in (mesh.n_nodes, mesh.n_cells, mesh.n_inner, mesh.n_bnd, mesh.n_faces):
+            f.write(f"{v}\n")
+
+
+def read_native(case_dir: str) -> Mesh:
+    d = os.path.join(case_dir, "polyMesh")
+    with open(os.path.join(d, "size")) as f:
+        n_nodes, n_cells, n_inner, n_bnd, n_faces = [int(f.readline()) for _ in range(5)]
+    pts = np.loadtxt(os.path.join(d, "points"), dtype=np.float64).reshape(n_nodes, 3)
+    node = np.full((n_faces, NOMAX), -1, dtype=np.int32)
+    nnodes = np.zeros(n_faces, dtype=np.int32)
+    with open(os.path.join(d, "faces")) as f:
+        for k in range(n_faces):
+            v = [int(t) for t in f.readline().split()]
+            nnodes[k] = v[0]
+            node[k, : v[0]] = np.array(v[1: 1 + v[0]]) - 1
+    owner = np.loadtxt(os.path.join(d, "owner"), dtype=np.int64).reshape(-1).astype(np.int32) - 1
+    nb = np.loadtxt(os.path.join(d, "neighbour"), dtype=np.int64).reshape(-1).astype(np.int32) - 1
+    names, types, nf, sf = [], [], [], []
+    with open(os.path.join(d, "boundary")) as f:
+        for line in f:
+            s = line.strip()
+            if not s or s.startswith("#"):
+                continue
+            t = s.split()
+            names.append(t[0]); types.append(t[1]); nf.append(int(t[2])); sf.append(int(t[3]))
+    m = Mesh(points=pts, nnodes=nnodes, node=node, owner=owner, neighbour=nb,
+             n_cells=n_cells, names=names, types=types,
+             nfaces=np.array(nf, np.int32), start_face=np.array(sf, np.int32))
+    m.check()
+    return m

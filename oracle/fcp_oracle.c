@@ -1,0 +1,923 @@
+/*
+ * fcp_oracle.c -- CPU restatement of the reference finite-volume path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see fcp_oracle.h).  Built with
+ * -O2 -ffp-contract=off and no fast-math so that every floating-point
+ * operation happens in the order the reference Fortran performs it; the
+ * reference (flang, x86-64 baseline ISA) has no FMA contraction either.
+ *
+ * Literal values: the reference writes several constants as default-REAL
+ * literals that are widened to REAL(8) (parameters.f90:6 `small = 1e-20`,
+ * linear_solvers.f90:276,447 `s0 = 1.e20`).  They are reproduced with the
+ * float->double widening below.
+ */
+#include "fcp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define SMALL ((double)1e-20f)  /* parameters.f90:6, gradients.f90:18 */
+#define S0INIT ((double)1e20f)  /* linear_solvers.f90:276,447          */
+static const double THIRD = 1.0 / 3.0; /* geometry.f90:151 third = 1./3._dp */
+
+double orc_small(void) { return SMALL; }
+double orc_s0_init(void) { return S0INIT; }
+
+/* ------------------------------------------------------------------------ */
+/* Geometry: geometry.f90:416-754                                           */
+/* ------------------------------------------------------------------------ */
+int orc_geometry(const orc_mesh *m, orc_geom *g) {
+  const int n = m->n_cells, F = m->n_faces, Fi = m->n_inner;
+  const double *P = m->points;
+  double *r8tmp = (double *)calloc((size_t)n, sizeof(double));
+  if (!r8tmp) return -1;
+  /* F7: the reference relies on zero-filled fresh allocations. */
+  memset(g->xc, 0, sizeof(double) * n);
+  memset(g->yc, 0, sizeof(double) * n);
+  memset(g->zc, 0, sizeof(double) * n);
+  memset(g->vol, 0, sizeof(double) * n);
+  memset(g->arx, 0, sizeof(double) * F);
+  memset(g->ary, 0, sizeof(double) * F);
+  memset(g->arz, 0, sizeof(double) * F);
+  memset(g->xf, 0, sizeof(double) * F);
+  memset(g->yf, 0, sizeof(double) * F);
+  memset(g->zf, 0, sizeof(double) * F);
+
+  for (int f = 0; f < F; ++f) {
+    const int *nd = m->node + (size_t)f * ORC_NOMAX;
+    const int inp = m->owner[f];
+    double areasum = 0.0;
+    const double x1 = P[3 * nd[0]], y1 = P[3 * nd[0] + 1], z1 = P[3 * nd[0] + 2];
+    for (int i = 0; i < m->nnodes[f] - 2; ++i) {
+      const double *pb = P + 3 * nd[i + 1];
+      const double *pc = P + 3 * nd[i + 2];
+      const double px = pb[0] - x1, py = pb[1] - y1, pz = pb[2] - z1;
+      const double qx = pc[0] - x1, qy = pc[1] - y1, qz = pc[2] - z1;
+      /* triangle_area_vector geometry.f90:1721-1749 */
+      const double nx = 0.5 * (py * qz - pz * qy);
+      const double ny = 0.5 * (pz * qx - px * qz);
+      const double nz = 0.5 * (px * qy - py * qx);
+      g->arx[f] += nx;
+      g->ary[f] += ny;
+      g->arz[f] += nz;
+      const double cx = THIRD * ((pc[0] + pb[0]) + x1);
+      const double cy = THIRD * ((pc[1] + pb[1]) + y1);
+      const double cz = THIRD * ((pc[2] + pb[2]) + z1);
+      const double are = sqrt((nx * nx + ny * ny) + nz * nz);
+      g->xf[f] += are * cx;
+      g->yf[f] += are * cy;
+      g->zf[f] += are * cz;
+      areasum += are;
+      double riSi = (cx * nx + cy * ny) + cz * nz;
+      g->vol[inp] += THIRD * riSi;
+      g->xc[inp] += 0.75 * riSi * cx;
+      g->yc[inp] += 0.75 * riSi * cy;
+      g->zc[inp] += 0.75 * riSi * cz;
+      r8tmp[inp] += riSi;
+      if (f < Fi) {
+        const int inn = m->neighbour[f];
+        riSi = -((cx * nx + cy * ny) + cz * nz);
+        g->vol[inn] += THIRD * riSi;
+        g->xc[inn] += 0.75 * riSi * cx;
+        g->yc[inn] += 0.75 * riSi * cy;
+        g->zc[inn] += 0.75 * riSi * cz;
+        r8tmp[inn] += riSi;
+      }
+    }
+    g->xf[f] = g->xf[f] / areasum;
+    g->yf[f] = g->yf[f] / areasum;
+    g->zf[f] = g->zf[f] / areasum;
+  }
+  for (int c = 0; c < n; ++c) {
+    g->xc[c] = g->xc[c] / r8tmp[c];
+    g->yc[c] = g->yc[c] / r8tmp[c];
+    g->zc[c] = g->zc[c] / r8tmp[c];
+  }
+  free(r8tmp);
+
+  /* Interpolation factor, variant 2 (geometry.f90:46, 581-606). */
+  for (int f = 0; f < Fi; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    double dx = g->xf[f] - g->xc[p], dy = g->yf[f] - g->yc[p], dz = g->zf[f] - g->zc[p];
+    const double djp = sqrt((dx * dx + dy * dy) + dz * dz);
+    dx = g->xf[f] - g->xc[q];
+    dy = g->yf[f] - g->yc[q];
+    dz = g->zf[f] - g->zc[q];
+    const double djn = sqrt((dx * dx + dy * dy) + dz * dz);
+    g->facint[f] = djp / (djp + djn);
+  }
+  /* Df = Sf.Sf/(Sf.dpn) (geometry.f90:648-664). */
+  for (int f = 0; f < Fi; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    const double xpn = g->xc[q] - g->xc[p], ypn = g->yc[q] - g->yc[p], zpn = g->zc[q] - g->zc[p];
+    const double are = (g->arx[f] * g->arx[f] + g->ary[f] * g->ary[f]) + g->arz[f] * g->arz[f];
+    g->Df[f] = are / ((g->arx[f] * xpn + g->ary[f] * ypn) + g->arz[f] * zpn);
+  }
+  /* Boundary normal distance and |S|/dn (geometry.f90:701-754), for every
+   * boundary face (the reference keeps them for wall/symmetry only). */
+  for (int f = Fi; f < F; ++f) {
+    const int p = m->owner[f];
+    const double are = sqrt((g->arx[f] * g->arx[f] + g->ary[f] * g->ary[f]) + g->arz[f] * g->arz[f]);
+    const double nxf = g->arx[f] / are, nyf = g->ary[f] / are, nzf = g->arz[f] / are;
+    const double dn = ((g->xf[f] - g->xc[p]) * nxf + (g->yf[f] - g->yc[p]) * nyf) +
+                      (g->zf[f] - g->zc[p]) * nzf;
+    g->dnb[f - Fi] = dn;
+    g->srdb[f - Fi] = are / dn;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CSR: sparse_matrix.f90:86-296, utils.f90:96-189                          */
+/* ------------------------------------------------------------------------ */
+static int has_periodic(const orc_mesh *m) {
+  for (int ib = 0; ib < m->n_patches; ++ib)
+    if (m->bctype[ib] == 7) return 1;
+  return 0;
+}
+
+int orc_csr_nnz(const orc_mesh *m) {
+  if (has_periodic(m)) return -2;
+  return 2 * m->n_inner + m->n_cells; /* sparse_matrix.f90:110 */
+}
+
+int orc_csr_build(const orc_mesh *m, int *ia, int *ja, int *diag, int *slot_pn,
+                  int *slot_np) {
+  const int n = m->n_cells, Fi = m->n_inner;
+  if (has_periodic(m)) return -2;
+  /* The reference sorts the COO list (diagonal + (P,N) + (N,P)) lexically
+   * (utils.f90:443) -- the sorted multiset is unique, so per-row counting
+   * followed by an in-row sort yields the same arrays. */
+  int *cnt = (int *)calloc((size_t)n + 1, sizeof(int));
+  if (!cnt) return -1;
+  for (int i = 0; i < n; ++i) cnt[i] = 1;
+  for (int f = 0; f < Fi; ++f) {
+    cnt[m->owner[f]]++;
+    cnt[m->neighbour[f]]++;
+  }
+  ia[0] = 0;
+  for (int i = 0; i < n; ++i) ia[i + 1] = ia[i] + cnt[i];
+  for (int i = 0; i < n; ++i) {
+    ja[ia[i]] = i;
+    cnt[i] = ia[i] + 1;
+  }
+  for (int f = 0; f < Fi; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    ja[cnt[p]++] = q;
+    ja[cnt[q]++] = p;
+  }
+  for (int i = 0; i < n; ++i) { /* insertion sort each row */
+    for (int k = ia[i] + 1; k < ia[i + 1]; ++k) {
+      const int v = ja[k];
+      int l = k - 1;
+      while (l >= ia[i] && ja[l] > v) {
+        ja[l + 1] = ja[l];
+        --l;
+      }
+      ja[l + 1] = v;
+    }
+  }
+  /* find_main_diag_element_positions utils.f90:149-168: the id-th entry with
+   * row==col in sorted order is the diagonal of row id. */
+  for (int i = 0; i < n; ++i) {
+    for (int k = ia[i]; k < ia[i + 1]; ++k)
+      if (ja[k] == i) {
+        diag[i] = k;
+        break;
+      }
+  }
+  /* csr_to_k utils.f90:96-146: first match in the row. */
+  for (int f = 0; f < Fi; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    int kpn = -1, knp = -1;
+    for (int k = ia[p]; k < ia[p + 1]; ++k)
+      if (ja[k] == q) { kpn = k; break; }
+    for (int k = ia[q]; k < ia[q + 1]; ++k)
+      if (ja[k] == p) { knp = k; break; }
+    slot_pn[f] = kpn;
+    slot_np[f] = knp;
+  }
+  free(cnt);
+  return ia[n];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Laplacian: laplacian.f90:1-151                                           */
+/* ------------------------------------------------------------------------ */
+void orc_laplacian(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                   const int *slot_pn, const int *slot_np, const double *mu,
+                   const double *phi, double *a, double *su) {
+  const int n = m->n_cells, Fi = m->n_inner;
+  memset(a, 0, sizeof(double) * c->nnz);
+  for (int f = 0; f < Fi; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    const double fxn = g->facint[f], fxp = 1.0 - g->facint[f];
+    const double xpn = g->xc[q] - g->xc[p], ypn = g->yc[q] - g->yc[p], zpn = g->zc[q] - g->zc[p];
+    const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
+    const double smdpn = ((ax * ax + ay * ay) + az * az) / ((ax * xpn + ay * ypn) + az * zpn);
+    const double cap = (fxp * mu[p] + fxn * mu[q]) * smdpn;
+    const double can = cap;
+    a[slot_pn[f]] = can;
+    a[slot_np[f]] = cap;
+    a[c->diag[p]] = a[c->diag[p]] - can;
+    a[c->diag[q]] = a[c->diag[q]] - cap;
+  }
+  for (int ib = 0; ib < m->n_patches; ++ib) {
+    for (int i = 0; i < m->nfaces[ib]; ++i) {
+      const int f = m->start_face[ib] + i;
+      const int p = m->owner[f];
+      const int ijb = n + (m->start_face[ib] - Fi) + i;
+      const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
+      const double are = sqrt((ax * ax + ay * ay) + az * az);
+      const double nxf = ax / are, nyf = ay / are, nzf = az / are;
+      const double dfn = ((g->xf[f] - g->xc[p]) * nxf + (g->yf[f] - g->yc[p]) * nyf) +
+                         (g->zf[f] - g->zc[p]) * nzf;
+      const double dcoef = mu[p] * are / dfn;
+      a[c->diag[p]] = a[c->diag[p]] - dcoef;
+      su[p] = su[p] - dcoef * phi[ijb];
+    }
+  }
+}
+
+void orc_laplacian_face(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                        const int *slot_pn, const int *slot_np, const double *mu_f,
+                        const unsigned char *bc_kind, const double *phi, double *a,
+                        double *su) {
+  const int n = m->n_cells, Fi = m->n_inner;
+  memset(a, 0, sizeof(double) * c->nnz);
+  for (int f = 0; f < Fi; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    const double cap = mu_f[f] * g->Df[f];
+    a[slot_pn[f]] = cap;
+    a[slot_np[f]] = cap;
+    a[c->diag[p]] = a[c->diag[p]] - cap;
+    a[c->diag[q]] = a[c->diag[q]] - cap;
+  }
+  for (int f = Fi; f < m->n_faces; ++f) {
+    if (bc_kind[f - Fi] != 0) continue;
+    const int p = m->owner[f];
+    const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
+    const double are = sqrt((ax * ax + ay * ay) + az * az);
+    const double nxf = ax / are, nyf = ay / are, nzf = az / are;
+    const double dfn = ((g->xf[f] - g->xc[p]) * nxf + (g->yf[f] - g->yc[p]) * nyf) +
+                       (g->zf[f] - g->zc[p]) * nzf;
+    const double dcoef = mu_f[f] * are / dfn;
+    a[c->diag[p]] = a[c->diag[p]] - dcoef;
+    su[p] = su[p] - dcoef * phi[n + (f - Fi)];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gauss gradient: gradients.f90:1607-1715                                  */
+/* ------------------------------------------------------------------------ */
+void orc_grad_gauss(const orc_mesh *m, const orc_geom *g, const double *u,
+                    double *dudxi) {
+  const int n = m->n_cells, Fi = m->n_inner, F = m->n_faces;
+  memset(dudxi, 0, sizeof(double) * 3 * (size_t)(n + F - Fi));
+  for (int f = 0; f < Fi; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    const double fie = u[p] + (u[q] - u[p]) * g->facint[f];
+    const double dfxe = fie * g->arx[f], dfye = fie * g->ary[f], dfze = fie * g->arz[f];
+    dudxi[3 * p] += dfxe;
+    dudxi[3 * p + 1] += dfye;
+    dudxi[3 * p + 2] += dfze;
+    dudxi[3 * q] -= dfxe;
+    dudxi[3 * q + 1] -= dfye;
+    dudxi[3 * q + 2] -= dfze;
+  }
+  for (int f = Fi; f < F; ++f) {
+    const int p = m->owner[f];
+    const double fi = u[n + (f - Fi)];
+    dudxi[3 * p] = dudxi[3 * p] + fi * g->arx[f];
+    dudxi[3 * p + 1] = dudxi[3 * p + 1] + fi * g->ary[f];
+    dudxi[3 * p + 2] = dudxi[3 * p + 2] + fi * g->arz[f];
+  }
+  for (int c = 0; c < n; ++c) {
+    const double volr = 1.0 / g->vol[c];
+    dudxi[3 * c] *= volr;
+    dudxi[3 * c + 1] *= volr;
+    dudxi[3 * c + 2] *= volr;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Least-squares gradients                                                  */
+/* ------------------------------------------------------------------------ */
+static void lsq_invert_sym(double *D) {
+  /* gradients.f90:745-777 (and :1296-1326): closed-form adjugate with +small */
+  const double d11 = D[0], d12 = D[1], d13 = D[2], d22 = D[3], d23 = D[4], d33 = D[5];
+  const double d21 = d12, d31 = d13, d32 = d23;
+  const double tmp = 1.0 / ((((((d11 * d22 * d33 - d11 * d23 * d32) - d12 * d21 * d33) +
+                                d12 * d23 * d31) + d13 * d21 * d32) - d13 * d22 * d31) + SMALL);
+  D[0] = (d22 * d33 - d23 * d32) * tmp;
+  D[1] = (d21 * d33 - d23 * d31) * tmp;
+  D[2] = (d21 * d32 - d22 * d31) * tmp;
+  D[3] = (d11 * d33 - d13 * d31) * tmp;
+  D[4] = (d12 * d33 - d13 * d32) * tmp;
+  D[5] = (d11 * d32 - d12 * d31) * tmp;
+  D[6] = (d12 * d23 - d13 * d22) * tmp;
+  D[7] = (d11 * d23 - d13 * d21) * tmp;
+  D[8] = (d11 * d22 - d12 * d21) * tmp;
+}
+
+static void lsq_accum(double *D, double w, double dx, double dy, double dz, int weighted) {
+  /* Dmat(1..6) += [w*]dx*dx, dx*dy, dx*dz, dy*dy, dy*dz, dz*dz in the
+   * reference's storage slots 1,2,3,4,5,6 = xx,xy,xz,yy,yz,zz. */
+  if (weighted) {
+    D[0] += w * dx * dx;
+    D[3] += w * dy * dy;
+    D[5] += w * dz * dz;
+    D[1] += w * dx * dy;
+    D[2] += w * dx * dz;
+    D[4] += w * dy * dz;
+  } else {
+    D[0] += dx * dx;
+    D[3] += dy * dy;
+    D[5] += dz * dz;
+    D[1] += dx * dy;
+    D[2] += dx * dz;
+    D[4] += dy * dz;
+  }
+}
+
+/* Thin QR of an l x 3 matrix by modified Gram-Schmidt (matrix.f90:366-419),
+ * returning R^{-1} Q^T (3 x l, column-major per neighbour) in out[3*6]. */
+static int mgs_rinv_qt(const double *A /* l x 3, row-major */, int l, double *out) {
+  double q[6][3], r[3][3] = {{0}};
+  for (int i = 0; i < l; ++i)
+    for (int j = 0; j < 3; ++j) q[i][j] = A[3 * i + j];
+  for (int j = 0; j < 3; ++j) {
+    double z = 0.0;
+    for (int i = 0; i < l; ++i) z = z + q[i][j] * q[i][j];
+    r[j][j] = sqrt(z);
+    for (int i = 0; i < l; ++i) q[i][j] = q[i][j] / r[j][j];
+    for (int k = j + 1; k < 3; ++k) {
+      z = 0.0;
+      for (int i = 0; i < l; ++i) z = z + q[i][j] * q[i][k];
+      r[j][k] = z;
+      for (int i = 0; i < l; ++i) q[i][k] = q[i][k] - r[j][k] * q[i][j];
+    }
+  }
+  /* R^{-1} for upper-triangular R */
+  double ri[3][3] = {{0}};
+  ri[0][0] = 1.0 / r[0][0];
+  ri[1][1] = 1.0 / r[1][1];
+  ri[2][2] = 1.0 / r[2][2];
+  ri[0][1] = -r[0][1] / (r[0][0] * r[1][1]);
+  ri[1][2] = -r[1][2] / (r[1][1] * r[2][2]);
+  ri[0][2] = (r[0][1] * r[1][2] - r[0][2] * r[1][1]) / (r[0][0] * r[1][1] * r[2][2]);
+  memset(out, 0, sizeof(double) * 18);
+  for (int k = 0; k < l; ++k)
+    for (int i = 0; i < 3; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < 3; ++j) s += ri[i][j] * q[k][j];
+      out[3 * k + i] = s;
+    }
+  return 0;
+}
+
+int orc_lsq_matrix(const orc_mesh *m, const orc_geom *g, int variant, int compat,
+                   double *dmat) {
+  const int n = m->n_cells, Fi = m->n_inner, F = m->n_faces;
+  (void)compat;
+  if (variant == 0 || variant == 1) {
+    const int weighted = (variant == 1);
+    memset(dmat, 0, sizeof(double) * 9 * (size_t)n);
+    for (int f = 0; f < Fi; ++f) {
+      const int p = m->owner[f], q = m->neighbour[f];
+      const double dx = g->xc[q] - g->xc[p], dy = g->yc[q] - g->yc[p], dz = g->zc[q] - g->zc[p];
+      double w = 0.0;
+      if (weighted) w = 1.0 / ((dx * dx + dy * dy) + dz * dz);
+      /* reference interleaves owner/neighbour per component; each Dmat
+       * entry still receives its face contributions in face order. */
+      lsq_accum(dmat + 9 * (size_t)p, w, dx, dy, dz, weighted);
+      lsq_accum(dmat + 9 * (size_t)q, w, dx, dy, dz, weighted);
+    }
+    for (int f = Fi; f < F; ++f) {
+      const int p = m->owner[f];
+      const double dx = g->xf[f] - g->xc[p], dy = g->yf[f] - g->yc[p], dz = g->zf[f] - g->zc[p];
+      double w = 0.0;
+      if (weighted) w = 1.0 / ((dx * dx + dy * dy) + dz * dz);
+      lsq_accum(dmat + 9 * (size_t)p, w, dx, dy, dz, weighted);
+    }
+    for (int c = 0; c < n; ++c) lsq_invert_sym(dmat + 9 * (size_t)c);
+    return 0;
+  }
+  if (variant == 2) {
+    /* neighbour slots in face-visit order (gradients.f90:966-990) */
+    double *A = (double *)calloc((size_t)n * 18, sizeof(double));
+    int *cnt = (int *)calloc((size_t)n, sizeof(int));
+    if (!A || !cnt) return -1;
+    for (int f = 0; f < Fi; ++f) {
+      const int p = m->owner[f], q = m->neighbour[f];
+      if (cnt[p] >= 6 || cnt[q] >= 6) { free(A); free(cnt); return -3; }
+      double *ap = A + 18 * (size_t)p + 3 * cnt[p]++;
+      ap[0] = g->xc[q] - g->xc[p]; ap[1] = g->yc[q] - g->yc[p]; ap[2] = g->zc[q] - g->zc[p];
+      double *aq = A + 18 * (size_t)q + 3 * cnt[q]++;
+      aq[0] = g->xc[p] - g->xc[q]; aq[1] = g->yc[p] - g->yc[q]; aq[2] = g->zc[p] - g->zc[q];
+    }
+    for (int f = Fi; f < F; ++f) {
+      const int p = m->owner[f];
+      if (cnt[p] >= 6) { free(A); free(cnt); return -3; }
+      double *ap = A + 18 * (size_t)p + 3 * cnt[p]++;
+      ap[0] = g->xf[f] - g->xc[p]; ap[1] = g->yf[f] - g->yc[p]; ap[2] = g->zf[f] - g->zc[p];
+    }
+    for (int c = 0; c < n; ++c) mgs_rinv_qt(A + 18 * (size_t)c, cnt[c], dmat + 18 * (size_t)c);
+    free(A);
+    free(cnt);
+    return 0;
+  }
+  return -4;
+}
+
+int orc_grad_lsq(const orc_mesh *m, const orc_geom *g, int variant, int compat,
+                 const double *dmat, const double *phi, double *dphidxi) {
+  const int n = m->n_cells, Fi = m->n_inner, F = m->n_faces;
+  const size_t ntot = (size_t)n + (size_t)(F - Fi);
+  memset(dphidxi, 0, sizeof(double) * 3 * ntot);
+  if (variant == 0 || variant == 1) {
+    for (int f = 0; f < Fi; ++f) {
+      const int p = m->owner[f], q = m->neighbour[f];
+      double Dx, Dy, Dz;
+      if (variant == 0) {
+        const double delta = phi[q] - phi[p];
+        Dx = (g->xc[q] - g->xc[p]) * delta;
+        Dy = (g->yc[q] - g->yc[p]) * delta;
+        Dz = (g->zc[q] - g->zc[p]) * delta;
+      } else {
+        const double dx = g->xc[q] - g->xc[p], dy = g->yc[q] - g->yc[p], dz = g->zc[q] - g->zc[p];
+        const double w = (phi[q] - phi[p]) / ((dx * dx + dy * dy) + dz * dz);
+        Dx = w * dx;
+        Dy = w * dy;
+        Dz = w * dz;
+      }
+      dphidxi[3 * p] += Dx; dphidxi[3 * p + 1] += Dy; dphidxi[3 * p + 2] += Dz;
+      dphidxi[3 * q] += Dx; dphidxi[3 * q + 1] += Dy; dphidxi[3 * q + 2] += Dz;
+    }
+    for (int f = Fi; f < F; ++f) {
+      const int p = m->owner[f];
+      const int ib = f - Fi;
+      const double delta = phi[n + ib] - phi[p];
+      double Dx, Dy, Dz;
+      if (variant == 0) {
+        Dx = (g->xf[f] - g->xc[p]) * delta;
+        Dy = (g->yf[f] - g->yc[p]) * delta;
+        Dz = (g->zf[f] - g->zc[p]) * delta;
+      } else {
+        /* gradients.f90:1459 weights with xf(i) (i = boundary counter) */
+        const int fw = compat ? ib : f;
+        const double wx = g->xf[fw] - g->xc[p], wy = g->yf[fw] - g->yc[p], wz = g->zf[fw] - g->zc[p];
+        const double w = delta / ((wx * wx + wy * wy) + wz * wz);
+        Dx = w * (g->xf[f] - g->xc[p]);
+        Dy = w * (g->yf[f] - g->yc[p]);
+        Dz = w * (g->zf[f] - g->zc[p]);
+      }
+      dphidxi[3 * p] += Dx; dphidxi[3 * p + 1] += Dy; dphidxi[3 * p + 2] += Dz;
+    }
+    for (int c = 0; c < n; ++c) {
+      const double *D = dmat + 9 * (size_t)c;
+      const double b1 = dphidxi[3 * c], b2 = dphidxi[3 * c + 1], b3 = dphidxi[3 * c + 2];
+      dphidxi[3 * c] = (b1 * D[0] - b2 * D[1]) + b3 * D[2];
+      if (compat)
+        dphidxi[3 * c + 1] = (b1 * D[3] - b2 * D[4]) - b3 * D[5];
+      else
+        dphidxi[3 * c + 1] = (-(b1 * D[4]) + b2 * D[3]) - b3 * D[5];
+      dphidxi[3 * c + 2] = (b1 * D[6] - b2 * D[7]) + b3 * D[8];
+    }
+    return 0;
+  }
+  if (variant == 2) {
+    double *b = (double *)calloc((size_t)n * 6, sizeof(double));
+    int *cnt = (int *)calloc((size_t)n, sizeof(int));
+    if (!b || !cnt) return -1;
+    for (int f = 0; f < Fi; ++f) {
+      const int p = m->owner[f], q = m->neighbour[f];
+      b[6 * (size_t)p + cnt[p]++] = phi[q] - phi[p];
+      b[6 * (size_t)q + cnt[q]++] = phi[p] - phi[q];
+    }
+    for (int f = Fi; f < F; ++f) {
+      const int p = m->owner[f];
+      b[6 * (size_t)p + cnt[p]++] = phi[n + (f - Fi)] - phi[p];
+    }
+    for (int c = 0; c < n; ++c) {
+      const double *D = dmat + 18 * (size_t)c;
+      for (int i = 0; i < 3; ++i) {
+        double s = 0.0; /* sum(D(i,1:l)*b(1:l)) */
+        for (int k = 0; k < cnt[c]; ++k) s = s + D[3 * k + i] * b[6 * (size_t)c + k];
+        dphidxi[3 * c + i] = s;
+      }
+    }
+    free(b);
+    free(cnt);
+    return 0;
+  }
+  return -4;
+}
+
+void orc_update_boundary(const orc_mesh *m, double *phi) {
+  const int n = m->n_cells, Fi = m->n_inner;
+  for (int ib = 0; ib < m->n_patches; ++ib) {
+    const int t = m->bctype[ib];
+    if (t == 4 || t == 2 || t == 5 || t == 6) {
+      for (int i = 0; i < m->nfaces[ib]; ++i) {
+        const int f = m->start_face[ib] + i;
+        phi[n + (f - Fi)] = phi[m->owner[f]];
+      }
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Linear algebra helpers                                                   */
+/* ------------------------------------------------------------------------ */
+void orc_spmv(const orc_csr *c, const double *a, const double *x, double *y) {
+  for (int i = 0; i < c->n; ++i) {
+    double s = 0.0;
+    for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) s = s + a[k] * x[c->ja[k]];
+    y[i] = s;
+  }
+}
+
+static void residual(const orc_csr *c, const double *a, const double *fi,
+                     const double *rhs, double *res) {
+  for (int i = 0; i < c->n; ++i) {
+    double r = rhs[i];
+    for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) r = r - a[k] * fi[c->ja[k]];
+    res[i] = r;
+  }
+}
+
+static double l1(const double *v, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s = s + fabs(v[i]);
+  return s;
+}
+
+static double dot(const double *u, const double *v, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s = s + u[i] * v[i];
+  return s;
+}
+
+static double factor_of(const orc_csr *c, const double *a, const double *fi) {
+  double s = 0.0;
+  for (int i = 0; i < c->n; ++i) s = s + fabs(a[c->diag[i]] * fi[i]);
+  return s + SMALL;
+}
+
+void orc_dic_diag(const orc_csr *c, const double *a, double *d) {
+  for (int i = 0; i < c->n; ++i) {
+    double di = a[c->diag[i]];
+    for (int k = c->ia[i]; k < c->diag[i]; ++k) di = di - a[k] * a[k] * d[c->ja[k]];
+    d[i] = 1.0 / di;
+  }
+}
+
+void orc_dilu_diag(const orc_csr *c, const double *a, double *d) {
+  const int n = c->n;
+  for (int i = 0; i < n; ++i) {
+    double di = a[c->diag[i]];
+    for (int k = c->ia[i]; k < c->diag[i]; ++k) {
+      const int j = c->ja[k];
+      int l;
+      for (l = c->diag[j]; l < c->ia[j + 1]; ++l)
+        if (c->ja[l] == i) break;
+      /* a miss leaves l one past the row (linear_solvers.f90:616-620) */
+      di = di - a[k] * d[j] * (l < c->nnz ? a[l] : 0.0);
+    }
+    d[i] = 1.0 / di;
+  }
+}
+
+void orc_dic_apply(const orc_csr *c, const double *a, const double *d,
+                   const double *r, double *z) {
+  const int n = c->n;
+  for (int i = 0; i < n; ++i) {
+    double zi = r[i];
+    for (int k = c->ia[i]; k < c->diag[i]; ++k) zi = zi - a[k] * z[c->ja[k]];
+    z[i] = zi * d[i];
+  }
+  for (int i = 0; i < n; ++i) z[i] = z[i] / (d[i] + SMALL);
+  for (int i = n - 1; i >= 0; --i) {
+    double zi = z[i];
+    for (int k = c->diag[i] + 1; k < c->ia[i + 1]; ++k) zi = zi - a[k] * z[c->ja[k]];
+    z[i] = zi * d[i];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solvers: linear_solvers.f90                                              */
+/* ------------------------------------------------------------------------ */
+int orc_iccg(const orc_csr *c, const double *a, double *fi, const double *rhs,
+             int itr_max, double tol_abs, double tol_rel, double *hist,
+             orc_solve_info *info) {
+  const int n = c->n;
+  double *res = (double *)calloc((size_t)n, sizeof(double));
+  double *pk = (double *)calloc((size_t)n, sizeof(double));
+  double *zk = (double *)calloc((size_t)n, sizeof(double));
+  double *d = (double *)calloc((size_t)n, sizeof(double));
+  double factor = 0.0, resor = 0.0, resl = 0.0;
+  int itr_used = 0;
+  residual(c, a, fi, rhs, res);                 /* :417-422 */
+  const double res0 = l1(res, n);               /* :425     */
+  if (res0 < tol_abs) {                         /* :428-432 */
+    resor = res0;
+    resl = res0;
+    goto done;
+  }
+  orc_dic_diag(c, a, d);                        /* :439-445 */
+  double s0 = S0INIT;                           /* :447     */
+  for (int l = 1; l <= itr_max; ++l) {
+    orc_dic_apply(c, a, d, res, zk);            /* :458-475 */
+    const double sk = dot(res, zk, n);          /* :478     */
+    const double bet = sk / s0;
+    for (int i = 0; i < n; ++i) pk[i] = zk[i] + bet * pk[i];
+    orc_spmv(c, a, pk, zk);                     /* :493-498 */
+    const double pkapk = dot(pk, zk, n);
+    const double alf = sk / pkapk;
+    for (int i = 0; i < n; ++i) fi[i] = fi[i] + alf * pk[i];
+    for (int i = 0; i < n; ++i) res[i] = res[i] - alf * zk[i];
+    resl = l1(res, n);
+    s0 = sk;
+    itr_used++;
+    if (hist) hist[l - 1] = resl;
+    if (l == 1) {
+      factor = factor_of(c, a, fi);
+      resor = res0 / factor;
+    }
+    const double rsm = resl / (res0 + SMALL);
+    if (rsm < tol_rel || resl < tol_abs) break;
+  }
+done:
+  if (info) {
+    info->resor = resor; info->res0 = res0; info->resl = resl;
+    info->factor = factor; info->iters = itr_used;
+  }
+  free(res); free(pk); free(zk); free(d);
+  return itr_used;
+}
+
+int orc_dpcg(const orc_csr *c, const double *a, double *fi, const double *rhs,
+             int itr_max, double tol_abs, double tol_rel, double *hist,
+             orc_solve_info *info) {
+  const int n = c->n;
+  double *res = (double *)calloc((size_t)n, sizeof(double));
+  double *pk = (double *)calloc((size_t)n, sizeof(double));
+  double *zk = (double *)calloc((size_t)n, sizeof(double));
+  double factor = 0.0, resor = 0.0, resl = 0.0;
+  int itr_used = 0;
+  residual(c, a, fi, rhs, res);
+  const double res0 = l1(res, n);
+  if (res0 < tol_abs) {
+    resor = res0;
+    resl = res0;
+    goto done;
+  }
+  double s0 = S0INIT;
+  for (int l = 1; l <= itr_max; ++l) {
+    for (int i = 0; i < n; ++i) zk[i] = res[i] / a[c->diag[i]];
+    const double sk = dot(res, zk, n);
+    const double bet = sk / s0;
+    for (int i = 0; i < n; ++i) pk[i] = zk[i] + bet * pk[i];
+    orc_spmv(c, a, pk, zk);
+    const double pkapk = dot(pk, zk, n);
+    const double alf = sk / pkapk;
+    for (int i = 0; i < n; ++i) fi[i] = fi[i] + alf * pk[i];
+    for (int i = 0; i < n; ++i) res[i] = res[i] - alf * zk[i];
+    resl = l1(res, n);
+    s0 = sk;
+    itr_used++;
+    if (hist) hist[l - 1] = resl;
+    if (l == 1) {
+      factor = factor_of(c, a, fi);
+      resor = res0 / factor;
+    }
+    const double rsm = resl / (res0 + SMALL);
+    if (rsm < tol_rel || resl < tol_abs) break;
+  }
+done:
+  if (info) {
+    info->resor = resor; info->res0 = res0; info->resl = resl;
+    info->factor = factor; info->iters = itr_used;
+  }
+  free(res); free(pk); free(zk);
+  return itr_used;
+}
+
+int orc_bicgstab(const orc_csr *c, const double *a, double *fi, const double *rhs,
+                 int itr_max, double tol_abs, double tol_rel, double *hist,
+                 orc_solve_info *info) {
+  const int n = c->n;
+  double *res = (double *)calloc((size_t)n, sizeof(double));
+  double *reso = (double *)calloc((size_t)n, sizeof(double));
+  double *pk = (double *)calloc((size_t)n, sizeof(double));
+  double *uk = (double *)calloc((size_t)n, sizeof(double));
+  double *zk = (double *)calloc((size_t)n, sizeof(double));
+  double *vk = (double *)calloc((size_t)n, sizeof(double));
+  double *d = (double *)calloc((size_t)n, sizeof(double));
+  double factor = 0.0, resor = 0.0, resl = 0.0;
+  int itr_used = 0;
+  residual(c, a, fi, rhs, res);                 /* :571-576 */
+  const double res0 = l1(res, n);
+  if (res0 < tol_abs) {
+    resor = res0;
+    resl = res0;
+    goto done;
+  }
+  orc_dilu_diag(c, a, d);                       /* :613-624 */
+  memcpy(reso, res, sizeof(double) * n);
+  double alf = 1.0, beto = 1.0, gam = 1.0;
+  for (int l = 1; l <= itr_max; ++l) {
+    const double bet = dot(res, reso, n);       /* :651     */
+    const double om = bet * gam / (alf * beto + SMALL);
+    beto = bet;
+    for (int i = 0; i < n; ++i) pk[i] = res[i] + om * (pk[i] - alf * uk[i]);
+    orc_dic_apply(c, a, d, pk, zk);             /* :664-683 */
+    orc_spmv(c, a, zk, uk);                     /* :689-694 */
+    const double ukreso = dot(uk, reso, n);
+    gam = bet / ukreso;
+    for (int i = 0; i < n; ++i) fi[i] = fi[i] + gam * zk[i];
+    for (int i = 0; i < n; ++i) res[i] = res[i] - gam * uk[i];
+    orc_dic_apply(c, a, d, res, zk);            /* :714-732 */
+    orc_spmv(c, a, zk, vk);                     /* :737-742 */
+    alf = dot(vk, res, n) / (dot(vk, vk, n) + SMALL);
+    for (int i = 0; i < n; ++i) fi[i] = fi[i] + alf * zk[i];
+    for (int i = 0; i < n; ++i) res[i] = res[i] - alf * vk[i];
+    resl = l1(res, n);
+    itr_used++;
+    if (hist) hist[l - 1] = resl;
+    if (l == 1) {
+      factor = factor_of(c, a, fi);
+      resor = res0 / factor;
+    }
+    const double rsm = resl / (res0 + SMALL);
+    if (rsm < tol_rel || resl < tol_abs) break;
+  }
+done:
+  if (info) {
+    info->resor = resor; info->res0 = res0; info->resl = resl;
+    info->factor = factor; info->iters = itr_used;
+  }
+  free(res); free(reso); free(pk); free(uk); free(zk); free(vk); free(d);
+  return itr_used;
+}
+
+int orc_gauss_seidel(const orc_csr *c, const double *a, double *fi,
+                     const double *rhs, int itr_max, double tol_abs,
+                     double tol_rel, double *hist, orc_solve_info *info) {
+  const int n = c->n;
+  double *res = (double *)calloc((size_t)n, sizeof(double));
+  double factor = 0.0, resor = 0.0, resl = 0.0, res0 = 0.0;
+  int itr_used = 0;
+  for (int l = 1; l <= itr_max; ++l) {
+    for (int i = 0; i < n; ++i) {              /* :147-153 */
+      double r = rhs[i];
+      for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) r = r - a[k] * fi[c->ja[k]];
+      res[i] = r;
+      fi[i] = fi[i] + res[i] / (a[c->diag[i]] + SMALL);
+    }
+    if (l == 1) {
+      res0 = l1(res, n);
+      if (res0 < tol_abs) {
+        resor = res0;
+        resl = res0;
+        itr_used = 1;
+        break;
+      }
+    }
+    resl = l1(res, n);
+    itr_used++;
+    if (hist) hist[l - 1] = resl;
+    if (l == 1) {
+      factor = factor_of(c, a, fi);
+      resor = res0 / factor;
+    }
+    const double rsm = resl / (res0 + SMALL);
+    if (rsm < tol_rel || resl < tol_abs) break;
+  }
+  if (info) {
+    info->resor = resor; info->res0 = res0; info->resl = resl;
+    info->factor = factor; info->iters = itr_used;
+  }
+  free(res);
+  return itr_used;
+}
+
+int orc_levels(const orc_csr *c, int upper, int *lev) {
+  const int n = c->n;
+  int nlev = 0;
+  if (!upper) {
+    for (int i = 0; i < n; ++i) {
+      int l = 0;
+      for (int k = c->ia[i]; k < c->diag[i]; ++k)
+        if (lev[c->ja[k]] + 1 > l) l = lev[c->ja[k]] + 1;
+      lev[i] = l;
+      if (l + 1 > nlev) nlev = l + 1;
+    }
+  } else {
+    for (int i = n - 1; i >= 0; --i) {
+      int l = 0;
+      for (int k = c->diag[i] + 1; k < c->ia[i + 1]; ++k)
+        if (lev[c->ja[k]] + 1 > l) l = lev[c->ja[k]] + 1;
+      lev[i] = l;
+      if (l + 1 > nlev) nlev = l + 1;
+    }
+  }
+  return nlev;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Convection-diffusion assembly (temperature.f90 + scalar_fluxes.f90)      */
+/* ------------------------------------------------------------------------ */
+static double face_value(int scheme, int ijp, int ijn, double xf, double yf, double zf,
+                         double lambda, const double *u, const double *gr,
+                         const orc_geom *g) {
+  if (scheme == 0) /* face_value_cds interpolation.f90:116-157 */
+    return u[ijp] + (u[ijn] - u[ijp]) * lambda;
+  /* face_value_2nd_upwind interpolation.f90:318-356 */
+  const double gradfidr = (gr[3 * ijp] * (xf - g->xc[ijp]) + gr[3 * ijp + 1] * (yf - g->yc[ijp])) +
+                          gr[3 * ijp + 2] * (zf - g->zc[ijp]);
+  return u[ijp] + gradfidr;
+}
+
+void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                  const int *slot_pn, const int *slot_np, const double *flmass,
+                  const double *dcoef, double *phi, const double *grad, double gam,
+                  int scheme, const unsigned char *bc_kind, double urf, double *a,
+                  double *su, double *sp) {
+  const int n = m->n_cells, Fi = m->n_inner, F = m->n_faces;
+  memset(a, 0, sizeof(double) * c->nnz);
+  memset(su, 0, sizeof(double) * n);
+  memset(sp, 0, sizeof(double) * n);
+  for (int f = 0; f < Fi; ++f) { /* temperature.f90:118-157 */
+    const int p = m->owner[f], q = m->neighbour[f];
+    const double fxn = g->facint[f], fxp = 1.0 - g->facint[f];
+    const double xpn = g->xc[q] - g->xc[p], ypn = g->yc[q] - g->yc[p], zpn = g->zc[q] - g->zc[p];
+    const double fm = flmass[f], dc = dcoef[f];
+    const double de = dc * g->Df[f];
+    const double ce = fm < 0.0 ? fm : 0.0;
+    const double cp = fm > 0.0 ? fm : 0.0;
+    const double can = -de + ce;
+    const double cap = -de - cp;
+    double dfixi = grad[3 * p] * fxp + grad[3 * q] * fxn;
+    double dfiyi = grad[3 * p + 1] * fxp + grad[3 * q + 1] * fxn;
+    double dfizi = grad[3 * p + 2] * fxp + grad[3 * q + 2] * fxn;
+    dfixi = dfixi * (g->arx[f] - g->Df[f] * xpn);
+    dfiyi = dfiyi * (g->ary[f] - g->Df[f] * ypn);
+    dfizi = dfizi * (g->arz[f] - g->Df[f] * zpn);
+    const double fdfie = dc * ((dfixi + dfiyi) + dfizi);
+    double fii;
+    if (fm >= 0.0)
+      fii = face_value(scheme, p, q, g->xf[f], g->yf[f], g->zf[f], fxp, phi, grad, g);
+    else
+      fii = face_value(scheme, q, p, g->xf[f], g->yf[f], g->zf[f], fxn, phi, grad, g);
+    double fcfie = fm * fii;
+    const double fcfii = ce * phi[q] + cp * phi[p];
+    fcfie = gam * (fcfie - fcfii);
+    const double suadd = -fcfie + fdfie;
+    a[slot_pn[f]] = can;
+    a[slot_np[f]] = cap;
+    a[c->diag[p]] = a[c->diag[p]] - can;
+    a[c->diag[q]] = a[c->diag[q]] - cap;
+    su[p] = su[p] + suadd;
+    su[q] = su[q] - suadd;
+  }
+  for (int f = Fi; f < F; ++f) { /* boundary loop temperature.f90:168-320 */
+    const int p = m->owner[f];
+    const int ijb = n + (f - Fi);
+    const int kind = bc_kind[f - Fi];
+    if (kind == 2) { /* facefluxsc_boundary scalar_fluxes.f90:256-337 */
+      const double xpn = g->xf[f] - g->xc[p], ypn = g->yf[f] - g->yc[p], zpn = g->zf[f] - g->zc[p];
+      const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
+      const double Dfi = ((ax * ax + ay * ay) + az * az) / ((xpn * ax + ypn * ay) + zpn * az);
+      const double de = dcoef[f] * Dfi;
+      const double fm = flmass[f];
+      const double ce = fm < 0.0 ? fm : 0.0;
+      const double cp = fm > 0.0 ? fm : 0.0;
+      const double can = -de + ce;
+      (void)cp;
+      double dfixi = grad[3 * p] * (ax - Dfi * xpn);
+      double dfiyi = grad[3 * p + 1] * (ay - Dfi * ypn);
+      double dfizi = grad[3 * p + 2] * (az - Dfi * zpn);
+      const double suadd = dcoef[f] * ((dfixi + dfiyi) + dfizi);
+      sp[p] = sp[p] - can;
+      su[p] = su[p] - can * phi[ijb] + suadd;
+    } else if (kind == 0) { /* Dirichlet wall :206-210 */
+      const double dw = dcoef[f] * g->srdb[f - Fi];
+      sp[p] = sp[p] + dw;
+      su[p] = su[p] + dw * phi[ijb];
+    } else { /* zero-gradient wall :224 */
+      phi[ijb] = phi[p];
+    }
+  }
+  const double urfr = 1.0 / urf, urfm = 1.0 - urf; /* :350-366 */
+  for (int i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) s = s + a[k];
+    const double off = s - a[c->diag[i]];
+    a[c->diag[i]] = sp[i] - off;
+    a[c->diag[i]] = a[c->diag[i]] * urfr;
+    su[i] = su[i] + urfm * a[c->diag[i]] * phi[i];
+  }
+}

@@ -1,0 +1,158 @@
+/*
+ * fcp_oracle.h -- CPU restatement of freeCappuccino's finite-volume
+ * assemble-and-solve path.  TEST INFRASTRUCTURE ONLY: this library is the
+ * checker for the HIP path (tests, __graft_entry__.smoke and the CPU
+ * baseline leg of bench.py).  Nothing in the product links or calls it.
+ *
+ * Conventions: every index is 0-based.  Field arrays have length
+ * numTotal = nCells + nBoundaryFaces with boundary-face values at
+ * nCells.. (geometry.f90:282-289).  Gradients are AoS [3*numTotal]
+ * (dPhidxi(3,numTotal), gradients.f90:1620).  Each function follows the
+ * reference's order of floating-point operations; file:line citations are
+ * relative to the reference repository root.
+ */
+#ifndef FCP_ORACLE_H
+#define FCP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NOMAX 6
+
+typedef struct {
+  int n_nodes, n_cells, n_inner, n_faces, n_patches;
+  const double *points;     /* [n_nodes*3] xyz AoS                        */
+  const int *nnodes;        /* [n_faces]                                  */
+  const int *node;          /* [n_faces*ORC_NOMAX] vertex ids             */
+  const int *owner;         /* [n_faces]                                  */
+  const int *neighbour;     /* [n_inner]                                  */
+  const int *nfaces;        /* [n_patches]                                */
+  const int *start_face;    /* [n_patches] id of the patch's first face    */
+  const int *bctype;        /* [n_patches] codes: 1 wall 2 symmetry 3 inlet
+                               4 outlet 5 pressure 6 empty 7 periodic      */
+} orc_mesh;
+
+typedef struct {
+  double *xc, *yc, *zc, *vol;               /* [n_cells]  */
+  double *arx, *ary, *arz, *xf, *yf, *zf;   /* [n_faces]  */
+  double *facint, *Df;                      /* [n_inner]  */
+  double *dnb, *srdb;                       /* [n_bnd]: normal distance face
+                                               centre -> owner centre and
+                                               |S|/dn for every boundary face */
+} orc_geom;
+
+/* geometry.f90:416-754 (native reader, interpolation variant 2). */
+int orc_geometry(const orc_mesh *m, orc_geom *g);
+
+/* sparse_matrix.f90:86-296: CSR pattern, diagonal and face->slot maps.
+ * ia[n+1] (ia[0]=0), ja[nnz], diag[n], slot_pn[F_i], slot_np[F_i].
+ * Returns nnz (>0) or <0 on error (periodic patches are not supported). */
+int orc_csr_nnz(const orc_mesh *m);
+int orc_csr_build(const orc_mesh *m, int *ia, int *ja, int *diag,
+                  int *slot_pn, int *slot_np);
+
+typedef struct {
+  int n, nnz;
+  const int *ia, *ja, *diag;
+} orc_csr;
+
+/* laplacian.f90:1-151 (cell mu, all boundary faces Dirichlet). a is zeroed,
+ * su is updated in place (not zeroed), exactly as the reference. */
+void orc_laplacian(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                   const int *slot_pn, const int *slot_np,
+                   const double *mu, const double *phi, double *a, double *su);
+
+/* Face-diffusivity extension of laplacian.f90: mu_f[n_faces] per face and
+ * bc_kind[n_bnd] (0 = Dirichlet, 1 = zero-gradient: no contribution).
+ * With mu_f = (1-lambda)*mu_P + lambda*mu_N on inner faces, mu_f = mu_P on
+ * boundary faces and every boundary Dirichlet it reproduces orc_laplacian
+ * bit for bit. */
+void orc_laplacian_face(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                        const int *slot_pn, const int *slot_np,
+                        const double *mu_f, const unsigned char *bc_kind,
+                        const double *phi, double *a, double *su);
+
+/* gradients.f90:1607-1715 grad_gauss (dudxi[3*numTotal], boundary rows 0). */
+void orc_grad_gauss(const orc_mesh *m, const orc_geom *g, const double *u,
+                    double *dudxi);
+
+/* LSQ gradients: variant 0 = lsq (gradients.f90:660-893), 1 = lsq_dm
+ * (:1157-1486), 2 = lsq_qr with modified Gram-Schmidt (:900-1152).
+ * compat=1 reproduces the reference arithmetic including its defects
+ * (y-row cofactors at :887/:1480, xf(i) for xf(iface) at :1459);
+ * compat=0 computes the corrected least-squares gradient.
+ * dmat: [9*n] for variants 0/1, [18*n] (3x6 per cell) for variant 2. */
+int orc_lsq_matrix(const orc_mesh *m, const orc_geom *g, int variant, int compat,
+                   double *dmat);
+int orc_grad_lsq(const orc_mesh *m, const orc_geom *g, int variant, int compat,
+                 const double *dmat, const double *phi, double *dphidxi);
+
+/* updateBoundary.f90:1-74 zero-gradient copy for outlet/symmetry/pressure/empty
+ * patches (periodic not supported). */
+void orc_update_boundary(const orc_mesh *m, double *phi);
+
+/* Linear solvers on CSR (linear_solvers.f90).  fi[n] in/out, rhs[n].
+ * hist (may be NULL) receives resl after each iteration (capacity itr_max).
+ * Return value: iterations used.  *resor: reference report value
+ * res0/factor;  *res0_out: initial L1 residual; *resl_out: final L1. */
+typedef struct {
+  double resor, res0, resl, factor;
+  int iters;
+} orc_solve_info;
+
+int orc_iccg(const orc_csr *c, const double *a, double *fi, const double *rhs,
+             int itr_max, double tol_abs, double tol_rel, double *hist,
+             orc_solve_info *info);
+int orc_dpcg(const orc_csr *c, const double *a, double *fi, const double *rhs,
+             int itr_max, double tol_abs, double tol_rel, double *hist,
+             orc_solve_info *info);
+int orc_bicgstab(const orc_csr *c, const double *a, double *fi, const double *rhs,
+                 int itr_max, double tol_abs, double tol_rel, double *hist,
+                 orc_solve_info *info);
+int orc_gauss_seidel(const orc_csr *c, const double *a, double *fi,
+                     const double *rhs, int itr_max, double tol_abs,
+                     double tol_rel, double *hist, orc_solve_info *info);
+
+/* DIC diagonal (linear_solvers.f90:439-445) and DILU diagonal (:613-624),
+ * both stored inverted (d = 1/pivot). */
+void orc_dic_diag(const orc_csr *c, const double *a, double *d);
+void orc_dilu_diag(const orc_csr *c, const double *a, double *d);
+/* One application z = M^{-1} r of the DIC/DILU preconditioner as the
+ * reference's sweeps do it (:458-475). */
+void orc_dic_apply(const orc_csr *c, const double *a, const double *d,
+                   const double *r, double *z);
+
+/* y = A x, row by row in CSR order (:493-498). */
+void orc_spmv(const orc_csr *c, const double *a, const double *x, double *y);
+
+/* Level sets of the strict-lower DAG: lev[i] = 0 if row i has no strict-lower
+ * entry, else 1 + max lev[ja[k]] over ia[i] <= k < diag[i].  upper=1 uses the
+ * strict-upper DAG processed from the last row.  Returns number of levels. */
+int orc_levels(const orc_csr *c, int upper, int *lev);
+
+/* Convection-diffusion assembly as calculate_temperature composes it
+ * (temperature.f90:75-366 with facefluxsc scalar_fluxes.f90:32-337 and
+ * face_value interpolation.f90:28-157,318-356), steady, no ddt.
+ * flmass[n_faces], dcoef[n_faces] (diffusivity per face), phi[numTotal],
+ * grad[3*numTotal] (AoS), gam = deferred-correction blend, scheme 0 = cds,
+ * 1 = linearUpwind.  bc_kind per boundary face: 0 = Dirichlet wall
+ * (:206-210), 1 = zero-gradient wall (:224, copies phi_P to phi_b),
+ * 2 = inlet/outlet/pressure flux boundary (:182-190).  urf: under-relaxation
+ * (1 = none).  Outputs a[nnz] (zeroed first), su[n], sp[n] (zeroed first).
+ * phi boundary entries of zero-gradient faces are updated in place. */
+void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                  const int *slot_pn, const int *slot_np,
+                  const double *flmass, const double *dcoef,
+                  double *phi, const double *grad, double gam, int scheme,
+                  const unsigned char *bc_kind, double urf,
+                  double *a, double *su, double *sp);
+
+/* Reference constants (parameters.f90:6-7 widened from default REAL). */
+double orc_small(void);
+double orc_s0_init(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,271 @@
+! ref_driver.f90 -- drives the reference freeCappuccino routines, compiled
+! from the reference sources by oracle/ref/Makefile, to write golden vectors.
+! TEST INFRASTRUCTURE ONLY: run in this container by tests/golden/make_golden.py.
+!
+! Usage (run inside a case directory that holds polyMesh/):
+!   ref_driver <native|foam> <jobfile>
+! Every file name in the job file is relative to the case directory.
+! Job lines (list-directed; names without '/'):
+!   dump                                      -> geometry and CSR arrays
+!   laplacian  mu phi su0 a_out su_out        (laplacian.f90)
+!   grad_gauss phi out                        (gradients.f90:1607)
+!   grad_lsq   variant phi out                variant: lsq | lsq_dm | lsq_qr
+!   solve      solver itr_max tol_abs tol_rel a fi0 rhs fi_out info_out
+!   convdiff   flmass dcoef phi grad gam scheme bckind a_out su_out sp_out
+! Binary files are raw little-endian stream files (real(8) / integer(4)).
+program ref_driver
+  use types
+  use parameters
+  use geometry
+  use sparse_matrix
+  use gradients
+  use linear_solvers
+  use scalar_fluxes
+  implicit none
+
+  character(len=256) :: fmt_arg, jobfile
+  character(len=32) :: cmd
+  integer :: ju, ios
+
+  call get_command_argument(1, fmt_arg)
+  call get_command_argument(2, jobfile)
+  if (trim(fmt_arg) == 'foam') then
+    mesh_format = 'foamMesh'
+  else
+    mesh_format = 'nativeMesh'
+  endif
+  ltest = .false.
+
+  call read_mesh
+  call create_CSR_matrix
+
+  allocate(res(numCells), reso(numCells), pk(numCells), zk(numCells), d(numCells), &
+           uk(numCells), vk(numCells))
+
+  open(newunit=ju, file=trim(jobfile), status='old', action='read')
+  do
+    read(ju, *, iostat=ios) cmd
+    if (ios /= 0) exit
+    backspace(ju)
+    select case (trim(cmd))
+    case ('dump')
+      read(ju, *) cmd
+      call job_dump()
+    case ('laplacian')
+      call job_laplacian(ju)
+    case ('grad_gauss')
+      call job_grad_gauss(ju)
+    case ('grad_lsq')
+      call job_grad_lsq(ju)
+    case ('solve')
+      call job_solve(ju)
+    case ('convdiff')
+      call job_convdiff(ju)
+    case default
+      read(ju, *) cmd
+      write(*,*) 'ref_driver: unknown job ', trim(cmd)
+      stop 2
+    end select
+  end do
+  close(ju)
+
+contains
+
+  subroutine wr8(fname, v, nv)
+    character(len=*), intent(in) :: fname
+    integer, intent(in) :: nv
+    real(dp), intent(in) :: v(nv)
+    integer :: u
+    open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+    write(u) v
+    close(u)
+  end subroutine
+
+  subroutine wi4(fname, v, nv)
+    character(len=*), intent(in) :: fname
+    integer, intent(in) :: nv
+    integer, intent(in) :: v(nv)
+    integer :: u
+    open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+    write(u) v
+    close(u)
+  end subroutine
+
+  subroutine rd8(fname, v, nv)
+    character(len=*), intent(in) :: fname
+    integer, intent(in) :: nv
+    real(dp), intent(out) :: v(nv)
+    integer :: u
+    open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='old')
+    read(u) v
+    close(u)
+  end subroutine
+
+  subroutine job_dump()
+    integer :: u
+    call wr8('g_xc.bin', xc, numCells)
+    call wr8('g_yc.bin', yc, numCells)
+    call wr8('g_zc.bin', zc, numCells)
+    call wr8('g_vol.bin', vol, numCells)
+    call wr8('g_arx.bin', arx, numFaces)
+    call wr8('g_ary.bin', ary, numFaces)
+    call wr8('g_arz.bin', arz, numFaces)
+    call wr8('g_xf.bin', xf, numFaces)
+    call wr8('g_yf.bin', yf, numFaces)
+    call wr8('g_zf.bin', zf, numFaces)
+    call wr8('g_facint.bin', facint, numInnerFaces)
+    call wr8('g_Df.bin', Df, numInnerFaces)
+    if (nwal > 0) then
+      call wr8('g_dnw.bin', dnw, nwal)
+      call wr8('g_srdw.bin', srdw, nwal)
+    endif
+    if (nsym > 0) then
+      call wr8('g_dns.bin', dns, nsym)
+      call wr8('g_srds.bin', srds, nsym)
+    endif
+    call wi4('c_ia.bin', ia, numCells+1)
+    call wi4('c_ja.bin', ja, nnz)
+    call wi4('c_diag.bin', diag, numCells)
+    call wi4('c_slot_pn.bin', icell_jcell_csr_index, numInnerFaces)
+    call wi4('c_slot_np.bin', jcell_icell_csr_index, numInnerFaces)
+    open(newunit=u, file='sizes.txt', status='replace')
+    write(u,*) numNodes, numCells, numInnerFaces, numBoundaryFaces, numFaces, nnz, nwal, nsym
+    close(u)
+  end subroutine
+
+  subroutine job_laplacian(ju)
+    integer, intent(in) :: ju
+    character(len=64) :: c, fmu, fphi, fsu0, fa, fsu
+    real(dp), allocatable :: mu(:), phi(:)
+    read(ju, *) c, fmu, fphi, fsu0, fa, fsu
+    allocate(mu(numCells), phi(numTotal))
+    call rd8(fmu, mu, numCells)
+    call rd8(fphi, phi, numTotal)
+    call rd8(fsu0, su, numCells)
+    call laplacian(mu, phi)
+    call wr8(fa, a, nnz)
+    call wr8(fsu, su, numCells)
+    deallocate(mu, phi)
+  end subroutine
+
+  subroutine job_grad_gauss(ju)
+    integer, intent(in) :: ju
+    character(len=64) :: c, fphi, fout
+    real(dp), allocatable :: phi(:), g(:,:)
+    read(ju, *) c, fphi, fout
+    allocate(phi(numTotal), g(3,numTotal))
+    call rd8(fphi, phi, numTotal)
+    call grad_gauss(phi, g)
+    call wr8(fout, g, 3*numTotal)
+    deallocate(phi, g)
+  end subroutine
+
+  subroutine job_grad_lsq(ju)
+    integer, intent(in) :: ju
+    character(len=64) :: c, variant, fphi, fout
+    real(dp), allocatable :: phi(:), g(:,:)
+    read(ju, *) c, variant, fphi, fout
+    lstsq = .false.; lstsq_qr = .false.; lstsq_dm = .false.
+    gauss = .false.; node_gauss = .false.
+    limiter = 'none'
+    select case (trim(variant))
+    case ('lsq')
+      lstsq = .true.
+    case ('lsq_dm')
+      lstsq_dm = .true.
+    case ('lsq_qr')
+      lstsq_qr = .true.
+    end select
+    call create_lsq_grad_matrix
+    allocate(phi(numTotal), g(3,numTotal))
+    call rd8(fphi, phi, numTotal)
+    call grad(phi, g)
+    call wr8(fout, g, 3*numTotal)
+    deallocate(phi, g)
+  end subroutine
+
+  subroutine job_solve(ju)
+    integer, intent(in) :: ju
+    character(len=64) :: c, solver, fa, ffi, frhs, fout, finfo
+    integer :: itr_max, u
+    real(dp) :: tol_abs, tol_rel, res0
+    real(dp), allocatable :: fi(:), rhs(:)
+    read(ju, *) c, solver, itr_max, tol_abs, tol_rel, fa, ffi, frhs, fout, finfo
+    allocate(fi(numTotal), rhs(numCells))
+    call rd8(fa, a, nnz)
+    fi = 0.0_dp
+    call rd8(ffi, fi, numCells)
+    call rd8(frhs, rhs, numCells)
+    res = 0.0_dp; reso = 0.0_dp; pk = 0.0_dp; zk = 0.0_dp; d = 0.0_dp
+    uk = 0.0_dp; vk = 0.0_dp
+    res0 = 0.0_dp
+    call csrsolve(trim(solver), fi, rhs, res0, itr_max, tol_abs, tol_rel, 'phi')
+    call wr8(fout, fi, numCells)
+    open(newunit=u, file=trim(finfo), status='replace')
+    write(u,'(es25.17)') res0
+    close(u)
+    deallocate(fi, rhs)
+  end subroutine
+
+  ! Steady convection-diffusion assembly with the reference flux routines,
+  ! composed as calculate_temperature does (temperature.f90:118-366).
+  ! bckind per boundary face: 0 Dirichlet wall, 1 zero-gradient, 2 flux bc.
+  subroutine job_convdiff(ju)
+    integer, intent(in) :: ju
+    character(len=64) :: c, ffm, fdc, fphi, fgr, fbk, fa, fsu, fsp
+    character(len=30) :: scheme
+    real(dp) :: gam, cap, can, suadd, dw, off
+    real(dp), allocatable :: fm(:), dc(:), phi(:), gr(:,:), bk(:), spl(:)
+    integer :: i, ijp, ijn, k, iface, ijb, inp
+    read(ju, *) c, ffm, fdc, fphi, fgr, gam, scheme, fbk, fa, fsu, fsp
+    allocate(fm(numFaces), dc(numFaces), phi(numTotal), gr(3,numTotal), bk(numBoundaryFaces), &
+             spl(numCells))
+    call rd8(ffm, fm, numFaces)
+    call rd8(fdc, dc, numFaces)
+    call rd8(fphi, phi, numTotal)
+    call rd8(fgr, gr, 3*numTotal)
+    call rd8(fbk, bk, numBoundaryFaces)
+    a = 0.0_dp; su = 0.0_dp; spl = 0.0_dp
+    do i = 1, numInnerFaces
+      ijp = owner(i)
+      ijn = neighbour(i)
+      call facefluxsc(i, ijp, ijn, xf(i), yf(i), zf(i), arx(i), ary(i), arz(i), &
+                      fm(i), facint(i), gam, scheme, phi, gr, dc(i), cap, can, suadd)
+      k = icell_jcell_csr_index(i); a(k) = can
+      k = jcell_icell_csr_index(i); a(k) = cap
+      k = diag(ijp); a(k) = a(k) - can
+      k = diag(ijn); a(k) = a(k) - cap
+      su(ijp) = su(ijp) + suadd
+      su(ijn) = su(ijn) - suadd
+    enddo
+    do i = 1, numBoundaryFaces
+      iface = numInnerFaces + i
+      ijp = owner(iface)
+      ijb = numCells + i
+      if (nint(bk(i)) == 2) then
+        call facefluxsc(ijp, xf(iface), yf(iface), zf(iface), arx(iface), ary(iface), arz(iface), &
+                        fm(iface), gr, dc(iface), cap, can, suadd)
+        spl(ijp) = spl(ijp) - can
+        su(ijp) = su(ijp) - can*phi(ijb) + suadd
+      elseif (nint(bk(i)) == 0) then
+        dw = dc(iface) * ( sqrt(arx(iface)**2+ary(iface)**2+arz(iface)**2) / &
+             ( (xf(iface)-xc(ijp))*(arx(iface)/sqrt(arx(iface)**2+ary(iface)**2+arz(iface)**2)) + &
+               (yf(iface)-yc(ijp))*(ary(iface)/sqrt(arx(iface)**2+ary(iface)**2+arz(iface)**2)) + &
+               (zf(iface)-zc(ijp))*(arz(iface)/sqrt(arx(iface)**2+ary(iface)**2+arz(iface)**2)) ) )
+        spl(ijp) = spl(ijp) + dw
+        su(ijp) = su(ijp) + dw*phi(ijb)
+      else
+        phi(ijb) = phi(ijp)
+      endif
+    enddo
+    do inp = 1, numCells
+      off = sum( a(ia(inp) : ia(inp+1)-1) ) - a(diag(inp))
+      a(diag(inp)) = spl(inp) - off
+    enddo
+    call wr8(fa, a, nnz)
+    call wr8(fsu, su, numCells)
+    call wr8(fsp, spl, numCells)
+    deallocate(fm, dc, phi, gr, bk, spl)
+  end subroutine
+
+end program ref_driver
