@@ -271,3 +271,61 @@ def read_native(case_dir: str) -> Mesh:
              nfaces=np.array(nf, np.int32), start_face=np.array(sf, np.int32))
     m.check()
     return m
+
+
+# --------------------------------------------------------------------------
+# OpenFOAM polyMesh (0-based lists with FoamFile headers), the format
+# read_mesh_openfoam reads (geometry.f90:846-1300).  polyMesh/boundary uses
+# the reference's simplified table "name type nFaces startFace".
+# --------------------------------------------------------------------------
+
+def _foam_list(path):
+    """Return the entries of the first top-level '(...)' list after the count."""
+    with open(path) as f:
+        txt = f.read()
+    # drop the FoamFile dictionary and comments
+    import re
+    txt = re.sub(r"/\*.*?\*/", " ", txt, flags=re.S)
+    txt = re.sub(r"//[^\n]*", " ", txt)
+    i = txt.find("FoamFile")
+    if i >= 0:
+        j = txt.find("}", i)
+        txt = txt[j + 1:]
+    m = re.search(r"(\d+)\s*\(", txt)
+    count = int(m.group(1))
+    body = txt[m.end():]
+    return count, body
+
+
+def read_foam(case_dir: str) -> Mesh:
+    import re
+    d = os.path.join(case_dir, "polyMesh")
+    npts, body = _foam_list(os.path.join(d, "points"))
+    vals = re.findall(r"\(([^()]*)\)", body)[:npts]
+    pts = np.array([[float(t) for t in v.split()] for v in vals], dtype=np.float64)
+    nf, body = _foam_list(os.path.join(d, "faces"))
+    faces = re.findall(r"(\d+)\s*\(([^()]*)\)", body)[:nf]
+    node = np.full((nf, NOMAX), -1, dtype=np.int32)
+    nnodes = np.zeros(nf, dtype=np.int32)
+    for k, (cnt, ids) in enumerate(faces):
+        v = [int(t) for t in ids.split()]
+        nnodes[k] = int(cnt)
+        node[k, : len(v)] = v
+    no, body = _foam_list(os.path.join(d, "owner"))
+    owner = np.array(body.split(")")[0].split(), dtype=np.int64)[:no].astype(np.int32)
+    nn, body = _foam_list(os.path.join(d, "neighbour"))
+    nb = np.array(body.split(")")[0].split(), dtype=np.int64)[:nn].astype(np.int32)
+    names, types, nfs, sfs = [], [], [], []
+    with open(os.path.join(d, "boundary")) as f:
+        for line in f:
+            s = line.strip()
+            if not s or s.startswith("#"):
+                continue
+            t = s.split()
+            names.append(t[0]); types.append(t[1]); nfs.append(int(t[2])); sfs.append(int(t[3]))
+    n_cells = int(max(owner.max(), nb.max() if nb.size else 0)) + 1
+    m = Mesh(points=pts, nnodes=nnodes, node=node, owner=owner, neighbour=nb,
+             n_cells=n_cells, names=names, types=types,
+             nfaces=np.array(nfs, np.int32), start_face=np.array(sfs, np.int32))
+    m.check()
+    return m

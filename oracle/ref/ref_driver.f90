@@ -12,6 +12,9 @@
 !   grad_lsq   variant phi out                variant: lsq | lsq_dm | lsq_qr
 !   solve      solver itr_max tol_abs tol_rel a fi0 rhs fi_out info_out
 !   convdiff   flmass dcoef phi grad gam scheme bckind a_out su_out sp_out
+!   solve_explicit solver n nnz itr_max tol_abs tol_rel ia ja diag a fi0 rhs fi_out info_out
+!              (explicit-array entry points iccg/dpcg/bicgstab/GaussSeidel on a
+!               CSR system of its own, 1-based int32 ia/ja/diag files)
 ! Binary files are raw little-endian stream files (real(8) / integer(4)).
 program ref_driver
   use types
@@ -61,6 +64,8 @@ program ref_driver
       call job_solve(ju)
     case ('convdiff')
       call job_convdiff(ju)
+    case ('solve_explicit')
+      call job_solve_explicit(ju)
     case default
       read(ju, *) cmd
       write(*,*) 'ref_driver: unknown job ', trim(cmd)
@@ -205,6 +210,45 @@ contains
     write(u,'(es25.17)') res0
     close(u)
     deallocate(fi, rhs)
+  end subroutine
+
+  subroutine job_solve_explicit(ju)
+    integer, intent(in) :: ju
+    character(len=64) :: c, solver, fia, fja, fdg, fa, ffi, frhs, fout, finfo
+    integer :: n, nz, itr_max, u
+    real(dp) :: tol_abs, tol_rel, resor
+    integer, allocatable :: eia(:), eja(:), edg(:)
+    real(dp), allocatable :: ea(:), fi(:), rhs(:)
+    read(ju, *) c, solver, n, nz, itr_max, tol_abs, tol_rel, fia, fja, fdg, fa, ffi, frhs, fout, finfo
+    allocate(eia(n+1), eja(nz), edg(n), ea(nz), fi(n), rhs(n))
+    open(newunit=u, file=trim(fia), access='stream', form='unformatted', status='old'); read(u) eia; close(u)
+    open(newunit=u, file=trim(fja), access='stream', form='unformatted', status='old'); read(u) eja; close(u)
+    open(newunit=u, file=trim(fdg), access='stream', form='unformatted', status='old'); read(u) edg; close(u)
+    call rd8(fa, ea, nz)
+    call rd8(ffi, fi, n)
+    call rd8(frhs, rhs, n)
+    deallocate(res, reso, pk, zk, d, uk, vk)
+    allocate(res(n), reso(n), pk(n), zk(n), d(n), uk(n), vk(n))
+    res = 0.0_dp; reso = 0.0_dp; pk = 0.0_dp; zk = 0.0_dp; d = 0.0_dp; uk = 0.0_dp; vk = 0.0_dp
+    resor = 0.0_dp
+    select case (trim(solver))
+    case ('iccg')
+      call iccg(n, nz, eia, eja, ea, edg, fi, rhs, resor, itr_max, tol_abs, tol_rel, 'x', .false.)
+    case ('dpcg')
+      call dpcg(n, nz, eia, eja, ea, edg, fi, rhs, resor, itr_max, tol_abs, tol_rel, 'x', .false.)
+    case ('bicgstab')
+      call bicgstab(n, nz, eia, eja, ea, edg, fi, rhs, resor, itr_max, tol_abs, tol_rel, 'x', .false.)
+    case ('gauss-seidel')
+      call GaussSeidel(n, nz, eia, eja, ea, edg, fi, rhs, resor, itr_max, tol_abs, tol_rel, 'x', .false.)
+    end select
+    call wr8(fout, fi, n)
+    open(newunit=u, file=trim(finfo), status='replace')
+    write(u,'(es25.17)') resor
+    close(u)
+    deallocate(res, reso, pk, zk, d, uk, vk)
+    allocate(res(numCells), reso(numCells), pk(numCells), zk(numCells), d(numCells), &
+             uk(numCells), vk(numCells))
+    deallocate(eia, eja, edg, ea, fi, rhs)
   end subroutine
 
   ! Steady convection-diffusion assembly with the reference flux routines,
