@@ -1,0 +1,258 @@
+"""Benchmark: cells/s to 1e-8 relative residual, mixed vs FP64, on one GPU.
+
+Workload (defaults): pressure-correction-style Poisson on 128^3 hexahedra with
+interior nodes jittered +-20% h, face diffusivity 10^U(-1,1), zero-mean
+source, Dirichlet on the x- patch and zero-gradient elsewhere.  One step =
+Laplacian assembly with face diffusivity + Gauss gradient of the previous
+iterate + explicit non-orthogonal correction + DIC-PCG solve from x0 = 0 to
+||b - A x||_2 / ||b||_2 <= 1e-8 (FP64 true residual).  Inputs are resident
+in HBM before the timed region; every step is a full assemble+solve.
+
+value = cells / (seconds per step) of the mixed-precision path (FP32 inner
+PCG inside FP64 iterative refinement); the plain FP64 path is timed in the
+same run and reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "freecappuccino-dev_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def alg_bytes(name, n, nnz, Fi, Fb):
+    """Algorithmic (compulsory) HBM bytes of one launch of a named kernel."""
+    lower = Fi  # strict-lower entries = inner faces
+    tab = {
+        "spmv_pq_f32": nnz * 8 + (n + 1) * 4 + 2 * n * 4,
+        "spmv_pq_f64": nnz * 12 + (n + 1) * 4 + 2 * n * 8,
+        "spmv_row_f64": nnz * 12 + (n + 1) * 4 + 2 * n * 8,
+        "sweep_fwd_f32": lower * 8 + 4 * n + 3 * n * 4,
+        "sweep_bwd_f32": lower * 8 + 4 * n + 3 * n * 4,
+        "sweep_fwd_f64": lower * 12 + 4 * n + 3 * n * 8,
+        "sweep_bwd_f64": lower * 12 + 4 * n + 3 * n * 8,
+        "dic_setup": lower * 12 + 4 * n + 2 * n * 8,
+        "residual_f64": nnz * 12 + (n + 1) * 4 + 3 * n * 8,
+        "dot_rz": 2 * n * 8,
+        "pupdate": 3 * n * 8,
+        "xr_update": 6 * n * 8,
+    }
+    return tab.get(name)
+
+
+def run_gpu(args):
+    import torch
+
+    from fcp import workloads
+    from fcp.api import Context, FCP_CRIT_TRUE_L2, FCP_PREC_FP64, FCP_PREC_MIXED
+
+    N = args.size
+    t0 = time.time()
+    mesh = workloads.poisson_jittered_mesh(N)
+    log(f"[bench] mesh {N}^3: n={mesh.n_cells} Fi={mesh.n_inner} ({time.time()-t0:.1f}s)")
+    ctx = Context(0)
+    ctx.read_mesh(mesh)
+    geo = ctx.geometry()
+    ctx.create_CSR_matrix(want_arrays=False)
+    n, nnz, Fi, Fb = mesh.n_cells, ctx.nnz, mesh.n_inner, mesh.n_bnd
+    log(f"[bench] geometry+CSR+levels on GPU done ({time.time()-t0:.1f}s) nnz={nnz}")
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    dev = torch.device("cuda:0")
+    T = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
+    mu_f = T(inp["mu_f"])
+    gamma = T(inp["gamma"])
+    bc_kind = T(inp["bc_kind"], torch.uint8)
+    phi_bc = T(inp["phi_bc"])
+    phi_prev = T(inp["phi_prev"])
+    sV = T(inp["source"] * geo["vol"])
+    su = torch.empty(n, dtype=torch.float64, device=dev)
+    x = torch.empty(n, dtype=torch.float64, device=dev)
+    grad = torch.empty((mesh.n_total, 3), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step(precision):
+        su.copy_(sV)
+        x.zero_()
+        torch.cuda.synchronize()
+        ctx.laplacian_face(mu_f, bc_kind, phi_bc, su)
+        ctx.grad_gauss(phi_prev, grad)
+        ctx.nonorth_correction(gamma, grad, su)
+        rc, info = ctx.csrsolve("iccg", x, su, itr_max=20000, tol_abs=0.0, tol_rel=1e-8,
+                                precision=precision, criterion=FCP_CRIT_TRUE_L2,
+                                inner_tol=args.inner_tol, check_every=args.check_every)
+        if rc != 0 or not info["rel_res_l2"] <= 1e-8:
+            raise RuntimeError(f"solve did not reach 1e-8: rc={rc} info={info}")
+        return info
+
+    results = {}
+    for name, prec in (("fp64", FCP_PREC_FP64), ("mixed", FCP_PREC_MIXED)):
+        for _ in range(args.warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        infos = [step(prec) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        ms = (te - ts) * 1e3 / args.steps
+        results[name] = dict(ms_per_step=ms, cells_per_s=n / (ms / 1e3),
+                             iterations=infos[-1]["iterations"],
+                             outer_iterations=infos[-1]["outer_iterations"],
+                             rel_res_l2=infos[-1]["rel_res_l2"], solve_ms=infos[-1]["time_ms"])
+        log(f"[bench] {name}: {ms:.2f} ms/step, {n/(ms/1e3)/1e6:.2f} Mcells/s, "
+            f"iters={infos[-1]['iterations']} outer={infos[-1]['outer_iterations']} "
+            f"relres={infos[-1]['rel_res_l2']:.2e}")
+    x_mixed = x.cpu().numpy().copy()
+
+    # error of the mixed solution against a tight FP64 GPU solve
+    su.copy_(sV)
+    torch.cuda.synchronize()
+    ctx.laplacian_face(mu_f, bc_kind, phi_bc, su)
+    ctx.grad_gauss(phi_prev, grad)
+    ctx.nonorth_correction(gamma, grad, su)
+    x_ref = torch.zeros(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ctx.csrsolve("iccg", x_ref, su, itr_max=50000, tol_rel=1e-13, precision=FCP_PREC_FP64,
+                 criterion=FCP_CRIT_TRUE_L2)
+    xr = x_ref.cpu().numpy()
+    err_mixed = float(np.linalg.norm(x_mixed - xr) / np.linalg.norm(xr))
+
+    # per-kernel device time (HIP events on the library stream) over profiled steps
+    ctx.profile(True)
+    for _ in range(args.prof_steps):
+        step(FCP_PREC_MIXED)
+    prof = ctx.profile_report()
+    ctx.profile(False)
+    total = sum(v[1] for v in prof.values())
+    dom = max(prof.items(), key=lambda kv: kv[1][1])
+    log("[bench] kernel profile (mixed, %d steps): " % args.prof_steps +
+        ", ".join(f"{k}={v[1]/args.prof_steps:.2f}ms/{v[0]//args.prof_steps}x" for k, v in
+                  sorted(prof.items(), key=lambda kv: -kv[1][1])[:8]))
+    kernels = {}
+    for k, (cnt, tms) in prof.items():
+        b = alg_bytes(k, n, nnz, Fi, Fb)
+        avg_us = tms / cnt * 1e3
+        kernels[k] = dict(launches_per_step=cnt / args.prof_steps, avg_us=avg_us,
+                          share=tms / total if total else 0.0)
+        if b:
+            kernels[k]["GBs"] = b / (avg_us * 1e-6) / 1e9
+    name, (cnt, tms) = dom
+    avg_s = tms / cnt / 1e3
+    b = alg_bytes(name, n, nnz, Fi, Fb)
+    roof = dict(bound="hbm", kernel=name, achieved=(b / avg_s / 1e9) if b else None,
+                peak=HBM_PEAK_GBS, unit="GB/s", frac=(b / avg_s / 1e9 / HBM_PEAK_GBS) if b else None,
+                traffic=None, alg_bytes_per_launch=b, avg_launch_us=avg_s * 1e6)
+
+    # standalone SpMV FP64 (reference row order) bandwidth
+    xs = torch.rand(n, dtype=torch.float64, device=dev)
+    ys = torch.empty(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    for _ in range(20):
+        ctx.spmv(xs, ys)
+    sp = ctx.profile_report().get("spmv_row_f64")
+    ctx.profile(False)
+    spmv = None
+    if sp:
+        avg = sp[1] / sp[0] / 1e3
+        bb = alg_bytes("spmv_row_f64", n, nnz, Fi, Fb)
+        spmv = dict(kernel="spmv_row_f64", avg_us=avg * 1e6, GBs=bb / avg / 1e9,
+                    frac=bb / avg / 1e9 / HBM_PEAK_GBS, alg_bytes=bb)
+    ctx.close()
+    return dict(n=n, nnz=nnz, Fi=Fi, results=results, err_mixed_vs_fp64_tight=err_mixed,
+                roofline=roof, kernels=kernels, spmv=spmv)
+
+
+def run_cpu_baseline(args, n_gpu_cells):
+    """The oracle (serial C restatement of the reference path) on the same
+    workload at a bounded size, 1 host core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from fcp import workloads
+    from oracle import Oracle
+
+    N = args.cpu_size
+    mesh = workloads.poisson_jittered_mesh(N)
+    o = Oracle(mesh)
+    g = o.geom
+    inp = workloads.poisson_jittered_inputs(mesh, g["vol"], g["xc"], g["yc"], g["zc"])
+    t0 = time.perf_counter()
+    a, su = o.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], inp["source"] * g["vol"])
+    gr = o.grad_gauss(inp["phi_prev"])
+    su = o.nonorth_correction(inp["gamma"], gr, su)
+    out = o.solve("iccg", a, su, itr_max=20000, tol_abs=0.0, tol_rel=1e-8)
+    t1 = time.perf_counter()
+    rel = float(np.linalg.norm(su - o.spmv(a, out["fi"])) / np.linalg.norm(su))
+    return dict(value=mesh.n_cells / (t1 - t0), unit="cells/s", cores=1, kind="port",
+                sample=f"same workload at {N}^3 ({mesh.n_cells} cells): oracle laplacian_face + "
+                       f"grad_gauss + non-orthogonal correction + ICCG (reference L1 criterion "
+                       f"tol_rel 1e-8, reached ||b-Ax||/||b|| = {rel:.1e}) in {t1-t0:.1f} s, "
+                       f"{out['iters']} iterations",
+                host=platform.processor() or platform.machine())
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--cpu-size", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prof-steps", type=int, default=1)
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--check-every", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
+              f"(asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+    g = run_gpu(args)
+    cpu = None if args.no_cpu else run_cpu_baseline(args, g["n"])
+    mixed, fp64 = g["results"]["mixed"], g["results"]["fp64"]
+    line = {
+        "metric": "cells/s to 1e-8 rel-res: mixed vs FP64 on GPU; SpMV HBM GB/s % of peak",
+        "value": mixed["cells_per_s"],
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": f"pressure-correction Poisson, {args.size}^3 hex jittered +-20%, "
+                               "face Gamma 10^U(-1,1), Dirichlet x-, Gauss grad + non-orth "
+                               "correction + DIC-PCG to ||b-Ax||/||b||<=1e-8",
+                   "cells": g["n"], "nnz": g["nnz"], "inner_faces": g["Fi"]},
+        "fp64": {"cells_per_s": fp64["cells_per_s"], "ms_per_step": fp64["ms_per_step"],
+                 "iterations": fp64["iterations"]},
+        "mixed": {"iterations": mixed["iterations"], "outer_iterations": mixed["outer_iterations"],
+                  "rel_res_l2": mixed["rel_res_l2"],
+                  "err_vs_fp64_1e-13": g["err_mixed_vs_fp64_tight"]},
+        "mixed_over_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
+        "spmv": g["spmv"],
+        "roofline": g["roofline"],
+        "cpu_baseline": cpu,
+        "kernels": g["kernels"],
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,679 @@
+// fcp_api.hip -- extern "C" entry points of libfcp (see include/fcp.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "fcp_internal.h"
+
+// ---------------------------------------------------------------------------
+// errors, pointer kinds, staging
+// ---------------------------------------------------------------------------
+int fcp_fail(fcp_ctx *ctx, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx) ctx->err = buf;
+  return code;
+}
+
+bool fcp_is_device_ptr(const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+namespace {
+
+// An input/output array that is either used in place (device pointer) or
+// staged through a device buffer (host pointer).
+template <typename T>
+struct Arg {
+  T *dev = nullptr;
+  T *host = nullptr;
+  size_t n = 0;
+  DBuf<T> own;
+};
+
+template <typename T>
+int arg_in(fcp_ctx *ctx, const T *p, size_t n, Arg<T> &a) {
+  a.n = n;
+  if (!p) return 0;
+  if (fcp_is_device_ptr(p)) {
+    a.dev = const_cast<T *>(p);
+    return 0;
+  }
+  a.host = const_cast<T *>(p);
+  FCP_HIP(ctx, a.own.alloc(n > 0 ? n : 1));
+  a.dev = a.own.p;
+  if (n) FCP_HIP(ctx, hipMemcpyAsync(a.dev, p, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return 0;
+}
+
+template <typename T>
+int arg_out(fcp_ctx *ctx, T *p, size_t n, Arg<T> &a) {
+  a.n = n;
+  if (!p) return 0;
+  if (fcp_is_device_ptr(p)) {
+    a.dev = p;
+    return 0;
+  }
+  a.host = p;
+  FCP_HIP(ctx, a.own.alloc(n > 0 ? n : 1));
+  a.dev = a.own.p;
+  return 0;
+}
+
+template <typename T>
+int arg_back(fcp_ctx *ctx, Arg<T> &a) {
+  if (a.host && a.n)
+    FCP_HIP(ctx, hipMemcpyAsync(a.host, a.dev, a.n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  return 0;
+}
+
+__global__ void k_sub1(const int *__restrict__ in, int *__restrict__ out, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i] - 1;
+}
+__global__ void k_add1(const int *__restrict__ in, int *__restrict__ out, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i] + 1;
+}
+__global__ void k_node_sub1(const int *__restrict__ in, const int *__restrict__ nn,
+                            int *__restrict__ out, int F) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  for (int k = 0; k < FCP_NOMAX; ++k) {
+    const int v = in[(size_t)f * FCP_NOMAX + k];
+    out[(size_t)f * FCP_NOMAX + k] = (k < nn[f]) ? v - 1 : 0;
+  }
+}
+__global__ void k_bpatch(const int *__restrict__ start, const int *__restrict__ nf, int np, int Fi,
+                         int *__restrict__ bpatch) {
+  int ib = blockIdx.x;
+  if (ib >= np) return;
+  for (int i = threadIdx.x; i < nf[ib]; i += blockDim.x) bpatch[start[ib] - Fi + i] = ib;
+}
+
+// Bounds checks of caller-supplied connectivity (1-based), run on the device
+// before any kernel indexes with it; violations only set a flag.
+__global__ void k_check_mesh(const int *__restrict__ nn, const int *__restrict__ node,
+                             const int *__restrict__ owner, const int *__restrict__ neighbour,
+                             int F, int Fi, int nNodes, int n, int *__restrict__ bad) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const int m = nn[f];
+  bool ok = (m >= 3 && m <= FCP_NOMAX);
+  if (ok)
+    for (int k = 0; k < m; ++k) {
+      const int v = node[(size_t)f * FCP_NOMAX + k];
+      ok = ok && v >= 1 && v <= nNodes;
+    }
+  ok = ok && owner[f] >= 1 && owner[f] <= n;
+  if (f < Fi) ok = ok && neighbour[f] >= 1 && neighbour[f] <= n && neighbour[f] != owner[f];
+  if (!ok) atomicOr(bad, 1);
+}
+
+bool csr_ok_host(int n, int nnz, const int *ia, const int *ja, const int *diag) {
+  if (ia[0] != 1 || ia[n] != nnz + 1) return false;
+  for (int i = 0; i < n; ++i) {
+    if (ia[i + 1] < ia[i]) return false;
+    const int d = diag[i];
+    if (d < ia[i] || d >= ia[i + 1]) return false;
+    if (ja[d - 1] != i + 1) return false;
+  }
+  for (int k = 0; k < nnz; ++k)
+    if (ja[k] < 1 || ja[k] > n) return false;
+  return true;
+}
+
+int out_index_plus1(fcp_ctx *ctx, const int *dev0, int *dst, size_t n) {
+  if (!dst || n == 0) return 0;
+  Arg<int> o;
+  FCP_TRY(arg_out(ctx, dst, n, o));
+  k_add1<<<grid_for((long long)n, 256), 256, 0, ctx->stream>>>(dev0, o.dev, n);
+  FCP_HIP(ctx, hipGetLastError());
+  FCP_TRY(arg_back(ctx, o));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int out_copy(fcp_ctx *ctx, const double *dev, double *dst, size_t n) {
+  if (!dst || n == 0) return 0;
+  hipMemcpyKind kind = fcp_is_device_ptr(dst) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  FCP_HIP(ctx, hipMemcpyAsync(dst, dev, n * sizeof(double), kind, ctx->stream));
+  return 0;
+}
+
+int need(fcp_ctx *ctx, bool mesh, bool geom, bool csr) {
+  if (!ctx) return FCP_ERR_ARG;
+  if (mesh && !ctx->have_mesh) return fcp_fail(ctx, FCP_ERR_STATE, "no mesh uploaded");
+  if (geom && !ctx->have_geom) return fcp_fail(ctx, FCP_ERR_STATE, "geometry not computed");
+  if (csr && !ctx->have_csr) return fcp_fail(ctx, FCP_ERR_STATE, "CSR pattern not built");
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// profiling
+// ---------------------------------------------------------------------------
+void fcp_prof_begin(fcp_ctx *ctx, const char *, hipEvent_t *ev) {
+  *ev = nullptr;
+  if (!ctx->prof) return;
+  hipEvent_t e;
+  if (!ctx->ev_pool.empty()) {
+    e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    return;
+  }
+  (void)hipEventRecord(e, ctx->stream);
+  *ev = e;
+}
+
+static void prof_flush(fcp_ctx *ctx) {
+  if (ctx->prof_pending.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &p : ctx->prof_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      auto &e = ctx->prof_tab[p.name];
+      e.launches++;
+      e.total_ms += ms;
+    }
+    ctx->ev_pool.push_back(p.a);
+    ctx->ev_pool.push_back(p.b);
+  }
+  ctx->prof_pending.clear();
+}
+
+void fcp_prof_end(fcp_ctx *ctx, const char *name, hipEvent_t ev) {
+  if (!ctx->prof || !ev) return;
+  hipEvent_t e;
+  if (!ctx->ev_pool.empty()) {
+    e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    return;
+  }
+  (void)hipEventRecord(e, ctx->stream);
+  ctx->prof_pending.push_back({name, ev, e});
+  if (ctx->prof_pending.size() > 4096) prof_flush(ctx);
+}
+
+extern "C" {
+
+int fcp_profile(fcp_ctx *ctx, int enable) {
+  if (!ctx) return FCP_ERR_ARG;
+  prof_flush(ctx);
+  ctx->prof = enable != 0;
+  if (enable) ctx->prof_tab.clear();
+  return 0;
+}
+
+int fcp_profile_report(fcp_ctx *ctx, char *buf, int len) {
+  if (!ctx || !buf || len <= 0) return FCP_ERR_ARG;
+  prof_flush(ctx);
+  std::string s;
+  char line[256];
+  for (auto &kv : ctx->prof_tab) {
+    snprintf(line, sizeof line, "%s %d %.6f\n", kv.first.c_str(), kv.second.launches,
+             kv.second.total_ms);
+    s += line;
+  }
+  snprintf(buf, (size_t)len, "%s", s.c_str());
+  return (int)s.size();
+}
+
+void *fcp_stream(fcp_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+int fcp_init(int device, fcp_ctx **out) {
+  if (!out) return FCP_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FCP_ERR_HIP;
+  if (device < 0 || device >= ndev) return FCP_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return FCP_ERR_HIP;
+  fcp_ctx *ctx = new fcp_ctx();
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->ncu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return FCP_ERR_HIP;
+  }
+  *out = ctx;
+  return 0;
+}
+
+void fcp_finalize(fcp_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  prof_flush(ctx);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+  if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *fcp_last_error(const fcp_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void fcp_solve_opts_default(fcp_solve_opts *o) {
+  if (!o) return;
+  memset(o, 0, sizeof *o);
+  o->precision = FCP_PREC_FP64;
+  o->criterion = FCP_CRIT_REF_L1;
+  o->inner_tol = 1e-4;
+  o->max_outer = 30;
+  o->check_every = 8;
+}
+
+// ---------------------------------------------------------------------------
+// mesh / geometry / CSR
+// ---------------------------------------------------------------------------
+int fcp_mesh_upload(fcp_ctx *ctx, int nNodes, int nCells, int nInnerFaces, int nBoundaryFaces,
+                    int nBoundaries, const double *xyz, const int *nnodes, const int *node,
+                    const int *owner, const int *neighbour, const int *nfaces,
+                    const int *startFace, const int *bctype) {
+  if (!ctx) return FCP_ERR_ARG;
+  if (nNodes <= 0 || nCells <= 0 || nInnerFaces < 0 || nBoundaryFaces < 0 || nBoundaries < 0 ||
+      !xyz || !nnodes || !node || !owner || (nInnerFaces > 0 && !neighbour) ||
+      (nBoundaries > 0 && (!nfaces || !startFace || !bctype)))
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_mesh_upload: bad sizes or null arrays");
+  (void)hipSetDevice(ctx->device);
+  const int F = nInnerFaces + nBoundaryFaces;
+  ctx->n = nCells;
+  ctx->F = F;
+  ctx->Fi = nInnerFaces;
+  ctx->Fb = nBoundaryFaces;
+  ctx->nNodes = nNodes;
+  ctx->nPatches = nBoundaries;
+  ctx->h_nfaces.assign(nfaces, nfaces + nBoundaries);
+  ctx->h_start.assign(startFace, startFace + nBoundaries);
+  ctx->h_bctype.assign(bctype, bctype + nBoundaries);
+  long long tot = 0;
+  for (int ib = 0; ib < nBoundaries; ++ib) {
+    if (startFace[ib] < nInnerFaces || startFace[ib] + nfaces[ib] > F || nfaces[ib] < 0)
+      return fcp_fail(ctx, FCP_ERR_ARG, "patch %d: faces outside the boundary range", ib);
+    if (ib > 0 && startFace[ib] < startFace[ib - 1] + nfaces[ib - 1])
+      return fcp_fail(ctx, FCP_ERR_ARG, "patches must be ordered by startFace");
+    tot += nfaces[ib];
+  }
+  if (tot != nBoundaryFaces) return fcp_fail(ctx, FCP_ERR_ARG, "patch faces do not sum to nBoundaryFaces");
+  ctx->have_mesh = ctx->have_geom = ctx->have_csr = false;
+  FCP_HIP(ctx, ctx->points.alloc((size_t)3 * nNodes));
+  FCP_HIP(ctx, ctx->nnodes.alloc(F));
+  FCP_HIP(ctx, ctx->node.alloc((size_t)FCP_NOMAX * F));
+  FCP_HIP(ctx, ctx->owner.alloc(F));
+  FCP_HIP(ctx, ctx->neighbour.alloc(nInnerFaces > 0 ? nInnerFaces : 1));
+  FCP_HIP(ctx, ctx->bpatch.alloc(nBoundaryFaces > 0 ? nBoundaryFaces : 1));
+  FCP_HIP(ctx, ctx->bctype.alloc(nBoundaries > 0 ? nBoundaries : 1));
+  {
+    Arg<double> p;
+    FCP_TRY(arg_in(ctx, xyz, (size_t)3 * nNodes, p));
+    FCP_HIP(ctx, hipMemcpyAsync(ctx->points.p, p.dev, sizeof(double) * 3 * nNodes,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+    Arg<int> nn, nd, ow, nb;
+    FCP_TRY(arg_in(ctx, nnodes, F, nn));
+    FCP_TRY(arg_in(ctx, node, (size_t)FCP_NOMAX * F, nd));
+    FCP_TRY(arg_in(ctx, owner, F, ow));
+    FCP_HIP(ctx, hipMemcpyAsync(ctx->nnodes.p, nn.dev, sizeof(int) * F, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+    if (nInnerFaces > 0) FCP_TRY(arg_in(ctx, neighbour, nInnerFaces, nb));
+    {
+      DBuf<int> bad;
+      FCP_HIP(ctx, bad.alloc(1));
+      FCP_HIP(ctx, hipMemsetAsync(bad.p, 0, sizeof(int), ctx->stream));
+      k_check_mesh<<<grid_for(F, 256), 256, 0, ctx->stream>>>(nn.dev, nd.dev, ow.dev, nb.dev, F,
+                                                              nInnerFaces, nNodes, nCells, bad.p);
+      int hbad = 0;
+      FCP_HIP(ctx, hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+      FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      if (hbad)
+        return fcp_fail(ctx, FCP_ERR_ARG,
+                        "fcp_mesh_upload: face vertex/owner/neighbour index out of range");
+    }
+    k_node_sub1<<<grid_for(F, 256), 256, 0, ctx->stream>>>(nd.dev, nn.dev, ctx->node.p, F);
+    k_sub1<<<grid_for(F, 256), 256, 0, ctx->stream>>>(ow.dev, ctx->owner.p, F);
+    if (nInnerFaces > 0) {
+      k_sub1<<<grid_for(nInnerFaces, 256), 256, 0, ctx->stream>>>(nb.dev, ctx->neighbour.p,
+                                                                   nInnerFaces);
+    }
+    if (nBoundaries > 0) {
+      DBuf<int> st, nf;
+      FCP_HIP(ctx, st.alloc(nBoundaries));
+      FCP_HIP(ctx, nf.alloc(nBoundaries));
+      FCP_HIP(ctx, hipMemcpyAsync(st.p, startFace, sizeof(int) * nBoundaries, hipMemcpyHostToDevice,
+                                  ctx->stream));
+      FCP_HIP(ctx, hipMemcpyAsync(nf.p, nfaces, sizeof(int) * nBoundaries, hipMemcpyHostToDevice,
+                                  ctx->stream));
+      FCP_HIP(ctx, hipMemcpyAsync(ctx->bctype.p, bctype, sizeof(int) * nBoundaries,
+                                  hipMemcpyHostToDevice, ctx->stream));
+      k_bpatch<<<nBoundaries, 256, 0, ctx->stream>>>(st.p, nf.p, nBoundaries, nInnerFaces,
+                                                     ctx->bpatch.p);
+      FCP_HIP(ctx, hipGetLastError());
+      FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    FCP_HIP(ctx, hipGetLastError());
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  FCP_TRY(fcp_build_cell_faces(ctx));
+  ctx->have_mesh = true;
+  return 0;
+}
+
+int fcp_geometry(fcp_ctx *ctx, double *xc, double *yc, double *zc, double *vol, double *arx,
+                 double *ary, double *arz, double *xf, double *yf, double *zf, double *facint,
+                 double *Df, double *dnb, double *srdb) {
+  FCP_TRY(need(ctx, true, false, false));
+  (void)hipSetDevice(ctx->device);
+  FCP_TRY(fcp_compute_geometry(ctx));
+  const size_t n = ctx->n, F = ctx->F, Fi = ctx->Fi, Fb = ctx->Fb;
+  FCP_TRY(out_copy(ctx, ctx->xc.p, xc, n));
+  FCP_TRY(out_copy(ctx, ctx->yc.p, yc, n));
+  FCP_TRY(out_copy(ctx, ctx->zc.p, zc, n));
+  FCP_TRY(out_copy(ctx, ctx->vol.p, vol, n));
+  FCP_TRY(out_copy(ctx, ctx->arx.p, arx, F));
+  FCP_TRY(out_copy(ctx, ctx->ary.p, ary, F));
+  FCP_TRY(out_copy(ctx, ctx->arz.p, arz, F));
+  FCP_TRY(out_copy(ctx, ctx->xf.p, xf, F));
+  FCP_TRY(out_copy(ctx, ctx->yf.p, yf, F));
+  FCP_TRY(out_copy(ctx, ctx->zf.p, zf, F));
+  FCP_TRY(out_copy(ctx, ctx->facint.p, facint, Fi));
+  FCP_TRY(out_copy(ctx, ctx->Df.p, Df, Fi));
+  FCP_TRY(out_copy(ctx, ctx->dnb.p, dnb, Fb));
+  FCP_TRY(out_copy(ctx, ctx->srdb.p, srdb, Fb));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_csr_build(fcp_ctx *ctx, int *nnz, int *ia, int *ja, int *diag, int *slot_pn, int *slot_np) {
+  FCP_TRY(need(ctx, true, false, false));
+  (void)hipSetDevice(ctx->device);
+  FCP_TRY(fcp_build_csr(ctx));
+  if (nnz) *nnz = ctx->nnz;
+  FCP_TRY(out_index_plus1(ctx, ctx->ia.p, ia, (size_t)ctx->n + 1));
+  FCP_TRY(out_index_plus1(ctx, ctx->ja.p, ja, ctx->nnz));
+  FCP_TRY(out_index_plus1(ctx, ctx->diag.p, diag, ctx->n));
+  FCP_TRY(out_index_plus1(ctx, ctx->slot_pn.p, slot_pn, ctx->Fi));
+  FCP_TRY(out_index_plus1(ctx, ctx->slot_np.p, slot_np, ctx->Fi));
+  return 0;
+}
+
+int fcp_levels(fcp_ctx *ctx, int upper, int *lev, int *nlev) {
+  FCP_TRY(need(ctx, true, false, true));
+  const Schedule &s = upper ? ctx->upper : ctx->lower;
+  if (nlev) *nlev = s.nlev;
+  if (lev) {
+    hipMemcpyKind kind = fcp_is_device_ptr(lev) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    FCP_HIP(ctx, hipMemcpyAsync(lev, s.lev.p, sizeof(int) * ctx->n, kind, ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// assembly
+// ---------------------------------------------------------------------------
+int fcp_laplacian(fcp_ctx *ctx, const double *mu, const double *phi, double *a, double *su) {
+  FCP_TRY(need(ctx, true, true, true));
+  if (!mu || !phi || !su) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_laplacian: null argument");
+  (void)hipSetDevice(ctx->device);
+  Arg<double> m, p, s;
+  FCP_TRY(arg_in(ctx, mu, ctx->n, m));
+  FCP_TRY(arg_in(ctx, phi, (size_t)ctx->n + ctx->Fb, p));
+  FCP_TRY(arg_in(ctx, (const double *)su, ctx->n, s));
+  FCP_TRY(fcp_launch_laplacian(ctx, m.dev, nullptr, nullptr, p.dev, s.dev));
+  FCP_TRY(arg_back(ctx, s));
+  FCP_TRY(out_copy(ctx, ctx->a.p, a, ctx->nnz));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_laplacian_face(fcp_ctx *ctx, const double *mu_f, const unsigned char *bc_kind,
+                       const double *phi, double *a, double *su) {
+  FCP_TRY(need(ctx, true, true, true));
+  if (!mu_f || !phi || !su) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_laplacian_face: null argument");
+  (void)hipSetDevice(ctx->device);
+  Arg<double> m, p, s;
+  Arg<unsigned char> bk;
+  FCP_TRY(arg_in(ctx, mu_f, ctx->F, m));
+  FCP_TRY(arg_in(ctx, phi, (size_t)ctx->n + ctx->Fb, p));
+  FCP_TRY(arg_in(ctx, (const double *)su, ctx->n, s));
+  if (bc_kind) FCP_TRY(arg_in(ctx, bc_kind, ctx->Fb, bk));
+  FCP_TRY(fcp_launch_laplacian(ctx, nullptr, m.dev, bk.dev, p.dev, s.dev));
+  FCP_TRY(arg_back(ctx, s));
+  FCP_TRY(out_copy(ctx, ctx->a.p, a, ctx->nnz));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_nonorth_correction(fcp_ctx *ctx, const double *gamma_f, const double *grad, double *su) {
+  FCP_TRY(need(ctx, true, true, false));
+  if (!gamma_f || !grad || !su) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_nonorth_correction: null argument");
+  (void)hipSetDevice(ctx->device);
+  Arg<double> g, gr, s;
+  FCP_TRY(arg_in(ctx, gamma_f, ctx->F, g));
+  FCP_TRY(arg_in(ctx, grad, (size_t)3 * (ctx->n + ctx->Fb), gr));
+  FCP_TRY(arg_in(ctx, (const double *)su, ctx->n, s));
+  FCP_TRY(fcp_launch_nonorth(ctx, g.dev, gr.dev, s.dev));
+  FCP_TRY(arg_back(ctx, s));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi) {
+  FCP_TRY(need(ctx, true, true, false));
+  if (!u || !dudxi) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_grad_gauss: null argument");
+  (void)hipSetDevice(ctx->device);
+  const size_t ntot = (size_t)ctx->n + ctx->Fb;
+  Arg<double> in, out;
+  FCP_TRY(arg_in(ctx, u, ntot, in));
+  FCP_TRY(arg_out(ctx, dudxi, 3 * ntot, out));
+  FCP_TRY(fcp_launch_grad_gauss(ctx, in.dev, out.dev));
+  FCP_TRY(arg_back(ctx, out));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_update_boundary(fcp_ctx *ctx, double *phi) {
+  FCP_TRY(need(ctx, true, false, false));
+  if (!phi) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_update_boundary: null argument");
+  (void)hipSetDevice(ctx->device);
+  Arg<double> p;
+  FCP_TRY(arg_in(ctx, (const double *)phi, (size_t)ctx->n + ctx->Fb, p));
+  FCP_TRY(fcp_launch_update_boundary(ctx, p.dev));
+  FCP_TRY(arg_back(ctx, p));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// linear algebra
+// ---------------------------------------------------------------------------
+int fcp_matrix_set(fcp_ctx *ctx, const double *a) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!a) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_matrix_set: null argument");
+  hipMemcpyKind kind = fcp_is_device_ptr(a) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  FCP_HIP(ctx, hipMemcpyAsync(ctx->a.p, a, sizeof(double) * ctx->nnz, kind, ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->a_version++;
+  return 0;
+}
+
+int fcp_spmv(fcp_ctx *ctx, const double *x, double *y) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!x || !y) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_spmv: null argument");
+  Arg<double> xi, yo;
+  FCP_TRY(arg_in(ctx, x, ctx->n, xi));
+  FCP_TRY(arg_out(ctx, y, ctx->n, yo));
+  FCP_TRY(fcp_launch_spmv64(ctx, ctx->a.p, xi.dev, yo.dev));
+  FCP_TRY(arg_back(ctx, yo));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!r || !z || (kind != 0 && kind != 1)) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_precond_apply: bad argument");
+  (void)hipSetDevice(ctx->device);
+  Arg<double> ri, zo, dout;
+  FCP_TRY(arg_in(ctx, r, ctx->n, ri));
+  FCP_TRY(arg_out(ctx, z, ctx->n, zo));
+  if (d) FCP_TRY(arg_out(ctx, d, ctx->n, dout));
+  FCP_TRY(fcp_precond_device(ctx, kind, ri.dev, zo.dev, dout.dev));
+  FCP_TRY(arg_back(ctx, zo));
+  FCP_TRY(arg_back(ctx, dout));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+static int solver_code(const char *s, int len) {
+  std::string name(s, s + (len > 0 ? len : strlen(s)));
+  while (!name.empty() && name.back() == ' ') name.pop_back();
+  if (name == "iccg") return 0;
+  if (name == "dpcg") return 1;
+  if (name == "bicgstab") return 2;
+  return -1;
+}
+
+int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi, const double *rhs,
+                 double *res0, int itr_max, double tol_abs, double tol_rel, const char *chvar,
+                 int chvar_len, int verbose, fcp_solve_opts *opts) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!solver || !fi || !rhs) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_csrsolve: null argument");
+  const int code = solver_code(solver, solver_len);
+  if (code < 0)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_csrsolve: unknown solver '%.*s'",
+                    solver_len > 0 ? solver_len : (int)strlen(solver), solver);
+  (void)hipSetDevice(ctx->device);
+  Arg<double> x, b;
+  FCP_TRY(arg_in(ctx, (const double *)fi, ctx->n, x));
+  FCP_TRY(arg_in(ctx, rhs, ctx->n, b));
+  fcp_solve_opts local;
+  fcp_solve_opts_default(&local);
+  fcp_solve_opts *o = opts ? opts : &local;
+  const int rc = fcp_solve_device(ctx, code, x.dev, b.dev, itr_max, tol_abs, tol_rel, verbose, o);
+  if (rc < 0) return rc;
+  FCP_TRY(arg_back(ctx, x));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (res0) *res0 = o->resor;
+  if (verbose >= 1) {
+    static const char *names[] = {"PCG(IC0)", "PCG(Jacobi)", "BiCGStab(ILU(0))"};
+    std::string var = chvar ? std::string(chvar, chvar + (chvar_len > 0 ? chvar_len : strlen(chvar)))
+                            : std::string("x");
+    while (!var.empty() && var.back() == ' ') var.pop_back();
+    const double factor = (o->res0 > 0 && o->resor > 0) ? o->res0 / o->resor : 1.0;
+    printf("  %s:  Solving for %s, Initial residual = %10.3E, Final residual = %10.3E, No Iterations %d\n",
+           names[code], var.c_str(), o->resor, o->resl / factor, o->iterations);
+  }
+  return rc;
+}
+
+// Explicit-array entry points on a library-owned context (device 0).
+static std::mutex g_explicit_mu;
+static fcp_ctx *g_explicit_ctx = nullptr;
+
+static int explicit_solve(int code, int n, int nnz, const int *ia, const int *ja, const double *a,
+                          const int *diag, double *fi, const double *rhs, double *resor,
+                          int itr_max, double tol_abs, double tol_rel, int verbose) {
+  std::lock_guard<std::mutex> lk(g_explicit_mu);
+  if (n <= 0 || nnz <= 0 || !ia || !ja || !a || !diag || !fi || !rhs) return FCP_ERR_ARG;
+  if (fcp_is_device_ptr(ia) || fcp_is_device_ptr(ja) || fcp_is_device_ptr(diag)) return FCP_ERR_ARG;
+  if (!csr_ok_host(n, nnz, ia, ja, diag)) return FCP_ERR_ARG;
+  if (!g_explicit_ctx) {
+    int rc = fcp_init(0, &g_explicit_ctx);
+    if (rc < 0) return rc;
+  }
+  fcp_ctx *ctx = g_explicit_ctx;
+  (void)hipSetDevice(ctx->device);
+  ctx->have_mesh = true;
+  ctx->have_csr = false;
+  ctx->n = n;
+  ctx->nnz = nnz;
+  ctx->Fi = 0;
+  ctx->Fb = 0;
+  FCP_HIP(ctx, ctx->ia.alloc(n + 1));
+  FCP_HIP(ctx, ctx->ja.alloc(nnz));
+  FCP_HIP(ctx, ctx->diag.alloc(n));
+  FCP_HIP(ctx, ctx->tslot.alloc(nnz));
+  FCP_HIP(ctx, ctx->row_of.alloc(nnz));
+  FCP_HIP(ctx, ctx->a.alloc(nnz));
+  {
+    Arg<int> i1, j1, d1;
+    FCP_TRY(arg_in(ctx, ia, (size_t)n + 1, i1));
+    FCP_TRY(arg_in(ctx, ja, nnz, j1));
+    FCP_TRY(arg_in(ctx, diag, n, d1));
+    k_sub1<<<grid_for(n + 1, 256), 256, 0, ctx->stream>>>(i1.dev, ctx->ia.p, (size_t)n + 1);
+    k_sub1<<<grid_for(nnz, 256), 256, 0, ctx->stream>>>(j1.dev, ctx->ja.p, nnz);
+    k_sub1<<<grid_for(n, 256), 256, 0, ctx->stream>>>(d1.dev, ctx->diag.p, n);
+    FCP_HIP(ctx, hipGetLastError());
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  FCP_TRY(fcp_build_schedules(ctx));
+  ctx->have_csr = true;
+  FCP_TRY(fcp_matrix_set(ctx, a));
+  // transposed-entry positions for DILU
+  {
+    std::vector<int> hia(n + 1), hja(nnz), ht(nnz);
+    for (int i = 0; i <= n; ++i) hia[i] = ia[i] - 1;
+    for (int k = 0; k < nnz; ++k) hja[k] = ja[k] - 1;
+    for (int i = 0; i < n; ++i)
+      for (int k = hia[i]; k < hia[i + 1]; ++k) {
+        const int j = hja[k];
+        int l = -1;
+        for (int m = hia[j]; m < hia[j + 1]; ++m)
+          if (hja[m] == i) {
+            l = m;
+            break;
+          }
+        ht[k] = l;
+      }
+    FCP_HIP(ctx, hipMemcpyAsync(ctx->tslot.p, ht.data(), sizeof(int) * nnz, hipMemcpyHostToDevice,
+                                ctx->stream));
+  }
+  fcp_solve_opts o;
+  fcp_solve_opts_default(&o);
+  Arg<double> x, b;
+  FCP_TRY(arg_in(ctx, (const double *)fi, n, x));
+  FCP_TRY(arg_in(ctx, rhs, n, b));
+  const int rc = fcp_solve_device(ctx, code, x.dev, b.dev, itr_max, tol_abs, tol_rel, verbose, &o);
+  if (rc < 0) return rc;
+  FCP_TRY(arg_back(ctx, x));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (resor) *resor = o.resor;
+  return rc;
+}
+
+int fcp_iccg(int n, int nnz, const int *ia, const int *ja, const double *a, const int *diag,
+             double *fi, const double *rhs, double *resor, int itr_max, double tol_abs,
+             double tol_rel, int verbose) {
+  return explicit_solve(0, n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, tol_abs, tol_rel, verbose);
+}
+int fcp_dpcg(int n, int nnz, const int *ia, const int *ja, const double *a, const int *diag,
+             double *fi, const double *rhs, double *resor, int itr_max, double tol_abs,
+             double tol_rel, int verbose) {
+  return explicit_solve(1, n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, tol_abs, tol_rel, verbose);
+}
+int fcp_bicgstab(int n, int nnz, const int *ia, const int *ja, const double *a, const int *diag,
+                 double *fi, const double *rhs, double *resor, int itr_max, double tol_abs,
+                 double tol_rel, int verbose) {
+  return explicit_solve(2, n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, tol_abs, tol_rel, verbose);
+}
+
+}  // extern "C"

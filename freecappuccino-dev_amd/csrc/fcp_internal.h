@@ -1,0 +1,189 @@
+// fcp_internal.h -- context, device buffers and kernel entry points of libfcp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/fcp.h"
+
+// ---------------------------------------------------------------------------
+// Reference literals (parameters.f90:6, linear_solvers.f90:276,447): default
+// REAL literals widened to REAL(8).
+// ---------------------------------------------------------------------------
+#define FCP_SMALL ((double)1e-20f)
+#define FCP_S0INIT ((double)1e20f)
+
+// Sweep outputs are pre-filled with a signalling-NaN pattern that arithmetic
+// never produces; a consumer spins until the producer overwrote it.
+#define FCP_SENT64 0x7FF4DEAD5EED0001ull
+#define FCP_SENT32 0x7FA5EED1u
+
+// Bounded spin of a sweep wait (iterations of s_sleep(1) ~ 64 cycles each):
+// ~0.5 s, after which the wave records FCP_ERR_SWEEP and moves on.
+#define FCP_SPIN_LIMIT (1 << 23)
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(&p, count * sizeof(T));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DBuf() { release(); }
+};
+
+// Level schedule of one triangular sweep: rows in processing order grouped
+// into tasks of <= 64 rows that share a level (one wavefront per task).
+struct Schedule {
+  int nlev = 0, ntasks = 0;
+  DBuf<int> order;       // [n] rows in level order
+  DBuf<int> task_ptr;    // [ntasks+1] offsets into order
+  DBuf<int> lev;         // [n] level of each row
+};
+
+// Device-resident scalars of one Krylov solve (single RHS).
+struct SolveScalars {
+  double sk, s0, pkapk, alf, bet, res0, resl, factor, resor, rsm;
+  double l2, l2b, om, gam, beto, ukreso, vv, vr;
+  int iter, done, status, pad;
+};
+
+struct ProfEntry {
+  int launches = 0;
+  double total_ms = 0.0;
+};
+
+struct fcp_ctx {
+  int device = 0;
+  int ncu = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // -------- mesh (0-based, resident) --------
+  bool have_mesh = false;
+  int n = 0, F = 0, Fi = 0, Fb = 0, nNodes = 0, nPatches = 0;
+  std::vector<int> h_nfaces, h_start, h_bctype;
+  DBuf<double> points;  // [3*nNodes]
+  DBuf<int> nnodes;     // [F]
+  DBuf<int> node;       // [NOMAX*F]
+  DBuf<int> owner;      // [F]
+  DBuf<int> neighbour;  // [Fi]
+  DBuf<int> bpatch;     // [Fb] patch of every boundary face
+  DBuf<int> bctype;     // [nPatches]
+  DBuf<int> cf_ptr;     // [n+1] cell -> faces (ascending face id)
+  DBuf<int> cf_face;    // [2*Fi+Fb]
+
+  // -------- geometry --------
+  bool have_geom = false;
+  DBuf<double> xc, yc, zc, vol;
+  DBuf<double> arx, ary, arz, xf, yf, zf;
+  DBuf<double> facint, Df, dnb, srdb;
+
+  // -------- CSR --------
+  bool have_csr = false;
+  int nnz = 0;
+  DBuf<int> ia, ja, diag, slot_pn, slot_np;
+  DBuf<int> row_of;     // [nnz] row of every entry
+  Schedule lower, upper;
+  DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)
+
+  // -------- operator values --------
+  DBuf<double> a;       // [nnz] resident matrix (fp64)
+  DBuf<float> a32;      // [nnz] fp32 shadow
+  uint64_t a_version = 0, a32_version = ~0ull, dic_version = ~0ull, d32_version = ~0ull;
+  int dic_kind = -1;    // 0 DIC, 1 DILU
+
+  // -------- solver workspace --------
+  DBuf<double> d, res, pk, zk, yk, uk, vk, reso, x, rhs, qk;
+  DBuf<float> d32, r32, p32, z32, y32, q32, e32;
+  DBuf<double> partials;     // per-block partial sums
+  DBuf<int> counters;        // [16] block-completion counters / tickets
+  DBuf<SolveScalars> scal;   // device scalars
+  SolveScalars *h_scal = nullptr;  // pinned mirror
+  DBuf<int> errflag;         // [1]
+
+  // -------- staging --------
+  DBuf<char> stage[4];
+
+  // -------- profiling --------
+  bool prof = false;
+  std::map<std::string, ProfEntry> prof_tab;
+  struct PendingEv {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<PendingEv> prof_pending;
+  std::vector<hipEvent_t> ev_pool;
+};
+
+// ---------------------------------------------------------------------------
+// helpers (fcp_api.hip)
+// ---------------------------------------------------------------------------
+int fcp_fail(fcp_ctx *ctx, int code, const char *fmt, ...);
+#define FCP_HIP(ctx, x)                                                          \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess)                                                        \
+      return fcp_fail(ctx, FCP_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #x, \
+                      hipGetErrorString(e_));                                    \
+  } while (0)
+#define FCP_TRY(x)            \
+  do {                        \
+    int rc_ = (x);            \
+    if (rc_ < 0) return rc_;  \
+  } while (0)
+
+bool fcp_is_device_ptr(const void *p);
+
+// Profiling brackets around a launch on ctx->stream.
+void fcp_prof_begin(fcp_ctx *ctx, const char *name, hipEvent_t *ev);
+void fcp_prof_end(fcp_ctx *ctx, const char *name, hipEvent_t ev);
+struct ProfScope {
+  fcp_ctx *ctx;
+  const char *name;
+  hipEvent_t ev = nullptr;
+  ProfScope(fcp_ctx *c, const char *nm) : ctx(c), name(nm) { fcp_prof_begin(ctx, name, &ev); }
+  ~ProfScope() { fcp_prof_end(ctx, name, ev); }
+};
+
+inline int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
+
+// ---------------------------------------------------------------------------
+// module entry points (host-side launchers)
+// ---------------------------------------------------------------------------
+// fcp_mesh.hip
+int fcp_build_cell_faces(fcp_ctx *ctx);
+int fcp_compute_geometry(fcp_ctx *ctx);
+int fcp_build_csr(fcp_ctx *ctx);
+int fcp_build_schedules(fcp_ctx *ctx);
+// fcp_assemble.hip
+int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
+                         const unsigned char *bc_kind, const double *phi, double *su);
+int fcp_launch_nonorth(fcp_ctx *ctx, const double *gamma_f, const double *grad, double *su);
+int fcp_launch_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi);
+int fcp_launch_update_boundary(fcp_ctx *ctx, double *phi);
+// fcp_krylov.hip
+int fcp_launch_spmv64(fcp_ctx *ctx, const double *a, const double *x, double *y);
+int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max,
+                     double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts);
+int fcp_ensure_workspace(fcp_ctx *ctx);
+int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
+
+// rocPRIM-based primitives (fcp_prim.hip)
+int fcp_exclusive_scan_int(fcp_ctx *ctx, const int *in, int *out, int n);
+int fcp_sort_pairs_int(fcp_ctx *ctx, const int *keys_in, int *keys_out,
+                       const int *vals_in, int *vals_out, int n, int max_key);

@@ -1,0 +1,939 @@
+// fcp_krylov.hip -- sparse kernels and Krylov solvers of libfcp.
+//
+// * SpMV: one thread per row in CSR order (same summation order as the
+//   reference's row loop, linear_solvers.f90:493-498) or LANES lanes per row
+//   with a shuffle reduction.
+// * Reductions: two-stage -- every block writes a partial (FP64), the last
+//   block to finish (atomic ticket) sums the partials in index order, so every
+//   run produces the same bits.  The last block also performs the scalar part
+//   of the Krylov step (alpha, beta, convergence test) on the device.
+// * DIC/DILU setup and triangular sweeps: rows grouped by level of the
+//   strict-lower (upper) DAG into tasks of <= 64 rows; persistent waves take
+//   tasks in level order from an atomic ticket and wait for their
+//   dependencies through the output array itself, pre-filled with a NaN
+//   sentinel (one agent-scope store per row, no per-level launch).  A wave
+//   only waits for rows of tasks with smaller tickets, which are owned by
+//   waves already running, so the sweep always drains; every wait is bounded.
+//   Each row is computed with the reference's operation order (:439-475),
+//   so sweeps are bit-equal to the CPU loops.
+// * PCG (DIC or Jacobi) and DILU-BiCGStab follow linear_solvers.f90:206-786
+//   step by step; scalars stay on the device, the host only polls a status
+//   word every `check_every` iterations.
+// * Mixed precision: FP32 matrix/preconditioner/vectors inside an FP64
+//   iterative-refinement loop (true residual, update and norms in FP64).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "fcp_internal.h"
+
+namespace {
+
+constexpr int BLK = 256;
+constexpr int MAXBLK = 1024;  // blocks of a reduction kernel (partials per slot)
+
+enum Slot { SLOT_A = 0, SLOT_B = 1, SLOT_C = 2, NSLOT = 4 };
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum of up to 3 values into lane 0 of warp 0 (deterministic order).
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double (&out)[NV]) {
+  __shared__ double sh[NV][BLK / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = wave_sum(v[i]);
+    if (lane == 0) sh[i][w] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    if (threadIdx.x == 0)
+      for (int k = 0; k < BLK / 64; ++k) s += sh[i][k];
+    out[i] = s;
+  }
+}
+
+// Publish NV block partials; returns true in the single last-arriving block,
+// whose thread 0 then holds the totals (summed in block order) in tot[].
+template <int NV>
+__device__ bool last_block_reduce(double (&part)[NV], double *__restrict__ partials, int slot,
+                                  int *__restrict__ counter, double (&tot)[NV]) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) partials[(size_t)(slot + i) * MAXBLK + blockIdx.x] = part[i];
+    __threadfence();
+    const int prev = atomicAdd(counter, 1);
+    is_last = (prev == (int)gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  // all threads of the last block reduce the partials in fixed order
+  double loc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLK)
+      s += __hip_atomic_load(&partials[(size_t)(slot + i) * MAXBLK + b], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    loc[i] = s;
+  }
+  block_sum<NV>(loc, tot);
+  if (threadIdx.x == 0) *counter = 0;
+  return true;
+}
+
+__device__ __forceinline__ bool solve_done(const SolveScalars *s) {
+  return __hip_atomic_load(&s->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// ---------------------------------------------------------------------------
+// sentinel wait/publish
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Bits;
+template <>
+struct Bits<double> {
+  using U = unsigned long long;
+  static constexpr U sent = FCP_SENT64;
+  static __device__ __forceinline__ U to(double v) { return __double_as_longlong(v); }
+  static __device__ __forceinline__ double from(U u) { return __longlong_as_double(u); }
+  static constexpr U qnan = 0x7FF8000000000000ull;
+};
+template <>
+struct Bits<float> {
+  using U = unsigned int;
+  static constexpr U sent = FCP_SENT32;
+  static __device__ __forceinline__ U to(float v) { return __float_as_uint(v); }
+  static __device__ __forceinline__ float from(U u) { return __uint_as_float(u); }
+  static constexpr U qnan = 0x7FC00000u;
+};
+
+template <typename T>
+__device__ __forceinline__ T wait_value(T *p, int *err) {
+  using B = Bits<T>;
+  typename B::U *q = reinterpret_cast<typename B::U *>(p);
+  typename B::U v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int spins = 0;
+  while (v == B::sent) {
+    __builtin_amdgcn_s_sleep(1);
+    v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (++spins > FCP_SPIN_LIMIT) {
+      __hip_atomic_store(err, FCP_ERR_SWEEP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      v = B::qnan;
+    }
+  }
+  return B::from(v);
+}
+
+template <typename T>
+__device__ __forceinline__ void publish(T *p, T v) {
+  using B = Bits<T>;
+  typename B::U u = B::to(v);
+  if (u == B::sent) u = B::qnan;
+  __hip_atomic_store(reinterpret_cast<typename B::U *>(p), u, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__global__ void k_fill_sent(T *__restrict__ x, T *__restrict__ y, int n, int *__restrict__ tickets,
+                            const SolveScalars *__restrict__ s) {
+  if (s && solve_done(s)) return;
+  using B = Bits<T>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && tickets) {
+    tickets[0] = 0;
+    tickets[1] = 0;
+  }
+  for (int k = i; k < n; k += gridDim.x * blockDim.x) {
+    reinterpret_cast<typename B::U *>(x)[k] = B::sent;
+    if (y) reinterpret_cast<typename B::U *>(y)[k] = B::sent;
+  }
+}
+
+// MODE 0: forward  y_i = (r_i - sum_{k<diag} a_k y_j) d_i            (:458-464)
+// MODE 1: backward z_i = (y_i/(d_i+small) - sum_{k>diag} a_k z_j) d_i   (:466-475)
+// MODE 2: DIC setup  d_i = 1/(a_ii - sum_{k<diag} a_k^2 d_j)          (:439-445)
+// MODE 3: DILU setup d_i = 1/(a_ii - sum_{k<diag} a_k d_j a_kT)       (:613-624)
+template <typename T, int MODE>
+__global__ __launch_bounds__(BLK) void k_sweep(
+    const int *__restrict__ task_ptr, const int *__restrict__ order, int ntasks,
+    const int *__restrict__ ia, const int *__restrict__ diag, const int *__restrict__ ja,
+    const T *__restrict__ a, const int *__restrict__ tslot, const T *__restrict__ d,
+    const T *__restrict__ in, T *out, int *__restrict__ ticket, int *__restrict__ err,
+    const SolveScalars *__restrict__ s) {
+  if (s && solve_done(s)) return;
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(ticket, 1);
+    t = __shfl(t, 0, 64);
+    if (t >= ntasks) break;
+    const int idx = task_ptr[t] + lane;
+    if (idx < task_ptr[t + 1]) {
+      const int row = order[idx];
+      const int dg = diag[row];
+      if (MODE == 0) {
+        T acc = in[row];
+        for (int k = ia[row]; k < dg; ++k) acc = acc - a[k] * wait_value<T>(out + ja[k], err);
+        publish<T>(out + row, acc * d[row]);
+      } else if (MODE == 1) {
+        const T dr = d[row];
+        T acc = in[row] / (dr + (T)FCP_SMALL);
+        const int e = ia[row + 1];
+        for (int k = dg + 1; k < e; ++k) acc = acc - a[k] * wait_value<T>(out + ja[k], err);
+        publish<T>(out + row, acc * dr);
+      } else if (MODE == 2) {
+        T acc = a[dg];
+        for (int k = ia[row]; k < dg; ++k) {
+          const T ak = a[k];
+          acc = acc - ak * ak * wait_value<T>(out + ja[k], err);
+        }
+        publish<T>(out + row, (T)1.0 / acc);
+      } else {
+        T acc = a[dg];
+        for (int k = ia[row]; k < dg; ++k) {
+          const int l = tslot[k];
+          const T dj = wait_value<T>(out + ja[k], err);
+          acc = acc - a[k] * dj * (l >= 0 ? a[l] : (T)0);
+        }
+        publish<T>(out + row, (T)1.0 / acc);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SpMV
+// ---------------------------------------------------------------------------
+// y = A x, one thread per row, CSR order (bit-equal to the reference loop).
+template <typename T>
+__global__ void k_spmv_row(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                           const T *__restrict__ a, const T *__restrict__ x, T *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T s = 0;
+  for (int k = ia[i]; k < ia[i + 1]; ++k) s = s + a[k] * x[ja[k]];
+  y[i] = s;
+}
+
+// res = b - A x (row order as :417-422) and the L1 / squared L2 norms of res
+// and the squared L2 norm of b.  The last block stores res0, l2, l2b and
+// initialises the Krylov scalars.
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_residual_init(
+    int n, const int *__restrict__ ia, const int *__restrict__ ja, const T *__restrict__ a,
+    const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ res, T *__restrict__ reso,
+    double *__restrict__ partials, int *__restrict__ counter, SolveScalars *__restrict__ s) {
+  double part[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    T r = b[i];
+    for (int k = ia[i]; k < ia[i + 1]; ++k) r = r - a[k] * x[ja[k]];
+    res[i] = r;
+    if (reso) reso[i] = r;
+    part[0] += fabs((double)r);
+    part[1] += (double)r * (double)r;
+    part[2] += (double)b[i] * (double)b[i];
+  }
+  double blk[3], tot[3];
+  block_sum<3>(part, blk);
+  if (last_block_reduce<3>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+    s->res0 = tot[0];
+    s->resl = tot[0];
+    s->l2 = sqrt(tot[1]);
+    s->l2b = sqrt(tot[2]);
+    s->s0 = FCP_S0INIT;
+    s->alf = 1.0;
+    s->beto = 1.0;
+    s->gam = 1.0;
+    s->iter = 0;
+    s->done = 0;
+    s->status = 0;
+    s->factor = 0.0;
+    s->resor = tot[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PCG step kernels (iccg :454-537, dpcg :282-352)
+// ---------------------------------------------------------------------------
+// sk = r.z ; bet = sk/s0
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_dot_rz(int n, const T *__restrict__ r,
+                                                const T *__restrict__ z,
+                                                double *__restrict__ partials,
+                                                int *__restrict__ counter,
+                                                SolveScalars *__restrict__ s) {
+  if (solve_done(s)) return;
+  double part[1] = {0.0};
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK)
+    part[0] += (double)r[i] * (double)z[i];
+  double blk[1], tot[1];
+  block_sum<1>(part, blk);
+  if (last_block_reduce<1>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+    s->sk = tot[0];
+    s->bet = tot[0] / s->s0;
+  }
+}
+
+// Jacobi preconditioner z = r / a_ii fused with sk = r.z (dpcg :288-293)
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_jacobi_dot(int n, const int *__restrict__ diag,
+                                                    const T *__restrict__ a,
+                                                    const T *__restrict__ r, T *__restrict__ z,
+                                                    double *__restrict__ partials,
+                                                    int *__restrict__ counter,
+                                                    SolveScalars *__restrict__ s) {
+  if (solve_done(s)) return;
+  double part[1] = {0.0};
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    const T zi = r[i] / a[diag[i]];
+    z[i] = zi;
+    part[0] += (double)r[i] * (double)zi;
+  }
+  double blk[1], tot[1];
+  block_sum<1>(part, blk);
+  if (last_block_reduce<1>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+    s->sk = tot[0];
+    s->bet = tot[0] / s->s0;
+  }
+}
+
+// p = z + bet*p
+template <typename T>
+__global__ void k_pupdate(int n, const T *__restrict__ z, T *__restrict__ p,
+                          const SolveScalars *__restrict__ s) {
+  if (solve_done(s)) return;
+  const T bet = (T)s->bet;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) p[i] = z[i] + bet * p[i];
+}
+
+// q = A p ; pkapk = p.q ; alf = sk/pkapk
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_spmv_pq(int n, const int *__restrict__ ia,
+                                                 const int *__restrict__ ja,
+                                                 const T *__restrict__ a, const T *__restrict__ p,
+                                                 T *__restrict__ q, double *__restrict__ partials,
+                                                 int *__restrict__ counter,
+                                                 SolveScalars *__restrict__ s) {
+  if (solve_done(s)) return;
+  double part[1] = {0.0};
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    T acc = 0;
+    for (int k = ia[i]; k < ia[i + 1]; ++k) acc = acc + a[k] * p[ja[k]];
+    q[i] = acc;
+    part[0] += (double)p[i] * (double)acc;
+  }
+  double blk[1], tot[1];
+  block_sum<1>(part, blk);
+  if (last_block_reduce<1>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+    s->pkapk = tot[0];
+    s->alf = s->sk / tot[0];
+  }
+}
+
+// x += alf p ; r -= alf q ; resl = |r|_1 ; |r|_2 ; iteration-1 factor;
+// convergence test (crit 0: reference L1, crit 1: relative L2 vs tol_rel).
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_xr_update(
+    int n, const int *__restrict__ diag, const T *__restrict__ a, T *__restrict__ x,
+    T *__restrict__ r, const T *__restrict__ p, const T *__restrict__ q,
+    double *__restrict__ partials, int *__restrict__ counter, SolveScalars *__restrict__ s,
+    int crit, double tol_abs, double tol_rel, int itr_max) {
+  if (solve_done(s)) return;
+  const T alf = (T)s->alf;
+  const bool first = (s->iter == 0);
+  double part[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    const T xi = x[i] + alf * p[i];
+    x[i] = xi;
+    const T ri = r[i] - alf * q[i];
+    r[i] = ri;
+    part[0] += fabs((double)ri);
+    part[1] += (double)ri * (double)ri;
+    if (first) part[2] += fabs((double)a[diag[i]] * (double)xi);
+  }
+  double blk[3], tot[3];
+  block_sum<3>(part, blk);
+  if (last_block_reduce<3>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+    const double resl = tot[0];
+    s->resl = resl;
+    s->l2 = sqrt(tot[1]);
+    s->s0 = s->sk;
+    s->iter += 1;
+    if (first) {
+      s->factor = tot[2] + FCP_SMALL;
+      s->resor = s->res0 / s->factor;
+    }
+    const double rsm = resl / (s->res0 + FCP_SMALL);
+    s->rsm = rsm;
+    bool conv;
+    if (crit == 0)
+      conv = (rsm < tol_rel) || (resl < tol_abs);
+    else
+      conv = (s->l2 <= tol_rel * s->l2b) || (resl < tol_abs);
+    if (!isfinite(resl)) {
+      s->status = FCP_ERR_NONFINITE;
+      conv = true;
+    }
+    if (conv || s->iter >= itr_max) s->done = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab step kernels (bicgstab :642-773)
+// ---------------------------------------------------------------------------
+// bet = r.reso ; om = bet*gam/(alf*beto+small) ; beto = bet
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_bicg_bet(int n, const T *__restrict__ r,
+                                                  const T *__restrict__ reso,
+                                                  double *__restrict__ partials,
+                                                  int *__restrict__ counter,
+                                                  SolveScalars *__restrict__ s) {
+  if (solve_done(s)) return;
+  double part[1] = {0.0};
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK)
+    part[0] += (double)r[i] * (double)reso[i];
+  double blk[1], tot[1];
+  block_sum<1>(part, blk);
+  if (last_block_reduce<1>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+    const double bet = tot[0];
+    s->sk = bet;
+    s->om = bet * s->gam / (s->alf * s->beto + FCP_SMALL);
+    s->beto = bet;
+  }
+}
+
+// p = r + om*(p - alf*u)
+template <typename T>
+__global__ void k_bicg_p(int n, const T *__restrict__ r, const T *__restrict__ u,
+                         T *__restrict__ p, const SolveScalars *__restrict__ s) {
+  if (solve_done(s)) return;
+  const T om = (T)s->om, alf = (T)s->alf;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK)
+    p[i] = r[i] + om * (p[i] - alf * u[i]);
+}
+
+// u = A z ; ukreso = u.reso ; gam = bet/ukreso
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_bicg_u(int n, const int *__restrict__ ia,
+                                                const int *__restrict__ ja,
+                                                const T *__restrict__ a, const T *__restrict__ z,
+                                                const T *__restrict__ reso, T *__restrict__ u,
+                                                double *__restrict__ partials,
+                                                int *__restrict__ counter,
+                                                SolveScalars *__restrict__ s) {
+  if (solve_done(s)) return;
+  double part[1] = {0.0};
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    T acc = 0;
+    for (int k = ia[i]; k < ia[i + 1]; ++k) acc = acc + a[k] * z[ja[k]];
+    u[i] = acc;
+    part[0] += (double)acc * (double)reso[i];
+  }
+  double blk[1], tot[1];
+  block_sum<1>(part, blk);
+  if (last_block_reduce<1>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+    s->ukreso = tot[0];
+    s->gam = s->sk / tot[0];
+  }
+}
+
+// x += gam z ; r -= gam u
+template <typename T>
+__global__ void k_bicg_xr1(int n, const T *__restrict__ z, const T *__restrict__ u,
+                           T *__restrict__ x, T *__restrict__ r,
+                           const SolveScalars *__restrict__ s) {
+  if (solve_done(s)) return;
+  const T gam = (T)s->gam;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    x[i] = x[i] + gam * z[i];
+    r[i] = r[i] - gam * u[i];
+  }
+}
+
+// v = A z ; alf = (v.r)/(v.v + small)
+template <typename T>
+__global__ __launch_bounds__(BLK) void k_bicg_v(int n, const int *__restrict__ ia,
+                                                const int *__restrict__ ja,
+                                                const T *__restrict__ a, const T *__restrict__ z,
+                                                const T *__restrict__ r, T *__restrict__ v,
+                                                double *__restrict__ partials,
+                                                int *__restrict__ counter,
+                                                SolveScalars *__restrict__ s) {
+  if (solve_done(s)) return;
+  double part[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    T acc = 0;
+    for (int k = ia[i]; k < ia[i + 1]; ++k) acc = acc + a[k] * z[ja[k]];
+    v[i] = acc;
+    part[0] += (double)acc * (double)r[i];
+    part[1] += (double)acc * (double)acc;
+  }
+  double blk[2], tot[2];
+  block_sum<2>(part, blk);
+  if (last_block_reduce<2>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+    s->vr = tot[0];
+    s->vv = tot[1];
+    s->alf = tot[0] / (tot[1] + FCP_SMALL);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// casts and refinement updates
+// ---------------------------------------------------------------------------
+__global__ void k_cast_a(int nnz, const double *__restrict__ a, float *__restrict__ a32) {
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < nnz; i += gridDim.x * BLK) a32[i] = (float)a[i];
+}
+
+// r32 = r64 * scale, e32 = 0
+__global__ void k_scale_to32(int n, const double *__restrict__ r, const SolveScalars *__restrict__ so,
+                             float *__restrict__ r32, float *__restrict__ e32) {
+  const double sc = 1.0 / so->l2;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    r32[i] = (float)(r[i] * sc);
+    e32[i] = 0.0f;
+  }
+}
+
+// x64 += l2 * e32  (FP64 update)
+__global__ void k_ir_update(int n, const float *__restrict__ e32, const SolveScalars *__restrict__ so,
+                            double *__restrict__ x) {
+  const double sc = so->l2;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK)
+    x[i] = x[i] + sc * (double)e32[i];
+}
+
+__global__ void k_zero_f32(int n, float *__restrict__ a, float *__restrict__ b) {
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    a[i] = 0.0f;
+    if (b) b[i] = 0.0f;
+  }
+}
+__global__ void k_zero_f64(int n, double *__restrict__ a, double *__restrict__ b,
+                           double *__restrict__ c) {
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    a[i] = 0.0;
+    if (b) b[i] = 0.0;
+    if (c) c[i] = 0.0;
+  }
+}
+
+// Copy of the FP32 inner-solve scalars' initial values: r32 given, x=0.
+__global__ __launch_bounds__(BLK) void k_init32(int n, const float *__restrict__ r,
+                                                double *__restrict__ partials,
+                                                int *__restrict__ counter,
+                                                SolveScalars *__restrict__ s) {
+  double part[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    part[0] += fabs((double)r[i]);
+    part[1] += (double)r[i] * (double)r[i];
+  }
+  double blk[2], tot[2];
+  block_sum<2>(part, blk);
+  if (last_block_reduce<2>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+    s->res0 = tot[0];
+    s->resl = tot[0];
+    s->l2 = sqrt(tot[1]);
+    s->l2b = sqrt(tot[1]);
+    s->s0 = FCP_S0INIT;
+    s->iter = 0;
+    s->done = 0;
+    s->status = 0;
+    s->factor = 0.0;
+  }
+}
+
+int red_grid(fcp_ctx *ctx, int n) {
+  int g = grid_for(n, BLK);
+  int cap = ctx->ncu > 0 ? ctx->ncu * 4 : 1024;
+  if (cap > MAXBLK) cap = MAXBLK;
+  return g < cap ? (g > 0 ? g : 1) : cap;
+}
+
+int sweep_grid(fcp_ctx *ctx, const Schedule &s) {
+  const int waves = (s.ntasks + 0);
+  int blocks = (waves + (BLK / 64) - 1) / (BLK / 64);
+  int cap = (ctx->ncu > 0 ? ctx->ncu : 256) * 8;
+  if (blocks > cap) blocks = cap;
+  return blocks > 0 ? blocks : 1;
+}
+
+}  // namespace
+
+// ===========================================================================
+// host side
+// ===========================================================================
+int fcp_ensure_workspace(fcp_ctx *ctx) {
+  const int n = ctx->n;
+  DBuf<double> *v64[] = {&ctx->d, &ctx->res, &ctx->pk, &ctx->zk, &ctx->yk, &ctx->uk, &ctx->vk,
+                         &ctx->reso, &ctx->x, &ctx->rhs, &ctx->qk};
+  for (auto *b : v64) FCP_HIP(ctx, b->alloc(n));
+  DBuf<float> *v32[] = {&ctx->d32, &ctx->r32, &ctx->p32, &ctx->z32, &ctx->y32, &ctx->q32, &ctx->e32};
+  for (auto *b : v32) FCP_HIP(ctx, b->alloc(n));
+  FCP_HIP(ctx, ctx->partials.alloc((size_t)MAXBLK * 4 * NSLOT));
+  FCP_HIP(ctx, ctx->counters.alloc(16));
+  FCP_HIP(ctx, ctx->scal.alloc(4));
+  FCP_HIP(ctx, ctx->errflag.alloc(1));
+  if (!ctx->h_scal) FCP_HIP(ctx, hipHostMalloc((void **)&ctx->h_scal, 4 * sizeof(SolveScalars)));
+  return 0;
+}
+
+int fcp_launch_spmv64(fcp_ctx *ctx, const double *a, const double *x, double *y) {
+  ProfScope ps(ctx, "spmv_row_f64");
+  k_spmv_row<double><<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(ctx->n, ctx->ia.p, ctx->ja.p,
+                                                                      a, x, y);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+namespace {
+
+// counters layout: [0..1] sweep tickets, [2] reduction counter
+int *tickets(fcp_ctx *ctx) { return ctx->counters.p; }
+int *redctr(fcp_ctx *ctx) { return ctx->counters.p + 2; }
+
+int check_err(fcp_ctx *ctx) {
+  int e = 0;
+  FCP_HIP(ctx, hipMemcpyAsync(&e, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (e) return fcp_fail(ctx, e, "sweep dependency wait exceeded its bound");
+  return 0;
+}
+
+// Diagonal of the DIC (kind 0) or DILU (kind 1) preconditioner in FP64,
+// recomputed when the matrix changed.
+template <typename T>
+int precond_setup(fcp_ctx *ctx, int kind, const T *a, T *d) {
+  const int n = ctx->n;
+  k_fill_sent<T><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(d, (T *)nullptr, n, tickets(ctx), nullptr);
+  ProfScope ps(ctx, kind ? "dilu_setup" : "dic_setup");
+  if (kind == 0)
+    k_sweep<T, 2><<<sweep_grid(ctx, ctx->lower), BLK, 0, ctx->stream>>>(
+        ctx->lower.task_ptr.p, ctx->lower.order.p, ctx->lower.ntasks, ctx->ia.p, ctx->diag.p,
+        ctx->ja.p, a, ctx->tslot.p, nullptr, nullptr, d, tickets(ctx), ctx->errflag.p, nullptr);
+  else
+    k_sweep<T, 3><<<sweep_grid(ctx, ctx->lower), BLK, 0, ctx->stream>>>(
+        ctx->lower.task_ptr.p, ctx->lower.order.p, ctx->lower.ntasks, ctx->ia.p, ctx->diag.p,
+        ctx->ja.p, a, ctx->tslot.p, nullptr, nullptr, d, tickets(ctx), ctx->errflag.p, nullptr);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+// z = M^{-1} r (forward into y, backward into z), both sentinel-filled first.
+template <typename T>
+int precond_apply(fcp_ctx *ctx, const T *a, const T *d, const T *r, T *y, T *z,
+                  const SolveScalars *s) {
+  const int n = ctx->n;
+  k_fill_sent<T><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(y, z, n, tickets(ctx), s);
+  {
+    ProfScope ps(ctx, sizeof(T) == 8 ? "sweep_fwd_f64" : "sweep_fwd_f32");
+    k_sweep<T, 0><<<sweep_grid(ctx, ctx->lower), BLK, 0, ctx->stream>>>(
+        ctx->lower.task_ptr.p, ctx->lower.order.p, ctx->lower.ntasks, ctx->ia.p, ctx->diag.p,
+        ctx->ja.p, a, nullptr, d, r, y, tickets(ctx), ctx->errflag.p, s);
+  }
+  {
+    ProfScope ps(ctx, sizeof(T) == 8 ? "sweep_bwd_f64" : "sweep_bwd_f32");
+    k_sweep<T, 1><<<sweep_grid(ctx, ctx->upper), BLK, 0, ctx->stream>>>(
+        ctx->upper.task_ptr.p, ctx->upper.order.p, ctx->upper.ntasks, ctx->ia.p, ctx->diag.p,
+        ctx->ja.p, a, nullptr, d, y, z, tickets(ctx) + 1, ctx->errflag.p, s);
+  }
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+struct Vecs {
+  void *x, *r, *p, *z, *y, *q, *u, *v, *reso;
+};
+
+// One PCG iteration (precond: 0 DIC, 1 Jacobi).
+template <typename T>
+int pcg_iteration(fcp_ctx *ctx, int precond, const T *a, const T *d, T *x, T *r, T *p, T *z,
+                  T *y, T *q, SolveScalars *s, int crit, double tol_abs, double tol_rel,
+                  int itr_max) {
+  const int n = ctx->n;
+  const int g = red_grid(ctx, n);
+  if (precond == 0) {
+    FCP_TRY(precond_apply<T>(ctx, a, d, r, y, z, s));
+    ProfScope ps(ctx, "dot_rz");
+    k_dot_rz<T><<<g, BLK, 0, ctx->stream>>>(n, r, z, ctx->partials.p, redctr(ctx), s);
+  } else {
+    ProfScope ps(ctx, "jacobi_dot");
+    k_jacobi_dot<T><<<g, BLK, 0, ctx->stream>>>(n, ctx->diag.p, a, r, z, ctx->partials.p,
+                                                 redctr(ctx), s);
+  }
+  {
+    ProfScope ps(ctx, "pupdate");
+    k_pupdate<T><<<g, BLK, 0, ctx->stream>>>(n, z, p, s);
+  }
+  {
+    ProfScope ps(ctx, sizeof(T) == 8 ? "spmv_pq_f64" : "spmv_pq_f32");
+    k_spmv_pq<T><<<g, BLK, 0, ctx->stream>>>(n, ctx->ia.p, ctx->ja.p, a, p, q, ctx->partials.p,
+                                              redctr(ctx), s);
+  }
+  {
+    ProfScope ps(ctx, "xr_update");
+    k_xr_update<T><<<g, BLK, 0, ctx->stream>>>(n, ctx->diag.p, a, x, r, p, q, ctx->partials.p,
+                                                redctr(ctx), s, crit, tol_abs, tol_rel, itr_max);
+  }
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int bicgstab_iteration(fcp_ctx *ctx, const T *a, const T *d, T *x, T *r, T *reso, T *p, T *z,
+                       T *y, T *u, T *v, SolveScalars *s, int crit, double tol_abs,
+                       double tol_rel, int itr_max) {
+  const int n = ctx->n;
+  const int g = red_grid(ctx, n);
+  k_bicg_bet<T><<<g, BLK, 0, ctx->stream>>>(n, r, reso, ctx->partials.p, redctr(ctx), s);
+  k_bicg_p<T><<<g, BLK, 0, ctx->stream>>>(n, r, u, p, s);
+  FCP_TRY(precond_apply<T>(ctx, a, d, p, y, z, s));
+  k_bicg_u<T><<<g, BLK, 0, ctx->stream>>>(n, ctx->ia.p, ctx->ja.p, a, z, reso, u, ctx->partials.p,
+                                           redctr(ctx), s);
+  k_bicg_xr1<T><<<g, BLK, 0, ctx->stream>>>(n, z, u, x, r, s);
+  FCP_TRY(precond_apply<T>(ctx, a, d, r, y, z, s));
+  k_bicg_v<T><<<g, BLK, 0, ctx->stream>>>(n, ctx->ia.p, ctx->ja.p, a, z, r, v, ctx->partials.p,
+                                           redctr(ctx), s);
+  // x += alf z ; r -= alf v ; L1 etc. (reuse the PCG update with p=z, q=v)
+  k_xr_update<T><<<g, BLK, 0, ctx->stream>>>(n, ctx->diag.p, a, x, r, z, v, ctx->partials.p,
+                                              redctr(ctx), s, crit, tol_abs, tol_rel, itr_max);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+// Runs iterations until the device reports done; returns 0 or error.
+template <typename F>
+int iterate_until_done(fcp_ctx *ctx, SolveScalars *s_dev, int slot, int itr_max, int check_every,
+                       int verbose, const char *tag, F &&one_iteration) {
+  int launched = 0;
+  const int ce = check_every > 0 ? check_every : 8;
+  while (launched < itr_max) {
+    const int batch = (itr_max - launched) < ce ? (itr_max - launched) : ce;
+    for (int k = 0; k < batch; ++k) FCP_TRY(one_iteration());
+    launched += batch;
+    FCP_HIP(ctx, hipMemcpyAsync(&ctx->h_scal[slot], s_dev, sizeof(SolveScalars),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    FCP_TRY(check_err(ctx));
+    if (verbose >= 2)
+      printf("                   %s iter = %4d resl = %10.3E rsm = %10.3E\n", tag,
+             ctx->h_scal[slot].iter, ctx->h_scal[slot].resl, ctx->h_scal[slot].rsm);
+    if (ctx->h_scal[slot].done) break;
+  }
+  return 0;
+}
+
+int solve_fp64(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, double tol_abs,
+               double tol_rel, int crit, int check_every, int verbose) {
+  const int n = ctx->n;
+  SolveScalars *s = ctx->scal.p;
+  const double *a = ctx->a.p;
+  const int g = red_grid(ctx, n);
+  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int), ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
+  k_zero_f64<<<g, BLK, 0, ctx->stream>>>(n, ctx->pk.p, ctx->uk.p, ctx->zk.p);
+  k_residual_init<double><<<g, BLK, 0, ctx->stream>>>(n, ctx->ia.p, ctx->ja.p, a, x, b, ctx->res.p,
+                                                       solver == 2 ? ctx->reso.p : nullptr,
+                                                       ctx->partials.p, redctr(ctx), s);
+  FCP_HIP(ctx, hipGetLastError());
+  FCP_HIP(ctx, hipMemcpyAsync(&ctx->h_scal[0], s, sizeof(SolveScalars), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->h_scal[0].res0 < tol_abs) return 0;  // :428-432
+  if (verbose >= 2) printf("                    res0 = %10.3E\n", ctx->h_scal[0].res0);
+  if (solver == 0 || solver == 2) {
+    const int kind = solver == 2 ? 1 : 0;
+    if (ctx->dic_version != ctx->a_version || ctx->dic_kind != kind) {
+      FCP_TRY(precond_setup<double>(ctx, kind, a, ctx->d.p));
+      ctx->dic_version = ctx->a_version;
+      ctx->dic_kind = kind;
+    }
+  }
+  if (solver == 0 || solver == 1) {
+    auto it = [&]() {
+      return pcg_iteration<double>(ctx, solver == 0 ? 0 : 1, a, ctx->d.p, x, ctx->res.p, ctx->pk.p,
+                                   ctx->zk.p, ctx->yk.p, ctx->qk.p, s, crit, tol_abs, tol_rel,
+                                   itr_max);
+    };
+    FCP_TRY(iterate_until_done(ctx, s, 0, itr_max, check_every, verbose, "PCG", it));
+  } else {
+    auto it = [&]() {
+      return bicgstab_iteration<double>(ctx, a, ctx->d.p, x, ctx->res.p, ctx->reso.p, ctx->pk.p,
+                                        ctx->zk.p, ctx->yk.p, ctx->uk.p, ctx->vk.p, s, crit,
+                                        tol_abs, tol_rel, itr_max);
+    };
+    FCP_TRY(iterate_until_done(ctx, s, 0, itr_max, check_every, verbose, "BiCGStab", it));
+  }
+  return 0;
+}
+
+// True FP64 residual r = b - A x with norms into scalars slot.
+int true_residual(fcp_ctx *ctx, const double *x, const double *b, SolveScalars *s) {
+  const int n = ctx->n;
+  ProfScope ps(ctx, "residual_f64");
+  k_residual_init<double><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(
+      n, ctx->ia.p, ctx->ja.p, ctx->a.p, x, b, ctx->res.p, nullptr, ctx->partials.p, redctr(ctx), s);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, double tol_abs,
+                double tol_rel, int check_every, int verbose, fcp_solve_opts *opts) {
+  const int n = ctx->n;
+  const int g = red_grid(ctx, n);
+  SolveScalars *so = ctx->scal.p;      // outer (FP64)
+  SolveScalars *si = ctx->scal.p + 1;  // inner (FP32)
+  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int), ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
+  // FP32 operator and preconditioner (diagonal computed in FP64 then rounded)
+  FCP_HIP(ctx, ctx->a32.alloc(ctx->nnz));
+  if (ctx->a32_version != ctx->a_version) {
+    k_cast_a<<<red_grid(ctx, ctx->nnz), BLK, 0, ctx->stream>>>(ctx->nnz, ctx->a.p, ctx->a32.p);
+    ctx->a32_version = ctx->a_version;
+  }
+  const int kind = solver == 2 ? 1 : 0;
+  if (solver != 1 && (ctx->dic_version != ctx->a_version || ctx->dic_kind != kind)) {
+    FCP_TRY(precond_setup<double>(ctx, kind, ctx->a.p, ctx->d.p));
+    ctx->dic_version = ctx->a_version;
+    ctx->dic_kind = kind;
+  }
+  if (solver != 1 && ctx->d32_version != ctx->dic_version) {
+    k_cast_a<<<g, BLK, 0, ctx->stream>>>(n, ctx->d.p, ctx->d32.p);
+    ctx->d32_version = ctx->dic_version;
+  }
+  FCP_TRY(true_residual(ctx, x, b, so));
+  const int max_outer = (opts && opts->max_outer > 0) ? opts->max_outer : 30;
+  const double inner_tol = (opts && opts->inner_tol > 0) ? opts->inner_tol : 1e-4;
+  int total_inner = 0, outer = 0;
+  double prev = 0.0;
+  bool fp64_fallback = false;
+  for (; outer < max_outer; ++outer) {
+    FCP_HIP(ctx, hipMemcpyAsync(&ctx->h_scal[0], so, sizeof(SolveScalars), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    const double rel = ctx->h_scal[0].l2 / (ctx->h_scal[0].l2b > 0 ? ctx->h_scal[0].l2b : 1.0);
+    if (outer == 0) {
+      if (opts) opts->res0 = ctx->h_scal[0].res0;
+    }
+    if (verbose >= 2) printf("                    IR outer %d  |r|/|b| = %10.3E\n", outer, rel);
+    if (!std::isfinite(rel)) return fcp_fail(ctx, FCP_ERR_NONFINITE, "non-finite residual in refinement");
+    if (rel <= tol_rel || ctx->h_scal[0].l2 == 0.0) break;
+    if (total_inner >= itr_max) break;
+    if (outer > 0 && rel > 0.5 * prev) fp64_fallback = true;  // stagnation
+    prev = rel;
+    if (fp64_fallback) {
+      // finish with FP64 inner iterations, warm-started from x
+      FCP_TRY(solve_fp64(ctx, solver, x, b, itr_max - total_inner, tol_abs, tol_rel, 1, check_every,
+                         verbose));
+      total_inner += ctx->h_scal[0].iter;
+      FCP_TRY(true_residual(ctx, x, b, so));
+      continue;
+    }
+    // inner FP32 solve of A e = r/|r|
+    k_scale_to32<<<g, BLK, 0, ctx->stream>>>(n, ctx->res.p, so, ctx->r32.p, ctx->e32.p);
+    k_zero_f32<<<g, BLK, 0, ctx->stream>>>(n, ctx->p32.p, ctx->q32.p);
+    k_init32<<<g, BLK, 0, ctx->stream>>>(n, ctx->r32.p, ctx->partials.p, redctr(ctx), si);
+    const int budget = itr_max - total_inner;
+    auto it = [&]() {
+      return pcg_iteration<float>(ctx, solver == 1 ? 1 : 0, ctx->a32.p, ctx->d32.p, ctx->e32.p,
+                                  ctx->r32.p, ctx->p32.p, ctx->z32.p, ctx->y32.p, ctx->q32.p, si, 1,
+                                  0.0, inner_tol, budget);
+    };
+    FCP_TRY(iterate_until_done(ctx, si, 1, budget, check_every, verbose, "PCG32", it));
+    total_inner += ctx->h_scal[1].iter;
+    k_ir_update<<<g, BLK, 0, ctx->stream>>>(n, ctx->e32.p, so, x);
+    FCP_TRY(true_residual(ctx, x, b, so));
+  }
+  if (opts) {
+    opts->iterations = total_inner;
+    opts->outer_iterations = outer;
+  }
+  ctx->h_scal[2].iter = total_inner;
+  return 0;
+}
+
+}  // namespace
+
+int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out) {
+  FCP_TRY(fcp_ensure_workspace(ctx));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
+  if (ctx->dic_version != ctx->a_version || ctx->dic_kind != kind) {
+    FCP_TRY(precond_setup<double>(ctx, kind, ctx->a.p, ctx->d.p));
+    ctx->dic_version = ctx->a_version;
+    ctx->dic_kind = kind;
+  }
+  FCP_TRY(precond_apply<double>(ctx, ctx->a.p, ctx->d.p, r, ctx->yk.p, z, nullptr));
+  if (d_out)
+    FCP_HIP(ctx, hipMemcpyAsync(d_out, ctx->d.p, sizeof(double) * ctx->n, hipMemcpyDeviceToDevice,
+                                ctx->stream));
+  FCP_TRY(check_err(ctx));
+  return 0;
+}
+
+// solver: 0 iccg, 1 dpcg, 2 bicgstab.  x, b device pointers of length n.
+int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max,
+                     double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts) {
+  FCP_TRY(fcp_ensure_workspace(ctx));
+  const int prec = opts ? opts->precision : FCP_PREC_FP64;
+  const int crit = opts ? opts->criterion : FCP_CRIT_REF_L1;
+  const int ce = (opts && opts->check_every > 0) ? opts->check_every : 8;
+  hipEvent_t e0, e1;
+  FCP_HIP(ctx, hipEventCreate(&e0));
+  FCP_HIP(ctx, hipEventCreate(&e1));
+  FCP_HIP(ctx, hipEventRecord(e0, ctx->stream));
+  int rc;
+  if (prec == FCP_PREC_MIXED)
+    rc = solve_mixed(ctx, solver, x, b, itr_max, tol_abs, tol_rel, ce, verbose, opts);
+  else
+    rc = solve_fp64(ctx, solver, x, b, itr_max, tol_abs, tol_rel, crit, ce, verbose);
+  if (rc < 0) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+  }
+  FCP_HIP(ctx, hipEventRecord(e1, ctx->stream));
+  // final true residual (L2) for the report
+  SolveScalars *sf = ctx->scal.p + 2;
+  FCP_TRY(true_residual(ctx, x, b, sf));
+  FCP_HIP(ctx, hipMemcpyAsync(&ctx->h_scal[0], ctx->scal.p, 3 * sizeof(SolveScalars),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  float ms = 0.f;
+  FCP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  const SolveScalars &S = ctx->h_scal[0];
+  const SolveScalars &F = ctx->h_scal[2];
+  int iters = (prec == FCP_PREC_MIXED) ? (opts ? opts->iterations : 0) : S.iter;
+  if (opts) {
+    if (prec != FCP_PREC_MIXED) {
+      opts->iterations = S.iter;
+      opts->outer_iterations = 1;
+      opts->res0 = S.res0;
+      opts->resl = S.resl;
+      opts->resor = S.resor;
+    } else {
+      opts->resl = F.res0;  // L1 of the final true residual
+      opts->resor = opts->res0 / (S.factor > 0 ? S.factor : 1.0);
+    }
+    opts->rel_res_l2 = F.l2 / (F.l2b > 0 ? F.l2b : 1.0);
+    opts->time_ms = ms;
+  }
+  if (prec != FCP_PREC_MIXED && S.status < 0) return fcp_fail(ctx, S.status, "non-finite residual");
+  if (prec != FCP_PREC_MIXED && !S.done && S.res0 >= tol_abs) return iters > 0 ? iters : 1;
+  if (prec == FCP_PREC_MIXED && F.l2 > tol_rel * F.l2b && crit == FCP_CRIT_TRUE_L2)
+    return iters > 0 ? iters : 1;
+  return 0;
+}

@@ -1,0 +1,456 @@
+// fcp_mesh.hip -- mesh connectivity, FP64 geometry, CSR pattern, face->slot
+// maps and level schedules, all built on the GPU.
+//
+// Determinism: the reference scatters per face into owner/neighbour cells in
+// ascending face order (geometry.f90:416-530).  Here every cell gathers its
+// faces in ascending face id from the cell->face adjacency, performing the
+// same floating-point operations in the same order, so results are bit-equal
+// without atomics on floating-point data.
+#include "fcp_internal.h"
+
+namespace {
+
+constexpr int BLK = 256;
+
+__global__ void k_cf_count(const int *__restrict__ owner, const int *__restrict__ neighbour,
+                           int F, int Fi, int *__restrict__ cnt) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  atomicAdd(&cnt[owner[f]], 1);
+  if (f < Fi) atomicAdd(&cnt[neighbour[f]], 1);
+}
+
+__global__ void k_cf_fill(const int *__restrict__ owner, const int *__restrict__ neighbour,
+                          int F, int Fi, const int *__restrict__ ptr, int *__restrict__ fill,
+                          int *__restrict__ cf) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  int c = owner[f];
+  cf[ptr[c] + atomicAdd(&fill[c], 1)] = f;
+  if (f < Fi) {
+    c = neighbour[f];
+    cf[ptr[c] + atomicAdd(&fill[c], 1)] = f;
+  }
+}
+
+// Sort each cell's face list ascending (insertion sort, short lists).
+__global__ void k_cf_sort(const int *__restrict__ ptr, int *__restrict__ cf, int n) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  int b = ptr[c], e = ptr[c + 1];
+  for (int k = b + 1; k < e; ++k) {
+    int v = cf[k];
+    int l = k - 1;
+    while (l >= b && cf[l] > v) {
+      cf[l + 1] = cf[l];
+      --l;
+    }
+    cf[l + 1] = v;
+  }
+}
+
+// Face area vectors and centroids by fan triangulation from the first vertex
+// (geometry.f90:416-516, triangle_area_vector :1721-1749).
+__global__ void k_face_geom(const double *__restrict__ P, const int *__restrict__ nnodes,
+                            const int *__restrict__ node, int F, double *__restrict__ arx,
+                            double *__restrict__ ary, double *__restrict__ arz,
+                            double *__restrict__ xf, double *__restrict__ yf,
+                            double *__restrict__ zf) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const int *nd = node + (size_t)f * FCP_NOMAX;
+  const double third = 1.0 / 3.0;
+  const double x1 = P[3 * nd[0]], y1 = P[3 * nd[0] + 1], z1 = P[3 * nd[0] + 2];
+  double sx = 0.0, sy = 0.0, sz = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, areasum = 0.0;
+  const int nt = nnodes[f] - 2;
+  for (int i = 0; i < nt; ++i) {
+    const double *pb = P + 3 * nd[i + 1];
+    const double *pc = P + 3 * nd[i + 2];
+    const double px = pb[0] - x1, py = pb[1] - y1, pz = pb[2] - z1;
+    const double qx = pc[0] - x1, qy = pc[1] - y1, qz = pc[2] - z1;
+    const double nx = 0.5 * (py * qz - pz * qy);
+    const double ny = 0.5 * (pz * qx - px * qz);
+    const double nz = 0.5 * (px * qy - py * qx);
+    sx += nx;
+    sy += ny;
+    sz += nz;
+    const double cx = third * ((pc[0] + pb[0]) + x1);
+    const double cy = third * ((pc[1] + pb[1]) + y1);
+    const double cz = third * ((pc[2] + pb[2]) + z1);
+    const double are = sqrt((nx * nx + ny * ny) + nz * nz);
+    fx += are * cx;
+    fy += are * cy;
+    fz += are * cz;
+    areasum += are;
+  }
+  arx[f] = sx;
+  ary[f] = sy;
+  arz[f] = sz;
+  xf[f] = fx / areasum;
+  yf[f] = fy / areasum;
+  zf[f] = fz / areasum;
+}
+
+// Cell volume and centroid (Wang 1999 as in geometry.f90:467-530): gather
+// over the cell's faces in ascending face id, triangles in fan order.
+__global__ void k_cell_geom(const double *__restrict__ P, const int *__restrict__ nnodes,
+                            const int *__restrict__ node, const int *__restrict__ owner,
+                            const int *__restrict__ cf_ptr, const int *__restrict__ cf, int n,
+                            double *__restrict__ xc, double *__restrict__ yc,
+                            double *__restrict__ zc, double *__restrict__ vol) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double third = 1.0 / 3.0;
+  double v = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, den = 0.0;
+  for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+    const int f = cf[k];
+    const bool own = (owner[f] == c);
+    const int *nd = node + (size_t)f * FCP_NOMAX;
+    const double x1 = P[3 * nd[0]], y1 = P[3 * nd[0] + 1], z1 = P[3 * nd[0] + 2];
+    const int nt = nnodes[f] - 2;
+    for (int i = 0; i < nt; ++i) {
+      const double *pb = P + 3 * nd[i + 1];
+      const double *pc = P + 3 * nd[i + 2];
+      const double px = pb[0] - x1, py = pb[1] - y1, pz = pb[2] - z1;
+      const double qx = pc[0] - x1, qy = pc[1] - y1, qz = pc[2] - z1;
+      const double nx = 0.5 * (py * qz - pz * qy);
+      const double ny = 0.5 * (pz * qx - px * qz);
+      const double nz = 0.5 * (px * qy - py * qx);
+      const double cx = third * ((pc[0] + pb[0]) + x1);
+      const double cy = third * ((pc[1] + pb[1]) + y1);
+      const double cz = third * ((pc[2] + pb[2]) + z1);
+      double riSi = (cx * nx + cy * ny) + cz * nz;
+      if (!own) riSi = -riSi;
+      v += third * riSi;
+      sx += 0.75 * riSi * cx;
+      sy += 0.75 * riSi * cy;
+      sz += 0.75 * riSi * cz;
+      den += riSi;
+    }
+  }
+  vol[c] = v;
+  xc[c] = sx / den;
+  yc[c] = sy / den;
+  zc[c] = sz / den;
+}
+
+// Interpolation factor (variant 2, geometry.f90:581-606), Df (:648-664) and
+// boundary normal distance / |S|/dn (:701-754).
+__global__ void k_face_geom2(const int *__restrict__ owner, const int *__restrict__ neighbour,
+                             int F, int Fi, const double *__restrict__ xc,
+                             const double *__restrict__ yc, const double *__restrict__ zc,
+                             const double *__restrict__ arx, const double *__restrict__ ary,
+                             const double *__restrict__ arz, const double *__restrict__ xf,
+                             const double *__restrict__ yf, const double *__restrict__ zf,
+                             double *__restrict__ facint, double *__restrict__ Df,
+                             double *__restrict__ dnb, double *__restrict__ srdb) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const int p = owner[f];
+  if (f < Fi) {
+    const int q = neighbour[f];
+    double dx = xf[f] - xc[p], dy = yf[f] - yc[p], dz = zf[f] - zc[p];
+    const double djp = sqrt((dx * dx + dy * dy) + dz * dz);
+    dx = xf[f] - xc[q];
+    dy = yf[f] - yc[q];
+    dz = zf[f] - zc[q];
+    const double djn = sqrt((dx * dx + dy * dy) + dz * dz);
+    facint[f] = djp / (djp + djn);
+    const double xpn = xc[q] - xc[p], ypn = yc[q] - yc[p], zpn = zc[q] - zc[p];
+    const double are = (arx[f] * arx[f] + ary[f] * ary[f]) + arz[f] * arz[f];
+    Df[f] = are / ((arx[f] * xpn + ary[f] * ypn) + arz[f] * zpn);
+  } else {
+    const double are = sqrt((arx[f] * arx[f] + ary[f] * ary[f]) + arz[f] * arz[f]);
+    const double nxf = arx[f] / are, nyf = ary[f] / are, nzf = arz[f] / are;
+    const double dn = ((xf[f] - xc[p]) * nxf + (yf[f] - yc[p]) * nyf) + (zf[f] - zc[p]) * nzf;
+    dnb[f - Fi] = dn;
+    srdb[f - Fi] = are / dn;
+  }
+}
+
+// ---- CSR ------------------------------------------------------------------
+__global__ void k_row_count(const int *__restrict__ cf_ptr, const int *__restrict__ cf, int n,
+                            int Fi, int *__restrict__ cnt) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > n) return;
+  if (c == n) {
+    cnt[n] = 0;
+    return;
+  }
+  int m = 1;
+  for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) m += (cf[k] < Fi);
+  cnt[c] = m;
+}
+
+// Row c holds {c} and the other cell of each inner face, sorted ascending
+// (the unique result of the reference's lexical COO sort, utils.f90:443).
+__global__ void k_row_fill(const int *__restrict__ cf_ptr, const int *__restrict__ cf,
+                           const int *__restrict__ owner, const int *__restrict__ neighbour,
+                           int n, int Fi, const int *__restrict__ ia, int *__restrict__ ja,
+                           int *__restrict__ diag) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int b = ia[c];
+  int e = b;
+  ja[e++] = c;
+  for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+    const int f = cf[k];
+    if (f >= Fi) continue;
+    ja[e++] = (owner[f] == c) ? neighbour[f] : owner[f];
+  }
+  for (int k = b + 1; k < e; ++k) {
+    int v = ja[k];
+    int l = k - 1;
+    while (l >= b && ja[l] > v) {
+      ja[l + 1] = ja[l];
+      --l;
+    }
+    ja[l + 1] = v;
+  }
+  int dpos = -1;
+  for (int k = b; k < e; ++k)
+    if (ja[k] == c) {
+      dpos = k;
+      break;
+    }
+  diag[c] = dpos;
+}
+
+__device__ __forceinline__ int first_in_row(const int *ja, int b, int e, int col) {
+  // lower_bound: first position with ja >= col; rows are sorted ascending
+  while (b < e) {
+    int m = (b + e) >> 1;
+    if (ja[m] < col)
+      b = m + 1;
+    else
+      e = m;
+  }
+  return b;
+}
+
+// csr_to_k (utils.f90:96-146): first match of (row,col).
+__global__ void k_slots(const int *__restrict__ owner, const int *__restrict__ neighbour, int Fi,
+                        const int *__restrict__ ia, const int *__restrict__ ja,
+                        int *__restrict__ slot_pn, int *__restrict__ slot_np) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= Fi) return;
+  const int p = owner[f], q = neighbour[f];
+  int k = first_in_row(ja, ia[p], ia[p + 1], q);
+  slot_pn[f] = (k < ia[p + 1] && ja[k] == q) ? k : -1;
+  k = first_in_row(ja, ia[q], ia[q + 1], p);
+  slot_np[f] = (k < ia[q + 1] && ja[k] == p) ? k : -1;
+}
+
+__global__ void k_row_of_tslot(const int *__restrict__ ia, const int *__restrict__ ja, int n,
+                               int *__restrict__ row_of, int *__restrict__ tslot) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = ia[i]; k < ia[i + 1]; ++k) {
+    row_of[k] = i;
+    const int j = ja[k];
+    int l = first_in_row(ja, ia[j], ia[j + 1], i);
+    tslot[k] = (l < ia[j + 1] && ja[l] == i) ? l : -1;
+  }
+}
+
+// ---- level sets -------------------------------------------------------------
+// lev[i] = 0 without strict-lower (upper) entries, else 1 + max over them.
+// In-place monotone relaxation: every value stays a lower bound of the true
+// level, so the fixed point is the exact level and independent of timing.
+__global__ void k_levels(const int *__restrict__ ia, const int *__restrict__ diag,
+                         const int *__restrict__ ja, int n, int upper, int *__restrict__ lev,
+                         int *__restrict__ changed) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = upper ? diag[i] + 1 : ia[i];
+  const int e = upper ? ia[i + 1] : diag[i];
+  int l = 0;
+  for (int k = b; k < e; ++k) {
+    const int v = lev[ja[k]] + 1;
+    l = v > l ? v : l;
+  }
+  if (l != lev[i]) {
+    lev[i] = l;
+    *changed = 1;
+  }
+}
+
+__global__ void k_iota(int *__restrict__ v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = i;
+}
+
+__global__ void k_hist(const int *__restrict__ lev, int n, int *__restrict__ cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&cnt[lev[i]], 1);
+}
+
+__global__ void k_max(const int *__restrict__ v, int n, int *__restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicMax(out, v[i]);
+}
+
+}  // namespace
+
+int fcp_build_cell_faces(fcp_ctx *ctx) {
+  const int n = ctx->n, F = ctx->F, Fi = ctx->Fi;
+  DBuf<int> cnt, fill;
+  FCP_HIP(ctx, cnt.alloc(n + 1));
+  FCP_HIP(ctx, fill.alloc(n));
+  FCP_HIP(ctx, ctx->cf_ptr.alloc(n + 1));
+  FCP_HIP(ctx, ctx->cf_face.alloc((size_t)2 * Fi + (F - Fi)));
+  FCP_HIP(ctx, hipMemsetAsync(cnt.p, 0, sizeof(int) * (n + 1), ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(fill.p, 0, sizeof(int) * n, ctx->stream));
+  k_cf_count<<<grid_for(F, BLK), BLK, 0, ctx->stream>>>(ctx->owner.p, ctx->neighbour.p, F, Fi, cnt.p);
+  FCP_TRY(fcp_exclusive_scan_int(ctx, cnt.p, ctx->cf_ptr.p, n + 1));
+  k_cf_fill<<<grid_for(F, BLK), BLK, 0, ctx->stream>>>(ctx->owner.p, ctx->neighbour.p, F, Fi,
+                                                         ctx->cf_ptr.p, fill.p, ctx->cf_face.p);
+  k_cf_sort<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(ctx->cf_ptr.p, ctx->cf_face.p, n);
+  FCP_HIP(ctx, hipGetLastError());
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_compute_geometry(fcp_ctx *ctx) {
+  const int n = ctx->n, F = ctx->F, Fi = ctx->Fi, Fb = ctx->Fb;
+  FCP_HIP(ctx, ctx->xc.alloc(n));
+  FCP_HIP(ctx, ctx->yc.alloc(n));
+  FCP_HIP(ctx, ctx->zc.alloc(n));
+  FCP_HIP(ctx, ctx->vol.alloc(n));
+  FCP_HIP(ctx, ctx->arx.alloc(F));
+  FCP_HIP(ctx, ctx->ary.alloc(F));
+  FCP_HIP(ctx, ctx->arz.alloc(F));
+  FCP_HIP(ctx, ctx->xf.alloc(F));
+  FCP_HIP(ctx, ctx->yf.alloc(F));
+  FCP_HIP(ctx, ctx->zf.alloc(F));
+  FCP_HIP(ctx, ctx->facint.alloc(Fi > 0 ? Fi : 1));
+  FCP_HIP(ctx, ctx->Df.alloc(Fi > 0 ? Fi : 1));
+  FCP_HIP(ctx, ctx->dnb.alloc(Fb > 0 ? Fb : 1));
+  FCP_HIP(ctx, ctx->srdb.alloc(Fb > 0 ? Fb : 1));
+  {
+    ProfScope ps(ctx, "face_geom");
+    k_face_geom<<<grid_for(F, BLK), BLK, 0, ctx->stream>>>(ctx->points.p, ctx->nnodes.p, ctx->node.p,
+                                                             F, ctx->arx.p, ctx->ary.p, ctx->arz.p,
+                                                             ctx->xf.p, ctx->yf.p, ctx->zf.p);
+  }
+  {
+    ProfScope ps(ctx, "cell_geom");
+    k_cell_geom<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(
+        ctx->points.p, ctx->nnodes.p, ctx->node.p, ctx->owner.p, ctx->cf_ptr.p, ctx->cf_face.p, n,
+        ctx->xc.p, ctx->yc.p, ctx->zc.p, ctx->vol.p);
+  }
+  {
+    ProfScope ps(ctx, "face_geom2");
+    k_face_geom2<<<grid_for(F, BLK), BLK, 0, ctx->stream>>>(
+        ctx->owner.p, ctx->neighbour.p, F, Fi, ctx->xc.p, ctx->yc.p, ctx->zc.p, ctx->arx.p,
+        ctx->ary.p, ctx->arz.p, ctx->xf.p, ctx->yf.p, ctx->zf.p, ctx->facint.p, ctx->Df.p,
+        ctx->dnb.p, ctx->srdb.p);
+  }
+  FCP_HIP(ctx, hipGetLastError());
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->have_geom = true;
+  return 0;
+}
+
+int fcp_build_csr(fcp_ctx *ctx) {
+  const int n = ctx->n, Fi = ctx->Fi;
+  for (int ib = 0; ib < ctx->nPatches; ++ib)
+    if (ctx->h_bctype[ib] == FCP_BC_PERIODIC)
+      return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "periodic patches are not supported");
+  const long long nnz = 2LL * Fi + n;  // sparse_matrix.f90:110
+  if (nnz > 0x7fffffffLL) return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "nnz exceeds int32");
+  ctx->nnz = (int)nnz;
+  DBuf<int> cnt;
+  FCP_HIP(ctx, cnt.alloc(n + 1));
+  FCP_HIP(ctx, ctx->ia.alloc(n + 1));
+  FCP_HIP(ctx, ctx->ja.alloc(nnz));
+  FCP_HIP(ctx, ctx->diag.alloc(n));
+  FCP_HIP(ctx, ctx->slot_pn.alloc(Fi > 0 ? Fi : 1));
+  FCP_HIP(ctx, ctx->slot_np.alloc(Fi > 0 ? Fi : 1));
+  FCP_HIP(ctx, ctx->row_of.alloc(nnz));
+  FCP_HIP(ctx, ctx->tslot.alloc(nnz));
+  k_row_count<<<grid_for(n + 1, BLK), BLK, 0, ctx->stream>>>(ctx->cf_ptr.p, ctx->cf_face.p, n, Fi, cnt.p);
+  FCP_TRY(fcp_exclusive_scan_int(ctx, cnt.p, ctx->ia.p, n + 1));
+  k_row_fill<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p,
+                                                          ctx->neighbour.p, n, Fi, ctx->ia.p,
+                                                          ctx->ja.p, ctx->diag.p);
+  if (Fi > 0)
+    k_slots<<<grid_for(Fi, BLK), BLK, 0, ctx->stream>>>(ctx->owner.p, ctx->neighbour.p, Fi, ctx->ia.p,
+                                                          ctx->ja.p, ctx->slot_pn.p, ctx->slot_np.p);
+  k_row_of_tslot<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(ctx->ia.p, ctx->ja.p, n, ctx->row_of.p,
+                                                              ctx->tslot.p);
+  FCP_HIP(ctx, hipGetLastError());
+  int last = 0;
+  FCP_HIP(ctx, hipMemcpyAsync(&last, ctx->ia.p + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (last != ctx->nnz) return fcp_fail(ctx, FCP_ERR_STATE, "CSR size mismatch %d vs %d", last, ctx->nnz);
+  FCP_HIP(ctx, ctx->a.alloc(nnz));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->a.p, 0, sizeof(double) * nnz, ctx->stream));
+  ctx->a_version++;
+  FCP_TRY(fcp_build_schedules(ctx));
+  ctx->have_csr = true;
+  return 0;
+}
+
+static int build_one_schedule(fcp_ctx *ctx, int upper, Schedule &s) {
+  const int n = ctx->n;
+  FCP_HIP(ctx, s.lev.alloc(n));
+  FCP_HIP(ctx, s.order.alloc(n));
+  DBuf<int> flag, iota, keys_out;
+  FCP_HIP(ctx, flag.alloc(4));
+  FCP_HIP(ctx, iota.alloc(n));
+  FCP_HIP(ctx, keys_out.alloc(n));
+  FCP_HIP(ctx, hipMemsetAsync(s.lev.p, 0, sizeof(int) * n, ctx->stream));
+  int *h_changed = nullptr;
+  FCP_HIP(ctx, hipHostMalloc(&h_changed, sizeof(int)));
+  // Relax in batches of passes; each pass can only raise values, so a
+  // quiescent batch has reached the fixed point.
+  for (int pass = 0; pass <= n; pass += 8) {
+    FCP_HIP(ctx, hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
+    for (int r = 0; r < 8; ++r)
+      k_levels<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(ctx->ia.p, ctx->diag.p, ctx->ja.p, n, upper,
+                                                            s.lev.p, flag.p);
+    FCP_HIP(ctx, hipMemcpyAsync(h_changed, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!*h_changed) break;
+  }
+  (void)hipHostFree(h_changed);
+  // number of levels
+  FCP_HIP(ctx, hipMemsetAsync(flag.p, 0, sizeof(int), ctx->stream));
+  k_max<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(s.lev.p, n, flag.p);
+  int maxlev = 0;
+  FCP_HIP(ctx, hipMemcpyAsync(&maxlev, flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  s.nlev = maxlev + 1;
+  // rows sorted by level (stable: ascending row id inside a level)
+  k_iota<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(iota.p, n);
+  FCP_TRY(fcp_sort_pairs_int(ctx, s.lev.p, keys_out.p, iota.p, s.order.p, n, maxlev));
+  DBuf<int> cnt;
+  FCP_HIP(ctx, cnt.alloc(s.nlev));
+  FCP_HIP(ctx, hipMemsetAsync(cnt.p, 0, sizeof(int) * s.nlev, ctx->stream));
+  k_hist<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(s.lev.p, n, cnt.p);
+  std::vector<int> h_cnt(s.nlev);
+  FCP_HIP(ctx, hipMemcpyAsync(h_cnt.data(), cnt.p, sizeof(int) * s.nlev, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int> tp;
+  tp.reserve(n / 64 + s.nlev + 1);
+  int off = 0;
+  for (int L = 0; L < s.nlev; ++L) {
+    for (int k = 0; k < h_cnt[L]; k += 64) tp.push_back(off + k);
+    off += h_cnt[L];
+  }
+  tp.push_back(off);
+  s.ntasks = (int)tp.size() - 1;
+  FCP_HIP(ctx, s.task_ptr.alloc(tp.size()));
+  FCP_HIP(ctx, hipMemcpyAsync(s.task_ptr.p, tp.data(), sizeof(int) * tp.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_build_schedules(fcp_ctx *ctx) {
+  FCP_TRY(build_one_schedule(ctx, 0, ctx->lower));
+  FCP_TRY(build_one_schedule(ctx, 1, ctx->upper));
+  return 0;
+}

@@ -1,0 +1,248 @@
+"""Python mirror of the reference's Fortran module API on top of libfcp.
+
+Names and argument meaning follow the reference: ``read_mesh`` hands a mesh
+over (geometry.f90:90), ``create_CSR_matrix`` (sparse_matrix.f90:86),
+``laplacian(mu, phi)`` (laplacian.f90:1), ``grad_gauss(u)``
+(gradients.f90:1607), ``csrsolve(solver, fi, rhs, ...)`` (linear_solvers.f90:40)
+and the explicit-array ``iccg``/``dpcg``/``bicgstab`` (:364, :206, :548).
+Arrays are numpy (staged through device memory) or torch CUDA tensors
+(used in place).  Indices in this module are 0-based; the C-ABI is 1-based
+like the Fortran API and the conversion happens here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .mesh import Mesh
+
+FCP_PREC_FP64 = 0
+FCP_PREC_MIXED = 1
+FCP_CRIT_REF_L1 = 0
+FCP_CRIT_TRUE_L2 = 1
+
+
+class FcpError(RuntimeError):
+    pass
+
+
+def _ptr(a):
+    """(void*, keepalive) for a numpy array or a torch tensor (host or CUDA)."""
+    if a is None:
+        return None, None
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return C.c_void_p(a.ctypes.data), a
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return C.c_void_p(a.data_ptr()), a
+    raise TypeError(f"unsupported array type {type(a)}")
+
+
+def _f64(a):
+    if isinstance(a, np.ndarray):
+        return np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+class Context:
+    """One libfcp context (device-resident mesh, geometry, CSR and matrix)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        rc = self.lib.fcp_init(device, C.byref(h))
+        if rc != 0:
+            raise FcpError(f"fcp_init({device}) failed with {rc}")
+        self.h = h
+        self.mesh = None
+        self.nnz = 0
+
+    def close(self):
+        if self.h:
+            self.lib.fcp_finalize(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc < 0:
+            msg = self.lib.fcp_last_error(self.h)
+            raise FcpError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+        return rc
+
+    @property
+    def stream(self):
+        return self.lib.fcp_stream(self.h)
+
+    # -- mesh --------------------------------------------------------------
+    def read_mesh(self, mesh: Mesh):
+        """Upload a mesh (geometry.f90:90 read_mesh hand-over)."""
+        self.mesh = mesh
+        node1 = np.where(mesh.node >= 0, mesh.node + 1, 0).astype(np.int32)
+        arrs = [np.ascontiguousarray(mesh.points, dtype=np.float64).reshape(-1),
+                np.ascontiguousarray(mesh.nnodes, dtype=np.int32),
+                np.ascontiguousarray(node1.reshape(-1)),
+                np.ascontiguousarray(mesh.owner + 1, dtype=np.int32),
+                np.ascontiguousarray(mesh.neighbour + 1, dtype=np.int32),
+                np.ascontiguousarray(mesh.nfaces, dtype=np.int32),
+                np.ascontiguousarray(mesh.start_face, dtype=np.int32),
+                np.ascontiguousarray(mesh.bctype_codes, dtype=np.int32)]
+        ptrs = [_ptr(a)[0] for a in arrs]
+        rc = self.lib.fcp_mesh_upload(self.h, mesh.n_nodes, mesh.n_cells, mesh.n_inner, mesh.n_bnd,
+                                      mesh.n_patches, *ptrs)
+        self._check(rc, "fcp_mesh_upload")
+
+    def geometry(self):
+        m = self.mesh
+        n, F, Fi, Fb = m.n_cells, m.n_faces, m.n_inner, m.n_bnd
+        sizes = dict(xc=n, yc=n, zc=n, vol=n, arx=F, ary=F, arz=F, xf=F, yf=F, zf=F, facint=Fi,
+                     Df=Fi, dnb=Fb, srdb=Fb)
+        out = {k: np.zeros(v) for k, v in sizes.items()}
+        rc = self.lib.fcp_geometry(self.h, *[_ptr(out[k])[0] for k in sizes])
+        self._check(rc, "fcp_geometry")
+        return out
+
+    def compute_geometry(self):
+        """Geometry on the device without copying anything back."""
+        rc = self.lib.fcp_geometry(self.h, *([None] * 14))
+        self._check(rc, "fcp_geometry")
+
+    def create_CSR_matrix(self, want_arrays: bool = True):
+        m = self.mesh
+        nnz = C.c_int()
+        if not want_arrays:
+            self._check(self.lib.fcp_csr_build(self.h, C.byref(nnz), None, None, None, None, None),
+                        "fcp_csr_build")
+            self.nnz = nnz.value
+            return None
+        nn = 2 * m.n_inner + m.n_cells
+        ia = np.zeros(m.n_cells + 1, np.int32)
+        ja = np.zeros(nn, np.int32)
+        dg = np.zeros(m.n_cells, np.int32)
+        spn = np.zeros(max(m.n_inner, 1), np.int32)
+        snp = np.zeros(max(m.n_inner, 1), np.int32)
+        rc = self.lib.fcp_csr_build(self.h, C.byref(nnz), *[_ptr(a)[0] for a in (ia, ja, dg, spn, snp)])
+        self._check(rc, "fcp_csr_build")
+        self.nnz = nnz.value
+        return dict(ia=ia - 1, ja=ja - 1, diag=dg - 1, slot_pn=spn[: m.n_inner] - 1,
+                    slot_np=snp[: m.n_inner] - 1)
+
+    def levels(self, upper: bool = False):
+        lev = np.zeros(self.mesh.n_cells, np.int32)
+        nlev = C.c_int()
+        self._check(self.lib.fcp_levels(self.h, int(upper), _ptr(lev)[0], C.byref(nlev)), "fcp_levels")
+        return lev, nlev.value
+
+    # -- assembly ----------------------------------------------------------
+    def laplacian(self, mu, phi, su, a_out=None):
+        """laplacian(mu,phi): resident matrix; su updated in place."""
+        mu, phi = _f64(mu), _f64(phi)
+        rc = self.lib.fcp_laplacian(self.h, _ptr(mu)[0], _ptr(phi)[0], _ptr(a_out)[0], _ptr(su)[0])
+        self._check(rc, "fcp_laplacian")
+        return a_out
+
+    def laplacian_face(self, mu_f, bc_kind, phi, su, a_out=None):
+        mu_f, phi = _f64(mu_f), _f64(phi)
+        if isinstance(bc_kind, np.ndarray):
+            bc_kind = np.ascontiguousarray(bc_kind, dtype=np.uint8)
+        rc = self.lib.fcp_laplacian_face(self.h, _ptr(mu_f)[0], _ptr(bc_kind)[0], _ptr(phi)[0],
+                                         _ptr(a_out)[0], _ptr(su)[0])
+        self._check(rc, "fcp_laplacian_face")
+        return a_out
+
+    def nonorth_correction(self, gamma_f, grad, su):
+        rc = self.lib.fcp_nonorth_correction(self.h, _ptr(_f64(gamma_f))[0], _ptr(_f64(grad))[0],
+                                             _ptr(su)[0])
+        self._check(rc, "fcp_nonorth_correction")
+
+    def grad_gauss(self, u, out=None):
+        if out is None:
+            out = np.zeros((self.mesh.n_total, 3))
+        rc = self.lib.fcp_grad_gauss(self.h, _ptr(_f64(u))[0], _ptr(out)[0])
+        self._check(rc, "fcp_grad_gauss")
+        return out
+
+    def update_boundary(self, phi):
+        self._check(self.lib.fcp_update_boundary(self.h, _ptr(phi)[0]), "fcp_update_boundary")
+        return phi
+
+    # -- linear algebra ----------------------------------------------------
+    def matrix_set(self, a):
+        self._check(self.lib.fcp_matrix_set(self.h, _ptr(_f64(a))[0]), "fcp_matrix_set")
+
+    def spmv(self, x, y=None):
+        if y is None:
+            y = np.zeros(self.mesh.n_cells)
+        self._check(self.lib.fcp_spmv(self.h, _ptr(_f64(x))[0], _ptr(y)[0]), "fcp_spmv")
+        return y
+
+    def precond_apply(self, r, kind: int = 0):
+        n = self.mesh.n_cells
+        z = np.zeros(n)
+        d = np.zeros(n)
+        rc = self.lib.fcp_precond_apply(self.h, kind, _ptr(_f64(r))[0], _ptr(z)[0], _ptr(d)[0])
+        self._check(rc, "fcp_precond_apply")
+        return z, d
+
+    def csrsolve(self, solver: str, fi, rhs, itr_max=1000, tol_abs=0.0, tol_rel=1e-10,
+                 chvar="phi", verbose=0, precision=FCP_PREC_FP64, criterion=FCP_CRIT_REF_L1,
+                 inner_tol=1e-4, max_outer=30, check_every=8):
+        """csrsolve on the resident matrix; fi updated in place.  Returns
+        (status, info dict)."""
+        opts = _lib.SolveOpts()
+        self.lib.fcp_solve_opts_default(C.byref(opts))
+        opts.precision = precision
+        opts.criterion = criterion
+        opts.inner_tol = inner_tol
+        opts.max_outer = max_outer
+        opts.check_every = check_every
+        res0 = C.c_double()
+        sb = solver.encode()
+        cb = chvar.encode()
+        rc = self.lib.fcp_csrsolve(self.h, sb, len(sb), _ptr(fi)[0], _ptr(_f64(rhs))[0],
+                                   C.byref(res0), int(itr_max), float(tol_abs), float(tol_rel), cb,
+                                   len(cb), int(verbose), C.byref(opts))
+        self._check(rc, "fcp_csrsolve")
+        info = {k: getattr(opts, k) for k, _ in _lib.SolveOpts._fields_}
+        info["status"] = rc
+        return rc, info
+
+    def profile(self, enable: bool):
+        self._check(self.lib.fcp_profile(self.h, int(enable)), "fcp_profile")
+
+    def profile_report(self):
+        buf = C.create_string_buffer(1 << 16)
+        self.lib.fcp_profile_report(self.h, buf, len(buf))
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
+
+
+def explicit_solve(solver: str, ia, ja, a, diag, fi, rhs, itr_max, tol_abs, tol_rel, verbose=0):
+    """iccg/dpcg/bicgstab(n,nnz,ia,ja,a,diag,fi,rhs,resor,...) with 0-based
+    ia/ja/diag (converted to the 1-based C-ABI).  fi updated in place."""
+    lib = _lib.load()
+    n = len(diag)
+    ia1 = np.ascontiguousarray(np.asarray(ia) + 1, np.int32)
+    ja1 = np.ascontiguousarray(np.asarray(ja) + 1, np.int32)
+    dg1 = np.ascontiguousarray(np.asarray(diag) + 1, np.int32)
+    a = np.ascontiguousarray(a, np.float64)
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    resor = C.c_double()
+    fn = {"iccg": lib.fcp_iccg, "dpcg": lib.fcp_dpcg, "bicgstab": lib.fcp_bicgstab}[solver]
+    rc = fn(n, len(ja1), _ptr(ia1)[0], _ptr(ja1)[0], _ptr(a)[0], _ptr(dg1)[0], _ptr(fi)[0],
+            _ptr(rhs)[0], C.byref(resor), int(itr_max), float(tol_abs), float(tol_rel), int(verbose))
+    if rc < 0:
+        raise FcpError(f"fcp_{solver} failed with {rc}")
+    return rc, resor.value

@@ -1,0 +1,204 @@
+/*
+ * fcp.h -- C-ABI of libfcp, the MI355X (gfx950) finite-volume
+ * assemble-and-solve path.
+ *
+ * Drop-in boundary for freeCappuccino's Fortran module API.  The reference
+ * interface each entry point replaces is cited (file:line, relative to the
+ * reference repository root).  Conventions follow the reference so a
+ * Fortran ISO_C_BINDING module can bind every symbol directly:
+ *   - default INTEGER = int32, REAL(8) = double, all arguments by pointer or
+ *     value as declared here (no hidden string lengths: lengths are explicit);
+ *   - every index array crossing the boundary is 1-based (owner, neighbour,
+ *     face vertex lists, ia/ja/diag, face->slot maps);
+ *   - field arrays have length numTotal = nCells + nBoundaryFaces with the
+ *     boundary-face values at positions nCells+1.. (geometry.f90:282-289);
+ *   - gradients are dPhidxi(3,numTotal), i.e. xyz-contiguous per cell.
+ * Array arguments may be host pointers (staged through device memory) or
+ * device pointers from hipMalloc (used in place, no copies).
+ *
+ * Return codes: 0 ok; > 0 solver did not converge (value = iterations);
+ * < 0 error (FCP_ERR_*), message via fcp_last_error().
+ *
+ * Threading: one context per host thread; calls on one context must not
+ * overlap.  Every call is ordered on the context's HIP stream and returns
+ * after its results are complete.
+ */
+#ifndef FCP_H
+#define FCP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FCP_NOMAX 6 /* max vertices per face, geometry.f90:40 */
+
+#define FCP_OK 0
+#define FCP_ERR_ARG (-1)
+#define FCP_ERR_HIP (-2)
+#define FCP_ERR_STATE (-3)
+#define FCP_ERR_UNSUPPORTED (-4)
+#define FCP_ERR_NONFINITE (-5)
+#define FCP_ERR_BREAKDOWN (-6)
+#define FCP_ERR_SWEEP (-7)
+
+/* Boundary-type codes of polyMesh/boundary (geometry.f90:234-261). */
+#define FCP_BC_WALL 1
+#define FCP_BC_SYMMETRY 2
+#define FCP_BC_INLET 3
+#define FCP_BC_OUTLET 4
+#define FCP_BC_PRESSURE 5
+#define FCP_BC_EMPTY 6
+#define FCP_BC_PERIODIC 7
+
+/* Precision of a solve. */
+#define FCP_PREC_FP64 0  /* reference arithmetic, FP64 throughout            */
+#define FCP_PREC_MIXED 1 /* FP32 inner Krylov + FP64 iterative refinement    */
+
+/* Stopping criterion of a solve. */
+#define FCP_CRIT_REF_L1 0  /* reference test: ||r||_1/||r0||_1 < tol_rel or
+                              ||r||_1 < tol_abs, linear_solvers.f90:512-532 */
+#define FCP_CRIT_TRUE_L2 1 /* ||b - A x||_2 / ||b||_2 <= tol_rel (FP64)       */
+
+typedef struct fcp_ctx fcp_ctx;
+
+typedef struct fcp_solve_opts {
+  /* inputs */
+  int precision;      /* FCP_PREC_*                                        */
+  int criterion;      /* FCP_CRIT_*                                        */
+  double inner_tol;   /* mixed precision: relative L2 reduction per inner
+                         FP32 solve (e.g. 1e-4)                            */
+  int max_outer;      /* mixed precision: max refinement steps             */
+  int check_every;    /* iterations between host convergence checks (>=1)  */
+  /* outputs */
+  int iterations;     /* total (inner) Krylov iterations                   */
+  int outer_iterations;
+  double res0;        /* initial L1 residual                               */
+  double resl;        /* final L1 residual (recursive)                     */
+  double resor;       /* reference report value res0/factor (:522-526)     */
+  double rel_res_l2;  /* final true ||b-Ax||_2/||b||_2                     */
+  double time_ms;     /* device time of the solve                          */
+} fcp_solve_opts;
+
+/* --- context ------------------------------------------------------------ */
+int fcp_init(int device, fcp_ctx **ctx);
+void fcp_finalize(fcp_ctx *ctx);
+const char *fcp_last_error(const fcp_ctx *ctx);
+/* Default solve options (FP64, reference criterion). */
+void fcp_solve_opts_default(fcp_solve_opts *opts);
+
+/* --- mesh and geometry -------------------------------------------------- */
+/* Replaces read_mesh_native's data hand-over (geometry.f90:118-406): the
+ * caller passes the arrays the reader produced.  xyz is [3*nNodes] (x,y,z per
+ * node); node is [FCP_NOMAX*nFaces] (node(nomax,numFaces), geometry.f90:386);
+ * startFace as in polyMesh/boundary (0-based offset of the patch's first
+ * face); bctype FCP_BC_* codes.  Resident in HBM until the next upload. */
+int fcp_mesh_upload(fcp_ctx *ctx, int nNodes, int nCells, int nInnerFaces,
+                    int nBoundaryFaces, int nBoundaries, const double *xyz,
+                    const int *nnodes, const int *node, const int *owner,
+                    const int *neighbour, const int *nfaces, const int *startFace,
+                    const int *bctype);
+
+/* FP64 geometry on the GPU, same arithmetic as geometry.f90:416-754
+ * (interpolation_coeff_variant = 2).  Any output pointer may be NULL.
+ * dnb/srdb: normal distance and |S|/dn for every boundary face (the
+ * reference's dnw/srdw, dns/srds are their wall/symmetry subsets). */
+int fcp_geometry(fcp_ctx *ctx, double *xc, double *yc, double *zc, double *vol,
+                 double *arx, double *ary, double *arz, double *xf, double *yf,
+                 double *zf, double *facint, double *Df, double *dnb,
+                 double *srdb);
+
+/* create_CSR_matrix (sparse_matrix.f90:86-296): CSR pattern, diagonal
+ * positions and face->slot maps icell_jcell_csr_index /
+ * jcell_icell_csr_index, built on the GPU and kept resident together with
+ * the level schedules of the strict-lower/upper parts.  Outputs (1-based,
+ * optional): ia[n+1], ja[nnz], diag[n], slot_pn[Fi], slot_np[Fi]. */
+int fcp_csr_build(fcp_ctx *ctx, int *nnz, int *ia, int *ja, int *diag,
+                  int *slot_pn, int *slot_np);
+
+/* Level sets of the strict-lower (upper=0) / strict-upper (upper=1) DAG:
+ * lev[i] (0-based level numbers), returns number of levels via *nlev. */
+int fcp_levels(fcp_ctx *ctx, int upper, int *lev, int *nlev);
+
+/* --- assembly ----------------------------------------------------------- */
+/* laplacian(mu,phi) (laplacian.f90:1-151): a = 0, inner faces
+ * a(PN)=a(NP)=Gamma_f*D_f with Gamma_f=(1-lambda)mu_P+lambda*mu_N, diagonal
+ * minus the row sum, every boundary face Dirichlet: a(PP) -= mu_P|S|/dn,
+ * su -= mu_P|S|/dn*phi_b.  The matrix stays resident as the solve operator;
+ * a (optional) receives a copy.  su is updated in place (not zeroed). */
+int fcp_laplacian(fcp_ctx *ctx, const double *mu, const double *phi, double *a,
+                  double *su);
+
+/* Face-diffusivity form of laplacian (extension, no reference counterpart
+ * for the per-face coefficient): mu_f[nFaces], bc_kind[nBoundaryFaces]
+ * 0 = Dirichlet (as laplacian.f90:93-96 with mu_f of the face),
+ * 1 = zero-gradient (no matrix/source term). */
+int fcp_laplacian_face(fcp_ctx *ctx, const double *mu_f,
+                       const unsigned char *bc_kind, const double *phi,
+                       double *a, double *su);
+
+/* Explicit non-orthogonal diffusion correction of facefluxsc
+ * (scalar_fluxes.f90:99-109,139): per inner face
+ * s = gamma_f*(grad_f . (S - D_f d_PN)), grad_f interpolated with lambda;
+ * su_P += s, su_N -= s (temperature.f90:153-154). */
+int fcp_nonorth_correction(fcp_ctx *ctx, const double *gamma_f,
+                           const double *grad, double *su);
+
+/* grad_gauss (gradients.f90:1607-1715). dudxi[3*numTotal], boundary rows 0. */
+int fcp_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi);
+
+/* updateBoundary (updateBoundary.f90:1-74): zero-gradient copy for
+ * outlet/symmetry/pressure/empty patches. */
+int fcp_update_boundary(fcp_ctx *ctx, double *phi);
+
+/* --- linear algebra ----------------------------------------------------- */
+/* Replace the resident matrix values (CSR order of fcp_csr_build). */
+int fcp_matrix_set(fcp_ctx *ctx, const double *a);
+
+/* y = A x with the resident matrix (ax_cr, linear_solvers.f90:1173;
+ * matvec of iccg :493-498).  x, y of length nCells. */
+int fcp_spmv(fcp_ctx *ctx, const double *x, double *y);
+
+/* One application z = M^{-1} r of the DIC (kind 0, iccg :439-475) or DILU
+ * (kind 1, bicgstab :613-683) preconditioner of the resident matrix, with
+ * the reference's operation order; d (optional, [n]) receives the inverted
+ * pivots.  r, z of length nCells. */
+int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d);
+
+/* csrsolve (linear_solvers.f90:40-91) on the resident matrix: solver is
+ * "iccg" | "dpcg" | "bicgstab" | "gauss-seidel"; fi[numTotal] (only 1..n
+ * used), rhs[n], res0 receives the reference's report value (resor).
+ * opts may be NULL (FP64, reference criterion).  verbose >= 1 prints the
+ * reference's report line, verbose >= 2 per-iteration residuals. */
+int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi,
+                 const double *rhs, double *res0, int itr_max, double tol_abs,
+                 double tol_rel, const char *chvar, int chvar_len, int verbose,
+                 fcp_solve_opts *opts);
+
+/* Explicit-array entry points with the reference signatures
+ * iccg/dpcg/bicgstab/GaussSeidel(n,nnz,ia,ja,a,diag,fi,rhs,resor,itr_max,
+ * tol_abs,tol_rel,chvar,verbose) (linear_solvers.f90:364,206,548,96).
+ * ia/ja/diag 1-based.  They run on a library-owned context on device 0. */
+int fcp_iccg(int n, int nnz, const int *ia, const int *ja, const double *a,
+             const int *diag, double *fi, const double *rhs, double *resor,
+             int itr_max, double tol_abs, double tol_rel, int verbose);
+int fcp_dpcg(int n, int nnz, const int *ia, const int *ja, const double *a,
+             const int *diag, double *fi, const double *rhs, double *resor,
+             int itr_max, double tol_abs, double tol_rel, int verbose);
+int fcp_bicgstab(int n, int nnz, const int *ia, const int *ja, const double *a,
+                 const int *diag, double *fi, const double *rhs, double *resor,
+                 int itr_max, double tol_abs, double tol_rel, int verbose);
+
+/* --- instrumentation ------------------------------------------------------ */
+/* Per-kernel device timing accumulated on the context's stream with HIP
+ * events (name -> total ms, launches).  enable=1 starts, 0 stops. */
+int fcp_profile(fcp_ctx *ctx, int enable);
+/* Writes "name launches total_ms\n" lines into buf (truncated to len). */
+int fcp_profile_report(fcp_ctx *ctx, char *buf, int len);
+
+/* Context stream (hipStream_t) for callers that enqueue their own work. */
+void *fcp_stream(fcp_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

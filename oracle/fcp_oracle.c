@@ -921,3 +921,21 @@ void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
     su[i] = su[i] + urfm * a[c->diag[i]] * phi[i];
   }
 }
+
+void orc_nonorth_correction(const orc_mesh *m, const orc_geom *g, const double *gamma,
+                            const double *grad, double *su) {
+  for (int f = 0; f < m->n_inner; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    const double fxn = g->facint[f], fxp = 1.0 - g->facint[f];
+    const double xpn = g->xc[q] - g->xc[p], ypn = g->yc[q] - g->yc[p], zpn = g->zc[q] - g->zc[p];
+    double dfixi = grad[3 * p] * fxp + grad[3 * q] * fxn;
+    double dfiyi = grad[3 * p + 1] * fxp + grad[3 * q + 1] * fxn;
+    double dfizi = grad[3 * p + 2] * fxp + grad[3 * q + 2] * fxn;
+    dfixi = dfixi * (g->arx[f] - g->Df[f] * xpn);
+    dfiyi = dfiyi * (g->ary[f] - g->Df[f] * ypn);
+    dfizi = dfizi * (g->arz[f] - g->Df[f] * zpn);
+    const double fdfie = gamma[f] * ((dfixi + dfiyi) + dfizi);
+    su[p] = su[p] + fdfie;
+    su[q] = su[q] - fdfie;
+  }
+}

@@ -148,6 +148,13 @@ void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
                   const unsigned char *bc_kind, double urf,
                   double *a, double *su, double *sp);
 
+/* Explicit non-orthogonal diffusion correction of facefluxsc
+ * (scalar_fluxes.f90:99-109) accumulated as calculate_temperature does
+ * (temperature.f90:153-154): su_P += s, su_N -= s, per inner face in face
+ * order, gamma per face, grad AoS [3*numTotal]. su updated in place. */
+void orc_nonorth_correction(const orc_mesh *m, const orc_geom *g, const double *gamma,
+                            const double *grad, double *su);
+
 /* Reference constants (parameters.f90:6-7 widened from default REAL). */
 double orc_small(void);
 double orc_s0_init(void);

@@ -151,6 +151,13 @@ class Oracle:
         assert rc == 0, rc
         return out.reshape(-1, 3)
 
+    def nonorth_correction(self, gamma_f, grad, su):
+        su = np.array(su, dtype=np.float64, copy=True)
+        lib().orc_nonorth_correction(C.byref(self.cm), C.byref(self.cg),
+                                     _p(np.ascontiguousarray(gamma_f, np.float64)),
+                                     _p(np.ascontiguousarray(grad, np.float64).reshape(-1)), _p(su))
+        return su
+
     def update_boundary(self, phi):
         phi = np.array(phi, dtype=np.float64, copy=True)
         lib().orc_update_boundary(C.byref(self.cm), _p(phi))

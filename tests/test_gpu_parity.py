@@ -1,0 +1,262 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle and the
+reference's golden fixtures.
+
+Integer structures (CSR, slot maps, level sets) and the deterministic
+gathers (geometry, Laplacian, Gauss gradient, non-orthogonal correction,
+SpMV, DIC/DILU pivots and sweeps) must be bit-exact.  Krylov solutions use
+different reduction orders than the sequential reference, so they are
+compared within stated tolerances.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from fcp.mesh import hex_mesh, read_foam
+from fcp import workloads
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+GEOM = ("xc", "yc", "zc", "vol", "arx", "ary", "arz", "xf", "yf", "zf", "facint", "Df", "dnb", "srdb")
+
+
+@pytest.fixture(scope="module")
+def api():
+    from fcp import api as _api
+    return _api
+
+
+def _setup(api, mesh):
+    ctx = api.Context(0)
+    ctx.read_mesh(mesh)
+    geo = ctx.geometry()
+    csr = ctx.create_CSR_matrix()
+    return ctx, geo, csr
+
+
+MESHES = {
+    "hex5j": lambda g: hex_mesh(5, jitter=0.2, seed=2,
+                                types=("wall", "symmetry", "outlet", "wall", "inlet", "pressure")),
+    "cavity": lambda g: read_foam(os.path.join(g, "cavity")),
+    "hex17j": lambda g: hex_mesh(17, jitter=0.2, seed=7),
+}
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_geometry_csr_levels_bitexact(api, golden_dir, name):
+    mesh = MESHES[name](golden_dir)
+    print(f"[case] geometry/csr/levels {name} n={mesh.n_cells}", flush=True)
+    ctx, geo, csr = _setup(api, mesh)
+    o = Oracle(mesh)
+    for k in GEOM:
+        assert np.array_equal(geo[k], o.geom[k]), k
+    for k in ("ia", "ja", "diag", "slot_pn", "slot_np"):
+        assert np.array_equal(csr[k], getattr(o, k)), k
+    for upper in (False, True):
+        lev, nlev = ctx.levels(upper)
+        lev_o, nlev_o = o.levels(upper)
+        assert nlev == nlev_o and np.array_equal(lev, lev_o)
+    if name in ("hex5j", "cavity"):
+        g = np.load(os.path.join(golden_dir, f"{name}.npz"))
+        for k in GEOM[:12]:
+            assert np.array_equal(geo[k], g["g_" + k]), k
+        for k in ("ia", "ja", "diag", "slot_pn", "slot_np"):
+            assert np.array_equal(csr[k], g["c_" + k]), k
+    ctx.close()
+
+
+def test_assembly_gradient_bitexact_vs_reference(api, golden_dir):
+    print("[case] laplacian/grad_gauss hex5j", flush=True)
+    mesh = MESHES["hex5j"](golden_dir)
+    g = np.load(os.path.join(golden_dir, "hex5j.npz"))
+    ctx, geo, csr = _setup(api, mesh)
+    su = g["in_su0"].copy()
+    a = np.zeros(csr["ja"].size)
+    ctx.laplacian(g["in_mu"], g["in_phi"], su, a_out=a)
+    assert np.array_equal(a, g["lap_a"]) and np.array_equal(su, g["lap_su"])
+    assert np.array_equal(ctx.grad_gauss(g["in_u"]), g["grad_gauss"])
+    # SpMV in CSR row order is bit-equal to the reference row loop
+    o = Oracle(mesh)
+    x = np.cos(np.arange(mesh.n_cells) * 0.37)
+    assert np.array_equal(ctx.spmv(x), o.spmv(a, x))
+    ctx.close()
+
+
+def test_cavity_gradient_and_laplacian(api, golden_dir):
+    print("[case] cavity", flush=True)
+    mesh = MESHES["cavity"](golden_dir)
+    g = np.load(os.path.join(golden_dir, "cavity.npz"))
+    ctx, geo, csr = _setup(api, mesh)
+    gg = ctx.grad_gauss(g["in_psi"])
+    assert np.array_equal(gg, g["grad_gauss"])
+    assert np.allclose(gg[: mesh.n_cells], 1.0, atol=1e-9)
+    su = np.zeros(mesh.n_cells)
+    a = np.zeros(csr["ja"].size)
+    ctx.laplacian(-np.ones(mesh.n_cells), g["in_psi"], su, a_out=a)
+    assert np.array_equal(a, g["lap_a"]) and np.array_equal(su, g["lap_su"])
+    ctx.close()
+
+
+def test_face_laplacian_nonorth_update_boundary(api):
+    print("[case] laplacian_face/nonorth 17^3", flush=True)
+    mesh = workloads.poisson_jittered_mesh(17)
+    ctx, geo, csr = _setup(api, mesh)
+    o = Oracle(mesh)
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    su = inp["source"] * geo["vol"]
+    su_o = su.copy()
+    a = np.zeros(csr["ja"].size)
+    ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su, a_out=a)
+    a_o, su_o = o.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su_o)
+    assert np.array_equal(a, a_o) and np.array_equal(su, su_o)
+    gr = ctx.grad_gauss(inp["phi_prev"])
+    gr_o = o.grad_gauss(inp["phi_prev"])
+    assert np.array_equal(gr, gr_o)
+    ctx.nonorth_correction(inp["gamma"], gr, su)
+    su_o = o.nonorth_correction(inp["gamma"], gr_o, su_o)
+    assert np.array_equal(su, su_o)
+    phi = np.linspace(-1, 1, mesh.n_total)
+    assert np.array_equal(ctx.update_boundary(phi.copy()), o.update_boundary(phi))
+    ctx.close()
+
+
+def test_precond_bitexact(api):
+    """DIC and DILU pivots and one preconditioner application are bit-equal
+    to the reference loops (level-scheduled sweeps change no arithmetic)."""
+    print("[case] DIC/DILU sweeps 20^3", flush=True)
+    mesh = hex_mesh(20, jitter=0.2, seed=4)
+    ctx, geo, csr = _setup(api, mesh)
+    o = Oracle(mesh)
+    mu = -(10.0 ** np.sin(np.arange(mesh.n_cells)))
+    a, _ = o.laplacian(mu, np.zeros(mesh.n_total))
+    ctx.matrix_set(a)
+    r = np.sin(np.arange(mesh.n_cells) * 0.1)
+    z, d = ctx.precond_apply(r, kind=0)
+    d_o = o.dic_diag(a)
+    assert np.array_equal(d, d_o)
+    assert np.array_equal(z, o.dic_apply(a, d_o, r))
+    # non-symmetric matrix: DILU
+    F = mesh.n_faces
+    bk = np.full(mesh.n_bnd, 2, np.uint8)
+    ac, _, _, _ = o.convdiff(np.cos(np.arange(F)), 10.0 ** np.sin(np.arange(F)),
+                             np.zeros(mesh.n_total), np.zeros((mesh.n_total, 3)), 1.0, 0, bk)
+    ctx.matrix_set(ac)
+    z2, d2 = ctx.precond_apply(r, kind=1)
+    d2_o = o.dilu_diag(ac)
+    assert np.array_equal(d2, d2_o)
+    assert np.array_equal(z2, o.dic_apply(ac, d2_o, r))
+    ctx.close()
+
+
+@pytest.mark.parametrize("solver", ["iccg", "dpcg", "bicgstab"])
+def test_solver_trajectories_vs_reference(api, golden_dir, solver):
+    print(f"[case] {solver} trajectories hex5j", flush=True)
+    mesh = MESHES["hex5j"](golden_dir)
+    g = np.load(os.path.join(golden_dir, "hex5j.npz"))
+    ctx, geo, csr = _setup(api, mesh)
+    ctx.matrix_set(g["lap_a"])
+    for it in (1, 2, 5, 20, 2000):
+        x = np.zeros(mesh.n_cells)
+        rc, info = ctx.csrsolve(solver, x, g["in_b"], itr_max=it, tol_abs=0.0, tol_rel=1e-10)
+        ref = g[f"sol_{solver}_{it}"]
+        # only the dot/norm reduction order differs from the reference
+        assert np.allclose(x, ref, rtol=1e-11, atol=1e-13 * np.abs(ref).max()), (solver, it)
+        assert np.isclose(info["resor"], float(g[f"sol_{solver}_{it}_resor"]), rtol=1e-12)
+    ctx.close()
+
+
+def test_poisson32_iccg_vs_reference(api, golden_dir):
+    """32^3 Poisson, ICCG to 1e-10 (reference L1 criterion)."""
+    print("[case] poisson32 iccg", flush=True)
+    g = np.load(os.path.join(golden_dir, "poisson32.npz"))
+    mesh = workloads.poisson_uniform_mesh(32)
+    ctx, geo, csr = _setup(api, mesh)
+    inp = workloads.poisson_uniform_inputs(mesh, geo["vol"])
+    su = inp["rhs"].copy()
+    ctx.laplacian(inp["mu"], inp["phi"], su)
+    assert np.array_equal(su, g["rhs"])
+    x = np.zeros(mesh.n_cells)
+    rc, info = ctx.csrsolve("iccg", x, su, itr_max=10000, tol_abs=0.0, tol_rel=1e-10, verbose=1)
+    assert rc == 0
+    assert abs(info["iterations"] - int(g["iccg_iters"])) <= 1
+    ref = g["iccg_fi"]
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-10
+    ctx.close()
+
+
+def test_explicit_entry_points_5x5(api, golden_dir):
+    print("[case] explicit 5x5", flush=True)
+    g = np.load(os.path.join(golden_dir, "kat5x5.npz"))
+    for key in g.files:
+        if not key.startswith("x_") or "gauss" in key:
+            continue
+        _, which, *rest = key.split("_")
+        itmax = int(rest[-1])
+        solver = "_".join(rest[:-1])
+        x = np.zeros(5)
+        api.explicit_solve(solver, g["ia"], g["ja"], g["A_" + which].reshape(-1), g["diag"], x,
+                           g["b_" + which], itmax, 1e-10, 1e-7)
+        assert np.allclose(x, g[key], rtol=1e-10, atol=1e-12), key
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_jittered_poisson_step_true_l2(api, precision):
+    """Pressure-correction-style step at 24^3: face diffusivity, Dirichlet on
+    one patch, non-orthogonal correction; solve to ||b-Ax||/||b|| <= 1e-8 in
+    FP64 and in FP32-inner/FP64-refinement, compared with a tight oracle
+    solve."""
+    print(f"[case] jittered step 24^3 precision={precision}", flush=True)
+    mesh = workloads.poisson_jittered_mesh(24)
+    ctx, geo, csr = _setup(api, mesh)
+    o = Oracle(mesh)
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    su = inp["source"] * geo["vol"]
+    a = np.zeros(csr["ja"].size)
+    ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su, a_out=a)
+    gr = ctx.grad_gauss(inp["phi_prev"])
+    ctx.nonorth_correction(inp["gamma"], gr, su)
+    x = np.zeros(mesh.n_cells)
+    rc, info = ctx.csrsolve("iccg", x, su, itr_max=5000, tol_rel=1e-8, precision=precision,
+                            criterion=1)
+    assert rc == 0 and info["rel_res_l2"] <= 1e-8, info
+    A_r = o.spmv(a, x)
+    assert np.linalg.norm(su - A_r) / np.linalg.norm(su) <= 1.01e-8
+    ref = o.solve("iccg", a, su, itr_max=20000, tol_abs=0.0, tol_rel=1e-15)["fi"]
+    err = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+    assert err < 1e-5, err
+    ctx.close()
+
+
+def test_mixed_precision_converges_below_fp32(api):
+    """Refinement reaches residuals far below FP32 resolution."""
+    print("[case] mixed to 1e-12 20^3", flush=True)
+    mesh = workloads.poisson_jittered_mesh(20)
+    ctx, geo, csr = _setup(api, mesh)
+    o = Oracle(mesh)
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    su = inp["source"] * geo["vol"]
+    a = np.zeros(csr["ja"].size)
+    ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su, a_out=a)
+    x = np.zeros(mesh.n_cells)
+    rc, info = ctx.csrsolve("iccg", x, su, itr_max=5000, tol_rel=1e-12, precision=1, criterion=1)
+    assert rc == 0 and info["rel_res_l2"] <= 1e-12 and info["outer_iterations"] >= 2, info
+    ref = o.solve("iccg", a, su, itr_max=20000, tol_abs=0.0, tol_rel=1e-15)["fi"]
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-9
+    ctx.close()
+
+
+def test_determinism_run_twice(api):
+    print("[case] determinism", flush=True)
+    mesh = workloads.poisson_jittered_mesh(16)
+    outs = []
+    for _ in range(2):
+        ctx, geo, csr = _setup(api, mesh)
+        inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+        su = inp["source"] * geo["vol"]
+        ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su)
+        x = np.zeros(mesh.n_cells)
+        ctx.csrsolve("iccg", x, su, itr_max=500, tol_rel=1e-9, precision=1, criterion=1)
+        outs.append(x)
+        ctx.close()
+    assert np.array_equal(outs[0], outs[1])
