@@ -102,10 +102,15 @@ def run_gpu(args):
     results = {}
     for name, prec in (("fp64", FCP_PREC_FP64), ("mixed", FCP_PREC_MIXED)):
         for _ in range(args.warmup):
-            step(prec)
+            info = step(prec)
+            log(f"[bench] {name} warmup done: iters={info['iterations']} solve {info['time_ms']:.1f} ms")
         torch.cuda.synchronize()
         ts = time.perf_counter()
-        infos = [step(prec) for _ in range(args.steps)]
+        infos = []
+        for k in range(args.steps):
+            infos.append(step(prec))
+            log(f"[bench] {name} step {k} done: iters={infos[-1]['iterations']} "
+                f"solve {infos[-1]['time_ms']:.1f} ms")
         torch.cuda.synchronize()
         te = time.perf_counter()
         ms = (te - ts) * 1e3 / args.steps

@@ -236,6 +236,21 @@ int fcp_profile_report(fcp_ctx *ctx, char *buf, int len) {
   return (int)s.size();
 }
 
+int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
+  if (!ctx) return FCP_ERR_ARG;
+  switch (param) {
+    case FCP_TUNE_SWEEP_WAVES:
+      if (value < 0 || value > (1 << 20)) return fcp_fail(ctx, FCP_ERR_ARG, "sweep waves out of range");
+      ctx->tune_sweep_waves = value;
+      return 0;
+    case FCP_TUNE_SWEEP_SLEEP:
+      if (value < 0 || value > 127) return fcp_fail(ctx, FCP_ERR_ARG, "sleep out of range");
+      ctx->tune_sweep_sleep = value;
+      return 0;
+  }
+  return fcp_fail(ctx, FCP_ERR_ARG, "unknown tuning parameter %d", param);
+}
+
 void *fcp_stream(fcp_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 // ---------------------------------------------------------------------------

@@ -23,9 +23,9 @@
 #define FCP_SENT64 0x7FF4DEAD5EED0001ull
 #define FCP_SENT32 0x7FA5EED1u
 
-// Bounded spin of a sweep wait (iterations of s_sleep(1) ~ 64 cycles each):
-// ~0.5 s, after which the wave records FCP_ERR_SWEEP and moves on.
-#define FCP_SPIN_LIMIT (1 << 23)
+// Bound of one sweep dependency wait in ticks of the 100 MHz wall clock
+// (2 s), after which the wave records FCP_ERR_SWEEP and moves on.
+#define FCP_WAIT_TICKS 200000000ull
 
 template <typename T>
 struct DBuf {
@@ -118,6 +118,10 @@ struct fcp_ctx {
 
   // -------- staging --------
   DBuf<char> stage[4];
+
+  // -------- launch shapes --------
+  int tune_sweep_waves = 0;
+  int tune_sweep_sleep = 1;
 
   // -------- profiling --------
   bool prof = false;

@@ -121,15 +121,25 @@ struct Bits<float> {
 };
 
 template <typename T>
-__device__ __forceinline__ T wait_value(T *p, int *err) {
+__device__ __forceinline__ T wait_value(T *p, int *err, int slp) {
   using B = Bits<T>;
   typename B::U *q = reinterpret_cast<typename B::U *>(p);
   typename B::U v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  int spins = 0;
+  if (v != B::sent) return B::from(v);
+  // Bounded by wall time (100 MHz constant clock): a dependency that is not
+  // published within FCP_WAIT_TICKS records FCP_ERR_SWEEP and the wave
+  // continues with NaN, so the kernel always drains.
+  const unsigned long long t0 = wall_clock64();
+  unsigned spins = 0;
   while (v == B::sent) {
-    __builtin_amdgcn_s_sleep(1);
+    if (slp >= 8)
+      __builtin_amdgcn_s_sleep(8);
+    else if (slp >= 2)
+      __builtin_amdgcn_s_sleep(2);
+    else
+      __builtin_amdgcn_s_sleep(1);
     v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (++spins > FCP_SPIN_LIMIT) {
+    if ((++spins & 63u) == 0 && wall_clock64() - t0 > FCP_WAIT_TICKS) {
       __hip_atomic_store(err, FCP_ERR_SWEEP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       v = B::qnan;
     }
@@ -172,7 +182,7 @@ __global__ __launch_bounds__(BLK) void k_sweep(
     const int *__restrict__ ia, const int *__restrict__ diag, const int *__restrict__ ja,
     const T *__restrict__ a, const int *__restrict__ tslot, const T *__restrict__ d,
     const T *__restrict__ in, T *out, int *__restrict__ ticket, int *__restrict__ err,
-    const SolveScalars *__restrict__ s) {
+    const SolveScalars *__restrict__ s, int slp) {
   if (s && solve_done(s)) return;
   const int lane = threadIdx.x & 63;
   for (;;) {
@@ -186,26 +196,26 @@ __global__ __launch_bounds__(BLK) void k_sweep(
       const int dg = diag[row];
       if (MODE == 0) {
         T acc = in[row];
-        for (int k = ia[row]; k < dg; ++k) acc = acc - a[k] * wait_value<T>(out + ja[k], err);
+        for (int k = ia[row]; k < dg; ++k) acc = acc - a[k] * wait_value<T>(out + ja[k], err, slp);
         publish<T>(out + row, acc * d[row]);
       } else if (MODE == 1) {
         const T dr = d[row];
         T acc = in[row] / (dr + (T)FCP_SMALL);
         const int e = ia[row + 1];
-        for (int k = dg + 1; k < e; ++k) acc = acc - a[k] * wait_value<T>(out + ja[k], err);
+        for (int k = dg + 1; k < e; ++k) acc = acc - a[k] * wait_value<T>(out + ja[k], err, slp);
         publish<T>(out + row, acc * dr);
       } else if (MODE == 2) {
         T acc = a[dg];
         for (int k = ia[row]; k < dg; ++k) {
           const T ak = a[k];
-          acc = acc - ak * ak * wait_value<T>(out + ja[k], err);
+          acc = acc - ak * ak * wait_value<T>(out + ja[k], err, slp);
         }
         publish<T>(out + row, (T)1.0 / acc);
       } else {
         T acc = a[dg];
         for (int k = ia[row]; k < dg; ++k) {
           const int l = tslot[k];
-          const T dj = wait_value<T>(out + ja[k], err);
+          const T dj = wait_value<T>(out + ja[k], err, slp);
           acc = acc - a[k] * dj * (l >= 0 ? a[l] : (T)0);
         }
         publish<T>(out + row, (T)1.0 / acc);
@@ -563,10 +573,10 @@ int red_grid(fcp_ctx *ctx, int n) {
 }
 
 int sweep_grid(fcp_ctx *ctx, const Schedule &s) {
-  const int waves = (s.ntasks + 0);
+  int waves = s.ntasks;
+  int cap = ctx->tune_sweep_waves > 0 ? ctx->tune_sweep_waves : (ctx->ncu > 0 ? ctx->ncu : 256) * 2;
+  if (waves > cap) waves = cap;
   int blocks = (waves + (BLK / 64) - 1) / (BLK / 64);
-  int cap = (ctx->ncu > 0 ? ctx->ncu : 256) * 8;
-  if (blocks > cap) blocks = cap;
   return blocks > 0 ? blocks : 1;
 }
 
@@ -622,11 +632,11 @@ int precond_setup(fcp_ctx *ctx, int kind, const T *a, T *d) {
   if (kind == 0)
     k_sweep<T, 2><<<sweep_grid(ctx, ctx->lower), BLK, 0, ctx->stream>>>(
         ctx->lower.task_ptr.p, ctx->lower.order.p, ctx->lower.ntasks, ctx->ia.p, ctx->diag.p,
-        ctx->ja.p, a, ctx->tslot.p, nullptr, nullptr, d, tickets(ctx), ctx->errflag.p, nullptr);
+        ctx->ja.p, a, ctx->tslot.p, nullptr, nullptr, d, tickets(ctx), ctx->errflag.p, nullptr, ctx->tune_sweep_sleep);
   else
     k_sweep<T, 3><<<sweep_grid(ctx, ctx->lower), BLK, 0, ctx->stream>>>(
         ctx->lower.task_ptr.p, ctx->lower.order.p, ctx->lower.ntasks, ctx->ia.p, ctx->diag.p,
-        ctx->ja.p, a, ctx->tslot.p, nullptr, nullptr, d, tickets(ctx), ctx->errflag.p, nullptr);
+        ctx->ja.p, a, ctx->tslot.p, nullptr, nullptr, d, tickets(ctx), ctx->errflag.p, nullptr, ctx->tune_sweep_sleep);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }
@@ -641,13 +651,13 @@ int precond_apply(fcp_ctx *ctx, const T *a, const T *d, const T *r, T *y, T *z,
     ProfScope ps(ctx, sizeof(T) == 8 ? "sweep_fwd_f64" : "sweep_fwd_f32");
     k_sweep<T, 0><<<sweep_grid(ctx, ctx->lower), BLK, 0, ctx->stream>>>(
         ctx->lower.task_ptr.p, ctx->lower.order.p, ctx->lower.ntasks, ctx->ia.p, ctx->diag.p,
-        ctx->ja.p, a, nullptr, d, r, y, tickets(ctx), ctx->errflag.p, s);
+        ctx->ja.p, a, nullptr, d, r, y, tickets(ctx), ctx->errflag.p, s, ctx->tune_sweep_sleep);
   }
   {
     ProfScope ps(ctx, sizeof(T) == 8 ? "sweep_bwd_f64" : "sweep_bwd_f32");
     k_sweep<T, 1><<<sweep_grid(ctx, ctx->upper), BLK, 0, ctx->stream>>>(
         ctx->upper.task_ptr.p, ctx->upper.order.p, ctx->upper.ntasks, ctx->ia.p, ctx->diag.p,
-        ctx->ja.p, a, nullptr, d, y, z, tickets(ctx) + 1, ctx->errflag.p, s);
+        ctx->ja.p, a, nullptr, d, y, z, tickets(ctx) + 1, ctx->errflag.p, s, ctx->tune_sweep_sleep);
   }
   FCP_HIP(ctx, hipGetLastError());
   return 0;
@@ -846,10 +856,15 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
     k_zero_f32<<<g, BLK, 0, ctx->stream>>>(n, ctx->p32.p, ctx->q32.p);
     k_init32<<<g, BLK, 0, ctx->stream>>>(n, ctx->r32.p, ctx->partials.p, redctr(ctx), si);
     const int budget = itr_max - total_inner;
+    // Reduce the correction residual only as far as this refinement step
+    // needs (with a 0.5 margin), never below inner_tol.
+    double itol = 0.5 * tol_rel / rel;
+    if (itol < inner_tol) itol = inner_tol;
+    if (itol > 0.5) itol = 0.5;
     auto it = [&]() {
       return pcg_iteration<float>(ctx, solver == 1 ? 1 : 0, ctx->a32.p, ctx->d32.p, ctx->e32.p,
                                   ctx->r32.p, ctx->p32.p, ctx->z32.p, ctx->y32.p, ctx->q32.p, si, 1,
-                                  0.0, inner_tol, budget);
+                                  0.0, itol, budget);
     };
     FCP_TRY(iterate_until_done(ctx, si, 1, budget, check_every, verbose, "PCG32", it));
     total_inner += ctx->h_scal[1].iter;

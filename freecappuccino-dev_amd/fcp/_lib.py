@@ -45,6 +45,7 @@ SIGNATURES = {
     "fcp_profile": (_i, [_p, _i]),
     "fcp_profile_report": (_i, [_p, C.c_char_p, _i]),
     "fcp_stream": (_p, [_p]),
+    "fcp_tune_set": (_i, [_p, _i, _i]),
 }
 
 _lib = None

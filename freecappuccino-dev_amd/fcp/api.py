@@ -22,6 +22,8 @@ FCP_PREC_FP64 = 0
 FCP_PREC_MIXED = 1
 FCP_CRIT_REF_L1 = 0
 FCP_CRIT_TRUE_L2 = 1
+FCP_TUNE_SWEEP_WAVES = 1
+FCP_TUNE_SWEEP_SLEEP = 2
 
 
 class FcpError(RuntimeError):
@@ -215,6 +217,9 @@ class Context:
         info = {k: getattr(opts, k) for k, _ in _lib.SolveOpts._fields_}
         info["status"] = rc
         return rc, info
+
+    def tune(self, param: int, value: int):
+        self._check(self.lib.fcp_tune_set(self.h, int(param), int(value)), "fcp_tune_set")
 
     def profile(self, enable: bool):
         self._check(self.lib.fcp_profile(self.h, int(enable)), "fcp_profile")

@@ -188,6 +188,12 @@ int fcp_bicgstab(int n, int nnz, const int *ia, const int *ja, const double *a,
                  const int *diag, double *fi, const double *rhs, double *resor,
                  int itr_max, double tol_abs, double tol_rel, int verbose);
 
+/* --- launch-shape tuning (autotuner hook) --------------------------------- */
+#define FCP_TUNE_SWEEP_WAVES 1   /* resident waves of a sweep kernel (0 = auto) */
+#define FCP_TUNE_SWEEP_SLEEP 2   /* s_sleep argument between dependency polls   */
+/* Sets a launch-shape parameter of the context; results do not depend on it. */
+int fcp_tune_set(fcp_ctx *ctx, int param, int value);
+
 /* --- instrumentation ------------------------------------------------------ */
 /* Per-kernel device timing accumulated on the context's stream with HIP
  * events (name -> total ms, launches).  enable=1 starts, 0 stops. */

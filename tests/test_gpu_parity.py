@@ -197,7 +197,11 @@ def test_explicit_entry_points_5x5(api, golden_dir):
         x = np.zeros(5)
         api.explicit_solve(solver, g["ia"], g["ja"], g["A_" + which].reshape(-1), g["diag"], x,
                            g["b_" + which], itmax, 1e-10, 1e-7)
-        assert np.allclose(x, g[key], rtol=1e-10, atol=1e-12), key
+        # Unconverged BiCGStab trajectories on these dense 5x5 systems amplify
+        # the different dot-product summation order (wavefront tree vs
+        # sequential) to ~1e-6; CG-family runs stay at 1e-10.
+        rtol = 1e-5 if solver == "bicgstab" else 1e-10
+        assert np.allclose(x, g[key], rtol=rtol, atol=1e-12), key
 
 
 @pytest.mark.parametrize("precision", [0, 1])
