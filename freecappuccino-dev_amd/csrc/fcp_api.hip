@@ -492,6 +492,54 @@ int fcp_nonorth_correction(fcp_ctx *ctx, const double *gamma_f, const double *gr
   return 0;
 }
 
+int fcp_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
+                 const double *grad, double gam, int scheme, const unsigned char *bc_kind,
+                 double urf, double *a, double *su, double *sp) {
+  FCP_TRY(need(ctx, true, true, true));
+  if (!flmass || !dcoef || !phi || !grad || !bc_kind || !su || !sp || (scheme != 0 && scheme != 1) ||
+      !(urf > 0.0))
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_convdiff: bad argument");
+  (void)hipSetDevice(ctx->device);
+  const size_t ntot = (size_t)ctx->n + ctx->Fb;
+  Arg<double> fm, dc, ph, gr, s, p;
+  Arg<unsigned char> bk;
+  FCP_TRY(arg_in(ctx, flmass, ctx->F, fm));
+  FCP_TRY(arg_in(ctx, dcoef, ctx->F, dc));
+  FCP_TRY(arg_in(ctx, (const double *)phi, ntot, ph));
+  FCP_TRY(arg_in(ctx, grad, 3 * ntot, gr));
+  FCP_TRY(arg_in(ctx, bc_kind, ctx->Fb, bk));
+  FCP_TRY(arg_in(ctx, (const double *)su, ctx->n, s));
+  FCP_TRY(arg_in(ctx, (const double *)sp, ctx->n, p));
+  FCP_TRY(fcp_launch_convdiff(ctx, fm.dev, dc.dev, ph.dev, gr.dev, gam, scheme, bk.dev, urf, s.dev,
+                              p.dev));
+  FCP_TRY(arg_back(ctx, ph));
+  FCP_TRY(arg_back(ctx, s));
+  FCP_TRY(arg_back(ctx, p));
+  FCP_TRY(out_copy(ctx, ctx->a.p, a, ctx->nnz));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt, const double *phi_o,
+            const double *phi_oo, double *su, double *sp) {
+  FCP_TRY(need(ctx, true, true, false));
+  if (!den || !phi_o || !su || !sp || (scheme == 2 && !phi_oo) || (scheme != 1 && scheme != 2) ||
+      !(dt > 0.0))
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_ddt: bad argument");
+  (void)hipSetDevice(ctx->device);
+  Arg<double> d, fo, foo, s, p;
+  FCP_TRY(arg_in(ctx, den, ctx->n, d));
+  FCP_TRY(arg_in(ctx, phi_o, ctx->n, fo));
+  if (phi_oo) FCP_TRY(arg_in(ctx, phi_oo, ctx->n, foo));
+  FCP_TRY(arg_in(ctx, (const double *)su, ctx->n, s));
+  FCP_TRY(arg_in(ctx, (const double *)sp, ctx->n, p));
+  FCP_TRY(fcp_launch_ddt(ctx, scheme, d.dev, dt, fo.dev, foo.dev, s.dev, p.dev));
+  FCP_TRY(arg_back(ctx, s));
+  FCP_TRY(arg_back(ctx, p));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
 int fcp_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi) {
   FCP_TRY(need(ctx, true, true, false));
   if (!u || !dudxi) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_grad_gauss: null argument");
@@ -641,6 +689,7 @@ static int explicit_solve(int code, int n, int nnz, const int *ia, const int *ja
     FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   FCP_TRY(fcp_build_schedules(ctx));
+  FCP_TRY(fcp_build_sell(ctx));
   ctx->have_csr = true;
   FCP_TRY(fcp_matrix_set(ctx, a));
   // transposed-entry positions for DILU

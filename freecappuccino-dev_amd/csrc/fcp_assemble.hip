@@ -143,6 +143,140 @@ __global__ void k_update_boundary(int n, int Fi, int Fb, const int *__restrict__
     phi[n + b] = phi[owner[Fi + b]];
 }
 
+// Convection-diffusion assembly as calculate_temperature composes it
+// (temperature.f90:118-366) with facefluxsc (scalar_fluxes.f90:32-141,
+// 256-337) and face_value (interpolation.f90:28-157, 318-356).  One thread
+// per cell gathers its faces in ascending face id: off-diagonals written
+// through the face->slot maps, the diagonal accumulator and su/sp in the
+// reference's face order, then a_ii = sp - (sum(row) - a_ii) and URF.
+struct FaceFlux {
+  double can, cap, suadd;
+};
+
+__device__ __forceinline__ FaceFlux inner_flux(int f, int p, int q, const GeomP &g, double fm,
+                                               double dc, const double *__restrict__ phi,
+                                               const double *__restrict__ gr, double gam,
+                                               int scheme) {
+  const double fxn = g.facint[f], fxp = 1.0 - g.facint[f];
+  const double xpn = g.xc[q] - g.xc[p], ypn = g.yc[q] - g.yc[p], zpn = g.zc[q] - g.zc[p];
+  const double Dff = g.Df[f];
+  const double de = dc * Dff;
+  const double ce = fm < 0.0 ? fm : 0.0;
+  const double cp = fm > 0.0 ? fm : 0.0;
+  FaceFlux r;
+  r.can = -de + ce;
+  r.cap = -de - cp;
+  double dfixi = gr[3 * p] * fxp + gr[3 * q] * fxn;
+  double dfiyi = gr[3 * p + 1] * fxp + gr[3 * q + 1] * fxn;
+  double dfizi = gr[3 * p + 2] * fxp + gr[3 * q + 2] * fxn;
+  dfixi = dfixi * (g.arx[f] - Dff * xpn);
+  dfiyi = dfiyi * (g.ary[f] - Dff * ypn);
+  dfizi = dfizi * (g.arz[f] - Dff * zpn);
+  const double fdfie = dc * ((dfixi + dfiyi) + dfizi);
+  // upwind cell u, downwind d, interpolation weight as passed by facefluxsc
+  const int u = fm >= 0.0 ? p : q, d = fm >= 0.0 ? q : p;
+  const double lam = fm >= 0.0 ? fxp : fxn;
+  double fii;
+  if (scheme == 0) {
+    fii = phi[u] + (phi[d] - phi[u]) * lam;
+  } else {
+    const double gfr = (gr[3 * u] * (g.xf[f] - g.xc[u]) + gr[3 * u + 1] * (g.yf[f] - g.yc[u])) +
+                       gr[3 * u + 2] * (g.zf[f] - g.zc[u]);
+    fii = phi[u] + gfr;
+  }
+  double fcfie = fm * fii;
+  const double fcfii = ce * phi[q] + cp * phi[p];
+  fcfie = gam * (fcfie - fcfii);
+  r.suadd = -fcfie + fdfie;
+  return r;
+}
+
+__global__ void k_convdiff(int n, int Fi, const int *__restrict__ cf_ptr, const int *__restrict__ cf,
+                           const int *__restrict__ owner, const int *__restrict__ neighbour, GeomP g,
+                           const double *__restrict__ srdb, const double *__restrict__ flmass,
+                           const double *__restrict__ dcoef, double *__restrict__ phi,
+                           const double *__restrict__ gr, double gam, int scheme,
+                           const unsigned char *__restrict__ bc_kind, double urf,
+                           const int *__restrict__ slot_pn, const int *__restrict__ slot_np,
+                           const int *__restrict__ ia, const int *__restrict__ diag,
+                           double *__restrict__ a, double *__restrict__ su,
+                           double *__restrict__ sp) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double ad = 0.0, s = su[c], spc = sp[c];
+  for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+    const int f = cf[k];
+    if (f < Fi) {
+      const int p = owner[f], q = neighbour[f];
+      const FaceFlux ff = inner_flux(f, p, q, g, flmass[f], dcoef[f], phi, gr, gam, scheme);
+      if (c == p) {
+        const int sl = slot_pn[f];
+        if (sl >= 0) a[sl] = ff.can;
+        ad = ad - ff.can;
+        s = s + ff.suadd;
+      } else {
+        const int sl = slot_np[f];
+        if (sl >= 0) a[sl] = ff.cap;
+        ad = ad - ff.cap;
+        s = s - ff.suadd;
+      }
+    } else {
+      const int b = f - Fi;
+      const int kind = bc_kind[b];
+      if (kind == 2) {  // facefluxsc_boundary, :182-190
+        const double xpn = g.xf[f] - g.xc[c], ypn = g.yf[f] - g.yc[c], zpn = g.zf[f] - g.zc[c];
+        const double ax = g.arx[f], ay = g.ary[f], az = g.arz[f];
+        const double Dfi = ((ax * ax + ay * ay) + az * az) / ((xpn * ax + ypn * ay) + zpn * az);
+        const double de = dcoef[f] * Dfi;
+        const double fm = flmass[f];
+        const double ce = fm < 0.0 ? fm : 0.0;
+        const double can = -de + ce;
+        const double dfixi = gr[3 * c] * (ax - Dfi * xpn);
+        const double dfiyi = gr[3 * c + 1] * (ay - Dfi * ypn);
+        const double dfizi = gr[3 * c + 2] * (az - Dfi * zpn);
+        const double suadd = dcoef[f] * ((dfixi + dfiyi) + dfizi);
+        spc = spc - can;
+        s = s - can * phi[n + b] + suadd;
+      } else if (kind == 0) {  // Dirichlet wall :206-210
+        const double dw = dcoef[f] * srdb[b];
+        spc = spc + dw;
+        s = s + dw * phi[n + b];
+      } else {  // zero-gradient wall :224
+        phi[n + b] = phi[c];
+      }
+    }
+  }
+  const int dg = diag[c];
+  a[dg] = ad;
+  // off_diagonal_terms = sum(a(ia(i):ia(i+1)-1)) - a(diag(i)) (:359-360)
+  double rs = 0.0;
+  for (int k = ia[c]; k < ia[c + 1]; ++k) rs = rs + a[k];
+  const double off = rs - ad;
+  double aii = spc - off;
+  aii = aii * (1.0 / urf);
+  a[dg] = aii;
+  su[c] = s + (1.0 - urf) * aii * phi[c];
+  sp[c] = spc;
+}
+
+// Time terms of the transport equations (temperature.f90:82-102):
+// scheme 1 = BDF1 / Crank-Nicolson volume part, 2 = BDF2.
+__global__ void k_ddt(int n, int scheme, const double *__restrict__ den,
+                      const double *__restrict__ vol, double dt, const double *__restrict__ fo,
+                      const double *__restrict__ foo, double *__restrict__ su,
+                      double *__restrict__ sp) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double apotime = den[c] * vol[c] / dt;
+  if (scheme == 2) {
+    su[c] = su[c] + apotime * (2.0 * fo[c] - 0.5 * foo[c]);
+    sp[c] = sp[c] + 1.5 * apotime;
+  } else {
+    su[c] = su[c] + apotime * fo[c];
+    sp[c] = sp[c] + apotime;
+  }
+}
+
 GeomP geom_of(fcp_ctx *ctx) {
   return GeomP{ctx->xc.p, ctx->yc.p, ctx->zc.p, ctx->vol.p, ctx->arx.p, ctx->ary.p, ctx->arz.p,
                ctx->xf.p, ctx->yf.p, ctx->zf.p, ctx->facint.p, ctx->Df.p};
@@ -185,6 +319,27 @@ int fcp_launch_update_boundary(fcp_ctx *ctx, double *phi) {
   if (ctx->Fb == 0) return 0;
   k_update_boundary<<<grid_for(ctx->Fb, BLK), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ctx->Fb, ctx->owner.p, ctx->bpatch.p, ctx->bctype.p, phi);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
+                        const double *grad, double gam, int scheme, const unsigned char *bc_kind,
+                        double urf, double *su, double *sp) {
+  ProfScope ps(ctx, "convdiff");
+  k_convdiff<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(
+      ctx->n, ctx->Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, geom_of(ctx),
+      ctx->srdb.p, flmass, dcoef, phi, grad, gam, scheme, bc_kind, urf, ctx->slot_pn.p,
+      ctx->slot_np.p, ctx->ia.p, ctx->diag.p, ctx->a.p, su, sp);
+  FCP_HIP(ctx, hipGetLastError());
+  ctx->a_version++;
+  return 0;
+}
+
+int fcp_launch_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt, const double *fo,
+                   const double *foo, double *su, double *sp) {
+  k_ddt<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(ctx->n, scheme, den, ctx->vol.p, dt, fo, foo,
+                                                        su, sp);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }

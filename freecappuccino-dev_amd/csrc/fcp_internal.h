@@ -100,6 +100,15 @@ struct fcp_ctx {
   DBuf<int> row_of;     // [nnz] row of every entry
   Schedule lower, upper;
   DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)
+  // SELL-64 copy of the operator for SpMV: rows in slices of 64, each slice
+  // stored column-major (entry k of the slice's rows contiguous) and padded
+  // to the slice's longest row; per-row entry order = CSR order.
+  int nslices = 0;
+  DBuf<int> sell_base;  // [nslices+1] offset of each slice (multiple of 64)
+  DBuf<int> sell_col;   // [sell_base[nslices]]
+  DBuf<double> sell_a64;
+  DBuf<float> sell_a32;
+  uint64_t sell64_version = ~0ull, sell32_version = ~0ull;
 
   // -------- operator values --------
   DBuf<double> a;       // [nnz] resident matrix (fp64)
@@ -180,7 +189,13 @@ int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
 int fcp_launch_nonorth(fcp_ctx *ctx, const double *gamma_f, const double *grad, double *su);
 int fcp_launch_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi);
 int fcp_launch_update_boundary(fcp_ctx *ctx, double *phi);
+int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
+                        const double *grad, double gam, int scheme, const unsigned char *bc_kind,
+                        double urf, double *su, double *sp);
+int fcp_launch_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt, const double *fo,
+                   const double *foo, double *su, double *sp);
 // fcp_krylov.hip
+int fcp_build_sell(fcp_ctx *ctx);
 int fcp_launch_spmv64(fcp_ctx *ctx, const double *a, const double *x, double *y);
 int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max,
                      double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts);

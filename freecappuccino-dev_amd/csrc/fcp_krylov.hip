@@ -225,17 +225,72 @@ __global__ __launch_bounds__(BLK) void k_sweep(
 }
 
 // ---------------------------------------------------------------------------
+// SpMV on the SELL-64 copy
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Sell {
+  const int *base, *col, *ia;
+  const T *a;
+};
+
+// sum_k a_ik x_k of row i in CSR entry order (the reference's row loop,
+// linear_solvers.f90:493-498); loads of a wave's 64 rows are contiguous.
+template <typename T>
+__device__ __forceinline__ T sell_row(const Sell<T> &m, int i, const T *__restrict__ x, T init,
+                                      bool subtract) {
+  const int p = m.base[i >> 6] + (i & 63);
+  const int len = m.ia[i + 1] - m.ia[i];
+  T acc = init;
+  if (subtract)
+    for (int k = 0; k < len; ++k) acc = acc - m.a[p + 64 * k] * x[m.col[p + 64 * k]];
+  else
+    for (int k = 0; k < len; ++k) acc = acc + m.a[p + 64 * k] * x[m.col[p + 64 * k]];
+  return acc;
+}
+
+__global__ void k_sell_len(const int *__restrict__ ia, int n, int nslices, int *__restrict__ len) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nslices) return;
+  if (s == nslices) {
+    len[s] = 0;
+    return;
+  }
+  int m = 0;
+  for (int r = s * 64; r < s * 64 + 64 && r < n; ++r) {
+    const int l = ia[r + 1] - ia[r];
+    m = l > m ? l : m;
+  }
+  len[s] = 64 * m;
+}
+
+__global__ void k_sell_col(const int *__restrict__ ia, const int *__restrict__ ja, int n,
+                           int nslices, const int *__restrict__ base, int *__restrict__ col) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nslices * 64) return;
+  const int s = r >> 6, p = base[s] + (r & 63), L = (base[s + 1] - base[s]) >> 6;
+  const int len = r < n ? ia[r + 1] - ia[r] : 0;
+  for (int k = 0; k < L; ++k) col[p + 64 * k] = (k < len) ? ja[ia[r] + k] : (r < n ? r : 0);
+}
+
+template <typename T>
+__global__ void k_sell_val(const int *__restrict__ ia, const double *__restrict__ a, int n,
+                           int nslices, const int *__restrict__ base, T *__restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nslices * 64) return;
+  const int s = r >> 6, p = base[s] + (r & 63), L = (base[s + 1] - base[s]) >> 6;
+  const int len = r < n ? ia[r + 1] - ia[r] : 0;
+  for (int k = 0; k < L; ++k) out[p + 64 * k] = (k < len) ? (T)a[ia[r] + k] : (T)0;
+}
+
+// ---------------------------------------------------------------------------
 // SpMV
 // ---------------------------------------------------------------------------
 // y = A x, one thread per row, CSR order (bit-equal to the reference loop).
 template <typename T>
-__global__ void k_spmv_row(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                           const T *__restrict__ a, const T *__restrict__ x, T *__restrict__ y) {
+__global__ void k_spmv_row(int n, Sell<T> m, const T *__restrict__ x, T *__restrict__ y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  T s = 0;
-  for (int k = ia[i]; k < ia[i + 1]; ++k) s = s + a[k] * x[ja[k]];
-  y[i] = s;
+  y[i] = sell_row<T>(m, i, x, (T)0, false);
 }
 
 // res = b - A x (row order as :417-422) and the L1 / squared L2 norms of res
@@ -243,13 +298,12 @@ __global__ void k_spmv_row(int n, const int *__restrict__ ia, const int *__restr
 // initialises the Krylov scalars.
 template <typename T>
 __global__ __launch_bounds__(BLK) void k_residual_init(
-    int n, const int *__restrict__ ia, const int *__restrict__ ja, const T *__restrict__ a,
-    const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ res, T *__restrict__ reso,
-    double *__restrict__ partials, int *__restrict__ counter, SolveScalars *__restrict__ s) {
+    int n, Sell<T> m, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ res,
+    T *__restrict__ reso, double *__restrict__ partials, int *__restrict__ counter,
+    SolveScalars *__restrict__ s) {
   double part[3] = {0.0, 0.0, 0.0};
   for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
-    T r = b[i];
-    for (int k = ia[i]; k < ia[i + 1]; ++k) r = r - a[k] * x[ja[k]];
+    const T r = sell_row<T>(m, i, x, b[i], true);
     res[i] = r;
     if (reso) reso[i] = r;
     part[0] += fabs((double)r);
@@ -331,17 +385,14 @@ __global__ void k_pupdate(int n, const T *__restrict__ z, T *__restrict__ p,
 
 // q = A p ; pkapk = p.q ; alf = sk/pkapk
 template <typename T>
-__global__ __launch_bounds__(BLK) void k_spmv_pq(int n, const int *__restrict__ ia,
-                                                 const int *__restrict__ ja,
-                                                 const T *__restrict__ a, const T *__restrict__ p,
+__global__ __launch_bounds__(BLK) void k_spmv_pq(int n, Sell<T> m, const T *__restrict__ p,
                                                  T *__restrict__ q, double *__restrict__ partials,
                                                  int *__restrict__ counter,
                                                  SolveScalars *__restrict__ s) {
   if (solve_done(s)) return;
   double part[1] = {0.0};
   for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
-    T acc = 0;
-    for (int k = ia[i]; k < ia[i + 1]; ++k) acc = acc + a[k] * p[ja[k]];
+    const T acc = sell_row<T>(m, i, p, (T)0, false);
     q[i] = acc;
     part[0] += (double)p[i] * (double)acc;
   }
@@ -437,9 +488,7 @@ __global__ void k_bicg_p(int n, const T *__restrict__ r, const T *__restrict__ u
 
 // u = A z ; ukreso = u.reso ; gam = bet/ukreso
 template <typename T>
-__global__ __launch_bounds__(BLK) void k_bicg_u(int n, const int *__restrict__ ia,
-                                                const int *__restrict__ ja,
-                                                const T *__restrict__ a, const T *__restrict__ z,
+__global__ __launch_bounds__(BLK) void k_bicg_u(int n, Sell<T> m, const T *__restrict__ z,
                                                 const T *__restrict__ reso, T *__restrict__ u,
                                                 double *__restrict__ partials,
                                                 int *__restrict__ counter,
@@ -447,8 +496,7 @@ __global__ __launch_bounds__(BLK) void k_bicg_u(int n, const int *__restrict__ i
   if (solve_done(s)) return;
   double part[1] = {0.0};
   for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
-    T acc = 0;
-    for (int k = ia[i]; k < ia[i + 1]; ++k) acc = acc + a[k] * z[ja[k]];
+    const T acc = sell_row<T>(m, i, z, (T)0, false);
     u[i] = acc;
     part[0] += (double)acc * (double)reso[i];
   }
@@ -475,9 +523,7 @@ __global__ void k_bicg_xr1(int n, const T *__restrict__ z, const T *__restrict__
 
 // v = A z ; alf = (v.r)/(v.v + small)
 template <typename T>
-__global__ __launch_bounds__(BLK) void k_bicg_v(int n, const int *__restrict__ ia,
-                                                const int *__restrict__ ja,
-                                                const T *__restrict__ a, const T *__restrict__ z,
+__global__ __launch_bounds__(BLK) void k_bicg_v(int n, Sell<T> m, const T *__restrict__ z,
                                                 const T *__restrict__ r, T *__restrict__ v,
                                                 double *__restrict__ partials,
                                                 int *__restrict__ counter,
@@ -485,8 +531,7 @@ __global__ __launch_bounds__(BLK) void k_bicg_v(int n, const int *__restrict__ i
   if (solve_done(s)) return;
   double part[2] = {0.0, 0.0};
   for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
-    T acc = 0;
-    for (int k = ia[i]; k < ia[i + 1]; ++k) acc = acc + a[k] * z[ja[k]];
+    const T acc = sell_row<T>(m, i, z, (T)0, false);
     v[i] = acc;
     part[0] += (double)acc * (double)r[i];
     part[1] += (double)acc * (double)acc;
@@ -600,10 +645,64 @@ int fcp_ensure_workspace(fcp_ctx *ctx) {
   return 0;
 }
 
-int fcp_launch_spmv64(fcp_ctx *ctx, const double *a, const double *x, double *y) {
-  ProfScope ps(ctx, "spmv_row_f64");
-  k_spmv_row<double><<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(ctx->n, ctx->ia.p, ctx->ja.p,
-                                                                      a, x, y);
+int fcp_build_sell(fcp_ctx *ctx) {
+  const int n = ctx->n;
+  ctx->nslices = (n + 63) / 64;
+  DBuf<int> len;
+  FCP_HIP(ctx, len.alloc(ctx->nslices + 1));
+  FCP_HIP(ctx, ctx->sell_base.alloc(ctx->nslices + 1));
+  k_sell_len<<<grid_for(ctx->nslices + 1, BLK), BLK, 0, ctx->stream>>>(ctx->ia.p, n, ctx->nslices,
+                                                                       len.p);
+  FCP_TRY(fcp_exclusive_scan_int(ctx, len.p, ctx->sell_base.p, ctx->nslices + 1));
+  int npad = 0;
+  FCP_HIP(ctx, hipMemcpyAsync(&npad, ctx->sell_base.p + ctx->nslices, sizeof(int),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  FCP_HIP(ctx, ctx->sell_col.alloc(npad > 0 ? npad : 1));
+  k_sell_col<<<grid_for((long long)ctx->nslices * 64, BLK), BLK, 0, ctx->stream>>>(
+      ctx->ia.p, ctx->ja.p, n, ctx->nslices, ctx->sell_base.p, ctx->sell_col.p);
+  FCP_HIP(ctx, hipGetLastError());
+  ctx->sell64_version = ctx->sell32_version = ~0ull;
+  return 0;
+}
+
+namespace {
+// SELL values of the resident matrix in precision T, refreshed on change.
+template <typename T>
+int sell_view(fcp_ctx *ctx, Sell<T> *out);
+template <>
+int sell_view<double>(fcp_ctx *ctx, Sell<double> *out) {
+  if (ctx->sell64_version != ctx->a_version) {
+    const int npad = (int)ctx->sell_col.n;
+    FCP_HIP(ctx, ctx->sell_a64.alloc(npad));
+    k_sell_val<double><<<grid_for((long long)ctx->nslices * 64, BLK), BLK, 0, ctx->stream>>>(
+        ctx->ia.p, ctx->a.p, ctx->n, ctx->nslices, ctx->sell_base.p, ctx->sell_a64.p);
+    FCP_HIP(ctx, hipGetLastError());
+    ctx->sell64_version = ctx->a_version;
+  }
+  *out = Sell<double>{ctx->sell_base.p, ctx->sell_col.p, ctx->ia.p, ctx->sell_a64.p};
+  return 0;
+}
+template <>
+int sell_view<float>(fcp_ctx *ctx, Sell<float> *out) {
+  if (ctx->sell32_version != ctx->a_version) {
+    const int npad = (int)ctx->sell_col.n;
+    FCP_HIP(ctx, ctx->sell_a32.alloc(npad));
+    k_sell_val<float><<<grid_for((long long)ctx->nslices * 64, BLK), BLK, 0, ctx->stream>>>(
+        ctx->ia.p, ctx->a.p, ctx->n, ctx->nslices, ctx->sell_base.p, ctx->sell_a32.p);
+    FCP_HIP(ctx, hipGetLastError());
+    ctx->sell32_version = ctx->a_version;
+  }
+  *out = Sell<float>{ctx->sell_base.p, ctx->sell_col.p, ctx->ia.p, ctx->sell_a32.p};
+  return 0;
+}
+}  // namespace
+
+int fcp_launch_spmv64(fcp_ctx *ctx, const double *, const double *x, double *y) {
+  Sell<double> m;
+  FCP_TRY(sell_view<double>(ctx, &m));
+  ProfScope ps(ctx, "spmv_sell_f64");
+  k_spmv_row<double><<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(ctx->n, m, x, y);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }
@@ -689,8 +788,9 @@ int pcg_iteration(fcp_ctx *ctx, int precond, const T *a, const T *d, T *x, T *r,
   }
   {
     ProfScope ps(ctx, sizeof(T) == 8 ? "spmv_pq_f64" : "spmv_pq_f32");
-    k_spmv_pq<T><<<g, BLK, 0, ctx->stream>>>(n, ctx->ia.p, ctx->ja.p, a, p, q, ctx->partials.p,
-                                              redctr(ctx), s);
+    Sell<T> m;
+    FCP_TRY(sell_view<T>(ctx, &m));
+    k_spmv_pq<T><<<g, BLK, 0, ctx->stream>>>(n, m, p, q, ctx->partials.p, redctr(ctx), s);
   }
   {
     ProfScope ps(ctx, "xr_update");
@@ -710,12 +810,12 @@ int bicgstab_iteration(fcp_ctx *ctx, const T *a, const T *d, T *x, T *r, T *reso
   k_bicg_bet<T><<<g, BLK, 0, ctx->stream>>>(n, r, reso, ctx->partials.p, redctr(ctx), s);
   k_bicg_p<T><<<g, BLK, 0, ctx->stream>>>(n, r, u, p, s);
   FCP_TRY(precond_apply<T>(ctx, a, d, p, y, z, s));
-  k_bicg_u<T><<<g, BLK, 0, ctx->stream>>>(n, ctx->ia.p, ctx->ja.p, a, z, reso, u, ctx->partials.p,
-                                           redctr(ctx), s);
+  Sell<T> m;
+  FCP_TRY(sell_view<T>(ctx, &m));
+  k_bicg_u<T><<<g, BLK, 0, ctx->stream>>>(n, m, z, reso, u, ctx->partials.p, redctr(ctx), s);
   k_bicg_xr1<T><<<g, BLK, 0, ctx->stream>>>(n, z, u, x, r, s);
   FCP_TRY(precond_apply<T>(ctx, a, d, r, y, z, s));
-  k_bicg_v<T><<<g, BLK, 0, ctx->stream>>>(n, ctx->ia.p, ctx->ja.p, a, z, r, v, ctx->partials.p,
-                                           redctr(ctx), s);
+  k_bicg_v<T><<<g, BLK, 0, ctx->stream>>>(n, m, z, r, v, ctx->partials.p, redctr(ctx), s);
   // x += alf z ; r -= alf v ; L1 etc. (reuse the PCG update with p=z, q=v)
   k_xr_update<T><<<g, BLK, 0, ctx->stream>>>(n, ctx->diag.p, a, x, r, z, v, ctx->partials.p,
                                               redctr(ctx), s, crit, tol_abs, tol_rel, itr_max);
@@ -754,9 +854,13 @@ int solve_fp64(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max
   FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int), ctx->stream));
   FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
   k_zero_f64<<<g, BLK, 0, ctx->stream>>>(n, ctx->pk.p, ctx->uk.p, ctx->zk.p);
-  k_residual_init<double><<<g, BLK, 0, ctx->stream>>>(n, ctx->ia.p, ctx->ja.p, a, x, b, ctx->res.p,
-                                                       solver == 2 ? ctx->reso.p : nullptr,
-                                                       ctx->partials.p, redctr(ctx), s);
+  {
+    Sell<double> m;
+    FCP_TRY(sell_view<double>(ctx, &m));
+    k_residual_init<double><<<g, BLK, 0, ctx->stream>>>(n, m, x, b, ctx->res.p,
+                                                         solver == 2 ? ctx->reso.p : nullptr,
+                                                         ctx->partials.p, redctr(ctx), s);
+  }
   FCP_HIP(ctx, hipGetLastError());
   FCP_HIP(ctx, hipMemcpyAsync(&ctx->h_scal[0], s, sizeof(SolveScalars), hipMemcpyDeviceToHost,
                               ctx->stream));
@@ -792,9 +896,11 @@ int solve_fp64(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max
 // True FP64 residual r = b - A x with norms into scalars slot.
 int true_residual(fcp_ctx *ctx, const double *x, const double *b, SolveScalars *s) {
   const int n = ctx->n;
+  Sell<double> m;
+  FCP_TRY(sell_view<double>(ctx, &m));
   ProfScope ps(ctx, "residual_f64");
   k_residual_init<double><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(
-      n, ctx->ia.p, ctx->ja.p, ctx->a.p, x, b, ctx->res.p, nullptr, ctx->partials.p, redctr(ctx), s);
+      n, m, x, b, ctx->res.p, nullptr, ctx->partials.p, redctr(ctx), s);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }

@@ -388,6 +388,7 @@ int fcp_build_csr(fcp_ctx *ctx) {
   FCP_HIP(ctx, hipMemsetAsync(ctx->a.p, 0, sizeof(double) * nnz, ctx->stream));
   ctx->a_version++;
   FCP_TRY(fcp_build_schedules(ctx));
+  FCP_TRY(fcp_build_sell(ctx));
   ctx->have_csr = true;
   return 0;
 }

@@ -33,6 +33,8 @@ SIGNATURES = {
     "fcp_laplacian_face": (_i, [_p, _p, _p, _p, _p, _p]),
     "fcp_nonorth_correction": (_i, [_p, _p, _p, _p]),
     "fcp_grad_gauss": (_i, [_p, _p, _p]),
+    "fcp_convdiff": (_i, [_p, _p, _p, _p, _p, _d, _i, _p, _d, _p, _p, _p]),
+    "fcp_ddt": (_i, [_p, _i, _p, _d, _p, _p, _p, _p]),
     "fcp_update_boundary": (_i, [_p, _p]),
     "fcp_matrix_set": (_i, [_p, _p]),
     "fcp_spmv": (_i, [_p, _p, _p]),

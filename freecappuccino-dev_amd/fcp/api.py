@@ -166,6 +166,24 @@ class Context:
                                              _ptr(su)[0])
         self._check(rc, "fcp_nonorth_correction")
 
+    def convdiff(self, flmass, dcoef, phi, grad, gam, scheme, bc_kind, su, sp, urf=1.0,
+                 a_out=None):
+        """facefluxsc assembly (scheme 0 'cds', 1 'linearUpwind'); su/sp
+        accumulate, phi boundary values of zero-gradient faces are updated."""
+        if isinstance(bc_kind, np.ndarray):
+            bc_kind = np.ascontiguousarray(bc_kind, dtype=np.uint8)
+        rc = self.lib.fcp_convdiff(self.h, _ptr(_f64(flmass))[0], _ptr(_f64(dcoef))[0],
+                                   _ptr(phi)[0], _ptr(_f64(grad))[0], float(gam), int(scheme),
+                                   _ptr(bc_kind)[0], float(urf), _ptr(a_out)[0], _ptr(su)[0],
+                                   _ptr(sp)[0])
+        self._check(rc, "fcp_convdiff")
+        return a_out
+
+    def ddt(self, scheme, den, dt, phi_o, su, sp, phi_oo=None):
+        rc = self.lib.fcp_ddt(self.h, int(scheme), _ptr(_f64(den))[0], float(dt),
+                              _ptr(_f64(phi_o))[0], _ptr(_f64(phi_oo))[0], _ptr(su)[0], _ptr(sp)[0])
+        self._check(rc, "fcp_ddt")
+
     def grad_gauss(self, u, out=None):
         if out is None:
             out = np.zeros((self.mesh.n_total, 3))

@@ -1,0 +1,202 @@
+! fcp_capi.f90 -- ISO_C_BINDING interfaces of libfcp (include/fcp.h).
+!
+! Every entry point is bound as declared in the C header: INTEGER is
+! integer(c_int), REAL(8) is real(c_double), arrays are passed by address,
+! scalars by value where the C prototype takes a value.  Index arrays are
+! 1-based, field arrays have length numTotal = numCells + numBoundaryFaces,
+! gradients are dPhidxi(3,numTotal) -- the reference's conventions
+! (geometry.f90:282-289, gradients.f90:1620).
+module fcp_capi
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: FCP_NOMAX = 6
+  integer(c_int), parameter :: FCP_PREC_FP64 = 0, FCP_PREC_MIXED = 1
+  integer(c_int), parameter :: FCP_CRIT_REF_L1 = 0, FCP_CRIT_TRUE_L2 = 1
+
+  type, bind(C) :: fcp_solve_opts
+    integer(c_int) :: precision
+    integer(c_int) :: criterion
+    real(c_double) :: inner_tol
+    integer(c_int) :: max_outer
+    integer(c_int) :: check_every
+    integer(c_int) :: iterations
+    integer(c_int) :: outer_iterations
+    real(c_double) :: res0
+    real(c_double) :: resl
+    real(c_double) :: resor
+    real(c_double) :: rel_res_l2
+    real(c_double) :: time_ms
+  end type
+
+  interface
+    integer(c_int) function fcp_init(device, ctx) bind(C, name='fcp_init')
+      import :: c_int, c_ptr
+      integer(c_int), value :: device
+      type(c_ptr) :: ctx
+    end function
+
+    subroutine fcp_finalize(ctx) bind(C, name='fcp_finalize')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+
+    type(c_ptr) function fcp_last_error(ctx) bind(C, name='fcp_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    subroutine fcp_solve_opts_default(opts) bind(C, name='fcp_solve_opts_default')
+      import :: fcp_solve_opts
+      type(fcp_solve_opts) :: opts
+    end subroutine
+
+    integer(c_int) function fcp_mesh_upload(ctx, nNodes, nCells, nInnerFaces, nBoundaryFaces, &
+        nBoundaries, xyz, face_nnodes, node, owner, neighbour, nfaces, startFace, bctype) &
+        bind(C, name='fcp_mesh_upload')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nNodes, nCells, nInnerFaces, nBoundaryFaces, nBoundaries
+      real(c_double) :: xyz(*)
+      integer(c_int) :: face_nnodes(*), node(*), owner(*), neighbour(*), nfaces(*), startFace(*), bctype(*)
+    end function
+
+    integer(c_int) function fcp_geometry(ctx, xc, yc, zc, vol, arx, ary, arz, xf, yf, zf, &
+        facint, Df, dnb, srdb) bind(C, name='fcp_geometry')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: xc(*), yc(*), zc(*), vol(*), arx(*), ary(*), arz(*), xf(*), yf(*), zf(*)
+      real(c_double) :: facint(*), Df(*), dnb(*), srdb(*)
+    end function
+
+    integer(c_int) function fcp_csr_build(ctx, nnz, ia, ja, diag, slot_pn, slot_np) &
+        bind(C, name='fcp_csr_build')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: nnz
+      integer(c_int) :: ia(*), ja(*), diag(*), slot_pn(*), slot_np(*)
+    end function
+
+    integer(c_int) function fcp_levels(ctx, upper, lev, nlev) bind(C, name='fcp_levels')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: upper
+      integer(c_int) :: lev(*), nlev
+    end function
+
+    integer(c_int) function fcp_laplacian(ctx, mu, phi, a, su) bind(C, name='fcp_laplacian')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: mu(*), phi(*), a(*), su(*)
+    end function
+
+    integer(c_int) function fcp_laplacian_face(ctx, mu_f, bc_kind, phi, a, su) &
+        bind(C, name='fcp_laplacian_face')
+      import :: c_int, c_double, c_ptr, c_signed_char
+      type(c_ptr), value :: ctx
+      real(c_double) :: mu_f(*), phi(*), a(*), su(*)
+      integer(c_signed_char) :: bc_kind(*)
+    end function
+
+    integer(c_int) function fcp_nonorth_correction(ctx, gamma_f, grad, su) &
+        bind(C, name='fcp_nonorth_correction')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: gamma_f(*), grad(*), su(*)
+    end function
+
+    integer(c_int) function fcp_grad_gauss(ctx, u, dudxi) bind(C, name='fcp_grad_gauss')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: u(*), dudxi(*)
+    end function
+
+    integer(c_int) function fcp_update_boundary(ctx, phi) bind(C, name='fcp_update_boundary')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: phi(*)
+    end function
+
+    integer(c_int) function fcp_matrix_set(ctx, a) bind(C, name='fcp_matrix_set')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: a(*)
+    end function
+
+    integer(c_int) function fcp_spmv(ctx, x, y) bind(C, name='fcp_spmv')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: x(*), y(*)
+    end function
+
+    integer(c_int) function fcp_precond_apply(ctx, kind, r, z, d) bind(C, name='fcp_precond_apply')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind
+      real(c_double) :: r(*), z(*), d(*)
+    end function
+
+    integer(c_int) function fcp_csrsolve(ctx, solver, solver_len, fi, rhs, res0, itr_max, &
+        tol_abs, tol_rel, chvar, chvar_len, verbose, opts) bind(C, name='fcp_csrsolve')
+      import :: c_int, c_double, c_ptr, c_char, fcp_solve_opts
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: solver(*), chvar(*)
+      integer(c_int), value :: solver_len, itr_max, chvar_len, verbose
+      real(c_double) :: fi(*), rhs(*), res0
+      real(c_double), value :: tol_abs, tol_rel
+      type(fcp_solve_opts) :: opts
+    end function
+
+    integer(c_int) function fcp_iccg(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, &
+        tol_abs, tol_rel, verbose) bind(C, name='fcp_iccg')
+      import :: c_int, c_double
+      integer(c_int), value :: n, nnz, itr_max, verbose
+      integer(c_int) :: ia(*), ja(*), diag(*)
+      real(c_double) :: a(*), fi(*), rhs(*), resor
+      real(c_double), value :: tol_abs, tol_rel
+    end function
+
+    integer(c_int) function fcp_dpcg(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, &
+        tol_abs, tol_rel, verbose) bind(C, name='fcp_dpcg')
+      import :: c_int, c_double
+      integer(c_int), value :: n, nnz, itr_max, verbose
+      integer(c_int) :: ia(*), ja(*), diag(*)
+      real(c_double) :: a(*), fi(*), rhs(*), resor
+      real(c_double), value :: tol_abs, tol_rel
+    end function
+
+    integer(c_int) function fcp_bicgstab(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, &
+        tol_abs, tol_rel, verbose) bind(C, name='fcp_bicgstab')
+      import :: c_int, c_double
+      integer(c_int), value :: n, nnz, itr_max, verbose
+      integer(c_int) :: ia(*), ja(*), diag(*)
+      real(c_double) :: a(*), fi(*), rhs(*), resor
+      real(c_double), value :: tol_abs, tol_rel
+    end function
+
+    integer(c_int) function fcp_tune_set(ctx, param, value) bind(C, name='fcp_tune_set')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: param, value
+    end function
+
+    integer(c_int) function fcp_profile(ctx, enable) bind(C, name='fcp_profile')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+    end function
+
+    integer(c_int) function fcp_profile_report(ctx, buf, len) bind(C, name='fcp_profile_report')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: buf(*)
+      integer(c_int), value :: len
+    end function
+
+    type(c_ptr) function fcp_stream(ctx) bind(C, name='fcp_stream')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+  end interface
+
+end module fcp_capi

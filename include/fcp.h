@@ -143,6 +143,29 @@ int fcp_laplacian_face(fcp_ctx *ctx, const double *mu_f,
 int fcp_nonorth_correction(fcp_ctx *ctx, const double *gamma_f,
                            const double *grad, double *su);
 
+/* Transport-equation assembly as calculate_temperature composes it
+ * (temperature.f90:118-366): per inner face facefluxsc (scalar_fluxes.f90:
+ * 32-141) -- implicit upwind convection + diffusion gamma*D_f, explicit
+ * non-orthogonal diffusion correction and deferred correction
+ * gam*(F*phi_f^HO - F*phi^UD) with face value scheme 0 = 'cds',
+ * 1 = 'linearUpwind' (interpolation.f90:116-157, 318-356); per boundary face
+ * bc_kind 0 = Dirichlet wall (:206-210), 1 = zero-gradient wall (phi_b =
+ * phi_P, :224), 2 = inlet/outlet/pressure flux face (facefluxsc_boundary
+ * :256-337, :182-190); then a_ii = sp - (row sum - a_ii) and under-relaxation
+ * a_ii /= urf, su += (1-urf) a_ii phi (:350-366).  flmass/dcoef [nFaces],
+ * phi [numTotal] (boundary values of zero-gradient faces updated), grad
+ * [3*numTotal]; su, sp [nCells] accumulate (pass them cleared, or holding
+ * the time terms of fcp_ddt).  The matrix stays resident; a optional copy. */
+int fcp_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
+                 const double *grad, double gam, int scheme,
+                 const unsigned char *bc_kind, double urf, double *a, double *su,
+                 double *sp);
+
+/* Time terms (temperature.f90:82-102): scheme 1 BDF1 (and the volume part of
+ * Crank-Nicolson), 2 BDF2; apotime = den*vol/dt; su/sp updated in place. */
+int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt,
+            const double *phi_o, const double *phi_oo, double *su, double *sp);
+
 /* grad_gauss (gradients.f90:1607-1715). dudxi[3*numTotal], boundary rows 0. */
 int fcp_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi);
 
