@@ -247,6 +247,18 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
       if (value < 0 || value > 127) return fcp_fail(ctx, FCP_ERR_ARG, "sleep out of range");
       ctx->tune_sweep_sleep = value;
       return 0;
+    case FCP_TUNE_SWEEP_KIND:
+      if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "sweep kind is 0 or 1");
+      ctx->tune_sweep_kind = value;
+      return 0;
+    case FCP_TUNE_SWEEP_ROWS:
+      if (value != 0 && (value < 64 || value > 16384 || (value & 63)))
+        return fcp_fail(ctx, FCP_ERR_ARG, "sweep rows must be 0 or a multiple of 64 in [64, 16384]");
+      if (value != ctx->tune_sweep_rows) {
+        ctx->tune_sweep_rows = value;
+        if (ctx->have_csr) return fcp_build_block_schedules(ctx, fcp_block_rows(ctx));
+      }
+      return 0;
   }
   return fcp_fail(ctx, FCP_ERR_ARG, "unknown tuning parameter %d", param);
 }
@@ -266,7 +278,10 @@ int fcp_init(int device, fcp_ctx **out) {
   fcp_ctx *ctx = new fcp_ctx();
   ctx->device = device;
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->ncu = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ctx->ncu = prop.multiProcessorCount;
+    if (prop.sharedMemPerBlock > 0) ctx->lds_cap = prop.sharedMemPerBlock;
+  }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return FCP_ERR_HIP;

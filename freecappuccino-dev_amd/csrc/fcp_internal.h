@@ -55,6 +55,16 @@ struct Schedule {
   DBuf<int> lev;         // [n] level of each row
 };
 
+// Row-block schedule of one triangular sweep: contiguous blocks of R rows,
+// each block's rows ordered by level ("steps"); one workgroup per block keeps
+// the block's results in LDS and waits only for rows of earlier blocks.
+struct BlockSchedule {
+  int R = 0, nblocks = 0, nsteps = 0;
+  DBuf<int> rows;      // [n] block-major, level-minor, ascending row inside a level
+  DBuf<int> step_ptr;  // [nsteps+1] offsets into rows
+  DBuf<int> blk_step;  // [nblocks+1] first step of every block
+};
+
 // Device-resident scalars of one Krylov solve (single RHS).
 struct SolveScalars {
   double sk, s0, pkapk, alf, bet, res0, resl, factor, resor, rsm;
@@ -99,6 +109,7 @@ struct fcp_ctx {
   DBuf<int> ia, ja, diag, slot_pn, slot_np;
   DBuf<int> row_of;     // [nnz] row of every entry
   Schedule lower, upper;
+  BlockSchedule blower, bupper;
   DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)
   // SELL-64 copy of the operator for SpMV: rows in slices of 64, each slice
   // stored column-major (entry k of the slice's rows contiguous) and padded
@@ -131,6 +142,9 @@ struct fcp_ctx {
   // -------- launch shapes --------
   int tune_sweep_waves = 0;
   int tune_sweep_sleep = 1;
+  int tune_sweep_kind = 0;     // 0 row-block (LDS) sweeps, 1 level-task sweeps
+  int tune_sweep_rows = 0;     // rows per block of the row-block sweeps (0 = auto)
+  size_t lds_cap = 65536;      // LDS bytes available to one workgroup
 
   // -------- profiling --------
   bool prof = false;
@@ -183,6 +197,17 @@ int fcp_build_cell_faces(fcp_ctx *ctx);
 int fcp_compute_geometry(fcp_ctx *ctx);
 int fcp_build_csr(fcp_ctx *ctx);
 int fcp_build_schedules(fcp_ctx *ctx);
+int fcp_build_block_schedules(fcp_ctx *ctx, int R);
+// Rows per block: the tuned value, or the largest multiple of 64 (<= 16384)
+// whose FP64 block fits the workgroup LDS limit with 1 KB to spare.
+inline int fcp_block_rows(const fcp_ctx *ctx) {
+  if (ctx->tune_sweep_rows > 0) return ctx->tune_sweep_rows;
+  long long r = ((long long)ctx->lds_cap - 1024) / 8;
+  r = (r / 64) * 64;
+  if (r > 16384) r = 16384;
+  if (r < 64) r = 64;
+  return (int)r;
+}
 // fcp_assemble.hip
 int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
                          const unsigned char *bc_kind, const double *phi, double *su);

@@ -224,6 +224,81 @@ __global__ __launch_bounds__(BLK) void k_sweep(
   }
 }
 
+// Row-block sweep: one workgroup per block of R contiguous rows (blocks taken
+// in ticket order, reversed for the backward sweep).  The block's rows are
+// processed step by step (one step = one level present in the block); a
+// dependency inside the block is read from LDS, written at an earlier step
+// (barrier in between); a dependency in an earlier block is waited for in
+// global memory like the level-task sweep.  Same per-row arithmetic as
+// k_sweep, so results are bit-equal.
+template <typename T, int MODE>
+__global__ __launch_bounds__(BLK) void k_bsweep(
+    const int *__restrict__ rows, const int *__restrict__ step_ptr,
+    const int *__restrict__ blk_step, int nblocks, int R, int n, const int *__restrict__ ia,
+    const int *__restrict__ diag, const int *__restrict__ ja, const T *__restrict__ a,
+    const int *__restrict__ tslot, const T *__restrict__ d, const T *__restrict__ in, T *out,
+    int *__restrict__ ticket, int *__restrict__ err, const SolveScalars *__restrict__ s, int slp) {
+  extern __shared__ unsigned char smem_raw[];
+  T *zl = reinterpret_cast<T *>(smem_raw);
+  __shared__ int sb;
+  if (s && solve_done(s)) return;
+  if (threadIdx.x == 0) sb = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int t = sb;
+  if (t >= nblocks) return;
+  const int b = (MODE == 1) ? nblocks - 1 - t : t;
+  const int lo = b * R;
+  const int hi = (lo + R < n) ? lo + R : n;
+  for (int st = blk_step[b]; st < blk_step[b + 1]; ++st) {
+    const int e = step_ptr[st + 1];
+    for (int idx = step_ptr[st] + threadIdx.x; idx < e; idx += BLK) {
+      const int row = rows[idx];
+      const int dg = diag[row];
+      T val;
+      if (MODE == 0) {
+        T acc = in[row];
+        for (int k = ia[row]; k < dg; ++k) {
+          const int j = ja[k];
+          const T v = (j >= lo) ? zl[j - lo] : wait_value<T>(out + j, err, slp);
+          acc = acc - a[k] * v;
+        }
+        val = acc * d[row];
+      } else if (MODE == 1) {
+        const T dr = d[row];
+        T acc = in[row] / (dr + (T)FCP_SMALL);
+        const int ee = ia[row + 1];
+        for (int k = dg + 1; k < ee; ++k) {
+          const int j = ja[k];
+          const T v = (j < hi) ? zl[j - lo] : wait_value<T>(out + j, err, slp);
+          acc = acc - a[k] * v;
+        }
+        val = acc * dr;
+      } else if (MODE == 2) {
+        T acc = a[dg];
+        for (int k = ia[row]; k < dg; ++k) {
+          const int j = ja[k];
+          const T ak = a[k];
+          const T v = (j >= lo) ? zl[j - lo] : wait_value<T>(out + j, err, slp);
+          acc = acc - ak * ak * v;
+        }
+        val = (T)1.0 / acc;
+      } else {
+        T acc = a[dg];
+        for (int k = ia[row]; k < dg; ++k) {
+          const int j = ja[k];
+          const int l = tslot[k];
+          const T v = (j >= lo) ? zl[j - lo] : wait_value<T>(out + j, err, slp);
+          acc = acc - a[k] * v * (l >= 0 ? a[l] : (T)0);
+        }
+        val = (T)1.0 / acc;
+      }
+      zl[row - lo] = val;
+      publish<T>(out + row, val);
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // SpMV on the SELL-64 copy
 // ---------------------------------------------------------------------------
@@ -721,6 +796,22 @@ int check_err(fcp_ctx *ctx) {
   return 0;
 }
 
+template <typename T, int MODE>
+int launch_bsweep(fcp_ctx *ctx, const BlockSchedule &b, const T *a, const T *d, const T *in, T *out,
+                  int *ticket, const SolveScalars *s) {
+  const size_t lds = (size_t)b.R * sizeof(T);
+  k_bsweep<T, MODE><<<b.nblocks, BLK, lds, ctx->stream>>>(
+      b.rows.p, b.step_ptr.p, b.blk_step.p, b.nblocks, b.R, ctx->n, ctx->ia.p, ctx->diag.p,
+      ctx->ja.p, a, ctx->tslot.p, d, in, out, ticket, ctx->errflag.p, s, ctx->tune_sweep_sleep);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+bool use_block_sweep(fcp_ctx *ctx, size_t elem) {
+  return ctx->tune_sweep_kind == 0 && ctx->blower.nblocks > 0 &&
+         (size_t)ctx->blower.R * elem + 256 <= ctx->lds_cap;
+}
+
 // Diagonal of the DIC (kind 0) or DILU (kind 1) preconditioner in FP64,
 // recomputed when the matrix changed.
 template <typename T>
@@ -728,6 +819,11 @@ int precond_setup(fcp_ctx *ctx, int kind, const T *a, T *d) {
   const int n = ctx->n;
   k_fill_sent<T><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(d, (T *)nullptr, n, tickets(ctx), nullptr);
   ProfScope ps(ctx, kind ? "dilu_setup" : "dic_setup");
+  if (use_block_sweep(ctx, sizeof(T))) {
+    if (kind == 0)
+      return launch_bsweep<T, 2>(ctx, ctx->blower, a, nullptr, nullptr, d, tickets(ctx), nullptr);
+    return launch_bsweep<T, 3>(ctx, ctx->blower, a, nullptr, nullptr, d, tickets(ctx), nullptr);
+  }
   if (kind == 0)
     k_sweep<T, 2><<<sweep_grid(ctx, ctx->lower), BLK, 0, ctx->stream>>>(
         ctx->lower.task_ptr.p, ctx->lower.order.p, ctx->lower.ntasks, ctx->ia.p, ctx->diag.p,
@@ -746,6 +842,15 @@ int precond_apply(fcp_ctx *ctx, const T *a, const T *d, const T *r, T *y, T *z,
                   const SolveScalars *s) {
   const int n = ctx->n;
   k_fill_sent<T><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(y, z, n, tickets(ctx), s);
+  if (use_block_sweep(ctx, sizeof(T))) {
+    {
+      ProfScope ps(ctx, sizeof(T) == 8 ? "sweep_fwd_f64" : "sweep_fwd_f32");
+      const int rc = launch_bsweep<T, 0>(ctx, ctx->blower, a, d, r, y, tickets(ctx), s);
+      if (rc < 0) return rc;
+    }
+    ProfScope ps(ctx, sizeof(T) == 8 ? "sweep_bwd_f64" : "sweep_bwd_f32");
+    return launch_bsweep<T, 1>(ctx, ctx->bupper, a, d, y, z, tickets(ctx) + 1, s);
+  }
   {
     ProfScope ps(ctx, sizeof(T) == 8 ? "sweep_fwd_f64" : "sweep_fwd_f32");
     k_sweep<T, 0><<<sweep_grid(ctx, ctx->lower), BLK, 0, ctx->stream>>>(

@@ -450,8 +450,69 @@ static int build_one_schedule(fcp_ctx *ctx, int upper, Schedule &s) {
   return 0;
 }
 
+namespace {
+__global__ void k_block_of(const int *__restrict__ order, int n, int R, int *__restrict__ key) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) key[i] = order[i] / R;
+}
+}  // namespace
+
+// Row-block schedule from a level schedule: a stable sort of the level-major
+// row order by block id gives every block's rows in level order; steps are
+// the runs of equal level inside a block.
+static int build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, BlockSchedule &b) {
+  const int n = ctx->n;
+  b.R = R;
+  b.nblocks = (n + R - 1) / R;
+  DBuf<int> key, key_out;
+  FCP_HIP(ctx, key.alloc(n));
+  FCP_HIP(ctx, key_out.alloc(n));
+  FCP_HIP(ctx, b.rows.alloc(n));
+  k_block_of<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(s.order.p, n, R, key.p);
+  FCP_TRY(fcp_sort_pairs_int(ctx, key.p, key_out.p, s.order.p, b.rows.p, n, b.nblocks));
+  std::vector<int> rows(n), lev(n);
+  FCP_HIP(ctx, hipMemcpyAsync(rows.data(), b.rows.p, sizeof(int) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(lev.data(), s.lev.p, sizeof(int) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int> sp, bs(b.nblocks + 1, 0);
+  sp.reserve(b.nblocks * 64 + 1);
+  int prev_blk = -1, prev_lev = -1;
+  for (int i = 0; i < n; ++i) {
+    const int r = rows[i], blk = r / R;
+    if (blk != prev_blk) {
+      for (int q = prev_blk + 1; q <= blk; ++q) bs[q] = (int)sp.size();
+      prev_blk = blk;
+      prev_lev = -1;
+    }
+    if (lev[r] != prev_lev) {
+      sp.push_back(i);
+      prev_lev = lev[r];
+    }
+  }
+  for (int q = prev_blk + 1; q <= b.nblocks; ++q) bs[q] = (int)sp.size();
+  b.nsteps = (int)sp.size();
+  sp.push_back(n);
+  FCP_HIP(ctx, b.step_ptr.alloc(sp.size()));
+  FCP_HIP(ctx, b.blk_step.alloc(bs.size()));
+  FCP_HIP(ctx, hipMemcpyAsync(b.step_ptr.p, sp.data(), sizeof(int) * sp.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(b.blk_step.p, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_build_block_schedules(fcp_ctx *ctx, int R) {
+  FCP_TRY(build_block_schedule(ctx, ctx->lower, R, ctx->blower));
+  FCP_TRY(build_block_schedule(ctx, ctx->upper, R, ctx->bupper));
+  return 0;
+}
+
 int fcp_build_schedules(fcp_ctx *ctx) {
   FCP_TRY(build_one_schedule(ctx, 0, ctx->lower));
   FCP_TRY(build_one_schedule(ctx, 1, ctx->upper));
+  FCP_TRY(fcp_build_block_schedules(ctx, fcp_block_rows(ctx)));
   return 0;
 }

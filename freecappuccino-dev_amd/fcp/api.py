@@ -24,6 +24,8 @@ FCP_CRIT_REF_L1 = 0
 FCP_CRIT_TRUE_L2 = 1
 FCP_TUNE_SWEEP_WAVES = 1
 FCP_TUNE_SWEEP_SLEEP = 2
+FCP_TUNE_SWEEP_KIND = 3
+FCP_TUNE_SWEEP_ROWS = 4
 
 
 class FcpError(RuntimeError):

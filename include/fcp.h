@@ -214,6 +214,8 @@ int fcp_bicgstab(int n, int nnz, const int *ia, const int *ja, const double *a,
 /* --- launch-shape tuning (autotuner hook) --------------------------------- */
 #define FCP_TUNE_SWEEP_WAVES 1   /* resident waves of a sweep kernel (0 = auto) */
 #define FCP_TUNE_SWEEP_SLEEP 2   /* s_sleep argument between dependency polls   */
+#define FCP_TUNE_SWEEP_KIND 3    /* 0 row-block LDS sweeps, 1 level-task sweeps */
+#define FCP_TUNE_SWEEP_ROWS 4    /* rows per block of the row-block sweeps      */
 /* Sets a launch-shape parameter of the context; results do not depend on it. */
 int fcp_tune_set(fcp_ctx *ctx, int param, int value);
 
