@@ -39,9 +39,9 @@ def alg_bytes(name, n, nnz, Fi, Fb):
     """Algorithmic (compulsory) HBM bytes of one launch of a named kernel."""
     lower = Fi  # strict-lower entries = inner faces
     tab = {
-        "spmv_pq_f32": nnz * 8 + (n + 1) * 4 + 2 * n * 4,
-        "spmv_pq_f64": nnz * 12 + (n + 1) * 4 + 2 * n * 8,
-        "spmv_row_f64": nnz * 12 + (n + 1) * 4 + 2 * n * 8,
+        "spmv_pq_f32": nnz * 8 + 2 * n * 4,
+        "spmv_pq_f64": nnz * 12 + 2 * n * 8,
+        "spmv_sell_f64": nnz * 12 + 2 * n * 8,
         "sweep_fwd_f32": lower * 8 + 4 * n + 3 * n * 4,
         "sweep_bwd_f32": lower * 8 + 4 * n + 3 * n * 4,
         "sweep_fwd_f64": lower * 12 + 4 * n + 3 * n * 8,
@@ -169,13 +169,13 @@ def run_gpu(args):
     ctx.profile(True)
     for _ in range(20):
         ctx.spmv(xs, ys)
-    sp = ctx.profile_report().get("spmv_row_f64")
+    sp = ctx.profile_report().get("spmv_sell_f64")
     ctx.profile(False)
     spmv = None
     if sp:
         avg = sp[1] / sp[0] / 1e3
-        bb = alg_bytes("spmv_row_f64", n, nnz, Fi, Fb)
-        spmv = dict(kernel="spmv_row_f64", avg_us=avg * 1e6, GBs=bb / avg / 1e9,
+        bb = alg_bytes("spmv_sell_f64", n, nnz, Fi, Fb)
+        spmv = dict(kernel="spmv_sell_f64", avg_us=avg * 1e6, GBs=bb / avg / 1e9,
                     frac=bb / avg / 1e9 / HBM_PEAK_GBS, alg_bytes=bb)
     ctx.close()
     return dict(n=n, nnz=nnz, Fi=Fi, results=results, err_mixed_vs_fp64_tight=err_mixed,

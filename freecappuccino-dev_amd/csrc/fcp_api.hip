@@ -252,11 +252,11 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
       ctx->tune_sweep_kind = value;
       return 0;
     case FCP_TUNE_SWEEP_ROWS:
-      if (value != 0 && (value < 64 || value > 16384 || (value & 63)))
-        return fcp_fail(ctx, FCP_ERR_ARG, "sweep rows must be 0 or a multiple of 64 in [64, 16384]");
+      if (value != 0 && (value < 64 || value > 32768 || (value & 63)))
+        return fcp_fail(ctx, FCP_ERR_ARG, "sweep rows must be 0 or a multiple of 64 in [64, 32768]");
       if (value != ctx->tune_sweep_rows) {
         ctx->tune_sweep_rows = value;
-        if (ctx->have_csr) return fcp_build_block_schedules(ctx, fcp_block_rows(ctx));
+        if (ctx->have_csr) return fcp_build_block_schedules(ctx, fcp_block_rows(ctx, 8));
       }
       return 0;
   }
@@ -627,6 +627,7 @@ static int solver_code(const char *s, int len) {
   if (name == "iccg") return 0;
   if (name == "dpcg") return 1;
   if (name == "bicgstab") return 2;
+  if (name == "gauss-seidel") return 3;
   return -1;
 }
 
@@ -652,7 +653,7 @@ int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi, c
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (res0) *res0 = o->resor;
   if (verbose >= 1) {
-    static const char *names[] = {"PCG(IC0)", "PCG(Jacobi)", "BiCGStab(ILU(0))"};
+    static const char *names[] = {"PCG(IC0)", "PCG(Jacobi)", "BiCGStab(ILU(0))", "Gauss-Seidel"};
     std::string var = chvar ? std::string(chvar, chvar + (chvar_len > 0 ? chvar_len : strlen(chvar)))
                             : std::string("x");
     while (!var.empty() && var.back() == ' ') var.pop_back();
@@ -660,6 +661,34 @@ int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi, c
     printf("  %s:  Solving for %s, Initial residual = %10.3E, Final residual = %10.3E, No Iterations %d\n",
            names[code], var.c_str(), o->resor, o->resl / factor, o->iterations);
   }
+  return rc;
+}
+
+int fcp_csrsolve_multi(fcp_ctx *ctx, const char *solver, int solver_len, int nrhs, double *fi,
+                       int ldfi, const double *rhs, double *res0, int itr_max, double tol_abs,
+                       double tol_rel, int verbose, fcp_solve_opts *opts) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!solver || !fi || !rhs) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_csrsolve_multi: null argument");
+  if (nrhs < 1 || nrhs > 4) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_csrsolve_multi: nrhs %d", nrhs);
+  if (ldfi < ctx->n) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_csrsolve_multi: ldfi < numCells");
+  const int code = solver_code(solver, solver_len);
+  if (code != 0 && code != 2)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_csrsolve_multi: solver must be iccg or bicgstab");
+  (void)hipSetDevice(ctx->device);
+  Arg<double> x, b;
+  FCP_TRY(arg_in(ctx, (const double *)fi, (size_t)ldfi * nrhs, x));
+  FCP_TRY(arg_in(ctx, rhs, (size_t)ctx->n * nrhs, b));
+  fcp_solve_opts local;
+  fcp_solve_opts_default(&local);
+  fcp_solve_opts *o = opts ? opts : &local;
+  const int rc = fcp_solve_multi_device(ctx, code, nrhs, x.dev, ldfi, b.dev, res0, itr_max,
+                                        tol_abs, tol_rel, verbose, o);
+  if (rc < 0) return rc;
+  FCP_TRY(arg_back(ctx, x));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (verbose >= 1)
+    printf("  %s x%d:  Initial residual = %10.3E, worst |b-Ax|/|b| = %10.3E, No Iterations %d\n",
+           code == 0 ? "PCG(IC0)" : "BiCGStab(ILU(0))", nrhs, o->res0, o->rel_res_l2, o->iterations);
   return rc;
 }
 
@@ -753,6 +782,11 @@ int fcp_bicgstab(int n, int nnz, const int *ia, const int *ja, const double *a, 
                  double *fi, const double *rhs, double *resor, int itr_max, double tol_abs,
                  double tol_rel, int verbose) {
   return explicit_solve(2, n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, tol_abs, tol_rel, verbose);
+}
+int fcp_gauss_seidel(int n, int nnz, const int *ia, const int *ja, const double *a,
+                     const int *diag, double *fi, const double *rhs, double *resor, int itr_max,
+                     double tol_abs, double tol_rel, int verbose) {
+  return explicit_solve(3, n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, tol_abs, tol_rel, verbose);
 }
 
 }  // extern "C"

@@ -1,0 +1,176 @@
+// fcp_device.h -- device helpers shared by the sparse/Krylov translation
+// units of libfcp: deterministic block reductions, the NaN-sentinel
+// publish/wait protocol of the sync-free sweeps and the SELL-64 row product.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fcp_internal.h"
+
+namespace fcpk {
+
+constexpr int BLK = 256;
+constexpr int MAXBLK = 1024;  // blocks of a reduction kernel (partials per slot)
+
+enum Slot { SLOT_A = 0, SLOT_B = 1, SLOT_C = 2, NSLOT = 4 };
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block-wide sum of up to 3 values into lane 0 of warp 0 (deterministic order).
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double (&out)[NV]) {
+  __shared__ double sh[NV][BLK / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = wave_sum(v[i]);
+    if (lane == 0) sh[i][w] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    if (threadIdx.x == 0)
+      for (int k = 0; k < BLK / 64; ++k) s += sh[i][k];
+    out[i] = s;
+  }
+}
+
+// Publish NV block partials; returns true in the single last-arriving block,
+// whose thread 0 then holds the totals (summed in block order) in tot[].
+template <int NV>
+__device__ bool last_block_reduce(double (&part)[NV], double *__restrict__ partials, int slot,
+                                  int *__restrict__ counter, double (&tot)[NV]) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) partials[(size_t)(slot + i) * MAXBLK + blockIdx.x] = part[i];
+    __threadfence();
+    const int prev = atomicAdd(counter, 1);
+    is_last = (prev == (int)gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!is_last) return false;
+  __threadfence();
+  // all threads of the last block reduce the partials in fixed order
+  double loc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLK)
+      s += __hip_atomic_load(&partials[(size_t)(slot + i) * MAXBLK + b], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    loc[i] = s;
+  }
+  block_sum<NV>(loc, tot);
+  if (threadIdx.x == 0) *counter = 0;
+  return true;
+}
+
+__device__ __forceinline__ bool solve_done(const SolveScalars *s) {
+  return __hip_atomic_load(&s->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+// ---------------------------------------------------------------------------
+// sentinel wait/publish
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Bits;
+template <>
+struct Bits<double> {
+  using U = unsigned long long;
+  static constexpr U sent = FCP_SENT64;
+  static __device__ __forceinline__ U to(double v) { return __double_as_longlong(v); }
+  static __device__ __forceinline__ double from(U u) { return __longlong_as_double(u); }
+  static constexpr U qnan = 0x7FF8000000000000ull;
+};
+template <>
+struct Bits<float> {
+  using U = unsigned int;
+  static constexpr U sent = FCP_SENT32;
+  static __device__ __forceinline__ U to(float v) { return __float_as_uint(v); }
+  static __device__ __forceinline__ float from(U u) { return __uint_as_float(u); }
+  static constexpr U qnan = 0x7FC00000u;
+};
+
+template <typename T>
+__device__ __forceinline__ T wait_value(T *p, int *err, int slp) {
+  using B = Bits<T>;
+  typename B::U *q = reinterpret_cast<typename B::U *>(p);
+  typename B::U v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (v != B::sent) return B::from(v);
+  // Bounded by wall time (100 MHz constant clock): a dependency that is not
+  // published within FCP_WAIT_TICKS records FCP_ERR_SWEEP and the wave
+  // continues with NaN, so the kernel always drains.
+  const unsigned long long t0 = wall_clock64();
+  unsigned spins = 0;
+  while (v == B::sent) {
+    if (slp >= 8)
+      __builtin_amdgcn_s_sleep(8);
+    else if (slp >= 2)
+      __builtin_amdgcn_s_sleep(2);
+    else
+      __builtin_amdgcn_s_sleep(1);
+    v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((++spins & 63u) == 0 && wall_clock64() - t0 > FCP_WAIT_TICKS) {
+      __hip_atomic_store(err, FCP_ERR_SWEEP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      v = B::qnan;
+    }
+  }
+  return B::from(v);
+}
+
+template <typename T>
+__device__ __forceinline__ void publish(T *p, T v) {
+  using B = Bits<T>;
+  typename B::U u = B::to(v);
+  if (u == B::sent) u = B::qnan;
+  __hip_atomic_store(reinterpret_cast<typename B::U *>(p), u, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// SELL-64 copy of the operator
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Sell {
+  const int *base, *col, *ia;
+  const T *a;
+};
+
+// sum_k a_ik x_k of row i in CSR entry order (the reference's row loop,
+// linear_solvers.f90:493-498); loads of a wave's 64 rows are contiguous.
+template <typename T>
+__device__ __forceinline__ T sell_row(const Sell<T> &m, int i, const T *__restrict__ x, T init,
+                                      bool subtract) {
+  const int p = m.base[i >> 6] + (i & 63);
+  const int len = m.ia[i + 1] - m.ia[i];
+  T acc = init;
+  if (subtract)
+    for (int k = 0; k < len; ++k) acc = acc - m.a[p + 64 * k] * x[m.col[p + 64 * k]];
+  else
+    for (int k = 0; k < len; ++k) acc = acc + m.a[p + 64 * k] * x[m.col[p + 64 * k]];
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers (fcp_krylov.hip)
+// ---------------------------------------------------------------------------
+int red_grid(fcp_ctx *ctx, int n);
+int sell_view64(fcp_ctx *ctx, Sell<double> *out);
+int sell_view32(fcp_ctx *ctx, Sell<float> *out);
+inline int *tickets(fcp_ctx *ctx) { return ctx->counters.p; }
+inline int *redctr(fcp_ctx *ctx) { return ctx->counters.p + 2; }
+int check_err(fcp_ctx *ctx);
+int prepare_precond(fcp_ctx *ctx, int kind, bool fp32);
+int pack_block64(fcp_ctx *ctx, BlockSchedule &b);
+int pack_block32(fcp_ctx *ctx, BlockSchedule &b);
+int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z);
+int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z);
+
+}  // namespace fcpk

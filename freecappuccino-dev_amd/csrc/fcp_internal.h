@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -59,13 +60,31 @@ struct Schedule {
 // each block's rows ordered by level ("steps"); one workgroup per block keeps
 // the block's results in LDS and waits only for rows of earlier blocks.
 struct BlockSchedule {
-  int R = 0, nblocks = 0, nsteps = 0;
+  int R = 0, nblocks = 0, nsteps = 0, max_steps = 0;
   DBuf<int> rows;      // [n] block-major, level-minor, ascending row inside a level
   DBuf<int> step_ptr;  // [nsteps+1] offsets into rows
   DBuf<int> blk_step;  // [nblocks+1] first step of every block
+  // Packed step-major operands for the pipelined sweep, position = index in
+  // `rows`: dependency count, dependency (>= 0: LDS slot inside the block,
+  // < 0: -(row+1) of an earlier block), CSR position of its coefficient.
+  int maxd = 0;
+  DBuf<int> prc;                // [2*n]  (row, count) pairs
+  DBuf<int> pdep;               // [PMAXD*n] dependencies, PMAXD per position
+  DBuf<int> psrc;               // [PMAXD*n] CSR positions of the coefficients
+  DBuf<double> pa64, pdx64;     // [PMAXD*n] coefficients; [2*n] (pivot, input)
+  DBuf<float> pa32, pdx32;
+  uint64_t pver64 = ~0ull, pver32 = ~0ull;
 };
+#define FCP_PMAXD 4  // dependencies per row handled by the pipelined sweep
 
 // Device-resident scalars of one Krylov solve (single RHS).
+// Row-block schedules of a batched (several right-hand sides) sweep, whose
+// block size depends on the bytes per row kept in LDS.
+struct MultiSched {
+  BlockSchedule lo, up;
+  DBuf<char> pin;  // packed sweep inputs, one row of values per position
+};
+
 struct SolveScalars {
   double sk, s0, pkapk, alf, bet, res0, resl, factor, resor, rsm;
   double l2, l2b, om, gam, beto, ukreso, vv, vr;
@@ -109,7 +128,8 @@ struct fcp_ctx {
   DBuf<int> ia, ja, diag, slot_pn, slot_np;
   DBuf<int> row_of;     // [nnz] row of every entry
   Schedule lower, upper;
-  BlockSchedule blower, bupper;
+  BlockSchedule blower, bupper;      // FP64-sized blocks
+  BlockSchedule blower4, bupper4;    // FP32-sized blocks (twice the rows)
   DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)
   // SELL-64 copy of the operator for SpMV: rows in slices of 64, each slice
   // stored column-major (entry k of the slice's rows contiguous) and padded
@@ -126,15 +146,21 @@ struct fcp_ctx {
   DBuf<float> a32;      // [nnz] fp32 shadow
   uint64_t a_version = 0, a32_version = ~0ull, dic_version = ~0ull, d32_version = ~0ull;
   int dic_kind = -1;    // 0 DIC, 1 DILU
+  uint64_t d_epoch = 0; // bumped at every preconditioner setup
 
   // -------- solver workspace --------
   DBuf<double> d, res, pk, zk, yk, uk, vk, reso, x, rhs, qk;
-  DBuf<float> d32, r32, p32, z32, y32, q32, e32;
+  DBuf<float> d32, r32, p32, z32, y32, q32, e32, u32, v32, reso32;
   DBuf<double> partials;     // per-block partial sums
   DBuf<int> counters;        // [16] block-completion counters / tickets
   DBuf<SolveScalars> scal;   // device scalars
   SolveScalars *h_scal = nullptr;  // pinned mirror
   DBuf<int> errflag;         // [1]
+
+  // -------- batched solves (interleaved vectors, NRP values per row) --------
+  std::map<int, std::unique_ptr<MultiSched>> msched;  // keyed by rows per block
+  DBuf<double> mv64[9];
+  DBuf<float> mv32[8];
 
   // -------- staging --------
   DBuf<char> stage[4];
@@ -198,16 +224,24 @@ int fcp_compute_geometry(fcp_ctx *ctx);
 int fcp_build_csr(fcp_ctx *ctx);
 int fcp_build_schedules(fcp_ctx *ctx);
 int fcp_build_block_schedules(fcp_ctx *ctx, int R);
-// Rows per block: the tuned value, or the largest multiple of 64 (<= 16384)
-// whose FP64 block fits the workgroup LDS limit with 1 KB to spare.
-inline int fcp_block_rows(const fcp_ctx *ctx) {
-  if (ctx->tune_sweep_rows > 0) return ctx->tune_sweep_rows;
-  long long r = ((long long)ctx->lds_cap - 1024) / 8;
+// Rows per block for elements of `elem` bytes: the tuned value (FP64 rows;
+// doubled for FP32), or the largest multiple of 64 (<= 65536) whose block
+// fits the workgroup LDS limit with 4 KB left for the step table.
+inline int fcp_block_rows(const fcp_ctx *ctx, int elem = 8) {
+  if (ctx->tune_sweep_rows > 0) {
+    long long t = (long long)ctx->tune_sweep_rows * 8 / elem;
+    t = (t / 64) * 64;
+    return (int)(t < 64 ? 64 : t);
+  }
+  long long r = ((long long)ctx->lds_cap - 4096) / elem;
   r = (r / 64) * 64;
-  if (r > 16384) r = 16384;
+  if (r > 65536) r = 65536;
   if (r < 64) r = 64;
   return (int)r;
 }
+// fcp_mesh.hip: row-block schedule of one triangle (upper = 1) with R rows
+// per block, packed static operands included.
+int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, BlockSchedule &b);
 // fcp_assemble.hip
 int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
                          const unsigned char *bc_kind, const double *phi, double *su);
@@ -226,6 +260,10 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
                      double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts);
 int fcp_ensure_workspace(fcp_ctx *ctx);
 int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
+// fcp_multi.hip: batched solve of nrhs columns (column-major, device memory)
+int fcp_solve_multi_device(fcp_ctx *ctx, int solver, int nrhs, double *fi, int ldfi,
+                           const double *rhs, double *res0_out, int itr_max, double tol_abs,
+                           double tol_rel, int verbose, fcp_solve_opts *opts);
 
 // rocPRIM-based primitives (fcp_prim.hip)
 int fcp_exclusive_scan_int(fcp_ctx *ctx, const int *in, int *out, int n);

@@ -26,135 +26,11 @@
 #include <cmath>
 #include <cstring>
 
-#include "fcp_internal.h"
+#include "fcp_device.h"
+
+using namespace fcpk;
 
 namespace {
-
-constexpr int BLK = 256;
-constexpr int MAXBLK = 1024;  // blocks of a reduction kernel (partials per slot)
-
-enum Slot { SLOT_A = 0, SLOT_B = 1, SLOT_C = 2, NSLOT = 4 };
-
-// ---------------------------------------------------------------------------
-// small device helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-
-// Block-wide sum of up to 3 values into lane 0 of warp 0 (deterministic order).
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double (&out)[NV]) {
-  __shared__ double sh[NV][BLK / 64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double s = wave_sum(v[i]);
-    if (lane == 0) sh[i][w] = s;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double s = 0.0;
-    if (threadIdx.x == 0)
-      for (int k = 0; k < BLK / 64; ++k) s += sh[i][k];
-    out[i] = s;
-  }
-}
-
-// Publish NV block partials; returns true in the single last-arriving block,
-// whose thread 0 then holds the totals (summed in block order) in tot[].
-template <int NV>
-__device__ bool last_block_reduce(double (&part)[NV], double *__restrict__ partials, int slot,
-                                  int *__restrict__ counter, double (&tot)[NV]) {
-  __shared__ int is_last;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) partials[(size_t)(slot + i) * MAXBLK + blockIdx.x] = part[i];
-    __threadfence();
-    const int prev = atomicAdd(counter, 1);
-    is_last = (prev == (int)gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  // all threads of the last block reduce the partials in fixed order
-  double loc[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLK)
-      s += __hip_atomic_load(&partials[(size_t)(slot + i) * MAXBLK + b], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    loc[i] = s;
-  }
-  block_sum<NV>(loc, tot);
-  if (threadIdx.x == 0) *counter = 0;
-  return true;
-}
-
-__device__ __forceinline__ bool solve_done(const SolveScalars *s) {
-  return __hip_atomic_load(&s->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-}
-
-// ---------------------------------------------------------------------------
-// sentinel wait/publish
-// ---------------------------------------------------------------------------
-template <typename T>
-struct Bits;
-template <>
-struct Bits<double> {
-  using U = unsigned long long;
-  static constexpr U sent = FCP_SENT64;
-  static __device__ __forceinline__ U to(double v) { return __double_as_longlong(v); }
-  static __device__ __forceinline__ double from(U u) { return __longlong_as_double(u); }
-  static constexpr U qnan = 0x7FF8000000000000ull;
-};
-template <>
-struct Bits<float> {
-  using U = unsigned int;
-  static constexpr U sent = FCP_SENT32;
-  static __device__ __forceinline__ U to(float v) { return __float_as_uint(v); }
-  static __device__ __forceinline__ float from(U u) { return __uint_as_float(u); }
-  static constexpr U qnan = 0x7FC00000u;
-};
-
-template <typename T>
-__device__ __forceinline__ T wait_value(T *p, int *err, int slp) {
-  using B = Bits<T>;
-  typename B::U *q = reinterpret_cast<typename B::U *>(p);
-  typename B::U v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (v != B::sent) return B::from(v);
-  // Bounded by wall time (100 MHz constant clock): a dependency that is not
-  // published within FCP_WAIT_TICKS records FCP_ERR_SWEEP and the wave
-  // continues with NaN, so the kernel always drains.
-  const unsigned long long t0 = wall_clock64();
-  unsigned spins = 0;
-  while (v == B::sent) {
-    if (slp >= 8)
-      __builtin_amdgcn_s_sleep(8);
-    else if (slp >= 2)
-      __builtin_amdgcn_s_sleep(2);
-    else
-      __builtin_amdgcn_s_sleep(1);
-    v = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((++spins & 63u) == 0 && wall_clock64() - t0 > FCP_WAIT_TICKS) {
-      __hip_atomic_store(err, FCP_ERR_SWEEP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      v = B::qnan;
-    }
-  }
-  return B::from(v);
-}
-
-template <typename T>
-__device__ __forceinline__ void publish(T *p, T v) {
-  using B = Bits<T>;
-  typename B::U u = B::to(v);
-  if (u == B::sent) u = B::qnan;
-  __hip_atomic_store(reinterpret_cast<typename B::U *>(p), u, __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
 
 template <typename T>
 __global__ void k_fill_sent(T *__restrict__ x, T *__restrict__ y, int n, int *__restrict__ tickets,
@@ -221,6 +97,85 @@ __global__ __launch_bounds__(BLK) void k_sweep(
         publish<T>(out + row, (T)1.0 / acc);
       }
     }
+  }
+}
+
+// Gauss-Seidel sweep (GaussSeidel, linear_solvers.f90:146-153): rows in the
+// level order of the strict lower triangle.  Columns left of the diagonal
+// take this sweep's values (waited for), the diagonal and the columns right
+// of it take the previous sweep's, summed in ascending column order as the
+// sequential loop does, so every sweep is bit-equal to it.
+__global__ __launch_bounds__(BLK) void k_gs_sweep(
+    const int *__restrict__ task_ptr, const int *__restrict__ order, int ntasks,
+    const int *__restrict__ ia, const int *__restrict__ diag, const int *__restrict__ ja,
+    const double *__restrict__ a, const double *__restrict__ rhs, const double *__restrict__ xold,
+    double *xnew, double *__restrict__ res, int *__restrict__ ticket, int *__restrict__ err,
+    const SolveScalars *__restrict__ s, int slp) {
+  if (solve_done(s)) return;
+  const int lane = threadIdx.x & 63;
+  for (;;) {
+    int t = 0;
+    if (lane == 0) t = atomicAdd(ticket, 1);
+    t = __shfl(t, 0, 64);
+    if (t >= ntasks) break;
+    const int idx = task_ptr[t] + lane;
+    if (idx < task_ptr[t + 1]) {
+      const int row = order[idx];
+      const int dg = diag[row];
+      const int e = ia[row + 1];
+      double acc = rhs[row];
+      for (int k = ia[row]; k < dg; ++k) acc = acc - a[k] * wait_value<double>(xnew + ja[k], err, slp);
+      for (int k = dg; k < e; ++k) acc = acc - a[k] * xold[ja[k]];
+      res[row] = acc;
+      publish<double>(xnew + row, xold[row] + acc / (a[dg] + FCP_SMALL));
+    }
+  }
+}
+
+// x = new sweep values; resl = |res|_1; on the first sweep res0 = resl and
+// factor = sum |a_ii x_i| + small (:156-180); convergence test.
+__global__ __launch_bounds__(BLK) void k_gs_update(
+    int n, const int *__restrict__ diag, const double *__restrict__ a,
+    const double *__restrict__ xnew, double *__restrict__ x, const double *__restrict__ res,
+    double *__restrict__ partials, int *__restrict__ counter, SolveScalars *__restrict__ s,
+    int crit, double tol_abs, double tol_rel, int itr_max) {
+  if (solve_done(s)) return;
+  const bool first = (s->iter == 0);
+  double part[3] = {0.0, 0.0, 0.0};
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    const double xi = xnew[i];
+    x[i] = xi;
+    const double ri = res[i];
+    part[0] += fabs(ri);
+    part[1] += ri * ri;
+    if (first) part[2] += fabs(a[diag[i]] * xi);
+  }
+  double blk[3], tot[3];
+  block_sum<3>(part, blk);
+  if (last_block_reduce<3>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+    const double resl = tot[0];
+    s->resl = resl;
+    s->l2 = sqrt(tot[1]);
+    s->iter += 1;
+    bool conv;
+    if (first) {
+      s->res0 = resl;
+      s->factor = tot[2] + FCP_SMALL;
+      s->resor = resl / s->factor;
+    }
+    const double rsm = resl / (s->res0 + FCP_SMALL);
+    s->rsm = rsm;
+    if (first && resl < tol_abs)
+      conv = true;  // :158-165
+    else if (crit == 0)
+      conv = (rsm < tol_rel) || (resl < tol_abs);
+    else
+      conv = (s->l2 <= tol_rel * s->l2b) || (resl < tol_abs);
+    if (!isfinite(resl)) {
+      s->status = FCP_ERR_NONFINITE;
+      conv = true;
+    }
+    if (conv || s->iter >= itr_max) s->done = 1;
   }
 }
 
@@ -300,29 +255,210 @@ __global__ __launch_bounds__(BLK) void k_bsweep(
 }
 
 // ---------------------------------------------------------------------------
-// SpMV on the SELL-64 copy
+// Pipelined row-block sweep on packed operands.
+//
+// Same block/step structure as k_bsweep, but every position's operands
+// (row id, dependency count and slots, coefficients, pivot, input value) are
+// stored step-major so a step's data is one contiguous load, and each thread
+// fetches them PD steps ahead into a ring of registers; the values of
+// dependencies in earlier blocks are fetched PD2 steps ahead (re-polled at
+// use time only if still unpublished).  Steps are separated by a workgroup
+// barrier; a thread handles at most one position per step in the pipelined
+// path and walks any further positions of an oversized step directly.
+// Prefetch loads are issued unconditionally (an unused slot reads position 0
+// and is flagged invalid) so the compiler's memory-counter waits stay exact:
+// a load inside a divergent branch makes every later wait drain all
+// outstanding prefetches.
 // ---------------------------------------------------------------------------
 template <typename T>
-struct Sell {
-  const int *base, *col, *ia;
-  const T *a;
+struct PSlot {
+  int row, cnt, ok;
+  T dv, xin;
+  int dep[FCP_PMAXD];
+  T av[FCP_PMAXD];
+  T rv[FCP_PMAXD];
 };
 
-// sum_k a_ik x_k of row i in CSR entry order (the reference's row loop,
-// linear_solvers.f90:493-498); loads of a wave's 64 rows are contiguous.
 template <typename T>
-__device__ __forceinline__ T sell_row(const Sell<T> &m, int i, const T *__restrict__ x, T init,
-                                      bool subtract) {
-  const int p = m.base[i >> 6] + (i & 63);
-  const int len = m.ia[i + 1] - m.ia[i];
-  T acc = init;
-  if (subtract)
-    for (int k = 0; k < len; ++k) acc = acc - m.a[p + 64 * k] * x[m.col[p + 64 * k]];
-  else
-    for (int k = 0; k < len; ++k) acc = acc + m.a[p + 64 * k] * x[m.col[p + 64 * k]];
-  return acc;
+struct PArgs {
+  const int *prc, *pdep;
+  const T *pa, *pdx;
+  int n;
+};
+
+// 16-byte vector loads of the packed operands (one int2, one int4, 16 or 32
+// bytes of coefficients and one (pivot, input) pair per position).
+template <typename T>
+__device__ __forceinline__ void p_fetch(const PArgs<T> &P, int idx, int &row, int &cnt, T &dv, T &xin,
+                                        int *dep, T *av) {
+  const int2 rc = reinterpret_cast<const int2 *>(P.prc)[idx];
+  row = rc.x;
+  cnt = rc.y;
+  const int4 dp = reinterpret_cast<const int4 *>(P.pdep)[idx];
+  dep[0] = dp.x;
+  dep[1] = dp.y;
+  dep[2] = dp.z;
+  dep[3] = dp.w;
+  if (sizeof(T) == 4) {
+    const float4 a4 = reinterpret_cast<const float4 *>(P.pa)[idx];
+    av[0] = (T)a4.x;
+    av[1] = (T)a4.y;
+    av[2] = (T)a4.z;
+    av[3] = (T)a4.w;
+    const float2 dx = reinterpret_cast<const float2 *>(P.pdx)[idx];
+    dv = (T)dx.x;
+    xin = (T)dx.y;
+  } else {
+    const double2 a0 = reinterpret_cast<const double2 *>(P.pa)[2 * idx];
+    const double2 a1 = reinterpret_cast<const double2 *>(P.pa)[2 * idx + 1];
+    av[0] = (T)a0.x;
+    av[1] = (T)a0.y;
+    av[2] = (T)a1.x;
+    av[3] = (T)a1.y;
+    const double2 dx = reinterpret_cast<const double2 *>(P.pdx)[idx];
+    dv = (T)dx.x;
+    xin = (T)dx.y;
+  }
 }
 
+// ns >= 1; sptr holds ns+1 step offsets.
+template <typename T>
+__device__ __forceinline__ void p_load1(PSlot<T> &S, const PArgs<T> &P, const int *sptr, int step,
+                                        int ns) {
+  const int sc = step < ns ? step : ns - 1;
+  const int idx = sptr[sc] + (int)threadIdx.x;
+  const bool ok = (step < ns) && (idx < sptr[sc + 1]);
+  S.ok = ok ? 1 : 0;
+  p_fetch<T>(P, ok ? idx : 0, S.row, S.cnt, S.dv, S.xin, S.dep, S.av);
+}
+
+template <typename T>
+__device__ __forceinline__ void p_load2(PSlot<T> &S, T *out) {
+  using B = Bits<T>;
+#pragma unroll
+  for (int m = 0; m < FCP_PMAXD; ++m) {
+    const bool need = S.ok && m < S.cnt && S.dep[m] < 0;
+    typename B::U *q = reinterpret_cast<typename B::U *>(out + (need ? (-S.dep[m] - 1) : 0));
+    S.rv[m] = B::from(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ T p_row_value(int cnt, T dv, T xin, const int *dep, const T *av,
+                                         const T *rv, const T *zl, T *out, int *err, int slp) {
+  using B = Bits<T>;
+  T acc = (MODE == 0) ? xin : xin / (dv + (T)FCP_SMALL);
+#pragma unroll
+  for (int m = 0; m < FCP_PMAXD; ++m) {
+    if (m < cnt) {
+      T v;
+      if (dep[m] >= 0)
+        v = zl[dep[m]];
+      else if (rv && B::to(rv[m]) != B::sent)
+        v = rv[m];
+      else
+        v = wait_value<T>(out + (-dep[m] - 1), err, slp);
+      acc = acc - av[m] * v;
+    }
+  }
+  return acc * dv;
+}
+
+template <typename T, int MODE>
+__device__ __forceinline__ void p_compute(const PSlot<T> &S, const PArgs<T> &P, const int *sptr,
+                                          int step, int ns, int lo, T *zl, T *out, int *err,
+                                          int slp) {
+  if (S.ok) {
+    const T val = p_row_value<T, MODE>(S.cnt, S.dv, S.xin, S.dep, S.av, S.rv, zl, out, err, slp);
+    zl[S.row - lo] = val;
+    publish<T>(out + S.row, val);
+  }
+  if (step >= ns) return;
+  // positions beyond one per thread (oversized steps): direct path
+  const int e = sptr[step + 1];
+  for (int idx = sptr[step] + (int)threadIdx.x + BLK; idx < e; idx += BLK) {
+    int row, cnt, dep[FCP_PMAXD];
+    T dv, xin, av[FCP_PMAXD];
+    p_fetch<T>(P, idx, row, cnt, dv, xin, dep, av);
+    const T val = p_row_value<T, MODE>(cnt, dv, xin, dep, av, nullptr, zl, out, err, slp);
+    zl[row - lo] = val;
+    publish<T>(out + row, val);
+  }
+}
+
+// Each step issues 8-9 vector memory loads and one store; PD steps in flight
+// stay under the 63 outstanding operations a wave can count.
+constexpr int PD = 6;   // steps of operand prefetch
+constexpr int PD2 = 3;  // steps of remote-value prefetch
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(BLK, 1) void k_psweep(PArgs<T> P, const int *__restrict__ step_ptr,
+                                                   const int *__restrict__ blk_step, int nblocks,
+                                                   int R, int sptr_off, T *out,
+                                                   int *__restrict__ ticket, int *__restrict__ err,
+                                                   const SolveScalars *__restrict__ s, int slp) {
+  extern __shared__ unsigned char smem_raw[];
+  T *zl = reinterpret_cast<T *>(smem_raw);
+  int *sptr = reinterpret_cast<int *>(smem_raw + sptr_off);
+  __shared__ int sb;
+  if (s && solve_done(s)) return;
+  if (threadIdx.x == 0) sb = atomicAdd(ticket, 1);
+  __syncthreads();
+  const int t = sb;
+  if (t >= nblocks) return;
+  const int b = (MODE == 1) ? nblocks - 1 - t : t;
+  const int lo = b * R;
+  const int st0 = blk_step[b];
+  const int ns = blk_step[b + 1] - st0;
+  for (int i = threadIdx.x; i <= ns; i += BLK) sptr[i] = step_ptr[st0 + i];
+  __syncthreads();
+  PSlot<T> S[PD];
+#pragma unroll
+  for (int j = 0; j < PD; ++j) p_load1<T>(S[j], P, sptr, j, ns);
+#pragma unroll
+  for (int j = 0; j < PD2; ++j) p_load2<T>(S[j], out);
+  // Steps past ns (at most PD-1 in the last group) are empty: their slots
+  // are invalid, so every group runs the same instruction stream.
+  for (int base = 0; base < ns; base += PD) {
+#pragma unroll
+    for (int j = 0; j < PD; ++j) {
+      const int step = base + j;
+      p_compute<T, MODE>(S[j], P, sptr, step, ns, lo, zl, out, err, slp);
+      __syncthreads();
+      p_load2<T>(S[(j + PD2) % PD], out);
+      p_load1<T>(S[j], P, sptr, step + PD, ns);
+    }
+  }
+}
+
+// pa/pd of a block schedule in precision T from the resident matrix and
+// pivots (repacked when either changed).
+template <typename T>
+__global__ void k_pack_vals(int n, const int *__restrict__ prc, const int *__restrict__ psrc,
+                            const double *__restrict__ a, const double *__restrict__ d,
+                            T *__restrict__ pa, T *__restrict__ pdx) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+#pragma unroll
+  for (int m = 0; m < FCP_PMAXD; ++m) {
+    const int k = psrc[(size_t)FCP_PMAXD * idx + m];
+    pa[(size_t)FCP_PMAXD * idx + m] = k >= 0 ? (T)a[k] : (T)0;
+  }
+  pdx[2 * (size_t)idx] = (T)d[prc[2 * idx]];
+}
+
+// Input values of a sweep in processing order, next to the pivots.
+template <typename T>
+__global__ void k_gather(int n, const int *__restrict__ prc, const T *__restrict__ in,
+                         T *__restrict__ pdx, const SolveScalars *__restrict__ s) {
+  if (s && solve_done(s)) return;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) pdx[2 * (size_t)idx + 1] = in[prc[2 * idx]];
+}
+
+// ---------------------------------------------------------------------------
+// SpMV on the SELL-64 copy
+// ---------------------------------------------------------------------------
 __global__ void k_sell_len(const int *__restrict__ ia, int n, int nslices, int *__restrict__ len) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s > nslices) return;
@@ -662,6 +798,7 @@ __global__ void k_zero_f64(int n, double *__restrict__ a, double *__restrict__ b
 
 // Copy of the FP32 inner-solve scalars' initial values: r32 given, x=0.
 __global__ __launch_bounds__(BLK) void k_init32(int n, const float *__restrict__ r,
+                                                float *__restrict__ reso,
                                                 double *__restrict__ partials,
                                                 int *__restrict__ counter,
                                                 SolveScalars *__restrict__ s) {
@@ -669,6 +806,7 @@ __global__ __launch_bounds__(BLK) void k_init32(int n, const float *__restrict__
   for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
     part[0] += fabs((double)r[i]);
     part[1] += (double)r[i] * (double)r[i];
+    if (reso) reso[i] = r[i];
   }
   double blk[2], tot[2];
   block_sum<2>(part, blk);
@@ -678,6 +816,9 @@ __global__ __launch_bounds__(BLK) void k_init32(int n, const float *__restrict__
     s->l2 = sqrt(tot[1]);
     s->l2b = sqrt(tot[1]);
     s->s0 = FCP_S0INIT;
+    s->alf = 1.0;
+    s->beto = 1.0;
+    s->gam = 1.0;
     s->iter = 0;
     s->done = 0;
     s->status = 0;
@@ -685,12 +826,6 @@ __global__ __launch_bounds__(BLK) void k_init32(int n, const float *__restrict__
   }
 }
 
-int red_grid(fcp_ctx *ctx, int n) {
-  int g = grid_for(n, BLK);
-  int cap = ctx->ncu > 0 ? ctx->ncu * 4 : 1024;
-  if (cap > MAXBLK) cap = MAXBLK;
-  return g < cap ? (g > 0 ? g : 1) : cap;
-}
 
 int sweep_grid(fcp_ctx *ctx, const Schedule &s) {
   int waves = s.ntasks;
@@ -705,12 +840,22 @@ int sweep_grid(fcp_ctx *ctx, const Schedule &s) {
 // ===========================================================================
 // host side
 // ===========================================================================
+namespace fcpk {
+int red_grid(fcp_ctx *ctx, int n) {
+  int g = grid_for(n, BLK);
+  int cap = ctx->ncu > 0 ? ctx->ncu * 4 : 1024;
+  if (cap > MAXBLK) cap = MAXBLK;
+  return g < cap ? (g > 0 ? g : 1) : cap;
+}
+}  // namespace fcpk
+
 int fcp_ensure_workspace(fcp_ctx *ctx) {
   const int n = ctx->n;
   DBuf<double> *v64[] = {&ctx->d, &ctx->res, &ctx->pk, &ctx->zk, &ctx->yk, &ctx->uk, &ctx->vk,
                          &ctx->reso, &ctx->x, &ctx->rhs, &ctx->qk};
   for (auto *b : v64) FCP_HIP(ctx, b->alloc(n));
-  DBuf<float> *v32[] = {&ctx->d32, &ctx->r32, &ctx->p32, &ctx->z32, &ctx->y32, &ctx->q32, &ctx->e32};
+  DBuf<float> *v32[] = {&ctx->d32, &ctx->r32, &ctx->p32, &ctx->z32, &ctx->y32,   &ctx->q32,
+                        &ctx->e32, &ctx->u32, &ctx->v32, &ctx->reso32};
   for (auto *b : v32) FCP_HIP(ctx, b->alloc(n));
   FCP_HIP(ctx, ctx->partials.alloc((size_t)MAXBLK * 4 * NSLOT));
   FCP_HIP(ctx, ctx->counters.alloc(16));
@@ -773,6 +918,18 @@ int sell_view<float>(fcp_ctx *ctx, Sell<float> *out) {
 }
 }  // namespace
 
+namespace fcpk {
+int check_err(fcp_ctx *ctx) {
+  int e = 0;
+  FCP_HIP(ctx, hipMemcpyAsync(&e, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (e) return fcp_fail(ctx, e, "sweep dependency wait exceeded its bound");
+  return 0;
+}
+int sell_view64(fcp_ctx *ctx, Sell<double> *out) { return sell_view<double>(ctx, out); }
+int sell_view32(fcp_ctx *ctx, Sell<float> *out) { return sell_view<float>(ctx, out); }
+}  // namespace fcpk
+
 int fcp_launch_spmv64(fcp_ctx *ctx, const double *, const double *x, double *y) {
   Sell<double> m;
   FCP_TRY(sell_view<double>(ctx, &m));
@@ -785,16 +942,6 @@ int fcp_launch_spmv64(fcp_ctx *ctx, const double *, const double *x, double *y) 
 namespace {
 
 // counters layout: [0..1] sweep tickets, [2] reduction counter
-int *tickets(fcp_ctx *ctx) { return ctx->counters.p; }
-int *redctr(fcp_ctx *ctx) { return ctx->counters.p + 2; }
-
-int check_err(fcp_ctx *ctx) {
-  int e = 0;
-  FCP_HIP(ctx, hipMemcpyAsync(&e, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (e) return fcp_fail(ctx, e, "sweep dependency wait exceeded its bound");
-  return 0;
-}
 
 template <typename T, int MODE>
 int launch_bsweep(fcp_ctx *ctx, const BlockSchedule &b, const T *a, const T *d, const T *in, T *out,
@@ -804,6 +951,79 @@ int launch_bsweep(fcp_ctx *ctx, const BlockSchedule &b, const T *a, const T *d, 
       b.rows.p, b.step_ptr.p, b.blk_step.p, b.nblocks, b.R, ctx->n, ctx->ia.p, ctx->diag.p,
       ctx->ja.p, a, ctx->tslot.p, d, in, out, ticket, ctx->errflag.p, s, ctx->tune_sweep_sleep);
   FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+struct PBufs;
+template <>
+struct PBufs<double> {
+  static DBuf<double> &pa(BlockSchedule &b) { return b.pa64; }
+  static DBuf<double> &pdx(BlockSchedule &b) { return b.pdx64; }
+  static uint64_t &ver(BlockSchedule &b) { return b.pver64; }
+};
+template <>
+struct PBufs<float> {
+  static DBuf<float> &pa(BlockSchedule &b) { return b.pa32; }
+  static DBuf<float> &pdx(BlockSchedule &b) { return b.pdx32; }
+  static uint64_t &ver(BlockSchedule &b) { return b.pver32; }
+};
+
+template <typename T>
+BlockSchedule &psched(fcp_ctx *ctx, bool upper) {
+  if (sizeof(T) == 4) return upper ? ctx->bupper4 : ctx->blower4;
+  return upper ? ctx->bupper : ctx->blower;
+}
+
+// The pipelined sweep applies when the packed operands cover every row and
+// the block plus its step table fit in LDS.
+template <typename T>
+bool use_psweep(fcp_ctx *ctx) {
+  if (ctx->tune_sweep_kind != 0) return false;
+  for (int u = 0; u < 2; ++u) {
+    BlockSchedule &b = psched<T>(ctx, u);
+    if (b.nblocks == 0 || b.maxd > FCP_PMAXD) return false;
+    const size_t lds = (size_t)b.R * sizeof(T) + (size_t)(b.max_steps + 2) * sizeof(int) + 256;
+    if (lds > ctx->lds_cap) return false;
+  }
+  return true;
+}
+
+template <typename T, int MODE>
+int launch_psweep(fcp_ctx *ctx, BlockSchedule &b, const T *in, T *out, int *ticket,
+                  const SolveScalars *s) {
+  const int n = ctx->n;
+  k_gather<T><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, b.prc.p, in, PBufs<T>::pdx(b).p, s);
+  PArgs<T> P{b.prc.p, b.pdep.p, PBufs<T>::pa(b).p, PBufs<T>::pdx(b).p, n};
+  const int sptr_off = (int)(((size_t)b.R * sizeof(T) + 15) & ~(size_t)15);
+  const size_t lds = (size_t)sptr_off + (size_t)(b.max_steps + 2) * sizeof(int);
+  k_psweep<T, MODE><<<b.nblocks, BLK, lds, ctx->stream>>>(P, b.step_ptr.p, b.blk_step.p, b.nblocks,
+                                                          b.R, sptr_off, out, ticket,
+                                                          ctx->errflag.p, s, ctx->tune_sweep_sleep);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+// Packs matrix coefficients and pivots of a block schedule when they changed
+// (keyed by the preconditioner setup epoch).
+template <typename T>
+int pack_block(fcp_ctx *ctx, BlockSchedule &b) {
+  const int n = ctx->n;
+  if (PBufs<T>::ver(b) == ctx->d_epoch) return 0;
+  FCP_HIP(ctx, PBufs<T>::pa(b).alloc((size_t)FCP_PMAXD * n));
+  FCP_HIP(ctx, PBufs<T>::pdx(b).alloc((size_t)2 * n));
+  k_pack_vals<T><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, b.prc.p, b.psrc.p, ctx->a.p,
+                                                            ctx->d.p, PBufs<T>::pa(b).p,
+                                                            PBufs<T>::pdx(b).p);
+  FCP_HIP(ctx, hipGetLastError());
+  PBufs<T>::ver(b) = ctx->d_epoch;
+  return 0;
+}
+
+// Packs both sweeps of the single-vector schedules.
+template <typename T>
+int psweep_pack(fcp_ctx *ctx) {
+  for (int u = 0; u < 2; ++u) FCP_TRY(pack_block<T>(ctx, psched<T>(ctx, u)));
   return 0;
 }
 
@@ -818,6 +1038,7 @@ template <typename T>
 int precond_setup(fcp_ctx *ctx, int kind, const T *a, T *d) {
   const int n = ctx->n;
   k_fill_sent<T><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(d, (T *)nullptr, n, tickets(ctx), nullptr);
+  ctx->d_epoch++;
   ProfScope ps(ctx, kind ? "dilu_setup" : "dic_setup");
   if (use_block_sweep(ctx, sizeof(T))) {
     if (kind == 0)
@@ -842,6 +1063,16 @@ int precond_apply(fcp_ctx *ctx, const T *a, const T *d, const T *r, T *y, T *z,
                   const SolveScalars *s) {
   const int n = ctx->n;
   k_fill_sent<T><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(y, z, n, tickets(ctx), s);
+  if (use_psweep<T>(ctx)) {
+    FCP_TRY(psweep_pack<T>(ctx));
+    {
+      ProfScope ps(ctx, sizeof(T) == 8 ? "sweep_fwd_f64" : "sweep_fwd_f32");
+      const int rc = launch_psweep<T, 0>(ctx, psched<T>(ctx, false), r, y, tickets(ctx), s);
+      if (rc < 0) return rc;
+    }
+    ProfScope ps(ctx, sizeof(T) == 8 ? "sweep_bwd_f64" : "sweep_bwd_f32");
+    return launch_psweep<T, 1>(ctx, psched<T>(ctx, true), y, z, tickets(ctx) + 1, s);
+  }
   if (use_block_sweep(ctx, sizeof(T))) {
     {
       ProfScope ps(ctx, sizeof(T) == 8 ? "sweep_fwd_f64" : "sweep_fwd_f32");
@@ -970,6 +1201,26 @@ int solve_fp64(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max
   FCP_HIP(ctx, hipMemcpyAsync(&ctx->h_scal[0], s, sizeof(SolveScalars), hipMemcpyDeviceToHost,
                               ctx->stream));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (solver == 3) {
+    // Gauss-Seidel: its residual norms come from the sweeps themselves
+    // (the early exit at :158-165 follows the first sweep).
+    auto it = [&]() {
+      k_fill_sent<double><<<g, BLK, 0, ctx->stream>>>(ctx->yk.p, nullptr, n, tickets(ctx), s);
+      {
+        ProfScope ps(ctx, "gs_sweep");
+        k_gs_sweep<<<sweep_grid(ctx, ctx->lower), BLK, 0, ctx->stream>>>(
+            ctx->lower.task_ptr.p, ctx->lower.order.p, ctx->lower.ntasks, ctx->ia.p, ctx->diag.p,
+            ctx->ja.p, a, b, x, ctx->yk.p, ctx->res.p, tickets(ctx), ctx->errflag.p, s,
+            ctx->tune_sweep_sleep);
+      }
+      k_gs_update<<<g, BLK, 0, ctx->stream>>>(n, ctx->diag.p, a, ctx->yk.p, x, ctx->res.p,
+                                               ctx->partials.p, redctr(ctx), s, crit, tol_abs,
+                                               tol_rel, itr_max);
+      FCP_HIP(ctx, hipGetLastError());
+      return 0;
+    };
+    return iterate_until_done(ctx, s, 0, itr_max, check_every, verbose, "GS", it);
+  }
   if (ctx->h_scal[0].res0 < tol_abs) return 0;  // :428-432
   if (verbose >= 2) printf("                    res0 = %10.3E\n", ctx->h_scal[0].res0);
   if (solver == 0 || solver == 2) {
@@ -1064,20 +1315,30 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
     }
     // inner FP32 solve of A e = r/|r|
     k_scale_to32<<<g, BLK, 0, ctx->stream>>>(n, ctx->res.p, so, ctx->r32.p, ctx->e32.p);
-    k_zero_f32<<<g, BLK, 0, ctx->stream>>>(n, ctx->p32.p, ctx->q32.p);
-    k_init32<<<g, BLK, 0, ctx->stream>>>(n, ctx->r32.p, ctx->partials.p, redctr(ctx), si);
+    k_zero_f32<<<g, BLK, 0, ctx->stream>>>(n, ctx->p32.p, solver == 2 ? ctx->u32.p : ctx->q32.p);
+    k_init32<<<g, BLK, 0, ctx->stream>>>(n, ctx->r32.p, solver == 2 ? ctx->reso32.p : nullptr,
+                                         ctx->partials.p, redctr(ctx), si);
     const int budget = itr_max - total_inner;
     // Reduce the correction residual only as far as this refinement step
     // needs (with a 0.5 margin), never below inner_tol.
     double itol = 0.5 * tol_rel / rel;
     if (itol < inner_tol) itol = inner_tol;
     if (itol > 0.5) itol = 0.5;
-    auto it = [&]() {
-      return pcg_iteration<float>(ctx, solver == 1 ? 1 : 0, ctx->a32.p, ctx->d32.p, ctx->e32.p,
-                                  ctx->r32.p, ctx->p32.p, ctx->z32.p, ctx->y32.p, ctx->q32.p, si, 1,
-                                  0.0, itol, budget);
-    };
-    FCP_TRY(iterate_until_done(ctx, si, 1, budget, check_every, verbose, "PCG32", it));
+    if (solver == 2) {
+      auto it = [&]() {
+        return bicgstab_iteration<float>(ctx, ctx->a32.p, ctx->d32.p, ctx->e32.p, ctx->r32.p,
+                                         ctx->reso32.p, ctx->p32.p, ctx->z32.p, ctx->y32.p,
+                                         ctx->u32.p, ctx->v32.p, si, 1, 0.0, itol, budget);
+      };
+      FCP_TRY(iterate_until_done(ctx, si, 1, budget, check_every, verbose, "BiCGStab32", it));
+    } else {
+      auto it = [&]() {
+        return pcg_iteration<float>(ctx, solver == 1 ? 1 : 0, ctx->a32.p, ctx->d32.p, ctx->e32.p,
+                                    ctx->r32.p, ctx->p32.p, ctx->z32.p, ctx->y32.p, ctx->q32.p, si,
+                                    1, 0.0, itol, budget);
+      };
+      FCP_TRY(iterate_until_done(ctx, si, 1, budget, check_every, verbose, "PCG32", it));
+    }
     total_inner += ctx->h_scal[1].iter;
     k_ir_update<<<g, BLK, 0, ctx->stream>>>(n, ctx->e32.p, so, x);
     FCP_TRY(true_residual(ctx, x, b, so));
@@ -1091,6 +1352,44 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
 }
 
 }  // namespace
+
+namespace fcpk {
+// FP64 pivots of the DIC (kind 0) / DILU (kind 1) preconditioner of the
+// resident matrix, plus FP32 copies of A and the pivots when fp32 is set.
+int prepare_precond(fcp_ctx *ctx, int kind, bool fp32) {
+  FCP_TRY(fcp_ensure_workspace(ctx));
+  if (ctx->dic_version != ctx->a_version || ctx->dic_kind != kind) {
+    FCP_TRY(precond_setup<double>(ctx, kind, ctx->a.p, ctx->d.p));
+    ctx->dic_version = ctx->a_version;
+    ctx->dic_kind = kind;
+  }
+  if (fp32) {
+    FCP_HIP(ctx, ctx->a32.alloc(ctx->nnz));
+    if (ctx->a32_version != ctx->a_version) {
+      k_cast_a<<<red_grid(ctx, ctx->nnz), BLK, 0, ctx->stream>>>(ctx->nnz, ctx->a.p, ctx->a32.p);
+      ctx->a32_version = ctx->a_version;
+    }
+    if (ctx->d32_version != ctx->dic_version) {
+      k_cast_a<<<red_grid(ctx, ctx->n), BLK, 0, ctx->stream>>>(ctx->n, ctx->d.p, ctx->d32.p);
+      ctx->d32_version = ctx->dic_version;
+    }
+  }
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+// Packed coefficients and pivots of a block schedule in FP64 / FP32.
+int pack_block64(fcp_ctx *ctx, BlockSchedule &b) { return pack_block<double>(ctx, b); }
+int pack_block32(fcp_ctx *ctx, BlockSchedule &b) { return pack_block<float>(ctx, b); }
+
+// z = M^{-1} r with the single-vector sweeps (current pivots).
+int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z) {
+  return precond_apply<double>(ctx, ctx->a.p, ctx->d.p, r, y, z, nullptr);
+}
+int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z) {
+  return precond_apply<float>(ctx, ctx->a32.p, ctx->d32.p, r, y, z, nullptr);
+}
+}  // namespace fcpk
 
 int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out) {
   FCP_TRY(fcp_ensure_workspace(ctx));
@@ -1108,11 +1407,15 @@ int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, doubl
   return 0;
 }
 
-// solver: 0 iccg, 1 dpcg, 2 bicgstab.  x, b device pointers of length n.
+// solver: 0 iccg, 1 dpcg, 2 bicgstab, 3 gauss-seidel.  x, b device pointers
+// of length n.
 int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max,
                      double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts) {
   FCP_TRY(fcp_ensure_workspace(ctx));
   const int prec = opts ? opts->precision : FCP_PREC_FP64;
+  if (solver == 3 && prec == FCP_PREC_MIXED)
+    return fcp_fail(ctx, FCP_ERR_UNSUPPORTED,
+                    "gauss-seidel runs in FP64 only (no refinement variant)");
   const int crit = opts ? opts->criterion : FCP_CRIT_REF_L1;
   const int ce = (opts && opts->check_every > 0) ? opts->check_every : 8;
   hipEvent_t e0, e1;

@@ -455,12 +455,45 @@ __global__ void k_block_of(const int *__restrict__ order, int n, int R, int *__r
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) key[i] = order[i] / R;
 }
+
+// Static part of the packed operands: dependency list of every position in
+// processing order (strict-lower entries for the forward sweep, strict-upper
+// for the backward sweep), in CSR order.
+__global__ void k_pack_static(const int *__restrict__ rows, int n, int R, int upper,
+                              const int *__restrict__ ia, const int *__restrict__ diag,
+                              const int *__restrict__ ja, int *__restrict__ prc,
+                              int *__restrict__ pdep, int *__restrict__ psrc,
+                              int *__restrict__ maxd) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const int row = rows[idx];
+  const int lo = (row / R) * R;
+  const int hi = (lo + R < n) ? lo + R : n;
+  const int b = upper ? diag[row] + 1 : ia[row];
+  const int e = upper ? ia[row + 1] : diag[row];
+  const int c = e - b;
+  prc[2 * idx] = row;
+  prc[2 * idx + 1] = c;
+  atomicMax(maxd, c);
+  for (int m = 0; m < FCP_PMAXD; ++m) {
+    int dep = 0, src = -1;
+    if (m < c) {
+      const int k = b + m;
+      const int j = ja[k];
+      const bool inblk = upper ? (j < hi) : (j >= lo);
+      dep = inblk ? (j - lo) : -(j + 1);
+      src = k;
+    }
+    pdep[(size_t)FCP_PMAXD * idx + m] = dep;
+    psrc[(size_t)FCP_PMAXD * idx + m] = src;
+  }
+}
 }  // namespace
 
 // Row-block schedule from a level schedule: a stable sort of the level-major
 // row order by block id gives every block's rows in level order; steps are
 // the runs of equal level inside a block.
-static int build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, BlockSchedule &b) {
+int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, BlockSchedule &b) {
   const int n = ctx->n;
   b.R = R;
   b.nblocks = (n + R - 1) / R;
@@ -476,6 +509,22 @@ static int build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, BlockSch
   FCP_HIP(ctx, hipMemcpyAsync(lev.data(), s.lev.p, sizeof(int) * n, hipMemcpyDeviceToHost,
                               ctx->stream));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // packed static operands
+  FCP_HIP(ctx, b.prc.alloc((size_t)2 * n));
+  FCP_HIP(ctx, b.pdep.alloc((size_t)FCP_PMAXD * n));
+  FCP_HIP(ctx, b.psrc.alloc((size_t)FCP_PMAXD * n));
+  {
+    DBuf<int> mx;
+    FCP_HIP(ctx, mx.alloc(1));
+    FCP_HIP(ctx, hipMemsetAsync(mx.p, 0, sizeof(int), ctx->stream));
+    k_pack_static<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(b.rows.p, n, R, upper, ctx->ia.p,
+                                                             ctx->diag.p, ctx->ja.p, b.prc.p,
+                                                             b.pdep.p, b.psrc.p, mx.p);
+    FCP_HIP(ctx, hipGetLastError());
+    FCP_HIP(ctx, hipMemcpyAsync(&b.maxd, mx.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  b.pver64 = b.pver32 = ~0ull;
   std::vector<int> sp, bs(b.nblocks + 1, 0);
   sp.reserve(b.nblocks * 64 + 1);
   int prev_blk = -1, prev_lev = -1;
@@ -494,6 +543,9 @@ static int build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, BlockSch
   for (int q = prev_blk + 1; q <= b.nblocks; ++q) bs[q] = (int)sp.size();
   b.nsteps = (int)sp.size();
   sp.push_back(n);
+  b.max_steps = 0;
+  for (int q = 0; q < b.nblocks; ++q)
+    if (bs[q + 1] - bs[q] > b.max_steps) b.max_steps = bs[q + 1] - bs[q];
   FCP_HIP(ctx, b.step_ptr.alloc(sp.size()));
   FCP_HIP(ctx, b.blk_step.alloc(bs.size()));
   FCP_HIP(ctx, hipMemcpyAsync(b.step_ptr.p, sp.data(), sizeof(int) * sp.size(), hipMemcpyHostToDevice,
@@ -505,14 +557,18 @@ static int build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, BlockSch
 }
 
 int fcp_build_block_schedules(fcp_ctx *ctx, int R) {
-  FCP_TRY(build_block_schedule(ctx, ctx->lower, R, ctx->blower));
-  FCP_TRY(build_block_schedule(ctx, ctx->upper, R, ctx->bupper));
+  ctx->msched.clear();
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, R, 0, ctx->blower));
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, R, 1, ctx->bupper));
+  const int R4 = fcp_block_rows(ctx, 4);
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, R4, 0, ctx->blower4));
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, R4, 1, ctx->bupper4));
   return 0;
 }
 
 int fcp_build_schedules(fcp_ctx *ctx) {
   FCP_TRY(build_one_schedule(ctx, 0, ctx->lower));
   FCP_TRY(build_one_schedule(ctx, 1, ctx->upper));
-  FCP_TRY(fcp_build_block_schedules(ctx, fcp_block_rows(ctx)));
+  FCP_TRY(fcp_build_block_schedules(ctx, fcp_block_rows(ctx, 8)));
   return 0;
 }

@@ -238,6 +238,31 @@ class Context:
         info["status"] = rc
         return rc, info
 
+    def csrsolve_multi(self, solver: str, fi, rhs, itr_max=1000, tol_abs=0.0, tol_rel=1e-10,
+                       verbose=0, precision=FCP_PREC_FP64, criterion=FCP_CRIT_REF_L1,
+                       inner_tol=1e-4, max_outer=30, check_every=8):
+        """Batched csrsolve of nrhs columns sharing the resident matrix.  fi and
+        rhs are (nrhs, n) row-major arrays (numpy or torch), i.e. Fortran
+        fi(n, nrhs); fi is updated in place.  Returns (status, info, res0[nrhs])."""
+        nrhs, n = rhs.shape
+        opts = _lib.SolveOpts()
+        self.lib.fcp_solve_opts_default(C.byref(opts))
+        opts.precision = precision
+        opts.criterion = criterion
+        opts.inner_tol = inner_tol
+        opts.max_outer = max_outer
+        opts.check_every = check_every
+        res0 = (C.c_double * nrhs)()
+        sb = solver.encode()
+        rc = self.lib.fcp_csrsolve_multi(self.h, sb, len(sb), int(nrhs), _ptr(fi)[0],
+                                         int(fi.shape[1]), _ptr(_f64(rhs))[0], res0,
+                                         int(itr_max), float(tol_abs), float(tol_rel),
+                                         int(verbose), C.byref(opts))
+        self._check(rc, "fcp_csrsolve_multi")
+        info = {k: getattr(opts, k) for k, _ in _lib.SolveOpts._fields_}
+        info["status"] = rc
+        return rc, info, np.array(res0[:])
+
     def tune(self, param: int, value: int):
         self._check(self.lib.fcp_tune_set(self.h, int(param), int(value)), "fcp_tune_set")
 
@@ -265,7 +290,8 @@ def explicit_solve(solver: str, ia, ja, a, diag, fi, rhs, itr_max, tol_abs, tol_
     a = np.ascontiguousarray(a, np.float64)
     rhs = np.ascontiguousarray(rhs, np.float64)
     resor = C.c_double()
-    fn = {"iccg": lib.fcp_iccg, "dpcg": lib.fcp_dpcg, "bicgstab": lib.fcp_bicgstab}[solver]
+    fn = {"iccg": lib.fcp_iccg, "dpcg": lib.fcp_dpcg, "bicgstab": lib.fcp_bicgstab,
+          "gauss-seidel": lib.fcp_gauss_seidel, "gauss_seidel": lib.fcp_gauss_seidel}[solver]
     rc = fn(n, len(ja1), _ptr(ia1)[0], _ptr(ja1)[0], _ptr(a)[0], _ptr(dg1)[0], _ptr(fi)[0],
             _ptr(rhs)[0], C.byref(resor), int(itr_max), float(tol_abs), float(tol_rel), int(verbose))
     if rc < 0:

@@ -59,3 +59,107 @@ def poisson_jittered_inputs(mesh, vol, xc, yc, zc):
     phi_prev[n:] = np.where(bc_kind == 0, 0.0, phi_prev[own_b])
     return dict(gamma=gamma, mu_f=-gamma, source=s, bc_kind=bc_kind, phi_prev=phi_prev,
                 phi_bc=np.zeros(mesh.n_total))
+
+
+# ---------------------------------------------------------------------------
+# Momentum-like convection-diffusion (192^3; 128^3 jittered for sensitivities)
+# ---------------------------------------------------------------------------
+# Unit cube of N^3 hexahedra; x+ is an outlet, the other five patches are
+# no-slip walls.  Velocity u(x) = sum_{m<8} a_m e_m sin(2 pi k_m.x + theta_m)
+# with a_m ~ N(0,1), k_m in {1..4}^3, theta_m ~ U[0,2pi), e_m a random unit
+# vector (seed 5, streams 30-35).  Face fluxes F_f = u(x_f).S_f on inner and
+# outlet faces (0 on walls) are made discretely divergence-free by one
+# Poisson projection (unit face diffusivity, p = 0 on the outlet, zero normal
+# gradient on walls): F_f <- F_f - L_PN (p_N - p_P), where L is the assembled
+# Laplacian, so the corrected cell divergence equals the projection solve's
+# residual.  Face diffusivity Gamma_f = max(|F_f|, F_min) / (Pe_f D_f) with
+# Pe_f = 10^U(-1,2) (stream 40), F_min = 1e-3 mean |F_inner|.  Components
+# u, v, w of the field are the three transported quantities; their sources
+# s ~ U[-1,1] (streams 50-52) times V, linear-upwind deferred correction
+# (gamma = 1) with Gauss gradients, under-relaxation 0.8, walls Dirichlet 0,
+# outlet zero-gradient.  All three share one matrix.
+CONVDIFF_SEED = 5
+CONVDIFF_URF = 0.8
+CONVDIFF_TYPES = ("wall", "outlet", "wall", "wall", "wall", "wall")
+
+
+def convdiff_mesh(N: int = 192, jitter: float = 0.0):
+    return hex_mesh(N, jitter=jitter, seed=2, types=CONVDIFF_TYPES)
+
+
+def _normal01(seed, s1, s2, n):
+    from .rng import uniform01
+    u1 = uniform01(seed, s1, np.arange(n, dtype=np.uint64))
+    u2 = uniform01(seed, s2, np.arange(n, dtype=np.uint64))
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def fourier_velocity(x, y, z, seed: int = CONVDIFF_SEED, modes: int = 8):
+    """u(x) of the convection-diffusion workloads at points (x, y, z)."""
+    from .rng import uniform01
+    a = _normal01(seed, 30, 31, modes)
+    k = 1.0 + np.floor(4.0 * uniform01(seed, 32, np.arange(3 * modes, dtype=np.uint64)))
+    k = k.reshape(modes, 3)
+    th = 2.0 * np.pi * uniform01(seed, 33, np.arange(modes, dtype=np.uint64))
+    e = _normal01(seed, 34, 35, 3 * modes).reshape(modes, 3)
+    e /= np.linalg.norm(e, axis=1, keepdims=True)
+    u = np.zeros((np.size(x), 3))
+    for m in range(modes):
+        w = a[m] * np.sin(2.0 * np.pi * (k[m, 0] * x + k[m, 1] * y + k[m, 2] * z) + th[m])
+        u += w[:, None] * e[m][None, :]
+    return u
+
+
+def convdiff_flux(mesh, geo, csr, ctx):
+    """Divergence-free face fluxes by one Poisson projection on ``ctx`` (a
+    Context holding ``mesh`` and its CSR arrays ``csr``, 0-based; the resident
+    matrix is overwritten).  Returns (flux[F], cell divergence[n])."""
+    n, Fi, F = mesh.n_cells, mesh.n_inner, mesh.n_faces
+    uf = fourier_velocity(geo["xf"], geo["yf"], geo["zf"])
+    flux = uf[:, 0] * geo["arx"] + uf[:, 1] * geo["ary"] + uf[:, 2] * geo["arz"]
+    patch = mesh.patch_of_bface()
+    outlet = np.asarray(mesh.bctype_codes()[patch] == 4)
+    flux[Fi:][~outlet] = 0.0
+    own, nb = mesh.owner, mesh.neighbour
+    div = np.bincount(own, weights=flux, minlength=n) - np.bincount(nb, weights=flux[:Fi],
+                                                                    minlength=n)
+    bc_kind = np.where(outlet, 0, 1).astype(np.uint8)
+    a = np.zeros(ctx.nnz)
+    rhs = -div
+    ctx.laplacian_face(-np.ones(F), bc_kind, np.zeros(mesh.n_total), rhs, a_out=a)
+    p = np.zeros(n)
+    rc, info = ctx.csrsolve("iccg", p, rhs, itr_max=20000, tol_abs=0.0, tol_rel=1e-13,
+                            criterion=1)
+    # L = -a: F_f -= L_PN (p_N - p_P) = F_f + a_PN (p_N - p_P)
+    flux[:Fi] += a[csr["slot_pn"]] * (p[nb] - p[own[:Fi]])
+    rowsum = np.add.reduceat(a, csr["ia"][:-1])
+    ob = np.nonzero(outlet)[0]
+    cells = own[Fi + ob]
+    flux[Fi + ob] += rowsum[cells] * p[cells]
+    div2 = np.bincount(own, weights=flux, minlength=n) - np.bincount(nb, weights=flux[:Fi],
+                                                                     minlength=n)
+    return flux, div2
+
+
+def convdiff_inputs(mesh, geo, flux):
+    """Face diffusivity, boundary kinds, sources and current fields of the
+    three transported components (see the block comment above)."""
+    n, Fi, F = mesh.n_cells, mesh.n_inner, mesh.n_faces
+    patch = mesh.patch_of_bface()
+    outlet = np.asarray(mesh.bctype_codes()[patch] == 4)
+    D = np.empty(F)
+    D[:Fi] = geo["Df"]
+    D[Fi:] = geo["srdb"]
+    pe = 10.0 ** uniform(CONVDIFF_SEED, 40, F, -1.0, 2.0)
+    fmin = 1e-3 * np.mean(np.abs(flux[:Fi]))
+    gamma = np.maximum(np.abs(flux), fmin) / (pe * D)
+    bc_kind = np.where(outlet, 2, 0).astype(np.uint8)   # kind 2 in/outlet, 0 Dirichlet wall
+    uc = fourier_velocity(geo["xc"], geo["yc"], geo["zc"])
+    own_b = mesh.owner[Fi:]
+    phi = np.zeros((3, mesh.n_total))
+    src = np.zeros((3, n))
+    for r in range(3):
+        phi[r, :n] = uc[:, r]
+        phi[r, n:] = np.where(outlet, uc[own_b, r], 0.0)
+        src[r] = uniform(CONVDIFF_SEED, 50 + r, n, -1.0, 1.0)
+    return dict(gamma=gamma, bc_kind=bc_kind, phi=phi, source=src, urf=CONVDIFF_URF)

@@ -147,6 +147,17 @@ module fcp_capi
       type(fcp_solve_opts) :: opts
     end function
 
+    integer(c_int) function fcp_csrsolve_multi(ctx, solver, solver_len, nrhs, fi, ldfi, rhs, res0, &
+        itr_max, tol_abs, tol_rel, verbose, opts) bind(C, name='fcp_csrsolve_multi')
+      import :: c_int, c_double, c_ptr, c_char, fcp_solve_opts
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: solver(*)
+      integer(c_int), value :: solver_len, nrhs, ldfi, itr_max, verbose
+      real(c_double) :: fi(*), rhs(*), res0(*)
+      real(c_double), value :: tol_abs, tol_rel
+      type(fcp_solve_opts) :: opts
+    end function
+
     integer(c_int) function fcp_iccg(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, &
         tol_abs, tol_rel, verbose) bind(C, name='fcp_iccg')
       import :: c_int, c_double
@@ -167,6 +178,15 @@ module fcp_capi
 
     integer(c_int) function fcp_bicgstab(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, &
         tol_abs, tol_rel, verbose) bind(C, name='fcp_bicgstab')
+      import :: c_int, c_double
+      integer(c_int), value :: n, nnz, itr_max, verbose
+      integer(c_int) :: ia(*), ja(*), diag(*)
+      real(c_double) :: a(*), fi(*), rhs(*), resor
+      real(c_double), value :: tol_abs, tol_rel
+    end function
+
+    integer(c_int) function fcp_gauss_seidel(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, &
+        tol_abs, tol_rel, verbose) bind(C, name='fcp_gauss_seidel')
       import :: c_int, c_double
       integer(c_int), value :: n, nnz, itr_max, verbose
       integer(c_int) :: ia(*), ja(*), diag(*)

@@ -8,8 +8,8 @@
 !   grad_gauss(u, dudxi)       gradients.f90:1607
 !   csrsolve(solver, fi, rhs, res0, itr_max, tol_abs, tol_rel, chvar)
 !                              linear_solvers.f90:40
-!   iccg / dpcg / bicgstab(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max,
-!                          tol_abs, tol_rel, chvar, verbose)  linear_solvers.f90:364,206,548
+!   iccg / dpcg / bicgstab / GaussSeidel(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max,
+!                          tol_abs, tol_rel, chvar, verbose)  linear_solvers.f90:364,206,548,96
 ! The mesh is handed over once with fcp_read_mesh (the arrays read_mesh
 ! produces, geometry.f90:118-406); geometry is computed on the GPU and
 ! returned in the module arrays below (geometry.f90:19-83 names).
@@ -169,6 +169,25 @@ contains
     rc = fcp_bicgstab(n, nz, ia_, ja_, a_, diag_, fi, rhs, resor, itr_max, tol_abs, tol_rel, vb)
     if (rc < 0) then
       write(*,'(3a,i0)') '  bicgstab (', trim(chvar), ') failed: ', rc
+      stop 1
+    endif
+  end subroutine
+
+  subroutine GaussSeidel(n, nz, ia_, ja_, a_, diag_, fi, rhs, resor, itr_max, tol_abs, tol_rel, chvar, &
+                         verbose)
+    integer, intent(in) :: n, nz, itr_max
+    integer, intent(in) :: ia_(n+1), ja_(nz), diag_(n)
+    real(dp), intent(in) :: a_(nz), rhs(n), tol_abs, tol_rel
+    real(dp), intent(inout) :: fi(n)
+    real(dp), intent(out) :: resor
+    character(len=*), intent(in) :: chvar
+    logical, intent(in) :: verbose
+    integer(c_int) :: rc, vb
+    vb = 0
+    if (verbose) vb = 2
+    rc = fcp_gauss_seidel(n, nz, ia_, ja_, a_, diag_, fi, rhs, resor, itr_max, tol_abs, tol_rel, vb)
+    if (rc < 0) then
+      write(*,'(3a,i0)') '  Gauss-Seidel (', trim(chvar), ') failed: ', rc
       stop 1
     endif
   end subroutine

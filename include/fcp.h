@@ -197,6 +197,21 @@ int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi,
                  double tol_rel, const char *chvar, int chvar_len, int verbose,
                  fcp_solve_opts *opts);
 
+/* Batched csrsolve: nrhs (1..4) systems sharing the resident matrix, as the
+ * three momentum components of velocity.f90:638-748 share one operator
+ * (there each is a separate csrsolve call).  fi is fi(ldfi, nrhs) and rhs is
+ * rhs(numCells, nrhs), column-major (Fortran order), ldfi >= numCells; only
+ * rows 1..numCells of each column are touched.  solver "iccg" | "bicgstab".
+ * Each column runs its own recurrence, bit-equal to fcp_csrsolve on that
+ * column; SpMV and the preconditioner sweeps read the operator once for all
+ * columns.  res0[nrhs] (may be NULL) receives each column's res0; opts
+ * reports the largest iteration count and the worst true relative residual.
+ * Returns 0, the iteration count when a column did not converge, or < 0. */
+int fcp_csrsolve_multi(fcp_ctx *ctx, const char *solver, int solver_len, int nrhs,
+                       double *fi, int ldfi, const double *rhs, double *res0,
+                       int itr_max, double tol_abs, double tol_rel, int verbose,
+                       fcp_solve_opts *opts);
+
 /* Explicit-array entry points with the reference signatures
  * iccg/dpcg/bicgstab/GaussSeidel(n,nnz,ia,ja,a,diag,fi,rhs,resor,itr_max,
  * tol_abs,tol_rel,chvar,verbose) (linear_solvers.f90:364,206,548,96).
@@ -210,6 +225,9 @@ int fcp_dpcg(int n, int nnz, const int *ia, const int *ja, const double *a,
 int fcp_bicgstab(int n, int nnz, const int *ia, const int *ja, const double *a,
                  const int *diag, double *fi, const double *rhs, double *resor,
                  int itr_max, double tol_abs, double tol_rel, int verbose);
+int fcp_gauss_seidel(int n, int nnz, const int *ia, const int *ja, const double *a,
+                     const int *diag, double *fi, const double *rhs, double *resor,
+                     int itr_max, double tol_abs, double tol_rel, int verbose);
 
 /* --- launch-shape tuning (autotuner hook) --------------------------------- */
 #define FCP_TUNE_SWEEP_WAVES 1   /* resident waves of a sweep kernel (0 = auto) */
