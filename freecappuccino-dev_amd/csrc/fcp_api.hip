@@ -569,6 +569,62 @@ int fcp_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi) {
   return 0;
 }
 
+namespace {
+__global__ void k_plus_one(int n, int *__restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] += 1;
+}
+}  // namespace
+
+int fcp_rcm(fcp_ctx *ctx, int pp_iters, int *perm) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!perm) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_rcm: null argument");
+  if (pp_iters < 0 || pp_iters > 64) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_rcm: pp_iters");
+  (void)hipSetDevice(ctx->device);
+  Arg<int> out;
+  FCP_TRY(arg_out(ctx, perm, (size_t)ctx->n, out));
+  FCP_TRY(fcp_rcm_device(ctx, pp_iters, out.dev));
+  k_plus_one<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(ctx->n, out.dev);
+  FCP_HIP(ctx, hipGetLastError());
+  FCP_TRY(arg_back(ctx, out));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_create_lsq_grad_matrix(fcp_ctx *ctx, int method, int compat, double *dmat) {
+  FCP_TRY(need(ctx, true, true, false));
+  if (method < 1 || method > 3)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_create_lsq_grad_matrix: method %d", method);
+  (void)hipSetDevice(ctx->device);
+  FCP_TRY(fcp_ensure_workspace(ctx));
+  FCP_TRY(fcp_launch_lsq_matrix(ctx, method, compat ? 1 : 0));
+  if (dmat) {
+    const size_t cnt = (method == 3 ? 18 : 9) * (size_t)ctx->n;
+    FCP_HIP(ctx, hipMemcpyAsync(dmat, ctx->lsq_dmat.p, cnt * sizeof(double), hipMemcpyDefault,
+                                ctx->stream));
+  }
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_grad(fcp_ctx *ctx, int method, const double *phi, double *dphidxi) {
+  FCP_TRY(need(ctx, true, true, false));
+  if (!phi || !dphidxi) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_grad: null argument");
+  if (method == FCP_GRAD_GAUSS) return fcp_grad_gauss(ctx, phi, dphidxi);
+  if (method != ctx->lsq_method)
+    return fcp_fail(ctx, FCP_ERR_STATE, "fcp_grad: least-squares matrices of method %d not created",
+                    method);
+  (void)hipSetDevice(ctx->device);
+  const size_t ntot = (size_t)ctx->n + ctx->Fb;
+  Arg<double> in, out;
+  FCP_TRY(arg_in(ctx, phi, ntot, in));
+  FCP_TRY(arg_out(ctx, dphidxi, 3 * ntot, out));
+  FCP_TRY(fcp_launch_grad_lsq(ctx, in.dev, out.dev));
+  FCP_TRY(arg_back(ctx, out));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
 int fcp_update_boundary(fcp_ctx *ctx, double *phi) {
   FCP_TRY(need(ctx, true, false, false));
   if (!phi) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_update_boundary: null argument");

@@ -88,6 +88,7 @@ struct MultiSched {
 struct SolveScalars {
   double sk, s0, pkapk, alf, bet, res0, resl, factor, resor, rsm;
   double l2, l2b, om, gam, beto, ukreso, vv, vr;
+  double tol;  // per-column relative tolerance of a batched inner solve
   int iter, done, status, pad;
 };
 
@@ -156,6 +157,10 @@ struct fcp_ctx {
   DBuf<SolveScalars> scal;   // device scalars
   SolveScalars *h_scal = nullptr;  // pinned mirror
   DBuf<int> errflag;         // [1]
+
+  // -------- least-squares gradient matrices --------
+  DBuf<double> lsq_dmat;  // Dmat(9,n) or D(3,6,n)
+  int lsq_method = 0, lsq_compat = 1;
 
   // -------- batched solves (interleaved vectors, NRP values per row) --------
   std::map<int, std::unique_ptr<MultiSched>> msched;  // keyed by rows per block
@@ -253,6 +258,9 @@ int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef,
                         double urf, double *su, double *sp);
 int fcp_launch_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt, const double *fo,
                    const double *foo, double *su, double *sp);
+// fcp_lsq.hip
+int fcp_launch_lsq_matrix(fcp_ctx *ctx, int method, int compat);
+int fcp_launch_grad_lsq(fcp_ctx *ctx, const double *phi, double *dphi);
 // fcp_krylov.hip
 int fcp_build_sell(fcp_ctx *ctx);
 int fcp_launch_spmv64(fcp_ctx *ctx, const double *a, const double *x, double *y);
@@ -267,5 +275,9 @@ int fcp_solve_multi_device(fcp_ctx *ctx, int solver, int nrhs, double *fi, int l
 
 // rocPRIM-based primitives (fcp_prim.hip)
 int fcp_exclusive_scan_int(fcp_ctx *ctx, const int *in, int *out, int n);
+int fcp_sort_keys_u64(fcp_ctx *ctx, const unsigned long long *keys_in,
+                      unsigned long long *keys_out, int n, int end_bit);
+// fcp_rcm.hip: reverse Cuthill-McKee permutation of the resident CSR graph
+int fcp_rcm_device(fcp_ctx *ctx, int pp_iters, int *new_of_old_dev);
 int fcp_sort_pairs_int(fcp_ctx *ctx, const int *keys_in, int *keys_out,
                        const int *vals_in, int *vals_out, int n, int max_key);

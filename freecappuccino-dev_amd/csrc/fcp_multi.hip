@@ -129,7 +129,9 @@ __global__ void mk_scale_to32(int n, const double *__restrict__ res,
 }
 
 // Inner (FP32) scalars: norms of the scaled residual; a column whose outer
-// residual already meets tol_rel starts (and stays) done.
+// residual already meets tol_rel starts (and stays) done.  Each column's
+// inner tolerance is what its own refinement step needs (0.5 tol/rel,
+// clamped to [inner_tol, 0.5]), the rule of the single-column solve.
 template <int NR, int NRP>
 __global__ __launch_bounds__(BLK) void mk_init32(int n, const float *__restrict__ r,
                                                  float *__restrict__ reso,
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(BLK) void mk_init32(int n, const float *__restrict_
                                                  int *__restrict__ counter,
                                                  SolveScalars *__restrict__ s,
                                                  const SolveScalars *__restrict__ so,
-                                                 double tol_rel) {
+                                                 double tol_rel, double inner_tol) {
   double part[2 * NR];
 #pragma unroll
   for (int q = 0; q < 2 * NR; ++q) part[q] = 0.0;
@@ -167,7 +169,12 @@ __global__ __launch_bounds__(BLK) void mk_init32(int n, const float *__restrict_
       S.iter = 0;
       S.status = 0;
       S.factor = 0.0;
-      S.done = (so[r].l2 <= tol_rel * so[r].l2b || so[r].l2 == 0.0) ? 1 : 0;
+      const double rel = so[r].l2 / (so[r].l2b > 0 ? so[r].l2b : 1.0);
+      double itol = 0.5 * tol_rel / rel;
+      if (itol < inner_tol) itol = inner_tol;
+      if (itol > 0.5) itol = 0.5;
+      S.tol = itol;
+      S.done = (rel <= tol_rel || so[r].l2 == 0.0) ? 1 : 0;
     }
   }
 }
@@ -352,10 +359,11 @@ __global__ __launch_bounds__(BLK) void mk_xr_update(
       const double rsm = resl / (S.res0 + FCP_SMALL);
       S.rsm = rsm;
       bool conv;
+      const double tr = tol_rel < 0.0 ? S.tol : tol_rel;  // < 0: per-column tolerance
       if (crit == 0)
-        conv = (rsm < tol_rel) || (resl < tol_abs);
+        conv = (rsm < tr) || (resl < tol_abs);
       else
-        conv = (S.l2 <= tol_rel * S.l2b) || (resl < tol_abs);
+        conv = (S.l2 <= tr * S.l2b) || (resl < tol_abs);
       if (!isfinite(resl)) {
         S.status = FCP_ERR_NONFINITE;
         conv = true;
@@ -1095,7 +1103,6 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
                                 ctx->stream));
     FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     bool all = true, stall = false;
-    double worst = 0.0;
     for (int c = 0; c < NR; ++c) {
       const double rel = h[c].l2 / (h[c].l2b > 0 ? h[c].l2b : 1.0);
       if (outer == 0) res.res0[c] = h[c].res0;
@@ -1106,7 +1113,6 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
         all = false;
         if (outer > 0 && rel > 0.5 * prev[c]) stall = true;
       }
-      worst = rel > worst ? rel : worst;
       prev[c] = rel;
     }
     if (all || total >= itr_max) break;
@@ -1139,11 +1145,10 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
     mk_scale_to32<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, V64.r, so, V32.r, V32.x);
     mk_zero2<float><<<g, BLK, 0, ctx->stream>>>(m, V32.p, V32.u);
     mk_init32<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, V32.r, solver == 2 ? V32.reso : nullptr,
-                                                   ctx->partials.p, redctr(ctx), si, so, tol_rel);
+                                                   ctx->partials.p, redctr(ctx), si, so, tol_rel,
+                                                   inner_tol);
     const int budget = itr_max - total;
-    double itol = 0.5 * tol_rel / worst;
-    if (itol < inner_tol) itol = inner_tol;
-    if (itol > 0.5) itol = 0.5;
+    const double itol = -1.0;  // per-column tolerances set by mk_init32
     if (solver == 2) {
       auto it = [&]() {
         return mbicgstab_iteration<float, NR, NRP>(ctx, ctx->a32.p, V32, si, 1, 0.0, itol, budget);

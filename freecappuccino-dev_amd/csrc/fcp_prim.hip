@@ -28,3 +28,14 @@ int fcp_sort_pairs_int(fcp_ctx *ctx, const int *keys_in, int *keys_out, const in
                                          vals_out, (size_t)n, 0u, end_bit, ctx->stream));
   return 0;
 }
+
+int fcp_sort_keys_u64(fcp_ctx *ctx, const unsigned long long *keys_in,
+                      unsigned long long *keys_out, int n, int end_bit) {
+  size_t bytes = 0;
+  FCP_HIP(ctx, rocprim::radix_sort_keys(nullptr, bytes, keys_in, keys_out, (size_t)n, 0u,
+                                        (unsigned)end_bit, ctx->stream));
+  FCP_HIP(ctx, ctx->stage[3].alloc(bytes + 16));
+  FCP_HIP(ctx, rocprim::radix_sort_keys(ctx->stage[3].p, bytes, keys_in, keys_out, (size_t)n, 0u,
+                                        (unsigned)end_bit, ctx->stream));
+  return 0;
+}

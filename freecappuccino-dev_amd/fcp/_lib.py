@@ -6,7 +6,8 @@ import ctypes as C
 import os
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "lib", "libfcp.so")
+# FCP_LIB selects another build of the same library (e.g. an instrumented one).
+LIB_PATH = os.environ.get("FCP_LIB") or os.path.join(PKG_DIR, "lib", "libfcp.so")
 
 _i = C.c_int
 _d = C.c_double
@@ -41,6 +42,9 @@ SIGNATURES = {
     "fcp_precond_apply": (_i, [_p, _i, _p, _p, _p]),
     "fcp_csrsolve": (_i, [_p, C.c_char_p, _i, _p, _p, C.POINTER(_d), _i, _d, _d, C.c_char_p, _i,
                           _i, C.POINTER(SolveOpts)]),
+    "fcp_rcm": (_i, [_p, _i, _p]),
+    "fcp_create_lsq_grad_matrix": (_i, [_p, _i, _i, _p]),
+    "fcp_grad": (_i, [_p, _i, _p, _p]),
     "fcp_csrsolve_multi": (_i, [_p, C.c_char_p, _i, _i, _p, _i, _p, _p, _i, _d, _d, _i,
                                 C.POINTER(SolveOpts)]),
     "fcp_iccg": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _d, _d, _i]),

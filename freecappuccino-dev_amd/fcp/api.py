@@ -193,6 +193,30 @@ class Context:
         self._check(rc, "fcp_grad_gauss")
         return out
 
+    def rcm(self, pp_iters: int = 4):
+        """GPU reverse Cuthill-McKee: new 0-based number of every cell."""
+        perm = np.zeros(self.mesh.n_cells, np.int32)
+        self._check(self.lib.fcp_rcm(self.h, int(pp_iters), _ptr(perm)[0]), "fcp_rcm")
+        return perm - 1
+
+    def create_lsq_grad_matrix(self, method: int, compat: bool = True, want: bool = False):
+        """create_lsq_grad_matrix for method 1 lsq, 2 lsq_dm, 3 lsq_qr."""
+        out = None
+        if want:
+            out = np.zeros((self.mesh.n_cells, 18 if method == 3 else 9))
+        rc = self.lib.fcp_create_lsq_grad_matrix(self.h, int(method), int(bool(compat)),
+                                                 _ptr(out)[0])
+        self._check(rc, "fcp_create_lsq_grad_matrix")
+        return out
+
+    def grad(self, method: int, phi, out=None):
+        """grad(phi, dPhidxi): method 0 Gauss, 1-3 least squares."""
+        if out is None:
+            out = np.zeros((self.mesh.n_total, 3))
+        rc = self.lib.fcp_grad(self.h, int(method), _ptr(_f64(phi))[0], _ptr(out)[0])
+        self._check(rc, "fcp_grad")
+        return out
+
     def update_boundary(self, phi):
         self._check(self.lib.fcp_update_boundary(self.h, _ptr(phi)[0]), "fcp_update_boundary")
         return phi

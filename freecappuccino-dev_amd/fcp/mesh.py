@@ -329,3 +329,160 @@ def read_foam(case_dir: str) -> Mesh:
              nfaces=np.array(nfs, np.int32), start_face=np.array(sfs, np.int32))
     m.check()
     return m
+
+
+# Kuhn split of a cube into 6 tetrahedra along its (0,0,0)-(1,1,1) diagonal:
+# tet p walks from corner 000 to 111 adding the unit steps of axis order
+# _KUHN[p]; every cube uses the same diagonal, so the split is conforming.
+_KUHN = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+
+
+def permute_cells(mesh: Mesh, new_of_old: np.ndarray) -> Mesh:
+    """Renumber cells (cell c becomes new_of_old[c]) and restore the
+    upper-triangular face order: inner faces get owner < neighbour (face
+    reversed when the pair swaps) sorted by (owner, neighbour); boundary
+    faces stay in their patch, sorted by owner.  Points are unchanged."""
+    new = np.asarray(new_of_old, dtype=np.int64)
+    Fi = mesh.n_inner
+    o = new[mesh.owner[:Fi]]
+    nb = new[mesh.neighbour]
+    swap = o > nb
+    lo = np.where(swap, nb, o)
+    hi = np.where(swap, o, nb)
+    nodes = mesh.node[:Fi].copy()
+    if swap.any():
+        nn = mesh.nnodes[:Fi]
+        for k in np.unique(nn[swap]):
+            sel = swap & (nn == k)
+            nodes[sel, :k] = nodes[sel, :k][:, ::-1]
+    order = np.lexsort((hi, lo))
+    inner_nodes = nodes[order]
+    inner_nn = mesh.nnodes[:Fi][order]
+    own = [lo[order]]
+    bnodes, bnn = [inner_nodes], [inner_nn]
+    for ib in range(mesh.n_patches):
+        s = int(mesh.start_face[ib])
+        e = s + int(mesh.nfaces[ib])
+        bo = new[mesh.owner[s:e]]
+        ordb = np.argsort(bo, kind="stable")
+        own.append(bo[ordb])
+        bnodes.append(mesh.node[s:e][ordb])
+        bnn.append(mesh.nnodes[s:e][ordb])
+    m = Mesh(points=mesh.points, nnodes=np.concatenate(bnn).astype(np.int32),
+             node=np.concatenate(bnodes).astype(np.int32),
+             owner=np.concatenate(own).astype(np.int32), neighbour=hi[order].astype(np.int32),
+             n_cells=mesh.n_cells, names=list(mesh.names), types=list(mesh.types),
+             nfaces=mesh.nfaces.copy(), start_face=mesh.start_face.copy())
+    m.check()
+    return m
+
+
+def fisher_yates(n: int, seed: int, stream: int = 200) -> np.ndarray:
+    """Permutation of range(n) by Fisher-Yates with SplitMix64 draws:
+    for i = n-1 .. 1, j = floor(u_i (i+1)), swap(a[i], a[j])."""
+    u = uniform01(seed, stream, np.arange(n, dtype=np.uint64))
+    j = np.minimum(np.floor(u * (np.arange(n) + 1)).astype(np.int64), np.arange(n))
+    a = list(range(n))
+    jl = j.tolist()
+    for i in range(n - 1, 0, -1):
+        k = jl[i]
+        a[i], a[k] = a[k], a[i]
+    return np.array(a, dtype=np.int64)
+
+
+def tet_mesh(N: int, jitter: float = 0.0, seed: int = 3, perm_seed=None,
+             types=("wall",) * 6) -> Mesh:
+    """Unit cube, N^3 cubes each split into 6 Kuhn tetrahedra (cell id
+    6*cube + p before any permutation, cube = i + N*j + N*N*k), interior
+    nodes jittered by +-jitter*h (seed), then -- with perm_seed -- the cell
+    order permuted by ``fisher_yates(n, perm_seed)`` (cell c becomes perm[c]).
+    Patches x-, x+, y-, y+, z-, z+ with 2 N^2 triangles each."""
+    N = int(N)
+    base = hex_mesh(1, types=types)  # for patch names only
+    M = N + 1
+    h = 1.0 / N
+    ii, jj, kk = np.meshgrid(np.arange(M), np.arange(M), np.arange(M), indexing="ij")
+    ii = ii.transpose(2, 1, 0).reshape(-1)
+    jj = jj.transpose(2, 1, 0).reshape(-1)
+    kk = kk.transpose(2, 1, 0).reshape(-1)
+    lat = np.stack([ii, jj, kk], axis=1).astype(np.int64)
+    pts = (lat / N).astype(np.float64)
+    if jitter:
+        nid = np.arange(M ** 3, dtype=np.uint64)
+        interior = (ii > 0) & (ii < N) & (jj > 0) & (jj < N) & (kk > 0) & (kk < N)
+        for c in range(3):
+            u = uniform01(seed, 100 + c, nid)
+            off = (2.0 * u - 1.0) * (jitter * h)
+            pts[:, c] = np.where(interior, pts[:, c] + off, pts[:, c])
+    ci, cj, ck = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    ci = ci.transpose(2, 1, 0).reshape(-1).astype(np.int64)
+    cj = cj.transpose(2, 1, 0).reshape(-1).astype(np.int64)
+    ck = ck.transpose(2, 1, 0).reshape(-1).astype(np.int64)
+    nc = N ** 3
+    tets = np.empty((nc, 6, 4), np.int64)
+    for p, ax in enumerate(_KUHN):
+        off = np.zeros(3, np.int64)
+        verts = []
+        for s in range(4):
+            verts.append((ci + off[0]) + M * (cj + off[1]) + M * M * (ck + off[2]))
+            if s < 3:
+                off[ax[s]] += 1
+        tets[:, p, :] = np.stack(verts, axis=1)
+    tets = tets.reshape(-1, 4)
+    ncell = 6 * nc
+    # the four faces of every tet, oriented away from the opposite vertex
+    fa = np.empty((ncell, 4, 3), np.int64)
+    opp = np.empty((ncell, 4), np.int64)
+    for m in range(4):
+        others = [q for q in range(4) if q != m]
+        fa[:, m, :] = tets[:, others]
+        opp[:, m] = tets[:, m]
+    fa = fa.reshape(-1, 3)
+    opp = opp.reshape(-1)
+    fcell = np.repeat(np.arange(ncell, dtype=np.int64), 4)
+    a, b, c = lat[fa[:, 0]], lat[fa[:, 1]], lat[fa[:, 2]]
+    nrm = np.cross(b - a, c - a)
+    flip = np.einsum("ij,ij->i", nrm, lat[opp] - a) > 0
+    fa[flip, 1], fa[flip, 2] = fa[flip, 2], fa[flip, 1].copy()
+    del a, b, c, nrm, flip, opp
+    key = np.sort(fa, axis=1)
+    order = np.lexsort((key[:, 2], key[:, 1], key[:, 0]))
+    ks = key[order]
+    same = np.all(ks[1:] == ks[:-1], axis=1)
+    first = np.nonzero(same)[0]          # pair (order[first], order[first+1])
+    paired = np.zeros(order.size, bool)
+    paired[first] = True
+    paired[first + 1] = True
+    r1, r2 = order[first], order[first + 1]
+    c1, c2 = fcell[r1], fcell[r2]
+    own_rec = np.where(c1 < c2, r1, r2)
+    own = np.minimum(c1, c2)
+    nbr = np.maximum(c1, c2)
+    io = np.lexsort((nbr, own))
+    inner_nodes = fa[own_rec[io]]
+    own, nbr = own[io], nbr[io]
+    brec = order[~paired]
+    bl = lat[fa[brec]]
+    patch = np.full(brec.size, -1, np.int64)
+    for p in range(6):
+        axis, hi = p // 2, p % 2
+        on = np.all(bl[:, :, axis] == (N if hi else 0), axis=1)
+        patch[on] = p
+    assert (patch >= 0).all()
+    bo = fcell[brec]
+    ob = np.lexsort((bo, patch))
+    bnodes = fa[brec[ob]]
+    bown = bo[ob]
+    nb_faces = np.bincount(patch, minlength=6).astype(np.int32)
+    start = own.size + np.concatenate([[0], np.cumsum(nb_faces)[:-1]]).astype(np.int32)
+    allq = np.concatenate([inner_nodes, bnodes], axis=0)
+    node = np.full((allq.shape[0], NOMAX), -1, dtype=np.int32)
+    node[:, :3] = allq
+    m = Mesh(points=pts, nnodes=np.full(allq.shape[0], 3, np.int32), node=node,
+             owner=np.concatenate([own, bown]).astype(np.int32), neighbour=nbr.astype(np.int32),
+             n_cells=ncell, names=list(base.names), types=list(types), nfaces=nb_faces,
+             start_face=start.astype(np.int32))
+    m.check()
+    if perm_seed is not None:
+        m = permute_cells(m, fisher_yates(ncell, perm_seed))
+    return m

@@ -118,7 +118,7 @@ def convdiff_flux(mesh, geo, csr, ctx):
     uf = fourier_velocity(geo["xf"], geo["yf"], geo["zf"])
     flux = uf[:, 0] * geo["arx"] + uf[:, 1] * geo["ary"] + uf[:, 2] * geo["arz"]
     patch = mesh.patch_of_bface()
-    outlet = np.asarray(mesh.bctype_codes()[patch] == 4)
+    outlet = np.asarray(mesh.bctype_codes[patch] == 4)
     flux[Fi:][~outlet] = 0.0
     own, nb = mesh.owner, mesh.neighbour
     div = np.bincount(own, weights=flux, minlength=n) - np.bincount(nb, weights=flux[:Fi],
@@ -146,7 +146,7 @@ def convdiff_inputs(mesh, geo, flux):
     three transported components (see the block comment above)."""
     n, Fi, F = mesh.n_cells, mesh.n_inner, mesh.n_faces
     patch = mesh.patch_of_bface()
-    outlet = np.asarray(mesh.bctype_codes()[patch] == 4)
+    outlet = np.asarray(mesh.bctype_codes[patch] == 4)
     D = np.empty(F)
     D[:Fi] = geo["Df"]
     D[Fi:] = geo["srdb"]
@@ -163,3 +163,44 @@ def convdiff_inputs(mesh, geo, flux):
         phi[r, n:] = np.where(outlet, uc[own_b, r], 0.0)
         src[r] = uniform(CONVDIFF_SEED, 50 + r, n, -1.0, 1.0)
     return dict(gamma=gamma, bc_kind=bc_kind, phi=phi, source=src, urf=CONVDIFF_URF)
+
+
+# ---------------------------------------------------------------------------
+# Unstructured tetrahedra (96^3 cubes x 6 Kuhn tets, cells permuted)
+# ---------------------------------------------------------------------------
+# Mesh: ``tet_mesh(N, jitter=0.15, seed=3, perm_seed=3)`` -- interior nodes
+# jittered by +-0.15h, cell order scrambled by Fisher-Yates (seed 3); the GPU
+# then renumbers it by reverse Cuthill-McKee.  Field for the gradients:
+# f = sin(2 pi x) cos(2 pi y) exp(z) + 1e-3 U[-1,1] at cell centres and
+# boundary-face centres (seed 3, streams 300/301).  Poisson: face diffusivity
+# 10^U(-2,2) (stream 302), zero-mean source U[-1,1] (stream 303), Dirichlet 0
+# on x-, zero normal gradient elsewhere.
+def tet_mesh_workload(N: int = 96):
+    from .mesh import tet_mesh
+    return tet_mesh(N, jitter=0.15, seed=3, perm_seed=3,
+                    types=("wall", "symmetry", "symmetry", "symmetry", "symmetry", "symmetry"))
+
+
+def analytic_field(x, y, z):
+    return np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * np.exp(z)
+
+
+def analytic_grad(x, y, z):
+    e = np.exp(z)
+    return np.stack([2 * np.pi * np.cos(2 * np.pi * x) * np.cos(2 * np.pi * y) * e,
+                     -2 * np.pi * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y) * e,
+                     np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y) * e], axis=1)
+
+
+def tet_inputs(mesh, geo):
+    n, Fi, F = mesh.n_cells, mesh.n_inner, mesh.n_faces
+    f = np.empty(mesh.n_total)
+    f[:n] = analytic_field(geo["xc"], geo["yc"], geo["zc"]) + 1e-3 * uniform(3, 300, n, -1.0, 1.0)
+    f[n:] = (analytic_field(geo["xf"][Fi:], geo["yf"][Fi:], geo["zf"][Fi:]) +
+             1e-3 * uniform(3, 301, F - Fi, -1.0, 1.0))
+    gamma = 10.0 ** uniform(3, 302, F, -2.0, 2.0)
+    s = uniform(3, 303, n, -1.0, 1.0)
+    s = s - s.mean()
+    bc_kind = np.where(mesh.patch_of_bface() == 0, 0, 1).astype(np.uint8)
+    return dict(f=f, gamma=gamma, mu_f=-gamma, source=s, bc_kind=bc_kind,
+                phi_bc=np.zeros(mesh.n_total))

@@ -105,6 +105,28 @@ module fcp_capi
       real(c_double) :: gamma_f(*), grad(*), su(*)
     end function
 
+    integer(c_int) function fcp_rcm(ctx, pp_iters, perm) bind(C, name='fcp_rcm')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: pp_iters
+      integer(c_int) :: perm(*)
+    end function
+
+    integer(c_int) function fcp_create_lsq_grad_matrix(ctx, method, compat, dmat) &
+        bind(C, name='fcp_create_lsq_grad_matrix')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: method, compat
+      real(c_double) :: dmat(*)
+    end function
+
+    integer(c_int) function fcp_grad(ctx, method, phi, dphidxi) bind(C, name='fcp_grad')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: method
+      real(c_double) :: phi(*), dphidxi(*)
+    end function
+
     integer(c_int) function fcp_grad_gauss(ctx, u, dudxi) bind(C, name='fcp_grad_gauss')
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

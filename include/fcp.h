@@ -169,6 +169,34 @@ int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt,
 /* grad_gauss (gradients.f90:1607-1715). dudxi[3*numTotal], boundary rows 0. */
 int fcp_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi);
 
+/* Reverse Cuthill-McKee renumbering of the cell graph of the resident mesh
+ * (adjacency of the CSR pattern), computed on the GPU; no counterpart in the
+ * reference.  Each connected component starts from its minimum-(degree, id)
+ * cell refined by at most pp_iters George-Liu pseudo-peripheral steps;
+ * Cuthill-McKee appends unvisited neighbours in (degree, id) order; the
+ * order is reversed.  perm[numCells] receives the new 1-based number of
+ * every cell (bit-identical to the CPU rule of the same definition). */
+int fcp_rcm(fcp_ctx *ctx, int pp_iters, int *perm);
+
+/* Gradient methods of grad_scalar_field (gradients.f90:106-163). */
+#define FCP_GRAD_GAUSS 0
+#define FCP_GRAD_LSQ 1     /* lstsq    : grad_lsq    (:782-893)   */
+#define FCP_GRAD_LSQ_DM 2  /* lstsq_dm : grad_lsq_dm (:1334-1486) */
+#define FCP_GRAD_LSQ_QR 3  /* lstsq_qr : grad_lsq_qr (:1063-1152) */
+
+/* create_lsq_grad_matrix (gradients.f90:72-101): builds and keeps the
+ * per-cell least-squares matrices of `method` (Dmat(9,numCells) for lsq and
+ * lsq_dm, D(3,6,numCells) for lsq_qr, which needs cells of <= 6 faces).
+ * compat = 1 reproduces the reference arithmetic including its defects
+ * (y-row cofactors at :887/:1480, boundary weight xf(i) at :1459); 0 computes
+ * the corrected gradient.  dmat (may be NULL) receives the matrices. */
+int fcp_create_lsq_grad_matrix(fcp_ctx *ctx, int method, int compat, double *dmat);
+
+/* grad(phi, dPhidxi) for a scalar field (gradients.f90:106-163, no limiter):
+ * FCP_GRAD_GAUSS, or the least-squares method whose matrices were created.
+ * dPhidxi[3*numTotal], boundary rows 0. */
+int fcp_grad(fcp_ctx *ctx, int method, const double *phi, double *dphidxi);
+
 /* updateBoundary (updateBoundary.f90:1-74): zero-gradient copy for
  * outlet/symmetry/pressure/empty patches. */
 int fcp_update_boundary(fcp_ctx *ctx, double *phi);

@@ -939,3 +939,111 @@ void orc_nonorth_correction(const orc_mesh *m, const orc_geom *g, const double *
     su[q] = su[q] - fdfie;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Reverse Cuthill-McKee (no reference counterpart; specification in the    */
+/* header)                                                                  */
+/* ------------------------------------------------------------------------ */
+static int rcm_less(const int *deg, int a, int b) {
+  return deg[a] < deg[b] || (deg[a] == deg[b] && a < b);
+}
+
+/* BFS over vertices with done[v] == 0 from r; returns the eccentricity and
+ * the minimum (degree, id) vertex of the last level in *last_min. */
+static int rcm_bfs(const orc_csr *c, const int *deg, const char *done, int r, int *dist,
+                   int *queue, int *last_min) {
+  const int n = c->n;
+  for (int i = 0; i < n; ++i) dist[i] = -1;
+  int head = 0, tail = 0;
+  queue[tail++] = r;
+  dist[r] = 0;
+  int ecc = 0;
+  while (head < tail) {
+    const int v = queue[head++];
+    for (int k = c->ia[v]; k < c->ia[v + 1]; ++k) {
+      const int u = c->ja[k];
+      if (u == v || done[u] || dist[u] >= 0) continue;
+      dist[u] = dist[v] + 1;
+      if (dist[u] > ecc) ecc = dist[u];
+      queue[tail++] = u;
+    }
+  }
+  int best = -1;
+  for (int q = 0; q < tail; ++q) {
+    const int v = queue[q];
+    if (dist[v] == ecc && (best < 0 || rcm_less(deg, v, best))) best = v;
+  }
+  *last_min = best;
+  return ecc;
+}
+
+static void rcm_sort_by_degree(const int *deg, int *v, int m) {
+  for (int i = 1; i < m; ++i) { /* insertion sort: lists are short */
+    const int x = v[i];
+    int j = i - 1;
+    while (j >= 0 && rcm_less(deg, x, v[j])) {
+      v[j + 1] = v[j];
+      --j;
+    }
+    v[j + 1] = x;
+  }
+}
+
+int orc_rcm(const orc_csr *c, int pp_iters, int *new_of_old) {
+  const int n = c->n;
+  int *deg = (int *)malloc(sizeof(int) * (size_t)n);
+  int *dist = (int *)malloc(sizeof(int) * (size_t)n);
+  int *queue = (int *)malloc(sizeof(int) * (size_t)n);
+  int *order = (int *)malloc(sizeof(int) * (size_t)n);
+  char *done = (char *)calloc((size_t)n, 1);
+  int *kids = (int *)malloc(sizeof(int) * (size_t)(n > 64 ? n : 64));
+  if (!deg || !dist || !queue || !order || !done || !kids) return -1;
+  for (int i = 0; i < n; ++i) {
+    int d = 0;
+    for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) d += (c->ja[k] != i);
+    deg[i] = d;
+  }
+  int pos = 0;
+  while (pos < n) {
+    int s = -1;
+    for (int i = 0; i < n; ++i)
+      if (!done[i] && (s < 0 || rcm_less(deg, i, s))) s = i;
+    int r = s, last;
+    int ecc = rcm_bfs(c, deg, done, r, dist, queue, &last);
+    for (int it = 0; it < pp_iters; ++it) {
+      int last2;
+      const int x = last;
+      const int ecc2 = rcm_bfs(c, deg, done, x, dist, queue, &last2);
+      if (ecc2 > ecc) {
+        r = x;
+        ecc = ecc2;
+        last = last2;
+      } else {
+        break;
+      }
+    }
+    int head = pos;
+    order[pos++] = r;
+    done[r] = 1;
+    while (head < pos) {
+      const int v = order[head++];
+      int m = 0;
+      for (int k = c->ia[v]; k < c->ia[v + 1]; ++k) {
+        const int u = c->ja[k];
+        if (u == v || done[u]) continue;
+        done[u] = 1;
+        kids[m++] = u;
+      }
+      rcm_sort_by_degree(deg, kids, m);
+      for (int q = 0; q < m; ++q) order[pos++] = kids[q];
+    }
+  }
+  for (int k = 0; k < n; ++k) new_of_old[order[k]] = n - 1 - k;
+  free(deg);
+  free(dist);
+  free(queue);
+  free(order);
+  free(done);
+  free(kids);
+  return 0;
+}

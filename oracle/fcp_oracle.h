@@ -131,6 +131,19 @@ void orc_spmv(const orc_csr *c, const double *a, const double *x, double *y);
  * strict-upper DAG processed from the last row.  Returns number of levels. */
 int orc_levels(const orc_csr *c, int upper, int *lev);
 
+/* Reverse Cuthill-McKee ordering of the matrix graph (rows adjacent through
+ * off-diagonal entries; degree = off-diagonal count).  The reference has no
+ * renumbering; this is the specification the GPU implementation follows:
+ *  - each connected component starts from the unvisited vertex of minimum
+ *    (degree, id), refined by at most pp_iters George-Liu steps: BFS from
+ *    the current root r, take x = min (degree, id) of the last BFS level,
+ *    and move to x when its eccentricity is larger than r's, else stop;
+ *  - Cuthill-McKee from the root: a FIFO queue; every dequeued vertex
+ *    appends its unvisited neighbours in (degree, id) order;
+ *  - the final order is reversed: new_of_old[order[k]] = n-1-k.
+ * Returns 0. */
+int orc_rcm(const orc_csr *c, int pp_iters, int *new_of_old);
+
 /* Convection-diffusion assembly as calculate_temperature composes it
  * (temperature.f90:75-366 with facefluxsc scalar_fluxes.f90:32-337 and
  * face_value interpolation.f90:28-157,318-356), steady, no ddt.

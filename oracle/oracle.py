@@ -191,6 +191,13 @@ class Oracle:
         nlev = lib().orc_levels(C.byref(self.cc), int(upper), _p(lev))
         return lev, nlev
 
+    def rcm(self, pp_iters: int = 4):
+        """Reverse Cuthill-McKee permutation (new id of every cell)."""
+        out = np.zeros(self.mesh.n_cells, np.int32)
+        rc = lib().orc_rcm(C.byref(self.cc), int(pp_iters), _p(out))
+        assert rc == 0, rc
+        return out
+
     def dic_diag(self, a):
         d = np.zeros(self.mesh.n_cells)
         lib().orc_dic_diag(C.byref(self.cc), _p(np.ascontiguousarray(a, np.float64)), _p(d))

@@ -62,7 +62,7 @@ def test_convdiff_flux_divergence_free(api):
         mesh.neighbour, weights=f0[:Fi], minlength=n)
     assert np.abs(div).max() < 1e-9 * np.abs(div0).max()
     # walls carry no flux; Peclet numbers span the stated range
-    bcode = mesh.bctype_codes()[mesh.patch_of_bface()]
+    bcode = mesh.bctype_codes[mesh.patch_of_bface()]
     assert np.all(flux[Fi:][bcode == 1] == 0.0)
     inp = workloads.convdiff_inputs(mesh, geo, flux)
     pe = np.abs(flux[:Fi]) / (inp["gamma"][:Fi] * geo["Df"])

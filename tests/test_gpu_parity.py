@@ -355,3 +355,45 @@ def test_convdiff_urf_ddt_vs_oracle(api):
     assert np.array_equal(a[off], a_o[off])
     assert np.allclose(sp, sp_o + sp0, rtol=1e-14)
     ctx.close()
+
+
+def test_lsq_gradients_vs_reference_and_oracle(api, golden_dir):
+    """Least-squares gradients: the reference's own lsq / lsq_dm outputs
+    (compat) bit for bit, every variant and mode against the oracle, and
+    exactness for a linear field in corrected mode."""
+    print("[case] lsq gradients hex5j + 12^3 jittered", flush=True)
+    mesh = MESHES["hex5j"](golden_dir)
+    g = np.load(os.path.join(golden_dir, "hex5j.npz"))
+    ctx, geo, csr = _setup(api, mesh)
+    for method, key in ((1, "grad_lsq"), (2, "grad_lsq_dm")):
+        ctx.create_lsq_grad_matrix(method, compat=True)
+        assert np.array_equal(ctx.grad(method, g["in_u"]), g[key]), key
+    ctx.close()
+    mesh = hex_mesh(12, jitter=0.2, seed=5,
+                    types=("wall", "symmetry", "outlet", "wall", "inlet", "pressure"))
+    ctx, geo, csr = _setup(api, mesh)
+    o = Oracle(mesh)
+    n, nt = mesh.n_cells, mesh.n_total
+    phi = np.sin(np.arange(nt) * 0.7) + 0.1 * np.arange(nt) / nt
+    for method in (1, 2, 3):
+        for compat in (True, False):
+            dm = ctx.create_lsq_grad_matrix(method, compat=compat, want=True)
+            dm_o = o.lsq_matrix(method - 1, int(compat)).reshape(n, -1)
+            gr = ctx.grad(method, phi)
+            gr_o = o.grad_lsq(method - 1, phi, compat=int(compat), dmat=dm_o.reshape(-1))
+            if method < 3:
+                assert np.array_equal(dm, dm_o) and np.array_equal(gr, gr_o), (method, compat)
+            else:
+                # MGS uses sqrt; device and host square roots may differ in the last bit
+                assert np.allclose(dm, dm_o, rtol=1e-13, atol=1e-13 * np.abs(dm_o).max())
+                assert np.allclose(gr, gr_o, rtol=1e-12, atol=1e-12 * np.abs(gr_o).max())
+    # linear field: corrected least squares is exact on any mesh
+    xc = np.concatenate([geo["xc"], geo["xf"][mesh.n_inner:]])
+    yc = np.concatenate([geo["yc"], geo["yf"][mesh.n_inner:]])
+    zc = np.concatenate([geo["zc"], geo["zf"][mesh.n_inner:]])
+    lin = 1.5 * xc - 2.0 * yc + 0.25 * zc + 3.0
+    for method in (1, 2, 3):
+        ctx.create_lsq_grad_matrix(method, compat=False)
+        gr = ctx.grad(method, lin)[:n]
+        assert np.allclose(gr, [1.5, -2.0, 0.25], rtol=0, atol=1e-9), method
+    ctx.close()

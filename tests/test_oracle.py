@@ -311,3 +311,93 @@ def test_convdiff_pure_diffusion_equals_laplacian_face():
     a2, su2 = o.laplacian_face(-gam, bk, phi)
     assert np.allclose(a1, a2, rtol=1e-14, atol=1e-17)
     assert np.allclose(su1, su2, rtol=1e-14, atol=1e-17)
+
+
+# ---------------------------------------------------------------------------
+# Reverse Cuthill-McKee (no reference counterpart: pinned against SciPy's
+# Cuthill-McKee rule, a hand-worked pseudo-peripheral case and invariants)
+# ---------------------------------------------------------------------------
+def _graph_csr(n, edges):
+    import scipy.sparse as sp
+    r = np.array([e[0] for e in edges] + list(range(n)))
+    c = np.array([e[1] for e in edges] + list(range(n)))
+    A = sp.coo_matrix((np.ones(r.size), (r, c)), shape=(n, n))
+    A = (A + A.T).tocsr()
+    A.sum_duplicates()
+    A.sort_indices()
+    ia = A.indptr.astype(np.int32)
+    ja = A.indices.astype(np.int32)
+    dg = np.array([ia[i] + np.searchsorted(ja[ia[i]:ia[i + 1]], i) for i in range(n)], np.int32)
+    return A, ia, ja, dg
+
+
+def _orc_rcm(ia, ja, dg, pp):
+    import ctypes as C
+    from oracle import lib
+    cc, keep = csr_struct(ia, ja, dg)
+    out = np.zeros(len(ia) - 1, np.int32)
+    assert lib().orc_rcm(C.byref(cc), int(pp), out.ctypes.data_as(C.c_void_p)) == 0
+    return out
+
+
+def test_rcm_follows_scipy_cuthill_mckee_rule():
+    """Without pseudo-peripheral refinement the order is SciPy's RCM (FIFO
+    queue, children by ascending degree, ties by id); SciPy picks among
+    equal-degree start vertices with an unstable sort, so a difference is
+    allowed only at a component start between vertices of equal degree."""
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    rng = np.random.default_rng(0)
+    exact = 0
+    for _ in range(200):
+        n = int(rng.integers(2, 40))
+        m = int(rng.integers(n, 4 * n))
+        edges = list(zip(rng.integers(0, n, m), rng.integers(0, n, m)))
+        A, ia, ja, dg = _graph_csr(n, edges)
+        new = _orc_rcm(ia, ja, dg, 0)
+        assert np.array_equal(np.sort(new), np.arange(n))
+        order = np.empty(n, int)
+        order[new] = np.arange(n)
+        sc = reverse_cuthill_mckee(A, symmetric_mode=True)
+        if np.array_equal(order, sc):
+            exact += 1
+            continue
+        a, b = order[::-1], sc[::-1]          # Cuthill-McKee orders
+        k = int(np.argmax(a != b))
+        deg = np.diff(ia) - 1
+        assert deg[a[k]] == deg[b[k]]
+        seen = set(a[:k].tolist())
+        for v in (a[k], b[k]):                # both must start a new component
+            assert not (set(ja[ia[v]:ia[v + 1]].tolist()) - {v}) & seen
+    assert exact > 150
+
+
+def test_rcm_pseudo_peripheral_root():
+    """A path 1-2-...-9 with a pendant vertex 0 on vertex 5: the start
+    (degree 1, id 0) has eccentricity 5; the last BFS level holds path ends 1
+    and 9, the smaller id 1 has eccentricity 8 > 5, so Cuthill-McKee starts
+    at 1 and walks the path, taking the pendant right after vertex 5
+    (neighbours of 5 by degree: 0 (deg 1) before 6 (deg 2))."""
+    edges = [(i, i + 1) for i in range(1, 9)] + [(5, 0)]
+    A, ia, ja, dg = _graph_csr(10, edges)
+    new = _orc_rcm(ia, ja, dg, 4)
+    cm = [1, 2, 3, 4, 5, 0, 6, 7, 8, 9]
+    expect = np.empty(10, np.int32)
+    for k, v in enumerate(cm):
+        expect[v] = 9 - k
+    assert np.array_equal(new, expect)
+    # without refinement the root stays at the pendant vertex
+    new0 = _orc_rcm(ia, ja, dg, 0)
+    assert new0[0] == 9
+
+
+def test_rcm_reduces_bandwidth_of_permuted_tets():
+    from fcp.mesh import tet_mesh
+    mesh = tet_mesh(6, jitter=0.15, perm_seed=3)
+    o = Oracle(mesh)
+    new = o.rcm()
+    n = mesh.n_cells
+    assert np.array_equal(np.sort(new), np.arange(n))
+    own, nb = mesh.owner[:mesh.n_inner], mesh.neighbour
+    bw0 = np.abs(own.astype(int) - nb).max()
+    bw1 = np.abs(new[own].astype(int) - new[nb]).max()
+    assert bw1 < bw0 / 5, (bw0, bw1)
