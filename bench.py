@@ -182,6 +182,284 @@ def run_gpu(args):
                 roofline=roof, kernels=kernels, spmv=spmv)
 
 
+def _timed(fn, reps=1):
+    import torch
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps, out
+
+
+def run_convdiff(args):
+    """Momentum-like convection-diffusion, three components sharing one
+    matrix: Gauss gradients + facefluxsc assembly (linear upwind) for u, v, w,
+    then one batched 3-column DILU-BiCGStab solve to ||b-Ax||/||b|| <= 1e-8
+    per column (FP64, and FP32 inner + FP64 refinement); the same columns
+    solved one at a time for comparison."""
+    import torch
+
+    from fcp import workloads
+    from fcp.api import Context, FCP_CRIT_TRUE_L2, FCP_PREC_FP64, FCP_PREC_MIXED
+
+    N = args.size
+    t0 = time.time()
+    mesh = workloads.convdiff_mesh(N)
+    ctx = Context(0)
+    ctx.read_mesh(mesh)
+    geo = ctx.geometry()
+    csr = ctx.create_CSR_matrix()
+    log(f"[convdiff] mesh+geometry+CSR {N}^3 n={mesh.n_cells} ({time.time()-t0:.1f}s)")
+    flux, div = workloads.convdiff_flux(mesh, geo, csr, ctx)
+    inp = workloads.convdiff_inputs(mesh, geo, flux)
+    log(f"[convdiff] projected flux, max|div|={np.abs(div).max():.1e} ({time.time()-t0:.1f}s)")
+    dev = torch.device("cuda:0")
+    T = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
+    n, nt = mesh.n_cells, mesh.n_total
+    F_, G_ = T(flux), T(inp["gamma"])
+    bk = T(inp["bc_kind"], torch.uint8)
+    phi0 = T(inp["phi"])
+    sV = T(inp["source"] * geo["vol"][None, :])
+    phi = torch.empty_like(phi0)
+    grad = torch.empty((nt, 3), dtype=torch.float64, device=dev)
+    rhs = torch.empty((3, n), dtype=torch.float64, device=dev)
+    sp = torch.empty(n, dtype=torch.float64, device=dev)
+    X = torch.zeros((3, n), dtype=torch.float64, device=dev)
+
+    def assemble():
+        phi.copy_(phi0)
+        rhs.copy_(sV)
+        for r in range(3):
+            sp.zero_()
+            ctx.grad_gauss(phi[r], grad)
+            ctx.convdiff(F_, G_, phi[r], grad, 1.0, 1, bk, rhs[r], sp, urf=inp["urf"])
+
+    out = {}
+    for name, prec in (("fp64", FCP_PREC_FP64), ("mixed", FCP_PREC_MIXED)):
+        def step():
+            assemble()
+            X.zero_()
+            rc, info, _ = ctx.csrsolve_multi("bicgstab", X, rhs, itr_max=20000, tol_rel=1e-8,
+                                             precision=prec, criterion=FCP_CRIT_TRUE_L2,
+                                             inner_tol=args.inner_tol, check_every=args.check_every)
+            if rc != 0:
+                raise RuntimeError(f"batched solve failed: {info}")
+            return info
+        step()
+        dt, info = _timed(step, args.steps)
+        out[name] = dict(ms_per_step=dt * 1e3, cells_per_s=n / dt, iterations=info["iterations"],
+                         outer=info["outer_iterations"], worst_rel_res=info["rel_res_l2"])
+        log(f"[convdiff] batched {name}: {dt*1e3:.1f} ms/step  iters={info['iterations']} "
+            f"rel={info['rel_res_l2']:.1e}")
+        # the same three columns one at a time
+        def seq():
+            assemble()
+            it = 0
+            for r in range(3):
+                x = torch.zeros(n, dtype=torch.float64, device=dev)
+                rc, inf1 = ctx.csrsolve("bicgstab", x, rhs[r].clone(), itr_max=20000, tol_rel=1e-8,
+                                        precision=prec, criterion=FCP_CRIT_TRUE_L2,
+                                        inner_tol=args.inner_tol, check_every=args.check_every)
+                it = max(it, inf1["iterations"])
+            return it
+        dts, it = _timed(seq, 1)
+        out[name]["sequential_ms"] = dts * 1e3
+        out[name]["batched_speedup"] = dts / dt
+        log(f"[convdiff] sequential {name}: {dts*1e3:.1f} ms  (batched speedup {dts/dt:.2f}x)")
+    ctx.close()
+    return {"metric": "cells/s to 1e-8 rel-res (3 components, batched DILU-BiCGStab)",
+            "value": out["mixed"]["cells_per_s"], "unit": "cells/s", "n_gpus": 1,
+            "steps": args.steps, "higher_is_better": True, "dtype": "fp32+fp64",
+            "data": "synthetic",
+            "config": {"workload": f"momentum-like convection-diffusion {N}^3 hex, 8 Fourier modes "
+                                   "projected divergence-free, Pe 0.1..100, linear upwind, "
+                                   "3 RHS sharing A", "cells": n},
+            "fp64": out["fp64"], "mixed": out["mixed"],
+            "mixed_over_fp64": out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]}
+
+
+def run_tets(args):
+    """Tetrahedra with scrambled cell order: GPU reverse Cuthill-McKee, the
+    renumbered mesh's gradients (Gauss, lsq, lsq_dm, lsq_qr) of the analytic
+    field against its exact gradient, and an FP64 Poisson solve."""
+    import torch
+
+    from fcp import workloads
+    from fcp.api import Context, FCP_CRIT_TRUE_L2, FCP_PREC_FP64, FCP_PREC_MIXED
+    from fcp.mesh import permute_cells
+
+    N = args.size
+    t0 = time.time()
+    mesh0 = workloads.tet_mesh_workload(N)
+    log(f"[tets] mesh {N}^3x6 n={mesh0.n_cells} ({time.time()-t0:.1f}s)")
+    ctx = Context(0)
+    ctx.read_mesh(mesh0)
+    ctx.compute_geometry()
+    ctx.create_CSR_matrix(want_arrays=False)
+    dt_rcm, perm = _timed(lambda: ctx.rcm(4))
+    own, nb = mesh0.owner[:mesh0.n_inner].astype(np.int64), mesh0.neighbour.astype(np.int64)
+    bw0 = int(np.abs(own - nb).max())
+    bw1 = int(np.abs(perm[own] - perm[nb]).max())
+    log(f"[tets] rcm {dt_rcm*1e3:.1f} ms, bandwidth {bw0} -> {bw1}")
+    ctx.close()
+    mesh = permute_cells(mesh0, perm)
+    del mesh0
+    ctx = Context(0)
+    ctx.read_mesh(mesh)
+    geo = ctx.geometry()
+    ctx.create_CSR_matrix(want_arrays=False)
+    n, nt, Fi = mesh.n_cells, mesh.n_total, mesh.n_inner
+    inp = workloads.tet_inputs(mesh, geo)
+    dev = torch.device("cuda:0")
+    f = torch.from_numpy(inp["f"]).to(dev)
+    gout = torch.empty((nt, 3), dtype=torch.float64, device=dev)
+    exact = workloads.analytic_grad(geo["xc"], geo["yc"], geo["zc"])
+    grads = {}
+    for method, name in ((0, "gauss"), (1, "lsq"), (2, "lsq_dm"), (3, "lsq_qr")):
+        if method:
+            ctx.create_lsq_grad_matrix(method, compat=False)
+        dtg, _ = _timed(lambda: ctx.grad(method, f, gout), 5)
+        g = gout[:n].cpu().numpy()
+        err = float(np.sqrt(np.mean(np.sum((g - exact) ** 2, axis=1))) /
+                    np.sqrt(np.mean(np.sum(exact ** 2, axis=1))))
+        grads[name] = dict(ms=dtg * 1e3, rel_rms_error=err)
+        log(f"[tets] grad {name}: {dtg*1e3:.2f} ms, rel rms error vs analytic {err:.2e}")
+    T = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
+    mu_f, bk, pb = T(inp["mu_f"]), T(inp["bc_kind"], torch.uint8), T(inp["phi_bc"])
+    sV = T(inp["source"] * geo["vol"])
+    su = torch.empty(n, dtype=torch.float64, device=dev)
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+    res = {}
+    for name, prec in (("fp64", FCP_PREC_FP64), ("mixed", FCP_PREC_MIXED)):
+        def step():
+            su.copy_(sV)
+            ctx.laplacian_face(mu_f, bk, pb, su)
+            x.zero_()
+            rc, info = ctx.csrsolve("iccg", x, su, itr_max=50000, tol_rel=1e-8, precision=prec,
+                                    criterion=FCP_CRIT_TRUE_L2, inner_tol=args.inner_tol,
+                                    check_every=args.check_every)
+            return info
+        step()
+        dts, info = _timed(step, args.steps)
+        res[name] = dict(ms_per_step=dts * 1e3, cells_per_s=n / dts, iterations=info["iterations"])
+        log(f"[tets] poisson {name}: {dts*1e3:.1f} ms/step iters={info['iterations']}")
+    ctx.close()
+    return {"metric": "tets: RCM ms, gradient errors, Poisson cells/s to 1e-8",
+            "value": res["mixed"]["cells_per_s"], "unit": "cells/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "fp32+fp64", "data": "synthetic",
+            "config": {"workload": f"{N}^3x6 Kuhn tets jittered +-15%, cells permuted (seed 3), "
+                                   "GPU RCM, Gamma 10^U(-2,2)", "cells": n},
+            "rcm_ms": dt_rcm * 1e3, "bandwidth_before": bw0, "bandwidth_after": bw1,
+            "gradients": grads, "poisson": res}
+
+
+def run_adjoint(args):
+    """Sensitivities of J = sum w phi V on the jittered convection-diffusion
+    problem: primal (2 deferred-correction passes) + adjoint on the GPU, and
+    central finite differences (step 1e-6) along random directions."""
+    import torch
+
+    from fcp import workloads
+    from fcp.adjoint import ConvDiffProblem, ConvDiffSolve
+    from fcp.api import Context
+
+    N = args.size
+    mesh = workloads.convdiff_mesh(N, jitter=0.2)
+    ctx = Context(0)
+    ctx.read_mesh(mesh)
+    geo = ctx.geometry()
+    csr = ctx.create_CSR_matrix()
+    flux, _ = workloads.convdiff_flux(mesh, geo, csr, ctx)
+    inp = workloads.adjoint_inputs(mesh, geo, flux)
+    dev = torch.device("cuda:0")
+    prob = ConvDiffProblem(ctx, mesh, geo, flux, inp["bc_kind"], K=inp["K"])
+    P = {k: torch.from_numpy(np.ascontiguousarray(inp[k])).to(dev) for k in ("gamma", "source", "phib", "w")}
+    vol = torch.from_numpy(geo["vol"]).to(dev)
+    g = P["gamma"].clone().requires_grad_(True)
+    s = P["source"].clone().requires_grad_(True)
+    b = P["phib"].clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    phi = ConvDiffSolve.apply(g, s, b, prob)
+    J = (P["w"] * phi * vol).sum()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t
+    t = time.perf_counter()
+    J.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t
+    log(f"[adjoint] forward {t_fwd*1e3:.0f} ms, backward {t_bwd*1e3:.0f} ms")
+    rng = np.random.default_rng(4)
+    dirichlet = torch.from_numpy(inp["bc_kind"] == 0).to(dev)
+    h = 1e-6
+    fds, ads = [], []
+    for k in range(args.fd_dirs):
+        vg = torch.from_numpy(rng.uniform(-1, 1, g.numel())).to(dev) * P["gamma"]
+        vs = torch.from_numpy(rng.uniform(-1, 1, s.numel())).to(dev)
+        vb = torch.from_numpy(rng.uniform(-1, 1, b.numel())).to(dev) * dirichlet
+        Jp = float((P["w"] * prob.forward(P["gamma"] + h * vg, P["source"] + h * vs,
+                                          P["phib"] + h * vb) * vol).sum())
+        Jm = float((P["w"] * prob.forward(P["gamma"] - h * vg, P["source"] - h * vs,
+                                          P["phib"] - h * vb) * vol).sum())
+        fd = (Jp - Jm) / (2 * h)
+        ad = float((g.grad * vg).sum() + (s.grad * vs).sum() + (b.grad * vb).sum())
+        fds.append(fd)
+        ads.append(ad)
+        log(f"[adjoint] direction {k}: fd {fd:.10e} adjoint {ad:.10e} rel {abs(fd-ad)/abs(ad):.1e}")
+    fds, ads = np.array(fds), np.array(ads)
+    worst = float(np.abs(fds - ads).max() / np.abs(ads).max())
+    worst_dir = float(np.max(np.abs(fds - ads) / np.abs(ads)))
+    ctx.close()
+    return {"metric": "adjoint gradient time and FD agreement", "value": (t_fwd + t_bwd) * 1e3,
+            "unit": "ms (primal + adjoint)", "n_gpus": 1, "higher_is_better": False,
+            "dtype": "fp64", "data": "synthetic",
+            "config": {"workload": f"{N}^3 jittered convection-diffusion, J = sum w phi V, "
+                                   f"K={inp['K']} deferred-correction passes", "cells": mesh.n_cells},
+            "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
+            "fd_directions": args.fd_dirs, "fd_step": h, "max_fd_mismatch_rel_to_max": worst,
+            "worst_per_direction_rel": worst_dir}
+
+
+def run_probe(args):
+    """A short fixed kernel sequence for hardware-counter runs: a calibration
+    copy of known size (256 MiB read + 256 MiB written), then 10 mixed and 10
+    FP64 PCG iterations and 5 SELL SpMVs on the benchmark system."""
+    import torch
+
+    from fcp import workloads
+    from fcp.api import Context, FCP_CRIT_TRUE_L2, FCP_PREC_FP64, FCP_PREC_MIXED
+
+    N = args.size
+    mesh = workloads.poisson_jittered_mesh(N)
+    ctx = Context(0)
+    ctx.read_mesh(mesh)
+    geo = ctx.geometry()
+    ctx.create_CSR_matrix(want_arrays=False)
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    dev = torch.device("cuda:0")
+    su = torch.from_numpy(inp["source"] * geo["vol"]).to(dev)
+    ctx.laplacian_face(torch.from_numpy(inp["mu_f"]).to(dev),
+                       torch.from_numpy(inp["bc_kind"]).to(dev), torch.zeros(mesh.n_total,
+                       dtype=torch.float64, device=dev), su)
+    src = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    x = torch.zeros(mesh.n_cells, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    for prec in (FCP_PREC_MIXED, FCP_PREC_FP64):  # 10 PCG iterations each
+        x.zero_()
+        ctx.csrsolve("iccg", x, su, itr_max=10, tol_abs=0.0, tol_rel=1e-30, precision=prec,
+                     criterion=FCP_CRIT_TRUE_L2, max_outer=1, check_every=10)
+    for _ in range(5):
+        ctx.spmv(x, y)
+    torch.cuda.synchronize()
+    ctx.close()
+    log(f"[probe] done: n={mesh.n_cells} copy=256MiB")
+
+
 def run_cpu_baseline(args, n_gpu_cells):
     """The oracle (serial C restatement of the reference path) on the same
     workload at a bounded size, 1 host core."""
@@ -215,16 +493,29 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=128)
-    ap.add_argument("--cpu-size", type=int, default=64)
+    ap.add_argument("--cpu-size", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=1)
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--check-every", type=int, default=16)
+    ap.add_argument("--probe", action="store_true",
+                    help="run only the fixed kernel sequence for counter collection")
+    ap.add_argument("--workload", default="poisson",
+                    choices=["poisson", "convdiff", "tets", "adjoint"],
+                    help="poisson (default, the headline) or one of the secondary workloads")
+    ap.add_argument("--fd-dirs", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
+    if args.probe:
+        run_probe(args)
+        return
+    if args.workload != "poisson":
+        fn = {"convdiff": run_convdiff, "tets": run_tets, "adjoint": run_adjoint}[args.workload]
+        print(json.dumps(fn(args)))
+        return
     g = run_gpu(args)
     cpu = None if args.no_cpu else run_cpu_baseline(args, g["n"])
     mixed, fp64 = g["results"]["mixed"], g["results"]["fp64"]

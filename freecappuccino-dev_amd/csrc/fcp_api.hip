@@ -251,6 +251,14 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
       if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "sweep kind is 0 or 1");
       ctx->tune_sweep_kind = value;
       return 0;
+    case FCP_TUNE_SWEEP_TILE:
+      if (value != 0 && value != 1 && (value < 64 || value > 32768))
+        return fcp_fail(ctx, FCP_ERR_ARG, "tile rows must be 0, 1 or in [64, 32768]");
+      if (value != ctx->tune_sweep_tile) {
+        ctx->tune_sweep_tile = value;
+        if (ctx->have_csr) return fcp_build_tile_schedules(ctx);
+      }
+      return 0;
     case FCP_TUNE_SWEEP_ROWS:
       if (value != 0 && (value < 64 || value > 32768 || (value & 63)))
         return fcp_fail(ctx, FCP_ERR_ARG, "sweep rows must be 0 or a multiple of 64 in [64, 32768]");
@@ -587,6 +595,94 @@ int fcp_rcm(fcp_ctx *ctx, int pp_iters, int *perm) {
   k_plus_one<<<(ctx->n + 255) / 256, 256, 0, ctx->stream>>>(ctx->n, out.dev);
   FCP_HIP(ctx, hipGetLastError());
   FCP_TRY(arg_back(ctx, out));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_matrix_transpose(fcp_ctx *ctx) {
+  FCP_TRY(need(ctx, true, false, true));
+  (void)hipSetDevice(ctx->device);
+  FCP_TRY(fcp_launch_transpose(ctx));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_convdiff_backward(fcp_ctx *ctx, const double *flmass, const double *dcoef,
+                          const double *phi_old, const double *grad_old, double gam, int scheme,
+                          const unsigned char *bc_kind, const double *lam, const double *phi_new,
+                          double *dcoef_bar, double *phi_old_bar, double *grad_old_bar) {
+  FCP_TRY(need(ctx, true, true, false));
+  if (!flmass || !dcoef || !phi_old || !grad_old || !bc_kind || !lam || !phi_new || !dcoef_bar ||
+      !phi_old_bar || !grad_old_bar)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_convdiff_backward: null argument");
+  if (scheme != 0 && scheme != 1) return fcp_fail(ctx, FCP_ERR_ARG, "scheme must be 0 or 1");
+  (void)hipSetDevice(ctx->device);
+  const size_t n = ctx->n, F = (size_t)ctx->Fi + ctx->Fb, nt = n + ctx->Fb;
+  Arg<double> fm, dc, po, go, la, pn, dcb, pob, gob;
+  Arg<unsigned char> bk;
+  FCP_TRY(arg_in(ctx, flmass, F, fm));
+  FCP_TRY(arg_in(ctx, dcoef, F, dc));
+  FCP_TRY(arg_in(ctx, phi_old, nt, po));
+  FCP_TRY(arg_in(ctx, grad_old, 3 * nt, go));
+  FCP_TRY(arg_in(ctx, bc_kind, (size_t)ctx->Fb, bk));
+  FCP_TRY(arg_in(ctx, lam, n, la));
+  FCP_TRY(arg_in(ctx, phi_new, n, pn));
+  FCP_TRY(arg_in(ctx, (const double *)dcoef_bar, F, dcb));
+  FCP_TRY(arg_in(ctx, (const double *)phi_old_bar, nt, pob));
+  FCP_TRY(arg_in(ctx, (const double *)grad_old_bar, 3 * nt, gob));
+  FCP_TRY(fcp_launch_convdiff_bwd(ctx, fm.dev, dc.dev, po.dev, go.dev, gam, scheme, bk.dev, la.dev,
+                                  pn.dev, dcb.dev, pob.dev, gob.dev));
+  FCP_TRY(arg_back(ctx, dcb));
+  FCP_TRY(arg_back(ctx, pob));
+  FCP_TRY(arg_back(ctx, gob));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_laplacian_face_backward(fcp_ctx *ctx, const double *mu_f, const unsigned char *bc_kind,
+                                const double *phi_bc, const double *gamma_f, const double *grad,
+                                const double *lam, const double *phi, double *mu_f_bar,
+                                double *phi_bc_bar, double *gamma_bar, double *grad_bar) {
+  FCP_TRY(need(ctx, true, true, false));
+  if (!mu_f || !phi_bc || !lam || !phi || !mu_f_bar || !phi_bc_bar)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_laplacian_face_backward: null argument");
+  if (gamma_f && (!grad || !gamma_bar || !grad_bar))
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_laplacian_face_backward: correction outputs missing");
+  (void)hipSetDevice(ctx->device);
+  const size_t n = ctx->n, F = (size_t)ctx->Fi + ctx->Fb, nt = n + ctx->Fb;
+  Arg<double> mf, pb, gf, gr, la, ph, mfb, pbb, gfb, grb;
+  Arg<unsigned char> bk;
+  FCP_TRY(arg_in(ctx, mu_f, F, mf));
+  FCP_TRY(arg_in(ctx, bc_kind, (size_t)ctx->Fb, bk));
+  FCP_TRY(arg_in(ctx, phi_bc, nt, pb));
+  FCP_TRY(arg_in(ctx, gamma_f, F, gf));
+  FCP_TRY(arg_in(ctx, grad, 3 * nt, gr));
+  FCP_TRY(arg_in(ctx, lam, n, la));
+  FCP_TRY(arg_in(ctx, phi, n, ph));
+  FCP_TRY(arg_in(ctx, (const double *)mu_f_bar, F, mfb));
+  FCP_TRY(arg_in(ctx, (const double *)phi_bc_bar, nt, pbb));
+  FCP_TRY(arg_in(ctx, (const double *)gamma_bar, F, gfb));
+  FCP_TRY(arg_in(ctx, (const double *)grad_bar, 3 * nt, grb));
+  FCP_TRY(fcp_launch_laplacian_bwd(ctx, mf.dev, bk.dev, pb.dev, gf.dev, gr.dev, la.dev, ph.dev,
+                                   mfb.dev, pbb.dev, gfb.dev, grb.dev));
+  FCP_TRY(arg_back(ctx, mfb));
+  FCP_TRY(arg_back(ctx, pbb));
+  FCP_TRY(arg_back(ctx, gfb));
+  FCP_TRY(arg_back(ctx, grb));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_grad_gauss_backward(fcp_ctx *ctx, const double *grad_bar, double *phi_bar) {
+  FCP_TRY(need(ctx, true, true, false));
+  if (!grad_bar || !phi_bar) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_grad_gauss_backward: null");
+  (void)hipSetDevice(ctx->device);
+  const size_t nt = (size_t)ctx->n + ctx->Fb;
+  Arg<double> gb, pb;
+  FCP_TRY(arg_in(ctx, grad_bar, 3 * nt, gb));
+  FCP_TRY(arg_in(ctx, (const double *)phi_bar, nt, pb));
+  FCP_TRY(fcp_launch_grad_gauss_bwd(ctx, gb.dev, pb.dev));
+  FCP_TRY(arg_back(ctx, pb));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return 0;
 }

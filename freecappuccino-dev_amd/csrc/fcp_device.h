@@ -168,6 +168,7 @@ inline int *tickets(fcp_ctx *ctx) { return ctx->counters.p; }
 inline int *redctr(fcp_ctx *ctx) { return ctx->counters.p + 2; }
 int check_err(fcp_ctx *ctx);
 int prepare_precond(fcp_ctx *ctx, int kind, bool fp32);
+bool sched_fits(fcp_ctx *ctx, const BlockSchedule &lo, const BlockSchedule &up, size_t row_bytes);
 int pack_block64(fcp_ctx *ctx, BlockSchedule &b);
 int pack_block32(fcp_ctx *ctx, BlockSchedule &b);
 int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z);

@@ -61,12 +61,15 @@ struct Schedule {
 // the block's results in LDS and waits only for rows of earlier blocks.
 struct BlockSchedule {
   int R = 0, nblocks = 0, nsteps = 0, max_steps = 0;
+  bool tiled = false;  // blocks are geometric tiles (else contiguous rows)
   DBuf<int> rows;      // [n] block-major, level-minor, ascending row inside a level
   DBuf<int> step_ptr;  // [nsteps+1] offsets into rows
   DBuf<int> blk_step;  // [nblocks+1] first step of every block
+  DBuf<int> blk_pos;   // [nblocks+1] first position (index in rows) of every block
   // Packed step-major operands for the pipelined sweep, position = index in
-  // `rows`: dependency count, dependency (>= 0: LDS slot inside the block,
-  // < 0: -(row+1) of an earlier block), CSR position of its coefficient.
+  // `rows`: dependency count, dependency (>= 0: position inside the block =
+  // its LDS slot, < 0: -(row+1) of an earlier block), CSR position of its
+  // coefficient.
   int maxd = 0;
   DBuf<int> prc;                // [2*n]  (row, count) pairs
   DBuf<int> pdep;               // [PMAXD*n] dependencies, PMAXD per position
@@ -77,7 +80,6 @@ struct BlockSchedule {
 };
 #define FCP_PMAXD 4  // dependencies per row handled by the pipelined sweep
 
-// Device-resident scalars of one Krylov solve (single RHS).
 // Row-block schedules of a batched (several right-hand sides) sweep, whose
 // block size depends on the bytes per row kept in LDS.
 struct MultiSched {
@@ -85,6 +87,7 @@ struct MultiSched {
   DBuf<char> pin;  // packed sweep inputs, one row of values per position
 };
 
+// Device-resident scalars of one Krylov solve (one column).
 struct SolveScalars {
   double sk, s0, pkapk, alf, bet, res0, resl, factor, resor, rsm;
   double l2, l2b, om, gam, beto, ukreso, vv, vr;
@@ -131,6 +134,8 @@ struct fcp_ctx {
   Schedule lower, upper;
   BlockSchedule blower, bupper;      // FP64-sized blocks
   BlockSchedule blower4, bupper4;    // FP32-sized blocks (twice the rows)
+  BlockSchedule tlower, tupper;      // geometric-tile blocks (when acyclic)
+  bool have_tiles = false;
   DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)
   // SELL-64 copy of the operator for SpMV: rows in slices of 64, each slice
   // stored column-major (entry k of the slice's rows contiguous) and padded
@@ -164,6 +169,7 @@ struct fcp_ctx {
 
   // -------- batched solves (interleaved vectors, NRP values per row) --------
   std::map<int, std::unique_ptr<MultiSched>> msched;  // keyed by rows per block
+  DBuf<char> tpin;  // batched sweep inputs on the tile schedules
   DBuf<double> mv64[9];
   DBuf<float> mv32[8];
 
@@ -175,6 +181,7 @@ struct fcp_ctx {
   int tune_sweep_sleep = 1;
   int tune_sweep_kind = 0;     // 0 row-block (LDS) sweeps, 1 level-task sweeps
   int tune_sweep_rows = 0;     // rows per block of the row-block sweeps (0 = auto)
+  int tune_sweep_tile = 0;     // rows per geometric tile (0 = auto, 1 = no tiles)
   size_t lds_cap = 65536;      // LDS bytes available to one workgroup
 
   // -------- profiling --------
@@ -246,7 +253,11 @@ inline int fcp_block_rows(const fcp_ctx *ctx, int elem = 8) {
 }
 // fcp_mesh.hip: row-block schedule of one triangle (upper = 1) with R rows
 // per block, packed static operands included.
-int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, BlockSchedule &b);
+// tile_rank (host, may be NULL): block rank of every row; blocks are then
+// the tiles in rank order (R = the largest tile) instead of row / R.
+int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, BlockSchedule &b,
+                             const int *tile_rank = nullptr, int ntiles = 0);
+int fcp_build_tile_schedules(fcp_ctx *ctx);
 // fcp_assemble.hip
 int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
                          const unsigned char *bc_kind, const double *phi, double *su);
@@ -258,6 +269,18 @@ int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef,
                         double urf, double *su, double *sp);
 int fcp_launch_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt, const double *fo,
                    const double *foo, double *su, double *sp);
+// fcp_adjoint.hip
+int fcp_launch_transpose(fcp_ctx *ctx);
+int fcp_launch_convdiff_bwd(fcp_ctx *ctx, const double *flmass, const double *dcoef,
+                            const double *phi_old, const double *grad_old, double gam, int scheme,
+                            const unsigned char *bc_kind, const double *lam,
+                            const double *phi_new, double *dcoef_bar, double *phi_old_bar,
+                            double *grad_old_bar);
+int fcp_launch_laplacian_bwd(fcp_ctx *ctx, const double *mu_f, const unsigned char *bc_kind,
+                             const double *phi_bc, const double *gamma, const double *grad,
+                             const double *lam, const double *phi, double *mu_f_bar,
+                             double *phi_bc_bar, double *gamma_bar, double *grad_bar);
+int fcp_launch_grad_gauss_bwd(fcp_ctx *ctx, const double *grad_bar, double *phi_bar);
 // fcp_lsq.hip
 int fcp_launch_lsq_matrix(fcp_ctx *ctx, int method, int compat);
 int fcp_launch_grad_lsq(fcp_ctx *ctx, const double *phi, double *dphi);

@@ -266,13 +266,14 @@ __global__ __launch_bounds__(BLK) void k_bsweep(
 // barrier; a thread handles at most one position per step in the pipelined
 // path and walks any further positions of an oversized step directly.
 // Prefetch loads are issued unconditionally (an unused slot reads position 0
-// and is flagged invalid) so the compiler's memory-counter waits stay exact:
+// or the row's own entry and is flagged invalid) so the compiler's
+// memory-counter waits stay exact:
 // a load inside a divergent branch makes every later wait drain all
 // outstanding prefetches.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct PSlot {
-  int row, cnt, ok;
+  int row, cnt, ok, pos;
   T dv, xin;
   int dep[FCP_PMAXD];
   T av[FCP_PMAXD];
@@ -329,16 +330,19 @@ __device__ __forceinline__ void p_load1(PSlot<T> &S, const PArgs<T> &P, const in
   const int idx = sptr[sc] + (int)threadIdx.x;
   const bool ok = (step < ns) && (idx < sptr[sc + 1]);
   S.ok = ok ? 1 : 0;
+  S.pos = ok ? idx : 0;
   p_fetch<T>(P, ok ? idx : 0, S.row, S.cnt, S.dv, S.xin, S.dep, S.av);
 }
 
+// idle: a row private to this lane, read by slots without a remote
+// dependency so idle lanes spread over memory instead of sharing an address
 template <typename T>
-__device__ __forceinline__ void p_load2(PSlot<T> &S, T *out) {
+__device__ __forceinline__ void p_load2(PSlot<T> &S, T *out, int idle) {
   using B = Bits<T>;
 #pragma unroll
   for (int m = 0; m < FCP_PMAXD; ++m) {
     const bool need = S.ok && m < S.cnt && S.dep[m] < 0;
-    typename B::U *q = reinterpret_cast<typename B::U *>(out + (need ? (-S.dep[m] - 1) : 0));
+    typename B::U *q = reinterpret_cast<typename B::U *>(out + (need ? (-S.dep[m] - 1) : idle));
     S.rv[m] = B::from(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   }
 }
@@ -370,7 +374,7 @@ __device__ __forceinline__ void p_compute(const PSlot<T> &S, const PArgs<T> &P, 
                                           int slp) {
   if (S.ok) {
     const T val = p_row_value<T, MODE>(S.cnt, S.dv, S.xin, S.dep, S.av, S.rv, zl, out, err, slp);
-    zl[S.row - lo] = val;
+    zl[S.pos - lo] = val;
     publish<T>(out + S.row, val);
   }
   if (step >= ns) return;
@@ -381,7 +385,7 @@ __device__ __forceinline__ void p_compute(const PSlot<T> &S, const PArgs<T> &P, 
     T dv, xin, av[FCP_PMAXD];
     p_fetch<T>(P, idx, row, cnt, dv, xin, dep, av);
     const T val = p_row_value<T, MODE>(cnt, dv, xin, dep, av, nullptr, zl, out, err, slp);
-    zl[row - lo] = val;
+    zl[idx - lo] = val;
     publish<T>(out + row, val);
   }
 }
@@ -393,8 +397,9 @@ constexpr int PD2 = 3;  // steps of remote-value prefetch
 
 template <typename T, int MODE>
 __global__ __launch_bounds__(BLK, 1) void k_psweep(PArgs<T> P, const int *__restrict__ step_ptr,
-                                                   const int *__restrict__ blk_step, int nblocks,
-                                                   int R, int sptr_off, T *out,
+                                                   const int *__restrict__ blk_step,
+                                                   const int *__restrict__ blk_pos, int nblocks,
+                                                   int sptr_off, T *out,
                                                    int *__restrict__ ticket, int *__restrict__ err,
                                                    const SolveScalars *__restrict__ s, int slp) {
   extern __shared__ unsigned char smem_raw[];
@@ -407,16 +412,17 @@ __global__ __launch_bounds__(BLK, 1) void k_psweep(PArgs<T> P, const int *__rest
   const int t = sb;
   if (t >= nblocks) return;
   const int b = (MODE == 1) ? nblocks - 1 - t : t;
-  const int lo = b * R;
+  const int lo = blk_pos[b];
   const int st0 = blk_step[b];
   const int ns = blk_step[b + 1] - st0;
   for (int i = threadIdx.x; i <= ns; i += BLK) sptr[i] = step_ptr[st0 + i];
   __syncthreads();
+  const int idle = (int)(((unsigned)b * BLK + threadIdx.x) % (unsigned)P.n);
   PSlot<T> S[PD];
 #pragma unroll
   for (int j = 0; j < PD; ++j) p_load1<T>(S[j], P, sptr, j, ns);
 #pragma unroll
-  for (int j = 0; j < PD2; ++j) p_load2<T>(S[j], out);
+  for (int j = 0; j < PD2; ++j) p_load2<T>(S[j], out, idle);
   // Steps past ns (at most PD-1 in the last group) are empty: their slots
   // are invalid, so every group runs the same instruction stream.
   for (int base = 0; base < ns; base += PD) {
@@ -425,7 +431,7 @@ __global__ __launch_bounds__(BLK, 1) void k_psweep(PArgs<T> P, const int *__rest
       const int step = base + j;
       p_compute<T, MODE>(S[j], P, sptr, step, ns, lo, zl, out, err, slp);
       __syncthreads();
-      p_load2<T>(S[(j + PD2) % PD], out);
+      p_load2<T>(S[(j + PD2) % PD], out, idle);
       p_load1<T>(S[j], P, sptr, step + PD, ns);
     }
   }
@@ -969,8 +975,30 @@ struct PBufs<float> {
   static uint64_t &ver(BlockSchedule &b) { return b.pver32; }
 };
 
+}  // namespace
+
+namespace fcpk {
+// Blocks of one sweep pair fit the pipelined kernel: at most FCP_PMAXD
+// dependencies per row, block values plus step table within the LDS limit.
+bool sched_fits(fcp_ctx *ctx, const BlockSchedule &lo, const BlockSchedule &up, size_t row_bytes) {
+  for (const BlockSchedule *b : {&lo, &up}) {
+    if (b->nblocks == 0 || b->maxd > FCP_PMAXD) return false;
+    const size_t lds = (((size_t)b->R * row_bytes + 15) & ~(size_t)15) +
+                       (size_t)(b->max_steps + 2) * sizeof(int) + 256;
+    if (lds > ctx->lds_cap) return false;
+  }
+  return true;
+}
+}  // namespace fcpk
+
+namespace {
+
+// Geometric tiles when they were built and fit, else contiguous row blocks
+// sized for precision T.
 template <typename T>
 BlockSchedule &psched(fcp_ctx *ctx, bool upper) {
+  if (ctx->have_tiles && sched_fits(ctx, ctx->tlower, ctx->tupper, sizeof(T)))
+    return upper ? ctx->tupper : ctx->tlower;
   if (sizeof(T) == 4) return upper ? ctx->bupper4 : ctx->blower4;
   return upper ? ctx->bupper : ctx->blower;
 }
@@ -980,13 +1008,7 @@ BlockSchedule &psched(fcp_ctx *ctx, bool upper) {
 template <typename T>
 bool use_psweep(fcp_ctx *ctx) {
   if (ctx->tune_sweep_kind != 0) return false;
-  for (int u = 0; u < 2; ++u) {
-    BlockSchedule &b = psched<T>(ctx, u);
-    if (b.nblocks == 0 || b.maxd > FCP_PMAXD) return false;
-    const size_t lds = (size_t)b.R * sizeof(T) + (size_t)(b.max_steps + 2) * sizeof(int) + 256;
-    if (lds > ctx->lds_cap) return false;
-  }
-  return true;
+  return sched_fits(ctx, psched<T>(ctx, false), psched<T>(ctx, true), sizeof(T));
 }
 
 template <typename T, int MODE>
@@ -997,8 +1019,9 @@ int launch_psweep(fcp_ctx *ctx, BlockSchedule &b, const T *in, T *out, int *tick
   PArgs<T> P{b.prc.p, b.pdep.p, PBufs<T>::pa(b).p, PBufs<T>::pdx(b).p, n};
   const int sptr_off = (int)(((size_t)b.R * sizeof(T) + 15) & ~(size_t)15);
   const size_t lds = (size_t)sptr_off + (size_t)(b.max_steps + 2) * sizeof(int);
-  k_psweep<T, MODE><<<b.nblocks, BLK, lds, ctx->stream>>>(P, b.step_ptr.p, b.blk_step.p, b.nblocks,
-                                                          b.R, sptr_off, out, ticket,
+  k_psweep<T, MODE><<<b.nblocks, BLK, lds, ctx->stream>>>(P, b.step_ptr.p, b.blk_step.p,
+                                                          b.blk_pos.p, b.nblocks, sptr_off, out,
+                                                          ticket,
                                                           ctx->errflag.p, s, ctx->tune_sweep_sleep);
   FCP_HIP(ctx, hipGetLastError());
   return 0;

@@ -6,6 +6,11 @@
 // faces in ascending face id from the cell->face adjacency, performing the
 // same floating-point operations in the same order, so results are bit-equal
 // without atomics on floating-point data.
+#include <algorithm>
+#include <cmath>
+#include <tuple>
+#include <vector>
+
 #include "fcp_internal.h"
 
 namespace {
@@ -451,24 +456,32 @@ static int build_one_schedule(fcp_ctx *ctx, int upper, Schedule &s) {
 }
 
 namespace {
-__global__ void k_block_of(const int *__restrict__ order, int n, int R, int *__restrict__ key) {
+__global__ void k_block_key(const int *__restrict__ order, int n, int R,
+                            const int *__restrict__ rank, int *__restrict__ key) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) key[i] = order[i] / R;
+  if (i < n) key[i] = rank ? rank[order[i]] : order[i] / R;
+}
+
+__global__ void k_pos_of(const int *__restrict__ rows, int n, int *__restrict__ pos) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pos[rows[i]] = i;
 }
 
 // Static part of the packed operands: dependency list of every position in
 // processing order (strict-lower entries for the forward sweep, strict-upper
-// for the backward sweep), in CSR order.
-__global__ void k_pack_static(const int *__restrict__ rows, int n, int R, int upper,
-                              const int *__restrict__ ia, const int *__restrict__ diag,
-                              const int *__restrict__ ja, int *__restrict__ prc,
-                              int *__restrict__ pdep, int *__restrict__ psrc,
-                              int *__restrict__ maxd) {
+// for the backward sweep), in CSR order.  A dependency inside the row's
+// block is stored as its position offset in the block (its LDS slot).
+__global__ void k_pack_static(const int *__restrict__ rows, int n, int upper,
+                              const int *__restrict__ blk_of, const int *__restrict__ blk_pos,
+                              const int *__restrict__ pos_of, const int *__restrict__ ia,
+                              const int *__restrict__ diag, const int *__restrict__ ja,
+                              int *__restrict__ prc, int *__restrict__ pdep,
+                              int *__restrict__ psrc, int *__restrict__ maxd) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n) return;
   const int row = rows[idx];
-  const int lo = (row / R) * R;
-  const int hi = (lo + R < n) ? lo + R : n;
+  const int blk = blk_of[row];
+  const int lo = blk_pos[blk];
   const int b = upper ? diag[row] + 1 : ia[row];
   const int e = upper ? ia[row + 1] : diag[row];
   const int c = e - b;
@@ -480,8 +493,7 @@ __global__ void k_pack_static(const int *__restrict__ rows, int n, int R, int up
     if (m < c) {
       const int k = b + m;
       const int j = ja[k];
-      const bool inblk = upper ? (j < hi) : (j >= lo);
-      dep = inblk ? (j - lo) : -(j + 1);
+      dep = (blk_of[j] == blk) ? (pos_of[j] - lo) : -(j + 1);
       src = k;
     }
     pdep[(size_t)FCP_PMAXD * idx + m] = dep;
@@ -492,46 +504,42 @@ __global__ void k_pack_static(const int *__restrict__ rows, int n, int R, int up
 
 // Row-block schedule from a level schedule: a stable sort of the level-major
 // row order by block id gives every block's rows in level order; steps are
-// the runs of equal level inside a block.
-int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, BlockSchedule &b) {
+// the runs of equal level inside a block.  Blocks are contiguous runs of R
+// rows, or -- with tile_rank -- the given tiles in rank order.
+int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, BlockSchedule &b,
+                             const int *tile_rank, int ntiles) {
   const int n = ctx->n;
-  b.R = R;
-  b.nblocks = (n + R - 1) / R;
-  DBuf<int> key, key_out;
+  b.tiled = tile_rank != nullptr;
+  b.nblocks = tile_rank ? ntiles : (n + R - 1) / R;
+  DBuf<int> key, key_out, rank_dev;
   FCP_HIP(ctx, key.alloc(n));
   FCP_HIP(ctx, key_out.alloc(n));
   FCP_HIP(ctx, b.rows.alloc(n));
-  k_block_of<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(s.order.p, n, R, key.p);
+  if (tile_rank) {
+    FCP_HIP(ctx, rank_dev.alloc(n));
+    FCP_HIP(ctx, hipMemcpyAsync(rank_dev.p, tile_rank, sizeof(int) * n, hipMemcpyHostToDevice,
+                                ctx->stream));
+  }
+  k_block_key<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(s.order.p, n, R, rank_dev.p, key.p);
   FCP_TRY(fcp_sort_pairs_int(ctx, key.p, key_out.p, s.order.p, b.rows.p, n, b.nblocks));
-  std::vector<int> rows(n), lev(n);
+  std::vector<int> rows(n), lev(n), blk_of(n);
   FCP_HIP(ctx, hipMemcpyAsync(rows.data(), b.rows.p, sizeof(int) * n, hipMemcpyDeviceToHost,
                               ctx->stream));
   FCP_HIP(ctx, hipMemcpyAsync(lev.data(), s.lev.p, sizeof(int) * n, hipMemcpyDeviceToHost,
                               ctx->stream));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  // packed static operands
-  FCP_HIP(ctx, b.prc.alloc((size_t)2 * n));
-  FCP_HIP(ctx, b.pdep.alloc((size_t)FCP_PMAXD * n));
-  FCP_HIP(ctx, b.psrc.alloc((size_t)FCP_PMAXD * n));
-  {
-    DBuf<int> mx;
-    FCP_HIP(ctx, mx.alloc(1));
-    FCP_HIP(ctx, hipMemsetAsync(mx.p, 0, sizeof(int), ctx->stream));
-    k_pack_static<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(b.rows.p, n, R, upper, ctx->ia.p,
-                                                             ctx->diag.p, ctx->ja.p, b.prc.p,
-                                                             b.pdep.p, b.psrc.p, mx.p);
-    FCP_HIP(ctx, hipGetLastError());
-    FCP_HIP(ctx, hipMemcpyAsync(&b.maxd, mx.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  }
-  b.pver64 = b.pver32 = ~0ull;
-  std::vector<int> sp, bs(b.nblocks + 1, 0);
+  for (int r = 0; r < n; ++r) blk_of[r] = tile_rank ? tile_rank[r] : r / R;
+  // step table, block starts and the largest block
+  std::vector<int> sp, bs(b.nblocks + 1, 0), bp(b.nblocks + 1, n);
   sp.reserve(b.nblocks * 64 + 1);
   int prev_blk = -1, prev_lev = -1;
   for (int i = 0; i < n; ++i) {
-    const int r = rows[i], blk = r / R;
+    const int r = rows[i], blk = blk_of[r];
     if (blk != prev_blk) {
-      for (int q = prev_blk + 1; q <= blk; ++q) bs[q] = (int)sp.size();
+      for (int q = prev_blk + 1; q <= blk; ++q) {
+        bs[q] = (int)sp.size();
+        bp[q] = i;
+      }
       prev_blk = blk;
       prev_lev = -1;
     }
@@ -540,19 +548,132 @@ int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, 
       prev_lev = lev[r];
     }
   }
-  for (int q = prev_blk + 1; q <= b.nblocks; ++q) bs[q] = (int)sp.size();
+  for (int q = prev_blk + 1; q <= b.nblocks; ++q) {
+    bs[q] = (int)sp.size();
+    bp[q] = n;
+  }
   b.nsteps = (int)sp.size();
   sp.push_back(n);
   b.max_steps = 0;
-  for (int q = 0; q < b.nblocks; ++q)
+  b.R = 0;
+  for (int q = 0; q < b.nblocks; ++q) {
     if (bs[q + 1] - bs[q] > b.max_steps) b.max_steps = bs[q + 1] - bs[q];
+    if (bp[q + 1] - bp[q] > b.R) b.R = bp[q + 1] - bp[q];
+  }
   FCP_HIP(ctx, b.step_ptr.alloc(sp.size()));
   FCP_HIP(ctx, b.blk_step.alloc(bs.size()));
+  FCP_HIP(ctx, b.blk_pos.alloc(bp.size()));
   FCP_HIP(ctx, hipMemcpyAsync(b.step_ptr.p, sp.data(), sizeof(int) * sp.size(), hipMemcpyHostToDevice,
                               ctx->stream));
   FCP_HIP(ctx, hipMemcpyAsync(b.blk_step.p, bs.data(), sizeof(int) * bs.size(), hipMemcpyHostToDevice,
                               ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(b.blk_pos.p, bp.data(), sizeof(int) * bp.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+  // packed static operands
+  FCP_HIP(ctx, b.prc.alloc((size_t)2 * n));
+  FCP_HIP(ctx, b.pdep.alloc((size_t)FCP_PMAXD * n));
+  FCP_HIP(ctx, b.psrc.alloc((size_t)FCP_PMAXD * n));
+  {
+    DBuf<int> mx, blk_dev, pos;
+    FCP_HIP(ctx, mx.alloc(1));
+    FCP_HIP(ctx, blk_dev.alloc(n));
+    FCP_HIP(ctx, pos.alloc(n));
+    FCP_HIP(ctx, hipMemcpyAsync(blk_dev.p, blk_of.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                                ctx->stream));
+    FCP_HIP(ctx, hipMemsetAsync(mx.p, 0, sizeof(int), ctx->stream));
+    k_pos_of<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(b.rows.p, n, pos.p);
+    k_pack_static<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(b.rows.p, n, upper, blk_dev.p,
+                                                             b.blk_pos.p, pos.p, ctx->ia.p,
+                                                             ctx->diag.p, ctx->ja.p, b.prc.p,
+                                                             b.pdep.p, b.psrc.p, mx.p);
+    FCP_HIP(ctx, hipGetLastError());
+    FCP_HIP(ctx, hipMemcpyAsync(&b.maxd, mx.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  b.pver64 = b.pver32 = ~0ull;
+  return 0;
+}
+
+// Geometric tiles: cells binned by centroid on a d0 x d1 x d2 grid of the
+// bounding box with about `target` cells per tile, tiles ranked by
+// (tx+ty+tz, tz, ty, tx).  Used only when every strict-lower dependency
+// lies in a tile of lower or equal rank and every strict-upper one in a tile
+// of higher or equal rank (then the forward sweep's tickets, and the
+// reversed backward tickets, only ever wait on earlier tickets); otherwise
+// the sweeps keep contiguous row blocks.
+int fcp_build_tile_schedules(fcp_ctx *ctx) {
+  ctx->have_tiles = false;
+  ctx->msched.clear();
+  if (!ctx->have_geom || ctx->tune_sweep_tile == 1) return 0;
+  const int n = ctx->n;
+  const int target = ctx->tune_sweep_tile >= 64 ? ctx->tune_sweep_tile : 4096;
+  if (n < 4 * target) return 0;
+  std::vector<double> x(n), y(n), z(n);
+  std::vector<int> ia(n + 1), ja(ctx->nnz), dg(n);
+  FCP_HIP(ctx, hipMemcpyAsync(x.data(), ctx->xc.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(y.data(), ctx->yc.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(z.data(), ctx->zc.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(ia.data(), ctx->ia.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(ja.data(), ctx->ja.p, sizeof(int) * ctx->nnz, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(dg.data(), ctx->diag.p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+  for (int i = 0; i < n; ++i) {
+    const double v[3] = {x[i], y[i], z[i]};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = v[a] < lo[a] ? v[a] : lo[a];
+      hi[a] = v[a] > hi[a] ? v[a] : hi[a];
+    }
+  }
+  // grid: cells per tile ~ target, tile edges proportional to the box
+  const double ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  const double vol = ext[0] * ext[1] * ext[2];
+  const double ntile = (double)n / target;
+  int d[3];
+  for (int a = 0; a < 3; ++a) {
+    double k = vol > 0 ? ext[a] * std::cbrt(ntile / vol) : std::cbrt(ntile);
+    d[a] = k < 1.0 ? 1 : (int)std::lround(k);
+  }
+  const int nt = d[0] * d[1] * d[2];
+  if (nt < 2) return 0;
+  std::vector<int> tid(n);
+  for (int i = 0; i < n; ++i) {
+    const double v[3] = {x[i], y[i], z[i]};
+    int t[3];
+    for (int a = 0; a < 3; ++a) {
+      const double u = ext[a] > 0 ? (v[a] - lo[a]) / ext[a] : 0.0;
+      int q = (int)(u * d[a]);
+      t[a] = q < 0 ? 0 : (q >= d[a] ? d[a] - 1 : q);
+    }
+    tid[i] = t[0] + d[0] * (t[1] + d[1] * t[2]);
+  }
+  // rank tiles by (tx+ty+tz, tz, ty, tx)
+  std::vector<int> tiles(nt), trank(nt);
+  for (int t = 0; t < nt; ++t) tiles[t] = t;
+  auto key = [&](int t) {
+    const int tx = t % d[0], ty = (t / d[0]) % d[1], tz = t / (d[0] * d[1]);
+    return std::make_tuple(tx + ty + tz, tz, ty, tx);
+  };
+  std::sort(tiles.begin(), tiles.end(), [&](int a, int b) { return key(a) < key(b); });
+  for (int r = 0; r < nt; ++r) trank[tiles[r]] = r;
+  std::vector<int> rank(n);
+  for (int i = 0; i < n; ++i) rank[i] = trank[tid[i]];
+  for (int i = 0; i < n; ++i) {
+    for (int k = ia[i]; k < dg[i]; ++k)
+      if (rank[ja[k]] > rank[i]) return 0;
+    for (int k = dg[i] + 1; k < ia[i + 1]; ++k)
+      if (rank[ja[k]] < rank[i]) return 0;
+  }
+  // compact away empty tiles (keep rank order)
+  std::vector<int> used(nt, 0), remap(nt, -1);
+  for (int i = 0; i < n; ++i) used[rank[i]] = 1;
+  int m = 0;
+  for (int r = 0; r < nt; ++r)
+    if (used[r]) remap[r] = m++;
+  for (int i = 0; i < n; ++i) rank[i] = remap[rank[i]];
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, 0, 0, ctx->tlower, rank.data(), m));
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, 0, 1, ctx->tupper, rank.data(), m));
+  ctx->have_tiles = true;
   return 0;
 }
 
@@ -570,5 +691,6 @@ int fcp_build_schedules(fcp_ctx *ctx) {
   FCP_TRY(build_one_schedule(ctx, 0, ctx->lower));
   FCP_TRY(build_one_schedule(ctx, 1, ctx->upper));
   FCP_TRY(fcp_build_block_schedules(ctx, fcp_block_rows(ctx, 8)));
+  FCP_TRY(fcp_build_tile_schedules(ctx));
   return 0;
 }

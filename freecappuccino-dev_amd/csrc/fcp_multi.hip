@@ -623,7 +623,7 @@ __device__ __forceinline__ void row_store(__amdgpu_buffer_rsrc_t rs, int off, co
 
 template <typename T, int NR, int NRP>
 struct MSlot {
-  int row, cnt, ok;
+  int row, cnt, ok, pos;
   T dv;
   T xin[NRP];
   int dep[FCP_PMAXD];
@@ -677,15 +677,19 @@ __device__ __forceinline__ void m_load1(MSlot<T, NR, NRP> &S, const MArgs<T> &P,
   const int idx = sptr[sc] + (int)threadIdx.x;
   const bool ok = (step < ns) && (idx < sptr[sc + 1]);
   S.ok = ok ? 1 : 0;
+  S.pos = ok ? idx : 0;
   m_fetch<T, NRP>(P, rin, ok ? idx : 0, S.row, S.cnt, S.dv, S.xin, S.dep, S.av);
 }
 
+// idle: a row private to this lane, read by slots without a remote
+// dependency so idle lanes spread over memory instead of sharing an address
 template <typename T, int NR, int NRP>
-__device__ __forceinline__ void m_load2(MSlot<T, NR, NRP> &S, __amdgpu_buffer_rsrc_t rout) {
+__device__ __forceinline__ void m_load2(MSlot<T, NR, NRP> &S, __amdgpu_buffer_rsrc_t rout,
+                                        int idle) {
 #pragma unroll
   for (int m = 0; m < FCP_PMAXD; ++m) {
     const bool need = S.ok && m < S.cnt && S.dep[m] < 0;
-    row_load<T, NRP>(rout, (need ? (-S.dep[m] - 1) : 0) * NRP * (int)sizeof(T), S.rv[m]);
+    row_load<T, NRP>(rout, (need ? (-S.dep[m] - 1) : idle) * NRP * (int)sizeof(T), S.rv[m]);
   }
 }
 
@@ -730,7 +734,7 @@ __device__ __forceinline__ void m_compute(const MSlot<T, NR, NRP> &S, const MArg
   if (S.ok) {
     m_row_value<T, NR, NRP, MODE>(S.cnt, S.dv, S.xin, S.dep, S.av, S.rv, zl, out, err, slp, val);
 #pragma unroll
-    for (int r = 0; r < NRP; ++r) zl[(S.row - lo) * NRP + r] = val[r];
+    for (int r = 0; r < NRP; ++r) zl[(S.pos - lo) * NRP + r] = val[r];
     row_store<T, NRP>(rout, S.row * NRP * (int)sizeof(T), val);
   }
   if (step >= ns) return;
@@ -741,7 +745,7 @@ __device__ __forceinline__ void m_compute(const MSlot<T, NR, NRP> &S, const MArg
     m_fetch<T, NRP>(P, rin, idx, row, cnt, dv, xin, dep, av);
     m_row_value<T, NR, NRP, MODE>(cnt, dv, xin, dep, av, nullptr, zl, out, err, slp, val);
 #pragma unroll
-    for (int r = 0; r < NRP; ++r) zl[(row - lo) * NRP + r] = val[r];
+    for (int r = 0; r < NRP; ++r) zl[(idx - lo) * NRP + r] = val[r];
     row_store<T, NRP>(rout, row * NRP * (int)sizeof(T), val);
   }
 }
@@ -757,8 +761,9 @@ struct MDepth {
 
 template <typename T, int NR, int NRP, int MODE>
 __global__ __launch_bounds__(BLK, 1) void mk_psweep(MArgs<T> P, const int *__restrict__ step_ptr,
-                                                    const int *__restrict__ blk_step, int nblocks,
-                                                    int R, int sptr_off, T *out,
+                                                    const int *__restrict__ blk_step,
+                                                    const int *__restrict__ blk_pos, int nblocks,
+                                                    int sptr_off, T *out,
                                                     int *__restrict__ ticket,
                                                     int *__restrict__ err,
                                                     const SolveScalars *__restrict__ s, int slp) {
@@ -773,7 +778,7 @@ __global__ __launch_bounds__(BLK, 1) void mk_psweep(MArgs<T> P, const int *__res
   const int t = sb;
   if (t >= nblocks) return;
   const int b = (MODE == 1) ? nblocks - 1 - t : t;
-  const int lo = b * R;
+  const int lo = blk_pos[b];
   const int st0 = blk_step[b];
   const int ns = blk_step[b + 1] - st0;
   for (int i = threadIdx.x; i <= ns; i += BLK) sptr[i] = step_ptr[st0 + i];
@@ -783,18 +788,19 @@ __global__ __launch_bounds__(BLK, 1) void mk_psweep(MArgs<T> P, const int *__res
       __builtin_amdgcn_make_buffer_rsrc((void *)P.pin, (short)0, vbytes, 0x00020000);
   const __amdgpu_buffer_rsrc_t rout =
       __builtin_amdgcn_make_buffer_rsrc((void *)out, (short)0, vbytes, 0x00020000);
+  const int idle = (int)(((unsigned)b * BLK + threadIdx.x) % (unsigned)P.n);
   MSlot<T, NR, NRP> S[PD];
 #pragma unroll
   for (int j = 0; j < PD; ++j) m_load1<T, NR, NRP>(S[j], P, rin, sptr, j, ns);
 #pragma unroll
-  for (int j = 0; j < PD2; ++j) m_load2<T, NR, NRP>(S[j], rout);
+  for (int j = 0; j < PD2; ++j) m_load2<T, NR, NRP>(S[j], rout, idle);
   for (int base = 0; base < ns; base += PD) {
 #pragma unroll
     for (int j = 0; j < PD; ++j) {
       const int step = base + j;
       m_compute<T, NR, NRP, MODE>(S[j], P, rin, rout, sptr, step, ns, lo, zl, out, err, slp);
       __syncthreads();
-      m_load2<T, NR, NRP>(S[(j + PD2) % PD], rout);
+      m_load2<T, NR, NRP>(S[(j + PD2) % PD], rout, idle);
       m_load1<T, NR, NRP>(S[j], P, rin, sptr, step + PD, ns);
     }
   }
@@ -863,11 +869,19 @@ const float *pdx_of<float>(BlockSchedule &b) {
 }
 
 // Batched sweeps apply when every row has at most FCP_PMAXD dependencies
-// and the block's values plus its step table fit the workgroup LDS.
+// and the block's values plus its step table fit the workgroup LDS: the
+// geometric tiles when they fit, else contiguous blocks sized for NRP values
+// per row.  *lo/*up/*pin receive the schedules and the input staging buffer.
 template <typename T, int NRP>
-int msched_for(fcp_ctx *ctx, MultiSched **out) {
-  *out = nullptr;
+int msched_for(fcp_ctx *ctx, BlockSchedule **lo, BlockSchedule **up, DBuf<char> **pin) {
+  *lo = *up = nullptr;
   if (ctx->tune_sweep_kind != 0) return 0;
+  if (ctx->have_tiles && sched_fits(ctx, ctx->tlower, ctx->tupper, NRP * sizeof(T))) {
+    *lo = &ctx->tlower;
+    *up = &ctx->tupper;
+    *pin = &ctx->tpin;
+    return 0;
+  }
   const int R = fcp_block_rows(ctx, NRP * (int)sizeof(T));
   auto it = ctx->msched.find(R);
   if (it == ctx->msched.end()) {
@@ -877,14 +891,10 @@ int msched_for(fcp_ctx *ctx, MultiSched **out) {
     it = ctx->msched.emplace(R, std::move(m)).first;
   }
   MultiSched *m = it->second.get();
-  for (int u = 0; u < 2; ++u) {
-    const BlockSchedule &b = u ? m->up : m->lo;
-    if (b.maxd > FCP_PMAXD) return 0;
-    const size_t lds = (((size_t)b.R * NRP * sizeof(T) + 15) & ~(size_t)15) +
-                       (size_t)(b.max_steps + 2) * sizeof(int);
-    if (lds > ctx->lds_cap) return 0;
-  }
-  *out = m;
+  if (!sched_fits(ctx, m->lo, m->up, NRP * sizeof(T))) return 0;
+  *lo = &m->lo;
+  *up = &m->up;
+  *pin = &m->pin;
   return 0;
 }
 
@@ -893,16 +903,17 @@ int msched_for(fcp_ctx *ctx, MultiSched **out) {
 template <typename T, int NR, int NRP>
 int mprecond_apply(fcp_ctx *ctx, const T *r, T *y, T *z, T *tmp3, const SolveScalars *s) {
   const int n = ctx->n;
-  MultiSched *ms = nullptr;
-  FCP_TRY((msched_for<T, NRP>(ctx, &ms)));
-  if (ms && (size_t)n * NRP * sizeof(T) < ((size_t)1 << 31)) {
+  BlockSchedule *blo = nullptr, *bup = nullptr;
+  DBuf<char> *pbuf = nullptr;
+  FCP_TRY((msched_for<T, NRP>(ctx, &blo, &bup, &pbuf)));
+  if (blo && (size_t)n * NRP * sizeof(T) < ((size_t)1 << 31)) {
     const size_t m = (size_t)n * NRP;
-    FCP_HIP(ctx, ms->pin.alloc(m * sizeof(T)));
+    FCP_HIP(ctx, pbuf->alloc(m * sizeof(T)));
     mk_fill_sent<T, NR><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(y, z, m, tickets(ctx), s);
     for (int u = 0; u < 2; ++u) {
-      BlockSchedule &b = u ? ms->up : ms->lo;
+      BlockSchedule &b = u ? *bup : *blo;
       FCP_TRY(pack_of<T>(ctx, b));
-      T *pin = reinterpret_cast<T *>(ms->pin.p);
+      T *pin = reinterpret_cast<T *>(pbuf->p);
       const T *in = u ? y : r;
       T *out = u ? z : y;
       mk_gather<T, NR, NRP><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, b.prc.p, in, pin, s);
@@ -913,11 +924,11 @@ int mprecond_apply(fcp_ctx *ctx, const T *r, T *y, T *z, T *tmp3, const SolveSca
                           : (sizeof(T) == 8 ? "msweep_fwd_f64" : "msweep_fwd_f32"));
       if (u)
         mk_psweep<T, NR, NRP, 1><<<b.nblocks, BLK, lds, ctx->stream>>>(
-            P, b.step_ptr.p, b.blk_step.p, b.nblocks, b.R, sptr_off, out, tickets(ctx) + 1,
-            ctx->errflag.p, s, ctx->tune_sweep_sleep);
+            P, b.step_ptr.p, b.blk_step.p, b.blk_pos.p, b.nblocks, sptr_off, out,
+            tickets(ctx) + 1, ctx->errflag.p, s, ctx->tune_sweep_sleep);
       else
         mk_psweep<T, NR, NRP, 0><<<b.nblocks, BLK, lds, ctx->stream>>>(
-            P, b.step_ptr.p, b.blk_step.p, b.nblocks, b.R, sptr_off, out, tickets(ctx),
+            P, b.step_ptr.p, b.blk_step.p, b.blk_pos.p, b.nblocks, sptr_off, out, tickets(ctx),
             ctx->errflag.p, s, ctx->tune_sweep_sleep);
       FCP_HIP(ctx, hipGetLastError());
     }

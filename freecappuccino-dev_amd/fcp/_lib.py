@@ -43,6 +43,10 @@ SIGNATURES = {
     "fcp_csrsolve": (_i, [_p, C.c_char_p, _i, _p, _p, C.POINTER(_d), _i, _d, _d, C.c_char_p, _i,
                           _i, C.POINTER(SolveOpts)]),
     "fcp_rcm": (_i, [_p, _i, _p]),
+    "fcp_matrix_transpose": (_i, [_p]),
+    "fcp_convdiff_backward": (_i, [_p, _p, _p, _p, _p, _d, _i, _p, _p, _p, _p, _p, _p]),
+    "fcp_laplacian_face_backward": (_i, [_p] + [_p] * 11),
+    "fcp_grad_gauss_backward": (_i, [_p, _p, _p]),
     "fcp_create_lsq_grad_matrix": (_i, [_p, _i, _i, _p]),
     "fcp_grad": (_i, [_p, _i, _p, _p]),
     "fcp_csrsolve_multi": (_i, [_p, C.c_char_p, _i, _i, _p, _i, _p, _p, _i, _d, _d, _i,

@@ -1,0 +1,130 @@
+"""Discrete adjoint of the steady convection-diffusion path on the GPU.
+
+Primal map (parameters: face diffusivity Gamma[F], cell source s[n], values
+phi_b of the Dirichlet boundary faces):
+
+    phi_0 = 0 in the cells; boundary entries = phi_b on Dirichlet faces,
+            the owner's value on every other boundary face
+    for k = 1..K (deferred-correction passes):
+        g_{k-1} = Gauss gradient of phi_{k-1}             (fcp_grad_gauss)
+        A, b    = facefluxsc assembly with Gamma, phi_{k-1}, g_{k-1},
+                  b = s V + face and boundary terms        (fcp_convdiff, urf 1)
+        phi_k   = A^{-1} b         (FP64 DILU-BiCGStab, ||b-Ax||/||b|| <= 1e-13)
+    J = sum_i w_i phi_K,i V_i
+
+A depends only on Gamma and the mass flux, so every pass shares it.  The
+reverse sweep solves A^T lam_k = phibar_k (A^T by the face-based coefficient
+swap, fcp_matrix_transpose), then accumulates with the backward kernels
+(fcp_convdiff_backward, fcp_grad_gauss_backward):
+
+    Gammabar += -lam^T (dA/dGamma) phi_k + lam^T db/dGamma
+    sbar     += lam V
+    phibar_{k-1} = (db/dphi_{k-1})^T lam + G^T (db/dg_{k-1})^T lam
+
+and routes the boundary entries of phibar_{k-1} to phi_b (Dirichlet faces)
+or to the owner cell (copied faces).  ``ConvDiffSolve`` wraps the map as a
+torch.autograd.Function.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .api import Context, FCP_CRIT_TRUE_L2, FCP_PREC_FP64
+
+
+class ConvDiffProblem:
+    """One mesh on one device with a fixed mass flux and boundary kinds."""
+
+    def __init__(self, ctx: Context, mesh, geo, flux, bc_kind, K: int = 2, gam: float = 1.0,
+                 scheme: int = 1, tol: float = 1e-13, itr_max: int = 20000,
+                 device: str = "cuda:0"):
+        self.ctx, self.mesh = ctx, mesh
+        self.K, self.gam, self.scheme, self.tol, self.itr_max = K, gam, scheme, tol, itr_max
+        dev = torch.device(device)
+        self.dev = dev
+        T = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
+        self.n, self.nt = mesh.n_cells, mesh.n_total
+        self.flux = T(flux)
+        self.bc_kind = T(np.asarray(bc_kind, np.uint8), torch.uint8)
+        self.dirichlet = T(np.asarray(bc_kind) == 0, torch.bool)
+        self.own_b = T(mesh.owner[mesh.n_inner:].astype(np.int64), torch.int64)
+        self.vol = T(geo["vol"])
+
+    def _full(self, phi_int, phib):
+        phi = torch.empty(self.nt, dtype=torch.float64, device=self.dev)
+        phi[: self.n] = phi_int
+        phi[self.n:] = torch.where(self.dirichlet, phib, phi_int[self.own_b])
+        return phi
+
+    def _solve(self, rhs):
+        x = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+        rc, info = self.ctx.csrsolve("bicgstab", x, rhs, itr_max=self.itr_max, tol_abs=0.0,
+                                     tol_rel=self.tol, precision=FCP_PREC_FP64,
+                                     criterion=FCP_CRIT_TRUE_L2)
+        if rc != 0:
+            raise RuntimeError(f"adjoint path: solve did not converge ({info})")
+        return x
+
+    def forward(self, gamma, s, phib):
+        """phi_K (cells) for parameters gamma[F], s[n], phib[F_b] (device)."""
+        n, nt = self.n, self.nt
+        gamma = gamma.contiguous()
+        phi_int = torch.zeros(n, dtype=torch.float64, device=self.dev)
+        self.states = []
+        for _ in range(self.K):
+            phi_old = self._full(phi_int, phib)
+            g = torch.zeros((nt, 3), dtype=torch.float64, device=self.dev)
+            self.ctx.grad_gauss(phi_old, out=g)
+            su = s * self.vol
+            sp = torch.zeros(n, dtype=torch.float64, device=self.dev)
+            work = phi_old.clone()  # fcp_convdiff rewrites zero-gradient boundary entries
+            self.ctx.convdiff(self.flux, gamma, work, g, self.gam, self.scheme, self.bc_kind, su, sp,
+                              urf=1.0)
+            phi_int = self._solve(su)
+            self.states.append((phi_old, g, phi_int))
+        self.gamma = gamma
+        return phi_int
+
+    def backward(self, phibar):
+        """(gamma_bar[F], s_bar[n], phib_bar[F_b]) for dJ/dphi_K = phibar."""
+        n, nt = self.n, self.nt
+        F = self.mesh.n_faces
+        gamma_bar = torch.zeros(F, dtype=torch.float64, device=self.dev)
+        s_bar = torch.zeros(n, dtype=torch.float64, device=self.dev)
+        phib_bar = torch.zeros(nt - n, dtype=torch.float64, device=self.dev)
+        src = phibar.contiguous()
+        self.ctx.matrix_transpose()
+        try:
+            for k in reversed(range(self.K)):
+                phi_old, g, phi_new = self.states[k]
+                lam = self._solve(src)
+                s_bar += lam * self.vol
+                pob = torch.zeros(nt, dtype=torch.float64, device=self.dev)
+                gob = torch.zeros((nt, 3), dtype=torch.float64, device=self.dev)
+                self.ctx.convdiff_backward(self.flux, self.gamma, phi_old, g, self.gam,
+                                           self.scheme, self.bc_kind, lam, phi_new, gamma_bar,
+                                           pob, gob)
+                self.ctx.grad_gauss_backward(gob, pob)
+                bnd = pob[n:]
+                phib_bar += torch.where(self.dirichlet, bnd, torch.zeros_like(bnd))
+                cells = pob[:n].clone()
+                cells.index_add_(0, self.own_b, torch.where(self.dirichlet, torch.zeros_like(bnd), bnd))
+                src = cells
+        finally:
+            self.ctx.matrix_transpose()
+        return gamma_bar, s_bar, phib_bar
+
+
+class ConvDiffSolve(torch.autograd.Function):
+    """phi_K = ConvDiffSolve.apply(gamma, s, phib, problem)."""
+
+    @staticmethod
+    def forward(ctx, gamma, s, phib, problem):
+        ctx.problem = problem
+        return problem.forward(gamma, s, phib)
+
+    @staticmethod
+    def backward(ctx, phibar):
+        g, s, b = ctx.problem.backward(phibar)
+        return g, s, b, None

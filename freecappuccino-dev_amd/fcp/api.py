@@ -26,6 +26,7 @@ FCP_TUNE_SWEEP_WAVES = 1
 FCP_TUNE_SWEEP_SLEEP = 2
 FCP_TUNE_SWEEP_KIND = 3
 FCP_TUNE_SWEEP_ROWS = 4
+FCP_TUNE_SWEEP_TILE = 5
 
 
 class FcpError(RuntimeError):
@@ -192,6 +193,34 @@ class Context:
         rc = self.lib.fcp_grad_gauss(self.h, _ptr(_f64(u))[0], _ptr(out)[0])
         self._check(rc, "fcp_grad_gauss")
         return out
+
+    # -- discrete adjoint building blocks (outputs accumulate) --------------
+    def matrix_transpose(self):
+        self._check(self.lib.fcp_matrix_transpose(self.h), "fcp_matrix_transpose")
+
+    def convdiff_backward(self, flmass, dcoef, phi_old, grad_old, gam, scheme, bc_kind, lam,
+                          phi_new, dcoef_bar, phi_old_bar, grad_old_bar):
+        if isinstance(bc_kind, np.ndarray):
+            bc_kind = np.ascontiguousarray(bc_kind, dtype=np.uint8)
+        rc = self.lib.fcp_convdiff_backward(
+            self.h, _ptr(_f64(flmass))[0], _ptr(_f64(dcoef))[0], _ptr(_f64(phi_old))[0],
+            _ptr(_f64(grad_old))[0], float(gam), int(scheme), _ptr(bc_kind)[0], _ptr(_f64(lam))[0],
+            _ptr(_f64(phi_new))[0], _ptr(dcoef_bar)[0], _ptr(phi_old_bar)[0], _ptr(grad_old_bar)[0])
+        self._check(rc, "fcp_convdiff_backward")
+
+    def laplacian_face_backward(self, mu_f, bc_kind, phi_bc, gamma_f, grad, lam, phi, mu_f_bar,
+                                phi_bc_bar, gamma_bar=None, grad_bar=None):
+        if isinstance(bc_kind, np.ndarray):
+            bc_kind = np.ascontiguousarray(bc_kind, dtype=np.uint8)
+        rc = self.lib.fcp_laplacian_face_backward(
+            self.h, _ptr(_f64(mu_f))[0], _ptr(bc_kind)[0], _ptr(_f64(phi_bc))[0],
+            _ptr(_f64(gamma_f))[0], _ptr(_f64(grad))[0], _ptr(_f64(lam))[0], _ptr(_f64(phi))[0],
+            _ptr(mu_f_bar)[0], _ptr(phi_bc_bar)[0], _ptr(gamma_bar)[0], _ptr(grad_bar)[0])
+        self._check(rc, "fcp_laplacian_face_backward")
+
+    def grad_gauss_backward(self, grad_bar, phi_bar):
+        rc = self.lib.fcp_grad_gauss_backward(self.h, _ptr(_f64(grad_bar))[0], _ptr(phi_bar)[0])
+        self._check(rc, "fcp_grad_gauss_backward")
 
     def rcm(self, pp_iters: int = 4):
         """GPU reverse Cuthill-McKee: new 0-based number of every cell."""

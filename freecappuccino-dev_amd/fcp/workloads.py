@@ -204,3 +204,22 @@ def tet_inputs(mesh, geo):
     bc_kind = np.where(mesh.patch_of_bface() == 0, 0, 1).astype(np.uint8)
     return dict(f=f, gamma=gamma, mu_f=-gamma, source=s, bc_kind=bc_kind,
                 phi_bc=np.zeros(mesh.n_total))
+
+
+# ---------------------------------------------------------------------------
+# Adjoint sensitivities (128^3 jittered, one transported scalar)
+# ---------------------------------------------------------------------------
+# Mesh ``convdiff_mesh(N, jitter=0.2)`` with the projected flux and face
+# diffusivity of the convection-diffusion workload; source s = component 0
+# of its sources; Dirichlet wall values phi_b = 0.1 U[-1,1] (seed 4,
+# stream 61); objective J = sum_i w_i phi_i V_i with w ~ U[0,1] (seed 4,
+# stream 60); K = 2 deferred-correction passes of the linear-upwind scheme.
+ADJOINT_K = 2
+
+
+def adjoint_inputs(mesh, geo, flux):
+    base = convdiff_inputs(mesh, geo, flux)
+    n, Fb = mesh.n_cells, mesh.n_bnd
+    phib = np.where(base["bc_kind"] == 0, 0.1 * uniform(4, 61, Fb, -1.0, 1.0), 0.0)
+    return dict(gamma=base["gamma"], source=base["source"][0], phib=phib,
+                bc_kind=base["bc_kind"], w=uniform(4, 60, n, 0.0, 1.0), K=ADJOINT_K)

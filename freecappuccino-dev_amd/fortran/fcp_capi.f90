@@ -105,6 +105,40 @@ module fcp_capi
       real(c_double) :: gamma_f(*), grad(*), su(*)
     end function
 
+    integer(c_int) function fcp_matrix_transpose(ctx) bind(C, name='fcp_matrix_transpose')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function fcp_convdiff_backward(ctx, flmass, dcoef, phi_old, grad_old, gam, &
+        scheme, bc_kind, lam, phi_new, dcoef_bar, phi_old_bar, grad_old_bar) &
+        bind(C, name='fcp_convdiff_backward')
+      import :: c_int, c_double, c_ptr, c_signed_char
+      type(c_ptr), value :: ctx
+      real(c_double) :: flmass(*), dcoef(*), phi_old(*), grad_old(*), lam(*), phi_new(*)
+      real(c_double) :: dcoef_bar(*), phi_old_bar(*), grad_old_bar(*)
+      real(c_double), value :: gam
+      integer(c_int), value :: scheme
+      integer(c_signed_char) :: bc_kind(*)
+    end function
+
+    integer(c_int) function fcp_laplacian_face_backward(ctx, mu_f, bc_kind, phi_bc, gamma_f, &
+        grad, lam, phi, mu_f_bar, phi_bc_bar, gamma_bar, grad_bar) &
+        bind(C, name='fcp_laplacian_face_backward')
+      import :: c_int, c_double, c_ptr, c_signed_char
+      type(c_ptr), value :: ctx
+      real(c_double) :: mu_f(*), phi_bc(*), gamma_f(*), grad(*), lam(*), phi(*)
+      real(c_double) :: mu_f_bar(*), phi_bc_bar(*), gamma_bar(*), grad_bar(*)
+      integer(c_signed_char) :: bc_kind(*)
+    end function
+
+    integer(c_int) function fcp_grad_gauss_backward(ctx, grad_bar, phi_bar) &
+        bind(C, name='fcp_grad_gauss_backward')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: grad_bar(*), phi_bar(*)
+    end function
+
     integer(c_int) function fcp_rcm(ctx, pp_iters, perm) bind(C, name='fcp_rcm')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

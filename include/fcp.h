@@ -169,6 +169,42 @@ int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt,
 /* grad_gauss (gradients.f90:1607-1715). dudxi[3*numTotal], boundary rows 0. */
 int fcp_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi);
 
+/* --- discrete adjoint ----------------------------------------------------
+ * No counterpart in the reference (its only transpose primitive is atx_cr,
+ * linear_solvers.f90:1079-1171).  For a system A(c) phi = b(c, phi_old,
+ * grad_old) assembled by the entry points above and an adjoint lam solving
+ * A^T lam = dJ/dphi, these accumulate (+=) the reverse-mode derivatives of J;
+ * every output is written by one thread (deterministic). */
+
+/* A <- A^T for the resident matrix (face-based swap of the slot_pn / slot_np
+ * coefficients); call twice to restore A.  A following csrsolve then solves
+ * the transposed system. */
+int fcp_matrix_transpose(fcp_ctx *ctx);
+
+/* Reverse of fcp_convdiff (urf = 1 only): inputs as given to the forward
+ * call (phi_old = its phi[numTotal], grad_old = its grad), lam[numCells],
+ * phi_new[numCells] = solution of the assembled system.  Accumulates
+ * dcoef_bar[nFaces], phi_old_bar[numTotal], grad_old_bar[3*numTotal]. */
+int fcp_convdiff_backward(fcp_ctx *ctx, const double *flmass, const double *dcoef,
+                          const double *phi_old, const double *grad_old, double gam,
+                          int scheme, const unsigned char *bc_kind, const double *lam,
+                          const double *phi_new, double *dcoef_bar, double *phi_old_bar,
+                          double *grad_old_bar);
+
+/* Reverse of fcp_laplacian_face (mu_f, bc_kind, phi) followed by
+ * fcp_nonorth_correction (gamma_f, grad) (gamma_f and grad may be NULL when
+ * no correction was applied).  Accumulates mu_f_bar[nFaces],
+ * phi_bc_bar[numTotal] (boundary entries), gamma_bar[nFaces],
+ * grad_bar[3*numTotal]. */
+int fcp_laplacian_face_backward(fcp_ctx *ctx, const double *mu_f,
+                                const unsigned char *bc_kind, const double *phi_bc,
+                                const double *gamma_f, const double *grad, const double *lam,
+                                const double *phi, double *mu_f_bar, double *phi_bc_bar,
+                                double *gamma_bar, double *grad_bar);
+
+/* Reverse of fcp_grad_gauss: phi_bar[numTotal] += G^T grad_bar. */
+int fcp_grad_gauss_backward(fcp_ctx *ctx, const double *grad_bar, double *phi_bar);
+
 /* Reverse Cuthill-McKee renumbering of the cell graph of the resident mesh
  * (adjacency of the CSR pattern), computed on the GPU; no counterpart in the
  * reference.  Each connected component starts from its minimum-(degree, id)
@@ -262,6 +298,8 @@ int fcp_gauss_seidel(int n, int nnz, const int *ia, const int *ja, const double 
 #define FCP_TUNE_SWEEP_SLEEP 2   /* s_sleep argument between dependency polls   */
 #define FCP_TUNE_SWEEP_KIND 3    /* 0 row-block LDS sweeps, 1 level-task sweeps */
 #define FCP_TUNE_SWEEP_ROWS 4    /* rows per block of the row-block sweeps      */
+#define FCP_TUNE_SWEEP_TILE 5    /* rows per geometric tile of the pipelined
+                                    sweeps (0 = auto, 1 = contiguous blocks)   */
 /* Sets a launch-shape parameter of the context; results do not depend on it. */
 int fcp_tune_set(fcp_ctx *ctx, int param, int value);
 
