@@ -74,7 +74,7 @@ struct BlockSchedule {
   DBuf<int> prc;                // [2*n]  (row, count) pairs
   DBuf<int> pdep;               // [PMAXD*n] dependencies, PMAXD per position
   DBuf<int> psrc;               // [PMAXD*n] CSR positions of the coefficients
-  DBuf<double> pa64, pdx64;     // [PMAXD*n] coefficients; [2*n] (pivot, input)
+  DBuf<double> pa64, pdx64;     // [PMAXD*n] coefficients; [2*n] (pivot, spare)
   DBuf<float> pa32, pdx32;
   uint64_t pver64 = ~0ull, pver32 = ~0ull;
 };

@@ -284,13 +284,15 @@ template <typename T>
 struct PArgs {
   const int *prc, *pdep;
   const T *pa, *pdx;
+  const T *in;  // sweep input, read by row
   int n;
 };
 
 // 16-byte vector loads of the packed operands (one int2, one int4, 16 or 32
-// bytes of coefficients and one (pivot, input) pair per position).
+// bytes of coefficients and the pivot of every position); the row's input
+// value is read in the second prefetch stage, once the row id has arrived.
 template <typename T>
-__device__ __forceinline__ void p_fetch(const PArgs<T> &P, int idx, int &row, int &cnt, T &dv, T &xin,
+__device__ __forceinline__ void p_fetch(const PArgs<T> &P, int idx, int &row, int &cnt, T &dv,
                                         int *dep, T *av) {
   const int2 rc = reinterpret_cast<const int2 *>(P.prc)[idx];
   row = rc.x;
@@ -308,7 +310,6 @@ __device__ __forceinline__ void p_fetch(const PArgs<T> &P, int idx, int &row, in
     av[3] = (T)a4.w;
     const float2 dx = reinterpret_cast<const float2 *>(P.pdx)[idx];
     dv = (T)dx.x;
-    xin = (T)dx.y;
   } else {
     const double2 a0 = reinterpret_cast<const double2 *>(P.pa)[2 * idx];
     const double2 a1 = reinterpret_cast<const double2 *>(P.pa)[2 * idx + 1];
@@ -318,7 +319,6 @@ __device__ __forceinline__ void p_fetch(const PArgs<T> &P, int idx, int &row, in
     av[3] = (T)a1.y;
     const double2 dx = reinterpret_cast<const double2 *>(P.pdx)[idx];
     dv = (T)dx.x;
-    xin = (T)dx.y;
   }
 }
 
@@ -331,14 +331,15 @@ __device__ __forceinline__ void p_load1(PSlot<T> &S, const PArgs<T> &P, const in
   const bool ok = (step < ns) && (idx < sptr[sc + 1]);
   S.ok = ok ? 1 : 0;
   S.pos = ok ? idx : 0;
-  p_fetch<T>(P, ok ? idx : 0, S.row, S.cnt, S.dv, S.xin, S.dep, S.av);
+  p_fetch<T>(P, ok ? idx : 0, S.row, S.cnt, S.dv, S.dep, S.av);
 }
 
 // idle: a row private to this lane, read by slots without a remote
 // dependency so idle lanes spread over memory instead of sharing an address
 template <typename T>
-__device__ __forceinline__ void p_load2(PSlot<T> &S, T *out, int idle) {
+__device__ __forceinline__ void p_load2(PSlot<T> &S, const PArgs<T> &P, T *out, int idle) {
   using B = Bits<T>;
+  S.xin = P.in[S.row];
 #pragma unroll
   for (int m = 0; m < FCP_PMAXD; ++m) {
     const bool need = S.ok && m < S.cnt && S.dep[m] < 0;
@@ -382,8 +383,9 @@ __device__ __forceinline__ void p_compute(const PSlot<T> &S, const PArgs<T> &P, 
   const int e = sptr[step + 1];
   for (int idx = sptr[step] + (int)threadIdx.x + BLK; idx < e; idx += BLK) {
     int row, cnt, dep[FCP_PMAXD];
-    T dv, xin, av[FCP_PMAXD];
-    p_fetch<T>(P, idx, row, cnt, dv, xin, dep, av);
+    T dv, av[FCP_PMAXD];
+    p_fetch<T>(P, idx, row, cnt, dv, dep, av);
+    const T xin = P.in[row];
     const T val = p_row_value<T, MODE>(cnt, dv, xin, dep, av, nullptr, zl, out, err, slp);
     zl[idx - lo] = val;
     publish<T>(out + row, val);
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(BLK, 1) void k_psweep(PArgs<T> P, const int *__rest
 #pragma unroll
   for (int j = 0; j < PD; ++j) p_load1<T>(S[j], P, sptr, j, ns);
 #pragma unroll
-  for (int j = 0; j < PD2; ++j) p_load2<T>(S[j], out, idle);
+  for (int j = 0; j < PD2; ++j) p_load2<T>(S[j], P, out, idle);
   // Steps past ns (at most PD-1 in the last group) are empty: their slots
   // are invalid, so every group runs the same instruction stream.
   for (int base = 0; base < ns; base += PD) {
@@ -431,7 +433,7 @@ __global__ __launch_bounds__(BLK, 1) void k_psweep(PArgs<T> P, const int *__rest
       const int step = base + j;
       p_compute<T, MODE>(S[j], P, sptr, step, ns, lo, zl, out, err, slp);
       __syncthreads();
-      p_load2<T>(S[(j + PD2) % PD], out, idle);
+      p_load2<T>(S[(j + PD2) % PD], P, out, idle);
       p_load1<T>(S[j], P, sptr, step + PD, ns);
     }
   }
@@ -453,14 +455,6 @@ __global__ void k_pack_vals(int n, const int *__restrict__ prc, const int *__res
   pdx[2 * (size_t)idx] = (T)d[prc[2 * idx]];
 }
 
-// Input values of a sweep in processing order, next to the pivots.
-template <typename T>
-__global__ void k_gather(int n, const int *__restrict__ prc, const T *__restrict__ in,
-                         T *__restrict__ pdx, const SolveScalars *__restrict__ s) {
-  if (s && solve_done(s)) return;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx < n) pdx[2 * (size_t)idx + 1] = in[prc[2 * idx]];
-}
 
 // ---------------------------------------------------------------------------
 // SpMV on the SELL-64 copy
@@ -1015,8 +1009,7 @@ template <typename T, int MODE>
 int launch_psweep(fcp_ctx *ctx, BlockSchedule &b, const T *in, T *out, int *ticket,
                   const SolveScalars *s) {
   const int n = ctx->n;
-  k_gather<T><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, b.prc.p, in, PBufs<T>::pdx(b).p, s);
-  PArgs<T> P{b.prc.p, b.pdep.p, PBufs<T>::pa(b).p, PBufs<T>::pdx(b).p, n};
+  PArgs<T> P{b.prc.p, b.pdep.p, PBufs<T>::pa(b).p, PBufs<T>::pdx(b).p, in, n};
   const int sptr_off = (int)(((size_t)b.R * sizeof(T) + 15) & ~(size_t)15);
   const size_t lds = (size_t)sptr_off + (size_t)(b.max_steps + 2) * sizeof(int);
   k_psweep<T, MODE><<<b.nblocks, BLK, lds, ctx->stream>>>(P, b.step_ptr.p, b.blk_step.p,
