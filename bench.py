@@ -373,7 +373,8 @@ def run_adjoint(args):
     flux, _ = workloads.convdiff_flux(mesh, geo, csr, ctx)
     inp = workloads.adjoint_inputs(mesh, geo, flux)
     dev = torch.device("cuda:0")
-    prob = ConvDiffProblem(ctx, mesh, geo, flux, inp["bc_kind"], K=inp["K"])
+    prob = ConvDiffProblem(ctx, mesh, geo, flux, inp["bc_kind"], K=inp["K"],
+                           log=lambda line: log(f"[adjoint] {line}"))
     P = {k: torch.from_numpy(np.ascontiguousarray(inp[k])).to(dev) for k in ("gamma", "source", "phib", "w")}
     vol = torch.from_numpy(geo["vol"]).to(dev)
     g = P["gamma"].clone().requires_grad_(True)

@@ -131,6 +131,10 @@ bool csr_ok_host(int n, int nnz, const int *ia, const int *ja, const int *diag) 
     const int d = diag[i];
     if (d < ia[i] || d >= ia[i + 1]) return false;
     if (ja[d - 1] != i + 1) return false;
+    // columns strictly ascending inside a row: the strict lower part is
+    // [ia(i), diag(i)) as the reference's sweeps and ilu_cr assume
+    for (int k = ia[i]; k + 1 < ia[i + 1]; ++k)
+      if (ja[k - 1] >= ja[k]) return false;
   }
   for (int k = 0; k < nnz; ++k)
     if (ja[k] < 1 || ja[k] > n) return false;
@@ -264,7 +268,7 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
         return fcp_fail(ctx, FCP_ERR_ARG, "sweep rows must be 0 or a multiple of 64 in [64, 32768]");
       if (value != ctx->tune_sweep_rows) {
         ctx->tune_sweep_rows = value;
-        if (ctx->have_csr) return fcp_build_block_schedules(ctx, fcp_block_rows(ctx, 8));
+        if (ctx->have_csr) return fcp_build_block_schedules(ctx);
       }
       return 0;
   }
@@ -760,13 +764,18 @@ int fcp_spmv(fcp_ctx *ctx, const double *x, double *y) {
 
 int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d) {
   FCP_TRY(need(ctx, true, false, true));
-  if (!r || !z || (kind != 0 && kind != 1)) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_precond_apply: bad argument");
+  if (!r || !z || kind < 0 || kind > 2)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_precond_apply: bad argument");
+  if (kind == 2) d = nullptr;  // ILU(0) has no pivot vector to return
   (void)hipSetDevice(ctx->device);
   Arg<double> ri, zo, dout;
   FCP_TRY(arg_in(ctx, r, ctx->n, ri));
   FCP_TRY(arg_out(ctx, z, ctx->n, zo));
   if (d) FCP_TRY(arg_out(ctx, d, ctx->n, dout));
-  FCP_TRY(fcp_precond_device(ctx, kind, ri.dev, zo.dev, dout.dev));
+  if (kind == 2)
+    FCP_TRY(fcp_ilu_apply_device(ctx, ri.dev, zo.dev));
+  else
+    FCP_TRY(fcp_precond_device(ctx, kind, ri.dev, zo.dev, dout.dev));
   FCP_TRY(arg_back(ctx, zo));
   FCP_TRY(arg_back(ctx, dout));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -780,7 +789,31 @@ static int solver_code(const char *s, int len) {
   if (name == "dpcg") return 1;
   if (name == "bicgstab") return 2;
   if (name == "gauss-seidel") return 3;
+  if (name == "pmgmres") return 4;
   return -1;
+}
+
+int fcp_spmv_transpose(fcp_ctx *ctx, const double *x, double *y) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!x || !y) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_spmv_transpose: null argument");
+  (void)hipSetDevice(ctx->device);
+  Arg<double> xi, yo;
+  FCP_TRY(arg_in(ctx, x, ctx->n, xi));
+  FCP_TRY(arg_out(ctx, y, ctx->n, yo));
+  FCP_TRY(fcp_atx_device(ctx, xi.dev, yo.dev));
+  FCP_TRY(arg_back(ctx, yo));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_ilu_factor(fcp_ctx *ctx, double *l) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!l) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_ilu_factor: null argument");
+  (void)hipSetDevice(ctx->device);
+  FCP_TRY(fcp_ensure_workspace(ctx));
+  FCP_TRY(fcp_ilu_factor_device(ctx));
+  FCP_TRY(out_copy(ctx, ctx->ilu.p, l, ctx->nnz));
+  return 0;
 }
 
 int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi, const double *rhs,
@@ -805,7 +838,8 @@ int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi, c
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (res0) *res0 = o->resor;
   if (verbose >= 1) {
-    static const char *names[] = {"PCG(IC0)", "PCG(Jacobi)", "BiCGStab(ILU(0))", "Gauss-Seidel"};
+    static const char *names[] = {"PCG(IC0)", "PCG(Jacobi)", "BiCGStab(ILU(0))", "Gauss-Seidel",
+                                  "PMGMRES_ILU(4)"};
     std::string var = chvar ? std::string(chvar, chvar + (chvar_len > 0 ? chvar_len : strlen(chvar)))
                             : std::string("x");
     while (!var.empty() && var.back() == ' ') var.pop_back();
@@ -850,7 +884,7 @@ static fcp_ctx *g_explicit_ctx = nullptr;
 
 static int explicit_solve(int code, int n, int nnz, const int *ia, const int *ja, const double *a,
                           const int *diag, double *fi, const double *rhs, double *resor,
-                          int itr_max, double tol_abs, double tol_rel, int verbose) {
+                          int itr_max, double tol_abs, double tol_rel, int verbose, int mr = 4) {
   std::lock_guard<std::mutex> lk(g_explicit_mu);
   if (n <= 0 || nnz <= 0 || !ia || !ja || !a || !diag || !fi || !rhs) return FCP_ERR_ARG;
   if (fcp_is_device_ptr(ia) || fcp_is_device_ptr(ja) || fcp_is_device_ptr(diag)) return FCP_ERR_ARG;
@@ -912,7 +946,9 @@ static int explicit_solve(int code, int n, int nnz, const int *ia, const int *ja
   Arg<double> x, b;
   FCP_TRY(arg_in(ctx, (const double *)fi, n, x));
   FCP_TRY(arg_in(ctx, rhs, n, b));
+  ctx->gm_restart = mr;
   const int rc = fcp_solve_device(ctx, code, x.dev, b.dev, itr_max, tol_abs, tol_rel, verbose, &o);
+  ctx->gm_restart = 4;
   if (rc < 0) return rc;
   FCP_TRY(arg_back(ctx, x));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -939,6 +975,12 @@ int fcp_gauss_seidel(int n, int nnz, const int *ia, const int *ja, const double 
                      const int *diag, double *fi, const double *rhs, double *resor, int itr_max,
                      double tol_abs, double tol_rel, int verbose) {
   return explicit_solve(3, n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, tol_abs, tol_rel, verbose);
+}
+int fcp_pmgmres(int n, int nnz, const int *ia, const int *ja, const double *a, const int *diag,
+                double *fi, const double *rhs, double *resor, int itr_max, int mr, double tol_abs,
+                double tol_rel, int verbose) {
+  return explicit_solve(4, n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, tol_abs, tol_rel,
+                        verbose, mr);
 }
 
 }  // extern "C"

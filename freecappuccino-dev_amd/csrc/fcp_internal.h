@@ -54,6 +54,7 @@ struct Schedule {
   DBuf<int> order;       // [n] rows in level order
   DBuf<int> task_ptr;    // [ntasks+1] offsets into order
   DBuf<int> lev;         // [n] level of each row
+  std::vector<int> lev_ptr;  // [nlev+1] host offsets of every level in order
 };
 
 // Row-block schedule of one triangular sweep: contiguous blocks of R rows,
@@ -92,7 +93,7 @@ struct SolveScalars {
   double sk, s0, pkapk, alf, bet, res0, resl, factor, resor, rsm;
   double l2, l2b, om, gam, beto, ukreso, vv, vr;
   double tol;  // per-column relative tolerance of a batched inner solve
-  int iter, done, status, pad;
+  int iter, done, status, conv;  // conv: the stopping test was met
 };
 
 struct ProfEntry {
@@ -163,12 +164,19 @@ struct fcp_ctx {
   SolveScalars *h_scal = nullptr;  // pinned mirror
   DBuf<int> errflag;         // [1]
 
+  // -------- restarted GMRES with ILU(0) --------
+  DBuf<double> ilu;       // [nnz] ILU(0) factors of `a` (ilu_cr layout)
+  uint64_t ilu_version = ~0ull;
+  DBuf<double> gm_v;      // [(mr+1)*n] Krylov basis, column-major
+  DBuf<char> gm_state;    // device GMRES scalars and Hessenberg matrix
+  int gm_restart = 4;     // restart length of solver "pmgmres" (csrsolve :85)
+
   // -------- least-squares gradient matrices --------
   DBuf<double> lsq_dmat;  // Dmat(9,n) or D(3,6,n)
   int lsq_method = 0, lsq_compat = 1;
 
   // -------- batched solves (interleaved vectors, NRP values per row) --------
-  std::map<int, std::unique_ptr<MultiSched>> msched;  // keyed by rows per block
+  std::map<long long, std::unique_ptr<MultiSched>> msched;  // keyed by rows per block (lo, up)
   DBuf<char> tpin;  // batched sweep inputs on the tile schedules
   DBuf<double> mv64[9];
   DBuf<float> mv32[8];
@@ -183,6 +191,7 @@ struct fcp_ctx {
   int tune_sweep_rows = 0;     // rows per block of the row-block sweeps (0 = auto)
   int tune_sweep_tile = 0;     // rows per geometric tile (0 = auto, 1 = no tiles)
   size_t lds_cap = 65536;      // LDS bytes available to one workgroup
+  int block_rows_cap[2] = {65536, 65536};  // rows per contiguous block (lower, upper sweep)
 
   // -------- profiling --------
   bool prof = false;
@@ -235,11 +244,13 @@ int fcp_build_cell_faces(fcp_ctx *ctx);
 int fcp_compute_geometry(fcp_ctx *ctx);
 int fcp_build_csr(fcp_ctx *ctx);
 int fcp_build_schedules(fcp_ctx *ctx);
-int fcp_build_block_schedules(fcp_ctx *ctx, int R);
+int fcp_build_block_schedules(fcp_ctx *ctx);
 // Rows per block for elements of `elem` bytes: the tuned value (FP64 rows;
 // doubled for FP32), or the largest multiple of 64 (<= 65536) whose block
 // fits the workgroup LDS limit with 4 KB left for the step table.
-inline int fcp_block_rows(const fcp_ctx *ctx, int elem = 8) {
+// The automatic value is further capped per sweep direction (upper = 1) by
+// the block size whose level steps pipeline best (chosen at schedule build).
+inline int fcp_block_rows(const fcp_ctx *ctx, int elem = 8, int upper = 0) {
   if (ctx->tune_sweep_rows > 0) {
     long long t = (long long)ctx->tune_sweep_rows * 8 / elem;
     t = (t / 64) * 64;
@@ -248,6 +259,7 @@ inline int fcp_block_rows(const fcp_ctx *ctx, int elem = 8) {
   long long r = ((long long)ctx->lds_cap - 4096) / elem;
   r = (r / 64) * 64;
   if (r > 65536) r = 65536;
+  if (r > ctx->block_rows_cap[upper ? 1 : 0]) r = ctx->block_rows_cap[upper ? 1 : 0];
   if (r < 64) r = 64;
   return (int)r;
 }
@@ -291,6 +303,13 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
                      double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts);
 int fcp_ensure_workspace(fcp_ctx *ctx);
 int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
+// fcp_gmres.hip: pmgmres_ilu (restart mr) on the resident matrix; ILU(0)
+// factors; y = A^T x
+int fcp_gmres_device(fcp_ctx *ctx, double *x, const double *b, int itr_max, int mr,
+                     double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts);
+int fcp_ilu_factor_device(fcp_ctx *ctx);
+int fcp_ilu_apply_device(fcp_ctx *ctx, const double *r, double *z);
+int fcp_atx_device(fcp_ctx *ctx, const double *x, double *y);
 // fcp_multi.hip: batched solve of nrhs columns (column-major, device memory)
 int fcp_solve_multi_device(fcp_ctx *ctx, int solver, int nrhs, double *fi, int ldfi,
                            const double *rhs, double *res0_out, int itr_max, double tol_abs,

@@ -175,6 +175,7 @@ __global__ __launch_bounds__(BLK) void k_gs_update(
       s->status = FCP_ERR_NONFINITE;
       conv = true;
     }
+    s->conv = (conv && s->status == 0) ? 1 : 0;
     if (conv || s->iter >= itr_max) s->done = 1;
   }
 }
@@ -659,6 +660,7 @@ __global__ __launch_bounds__(BLK) void k_xr_update(
       s->status = FCP_ERR_NONFINITE;
       conv = true;
     }
+    s->conv = (conv && s->status == 0) ? 1 : 0;
     if (conv || s->iter >= itr_max) s->done = 1;
   }
 }
@@ -1423,8 +1425,8 @@ int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, doubl
   return 0;
 }
 
-// solver: 0 iccg, 1 dpcg, 2 bicgstab, 3 gauss-seidel.  x, b device pointers
-// of length n.
+// solver: 0 iccg, 1 dpcg, 2 bicgstab, 3 gauss-seidel, 4 pmgmres (restart
+// ctx->gm_restart).  x, b device pointers of length n.
 int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max,
                      double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts) {
   FCP_TRY(fcp_ensure_workspace(ctx));
@@ -1439,6 +1441,28 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
   FCP_HIP(ctx, hipEventCreate(&e1));
   FCP_HIP(ctx, hipEventRecord(e0, ctx->stream));
   int rc;
+  if (solver == 4) {
+    // GMRES fills opts itself; the final true residual is added here.
+    rc = fcp_gmres_device(ctx, x, b, itr_max, ctx->gm_restart, tol_abs, tol_rel, verbose, opts);
+    if (rc >= 0) {
+      FCP_HIP(ctx, hipEventRecord(e1, ctx->stream));
+      SolveScalars *sf = ctx->scal.p + 2;
+      FCP_TRY(true_residual(ctx, x, b, sf));
+      FCP_HIP(ctx, hipMemcpyAsync(&ctx->h_scal[2], sf, sizeof(SolveScalars),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+      FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      float ms = 0.f;
+      FCP_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+      if (opts) {
+        const SolveScalars &F = ctx->h_scal[2];
+        opts->rel_res_l2 = F.l2 / (F.l2b > 0 ? F.l2b : 1.0);
+        opts->time_ms = ms;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+  }
   if (prec == FCP_PREC_MIXED)
     rc = solve_mixed(ctx, solver, x, b, itr_max, tol_abs, tol_rel, ce, verbose, opts);
   else
@@ -1477,7 +1501,7 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
     opts->time_ms = ms;
   }
   if (prec != FCP_PREC_MIXED && S.status < 0) return fcp_fail(ctx, S.status, "non-finite residual");
-  if (prec != FCP_PREC_MIXED && !S.done && S.res0 >= tol_abs) return iters > 0 ? iters : 1;
+  if (prec != FCP_PREC_MIXED && !S.conv && S.res0 >= tol_abs) return iters > 0 ? iters : 1;
   if (prec == FCP_PREC_MIXED && F.l2 > tol_rel * F.l2b && crit == FCP_CRIT_TRUE_L2)
     return iters > 0 ? iters : 1;
   return 0;

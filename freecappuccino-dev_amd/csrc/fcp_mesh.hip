@@ -441,10 +441,12 @@ static int build_one_schedule(fcp_ctx *ctx, int upper, Schedule &s) {
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<int> tp;
   tp.reserve(n / 64 + s.nlev + 1);
+  s.lev_ptr.assign(1, 0);
   int off = 0;
   for (int L = 0; L < s.nlev; ++L) {
     for (int k = 0; k < h_cnt[L]; k += 64) tp.push_back(off + k);
     off += h_cnt[L];
+    s.lev_ptr.push_back(off);
   }
   tp.push_back(off);
   s.ntasks = (int)tp.size() - 1;
@@ -543,7 +545,10 @@ int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, 
       prev_blk = blk;
       prev_lev = -1;
     }
-    if (lev[r] != prev_lev) {
+    // a new step at every level change, and after BLK rows of one level:
+    // rows of a level are independent, and a step the sweep kernels take
+    // with one row per thread is pipelined (wider steps are not)
+    if (lev[r] != prev_lev || i - sp.back() >= BLK) {
       sp.push_back(i);
       prev_lev = lev[r];
     }
@@ -596,30 +601,21 @@ int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, 
 
 // Geometric tiles: cells binned by centroid on a d0 x d1 x d2 grid of the
 // bounding box with about `target` cells per tile, tiles ranked by
-// (tx+ty+tz, tz, ty, tx).  Used only when every strict-lower dependency
+// (tx+ty+tz, tz, ty, tx).  Usable only when every strict-lower dependency
 // lies in a tile of lower or equal rank and every strict-upper one in a tile
 // of higher or equal rank (then the forward sweep's tickets, and the
-// reversed backward tickets, only ever wait on earlier tickets); otherwise
-// the sweeps keep contiguous row blocks.
-int fcp_build_tile_schedules(fcp_ctx *ctx) {
-  ctx->have_tiles = false;
-  ctx->msched.clear();
-  if (!ctx->have_geom || ctx->tune_sweep_tile == 1) return 0;
-  const int n = ctx->n;
-  const int target = ctx->tune_sweep_tile >= 64 ? ctx->tune_sweep_tile : 4096;
-  if (n < 4 * target) return 0;
-  std::vector<double> x(n), y(n), z(n);
-  std::vector<int> ia(n + 1), ja(ctx->nnz), dg(n);
-  FCP_HIP(ctx, hipMemcpyAsync(x.data(), ctx->xc.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
-  FCP_HIP(ctx, hipMemcpyAsync(y.data(), ctx->yc.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
-  FCP_HIP(ctx, hipMemcpyAsync(z.data(), ctx->zc.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
-  FCP_HIP(ctx, hipMemcpyAsync(ia.data(), ctx->ia.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
-  FCP_HIP(ctx, hipMemcpyAsync(ja.data(), ctx->ja.p, sizeof(int) * ctx->nnz, hipMemcpyDeviceToHost, ctx->stream));
-  FCP_HIP(ctx, hipMemcpyAsync(dg.data(), ctx->diag.p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
-  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+// reversed backward tickets, only ever wait on earlier tickets).  Returns
+// the number of tiles, 0 when the check fails.
+namespace {
+struct TileInput {
+  std::vector<double> x, y, z;
+  std::vector<int> ia, ja, dg;
+};
+
+int tile_ranks(const TileInput &in, int n, int target, std::vector<int> &rank) {
+  double lo[3] = {in.x[0], in.y[0], in.z[0]}, hi[3] = {in.x[0], in.y[0], in.z[0]};
   for (int i = 0; i < n; ++i) {
-    const double v[3] = {x[i], y[i], z[i]};
+    const double v[3] = {in.x[i], in.y[i], in.z[i]};
     for (int a = 0; a < 3; ++a) {
       lo[a] = v[a] < lo[a] ? v[a] : lo[a];
       hi[a] = v[a] > hi[a] ? v[a] : hi[a];
@@ -631,14 +627,15 @@ int fcp_build_tile_schedules(fcp_ctx *ctx) {
   const double ntile = (double)n / target;
   int d[3];
   for (int a = 0; a < 3; ++a) {
-    double k = vol > 0 ? ext[a] * std::cbrt(ntile / vol) : std::cbrt(ntile);
-    d[a] = k < 1.0 ? 1 : (int)std::lround(k);
+    // round up so tiles stay at or below the target on average
+    const double k = vol > 0 ? ext[a] * std::cbrt(ntile / vol) : std::cbrt(ntile);
+    d[a] = k < 1.0 ? 1 : (int)std::ceil(k - 1e-9);
   }
   const int nt = d[0] * d[1] * d[2];
   if (nt < 2) return 0;
   std::vector<int> tid(n);
   for (int i = 0; i < n; ++i) {
-    const double v[3] = {x[i], y[i], z[i]};
+    const double v[3] = {in.x[i], in.y[i], in.z[i]};
     int t[3];
     for (int a = 0; a < 3; ++a) {
       const double u = ext[a] > 0 ? (v[a] - lo[a]) / ext[a] : 0.0;
@@ -656,13 +653,15 @@ int fcp_build_tile_schedules(fcp_ctx *ctx) {
   };
   std::sort(tiles.begin(), tiles.end(), [&](int a, int b) { return key(a) < key(b); });
   for (int r = 0; r < nt; ++r) trank[tiles[r]] = r;
-  std::vector<int> rank(n);
+  rank.assign(n, 0);
   for (int i = 0; i < n; ++i) rank[i] = trank[tid[i]];
+  // acyclic only if every lower dependency has a rank <= its row's, every
+  // upper dependency a rank >= it
   for (int i = 0; i < n; ++i) {
-    for (int k = ia[i]; k < dg[i]; ++k)
-      if (rank[ja[k]] > rank[i]) return 0;
-    for (int k = dg[i] + 1; k < ia[i + 1]; ++k)
-      if (rank[ja[k]] < rank[i]) return 0;
+    for (int k = in.ia[i]; k < in.dg[i]; ++k)
+      if (rank[in.ja[k]] > rank[i]) return 0;
+    for (int k = in.dg[i] + 1; k < in.ia[i + 1]; ++k)
+      if (rank[in.ja[k]] < rank[i]) return 0;
   }
   // compact away empty tiles (keep rank order)
   std::vector<int> used(nt, 0), remap(nt, -1);
@@ -671,26 +670,112 @@ int fcp_build_tile_schedules(fcp_ctx *ctx) {
   for (int r = 0; r < nt; ++r)
     if (used[r]) remap[r] = m++;
   for (int i = 0; i < n; ++i) rank[i] = remap[rank[i]];
+  return m;
+}
+}  // namespace
+
+int fcp_build_tile_schedules(fcp_ctx *ctx) {
+  ctx->have_tiles = false;
+  ctx->msched.clear();
+  if (!ctx->have_geom || ctx->tune_sweep_tile == 1) return 0;
+  const int n = ctx->n;
+  const int target = ctx->tune_sweep_tile >= 64 ? ctx->tune_sweep_tile : 4096;
+  if (n < 4 * target) return 0;
+  TileInput in;
+  in.x.resize(n);
+  in.y.resize(n);
+  in.z.resize(n);
+  in.ia.resize(n + 1);
+  in.ja.resize(ctx->nnz);
+  in.dg.resize(n);
+  FCP_HIP(ctx, hipMemcpyAsync(in.x.data(), ctx->xc.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(in.y.data(), ctx->yc.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(in.z.data(), ctx->zc.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(in.ia.data(), ctx->ia.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(in.ja.data(), ctx->ja.p, sizeof(int) * ctx->nnz, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(in.dg.data(), ctx->diag.p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<int> rank;
+  const int m = tile_ranks(in, n, target, rank);
+  if (m == 0) return 0;
   FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, 0, 0, ctx->tlower, rank.data(), m));
   FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, 0, 1, ctx->tupper, rank.data(), m));
   ctx->have_tiles = true;
   return 0;
 }
 
-int fcp_build_block_schedules(fcp_ctx *ctx, int R) {
+int fcp_build_block_schedules(fcp_ctx *ctx) {
   ctx->msched.clear();
-  FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, R, 0, ctx->blower));
-  FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, R, 1, ctx->bupper));
-  const int R4 = fcp_block_rows(ctx, 4);
-  FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, R4, 0, ctx->blower4));
-  FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, R4, 1, ctx->bupper4));
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, fcp_block_rows(ctx, 8, 0), 0, ctx->blower));
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, fcp_block_rows(ctx, 8, 1), 1, ctx->bupper));
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, fcp_block_rows(ctx, 4, 0), 0, ctx->blower4));
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, fcp_block_rows(ctx, 4, 1), 1, ctx->bupper4));
+  return 0;
+}
+
+// Estimated critical path, in pipelined steps, of one sweep over contiguous
+// blocks of R rows: blocks run concurrently and a block's step for a level
+// follows its steps for the levels before, so the path is the sum over
+// levels of the widest block's sub-steps at that level (ceil(rows/BLK)).
+static long long sweep_path_steps(const std::vector<int> &lev, int nlev, int n, int R,
+                                  std::vector<int> &cnt, std::vector<int> &widest) {
+  std::fill(widest.begin(), widest.end(), 0);
+  for (int b0 = 0; b0 < n; b0 += R) {
+    const int b1 = n < b0 + R ? n : b0 + R;
+    for (int i = b0; i < b1; ++i) ++cnt[lev[i]];
+    for (int i = b0; i < b1; ++i) {
+      const int L = lev[i];
+      if (cnt[L] == 0) continue;
+      const int sub = (cnt[L] + BLK - 1) / BLK;
+      widest[L] = sub > widest[L] ? sub : widest[L];
+      cnt[L] = 0;
+    }
+  }
+  long long steps = 0;
+  for (int L = 0; L < nlev; ++L) steps += widest[L];
+  return steps;
+}
+
+// Rows per contiguous block of one triangle: the largest candidate (Rmax
+// halved down to 256 rows) whose estimated path is within 10% of the best.
+// Numberings with wide level fronts (RCM) need small blocks, lexicographic
+// ones do not.
+static int choose_block_rows(const std::vector<int> &lev, int nlev, int n, int Rmax) {
+  std::vector<int> cand;
+  for (int R = Rmax; R >= 256; R = ((R / 2) / 64) * 64) cand.push_back(R);
+  if (cand.empty()) return Rmax;
+  std::vector<int> cnt(nlev > 0 ? nlev : 1, 0), widest(nlev > 0 ? nlev : 1, 0);
+  std::vector<long long> cost;
+  long long best = -1;
+  for (int R : cand) {
+    cost.push_back(sweep_path_steps(lev, nlev, n, R, cnt, widest));
+    if (best < 0 || cost.back() < best) best = cost.back();
+  }
+  for (size_t k = 0; k < cand.size(); ++k)
+    if (cost[k] * 10 <= best * 11) return cand[k];
+  return cand.back();
+}
+
+static int set_block_rows_caps(fcp_ctx *ctx) {
+  const int n = ctx->n;
+  std::vector<int> lev(n);
+  const long long rmax = ((long long)ctx->lds_cap - 4096) / 4 / 64 * 64;
+  const int Rmax = (int)(rmax > 65536 ? 65536 : (rmax < 64 ? 64 : rmax));
+  for (int u = 0; u < 2; ++u) {
+    const Schedule &s = u ? ctx->upper : ctx->lower;
+    FCP_HIP(ctx, hipMemcpyAsync(lev.data(), s.lev.p, sizeof(int) * n, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->block_rows_cap[u] = choose_block_rows(lev, s.nlev, n, Rmax);
+  }
   return 0;
 }
 
 int fcp_build_schedules(fcp_ctx *ctx) {
   FCP_TRY(build_one_schedule(ctx, 0, ctx->lower));
   FCP_TRY(build_one_schedule(ctx, 1, ctx->upper));
-  FCP_TRY(fcp_build_block_schedules(ctx, fcp_block_rows(ctx, 8)));
+  FCP_TRY(set_block_rows_caps(ctx));
+  FCP_TRY(fcp_build_block_schedules(ctx));
   FCP_TRY(fcp_build_tile_schedules(ctx));
   return 0;
 }

@@ -368,6 +368,7 @@ __global__ __launch_bounds__(BLK) void mk_xr_update(
         S.status = FCP_ERR_NONFINITE;
         conv = true;
       }
+      S.conv = (conv && S.status == 0) ? 1 : 0;
       if (conv || S.iter >= itr_max) S.done = 1;
     }
   }
@@ -882,13 +883,15 @@ int msched_for(fcp_ctx *ctx, BlockSchedule **lo, BlockSchedule **up, DBuf<char> 
     *pin = &ctx->tpin;
     return 0;
   }
-  const int R = fcp_block_rows(ctx, NRP * (int)sizeof(T));
-  auto it = ctx->msched.find(R);
+  const int Rlo = fcp_block_rows(ctx, NRP * (int)sizeof(T), 0);
+  const int Rup = fcp_block_rows(ctx, NRP * (int)sizeof(T), 1);
+  const long long key = ((long long)Rlo << 20) | Rup;
+  auto it = ctx->msched.find(key);
   if (it == ctx->msched.end()) {
     std::unique_ptr<MultiSched> m(new MultiSched());
-    FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, R, 0, m->lo));
-    FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, R, 1, m->up));
-    it = ctx->msched.emplace(R, std::move(m)).first;
+    FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, Rlo, 0, m->lo));
+    FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, Rup, 1, m->up));
+    it = ctx->msched.emplace(key, std::move(m)).first;
   }
   MultiSched *m = it->second.get();
   if (!sched_fits(ctx, m->lo, m->up, NRP * sizeof(T))) return 0;
@@ -1036,7 +1039,7 @@ int miterate(fcp_ctx *ctx, SolveScalars *s_dev, SolveScalars *h, int itr_max, in
 }
 
 struct MultiResult {
-  int iters[MAXNR], outer;
+  int iters[MAXNR], conv[MAXNR], outer;
   double res0[MAXNR], resl[MAXNR], resor[MAXNR];
 };
 
@@ -1063,7 +1066,10 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
                     ctx->mv64[4].p, ctx->mv64[5].p, ctx->mv64[6].p, ctx->mv64[7].p};
   Sell<double> m64;
   FCP_TRY(sell_view64(ctx, &m64));
-  for (int c = 0; c < MAXNR; ++c) res.iters[c] = 0;
+  for (int c = 0; c < MAXNR; ++c) {
+    res.iters[c] = 0;
+    res.conv[c] = 1;  // FP64 solves record their stopping test below
+  }
   res.outer = 0;
   auto residual = [&](double *reso) {
     ProfScope ps(ctx, "mresidual_f64");
@@ -1096,6 +1102,7 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
       res.iters[c] = h[c].iter;
       res.resl[c] = h[c].resl;
       res.resor[c] = h[c].resor;
+      res.conv[c] = (h[c].conv || h[c].res0 < tol_abs) ? 1 : 0;
       if (h[c].status < 0) return fcp_fail(ctx, h[c].status, "non-finite residual (column %d)", c);
     }
     res.outer = 1;
@@ -1279,6 +1286,7 @@ int fcp_solve_multi_device(fcp_ctx *ctx, int solver, int nrhs, double *fi, int l
     worst_it = R.iters[c] > worst_it ? R.iters[c] : worst_it;
     if (res0_out) res0_out[c] = prec == FCP_PREC_MIXED ? R.res0[c] : R.resor[c];
     if (crit == FCP_CRIT_TRUE_L2 && rel > tol_rel) conv = false;
+    if (!R.conv[c]) conv = false;
   }
   if (opts) {
     opts->iterations = worst_it;

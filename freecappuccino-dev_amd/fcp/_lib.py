@@ -40,6 +40,8 @@ SIGNATURES = {
     "fcp_matrix_set": (_i, [_p, _p]),
     "fcp_spmv": (_i, [_p, _p, _p]),
     "fcp_precond_apply": (_i, [_p, _i, _p, _p, _p]),
+    "fcp_spmv_transpose": (_i, [_p, _p, _p]),
+    "fcp_ilu_factor": (_i, [_p, _p]),
     "fcp_csrsolve": (_i, [_p, C.c_char_p, _i, _p, _p, C.POINTER(_d), _i, _d, _d, C.c_char_p, _i,
                           _i, C.POINTER(SolveOpts)]),
     "fcp_rcm": (_i, [_p, _i, _p]),
@@ -55,6 +57,7 @@ SIGNATURES = {
     "fcp_dpcg": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _d, _d, _i]),
     "fcp_bicgstab": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _d, _d, _i]),
     "fcp_gauss_seidel": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _d, _d, _i]),
+    "fcp_pmgmres": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _i, _d, _d, _i]),
     "fcp_profile": (_i, [_p, _i]),
     "fcp_profile_report": (_i, [_p, C.c_char_p, _i]),
     "fcp_stream": (_p, [_p]),

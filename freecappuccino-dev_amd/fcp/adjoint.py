@@ -9,7 +9,8 @@ phi_b of the Dirichlet boundary faces):
         g_{k-1} = Gauss gradient of phi_{k-1}             (fcp_grad_gauss)
         A, b    = facefluxsc assembly with Gamma, phi_{k-1}, g_{k-1},
                   b = s V + face and boundary terms        (fcp_convdiff, urf 1)
-        phi_k   = A^{-1} b         (FP64 DILU-BiCGStab, ||b-Ax||/||b|| <= 1e-13)
+        phi_k   = A^{-1} b         (FP64 DILU-BiCGStab, ||b-Ax||/||b|| <= 1e-13,
+                                    then one FP64 refinement pass)
     J = sum_i w_i phi_K,i V_i
 
 A depends only on Gamma and the mass flux, so every pass shares it.  The
@@ -37,10 +38,12 @@ class ConvDiffProblem:
     """One mesh on one device with a fixed mass flux and boundary kinds."""
 
     def __init__(self, ctx: Context, mesh, geo, flux, bc_kind, K: int = 2, gam: float = 1.0,
-                 scheme: int = 1, tol: float = 1e-13, itr_max: int = 20000,
-                 device: str = "cuda:0"):
+                 scheme: int = 1, tol: float = 1e-13, itr_max: int = 5000,
+                 device: str = "cuda:0", log=None, refine: int = 1):
         self.ctx, self.mesh = ctx, mesh
         self.K, self.gam, self.scheme, self.tol, self.itr_max = K, gam, scheme, tol, itr_max
+        self.refine = refine
+        self.log = log  # callable taking one line per linear solve, or None
         dev = torch.device(device)
         self.dev = dev
         T = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
@@ -57,13 +60,41 @@ class ConvDiffProblem:
         phi[self.n:] = torch.where(self.dirichlet, phib, phi_int[self.own_b])
         return phi
 
-    def _solve(self, rhs):
+    def _solve_once(self, rhs, tol):
+        """e = A^{-1} rhs to ||rhs - A e||/||rhs|| <= tol.  The reference's
+        BiCGStab guards its denominators with an absolute 1e-20
+        (linear_solvers.f90:652,747), which stalls it once the squared norms
+        of its vectors approach that value; rhs is therefore scaled by a power
+        of two to a norm of about 2^20 (exact in binary arithmetic) and the
+        solution scaled back."""
+        nrm = float(torch.linalg.vector_norm(rhs))
+        k = 20 - int(np.floor(np.log2(nrm))) if nrm > 0 else 0
         x = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
-        rc, info = self.ctx.csrsolve("bicgstab", x, rhs, itr_max=self.itr_max, tol_abs=0.0,
-                                     tol_rel=self.tol, precision=FCP_PREC_FP64,
-                                     criterion=FCP_CRIT_TRUE_L2)
+        rc, info = self.ctx.csrsolve("bicgstab", x, rhs * (2.0 ** k),
+                                     itr_max=self.itr_max, tol_abs=0.0, tol_rel=tol,
+                                     precision=FCP_PREC_FP64, criterion=FCP_CRIT_TRUE_L2)
+        if self.log is not None:
+            self.log(f"solve: {info['iterations']} iterations, rel. residual "
+                     f"{info['rel_res_l2']:.2e}, {info['time_ms']:.0f} ms")
         if rc != 0:
             raise RuntimeError(f"adjoint path: solve did not converge ({info})")
+        return x * (2.0 ** -k)
+
+    def _solve(self, rhs):
+        """x = A^{-1} rhs (the resident matrix: A, or A^T during backward):
+        one solve to tol, then `refine` FP64 refinement passes that solve for
+        the correction of the true residual to 1e-6 of it.  Central
+        differences with step h resolve the sensitivities only to about
+        (solution error)/h, so the solutions must be accurate far below the
+        Krylov tolerance."""
+        x = self._solve_once(rhs, self.tol)
+        y = torch.empty_like(x)
+        for _ in range(self.refine):
+            self.ctx.spmv(x, y)
+            r = rhs - y
+            if float(torch.linalg.vector_norm(r)) == 0.0:
+                break
+            x = x + self._solve_once(r, 1e-6)
         return x
 
     def forward(self, gamma, s, phib):

@@ -260,13 +260,30 @@ class Context:
         self._check(self.lib.fcp_spmv(self.h, _ptr(_f64(x))[0], _ptr(y)[0]), "fcp_spmv")
         return y
 
+    def spmv_transpose(self, x, y=None):
+        """y = A^T x (atx_cr)."""
+        if y is None:
+            y = np.zeros(self.mesh.n_cells)
+        self._check(self.lib.fcp_spmv_transpose(self.h, _ptr(_f64(x))[0], _ptr(y)[0]),
+                    "fcp_spmv_transpose")
+        return y
+
     def precond_apply(self, r, kind: int = 0):
+        """z = M^{-1} r: kind 0 DIC, 1 DILU (returns (z, inverted pivots)),
+        2 ILU(0) of pmgmres_ilu (returns (z, None))."""
         n = self.mesh.n_cells
         z = np.zeros(n)
-        d = np.zeros(n)
-        rc = self.lib.fcp_precond_apply(self.h, kind, _ptr(_f64(r))[0], _ptr(z)[0], _ptr(d)[0])
+        d = np.zeros(n) if kind != 2 else None
+        rc = self.lib.fcp_precond_apply(self.h, kind, _ptr(_f64(r))[0], _ptr(z)[0],
+                                        _ptr(d)[0] if d is not None else None)
         self._check(rc, "fcp_precond_apply")
         return z, d
+
+    def ilu_factor(self, nnz):
+        """ILU(0) factors of the resident matrix (ilu_cr layout, CSR order)."""
+        l = np.zeros(nnz)
+        self._check(self.lib.fcp_ilu_factor(self.h, _ptr(l)[0]), "fcp_ilu_factor")
+        return l
 
     def csrsolve(self, solver: str, fi, rhs, itr_max=1000, tol_abs=0.0, tol_rel=1e-10,
                  chvar="phi", verbose=0, precision=FCP_PREC_FP64, criterion=FCP_CRIT_REF_L1,
@@ -332,9 +349,11 @@ class Context:
         return out
 
 
-def explicit_solve(solver: str, ia, ja, a, diag, fi, rhs, itr_max, tol_abs, tol_rel, verbose=0):
-    """iccg/dpcg/bicgstab(n,nnz,ia,ja,a,diag,fi,rhs,resor,...) with 0-based
-    ia/ja/diag (converted to the 1-based C-ABI).  fi updated in place."""
+def explicit_solve(solver: str, ia, ja, a, diag, fi, rhs, itr_max, tol_abs, tol_rel, verbose=0,
+                   mr=4):
+    """iccg/dpcg/bicgstab/GaussSeidel/pmgmres_ilu(n,nnz,ia,ja,a,diag,fi,rhs,
+    resor,...) with 0-based ia/ja/diag (converted to the 1-based C-ABI).  fi
+    updated in place; mr is the restart length of pmgmres."""
     lib = _lib.load()
     n = len(diag)
     ia1 = np.ascontiguousarray(np.asarray(ia) + 1, np.int32)
@@ -343,10 +362,16 @@ def explicit_solve(solver: str, ia, ja, a, diag, fi, rhs, itr_max, tol_abs, tol_
     a = np.ascontiguousarray(a, np.float64)
     rhs = np.ascontiguousarray(rhs, np.float64)
     resor = C.c_double()
-    fn = {"iccg": lib.fcp_iccg, "dpcg": lib.fcp_dpcg, "bicgstab": lib.fcp_bicgstab,
-          "gauss-seidel": lib.fcp_gauss_seidel, "gauss_seidel": lib.fcp_gauss_seidel}[solver]
-    rc = fn(n, len(ja1), _ptr(ia1)[0], _ptr(ja1)[0], _ptr(a)[0], _ptr(dg1)[0], _ptr(fi)[0],
-            _ptr(rhs)[0], C.byref(resor), int(itr_max), float(tol_abs), float(tol_rel), int(verbose))
+    if solver == "pmgmres":
+        rc = lib.fcp_pmgmres(n, len(ja1), _ptr(ia1)[0], _ptr(ja1)[0], _ptr(a)[0], _ptr(dg1)[0],
+                             _ptr(fi)[0], _ptr(rhs)[0], C.byref(resor), int(itr_max), int(mr),
+                             float(tol_abs), float(tol_rel), int(verbose))
+    else:
+        fn = {"iccg": lib.fcp_iccg, "dpcg": lib.fcp_dpcg, "bicgstab": lib.fcp_bicgstab,
+              "gauss-seidel": lib.fcp_gauss_seidel, "gauss_seidel": lib.fcp_gauss_seidel}[solver]
+        rc = fn(n, len(ja1), _ptr(ia1)[0], _ptr(ja1)[0], _ptr(a)[0], _ptr(dg1)[0], _ptr(fi)[0],
+                _ptr(rhs)[0], C.byref(resor), int(itr_max), float(tol_abs), float(tol_rel),
+                int(verbose))
     if rc < 0:
         raise FcpError(f"fcp_{solver} failed with {rc}")
     return rc, resor.value

@@ -192,6 +192,18 @@ module fcp_capi
       real(c_double) :: r(*), z(*), d(*)
     end function
 
+    integer(c_int) function fcp_spmv_transpose(ctx, x, y) bind(C, name='fcp_spmv_transpose')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: x(*), y(*)
+    end function
+
+    integer(c_int) function fcp_ilu_factor(ctx, l) bind(C, name='fcp_ilu_factor')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: l(*)
+    end function
+
     integer(c_int) function fcp_csrsolve(ctx, solver, solver_len, fi, rhs, res0, itr_max, &
         tol_abs, tol_rel, chvar, chvar_len, verbose, opts) bind(C, name='fcp_csrsolve')
       import :: c_int, c_double, c_ptr, c_char, fcp_solve_opts
@@ -245,6 +257,15 @@ module fcp_capi
         tol_abs, tol_rel, verbose) bind(C, name='fcp_gauss_seidel')
       import :: c_int, c_double
       integer(c_int), value :: n, nnz, itr_max, verbose
+      integer(c_int) :: ia(*), ja(*), diag(*)
+      real(c_double) :: a(*), fi(*), rhs(*), resor
+      real(c_double), value :: tol_abs, tol_rel
+    end function
+
+    integer(c_int) function fcp_pmgmres(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, mr, &
+        tol_abs, tol_rel, verbose) bind(C, name='fcp_pmgmres')
+      import :: c_int, c_double
+      integer(c_int), value :: n, nnz, itr_max, mr, verbose
       integer(c_int) :: ia(*), ja(*), diag(*)
       real(c_double) :: a(*), fi(*), rhs(*), resor
       real(c_double), value :: tol_abs, tol_rel

@@ -10,6 +10,8 @@
 !                              linear_solvers.f90:40
 !   iccg / dpcg / bicgstab / GaussSeidel(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max,
 !                          tol_abs, tol_rel, chvar, verbose)  linear_solvers.f90:364,206,548,96
+!   pmgmres_ilu(n, nz_num, ia, ja, a, ua, x, rhs, resor, itr_max, mr, tol_abs, tol_rel,
+!               chvar, verbose)                               linear_solvers.f90:789
 ! The mesh is handed over once with fcp_read_mesh (the arrays read_mesh
 ! produces, geometry.f90:118-406); geometry is computed on the GPU and
 ! returned in the module arrays below (geometry.f90:19-83 names).
@@ -188,6 +190,25 @@ contains
     rc = fcp_gauss_seidel(n, nz, ia_, ja_, a_, diag_, fi, rhs, resor, itr_max, tol_abs, tol_rel, vb)
     if (rc < 0) then
       write(*,'(3a,i0)') '  Gauss-Seidel (', trim(chvar), ') failed: ', rc
+      stop 1
+    endif
+  end subroutine
+
+  subroutine pmgmres_ilu(n, nz_num, ia_, ja_, a_, ua, x, rhs, resor, itr_max, mr, tol_abs, &
+                         tol_rel, chvar, verbose)
+    integer, intent(in) :: n, nz_num, itr_max, mr
+    integer, intent(in) :: ia_(n+1), ja_(nz_num), ua(n)
+    real(dp), intent(in) :: a_(nz_num), rhs(n), tol_abs, tol_rel
+    real(dp), intent(inout) :: x(n)
+    real(dp), intent(out) :: resor
+    character(len=*), intent(in) :: chvar
+    logical, intent(in) :: verbose
+    integer(c_int) :: rc, vb
+    vb = 0
+    if (verbose) vb = 2
+    rc = fcp_pmgmres(n, nz_num, ia_, ja_, a_, ua, x, rhs, resor, itr_max, mr, tol_abs, tol_rel, vb)
+    if (rc < 0) then
+      write(*,'(3a,i0)') '  PMGMRES_ILU (', trim(chvar), ') failed: ', rc
       stop 1
     endif
   end subroutine

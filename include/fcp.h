@@ -245,17 +245,37 @@ int fcp_matrix_set(fcp_ctx *ctx, const double *a);
  * matvec of iccg :493-498).  x, y of length nCells. */
 int fcp_spmv(fcp_ctx *ctx, const double *x, double *y);
 
-/* One application z = M^{-1} r of the DIC (kind 0, iccg :439-475) or DILU
- * (kind 1, bicgstab :613-683) preconditioner of the resident matrix, with
- * the reference's operation order; d (optional, [n]) receives the inverted
- * pivots.  r, z of length nCells. */
+/* y = A^T x with the resident matrix (atx_cr, linear_solvers.f90:1079-1171),
+ * summed in the reference's row order.  Needs a symmetric sparsity pattern
+ * (every finite-volume matrix has one), else FCP_ERR_UNSUPPORTED. */
+int fcp_spmv_transpose(fcp_ctx *ctx, const double *x, double *y);
+
+/* One application z = M^{-1} r of the DIC (kind 0, iccg :439-475), DILU
+ * (kind 1, bicgstab :613-683) or ILU(0) (kind 2, ilu_cr + lus_cr
+ * :1267-1477, the preconditioner of pmgmres_ilu) preconditioner of the
+ * resident matrix, with the reference's operation order; d (optional, [n])
+ * receives the inverted pivots of kinds 0 and 1 and is ignored for kind 2
+ * (fcp_ilu_factor returns those factors).  r, z of length nCells. */
 int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d);
 
+/* ILU(0) factors of the resident matrix as ilu_cr leaves them
+ * (linear_solvers.f90:1267-1385): l[nnz] in CSR order, multipliers left of
+ * the diagonal, U from it on.  FCP_ERR_BREAKDOWN on a zero pivot (the
+ * reference stops there). */
+int fcp_ilu_factor(fcp_ctx *ctx, double *l);
+
 /* csrsolve (linear_solvers.f90:40-91) on the resident matrix: solver is
- * "iccg" | "dpcg" | "bicgstab" | "gauss-seidel"; fi[numTotal] (only 1..n
- * used), rhs[n], res0 receives the reference's report value (resor).
- * opts may be NULL (FP64, reference criterion).  verbose >= 1 prints the
- * reference's report line, verbose >= 2 per-iteration residuals. */
+ * "iccg" | "dpcg" | "bicgstab" | "gauss-seidel" | "pmgmres"; fi[numTotal]
+ * (only 1..n used), rhs[n], res0 receives the reference's report value
+ * (resor).  opts may be NULL (FP64, reference criterion).  verbose >= 1
+ * prints the reference's report line, verbose >= 2 per-iteration residuals.
+ * "pmgmres" is GMRES(4) with ILU(0) (pmgmres_ilu :789-1077, restart fixed
+ * at 4 as :85 does; itr_max counts restart cycles), FP64 only.  With
+ * FCP_CRIT_REF_L1 it reproduces the reference, including its stopping test
+ * resl/resor <= resor*tol_rel and its normalisation of the preconditioned
+ * residual by the unpreconditioned norm (:970-978), which can stop early
+ * on a wrong solution; FCP_CRIT_TRUE_L2 normalises by the preconditioned
+ * norm and stops on the true relative L2 residual at a restart. */
 int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi,
                  const double *rhs, double *res0, int itr_max, double tol_abs,
                  double tol_rel, const char *chvar, int chvar_len, int verbose,
@@ -292,6 +312,12 @@ int fcp_bicgstab(int n, int nnz, const int *ia, const int *ja, const double *a,
 int fcp_gauss_seidel(int n, int nnz, const int *ia, const int *ja, const double *a,
                      const int *diag, double *fi, const double *rhs, double *resor,
                      int itr_max, double tol_abs, double tol_rel, int verbose);
+/* pmgmres_ilu(n,nz_num,ia,ja,a,ua,x,rhs,resor,itr_max,mr,tol_abs,tol_rel,
+ * chvar,verbose) (linear_solvers.f90:789): restart length mr (1..32, < n),
+ * itr_max restart cycles, reference stopping test. */
+int fcp_pmgmres(int n, int nnz, const int *ia, const int *ja, const double *a,
+                const int *diag, double *fi, const double *rhs, double *resor,
+                int itr_max, int mr, double tol_abs, double tol_rel, int verbose);
 
 /* --- launch-shape tuning (autotuner hook) --------------------------------- */
 #define FCP_TUNE_SWEEP_WAVES 1   /* resident waves of a sweep kernel (0 = auto) */

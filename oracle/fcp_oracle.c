@@ -804,6 +804,176 @@ int orc_gauss_seidel(const orc_csr *c, const double *a, double *fi,
   return itr_used;
 }
 
+/* ------------------------------------------------------------------------ */
+/* GMRES(m) with ILU(0): linear_solvers.f90:789-1555                        */
+/* ------------------------------------------------------------------------ */
+void orc_atx(const orc_csr *c, const double *a, const double *x, double *y) {
+  const int n = c->n;
+  for (int i = 0; i < n; ++i) y[i] = 0.0;                       /* :1162 */
+  for (int i = 0; i < n; ++i)                                    /* :1164-1168 */
+    for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) y[c->ja[k]] = y[c->ja[k]] + a[k] * x[i];
+}
+
+int orc_ilu0(const orc_csr *c, const double *a, double *l) {
+  const int n = c->n, nnz = c->nnz;
+  int *iw = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  int *ua = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  if (!iw || !ua) {
+    free(iw);
+    free(ua);
+    return -1;
+  }
+  memcpy(l, a, sizeof(double) * (size_t)nnz);                    /* :1332 */
+  memcpy(ua, c->diag, sizeof(int) * (size_t)n);
+  /* iw(1:n) = -1 per row (:1338) is kept as "entries of row i only": set
+   * before the row and cleared after it, which reads the same values. */
+  for (int i = 0; i < n; ++i) iw[i] = -1;
+  int rc = 0;
+  for (int i = 0; i < n && rc == 0; ++i) {
+    for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) iw[c->ja[k]] = k;  /* :1340-1342 */
+    int j = c->ia[i], jrow = -1;
+    for (; j < c->ia[i + 1]; ++j) {                               /* :1344-1357 */
+      jrow = c->ja[j];
+      if (i <= jrow) break;
+      const double tl = l[j] * l[ua[jrow]];
+      l[j] = tl;
+      for (int jj = ua[jrow] + 1; jj < c->ia[jrow + 1]; ++jj) {
+        const int jw = iw[c->ja[jj]];
+        if (jw != -1) l[jw] = l[jw] - tl * l[jj];
+      }
+    }
+    ua[i] = j;                                                    /* :1359 */
+    if (jrow != i || l[j] == 0.0) rc = -(i + 1);                  /* :1361-1376 */
+    else l[j] = 1.0 / l[j];                                       /* :1378 */
+    for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) iw[c->ja[k]] = -1;
+  }
+  if (rc == 0)
+    for (int i = 0; i < n; ++i) l[ua[i]] = 1.0 / l[ua[i]];        /* :1382 */
+  free(iw);
+  free(ua);
+  return rc;
+}
+
+void orc_ilu_apply(const orc_csr *c, const double *l, const double *r, double *z) {
+  const int n = c->n;
+  double *w = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  if (!w) return;
+  for (int i = 0; i < n; ++i) w[i] = r[i];                        /* :1453 */
+  for (int i = 1; i < n; ++i)                                     /* :1457-1461 */
+    for (int j = c->ia[i]; j < c->diag[i]; ++j) w[i] = w[i] - l[j] * w[c->ja[j]];
+  for (int i = n - 1; i >= 0; --i) {                              /* :1465-1470 */
+    for (int j = c->diag[i] + 1; j < c->ia[i + 1]; ++j) w[i] = w[i] - l[j] * w[c->ja[j]];
+    w[i] = w[i] / l[c->diag[i]];
+  }
+  for (int i = 0; i < n; ++i) z[i] = w[i];                        /* :1474 */
+  free(w);
+}
+
+/* mult_givens (:1548-1552) on g(k), g(k+1), 0-based k. */
+static void givens(double cs, double sn, int k, double *g) {
+  const double g1 = cs * g[k] - sn * g[k + 1];
+  const double g2 = sn * g[k] + cs * g[k + 1];
+  g[k] = g1;
+  g[k + 1] = g2;
+}
+
+int orc_pmgmres_ilu(const orc_csr *c, const double *a, double *x, const double *rhs,
+                    int itr_max, int mr, double tol_abs, double tol_rel, double *hist,
+                    orc_solve_info *info) {
+  const int n = c->n;
+  const int ld = mr + 1;
+  const double delta = 1.0e-3;                                    /* :903 */
+  double *r = (double *)calloc((size_t)n, sizeof(double));
+  double *v = (double *)calloc((size_t)n * (size_t)(mr + 1), sizeof(double));
+  double *l = (double *)calloc((size_t)c->nnz, sizeof(double));
+  double *h = (double *)calloc((size_t)ld * (size_t)mr, sizeof(double));
+  double *cs = (double *)calloc((size_t)ld, sizeof(double));
+  double *sn = (double *)calloc((size_t)ld, sizeof(double));
+  double *g = (double *)calloc((size_t)ld, sizeof(double));
+  double *y = (double *)calloc((size_t)ld, sizeof(double));
+  int itr_used = 0, k_copy = 0, rc;
+  double rho_tol = 0.0, resor = 0.0, resl = 0.0;
+#define H(i, k) h[(i) + ld * (k)]
+  rc = orc_ilu0(c, a, l);                                         /* :947 */
+  if (rc < 0) goto done;
+  for (int itr = 1; itr <= itr_max; ++itr) {
+    orc_spmv(c, a, x, r);                                         /* :959-960 */
+    for (int i = 0; i < n; ++i) r[i] = rhs[i] - r[i];
+    if (itr == 1) {                                               /* :962-968 */
+      resor = l1(r, n);
+      rho_tol = resor * tol_rel;
+    }
+    const double rho = sqrt(dot(r, r, n));                        /* :970 */
+    if (rho == 0.0) break;  /* guard: see the header */
+    orc_ilu_apply(c, l, r, r);                                    /* :976 */
+    for (int i = 0; i < n; ++i) v[i] = r[i] / rho;                /* :978 */
+    g[0] = rho;                                                   /* :980-983 */
+    for (int i = 1; i <= mr; ++i) g[i] = 0.0;
+    for (int i = 0; i < ld * mr; ++i) h[i] = 0.0;
+    for (int k = 0; k < mr; ++k) {                                /* :985-1043 */
+      k_copy = k + 1;
+      double *vk = v + (size_t)k * n, *w = v + (size_t)(k + 1) * n;
+      orc_spmv(c, a, vk, w);
+      orc_ilu_apply(c, l, w, w);
+      const double av = sqrt(dot(w, w, n));
+      for (int j = 0; j <= k; ++j) {
+        const double *vj = v + (size_t)j * n;
+        H(j, k) = dot(w, vj, n);
+        for (int i = 0; i < n; ++i) w[i] = w[i] - vj[i] * H(j, k);
+      }
+      H(k + 1, k) = sqrt(dot(w, w, n));
+      if (av + delta * H(k + 1, k) == av) {                       /* :1002-1009 */
+        for (int j = 0; j <= k; ++j) {
+          const double *vj = v + (size_t)j * n;
+          const double htmp = dot(w, vj, n);
+          H(j, k) = H(j, k) + htmp;
+          for (int i = 0; i < n; ++i) w[i] = w[i] - htmp * vj[i];
+        }
+        H(k + 1, k) = sqrt(dot(w, w, n));
+      }
+      if (H(k + 1, k) != 0.0)                                     /* :1011-1013 */
+        for (int i = 0; i < n; ++i) w[i] = w[i] / H(k + 1, k);
+      if (k > 0) {                                                /* :1015-1021 */
+        for (int i = 0; i <= k + 1; ++i) y[i] = H(i, k);
+        for (int j = 0; j < k; ++j) givens(cs[j], sn[j], j, y);
+        for (int i = 0; i <= k + 1; ++i) H(i, k) = y[i];
+      }
+      const double mu = sqrt(H(k, k) * H(k, k) + H(k + 1, k) * H(k + 1, k));
+      cs[k] = H(k, k) / mu;                                       /* :1023-1029 */
+      sn[k] = -H(k + 1, k) / mu;
+      H(k, k) = cs[k] * H(k, k) - sn[k] * H(k + 1, k);
+      H(k + 1, k) = 0.0;
+      givens(cs[k], sn[k], k, g);
+      resl = fabs(g[k + 1]);                                      /* :1031-1033 */
+      if (hist) hist[itr_used] = resl;
+      itr_used++;
+      if (resl / resor <= rho_tol || resl <= tol_abs) break;      /* :1039-1041 */
+    }
+    const int m = k_copy - 1;                                     /* :1045-1051 */
+    y[m] = g[m] / H(m, m);
+    for (int i = m - 1; i >= 0; --i) {
+      double s = 0.0;
+      for (int q = i + 1; q <= m; ++q) s = s + H(i, q) * y[q];
+      y[i] = (g[i] - s) / H(i, i);
+    }
+    for (int i = 0; i < n; ++i) {                                 /* :1053-1055 */
+      double s = 0.0;
+      for (int q = 0; q <= m; ++q) s = s + v[(size_t)q * n + i] * y[q];
+      x[i] = x[i] + s;
+    }
+    if (resl / resor <= rho_tol || resl <= tol_abs) break;        /* :1057-1059 */
+  }
+#undef H
+  rc = itr_used;
+done:
+  if (info) {
+    info->resor = resor; info->res0 = resor; info->resl = resl;
+    info->factor = 1.0; info->iters = itr_used;
+  }
+  free(r); free(v); free(l); free(h); free(cs); free(sn); free(g); free(y);
+  return rc;
+}
+
 int orc_levels(const orc_csr *c, int upper, int *lev) {
   const int n = c->n;
   int nlev = 0;

@@ -168,6 +168,34 @@ void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
 void orc_nonorth_correction(const orc_mesh *m, const orc_geom *g, const double *gamma,
                             const double *grad, double *su);
 
+/* y = A^T x by the row-order scatter of atx_cr (linear_solvers.f90:1079-1171):
+ * y = 0, then for every row i in ascending order y[ja[k]] += a[k]*x[i]. */
+void orc_atx(const orc_csr *c, const double *a, const double *x, double *y);
+
+/* ILU(0) factors of ilu_cr (linear_solvers.f90:1267-1385) into l[nnz]: the
+ * strict lower part holds the multipliers, the rest U; every pivot is
+ * inverted at the end of its row and inverted back at the end (:1378,1382),
+ * as the reference does.  Rows must be sorted by column.  Returns 0, or
+ * -(i+1) when row i has no diagonal or a zero pivot (the reference stops). */
+int orc_ilu0(const orc_csr *c, const double *a, double *l);
+
+/* z = (LU)^{-1} r of lus_cr (:1386-1477, sweeps :1457-1470): unit-lower forward sweep, then
+ * the backward sweep divided by the pivot.  z may alias r. */
+void orc_ilu_apply(const orc_csr *c, const double *l, const double *r, double *z);
+
+/* Restarted GMRES(mr) with the ILU(0) preconditioner, pmgmres_ilu
+ * (:789-1077; csrsolve passes mr = 4 at :85).  itr_max counts restart
+ * cycles.  hist (may be NULL, capacity itr_max*mr) receives resl after every
+ * inner step.  info: resor = res0 = L1 norm of the first residual, resl =
+ * the last estimate |g(k+1)|, iters = inner steps (itr_used).  Stopping
+ * test as the reference writes it: resl/resor <= resor*tol_rel or
+ * resl <= tol_abs.  One guard departs from the reference: a zero initial
+ * residual returns at once (the reference divides by it and fills x with
+ * NaN).  Returns the inner steps, or the orc_ilu0 error. */
+int orc_pmgmres_ilu(const orc_csr *c, const double *a, double *x, const double *rhs,
+                    int itr_max, int mr, double tol_abs, double tol_rel, double *hist,
+                    orc_solve_info *info);
+
 /* Reference constants (parameters.f90:6-7 widened from default REAL). */
 double orc_small(void);
 double orc_s0_init(void);

@@ -216,9 +216,9 @@ class Oracle:
         return z
 
     def solve(self, solver: str, a, rhs, fi0=None, itr_max=1000, tol_abs=0.0,
-              tol_rel=1e-10, history=False):
+              tol_rel=1e-10, history=False, mr=4):
         return solve_csr(self.cc, solver, a, rhs, fi0, itr_max, tol_abs, tol_rel, history,
-                         self.mesh.n_cells)
+                         self.mesh.n_cells, mr)
 
 
 _SOLVERS = {"iccg": "orc_iccg", "dpcg": "orc_dpcg", "bicgstab": "orc_bicgstab",
@@ -232,16 +232,60 @@ def csr_struct(ia, ja, diag):
     return cc, keep
 
 
-def solve_csr(cc, solver, a, rhs, fi0, itr_max, tol_abs, tol_rel, history, n):
+def solve_csr(cc, solver, a, rhs, fi0, itr_max, tol_abs, tol_rel, history, n, mr=4):
     fi = np.zeros(n) if fi0 is None else np.array(fi0, dtype=np.float64, copy=True)
-    hist = np.zeros(max(itr_max, 1))
     info = _Info()
-    fn = getattr(lib(), _SOLVERS[solver])
-    fn(C.byref(cc), _p(np.ascontiguousarray(a, np.float64)), _p(fi),
-       _p(np.ascontiguousarray(rhs, np.float64)), int(itr_max), C.c_double(tol_abs),
-       C.c_double(tol_rel), _p(hist), C.byref(info))
+    a = np.ascontiguousarray(a, np.float64)
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    if solver == "pmgmres":
+        # csrsolve hard-codes the restart length mr = 4 (linear_solvers.f90:85)
+        hist = np.zeros(max(itr_max * mr, 1))
+        rc = lib().orc_pmgmres_ilu(C.byref(cc), _p(a), _p(fi), _p(rhs), int(itr_max), int(mr),
+                                   C.c_double(tol_abs), C.c_double(tol_rel), _p(hist),
+                                   C.byref(info))
+        assert rc >= 0, f"ILU(0) failed at row {-rc - 1}"
+    else:
+        hist = np.zeros(max(itr_max, 1))
+        fn = getattr(lib(), _SOLVERS[solver])
+        fn(C.byref(cc), _p(a), _p(fi), _p(rhs), int(itr_max), C.c_double(tol_abs),
+           C.c_double(tol_rel), _p(hist), C.byref(info))
     out = dict(fi=fi, iters=info.iters, resor=info.resor, res0=info.res0, resl=info.resl,
                factor=info.factor)
     if history:
         out["hist"] = hist[: info.iters].copy()
     return out
+
+
+# -- explicit-CSR helpers (0-based ia/ja/diag) -----------------------------
+def ilu0(ia, ja, diag, a):
+    """ILU(0) factors of ilu_cr (linear_solvers.f90:1267-1385)."""
+    cc, keep = csr_struct(ia, ja, diag)
+    l = np.zeros(len(keep[1]))
+    rc = lib().orc_ilu0(C.byref(cc), _p(np.ascontiguousarray(a, np.float64)), _p(l))
+    assert rc == 0, f"ILU(0) failed at row {-rc - 1}"
+    return l
+
+
+def ilu_apply(ia, ja, diag, l, r):
+    """z = (LU)^{-1} r of lus_cr (linear_solvers.f90:1386-1477)."""
+    cc, keep = csr_struct(ia, ja, diag)
+    z = np.zeros(len(keep[0]) - 1)
+    lib().orc_ilu_apply(C.byref(cc), _p(np.ascontiguousarray(l, np.float64)),
+                        _p(np.ascontiguousarray(r, np.float64)), _p(z))
+    return z
+
+
+def atx(ia, ja, diag, a, x):
+    """y = A^T x of atx_cr (linear_solvers.f90:1079-1171)."""
+    cc, keep = csr_struct(ia, ja, diag)
+    y = np.zeros(len(keep[0]) - 1)
+    lib().orc_atx(C.byref(cc), _p(np.ascontiguousarray(a, np.float64)),
+                  _p(np.ascontiguousarray(x, np.float64)), _p(y))
+    return y
+
+
+def solve_explicit(solver, ia, ja, diag, a, rhs, fi0=None, itr_max=1000, tol_abs=0.0,
+                   tol_rel=1e-10, history=False, mr=4):
+    cc, keep = csr_struct(ia, ja, diag)
+    return solve_csr(cc, solver, a, rhs, fi0, itr_max, tol_abs, tol_rel, history,
+                     len(keep[0]) - 1, mr)

@@ -66,6 +66,8 @@ program ref_driver
       call job_convdiff(ju)
     case ('solve_explicit')
       call job_solve_explicit(ju)
+    case ('ilu')
+      call job_ilu(ju)
     case default
       read(ju, *) cmd
       write(*,*) 'ref_driver: unknown job ', trim(cmd)
@@ -240,6 +242,8 @@ contains
       call bicgstab(n, nz, eia, eja, ea, edg, fi, rhs, resor, itr_max, tol_abs, tol_rel, 'x', .false.)
     case ('gauss-seidel')
       call GaussSeidel(n, nz, eia, eja, ea, edg, fi, rhs, resor, itr_max, tol_abs, tol_rel, 'x', .false.)
+    case ('pmgmres')
+      call pmgmres_ilu(n, nz, eia, eja, ea, edg, fi, rhs, resor, itr_max, 4, tol_abs, tol_rel, 'x', .false.)
     end select
     call wr8(fout, fi, n)
     open(newunit=u, file=trim(finfo), status='replace')
@@ -249,6 +253,30 @@ contains
     allocate(res(numCells), reso(numCells), pk(numCells), zk(numCells), d(numCells), &
              uk(numCells), vk(numCells))
     deallocate(eia, eja, edg, ea, fi, rhs)
+  end subroutine
+
+  ! ILU(0) factors (ilu_cr), one preconditioner application (lus_cr) and
+  ! the transposed product (atx_cr) of an explicit CSR matrix.
+  subroutine job_ilu(ju)
+    integer, intent(in) :: ju
+    character(len=64) :: c, fia, fja, fdg, fa, fr, fl, fz, fw
+    integer :: n, nz, u
+    integer, allocatable :: eia(:), eja(:), edg(:)
+    real(dp), allocatable :: ea(:), el(:), r(:), z(:), w(:)
+    read(ju, *) c, n, nz, fia, fja, fdg, fa, fr, fl, fz, fw
+    allocate(eia(n+1), eja(nz), edg(n), ea(nz), el(nz), r(n), z(n), w(n))
+    open(newunit=u, file=trim(fia), access='stream', form='unformatted', status='old'); read(u) eia; close(u)
+    open(newunit=u, file=trim(fja), access='stream', form='unformatted', status='old'); read(u) eja; close(u)
+    open(newunit=u, file=trim(fdg), access='stream', form='unformatted', status='old'); read(u) edg; close(u)
+    call rd8(fa, ea, nz)
+    call rd8(fr, r, n)
+    call ilu_cr(n, nz, eia, eja, ea, edg, el)
+    call lus_cr(n, nz, eia, eja, el, edg, r, z)
+    call atx_cr(n, nz, eia, eja, ea, r, w)
+    call wr8(fl, el, nz)
+    call wr8(fz, z, n)
+    call wr8(fw, w, n)
+    deallocate(eia, eja, edg, ea, el, r, z, w)
   end subroutine
 
   ! Steady convection-diffusion assembly with the reference flux routines,

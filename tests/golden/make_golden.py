@@ -1,7 +1,7 @@
 """Writes the golden fixtures in tests/golden/ from the flang-built reference.
 
 Run in the build container only (needs /root/reference and flang):
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]     (default: every case)
 The reference routines are compiled from the reference sources by
 oracle/ref/Makefile into oracle/_ref/ and driven by oracle/ref/ref_driver.f90.
 Fixtures hold inputs and the reference's outputs; large integer arrays are
@@ -191,9 +191,86 @@ def kat_5x5_case():
     print("wrote kat5x5")
 
 
+def _report(log, tag):
+    """(iterations, final residual) of the last solver report line with tag."""
+    line = [ln for ln in log.splitlines() if tag in ln][-1]
+    its = int(line.split("No Iterations")[-1])
+    resl = float(line.split("Final residual =")[-1].split(",")[0])
+    return its, resl
+
+
+def gmres_case():
+    """pmgmres_ilu, ilu_cr, lus_cr and atx_cr (linear_solvers.f90:789-1555) on
+    the hex5j Laplacian (SPD) and linearUpwind convection-diffusion (non-
+    symmetric) matrices, the two dense 5x5 systems, and the 32^3 Poisson."""
+    hx = np.load(os.path.join(HERE, "hex5j.npz"))
+    kat = np.load(os.path.join(HERE, "kat5x5.npz"))
+    out = {}
+    rc = RefCase(hex_mesh(2))
+    systems = {
+        "lap": (hx["c_ia"], hx["c_ja"], hx["c_diag"], hx["lap_a"], hx["in_b"]),
+        "cd": (hx["c_ia"], hx["c_ja"], hx["c_diag"], hx["cd_linearUpwind_a"], hx["in_b"]),
+        "ns": (kat["ia"], kat["ja"], kat["diag"], kat["A_ns"].reshape(-1), kat["b_ns"]),
+        "sp": (kat["ia"], kat["ja"], kat["diag"], kat["A_sp"].reshape(-1), kat["b_sp"]),
+    }
+    for key, (ia, ja, dg, a, b) in systems.items():
+        n, nz = len(ia) - 1, len(ja)
+        (ia + 1).astype(np.int32).tofile(os.path.join(rc.dir, "eia.bin"))
+        (ja + 1).astype(np.int32).tofile(os.path.join(rc.dir, "eja.bin"))
+        (dg + 1).astype(np.int32).tofile(os.path.join(rc.dir, "edg.bin"))
+        rc.put("ea.bin", a)
+        rc.put("eb.bin", b)
+        rc.put("x0.bin", np.zeros(n))
+        rc.run([f"ilu {n} {nz} eia.bin eja.bin edg.bin ea.bin eb.bin el.bin ez.bin ew.bin"])
+        out[f"{key}_ilu"] = rc.get("el.bin")
+        out[f"{key}_lus_b"] = rc.get("ez.bin")
+        out[f"{key}_atx_b"] = rc.get("ew.bin")
+        for itmax in (1, 2, 5, 20, 500):
+            log = rc.run([f"solve_explicit pmgmres {n} {nz} {itmax} 0.0 1.0d-10 eia.bin eja.bin "
+                          f"edg.bin ea.bin x0.bin eb.bin ex.bin einfo.txt"])
+            out[f"{key}_gm_{itmax}"] = rc.get("ex.bin")
+            out[f"{key}_gm_{itmax}_resor"] = np.array(
+                float(open(os.path.join(rc.dir, "einfo.txt")).read()))
+            its, resl = _report(log, "PMGMRES_ILU")
+            out[f"{key}_gm_{itmax}_iters"] = np.array(its)
+            out[f"{key}_gm_{itmax}_resl"] = np.array(resl)
+    rc.close()
+    # 32^3 Poisson through csrsolve('pmgmres'), mr = 4 (:85)
+    mesh = workloads.poisson_uniform_mesh(32)
+    rc = RefCase(mesh)
+    g, c, sizes = rc.dump()
+    n = mesh.n_cells
+    inp = workloads.poisson_uniform_inputs(mesh, g["vol"])
+    rc.put("mu.bin", inp["mu"])
+    rc.put("phi.bin", inp["phi"])
+    rc.put("su0.bin", inp["rhs"])
+    rc.put("fi0.bin", np.zeros(n))
+    rc.run(["laplacian mu.bin phi.bin su0.bin a.bin su.bin"])
+    out["p32_rhs"] = rc.get("su.bin")
+    out["p32_sha_a"] = np.array(sha(rc.get("a.bin")))
+    for itmax in (3, 3000):
+        log = rc.run([f"solve pmgmres {itmax} 0.0 1.0d-10 a.bin fi0.bin su.bin fi.bin info.txt"])
+        out[f"p32_gm_{itmax}"] = rc.get("fi.bin")
+        out[f"p32_gm_{itmax}_resor"] = np.array(
+            float(open(os.path.join(rc.dir, "info.txt")).read()))
+        its, resl = _report(log, "PMGMRES_ILU")
+        out[f"p32_gm_{itmax}_iters"] = np.array(its)
+        out[f"p32_gm_{itmax}_resl"] = np.array(resl)
+    rc.close()
+    np.savez_compressed(os.path.join(HERE, "gmres.npz"), **out)
+    print("wrote gmres", int(out["p32_gm_3000_iters"]))
+
+
+CASES = {
+    "hex5j": lambda: small_case(
+        hex_mesh(5, jitter=0.2, seed=2,
+                 types=("wall", "symmetry", "outlet", "wall", "inlet", "pressure")), "hex5j"),
+    "cavity": cavity_case,
+    "kat5x5": kat_5x5_case,
+    "poisson32": poisson32_case,
+    "gmres": gmres_case,  # reads hex5j.npz and kat5x5.npz
+}
+
 if __name__ == "__main__":
-    small_case(hex_mesh(5, jitter=0.2, seed=2,
-                        types=("wall", "symmetry", "outlet", "wall", "inlet", "pressure")), "hex5j")
-    cavity_case()
-    kat_5x5_case()
-    poisson32_case()
+    for name in (sys.argv[1:] or list(CASES)):
+        CASES[name]()

@@ -130,7 +130,8 @@ def test_multi_reference_criterion_matches_reference_trajectory(api):
     o = Oracle(mesh)
     for it in (1, 3, 10):
         X = np.zeros((3, mesh.n_cells))
-        ctx.csrsolve_multi("bicgstab", X, rhs, itr_max=it, tol_rel=1e-30)
+        rc, info, _ = ctx.csrsolve_multi("bicgstab", X, rhs, itr_max=it, tol_rel=1e-30)
+        assert rc == it  # tol_rel 1e-30 is never met: not converged, iterations used
         for r in range(3):
             ref = o.solve("bicgstab", a, rhs[r], itr_max=it, tol_abs=0.0, tol_rel=1e-30)["fi"]
             assert np.allclose(X[r], ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max()), (it, r)
