@@ -35,6 +35,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweeps_spmv_128cube_hbm_traffic.md")
+
+
+def measured_traffic(kernel, N):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy of known size),
+    which covers the 128^3 workload only; None otherwise."""
+    if N != 128 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            t = line.split()
+            if len(t) >= 3 and t[0] == "traffic" and t[1] == kernel:
+                return float(t[2])
+    return None
+
+
 def alg_bytes(name, n, nnz, Fi, Fb):
     """Algorithmic (compulsory) HBM bytes of one launch of a named kernel."""
     lower = Fi  # strict-lower entries = inner faces
@@ -160,7 +177,8 @@ def run_gpu(args):
     b = alg_bytes(name, n, nnz, Fi, Fb)
     roof = dict(bound="hbm", kernel=name, achieved=(b / avg_s / 1e9) if b else None,
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=(b / avg_s / 1e9 / HBM_PEAK_GBS) if b else None,
-                traffic=None, alg_bytes_per_launch=b, avg_launch_us=avg_s * 1e6)
+                traffic=measured_traffic(name, N), alg_bytes_per_launch=b,
+                avg_launch_us=avg_s * 1e6)
 
     # standalone SpMV FP64 (reference row order) bandwidth
     xs = torch.rand(n, dtype=torch.float64, device=dev)

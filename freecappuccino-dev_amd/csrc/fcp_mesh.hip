@@ -627,9 +627,8 @@ int tile_ranks(const TileInput &in, int n, int target, std::vector<int> &rank) {
   const double ntile = (double)n / target;
   int d[3];
   for (int a = 0; a < 3; ++a) {
-    // round up so tiles stay at or below the target on average
     const double k = vol > 0 ? ext[a] * std::cbrt(ntile / vol) : std::cbrt(ntile);
-    d[a] = k < 1.0 ? 1 : (int)std::ceil(k - 1e-9);
+    d[a] = k < 1.0 ? 1 : (int)std::lround(k);
   }
   const int nt = d[0] * d[1] * d[2];
   if (nt < 2) return 0;
