@@ -918,15 +918,13 @@ static int explicit_solve(int code, int n, int nnz, const int *ia, const int *ja
     FCP_HIP(ctx, hipGetLastError());
     FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
-  FCP_TRY(fcp_build_schedules(ctx));
-  FCP_TRY(fcp_build_sell(ctx));
-  ctx->have_csr = true;
-  FCP_TRY(fcp_matrix_set(ctx, a));
-  // transposed-entry positions for DILU
+  // transposed-entry positions for DILU; a missing one marks the pattern
+  // non-symmetric (the sweeps then publish every row)
   {
     std::vector<int> hia(n + 1), hja(nnz), ht(nnz);
     for (int i = 0; i <= n; ++i) hia[i] = ia[i] - 1;
     for (int k = 0; k < nnz; ++k) hja[k] = ja[k] - 1;
+    bool sym = true;
     for (int i = 0; i < n; ++i)
       for (int k = hia[i]; k < hia[i + 1]; ++k) {
         const int j = hja[k];
@@ -937,10 +935,17 @@ static int explicit_solve(int code, int n, int nnz, const int *ia, const int *ja
             break;
           }
         ht[k] = l;
+        if (l < 0) sym = false;
       }
+    ctx->sym_pattern = sym;
     FCP_HIP(ctx, hipMemcpyAsync(ctx->tslot.p, ht.data(), sizeof(int) * nnz, hipMemcpyHostToDevice,
                                 ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
+  FCP_TRY(fcp_build_schedules(ctx));
+  FCP_TRY(fcp_build_sell(ctx));
+  ctx->have_csr = true;
+  FCP_TRY(fcp_matrix_set(ctx, a));
   fcp_solve_opts o;
   fcp_solve_opts_default(&o);
   Arg<double> x, b;

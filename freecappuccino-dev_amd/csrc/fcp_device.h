@@ -110,8 +110,15 @@ __device__ __forceinline__ T wait_value(T *p, int *err, int slp) {
   const unsigned long long t0 = wall_clock64();
   unsigned spins = 0;
   while (v == B::sent) {
-    if (slp >= 8)
+    // s_sleep n idles the wave for about 64*n clocks between polls
+    if (slp >= 32)
+      __builtin_amdgcn_s_sleep(32);
+    else if (slp >= 16)
+      __builtin_amdgcn_s_sleep(16);
+    else if (slp >= 8)
       __builtin_amdgcn_s_sleep(8);
+    else if (slp >= 4)
+      __builtin_amdgcn_s_sleep(4);
     else if (slp >= 2)
       __builtin_amdgcn_s_sleep(2);
     else
@@ -133,6 +140,54 @@ __device__ __forceinline__ void publish(T *p, T v) {
   __hip_atomic_store(reinterpret_cast<typename B::U *>(p), u, __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// ---------------------------------------------------------------------------
+// raw buffer accesses: 32-bit byte offsets from one SGPR resource; an offset
+// at or beyond the resource's size reads 0 and issues no memory request, so
+// lanes that need nothing can take part in a load without diverging
+// ---------------------------------------------------------------------------
+constexpr int CPOL_SC1 = 16;        // cache-policy bit of agent-coherent accesses
+constexpr int FCP_OOB = 0x7FFFFF00;  // an offset beyond every resource (bytes < FCP_OOB)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
+}
+
+template <int BYTES>
+struct RawVec;
+template <>
+struct RawVec<4> {
+  typedef unsigned int U;
+  template <int CP>
+  static __device__ __forceinline__ U load(__amdgpu_buffer_rsrc_t rs, int off) {
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, CP);
+  }
+  static __device__ __forceinline__ void store(U v, __amdgpu_buffer_rsrc_t rs, int off) {
+    __builtin_amdgcn_raw_buffer_store_b32(v, rs, off, 0, CPOL_SC1);
+  }
+};
+template <>
+struct RawVec<8> {
+  typedef unsigned int U __attribute__((ext_vector_type(2)));
+  template <int CP>
+  static __device__ __forceinline__ U load(__amdgpu_buffer_rsrc_t rs, int off) {
+    return __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, CP);
+  }
+  static __device__ __forceinline__ void store(U v, __amdgpu_buffer_rsrc_t rs, int off) {
+    __builtin_amdgcn_raw_buffer_store_b64(v, rs, off, 0, CPOL_SC1);
+  }
+};
+template <>
+struct RawVec<16> {
+  typedef unsigned int U __attribute__((ext_vector_type(4)));
+  template <int CP>
+  static __device__ __forceinline__ U load(__amdgpu_buffer_rsrc_t rs, int off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, CP);
+  }
+  static __device__ __forceinline__ void store(U v, __amdgpu_buffer_rsrc_t rs, int off) {
+    __builtin_amdgcn_raw_buffer_store_b128(v, rs, off, 0, CPOL_SC1);
+  }
+};
 
 // ---------------------------------------------------------------------------
 // SELL-64 copy of the operator

@@ -138,6 +138,7 @@ struct fcp_ctx {
   BlockSchedule tlower, tupper;      // geometric-tile blocks (when acyclic)
   bool have_tiles = false;
   DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)
+  bool sym_pattern = true;  // every entry has its transposed entry
   // SELL-64 copy of the operator for SpMV: rows in slices of 64, each slice
   // stored column-major (entry k of the slice's rows contiguous) and padded
   // to the slice's longest row; per-row entry order = CSR order.

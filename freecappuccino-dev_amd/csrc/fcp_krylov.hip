@@ -264,17 +264,17 @@ __global__ __launch_bounds__(BLK) void k_bsweep(
 // fetches them PD steps ahead into a ring of registers; the values of
 // dependencies in earlier blocks are fetched PD2 steps ahead (re-polled at
 // use time only if still unpublished).  Steps are separated by a workgroup
-// barrier; a thread handles at most one position per step in the pipelined
-// path and walks any further positions of an oversized step directly.
-// Prefetch loads are issued unconditionally (an unused slot reads position 0
-// or the row's own entry and is flagged invalid) so the compiler's
-// memory-counter waits stay exact:
-// a load inside a divergent branch makes every later wait drain all
-// outstanding prefetches.
+// barrier; a thread handles one position per step (the schedule splits
+// steps wider than the workgroup).  Prefetch loads are issued
+// unconditionally (an unused slot reads position 0, or an out-of-range
+// buffer offset, and is flagged invalid) so the compiler's memory-counter
+// waits stay exact: a load inside a divergent branch makes every later wait
+// drain all outstanding prefetches.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct PSlot {
   int row, cnt, ok, pos;
+  int pubo;  // byte offset of the row in the output for an external row, else FCP_OOB
   T dv, xin;
   int dep[FCP_PMAXD];
   T av[FCP_PMAXD];
@@ -293,11 +293,12 @@ struct PArgs {
 // bytes of coefficients and the pivot of every position); the row's input
 // value is read in the second prefetch stage, once the row id has arrived.
 template <typename T>
-__device__ __forceinline__ void p_fetch(const PArgs<T> &P, int idx, int &row, int &cnt, T &dv,
-                                        int *dep, T *av) {
+__device__ __forceinline__ void p_fetch(const PArgs<T> &P, int idx, int &row, int &cnt, int &ext,
+                                        T &dv, int *dep, T *av) {
   const int2 rc = reinterpret_cast<const int2 *>(P.prc)[idx];
   row = rc.x;
-  cnt = rc.y;
+  cnt = rc.y & 0xFFFF;
+  ext = rc.y >> 16;
   const int4 dp = reinterpret_cast<const int4 *>(P.pdep)[idx];
   dep[0] = dp.x;
   dep[1] = dp.y;
@@ -332,20 +333,25 @@ __device__ __forceinline__ void p_load1(PSlot<T> &S, const PArgs<T> &P, const in
   const bool ok = (step < ns) && (idx < sptr[sc + 1]);
   S.ok = ok ? 1 : 0;
   S.pos = ok ? idx : 0;
-  p_fetch<T>(P, ok ? idx : 0, S.row, S.cnt, S.dv, S.dep, S.av);
+  int ext;
+  p_fetch<T>(P, ok ? idx : 0, S.row, S.cnt, ext, S.dv, S.dep, S.av);
+  S.pubo = (ok && ext) ? S.row * (int)sizeof(T) : FCP_OOB;
 }
 
-// idle: a row private to this lane, read by slots without a remote
-// dependency so idle lanes spread over memory instead of sharing an address
+// Second prefetch stage, once the row and dependency ids have arrived: the
+// row's input value, and the values of dependencies in earlier blocks with
+// agent-coherent buffer loads; lanes without such a dependency pass an
+// out-of-range offset and issue no memory request.
 template <typename T>
-__device__ __forceinline__ void p_load2(PSlot<T> &S, const PArgs<T> &P, T *out, int idle) {
-  using B = Bits<T>;
+__device__ __forceinline__ void p_load2(PSlot<T> &S, const PArgs<T> &P,
+                                        __amdgpu_buffer_rsrc_t rout) {
+  using R = RawVec<sizeof(T)>;
   S.xin = P.in[S.row];
 #pragma unroll
   for (int m = 0; m < FCP_PMAXD; ++m) {
     const bool need = S.ok && m < S.cnt && S.dep[m] < 0;
-    typename B::U *q = reinterpret_cast<typename B::U *>(out + (need ? (-S.dep[m] - 1) : idle));
-    S.rv[m] = B::from(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const int off = need ? (-S.dep[m] - 1) * (int)sizeof(T) : FCP_OOB;
+    S.rv[m] = __builtin_bit_cast(T, R::template load<CPOL_SC1>(rout, off));
   }
 }
 
@@ -370,44 +376,68 @@ __device__ __forceinline__ T p_row_value(int cnt, T dv, T xin, const int *dep, c
   return acc * dv;
 }
 
+// One position of a step.  Fast path without branches: every dependency is
+// read from LDS (slot 0 for remote ones) and the prefetched remote value is
+// selected where the dependency lies in an earlier block; unused dependency
+// slots are masked by a select, so the sums are formed in the same order as
+// p_row_value.  Only a lane whose prefetched remote value was still
+// unpublished takes p_row_value, which waits for it.  Steps never hold more
+// positions than the workgroup has threads (the schedule splits them).
 template <typename T, int MODE>
-__device__ __forceinline__ void p_compute(const PSlot<T> &S, const PArgs<T> &P, const int *sptr,
-                                          int step, int ns, int lo, T *zl, T *out, int *err,
-                                          int slp) {
-  if (S.ok) {
-    const T val = p_row_value<T, MODE>(S.cnt, S.dv, S.xin, S.dep, S.av, S.rv, zl, out, err, slp);
-    zl[S.pos - lo] = val;
-    publish<T>(out + S.row, val);
+__device__ __forceinline__ void p_compute(const PSlot<T> &S, int lo, T *zl, int *zr, T *out,
+                                          __amdgpu_buffer_rsrc_t rout, int *err, int slp) {
+  using B = Bits<T>;
+  if (!S.ok) return;
+  T acc = (MODE == 0) ? S.xin : S.xin / (S.dv + (T)FCP_SMALL);
+  unsigned miss = 0;  // bitwise flags: no short-circuit control flow
+#pragma unroll
+  for (int m = 0; m < FCP_PMAXD; ++m) {
+    const int d = S.dep[m];
+    const T lv = zl[d > 0 ? d : 0];
+    const bool use = m < S.cnt;
+    const bool remote = d < 0;
+    miss |= (unsigned)use & (unsigned)remote & (unsigned)(B::to(S.rv[m]) == B::sent);
+    const T v = remote ? S.rv[m] : lv;
+    const T na = acc - S.av[m] * v;
+    acc = use ? na : acc;
   }
-  if (step >= ns) return;
-  // positions beyond one per thread (oversized steps): direct path
-  const int e = sptr[step + 1];
-  for (int idx = sptr[step] + (int)threadIdx.x + BLK; idx < e; idx += BLK) {
-    int row, cnt, dep[FCP_PMAXD];
-    T dv, av[FCP_PMAXD];
-    p_fetch<T>(P, idx, row, cnt, dv, dep, av);
-    const T xin = P.in[row];
-    const T val = p_row_value<T, MODE>(cnt, dv, xin, dep, av, nullptr, zl, out, err, slp);
-    zl[idx - lo] = val;
-    publish<T>(out + row, val);
-  }
+  T val = acc * S.dv;
+  if (miss) val = p_row_value<T, MODE>(S.cnt, S.dv, S.xin, S.dep, S.av, S.rv, zl, out, err, slp);
+  zl[S.pos - lo] = val;
+  zr[S.pos - lo] = S.row;
+  // external rows are published for the other blocks now (agent-coherent
+  // store; lanes of other rows pass an out-of-range offset); every row is
+  // written to the output when the block ends
+  using B = Bits<T>;
+  typename B::U u = B::to(val);
+  if (u == B::sent) u = B::qnan;
+  RawVec<sizeof(T)>::store(__builtin_bit_cast(typename RawVec<sizeof(T)>::U, u), rout, S.pubo);
 }
 
 // Each step issues 8-9 vector memory loads and one store; PD steps in flight
 // stay under the 63 outstanding operations a wave can count.
-constexpr int PD = 6;   // steps of operand prefetch
-constexpr int PD2 = 3;  // steps of remote-value prefetch
+#ifndef FCP_PD
+#define FCP_PD 6
+#endif
+#ifndef FCP_PD2
+#define FCP_PD2 3
+#endif
+constexpr int PD = FCP_PD;    // steps of operand prefetch
+constexpr int PD2 = FCP_PD2;  // steps of input and remote-value prefetch
+static_assert(PD2 >= 1 && PD2 < PD, "remote prefetch must trail the operand prefetch");
 
+// LDS: block values zl[R] | step table sptr[max_steps+2] | row ids zr[R].
 template <typename T, int MODE>
 __global__ __launch_bounds__(BLK, 1) void k_psweep(PArgs<T> P, const int *__restrict__ step_ptr,
                                                    const int *__restrict__ blk_step,
                                                    const int *__restrict__ blk_pos, int nblocks,
-                                                   int sptr_off, T *out,
+                                                   int sptr_off, int zr_off, T *out,
                                                    int *__restrict__ ticket, int *__restrict__ err,
                                                    const SolveScalars *__restrict__ s, int slp) {
   extern __shared__ unsigned char smem_raw[];
   T *zl = reinterpret_cast<T *>(smem_raw);
   int *sptr = reinterpret_cast<int *>(smem_raw + sptr_off);
+  int *zr = reinterpret_cast<int *>(smem_raw + zr_off);
   __shared__ int sb;
   if (s && solve_done(s)) return;
   if (threadIdx.x == 0) sb = atomicAdd(ticket, 1);
@@ -420,24 +450,27 @@ __global__ __launch_bounds__(BLK, 1) void k_psweep(PArgs<T> P, const int *__rest
   const int ns = blk_step[b + 1] - st0;
   for (int i = threadIdx.x; i <= ns; i += BLK) sptr[i] = step_ptr[st0 + i];
   __syncthreads();
-  const int idle = (int)(((unsigned)b * BLK + threadIdx.x) % (unsigned)P.n);
+  const __amdgpu_buffer_rsrc_t rout = raw_rsrc(out, P.n * (int)sizeof(T));
   PSlot<T> S[PD];
 #pragma unroll
   for (int j = 0; j < PD; ++j) p_load1<T>(S[j], P, sptr, j, ns);
 #pragma unroll
-  for (int j = 0; j < PD2; ++j) p_load2<T>(S[j], P, out, idle);
+  for (int j = 0; j < PD2; ++j) p_load2<T>(S[j], P, rout);
   // Steps past ns (at most PD-1 in the last group) are empty: their slots
   // are invalid, so every group runs the same instruction stream.
   for (int base = 0; base < ns; base += PD) {
 #pragma unroll
     for (int j = 0; j < PD; ++j) {
       const int step = base + j;
-      p_compute<T, MODE>(S[j], P, sptr, step, ns, lo, zl, out, err, slp);
+      p_compute<T, MODE>(S[j], lo, zl, zr, out, rout, err, slp);
       __syncthreads();
-      p_load2<T>(S[(j + PD2) % PD], P, out, idle);
+      p_load2<T>(S[(j + PD2) % PD], P, rout);
       p_load1<T>(S[j], P, sptr, step + PD, ns);
     }
   }
+  // every row of the block to the output (read by later kernels)
+  const int nr = blk_pos[b + 1] - lo;
+  for (int i = threadIdx.x; i < nr; i += BLK) out[zr[i]] = zl[i];
 }
 
 // pa/pd of a block schedule in precision T from the resident matrix and
@@ -993,7 +1026,7 @@ namespace {
 // sized for precision T.
 template <typename T>
 BlockSchedule &psched(fcp_ctx *ctx, bool upper) {
-  if (ctx->have_tiles && sched_fits(ctx, ctx->tlower, ctx->tupper, sizeof(T)))
+  if (ctx->have_tiles && sched_fits(ctx, ctx->tlower, ctx->tupper, sizeof(T) + sizeof(int)))
     return upper ? ctx->tupper : ctx->tlower;
   if (sizeof(T) == 4) return upper ? ctx->bupper4 : ctx->blower4;
   return upper ? ctx->bupper : ctx->blower;
@@ -1004,7 +1037,10 @@ BlockSchedule &psched(fcp_ctx *ctx, bool upper) {
 template <typename T>
 bool use_psweep(fcp_ctx *ctx) {
   if (ctx->tune_sweep_kind != 0) return false;
-  return sched_fits(ctx, psched<T>(ctx, false), psched<T>(ctx, true), sizeof(T));
+  // remote values are read through a buffer resource of n*sizeof(T) bytes
+  if ((long long)ctx->n * (long long)sizeof(T) >= FCP_OOB) return false;
+  // per row: its value and its row id in LDS
+  return sched_fits(ctx, psched<T>(ctx, false), psched<T>(ctx, true), sizeof(T) + sizeof(int));
 }
 
 template <typename T, int MODE>
@@ -1013,11 +1049,13 @@ int launch_psweep(fcp_ctx *ctx, BlockSchedule &b, const T *in, T *out, int *tick
   const int n = ctx->n;
   PArgs<T> P{b.prc.p, b.pdep.p, PBufs<T>::pa(b).p, PBufs<T>::pdx(b).p, in, n};
   const int sptr_off = (int)(((size_t)b.R * sizeof(T) + 15) & ~(size_t)15);
-  const size_t lds = (size_t)sptr_off + (size_t)(b.max_steps + 2) * sizeof(int);
+  const int zr_off =
+      (int)(((size_t)sptr_off + (size_t)(b.max_steps + 2) * sizeof(int) + 15) & ~(size_t)15);
+  const size_t lds = (size_t)zr_off + (size_t)b.R * sizeof(int);
   k_psweep<T, MODE><<<b.nblocks, BLK, lds, ctx->stream>>>(P, b.step_ptr.p, b.blk_step.p,
-                                                          b.blk_pos.p, b.nblocks, sptr_off, out,
-                                                          ticket,
-                                                          ctx->errflag.p, s, ctx->tune_sweep_sleep);
+                                                          b.blk_pos.p, b.nblocks, sptr_off, zr_off,
+                                                          out, ticket, ctx->errflag.p, s,
+                                                          ctx->tune_sweep_sleep);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }

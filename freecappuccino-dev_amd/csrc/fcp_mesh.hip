@@ -359,6 +359,7 @@ int fcp_compute_geometry(fcp_ctx *ctx) {
 
 int fcp_build_csr(fcp_ctx *ctx) {
   const int n = ctx->n, Fi = ctx->Fi;
+  ctx->sym_pattern = true;  // (owner, neighbour) and (neighbour, owner) for every inner face
   for (int ib = 0; ib < ctx->nPatches; ++ib)
     if (ctx->h_bctype[ib] == FCP_BC_PERIODIC)
       return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "periodic patches are not supported");
@@ -473,7 +474,12 @@ __global__ void k_pos_of(const int *__restrict__ rows, int n, int *__restrict__ 
 // processing order (strict-lower entries for the forward sweep, strict-upper
 // for the backward sweep), in CSR order.  A dependency inside the row's
 // block is stored as its position offset in the block (its LDS slot).
-__global__ void k_pack_static(const int *__restrict__ rows, int n, int upper,
+// (row, count | external << 16) per position: external rows have a consumer
+// in another block (their values are published during the sweep).  With a
+// symmetric pattern the consumers of a forward-sweep row are its strict-upper
+// neighbours, those of a backward-sweep row its strict-lower ones; without
+// one (sym = 0) every row counts as external.
+__global__ void k_pack_static(const int *__restrict__ rows, int n, int upper, int sym,
                               const int *__restrict__ blk_of, const int *__restrict__ blk_pos,
                               const int *__restrict__ pos_of, const int *__restrict__ ia,
                               const int *__restrict__ diag, const int *__restrict__ ja,
@@ -487,8 +493,16 @@ __global__ void k_pack_static(const int *__restrict__ rows, int n, int upper,
   const int b = upper ? diag[row] + 1 : ia[row];
   const int e = upper ? ia[row + 1] : diag[row];
   const int c = e - b;
+  int ext = 1;
+  if (sym) {
+    ext = 0;
+    const int cb = upper ? ia[row] : diag[row] + 1;
+    const int ce = upper ? diag[row] : ia[row + 1];
+    for (int k = cb; k < ce; ++k)
+      if (blk_of[ja[k]] != blk) ext = 1;
+  }
   prc[2 * idx] = row;
-  prc[2 * idx + 1] = c;
+  prc[2 * idx + 1] = c | (ext << 16);
   atomicMax(maxd, c);
   for (int m = 0; m < FCP_PMAXD; ++m) {
     int dep = 0, src = -1;
@@ -559,6 +573,10 @@ int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, 
   }
   b.nsteps = (int)sp.size();
   sp.push_back(n);
+  // the pipelined sweeps take one position per thread and step
+  for (size_t k = 0; k + 1 < sp.size(); ++k)
+    if (sp[k + 1] - sp[k] > BLK)
+      return fcp_fail(ctx, FCP_ERR_STATE, "block schedule: step of %d rows", sp[k + 1] - sp[k]);
   b.max_steps = 0;
   b.R = 0;
   for (int q = 0; q < b.nblocks; ++q) {
@@ -587,7 +605,8 @@ int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, 
                                 ctx->stream));
     FCP_HIP(ctx, hipMemsetAsync(mx.p, 0, sizeof(int), ctx->stream));
     k_pos_of<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(b.rows.p, n, pos.p);
-    k_pack_static<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(b.rows.p, n, upper, blk_dev.p,
+    k_pack_static<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(b.rows.p, n, upper,
+                                                             ctx->sym_pattern ? 1 : 0, blk_dev.p,
                                                              b.blk_pos.p, pos.p, ctx->ia.p,
                                                              ctx->diag.p, ctx->ja.p, b.prc.p,
                                                              b.pdep.p, b.psrc.p, mx.p);

@@ -530,44 +530,6 @@ __global__ __launch_bounds__(BLK) void mk_bicg_v(int n, Sell<T> m, const T *__re
 // pipelined sweep; a row's NR values are kept together in LDS, published
 // with one vector store and fetched with one vector load (L2-coherent
 // buffer accesses, each value individually sentinel-checked).
-constexpr int CPOL_SC1 = 16;  // cache-policy bit of agent-coherent accesses
-
-template <int BYTES>
-struct RawVec;
-template <>
-struct RawVec<4> {
-  typedef unsigned int U;
-  template <int CP>
-  static __device__ __forceinline__ U load(__amdgpu_buffer_rsrc_t rs, int off) {
-    return __builtin_amdgcn_raw_buffer_load_b32(rs, off, 0, CP);
-  }
-  static __device__ __forceinline__ void store(U v, __amdgpu_buffer_rsrc_t rs, int off) {
-    __builtin_amdgcn_raw_buffer_store_b32(v, rs, off, 0, CPOL_SC1);
-  }
-};
-template <>
-struct RawVec<8> {
-  typedef unsigned int U __attribute__((ext_vector_type(2)));
-  template <int CP>
-  static __device__ __forceinline__ U load(__amdgpu_buffer_rsrc_t rs, int off) {
-    return __builtin_amdgcn_raw_buffer_load_b64(rs, off, 0, CP);
-  }
-  static __device__ __forceinline__ void store(U v, __amdgpu_buffer_rsrc_t rs, int off) {
-    __builtin_amdgcn_raw_buffer_store_b64(v, rs, off, 0, CPOL_SC1);
-  }
-};
-template <>
-struct RawVec<16> {
-  typedef unsigned int U __attribute__((ext_vector_type(4)));
-  template <int CP>
-  static __device__ __forceinline__ U load(__amdgpu_buffer_rsrc_t rs, int off) {
-    return __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, CP);
-  }
-  static __device__ __forceinline__ void store(U v, __amdgpu_buffer_rsrc_t rs, int off) {
-    __builtin_amdgcn_raw_buffer_store_b128(v, rs, off, 0, CPOL_SC1);
-  }
-};
-
 // NRP values of T at byte offset off (4 to 32 bytes); CP = cache policy
 // (CPOL_SC1 for values other workgroups publish during the kernel).
 template <typename T, int NRP, int CP = CPOL_SC1>
@@ -645,7 +607,7 @@ __device__ __forceinline__ void m_fetch(const MArgs<T> &P, __amdgpu_buffer_rsrc_
                                         T *av) {
   const int2 rc = reinterpret_cast<const int2 *>(P.prc)[idx];
   row = rc.x;
-  cnt = rc.y;
+  cnt = rc.y & 0xFFFF;
   const int4 dp = reinterpret_cast<const int4 *>(P.pdep)[idx];
   dep[0] = dp.x;
   dep[1] = dp.y;
@@ -682,15 +644,14 @@ __device__ __forceinline__ void m_load1(MSlot<T, NR, NRP> &S, const MArgs<T> &P,
   m_fetch<T, NRP>(P, rin, ok ? idx : 0, S.row, S.cnt, S.dv, S.xin, S.dep, S.av);
 }
 
-// idle: a row private to this lane, read by slots without a remote
-// dependency so idle lanes spread over memory instead of sharing an address
+// Values of dependencies in earlier blocks; lanes without one pass an
+// out-of-range offset and issue no memory request.
 template <typename T, int NR, int NRP>
-__device__ __forceinline__ void m_load2(MSlot<T, NR, NRP> &S, __amdgpu_buffer_rsrc_t rout,
-                                        int idle) {
+__device__ __forceinline__ void m_load2(MSlot<T, NR, NRP> &S, __amdgpu_buffer_rsrc_t rout) {
 #pragma unroll
   for (int m = 0; m < FCP_PMAXD; ++m) {
     const bool need = S.ok && m < S.cnt && S.dep[m] < 0;
-    row_load<T, NRP>(rout, (need ? (-S.dep[m] - 1) : idle) * NRP * (int)sizeof(T), S.rv[m]);
+    row_load<T, NRP>(rout, need ? (-S.dep[m] - 1) * NRP * (int)sizeof(T) : FCP_OOB, S.rv[m]);
   }
 }
 
@@ -789,19 +750,18 @@ __global__ __launch_bounds__(BLK, 1) void mk_psweep(MArgs<T> P, const int *__res
       __builtin_amdgcn_make_buffer_rsrc((void *)P.pin, (short)0, vbytes, 0x00020000);
   const __amdgpu_buffer_rsrc_t rout =
       __builtin_amdgcn_make_buffer_rsrc((void *)out, (short)0, vbytes, 0x00020000);
-  const int idle = (int)(((unsigned)b * BLK + threadIdx.x) % (unsigned)P.n);
   MSlot<T, NR, NRP> S[PD];
 #pragma unroll
   for (int j = 0; j < PD; ++j) m_load1<T, NR, NRP>(S[j], P, rin, sptr, j, ns);
 #pragma unroll
-  for (int j = 0; j < PD2; ++j) m_load2<T, NR, NRP>(S[j], rout, idle);
+  for (int j = 0; j < PD2; ++j) m_load2<T, NR, NRP>(S[j], rout);
   for (int base = 0; base < ns; base += PD) {
 #pragma unroll
     for (int j = 0; j < PD; ++j) {
       const int step = base + j;
       m_compute<T, NR, NRP, MODE>(S[j], P, rin, rout, sptr, step, ns, lo, zl, out, err, slp);
       __syncthreads();
-      m_load2<T, NR, NRP>(S[(j + PD2) % PD], rout, idle);
+      m_load2<T, NR, NRP>(S[(j + PD2) % PD], rout);
       m_load1<T, NR, NRP>(S[j], P, rin, sptr, step + PD, ns);
     }
   }
@@ -877,6 +837,8 @@ template <typename T, int NRP>
 int msched_for(fcp_ctx *ctx, BlockSchedule **lo, BlockSchedule **up, DBuf<char> **pin) {
   *lo = *up = nullptr;
   if (ctx->tune_sweep_kind != 0) return 0;
+  // interleaved vectors are read through a buffer resource of n*NRP*sizeof(T) bytes
+  if ((long long)ctx->n * NRP * (long long)sizeof(T) >= FCP_OOB) return 0;
   if (ctx->have_tiles && sched_fits(ctx, ctx->tlower, ctx->tupper, NRP * sizeof(T))) {
     *lo = &ctx->tlower;
     *up = &ctx->tupper;
