@@ -52,8 +52,10 @@ def measured_traffic(kernel, N):
     return None
 
 
-def alg_bytes(name, n, nnz, Fi, Fb):
-    """Algorithmic (compulsory) HBM bytes of one launch of a named kernel."""
+def alg_bytes(name, n, nnz, Fi, Fb, local=None):
+    """Algorithmic (compulsory) HBM bytes of one launch of a named kernel;
+    `local`: strict-lower entries inside the tile-local preconditioner's tiles
+    (inner faces whose two cells share a tile)."""
     lower = Fi  # strict-lower entries = inner faces
     tab = {
         "spmv_pq_f32": nnz * 8 + 2 * n * 4,
@@ -64,6 +66,8 @@ def alg_bytes(name, n, nnz, Fi, Fb):
         "sweep_fwd_f64": lower * 12 + 4 * n + 3 * n * 8,
         "sweep_bwd_f64": lower * 12 + 4 * n + 3 * n * 8,
         "dic_setup": lower * 12 + 4 * n + 2 * n * 8,
+        "bj_sweep_fwd_f32": local * 8 + 4 * n + 3 * n * 4 if local is not None else None,
+        "bj_sweep_bwd_f32": local * 8 + 4 * n + 3 * n * 4 if local is not None else None,
         "residual_f64": nnz * 12 + (n + 1) * 4 + 3 * n * 8,
         "dot_rz": 2 * n * 8,
         "pupdate": 3 * n * 8,
@@ -88,6 +92,8 @@ def run_gpu(args):
     ctx.create_CSR_matrix(want_arrays=False)
     n, nnz, Fi, Fb = mesh.n_cells, ctx.nnz, mesh.n_inner, mesh.n_bnd
     log(f"[bench] geometry+CSR+levels on GPU done ({time.time()-t0:.1f}s) nnz={nnz}")
+    tiles = ctx.sweep_tiles()
+    local = int(np.count_nonzero(tiles[mesh.owner[:Fi]] == tiles[mesh.neighbour]))
     inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
     dev = torch.device("cuda:0")
     T = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
@@ -166,7 +172,7 @@ def run_gpu(args):
                   sorted(prof.items(), key=lambda kv: -kv[1][1])[:8]))
     kernels = {}
     for k, (cnt, tms) in prof.items():
-        b = alg_bytes(k, n, nnz, Fi, Fb)
+        b = alg_bytes(k, n, nnz, Fi, Fb, local)
         avg_us = tms / cnt * 1e3
         kernels[k] = dict(launches_per_step=cnt / args.prof_steps, avg_us=avg_us,
                           share=tms / total if total else 0.0)
@@ -174,7 +180,7 @@ def run_gpu(args):
             kernels[k]["GBs"] = b / (avg_us * 1e-6) / 1e9
     name, (cnt, tms) = dom
     avg_s = tms / cnt / 1e3
-    b = alg_bytes(name, n, nnz, Fi, Fb)
+    b = alg_bytes(name, n, nnz, Fi, Fb, local)
     roof = dict(bound="hbm", kernel=name, achieved=(b / avg_s / 1e9) if b else None,
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=(b / avg_s / 1e9 / HBM_PEAK_GBS) if b else None,
                 traffic=measured_traffic(name, N), alg_bytes_per_launch=b,

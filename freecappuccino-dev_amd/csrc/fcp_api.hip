@@ -3,6 +3,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -262,6 +263,10 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
         ctx->tune_sweep_tile = value;
         if (ctx->have_csr) return fcp_build_tile_schedules(ctx);
       }
+      return 0;
+    case FCP_TUNE_INNER_PRECOND:
+      if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "inner preconditioner must be 0 or 1");
+      ctx->inner_precond = value;
       return 0;
     case FCP_TUNE_SWEEP_ROWS:
       if (value != 0 && (value < 64 || value > 32768 || (value & 63)))
@@ -764,7 +769,7 @@ int fcp_spmv(fcp_ctx *ctx, const double *x, double *y) {
 
 int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d) {
   FCP_TRY(need(ctx, true, false, true));
-  if (!r || !z || kind < 0 || kind > 2)
+  if (!r || !z || kind < 0 || kind > 4)
     return fcp_fail(ctx, FCP_ERR_ARG, "fcp_precond_apply: bad argument");
   if (kind == 2) d = nullptr;  // ILU(0) has no pivot vector to return
   (void)hipSetDevice(ctx->device);
@@ -774,6 +779,8 @@ int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double
   if (d) FCP_TRY(arg_out(ctx, d, ctx->n, dout));
   if (kind == 2)
     FCP_TRY(fcp_ilu_apply_device(ctx, ri.dev, zo.dev));
+  else if (kind >= 3)
+    FCP_TRY(fcp_bj_device(ctx, kind - 3, ri.dev, zo.dev, dout.dev));
   else
     FCP_TRY(fcp_precond_device(ctx, kind, ri.dev, zo.dev, dout.dev));
   FCP_TRY(arg_back(ctx, zo));
@@ -803,6 +810,19 @@ int fcp_spmv_transpose(fcp_ctx *ctx, const double *x, double *y) {
   FCP_TRY(fcp_atx_device(ctx, xi.dev, yo.dev));
   FCP_TRY(arg_back(ctx, yo));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_sweep_tiles(fcp_ctx *ctx, int *tile) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!tile) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_sweep_tiles: null argument");
+  (void)hipSetDevice(ctx->device);
+  if (!ctx->have_bj) FCP_TRY(fcp_build_bj_schedules(ctx));
+  if (fcp_is_device_ptr(tile)) {
+    FCP_HIP(ctx, hipMemcpy(tile, ctx->bj_tile.data(), sizeof(int) * ctx->n, hipMemcpyHostToDevice));
+  } else {
+    std::copy(ctx->bj_tile.begin(), ctx->bj_tile.end(), tile);
+  }
   return 0;
 }
 

@@ -43,20 +43,27 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (&out)[NV]) {
 
 // Publish NV block partials; returns true in the single last-arriving block,
 // whose thread 0 then holds the totals (summed in block order) in tot[].
+// The partials are written with agent-coherent stores and read back with
+// agent-coherent loads; waiting for the stores to complete before the
+// counter increment orders them without a device fence (which would write
+// back every dirty L2 line in every block).
 template <int NV>
 __device__ bool last_block_reduce(double (&part)[NV], double *__restrict__ partials, int slot,
                                   int *__restrict__ counter, double (&tot)[NV]) {
   __shared__ int is_last;
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) partials[(size_t)(slot + i) * MAXBLK + blockIdx.x] = part[i];
-    __threadfence();
-    const int prev = atomicAdd(counter, 1);
+    for (int i = 0; i < NV; ++i)
+      __hip_atomic_store(
+          reinterpret_cast<unsigned long long *>(&partials[(size_t)(slot + i) * MAXBLK + blockIdx.x]),
+          (unsigned long long)__double_as_longlong(part[i]), __ATOMIC_RELAXED,
+          __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // every memory counter at zero: the stores are complete
+    const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     is_last = (prev == (int)gridDim.x - 1);
   }
   __syncthreads();
   if (!is_last) return false;
-  __threadfence();
   // all threads of the last block reduce the partials in fixed order
   double loc[NV];
 #pragma unroll
@@ -198,18 +205,46 @@ struct Sell {
   const T *a;
 };
 
+// Entries of a row whose loads are issued together (rows longer than this
+// finish in a plain loop).
+constexpr int SELL_KU = 8;
+
+// Grid-stride trips taken GS_U at a time by the streaming kernels: the loads
+// of GS_U trips are issued before their first use, the trips themselves stay
+// in the plain loop's order (i, i + stride, ...), so per-thread sums do not
+// change.
+constexpr int GS_U = 4;
+
 // sum_k a_ik x_k of row i in CSR entry order (the reference's row loop,
 // linear_solvers.f90:493-498); loads of a wave's 64 rows are contiguous.
+// The first SELL_KU entries of the slice (padding: a = 0, col = row) are
+// loaded before any product, so a row costs two memory latencies.
 template <typename T>
 __device__ __forceinline__ T sell_row(const Sell<T> &m, int i, const T *__restrict__ x, T init,
                                       bool subtract) {
-  const int p = m.base[i >> 6] + (i & 63);
+  const int sl = i >> 6;
+  const int b0 = m.base[sl];
+  const int p = b0 + (i & 63);
+  const int L = (m.base[sl + 1] - b0) >> 6;  // slice width: k < L exists for all its rows
   const int len = m.ia[i + 1] - m.ia[i];
+  T av[SELL_KU], xv[SELL_KU];
+  int cv[SELL_KU];
+#pragma unroll
+  for (int k = 0; k < SELL_KU; ++k)
+    if (k < L) {
+      av[k] = m.a[p + 64 * k];
+      cv[k] = m.col[p + 64 * k];
+    }
+#pragma unroll
+  for (int k = 0; k < SELL_KU; ++k)
+    if (k < len) xv[k] = x[cv[k]];
   T acc = init;
-  if (subtract)
-    for (int k = 0; k < len; ++k) acc = acc - m.a[p + 64 * k] * x[m.col[p + 64 * k]];
-  else
-    for (int k = 0; k < len; ++k) acc = acc + m.a[p + 64 * k] * x[m.col[p + 64 * k]];
+#pragma unroll
+  for (int k = 0; k < SELL_KU; ++k)
+    if (k < len) acc = subtract ? acc - av[k] * xv[k] : acc + av[k] * xv[k];
+  for (int k = SELL_KU; k < len; ++k)
+    acc = subtract ? acc - m.a[p + 64 * k] * x[m.col[p + 64 * k]]
+                   : acc + m.a[p + 64 * k] * x[m.col[p + 64 * k]];
   return acc;
 }
 
@@ -226,6 +261,9 @@ int prepare_precond(fcp_ctx *ctx, int kind, bool fp32);
 bool sched_fits(fcp_ctx *ctx, const BlockSchedule &lo, const BlockSchedule &up, size_t row_bytes);
 int pack_block64(fcp_ctx *ctx, BlockSchedule &b);
 int pack_block32(fcp_ctx *ctx, BlockSchedule &b);
+// Pivots and packed FP32 operands of the tile-local preconditioner (schedules
+// jlower/jupper, pivots dbj); *ok false when it does not fit this mesh.
+int bj_begin(fcp_ctx *ctx, int kind, bool *ok);
 int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z);
 int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z);
 

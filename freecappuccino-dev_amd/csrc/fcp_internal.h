@@ -137,6 +137,16 @@ struct fcp_ctx {
   BlockSchedule blower4, bupper4;    // FP32-sized blocks (twice the rows)
   BlockSchedule tlower, tupper;      // geometric-tile blocks (when acyclic)
   bool have_tiles = false;
+  // tile-local preconditioner of the FP32 inner solves: geometric tiles (any
+  // numbering) or contiguous blocks, dependencies inside the block only
+  BlockSchedule jlower, jupper;
+  bool have_bj = false;
+  std::vector<int> bj_tile;          // [n] block of every row (host)
+  DBuf<double> dbj;                  // [n] tile-local DIC / DILU pivots (inverted)
+  uint64_t bj_version = ~0ull, bj_epoch = 0;
+  int bj_kind = -1;
+  int inner_precond = 1;             // 0 global DIC/DILU, 1 tile-local (FP32 inner solves)
+  bool bj_active = false;            // FP32 sweeps use the tile-local preconditioner
   DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)
   bool sym_pattern = true;  // every entry has its transposed entry
   // SELL-64 copy of the operator for SpMV: rows in slices of 64, each slice
@@ -179,6 +189,7 @@ struct fcp_ctx {
   // -------- batched solves (interleaved vectors, NRP values per row) --------
   std::map<long long, std::unique_ptr<MultiSched>> msched;  // keyed by rows per block (lo, up)
   DBuf<char> tpin;  // batched sweep inputs on the tile schedules
+  DBuf<char> jpin;  // batched sweep inputs on the tile-local schedules
   DBuf<double> mv64[9];
   DBuf<float> mv32[8];
 
@@ -268,8 +279,12 @@ inline int fcp_block_rows(const fcp_ctx *ctx, int elem = 8, int upper = 0) {
 // per block, packed static operands included.
 // tile_rank (host, may be NULL): block rank of every row; blocks are then
 // the tiles in rank order (R = the largest tile) instead of row / R.
+// local_only: dependencies in other blocks are dropped (tile-local
+// preconditioner) and no row is external.
 int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, BlockSchedule &b,
-                             const int *tile_rank = nullptr, int ntiles = 0);
+                             const int *tile_rank = nullptr, int ntiles = 0,
+                             bool local_only = false);
+int fcp_build_bj_schedules(fcp_ctx *ctx);
 int fcp_build_tile_schedules(fcp_ctx *ctx);
 // fcp_assemble.hip
 int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
@@ -304,6 +319,7 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
                      double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts);
 int fcp_ensure_workspace(fcp_ctx *ctx);
 int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
+int fcp_bj_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
 // fcp_gmres.hip: pmgmres_ilu (restart mr) on the resident matrix; ILU(0)
 // factors; y = A^T x
 int fcp_gmres_device(fcp_ctx *ctx, double *x, const double *b, int itr_max, int mr,

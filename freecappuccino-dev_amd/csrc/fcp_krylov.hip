@@ -586,8 +586,19 @@ __global__ __launch_bounds__(BLK) void k_dot_rz(int n, const T *__restrict__ r,
                                                 SolveScalars *__restrict__ s) {
   if (solve_done(s)) return;
   double part[1] = {0.0};
-  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK)
-    part[0] += (double)r[i] * (double)z[i];
+  const int stride = gridDim.x * BLK;
+  for (int i0 = blockIdx.x * BLK + threadIdx.x; i0 < n; i0 += GS_U * stride) {
+    T rv[GS_U], zv[GS_U];
+#pragma unroll
+    for (int u = 0; u < GS_U; ++u)
+      if (i0 + u * stride < n) {
+        rv[u] = r[i0 + u * stride];
+        zv[u] = z[i0 + u * stride];
+      }
+#pragma unroll
+    for (int u = 0; u < GS_U; ++u)
+      if (i0 + u * stride < n) part[0] += (double)rv[u] * (double)zv[u];
+  }
   double blk[1], tot[1];
   block_sum<1>(part, blk);
   if (last_block_reduce<1>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
@@ -625,7 +636,19 @@ __global__ void k_pupdate(int n, const T *__restrict__ z, T *__restrict__ p,
                           const SolveScalars *__restrict__ s) {
   if (solve_done(s)) return;
   const T bet = (T)s->bet;
-  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) p[i] = z[i] + bet * p[i];
+  const int stride = gridDim.x * BLK;
+  for (int i0 = blockIdx.x * BLK + threadIdx.x; i0 < n; i0 += GS_U * stride) {
+    T zv[GS_U], pv[GS_U];
+#pragma unroll
+    for (int u = 0; u < GS_U; ++u)
+      if (i0 + u * stride < n) {
+        zv[u] = z[i0 + u * stride];
+        pv[u] = p[i0 + u * stride];
+      }
+#pragma unroll
+    for (int u = 0; u < GS_U; ++u)
+      if (i0 + u * stride < n) p[i0 + u * stride] = zv[u] + bet * pv[u];
+  }
 }
 
 // q = A p ; pkapk = p.q ; alf = sk/pkapk
@@ -661,14 +684,33 @@ __global__ __launch_bounds__(BLK) void k_xr_update(
   const T alf = (T)s->alf;
   const bool first = (s->iter == 0);
   double part[3] = {0.0, 0.0, 0.0};
-  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
-    const T xi = x[i] + alf * p[i];
-    x[i] = xi;
-    const T ri = r[i] - alf * q[i];
-    r[i] = ri;
-    part[0] += fabs((double)ri);
-    part[1] += (double)ri * (double)ri;
-    if (first) part[2] += fabs((double)a[diag[i]] * (double)xi);
+  const int stride = gridDim.x * BLK;
+  for (int i0 = blockIdx.x * BLK + threadIdx.x; i0 < n; i0 += GS_U * stride) {
+    T xv[GS_U], pv[GS_U], rv[GS_U], qv[GS_U], av[GS_U];
+#pragma unroll
+    for (int u = 0; u < GS_U; ++u) {
+      const int i = i0 + u * stride;
+      if (i < n) {
+        xv[u] = x[i];
+        pv[u] = p[i];
+        rv[u] = r[i];
+        qv[u] = q[i];
+        if (first) av[u] = a[diag[i]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < GS_U; ++u) {
+      const int i = i0 + u * stride;
+      if (i < n) {
+        const T xi = xv[u] + alf * pv[u];
+        x[i] = xi;
+        const T ri = rv[u] - alf * qv[u];
+        r[i] = ri;
+        part[0] += fabs((double)ri);
+        part[1] += (double)ri * (double)ri;
+        if (first) part[2] += fabs((double)av[u] * (double)xi);
+      }
+    }
   }
   double blk[3], tot[3];
   block_sum<3>(part, blk);
@@ -1063,17 +1105,22 @@ int launch_psweep(fcp_ctx *ctx, BlockSchedule &b, const T *in, T *out, int *tick
 // Packs matrix coefficients and pivots of a block schedule when they changed
 // (keyed by the preconditioner setup epoch).
 template <typename T>
-int pack_block(fcp_ctx *ctx, BlockSchedule &b) {
+int pack_block(fcp_ctx *ctx, BlockSchedule &b, const double *d, uint64_t ver) {
   const int n = ctx->n;
-  if (PBufs<T>::ver(b) == ctx->d_epoch) return 0;
+  if (PBufs<T>::ver(b) == ver) return 0;
   FCP_HIP(ctx, PBufs<T>::pa(b).alloc((size_t)FCP_PMAXD * n));
   FCP_HIP(ctx, PBufs<T>::pdx(b).alloc((size_t)2 * n));
-  k_pack_vals<T><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, b.prc.p, b.psrc.p, ctx->a.p,
-                                                            ctx->d.p, PBufs<T>::pa(b).p,
+  k_pack_vals<T><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, b.prc.p, b.psrc.p, ctx->a.p, d,
+                                                            PBufs<T>::pa(b).p,
                                                             PBufs<T>::pdx(b).p);
   FCP_HIP(ctx, hipGetLastError());
-  PBufs<T>::ver(b) = ctx->d_epoch;
+  PBufs<T>::ver(b) = ver;
   return 0;
+}
+
+template <typename T>
+int pack_block(fcp_ctx *ctx, BlockSchedule &b) {
+  return pack_block<T>(ctx, b, ctx->d.p, ctx->d_epoch);
 }
 
 // Packs both sweeps of the single-vector schedules.
@@ -1113,11 +1160,119 @@ int precond_setup(fcp_ctx *ctx, int kind, const T *a, T *d) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Tile-local DIC / DILU (preconditioner of the FP32 inner solves): the same
+// per-row recurrences restricted to dependencies inside a block, so every
+// block of a sweep runs independently (blocks never wait on one another).
+// ---------------------------------------------------------------------------
+
+// Pivots over the block schedule jlower, one workgroup per block, steps in
+// order: d_i = 1/(a_ii - sum a_ik^2 d_k) (kind 0) or
+// 1/(a_ii - sum a_ik d_k a_ki) (kind 1) over the block's own k < i, summed in
+// CSR order as the global setup does.
+template <int KIND>
+__global__ __launch_bounds__(BLK) void k_bj_setup(const int *__restrict__ prc,
+                                                  const int *__restrict__ pdep,
+                                                  const int *__restrict__ psrc,
+                                                  const int *__restrict__ step_ptr,
+                                                  const int *__restrict__ blk_step,
+                                                  const int *__restrict__ blk_pos,
+                                                  const int *__restrict__ diag,
+                                                  const double *__restrict__ a,
+                                                  const int *__restrict__ tslot,
+                                                  double *__restrict__ dbj) {
+  extern __shared__ double dl[];
+  const int b = blockIdx.x;
+  const int lo = blk_pos[b];
+  for (int st = blk_step[b], se = blk_step[b + 1]; st < se; ++st) {
+    const int p = step_ptr[st] + (int)threadIdx.x;
+    if (p < step_ptr[st + 1]) {
+      const int row = prc[2 * p];
+      const int cnt = prc[2 * p + 1] & 0xFFFF;
+      double acc = a[diag[row]];
+      for (int m = 0; m < cnt; ++m) {
+        const int k = psrc[(size_t)FCP_PMAXD * p + m];
+        const double dj = dl[pdep[(size_t)FCP_PMAXD * p + m]];
+        if (KIND == 0) {
+          const double ak = a[k];
+          acc = acc - ak * ak * dj;
+        } else {
+          const int l = tslot[k];
+          acc = acc - a[k] * dj * (l >= 0 ? a[l] : 0.0);
+        }
+      }
+      const double dv = 1.0 / acc;
+      dl[p - lo] = dv;
+      dbj[row] = dv;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_reset_tickets(int *__restrict__ tickets) {
+  tickets[0] = 0;
+  tickets[1] = 0;
+}
+
+bool bj_usable(fcp_ctx *ctx) {
+  return ctx->have_bj && ctx->tune_sweep_kind == 0 &&
+         (long long)ctx->n * (long long)sizeof(float) < FCP_OOB &&
+         sched_fits(ctx, ctx->jlower, ctx->jupper, sizeof(float) + sizeof(int)) &&
+         (size_t)ctx->jlower.R * sizeof(double) + 256 <= ctx->lds_cap;
+}
+
+// Schedules, FP64 pivots and packed FP32 operands of the tile-local
+// preconditioner of kind 0 (DIC) or 1 (DILU); false when it does not apply.
+int bj_prepare(fcp_ctx *ctx, int kind, bool *ok) {
+  *ok = false;
+  if (!ctx->have_bj) FCP_TRY(fcp_build_bj_schedules(ctx));
+  if (!bj_usable(ctx)) return 0;
+  const int n = ctx->n;
+  if (ctx->bj_version != ctx->a_version || ctx->bj_kind != kind) {
+    FCP_HIP(ctx, ctx->dbj.alloc(n));
+    const BlockSchedule &b = ctx->jlower;
+    const size_t lds = (size_t)b.R * sizeof(double);
+    ProfScope ps(ctx, kind ? "bj_dilu_setup" : "bj_dic_setup");
+    if (kind == 0)
+      k_bj_setup<0><<<b.nblocks, BLK, lds, ctx->stream>>>(
+          b.prc.p, b.pdep.p, b.psrc.p, b.step_ptr.p, b.blk_step.p, b.blk_pos.p, ctx->diag.p,
+          ctx->a.p, ctx->tslot.p, ctx->dbj.p);
+    else
+      k_bj_setup<1><<<b.nblocks, BLK, lds, ctx->stream>>>(
+          b.prc.p, b.pdep.p, b.psrc.p, b.step_ptr.p, b.blk_step.p, b.blk_pos.p, ctx->diag.p,
+          ctx->a.p, ctx->tslot.p, ctx->dbj.p);
+    FCP_HIP(ctx, hipGetLastError());
+    ctx->bj_version = ctx->a_version;
+    ctx->bj_kind = kind;
+    ctx->bj_epoch++;
+  }
+  for (BlockSchedule *b : {&ctx->jlower, &ctx->jupper})
+    FCP_TRY(pack_block<float>(ctx, *b, ctx->dbj.p, ctx->bj_epoch | (1ull << 63)));
+  *ok = true;
+  return 0;
+}
+
+// z = M^{-1} r with the tile-local sweeps (forward into y, backward into z);
+// no dependency crosses a block, so no sentinel fill is needed.
+int bj_apply32(fcp_ctx *ctx, const float *r, float *y, float *z, const SolveScalars *s) {
+  k_reset_tickets<<<1, 1, 0, ctx->stream>>>(tickets(ctx));
+  {
+    ProfScope ps(ctx, "bj_sweep_fwd_f32");
+    const int rc = launch_psweep<float, 0>(ctx, ctx->jlower, r, y, tickets(ctx), s);
+    if (rc < 0) return rc;
+  }
+  ProfScope ps(ctx, "bj_sweep_bwd_f32");
+  return launch_psweep<float, 1>(ctx, ctx->jupper, y, z, tickets(ctx) + 1, s);
+}
+
 // z = M^{-1} r (forward into y, backward into z), both sentinel-filled first.
 template <typename T>
 int precond_apply(fcp_ctx *ctx, const T *a, const T *d, const T *r, T *y, T *z,
                   const SolveScalars *s) {
   const int n = ctx->n;
+  if (sizeof(T) == 4 && ctx->bj_active)
+    return bj_apply32(ctx, reinterpret_cast<const float *>(r), reinterpret_cast<float *>(y),
+                      reinterpret_cast<float *>(z), s);
   k_fill_sent<T><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(y, z, n, tickets(ctx), s);
   if (use_psweep<T>(ctx)) {
     FCP_TRY(psweep_pack<T>(ctx));
@@ -1341,6 +1496,15 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
     k_cast_a<<<g, BLK, 0, ctx->stream>>>(n, ctx->d.p, ctx->d32.p);
     ctx->d32_version = ctx->dic_version;
   }
+  // FP32 inner sweeps with the tile-local preconditioner when selected (the
+  // FP64 refinement and any FP64 fallback keep the global one)
+  bool use_bj = false;
+  if (solver != 1 && ctx->inner_precond == 1) FCP_TRY(bj_prepare(ctx, kind, &use_bj));
+  struct BjScope {
+    fcp_ctx *c;
+    ~BjScope() { c->bj_active = false; }
+  } bj_scope{ctx};
+  ctx->bj_active = use_bj;
   FCP_TRY(true_residual(ctx, x, b, so));
   const int max_outer = (opts && opts->max_outer > 0) ? opts->max_outer : 30;
   const double inner_tol = (opts && opts->inner_tol > 0) ? opts->inner_tol : 1e-4;
@@ -1437,6 +1601,7 @@ int prepare_precond(fcp_ctx *ctx, int kind, bool fp32) {
 // Packed coefficients and pivots of a block schedule in FP64 / FP32.
 int pack_block64(fcp_ctx *ctx, BlockSchedule &b) { return pack_block<double>(ctx, b); }
 int pack_block32(fcp_ctx *ctx, BlockSchedule &b) { return pack_block<float>(ctx, b); }
+int bj_begin(fcp_ctx *ctx, int kind, bool *ok) { return bj_prepare(ctx, kind, ok); }
 
 // z = M^{-1} r with the single-vector sweeps (current pivots).
 int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z) {
@@ -1446,6 +1611,28 @@ int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z) {
   return precond_apply<float>(ctx, ctx->a32.p, ctx->d32.p, r, y, z, nullptr);
 }
 }  // namespace fcpk
+
+// Tile-local DIC (kind 0) / DILU (kind 1) applied in FP64, for checking the
+// FP32 inner preconditioner; d_out (optional) receives its pivots.
+int fcp_bj_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out) {
+  FCP_TRY(fcp_ensure_workspace(ctx));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
+  bool ok = false;
+  FCP_TRY(bj_prepare(ctx, kind, &ok));
+  if (!ok || (long long)ctx->n * 8 >= FCP_OOB ||
+      !sched_fits(ctx, ctx->jlower, ctx->jupper, sizeof(double) + sizeof(int)))
+    return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "tile-local preconditioner does not fit this mesh");
+  for (BlockSchedule *b : {&ctx->jlower, &ctx->jupper})
+    FCP_TRY(pack_block<double>(ctx, *b, ctx->dbj.p, ctx->bj_epoch | (1ull << 63)));
+  k_reset_tickets<<<1, 1, 0, ctx->stream>>>(tickets(ctx));
+  FCP_TRY((launch_psweep<double, 0>(ctx, ctx->jlower, r, ctx->yk.p, tickets(ctx), nullptr)));
+  FCP_TRY((launch_psweep<double, 1>(ctx, ctx->jupper, ctx->yk.p, z, tickets(ctx) + 1, nullptr)));
+  if (d_out)
+    FCP_HIP(ctx, hipMemcpyAsync(d_out, ctx->dbj.p, sizeof(double) * ctx->n,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  FCP_TRY(check_err(ctx));
+  return 0;
+}
 
 int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out) {
   FCP_TRY(fcp_ensure_workspace(ctx));

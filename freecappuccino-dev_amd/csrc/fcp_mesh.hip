@@ -479,7 +479,7 @@ __global__ void k_pos_of(const int *__restrict__ rows, int n, int *__restrict__ 
 // symmetric pattern the consumers of a forward-sweep row are its strict-upper
 // neighbours, those of a backward-sweep row its strict-lower ones; without
 // one (sym = 0) every row counts as external.
-__global__ void k_pack_static(const int *__restrict__ rows, int n, int upper, int sym,
+__global__ void k_pack_static(const int *__restrict__ rows, int n, int upper, int sym, int local,
                               const int *__restrict__ blk_of, const int *__restrict__ blk_pos,
                               const int *__restrict__ pos_of, const int *__restrict__ ia,
                               const int *__restrict__ diag, const int *__restrict__ ja,
@@ -492,9 +492,13 @@ __global__ void k_pack_static(const int *__restrict__ rows, int n, int upper, in
   const int lo = blk_pos[blk];
   const int b = upper ? diag[row] + 1 : ia[row];
   const int e = upper ? ia[row + 1] : diag[row];
-  const int c = e - b;
-  int ext = 1;
-  if (sym) {
+  int c = e - b;
+  if (local) {  // only the dependencies inside the block count
+    c = 0;
+    for (int k = b; k < e; ++k) c += blk_of[ja[k]] == blk;
+  }
+  int ext = local ? 0 : 1;
+  if (sym && !local) {
     ext = 0;
     const int cb = upper ? ia[row] : diag[row] + 1;
     const int ce = upper ? diag[row] : ia[row + 1];
@@ -504,13 +508,17 @@ __global__ void k_pack_static(const int *__restrict__ rows, int n, int upper, in
   prc[2 * idx] = row;
   prc[2 * idx + 1] = c | (ext << 16);
   atomicMax(maxd, c);
+  // the m-th counted dependency, in CSR order
+  int k = b;
   for (int m = 0; m < FCP_PMAXD; ++m) {
     int dep = 0, src = -1;
     if (m < c) {
-      const int k = b + m;
+      if (local)
+        while (blk_of[ja[k]] != blk) ++k;
       const int j = ja[k];
       dep = (blk_of[j] == blk) ? (pos_of[j] - lo) : -(j + 1);
       src = k;
+      ++k;
     }
     pdep[(size_t)FCP_PMAXD * idx + m] = dep;
     psrc[(size_t)FCP_PMAXD * idx + m] = src;
@@ -523,7 +531,7 @@ __global__ void k_pack_static(const int *__restrict__ rows, int n, int upper, in
 // the runs of equal level inside a block.  Blocks are contiguous runs of R
 // rows, or -- with tile_rank -- the given tiles in rank order.
 int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, BlockSchedule &b,
-                             const int *tile_rank, int ntiles) {
+                             const int *tile_rank, int ntiles, bool local_only) {
   const int n = ctx->n;
   b.tiled = tile_rank != nullptr;
   b.nblocks = tile_rank ? ntiles : (n + R - 1) / R;
@@ -606,7 +614,8 @@ int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, 
     FCP_HIP(ctx, hipMemsetAsync(mx.p, 0, sizeof(int), ctx->stream));
     k_pos_of<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(b.rows.p, n, pos.p);
     k_pack_static<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(b.rows.p, n, upper,
-                                                             ctx->sym_pattern ? 1 : 0, blk_dev.p,
+                                                             ctx->sym_pattern ? 1 : 0,
+                                                             local_only ? 1 : 0, blk_dev.p,
                                                              b.blk_pos.p, pos.p, ctx->ia.p,
                                                              ctx->diag.p, ctx->ja.p, b.prc.p,
                                                              b.pdep.p, b.psrc.p, mx.p);
@@ -631,7 +640,8 @@ struct TileInput {
   std::vector<int> ia, ja, dg;
 };
 
-int tile_ranks(const TileInput &in, int n, int target, std::vector<int> &rank) {
+int tile_ranks(const TileInput &in, int n, int target, std::vector<int> &rank,
+               bool check_acyclic = true) {
   double lo[3] = {in.x[0], in.y[0], in.z[0]}, hi[3] = {in.x[0], in.y[0], in.z[0]};
   for (int i = 0; i < n; ++i) {
     const double v[3] = {in.x[i], in.y[i], in.z[i]};
@@ -675,7 +685,7 @@ int tile_ranks(const TileInput &in, int n, int target, std::vector<int> &rank) {
   for (int i = 0; i < n; ++i) rank[i] = trank[tid[i]];
   // acyclic only if every lower dependency has a rank <= its row's, every
   // upper dependency a rank >= it
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; i < n && check_acyclic; ++i) {
     for (int k = in.ia[i]; k < in.dg[i]; ++k)
       if (rank[in.ja[k]] > rank[i]) return 0;
     for (int k = in.dg[i] + 1; k < in.ia[i + 1]; ++k)
@@ -692,14 +702,8 @@ int tile_ranks(const TileInput &in, int n, int target, std::vector<int> &rank) {
 }
 }  // namespace
 
-int fcp_build_tile_schedules(fcp_ctx *ctx) {
-  ctx->have_tiles = false;
-  ctx->msched.clear();
-  if (!ctx->have_geom || ctx->tune_sweep_tile == 1) return 0;
+static int tile_input(fcp_ctx *ctx, TileInput &in) {
   const int n = ctx->n;
-  const int target = ctx->tune_sweep_tile >= 64 ? ctx->tune_sweep_tile : 4096;
-  if (n < 4 * target) return 0;
-  TileInput in;
   in.x.resize(n);
   in.y.resize(n);
   in.z.resize(n);
@@ -713,12 +717,52 @@ int fcp_build_tile_schedules(fcp_ctx *ctx) {
   FCP_HIP(ctx, hipMemcpyAsync(in.ja.data(), ctx->ja.p, sizeof(int) * ctx->nnz, hipMemcpyDeviceToHost, ctx->stream));
   FCP_HIP(ctx, hipMemcpyAsync(in.dg.data(), ctx->diag.p, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_build_tile_schedules(fcp_ctx *ctx) {
+  ctx->have_tiles = false;
+  ctx->have_bj = false;
+  ctx->msched.clear();
+  if (!ctx->have_geom || ctx->tune_sweep_tile == 1) return 0;
+  const int n = ctx->n;
+  const int target = ctx->tune_sweep_tile >= 64 ? ctx->tune_sweep_tile : 4096;
+  if (n < 4 * target) return 0;
+  TileInput in;
+  FCP_TRY(tile_input(ctx, in));
   std::vector<int> rank;
   const int m = tile_ranks(in, n, target, rank);
   if (m == 0) return 0;
   FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, 0, 0, ctx->tlower, rank.data(), m));
   FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, 0, 1, ctx->tupper, rank.data(), m));
   ctx->have_tiles = true;
+  return 0;
+}
+
+// Blocks of the tile-local preconditioner: geometric tiles of about 4096
+// cells when the mesh has geometry (any numbering: blocks never wait on one
+// another), else runs of 4096 contiguous rows.
+int fcp_build_bj_schedules(fcp_ctx *ctx) {
+  ctx->have_bj = false;
+  const int n = ctx->n;
+  const int target = ctx->tune_sweep_tile >= 64 ? ctx->tune_sweep_tile : 4096;
+  std::vector<int> rank;
+  int m = 0;
+  if (ctx->have_geom && ctx->tune_sweep_tile != 1 && n >= 4 * target) {
+    TileInput in;
+    FCP_TRY(tile_input(ctx, in));
+    m = tile_ranks(in, n, target, rank, false);
+  }
+  if (m == 0) {
+    rank.resize(n);
+    for (int i = 0; i < n; ++i) rank[i] = i / target;
+    m = (n + target - 1) / target;
+  }
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, 0, 0, ctx->jlower, rank.data(), m, true));
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, 0, 1, ctx->jupper, rank.data(), m, true));
+  ctx->bj_tile.swap(rank);
+  ctx->bj_version = ~0ull;
+  ctx->have_bj = true;
   return 0;
 }
 
@@ -790,6 +834,7 @@ static int set_block_rows_caps(fcp_ctx *ctx) {
 }
 
 int fcp_build_schedules(fcp_ctx *ctx) {
+  ctx->have_bj = false;
   FCP_TRY(build_one_schedule(ctx, 0, ctx->lower));
   FCP_TRY(build_one_schedule(ctx, 1, ctx->upper));
   FCP_TRY(set_block_rows_caps(ctx));

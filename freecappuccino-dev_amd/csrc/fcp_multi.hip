@@ -50,9 +50,28 @@ __device__ __forceinline__ void done_mask(const SolveScalars *s, bool (&d)[NR]) 
 template <typename T, int NR, int NRP>
 __device__ __forceinline__ void msell_row(const Sell<T> &m, int i, const T *__restrict__ x,
                                           T (&acc)[NR], bool subtract) {
-  const int p = m.base[i >> 6] + (i & 63);
+  const int sl = i >> 6;
+  const int b0 = m.base[sl];
+  const int p = b0 + (i & 63);
+  const int L = (m.base[sl + 1] - b0) >> 6;
   const int len = m.ia[i + 1] - m.ia[i];
-  for (int k = 0; k < len; ++k) {
+  T av[SELL_KU];
+  int cv[SELL_KU];
+#pragma unroll
+  for (int k = 0; k < SELL_KU; ++k)
+    if (k < L) {
+      av[k] = m.a[p + 64 * k];
+      cv[k] = m.col[p + 64 * k];
+    }
+#pragma unroll
+  for (int k = 0; k < SELL_KU; ++k)
+    if (k < len) {
+      const T *xc = x + (size_t)cv[k] * NRP;
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        acc[r] = subtract ? acc[r] - av[k] * xc[r] : acc[r] + av[k] * xc[r];
+    }
+  for (int k = SELL_KU; k < len; ++k) {
     const T a = m.a[p + 64 * k];
     const T *xc = x + (size_t)m.col[p + 64 * k] * NRP;
 #pragma unroll
@@ -870,14 +889,29 @@ int mprecond_apply(fcp_ctx *ctx, const T *r, T *y, T *z, T *tmp3, const SolveSca
   const int n = ctx->n;
   BlockSchedule *blo = nullptr, *bup = nullptr;
   DBuf<char> *pbuf = nullptr;
-  FCP_TRY((msched_for<T, NRP>(ctx, &blo, &bup, &pbuf)));
+  // FP32 sweeps of the tile-local preconditioner: operands already packed with
+  // its pivots, and no dependency crosses a block (no sentinel fill needed)
+  const bool bj = sizeof(T) == 4 && ctx->bj_active;
+  if (bj) {
+    if ((long long)n * NRP * (long long)sizeof(T) < FCP_OOB &&
+        sched_fits(ctx, ctx->jlower, ctx->jupper, NRP * sizeof(T))) {
+      blo = &ctx->jlower;
+      bup = &ctx->jupper;
+      pbuf = &ctx->jpin;
+    }
+  } else {
+    FCP_TRY((msched_for<T, NRP>(ctx, &blo, &bup, &pbuf)));
+  }
   if (blo && (size_t)n * NRP * sizeof(T) < ((size_t)1 << 31)) {
     const size_t m = (size_t)n * NRP;
     FCP_HIP(ctx, pbuf->alloc(m * sizeof(T)));
-    mk_fill_sent<T, NR><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(y, z, m, tickets(ctx), s);
+    if (bj)
+      mk_fill_sent<T, NR><<<1, 1, 0, ctx->stream>>>(y, z, 0, tickets(ctx), s);
+    else
+      mk_fill_sent<T, NR><<<red_grid(ctx, n), BLK, 0, ctx->stream>>>(y, z, m, tickets(ctx), s);
     for (int u = 0; u < 2; ++u) {
       BlockSchedule &b = u ? *bup : *blo;
-      FCP_TRY(pack_of<T>(ctx, b));
+      if (!bj) FCP_TRY(pack_of<T>(ctx, b));
       T *pin = reinterpret_cast<T *>(pbuf->p);
       const T *in = u ? y : r;
       T *out = u ? z : y;
@@ -1070,7 +1104,16 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
     res.outer = 1;
     return 0;
   }
-  // FP32 inner Krylov on A e = r/|r| inside FP64 refinement, per column
+  // FP32 inner Krylov on A e = r/|r| inside FP64 refinement, per column; the
+  // inner sweeps use the tile-local preconditioner when selected (the FP64
+  // refinement and the FP64 finish keep the global one)
+  bool use_bj = false;
+  if (ctx->inner_precond == 1) FCP_TRY(bj_begin(ctx, kind, &use_bj));
+  struct BjScope {
+    fcp_ctx *c;
+    ~BjScope() { c->bj_active = false; }
+  } bj_scope{ctx};
+  ctx->bj_active = use_bj;
   for (int k = 0; k < 9; ++k) FCP_HIP(ctx, ctx->mv32[k].alloc(k == 8 ? 3 * (size_t)n : m));
   MVecs<float> V32{ctx->mv32[0].p, ctx->mv32[1].p, ctx->mv32[2].p, ctx->mv32[3].p,
                    ctx->mv32[4].p, ctx->mv32[5].p, ctx->mv32[6].p, ctx->mv32[7].p,

@@ -27,6 +27,7 @@ FCP_TUNE_SWEEP_SLEEP = 2
 FCP_TUNE_SWEEP_KIND = 3
 FCP_TUNE_SWEEP_ROWS = 4
 FCP_TUNE_SWEEP_TILE = 5
+FCP_TUNE_INNER_PRECOND = 6
 
 
 class FcpError(RuntimeError):
@@ -268,9 +269,16 @@ class Context:
                     "fcp_spmv_transpose")
         return y
 
+    def sweep_tiles(self):
+        """Tile of every cell in the tile-local preconditioner (0-based)."""
+        t = np.zeros(self.mesh.n_cells, np.int32)
+        self._check(self.lib.fcp_sweep_tiles(self.h, _ptr(t)[0]), "fcp_sweep_tiles")
+        return t
+
     def precond_apply(self, r, kind: int = 0):
         """z = M^{-1} r: kind 0 DIC, 1 DILU (returns (z, inverted pivots)),
-        2 ILU(0) of pmgmres_ilu (returns (z, None))."""
+        2 ILU(0) of pmgmres_ilu (returns (z, None)), 3 / 4 tile-local DIC /
+        DILU of the FP32 inner solves (returns (z, pivots))."""
         n = self.mesh.n_cells
         z = np.zeros(n)
         d = np.zeros(n) if kind != 2 else None

@@ -198,6 +198,12 @@ module fcp_capi
       real(c_double) :: x(*), y(*)
     end function
 
+    integer(c_int) function fcp_sweep_tiles(ctx, tile) bind(C, name='fcp_sweep_tiles')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: tile(*)
+    end function
+
     integer(c_int) function fcp_ilu_factor(ctx, l) bind(C, name='fcp_ilu_factor')
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

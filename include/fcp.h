@@ -255,8 +255,16 @@ int fcp_spmv_transpose(fcp_ctx *ctx, const double *x, double *y);
  * :1267-1477, the preconditioner of pmgmres_ilu) preconditioner of the
  * resident matrix, with the reference's operation order; d (optional, [n])
  * receives the inverted pivots of kinds 0 and 1 and is ignored for kind 2
- * (fcp_ilu_factor returns those factors).  r, z of length nCells. */
+ * (fcp_ilu_factor returns those factors).  Kinds 3 and 4 apply, in FP64,
+ * the tile-local DIC / DILU that preconditions the FP32 inner solves of
+ * FCP_PREC_MIXED (no reference counterpart: the DIC / DILU recurrences with
+ * the couplings between tiles dropped; fcp_sweep_tiles gives the tiles).
+ * r, z of length nCells. */
 int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d);
+
+/* Tile (0-based block number) of every cell in the tile-local
+ * preconditioner of the FP32 inner solves; tile[nCells]. */
+int fcp_sweep_tiles(fcp_ctx *ctx, int *tile);
 
 /* ILU(0) factors of the resident matrix as ilu_cr leaves them
  * (linear_solvers.f90:1267-1385): l[nnz] in CSR order, multipliers left of
@@ -326,7 +334,16 @@ int fcp_pmgmres(int n, int nnz, const int *ia, const int *ja, const double *a,
 #define FCP_TUNE_SWEEP_ROWS 4    /* rows per block of the row-block sweeps      */
 #define FCP_TUNE_SWEEP_TILE 5    /* rows per geometric tile of the pipelined
                                     sweeps (0 = auto, 1 = contiguous blocks)   */
-/* Sets a launch-shape parameter of the context; results do not depend on it. */
+#define FCP_TUNE_INNER_PRECOND 6 /* preconditioner of the FP32 inner solves of
+                                    FCP_PREC_MIXED: 1 (default) tile-local
+                                    DIC/DILU (the reference recurrences
+                                    restricted to ~4096-cell tiles, which all
+                                    run at once), 0 the global DIC/DILU of the
+                                    FP64 solve.  FP64 solves and the FP64
+                                    refinement always use the global one.      */
+/* Sets a launch-shape parameter of the context; results do not depend on it
+ * (FCP_TUNE_INNER_PRECOND changes the path of a mixed-precision solve, not
+ * its stopping test). */
 int fcp_tune_set(fcp_ctx *ctx, int param, int value);
 
 /* --- instrumentation ------------------------------------------------------ */

@@ -407,7 +407,10 @@ def test_lsq_gradients_vs_reference_and_oracle(api, golden_dir):
 def test_tiled_and_contiguous_sweeps_bitexact(api, tile):
     """Pipelined sweeps on geometric tiles (forced small) and on contiguous
     row blocks give the reference DIC/DILU application bit for bit, single
-    and batched, and PCG solves identical to the level-task sweeps."""
+    and batched, and PCG solves identical to the level-task sweeps (global
+    inner preconditioner; the level-task sweeps have no tile-local one).
+    Batched solves equal single-column solves with either inner
+    preconditioner."""
     print(f"[case] sweeps tile={tile} 20^3", flush=True)
     mesh = hex_mesh(20, jitter=0.2, seed=4)
     ctx, geo, csr = _setup(api, mesh)
@@ -424,6 +427,7 @@ def test_tiled_and_contiguous_sweeps_bitexact(api, tile):
     assert np.array_equal(z, o.dic_apply(a, d_o, r))
     b = r * 3.0
     sols = {}
+    ctx.tune(api.FCP_TUNE_INNER_PRECOND, 0)
     for kind in (0, 1):
         ctx.tune(api.FCP_TUNE_SWEEP_KIND, kind)
         for prec in (0, 1):
@@ -433,8 +437,66 @@ def test_tiled_and_contiguous_sweeps_bitexact(api, tile):
     assert np.array_equal(sols[(0, 0)], sols[(1, 0)]) and np.array_equal(sols[(0, 1)], sols[(1, 1)])
     ctx.tune(api.FCP_TUNE_SWEEP_KIND, 0)
     B = np.stack([b, np.cos(np.arange(n))])
-    for prec in (0, 1):
-        X = np.zeros((2, n))
-        ctx.csrsolve_multi("iccg", X, B, itr_max=2000, tol_rel=1e-10, precision=prec, criterion=1)
-        assert np.array_equal(X[0], sols[(0, prec)])
+    for ip in (0, 1):
+        ctx.tune(api.FCP_TUNE_INNER_PRECOND, ip)
+        for prec in (0, 1):
+            x = np.zeros(n)
+            ctx.csrsolve("iccg", x, b, itr_max=2000, tol_rel=1e-10, precision=prec, criterion=1)
+            X = np.zeros((2, n))
+            ctx.csrsolve_multi("iccg", X, B, itr_max=2000, tol_rel=1e-10, precision=prec,
+                               criterion=1)
+            assert np.array_equal(X[0], x), (ip, prec)
+            assert np.array_equal(x, sols[(0, prec)]) or (ip == 1 and prec == 1), (ip, prec)
+    ctx.close()
+
+
+@pytest.mark.parametrize("kind", [3, 4])
+def test_tile_local_preconditioner_bitexact(api, kind):
+    """Tile-local DIC (3) / DILU (4) of the FP32 inner solves equal the
+    oracle's DIC / DILU of the block-diagonal matrix (couplings between the
+    library's tiles set to zero): pivots and one application, bit for bit."""
+    print(f"[case] tile-local precond kind {kind} 32^3", flush=True)
+    mesh = hex_mesh(32, jitter=0.2, seed=4)
+    ctx, geo, csr = _setup(api, mesh)
+    o = Oracle(mesh)
+    if kind == 3:
+        mu = -(10.0 ** np.sin(np.arange(mesh.n_cells)))
+        a, _ = o.laplacian(mu, np.zeros(mesh.n_total))
+    else:
+        F = mesh.n_faces
+        bk = np.full(mesh.n_bnd, 2, np.uint8)
+        a, _, _, _ = o.convdiff(np.cos(np.arange(F)), 10.0 ** np.sin(np.arange(F)),
+                                np.zeros(mesh.n_total), np.zeros((mesh.n_total, 3)), 1.0, 0, bk)
+    ctx.matrix_set(a)
+    tile = ctx.sweep_tiles()
+    assert tile.max() >= 3  # several tiles
+    rows = np.repeat(np.arange(mesh.n_cells), np.diff(o.ia))
+    ab = np.where(tile[rows] == tile[o.ja], a, 0.0)
+    r = np.sin(np.arange(mesh.n_cells) * 0.1)
+    z, d = ctx.precond_apply(r, kind=kind)
+    d_o = o.dic_diag(ab) if kind == 3 else o.dilu_diag(ab)
+    assert np.array_equal(d, d_o)
+    assert np.array_equal(z, o.dic_apply(ab, d_o, r))
+    ctx.close()
+
+
+@pytest.mark.parametrize("inner", [0, 1])
+def test_mixed_inner_preconditioners_reach_tolerance(api, inner):
+    """FP32 inner solves with the global (0) or tile-local (1) preconditioner:
+    the FP64 refinement reaches the requested true relative residual."""
+    print(f"[case] mixed inner precond {inner} 48^3", flush=True)
+    mesh = workloads.poisson_jittered_mesh(48)
+    ctx, geo, csr = _setup(api, mesh)
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    su = inp["source"] * geo["vol"]
+    ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su)
+    ctx.tune(api.FCP_TUNE_INNER_PRECOND, inner)
+    x = np.zeros(mesh.n_cells)
+    rc, info = ctx.csrsolve("iccg", x, su, itr_max=5000, tol_abs=0.0, tol_rel=1e-10,
+                            precision=api.FCP_PREC_MIXED, criterion=api.FCP_CRIT_TRUE_L2)
+    assert rc == 0 and info["rel_res_l2"] <= 1e-10, info
+    x64 = np.zeros(mesh.n_cells)
+    ctx.csrsolve("iccg", x64, su, itr_max=5000, tol_abs=0.0, tol_rel=1e-12,
+                 criterion=api.FCP_CRIT_TRUE_L2)
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-7
     ctx.close()
