@@ -265,7 +265,8 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
       }
       return 0;
     case FCP_TUNE_INNER_PRECOND:
-      if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "inner preconditioner must be 0 or 1");
+      if (value < 0 || value > 2)
+        return fcp_fail(ctx, FCP_ERR_ARG, "inner preconditioner must be 0, 1 or 2");
       ctx->inner_precond = value;
       return 0;
     case FCP_TUNE_SWEEP_ROWS:
@@ -769,7 +770,7 @@ int fcp_spmv(fcp_ctx *ctx, const double *x, double *y) {
 
 int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d) {
   FCP_TRY(need(ctx, true, false, true));
-  if (!r || !z || kind < 0 || kind > 4)
+  if (!r || !z || kind < 0 || kind > 6)
     return fcp_fail(ctx, FCP_ERR_ARG, "fcp_precond_apply: bad argument");
   if (kind == 2) d = nullptr;  // ILU(0) has no pivot vector to return
   (void)hipSetDevice(ctx->device);
@@ -779,6 +780,8 @@ int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double
   if (d) FCP_TRY(arg_out(ctx, d, ctx->n, dout));
   if (kind == 2)
     FCP_TRY(fcp_ilu_apply_device(ctx, ri.dev, zo.dev));
+  else if (kind >= 5)
+    FCP_TRY(fcp_bj32_device(ctx, kind - 5, ri.dev, zo.dev, dout.dev));
   else if (kind >= 3)
     FCP_TRY(fcp_bj_device(ctx, kind - 3, ri.dev, zo.dev, dout.dev));
   else

@@ -9,7 +9,7 @@
 namespace fcpk {
 
 constexpr int BLK = 256;
-constexpr int MAXBLK = 1024;  // blocks of a reduction kernel (partials per slot)
+constexpr int MAXBLK = 4096;  // blocks of a reduction kernel (partials per slot)
 
 enum Slot { SLOT_A = 0, SLOT_B = 1, SLOT_C = 2, NSLOT = 4 };
 
@@ -264,6 +264,12 @@ int pack_block32(fcp_ctx *ctx, BlockSchedule &b);
 // Pivots and packed FP32 operands of the tile-local preconditioner (schedules
 // jlower/jupper, pivots dbj); *ok false when it does not fit this mesh.
 int bj_begin(fcp_ctx *ctx, int kind, bool *ok);
+// Inner FP32 PCG in tile order (fcp_inner.hip).
+bool tin_usable(fcp_ctx *ctx, int nrp);
+int tin_pcg(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScalars *so,
+            SolveScalars *si, double tol_rel, double inner_tol, int budget, int check_every,
+            int verbose, SolveScalars *h);
+int tin_apply(fcp_ctx *ctx, const double *r, double *z);
 int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z);
 int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z);
 

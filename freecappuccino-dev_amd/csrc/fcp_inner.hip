@@ -1,0 +1,742 @@
+// FP32 inner PCG of the mixed-precision solves, run in the processing order
+// of the tile-local preconditioner ("tile order").
+//
+// Position p of the tile order is the p-th row of the tile-local forward
+// schedule: tiles one after another, rows of a tile by level of the strict
+// lower DAG (equal levels split into steps of at most BLK rows).  Every
+// neighbour pair keeps its relative order (a row's lower neighbours have
+// smaller levels), so the strict lower / upper triangles of P A P^T are the
+// permuted strict lower / upper triangles of A and the tile-local DIC / DILU
+// is the same preconditioner, with the same per-row arithmetic, as the one
+// the row-gather sweeps apply.  The backward sweep processes a tile's steps
+// in reverse (a row's upper neighbours in the tile have larger levels).
+//
+// In tile order a tile's rows, their sweep operands and their part of every
+// vector are contiguous, and a tile's preconditioner application reads only
+// the tile's own residual.  One PCG iteration is two kernels:
+//   tk_pre  (one workgroup per tile): x += alf p, r -= alf q with the norms
+//           and stopping test of the previous iteration (iccg :506-512),
+//           then z = M^-1 r with both sweeps in LDS and r.z (:458-478);
+//   tk_spmv (grid-stride rows): p = z + bet p (:488) computed on the fly
+//           for every operand, q = A p and p.q (:493-501).
+// NR right-hand sides are interleaved (NRP values per position); every
+// column's arithmetic is independent of NR, so a batched solve's columns
+// equal single-column solves bit for bit.
+#include "fcp_internal.h"
+#include "fcp_device.h"
+
+using namespace fcpk;
+
+namespace {
+
+constexpr unsigned short TK_NONE = 0xFFFF;  // unused dependency slot
+constexpr int TPD = 4;                      // sweep steps of operand prefetch
+
+template <int NR>
+__device__ __forceinline__ bool tk_all_done(const SolveScalars *s) {
+#pragma unroll
+  for (int c = 0; c < NR; ++c)
+    if (!solve_done(s + c)) return false;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// pattern and values
+// ---------------------------------------------------------------------------
+__global__ void tk_iperm(int n, const int *__restrict__ perm, int *__restrict__ iperm) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) iperm[perm[p]] = p;
+}
+
+__device__ __forceinline__ int tk_block_of(const int *__restrict__ bpos, int nb, int p) {
+  int lo = 0, hi = nb - 1;  // largest b with bpos[b] <= p
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (bpos[mid] <= p)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int2 tk_pack16(const int (&d)[FCP_PMAXD]) {
+  return make_int2((int)((unsigned)(d[0] & 0xFFFF) | ((unsigned)(d[1] & 0xFFFF) << 16)),
+                   (int)((unsigned)(d[2] & 0xFFFF) | ((unsigned)(d[3] & 0xFFFF) << 16)));
+}
+
+// Dependencies in LDS slots (16 bits) of both sweeps: the forward ones from
+// the tile-local forward schedule, the backward ones are the row's upper
+// entries (CSR order) whose column lies in the same tile.
+__global__ void tk_pattern(int n, int nb, const int *__restrict__ perm,
+                           const int *__restrict__ iperm, const int *__restrict__ bpos,
+                           const int *__restrict__ prc, const int *__restrict__ pdep,
+                           const int *__restrict__ ia, const int *__restrict__ diag,
+                           const int *__restrict__ ja, int2 *__restrict__ dl,
+                           int2 *__restrict__ du, int *__restrict__ err) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int cnt = prc[2 * p + 1] & 0xFFFF;
+  int d[FCP_PMAXD];
+#pragma unroll
+  for (int m = 0; m < FCP_PMAXD; ++m) d[m] = m < cnt ? pdep[(size_t)FCP_PMAXD * p + m] : TK_NONE;
+  dl[p] = tk_pack16(d);
+  const int b = tk_block_of(bpos, nb, p);
+  const int lo = bpos[b], hi = bpos[b + 1];
+  const int row = perm[p];
+  int m = 0;
+#pragma unroll
+  for (int k = 0; k < FCP_PMAXD; ++k) d[k] = TK_NONE;
+  for (int k = diag[row] + 1; k < ia[row + 1]; ++k) {
+    const int q = iperm[ja[k]];
+    if (q >= lo && q < hi) {
+      if (m < FCP_PMAXD) d[m] = q - lo;
+      ++m;
+    }
+  }
+  if (m > FCP_PMAXD) atomicOr(err, 1);
+  du[p] = tk_pack16(d);
+}
+
+// Coefficients (FP32 roundings of the FP64 matrix) of both sweeps and the
+// FP32 rounding of the tile-local pivots.
+__global__ void tk_sweep_vals(int n, int nb, const int *__restrict__ perm,
+                              const int *__restrict__ iperm, const int *__restrict__ bpos,
+                              const int *__restrict__ psrc, const int *__restrict__ ia,
+                              const int *__restrict__ diag, const int *__restrict__ ja,
+                              const double *__restrict__ a, const double *__restrict__ dbj,
+                              float4 *__restrict__ al, float4 *__restrict__ au,
+                              float *__restrict__ dv) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float v[FCP_PMAXD];
+#pragma unroll
+  for (int m = 0; m < FCP_PMAXD; ++m) {
+    const int k = psrc[(size_t)FCP_PMAXD * p + m];
+    v[m] = k >= 0 ? (float)a[k] : 0.0f;
+  }
+  al[p] = make_float4(v[0], v[1], v[2], v[3]);
+  const int b = tk_block_of(bpos, nb, p);
+  const int lo = bpos[b], hi = bpos[b + 1];
+  const int row = perm[p];
+  int m = 0;
+#pragma unroll
+  for (int k = 0; k < FCP_PMAXD; ++k) v[k] = 0.0f;
+  for (int k = diag[row] + 1; k < ia[row + 1] && m < FCP_PMAXD; ++k) {
+    const int q = iperm[ja[k]];
+    if (q >= lo && q < hi) v[m++] = (float)a[k];
+  }
+  au[p] = make_float4(v[0], v[1], v[2], v[3]);
+  dv[p] = (float)dbj[row];
+}
+
+// SELL-64 copy of P A P^T: slice widths, columns, values.  A row keeps its
+// CSR entry order; padding entries have value 0 and the row's own column.
+__global__ void tk_sell_len(int n, int nslices, const int *__restrict__ perm,
+                            const int *__restrict__ ia, int *__restrict__ len) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > nslices) return;
+  if (s == nslices) {
+    len[s] = 0;
+    return;
+  }
+  int w = 0;
+  for (int p = s * 64; p < s * 64 + 64 && p < n; ++p) {
+    const int row = perm[p];
+    const int l = ia[row + 1] - ia[row];
+    w = l > w ? l : w;
+  }
+  len[s] = 64 * w;
+}
+
+__global__ void tk_sell_col(int n, int nslices, const int *__restrict__ perm,
+                            const int *__restrict__ iperm, const int *__restrict__ ia,
+                            const int *__restrict__ ja, const int *__restrict__ base,
+                            int *__restrict__ col) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nslices * 64) return;
+  const int s = p >> 6, q = base[s] + (p & 63), L = (base[s + 1] - base[s]) >> 6;
+  const int row = p < n ? perm[p] : -1;
+  const int len = row >= 0 ? ia[row + 1] - ia[row] : 0;
+  for (int k = 0; k < L; ++k)
+    col[q + 64 * k] = k < len ? iperm[ja[ia[row] + k]] : (p < n ? p : 0);
+}
+
+__global__ void tk_sell_val(int n, int nslices, const int *__restrict__ perm,
+                            const int *__restrict__ ia, const double *__restrict__ a,
+                            const int *__restrict__ base, float *__restrict__ val) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nslices * 64) return;
+  const int s = p >> 6, q = base[s] + (p & 63), L = (base[s + 1] - base[s]) >> 6;
+  const int row = p < n ? perm[p] : -1;
+  const int len = row >= 0 ? ia[row + 1] - ia[row] : 0;
+  for (int k = 0; k < L; ++k) val[q + 64 * k] = k < len ? (float)a[ia[row] + k] : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// sweeps in LDS
+// ---------------------------------------------------------------------------
+struct TSlot {
+  int ok, i;  // position valid; its LDS slot
+  int2 d;     // 4 x 16-bit dependency slots
+  float4 a;
+  float dv;
+};
+
+// Operands of step st (valid: the step exists) for this thread.
+__device__ __forceinline__ void tk_load(TSlot &S, const int *sptr, int st, bool valid, int lo,
+                                        const int2 *__restrict__ dd,
+                                        const float4 *__restrict__ aa,
+                                        const float *__restrict__ dv) {
+  const int sc = valid ? st : 0;
+  const int i = sptr[sc] + (int)threadIdx.x;
+  S.ok = (valid && i < sptr[sc + 1]) ? 1 : 0;
+  const int ii = S.ok ? i : lo;
+  S.i = ii - lo;
+  S.d = dd[ii];
+  S.a = aa[ii];
+  S.dv = dv[ii];
+}
+
+// One position of a step, in place: forward (MODE 0)
+// v_i = (v_i - sum_m a_m v_dep(m)) * d_i; backward (MODE 1)
+// v_i = (v_i / (d_i + small) - sum_m a_m v_dep(m)) * d_i; dependencies in
+// CSR order, unused slots masked by a select (the row-gather sweeps' order).
+template <int NR, int NRP, int MODE>
+__device__ __forceinline__ void tk_step(const TSlot &S, float *V) {
+  if (!S.ok) return;
+  const unsigned dx = (unsigned)S.d.x, dy = (unsigned)S.d.y;
+  const int dep[FCP_PMAXD] = {(int)(dx & 0xFFFF), (int)(dx >> 16), (int)(dy & 0xFFFF),
+                              (int)(dy >> 16)};
+  const float av[FCP_PMAXD] = {S.a.x, S.a.y, S.a.z, S.a.w};
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    float acc = V[S.i * NRP + c];
+    if (MODE == 1) acc = acc / (S.dv + (float)FCP_SMALL);
+#pragma unroll
+    for (int m = 0; m < FCP_PMAXD; ++m) {
+      const bool use = dep[m] != TK_NONE;
+      const float v = V[(use ? dep[m] : 0) * NRP + c];
+      const float na = acc - av[m] * v;
+      acc = use ? na : acc;
+    }
+    V[S.i * NRP + c] = acc * S.dv;
+  }
+}
+
+// Both sweeps of one tile over V (ns >= 1 steps, step offsets in sptr).
+template <int NR, int NRP>
+__device__ __forceinline__ void tk_sweeps(float *V, const int *sptr, int ns, int lo,
+                                          const int2 *__restrict__ dl,
+                                          const float4 *__restrict__ al,
+                                          const int2 *__restrict__ du,
+                                          const float4 *__restrict__ au,
+                                          const float *__restrict__ dv) {
+  TSlot S[TPD];
+#pragma unroll
+  for (int j = 0; j < TPD; ++j) tk_load(S[j], sptr, j, j < ns, lo, dl, al, dv);
+  for (int base = 0; base < ns; base += TPD) {
+#pragma unroll
+    for (int j = 0; j < TPD; ++j) {
+      tk_step<NR, NRP, 0>(S[j], V);
+      __syncthreads();
+      const int nx = base + j + TPD;
+      tk_load(S[j], sptr, nx, nx < ns, lo, dl, al, dv);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < TPD; ++j) tk_load(S[j], sptr, ns - 1 - j, j < ns, lo, du, au, dv);
+  for (int base = 0; base < ns; base += TPD) {
+#pragma unroll
+    for (int j = 0; j < TPD; ++j) {
+      tk_step<NR, NRP, 1>(S[j], V);
+      __syncthreads();
+      const int nx = base + j + TPD;
+      tk_load(S[j], sptr, ns - 1 - nx, nx < ns, lo, du, au, dv);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// iteration kernels
+// ---------------------------------------------------------------------------
+// One workgroup per tile (tiles nb..: loop).  upd: apply the previous
+// iteration's update x += alf p, r -= alf q with |r|_1, |r|_2 and its
+// stopping test (crit: relative L2 against the column's tolerance); then
+// z = M^-1 r and sk = r.z, bet = sk / s0.  Columns already done are left
+// untouched.
+template <int NR, int NRP>
+__global__ __launch_bounds__(BLK) void tk_pre(
+    int nb, int vslots, const int *__restrict__ bpos, const int *__restrict__ blk_step,
+    const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const float4 *__restrict__ al,
+    const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
+    float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
+    const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
+    int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs,
+    int itr_max) {
+  extern __shared__ float V[];  // [vslots] values, then the step table
+  int *sptr = reinterpret_cast<int *>(V + vslots);
+  if (tk_all_done<NR>(s)) return;
+  bool act[NR];
+  float alf[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    act[c] = !solve_done(s + c);
+    alf[c] = (float)s[c].alf;
+  }
+  double part[3 * NR];
+#pragma unroll
+  for (int k = 0; k < 3 * NR; ++k) part[k] = 0.0;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int lo = bpos[b], hi = bpos[b + 1];
+    const int st0 = blk_step[b], ns = blk_step[b + 1] - st0;
+    for (int k = threadIdx.x; k <= ns; k += BLK) sptr[k] = step_ptr[st0 + k];
+    for (int i0 = lo + (int)threadIdx.x; i0 < hi; i0 += GS_U * BLK) {
+      float xv[GS_U][NR], pv[GS_U][NR], rv[GS_U][NR], qv[GS_U][NR];
+#pragma unroll
+      for (int u = 0; u < GS_U; ++u) {
+        const int i = i0 + u * BLK;
+        if (i < hi) {
+#pragma unroll
+          for (int c = 0; c < NR; ++c) {
+            const size_t o = (size_t)i * NRP + c;
+            rv[u][c] = r[o];
+            if (upd && act[c]) {
+              xv[u][c] = x[o];
+              pv[u][c] = p[o];
+              qv[u][c] = q[o];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < GS_U; ++u) {
+        const int i = i0 + u * BLK;
+        if (i < hi) {
+#pragma unroll
+          for (int c = 0; c < NR; ++c) {
+            const size_t o = (size_t)i * NRP + c;
+            float ri = rv[u][c];
+            if (upd && act[c]) {
+              x[o] = xv[u][c] + alf[c] * pv[u][c];
+              ri = ri - alf[c] * qv[u][c];
+              r[o] = ri;
+              part[3 * c] += fabs((double)ri);
+              part[3 * c + 1] += (double)ri * (double)ri;
+            }
+            V[(i - lo) * NRP + c] = ri;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (ns > 0) tk_sweeps<NR, NRP>(V, sptr, ns, lo, dl, al, du, au, dv);
+    // z out; r.z with the residual this thread wrote (or read) above
+    for (int i = lo + (int)threadIdx.x; i < hi; i += BLK) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const size_t o = (size_t)i * NRP + c;
+        const float zi = V[(i - lo) * NRP + c];
+        z[o] = zi;
+        part[3 * c + 2] += (double)r[o] * (double)zi;
+      }
+    }
+    __syncthreads();  // V and sptr are reused by the next tile
+  }
+  double blk[3 * NR], tot[3 * NR];
+  block_sum<3 * NR>(part, blk);
+  if (last_block_reduce<3 * NR>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (!act[c]) continue;
+      SolveScalars &S = s[c];
+      bool fin = false;
+      if (upd) {
+        const double resl = tot[3 * c];
+        S.resl = resl;
+        S.l2 = sqrt(tot[3 * c + 1]);
+        S.s0 = S.sk;
+        S.iter += 1;
+        S.rsm = resl / (S.res0 + FCP_SMALL);
+        bool conv = (S.l2 <= S.tol * S.l2b) || (resl < tol_abs);
+        if (!isfinite(resl)) {
+          S.status = FCP_ERR_NONFINITE;
+          conv = true;
+        }
+        S.conv = (conv && S.status == 0) ? 1 : 0;
+        if (conv || S.iter >= itr_max) {
+          S.done = 1;
+          fin = true;
+        }
+      }
+      if (!fin) {
+        S.sk = tot[3 * c + 2];
+        S.bet = S.sk / S.s0;
+      }
+    }
+  }
+}
+
+// p_new = z + bet p_old (every operand computed where it is used), q = A p_new
+// over all slice entries (padding adds 0 * p_i), pkapk = p.q, alf = sk/pkapk.
+template <int NR, int NRP>
+__global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ base,
+                                               const int *__restrict__ col,
+                                               const float *__restrict__ val,
+                                               const float *__restrict__ z,
+                                               const float *__restrict__ pold,
+                                               float *__restrict__ pnew, float *__restrict__ q,
+                                               double *__restrict__ partials,
+                                               int *__restrict__ counter,
+                                               SolveScalars *__restrict__ s) {
+  if (tk_all_done<NR>(s)) return;
+  bool act[NR];
+  float bet[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    act[c] = !solve_done(s + c);
+    bet[c] = (float)s[c].bet;
+  }
+  double part[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) part[c] = 0.0;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    const int sl = i >> 6;
+    const int b0 = base[sl];
+    const int pp = b0 + (i & 63);
+    const int L = (base[sl + 1] - b0) >> 6;
+    float av[SELL_KU];
+    int cv[SELL_KU];
+#pragma unroll
+    for (int k = 0; k < SELL_KU; ++k)
+      if (k < L) {
+        av[k] = val[pp + 64 * k];
+        cv[k] = col[pp + 64 * k];
+      }
+    float acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < SELL_KU; ++k)
+      if (k < L) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          const size_t o = (size_t)cv[k] * NRP + c;
+          const float pj = z[o] + bet[c] * pold[o];
+          acc[c] = acc[c] + av[k] * pj;
+        }
+      }
+    for (int k = SELL_KU; k < L; ++k) {
+      const float a = val[pp + 64 * k];
+      const int j = col[pp + 64 * k];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const size_t o = (size_t)j * NRP + c;
+        const float pj = z[o] + bet[c] * pold[o];
+        acc[c] = acc[c] + a * pj;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (!act[c]) continue;
+      const size_t o = (size_t)i * NRP + c;
+      const float pi = z[o] + bet[c] * pold[o];
+      pnew[o] = pi;
+      q[o] = acc[c];
+      part[c] += (double)pi * (double)acc[c];
+    }
+  }
+  double blk[NR], tot[NR];
+  block_sum<NR>(part, blk);
+  if (last_block_reduce<NR>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (!act[c]) continue;
+      s[c].pkapk = tot[c];
+      s[c].alf = s[c].sk / tot[c];
+    }
+  }
+}
+
+// r~ = P (res / |res|_2) in FP32, x~ = 0, p~ = 0, and the inner scalars of
+// every column (tolerance from the outer residual as for the row-order inner
+// solves; columns whose outer residual already meets tol_rel start done).
+template <int NR, int NRP>
+__global__ __launch_bounds__(BLK) void tk_init(int n, const int *__restrict__ perm,
+                                               const double *__restrict__ res,
+                                               const SolveScalars *__restrict__ so,
+                                               float *__restrict__ r, float *__restrict__ x,
+                                               float *__restrict__ p0,
+                                               double *__restrict__ partials,
+                                               int *__restrict__ counter,
+                                               SolveScalars *__restrict__ s, double tol_rel,
+                                               double inner_tol) {
+  double inv[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) inv[c] = so[c].l2 > 0.0 ? 1.0 / so[c].l2 : 0.0;
+  double part[2 * NR];
+#pragma unroll
+  for (int k = 0; k < 2 * NR; ++k) part[k] = 0.0;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    const size_t row = (size_t)perm[i];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      const size_t o = (size_t)i * NRP + c;
+      const float v = (float)(res[row * NRP + c] * inv[c]);
+      r[o] = v;
+      x[o] = 0.0f;
+      p0[o] = 0.0f;
+      part[2 * c] += fabs((double)v);
+      part[2 * c + 1] += (double)v * (double)v;
+    }
+  }
+  double blk[2 * NR], tot[2 * NR];
+  block_sum<2 * NR>(part, blk);
+  if (last_block_reduce<2 * NR>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      SolveScalars &S = s[c];
+      S.res0 = tot[2 * c];
+      S.resl = tot[2 * c];
+      S.l2 = sqrt(tot[2 * c + 1]);
+      S.l2b = S.l2;
+      S.s0 = FCP_S0INIT;
+      S.alf = 1.0;
+      S.beto = 1.0;
+      S.gam = 1.0;
+      S.iter = 0;
+      S.status = 0;
+      S.conv = 0;
+      S.factor = 0.0;
+      const double rel = so[c].l2 / (so[c].l2b > 0 ? so[c].l2b : 1.0);
+      double itol = 0.5 * tol_rel / rel;
+      if (itol < inner_tol) itol = inner_tol;
+      if (itol > 0.5) itol = 0.5;
+      S.tol = itol;
+      S.done = (rel <= tol_rel || so[c].l2 == 0.0) ? 1 : 0;
+    }
+  }
+}
+
+// x += |res|_2 P^T x~ (FP64, natural order)
+template <int NR, int NRP>
+__global__ void tk_ir_update(int n, const int *__restrict__ perm, const float *__restrict__ e,
+                             const SolveScalars *__restrict__ so, double *__restrict__ x) {
+  double sc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) sc[c] = so[c].l2;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    const size_t row = (size_t)perm[i];
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+      x[row * NRP + c] = x[row * NRP + c] + sc[c] * (double)e[(size_t)i * NRP + c];
+  }
+}
+
+// z = P^T M^-1 P r in FP32 for one column (the inner solves' application).
+__global__ void tk_gather_in(int n, const int *__restrict__ perm, const double *__restrict__ r,
+                             float *__restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (float)r[perm[i]];
+}
+__global__ void tk_scatter_out(int n, const int *__restrict__ perm, const float *__restrict__ v,
+                               double *__restrict__ z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[perm[i]] = (double)v[i];
+}
+
+size_t tk_lds(const fcp_ctx *ctx, int nrp) {
+  const BlockSchedule &b = ctx->jlower;
+  return (size_t)b.R * nrp * sizeof(float) + (size_t)(b.max_steps + 2) * sizeof(int);
+}
+
+int tk_grid(const fcp_ctx *ctx) {
+  const int nb = ctx->jlower.nblocks;
+  return nb < MAXBLK ? nb : MAXBLK;
+}
+
+// Pattern (once per tile-local schedule) and values (per pivot epoch).
+int tk_prepare(fcp_ctx *ctx) {
+  const int n = ctx->n;
+  const BlockSchedule &b = ctx->jlower;
+  const int g = grid_for(n, BLK);
+  if (!ctx->tin_have) {
+    FCP_HIP(ctx, ctx->tin_iperm.alloc(n));
+    FCP_HIP(ctx, ctx->tin_dl.alloc(2 * (size_t)n));
+    FCP_HIP(ctx, ctx->tin_du.alloc(2 * (size_t)n));
+    tk_iperm<<<g, BLK, 0, ctx->stream>>>(n, b.rows.p, ctx->tin_iperm.p);
+    FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
+    tk_pattern<<<g, BLK, 0, ctx->stream>>>(
+        n, b.nblocks, b.rows.p, ctx->tin_iperm.p, b.blk_pos.p, b.prc.p, b.pdep.p, ctx->ia.p,
+        ctx->diag.p, ctx->ja.p, reinterpret_cast<int2 *>(ctx->tin_dl.p),
+        reinterpret_cast<int2 *>(ctx->tin_du.p), ctx->errflag.p);
+    const int nsl = (n + 63) / 64;
+    DBuf<int> len;
+    FCP_HIP(ctx, len.alloc(nsl + 1));
+    FCP_HIP(ctx, ctx->tin_base.alloc(nsl + 1));
+    tk_sell_len<<<grid_for(nsl + 1, BLK), BLK, 0, ctx->stream>>>(n, nsl, b.rows.p, ctx->ia.p,
+                                                                 len.p);
+    FCP_TRY(fcp_exclusive_scan_int(ctx, len.p, ctx->tin_base.p, nsl + 1));
+    int npad = 0;
+    FCP_HIP(ctx, hipMemcpyAsync(&npad, ctx->tin_base.p + nsl, sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int err = 0;
+    FCP_HIP(ctx, hipMemcpy(&err, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return FCP_ERR_UNSUPPORTED;  // more upper dependencies than slots
+    FCP_HIP(ctx, ctx->tin_col.alloc(npad));
+    FCP_HIP(ctx, ctx->tin_val.alloc(npad));
+    tk_sell_col<<<grid_for(nsl * 64, BLK), BLK, 0, ctx->stream>>>(
+        n, nsl, b.rows.p, ctx->tin_iperm.p, ctx->ia.p, ctx->ja.p, ctx->tin_base.p,
+        ctx->tin_col.p);
+    FCP_HIP(ctx, hipGetLastError());
+    ctx->tin_have = true;
+    ctx->tin_epoch = ~0ull;
+  }
+  if (ctx->tin_epoch != ctx->bj_epoch) {
+    FCP_HIP(ctx, ctx->tin_al.alloc(4 * (size_t)n));
+    FCP_HIP(ctx, ctx->tin_au.alloc(4 * (size_t)n));
+    FCP_HIP(ctx, ctx->tin_dv.alloc(n));
+    tk_sweep_vals<<<g, BLK, 0, ctx->stream>>>(
+        n, b.nblocks, b.rows.p, ctx->tin_iperm.p, b.blk_pos.p, b.psrc.p, ctx->ia.p, ctx->diag.p,
+        ctx->ja.p, ctx->a.p, ctx->dbj.p, reinterpret_cast<float4 *>(ctx->tin_al.p),
+        reinterpret_cast<float4 *>(ctx->tin_au.p), ctx->tin_dv.p);
+    const int nsl = (n + 63) / 64;
+    tk_sell_val<<<grid_for(nsl * 64, BLK), BLK, 0, ctx->stream>>>(
+        n, nsl, b.rows.p, ctx->ia.p, ctx->a.p, ctx->tin_base.p, ctx->tin_val.p);
+    FCP_HIP(ctx, hipGetLastError());
+    ctx->tin_epoch = ctx->bj_epoch;
+  }
+  return 0;
+}
+
+struct TkBufs {
+  float *x, *r, *z, *q, *p[2];
+};
+
+template <int NR, int NRP>
+int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *si, int upd,
+                  int itr_max) {
+  const BlockSchedule &b = ctx->jlower;
+  ProfScope ps(ctx, "tin_pre");
+  tk_pre<NR, NRP><<<tk_grid(ctx), BLK, tk_lds(ctx, NRP), ctx->stream>>>(
+      b.nblocks, b.R * NRP, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
+      reinterpret_cast<const int2 *>(ctx->tin_dl.p),
+      reinterpret_cast<const float4 *>(ctx->tin_al.p),
+      reinterpret_cast<const int2 *>(ctx->tin_du.p),
+      reinterpret_cast<const float4 *>(ctx->tin_au.p), ctx->tin_dv.p, B.x, B.r, p, B.q, B.z,
+      ctx->partials.p, redctr(ctx), si, upd, 0.0, itr_max);
+  return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
+}
+
+template <int NR, int NRP>
+int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, SolveScalars *si,
+           double tol_rel, double inner_tol, int budget, int check_every, int verbose,
+           SolveScalars *h) {
+  const int n = ctx->n;
+  const size_t m = (size_t)n * NRP;
+  for (int k = 0; k < 6; ++k) FCP_HIP(ctx, ctx->tin_vec[k].alloc(m));
+  TkBufs B{ctx->tin_vec[0].p, ctx->tin_vec[1].p, ctx->tin_vec[2].p, ctx->tin_vec[3].p,
+           {ctx->tin_vec[4].p, ctx->tin_vec[5].p}};
+  const int g = red_grid(ctx, n);
+  tk_init<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, res, so, B.r, B.x, B.p[0],
+                                               ctx->partials.p, redctr(ctx), si, tol_rel,
+                                               inner_tol);
+  FCP_TRY((tk_launch_pre<NR, NRP>(ctx, B, B.p[0], si, 0, budget)));
+  const int ce = check_every > 0 ? check_every : 8;
+  int launched = 0, k = 0;
+  while (launched < budget) {
+    const int batch = (budget - launched) < ce ? (budget - launched) : ce;
+    for (int j = 0; j < batch; ++j, ++k) {
+      float *pold = B.p[k & 1], *pnew = B.p[(k + 1) & 1];
+      {
+        ProfScope ps(ctx, "tin_spmv");
+        tk_spmv<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->tin_base.p, ctx->tin_col.p,
+                                                     ctx->tin_val.p, B.z, pold, pnew, B.q,
+                                                     ctx->partials.p, redctr(ctx), si);
+      }
+      FCP_TRY((tk_launch_pre<NR, NRP>(ctx, B, pnew, si, 1, budget)));
+    }
+    launched += batch;
+    FCP_HIP(ctx, hipMemcpyAsync(h, si, NR * sizeof(SolveScalars), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    FCP_TRY(check_err(ctx));
+    bool all = true;
+    for (int c = 0; c < NR; ++c) {
+      if (verbose >= 2)
+        printf("                   PCG32[%d] iter = %4d resl = %10.3E rsm = %10.3E\n", c,
+               h[c].iter, h[c].resl, h[c].rsm);
+      all = all && h[c].done;
+    }
+    if (all) break;
+  }
+  tk_ir_update<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, B.x, so, x);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+namespace fcpk {
+// The tile-order inner PCG applies: tile-local preconditioner built, at most
+// FCP_PMAXD dependencies per row in either sweep, slots addressable in 16
+// bits, one tile's values and step table in LDS.
+bool tin_usable(fcp_ctx *ctx, int nrp) {
+  const BlockSchedule &b = ctx->jlower;
+  return ctx->have_bj && b.nblocks > 0 && b.maxd <= FCP_PMAXD && ctx->jupper.maxd <= FCP_PMAXD &&
+         b.R < (int)TK_NONE && tk_lds(ctx, nrp) + 256 <= ctx->lds_cap;
+}
+
+// Inner FP32 PCG of nr interleaved columns (NRP = 1, 2, 4) on the FP64
+// residual res (scaled by so[c].l2), adding the correction to x; the
+// tile-local pivots must be current (bj_begin).  h receives the final inner
+// scalars.  FCP_ERR_UNSUPPORTED: the mesh does not fit (nothing launched).
+int tin_pcg(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScalars *so,
+            SolveScalars *si, double tol_rel, double inner_tol, int budget, int check_every,
+            int verbose, SolveScalars *h) {
+  const int nrp = nr <= 1 ? 1 : (nr <= 2 ? 2 : 4);
+  if (!tin_usable(ctx, nrp)) return FCP_ERR_UNSUPPORTED;
+  const int rc = tk_prepare(ctx);
+  if (rc == FCP_ERR_UNSUPPORTED) {
+    ctx->tin_have = false;
+    return rc;
+  }
+  FCP_TRY(rc);
+  switch (nr) {
+    case 1:
+      return tk_pcg<1, 1>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every, verbose,
+                          h);
+    case 2:
+      return tk_pcg<2, 2>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every, verbose,
+                          h);
+    case 3:
+      return tk_pcg<3, 4>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every, verbose,
+                          h);
+    case 4:
+      return tk_pcg<4, 4>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every, verbose,
+                          h);
+    default:
+      return FCP_ERR_ARG;
+  }
+}
+
+// z = M^-1 r with the tile-local preconditioner in FP32 exactly as the inner
+// solves apply it (one tile-order application, pivots current).
+int tin_apply(fcp_ctx *ctx, const double *r, double *z) {
+  if (!tin_usable(ctx, 1)) return FCP_ERR_UNSUPPORTED;
+  FCP_TRY(tk_prepare(ctx));
+  const int n = ctx->n;
+  for (int k = 0; k < 6; ++k) FCP_HIP(ctx, ctx->tin_vec[k].alloc(n));
+  TkBufs B{ctx->tin_vec[0].p, ctx->tin_vec[1].p, ctx->tin_vec[2].p, ctx->tin_vec[3].p,
+           {ctx->tin_vec[4].p, ctx->tin_vec[5].p}};
+  // scalars in the spare slot: not done, unit tolerance, no update
+  SolveScalars *si = ctx->scal.p + 3;
+  FCP_HIP(ctx, hipMemsetAsync(si, 0, sizeof(SolveScalars), ctx->stream));
+  tk_gather_in<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, r, B.r);
+  FCP_TRY((tk_launch_pre<1, 1>(ctx, B, B.p[0], si, 0, 1)));
+  tk_scatter_out<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, B.z, z);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+}  // namespace fcpk

@@ -145,7 +145,17 @@ struct fcp_ctx {
   DBuf<double> dbj;                  // [n] tile-local DIC / DILU pivots (inverted)
   uint64_t bj_version = ~0ull, bj_epoch = 0;
   int bj_kind = -1;
-  int inner_precond = 1;             // 0 global DIC/DILU, 1 tile-local (FP32 inner solves)
+  int inner_precond = 1;             // 0 global DIC/DILU, 1 tile-local in tile order,
+                                     // 2 tile-local with row-gather sweeps
+  // FP32 inner PCG in the tile-local preconditioner's processing order
+  bool tin_have = false;             // pattern of the current tile-local schedule
+  uint64_t tin_epoch = ~0ull;        // pivot epoch of the packed values
+  DBuf<int> tin_iperm;               // [n] position of every row
+  DBuf<int> tin_dl, tin_du;          // [2n] 4 x 16-bit LDS slots of both sweeps
+  DBuf<float> tin_al, tin_au, tin_dv;  // [4n] coefficients, [n] pivots
+  DBuf<int> tin_base, tin_col;       // SELL-64 copy of P A P^T
+  DBuf<float> tin_val;
+  DBuf<float> tin_vec[6];            // x, r, z, q, p (two buffers)
   bool bj_active = false;            // FP32 sweeps use the tile-local preconditioner
   DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)
   bool sym_pattern = true;  // every entry has its transposed entry
@@ -320,6 +330,7 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
 int fcp_ensure_workspace(fcp_ctx *ctx);
 int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
 int fcp_bj_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
+int fcp_bj32_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
 // fcp_gmres.hip: pmgmres_ilu (restart mr) on the resident matrix; ILU(0)
 // factors; y = A^T x
 int fcp_gmres_device(fcp_ctx *ctx, double *x, const double *b, int itr_max, int mr,

@@ -1499,7 +1499,7 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
   // FP32 inner sweeps with the tile-local preconditioner when selected (the
   // FP64 refinement and any FP64 fallback keep the global one)
   bool use_bj = false;
-  if (solver != 1 && ctx->inner_precond == 1) FCP_TRY(bj_prepare(ctx, kind, &use_bj));
+  if (solver != 1 && ctx->inner_precond >= 1) FCP_TRY(bj_prepare(ctx, kind, &use_bj));
   struct BjScope {
     fcp_ctx *c;
     ~BjScope() { c->bj_active = false; }
@@ -1533,12 +1533,20 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
       FCP_TRY(true_residual(ctx, x, b, so));
       continue;
     }
+    const int budget = itr_max - total_inner;
+    // inner FP32 PCG in tile order (tile-local preconditioner)
+    if (solver == 0 && use_bj && ctx->inner_precond == 1 && tin_usable(ctx, 1)) {
+      FCP_TRY(tin_pcg(ctx, 1, ctx->res.p, x, so, si, tol_rel, inner_tol, budget, check_every,
+                      verbose, &ctx->h_scal[1]));
+      total_inner += ctx->h_scal[1].iter;
+      FCP_TRY(true_residual(ctx, x, b, so));
+      continue;
+    }
     // inner FP32 solve of A e = r/|r|
     k_scale_to32<<<g, BLK, 0, ctx->stream>>>(n, ctx->res.p, so, ctx->r32.p, ctx->e32.p);
     k_zero_f32<<<g, BLK, 0, ctx->stream>>>(n, ctx->p32.p, solver == 2 ? ctx->u32.p : ctx->q32.p);
     k_init32<<<g, BLK, 0, ctx->stream>>>(n, ctx->r32.p, solver == 2 ? ctx->reso32.p : nullptr,
                                          ctx->partials.p, redctr(ctx), si);
-    const int budget = itr_max - total_inner;
     // Reduce the correction residual only as far as this refinement step
     // needs (with a 0.5 margin), never below inner_tol.
     double itol = 0.5 * tol_rel / rel;
@@ -1611,6 +1619,26 @@ int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z) {
   return precond_apply<float>(ctx, ctx->a32.p, ctx->d32.p, r, y, z, nullptr);
 }
 }  // namespace fcpk
+
+// Tile-local DIC (kind 0) / DILU (kind 1) applied in FP32 by the tile-order
+// kernel of the inner solves (input rounded to FP32, output widened); d_out
+// (optional) receives the FP64 pivots.
+int fcp_bj32_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out) {
+  FCP_TRY(fcp_ensure_workspace(ctx));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int), ctx->stream));
+  bool ok = false;
+  FCP_TRY(bj_prepare(ctx, kind, &ok));
+  const int rc = ok ? tin_apply(ctx, r, z) : FCP_ERR_UNSUPPORTED;
+  if (rc == FCP_ERR_UNSUPPORTED)
+    return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "tile-order preconditioner does not fit this mesh");
+  FCP_TRY(rc);
+  if (d_out)
+    FCP_HIP(ctx, hipMemcpyAsync(d_out, ctx->dbj.p, sizeof(double) * ctx->n,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+  FCP_TRY(check_err(ctx));
+  return 0;
+}
 
 // Tile-local DIC (kind 0) / DILU (kind 1) applied in FP64, for checking the
 // FP32 inner preconditioner; d_out (optional) receives its pivots.

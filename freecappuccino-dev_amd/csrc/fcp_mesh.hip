@@ -744,6 +744,7 @@ int fcp_build_tile_schedules(fcp_ctx *ctx) {
 // another), else runs of 4096 contiguous rows.
 int fcp_build_bj_schedules(fcp_ctx *ctx) {
   ctx->have_bj = false;
+  ctx->tin_have = false;
   const int n = ctx->n;
   const int target = ctx->tune_sweep_tile >= 64 ? ctx->tune_sweep_tile : 4096;
   std::vector<int> rank;

@@ -1108,7 +1108,7 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
   // inner sweeps use the tile-local preconditioner when selected (the FP64
   // refinement and the FP64 finish keep the global one)
   bool use_bj = false;
-  if (ctx->inner_precond == 1) FCP_TRY(bj_begin(ctx, kind, &use_bj));
+  if (ctx->inner_precond >= 1) FCP_TRY(bj_begin(ctx, kind, &use_bj));
   struct BjScope {
     fcp_ctx *c;
     ~BjScope() { c->bj_active = false; }
@@ -1156,6 +1156,19 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
         };
         FCP_TRY(miterate<NR>(ctx, so, h, itr_max - total, check_every, verbose, "PCG", it));
       }
+      int mx = 0;
+      for (int c = 0; c < NR; ++c) {
+        res.iters[c] += h[c].iter;
+        mx = h[c].iter > mx ? h[c].iter : mx;
+      }
+      total += mx;
+      FCP_TRY(residual(nullptr));
+      continue;
+    }
+    if (solver == 0 && use_bj && ctx->inner_precond == 1 && tin_usable(ctx, NRP)) {
+      // inner FP32 PCG in tile order (tile-local preconditioner), x updated
+      FCP_TRY(tin_pcg(ctx, NR, V64.r, x, so, si, tol_rel, inner_tol, itr_max - total,
+                      check_every, verbose, h));
       int mx = 0;
       for (int c = 0; c < NR; ++c) {
         res.iters[c] += h[c].iter;
