@@ -264,6 +264,15 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
         if (ctx->have_csr) return fcp_build_tile_schedules(ctx);
       }
       return 0;
+    case FCP_TUNE_INNER_TILE:
+      if (value != 0 && (value < 64 || value > 32768))
+        return fcp_fail(ctx, FCP_ERR_ARG, "inner tile rows must be 0 or in [64, 32768]");
+      if (value != ctx->tune_inner_tile) {
+        ctx->tune_inner_tile = value;
+        ctx->have_bj = false;
+        ctx->tin_have = false;
+      }
+      return 0;
     case FCP_TUNE_INNER_PRECOND:
       if (value < 0 || value > 2)
         return fcp_fail(ctx, FCP_ERR_ARG, "inner preconditioner must be 0, 1 or 2");

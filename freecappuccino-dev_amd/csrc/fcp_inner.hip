@@ -30,7 +30,10 @@ using namespace fcpk;
 namespace {
 
 constexpr unsigned short TK_NONE = 0xFFFF;  // unused dependency slot
-constexpr int TPD = 4;                      // sweep steps of operand prefetch
+#ifndef FCP_TPD
+#define FCP_TPD 4
+#endif
+constexpr int TPD = FCP_TPD;  // sweep steps of operand prefetch
 
 template <int NR>
 __device__ __forceinline__ bool tk_all_done(const SolveScalars *s) {
@@ -183,14 +186,14 @@ struct TSlot {
   float dv;
 };
 
-// Operands of step st (valid: the step exists) for this thread.
-__device__ __forceinline__ void tk_load(TSlot &S, const int *sptr, int st, bool valid, int lo,
+// Operands for this thread of the step with position bounds sb (valid: the
+// step exists).
+__device__ __forceinline__ void tk_load(TSlot &S, int2 sb, bool valid, int lo,
                                         const int2 *__restrict__ dd,
                                         const float4 *__restrict__ aa,
                                         const float *__restrict__ dv) {
-  const int sc = valid ? st : 0;
-  const int i = sptr[sc] + (int)threadIdx.x;
-  S.ok = (valid && i < sptr[sc + 1]) ? 1 : 0;
+  const int i = sb.x + (int)threadIdx.x;
+  S.ok = (valid && i < sb.y) ? 1 : 0;
   const int ii = S.ok ? i : lo;
   S.i = ii - lo;
   S.d = dd[ii];
@@ -224,9 +227,12 @@ __device__ __forceinline__ void tk_step(const TSlot &S, float *V) {
   }
 }
 
-// Both sweeps of one tile over V (ns >= 1 steps, step offsets in sptr).
+// Both sweeps of one tile over V (ns >= 1 steps; sb[k] = position bounds of
+// step k, in LDS).  Operands are prefetched TPD steps ahead; the bounds of
+// the step to prefetch are read one step early (always from a valid slot),
+// so a step's critical path holds only its own LDS reads and write.
 template <int NR, int NRP>
-__device__ __forceinline__ void tk_sweeps(float *V, const int *sptr, int ns, int lo,
+__device__ __forceinline__ void tk_sweeps(float *V, const int2 *sb, int ns, int lo,
                                           const int2 *__restrict__ dl,
                                           const float4 *__restrict__ al,
                                           const int2 *__restrict__ du,
@@ -234,25 +240,32 @@ __device__ __forceinline__ void tk_sweeps(float *V, const int *sptr, int ns, int
                                           const float *__restrict__ dv) {
   TSlot S[TPD];
 #pragma unroll
-  for (int j = 0; j < TPD; ++j) tk_load(S[j], sptr, j, j < ns, lo, dl, al, dv);
+  for (int j = 0; j < TPD; ++j) tk_load(S[j], sb[j < ns ? j : 0], j < ns, lo, dl, al, dv);
+  int2 nxt = sb[TPD < ns ? TPD : 0];
   for (int base = 0; base < ns; base += TPD) {
 #pragma unroll
     for (int j = 0; j < TPD; ++j) {
+      const int nx = base + j + TPD;
+      const int2 nn = sb[nx + 1 < ns ? nx + 1 : 0];
       tk_step<NR, NRP, 0>(S[j], V);
       __syncthreads();
-      const int nx = base + j + TPD;
-      tk_load(S[j], sptr, nx, nx < ns, lo, dl, al, dv);
+      tk_load(S[j], nxt, nx < ns, lo, dl, al, dv);
+      nxt = nn;
     }
   }
 #pragma unroll
-  for (int j = 0; j < TPD; ++j) tk_load(S[j], sptr, ns - 1 - j, j < ns, lo, du, au, dv);
+  for (int j = 0; j < TPD; ++j)
+    tk_load(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv);
+  nxt = sb[TPD < ns ? ns - 1 - TPD : 0];
   for (int base = 0; base < ns; base += TPD) {
 #pragma unroll
     for (int j = 0; j < TPD; ++j) {
+      const int nx = base + j + TPD;
+      const int2 nn = sb[nx + 1 < ns ? ns - 2 - nx : 0];
       tk_step<NR, NRP, 1>(S[j], V);
       __syncthreads();
-      const int nx = base + j + TPD;
-      tk_load(S[j], sptr, ns - 1 - nx, nx < ns, lo, du, au, dv);
+      tk_load(S[j], nxt, nx < ns, lo, du, au, dv);
+      nxt = nn;
     }
   }
 }
@@ -274,8 +287,8 @@ __global__ __launch_bounds__(BLK) void tk_pre(
     const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
     int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs,
     int itr_max) {
-  extern __shared__ float V[];  // [vslots] values, then the step table
-  int *sptr = reinterpret_cast<int *>(V + vslots);
+  extern __shared__ float V[];  // [vslots] values, then the step bounds
+  int2 *sb = reinterpret_cast<int2 *>(V + vslots);
   if (tk_all_done<NR>(s)) return;
   bool act[NR];
   float alf[NR];
@@ -290,7 +303,8 @@ __global__ __launch_bounds__(BLK) void tk_pre(
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
     const int lo = bpos[b], hi = bpos[b + 1];
     const int st0 = blk_step[b], ns = blk_step[b + 1] - st0;
-    for (int k = threadIdx.x; k <= ns; k += BLK) sptr[k] = step_ptr[st0 + k];
+    for (int k = threadIdx.x; k < ns; k += BLK)
+      sb[k] = make_int2(step_ptr[st0 + k], step_ptr[st0 + k + 1]);
     for (int i0 = lo + (int)threadIdx.x; i0 < hi; i0 += GS_U * BLK) {
       float xv[GS_U][NR], pv[GS_U][NR], rv[GS_U][NR], qv[GS_U][NR];
 #pragma unroll
@@ -330,7 +344,7 @@ __global__ __launch_bounds__(BLK) void tk_pre(
       }
     }
     __syncthreads();
-    if (ns > 0) tk_sweeps<NR, NRP>(V, sptr, ns, lo, dl, al, du, au, dv);
+    if (ns > 0) tk_sweeps<NR, NRP>(V, sb, ns, lo, dl, al, du, au, dv);
     // z out; r.z with the residual this thread wrote (or read) above
     for (int i = lo + (int)threadIdx.x; i < hi; i += BLK) {
 #pragma unroll
@@ -341,7 +355,7 @@ __global__ __launch_bounds__(BLK) void tk_pre(
         part[3 * c + 2] += (double)r[o] * (double)zi;
       }
     }
-    __syncthreads();  // V and sptr are reused by the next tile
+    __syncthreads();  // V and sb are reused by the next tile
   }
   double blk[3 * NR], tot[3 * NR];
   block_sum<3 * NR>(part, blk);
@@ -547,7 +561,8 @@ __global__ void tk_scatter_out(int n, const int *__restrict__ perm, const float 
 
 size_t tk_lds(const fcp_ctx *ctx, int nrp) {
   const BlockSchedule &b = ctx->jlower;
-  return (size_t)b.R * nrp * sizeof(float) + (size_t)(b.max_steps + 2) * sizeof(int);
+  // values, then 8-byte aligned step bounds
+  return (size_t)((b.R * nrp + 1) & ~1) * sizeof(float) + (size_t)(b.max_steps + 1) * sizeof(int2);
 }
 
 int tk_grid(const fcp_ctx *ctx) {
@@ -620,7 +635,7 @@ int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *s
   const BlockSchedule &b = ctx->jlower;
   ProfScope ps(ctx, "tin_pre");
   tk_pre<NR, NRP><<<tk_grid(ctx), BLK, tk_lds(ctx, NRP), ctx->stream>>>(
-      b.nblocks, b.R * NRP, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
+      b.nblocks, (b.R * NRP + 1) & ~1, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
       reinterpret_cast<const int2 *>(ctx->tin_dl.p),
       reinterpret_cast<const float4 *>(ctx->tin_al.p),
       reinterpret_cast<const int2 *>(ctx->tin_du.p),

@@ -212,6 +212,7 @@ struct fcp_ctx {
   int tune_sweep_kind = 0;     // 0 row-block (LDS) sweeps, 1 level-task sweeps
   int tune_sweep_rows = 0;     // rows per block of the row-block sweeps (0 = auto)
   int tune_sweep_tile = 0;     // rows per geometric tile (0 = auto, 1 = no tiles)
+  int tune_inner_tile = 0;     // rows per tile of the tile-local preconditioner (0 = auto)
   size_t lds_cap = 65536;      // LDS bytes available to one workgroup
   int block_rows_cap[2] = {65536, 65536};  // rows per contiguous block (lower, upper sweep)
 

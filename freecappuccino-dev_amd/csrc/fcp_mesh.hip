@@ -746,7 +746,7 @@ int fcp_build_bj_schedules(fcp_ctx *ctx) {
   ctx->have_bj = false;
   ctx->tin_have = false;
   const int n = ctx->n;
-  const int target = ctx->tune_sweep_tile >= 64 ? ctx->tune_sweep_tile : 4096;
+  const int target = ctx->tune_inner_tile >= 64 ? ctx->tune_inner_tile : 4096;
   std::vector<int> rank;
   int m = 0;
   if (ctx->have_geom && ctx->tune_sweep_tile != 1 && n >= 4 * target) {

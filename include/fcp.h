@@ -341,6 +341,8 @@ int fcp_pmgmres(int n, int nnz, const int *ia, const int *ja, const double *a,
                                     run at once), 0 the global DIC/DILU of the
                                     FP64 solve.  FP64 solves and the FP64
                                     refinement always use the global one.      */
+#define FCP_TUNE_INNER_TILE 7    /* rows per tile of the tile-local
+                                    preconditioner (0 = auto: 4096)            */
 /* Sets a launch-shape parameter of the context; results do not depend on it
  * (FCP_TUNE_INNER_PRECOND changes the path of a mixed-precision solve, not
  * its stopping test). */
