@@ -66,6 +66,12 @@ def alg_bytes(name, n, nnz, Fi, Fb, local=None):
         "sweep_fwd_f64": lower * 12 + 4 * n + 3 * n * 8,
         "sweep_bwd_f64": lower * 12 + 4 * n + 3 * n * 8,
         "dic_setup": lower * 12 + 4 * n + 2 * n * 8,
+        # inner PCG in tile order: per position x, p, r, q read and x, r, z
+        # written, the pivot, 2 x (4-byte coefficient + 2-byte slot) per
+        # in-tile dependency; SpMV values + columns, z and p read, p and q
+        # written
+        "tin_pre": 32 * n + 12 * local if local is not None else None,
+        "tin_spmv": nnz * 8 + 4 * n * 4,
         "bj_sweep_fwd_f32": local * 8 + 4 * n + 3 * n * 4 if local is not None else None,
         "bj_sweep_bwd_f32": local * 8 + 4 * n + 3 * n * 4 if local is not None else None,
         "residual_f64": nnz * 12 + (n + 1) * 4 + 3 * n * 8,

@@ -264,6 +264,10 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
         if (ctx->have_csr) return fcp_build_tile_schedules(ctx);
       }
       return 0;
+    case FCP_TUNE_GRAPH:
+      if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "graph replay is 0 or 1");
+      ctx->tune_graph = value;
+      return 0;
     case FCP_TUNE_INNER_TILE:
       if (value != 0 && (value < 64 || value > 32768))
         return fcp_fail(ctx, FCP_ERR_ARG, "inner tile rows must be 0 or in [64, 32768]");
@@ -324,6 +328,7 @@ void fcp_finalize(fcp_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
+  if (ctx->tin_graph) (void)hipGraphExecDestroy(ctx->tin_graph);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

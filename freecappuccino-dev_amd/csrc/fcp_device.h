@@ -9,7 +9,11 @@
 namespace fcpk {
 
 constexpr int BLK = 256;
-constexpr int MAXBLK = 4096;  // blocks of a reduction kernel (partials per slot)
+constexpr int MAXBLK = 4096;
+// Arrival counters of the block reductions: RED_GROUPS sub-counters and one
+// top counter, each on its own 128-byte line, after the two sweep tickets.
+constexpr int RED_GROUPS = 32;
+constexpr int CTR_INTS = 32 + (RED_GROUPS + 1) * 32;  // blocks of a reduction kernel (partials per slot)
 
 enum Slot { SLOT_A = 0, SLOT_B = 1, SLOT_C = 2, NSLOT = 4 };
 
@@ -44,9 +48,14 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (&out)[NV]) {
 // Publish NV block partials; returns true in the single last-arriving block,
 // whose thread 0 then holds the totals (summed in block order) in tot[].
 // The partials are written with agent-coherent stores and read back with
-// agent-coherent loads; waiting for the stores to complete before the
-// counter increment orders them without a device fence (which would write
-// back every dirty L2 line in every block).
+// agent-coherent loads; waiting for the stores to complete before counting
+// the arrival orders them without a device fence (which would write back
+// every dirty L2 line in every block).  Arrivals are counted in two levels
+// (block b at sub-counter b % RED_GROUPS, the last of each group at the top
+// counter), so no single address takes more than ~gridDim / RED_GROUPS
+// serialized atomics; every counter is reset by its last arrival.  counter
+// points to (RED_GROUPS + 1) lines of 32 ints: the top counter, then the
+// sub-counters.
 template <int NV>
 __device__ bool last_block_reduce(double (&part)[NV], double *__restrict__ partials, int slot,
                                   int *__restrict__ counter, double (&tot)[NV]) {
@@ -59,8 +68,19 @@ __device__ bool last_block_reduce(double (&part)[NV], double *__restrict__ parti
           (unsigned long long)__double_as_longlong(part[i]), __ATOMIC_RELAXED,
           __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);  // every memory counter at zero: the stores are complete
-    const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (prev == (int)gridDim.x - 1);
+    const int nb = (int)gridDim.x;
+    const int g = (int)blockIdx.x % RED_GROUPS;
+    const int members = (nb - g + RED_GROUPS - 1) / RED_GROUPS;
+    const int groups = nb < RED_GROUPS ? nb : RED_GROUPS;
+    int *sub = counter + 32 * (1 + g);
+    int last = 0;
+    if (__hip_atomic_fetch_add(sub, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
+      // every member's stores completed before its arrival was counted
+      __hip_atomic_store(sub, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             groups - 1;
+    }
+    is_last = last;
   }
   __syncthreads();
   if (!is_last) return false;
@@ -75,7 +95,7 @@ __device__ bool last_block_reduce(double (&part)[NV], double *__restrict__ parti
     loc[i] = s;
   }
   block_sum<NV>(loc, tot);
-  if (threadIdx.x == 0) *counter = 0;
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return true;
 }
 
@@ -255,9 +275,9 @@ int red_grid(fcp_ctx *ctx, int n);
 int sell_view64(fcp_ctx *ctx, Sell<double> *out);
 int sell_view32(fcp_ctx *ctx, Sell<float> *out);
 inline int *tickets(fcp_ctx *ctx) { return ctx->counters.p; }
-inline int *redctr(fcp_ctx *ctx) { return ctx->counters.p + 2; }
+inline int *redctr(fcp_ctx *ctx) { return ctx->counters.p + 32; }
 int check_err(fcp_ctx *ctx);
-int prepare_precond(fcp_ctx *ctx, int kind, bool fp32);
+int prepare_precond(fcp_ctx *ctx, int kind, bool fp32, bool pivots = true);
 bool sched_fits(fcp_ctx *ctx, const BlockSchedule &lo, const BlockSchedule &up, size_t row_bytes);
 int pack_block64(fcp_ctx *ctx, BlockSchedule &b);
 int pack_block32(fcp_ctx *ctx, BlockSchedule &b);

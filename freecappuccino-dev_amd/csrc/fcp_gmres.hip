@@ -496,7 +496,7 @@ int fcp_gmres_device(fcp_ctx *ctx, double *x, const double *b, int itr_max, int 
   const SolveScalars *gate = &st->gate;
   double *V = ctx->gm_v.p;
   FCP_HIP(ctx, hipMemsetAsync(st, 0, sizeof(GmState), ctx->stream));
-  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int), ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, CTR_INTS * sizeof(int), ctx->stream));
   FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
   Sell<double> m;
   FCP_TRY(sell_view64(ctx, &m));

@@ -33,6 +33,10 @@ constexpr unsigned short TK_NONE = 0xFFFF;  // unused dependency slot
 #ifndef FCP_TPD
 #define FCP_TPD 4
 #endif
+#ifndef FCP_TK_U
+#define FCP_TK_U 1
+#endif
+constexpr int TK_U = FCP_TK_U;  // SpMV rows per thread in flight together
 constexpr int TPD = FCP_TPD;  // sweep steps of operand prefetch
 
 template <int NR>
@@ -187,7 +191,9 @@ struct TSlot {
 };
 
 // Operands for this thread of the step with position bounds sb (valid: the
-// step exists).
+// step exists).  Lanes without a position load the tile's first one: a
+// branch around the loads would make the compiler drain every outstanding
+// prefetch where the paths join.
 __device__ __forceinline__ void tk_load(TSlot &S, int2 sb, bool valid, int lo,
                                         const int2 *__restrict__ dd,
                                         const float4 *__restrict__ aa,
@@ -275,7 +281,8 @@ __device__ __forceinline__ void tk_sweeps(float *V, const int2 *sb, int ns, int 
 // ---------------------------------------------------------------------------
 // One workgroup per tile (tiles nb..: loop).  upd: apply the previous
 // iteration's update x += alf p, r -= alf q with |r|_1, |r|_2 and its
-// stopping test (crit: relative L2 against the column's tolerance); then
+// stopping test (relative L2 against the column's tolerance, at most
+// s->imax iterations); then
 // z = M^-1 r and sk = r.z, bet = sk / s0.  Columns already done are left
 // untouched.
 template <int NR, int NRP>
@@ -285,8 +292,7 @@ __global__ __launch_bounds__(BLK) void tk_pre(
     const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
     float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
     const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
-    int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs,
-    int itr_max) {
+    int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs) {
   extern __shared__ float V[];  // [vslots] values, then the step bounds
   int2 *sb = reinterpret_cast<int2 *>(V + vslots);
   if (tk_all_done<NR>(s)) return;
@@ -378,7 +384,7 @@ __global__ __launch_bounds__(BLK) void tk_pre(
           conv = true;
         }
         S.conv = (conv && S.status == 0) ? 1 : 0;
-        if (conv || S.iter >= itr_max) {
+        if (conv || S.iter >= S.imax) {
           S.done = 1;
           fin = true;
         }
@@ -414,50 +420,75 @@ __global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ ba
   double part[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) part[c] = 0.0;
-  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
-    const int sl = i >> 6;
-    const int b0 = base[sl];
-    const int pp = b0 + (i & 63);
-    const int L = (base[sl + 1] - b0) >> 6;
-    float av[SELL_KU];
-    int cv[SELL_KU];
+  // TK_U rows per thread at a time, in the plain grid-stride order (the
+  // per-thread sums are those of one row at a time); all their matrix loads,
+  // then all their gathers, are in flight together
+  const int stride = gridDim.x * BLK;
+  for (int i0 = blockIdx.x * BLK + threadIdx.x; i0 < n; i0 += TK_U * stride) {
+    int pp[TK_U], L[TK_U];
 #pragma unroll
-    for (int k = 0; k < SELL_KU; ++k)
-      if (k < L) {
-        av[k] = val[pp + 64 * k];
-        cv[k] = col[pp + 64 * k];
-      }
-    float acc[NR];
-#pragma unroll
-    for (int c = 0; c < NR; ++c) acc[c] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < SELL_KU; ++k)
-      if (k < L) {
-#pragma unroll
-        for (int c = 0; c < NR; ++c) {
-          const size_t o = (size_t)cv[k] * NRP + c;
-          const float pj = z[o] + bet[c] * pold[o];
-          acc[c] = acc[c] + av[k] * pj;
-        }
-      }
-    for (int k = SELL_KU; k < L; ++k) {
-      const float a = val[pp + 64 * k];
-      const int j = col[pp + 64 * k];
-#pragma unroll
-      for (int c = 0; c < NR; ++c) {
-        const size_t o = (size_t)j * NRP + c;
-        const float pj = z[o] + bet[c] * pold[o];
-        acc[c] = acc[c] + a * pj;
+    for (int u = 0; u < TK_U; ++u) {
+      const int i = i0 + u * stride;
+      L[u] = 0;
+      pp[u] = 0;
+      if (i < n) {
+        const int sl = i >> 6;
+        const int b0 = base[sl];
+        pp[u] = b0 + (i & 63);
+        L[u] = (base[sl + 1] - b0) >> 6;
       }
     }
+    float av[TK_U][SELL_KU];
+    int cv[TK_U][SELL_KU];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) {
-      if (!act[c]) continue;
-      const size_t o = (size_t)i * NRP + c;
-      const float pi = z[o] + bet[c] * pold[o];
-      pnew[o] = pi;
-      q[o] = acc[c];
-      part[c] += (double)pi * (double)acc[c];
+    for (int u = 0; u < TK_U; ++u)
+#pragma unroll
+      for (int k = 0; k < SELL_KU; ++k)
+        if (k < L[u]) {
+          av[u][k] = val[pp[u] + 64 * k];
+          cv[u][k] = col[pp[u] + 64 * k];
+        }
+    float acc[TK_U][NR];
+#pragma unroll
+    for (int u = 0; u < TK_U; ++u)
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[u][c] = 0.0f;
+#pragma unroll
+    for (int u = 0; u < TK_U; ++u)
+#pragma unroll
+      for (int k = 0; k < SELL_KU; ++k)
+        if (k < L[u]) {
+#pragma unroll
+          for (int c = 0; c < NR; ++c) {
+            const size_t o = (size_t)cv[u][k] * NRP + c;
+            const float pj = z[o] + bet[c] * pold[o];
+            acc[u][c] = acc[u][c] + av[u][k] * pj;
+          }
+        }
+#pragma unroll
+    for (int u = 0; u < TK_U; ++u) {
+      for (int k = SELL_KU; k < L[u]; ++k) {
+        const float a = val[pp[u] + 64 * k];
+        const int j = col[pp[u] + 64 * k];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          const size_t o = (size_t)j * NRP + c;
+          const float pj = z[o] + bet[c] * pold[o];
+          acc[u][c] = acc[u][c] + a * pj;
+        }
+      }
+      const int i = i0 + u * stride;
+      if (i < n) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          if (!act[c]) continue;
+          const size_t o = (size_t)i * NRP + c;
+          const float pi = z[o] + bet[c] * pold[o];
+          pnew[o] = pi;
+          q[o] = acc[u][c];
+          part[c] += (double)pi * (double)acc[u][c];
+        }
+      }
     }
   }
   double blk[NR], tot[NR];
@@ -484,7 +515,7 @@ __global__ __launch_bounds__(BLK) void tk_init(int n, const int *__restrict__ pe
                                                double *__restrict__ partials,
                                                int *__restrict__ counter,
                                                SolveScalars *__restrict__ s, double tol_rel,
-                                               double inner_tol) {
+                                               double inner_tol, int imax) {
   double inv[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) inv[c] = so[c].l2 > 0.0 ? 1.0 / so[c].l2 : 0.0;
@@ -528,6 +559,7 @@ __global__ __launch_bounds__(BLK) void tk_init(int n, const int *__restrict__ pe
       if (itol > 0.5) itol = 0.5;
       S.tol = itol;
       S.done = (rel <= tol_rel || so[c].l2 == 0.0) ? 1 : 0;
+      S.imax = imax;
     }
   }
 }
@@ -563,6 +595,16 @@ size_t tk_lds(const fcp_ctx *ctx, int nrp) {
   const BlockSchedule &b = ctx->jlower;
   // values, then 8-byte aligned step bounds
   return (size_t)((b.R * nrp + 1) & ~1) * sizeof(float) + (size_t)(b.max_steps + 1) * sizeof(int2);
+}
+
+#ifndef FCP_TK_SPMV_BLOCKS
+#define FCP_TK_SPMV_BLOCKS 1024
+#endif
+// Workgroups of the SpMV (grid-stride beyond that).
+int tk_spmv_grid(const fcp_ctx *ctx) {
+  const int g = grid_for(ctx->n, BLK);
+  const int cap = FCP_TK_SPMV_BLOCKS < MAXBLK ? FCP_TK_SPMV_BLOCKS : MAXBLK;
+  return g < cap ? (g > 0 ? g : 1) : cap;
 }
 
 int tk_grid(const fcp_ctx *ctx) {
@@ -630,8 +672,7 @@ struct TkBufs {
 };
 
 template <int NR, int NRP>
-int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *si, int upd,
-                  int itr_max) {
+int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *si, int upd) {
   const BlockSchedule &b = ctx->jlower;
   ProfScope ps(ctx, "tin_pre");
   tk_pre<NR, NRP><<<tk_grid(ctx), BLK, tk_lds(ctx, NRP), ctx->stream>>>(
@@ -640,8 +681,56 @@ int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *s
       reinterpret_cast<const float4 *>(ctx->tin_al.p),
       reinterpret_cast<const int2 *>(ctx->tin_du.p),
       reinterpret_cast<const float4 *>(ctx->tin_au.p), ctx->tin_dv.p, B.x, B.r, p, B.q, B.z,
-      ctx->partials.p, redctr(ctx), si, upd, 0.0, itr_max);
+      ctx->partials.p, redctr(ctx), si, upd, 0.0);
   return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
+}
+
+// Inner iteration k: p_new = z + bet p_old, q = A p_new (tk_spmv), then the
+// update, stopping test and next preconditioner application (tk_pre).
+template <int NR, int NRP>
+int tk_iteration(fcp_ctx *ctx, const TkBufs &B, SolveScalars *si, int k) {
+  float *pold = B.p[k & 1], *pnew = B.p[(k + 1) & 1];
+  {
+    ProfScope ps(ctx, "tin_spmv");
+    tk_spmv<NR, NRP><<<tk_spmv_grid(ctx), BLK, 0, ctx->stream>>>(
+        ctx->n, ctx->tin_base.p, ctx->tin_col.p, ctx->tin_val.p, B.z, pold, pnew, B.q,
+        ctx->partials.p, redctr(ctx), si);
+  }
+  return tk_launch_pre<NR, NRP>(ctx, B, pnew, si, 1);
+}
+
+// Executable graph of two inner iterations (k even, then odd) for the current
+// buffers, captured once and reused while every pointer it holds is unchanged.
+template <int NR, int NRP>
+int tk_graph(fcp_ctx *ctx, const TkBufs &B, SolveScalars *si, hipGraphExec_t *out) {
+  const BlockSchedule &b = ctx->jlower;
+  const std::vector<const void *> key = {
+      (const void *)(size_t)NR, B.x, B.r, B.z, B.q, B.p[0], B.p[1], si, ctx->partials.p,
+      ctx->counters.p, ctx->tin_dl.p, ctx->tin_du.p, ctx->tin_al.p, ctx->tin_au.p, ctx->tin_dv.p,
+      ctx->tin_base.p, ctx->tin_col.p, ctx->tin_val.p, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
+      (const void *)(size_t)b.nblocks, (const void *)(size_t)b.R, (const void *)(size_t)b.max_steps,
+      (const void *)(size_t)ctx->n};
+  if (ctx->tin_graph && ctx->tin_graph_key == key) {
+    *out = ctx->tin_graph;
+    return 0;
+  }
+  if (ctx->tin_graph) {
+    FCP_HIP(ctx, hipGraphExecDestroy(ctx->tin_graph));
+    ctx->tin_graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  FCP_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = tk_iteration<NR, NRP>(ctx, B, si, 0);
+  if (rc == 0) rc = tk_iteration<NR, NRP>(ctx, B, si, 1);
+  const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+  FCP_TRY(rc);
+  FCP_HIP(ctx, ec);
+  const hipError_t ei = hipGraphInstantiate(&ctx->tin_graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  FCP_HIP(ctx, ei);
+  ctx->tin_graph_key = key;
+  *out = ctx->tin_graph;
+  return 0;
 }
 
 template <int NR, int NRP>
@@ -656,22 +745,18 @@ int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, S
   const int g = red_grid(ctx, n);
   tk_init<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, res, so, B.r, B.x, B.p[0],
                                                ctx->partials.p, redctr(ctx), si, tol_rel,
-                                               inner_tol);
-  FCP_TRY((tk_launch_pre<NR, NRP>(ctx, B, B.p[0], si, 0, budget)));
+                                               inner_tol, budget);
+  FCP_TRY((tk_launch_pre<NR, NRP>(ctx, B, B.p[0], si, 0)));
   const int ce = check_every > 0 ? check_every : 8;
+  hipGraphExec_t ge = nullptr;
+  if (ctx->tune_graph && !ctx->prof) FCP_TRY((tk_graph<NR, NRP>(ctx, B, si, &ge)));
   int launched = 0, k = 0;
   while (launched < budget) {
     const int batch = (budget - launched) < ce ? (budget - launched) : ce;
-    for (int j = 0; j < batch; ++j, ++k) {
-      float *pold = B.p[k & 1], *pnew = B.p[(k + 1) & 1];
-      {
-        ProfScope ps(ctx, "tin_spmv");
-        tk_spmv<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->tin_base.p, ctx->tin_col.p,
-                                                     ctx->tin_val.p, B.z, pold, pnew, B.q,
-                                                     ctx->partials.p, redctr(ctx), si);
-      }
-      FCP_TRY((tk_launch_pre<NR, NRP>(ctx, B, pnew, si, 1, budget)));
-    }
+    int j = 0;
+    if (ge && (k & 1) == 0)  // the graph starts from an even iteration
+      for (; j + 2 <= batch; j += 2, k += 2) FCP_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
+    for (; j < batch; ++j, ++k) FCP_TRY((tk_iteration<NR, NRP>(ctx, B, si, k)));
     launched += batch;
     FCP_HIP(ctx, hipMemcpyAsync(h, si, NR * sizeof(SolveScalars), hipMemcpyDeviceToHost,
                                 ctx->stream));
@@ -749,7 +834,7 @@ int tin_apply(fcp_ctx *ctx, const double *r, double *z) {
   SolveScalars *si = ctx->scal.p + 3;
   FCP_HIP(ctx, hipMemsetAsync(si, 0, sizeof(SolveScalars), ctx->stream));
   tk_gather_in<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, r, B.r);
-  FCP_TRY((tk_launch_pre<1, 1>(ctx, B, B.p[0], si, 0, 1)));
+  FCP_TRY((tk_launch_pre<1, 1>(ctx, B, B.p[0], si, 0)));
   tk_scatter_out<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, B.z, z);
   FCP_HIP(ctx, hipGetLastError());
   return 0;

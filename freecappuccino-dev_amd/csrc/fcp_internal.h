@@ -94,6 +94,7 @@ struct SolveScalars {
   double l2, l2b, om, gam, beto, ukreso, vv, vr;
   double tol;  // per-column relative tolerance of a batched inner solve
   int iter, done, status, conv;  // conv: the stopping test was met
+  int imax;                      // iteration limit of a tile-order inner solve
 };
 
 struct ProfEntry {
@@ -213,6 +214,10 @@ struct fcp_ctx {
   int tune_sweep_rows = 0;     // rows per block of the row-block sweeps (0 = auto)
   int tune_sweep_tile = 0;     // rows per geometric tile (0 = auto, 1 = no tiles)
   int tune_inner_tile = 0;     // rows per tile of the tile-local preconditioner (0 = auto)
+  int tune_graph = 0;          // inner iterations replayed from a captured HIP graph
+  // captured pair of tile-order inner iterations and the launch arguments it holds
+  hipGraphExec_t tin_graph = nullptr;
+  std::vector<const void *> tin_graph_key;
   size_t lds_cap = 65536;      // LDS bytes available to one workgroup
   int block_rows_cap[2] = {65536, 65536};  // rows per contiguous block (lower, upper sweep)
 

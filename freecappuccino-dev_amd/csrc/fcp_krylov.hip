@@ -918,9 +918,12 @@ int sweep_grid(fcp_ctx *ctx, const Schedule &s) {
 // host side
 // ===========================================================================
 namespace fcpk {
+#ifndef FCP_RED_BPC
+#define FCP_RED_BPC 4
+#endif
 int red_grid(fcp_ctx *ctx, int n) {
   int g = grid_for(n, BLK);
-  int cap = ctx->ncu > 0 ? ctx->ncu * 4 : 1024;
+  int cap = ctx->ncu > 0 ? ctx->ncu * FCP_RED_BPC : 1024;
   if (cap > MAXBLK) cap = MAXBLK;
   return g < cap ? (g > 0 ? g : 1) : cap;
 }
@@ -935,7 +938,7 @@ int fcp_ensure_workspace(fcp_ctx *ctx) {
                         &ctx->e32, &ctx->u32, &ctx->v32, &ctx->reso32};
   for (auto *b : v32) FCP_HIP(ctx, b->alloc(n));
   FCP_HIP(ctx, ctx->partials.alloc((size_t)MAXBLK * 4 * NSLOT));
-  FCP_HIP(ctx, ctx->counters.alloc(16));
+  FCP_HIP(ctx, ctx->counters.alloc(CTR_INTS));
   FCP_HIP(ctx, ctx->scal.alloc(4));
   FCP_HIP(ctx, ctx->errflag.alloc(1));
   if (!ctx->h_scal) FCP_HIP(ctx, hipHostMalloc((void **)&ctx->h_scal, 4 * sizeof(SolveScalars)));
@@ -1398,7 +1401,7 @@ int solve_fp64(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max
   SolveScalars *s = ctx->scal.p;
   const double *a = ctx->a.p;
   const int g = red_grid(ctx, n);
-  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int), ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, CTR_INTS * sizeof(int), ctx->stream));
   FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
   k_zero_f64<<<g, BLK, 0, ctx->stream>>>(n, ctx->pk.p, ctx->uk.p, ctx->zk.p);
   {
@@ -1478,7 +1481,7 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
   const int g = red_grid(ctx, n);
   SolveScalars *so = ctx->scal.p;      // outer (FP64)
   SolveScalars *si = ctx->scal.p + 1;  // inner (FP32)
-  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int), ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, CTR_INTS * sizeof(int), ctx->stream));
   FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
   // FP32 operator and preconditioner (diagonal computed in FP64 then rounded)
   FCP_HIP(ctx, ctx->a32.alloc(ctx->nnz));
@@ -1487,19 +1490,26 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
     ctx->a32_version = ctx->a_version;
   }
   const int kind = solver == 2 ? 1 : 0;
-  if (solver != 1 && (ctx->dic_version != ctx->a_version || ctx->dic_kind != kind)) {
-    FCP_TRY(precond_setup<double>(ctx, kind, ctx->a.p, ctx->d.p));
-    ctx->dic_version = ctx->a_version;
-    ctx->dic_kind = kind;
-  }
-  if (solver != 1 && ctx->d32_version != ctx->dic_version) {
-    k_cast_a<<<g, BLK, 0, ctx->stream>>>(n, ctx->d.p, ctx->d32.p);
-    ctx->d32_version = ctx->dic_version;
-  }
+  // global DIC / DILU pivots (FP64 and their FP32 rounding), computed when a
+  // sweep needs them
+  auto global_pivots = [&]() -> int {
+    if (solver == 1) return 0;
+    if (ctx->dic_version != ctx->a_version || ctx->dic_kind != kind) {
+      FCP_TRY(precond_setup<double>(ctx, kind, ctx->a.p, ctx->d.p));
+      ctx->dic_version = ctx->a_version;
+      ctx->dic_kind = kind;
+    }
+    if (ctx->d32_version != ctx->dic_version) {
+      k_cast_a<<<g, BLK, 0, ctx->stream>>>(n, ctx->d.p, ctx->d32.p);
+      ctx->d32_version = ctx->dic_version;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
+  };
   // FP32 inner sweeps with the tile-local preconditioner when selected (the
   // FP64 refinement and any FP64 fallback keep the global one)
   bool use_bj = false;
   if (solver != 1 && ctx->inner_precond >= 1) FCP_TRY(bj_prepare(ctx, kind, &use_bj));
+  if (!use_bj) FCP_TRY(global_pivots());
   struct BjScope {
     fcp_ctx *c;
     ~BjScope() { c->bj_active = false; }
@@ -1527,6 +1537,7 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
     prev = rel;
     if (fp64_fallback) {
       // finish with FP64 inner iterations, warm-started from x
+      FCP_TRY(global_pivots());
       FCP_TRY(solve_fp64(ctx, solver, x, b, itr_max - total_inner, tol_abs, tol_rel, 1, check_every,
                          verbose));
       total_inner += ctx->h_scal[0].iter;
@@ -1583,10 +1594,11 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
 
 namespace fcpk {
 // FP64 pivots of the DIC (kind 0) / DILU (kind 1) preconditioner of the
-// resident matrix, plus FP32 copies of A and the pivots when fp32 is set.
-int prepare_precond(fcp_ctx *ctx, int kind, bool fp32) {
+// resident matrix (unless pivots is false), plus the FP32 copy of A and of
+// the current pivots when fp32 is set.
+int prepare_precond(fcp_ctx *ctx, int kind, bool fp32, bool pivots) {
   FCP_TRY(fcp_ensure_workspace(ctx));
-  if (ctx->dic_version != ctx->a_version || ctx->dic_kind != kind) {
+  if (pivots && (ctx->dic_version != ctx->a_version || ctx->dic_kind != kind)) {
     FCP_TRY(precond_setup<double>(ctx, kind, ctx->a.p, ctx->d.p));
     ctx->dic_version = ctx->a_version;
     ctx->dic_kind = kind;
@@ -1597,7 +1609,7 @@ int prepare_precond(fcp_ctx *ctx, int kind, bool fp32) {
       k_cast_a<<<red_grid(ctx, ctx->nnz), BLK, 0, ctx->stream>>>(ctx->nnz, ctx->a.p, ctx->a32.p);
       ctx->a32_version = ctx->a_version;
     }
-    if (ctx->d32_version != ctx->dic_version) {
+    if (pivots && ctx->d32_version != ctx->dic_version) {
       k_cast_a<<<red_grid(ctx, ctx->n), BLK, 0, ctx->stream>>>(ctx->n, ctx->d.p, ctx->d32.p);
       ctx->d32_version = ctx->dic_version;
     }
@@ -1626,7 +1638,7 @@ int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z) {
 int fcp_bj32_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out) {
   FCP_TRY(fcp_ensure_workspace(ctx));
   FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
-  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int), ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, CTR_INTS * sizeof(int), ctx->stream));
   bool ok = false;
   FCP_TRY(bj_prepare(ctx, kind, &ok));
   const int rc = ok ? tin_apply(ctx, r, z) : FCP_ERR_UNSUPPORTED;

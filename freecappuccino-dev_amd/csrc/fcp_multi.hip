@@ -1049,13 +1049,16 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
   const size_t m = (size_t)n * NRP;
   const int g = red_grid(ctx, n);
   const int kind = solver == 2 ? 1 : 0;
-  FCP_TRY(prepare_precond(ctx, kind, prec == FCP_PREC_MIXED));
+  // global pivots now for FP64 solves; a mixed solve computes them when a
+  // sweep needs them (FP32 inner sweeps without the tile-local
+  // preconditioner, FP64 finishing iterations)
+  FCP_TRY(prepare_precond(ctx, kind, prec == FCP_PREC_MIXED, prec != FCP_PREC_MIXED));
   FCP_HIP(ctx, ctx->scal.alloc(4 + 2 * MAXNR));
   FCP_HIP(ctx, ctx->partials.alloc((size_t)MAXBLK * 16));
   SolveScalars *so = ctx->scal.p + 4;        // outer / FP64 columns
   SolveScalars *si = ctx->scal.p + 4 + MAXNR;  // inner FP32 columns
   SolveScalars h[MAXNR];
-  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, 16 * sizeof(int), ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, CTR_INTS * sizeof(int), ctx->stream));
   FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
   for (int k = 0; k < 8; ++k) FCP_HIP(ctx, ctx->mv64[k].alloc(k == 7 ? 3 * (size_t)n : m));
   MVecs<double> V64{x, ctx->mv64[0].p, ctx->mv64[1].p, ctx->mv64[2].p, ctx->mv64[3].p,
@@ -1114,6 +1117,7 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
     ~BjScope() { c->bj_active = false; }
   } bj_scope{ctx};
   ctx->bj_active = use_bj;
+  if (!use_bj) FCP_TRY(prepare_precond(ctx, kind, true, true));
   for (int k = 0; k < 9; ++k) FCP_HIP(ctx, ctx->mv32[k].alloc(k == 8 ? 3 * (size_t)n : m));
   MVecs<float> V32{ctx->mv32[0].p, ctx->mv32[1].p, ctx->mv32[2].p, ctx->mv32[3].p,
                    ctx->mv32[4].p, ctx->mv32[5].p, ctx->mv32[6].p, ctx->mv32[7].p,
@@ -1141,6 +1145,7 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
     if (all || total >= itr_max) break;
     if (stall) {
       // finish every column with FP64 iterations, warm-started from x
+      FCP_TRY(prepare_precond(ctx, kind, true, true));
       mk_zero2<double><<<g, BLK, 0, ctx->stream>>>(m, V64.p, V64.u);
       FCP_TRY(residual(solver == 2 ? V64.reso : nullptr));
       if (solver == 2) {

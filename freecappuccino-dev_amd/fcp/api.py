@@ -29,6 +29,7 @@ FCP_TUNE_SWEEP_ROWS = 4
 FCP_TUNE_SWEEP_TILE = 5
 FCP_TUNE_INNER_PRECOND = 6
 FCP_TUNE_INNER_TILE = 7
+FCP_TUNE_GRAPH = 8
 
 
 class FcpError(RuntimeError):

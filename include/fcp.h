@@ -343,6 +343,13 @@ int fcp_pmgmres(int n, int nnz, const int *ia, const int *ja, const double *a,
                                     refinement always use the global one.      */
 #define FCP_TUNE_INNER_TILE 7    /* rows per tile of the tile-local
                                     preconditioner (0 = auto: 4096)            */
+#define FCP_TUNE_GRAPH 8         /* 1: inner FP32 iterations of a mixed-
+                                    precision PCG are replayed from a captured
+                                    HIP graph (two iterations per replay);
+                                    0 (default): launched one kernel at a
+                                    time (as fast on MI355X: the gap between
+                                    dependent kernels is on the device side).
+                                    Profiling launches them one at a time.     */
 /* Sets a launch-shape parameter of the context; results do not depend on it
  * (FCP_TUNE_INNER_PRECOND changes the path of a mixed-precision solve, not
  * its stopping test). */

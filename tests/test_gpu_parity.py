@@ -540,6 +540,38 @@ def test_tile_order_preconditioner_fp32_bitexact(api, kind):
     ctx.close()
 
 
+def test_inner_graph_replay_bitequal_to_launches(api):
+    """Inner FP32 iterations replayed from the captured HIP graph (two
+    iterations per replay, eager refinement work and host polls between
+    replays) give the same solutions, bit for bit, as launching every kernel,
+    for single and batched mixed solves, twice in a row."""
+    print("[case] graph replay 24^3", flush=True)
+    mesh = workloads.poisson_jittered_mesh(24)
+    ctx, geo, csr = _setup(api, mesh)
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    su = inp["source"] * geo["vol"]
+    ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su)
+    n = mesh.n_cells
+    B = np.stack([su, np.cos(np.arange(n)) * 1e-3, np.sin(np.arange(n) * 0.3) * 1e-3])
+    out = {}
+    for graph in (0, 1, 1):
+        ctx.tune(api.FCP_TUNE_GRAPH, graph)
+        x = np.zeros(n)
+        rc, info = ctx.csrsolve("iccg", x, su.copy(), itr_max=5000, tol_abs=0.0, tol_rel=1e-10,
+                                precision=api.FCP_PREC_MIXED, criterion=api.FCP_CRIT_TRUE_L2,
+                                check_every=6)
+        assert rc == 0 and info["rel_res_l2"] <= 1e-10, info
+        X = np.zeros((3, n))
+        ctx.csrsolve_multi("iccg", X, B, itr_max=5000, tol_rel=1e-10,
+                           precision=api.FCP_PREC_MIXED, criterion=api.FCP_CRIT_TRUE_L2,
+                           check_every=6)
+        out.setdefault(graph, []).append((x, X))
+    for x, X in out[1]:
+        assert np.array_equal(x, out[0][0][0])
+        assert np.array_equal(X, out[0][0][1])
+    ctx.close()
+
+
 @pytest.mark.parametrize("inner", [0, 1, 2])
 def test_mixed_inner_preconditioners_reach_tolerance(api, inner):
     """FP32 inner solves with the global (0) or tile-local preconditioner
