@@ -29,7 +29,7 @@ using namespace fcpk;
 
 namespace {
 
-constexpr unsigned short TK_NONE = 0xFFFF;  // unused dependency slot
+constexpr int TK_SLOTS = 0xFFFF;  // LDS slots a 16-bit dependency can name
 #ifndef FCP_TPD
 #define FCP_TPD 4
 #endif
@@ -74,8 +74,9 @@ __device__ __forceinline__ int2 tk_pack16(const int (&d)[FCP_PMAXD]) {
 
 // Dependencies in LDS slots (16 bits) of both sweeps: the forward ones from
 // the tile-local forward schedule, the backward ones are the row's upper
-// entries (CSR order) whose column lies in the same tile.
-__global__ void tk_pattern(int n, int nb, const int *__restrict__ perm,
+// entries (CSR order) whose column lies in the same tile.  Unused slots name
+// the zero-valued slot `dummy` (their coefficient is 0).
+__global__ void tk_pattern(int n, int nb, int dummy, const int *__restrict__ perm,
                            const int *__restrict__ iperm, const int *__restrict__ bpos,
                            const int *__restrict__ prc, const int *__restrict__ pdep,
                            const int *__restrict__ ia, const int *__restrict__ diag,
@@ -86,14 +87,14 @@ __global__ void tk_pattern(int n, int nb, const int *__restrict__ perm,
   const int cnt = prc[2 * p + 1] & 0xFFFF;
   int d[FCP_PMAXD];
 #pragma unroll
-  for (int m = 0; m < FCP_PMAXD; ++m) d[m] = m < cnt ? pdep[(size_t)FCP_PMAXD * p + m] : TK_NONE;
+  for (int m = 0; m < FCP_PMAXD; ++m) d[m] = m < cnt ? pdep[(size_t)FCP_PMAXD * p + m] : dummy;
   dl[p] = tk_pack16(d);
   const int b = tk_block_of(bpos, nb, p);
   const int lo = bpos[b], hi = bpos[b + 1];
   const int row = perm[p];
   int m = 0;
 #pragma unroll
-  for (int k = 0; k < FCP_PMAXD; ++k) d[k] = TK_NONE;
+  for (int k = 0; k < FCP_PMAXD; ++k) d[k] = dummy;
   for (int k = diag[row] + 1; k < ia[row + 1]; ++k) {
     const int q = iperm[ja[k]];
     if (q >= lo && q < hi) {
@@ -210,7 +211,9 @@ __device__ __forceinline__ void tk_load(TSlot &S, int2 sb, bool valid, int lo,
 // One position of a step, in place: forward (MODE 0)
 // v_i = (v_i - sum_m a_m v_dep(m)) * d_i; backward (MODE 1)
 // v_i = (v_i / (d_i + small) - sum_m a_m v_dep(m)) * d_i; dependencies in
-// CSR order, unused slots masked by a select (the row-gather sweeps' order).
+// CSR order.  An unused slot subtracts 0 * 0 from the sum, which leaves every
+// value (signed zeros, infinities and NaN included) unchanged, so the result
+// is the one of the sum over the used slots only.
 template <int NR, int NRP, int MODE>
 __device__ __forceinline__ void tk_step(const TSlot &S, float *V) {
   if (!S.ok) return;
@@ -223,12 +226,7 @@ __device__ __forceinline__ void tk_step(const TSlot &S, float *V) {
     float acc = V[S.i * NRP + c];
     if (MODE == 1) acc = acc / (S.dv + (float)FCP_SMALL);
 #pragma unroll
-    for (int m = 0; m < FCP_PMAXD; ++m) {
-      const bool use = dep[m] != TK_NONE;
-      const float v = V[(use ? dep[m] : 0) * NRP + c];
-      const float na = acc - av[m] * v;
-      acc = use ? na : acc;
-    }
+    for (int m = 0; m < FCP_PMAXD; ++m) acc = acc - av[m] * V[dep[m] * NRP + c];
     V[S.i * NRP + c] = acc * S.dv;
   }
 }
@@ -287,13 +285,13 @@ __device__ __forceinline__ void tk_sweeps(float *V, const int2 *sb, int ns, int 
 // untouched.
 template <int NR, int NRP>
 __global__ __launch_bounds__(BLK) void tk_pre(
-    int nb, int vslots, const int *__restrict__ bpos, const int *__restrict__ blk_step,
+    int nb, int vslots, int dslot, const int *__restrict__ bpos, const int *__restrict__ blk_step,
     const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const float4 *__restrict__ al,
     const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
     float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
     const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
     int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs) {
-  extern __shared__ float V[];  // [vslots] values, then the step bounds
+  extern __shared__ float V[];  // [vslots] values (slot dslot = 0), then the step bounds
   int2 *sb = reinterpret_cast<int2 *>(V + vslots);
   if (tk_all_done<NR>(s)) return;
   bool act[NR];
@@ -349,6 +347,7 @@ __global__ __launch_bounds__(BLK) void tk_pre(
         }
       }
     }
+    if (threadIdx.x < NRP) V[dslot * NRP + threadIdx.x] = 0.0f;  // the unused-dependency slot
     __syncthreads();
     if (ns > 0) tk_sweeps<NR, NRP>(V, sb, ns, lo, dl, al, du, au, dv);
     // z out; r.z with the residual this thread wrote (or read) above
@@ -593,8 +592,9 @@ __global__ void tk_scatter_out(int n, const int *__restrict__ perm, const float 
 
 size_t tk_lds(const fcp_ctx *ctx, int nrp) {
   const BlockSchedule &b = ctx->jlower;
-  // values, then 8-byte aligned step bounds
-  return (size_t)((b.R * nrp + 1) & ~1) * sizeof(float) + (size_t)(b.max_steps + 1) * sizeof(int2);
+  // values (R slots and the zero slot), then 8-byte aligned step bounds
+  return (size_t)(((b.R + 1) * nrp + 1) & ~1) * sizeof(float) +
+         (size_t)(b.max_steps + 1) * sizeof(int2);
 }
 
 #ifndef FCP_TK_SPMV_BLOCKS
@@ -624,7 +624,7 @@ int tk_prepare(fcp_ctx *ctx) {
     tk_iperm<<<g, BLK, 0, ctx->stream>>>(n, b.rows.p, ctx->tin_iperm.p);
     FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
     tk_pattern<<<g, BLK, 0, ctx->stream>>>(
-        n, b.nblocks, b.rows.p, ctx->tin_iperm.p, b.blk_pos.p, b.prc.p, b.pdep.p, ctx->ia.p,
+        n, b.nblocks, b.R, b.rows.p, ctx->tin_iperm.p, b.blk_pos.p, b.prc.p, b.pdep.p, ctx->ia.p,
         ctx->diag.p, ctx->ja.p, reinterpret_cast<int2 *>(ctx->tin_dl.p),
         reinterpret_cast<int2 *>(ctx->tin_du.p), ctx->errflag.p);
     const int nsl = (n + 63) / 64;
@@ -676,7 +676,7 @@ int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *s
   const BlockSchedule &b = ctx->jlower;
   ProfScope ps(ctx, "tin_pre");
   tk_pre<NR, NRP><<<tk_grid(ctx), BLK, tk_lds(ctx, NRP), ctx->stream>>>(
-      b.nblocks, (b.R * NRP + 1) & ~1, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
+      b.nblocks, ((b.R + 1) * NRP + 1) & ~1, b.R, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
       reinterpret_cast<const int2 *>(ctx->tin_dl.p),
       reinterpret_cast<const float4 *>(ctx->tin_al.p),
       reinterpret_cast<const int2 *>(ctx->tin_du.p),
@@ -785,7 +785,7 @@ namespace fcpk {
 bool tin_usable(fcp_ctx *ctx, int nrp) {
   const BlockSchedule &b = ctx->jlower;
   return ctx->have_bj && b.nblocks > 0 && b.maxd <= FCP_PMAXD && ctx->jupper.maxd <= FCP_PMAXD &&
-         b.R < (int)TK_NONE && tk_lds(ctx, nrp) + 256 <= ctx->lds_cap;
+         b.R < TK_SLOTS && tk_lds(ctx, nrp) + 256 <= ctx->lds_cap;
 }
 
 // Inner FP32 PCG of nr interleaved columns (NRP = 1, 2, 4) on the FP64
