@@ -289,6 +289,9 @@ bool tin_usable(fcp_ctx *ctx, int nrp);
 int tin_pcg(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScalars *so,
             SolveScalars *si, double tol_rel, double inner_tol, int budget, int check_every,
             int verbose, SolveScalars *h);
+int tin_bicgstab(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScalars *so,
+                 SolveScalars *si, double tol_rel, double inner_tol, int budget, int check_every,
+                 int verbose, SolveScalars *h);
 int tin_apply(fcp_ctx *ctx, const double *r, double *z);
 int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z);
 int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z);

@@ -22,6 +22,8 @@
 // NR right-hand sides are interleaved (NRP values per position); every
 // column's arithmetic is independent of NR, so a batched solve's columns
 // equal single-column solves bit for bit.
+#include <type_traits>
+
 #include "fcp_internal.h"
 #include "fcp_device.h"
 
@@ -502,7 +504,8 @@ __global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ ba
   }
 }
 
-// r~ = P (res / |res|_2) in FP32, x~ = 0, p~ = 0, and the inner scalars of
+// r~ = P (res / |res|_2) in FP32, x~ = 0, p~ = 0 (BiCGStab: reso = r~, u = 0),
+// and the inner scalars of
 // every column (tolerance from the outer residual as for the row-order inner
 // solves; columns whose outer residual already meets tol_rel start done).
 template <int NR, int NRP>
@@ -514,7 +517,8 @@ __global__ __launch_bounds__(BLK) void tk_init(int n, const int *__restrict__ pe
                                                double *__restrict__ partials,
                                                int *__restrict__ counter,
                                                SolveScalars *__restrict__ s, double tol_rel,
-                                               double inner_tol, int imax) {
+                                               double inner_tol, int imax,
+                                               float *__restrict__ reso, float *__restrict__ u0) {
   double inv[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) inv[c] = so[c].l2 > 0.0 ? 1.0 / so[c].l2 : 0.0;
@@ -530,6 +534,8 @@ __global__ __launch_bounds__(BLK) void tk_init(int n, const int *__restrict__ pe
       r[o] = v;
       x[o] = 0.0f;
       p0[o] = 0.0f;
+      if (reso) reso[o] = v;
+      if (u0) u0[o] = 0.0f;
       part[2 * c] += fabs((double)v);
       part[2 * c + 1] += (double)v * (double)v;
     }
@@ -575,6 +581,273 @@ __global__ void tk_ir_update(int n, const int *__restrict__ perm, const float *_
 #pragma unroll
     for (int c = 0; c < NR; ++c)
       x[row * NRP + c] = x[row * NRP + c] + sc[c] * (double)e[(size_t)i * NRP + c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BiCGStab (DILU) inner iterations in tile order (bicgstab :642-773): five
+// kernels per iteration, the scalars in device memory.
+//   tb_bet  (rows): x += alf z, r -= alf v with |r|_1, |r|_2 and the stopping
+//           test of the previous iteration; bet = r.reso, om, beto (:647-650)
+//   tb_tile<1> (tiles): p = r + om (p - alf u), then z = M^-1 p (:653-683)
+//   tb_spmv<0> (rows): u = A z, ukreso = u.reso, gam = bet / ukreso (:689-700)
+//   tb_tile<2> (tiles): x += gam z, r -= gam u, then z = M^-1 r (:703-732)
+//   tb_spmv<1> (rows): v = A z, alf = v.r / (v.v + small) (:737-748)
+// Each kernel keeps the per-row arithmetic of the row-order kernels.
+// ---------------------------------------------------------------------------
+template <int NR, int NRP, int MODE>
+__global__ __launch_bounds__(BLK) void tb_tile(
+    int nb, int vslots, int dslot, const int *__restrict__ bpos, const int *__restrict__ blk_step,
+    const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const float4 *__restrict__ al,
+    const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
+    float *__restrict__ x, float *__restrict__ r, float *__restrict__ p,
+    const float *__restrict__ u, float *z, const SolveScalars *__restrict__ s) {
+  extern __shared__ float V[];  // [vslots] values (slot dslot = 0), then the step bounds
+  int2 *sb = reinterpret_cast<int2 *>(V + vslots);
+  if (tk_all_done<NR>(s)) return;
+  bool act[NR];
+  float c1[NR], c2[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    act[c] = !solve_done(s + c);
+    c1[c] = MODE == 1 ? (float)s[c].om : (float)s[c].gam;
+    c2[c] = (float)s[c].alf;
+  }
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int lo = bpos[b], hi = bpos[b + 1];
+    const int st0 = blk_step[b], ns = blk_step[b + 1] - st0;
+    for (int k = threadIdx.x; k < ns; k += BLK)
+      sb[k] = make_int2(step_ptr[st0 + k], step_ptr[st0 + k + 1]);
+    for (int i0 = lo + (int)threadIdx.x; i0 < hi; i0 += GS_U * BLK) {
+      float av[GS_U][NR], bv[GS_U][NR], cv[GS_U][NR];
+#pragma unroll
+      for (int u2 = 0; u2 < GS_U; ++u2) {
+        const int i = i0 + u2 * BLK;
+        if (i < hi) {
+#pragma unroll
+          for (int c = 0; c < NR; ++c) {
+            const size_t o = (size_t)i * NRP + c;
+            av[u2][c] = MODE == 1 ? p[o] : r[o];
+            if (act[c]) {
+              bv[u2][c] = MODE == 1 ? r[o] : x[o];
+              cv[u2][c] = u[o];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int u2 = 0; u2 < GS_U; ++u2) {
+        const int i = i0 + u2 * BLK;
+        if (i < hi) {
+#pragma unroll
+          for (int c = 0; c < NR; ++c) {
+            const size_t o = (size_t)i * NRP + c;
+            float val = av[u2][c];
+            if (act[c]) {
+              if (MODE == 1) {  // p = r + om (p - alf u)
+                val = bv[u2][c] + c1[c] * (av[u2][c] - c2[c] * cv[u2][c]);
+                p[o] = val;
+              } else {  // x += gam z ; r -= gam u
+                x[o] = bv[u2][c] + c1[c] * z[o];
+                val = av[u2][c] - c1[c] * cv[u2][c];
+                r[o] = val;
+              }
+            }
+            V[(i - lo) * NRP + c] = val;
+          }
+        }
+      }
+    }
+    if (threadIdx.x < NRP) V[dslot * NRP + threadIdx.x] = 0.0f;  // the unused-dependency slot
+    __syncthreads();
+    if (ns > 0) tk_sweeps<NR, NRP>(V, sb, ns, lo, dl, al, du, au, dv);
+    for (int i = lo + (int)threadIdx.x; i < hi; i += BLK) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) z[(size_t)i * NRP + c] = V[(i - lo) * NRP + c];
+    }
+    __syncthreads();  // V and sb are reused by the next tile
+  }
+}
+
+// out = A z over the slice's entries (padding adds 0 * z_i); MODE 0:
+// ukreso = out.w, gam = bet / ukreso (w = reso); MODE 1: vr = out.w,
+// vv = out.out, alf = vr / (vv + small) (w = r).
+template <int NR, int NRP, int MODE>
+__global__ __launch_bounds__(BLK) void tb_spmv(int n, const int *__restrict__ base,
+                                               const int *__restrict__ col,
+                                               const float *__restrict__ val,
+                                               const float *__restrict__ z,
+                                               const float *__restrict__ w, float *__restrict__ out,
+                                               double *__restrict__ partials,
+                                               int *__restrict__ counter,
+                                               SolveScalars *__restrict__ s) {
+  if (tk_all_done<NR>(s)) return;
+  bool act[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) act[c] = !solve_done(s + c);
+  constexpr int NV = MODE == 0 ? NR : 2 * NR;
+  double part[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) part[k] = 0.0;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    const int sl = i >> 6;
+    const int b0 = base[sl];
+    const int pp = b0 + (i & 63);
+    const int L = (base[sl + 1] - b0) >> 6;
+    float av[SELL_KU];
+    int cv[SELL_KU];
+#pragma unroll
+    for (int k = 0; k < SELL_KU; ++k)
+      if (k < L) {
+        av[k] = val[pp + 64 * k];
+        cv[k] = col[pp + 64 * k];
+      }
+    float acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < SELL_KU; ++k)
+      if (k < L) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] = acc[c] + av[k] * z[(size_t)cv[k] * NRP + c];
+      }
+    for (int k = SELL_KU; k < L; ++k) {
+      const float a = val[pp + 64 * k];
+      const int j = col[pp + 64 * k];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = acc[c] + a * z[(size_t)j * NRP + c];
+    }
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (!act[c]) continue;
+      const size_t o = (size_t)i * NRP + c;
+      out[o] = acc[c];
+      if (MODE == 0) {
+        part[c] += (double)acc[c] * (double)w[o];
+      } else {
+        part[2 * c] += (double)acc[c] * (double)w[o];
+        part[2 * c + 1] += (double)acc[c] * (double)acc[c];
+      }
+    }
+  }
+  double blk[NV], tot[NV];
+  block_sum<NV>(part, blk);
+  if (last_block_reduce<NV>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (!act[c]) continue;
+      if (MODE == 0) {
+        s[c].ukreso = tot[c];
+        s[c].gam = s[c].sk / tot[c];
+      } else {
+        s[c].vr = tot[2 * c];
+        s[c].vv = tot[2 * c + 1];
+        s[c].alf = tot[2 * c] / (tot[2 * c + 1] + FCP_SMALL);
+      }
+    }
+  }
+}
+
+// upd: x += alf z, r -= alf v with |r|_1, |r|_2 and the stopping test of the
+// previous iteration (relative L2 against the column's tolerance, at most
+// s->imax iterations); then bet = r.reso, om = bet gam / (alf beto + small),
+// beto = bet.
+template <int NR, int NRP>
+__global__ __launch_bounds__(BLK) void tb_bet(int n, float *__restrict__ x, float *__restrict__ r,
+                                              const float *__restrict__ z,
+                                              const float *__restrict__ v,
+                                              const float *__restrict__ reso,
+                                              double *__restrict__ partials,
+                                              int *__restrict__ counter,
+                                              SolveScalars *__restrict__ s, int upd,
+                                              double tol_abs) {
+  if (tk_all_done<NR>(s)) return;
+  bool act[NR];
+  float alf[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    act[c] = !solve_done(s + c);
+    alf[c] = (float)s[c].alf;
+  }
+  double part[3 * NR];
+#pragma unroll
+  for (int k = 0; k < 3 * NR; ++k) part[k] = 0.0;
+  const int stride = gridDim.x * BLK;
+  for (int i0 = blockIdx.x * BLK + threadIdx.x; i0 < n; i0 += GS_U * stride) {
+    float xv[GS_U][NR], zv[GS_U][NR], rv[GS_U][NR], vv[GS_U][NR], ov[GS_U][NR];
+#pragma unroll
+    for (int u = 0; u < GS_U; ++u) {
+      const int i = i0 + u * stride;
+      if (i < n) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          const size_t o = (size_t)i * NRP + c;
+          if (act[c]) {
+            rv[u][c] = r[o];
+            ov[u][c] = reso[o];
+            if (upd) {
+              xv[u][c] = x[o];
+              zv[u][c] = z[o];
+              vv[u][c] = v[o];
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < GS_U; ++u) {
+      const int i = i0 + u * stride;
+      if (i < n) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          if (!act[c]) continue;
+          const size_t o = (size_t)i * NRP + c;
+          float ri = rv[u][c];
+          if (upd) {
+            x[o] = xv[u][c] + alf[c] * zv[u][c];
+            ri = ri - alf[c] * vv[u][c];
+            r[o] = ri;
+            part[3 * c] += fabs((double)ri);
+            part[3 * c + 1] += (double)ri * (double)ri;
+          }
+          part[3 * c + 2] += (double)ri * (double)ov[u][c];
+        }
+      }
+    }
+  }
+  double blk[3 * NR], tot[3 * NR];
+  block_sum<3 * NR>(part, blk);
+  if (last_block_reduce<3 * NR>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (!act[c]) continue;
+      SolveScalars &S = s[c];
+      bool fin = false;
+      if (upd) {
+        const double resl = tot[3 * c];
+        S.resl = resl;
+        S.l2 = sqrt(tot[3 * c + 1]);
+        S.s0 = S.sk;
+        S.iter += 1;
+        S.rsm = resl / (S.res0 + FCP_SMALL);
+        bool conv = (S.l2 <= S.tol * S.l2b) || (resl < tol_abs);
+        if (!isfinite(resl)) {
+          S.status = FCP_ERR_NONFINITE;
+          conv = true;
+        }
+        S.conv = (conv && S.status == 0) ? 1 : 0;
+        if (conv || S.iter >= S.imax) {
+          S.done = 1;
+          fin = true;
+        }
+      }
+      if (!fin) {
+        const double bet = tot[3 * c + 2];
+        S.sk = bet;
+        S.om = bet * S.gam / (S.alf * S.beto + FCP_SMALL);
+        S.beto = bet;
+      }
+    }
   }
 }
 
@@ -745,7 +1018,7 @@ int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, S
   const int g = red_grid(ctx, n);
   tk_init<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, res, so, B.r, B.x, B.p[0],
                                                ctx->partials.p, redctr(ctx), si, tol_rel,
-                                               inner_tol, budget);
+                                               inner_tol, budget, nullptr, nullptr);
   FCP_TRY((tk_launch_pre<NR, NRP>(ctx, B, B.p[0], si, 0)));
   const int ce = check_every > 0 ? check_every : 8;
   hipGraphExec_t ge = nullptr;
@@ -772,6 +1045,79 @@ int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, S
     if (all) break;
   }
   tk_ir_update<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, B.x, so, x);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+
+template <int NR, int NRP>
+int tb_bicgstab(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so,
+                SolveScalars *si, double tol_rel, double inner_tol, int budget, int check_every,
+                int verbose, SolveScalars *h) {
+  const int n = ctx->n;
+  const size_t m = (size_t)n * NRP;
+  for (int k = 0; k < 7; ++k) FCP_HIP(ctx, ctx->tin_vec[k].alloc(m));
+  float *X = ctx->tin_vec[0].p, *R = ctx->tin_vec[1].p, *Z = ctx->tin_vec[2].p,
+        *P = ctx->tin_vec[3].p, *U = ctx->tin_vec[4].p, *Vv = ctx->tin_vec[5].p,
+        *Reso = ctx->tin_vec[6].p;
+  const BlockSchedule &b = ctx->jlower;
+  const int g = red_grid(ctx, n);
+  const size_t lds = tk_lds(ctx, NRP);
+  const int vslots = ((b.R + 1) * NRP + 1) & ~1;
+  const int2 *dl = reinterpret_cast<const int2 *>(ctx->tin_dl.p);
+  const int2 *du = reinterpret_cast<const int2 *>(ctx->tin_du.p);
+  const float4 *al = reinterpret_cast<const float4 *>(ctx->tin_al.p);
+  const float4 *au = reinterpret_cast<const float4 *>(ctx->tin_au.p);
+  tk_init<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, b.rows.p, res, so, R, X, P, ctx->partials.p,
+                                               redctr(ctx), si, tol_rel, inner_tol, budget, Reso,
+                                               U);
+  auto bet = [&](int upd) {
+    ProfScope ps(ctx, "tinb_bet");
+    tb_bet<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, X, R, Z, Vv, Reso, ctx->partials.p,
+                                                redctr(ctx), si, upd, 0.0);
+  };
+  auto tile = [&](auto mode) {
+    constexpr int MODE = decltype(mode)::value;
+    ProfScope ps(ctx, MODE == 1 ? "tinb_pz" : "tinb_xz");
+    tb_tile<NR, NRP, MODE><<<tk_grid(ctx), BLK, lds, ctx->stream>>>(
+        b.nblocks, vslots, b.R, b.blk_pos.p, b.blk_step.p, b.step_ptr.p, dl, al, du, au,
+        ctx->tin_dv.p, X, R, P, U, Z, si);
+  };
+  auto spmv = [&](auto mode) {
+    constexpr int MODE = decltype(mode)::value;
+    ProfScope ps(ctx, MODE == 0 ? "tinb_u" : "tinb_v");
+    tb_spmv<NR, NRP, MODE><<<tk_spmv_grid(ctx), BLK, 0, ctx->stream>>>(
+        n, ctx->tin_base.p, ctx->tin_col.p, ctx->tin_val.p, Z, MODE == 0 ? Reso : R,
+        MODE == 0 ? U : Vv, ctx->partials.p, redctr(ctx), si);
+  };
+  bet(0);
+  const int ce = check_every > 0 ? check_every : 8;
+  int launched = 0;
+  while (launched < budget) {
+    const int batch = (budget - launched) < ce ? (budget - launched) : ce;
+    for (int j = 0; j < batch; ++j) {
+      tile(std::integral_constant<int, 1>());
+      spmv(std::integral_constant<int, 0>());
+      tile(std::integral_constant<int, 2>());
+      spmv(std::integral_constant<int, 1>());
+      bet(1);
+    }
+    FCP_HIP(ctx, hipGetLastError());
+    launched += batch;
+    FCP_HIP(ctx, hipMemcpyAsync(h, si, NR * sizeof(SolveScalars), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    FCP_TRY(check_err(ctx));
+    bool all = true;
+    for (int c = 0; c < NR; ++c) {
+      if (verbose >= 2)
+        printf("                   BiCGStab32[%d] iter = %4d resl = %10.3E rsm = %10.3E\n", c,
+               h[c].iter, h[c].resl, h[c].rsm);
+      all = all && h[c].done;
+    }
+    if (all) break;
+  }
+  tk_ir_update<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, b.rows.p, X, so, x);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }
@@ -816,6 +1162,36 @@ int tin_pcg(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScala
     case 4:
       return tk_pcg<4, 4>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every, verbose,
                           h);
+    default:
+      return FCP_ERR_ARG;
+  }
+}
+
+// Inner FP32 DILU-BiCGStab in tile order; arguments as tin_pcg.
+int tin_bicgstab(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScalars *so,
+                 SolveScalars *si, double tol_rel, double inner_tol, int budget, int check_every,
+                 int verbose, SolveScalars *h) {
+  const int nrp = nr <= 1 ? 1 : (nr <= 2 ? 2 : 4);
+  if (!tin_usable(ctx, nrp)) return FCP_ERR_UNSUPPORTED;
+  const int rc = tk_prepare(ctx);
+  if (rc == FCP_ERR_UNSUPPORTED) {
+    ctx->tin_have = false;
+    return rc;
+  }
+  FCP_TRY(rc);
+  switch (nr) {
+    case 1:
+      return tb_bicgstab<1, 1>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every,
+                               verbose, h);
+    case 2:
+      return tb_bicgstab<2, 2>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every,
+                               verbose, h);
+    case 3:
+      return tb_bicgstab<3, 4>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every,
+                               verbose, h);
+    case 4:
+      return tb_bicgstab<4, 4>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every,
+                               verbose, h);
     default:
       return FCP_ERR_ARG;
   }

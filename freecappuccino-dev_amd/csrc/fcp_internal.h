@@ -156,7 +156,7 @@ struct fcp_ctx {
   DBuf<float> tin_al, tin_au, tin_dv;  // [4n] coefficients, [n] pivots
   DBuf<int> tin_base, tin_col;       // SELL-64 copy of P A P^T
   DBuf<float> tin_val;
-  DBuf<float> tin_vec[6];            // x, r, z, q, p (two buffers)
+  DBuf<float> tin_vec[7];            // PCG: x, r, z, q, p (two); BiCGStab: x, r, z, p, u, v, reso
   bool bj_active = false;            // FP32 sweeps use the tile-local preconditioner
   DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)
   bool sym_pattern = true;  // every entry has its transposed entry

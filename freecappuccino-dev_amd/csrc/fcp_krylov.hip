@@ -1545,10 +1545,14 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
       continue;
     }
     const int budget = itr_max - total_inner;
-    // inner FP32 PCG in tile order (tile-local preconditioner)
-    if (solver == 0 && use_bj && ctx->inner_precond == 1 && tin_usable(ctx, 1)) {
-      FCP_TRY(tin_pcg(ctx, 1, ctx->res.p, x, so, si, tol_rel, inner_tol, budget, check_every,
-                      verbose, &ctx->h_scal[1]));
+    // inner FP32 PCG / BiCGStab in tile order (tile-local preconditioner)
+    if ((solver == 0 || solver == 2) && use_bj && ctx->inner_precond == 1 && tin_usable(ctx, 1)) {
+      if (solver == 0)
+        FCP_TRY(tin_pcg(ctx, 1, ctx->res.p, x, so, si, tol_rel, inner_tol, budget, check_every,
+                        verbose, &ctx->h_scal[1]));
+      else
+        FCP_TRY(tin_bicgstab(ctx, 1, ctx->res.p, x, so, si, tol_rel, inner_tol, budget,
+                             check_every, verbose, &ctx->h_scal[1]));
       total_inner += ctx->h_scal[1].iter;
       FCP_TRY(true_residual(ctx, x, b, so));
       continue;

@@ -1170,10 +1170,14 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
       FCP_TRY(residual(nullptr));
       continue;
     }
-    if (solver == 0 && use_bj && ctx->inner_precond == 1 && tin_usable(ctx, NRP)) {
-      // inner FP32 PCG in tile order (tile-local preconditioner), x updated
-      FCP_TRY(tin_pcg(ctx, NR, V64.r, x, so, si, tol_rel, inner_tol, itr_max - total,
-                      check_every, verbose, h));
+    if (use_bj && ctx->inner_precond == 1 && tin_usable(ctx, NRP)) {
+      // inner FP32 PCG / BiCGStab in tile order (tile-local preconditioner), x updated
+      if (solver == 2)
+        FCP_TRY(tin_bicgstab(ctx, NR, V64.r, x, so, si, tol_rel, inner_tol, itr_max - total,
+                             check_every, verbose, h));
+      else
+        FCP_TRY(tin_pcg(ctx, NR, V64.r, x, so, si, tol_rel, inner_tol, itr_max - total,
+                        check_every, verbose, h));
       int mx = 0;
       for (int c = 0; c < NR; ++c) {
         res.iters[c] += h[c].iter;

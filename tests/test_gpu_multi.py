@@ -71,13 +71,18 @@ def test_convdiff_flux_divergence_free(api):
     ctx.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_multi_bicgstab_columns_bitequal_single(api, precision):
-    """3-column DILU-BiCGStab (FP64, and FP32 inner + FP64 refinement) equals
-    three single-column solves bit for bit, and reaches the tolerance."""
-    print(f"[case] multi bicgstab 14^3 precision={precision}", flush=True)
-    mesh, ctx, geo, csr, flux, div, a, rhs = _convdiff_system(api, 14)
+@pytest.mark.parametrize("precision,inner,N", [(0, 1, 14), (1, 1, 14), (1, 1, 26), (1, 2, 26)])
+def test_multi_bicgstab_columns_bitequal_single(api, precision, inner, N):
+    """3-column DILU-BiCGStab (FP64, and FP32 inner + FP64 refinement with the
+    tile-local preconditioner: inner iterations in tile order (1) or with the
+    row-gather sweeps (2)) equals three single-column solves bit for bit, and
+    reaches the tolerance.  At 26^3 the preconditioner has several tiles."""
+    print(f"[case] multi bicgstab {N}^3 precision={precision} inner={inner}", flush=True)
+    mesh, ctx, geo, csr, flux, div, a, rhs = _convdiff_system(api, N)
     n = mesh.n_cells
+    ctx.tune(api.FCP_TUNE_INNER_PRECOND, inner)
+    if N == 26:
+        assert ctx.sweep_tiles().max() >= 3
     ctx.matrix_set(a)
     X = np.zeros((3, n))
     rc, info, res0 = ctx.csrsolve_multi("bicgstab", X, rhs, itr_max=3000, tol_rel=1e-9,
