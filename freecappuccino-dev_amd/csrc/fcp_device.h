@@ -26,9 +26,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Block-wide sum of up to 3 values into lane 0 of warp 0 (deterministic order).
-template <int NV>
+template <int NV, int NT = BLK>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double (&out)[NV]) {
-  __shared__ double sh[NV][BLK / 64];
+  __shared__ double sh[NV][NT / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
@@ -40,7 +40,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (&out)[NV]) {
   for (int i = 0; i < NV; ++i) {
     double s = 0.0;
     if (threadIdx.x == 0)
-      for (int k = 0; k < BLK / 64; ++k) s += sh[i][k];
+      for (int k = 0; k < NT / 64; ++k) s += sh[i][k];
     out[i] = s;
   }
 }
@@ -56,7 +56,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (&out)[NV]) {
 // serialized atomics; every counter is reset by its last arrival.  counter
 // points to (RED_GROUPS + 1) lines of 32 ints: the top counter, then the
 // sub-counters.
-template <int NV>
+template <int NV, int NT = BLK>
 __device__ bool last_block_reduce(double (&part)[NV], double *__restrict__ partials, int slot,
                                   int *__restrict__ counter, double (&tot)[NV]) {
   __shared__ int is_last;
@@ -89,12 +89,12 @@ __device__ bool last_block_reduce(double (&part)[NV], double *__restrict__ parti
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
     double s = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += BLK)
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += NT)
       s += __hip_atomic_load(&partials[(size_t)(slot + i) * MAXBLK + b], __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
     loc[i] = s;
   }
-  block_sum<NV>(loc, tot);
+  block_sum<NV, NT>(loc, tot);
   if (threadIdx.x == 0) __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return true;
 }

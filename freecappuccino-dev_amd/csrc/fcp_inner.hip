@@ -40,6 +40,10 @@ constexpr int TK_SLOTS = 0xFFFF;  // LDS slots a 16-bit dependency can name
 #endif
 constexpr int TK_U = FCP_TK_U;  // SpMV rows per thread in flight together
 constexpr int TPD = FCP_TPD;  // sweep steps of operand prefetch
+#ifndef FCP_TK_NT
+#define FCP_TK_NT 256
+#endif
+constexpr int TK_NT = FCP_TK_NT;  // threads of a tk_pre workgroup (BLK / TK_NT positions each)
 
 template <int NR>
 __device__ __forceinline__ bool tk_all_done(const SolveScalars *s) {
@@ -186,91 +190,155 @@ __global__ void tk_sell_val(int n, int nslices, const int *__restrict__ perm,
 // ---------------------------------------------------------------------------
 // sweeps in LDS
 // ---------------------------------------------------------------------------
+// Operands of the PP positions (PP = BLK / NT) a thread of an NT-thread
+// workgroup handles in one step: position sb.x + t + q NT for q < PP.
+template <int PP>
 struct TSlot {
-  int ok, i;  // position valid; its LDS slot
-  int2 d;     // 4 x 16-bit dependency slots
-  float4 a;
-  float dv;
+  int ok[PP], i[PP];  // position valid; its LDS slot
+  int2 d[PP];         // 4 x 16-bit dependency slots
+  float4 a[PP];
+  float dv[PP];
 };
 
 // Operands for this thread of the step with position bounds sb (valid: the
 // step exists).  Lanes without a position load the tile's first one: a
 // branch around the loads would make the compiler drain every outstanding
 // prefetch where the paths join.
-__device__ __forceinline__ void tk_load(TSlot &S, int2 sb, bool valid, int lo,
+template <int PP, int NT>
+__device__ __forceinline__ void tk_load(TSlot<PP> &S, int2 sb, bool valid, int lo,
                                         const int2 *__restrict__ dd,
                                         const float4 *__restrict__ aa,
-                                        const float *__restrict__ dv) {
-  const int i = sb.x + (int)threadIdx.x;
-  S.ok = (valid && i < sb.y) ? 1 : 0;
-  const int ii = S.ok ? i : lo;
-  S.i = ii - lo;
-  S.d = dd[ii];
-  S.a = aa[ii];
-  S.dv = dv[ii];
+                                        const float *__restrict__ dv, int t = (int)threadIdx.x) {
+#pragma unroll
+  for (int q = 0; q < PP; ++q) {
+    const int i = sb.x + t + q * NT;
+    S.ok[q] = (valid && i < sb.y) ? 1 : 0;
+    const int ii = S.ok[q] ? i : lo;
+    S.i[q] = ii - lo;
+    S.d[q] = dd[ii];
+    S.a[q] = aa[ii];
+    S.dv[q] = dv[ii];
+  }
 }
 
-// One position of a step, in place: forward (MODE 0)
+// The positions of a step, in place: forward (MODE 0)
 // v_i = (v_i - sum_m a_m v_dep(m)) * d_i; backward (MODE 1)
 // v_i = (v_i / (d_i + small) - sum_m a_m v_dep(m)) * d_i; dependencies in
 // CSR order.  An unused slot subtracts 0 * 0 from the sum, which leaves every
 // value (signed zeros, infinities and NaN included) unchanged, so the result
-// is the one of the sum over the used slots only.
-template <int NR, int NRP, int MODE>
-__device__ __forceinline__ void tk_step(const TSlot &S, float *V) {
-  if (!S.ok) return;
-  const unsigned dx = (unsigned)S.d.x, dy = (unsigned)S.d.y;
-  const int dep[FCP_PMAXD] = {(int)(dx & 0xFFFF), (int)(dx >> 16), (int)(dy & 0xFFFF),
-                              (int)(dy >> 16)};
-  const float av[FCP_PMAXD] = {S.a.x, S.a.y, S.a.z, S.a.w};
+// is the one of the sum over the used slots only.  A step's positions are
+// independent, so a thread's PP chains overlap.
+template <int NR, int NRP, int MODE, int PP>
+__device__ __forceinline__ void tk_step(const TSlot<PP> &S, float *V) {
 #pragma unroll
-  for (int c = 0; c < NR; ++c) {
-    float acc = V[S.i * NRP + c];
-    if (MODE == 1) acc = acc / (S.dv + (float)FCP_SMALL);
+  for (int q = 0; q < PP; ++q) {
+    if (!S.ok[q]) continue;
+    const unsigned dx = (unsigned)S.d[q].x, dy = (unsigned)S.d[q].y;
+    const int dep[FCP_PMAXD] = {(int)(dx & 0xFFFF), (int)(dx >> 16), (int)(dy & 0xFFFF),
+                                (int)(dy >> 16)};
+    const float av[FCP_PMAXD] = {S.a[q].x, S.a[q].y, S.a[q].z, S.a[q].w};
 #pragma unroll
-    for (int m = 0; m < FCP_PMAXD; ++m) acc = acc - av[m] * V[dep[m] * NRP + c];
-    V[S.i * NRP + c] = acc * S.dv;
+    for (int c = 0; c < NR; ++c) {
+      float acc = V[S.i[q] * NRP + c];
+      if (MODE == 1) acc = acc / (S.dv[q] + (float)FCP_SMALL);
+#pragma unroll
+      for (int m = 0; m < FCP_PMAXD; ++m) acc = acc - av[m] * V[dep[m] * NRP + c];
+      V[S.i[q] * NRP + c] = acc * S.dv[q];
+    }
   }
 }
 
 // Both sweeps of one tile over V (ns >= 1 steps; sb[k] = position bounds of
-// step k, in LDS).  Operands are prefetched TPD steps ahead; the bounds of
-// the step to prefetch are read one step early (always from a valid slot),
-// so a step's critical path holds only its own LDS reads and write.
-template <int NR, int NRP>
+// step k, in LDS) by an NT-thread workgroup.  Operands are prefetched TPD
+// steps ahead; the bounds of the step to prefetch are read one step early
+// (always from a valid slot), so a step's critical path holds only its own
+// LDS reads and write.
+template <int NR, int NRP, int NT = BLK>
 __device__ __forceinline__ void tk_sweeps(float *V, const int2 *sb, int ns, int lo,
                                           const int2 *__restrict__ dl,
                                           const float4 *__restrict__ al,
                                           const int2 *__restrict__ du,
                                           const float4 *__restrict__ au,
                                           const float *__restrict__ dv) {
-  TSlot S[TPD];
+  constexpr int PP = BLK / NT;
+  TSlot<PP> S[TPD];
 #pragma unroll
-  for (int j = 0; j < TPD; ++j) tk_load(S[j], sb[j < ns ? j : 0], j < ns, lo, dl, al, dv);
+  for (int j = 0; j < TPD; ++j)
+    tk_load<PP, NT>(S[j], sb[j < ns ? j : 0], j < ns, lo, dl, al, dv);
   int2 nxt = sb[TPD < ns ? TPD : 0];
   for (int base = 0; base < ns; base += TPD) {
 #pragma unroll
     for (int j = 0; j < TPD; ++j) {
       const int nx = base + j + TPD;
       const int2 nn = sb[nx + 1 < ns ? nx + 1 : 0];
-      tk_step<NR, NRP, 0>(S[j], V);
+      tk_step<NR, NRP, 0, PP>(S[j], V);
       __syncthreads();
-      tk_load(S[j], nxt, nx < ns, lo, dl, al, dv);
+      tk_load<PP, NT>(S[j], nxt, nx < ns, lo, dl, al, dv);
       nxt = nn;
     }
   }
 #pragma unroll
   for (int j = 0; j < TPD; ++j)
-    tk_load(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv);
+    tk_load<PP, NT>(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv);
   nxt = sb[TPD < ns ? ns - 1 - TPD : 0];
   for (int base = 0; base < ns; base += TPD) {
 #pragma unroll
     for (int j = 0; j < TPD; ++j) {
       const int nx = base + j + TPD;
       const int2 nn = sb[nx + 1 < ns ? ns - 2 - nx : 0];
-      tk_step<NR, NRP, 1>(S[j], V);
+      tk_step<NR, NRP, 1, PP>(S[j], V);
       __syncthreads();
-      tk_load(S[j], nxt, nx < ns, lo, du, au, dv);
+      tk_load<PP, NT>(S[j], nxt, nx < ns, lo, du, au, dv);
+      nxt = nn;
+    }
+  }
+}
+
+#ifndef FCP_TPDW
+#define FCP_TPDW 8
+#endif
+constexpr int TPDW = FCP_TPDW;  // steps of operand prefetch of a one-wave tile sweep
+
+// Both sweeps of one tile by one wave (steps of at most 64 positions): the
+// wave's own LDS accesses are performed in program order, so no workgroup
+// barrier separates the steps; a wave barrier only keeps the compiler from
+// moving LDS accesses across a step.  Steps are short, so operands are
+// prefetched TPDW steps ahead.
+template <int NR, int NRP>
+__device__ __forceinline__ void tk_sweeps_w(float *V, const int2 *sb, int ns, int lo, int lane,
+                                            const int2 *__restrict__ dl,
+                                            const float4 *__restrict__ al,
+                                            const int2 *__restrict__ du,
+                                            const float4 *__restrict__ au,
+                                            const float *__restrict__ dv) {
+  TSlot<1> S[TPDW];
+#pragma unroll
+  for (int j = 0; j < TPDW; ++j)
+    tk_load<1, 64>(S[j], sb[j < ns ? j : 0], j < ns, lo, dl, al, dv, lane);
+  int2 nxt = sb[TPDW < ns ? TPDW : 0];
+  for (int base = 0; base < ns; base += TPDW) {
+#pragma unroll
+    for (int j = 0; j < TPDW; ++j) {
+      const int nx = base + j + TPDW;
+      const int2 nn = sb[nx + 1 < ns ? nx + 1 : 0];
+      tk_step<NR, NRP, 0, 1>(S[j], V);
+      __builtin_amdgcn_wave_barrier();
+      tk_load<1, 64>(S[j], nxt, nx < ns, lo, dl, al, dv, lane);
+      nxt = nn;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < TPDW; ++j)
+    tk_load<1, 64>(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv, lane);
+  nxt = sb[TPDW < ns ? ns - 1 - TPDW : 0];
+  for (int base = 0; base < ns; base += TPDW) {
+#pragma unroll
+    for (int j = 0; j < TPDW; ++j) {
+      const int nx = base + j + TPDW;
+      const int2 nn = sb[nx + 1 < ns ? ns - 2 - nx : 0];
+      tk_step<NR, NRP, 1, 1>(S[j], V);
+      __builtin_amdgcn_wave_barrier();
+      tk_load<1, 64>(S[j], nxt, nx < ns, lo, du, au, dv, lane);
       nxt = nn;
     }
   }
@@ -285,8 +353,8 @@ __device__ __forceinline__ void tk_sweeps(float *V, const int2 *sb, int ns, int 
 // s->imax iterations); then
 // z = M^-1 r and sk = r.z, bet = sk / s0.  Columns already done are left
 // untouched.
-template <int NR, int NRP>
-__global__ __launch_bounds__(BLK) void tk_pre(
+template <int NR, int NRP, int NT>
+__global__ __launch_bounds__(NT) void tk_pre(
     int nb, int vslots, int dslot, const int *__restrict__ bpos, const int *__restrict__ blk_step,
     const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const float4 *__restrict__ al,
     const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
@@ -309,13 +377,13 @@ __global__ __launch_bounds__(BLK) void tk_pre(
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
     const int lo = bpos[b], hi = bpos[b + 1];
     const int st0 = blk_step[b], ns = blk_step[b + 1] - st0;
-    for (int k = threadIdx.x; k < ns; k += BLK)
+    for (int k = threadIdx.x; k < ns; k += NT)
       sb[k] = make_int2(step_ptr[st0 + k], step_ptr[st0 + k + 1]);
-    for (int i0 = lo + (int)threadIdx.x; i0 < hi; i0 += GS_U * BLK) {
+    for (int i0 = lo + (int)threadIdx.x; i0 < hi; i0 += GS_U * NT) {
       float xv[GS_U][NR], pv[GS_U][NR], rv[GS_U][NR], qv[GS_U][NR];
 #pragma unroll
       for (int u = 0; u < GS_U; ++u) {
-        const int i = i0 + u * BLK;
+        const int i = i0 + u * NT;
         if (i < hi) {
 #pragma unroll
           for (int c = 0; c < NR; ++c) {
@@ -331,7 +399,7 @@ __global__ __launch_bounds__(BLK) void tk_pre(
       }
 #pragma unroll
       for (int u = 0; u < GS_U; ++u) {
-        const int i = i0 + u * BLK;
+        const int i = i0 + u * NT;
         if (i < hi) {
 #pragma unroll
           for (int c = 0; c < NR; ++c) {
@@ -351,9 +419,9 @@ __global__ __launch_bounds__(BLK) void tk_pre(
     }
     if (threadIdx.x < NRP) V[dslot * NRP + threadIdx.x] = 0.0f;  // the unused-dependency slot
     __syncthreads();
-    if (ns > 0) tk_sweeps<NR, NRP>(V, sb, ns, lo, dl, al, du, au, dv);
+    if (ns > 0) tk_sweeps<NR, NRP, NT>(V, sb, ns, lo, dl, al, du, au, dv);
     // z out; r.z with the residual this thread wrote (or read) above
-    for (int i = lo + (int)threadIdx.x; i < hi; i += BLK) {
+    for (int i = lo + (int)threadIdx.x; i < hi; i += NT) {
 #pragma unroll
       for (int c = 0; c < NR; ++c) {
         const size_t o = (size_t)i * NRP + c;
@@ -363,6 +431,127 @@ __global__ __launch_bounds__(BLK) void tk_pre(
       }
     }
     __syncthreads();  // V and sb are reused by the next tile
+  }
+  double blk[3 * NR], tot[3 * NR];
+  block_sum<3 * NR, NT>(part, blk);
+  if (last_block_reduce<3 * NR, NT>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (!act[c]) continue;
+      SolveScalars &S = s[c];
+      bool fin = false;
+      if (upd) {
+        const double resl = tot[3 * c];
+        S.resl = resl;
+        S.l2 = sqrt(tot[3 * c + 1]);
+        S.s0 = S.sk;
+        S.iter += 1;
+        S.rsm = resl / (S.res0 + FCP_SMALL);
+        bool conv = (S.l2 <= S.tol * S.l2b) || (resl < tol_abs);
+        if (!isfinite(resl)) {
+          S.status = FCP_ERR_NONFINITE;
+          conv = true;
+        }
+        S.conv = (conv && S.status == 0) ? 1 : 0;
+        if (conv || S.iter >= S.imax) {
+          S.done = 1;
+          fin = true;
+        }
+      }
+      if (!fin) {
+        S.sk = tot[3 * c + 2];
+        S.bet = S.sk / S.s0;
+      }
+    }
+  }
+}
+
+// tk_pre with one wave per tile: four tiles per workgroup, no workgroup
+// barrier inside a tile (tiles of one schedule built with steps of at most
+// 64 positions).
+template <int NR, int NRP>
+__global__ __launch_bounds__(BLK) void tk_pre_w(
+    int nb, int vstride, int vslots, int dslot, const int *__restrict__ bpos, const int *__restrict__ blk_step,
+    const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const float4 *__restrict__ al,
+    const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
+    float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
+    const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
+    int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs) {
+  // one region of vstride floats per wave: [vslots] values (slot dslot = 0),
+  // then the step bounds
+  extern __shared__ float VW[];
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  float *V = VW + (size_t)wv * vstride;
+  int2 *sb = reinterpret_cast<int2 *>(V + vslots);
+  if (tk_all_done<NR>(s)) return;
+  bool act[NR];
+  float alf[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    act[c] = !solve_done(s + c);
+    alf[c] = (float)s[c].alf;
+  }
+  double part[3 * NR];
+#pragma unroll
+  for (int k = 0; k < 3 * NR; ++k) part[k] = 0.0;
+  for (int b = blockIdx.x * (BLK / 64) + wv; b < nb; b += gridDim.x * (BLK / 64)) {
+    const int lo = bpos[b], hi = bpos[b + 1];
+    const int st0 = blk_step[b], ns = blk_step[b + 1] - st0;
+    for (int k = lane; k < ns; k += 64)
+      sb[k] = make_int2(step_ptr[st0 + k], step_ptr[st0 + k + 1]);
+    for (int i0 = lo + lane; i0 < hi; i0 += GS_U * 64) {
+      float xv[GS_U][NR], pv[GS_U][NR], rv[GS_U][NR], qv[GS_U][NR];
+#pragma unroll
+      for (int u = 0; u < GS_U; ++u) {
+        const int i = i0 + u * 64;
+        if (i < hi) {
+#pragma unroll
+          for (int c = 0; c < NR; ++c) {
+            const size_t o = (size_t)i * NRP + c;
+            rv[u][c] = r[o];
+            if (upd && act[c]) {
+              xv[u][c] = x[o];
+              pv[u][c] = p[o];
+              qv[u][c] = q[o];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < GS_U; ++u) {
+        const int i = i0 + u * 64;
+        if (i < hi) {
+#pragma unroll
+          for (int c = 0; c < NR; ++c) {
+            const size_t o = (size_t)i * NRP + c;
+            float ri = rv[u][c];
+            if (upd && act[c]) {
+              x[o] = xv[u][c] + alf[c] * pv[u][c];
+              ri = ri - alf[c] * qv[u][c];
+              r[o] = ri;
+              part[3 * c] += fabs((double)ri);
+              part[3 * c + 1] += (double)ri * (double)ri;
+            }
+            V[(i - lo) * NRP + c] = ri;
+          }
+        }
+      }
+    }
+    if (lane < NRP) V[dslot * NRP + lane] = 0.0f;  // the unused-dependency slot
+    __builtin_amdgcn_wave_barrier();
+    if (ns > 0) tk_sweeps_w<NR, NRP>(V, sb, ns, lo, lane, dl, al, du, au, dv);
+    __builtin_amdgcn_wave_barrier();
+    // z out; r.z with the residual this thread wrote (or read) above
+    for (int i = lo + lane; i < hi; i += 64) {
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const size_t o = (size_t)i * NRP + c;
+        const float zi = V[(i - lo) * NRP + c];
+        z[o] = zi;
+        part[3 * c + 2] += (double)r[o] * (double)zi;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // V and sb are reused by the wave's next tile
   }
   double blk[3 * NR], tot[3 * NR];
   block_sum<3 * NR>(part, blk);
@@ -863,8 +1052,15 @@ __global__ void tk_scatter_out(int n, const int *__restrict__ perm, const float 
   if (i < n) z[perm[i]] = (double)v[i];
 }
 
+// LDS of one wave's tile region, in floats (values, then the step bounds).
+int tk_wave_stride(const fcp_ctx *ctx, int nrp) {
+  const BlockSchedule &b = ctx->jlower;
+  return (((b.R + 1) * nrp + 1) & ~1) + 2 * (b.max_steps + 1);
+}
+
 size_t tk_lds(const fcp_ctx *ctx, int nrp) {
   const BlockSchedule &b = ctx->jlower;
+  if (ctx->bj_wave) return (size_t)(BLK / 64) * tk_wave_stride(ctx, nrp) * sizeof(float);
   // values (R slots and the zero slot), then 8-byte aligned step bounds
   return (size_t)(((b.R + 1) * nrp + 1) & ~1) * sizeof(float) +
          (size_t)(b.max_steps + 1) * sizeof(int2);
@@ -948,7 +1144,19 @@ template <int NR, int NRP>
 int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *si, int upd) {
   const BlockSchedule &b = ctx->jlower;
   ProfScope ps(ctx, "tin_pre");
-  tk_pre<NR, NRP><<<tk_grid(ctx), BLK, tk_lds(ctx, NRP), ctx->stream>>>(
+  if (ctx->bj_wave) {
+    int g = (b.nblocks + BLK / 64 - 1) / (BLK / 64);
+    g = g < MAXBLK ? g : MAXBLK;
+    tk_pre_w<NR, NRP><<<g, BLK, tk_lds(ctx, NRP), ctx->stream>>>(
+        b.nblocks, tk_wave_stride(ctx, NRP), ((b.R + 1) * NRP + 1) & ~1, b.R, b.blk_pos.p,
+        b.blk_step.p, b.step_ptr.p, reinterpret_cast<const int2 *>(ctx->tin_dl.p),
+        reinterpret_cast<const float4 *>(ctx->tin_al.p),
+        reinterpret_cast<const int2 *>(ctx->tin_du.p),
+        reinterpret_cast<const float4 *>(ctx->tin_au.p), ctx->tin_dv.p, B.x, B.r, p, B.q, B.z,
+        ctx->partials.p, redctr(ctx), si, upd, 0.0);
+    return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
+  }
+  tk_pre<NR, NRP, TK_NT><<<tk_grid(ctx), TK_NT, tk_lds(ctx, NRP), ctx->stream>>>(
       b.nblocks, ((b.R + 1) * NRP + 1) & ~1, b.R, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
       reinterpret_cast<const int2 *>(ctx->tin_dl.p),
       reinterpret_cast<const float4 *>(ctx->tin_al.p),

@@ -142,6 +142,7 @@ struct fcp_ctx {
   // numbering) or contiguous blocks, dependencies inside the block only
   BlockSchedule jlower, jupper;
   bool have_bj = false;
+  bool bj_wave = false;              // tiles small enough for one wave each (steps <= 64 rows)
   std::vector<int> bj_tile;          // [n] block of every row (host)
   DBuf<double> dbj;                  // [n] tile-local DIC / DILU pivots (inverted)
   uint64_t bj_version = ~0ull, bj_epoch = 0;
@@ -299,7 +300,7 @@ inline int fcp_block_rows(const fcp_ctx *ctx, int elem = 8, int upper = 0) {
 // preconditioner) and no row is external.
 int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, BlockSchedule &b,
                              const int *tile_rank = nullptr, int ntiles = 0,
-                             bool local_only = false);
+                             bool local_only = false, int step_max = 256);
 int fcp_build_bj_schedules(fcp_ctx *ctx);
 int fcp_build_tile_schedules(fcp_ctx *ctx);
 // fcp_assemble.hip

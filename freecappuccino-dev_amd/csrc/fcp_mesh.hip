@@ -531,7 +531,7 @@ __global__ void k_pack_static(const int *__restrict__ rows, int n, int upper, in
 // the runs of equal level inside a block.  Blocks are contiguous runs of R
 // rows, or -- with tile_rank -- the given tiles in rank order.
 int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, BlockSchedule &b,
-                             const int *tile_rank, int ntiles, bool local_only) {
+                             const int *tile_rank, int ntiles, bool local_only, int step_max) {
   const int n = ctx->n;
   b.tiled = tile_rank != nullptr;
   b.nblocks = tile_rank ? ntiles : (n + R - 1) / R;
@@ -567,10 +567,10 @@ int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, 
       prev_blk = blk;
       prev_lev = -1;
     }
-    // a new step at every level change, and after BLK rows of one level:
-    // rows of a level are independent, and a step the sweep kernels take
-    // with one row per thread is pipelined (wider steps are not)
-    if (lev[r] != prev_lev || i - sp.back() >= BLK) {
+    // a new step at every level change, and after step_max rows of one
+    // level: rows of a level are independent, and a step the sweep kernels
+    // take with one row per thread is pipelined (wider steps are not)
+    if (lev[r] != prev_lev || i - sp.back() >= step_max) {
       sp.push_back(i);
       prev_lev = lev[r];
     }
@@ -583,7 +583,7 @@ int fcp_build_block_schedule(fcp_ctx *ctx, const Schedule &s, int R, int upper, 
   sp.push_back(n);
   // the pipelined sweeps take one position per thread and step
   for (size_t k = 0; k + 1 < sp.size(); ++k)
-    if (sp[k + 1] - sp[k] > BLK)
+    if (sp[k + 1] - sp[k] > step_max)
       return fcp_fail(ctx, FCP_ERR_STATE, "block schedule: step of %d rows", sp[k + 1] - sp[k]);
   b.max_steps = 0;
   b.R = 0;
@@ -759,8 +759,15 @@ int fcp_build_bj_schedules(fcp_ctx *ctx) {
     for (int i = 0; i < n; ++i) rank[i] = i / target;
     m = (n + target - 1) / target;
   }
-  FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, 0, 0, ctx->jlower, rank.data(), m, true));
-  FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, 0, 1, ctx->jupper, rank.data(), m, true));
+  // tiles of up to 2048 cells are swept by one wave each: steps of at most
+  // 64 rows
+  const bool wave = target <= 2048;
+  const int step_max = wave ? 64 : 256;
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, 0, 0, ctx->jlower, rank.data(), m, true,
+                                   step_max));
+  FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, 0, 1, ctx->jupper, rank.data(), m, true,
+                                   step_max));
+  ctx->bj_wave = wave;
   ctx->bj_tile.swap(rank);
   ctx->bj_version = ~0ull;
   ctx->have_bj = true;

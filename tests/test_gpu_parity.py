@@ -506,16 +506,18 @@ def _tile_sweeps(o, a, d, r, tile, dtype):
     return z
 
 
-@pytest.mark.parametrize("kind", [5, 6])
-def test_tile_order_preconditioner_fp32_bitexact(api, kind):
+@pytest.mark.parametrize("kind,tile", [(5, 0), (6, 0), (5, 512), (6, 512)])
+def test_tile_order_preconditioner_fp32_bitexact(api, kind, tile):
     """The FP32 tile-order kernel of the inner solves (kinds 5 DIC / 6 DILU)
     applies the tile-local preconditioner with FP32 roundings of the
     coefficients, pivots and input, bit for bit as the scalar FP32 recurrence;
     that recurrence run in FP64 equals the oracle's application to the
-    block-diagonal matrix bit for bit."""
-    print(f"[case] tile-order precond kind {kind} 32^3", flush=True)
+    block-diagonal matrix bit for bit.  Tiles of 4096 rows (one workgroup per
+    tile) and of 512 rows (one wave per tile)."""
+    print(f"[case] tile-order precond kind {kind} tile {tile} 32^3", flush=True)
     mesh = hex_mesh(32, jitter=0.2, seed=4)
     ctx, geo, csr = _setup(api, mesh)
+    ctx.tune(api.FCP_TUNE_INNER_TILE, tile)
     o = Oracle(mesh)
     if kind == 5:
         mu = -(10.0 ** np.sin(np.arange(mesh.n_cells)))
@@ -569,6 +571,32 @@ def test_inner_graph_replay_bitequal_to_launches(api):
     for x, X in out[1]:
         assert np.array_equal(x, out[0][0][0])
         assert np.array_equal(X, out[0][0][1])
+    ctx.close()
+
+
+def test_inner_wave_tiles_batched_equal_single(api):
+    """Inner PCG with 512-row tiles (one wave per tile): two batched columns
+    equal single-column solves bit for bit and reach the tolerance."""
+    print("[case] wave tiles 24^3", flush=True)
+    mesh = workloads.poisson_jittered_mesh(24)
+    ctx, geo, csr = _setup(api, mesh)
+    ctx.tune(api.FCP_TUNE_INNER_TILE, 512)
+    assert ctx.sweep_tiles().max() >= 15
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    su = inp["source"] * geo["vol"]
+    ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su)
+    n = mesh.n_cells
+    B = np.stack([su, np.cos(np.arange(n)) * 1e-3])
+    X = np.zeros((2, n))
+    rc, info, _ = ctx.csrsolve_multi("iccg", X, B, itr_max=5000, tol_rel=1e-10,
+                                     precision=api.FCP_PREC_MIXED, criterion=api.FCP_CRIT_TRUE_L2)
+    assert rc == 0 and info["rel_res_l2"] <= 1e-10, info
+    for r in range(2):
+        x = np.zeros(n)
+        rc1, info1 = ctx.csrsolve("iccg", x, B[r].copy(), itr_max=5000, tol_rel=1e-10,
+                                  precision=api.FCP_PREC_MIXED, criterion=api.FCP_CRIT_TRUE_L2)
+        assert rc1 == 0 and info1["rel_res_l2"] <= 1e-10
+        assert np.array_equal(X[r], x), r
     ctx.close()
 
 
