@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 #include "fcp_internal.h"
 
@@ -885,6 +887,133 @@ int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi, c
            names[code], var.c_str(), o->resor, o->resl / factor, o->iterations);
   }
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// autotuner of the tile size of the FP32 inner solves' preconditioner
+// ---------------------------------------------------------------------------
+namespace {
+// 64-bit finaliser of MurmurHash3 (public domain): mixes one pattern entry.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long v) {
+  v ^= v >> 33;
+  v *= 0xff51afd7ed558ccdULL;
+  v ^= v >> 33;
+  v *= 0xc4ceb9fe1a85ec53ULL;
+  v ^= v >> 33;
+  return v;
+}
+
+// Per-block sums (mod 2^64) of the mixed (position, value) pairs of ia and
+// ja: an order-independent signature of the CSR pattern.
+__global__ void k_pattern_sig(const int *__restrict__ ia, const int *__restrict__ ja, int n,
+                              int nnz, unsigned long long *__restrict__ part) {
+  __shared__ unsigned long long sh[256];
+  unsigned long long s = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nnz; k += gridDim.x * blockDim.x)
+    s += mix64(((unsigned long long)(unsigned)ja[k] << 32) | (unsigned)k);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= n; k += gridDim.x * blockDim.x)
+    s += mix64((((unsigned long long)(unsigned)ia[k] << 32) | (unsigned)k) ^ 0x9e3779b97f4a7c15ULL);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_fill64(double *__restrict__ x, int n, double v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
+}
+
+// (cells, entries, pattern signature, solver) -> chosen tile rows
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, unsigned long long, int>, int> g_tune_cache;
+
+void set_inner_tile(fcp_ctx *ctx, int t) {
+  if (t == ctx->tune_inner_tile) return;
+  ctx->tune_inner_tile = t;
+  ctx->have_bj = false;
+  ctx->tin_have = false;
+}
+}  // namespace
+
+extern "C" {
+
+int fcp_autotune(fcp_ctx *ctx, const char *solver, int solver_len, int *tile_out, int *cached) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!solver) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_autotune: null solver");
+  const int code = solver_code(solver, solver_len);
+  if (code != 0 && code != 2)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_autotune: solver must be iccg or bicgstab");
+  (void)hipSetDevice(ctx->device);
+  const int n = ctx->n;
+  constexpr int G = 256;
+  DBuf<unsigned long long> part;
+  FCP_HIP(ctx, part.alloc(G));
+  k_pattern_sig<<<G, 256, 0, ctx->stream>>>(ctx->ia.p, ctx->ja.p, n, ctx->nnz, part.p);
+  std::vector<unsigned long long> hp(G);
+  FCP_HIP(ctx, hipMemcpyAsync(hp.data(), part.p, sizeof(unsigned long long) * G,
+                              hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  unsigned long long sig = 0;
+  for (unsigned long long v : hp) sig += v;
+  const auto key = std::make_tuple(n, ctx->nnz, sig, code);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      set_inner_tile(ctx, it->second);
+      if (tile_out) *tile_out = it->second;
+      if (cached) *cached = 1;
+      return 0;
+    }
+  }
+  // trials: the mixed solve of A x = A 1 to a relative L2 residual of 1e-6,
+  // once to build the tile schedules and once timed, per candidate
+  DBuf<double> ones, b, x;
+  FCP_HIP(ctx, ones.alloc(n));
+  FCP_HIP(ctx, b.alloc(n));
+  FCP_HIP(ctx, x.alloc(n));
+  k_fill64<<<grid_for(n, 256) < 1024 ? grid_for(n, 256) : 1024, 256, 0, ctx->stream>>>(ones.p, n, 1.0);
+  FCP_TRY(fcp_launch_spmv64(ctx, ctx->a.p, ones.p, b.p));
+  const int prev = ctx->tune_inner_tile;
+  int best = prev;
+  double best_ms = 0.0;
+  bool found = false;
+  for (int t : {1024, 2048, 4096, 8192}) {
+    if (n < 4 * t) continue;
+    set_inner_tile(ctx, t);
+    for (int rep = 0; rep < 2; ++rep) {
+      k_fill64<<<grid_for(n, 256) < 1024 ? grid_for(n, 256) : 1024, 256, 0, ctx->stream>>>(
+          x.p, n, 0.0);
+      fcp_solve_opts o;
+      fcp_solve_opts_default(&o);
+      o.precision = FCP_PREC_MIXED;
+      o.criterion = FCP_CRIT_TRUE_L2;
+      o.check_every = 16;
+      const int rc = fcp_solve_device(ctx, code, x.p, b.p, 20000, 0.0, 1e-6, 0, &o);
+      if (rc < 0) {
+        set_inner_tile(ctx, prev);
+        return rc;
+      }
+      if (rep == 1 && rc == 0 && (!found || o.time_ms < best_ms)) {
+        found = true;
+        best = t;
+        best_ms = o.time_ms;
+      }
+    }
+  }
+  set_inner_tile(ctx, found ? best : prev);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = ctx->tune_inner_tile;
+  }
+  if (tile_out) *tile_out = ctx->tune_inner_tile;
+  if (cached) *cached = 0;
+  return 0;
 }
 
 int fcp_csrsolve_multi(fcp_ctx *ctx, const char *solver, int solver_len, int nrhs, double *fi,

@@ -43,6 +43,7 @@ SIGNATURES = {
     "fcp_spmv_transpose": (_i, [_p, _p, _p]),
     "fcp_ilu_factor": (_i, [_p, _p]),
     "fcp_sweep_tiles": (_i, [_p, _p]),
+    "fcp_autotune": (_i, [_p, C.c_char_p, _i, C.POINTER(_i), C.POINTER(_i)]),
     "fcp_csrsolve": (_i, [_p, C.c_char_p, _i, _p, _p, C.POINTER(_d), _i, _d, _d, C.c_char_p, _i,
                           _i, C.POINTER(SolveOpts)]),
     "fcp_rcm": (_i, [_p, _i, _p]),

@@ -271,6 +271,15 @@ class Context:
                     "fcp_spmv_transpose")
         return y
 
+    def autotune(self, solver: str = "iccg"):
+        """fcp_autotune: picks the tile size of the FP32 inner solves'
+        preconditioner for this mesh; returns (tile_rows, cached)."""
+        t, c = C.c_int(0), C.c_int(0)
+        sb = solver.encode()
+        self._check(self.lib.fcp_autotune(self.h, sb, len(sb), C.byref(t), C.byref(c)),
+                    "fcp_autotune")
+        return t.value, bool(c.value)
+
     def sweep_tiles(self):
         """Tile of every cell in the tile-local preconditioner (0-based)."""
         t = np.zeros(self.mesh.n_cells, np.int32)

@@ -204,6 +204,15 @@ module fcp_capi
       integer(c_int) :: tile(*)
     end function
 
+    integer(c_int) function fcp_autotune(ctx, solver, solver_len, tile_out, cached) &
+        bind(C, name='fcp_autotune')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: solver(*)
+      integer(c_int), value :: solver_len
+      integer(c_int) :: tile_out, cached
+    end function
+
     integer(c_int) function fcp_ilu_factor(ctx, l) bind(C, name='fcp_ilu_factor')
       import :: c_int, c_double, c_ptr
       type(c_ptr), value :: ctx

@@ -355,6 +355,18 @@ int fcp_pmgmres(int n, int nnz, const int *ia, const int *ja, const double *a,
  * its stopping test). */
 int fcp_tune_set(fcp_ctx *ctx, int param, int value);
 
+/* Autotuner (no counterpart in the reference; the launch-shape search of the
+ * mixed-precision path): for solver "iccg" or "bicgstab", times the mixed
+ * FP32-inner solve of A x = A*1 on the resident matrix to a true relative
+ * residual of 1e-6 with tile-local preconditioner tiles of 1024, 2048, 4096
+ * and 8192 cells (those with at least four tiles), keeps the fastest in the
+ * context (FCP_TUNE_INNER_TILE) and caches it for the rest of the process
+ * under the CSR pattern (cell count, entry count, a 64-bit signature of ia
+ * and ja) and the solver, so later contexts on the same mesh take it
+ * without trials.  tile_out (optional) receives the tile rows chosen (0:
+ * the default), cached (optional) 1 when the cache supplied it. */
+int fcp_autotune(fcp_ctx *ctx, const char *solver, int solver_len, int *tile_out, int *cached);
+
 /* --- instrumentation ------------------------------------------------------ */
 /* Per-kernel device timing accumulated on the context's stream with HIP
  * events (name -> total ms, launches).  enable=1 starts, 0 stops. */

@@ -574,6 +574,32 @@ def test_inner_graph_replay_bitequal_to_launches(api):
     ctx.close()
 
 
+def test_autotune_picks_and_caches(api):
+    """fcp_autotune picks one of the candidate tile sizes for the mesh; a
+    second call, and a new context on the same mesh, take it from the cache;
+    the tuned mixed solve reaches its tolerance."""
+    print("[case] autotune 40^3", flush=True)
+    mesh = workloads.poisson_jittered_mesh(40)
+    ctxs = []
+    for k in range(2):
+        ctx, geo, csr = _setup(api, mesh)
+        inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+        su = inp["source"] * geo["vol"]
+        ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su)
+        ctxs.append((ctx, su))
+    ctx, su = ctxs[0]
+    t, cached = ctx.autotune("iccg")
+    assert t in (1024, 2048, 4096, 8192) and not cached, (t, cached)
+    assert ctx.autotune("iccg") == (t, True)
+    x = np.zeros(mesh.n_cells)
+    rc, info = ctx.csrsolve("iccg", x, su.copy(), itr_max=5000, tol_abs=0.0, tol_rel=1e-10,
+                            precision=api.FCP_PREC_MIXED, criterion=api.FCP_CRIT_TRUE_L2)
+    assert rc == 0 and info["rel_res_l2"] <= 1e-10, info
+    assert ctxs[1][0].autotune("iccg") == (t, True)
+    for c, _ in ctxs:
+        c.close()
+
+
 def test_inner_wave_tiles_batched_equal_single(api):
     """Inner PCG with 512-row tiles (one wave per tile): two batched columns
     equal single-column solves bit for bit and reach the tolerance."""
