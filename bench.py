@@ -360,6 +360,12 @@ def run_tets(args):
     sV = T(inp["source"] * geo["vol"])
     su = torch.empty(n, dtype=torch.float64, device=dev)
     x = torch.zeros(n, dtype=torch.float64, device=dev)
+    # mesh setup: tile size of the FP32 inner solves' preconditioner
+    su.copy_(sV)
+    ctx.laplacian_face(mu_f, bk, pb, su)
+    t0 = time.time()
+    tile, _ = ctx.autotune("iccg")
+    log(f"[tets] autotune: inner tiles of {tile} cells ({time.time()-t0:.1f}s)")
     res = {}
     for name, prec in (("fp64", FCP_PREC_FP64), ("mixed", FCP_PREC_MIXED)):
         def step():
@@ -381,7 +387,7 @@ def run_tets(args):
             "config": {"workload": f"{N}^3x6 Kuhn tets jittered +-15%, cells permuted (seed 3), "
                                    "GPU RCM, Gamma 10^U(-2,2)", "cells": n},
             "rcm_ms": dt_rcm * 1e3, "bandwidth_before": bw0, "bandwidth_after": bw1,
-            "gradients": grads, "poisson": res}
+            "gradients": grads, "poisson": res, "inner_tile": tile}
 
 
 def run_adjoint(args):
