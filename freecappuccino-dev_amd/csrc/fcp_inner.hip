@@ -35,10 +35,7 @@ constexpr int TK_SLOTS = 0xFFFF;  // LDS slots a 16-bit dependency can name
 #ifndef FCP_TPD
 #define FCP_TPD 4
 #endif
-#ifndef FCP_TK_U
-#define FCP_TK_U 1
-#endif
-constexpr int TK_U = FCP_TK_U;  // SpMV rows per thread in flight together
+
 constexpr int TPD = FCP_TPD;  // sweep steps of operand prefetch
 #ifndef FCP_TK_NT
 #define FCP_TK_NT 256
@@ -51,6 +48,26 @@ __device__ __forceinline__ bool tk_all_done(const SolveScalars *s) {
   for (int c = 0; c < NR; ++c)
     if (!solve_done(s + c)) return false;
   return true;
+}
+
+#ifndef FCP_XCD_ROWS
+#define FCP_XCD_ROWS 1
+#endif
+// Rows of a grid-stride loop, XCD-chunked.  Workgroups are dispatched to the
+// eight XCDs round-robin (workgroup b on XCD b % 8), each XCD with its own L2.
+// With a grid that is a multiple of 8, XCD x takes the x-th eighth of the
+// rows (in 256-row pieces), so the vector lines that neighbouring rows
+// gather stay in one L2 instead of being fetched by several.
+struct XRows {
+  int i, end, step;
+};
+__device__ __forceinline__ XRows xcd_rows(int n) {
+  const int G = (int)gridDim.x, b = (int)blockIdx.x, t = (int)threadIdx.x;
+  if (!FCP_XCD_ROWS || (G & 7) != 0) return {b * BLK + t, n, G * BLK};
+  const int chunk = ((n + 8 * BLK - 1) / (8 * BLK)) * BLK;
+  const int lo = (b & 7) * chunk;
+  const int hi = n < lo + chunk ? n : lo + chunk;
+  return {lo + (b >> 3) * BLK + t, hi, (G >> 3) * BLK};
 }
 
 // ---------------------------------------------------------------------------
@@ -610,75 +627,51 @@ __global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ ba
   double part[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) part[c] = 0.0;
-  // TK_U rows per thread at a time, in the plain grid-stride order (the
-  // per-thread sums are those of one row at a time); all their matrix loads,
-  // then all their gathers, are in flight together
-  const int stride = gridDim.x * BLK;
-  for (int i0 = blockIdx.x * BLK + threadIdx.x; i0 < n; i0 += TK_U * stride) {
-    int pp[TK_U], L[TK_U];
+  for (XRows X = xcd_rows(n); X.i < X.end; X.i += X.step) {
+    const int i = X.i;
+    const int sl = i >> 6;
+    const int b0 = base[sl];
+    const int pp = b0 + (i & 63);
+    const int L = (base[sl + 1] - b0) >> 6;
+    float av[SELL_KU];
+    int cv[SELL_KU];
 #pragma unroll
-    for (int u = 0; u < TK_U; ++u) {
-      const int i = i0 + u * stride;
-      L[u] = 0;
-      pp[u] = 0;
-      if (i < n) {
-        const int sl = i >> 6;
-        const int b0 = base[sl];
-        pp[u] = b0 + (i & 63);
-        L[u] = (base[sl + 1] - b0) >> 6;
+    for (int k = 0; k < SELL_KU; ++k)
+      if (k < L) {
+        av[k] = val[pp + 64 * k];
+        cv[k] = col[pp + 64 * k];
+      }
+    float acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < SELL_KU; ++k)
+      if (k < L) {
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          const size_t o = (size_t)cv[k] * NRP + c;
+          const float pj = z[o] + bet[c] * pold[o];
+          acc[c] = acc[c] + av[k] * pj;
+        }
+      }
+    for (int k = SELL_KU; k < L; ++k) {
+      const float a = val[pp + 64 * k];
+      const int j = col[pp + 64 * k];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const size_t o = (size_t)j * NRP + c;
+        const float pj = z[o] + bet[c] * pold[o];
+        acc[c] = acc[c] + a * pj;
       }
     }
-    float av[TK_U][SELL_KU];
-    int cv[TK_U][SELL_KU];
 #pragma unroll
-    for (int u = 0; u < TK_U; ++u)
-#pragma unroll
-      for (int k = 0; k < SELL_KU; ++k)
-        if (k < L[u]) {
-          av[u][k] = val[pp[u] + 64 * k];
-          cv[u][k] = col[pp[u] + 64 * k];
-        }
-    float acc[TK_U][NR];
-#pragma unroll
-    for (int u = 0; u < TK_U; ++u)
-#pragma unroll
-      for (int c = 0; c < NR; ++c) acc[u][c] = 0.0f;
-#pragma unroll
-    for (int u = 0; u < TK_U; ++u)
-#pragma unroll
-      for (int k = 0; k < SELL_KU; ++k)
-        if (k < L[u]) {
-#pragma unroll
-          for (int c = 0; c < NR; ++c) {
-            const size_t o = (size_t)cv[u][k] * NRP + c;
-            const float pj = z[o] + bet[c] * pold[o];
-            acc[u][c] = acc[u][c] + av[u][k] * pj;
-          }
-        }
-#pragma unroll
-    for (int u = 0; u < TK_U; ++u) {
-      for (int k = SELL_KU; k < L[u]; ++k) {
-        const float a = val[pp[u] + 64 * k];
-        const int j = col[pp[u] + 64 * k];
-#pragma unroll
-        for (int c = 0; c < NR; ++c) {
-          const size_t o = (size_t)j * NRP + c;
-          const float pj = z[o] + bet[c] * pold[o];
-          acc[u][c] = acc[u][c] + a * pj;
-        }
-      }
-      const int i = i0 + u * stride;
-      if (i < n) {
-#pragma unroll
-        for (int c = 0; c < NR; ++c) {
-          if (!act[c]) continue;
-          const size_t o = (size_t)i * NRP + c;
-          const float pi = z[o] + bet[c] * pold[o];
-          pnew[o] = pi;
-          q[o] = acc[u][c];
-          part[c] += (double)pi * (double)acc[u][c];
-        }
-      }
+    for (int c = 0; c < NR; ++c) {
+      if (!act[c]) continue;
+      const size_t o = (size_t)i * NRP + c;
+      const float pi = z[o] + bet[c] * pold[o];
+      pnew[o] = pi;
+      q[o] = acc[c];
+      part[c] += (double)pi * (double)acc[c];
     }
   }
   double blk[NR], tot[NR];
@@ -878,7 +871,8 @@ __global__ __launch_bounds__(BLK) void tb_spmv(int n, const int *__restrict__ ba
   double part[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) part[k] = 0.0;
-  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+  for (XRows X = xcd_rows(n); X.i < X.end; X.i += X.step) {
+    const int i = X.i;
     const int sl = i >> 6;
     const int b0 = base[sl];
     const int pp = b0 + (i & 63);
