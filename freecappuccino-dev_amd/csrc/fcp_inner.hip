@@ -221,7 +221,7 @@ struct TSlot {
 // step exists).  Lanes without a position load the tile's first one: a
 // branch around the loads would make the compiler drain every outstanding
 // prefetch where the paths join.
-template <int PP, int NT>
+template <int PP, int NT, bool DV = true>
 __device__ __forceinline__ void tk_load(TSlot<PP> &S, int2 sb, bool valid, int lo,
                                         const int2 *__restrict__ dd,
                                         const float4 *__restrict__ aa,
@@ -234,7 +234,7 @@ __device__ __forceinline__ void tk_load(TSlot<PP> &S, int2 sb, bool valid, int l
     S.i[q] = ii - lo;
     S.d[q] = dd[ii];
     S.a[q] = aa[ii];
-    S.dv[q] = dv[ii];
+    if (DV) S.dv[q] = dv[ii];  // the backward sweep takes the pivot from LDS
   }
 }
 
@@ -246,7 +246,7 @@ __device__ __forceinline__ void tk_load(TSlot<PP> &S, int2 sb, bool valid, int l
 // is the one of the sum over the used slots only.  A step's positions are
 // independent, so a thread's PP chains overlap.
 template <int NR, int NRP, int MODE, int PP>
-__device__ __forceinline__ void tk_step(const TSlot<PP> &S, float *V) {
+__device__ __forceinline__ void tk_step(const TSlot<PP> &S, float *V, float *D) {
 #pragma unroll
   for (int q = 0; q < PP; ++q) {
     if (!S.ok[q]) continue;
@@ -254,13 +254,20 @@ __device__ __forceinline__ void tk_step(const TSlot<PP> &S, float *V) {
     const int dep[FCP_PMAXD] = {(int)(dx & 0xFFFF), (int)(dx >> 16), (int)(dy & 0xFFFF),
                                 (int)(dy >> 16)};
     const float av[FCP_PMAXD] = {S.a[q].x, S.a[q].y, S.a[q].z, S.a[q].w};
+    float dvq;
+    if (MODE == 0) {
+      dvq = S.dv[q];
+      D[S.i[q]] = dvq;  // kept for the backward sweep
+    } else {
+      dvq = D[S.i[q]];
+    }
 #pragma unroll
     for (int c = 0; c < NR; ++c) {
       float acc = V[S.i[q] * NRP + c];
-      if (MODE == 1) acc = acc / (S.dv[q] + (float)FCP_SMALL);
+      if (MODE == 1) acc = acc / (dvq + (float)FCP_SMALL);
 #pragma unroll
       for (int m = 0; m < FCP_PMAXD; ++m) acc = acc - av[m] * V[dep[m] * NRP + c];
-      V[S.i[q] * NRP + c] = acc * S.dv[q];
+      V[S.i[q] * NRP + c] = acc * dvq;
     }
   }
 }
@@ -271,7 +278,7 @@ __device__ __forceinline__ void tk_step(const TSlot<PP> &S, float *V) {
 // (always from a valid slot), so a step's critical path holds only its own
 // LDS reads and write.
 template <int NR, int NRP, int NT = BLK>
-__device__ __forceinline__ void tk_sweeps(float *V, const int2 *sb, int ns, int lo,
+__device__ __forceinline__ void tk_sweeps(float *V, float *D, const int2 *sb, int ns, int lo,
                                           const int2 *__restrict__ dl,
                                           const float4 *__restrict__ al,
                                           const int2 *__restrict__ du,
@@ -288,7 +295,7 @@ __device__ __forceinline__ void tk_sweeps(float *V, const int2 *sb, int ns, int 
     for (int j = 0; j < TPD; ++j) {
       const int nx = base + j + TPD;
       const int2 nn = sb[nx + 1 < ns ? nx + 1 : 0];
-      tk_step<NR, NRP, 0, PP>(S[j], V);
+      tk_step<NR, NRP, 0, PP>(S[j], V, D);
       __syncthreads();
       tk_load<PP, NT>(S[j], nxt, nx < ns, lo, dl, al, dv);
       nxt = nn;
@@ -296,16 +303,16 @@ __device__ __forceinline__ void tk_sweeps(float *V, const int2 *sb, int ns, int 
   }
 #pragma unroll
   for (int j = 0; j < TPD; ++j)
-    tk_load<PP, NT>(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv);
+    tk_load<PP, NT, false>(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv);
   nxt = sb[TPD < ns ? ns - 1 - TPD : 0];
   for (int base = 0; base < ns; base += TPD) {
 #pragma unroll
     for (int j = 0; j < TPD; ++j) {
       const int nx = base + j + TPD;
       const int2 nn = sb[nx + 1 < ns ? ns - 2 - nx : 0];
-      tk_step<NR, NRP, 1, PP>(S[j], V);
+      tk_step<NR, NRP, 1, PP>(S[j], V, D);
       __syncthreads();
-      tk_load<PP, NT>(S[j], nxt, nx < ns, lo, du, au, dv);
+      tk_load<PP, NT, false>(S[j], nxt, nx < ns, lo, du, au, dv);
       nxt = nn;
     }
   }
@@ -322,7 +329,8 @@ constexpr int TPDW = FCP_TPDW;  // steps of operand prefetch of a one-wave tile 
 // moving LDS accesses across a step.  Steps are short, so operands are
 // prefetched TPDW steps ahead.
 template <int NR, int NRP>
-__device__ __forceinline__ void tk_sweeps_w(float *V, const int2 *sb, int ns, int lo, int lane,
+__device__ __forceinline__ void tk_sweeps_w(float *V, float *D, const int2 *sb, int ns, int lo,
+                                            int lane,
                                             const int2 *__restrict__ dl,
                                             const float4 *__restrict__ al,
                                             const int2 *__restrict__ du,
@@ -338,7 +346,7 @@ __device__ __forceinline__ void tk_sweeps_w(float *V, const int2 *sb, int ns, in
     for (int j = 0; j < TPDW; ++j) {
       const int nx = base + j + TPDW;
       const int2 nn = sb[nx + 1 < ns ? nx + 1 : 0];
-      tk_step<NR, NRP, 0, 1>(S[j], V);
+      tk_step<NR, NRP, 0, 1>(S[j], V, D);
       __builtin_amdgcn_wave_barrier();
       tk_load<1, 64>(S[j], nxt, nx < ns, lo, dl, al, dv, lane);
       nxt = nn;
@@ -346,16 +354,16 @@ __device__ __forceinline__ void tk_sweeps_w(float *V, const int2 *sb, int ns, in
   }
 #pragma unroll
   for (int j = 0; j < TPDW; ++j)
-    tk_load<1, 64>(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv, lane);
+    tk_load<1, 64, false>(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv, lane);
   nxt = sb[TPDW < ns ? ns - 1 - TPDW : 0];
   for (int base = 0; base < ns; base += TPDW) {
 #pragma unroll
     for (int j = 0; j < TPDW; ++j) {
       const int nx = base + j + TPDW;
       const int2 nn = sb[nx + 1 < ns ? ns - 2 - nx : 0];
-      tk_step<NR, NRP, 1, 1>(S[j], V);
+      tk_step<NR, NRP, 1, 1>(S[j], V, D);
       __builtin_amdgcn_wave_barrier();
-      tk_load<1, 64>(S[j], nxt, nx < ns, lo, du, au, dv, lane);
+      tk_load<1, 64, false>(S[j], nxt, nx < ns, lo, du, au, dv, lane);
       nxt = nn;
     }
   }
@@ -378,8 +386,10 @@ __global__ __launch_bounds__(NT) void tk_pre(
     float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
     const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
     int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs) {
-  extern __shared__ float V[];  // [vslots] values (slot dslot = 0), then the step bounds
-  int2 *sb = reinterpret_cast<int2 *>(V + vslots);
+  // [vslots] values (slot dslot = 0), the tile's pivots, then the step bounds
+  extern __shared__ float V[];
+  float *D = V + vslots;
+  int2 *sb = reinterpret_cast<int2 *>(D + ((dslot + 1) & ~1));
   if (tk_all_done<NR>(s)) return;
   bool act[NR];
   float alf[NR];
@@ -436,7 +446,7 @@ __global__ __launch_bounds__(NT) void tk_pre(
     }
     if (threadIdx.x < NRP) V[dslot * NRP + threadIdx.x] = 0.0f;  // the unused-dependency slot
     __syncthreads();
-    if (ns > 0) tk_sweeps<NR, NRP, NT>(V, sb, ns, lo, dl, al, du, au, dv);
+    if (ns > 0) tk_sweeps<NR, NRP, NT>(V, D, sb, ns, lo, dl, al, du, au, dv);
     // z out; r.z with the residual this thread wrote (or read) above
     for (int i = lo + (int)threadIdx.x; i < hi; i += NT) {
 #pragma unroll
@@ -495,11 +505,12 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
     const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
     int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs) {
   // one region of vstride floats per wave: [vslots] values (slot dslot = 0),
-  // then the step bounds
+  // the tile's pivots, then the step bounds
   extern __shared__ float VW[];
   const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
   float *V = VW + (size_t)wv * vstride;
-  int2 *sb = reinterpret_cast<int2 *>(V + vslots);
+  float *D = V + vslots;
+  int2 *sb = reinterpret_cast<int2 *>(D + ((dslot + 1) & ~1));
   if (tk_all_done<NR>(s)) return;
   bool act[NR];
   float alf[NR];
@@ -556,7 +567,7 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
     }
     if (lane < NRP) V[dslot * NRP + lane] = 0.0f;  // the unused-dependency slot
     __builtin_amdgcn_wave_barrier();
-    if (ns > 0) tk_sweeps_w<NR, NRP>(V, sb, ns, lo, lane, dl, al, du, au, dv);
+    if (ns > 0) tk_sweeps_w<NR, NRP>(V, D, sb, ns, lo, lane, dl, al, du, au, dv);
     __builtin_amdgcn_wave_barrier();
     // z out; r.z with the residual this thread wrote (or read) above
     for (int i = lo + lane; i < hi; i += 64) {
@@ -784,8 +795,10 @@ __global__ __launch_bounds__(BLK) void tb_tile(
     const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
     float *__restrict__ x, float *__restrict__ r, float *__restrict__ p,
     const float *__restrict__ u, float *z, const SolveScalars *__restrict__ s) {
-  extern __shared__ float V[];  // [vslots] values (slot dslot = 0), then the step bounds
-  int2 *sb = reinterpret_cast<int2 *>(V + vslots);
+  // [vslots] values (slot dslot = 0), the tile's pivots, then the step bounds
+  extern __shared__ float V[];
+  float *D = V + vslots;
+  int2 *sb = reinterpret_cast<int2 *>(D + ((dslot + 1) & ~1));
   if (tk_all_done<NR>(s)) return;
   bool act[NR];
   float c1[NR], c2[NR];
@@ -842,7 +855,7 @@ __global__ __launch_bounds__(BLK) void tb_tile(
     }
     if (threadIdx.x < NRP) V[dslot * NRP + threadIdx.x] = 0.0f;  // the unused-dependency slot
     __syncthreads();
-    if (ns > 0) tk_sweeps<NR, NRP>(V, sb, ns, lo, dl, al, du, au, dv);
+    if (ns > 0) tk_sweeps<NR, NRP>(V, D, sb, ns, lo, dl, al, du, au, dv);
     for (int i = lo + (int)threadIdx.x; i < hi; i += BLK) {
 #pragma unroll
       for (int c = 0; c < NR; ++c) z[(size_t)i * NRP + c] = V[(i - lo) * NRP + c];
@@ -1049,14 +1062,14 @@ __global__ void tk_scatter_out(int n, const int *__restrict__ perm, const float 
 // LDS of one wave's tile region, in floats (values, then the step bounds).
 int tk_wave_stride(const fcp_ctx *ctx, int nrp) {
   const BlockSchedule &b = ctx->jlower;
-  return (((b.R + 1) * nrp + 1) & ~1) + 2 * (b.max_steps + 1);
+  return (((b.R + 1) * nrp + 1) & ~1) + ((b.R + 1) & ~1) + 2 * (b.max_steps + 1);
 }
 
 size_t tk_lds(const fcp_ctx *ctx, int nrp) {
   const BlockSchedule &b = ctx->jlower;
   if (ctx->bj_wave) return (size_t)(BLK / 64) * tk_wave_stride(ctx, nrp) * sizeof(float);
-  // values (R slots and the zero slot), then 8-byte aligned step bounds
-  return (size_t)(((b.R + 1) * nrp + 1) & ~1) * sizeof(float) +
+  // values (R slots and the zero slot), pivots, then 8-byte aligned step bounds
+  return (size_t)((((b.R + 1) * nrp + 1) & ~1) + ((b.R + 1) & ~1)) * sizeof(float) +
          (size_t)(b.max_steps + 1) * sizeof(int2);
 }
 
