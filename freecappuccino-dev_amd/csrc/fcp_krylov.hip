@@ -531,9 +531,13 @@ __global__ void k_sell_val(const int *__restrict__ ia, const double *__restrict_
 // SpMV
 // ---------------------------------------------------------------------------
 // y = A x, one thread per row, CSR order (bit-equal to the reference loop).
+// The grid is a multiple of 8 workgroups; workgroups run on the eight XCDs
+// round-robin (b on XCD b % 8), and XCD x takes the x-th eighth of the row
+// blocks, so the x lines that neighbouring rows gather stay in one L2.
 template <typename T>
 __global__ void k_spmv_row(int n, Sell<T> m, const T *__restrict__ x, T *__restrict__ y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = (int)blockIdx.x, per = (int)gridDim.x >> 3;
+  const int i = ((b & 7) * per + (b >> 3)) * (int)blockDim.x + (int)threadIdx.x;
   if (i >= n) return;
   y[i] = sell_row<T>(m, i, x, (T)0, false);
 }
@@ -1014,7 +1018,7 @@ int fcp_launch_spmv64(fcp_ctx *ctx, const double *, const double *x, double *y) 
   Sell<double> m;
   FCP_TRY(sell_view<double>(ctx, &m));
   ProfScope ps(ctx, "spmv_sell_f64");
-  k_spmv_row<double><<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(ctx->n, m, x, y);
+  k_spmv_row<double><<<(grid_for(ctx->n, BLK) + 7) / 8 * 8, BLK, 0, ctx->stream>>>(ctx->n, m, x, y);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }
