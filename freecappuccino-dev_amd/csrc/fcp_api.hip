@@ -277,7 +277,12 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
         ctx->tune_inner_tile = value;
         ctx->have_bj = false;
         ctx->tin_have = false;
+        ctx->tc_have = false;
       }
+      return 0;
+    case FCP_TUNE_INNER_COARSE:
+      if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "coarse correction is 0 or 1");
+      ctx->inner_coarse = value;
       return 0;
     case FCP_TUNE_INNER_PRECOND:
       if (value < 0 || value > 2)
@@ -786,7 +791,7 @@ int fcp_spmv(fcp_ctx *ctx, const double *x, double *y) {
 
 int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d) {
   FCP_TRY(need(ctx, true, false, true));
-  if (!r || !z || kind < 0 || kind > 6)
+  if (!r || !z || kind < 0 || kind > 7)
     return fcp_fail(ctx, FCP_ERR_ARG, "fcp_precond_apply: bad argument");
   if (kind == 2) d = nullptr;  // ILU(0) has no pivot vector to return
   (void)hipSetDevice(ctx->device);
@@ -937,6 +942,7 @@ void set_inner_tile(fcp_ctx *ctx, int t) {
   ctx->tune_inner_tile = t;
   ctx->have_bj = false;
   ctx->tin_have = false;
+  ctx->tc_have = false;
 }
 }  // namespace
 

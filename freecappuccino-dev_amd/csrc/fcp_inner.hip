@@ -22,6 +22,7 @@
 // NR right-hand sides are interleaved (NRP values per position); every
 // column's arithmetic is independent of NR, so a batched solve's columns
 // equal single-column solves bit for bit.
+#include <algorithm>
 #include <type_traits>
 
 #include "fcp_internal.h"
@@ -385,7 +386,8 @@ __global__ __launch_bounds__(NT) void tk_pre(
     const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
     float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
     const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
-    int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs) {
+    int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs,
+    double *__restrict__ r0, int ncp) {
   // [vslots] values (slot dslot = 0), the tile's pivots, then the step bounds
   extern __shared__ float V[];
   float *D = V + vslots;
@@ -406,6 +408,9 @@ __global__ __launch_bounds__(NT) void tk_pre(
     const int st0 = blk_step[b], ns = blk_step[b + 1] - st0;
     for (int k = threadIdx.x; k < ns; k += NT)
       sb[k] = make_int2(step_ptr[st0 + k], step_ptr[st0 + k + 1]);
+    double rs[NR];  // the tile's residual sum (coarse correction)
+#pragma unroll
+    for (int c = 0; c < NR; ++c) rs[c] = 0.0;
     for (int i0 = lo + (int)threadIdx.x; i0 < hi; i0 += GS_U * NT) {
       float xv[GS_U][NR], pv[GS_U][NR], rv[GS_U][NR], qv[GS_U][NR];
 #pragma unroll
@@ -440,11 +445,19 @@ __global__ __launch_bounds__(NT) void tk_pre(
               part[3 * c + 1] += (double)ri * (double)ri;
             }
             V[(i - lo) * NRP + c] = ri;
+            rs[c] += (double)ri;
           }
         }
       }
     }
     if (threadIdx.x < NRP) V[dslot * NRP + threadIdx.x] = 0.0f;  // the unused-dependency slot
+    if (r0) {  // uniform
+      double ts[NR];
+      block_sum<NR, NT>(rs, ts);
+      if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) r0[(size_t)c * ncp + b] = ts[c];
+    }
     __syncthreads();
     if (ns > 0) tk_sweeps<NR, NRP, NT>(V, D, sb, ns, lo, dl, al, du, au, dv);
     // z out; r.z with the residual this thread wrote (or read) above
@@ -503,7 +516,8 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
     const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
     float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
     const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
-    int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs) {
+    int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs,
+    double *__restrict__ r0, int ncp) {
   // one region of vstride floats per wave: [vslots] values (slot dslot = 0),
   // the tile's pivots, then the step bounds
   extern __shared__ float VW[];
@@ -527,6 +541,9 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
     const int st0 = blk_step[b], ns = blk_step[b + 1] - st0;
     for (int k = lane; k < ns; k += 64)
       sb[k] = make_int2(step_ptr[st0 + k], step_ptr[st0 + k + 1]);
+    double rs[NR];  // the tile's residual sum (coarse correction)
+#pragma unroll
+    for (int c = 0; c < NR; ++c) rs[c] = 0.0;
     for (int i0 = lo + lane; i0 < hi; i0 += GS_U * 64) {
       float xv[GS_U][NR], pv[GS_U][NR], rv[GS_U][NR], qv[GS_U][NR];
 #pragma unroll
@@ -561,11 +578,19 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
               part[3 * c + 1] += (double)ri * (double)ri;
             }
             V[(i - lo) * NRP + c] = ri;
+            rs[c] += (double)ri;
           }
         }
       }
     }
     if (lane < NRP) V[dslot * NRP + lane] = 0.0f;  // the unused-dependency slot
+    if (r0) {  // uniform
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        const double ts = wave_sum(rs[c]);
+        if (lane == 0) r0[(size_t)c * ncp + b] = ts;
+      }
+    }
     __builtin_amdgcn_wave_barrier();
     if (ns > 0) tk_sweeps_w<NR, NRP>(V, D, sb, ns, lo, lane, dl, al, du, au, dv);
     __builtin_amdgcn_wave_barrier();
@@ -1047,7 +1072,387 @@ __global__ __launch_bounds__(BLK) void tb_bet(int n, float *__restrict__ x, floa
   }
 }
 
-// z = P^T M^-1 P r in FP32 for one column (the inner solves' application).
+// ---------------------------------------------------------------------------
+// Coarse correction of the inner PCG: z += P0 A0^-1 P0^T r with P0 the
+// indicator of the tiles and A0 = P0^T A P0 (one unknown per tile, the
+// additive two-level preconditioner).  A0 is factored A0 = L L^T in 64 x 64
+// blocks (diagonal block factor and inverse in LDS, panel, trailing update),
+// W = L^-1 is formed and A0^-1 = W^T W, so an iteration's coarse solve is one
+// dense product e0 = A0^-1 r0 (r0 the tile sums of r, formed by the
+// preconditioner kernel).  Every sum is taken in a fixed order
+// (deterministic results).
+// ---------------------------------------------------------------------------
+constexpr int TCB = 64;         // block size of the coarse factorisation
+constexpr int TC_MAXN = 1024;   // most tiles with a coarse correction
+constexpr int TC_MAXDEG = 32;   // widest tile adjacency (the tile itself included)
+
+__global__ void tc_tile_of_pos(int n, int nb, const int *__restrict__ bpos,
+                               int *__restrict__ tpos) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) tpos[p] = tk_block_of(bpos, nb, p);
+}
+
+// mark[T * nc + U] = 1 where a row of tile T has an entry in tile U
+__global__ void tc_mark(int n, int nc, const int *__restrict__ perm, const int *__restrict__ iperm,
+                        const int *__restrict__ tpos, const int *__restrict__ ia,
+                        const int *__restrict__ ja, unsigned char *__restrict__ mark) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int T = tpos[p], row = perm[p];
+  for (int e = ia[row]; e < ia[row + 1]; ++e) mark[(size_t)T * nc + tpos[iperm[ja[e]]]] = 1;
+}
+
+// A0[T][U] = sum of a_ij over rows i of tile T and columns j of tile U; one
+// workgroup per tile, each thread sums its rows in order into its own LDS
+// column per adjacent tile, the columns are then added in thread order.
+// Dynamic LDS: maxdeg x BLK doubles, then nc ints.
+__global__ __launch_bounds__(BLK) void tc_vals(int nc, int ncp, int maxdeg,
+                                               const int *__restrict__ bpos,
+                                               const int *__restrict__ perm,
+                                               const int *__restrict__ iperm,
+                                               const int *__restrict__ tpos,
+                                               const int *__restrict__ ia,
+                                               const int *__restrict__ ja,
+                                               const double *__restrict__ a,
+                                               const int *__restrict__ cptr,
+                                               const int *__restrict__ cadj,
+                                               double *__restrict__ A) {
+  extern __shared__ double SV[];
+  int *slot = reinterpret_cast<int *>(SV + (size_t)maxdeg * BLK);
+  const int T = blockIdx.x, t = threadIdx.x;
+  const int c0 = cptr[T], deg = cptr[T + 1] - c0;
+  for (int u = t; u < nc; u += BLK) slot[u] = 0;
+  __syncthreads();
+  for (int k = t; k < deg; k += BLK) slot[cadj[c0 + k]] = k;
+  for (int k = 0; k < deg; ++k) SV[(size_t)k * BLK + t] = 0.0;
+  __syncthreads();
+  for (int i = bpos[T] + t; i < bpos[T + 1]; i += BLK) {
+    const int row = perm[i];
+    for (int e = ia[row]; e < ia[row + 1]; ++e)
+      SV[(size_t)slot[tpos[iperm[ja[e]]]] * BLK + t] += a[e];
+  }
+  __syncthreads();
+  for (int k = t; k < deg; k += BLK) {
+    double sum = 0.0;
+    for (int u = 0; u < BLK; ++u) sum += SV[(size_t)k * BLK + u];
+    A[(size_t)T * ncp + cadj[c0 + k]] = sum;
+  }
+}
+
+__global__ void tc_pad(int nc, int ncp, double *__restrict__ A) {
+  const int i = nc + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ncp) A[(size_t)i * ncp + i] = 1.0;  // padding unknowns: identity
+}
+
+typedef double TcTile[TCB][TCB + 1];
+
+__device__ __forceinline__ void tc_load(TcTile &B, const double *__restrict__ M, int ncp, int bi,
+                                        int bj) {
+  for (int e = threadIdx.x; e < TCB * TCB; e += blockDim.x) {
+    const int r = e / TCB, c = e % TCB;
+    B[r][c] = M[(size_t)(bi * TCB + r) * ncp + bj * TCB + c];
+  }
+}
+
+// Operand of a block product, stored k-major: S[t][r] = block[r][t] (trans)
+// or block[t][r].
+__device__ __forceinline__ void tc_op(TcTile &S, const double *__restrict__ M, int ncp, int bi,
+                                      int bj, bool trans) {
+  for (int e = threadIdx.x; e < TCB * TCB; e += 256) {
+    const int r = e / TCB, c = e % TCB;
+    const double v = M[(size_t)(bi * TCB + r) * ncp + bj * TCB + c];
+    if (trans)
+      S[c][r] = v;
+    else
+      S[r][c] = v;
+  }
+}
+
+// acc[i][j] += sum_t A[t][ty + 16 i] B[t][tx + 16 j] (256 threads, thread
+// (ty, tx) = (tid / 16, tid % 16) holding a 4 x 4 output set; t ascending).
+__device__ __forceinline__ void tc_mm(const TcTile &A, const TcTile &B, double (&acc)[4][4]) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  for (int t = 0; t < TCB; ++t) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) av[i] = A[t][ty + 16 * i];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bv[j] = B[t][tx + 16 * j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+  }
+}
+
+__device__ __forceinline__ void tc_zero(double (&acc)[4][4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+}
+
+// block (bi, bj) of M = sign * acc
+__device__ __forceinline__ void tc_store(double *__restrict__ M, int ncp, int bi, int bj,
+                                         const double (&acc)[4][4], double sign) {
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      M[(size_t)(bi * TCB + ty + 16 * i) * ncp + bj * TCB + tx + 16 * j] = sign * acc[i][j];
+}
+
+// Diagonal block k: Cholesky, written back with its upper triangle zeroed,
+// and the inverse of the factor written to W's block (k, k).  The factor is
+// right-looking with the block in registers (thread (ty, tx) holds rows
+// ty + 16 i, columns tx + 16 j), column j scaled by its owners and broadcast
+// through LDS; the inverse is formed row by row in LDS, X[r][c] =
+// -(sum over c <= m < r of L[r][m] X[m][c]) / L[r][r], four threads per
+// column.
+__global__ __launch_bounds__(256) void tc_potrf(int ncp, int k, double *__restrict__ A,
+                                                double *__restrict__ W, int *__restrict__ err) {
+  __shared__ TcTile B;
+  __shared__ TcTile X;
+  __shared__ double col[TCB], piv[TCB];
+  __shared__ int bad;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double v[4][4], d0 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      v[i][q] = A[(size_t)(k * TCB + ty + 16 * i) * ncp + k * TCB + tx + 16 * q];
+  if (tx == ty) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) piv[ty + 16 * i] = v[i][i];  // original diagonal
+  }
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  for (int j = 0; j < TCB; ++j) {
+    const int jo = j & 15, jq = j >> 4;
+    if (tx == jo && ty == jo) {  // owner of (j, j)
+      double d = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q == jq) d = v[q][q];
+      d0 = piv[j];
+      // a pivot lost to cancellation marks a singular coarse operator
+      if (!(d > 1e-10 * d0)) bad = 1;
+      piv[j] = sqrt(d);
+    }
+    __syncthreads();
+    if (bad) {  // uniform
+      if (threadIdx.x == 0) atomicOr(err, 2);
+      return;
+    }
+    const double l = piv[j];
+    if (tx == jo) {  // owners of column j
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q == jq) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int r = ty + 16 * i;
+            if (r > j) {
+              v[i][q] = v[i][q] / l;
+              col[r] = v[i][q];
+            } else if (r == j) {
+              v[i][q] = l;
+            }
+          }
+        }
+    }
+    __syncthreads();
+    double cr[4], cc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) cr[i] = col[ty + 16 * i];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cc[q] = col[tx + 16 * q];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = ty + 16 * i, c = tx + 16 * q;
+        if (c > j && c <= r) v[i][q] = v[i][q] - cr[i] * cc[q];
+      }
+    // col is next written after the next column's first barrier
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = ty + 16 * i, c = tx + 16 * q;
+      B[r][c] = c <= r ? v[i][q] : 0.0;
+      X[r][c] = 0.0;
+    }
+  __syncthreads();
+  const int c = threadIdx.x >> 2, part = threadIdx.x & 3;
+  for (int r = 0; r < TCB; ++r) {
+    double sum = 0.0;
+    for (int m = c + part; m < r; m += 4) sum += B[r][m] * X[m][c];
+    sum += __shfl_xor(sum, 1);
+    sum += __shfl_xor(sum, 2);
+    if (part == 0) {
+      if (c < r) X[r][c] = -sum / B[r][r];
+      if (c == r) X[r][c] = 1.0 / B[r][r];
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < TCB * TCB; e += blockDim.x) {
+    const int r = e / TCB, cc = e % TCB;
+    const size_t o = (size_t)(k * TCB + r) * ncp + k * TCB + cc;
+    A[o] = B[r][cc];
+    W[o] = X[r][cc];
+  }
+}
+
+// Panel below diagonal block k: L_ik = A_ik L_kk^-T = A_ik W_kk^T (block row
+// i = k+1+blockIdx.x).
+__global__ __launch_bounds__(256) void tc_panel(int ncp, int k, double *__restrict__ A,
+                                                const double *__restrict__ W) {
+  __shared__ TcTile P;
+  __shared__ TcTile Q;
+  const int i = k + 1 + blockIdx.x;
+  tc_op(P, A, ncp, i, k, true);
+  tc_op(Q, W, ncp, k, k, true);
+  __syncthreads();
+  double acc[4][4];
+  tc_zero(acc);
+  tc_mm(P, Q, acc);
+  tc_store(A, ncp, i, k, acc, 1.0);
+}
+
+// Trailing update after block column k: A_ij -= L_ik L_jk^T for k < j <= i
+// (blockIdx.x enumerates the pairs row by row).
+__global__ __launch_bounds__(256) void tc_update(int ncp, int k, double *__restrict__ A) {
+  __shared__ TcTile P;
+  __shared__ TcTile Q;
+  int e0 = blockIdx.x, ii = 0;
+  while (e0 > ii) {
+    e0 -= ii + 1;
+    ++ii;
+  }
+  const int i = k + 1 + ii, j = k + 1 + e0;
+  tc_op(P, A, ncp, i, k, true);
+  tc_op(Q, A, ncp, j, k, true);
+  __syncthreads();
+  double acc[4][4];
+  tc_zero(acc);
+  tc_mm(P, Q, acc);
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const size_t o = (size_t)(i * TCB + ty + 16 * a) * ncp + j * TCB + tx + 16 * b;
+      A[o] = A[o] - acc[a][b];
+    }
+}
+
+// Block row i of W = L^-1: W_ij = -W_ii sum_{m=j..i-1} L_im W_mj (j = blockIdx.x).
+__global__ __launch_bounds__(256) void tc_inv(int ncp, int i, const double *__restrict__ A,
+                                              double *__restrict__ W) {
+  __shared__ TcTile P;
+  __shared__ TcTile Q;
+  const int j = blockIdx.x;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double acc[4][4];
+  tc_zero(acc);
+  for (int m = j; m < i; ++m) {
+    tc_op(P, A, ncp, i, m, true);
+    tc_op(Q, W, ncp, m, j, false);
+    __syncthreads();
+    tc_mm(P, Q, acc);
+    __syncthreads();
+  }
+  // S (acc) to LDS k-major, then -W_ii S
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) Q[ty + 16 * a][tx + 16 * b] = acc[a][b];
+  tc_op(P, W, ncp, i, i, true);
+  __syncthreads();
+  tc_zero(acc);
+  tc_mm(P, Q, acc);
+  tc_store(W, ncp, i, j, acc, -1.0);
+}
+
+// A0^-1 = W^T W (block (bi, bj) = sum over block rows m >= max(bi, bj) of
+// W_m,bi^T W_m,bj), written over the factor.
+__global__ __launch_bounds__(256) void tc_gram(int ncp, const double *__restrict__ W,
+                                               double *__restrict__ G) {
+  __shared__ TcTile P;
+  __shared__ TcTile Q;
+  const int bi = blockIdx.x, bj = blockIdx.y, nbk = ncp / TCB;
+  double acc[4][4];
+  tc_zero(acc);
+  for (int m = bi > bj ? bi : bj; m < nbk; ++m) {
+    tc_op(P, W, ncp, m, bi, false);
+    tc_op(Q, W, ncp, m, bj, false);
+    __syncthreads();
+    tc_mm(P, Q, acc);
+    __syncthreads();
+  }
+  tc_store(G, ncp, bi, bj, acc, 1.0);
+}
+
+// z += e0[T] on the positions of tile T, e0 = A0^-1 r0 the coarse correction
+// (r0 the tile sums of r, ncp per column), then sk += e0 . r0 (the coarse
+// part of r.z) and bet = sk / s0.  Workgroup w updates part w % parts of
+// tile w / parts; every workgroup of a tile forms its entry of e0 with the
+// same summation order, wave c for column c.
+template <int NR, int NRP>
+__global__ __launch_bounds__(BLK) void tc_zc(int parts, int ncp, const int *__restrict__ bpos,
+                                             const int *__restrict__ failed,
+                                             const double *__restrict__ G,
+                                             const double *__restrict__ r0,
+                                             float *__restrict__ z, double *__restrict__ partials,
+                                             int *__restrict__ counter,
+                                             SolveScalars *__restrict__ s) {
+  static_assert(NR <= BLK / 64, "one wave per column");
+  __shared__ double E[NR];
+  if (tk_all_done<NR>(s)) return;
+  bool act[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) act[c] = !solve_done(s + c);
+  const int T = blockIdx.x / parts, part = blockIdx.x % parts;
+  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+  if (wv < NR) {
+    const double *g = G + (size_t)T * ncp, *rc = r0 + (size_t)wv * ncp;
+    double sum = 0.0;
+    for (int m = lane; m < ncp; m += 64) sum += g[m] * rc[m];
+    sum = wave_sum(sum);
+    if (lane == 0) E[wv] = *failed ? 0.0 : sum;  // no factor: tile-local only
+  }
+  __syncthreads();
+  float e[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) e[c] = (float)E[c];
+  const int lo = bpos[T], len = bpos[T + 1] - lo;
+  const int chunk = (len + parts - 1) / parts;
+  const int i0 = lo + part * chunk, i1 = min(lo + len, i0 + chunk);
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += BLK) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (!act[c]) continue;
+      const size_t o = (size_t)i * NRP + c;
+      z[o] = z[o] + e[c];
+    }
+  }
+  double pt[NR], tot[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) pt[c] = part == 0 ? E[c] * r0[(size_t)c * ncp + T] : 0.0;
+  if (last_block_reduce<NR>(pt, partials, 0, counter, tot) && threadIdx.x == 0) {
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      if (!act[c]) continue;
+      s[c].sk = s[c].sk + tot[c];
+      s[c].bet = s[c].sk / s[c].s0;
+    }
+  }
+}
+
 __global__ void tk_gather_in(int n, const int *__restrict__ perm, const double *__restrict__ r,
                              float *__restrict__ v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1145,7 +1550,110 @@ int tk_prepare(fcp_ctx *ctx) {
 
 struct TkBufs {
   float *x, *r, *z, *q, *p[2];
+  double *r0 = nullptr;  // tile sums of r (null: no coarse correction)
+  int ncp = 0;
 };
+
+// Coarse operator of the current tile schedule and matrix: the tile
+// adjacency once per schedule, A0, its factor and inverse once per matrix.
+// Leaves tc_nc = 0 (no coarse correction) for fewer than two or more than
+// TC_MAXN tiles or a tile adjacency wider than TC_MAXDEG.  A coarse operator
+// that is not positive definite sets tc_flag on the device (no host wait);
+// the iterations then add no correction.
+int tc_prepare(fcp_ctx *ctx) {
+  const BlockSchedule &b = ctx->jlower;
+  const int n = ctx->n, nc = b.nblocks;
+  if (!ctx->tc_have) {
+    ctx->tc_nc = 0;
+    ctx->tc_have = true;
+    if (nc < 2 || nc > TC_MAXN) return 0;
+    FCP_HIP(ctx, ctx->tc_tpos.alloc(n));
+    tc_tile_of_pos<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, nc, b.blk_pos.p,
+                                                             ctx->tc_tpos.p);
+    DBuf<unsigned char> mark;
+    FCP_HIP(ctx, mark.alloc((size_t)nc * nc));
+    FCP_HIP(ctx, hipMemsetAsync(mark.p, 0, (size_t)nc * nc, ctx->stream));
+    tc_mark<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, nc, b.rows.p, ctx->tin_iperm.p,
+                                                      ctx->tc_tpos.p, ctx->ia.p, ctx->ja.p,
+                                                      mark.p);
+    FCP_HIP(ctx, hipGetLastError());
+    std::vector<unsigned char> hm((size_t)nc * nc);
+    FCP_HIP(ctx, hipMemcpyAsync(hm.data(), mark.p, hm.size(), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    std::vector<int> cptr(nc + 1, 0), cadj;
+    int maxdeg = 0;
+    for (int T = 0; T < nc; ++T) {
+      for (int U = 0; U < nc; ++U)
+        if (hm[(size_t)T * nc + U]) cadj.push_back(U);
+      cptr[T + 1] = (int)cadj.size();
+      maxdeg = std::max(maxdeg, cptr[T + 1] - cptr[T]);
+    }
+    if (maxdeg > TC_MAXDEG) return 0;
+    FCP_HIP(ctx, ctx->tc_cptr.alloc(nc + 1));
+    FCP_HIP(ctx, ctx->tc_cadj.alloc(cadj.size()));
+    FCP_HIP(ctx, hipMemcpy(ctx->tc_cptr.p, cptr.data(), sizeof(int) * (nc + 1),
+                           hipMemcpyHostToDevice));
+    FCP_HIP(ctx, hipMemcpy(ctx->tc_cadj.p, cadj.data(), sizeof(int) * cadj.size(),
+                           hipMemcpyHostToDevice));
+    ctx->tc_nc = nc;
+    ctx->tc_ncp = (nc + TCB - 1) / TCB * TCB;
+    ctx->tc_maxdeg = maxdeg;
+    ctx->tc_epoch = ~0ull;
+  }
+  if (ctx->tc_nc == 0 || ctx->tc_epoch == ctx->a_version) return 0;
+  ProfScope ps(ctx, "coarse_setup");
+  const int ncp = ctx->tc_ncp, nbk = ncp / TCB;
+  const size_t nn = (size_t)ncp * ncp;
+  FCP_HIP(ctx, ctx->tc_L.alloc(nn));
+  FCP_HIP(ctx, ctx->tc_W.alloc(nn));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_L.p, 0, nn * sizeof(double), ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_W.p, 0, nn * sizeof(double), ctx->stream));
+  FCP_HIP(ctx, ctx->tc_flag.alloc(1));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_flag.p, 0, sizeof(int), ctx->stream));
+  const size_t lds = (size_t)ctx->tc_maxdeg * BLK * sizeof(double) + (size_t)nc * sizeof(int);
+  {
+    ProfScope ps1(ctx, "coarse_vals");
+    tc_vals<<<nc, BLK, lds, ctx->stream>>>(nc, ncp, ctx->tc_maxdeg, b.blk_pos.p, b.rows.p,
+                                           ctx->tin_iperm.p, ctx->tc_tpos.p, ctx->ia.p,
+                                           ctx->ja.p, ctx->a.p, ctx->tc_cptr.p, ctx->tc_cadj.p,
+                                           ctx->tc_L.p);
+    if (ncp > nc) tc_pad<<<1, TCB, 0, ctx->stream>>>(nc, ncp, ctx->tc_L.p);
+  }
+  {
+    ProfScope ps1(ctx, "coarse_chol");
+    for (int k = 0; k < nbk; ++k) {
+      tc_potrf<<<1, 256, 0, ctx->stream>>>(ncp, k, ctx->tc_L.p, ctx->tc_W.p, ctx->tc_flag.p);
+      const int m = nbk - k - 1;
+      if (m == 0) break;
+      tc_panel<<<m, 256, 0, ctx->stream>>>(ncp, k, ctx->tc_L.p, ctx->tc_W.p);
+      tc_update<<<m * (m + 1) / 2, 256, 0, ctx->stream>>>(ncp, k, ctx->tc_L.p);
+    }
+  }
+  {
+    ProfScope ps1(ctx, "coarse_inv");
+    for (int i = 1; i < nbk; ++i)
+      tc_inv<<<i, 256, 0, ctx->stream>>>(ncp, i, ctx->tc_L.p, ctx->tc_W.p);
+    tc_gram<<<dim3(nbk, nbk), 256, 0, ctx->stream>>>(ncp, ctx->tc_W.p, ctx->tc_L.p);
+  }
+  FCP_HIP(ctx, hipGetLastError());
+  ctx->tc_epoch = ctx->a_version;
+  return 0;
+}
+
+// Coarse buffers of an inner solve of nr columns (r0 cleared: entries past
+// the last tile stay zero), or none where the correction does not apply.
+int tc_bufs(fcp_ctx *ctx, int nr, TkBufs &B) {
+  if (!ctx->inner_coarse) return 0;
+  FCP_TRY(tc_prepare(ctx));
+  if (ctx->tc_nc == 0) return 0;
+  const size_t m = (size_t)nr * ctx->tc_ncp;
+  FCP_HIP(ctx, ctx->tc_r0.alloc(m));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_r0.p, 0, m * sizeof(double), ctx->stream));
+  B.r0 = ctx->tc_r0.p;
+  B.ncp = ctx->tc_ncp;
+  return 0;
+}
 
 template <int NR, int NRP>
 int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *si, int upd) {
@@ -1160,7 +1668,7 @@ int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *s
         reinterpret_cast<const float4 *>(ctx->tin_al.p),
         reinterpret_cast<const int2 *>(ctx->tin_du.p),
         reinterpret_cast<const float4 *>(ctx->tin_au.p), ctx->tin_dv.p, B.x, B.r, p, B.q, B.z,
-        ctx->partials.p, redctr(ctx), si, upd, 0.0);
+        ctx->partials.p, redctr(ctx), si, upd, 0.0, B.r0, B.ncp);
     return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
   }
   tk_pre<NR, NRP, TK_NT><<<tk_grid(ctx), TK_NT, tk_lds(ctx, NRP), ctx->stream>>>(
@@ -1169,14 +1677,32 @@ int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *s
       reinterpret_cast<const float4 *>(ctx->tin_al.p),
       reinterpret_cast<const int2 *>(ctx->tin_du.p),
       reinterpret_cast<const float4 *>(ctx->tin_au.p), ctx->tin_dv.p, B.x, B.r, p, B.q, B.z,
-      ctx->partials.p, redctr(ctx), si, upd, 0.0);
+      ctx->partials.p, redctr(ctx), si, upd, 0.0, B.r0, B.ncp);
   return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
 }
 
 // Inner iteration k: p_new = z + bet p_old, q = A p_new (tk_spmv), then the
 // update, stopping test and next preconditioner application (tk_pre).
+// With the coarse correction, tc_zc first adds it to z and completes r.z and
+// bet.
+int tc_parts(const fcp_ctx *ctx) {
+  const int nb = ctx->jlower.nblocks, p = 2048 / nb;
+  return p < 1 ? 1 : (p > 8 ? 8 : p);
+}
+
+template <int NR, int NRP>
+int tc_launch_zc(fcp_ctx *ctx, const TkBufs &B, SolveScalars *si) {
+  ProfScope ps(ctx, "tin_coarse");
+  const int parts = tc_parts(ctx);
+  tc_zc<NR, NRP><<<ctx->jlower.nblocks * parts, BLK, 0, ctx->stream>>>(
+      parts, B.ncp, ctx->jlower.blk_pos.p, ctx->tc_flag.p, ctx->tc_L.p, B.r0, B.z,
+      ctx->partials.p, redctr(ctx), si);
+  return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
+}
+
 template <int NR, int NRP>
 int tk_iteration(fcp_ctx *ctx, const TkBufs &B, SolveScalars *si, int k) {
+  if (B.r0) FCP_TRY((tc_launch_zc<NR, NRP>(ctx, B, si)));
   float *pold = B.p[k & 1], *pnew = B.p[(k + 1) & 1];
   {
     ProfScope ps(ctx, "tin_spmv");
@@ -1197,7 +1723,7 @@ int tk_graph(fcp_ctx *ctx, const TkBufs &B, SolveScalars *si, hipGraphExec_t *ou
       ctx->counters.p, ctx->tin_dl.p, ctx->tin_du.p, ctx->tin_al.p, ctx->tin_au.p, ctx->tin_dv.p,
       ctx->tin_base.p, ctx->tin_col.p, ctx->tin_val.p, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
       (const void *)(size_t)b.nblocks, (const void *)(size_t)b.R, (const void *)(size_t)b.max_steps,
-      (const void *)(size_t)ctx->n};
+      (const void *)(size_t)ctx->n, B.r0, (const void *)(size_t)B.ncp, ctx->tc_L.p};
   if (ctx->tin_graph && ctx->tin_graph_key == key) {
     *out = ctx->tin_graph;
     return 0;
@@ -1230,6 +1756,7 @@ int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, S
   for (int k = 0; k < 6; ++k) FCP_HIP(ctx, ctx->tin_vec[k].alloc(m));
   TkBufs B{ctx->tin_vec[0].p, ctx->tin_vec[1].p, ctx->tin_vec[2].p, ctx->tin_vec[3].p,
            {ctx->tin_vec[4].p, ctx->tin_vec[5].p}};
+  FCP_TRY(tc_bufs(ctx, NR, B));
   const int g = red_grid(ctx, n);
   tk_init<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, res, so, B.r, B.x, B.p[0],
                                                ctx->partials.p, redctr(ctx), si, tol_rel,
@@ -1413,19 +1940,34 @@ int tin_bicgstab(fcp_ctx *ctx, int nr, const double *res, double *x, const Solve
 }
 
 // z = M^-1 r with the tile-local preconditioner in FP32 exactly as the inner
-// solves apply it (one tile-order application, pivots current).
-int tin_apply(fcp_ctx *ctx, const double *r, double *z) {
+// solves apply it (one tile-order application, pivots current); coarse: plus
+// the coarse correction (FCP_ERR_UNSUPPORTED where it does not apply).
+int tin_apply(fcp_ctx *ctx, const double *r, double *z, bool coarse) {
   if (!tin_usable(ctx, 1)) return FCP_ERR_UNSUPPORTED;
   FCP_TRY(tk_prepare(ctx));
   const int n = ctx->n;
   for (int k = 0; k < 6; ++k) FCP_HIP(ctx, ctx->tin_vec[k].alloc(n));
   TkBufs B{ctx->tin_vec[0].p, ctx->tin_vec[1].p, ctx->tin_vec[2].p, ctx->tin_vec[3].p,
            {ctx->tin_vec[4].p, ctx->tin_vec[5].p}};
+  if (coarse) {
+    const int on = ctx->inner_coarse;
+    ctx->inner_coarse = 1;
+    const int rc = tc_bufs(ctx, 1, B);
+    ctx->inner_coarse = on;
+    FCP_TRY(rc);
+    if (!B.r0) return FCP_ERR_UNSUPPORTED;
+    int failed = 0;
+    FCP_HIP(ctx, hipMemcpyAsync(&failed, ctx->tc_flag.p, sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (failed) return FCP_ERR_UNSUPPORTED;
+  }
   // scalars in the spare slot: not done, unit tolerance, no update
   SolveScalars *si = ctx->scal.p + 3;
   FCP_HIP(ctx, hipMemsetAsync(si, 0, sizeof(SolveScalars), ctx->stream));
   tk_gather_in<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, r, B.r);
   FCP_TRY((tk_launch_pre<1, 1>(ctx, B, B.p[0], si, 0)));
+  if (coarse) FCP_TRY((tc_launch_zc<1, 1>(ctx, B, si)));
   tk_scatter_out<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, B.z, z);
   FCP_HIP(ctx, hipGetLastError());
   return 0;

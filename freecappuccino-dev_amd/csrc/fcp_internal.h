@@ -157,6 +157,15 @@ struct fcp_ctx {
   DBuf<float> tin_al, tin_au, tin_dv;  // [4n] coefficients, [n] pivots
   DBuf<int> tin_base, tin_col;       // SELL-64 copy of P A P^T
   DBuf<float> tin_val;
+  // coarse correction of the inner PCG (one unknown per tile): tile of every
+  // position, tile adjacency, the coarse operator (then its factor, then its
+  // inverse; ncp x ncp row-major) and the inverse factor
+  int inner_coarse = 1;              // add the coarse correction when it applies
+  int tc_nc = 0, tc_ncp = 0, tc_maxdeg = 0;  // tiles (0: none), padded to 64, widest adjacency
+  bool tc_have = false;              // adjacency of the current tile schedule examined
+  uint64_t tc_epoch = ~0ull;         // matrix version of the factor
+  DBuf<int> tc_tpos, tc_cptr, tc_cadj, tc_flag;  // tc_flag: the factor failed
+  DBuf<double> tc_L, tc_W, tc_r0;
   DBuf<float> tin_vec[7];            // PCG: x, r, z, q, p (two); BiCGStab: x, r, z, p, u, v, reso
   bool bj_active = false;            // FP32 sweeps use the tile-local preconditioner
   DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)

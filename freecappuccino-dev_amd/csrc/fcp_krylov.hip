@@ -1640,16 +1640,18 @@ int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z) {
 }
 }  // namespace fcpk
 
-// Tile-local DIC (kind 0) / DILU (kind 1) applied in FP32 by the tile-order
-// kernel of the inner solves (input rounded to FP32, output widened); d_out
+// Tile-local DIC (kind 0) / DILU (kind 1) / DIC with the coarse correction
+// (kind 2) applied in FP32 by the tile-order kernels of the inner solves
+// (input rounded to FP32, output widened); d_out
 // (optional) receives the FP64 pivots.
 int fcp_bj32_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out) {
   FCP_TRY(fcp_ensure_workspace(ctx));
   FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
   FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, CTR_INTS * sizeof(int), ctx->stream));
   bool ok = false;
-  FCP_TRY(bj_prepare(ctx, kind, &ok));
-  const int rc = ok ? tin_apply(ctx, r, z) : FCP_ERR_UNSUPPORTED;
+  const bool coarse = kind == 2;  // tile-local DIC plus the coarse correction
+  FCP_TRY(bj_prepare(ctx, coarse ? 0 : kind, &ok));
+  const int rc = ok ? tin_apply(ctx, r, z, coarse) : FCP_ERR_UNSUPPORTED;
   if (rc == FCP_ERR_UNSUPPORTED)
     return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "tile-order preconditioner does not fit this mesh");
   FCP_TRY(rc);

@@ -745,6 +745,7 @@ int fcp_build_tile_schedules(fcp_ctx *ctx) {
 int fcp_build_bj_schedules(fcp_ctx *ctx) {
   ctx->have_bj = false;
   ctx->tin_have = false;
+  ctx->tc_have = false;
   const int n = ctx->n;
   const int target = ctx->tune_inner_tile >= 64 ? ctx->tune_inner_tile : 4096;
   std::vector<int> rank;

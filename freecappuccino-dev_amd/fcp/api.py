@@ -30,6 +30,7 @@ FCP_TUNE_SWEEP_TILE = 5
 FCP_TUNE_INNER_PRECOND = 6
 FCP_TUNE_INNER_TILE = 7
 FCP_TUNE_GRAPH = 8
+FCP_TUNE_INNER_COARSE = 9
 
 
 class FcpError(RuntimeError):
@@ -289,7 +290,9 @@ class Context:
     def precond_apply(self, r, kind: int = 0):
         """z = M^{-1} r: kind 0 DIC, 1 DILU (returns (z, inverted pivots)),
         2 ILU(0) of pmgmres_ilu (returns (z, None)), 3 / 4 tile-local DIC /
-        DILU of the FP32 inner solves (returns (z, pivots))."""
+        DILU of the FP32 inner solves in FP64, 5 / 6 the same in FP32 by the
+        inner solves' kernel, 7 kind 5 plus the coarse correction (returns
+        (z, pivots))."""
         n = self.mesh.n_cells
         z = np.zeros(n)
         d = np.zeros(n) if kind != 2 else None

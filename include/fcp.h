@@ -259,7 +259,10 @@ int fcp_spmv_transpose(fcp_ctx *ctx, const double *x, double *y);
  * the tile-local DIC / DILU that preconditions the FP32 inner solves of
  * FCP_PREC_MIXED (no reference counterpart: the DIC / DILU recurrences with
  * the couplings between tiles dropped; fcp_sweep_tiles gives the tiles).
- * r, z of length nCells. */
+ * Kinds 5 and 6 apply the same in FP32 with the tile-order kernel of the
+ * inner solves (input rounded to FP32); kind 7 is kind 5 plus the coarse
+ * correction P0 A0^-1 P0^T r of FCP_TUNE_INNER_COARSE (FCP_ERR_UNSUPPORTED
+ * where it does not apply).  r, z of length nCells. */
 int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d);
 
 /* Tile (0-based block number) of every cell in the tile-local
@@ -350,9 +353,16 @@ int fcp_pmgmres(int n, int nnz, const int *ia, const int *ja, const double *a,
                                     time (as fast on MI355X: the gap between
                                     dependent kernels is on the device side).
                                     Profiling launches them one at a time.     */
+#define FCP_TUNE_INNER_COARSE 9  /* 1 (default): the tile-local DIC of the
+                                    FP32 inner PCG is completed by a coarse
+                                    correction with one unknown per tile
+                                    (z += P0 A0^-1 P0^T r, A0 = P0^T A P0, P0
+                                    the tile indicator; up to 1024 tiles),
+                                    which cuts the inner iterations several
+                                    times on large meshes; 0: tile-local only */
 /* Sets a launch-shape parameter of the context; results do not depend on it
- * (FCP_TUNE_INNER_PRECOND changes the path of a mixed-precision solve, not
- * its stopping test). */
+ * (FCP_TUNE_INNER_PRECOND and FCP_TUNE_INNER_COARSE change the path of a
+ * mixed-precision solve, not its stopping test). */
 int fcp_tune_set(fcp_ctx *ctx, int param, int value);
 
 /* Autotuner (no counterpart in the reference; the launch-shape search of the
