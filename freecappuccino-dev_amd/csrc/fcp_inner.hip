@@ -1102,34 +1102,54 @@ __global__ void tc_mark(int n, int nc, const int *__restrict__ perm, const int *
   for (int e = ia[row]; e < ia[row + 1]; ++e) mark[(size_t)T * nc + tpos[iperm[ja[e]]]] = 1;
 }
 
+// Slot of every CSR entry of the tiled rows in its row tile's adjacency list:
+// eslot[e] = k where cadj[cptr[T] + k] is the tile of the entry's column.
+__global__ void tc_eslot(int n, const int *__restrict__ perm, const int *__restrict__ iperm,
+                         const int *__restrict__ tpos, const int *__restrict__ ia,
+                         const int *__restrict__ ja, const int *__restrict__ cptr,
+                         const int *__restrict__ cadj, unsigned char *__restrict__ eslot) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int T = tpos[p], row = perm[p], c0 = cptr[T], deg = cptr[T + 1] - c0;
+  for (int e = ia[row]; e < ia[row + 1]; ++e) {
+    const int U = tpos[iperm[ja[e]]];
+    int k = 0;
+    while (k < deg - 1 && cadj[c0 + k] != U) ++k;
+    eslot[e] = (unsigned char)k;
+  }
+}
+
 // A0[T][U] = sum of a_ij over rows i of tile T and columns j of tile U; one
 // workgroup per tile, each thread sums its rows in order into its own LDS
 // column per adjacent tile, the columns are then added in thread order.
-// Dynamic LDS: maxdeg x BLK doubles, then nc ints.
-__global__ __launch_bounds__(BLK) void tc_vals(int nc, int ncp, int maxdeg,
-                                               const int *__restrict__ bpos,
+// Dynamic LDS: maxdeg x BLK doubles.
+__global__ __launch_bounds__(BLK) void tc_vals(int ncp, int maxdeg, const int *__restrict__ bpos,
                                                const int *__restrict__ perm,
-                                               const int *__restrict__ iperm,
-                                               const int *__restrict__ tpos,
                                                const int *__restrict__ ia,
-                                               const int *__restrict__ ja,
+                                               const unsigned char *__restrict__ eslot,
                                                const double *__restrict__ a,
                                                const int *__restrict__ cptr,
                                                const int *__restrict__ cadj,
                                                double *__restrict__ A) {
   extern __shared__ double SV[];
-  int *slot = reinterpret_cast<int *>(SV + (size_t)maxdeg * BLK);
   const int T = blockIdx.x, t = threadIdx.x;
   const int c0 = cptr[T], deg = cptr[T + 1] - c0;
-  for (int u = t; u < nc; u += BLK) slot[u] = 0;
-  __syncthreads();
-  for (int k = t; k < deg; k += BLK) slot[cadj[c0 + k]] = k;
   for (int k = 0; k < deg; ++k) SV[(size_t)k * BLK + t] = 0.0;
-  __syncthreads();
   for (int i = bpos[T] + t; i < bpos[T + 1]; i += BLK) {
     const int row = perm[i];
-    for (int e = ia[row]; e < ia[row + 1]; ++e)
-      SV[(size_t)slot[tpos[iperm[ja[e]]]] * BLK + t] += a[e];
+    const int e0 = ia[row], len = ia[row + 1] - e0;
+    double av[8];  // a row's first entries loaded together
+    int sv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < len) {
+        av[k] = a[e0 + k];
+        sv[k] = eslot[e0 + k];
+      }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k < len) SV[(size_t)sv[k] * BLK + t] += av[k];
+    for (int e = e0 + 8; e < e0 + len; ++e) SV[(size_t)eslot[e] * BLK + t] += a[e];
   }
   __syncthreads();
   for (int k = t; k < deg; k += BLK) {
@@ -1146,25 +1166,23 @@ __global__ void tc_pad(int nc, int ncp, double *__restrict__ A) {
 
 typedef double TcTile[TCB][TCB + 1];
 
-__device__ __forceinline__ void tc_load(TcTile &B, const double *__restrict__ M, int ncp, int bi,
-                                        int bj) {
-  for (int e = threadIdx.x; e < TCB * TCB; e += blockDim.x) {
-    const int r = e / TCB, c = e % TCB;
-    B[r][c] = M[(size_t)(bi * TCB + r) * ncp + bj * TCB + c];
-  }
-}
-
-// Operand of a block product, stored k-major: S[t][r] = block[r][t] (trans)
-// or block[t][r].
+// Operand of a block product (256 threads), stored k-major: S[t][r] =
+// block[r][t] (trans) or block[t][r]; every load issued before the stores.
 __device__ __forceinline__ void tc_op(TcTile &S, const double *__restrict__ M, int ncp, int bi,
                                       int bj, bool trans) {
-  for (int e = threadIdx.x; e < TCB * TCB; e += 256) {
-    const int r = e / TCB, c = e % TCB;
-    const double v = M[(size_t)(bi * TCB + r) * ncp + bj * TCB + c];
+  double v[TCB * TCB / 256];
+#pragma unroll
+  for (int u = 0; u < TCB * TCB / 256; ++u) {
+    const int e = (int)threadIdx.x + 256 * u, r = e / TCB, c = e % TCB;
+    v[u] = M[(size_t)(bi * TCB + r) * ncp + bj * TCB + c];
+  }
+#pragma unroll
+  for (int u = 0; u < TCB * TCB / 256; ++u) {
+    const int e = (int)threadIdx.x + 256 * u, r = e / TCB, c = e % TCB;
     if (trans)
-      S[c][r] = v;
+      S[c][r] = v[u];
     else
-      S[r][c] = v;
+      S[r][c] = v[u];
   }
 }
 
@@ -1209,7 +1227,7 @@ __device__ __forceinline__ void tc_store(double *__restrict__ M, int ncp, int bi
 // ty + 16 i, columns tx + 16 j), column j scaled by its owners and broadcast
 // through LDS; the inverse is formed row by row in LDS, X[r][c] =
 // -(sum over c <= m < r of L[r][m] X[m][c]) / L[r][r], four threads per
-// column.
+// column (thread part p summing the m = p mod 4).
 __global__ __launch_bounds__(256) void tc_potrf(int ncp, int k, double *__restrict__ A,
                                                 double *__restrict__ W, int *__restrict__ err) {
   __shared__ TcTile B;
@@ -1249,19 +1267,17 @@ __global__ __launch_bounds__(256) void tc_potrf(int ncp, int k, double *__restri
     const double l = piv[j];
     if (tx == jo) {  // owners of column j
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (q == jq) {
+      for (int i = 0; i < 4; ++i) {
+        const int r = ty + 16 * i;
+        double cv = v[i][0];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int r = ty + 16 * i;
-            if (r > j) {
-              v[i][q] = v[i][q] / l;
-              col[r] = v[i][q];
-            } else if (r == j) {
-              v[i][q] = l;
-            }
-          }
-        }
+        for (int q = 1; q < 4; ++q) cv = q == jq ? v[i][q] : cv;
+        const double sc = cv / l;
+        cv = r > j ? sc : (r == j ? l : cv);
+        col[r] = cv;  // read only for rows below j
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[i][q] = q == jq ? cv : v[i][q];
+      }
     }
     __syncthreads();
     double cr[4], cc[4];
@@ -1290,7 +1306,12 @@ __global__ __launch_bounds__(256) void tc_potrf(int ncp, int k, double *__restri
   const int c = threadIdx.x >> 2, part = threadIdx.x & 3;
   for (int r = 0; r < TCB; ++r) {
     double sum = 0.0;
-    for (int m = c + part; m < r; m += 4) sum += B[r][m] * X[m][c];
+#pragma unroll
+    for (int u = 0; u < TCB / 4; ++u) {  // every load issued, the terms masked
+      const int m = 4 * u + part;
+      const double w = (m >= c && m < r) ? 1.0 : 0.0;
+      sum += (B[r][m] * X[m][c]) * w;  // finite operands: a masked term adds 0
+    }
     sum += __shfl_xor(sum, 1);
     sum += __shfl_xor(sum, 2);
     if (part == 0) {
@@ -1596,6 +1617,12 @@ int tc_prepare(fcp_ctx *ctx) {
                            hipMemcpyHostToDevice));
     FCP_HIP(ctx, hipMemcpy(ctx->tc_cadj.p, cadj.data(), sizeof(int) * cadj.size(),
                            hipMemcpyHostToDevice));
+    FCP_HIP(ctx, ctx->tc_eslot.alloc(ctx->nnz));
+    tc_eslot<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, b.rows.p, ctx->tin_iperm.p,
+                                                       ctx->tc_tpos.p, ctx->ia.p, ctx->ja.p,
+                                                       ctx->tc_cptr.p, ctx->tc_cadj.p,
+                                                       ctx->tc_eslot.p);
+    FCP_HIP(ctx, hipGetLastError());
     ctx->tc_nc = nc;
     ctx->tc_ncp = (nc + TCB - 1) / TCB * TCB;
     ctx->tc_maxdeg = maxdeg;
@@ -1611,13 +1638,12 @@ int tc_prepare(fcp_ctx *ctx) {
   FCP_HIP(ctx, hipMemsetAsync(ctx->tc_W.p, 0, nn * sizeof(double), ctx->stream));
   FCP_HIP(ctx, ctx->tc_flag.alloc(1));
   FCP_HIP(ctx, hipMemsetAsync(ctx->tc_flag.p, 0, sizeof(int), ctx->stream));
-  const size_t lds = (size_t)ctx->tc_maxdeg * BLK * sizeof(double) + (size_t)nc * sizeof(int);
+  const size_t lds = (size_t)ctx->tc_maxdeg * BLK * sizeof(double);
   {
     ProfScope ps1(ctx, "coarse_vals");
-    tc_vals<<<nc, BLK, lds, ctx->stream>>>(nc, ncp, ctx->tc_maxdeg, b.blk_pos.p, b.rows.p,
-                                           ctx->tin_iperm.p, ctx->tc_tpos.p, ctx->ia.p,
-                                           ctx->ja.p, ctx->a.p, ctx->tc_cptr.p, ctx->tc_cadj.p,
-                                           ctx->tc_L.p);
+    tc_vals<<<nc, BLK, lds, ctx->stream>>>(ncp, ctx->tc_maxdeg, b.blk_pos.p, b.rows.p, ctx->ia.p,
+                                           ctx->tc_eslot.p, ctx->a.p, ctx->tc_cptr.p,
+                                           ctx->tc_cadj.p, ctx->tc_L.p);
     if (ncp > nc) tc_pad<<<1, TCB, 0, ctx->stream>>>(nc, ncp, ctx->tc_L.p);
   }
   {

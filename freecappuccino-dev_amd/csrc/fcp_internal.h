@@ -165,6 +165,7 @@ struct fcp_ctx {
   bool tc_have = false;              // adjacency of the current tile schedule examined
   uint64_t tc_epoch = ~0ull;         // matrix version of the factor
   DBuf<int> tc_tpos, tc_cptr, tc_cadj, tc_flag;  // tc_flag: the factor failed
+  DBuf<unsigned char> tc_eslot;      // adjacency slot of every CSR entry
   DBuf<double> tc_L, tc_W, tc_r0;
   DBuf<float> tin_vec[7];            // PCG: x, r, z, q, p (two); BiCGStab: x, r, z, p, u, v, reso
   bool bj_active = false;            // FP32 sweeps use the tile-local preconditioner
