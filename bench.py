@@ -72,6 +72,9 @@ def alg_bytes(name, n, nnz, Fi, Fb, local=None):
         # written
         "tin_pre": 32 * n + 12 * local if local is not None else None,
         "tin_spmv": nnz * 8 + 4 * n * 4,
+        # coarse correction: z read and written (the dense coarse rows stay
+        # in L2)
+        "tin_coarse": 2 * n * 4,
         "bj_sweep_fwd_f32": local * 8 + 4 * n + 3 * n * 4 if local is not None else None,
         "bj_sweep_bwd_f32": local * 8 + 4 * n + 3 * n * 4 if local is not None else None,
         "residual_f64": nnz * 12 + (n + 1) * 4 + 3 * n * 8,
