@@ -333,6 +333,12 @@ void fcp_finalize(fcp_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   prof_flush(ctx);
   (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->tc_stream) {
+    (void)hipStreamSynchronize(ctx->tc_stream);
+    (void)hipStreamDestroy(ctx->tc_stream);
+    (void)hipEventDestroy(ctx->tc_ready);
+    (void)hipEventDestroy(ctx->tc_done);
+  }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
   if (ctx->tin_graph) (void)hipGraphExecDestroy(ctx->tin_graph);

@@ -1575,15 +1575,49 @@ struct TkBufs {
   int ncp = 0;
 };
 
+// A0 of the current matrix, its factor and A0^-1 (over the factor), on
+// stream st; the factor's failure goes to tc_flag on the device.
+int tc_factor(fcp_ctx *ctx, hipStream_t st) {
+  const BlockSchedule &b = ctx->jlower;
+  const int nc = ctx->tc_nc, ncp = ctx->tc_ncp, nbk = ncp / TCB;
+  const size_t nn = (size_t)ncp * ncp;
+  FCP_HIP(ctx, ctx->tc_L.alloc(nn));
+  FCP_HIP(ctx, ctx->tc_W.alloc(nn));
+  FCP_HIP(ctx, ctx->tc_flag.alloc(1));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_L.p, 0, nn * sizeof(double), st));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_W.p, 0, nn * sizeof(double), st));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_flag.p, 0, sizeof(int), st));
+  const size_t lds = (size_t)ctx->tc_maxdeg * BLK * sizeof(double);
+  tc_vals<<<nc, BLK, lds, st>>>(ncp, ctx->tc_maxdeg, b.blk_pos.p, b.rows.p, ctx->ia.p,
+                                ctx->tc_eslot.p, ctx->a.p, ctx->tc_cptr.p, ctx->tc_cadj.p,
+                                ctx->tc_L.p);
+  if (ncp > nc) tc_pad<<<1, TCB, 0, st>>>(nc, ncp, ctx->tc_L.p);
+  for (int k = 0; k < nbk; ++k) {
+    tc_potrf<<<1, 256, 0, st>>>(ncp, k, ctx->tc_L.p, ctx->tc_W.p, ctx->tc_flag.p);
+    const int m = nbk - k - 1;
+    if (m == 0) break;
+    tc_panel<<<m, 256, 0, st>>>(ncp, k, ctx->tc_L.p, ctx->tc_W.p);
+    tc_update<<<m * (m + 1) / 2, 256, 0, st>>>(ncp, k, ctx->tc_L.p);
+  }
+  for (int i = 1; i < nbk; ++i) tc_inv<<<i, 256, 0, st>>>(ncp, i, ctx->tc_L.p, ctx->tc_W.p);
+  tc_gram<<<dim3(nbk, nbk), 256, 0, st>>>(ncp, ctx->tc_W.p, ctx->tc_L.p);
+  FCP_HIP(ctx, hipGetLastError());
+  ctx->tc_epoch = ctx->a_version;
+  return 0;
+}
+
 // Coarse operator of the current tile schedule and matrix: the tile
 // adjacency once per schedule, A0, its factor and inverse once per matrix.
 // Leaves tc_nc = 0 (no coarse correction) for fewer than two or more than
 // TC_MAXN tiles or a tile adjacency wider than TC_MAXDEG.  A coarse operator
 // that is not positive definite sets tc_flag on the device (no host wait);
 // the iterations then add no correction.
+int tc_join(fcp_ctx *ctx);
+
 int tc_prepare(fcp_ctx *ctx) {
   const BlockSchedule &b = ctx->jlower;
   const int n = ctx->n, nc = b.nblocks;
+  FCP_TRY(tc_join(ctx));
   if (!ctx->tc_have) {
     ctx->tc_nc = 0;
     ctx->tc_have = true;
@@ -1630,40 +1664,37 @@ int tc_prepare(fcp_ctx *ctx) {
   }
   if (ctx->tc_nc == 0 || ctx->tc_epoch == ctx->a_version) return 0;
   ProfScope ps(ctx, "coarse_setup");
-  const int ncp = ctx->tc_ncp, nbk = ncp / TCB;
-  const size_t nn = (size_t)ncp * ncp;
-  FCP_HIP(ctx, ctx->tc_L.alloc(nn));
-  FCP_HIP(ctx, ctx->tc_W.alloc(nn));
-  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_L.p, 0, nn * sizeof(double), ctx->stream));
-  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_W.p, 0, nn * sizeof(double), ctx->stream));
-  FCP_HIP(ctx, ctx->tc_flag.alloc(1));
-  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_flag.p, 0, sizeof(int), ctx->stream));
-  const size_t lds = (size_t)ctx->tc_maxdeg * BLK * sizeof(double);
-  {
-    ProfScope ps1(ctx, "coarse_vals");
-    tc_vals<<<nc, BLK, lds, ctx->stream>>>(ncp, ctx->tc_maxdeg, b.blk_pos.p, b.rows.p, ctx->ia.p,
-                                           ctx->tc_eslot.p, ctx->a.p, ctx->tc_cptr.p,
-                                           ctx->tc_cadj.p, ctx->tc_L.p);
-    if (ncp > nc) tc_pad<<<1, TCB, 0, ctx->stream>>>(nc, ncp, ctx->tc_L.p);
+  return tc_factor(ctx, ctx->stream);
+}
+
+// Waits (on the library stream) for a coarse factorisation started on the
+// side stream.
+int tc_join(fcp_ctx *ctx) {
+  if (!ctx->tc_pending) return 0;
+  ctx->tc_pending = false;
+  FCP_HIP(ctx, hipStreamWaitEvent(ctx->stream, ctx->tc_done, 0));
+  return 0;
+}
+
+// Starts the coarse factorisation of the current matrix on a side stream
+// (default priority), after the work already queued on the library stream,
+// so that it overlaps the inner solve's own set-up; tc_bufs joins it.  Needs
+// the adjacency of the current tile schedule (built by an earlier solve);
+// otherwise tc_prepare does the whole set-up in order.
+int tc_begin(fcp_ctx *ctx) {
+  if (!ctx->inner_coarse || !ctx->tc_have || ctx->tc_nc == 0 || ctx->tc_pending ||
+      ctx->tc_epoch == ctx->a_version)
+    return 0;
+  if (!ctx->tc_stream) {
+    FCP_HIP(ctx, hipStreamCreateWithFlags(&ctx->tc_stream, hipStreamNonBlocking));
+    FCP_HIP(ctx, hipEventCreateWithFlags(&ctx->tc_ready, hipEventDisableTiming));
+    FCP_HIP(ctx, hipEventCreateWithFlags(&ctx->tc_done, hipEventDisableTiming));
   }
-  {
-    ProfScope ps1(ctx, "coarse_chol");
-    for (int k = 0; k < nbk; ++k) {
-      tc_potrf<<<1, 256, 0, ctx->stream>>>(ncp, k, ctx->tc_L.p, ctx->tc_W.p, ctx->tc_flag.p);
-      const int m = nbk - k - 1;
-      if (m == 0) break;
-      tc_panel<<<m, 256, 0, ctx->stream>>>(ncp, k, ctx->tc_L.p, ctx->tc_W.p);
-      tc_update<<<m * (m + 1) / 2, 256, 0, ctx->stream>>>(ncp, k, ctx->tc_L.p);
-    }
-  }
-  {
-    ProfScope ps1(ctx, "coarse_inv");
-    for (int i = 1; i < nbk; ++i)
-      tc_inv<<<i, 256, 0, ctx->stream>>>(ncp, i, ctx->tc_L.p, ctx->tc_W.p);
-    tc_gram<<<dim3(nbk, nbk), 256, 0, ctx->stream>>>(ncp, ctx->tc_W.p, ctx->tc_L.p);
-  }
-  FCP_HIP(ctx, hipGetLastError());
-  ctx->tc_epoch = ctx->a_version;
+  FCP_HIP(ctx, hipEventRecord(ctx->tc_ready, ctx->stream));
+  FCP_HIP(ctx, hipStreamWaitEvent(ctx->tc_stream, ctx->tc_ready, 0));
+  FCP_TRY(tc_factor(ctx, ctx->tc_stream));
+  FCP_HIP(ctx, hipEventRecord(ctx->tc_done, ctx->tc_stream));
+  ctx->tc_pending = true;
   return 0;
 }
 
@@ -1911,6 +1942,13 @@ int tin_pcg(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScala
             int verbose, SolveScalars *h) {
   const int nrp = nr <= 1 ? 1 : (nr <= 2 ? 2 : 4);
   if (!tin_usable(ctx, nrp)) return FCP_ERR_UNSUPPORTED;
+  // the coarse factorisation overlaps the tile operands' set-up and the first
+  // preconditioner application; joined before its first use or on return
+  FCP_TRY(tc_begin(ctx));
+  struct Join {
+    fcp_ctx *c;
+    ~Join() { (void)tc_join(c); }
+  } join{ctx};
   const int rc = tk_prepare(ctx);
   if (rc == FCP_ERR_UNSUPPORTED) {
     ctx->tin_have = false;

@@ -166,6 +166,9 @@ struct fcp_ctx {
   uint64_t tc_epoch = ~0ull;         // matrix version of the factor
   DBuf<int> tc_tpos, tc_cptr, tc_cadj, tc_flag;  // tc_flag: the factor failed
   DBuf<unsigned char> tc_eslot;      // adjacency slot of every CSR entry
+  hipStream_t tc_stream = nullptr;   // side stream of the coarse factorisation
+  hipEvent_t tc_ready = nullptr, tc_done = nullptr;
+  bool tc_pending = false;           // a factorisation on tc_stream not yet joined
   DBuf<double> tc_L, tc_W, tc_r0;
   DBuf<float> tin_vec[7];            // PCG: x, r, z, q, p (two); BiCGStab: x, r, z, p, u, v, reso
   bool bj_active = false;            // FP32 sweeps use the tile-local preconditioner
