@@ -293,6 +293,8 @@ int tin_bicgstab(fcp_ctx *ctx, int nr, const double *res, double *x, const Solve
                  SolveScalars *si, double tol_rel, double inner_tol, int budget, int check_every,
                  int verbose, SolveScalars *h);
 int tin_apply(fcp_ctx *ctx, const double *r, double *z, bool coarse);
+int tin_coarse_begin(fcp_ctx *ctx);
+int tin_coarse_join(fcp_ctx *ctx);
 int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z);
 int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z);
 

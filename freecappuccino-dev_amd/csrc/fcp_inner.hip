@@ -1682,8 +1682,8 @@ int tc_join(fcp_ctx *ctx) {
 // the adjacency of the current tile schedule (built by an earlier solve);
 // otherwise tc_prepare does the whole set-up in order.
 int tc_begin(fcp_ctx *ctx) {
-  if (!ctx->inner_coarse || !ctx->tc_have || ctx->tc_nc == 0 || ctx->tc_pending ||
-      ctx->tc_epoch == ctx->a_version)
+  if (!ctx->inner_coarse || !ctx->have_bj || !ctx->tc_have || ctx->tc_nc == 0 ||
+      ctx->tc_pending || ctx->tc_epoch == ctx->a_version)
     return 0;
   if (!ctx->tc_stream) {
     FCP_HIP(ctx, hipStreamCreateWithFlags(&ctx->tc_stream, hipStreamNonBlocking));
@@ -1972,6 +1972,13 @@ int tin_pcg(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScala
       return FCP_ERR_ARG;
   }
 }
+
+// Starts the coarse factorisation of the inner PCG for the current matrix on
+// the side stream (nothing when it is current, does not apply or its tile
+// adjacency is not built yet); tin_coarse_join makes the library stream wait
+// for it.
+int tin_coarse_begin(fcp_ctx *ctx) { return tc_begin(ctx); }
+int tin_coarse_join(fcp_ctx *ctx) { return tc_join(ctx); }
 
 // Inner FP32 DILU-BiCGStab in tile order; arguments as tin_pcg.
 int tin_bicgstab(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScalars *so,

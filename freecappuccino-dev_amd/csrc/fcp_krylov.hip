@@ -1511,6 +1511,13 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
   };
   // FP32 inner sweeps with the tile-local preconditioner when selected (the
   // FP64 refinement and any FP64 fallback keep the global one)
+  // the coarse factorisation of the inner PCG overlaps the set-up below;
+  // joined before its first use or on return
+  struct CoarseJoin {
+    fcp_ctx *c;
+    ~CoarseJoin() { (void)tin_coarse_join(c); }
+  } coarse_join{ctx};
+  if (solver == 0 && ctx->inner_precond == 1) FCP_TRY(tin_coarse_begin(ctx));
   bool use_bj = false;
   if (solver != 1 && ctx->inner_precond >= 1) FCP_TRY(bj_prepare(ctx, kind, &use_bj));
   if (!use_bj) FCP_TRY(global_pivots());

@@ -723,6 +723,7 @@ static int tile_input(fcp_ctx *ctx, TileInput &in) {
 int fcp_build_tile_schedules(fcp_ctx *ctx) {
   ctx->have_tiles = false;
   ctx->have_bj = false;
+  ctx->tc_have = false;
   ctx->msched.clear();
   if (!ctx->have_geom || ctx->tune_sweep_tile == 1) return 0;
   const int n = ctx->n;
@@ -844,6 +845,7 @@ static int set_block_rows_caps(fcp_ctx *ctx) {
 
 int fcp_build_schedules(fcp_ctx *ctx) {
   ctx->have_bj = false;
+  ctx->tc_have = false;
   FCP_TRY(build_one_schedule(ctx, 0, ctx->lower));
   FCP_TRY(build_one_schedule(ctx, 1, ctx->upper));
   FCP_TRY(set_block_rows_caps(ctx));
