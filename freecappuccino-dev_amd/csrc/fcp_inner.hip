@@ -1422,7 +1422,8 @@ __global__ __launch_bounds__(256) void tc_gram(int ncp, const double *__restrict
 // (r0 the tile sums of r, ncp per column), then sk += e0 . r0 (the coarse
 // part of r.z) and bet = sk / s0.  Workgroup w updates part w % parts of
 // tile w / parts; every workgroup of a tile forms its entry of e0 with the
-// same summation order, wave c for column c.
+// same summation order (thread t sums the m = t mod BLK, then the waves in
+// order).
 template <int NR, int NRP>
 __global__ __launch_bounds__(BLK) void tc_zc(int parts, int ncp, const int *__restrict__ bpos,
                                              const int *__restrict__ failed,
@@ -1431,29 +1432,54 @@ __global__ __launch_bounds__(BLK) void tc_zc(int parts, int ncp, const int *__re
                                              float *__restrict__ z, double *__restrict__ partials,
                                              int *__restrict__ counter,
                                              SolveScalars *__restrict__ s) {
-  static_assert(NR <= BLK / 64, "one wave per column");
-  __shared__ double E[NR];
+  __shared__ double S[NR][BLK / 64];
   if (tk_all_done<NR>(s)) return;
   bool act[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) act[c] = !solve_done(s + c);
   const int T = blockIdx.x / parts, part = blockIdx.x % parts;
-  const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
-  if (wv < NR) {
-    const double *g = G + (size_t)T * ncp, *rc = r0 + (size_t)wv * ncp;
-    double sum = 0.0;
-    for (int m = lane; m < ncp; m += 64) sum += g[m] * rc[m];
-    sum = wave_sum(sum);
-    if (lane == 0) E[wv] = *failed ? 0.0 : sum;  // no factor: tile-local only
-  }
-  __syncthreads();
-  float e[NR];
-#pragma unroll
-  for (int c = 0; c < NR; ++c) e[c] = (float)E[c];
+  const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6;
   const int lo = bpos[T], len = bpos[T + 1] - lo;
   const int chunk = (len + parts - 1) / parts;
   const int i0 = lo + part * chunk, i1 = min(lo + len, i0 + chunk);
-  for (int i = i0 + (int)threadIdx.x; i < i1; i += BLK) {
+  // the first positions' z loaded while the coarse row is summed
+  float zv[GS_U][NR];
+#pragma unroll
+  for (int u = 0; u < GS_U; ++u) {
+    const int i = i0 + t + u * BLK;
+    if (i < i1)
+#pragma unroll
+      for (int c = 0; c < NR; ++c) zv[u][c] = z[(size_t)i * NRP + c];
+  }
+  const double *g = G + (size_t)T * ncp;
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    double sum = 0.0;
+    for (int m = t; m < ncp; m += BLK) sum += g[m] * r0[(size_t)c * ncp + m];
+    sum = wave_sum(sum);
+    if (lane == 0) S[c][wv] = sum;
+  }
+  __syncthreads();
+  const bool off = *failed != 0;  // no factor: tile-local only
+  double E[NR];
+  float e[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < BLK / 64; ++w) v += S[c][w];
+    E[c] = off ? 0.0 : v;
+    e[c] = (float)E[c];
+  }
+#pragma unroll
+  for (int u = 0; u < GS_U; ++u) {
+    const int i = i0 + t + u * BLK;
+    if (i < i1)
+#pragma unroll
+      for (int c = 0; c < NR; ++c)
+        if (act[c]) z[(size_t)i * NRP + c] = zv[u][c] + e[c];
+  }
+  for (int i = i0 + t + GS_U * BLK; i < i1; i += BLK) {
 #pragma unroll
     for (int c = 0; c < NR; ++c) {
       if (!act[c]) continue;
