@@ -357,7 +357,7 @@ int fcp_pmgmres(int n, int nnz, const int *ia, const int *ja, const double *a,
                                     FP32 inner PCG is completed by a coarse
                                     correction with one unknown per tile
                                     (z += P0 A0^-1 P0^T r, A0 = P0^T A P0, P0
-                                    the tile indicator; up to 1024 tiles),
+                                    the tile indicator; up to 2048 tiles),
                                     which cuts the inner iterations several
                                     times on large meshes; 0: tile-local only */
 /* Sets a launch-shape parameter of the context; results do not depend on it

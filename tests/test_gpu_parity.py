@@ -542,14 +542,14 @@ def test_tile_order_preconditioner_fp32_bitexact(api, kind, tile):
     ctx.close()
 
 
-@pytest.mark.parametrize("n,tile", [(32, 0), (32, 512), (48, 256)])
+@pytest.mark.parametrize("n,tile", [(32, 0), (32, 512), (48, 256), (48, 64)])
 def test_coarse_correction_matches_dense_solve(api, n, tile):
     """Kind 7 adds the coarse correction P0 A0^-1 P0^T r (A0 = P0^T A P0, P0
     the indicator of the tiles) to the FP32 tile-local DIC of kind 5: the
     difference of the two applications equals the dense solve of the coarse
     system, taken from the CSR matrix with scipy, on every cell, within FP32
-    rounding of z.  Coarse operators of 8 to several hundred tiles (one to
-    seven 64-row blocks of the blocked factorisation)."""
+    rounding of z.  Coarse operators of 8 to 1728 tiles (one to 27 64-row
+    blocks of the blocked factorisation)."""
     print(f"[case] coarse correction {n}^3 tile {tile}", flush=True)
     import scipy.sparse as sp
     mesh = hex_mesh(n, jitter=0.2, seed=4)
