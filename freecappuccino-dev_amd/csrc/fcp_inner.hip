@@ -1639,6 +1639,7 @@ int tc_factor(fcp_ctx *ctx, hipStream_t st) {
 // that is not positive definite sets tc_flag on the device (no host wait);
 // the iterations then add no correction.
 int tc_join(fcp_ctx *ctx);
+int tc_stream_init(fcp_ctx *ctx);
 
 int tc_prepare(fcp_ctx *ctx) {
   const BlockSchedule &b = ctx->jlower;
@@ -1687,10 +1688,25 @@ int tc_prepare(fcp_ctx *ctx) {
     ctx->tc_ncp = (nc + TCB - 1) / TCB * TCB;
     ctx->tc_maxdeg = maxdeg;
     ctx->tc_epoch = ~0ull;
+    FCP_TRY(tc_stream_init(ctx));
   }
   if (ctx->tc_nc == 0 || ctx->tc_epoch == ctx->a_version) return 0;
   ProfScope ps(ctx, "coarse_setup");
   return tc_factor(ctx, ctx->stream);
+}
+
+// The side stream of the coarse factorisation and its events, created (and
+// its queue brought up by a first operation) with the adjacency, so that the
+// first overlapped factorisation does not pay for them.
+int tc_stream_init(fcp_ctx *ctx) {
+  if (ctx->tc_stream) return 0;
+  FCP_HIP(ctx, hipStreamCreateWithFlags(&ctx->tc_stream, hipStreamNonBlocking));
+  FCP_HIP(ctx, hipEventCreateWithFlags(&ctx->tc_ready, hipEventDisableTiming));
+  FCP_HIP(ctx, hipEventCreateWithFlags(&ctx->tc_done, hipEventDisableTiming));
+  FCP_HIP(ctx, ctx->tc_flag.alloc(1));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->tc_flag.p, 0, sizeof(int), ctx->tc_stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->tc_stream));
+  return 0;
 }
 
 // Waits (on the library stream) for a coarse factorisation started on the
@@ -1711,11 +1727,7 @@ int tc_begin(fcp_ctx *ctx) {
   if (!ctx->inner_coarse || !ctx->have_bj || !ctx->tc_have || ctx->tc_nc == 0 ||
       ctx->tc_pending || ctx->tc_epoch == ctx->a_version)
     return 0;
-  if (!ctx->tc_stream) {
-    FCP_HIP(ctx, hipStreamCreateWithFlags(&ctx->tc_stream, hipStreamNonBlocking));
-    FCP_HIP(ctx, hipEventCreateWithFlags(&ctx->tc_ready, hipEventDisableTiming));
-    FCP_HIP(ctx, hipEventCreateWithFlags(&ctx->tc_done, hipEventDisableTiming));
-  }
+  FCP_TRY(tc_stream_init(ctx));
   FCP_HIP(ctx, hipEventRecord(ctx->tc_ready, ctx->stream));
   FCP_HIP(ctx, hipStreamWaitEvent(ctx->tc_stream, ctx->tc_ready, 0));
   FCP_TRY(tc_factor(ctx, ctx->tc_stream));
