@@ -222,6 +222,7 @@ int grid(long long m) { return (int)((m + BLK - 1) / BLK); }
 }  // namespace
 
 int fcp_launch_transpose(fcp_ctx *ctx) {
+  FCP_TRY(fcp_coarse_wait(ctx));
   const int nnz = ctx->nnz;
   DBuf<double> at;
   FCP_HIP(ctx, at.alloc(nnz));

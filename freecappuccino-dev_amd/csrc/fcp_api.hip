@@ -776,6 +776,7 @@ int fcp_update_boundary(fcp_ctx *ctx, double *phi) {
 int fcp_matrix_set(fcp_ctx *ctx, const double *a) {
   FCP_TRY(need(ctx, true, false, true));
   if (!a) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_matrix_set: null argument");
+  FCP_TRY(fcp_coarse_join(ctx));
   hipMemcpyKind kind = fcp_is_device_ptr(a) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   FCP_HIP(ctx, hipMemcpyAsync(ctx->a.p, a, sizeof(double) * ctx->nnz, kind, ctx->stream));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -286,13 +286,14 @@ GeomP geom_of(fcp_ctx *ctx) {
 
 int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
                          const unsigned char *bc_kind, const double *phi, double *su) {
+  FCP_TRY(fcp_coarse_join(ctx));
   ProfScope ps(ctx, mu_f ? "laplacian_face" : "laplacian");
   k_laplacian<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, geom_of(ctx),
       mu, mu_f, bc_kind, phi, ctx->slot_pn.p, ctx->slot_np.p, ctx->diag.p, ctx->a.p, su);
   FCP_HIP(ctx, hipGetLastError());
   ctx->a_version++;
-  return 0;
+  return fcp_coarse_begin(ctx);
 }
 
 int fcp_launch_nonorth(fcp_ctx *ctx, const double *gamma_f, const double *grad, double *su) {
@@ -326,6 +327,7 @@ int fcp_launch_update_boundary(fcp_ctx *ctx, double *phi) {
 int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
                         const double *grad, double gam, int scheme, const unsigned char *bc_kind,
                         double urf, double *su, double *sp) {
+  FCP_TRY(fcp_coarse_join(ctx));
   ProfScope ps(ctx, "convdiff");
   k_convdiff<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, geom_of(ctx),

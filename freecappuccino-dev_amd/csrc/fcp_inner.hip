@@ -2082,3 +2082,16 @@ int tin_apply(fcp_ctx *ctx, const double *r, double *z, bool coarse) {
   return 0;
 }
 }  // namespace fcpk
+
+// Matrix writers join a pending coarse factorisation before they change the
+// matrix (the side stream reads it); the Laplacian assembly starts the next
+// one (for a context whose inner PCG built the coarse adjacency), so that it
+// overlaps the gradient and source-term work before the solve.
+int fcp_coarse_join(fcp_ctx *ctx) { return tc_join(ctx); }
+// ... and before buffers it reads are freed or reallocated
+int fcp_coarse_wait(fcp_ctx *ctx) {
+  FCP_TRY(tc_join(ctx));
+  if (ctx->tc_stream) FCP_HIP(ctx, hipStreamSynchronize(ctx->tc_stream));
+  return 0;
+}
+int fcp_coarse_begin(fcp_ctx *ctx) { return tc_begin(ctx); }

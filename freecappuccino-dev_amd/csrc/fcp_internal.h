@@ -351,6 +351,12 @@ int fcp_ensure_workspace(fcp_ctx *ctx);
 int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
 int fcp_bj_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
 int fcp_bj32_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
+// fcp_inner.hip: a pending coarse factorisation of the inner PCG (side
+// stream) joined before the matrix changes / started after a Laplacian
+// assembly
+int fcp_coarse_join(fcp_ctx *ctx);
+int fcp_coarse_wait(fcp_ctx *ctx);
+int fcp_coarse_begin(fcp_ctx *ctx);
 // fcp_gmres.hip: pmgmres_ilu (restart mr) on the resident matrix; ILU(0)
 // factors; y = A^T x
 int fcp_gmres_device(fcp_ctx *ctx, double *x, const double *b, int itr_max, int mr,

@@ -358,6 +358,7 @@ int fcp_compute_geometry(fcp_ctx *ctx) {
 }
 
 int fcp_build_csr(fcp_ctx *ctx) {
+  FCP_TRY(fcp_coarse_wait(ctx));
   const int n = ctx->n, Fi = ctx->Fi;
   ctx->sym_pattern = true;  // (owner, neighbour) and (neighbour, owner) for every inner face
   for (int ib = 0; ib < ctx->nPatches; ++ib)
@@ -721,6 +722,7 @@ static int tile_input(fcp_ctx *ctx, TileInput &in) {
 }
 
 int fcp_build_tile_schedules(fcp_ctx *ctx) {
+  FCP_TRY(fcp_coarse_wait(ctx));  // a pending coarse factorisation reads the schedules
   ctx->have_tiles = false;
   ctx->have_bj = false;
   ctx->tc_have = false;
@@ -744,6 +746,7 @@ int fcp_build_tile_schedules(fcp_ctx *ctx) {
 // cells when the mesh has geometry (any numbering: blocks never wait on one
 // another), else runs of 4096 contiguous rows.
 int fcp_build_bj_schedules(fcp_ctx *ctx) {
+  FCP_TRY(fcp_coarse_wait(ctx));  // a pending coarse factorisation reads the schedules
   ctx->have_bj = false;
   ctx->tin_have = false;
   ctx->tc_have = false;
@@ -777,6 +780,7 @@ int fcp_build_bj_schedules(fcp_ctx *ctx) {
 }
 
 int fcp_build_block_schedules(fcp_ctx *ctx) {
+  FCP_TRY(fcp_coarse_wait(ctx));  // a pending coarse factorisation reads the schedules
   ctx->msched.clear();
   FCP_TRY(fcp_build_block_schedule(ctx, ctx->lower, fcp_block_rows(ctx, 8, 0), 0, ctx->blower));
   FCP_TRY(fcp_build_block_schedule(ctx, ctx->upper, fcp_block_rows(ctx, 8, 1), 1, ctx->bupper));
@@ -844,6 +848,7 @@ static int set_block_rows_caps(fcp_ctx *ctx) {
 }
 
 int fcp_build_schedules(fcp_ctx *ctx) {
+  FCP_TRY(fcp_coarse_wait(ctx));  // a pending coarse factorisation reads the schedules
   ctx->have_bj = false;
   ctx->tc_have = false;
   FCP_TRY(build_one_schedule(ctx, 0, ctx->lower));
