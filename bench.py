@@ -35,7 +35,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweeps_spmv_128cube_hbm_traffic.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweeps_spmv_128cube_hbm_traffic.txt")
 
 
 def measured_traffic(kernel, N):
