@@ -1223,19 +1223,21 @@ __device__ __forceinline__ void tc_store(double *__restrict__ M, int ncp, int bi
 
 // Diagonal block k: Cholesky, written back with its upper triangle zeroed,
 // and the inverse of the factor written to W's block (k, k).  The factor is
-// right-looking with the block in registers (thread (ty, tx) holds rows
-// ty + 16 i, columns tx + 16 j), column j scaled by its owners and broadcast
-// through LDS; the inverse is formed row by row in LDS, X[r][c] =
-// -(sum over c <= m < r of L[r][m] X[m][c]) / L[r][r], four threads per
-// column (thread part p summing the m = p mod 4).
+// right-looking on the unscaled columns (thread (ty, tx) holds rows
+// ty + 16 i, columns tx + 16 j in registers): step j subtracts
+// v[r][j] v[c][j] / d_j from the trailing block, with column j and its pivot
+// d_j = v[j][j] published by the column's owners at the end of step j - 1
+// into one of two LDS buffers, so a step costs one barrier; at the end
+// L[r][j] = v[r][j] / sqrt(d_j), L[j][j] = sqrt(d_j).  The inverse is formed
+// row by row, X[r][c] = -(sum over c <= m < r of L[r][m] X[m][c]) / L[r][r],
+// by the four lanes of column c (one wave: no workgroup barrier).
 __global__ __launch_bounds__(256) void tc_potrf(int ncp, int k, double *__restrict__ A,
                                                 double *__restrict__ W, int *__restrict__ err) {
   __shared__ TcTile B;
   __shared__ TcTile X;
-  __shared__ double col[TCB], piv[TCB];
-  __shared__ int bad;
+  __shared__ double col[2][TCB], piv[TCB], d0s[TCB];
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double v[4][4], d0 = 0.0;
+  double v[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -1243,63 +1245,55 @@ __global__ __launch_bounds__(256) void tc_potrf(int ncp, int k, double *__restri
       v[i][q] = A[(size_t)(k * TCB + ty + 16 * i) * ncp + k * TCB + tx + 16 * q];
   if (tx == ty) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) piv[ty + 16 * i] = v[i][i];  // original diagonal
+    for (int i = 0; i < 4; ++i) d0s[ty + 16 * i] = v[i][i];  // original diagonal
   }
-  if (threadIdx.x == 0) bad = 0;
-  __syncthreads();
-  for (int j = 0; j < TCB; ++j) {
-    const int jo = j & 15, jq = j >> 4;
-    if (tx == jo && ty == jo) {  // owner of (j, j)
-      double d = 0.0;
+  if (tx == 0) {  // column 0 (unscaled) for step 0
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (q == jq) d = v[q][q];
-      d0 = piv[j];
-      // a pivot lost to cancellation marks a singular coarse operator
-      if (!(d > 1e-10 * d0)) bad = 1;
-      piv[j] = sqrt(d);
-    }
+    for (int i = 0; i < 4; ++i) col[0][ty + 16 * i] = v[i][0];
+  }
+  for (int j = 0; j < TCB; ++j) {
     __syncthreads();
-    if (bad) {  // uniform
+    const double *cj = col[j & 1];
+    const double d = cj[j];
+    // uniform: a pivot lost to cancellation marks a singular coarse operator
+    if (!(d > 1e-10 * d0s[j])) {
       if (threadIdx.x == 0) atomicOr(err, 2);
       return;
     }
-    const double l = piv[j];
-    if (tx == jo) {  // owners of column j
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = ty + 16 * i;
-        double cv = v[i][0];
-#pragma unroll
-        for (int q = 1; q < 4; ++q) cv = q == jq ? v[i][q] : cv;
-        const double sc = cv / l;
-        cv = r > j ? sc : (r == j ? l : cv);
-        col[r] = cv;  // read only for rows below j
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[i][q] = q == jq ? cv : v[i][q];
-      }
-    }
-    __syncthreads();
+    if (threadIdx.x == 0) piv[j] = d;
+    const double rd = 1.0 / d;
     double cr[4], cc[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) cr[i] = col[ty + 16 * i];
+    for (int i = 0; i < 4; ++i) cr[i] = cj[ty + 16 * i];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) cc[q] = col[tx + 16 * q];
+    for (int q = 0; q < 4; ++q) cc[q] = cj[tx + 16 * q];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int r = ty + 16 * i, c = tx + 16 * q;
-        if (c > j && c <= r) v[i][q] = v[i][q] - cr[i] * cc[q];
+        if (c > j && c <= r) v[i][q] = v[i][q] - (cr[i] * cc[q]) * rd;
       }
-    // col is next written after the next column's first barrier
+    const int jn = j + 1;
+    if (jn < TCB && tx == (jn & 15)) {  // owners of column j + 1 publish it
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        double cv = v[i][0];
+#pragma unroll
+        for (int q = 1; q < 4; ++q) cv = q == (jn >> 4) ? v[i][q] : cv;
+        col[jn & 1][ty + 16 * i] = cv;  // read only for rows >= j + 1
+      }
+    }
   }
+  __syncthreads();
+  // scale the columns: L[r][c] = v[r][c] / sqrt(d_c) below the diagonal
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int r = ty + 16 * i, c = tx + 16 * q;
-      B[r][c] = c <= r ? v[i][q] : 0.0;
+      const double l = sqrt(piv[c]);
+      B[r][c] = c < r ? v[i][q] / l : (c == r ? l : 0.0);
       X[r][c] = 0.0;
     }
   __syncthreads();
@@ -1318,8 +1312,11 @@ __global__ __launch_bounds__(256) void tc_potrf(int ncp, int k, double *__restri
       if (c < r) X[r][c] = -sum / B[r][r];
       if (c == r) X[r][c] = 1.0 / B[r][r];
     }
-    __syncthreads();
+    // column c's lanes are one wave and LDS operations of a wave complete in
+    // order: the next row reads what this one wrote without a barrier
+    __builtin_amdgcn_wave_barrier();
   }
+  __syncthreads();
   for (int e = threadIdx.x; e < TCB * TCB; e += blockDim.x) {
     const int r = e / TCB, cc = e % TCB;
     const size_t o = (size_t)(k * TCB + r) * ncp + k * TCB + cc;
