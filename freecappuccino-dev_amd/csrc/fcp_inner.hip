@@ -1186,6 +1186,28 @@ __device__ __forceinline__ void tc_op(TcTile &S, const double *__restrict__ M, i
   }
 }
 
+// tc_op in two halves, so the next operand's loads can be in flight while
+// the current product is formed: block (bi, bj) into registers, then into LDS.
+constexpr int TC_PER = TCB * TCB / 256;
+__device__ __forceinline__ void tc_fetch(double (&v)[TC_PER], const double *__restrict__ M,
+                                         int ncp, int bi, int bj) {
+#pragma unroll
+  for (int u = 0; u < TC_PER; ++u) {
+    const int e = (int)threadIdx.x + 256 * u, r = e / TCB, c = e % TCB;
+    v[u] = M[(size_t)(bi * TCB + r) * ncp + bj * TCB + c];
+  }
+}
+__device__ __forceinline__ void tc_put(TcTile &S, const double (&v)[TC_PER], bool trans) {
+#pragma unroll
+  for (int u = 0; u < TC_PER; ++u) {
+    const int e = (int)threadIdx.x + 256 * u, r = e / TCB, c = e % TCB;
+    if (trans)
+      S[c][r] = v[u];
+    else
+      S[r][c] = v[u];
+  }
+}
+
 // acc[i][j] += sum_t A[t][ty + 16 i] B[t][tx + 16 j] (256 threads, thread
 // (ty, tx) = (tid / 16, tid % 16) holding a 4 x 4 output set; t ascending).
 __device__ __forceinline__ void tc_mm(const TcTile &A, const TcTile &B, double (&acc)[4][4]) {
@@ -1377,10 +1399,19 @@ __global__ __launch_bounds__(256) void tc_inv(int ncp, int i, const double *__re
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   double acc[4][4];
   tc_zero(acc);
+  double pv[TC_PER], qv[TC_PER];
+  tc_fetch(pv, A, ncp, i, j);
+  tc_fetch(qv, W, ncp, j, j);
   for (int m = j; m < i; ++m) {
-    tc_op(P, A, ncp, i, m, true);
-    tc_op(Q, W, ncp, m, j, false);
+    tc_put(P, pv, true);
+    tc_put(Q, qv, false);
     __syncthreads();
+    if (m + 1 < i) {  // the next operands load during this product
+      tc_fetch(pv, A, ncp, i, m + 1);
+      tc_fetch(qv, W, ncp, m + 1, j);
+    } else {
+      tc_fetch(pv, W, ncp, i, i);
+    }
     tc_mm(P, Q, acc);
     __syncthreads();
   }
@@ -1389,7 +1420,7 @@ __global__ __launch_bounds__(256) void tc_inv(int ncp, int i, const double *__re
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 4; ++b) Q[ty + 16 * a][tx + 16 * b] = acc[a][b];
-  tc_op(P, W, ncp, i, i, true);
+  tc_put(P, pv, true);
   __syncthreads();
   tc_zero(acc);
   tc_mm(P, Q, acc);
@@ -1405,10 +1436,18 @@ __global__ __launch_bounds__(256) void tc_gram(int ncp, const double *__restrict
   const int bi = blockIdx.x, bj = blockIdx.y, nbk = ncp / TCB;
   double acc[4][4];
   tc_zero(acc);
-  for (int m = bi > bj ? bi : bj; m < nbk; ++m) {
-    tc_op(P, W, ncp, m, bi, false);
-    tc_op(Q, W, ncp, m, bj, false);
+  const int m0 = bi > bj ? bi : bj;
+  double pv[TC_PER], qv[TC_PER];
+  tc_fetch(pv, W, ncp, m0, bi);
+  tc_fetch(qv, W, ncp, m0, bj);
+  for (int m = m0; m < nbk; ++m) {
+    tc_put(P, pv, false);
+    tc_put(Q, qv, false);
     __syncthreads();
+    if (m + 1 < nbk) {  // the next operands load during this product
+      tc_fetch(pv, W, ncp, m + 1, bi);
+      tc_fetch(qv, W, ncp, m + 1, bj);
+    }
     tc_mm(P, Q, acc);
     __syncthreads();
   }
