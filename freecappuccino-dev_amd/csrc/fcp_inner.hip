@@ -132,36 +132,42 @@ __global__ void tk_pattern(int n, int nb, int dummy, const int *__restrict__ per
 
 // Coefficients (FP32 roundings of the FP64 matrix) of both sweeps and the
 // FP32 rounding of the tile-local pivots.
-__global__ void tk_sweep_vals(int n, int nb, const int *__restrict__ perm,
-                              const int *__restrict__ iperm, const int *__restrict__ bpos,
-                              const int *__restrict__ psrc, const int *__restrict__ ia,
-                              const int *__restrict__ diag, const int *__restrict__ ja,
-                              const double *__restrict__ a, const double *__restrict__ dbj,
-                              float4 *__restrict__ al, float4 *__restrict__ au,
-                              float *__restrict__ dv) {
+// CSR entry of every in-tile upper dependency slot of every position (-1:
+// unused), in the order the backward sweep takes them (once per pattern).
+__global__ void tk_upper_src(int n, int nb, const int *__restrict__ perm,
+                             const int *__restrict__ iperm, const int *__restrict__ bpos,
+                             const int *__restrict__ ia, const int *__restrict__ diag,
+                             const int *__restrict__ ja, int *__restrict__ usrc) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  float v[FCP_PMAXD];
-#pragma unroll
-  for (int m = 0; m < FCP_PMAXD; ++m) {
-    const int k = psrc[(size_t)FCP_PMAXD * p + m];
-    v[m] = k >= 0 ? (float)a[k] : 0.0f;
-  }
-  al[p] = make_float4(v[0], v[1], v[2], v[3]);
   const int b = tk_block_of(bpos, nb, p);
   const int lo = bpos[b], hi = bpos[b + 1];
   const int row = perm[p];
   int m = 0;
-#pragma unroll
-  for (int k = 0; k < FCP_PMAXD; ++k) v[k] = 0.0f;
   for (int k = diag[row] + 1; k < ia[row + 1] && m < FCP_PMAXD; ++k) {
     const int q = iperm[ja[k]];
-    if (q >= lo && q < hi) v[m++] = (float)a[k];
+    if (q >= lo && q < hi) usrc[(size_t)FCP_PMAXD * p + m++] = k;
   }
-  au[p] = make_float4(v[0], v[1], v[2], v[3]);
-  dv[p] = (float)dbj[row];
+  for (; m < FCP_PMAXD; ++m) usrc[(size_t)FCP_PMAXD * p + m] = -1;
 }
 
+// FP32 sweep operands of the current matrix in tile order: lower and upper
+// in-tile coefficients (psrc / usrc name their CSR entries) and the pivots.
+__global__ void tk_sweep_vals(int n, const int *__restrict__ perm, const int *__restrict__ psrc,
+                              const int *__restrict__ usrc, const double *__restrict__ a,
+                              const double *__restrict__ dbj, float4 *__restrict__ al,
+                              float4 *__restrict__ au, float *__restrict__ dv) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int4 ls = reinterpret_cast<const int4 *>(psrc)[p];
+  const int4 us = reinterpret_cast<const int4 *>(usrc)[p];
+  const int row = perm[p];
+  al[p] = make_float4(ls.x >= 0 ? (float)a[ls.x] : 0.0f, ls.y >= 0 ? (float)a[ls.y] : 0.0f,
+                      ls.z >= 0 ? (float)a[ls.z] : 0.0f, ls.w >= 0 ? (float)a[ls.w] : 0.0f);
+  au[p] = make_float4(us.x >= 0 ? (float)a[us.x] : 0.0f, us.y >= 0 ? (float)a[us.y] : 0.0f,
+                      us.z >= 0 ? (float)a[us.z] : 0.0f, us.w >= 0 ? (float)a[us.w] : 0.0f);
+  dv[p] = (float)dbj[row];
+}
 // SELL-64 copy of P A P^T: slice widths, columns, values.  A row keeps its
 // CSR entry order; padding entries have value 0 and the row's own column.
 __global__ void tk_sell_len(int n, int nslices, const int *__restrict__ perm,
@@ -1605,6 +1611,10 @@ int tk_prepare(fcp_ctx *ctx) {
     int err = 0;
     FCP_HIP(ctx, hipMemcpy(&err, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost));
     if (err) return FCP_ERR_UNSUPPORTED;  // more upper dependencies than slots
+    FCP_HIP(ctx, ctx->tin_usrc.alloc((size_t)FCP_PMAXD * n));
+    tk_upper_src<<<g, BLK, 0, ctx->stream>>>(n, b.nblocks, b.rows.p, ctx->tin_iperm.p,
+                                            b.blk_pos.p, ctx->ia.p, ctx->diag.p, ctx->ja.p,
+                                            ctx->tin_usrc.p);
     FCP_HIP(ctx, ctx->tin_col.alloc(npad));
     FCP_HIP(ctx, ctx->tin_val.alloc(npad));
     tk_sell_col<<<grid_for(nsl * 64, BLK), BLK, 0, ctx->stream>>>(
@@ -1619,9 +1629,9 @@ int tk_prepare(fcp_ctx *ctx) {
     FCP_HIP(ctx, ctx->tin_au.alloc(4 * (size_t)n));
     FCP_HIP(ctx, ctx->tin_dv.alloc(n));
     tk_sweep_vals<<<g, BLK, 0, ctx->stream>>>(
-        n, b.nblocks, b.rows.p, ctx->tin_iperm.p, b.blk_pos.p, b.psrc.p, ctx->ia.p, ctx->diag.p,
-        ctx->ja.p, ctx->a.p, ctx->dbj.p, reinterpret_cast<float4 *>(ctx->tin_al.p),
-        reinterpret_cast<float4 *>(ctx->tin_au.p), ctx->tin_dv.p);
+        n, b.rows.p, b.psrc.p, ctx->tin_usrc.p, ctx->a.p, ctx->dbj.p,
+        reinterpret_cast<float4 *>(ctx->tin_al.p), reinterpret_cast<float4 *>(ctx->tin_au.p),
+        ctx->tin_dv.p);
     const int nsl = (n + 63) / 64;
     tk_sell_val<<<grid_for(nsl * 64, BLK), BLK, 0, ctx->stream>>>(
         n, nsl, b.rows.p, ctx->ia.p, ctx->a.p, ctx->tin_base.p, ctx->tin_val.p);

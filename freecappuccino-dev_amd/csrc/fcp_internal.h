@@ -164,6 +164,7 @@ struct fcp_ctx {
   int tc_nc = 0, tc_ncp = 0, tc_maxdeg = 0;  // tiles (0: none), padded to 64, widest adjacency
   bool tc_have = false;              // adjacency of the current tile schedule examined
   uint64_t tc_epoch = ~0ull;         // matrix version of the factor
+  DBuf<int> tin_usrc;                // CSR entry of every in-tile upper sweep slot (-1: none)
   DBuf<int> tc_tpos, tc_cptr, tc_cadj, tc_flag;  // tc_flag: the factor failed
   DBuf<unsigned char> tc_eslot;      // adjacency slot of every CSR entry
   hipStream_t tc_stream = nullptr;   // side stream of the coarse factorisation
