@@ -200,15 +200,26 @@ __global__ void tk_sell_col(int n, int nslices, const int *__restrict__ perm,
     col[q + 64 * k] = k < len ? iperm[ja[ia[row] + k]] : (p < n ? p : 0);
 }
 
-__global__ void tk_sell_val(int n, int nslices, const int *__restrict__ perm,
-                            const int *__restrict__ ia, const double *__restrict__ a,
-                            const int *__restrict__ base, float *__restrict__ val) {
+// CSR start and length of the row at every tile-order position (once per
+// pattern).
+__global__ void tk_row_src(int n, const int *__restrict__ perm, const int *__restrict__ ia,
+                           int2 *__restrict__ rs) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int row = perm[p];
+  rs[p] = make_int2(ia[row], ia[row + 1] - ia[row]);
+}
+// Values of the SELL-64 copy: thread (position, blockIdx.y) fills entries
+// k = blockIdx.y, blockIdx.y + gridDim.y, ... of its row.
+__global__ void tk_sell_val(int n, int nslices, const int2 *__restrict__ rs,
+                            const double *__restrict__ a, const int *__restrict__ base,
+                            float *__restrict__ val) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nslices * 64) return;
   const int s = p >> 6, q = base[s] + (p & 63), L = (base[s + 1] - base[s]) >> 6;
-  const int row = p < n ? perm[p] : -1;
-  const int len = row >= 0 ? ia[row + 1] - ia[row] : 0;
-  for (int k = 0; k < L; ++k) val[q + 64 * k] = k < len ? (float)a[ia[row] + k] : 0.0f;
+  const int2 r = p < n ? rs[p] : make_int2(0, 0);
+  for (int k = blockIdx.y; k < L; k += gridDim.y)
+    val[q + 64 * k] = k < r.y ? (float)a[r.x + k] : 0.0f;
 }
 
 // ---------------------------------------------------------------------------
@@ -1611,6 +1622,9 @@ int tk_prepare(fcp_ctx *ctx) {
     int err = 0;
     FCP_HIP(ctx, hipMemcpy(&err, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost));
     if (err) return FCP_ERR_UNSUPPORTED;  // more upper dependencies than slots
+    FCP_HIP(ctx, ctx->tin_rs.alloc(2 * (size_t)n));
+    tk_row_src<<<g, BLK, 0, ctx->stream>>>(n, b.rows.p, ctx->ia.p,
+                                          reinterpret_cast<int2 *>(ctx->tin_rs.p));
     FCP_HIP(ctx, ctx->tin_usrc.alloc((size_t)FCP_PMAXD * n));
     tk_upper_src<<<g, BLK, 0, ctx->stream>>>(n, b.nblocks, b.rows.p, ctx->tin_iperm.p,
                                             b.blk_pos.p, ctx->ia.p, ctx->diag.p, ctx->ja.p,
@@ -1633,8 +1647,9 @@ int tk_prepare(fcp_ctx *ctx) {
         reinterpret_cast<float4 *>(ctx->tin_al.p), reinterpret_cast<float4 *>(ctx->tin_au.p),
         ctx->tin_dv.p);
     const int nsl = (n + 63) / 64;
-    tk_sell_val<<<grid_for(nsl * 64, BLK), BLK, 0, ctx->stream>>>(
-        n, nsl, b.rows.p, ctx->ia.p, ctx->a.p, ctx->tin_base.p, ctx->tin_val.p);
+    tk_sell_val<<<dim3(grid_for((size_t)nsl * 64, BLK), SELL_KU), BLK, 0, ctx->stream>>>(
+        n, nsl, reinterpret_cast<const int2 *>(ctx->tin_rs.p), ctx->a.p, ctx->tin_base.p,
+        ctx->tin_val.p);
     FCP_HIP(ctx, hipGetLastError());
     ctx->tin_epoch = ctx->bj_epoch;
   }

@@ -165,6 +165,7 @@ struct fcp_ctx {
   bool tc_have = false;              // adjacency of the current tile schedule examined
   uint64_t tc_epoch = ~0ull;         // matrix version of the factor
   DBuf<int> tin_usrc;                // CSR entry of every in-tile upper sweep slot (-1: none)
+  DBuf<int> tin_rs;                  // CSR start and length of every position's row
   DBuf<int> tc_tpos, tc_cptr, tc_cadj, tc_flag;  // tc_flag: the factor failed
   DBuf<unsigned char> tc_eslot;      // adjacency slot of every CSR entry
   hipStream_t tc_stream = nullptr;   // side stream of the coarse factorisation
