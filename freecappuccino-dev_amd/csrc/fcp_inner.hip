@@ -1141,8 +1141,7 @@ __global__ void tc_eslot(int n, const int *__restrict__ perm, const int *__restr
 // column per adjacent tile, the columns are then added in thread order.
 // Dynamic LDS: maxdeg x BLK doubles.
 __global__ __launch_bounds__(BLK) void tc_vals(int ncp, int maxdeg, const int *__restrict__ bpos,
-                                               const int *__restrict__ perm,
-                                               const int *__restrict__ ia,
+                                               const int2 *__restrict__ rs,
                                                const unsigned char *__restrict__ eslot,
                                                const double *__restrict__ a,
                                                const int *__restrict__ cptr,
@@ -1151,22 +1150,30 @@ __global__ __launch_bounds__(BLK) void tc_vals(int ncp, int maxdeg, const int *_
   extern __shared__ double SV[];
   const int T = blockIdx.x, t = threadIdx.x;
   const int c0 = cptr[T], deg = cptr[T + 1] - c0;
+  const int hi = bpos[T + 1];
   for (int k = 0; k < deg; ++k) SV[(size_t)k * BLK + t] = 0.0;
-  for (int i = bpos[T] + t; i < bpos[T + 1]; i += BLK) {
-    const int row = perm[i];
-    const int e0 = ia[row], len = ia[row + 1] - e0;
-    double av[8];  // a row's first entries loaded together
-    int sv[8];
+  constexpr int RU = 4, KU = 8;  // rows, and entries per row, loaded together
+  for (int i0 = bpos[T] + t; i0 < hi; i0 += RU * BLK) {
+    int2 r[RU];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k < len) {
-        av[k] = a[e0 + k];
-        sv[k] = eslot[e0 + k];
-      }
+    for (int u = 0; u < RU; ++u) r[u] = i0 + u * BLK < hi ? rs[i0 + u * BLK] : make_int2(0, 0);
+    double av[RU][KU];
+    int sv[RU][KU];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k < len) SV[(size_t)sv[k] * BLK + t] += av[k];
-    for (int e = e0 + 8; e < e0 + len; ++e) SV[(size_t)eslot[e] * BLK + t] += a[e];
+    for (int u = 0; u < RU; ++u)
+#pragma unroll
+      for (int k = 0; k < KU; ++k)
+        if (k < r[u].y) {
+          av[u][k] = a[r[u].x + k];
+          sv[u][k] = eslot[r[u].x + k];
+        }
+#pragma unroll
+    for (int u = 0; u < RU; ++u) {  // rows in the order i0, i0 + BLK, ...
+#pragma unroll
+      for (int k = 0; k < KU; ++k)
+        if (k < r[u].y) SV[(size_t)sv[u][k] * BLK + t] += av[u][k];
+      for (int e = r[u].x + KU; e < r[u].x + r[u].y; ++e) SV[(size_t)eslot[e] * BLK + t] += a[e];
+    }
   }
   __syncthreads();
   for (int k = t; k < deg; k += BLK) {
@@ -1675,9 +1682,9 @@ int tc_factor(fcp_ctx *ctx, hipStream_t st) {
   FCP_HIP(ctx, hipMemsetAsync(ctx->tc_W.p, 0, nn * sizeof(double), st));
   FCP_HIP(ctx, hipMemsetAsync(ctx->tc_flag.p, 0, sizeof(int), st));
   const size_t lds = (size_t)ctx->tc_maxdeg * BLK * sizeof(double);
-  tc_vals<<<nc, BLK, lds, st>>>(ncp, ctx->tc_maxdeg, b.blk_pos.p, b.rows.p, ctx->ia.p,
-                                ctx->tc_eslot.p, ctx->a.p, ctx->tc_cptr.p, ctx->tc_cadj.p,
-                                ctx->tc_L.p);
+  tc_vals<<<nc, BLK, lds, st>>>(ncp, ctx->tc_maxdeg, b.blk_pos.p,
+                                reinterpret_cast<const int2 *>(ctx->tin_rs.p), ctx->tc_eslot.p,
+                                ctx->a.p, ctx->tc_cptr.p, ctx->tc_cadj.p, ctx->tc_L.p);
   if (ncp > nc) tc_pad<<<1, TCB, 0, st>>>(nc, ncp, ctx->tc_L.p);
   for (int k = 0; k < nbk; ++k) {
     tc_potrf<<<1, 256, 0, st>>>(ncp, k, ctx->tc_L.p, ctx->tc_W.p, ctx->tc_flag.p);
