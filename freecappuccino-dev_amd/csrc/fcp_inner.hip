@@ -804,18 +804,19 @@ __global__ __launch_bounds__(BLK) void tk_init(int n, const int *__restrict__ pe
   }
 }
 
-// x += |res|_2 P^T x~ (FP64, natural order)
+// x += |res|_2 P^T x~ (FP64), in natural row order (x read and written
+// contiguously, x~ gathered through the inverse permutation)
 template <int NR, int NRP>
-__global__ void tk_ir_update(int n, const int *__restrict__ perm, const float *__restrict__ e,
+__global__ void tk_ir_update(int n, const int *__restrict__ iperm, const float *__restrict__ e,
                              const SolveScalars *__restrict__ so, double *__restrict__ x) {
   double sc[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) sc[c] = so[c].l2;
-  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
-    const size_t row = (size_t)perm[i];
+  for (int row = blockIdx.x * BLK + threadIdx.x; row < n; row += gridDim.x * BLK) {
+    const size_t i = (size_t)iperm[row];
 #pragma unroll
     for (int c = 0; c < NR; ++c)
-      x[row * NRP + c] = x[row * NRP + c] + sc[c] * (double)e[(size_t)i * NRP + c];
+      x[(size_t)row * NRP + c] = x[(size_t)row * NRP + c] + sc[c] * (double)e[i * NRP + c];
   }
 }
 
@@ -1951,7 +1952,7 @@ int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, S
     }
     if (all) break;
   }
-  tk_ir_update<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, B.x, so, x);
+  tk_ir_update<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->tin_iperm.p, B.x, so, x);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }
@@ -2024,7 +2025,7 @@ int tb_bicgstab(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *
     }
     if (all) break;
   }
-  tk_ir_update<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, b.rows.p, X, so, x);
+  tk_ir_update<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->tin_iperm.p, X, so, x);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }
