@@ -19,6 +19,10 @@
 //           then z = M^-1 r with both sweeps in LDS and r.z (:458-478);
 //   tk_spmv (grid-stride rows): p = z + bet p (:488) computed on the fly
 //           for every operand, q = A p and p.q (:493-501).
+// With the two-level preconditioner (tile-local DIC plus a coarse correction
+// with one unknown per tile, no counterpart in the reference) tk_pre also
+// writes each tile's residual sum and tc_zc, between the two, adds
+// (A0^-1 r0)[tile] to z and completes r.z and bet.
 // NR right-hand sides are interleaved (NRP values per position); every
 // column's arithmetic is independent of NR, so a batched solve's columns
 // equal single-column solves bit for bit.
