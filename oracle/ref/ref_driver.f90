@@ -12,6 +12,9 @@
 !   grad_lsq   variant phi out                variant: lsq | lsq_dm | lsq_qr
 !   solve      solver itr_max tol_abs tol_rel a fi0 rhs fi_out info_out
 !   convdiff   flmass dcoef phi grad gam scheme bckind a_out su_out sp_out
+!   nonorth    gamma grad su0 su_out         (facefluxsc with zero mass flux:
+!              the explicit non-orthogonal part only, scalar_fluxes.f90:99-109,139)
+!   update_boundary phi phi_out               (updateBoundary.f90)
 !   solve_explicit solver n nnz itr_max tol_abs tol_rel ia ja diag a fi0 rhs fi_out info_out
 !              (explicit-array entry points iccg/dpcg/bicgstab/GaussSeidel on a
 !               CSR system of its own, 1-based int32 ia/ja/diag files)
@@ -64,6 +67,10 @@ program ref_driver
       call job_solve(ju)
     case ('convdiff')
       call job_convdiff(ju)
+    case ('nonorth')
+      call job_nonorth(ju)
+    case ('update_boundary')
+      call job_update_boundary(ju)
     case ('solve_explicit')
       call job_solve_explicit(ju)
     case ('ilu')
@@ -338,6 +345,47 @@ contains
     call wr8(fsu, su, numCells)
     call wr8(fsp, spl, numCells)
     deallocate(fm, dc, phi, gr, bk, spl)
+  end subroutine
+
+  ! The explicit non-orthogonal correction of the inner faces, taken from
+  ! the reference's facefluxsc with zero mass flux (then fcfie = fcfii = 0
+  ! and suadd = fdfie exactly), scattered as temperature.f90:152-155 does.
+  subroutine job_nonorth(ju)
+    integer, intent(in) :: ju
+    character(len=64) :: c, fdc, fgr, fsu0, fsu
+    character(len=30) :: scheme
+    real(dp) :: cap, can, suadd
+    real(dp), allocatable :: dc(:), phi(:), gr(:,:)
+    integer :: i, ijp, ijn
+    read(ju, *) c, fdc, fgr, fsu0, fsu
+    allocate(dc(numFaces), phi(numTotal), gr(3,numTotal))
+    call rd8(fdc, dc, numFaces)
+    call rd8(fgr, gr, 3*numTotal)
+    call rd8(fsu0, su, numCells)
+    phi = 0.0_dp
+    scheme = 'cds'
+    do i = 1, numInnerFaces
+      ijp = owner(i)
+      ijn = neighbour(i)
+      call facefluxsc(i, ijp, ijn, xf(i), yf(i), zf(i), arx(i), ary(i), arz(i), &
+                      0.0_dp, facint(i), 1.0_dp, scheme, phi, gr, dc(i), cap, can, suadd)
+      su(ijp) = su(ijp) + suadd
+      su(ijn) = su(ijn) - suadd
+    enddo
+    call wr8(fsu, su, numCells)
+    deallocate(dc, phi, gr)
+  end subroutine
+
+  subroutine job_update_boundary(ju)
+    integer, intent(in) :: ju
+    character(len=64) :: c, fphi, fout
+    real(dp), allocatable :: phi(:)
+    read(ju, *) c, fphi, fout
+    allocate(phi(numTotal))
+    call rd8(fphi, phi, numTotal)
+    call updateBoundary(phi)
+    call wr8(fout, phi, numTotal)
+    deallocate(phi)
   end subroutine
 
 end program ref_driver

@@ -81,6 +81,46 @@ def small_case(mesh, tag):
     print("wrote", tag)
 
 
+def bc_case():
+    """Boundary handling and the explicit non-orthogonal correction on a
+    jittered 5^3 hex whose six patches carry six different boundary types:
+    updateBoundary.f90 on a random field; the inner-face non-orthogonal
+    correction taken from facefluxsc with zero mass flux; and the
+    calculate_temperature-style assembly with zero mass flux, Dirichlet on the
+    inlet and wall patches and zero-gradient elsewhere (the face-diffusivity
+    Laplacian with a Neumann mask, up to sign)."""
+    types = ("inlet", "outlet", "wall", "symmetry", "pressure", "empty")
+    mesh = hex_mesh(5, jitter=0.2, seed=7, types=types)
+    rc = RefCase(mesh)
+    g, c, sizes = rc.dump()
+    n, nt, F, Fi = mesh.n_cells, mesh.n_total, mesh.n_faces, mesh.n_inner
+    out = {f"g_{k}": v for k, v in g.items()}
+    out.update({f"c_{k}": v for k, v in c.items()})
+    out["sizes"] = np.array(sizes, np.int64)
+    gam = 10.0 ** uniform(6, 1, F, -1.0, 1.0)
+    gr = uniform(6, 2, 3 * nt, -1.0, 1.0)
+    su0 = uniform(6, 3, n, -1.0, 1.0)
+    phi = uniform(6, 4, nt, -1.0, 1.0)
+    patch = mesh.patch_of_bface()
+    bk = np.array([0 if types[p] in ("inlet", "wall") else 1 for p in patch], np.float64)
+    out.update(in_gamma=gam, in_grad=gr.reshape(-1, 3), in_su0=su0, in_phi=phi,
+               in_bckind=bk.astype(np.uint8))
+    for k, v in (("gam", gam), ("gr", gr), ("su0", su0), ("phi", phi), ("bk", bk),
+                 ("zero", np.zeros(F))):
+        rc.put(k + ".bin", v)
+    rc.run(["nonorth gam.bin gr.bin su0.bin no_su.bin",
+            "update_boundary phi.bin ub.bin",
+            "convdiff zero.bin gam.bin phi.bin gr.bin 1.0d0 cds bk.bin ca.bin csu.bin csp.bin"])
+    out["nonorth_su"] = rc.get("no_su.bin")
+    out["update_boundary"] = rc.get("ub.bin")
+    out["cd0_a"] = rc.get("ca.bin")
+    out["cd0_su"] = rc.get("csu.bin")
+    out["cd0_sp"] = rc.get("csp.bin")
+    rc.close()
+    np.savez_compressed(os.path.join(HERE, "hexbc.npz"), **out)
+    print("wrote hexbc")
+
+
 def cavity_case():
     """The reference's own test mesh (test/testFieldOperations/polyMesh).
     The reference's OpenFOAM reader aborts under the flang runtime
@@ -265,6 +305,7 @@ CASES = {
     "hex5j": lambda: small_case(
         hex_mesh(5, jitter=0.2, seed=2,
                  types=("wall", "symmetry", "outlet", "wall", "inlet", "pressure")), "hex5j"),
+    "hexbc": bc_case,
     "cavity": cavity_case,
     "kat5x5": kat_5x5_case,
     "poisson32": poisson32_case,

@@ -121,6 +121,33 @@ def test_face_laplacian_nonorth_update_boundary(api):
     ctx.close()
 
 
+def test_nonorth_update_boundary_laplacian_face_vs_reference(api, golden_dir):
+    """k_nonorth, k_update_boundary and the zero-gradient branch of
+    k_laplacian against the reference's own facefluxsc (zero mass flux),
+    updateBoundary and transport assembly on a mesh with all six non-periodic
+    boundary types (golden fixture hexbc)."""
+    print("[case] nonorth/update_boundary/laplacian_face hexbc", flush=True)
+    mesh = hex_mesh(5, jitter=0.2, seed=7,
+                    types=("inlet", "outlet", "wall", "symmetry", "pressure", "empty"))
+    g = np.load(os.path.join(golden_dir, "hexbc.npz"))
+    ctx, geo, csr = _setup(api, mesh)
+    su = g["in_su0"].copy()
+    ctx.nonorth_correction(g["in_gamma"], g["in_grad"], su)
+    assert np.array_equal(su, g["nonorth_su"])
+    assert np.array_equal(ctx.update_boundary(g["in_phi"].copy()), g["update_boundary"])
+    gam, bk, phi = g["in_gamma"], g["in_bckind"], g["in_phi"]
+    su = np.zeros(mesh.n_cells)
+    ctx.nonorth_correction(gam, g["in_grad"], su)
+    a = np.zeros(csr["ja"].size)
+    ctx.laplacian_face(-gam, bk, phi, su, a_out=a)
+    off = np.ones(a.size, bool)
+    off[csr["diag"]] = False
+    assert np.array_equal(a[off], g["cd0_a"][off])
+    assert np.allclose(a[csr["diag"]], g["cd0_a"][csr["diag"]], rtol=1e-14, atol=0)
+    assert np.allclose(su, g["cd0_su"], rtol=0, atol=1e-14 * np.abs(g["cd0_su"]).max())
+    ctx.close()
+
+
 def test_precond_bitexact(api):
     """DIC and DILU pivots and one preconditioner application are bit-equal
     to the reference loops (level-scheduled sweeps change no arithmetic)."""
