@@ -117,21 +117,21 @@ def run_gpu(args):
     grad = torch.empty((mesh.n_total, 3), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
-    def step(precision):
+    def step(precision, tol=1e-8):
         su.copy_(sV)
         x.zero_()
         torch.cuda.synchronize()
         ctx.laplacian_face(mu_f, bc_kind, phi_bc, su)
         ctx.grad_gauss(phi_prev, grad)
         ctx.nonorth_correction(gamma, grad, su)
-        rc, info = ctx.csrsolve("iccg", x, su, itr_max=20000, tol_abs=0.0, tol_rel=1e-8,
+        rc, info = ctx.csrsolve("iccg", x, su, itr_max=20000, tol_abs=0.0, tol_rel=tol,
                                 precision=precision, criterion=FCP_CRIT_TRUE_L2,
                                 inner_tol=args.inner_tol, check_every=args.check_every)
-        if rc != 0 or not info["rel_res_l2"] <= 1e-8:
-            raise RuntimeError(f"solve did not reach 1e-8: rc={rc} info={info}")
+        if rc != 0 or not info["rel_res_l2"] <= tol:
+            raise RuntimeError(f"solve did not reach {tol:g}: rc={rc} info={info}")
         return info
 
-    results = {}
+    results, xs = {}, {}
     for name, prec in (("fp64", FCP_PREC_FP64), ("mixed", FCP_PREC_MIXED)):
         for _ in range(args.warmup):
             info = step(prec)
@@ -153,7 +153,7 @@ def run_gpu(args):
         log(f"[bench] {name}: {ms:.2f} ms/step, {n/(ms/1e3)/1e6:.2f} Mcells/s, "
             f"iters={infos[-1]['iterations']} outer={infos[-1]['outer_iterations']} "
             f"relres={infos[-1]['rel_res_l2']:.2e}")
-    x_mixed = x.cpu().numpy().copy()
+        xs[name] = x.cpu().numpy().copy()
 
     # error of the mixed solution against a tight FP64 GPU solve
     su.copy_(sV)
@@ -166,53 +166,126 @@ def run_gpu(args):
     ctx.csrsolve("iccg", x_ref, su, itr_max=50000, tol_rel=1e-13, precision=FCP_PREC_FP64,
                  criterion=FCP_CRIT_TRUE_L2)
     xr = x_ref.cpu().numpy()
-    err_mixed = float(np.linalg.norm(x_mixed - xr) / np.linalg.norm(xr))
+    nxr = np.linalg.norm(xr)
+    errs = {k: float(np.linalg.norm(v - xr) / nxr) for k, v in xs.items()}
+    log(f"[bench] error vs the FP64 solve to 1e-13: fp64 {errs['fp64']:.2e} mixed {errs['mixed']:.2e}")
+
+    # the same step solved to ||b-Ax||/||b|| <= 1e-12 (mixed and FP64)
+    tight = {}
+    for name, prec in (("fp64", FCP_PREC_FP64), ("mixed", FCP_PREC_MIXED)):
+        step(prec, 1e-12)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(args.tight_steps):
+            info = step(prec, 1e-12)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - ts) * 1e3 / args.tight_steps
+        err = float(np.linalg.norm(x.cpu().numpy() - xr) / nxr)
+        tight[name] = dict(ms_per_step=ms, cells_per_s=n / (ms / 1e3), iterations=info["iterations"],
+                           outer_iterations=info["outer_iterations"],
+                           rel_res_l2=info["rel_res_l2"], err_vs_fp64_1e13=err)
+        log(f"[bench] {name} to 1e-12: {ms:.2f} ms/step iters={info['iterations']} "
+            f"relres={info['rel_res_l2']:.2e} err={err:.2e}")
+    tight["mixed_over_fp64"] = tight["fp64"]["ms_per_step"] / tight["mixed"]["ms_per_step"]
 
     # per-kernel device time (HIP events on the library stream) over profiled steps
     ctx.profile(True)
+    its = outer = 0
     for _ in range(args.prof_steps):
-        step(FCP_PREC_MIXED)
+        info = step(FCP_PREC_MIXED)
+        its += info["iterations"]
+        outer += info["outer_iterations"]
     prof = ctx.profile_report()
     ctx.profile(False)
+    # Launches that did work: every inner solve applies the preconditioner once
+    # before its first iteration, then once per iteration (tk_pre, tc_zc); the
+    # SpMV runs once per iteration.  Launches queued after the device-side
+    # stopping test exit at once; their few microseconds stay in the time
+    # divided by the working count (a conservative per-launch figure).
+    working = {"tin_pre": its + outer, "tin_coarse": its + outer, "tin_spmv": its}
     total = sum(v[1] for v in prof.values())
-    dom = max(prof.items(), key=lambda kv: kv[1][1])
     log("[bench] kernel profile (mixed, %d steps): " % args.prof_steps +
         ", ".join(f"{k}={v[1]/args.prof_steps:.2f}ms/{v[0]//args.prof_steps}x" for k, v in
                   sorted(prof.items(), key=lambda kv: -kv[1][1])[:8]))
     kernels = {}
     for k, (cnt, tms) in prof.items():
         b = alg_bytes(k, n, nnz, Fi, Fb, local)
-        avg_us = tms / cnt * 1e3
-        kernels[k] = dict(launches_per_step=cnt / args.prof_steps, avg_us=avg_us,
+        wk = min(cnt, working.get(k, cnt))
+        avg_us = tms / wk * 1e3
+        kernels[k] = dict(launches_per_step=cnt / args.prof_steps,
+                          working_launches_per_step=wk / args.prof_steps, avg_us=avg_us,
                           share=tms / total if total else 0.0)
         if b:
             kernels[k]["GBs"] = b / (avg_us * 1e-6) / 1e9
-    name, (cnt, tms) = dom
-    avg_s = tms / cnt / 1e3
+    name, (cnt, tms) = max(prof.items(), key=lambda kv: kv[1][1])
+    wk = min(cnt, working.get(name, cnt))
+    avg_s = tms / wk / 1e3
     b = alg_bytes(name, n, nnz, Fi, Fb, local)
     roof = dict(bound="hbm", kernel=name, achieved=(b / avg_s / 1e9) if b else None,
                 peak=HBM_PEAK_GBS, unit="GB/s", frac=(b / avg_s / 1e9 / HBM_PEAK_GBS) if b else None,
                 traffic=measured_traffic(name, N), alg_bytes_per_launch=b,
-                avg_launch_us=avg_s * 1e6)
+                avg_launch_us=avg_s * 1e6, launches=cnt, working_launches=wk)
 
-    # standalone SpMV FP64 (reference row order) bandwidth
-    xs = torch.rand(n, dtype=torch.float64, device=dev)
+    # standalone SpMV FP64 (reference row order) bandwidth; the 128^3 operator
+    # and vectors (209 MB) fit in the 256 MB Infinity Cache
+    spmv = spmv_rate(ctx, n, nnz, Fi, Fb)
+    if spmv:
+        spmv["note"] = "operator + vectors fit in the 256 MB Infinity Cache"
+    ctx.close()
+    out = dict(n=n, nnz=nnz, Fi=Fi, results=results, errs=errs, tight=tight,
+               roofline=roof, kernels=kernels, spmv=spmv)
+    if args.spmv_size:
+        out["spmv_big"] = run_spmv_big(args)
+    return out
+
+
+def spmv_rate(ctx, n, nnz, Fi, Fb, reps=20):
+    """Average device time and algorithmic bandwidth of the FP64 SELL SpMV
+    (reference row order), two input vectors alternated."""
+    import torch
+    dev = torch.device("cuda:0")
+    xs = [torch.rand(n, dtype=torch.float64, device=dev) for _ in range(2)]
     ys = torch.empty(n, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     ctx.profile(True)
-    for _ in range(20):
-        ctx.spmv(xs, ys)
+    for k in range(reps):
+        ctx.spmv(xs[k & 1], ys)
     sp = ctx.profile_report().get("spmv_sell_f64")
     ctx.profile(False)
-    spmv = None
-    if sp:
-        avg = sp[1] / sp[0] / 1e3
-        bb = alg_bytes("spmv_sell_f64", n, nnz, Fi, Fb)
-        spmv = dict(kernel="spmv_sell_f64", avg_us=avg * 1e6, GBs=bb / avg / 1e9,
-                    frac=bb / avg / 1e9 / HBM_PEAK_GBS, alg_bytes=bb)
+    if not sp:
+        return None
+    avg = sp[1] / sp[0] / 1e3
+    bb = alg_bytes("spmv_sell_f64", n, nnz, Fi, Fb)
+    return dict(kernel="spmv_sell_f64", avg_us=avg * 1e6, GBs=bb / avg / 1e9,
+                frac=bb / avg / 1e9 / HBM_PEAK_GBS, alg_bytes=bb, launches=sp[0])
+
+
+def run_spmv_big(args):
+    """The FP64 SpMV on the same kind of operator at a size whose operator
+    alone exceeds the Infinity Cache (192^3: 733 MB of algorithmic bytes)."""
+    import torch
+
+    from fcp import workloads
+    from fcp.api import Context
+
+    N = args.spmv_size
+    mesh = workloads.poisson_jittered_mesh(N)
+    ctx = Context(0)
+    ctx.read_mesh(mesh)
+    geo = ctx.geometry()
+    ctx.create_CSR_matrix(want_arrays=False)
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    dev = torch.device("cuda:0")
+    T = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
+    su = T(inp["source"] * geo["vol"])
+    ctx.laplacian_face(T(inp["mu_f"]), T(inp["bc_kind"], torch.uint8), T(inp["phi_bc"]), su)
+    out = spmv_rate(ctx, mesh.n_cells, ctx.nnz, mesh.n_inner, mesh.n_bnd)
+    if out:
+        out.update(cells=mesh.n_cells, nnz=ctx.nnz, size=f"{N}^3")
+        log(f"[bench] SpMV FP64 {N}^3: {out['avg_us']:.1f} us, {out['GBs']:.0f} GB/s "
+            f"({100 * out['frac']:.1f}% of peak)")
     ctx.close()
-    return dict(n=n, nnz=nnz, Fi=Fi, results=results, err_mixed_vs_fp64_tight=err_mixed,
-                roofline=roof, kernels=kernels, spmv=spmv)
+    return out
 
 
 def _timed(fn, reps=1):
@@ -516,15 +589,32 @@ def run_cpu_baseline(args, n_gpu_cells):
     a, su = o.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], inp["source"] * g["vol"])
     gr = o.grad_gauss(inp["phi_prev"])
     su = o.nonorth_correction(inp["gamma"], gr, su)
-    out = o.solve("iccg", a, su, itr_max=20000, tol_abs=0.0, tol_rel=1e-8)
+    # the reference's DIC-PCG iterations, stopped on the GPU legs' test
+    # ||b - A x||_2 <= 1e-8 ||b||_2 (x0 = 0) through the recursive residual
+    out = o.solve("iccg-l2", a, su, itr_max=20000, tol_abs=0.0, tol_rel=1e-8)
     t1 = time.perf_counter()
     rel = float(np.linalg.norm(su - o.spmv(a, out["fi"])) / np.linalg.norm(su))
     return dict(value=mesh.n_cells / (t1 - t0), unit="cells/s", cores=1, kind="port",
                 sample=f"same workload at {N}^3 ({mesh.n_cells} cells): oracle laplacian_face + "
-                       f"grad_gauss + non-orthogonal correction + ICCG (reference L1 criterion "
-                       f"tol_rel 1e-8, reached ||b-Ax||/||b|| = {rel:.1e}) in {t1-t0:.1f} s, "
-                       f"{out['iters']} iterations",
-                host=platform.processor() or platform.machine())
+                       f"grad_gauss + non-orthogonal correction + ICCG stopped at "
+                       f"||r||_2/||b||_2 <= 1e-8 (true ||b-Ax||/||b|| = {rel:.1e}) in "
+                       f"{t1-t0:.1f} s, {out['iters']} iterations, one thread",
+                host=host_cpu())
+
+
+def host_cpu():
+    """CPU model and logical CPU count of this host."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None}
 
 
 def main():
@@ -544,6 +634,10 @@ def main():
                     choices=["poisson", "convdiff", "tets", "adjoint"],
                     help="poisson (default, the headline) or one of the secondary workloads")
     ap.add_argument("--fd-dirs", type=int, default=8)
+    ap.add_argument("--tight-steps", type=int, default=2,
+                    help="timed steps of the 1e-12 measurement point")
+    ap.add_argument("--spmv-size", type=int, default=192,
+                    help="mesh size of the SpMV measured beyond the 256 MB Infinity Cache (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -577,12 +671,15 @@ def main():
                                "correction + DIC-PCG to ||b-Ax||/||b||<=1e-8",
                    "cells": g["n"], "nnz": g["nnz"], "inner_faces": g["Fi"]},
         "fp64": {"cells_per_s": fp64["cells_per_s"], "ms_per_step": fp64["ms_per_step"],
-                 "iterations": fp64["iterations"]},
+                 "iterations": fp64["iterations"], "rel_res_l2": fp64["rel_res_l2"],
+                 "err_vs_fp64_1e-13": g["errs"]["fp64"]},
         "mixed": {"iterations": mixed["iterations"], "outer_iterations": mixed["outer_iterations"],
                   "rel_res_l2": mixed["rel_res_l2"],
-                  "err_vs_fp64_1e-13": g["err_mixed_vs_fp64_tight"]},
+                  "err_vs_fp64_1e-13": g["errs"]["mixed"]},
         "mixed_over_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
+        "tol_1e-12": g["tight"],
         "spmv": g["spmv"],
+        "spmv_beyond_mall": g.get("spmv_big"),
         "roofline": g["roofline"],
         "cpu_baseline": cpu,
         "kernels": g["kernels"],

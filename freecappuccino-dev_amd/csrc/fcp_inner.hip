@@ -1932,12 +1932,13 @@ int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, S
                                                ctx->partials.p, redctr(ctx), si, tol_rel,
                                                inner_tol, budget, nullptr, nullptr);
   FCP_TRY((tk_launch_pre<NR, NRP>(ctx, B, B.p[0], si, 0)));
-  const int ce = check_every > 0 ? check_every : 8;
+  BatchSizer bs(check_every, BATCH_INNER, 0.0);
+  int next = bs.cap;
   hipGraphExec_t ge = nullptr;
   if (ctx->tune_graph && !ctx->prof) FCP_TRY((tk_graph<NR, NRP>(ctx, B, si, &ge)));
   int launched = 0, k = 0;
   while (launched < budget) {
-    const int batch = (budget - launched) < ce ? (budget - launched) : ce;
+    const int batch = (budget - launched) < next ? (budget - launched) : next;
     int j = 0;
     if (ge && (k & 1) == 0)  // the graph starts from an even iteration
       for (; j + 2 <= batch; j += 2, k += 2) FCP_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
@@ -1955,6 +1956,7 @@ int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, S
       all = all && h[c].done;
     }
     if (all) break;
+    next = bs.next(h, NR);
   }
   tk_ir_update<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->tin_iperm.p, B.x, so, x);
   FCP_HIP(ctx, hipGetLastError());
@@ -2003,10 +2005,11 @@ int tb_bicgstab(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *
         MODE == 0 ? U : Vv, ctx->partials.p, redctr(ctx), si);
   };
   bet(0);
-  const int ce = check_every > 0 ? check_every : 8;
+  BatchSizer bs(check_every, BATCH_INNER, 0.0);
+  int next = bs.cap;
   int launched = 0;
   while (launched < budget) {
-    const int batch = (budget - launched) < ce ? (budget - launched) : ce;
+    const int batch = (budget - launched) < next ? (budget - launched) : next;
     for (int j = 0; j < batch; ++j) {
       tile(std::integral_constant<int, 1>());
       spmv(std::integral_constant<int, 0>());
@@ -2028,6 +2031,7 @@ int tb_bicgstab(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *
       all = all && h[c].done;
     }
     if (all) break;
+    next = bs.next(h, NR);
   }
   tk_ir_update<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->tin_iperm.p, X, so, x);
   FCP_HIP(ctx, hipGetLastError());

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -95,6 +96,55 @@ struct SolveScalars {
   double tol;  // per-column relative tolerance of a batched inner solve
   int iter, done, status, conv;  // conv: the stopping test was met
   int imax;                      // iteration limit of a tile-order inner solve
+};
+
+// Which residual ratio a polling loop watches: the reference's L1 ratio rsm
+// against tol_rel, the L2 ratio l2/l2b against tol_rel, or the L2 ratio
+// against each column's own inner tolerance S.tol.
+enum { BATCH_L1 = 0, BATCH_L2 = 1, BATCH_INNER = 2 };
+
+// Length of the next batch of iterations launched between two host polls of
+// the device scalars.  At each poll, every unfinished column's residual ratio
+// and iteration count are compared with the previous poll; the geometric rate
+// per iteration since then predicts how many more iterations the column needs
+// to meet its stopping test.  The batch covers the slowest column plus one
+// iteration (an iteration's stopping test runs in the first kernel of the
+// next), within [1, cap]; without a usable rate (first poll, a residual that
+// did not decrease) it is cap.  Iterations launched after the device set
+// `done` do no work, so a short estimate costs one more poll and a long one
+// only launches that exit at once.
+struct BatchSizer {
+  static constexpr int MAXC = 8;
+  int cap, mode;
+  double tol;
+  double rho[MAXC];
+  int it[MAXC];
+  BatchSizer(int cap_, int mode_, double tol_) : cap(cap_ > 0 ? cap_ : 8), mode(mode_), tol(tol_) {
+    for (int c = 0; c < MAXC; ++c) {
+      rho[c] = 0.0;
+      it[c] = -1;
+    }
+  }
+  // Batch length after a poll of ncol columns' host copies h.
+  int next(const SolveScalars *h, int ncol) {
+    int out = 1;
+    for (int c = 0; c < ncol && c < MAXC; ++c) {
+      const SolveScalars &S = h[c];
+      if (S.done) continue;
+      const double ratio = mode == BATCH_L1 ? S.rsm : (S.l2b > 0.0 ? S.l2 / S.l2b : 0.0);
+      const double target = mode == BATCH_INNER ? S.tol : tol;
+      int est = cap;
+      if (it[c] >= 0 && S.iter > it[c] && ratio > 0.0 && ratio < rho[c] && target > 0.0) {
+        const double q = log(ratio / rho[c]) / (double)(S.iter - it[c]);
+        const double need = target < ratio ? log(target / ratio) / q : 0.0;
+        est = need < (double)cap ? (int)ceil(need) + 1 : cap;
+      }
+      rho[c] = ratio;
+      it[c] = S.iter;
+      if (est > out) out = est;
+    }
+    return out > cap ? cap : out;
+  }
 };
 
 struct ProfEntry {

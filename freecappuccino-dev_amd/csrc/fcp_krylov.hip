@@ -1380,11 +1380,12 @@ int bicgstab_iteration(fcp_ctx *ctx, const T *a, const T *d, T *x, T *r, T *reso
 // Runs iterations until the device reports done; returns 0 or error.
 template <typename F>
 int iterate_until_done(fcp_ctx *ctx, SolveScalars *s_dev, int slot, int itr_max, int check_every,
-                       int verbose, const char *tag, F &&one_iteration) {
+                       int verbose, const char *tag, int mode, double tol, F &&one_iteration) {
   int launched = 0;
-  const int ce = check_every > 0 ? check_every : 8;
+  BatchSizer bs(check_every, mode, tol);
+  int next = bs.cap;
   while (launched < itr_max) {
-    const int batch = (itr_max - launched) < ce ? (itr_max - launched) : ce;
+    const int batch = (itr_max - launched) < next ? (itr_max - launched) : next;
     for (int k = 0; k < batch; ++k) FCP_TRY(one_iteration());
     launched += batch;
     FCP_HIP(ctx, hipMemcpyAsync(&ctx->h_scal[slot], s_dev, sizeof(SolveScalars),
@@ -1395,6 +1396,7 @@ int iterate_until_done(fcp_ctx *ctx, SolveScalars *s_dev, int slot, int itr_max,
       printf("                   %s iter = %4d resl = %10.3E rsm = %10.3E\n", tag,
              ctx->h_scal[slot].iter, ctx->h_scal[slot].resl, ctx->h_scal[slot].rsm);
     if (ctx->h_scal[slot].done) break;
+    next = bs.next(&ctx->h_scal[slot], 1);
   }
   return 0;
 }
@@ -1437,7 +1439,8 @@ int solve_fp64(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max
       FCP_HIP(ctx, hipGetLastError());
       return 0;
     };
-    return iterate_until_done(ctx, s, 0, itr_max, check_every, verbose, "GS", it);
+    return iterate_until_done(ctx, s, 0, itr_max, check_every, verbose, "GS",
+                              crit == 0 ? BATCH_L1 : BATCH_L2, tol_rel, it);
   }
   if (ctx->h_scal[0].res0 < tol_abs) return 0;  // :428-432
   if (verbose >= 2) printf("                    res0 = %10.3E\n", ctx->h_scal[0].res0);
@@ -1455,14 +1458,16 @@ int solve_fp64(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max
                                    ctx->zk.p, ctx->yk.p, ctx->qk.p, s, crit, tol_abs, tol_rel,
                                    itr_max);
     };
-    FCP_TRY(iterate_until_done(ctx, s, 0, itr_max, check_every, verbose, "PCG", it));
+    FCP_TRY(iterate_until_done(ctx, s, 0, itr_max, check_every, verbose, "PCG",
+                               crit == 0 ? BATCH_L1 : BATCH_L2, tol_rel, it));
   } else {
     auto it = [&]() {
       return bicgstab_iteration<double>(ctx, a, ctx->d.p, x, ctx->res.p, ctx->reso.p, ctx->pk.p,
                                         ctx->zk.p, ctx->yk.p, ctx->uk.p, ctx->vk.p, s, crit,
                                         tol_abs, tol_rel, itr_max);
     };
-    FCP_TRY(iterate_until_done(ctx, s, 0, itr_max, check_every, verbose, "BiCGStab", it));
+    FCP_TRY(iterate_until_done(ctx, s, 0, itr_max, check_every, verbose, "BiCGStab",
+                               crit == 0 ? BATCH_L1 : BATCH_L2, tol_rel, it));
   }
   return 0;
 }
@@ -1584,14 +1589,16 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
                                          ctx->reso32.p, ctx->p32.p, ctx->z32.p, ctx->y32.p,
                                          ctx->u32.p, ctx->v32.p, si, 1, 0.0, itol, budget);
       };
-      FCP_TRY(iterate_until_done(ctx, si, 1, budget, check_every, verbose, "BiCGStab32", it));
+      FCP_TRY(iterate_until_done(ctx, si, 1, budget, check_every, verbose, "BiCGStab32",
+                                 BATCH_L2, itol, it));
     } else {
       auto it = [&]() {
         return pcg_iteration<float>(ctx, solver == 1 ? 1 : 0, ctx->a32.p, ctx->d32.p, ctx->e32.p,
                                     ctx->r32.p, ctx->p32.p, ctx->z32.p, ctx->y32.p, ctx->q32.p, si,
                                     1, 0.0, itol, budget);
       };
-      FCP_TRY(iterate_until_done(ctx, si, 1, budget, check_every, verbose, "PCG32", it));
+      FCP_TRY(iterate_until_done(ctx, si, 1, budget, check_every, verbose, "PCG32",
+                                 BATCH_L2, itol, it));
     }
     total_inner += ctx->h_scal[1].iter;
     k_ir_update<<<g, BLK, 0, ctx->stream>>>(n, ctx->e32.p, so, x);

@@ -1011,11 +1011,12 @@ int mbicgstab_iteration(fcp_ctx *ctx, const T *a, MVecs<T> &V, SolveScalars *s, 
 // Launch iterations in batches, polling the NR device scalars in between.
 template <int NR, typename F>
 int miterate(fcp_ctx *ctx, SolveScalars *s_dev, SolveScalars *h, int itr_max, int check_every,
-             int verbose, const char *tag, F &&one) {
+             int verbose, const char *tag, int mode, double tol, F &&one) {
   int launched = 0;
-  const int ce = check_every > 0 ? check_every : 8;
+  BatchSizer bs(check_every, mode, tol);
+  int next = bs.cap;
   while (launched < itr_max) {
-    const int batch = (itr_max - launched) < ce ? (itr_max - launched) : ce;
+    const int batch = (itr_max - launched) < next ? (itr_max - launched) : next;
     for (int k = 0; k < batch; ++k) FCP_TRY(one());
     launched += batch;
     FCP_HIP(ctx, hipMemcpyAsync(h, s_dev, NR * sizeof(SolveScalars), hipMemcpyDeviceToHost,
@@ -1030,6 +1031,7 @@ int miterate(fcp_ctx *ctx, SolveScalars *s_dev, SolveScalars *h, int itr_max, in
       all = all && h[c].done;
     }
     if (all) break;
+    next = bs.next(h, NR);
   }
   return 0;
 }
@@ -1089,13 +1091,15 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
         return mbicgstab_iteration<double, NR, NRP>(ctx, ctx->a.p, V64, so, crit, tol_abs,
                                                     tol_rel, itr_max);
       };
-      FCP_TRY(miterate<NR>(ctx, so, h, itr_max, check_every, verbose, "BiCGStab", it));
+      FCP_TRY(miterate<NR>(ctx, so, h, itr_max, check_every, verbose, "BiCGStab",
+                           crit == 0 ? BATCH_L1 : BATCH_L2, tol_rel, it));
     } else {
       auto it = [&]() {
         return mpcg_iteration<double, NR, NRP>(ctx, 0, ctx->a.p, V64, so, crit, tol_abs, tol_rel,
                                                itr_max);
       };
-      FCP_TRY(miterate<NR>(ctx, so, h, itr_max, check_every, verbose, "PCG", it));
+      FCP_TRY(miterate<NR>(ctx, so, h, itr_max, check_every, verbose, "PCG",
+                           crit == 0 ? BATCH_L1 : BATCH_L2, tol_rel, it));
     }
     for (int c = 0; c < NR; ++c) {
       res.iters[c] = h[c].iter;
@@ -1153,13 +1157,15 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
           return mbicgstab_iteration<double, NR, NRP>(ctx, ctx->a.p, V64, so, 1, tol_abs, tol_rel,
                                                       itr_max - total);
         };
-        FCP_TRY(miterate<NR>(ctx, so, h, itr_max - total, check_every, verbose, "BiCGStab", it));
+        FCP_TRY(miterate<NR>(ctx, so, h, itr_max - total, check_every, verbose, "BiCGStab",
+                             BATCH_L2, tol_rel, it));
       } else {
         auto it = [&]() {
           return mpcg_iteration<double, NR, NRP>(ctx, 0, ctx->a.p, V64, so, 1, tol_abs, tol_rel,
                                                  itr_max - total);
         };
-        FCP_TRY(miterate<NR>(ctx, so, h, itr_max - total, check_every, verbose, "PCG", it));
+        FCP_TRY(miterate<NR>(ctx, so, h, itr_max - total, check_every, verbose, "PCG",
+                             BATCH_L2, tol_rel, it));
       }
       int mx = 0;
       for (int c = 0; c < NR; ++c) {
@@ -1198,12 +1204,14 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
       auto it = [&]() {
         return mbicgstab_iteration<float, NR, NRP>(ctx, ctx->a32.p, V32, si, 1, 0.0, itol, budget);
       };
-      FCP_TRY(miterate<NR>(ctx, si, h, budget, check_every, verbose, "BiCGStab32", it));
+      FCP_TRY(miterate<NR>(ctx, si, h, budget, check_every, verbose, "BiCGStab32",
+                             BATCH_INNER, 0.0, it));
     } else {
       auto it = [&]() {
         return mpcg_iteration<float, NR, NRP>(ctx, 0, ctx->a32.p, V32, si, 1, 0.0, itol, budget);
       };
-      FCP_TRY(miterate<NR>(ctx, si, h, budget, check_every, verbose, "PCG32", it));
+      FCP_TRY(miterate<NR>(ctx, si, h, budget, check_every, verbose, "PCG32",
+                             BATCH_INNER, 0.0, it));
     }
     int mx = 0;
     for (int c = 0; c < NR; ++c) {

@@ -75,6 +75,13 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libfcp.so not found at {LIB_PATH}; run __graft_entry__.build()")
+        # PyTorch ships its own HIP and HSA runtime libraries.  Loaded first,
+        # they also serve libfcp (same sonames), so both share one runtime and
+        # exchange device pointers; loaded after libfcp's, torch finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

@@ -97,43 +97,58 @@ class ConvDiffProblem:
             x = x + self._solve_once(r, 1e-6)
         return x
 
-    def forward(self, gamma, s, phib):
-        """phi_K (cells) for parameters gamma[F], s[n], phib[F_b] (device)."""
+    def _assemble(self, gamma, phi_old, g, su):
+        """A (resident in the context) and b (into su) of one pass."""
+        sp = torch.zeros(self.n, dtype=torch.float64, device=self.dev)
+        work = phi_old.clone()  # fcp_convdiff rewrites zero-gradient boundary entries
+        self.ctx.convdiff(self.flux, gamma, work, g, self.gam, self.scheme, self.bc_kind, su, sp,
+                          urf=1.0)
+
+    def run(self, gamma, s, phib):
+        """(phi_K, tape): the cells' solution for parameters gamma[F], s[n],
+        phib[F_b] (device) and the per-call record the reverse sweep needs
+        (gamma and every pass's input field, gradient and solution)."""
         n, nt = self.n, self.nt
-        gamma = gamma.contiguous()
+        gamma = gamma.detach().contiguous().clone()
         phi_int = torch.zeros(n, dtype=torch.float64, device=self.dev)
-        self.states = []
+        states = []
         for _ in range(self.K):
             phi_old = self._full(phi_int, phib)
             g = torch.zeros((nt, 3), dtype=torch.float64, device=self.dev)
             self.ctx.grad_gauss(phi_old, out=g)
             su = s * self.vol
-            sp = torch.zeros(n, dtype=torch.float64, device=self.dev)
-            work = phi_old.clone()  # fcp_convdiff rewrites zero-gradient boundary entries
-            self.ctx.convdiff(self.flux, gamma, work, g, self.gam, self.scheme, self.bc_kind, su, sp,
-                              urf=1.0)
+            self._assemble(gamma, phi_old, g, su)
             phi_int = self._solve(su)
-            self.states.append((phi_old, g, phi_int))
-        self.gamma = gamma
-        return phi_int
+            states.append((phi_old, g, phi_int))
+        return phi_int, {"gamma": gamma, "states": states}
 
-    def backward(self, phibar):
-        """(gamma_bar[F], s_bar[n], phib_bar[F_b]) for dJ/dphi_K = phibar."""
+    def forward(self, gamma, s, phib):
+        """phi_K (cells) for parameters gamma[F], s[n], phib[F_b] (device)."""
+        return self.run(gamma, s, phib)[0]
+
+    def backward(self, phibar, tape):
+        """(gamma_bar[F], s_bar[n], phib_bar[F_b]) for dJ/dphi_K = phibar at
+        the forward call that recorded `tape`.  The context's matrix may have
+        changed since (other forward calls, other assemblies), so A is first
+        re-assembled from the taped gamma, then transposed in place."""
         n, nt = self.n, self.nt
         F = self.mesh.n_faces
+        gamma, states = tape["gamma"], tape["states"]
         gamma_bar = torch.zeros(F, dtype=torch.float64, device=self.dev)
         s_bar = torch.zeros(n, dtype=torch.float64, device=self.dev)
         phib_bar = torch.zeros(nt - n, dtype=torch.float64, device=self.dev)
         src = phibar.contiguous()
+        self._assemble(gamma, states[0][0], states[0][1],
+                       torch.zeros(n, dtype=torch.float64, device=self.dev))
         self.ctx.matrix_transpose()
         try:
             for k in reversed(range(self.K)):
-                phi_old, g, phi_new = self.states[k]
+                phi_old, g, phi_new = states[k]
                 lam = self._solve(src)
                 s_bar += lam * self.vol
                 pob = torch.zeros(nt, dtype=torch.float64, device=self.dev)
                 gob = torch.zeros((nt, 3), dtype=torch.float64, device=self.dev)
-                self.ctx.convdiff_backward(self.flux, self.gamma, phi_old, g, self.gam,
+                self.ctx.convdiff_backward(self.flux, gamma, phi_old, g, self.gam,
                                            self.scheme, self.bc_kind, lam, phi_new, gamma_bar,
                                            pob, gob)
                 self.ctx.grad_gauss_backward(gob, pob)
@@ -152,10 +167,12 @@ class ConvDiffSolve(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, gamma, s, phib, problem):
+        phi, tape = problem.run(gamma, s, phib)
         ctx.problem = problem
-        return problem.forward(gamma, s, phib)
+        ctx.tape = tape
+        return phi
 
     @staticmethod
     def backward(ctx, phibar):
-        g, s, b = ctx.problem.backward(phibar)
+        g, s, b = ctx.problem.backward(phibar, ctx.tape)
         return g, s, b, None

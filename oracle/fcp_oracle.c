@@ -609,9 +609,13 @@ void orc_dic_apply(const orc_csr *c, const double *a, const double *d,
 /* ------------------------------------------------------------------------ */
 /* Solvers: linear_solvers.f90                                              */
 /* ------------------------------------------------------------------------ */
-int orc_iccg(const orc_csr *c, const double *a, double *fi, const double *rhs,
-             int itr_max, double tol_abs, double tol_rel, double *hist,
-             orc_solve_info *info) {
+/* crit 0: the reference's L1 test rsm < tol_rel (:528-532); crit 1: the
+ * recursively updated residual's L2 norm against tol_rel * ||r0||_2 (the
+ * stopping test of the mixed-precision GPU solves, used only for the CPU
+ * baseline timing). */
+static int iccg_impl(const orc_csr *c, const double *a, double *fi, const double *rhs,
+                     int itr_max, double tol_abs, double tol_rel, double *hist,
+                     orc_solve_info *info, int crit) {
   const int n = c->n;
   double *res = (double *)calloc((size_t)n, sizeof(double));
   double *pk = (double *)calloc((size_t)n, sizeof(double));
@@ -621,6 +625,7 @@ int orc_iccg(const orc_csr *c, const double *a, double *fi, const double *rhs,
   int itr_used = 0;
   residual(c, a, fi, rhs, res);                 /* :417-422 */
   const double res0 = l1(res, n);               /* :425     */
+  const double l2r0 = sqrt(dot(res, res, n));
   if (res0 < tol_abs) {                         /* :428-432 */
     resor = res0;
     resl = res0;
@@ -647,7 +652,8 @@ int orc_iccg(const orc_csr *c, const double *a, double *fi, const double *rhs,
       resor = res0 / factor;
     }
     const double rsm = resl / (res0 + SMALL);
-    if (rsm < tol_rel || resl < tol_abs) break;
+    if (crit == 0 && (rsm < tol_rel || resl < tol_abs)) break;
+    if (crit == 1 && (sqrt(dot(res, res, n)) <= tol_rel * l2r0 || resl < tol_abs)) break;
   }
 done:
   if (info) {
@@ -656,6 +662,18 @@ done:
   }
   free(res); free(pk); free(zk); free(d);
   return itr_used;
+}
+
+int orc_iccg(const orc_csr *c, const double *a, double *fi, const double *rhs,
+             int itr_max, double tol_abs, double tol_rel, double *hist,
+             orc_solve_info *info) {
+  return iccg_impl(c, a, fi, rhs, itr_max, tol_abs, tol_rel, hist, info, 0);
+}
+
+int orc_iccg_l2(const orc_csr *c, const double *a, double *fi, const double *rhs,
+                int itr_max, double tol_abs, double tol_rel, double *hist,
+                orc_solve_info *info) {
+  return iccg_impl(c, a, fi, rhs, itr_max, tol_abs, tol_rel, hist, info, 1);
 }
 
 int orc_dpcg(const orc_csr *c, const double *a, double *fi, const double *rhs,

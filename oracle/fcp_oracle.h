@@ -104,6 +104,11 @@ typedef struct {
 int orc_iccg(const orc_csr *c, const double *a, double *fi, const double *rhs,
              int itr_max, double tol_abs, double tol_rel, double *hist,
              orc_solve_info *info);
+/* orc_iccg with the stopping test ||r||_2 <= tol_rel ||r0||_2 on the
+ * recursively updated residual instead of the reference's L1 ratio. */
+int orc_iccg_l2(const orc_csr *c, const double *a, double *fi, const double *rhs,
+                int itr_max, double tol_abs, double tol_rel, double *hist,
+                orc_solve_info *info);
 int orc_dpcg(const orc_csr *c, const double *a, double *fi, const double *rhs,
              int itr_max, double tol_abs, double tol_rel, double *hist,
              orc_solve_info *info);

@@ -222,7 +222,7 @@ class Oracle:
 
 
 _SOLVERS = {"iccg": "orc_iccg", "dpcg": "orc_dpcg", "bicgstab": "orc_bicgstab",
-            "gauss-seidel": "orc_gauss_seidel"}
+            "gauss-seidel": "orc_gauss_seidel", "iccg-l2": "orc_iccg_l2"}
 
 
 def csr_struct(ia, ja, diag):

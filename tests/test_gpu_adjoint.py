@@ -127,3 +127,35 @@ def test_adjoint_vs_cpu_reference_gradient():
         print(f"{name}: max |gpu-cpu|/max|cpu| = {err:.2e}", flush=True)
         assert err < 1e-5, name
     ctx.close()
+
+
+def test_adjoint_two_calls_before_backward():
+    """Two forward calls with different face diffusivity before either
+    backward: each backward must return the gradients of its own call (the
+    taped gamma is re-assembled before the transpose), equal to a forward and
+    backward of that call alone."""
+    import torch
+    from fcp.adjoint import ConvDiffSolve
+    print("[case] adjoint two calls before backward 6^3", flush=True)
+    mesh, ctx, geo, csr, flux, inp, prob, P, vol = _problem(6)
+    gammas = [P["gamma"], P["gamma"] * (1.5 + 0.5 * torch.cos(torch.arange(
+        P["gamma"].numel(), device=P["gamma"].device, dtype=torch.float64)))]
+
+    alone = []
+    for gm in gammas:
+        g = gm.clone().requires_grad_(True)
+        s = P["source"].clone().requires_grad_(True)
+        b = P["phib"].clone().requires_grad_(True)
+        (P["w"] * ConvDiffSolve.apply(g, s, b, prob) * vol).sum().backward()
+        alone.append((g.grad.clone(), s.grad.clone(), b.grad.clone()))
+    leaves = [[t.clone().requires_grad_(True) for t in (gm, P["source"], P["phib"])]
+              for gm in gammas]
+    Js = [(P["w"] * ConvDiffSolve.apply(*lv, prob) * vol).sum() for lv in leaves]
+    # backward of the earlier call after the later call replaced the matrix
+    for k in (0, 1):
+        Js[k].backward()
+        for t, ref in zip(leaves[k], alone[k]):
+            assert torch.allclose(t.grad, ref, rtol=1e-9, atol=1e-12 * float(ref.abs().max())), k
+    # the two calls' gradients differ (the test would not see a mix-up otherwise)
+    assert not torch.allclose(alone[0][0], alone[1][0], rtol=1e-3)
+    ctx.close()

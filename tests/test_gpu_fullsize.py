@@ -2,7 +2,8 @@
 cells, the benchmark workload), through size-independent properties:
 
 - the mixed-precision and the FP64 ICCG reach the requested true relative
-  residual and agree with each other within what that residual allows;
+  residual, and the mixed solution's error against a tight FP64 solve stays
+  within three times the FP64 solution's own error at the same tolerance;
 - the tile-local DIC of the FP32 inner solves is a symmetric operator
   (r2 . M^-1 r1 = r1 . M^-1 r2) in FP64 and, within FP32 rounding, in the
   FP32 tile-order kernel;
@@ -39,20 +40,32 @@ def system():
     ctx.close()
 
 
-def test_fullsize_mixed_and_fp64_reach_tolerance_and_agree(system):
+def test_fullsize_mixed_error_within_fp64_error(system):
+    """Mixed and FP64 ICCG reach the requested true relative residual, and the
+    mixed solution's error against a tight FP64 solve (1e-13) is at most three
+    times the FP64 solution's own error at the same tolerance (1e-8 and
+    1e-12): the FP32 inner iterations cost no accuracy beyond what the
+    stopping test allows."""
     api, mesh, ctx, su = system
-    print(f"[case] full size {N}^3 mixed vs fp64", flush=True)
-    sols = {}
-    for prec in (api.FCP_PREC_MIXED, api.FCP_PREC_FP64):
+    print(f"[case] full size {N}^3 mixed vs fp64 errors", flush=True)
+
+    def solve(prec, tol):
         x = np.zeros(mesh.n_cells)
-        rc, info = ctx.csrsolve("iccg", x, su.copy(), itr_max=20000, tol_abs=0.0, tol_rel=1e-8,
+        rc, info = ctx.csrsolve("iccg", x, su.copy(), itr_max=50000, tol_abs=0.0, tol_rel=tol,
                                 precision=prec, criterion=api.FCP_CRIT_TRUE_L2, check_every=16)
-        assert rc == 0 and info["rel_res_l2"] <= 1e-8, (prec, info)
-        sols[prec] = x
-    xm, xd = sols[api.FCP_PREC_MIXED], sols[api.FCP_PREC_FP64]
-    # both residuals are <= 1e-8 of |b|; the solutions differ by at most
-    # cond(A) times that, far below 1e-3 for this operator
-    assert np.linalg.norm(xm - xd) / np.linalg.norm(xd) < 1e-4
+        assert rc == 0 and info["rel_res_l2"] <= tol, (prec, tol, info)
+        return x
+
+    xr = solve(api.FCP_PREC_FP64, 1e-13)
+    nr = np.linalg.norm(xr)
+    err = {}
+    for tol in (1e-8, 1e-12):
+        for prec in (api.FCP_PREC_MIXED, api.FCP_PREC_FP64):
+            err[prec, tol] = np.linalg.norm(solve(prec, tol) - xr) / nr
+        print(f"[case] tol {tol:g}: err mixed {err[api.FCP_PREC_MIXED, tol]:.3e} "
+              f"fp64 {err[api.FCP_PREC_FP64, tol]:.3e}", flush=True)
+        assert err[api.FCP_PREC_MIXED, tol] <= 3.0 * err[api.FCP_PREC_FP64, tol], (tol, err)
+    assert err[api.FCP_PREC_MIXED, 1e-12] <= 1e-9, err
 
 
 @pytest.mark.parametrize("kind,tol", [(3, 1e-12), (5, 1e-5)])

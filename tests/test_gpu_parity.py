@@ -605,6 +605,57 @@ def test_coarse_correction_matches_dense_solve(api, n, tile):
     ctx.close()
 
 
+def test_coarse_factorisation_after_reassembly(api):
+    """A second assembly on a context whose inner PCG already built the coarse
+    adjacency starts the coarse factorisation on the side stream.  The mixed
+    solve that follows must equal, bit for bit and in inner iterations, the
+    same solve on a fresh context that assembled only the second matrix; and
+    the coarse correction applied after the re-assembly must be the dense
+    solve of the second matrix's coarse operator."""
+    print("[case] coarse factorisation after re-assembly 40^3", flush=True)
+    import scipy.sparse as sp
+    mesh = workloads.poisson_jittered_mesh(40)
+    ctx, geo, csr = _setup(api, mesh)
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    nn = mesh.n_cells
+    mu1 = inp["mu_f"]
+    mu2 = inp["mu_f"] * (1.0 + 0.5 * np.sin(np.arange(mesh.n_faces) * 0.7))
+
+    def solve(c, mu):
+        su = inp["source"] * geo["vol"]
+        a = np.zeros(csr["ja"].size)
+        c.laplacian_face(mu, inp["bc_kind"], inp["phi_bc"], su, a_out=a)
+        x = np.zeros(nn)
+        rc, info = c.csrsolve("iccg", x, su, itr_max=5000, tol_abs=0.0, tol_rel=1e-10,
+                              precision=api.FCP_PREC_MIXED, criterion=api.FCP_CRIT_TRUE_L2)
+        assert rc == 0 and info["rel_res_l2"] <= 1e-10, info
+        return x, info, a
+
+    solve(ctx, mu1)
+    x2, info2, a2 = solve(ctx, mu2)
+    tiles = ctx.sweep_tiles()
+    nc = int(tiles.max()) + 1
+    assert nc >= 8
+    fresh = api.Context(0)
+    fresh.read_mesh(mesh)
+    fresh.geometry()
+    fresh.create_CSR_matrix()
+    x2f, info2f, _ = solve(fresh, mu2)
+    fresh.close()
+    assert info2["iterations"] == info2f["iterations"], (info2, info2f)
+    assert np.array_equal(x2, x2f)
+    A = sp.csr_matrix((a2, csr["ja"], csr["ia"]), shape=(nn, nn))
+    P0 = sp.csr_matrix((np.ones(nn), (np.arange(nn), tiles)), shape=(nn, nc))
+    A0 = (P0.T @ A @ P0).toarray()
+    r = np.sin(np.arange(nn) * 0.23)
+    z5, _ = ctx.precond_apply(r, kind=5)
+    z7, _ = ctx.precond_apply(r, kind=7)
+    e0 = np.linalg.solve(A0, P0.T @ r.astype(np.float32).astype(np.float64))
+    scale = np.abs(z7).max() + np.abs(e0).max()
+    assert np.abs((z7 - z5) - e0[tiles]).max() <= 2e-6 * scale
+    ctx.close()
+
+
 def test_inner_graph_replay_bitequal_to_launches(api):
     """Inner FP32 iterations replayed from the captured HIP graph (two
     iterations per replay, eager refinement work and host polls between

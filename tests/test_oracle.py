@@ -463,3 +463,18 @@ def test_rcm_reduces_bandwidth_of_permuted_tets():
     bw0 = np.abs(own.astype(int) - nb).max()
     bw1 = np.abs(new[own].astype(int) - new[nb]).max()
     assert bw1 < bw0 / 5, (bw0, bw1)
+
+
+def test_iccg_l2_criterion_same_trajectory(hex5j):
+    """orc_iccg_l2 runs the reference's DIC-PCG iterations unchanged (a fixed
+    iteration count gives the reference's golden vector) and stops at the
+    first iteration whose recursive residual meets ||r||_2 <= tol ||r0||_2."""
+    _, o, g = hex5j
+    out = o.solve("iccg-l2", g["lap_a"], g["in_b"], itr_max=5, tol_abs=0.0, tol_rel=0.0)
+    assert np.array_equal(out["fi"], g["sol_iccg_5"])
+    b = g["in_b"]
+    out = o.solve("iccg-l2", g["lap_a"], b, itr_max=500, tol_abs=0.0, tol_rel=1e-8)
+    rel = np.linalg.norm(b - o.spmv(g["lap_a"], out["fi"])) / np.linalg.norm(b)
+    assert rel <= 1.5e-8
+    prev = o.solve("iccg-l2", g["lap_a"], b, itr_max=out["iters"] - 1, tol_abs=0.0, tol_rel=0.0)
+    assert np.linalg.norm(b - o.spmv(g["lap_a"], prev["fi"])) / np.linalg.norm(b) > 1e-8
