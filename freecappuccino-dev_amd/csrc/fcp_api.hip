@@ -346,7 +346,11 @@ void fcp_finalize(fcp_ctx *ctx) {
   delete ctx;
 }
 
-const char *fcp_last_error(const fcp_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *fcp_last_error(const fcp_ctx *ctx) {
+  if (ctx) return ctx->err.c_str();
+  const char *m = fcpk::noalloc_message();
+  return m[0] ? m : "null context";
+}
 
 void fcp_solve_opts_default(fcp_solve_opts *o) {
   if (!o) return;

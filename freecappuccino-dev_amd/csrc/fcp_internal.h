@@ -330,6 +330,10 @@ struct ProfScope {
 
 inline int grid_for(long long n, int block) { return (int)((n + block - 1) / block); }
 
+namespace fcpk {
+const char *noalloc_message();  // fcp_noalloc.hip
+}
+
 // ---------------------------------------------------------------------------
 // module entry points (host-side launchers)
 // ---------------------------------------------------------------------------

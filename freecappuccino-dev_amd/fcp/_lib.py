@@ -60,6 +60,7 @@ SIGNATURES = {
     "fcp_bicgstab": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _d, _d, _i]),
     "fcp_gauss_seidel": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _d, _d, _i]),
     "fcp_pmgmres": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _i, _d, _d, _i]),
+    "fcp_dpcg_noalloc": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _d, _d, _p]),
     "fcp_profile": (_i, [_p, _i]),
     "fcp_profile_report": (_i, [_p, C.c_char_p, _i]),
     "fcp_stream": (_p, [_p]),

@@ -330,6 +330,25 @@ int fcp_pmgmres(int n, int nnz, const int *ia, const int *ja, const double *a,
                 const int *diag, double *fi, const double *rhs, double *resor,
                 int itr_max, int mr, double tol_abs, double tol_rel, int verbose);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the Jacobi-preconditioned CG of fcp_dpcg (dpcg's
+ * recurrence and L1 stopping test) on DEVICE arrays of the current device,
+ * ia/ja/diag 1-based as there, on the given stream (NULL: the null stream).
+ * Its work vectors and reduction state live in a fixed device array of the
+ * library (about 6 MB, held by every device it runs on until the process
+ * ends), so it allocates nothing and takes at most FCP_NOALLOC_MAX_ROWS rows;
+ * larger systems, host arrays, arrays shorter than their stated sizes and
+ * index arrays that are not a CSR matrix with its diagonal are rejected with
+ * FCP_ERR_ARG before any solve kernel runs (message: fcp_last_error(NULL)).
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device race.  resor (host) receives the reference's resor.  Returns 0,
+ * the iteration count when the stopping test was not met, or < 0. */
+#define FCP_NOALLOC_MAX_ROWS (1 << 18)
+int fcp_dpcg_noalloc(int n, int nnz, const int *ia, const int *ja, const double *a,
+                     const int *diag, double *fi, const double *rhs, double *resor,
+                     int itr_max, double tol_abs, double tol_rel, void *stream);
+
 /* --- launch-shape tuning (autotuner hook) --------------------------------- */
 #define FCP_TUNE_SWEEP_WAVES 1   /* resident waves of a sweep kernel (0 = auto) */
 #define FCP_TUNE_SWEEP_SLEEP 2   /* s_sleep argument between dependency polls   */
