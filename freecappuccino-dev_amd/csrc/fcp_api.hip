@@ -385,6 +385,8 @@ int fcp_mesh_upload(fcp_ctx *ctx, int nNodes, int nCells, int nInnerFaces, int n
   ctx->h_nfaces.assign(nfaces, nfaces + nBoundaries);
   ctx->h_start.assign(startFace, startFace + nBoundaries);
   ctx->h_bctype.assign(bctype, bctype + nBoundaries);
+  ctx->tr_kind.release();  // the transport step's boundary table belongs to the old mesh
+  ctx->tr_wf.clear();
   long long tot = 0;
   for (int ib = 0; ib < nBoundaries; ++ib) {
     if (startFace[ib] < nInnerFaces || startFace[ib] + nfaces[ib] > F || nfaces[ib] < 0)
@@ -902,6 +904,124 @@ int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi, c
     printf("  %s:  Solving for %s, Initial residual = %10.3E, Final residual = %10.3E, No Iterations %d\n",
            names[code], var.c_str(), o->resor, o->resl / factor, o->iterations);
   }
+  return rc;
+}
+
+int fcp_transport(fcp_ctx *ctx, double *phi, const double *phi_o, const double *phi_oo,
+                  const double *den, const double *vis, const double *flmass,
+                  const double *wall_frac, double viscos, double pranl, double sigt, double urf,
+                  double gds, double dt, int time_scheme, int conv_scheme, int grad_method,
+                  const char *solver, int solver_len, int itr_max, double tol_abs, double tol_rel,
+                  int clip, double *grad, double *a, double *su, double *sp, double *resor,
+                  double *fimin, double *fimax, fcp_solve_opts *opts) {
+  FCP_TRY(need(ctx, true, true, true));
+  const bool transient = time_scheme != 0;
+  if (!phi || !vis || !flmass || !solver || time_scheme < 0 || time_scheme > 3 ||
+      (conv_scheme != 0 && conv_scheme != 1) || !(urf > 0.0) || !(pranl != 0.0) ||
+      !(sigt != 0.0) || (transient && (!den || !phi_o || !(dt > 0.0))) ||
+      (time_scheme == 2 && !phi_oo) || (ctx->nPatches > 0 && !wall_frac))
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_transport: bad argument");
+  if (grad_method != FCP_GRAD_GAUSS && grad_method != ctx->lsq_method)
+    return fcp_fail(ctx, FCP_ERR_STATE,
+                    "fcp_transport: least-squares matrices of method %d not created", grad_method);
+  const int code = solver_code(solver, solver_len);
+  if (code < 0) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_transport: unknown solver");
+  (void)hipSetDevice(ctx->device);
+  const int n = ctx->n, Fb = ctx->Fb, np = ctx->nPatches;
+  const size_t ntot = (size_t)n + Fb;
+  // boundary treatment per face (temperature.f90:168-320), rebuilt when the
+  // wall fractions change
+  std::vector<double> wf(np);
+  if (np > 0) {
+    if (fcp_is_device_ptr(wall_frac)) {
+      FCP_HIP(ctx, hipMemcpy(wf.data(), wall_frac, sizeof(double) * np, hipMemcpyDeviceToHost));
+    } else {
+      std::copy(wall_frac, wall_frac + np, wf.begin());
+    }
+  }
+  if (!ctx->tr_kind.p || ctx->tr_wf != wf) {
+    std::vector<unsigned char> kind(Fb > 0 ? Fb : 1, 3);
+    std::vector<int> srd(Fb > 0 ? Fb : 1);
+    std::vector<int> walls;
+    for (int ib = 0; ib < np; ++ib)
+      if (ctx->h_bctype[ib] == FCP_BC_WALL)
+        for (int i = 0; i < ctx->h_nfaces[ib]; ++i) walls.push_back(ctx->h_start[ib] - ctx->Fi + i);
+    size_t iwall = 0;
+    for (int ib = 0; ib < np; ++ib) {
+      const int t = ctx->h_bctype[ib], s0 = ctx->h_start[ib] - ctx->Fi, nf = ctx->h_nfaces[ib];
+      for (int i = 0; i < nf; ++i) srd[s0 + i] = s0 + i;
+      if (t == FCP_BC_PERIODIC)
+        return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "fcp_transport: periodic patches");
+      unsigned char k = 3;
+      if (t == FCP_BC_INLET || t == FCP_BC_OUTLET || t == FCP_BC_PRESSURE) k = 2;
+      else if ((t == FCP_BC_WALL && wf[ib] == 1.0) || t == FCP_BC_WALL_QFLUX) k = t == FCP_BC_WALL ? 0 : 4;
+      else if (t == FCP_BC_WALL && wf[ib] == 0.0) k = 1;
+      if (k == 0 || k == 4) {
+        // srdw(iWall): the counter runs over these faces, srdw over the wall patches' faces
+        if (iwall + nf > walls.size())
+          return fcp_fail(ctx, FCP_ERR_ARG, "fcp_transport: more prescribed-wall faces than wall faces");
+        for (int i = 0; i < nf; ++i) srd[s0 + i] = walls[iwall + i];
+        iwall += nf;
+      }
+      for (int i = 0; i < nf; ++i) kind[s0 + i] = k;
+    }
+    FCP_HIP(ctx, ctx->tr_kind.alloc(kind.size()));
+    FCP_HIP(ctx, ctx->tr_srd.alloc(srd.size()));
+    FCP_HIP(ctx, hipMemcpyAsync(ctx->tr_kind.p, kind.data(), kind.size(), hipMemcpyHostToDevice,
+                                ctx->stream));
+    FCP_HIP(ctx, hipMemcpyAsync(ctx->tr_srd.p, srd.data(), srd.size() * sizeof(int),
+                                hipMemcpyHostToDevice, ctx->stream));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->tr_wf = wf;
+  }
+  FCP_HIP(ctx, ctx->tr_grad.alloc(3 * ntot));
+  FCP_HIP(ctx, ctx->tr_dcoef.alloc(ctx->F));
+  FCP_HIP(ctx, ctx->tr_su.alloc(n));
+  FCP_HIP(ctx, ctx->tr_sp.alloc(n));
+  FCP_HIP(ctx, ctx->tr_mm.alloc(2 * 1024));
+  Arg<double> ph, fo, foo, dn, vs, fm;
+  FCP_TRY(arg_in(ctx, (const double *)phi, ntot, ph));
+  FCP_TRY(arg_in(ctx, vis, n, vs));
+  FCP_TRY(arg_in(ctx, flmass, ctx->F, fm));
+  if (transient) {
+    FCP_TRY(arg_in(ctx, den, n, dn));
+    FCP_TRY(arg_in(ctx, phi_o, n, fo));
+    if (time_scheme == 2) FCP_TRY(arg_in(ctx, phi_oo, n, foo));
+  }
+  // grad(T, dTdxi) (:72)
+  if (grad_method == FCP_GRAD_GAUSS)
+    FCP_TRY(fcp_launch_grad_gauss(ctx, ph.dev, ctx->tr_grad.p));
+  else
+    FCP_TRY(fcp_launch_grad_lsq(ctx, ph.dev, ctx->tr_grad.p));
+  FCP_TRY(fcp_launch_transport_dcoef(ctx, vs.dev, ctx->tr_kind.p, viscos, pranl, sigt,
+                                     ctx->tr_dcoef.p));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->tr_su.p, 0, sizeof(double) * n, ctx->stream));
+  FCP_HIP(ctx, hipMemsetAsync(ctx->tr_sp.p, 0, sizeof(double) * n, ctx->stream));
+  if (transient)  // :82-102; Crank-Nicolson shares BDF1's volume terms
+    FCP_TRY(fcp_launch_ddt(ctx, time_scheme == 2 ? 2 : 1, dn.dev, dt, fo.dev, foo.dev,
+                           ctx->tr_su.p, ctx->tr_sp.p));
+  FCP_TRY(fcp_launch_convdiff(ctx, fm.dev, ctx->tr_dcoef.p, ph.dev, ctx->tr_grad.p, gds,
+                              conv_scheme, ctx->tr_kind.p, urf, ctx->tr_su.p, ctx->tr_sp.p,
+                              ctx->tr_srd.p, time_scheme == 3, dn.dev, dt, fo.dev));
+  FCP_TRY(out_copy(ctx, ctx->a.p, a, ctx->nnz));
+  FCP_TRY(out_copy(ctx, ctx->tr_su.p, su, n));
+  FCP_TRY(out_copy(ctx, ctx->tr_sp.p, sp, n));
+  // csrsolve(lSolverT, T, su, resor(7), ...) (:369) from the current cell values
+  fcp_solve_opts local;
+  fcp_solve_opts_default(&local);
+  fcp_solve_opts *o = opts ? opts : &local;
+  const int rc = fcp_solve_device(ctx, code, ph.dev, ctx->tr_su.p, itr_max, tol_abs, tol_rel, 0, o);
+  if (rc < 0) return rc;
+  FCP_TRY(fcp_launch_update_boundary(ctx, ph.dev));  // :373
+  FCP_TRY(fcp_launch_minmax_clip(ctx, ph.dev, clip, ctx->tr_mm.p));  // :377-381
+  double mm[2];
+  FCP_HIP(ctx, hipMemcpyAsync(mm, ctx->tr_mm.p, sizeof mm, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_TRY(arg_back(ctx, ph));
+  FCP_TRY(out_copy(ctx, ctx->tr_grad.p, grad, 3 * ntot));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (resor) *resor = o->resor;
+  if (fimin) *fimin = mm[0];
+  if (fimax) *fimax = mm[1];
   return rc;
 }
 

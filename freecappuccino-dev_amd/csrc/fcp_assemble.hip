@@ -191,6 +191,13 @@ __device__ __forceinline__ FaceFlux inner_flux(int f, int p, int q, const GeomP 
   return r;
 }
 
+// Boundary kinds: 0 Dirichlet wall, 1 zero-gradient wall, 2 inlet/outlet/
+// pressure flux face, 3 no term, 4 prescribed normal gradient (wallQFlux,
+// :231-262).  srd_map (may be null) names, per boundary face, the face whose
+// |S|/dn a wall term reads.  With cn the Crank-Nicolson block (:325-347)
+// runs between the boundary terms and the diagonal: the row is halved, the
+// old neighbour values enter su through the halved coefficients, and
+// (den V/dt + halved off-diagonal sum) phi_o and den V/dt join su and sp.
 __global__ void k_convdiff(int n, int Fi, const int *__restrict__ cf_ptr, const int *__restrict__ cf,
                            const int *__restrict__ owner, const int *__restrict__ neighbour, GeomP g,
                            const double *__restrict__ srdb, const double *__restrict__ flmass,
@@ -200,7 +207,9 @@ __global__ void k_convdiff(int n, int Fi, const int *__restrict__ cf_ptr, const 
                            const int *__restrict__ slot_pn, const int *__restrict__ slot_np,
                            const int *__restrict__ ia, const int *__restrict__ diag,
                            double *__restrict__ a, double *__restrict__ su,
-                           double *__restrict__ sp) {
+                           double *__restrict__ sp, const int *__restrict__ srd_map, int cn,
+                           const double *__restrict__ den, double dt,
+                           const double *__restrict__ phi_o) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   double ad = 0.0, s = su[c], spc = sp[c];
@@ -238,20 +247,47 @@ __global__ void k_convdiff(int n, int Fi, const int *__restrict__ cf_ptr, const 
         spc = spc - can;
         s = s - can * phi[n + b] + suadd;
       } else if (kind == 0) {  // Dirichlet wall :206-210
-        const double dw = dcoef[f] * srdb[b];
+        const double dw = dcoef[f] * srdb[srd_map ? srd_map[b] : b];
         spc = spc + dw;
         s = s + dw * phi[n + b];
-      } else {  // zero-gradient wall :224
+      } else if (kind == 1) {  // zero-gradient wall :224
         phi[n + b] = phi[c];
+      } else if (kind == 4) {  // prescribed normal gradient :231-262
+        const double dw = dcoef[f] * srdb[srd_map ? srd_map[b] : b];
+        const double ax = g.arx[f], ay = g.ary[f], az = g.arz[f];
+        const double are = sqrt((ax * ax + ay * ay) + az * az);
+        const double nxf = ax / are, nyf = ay / are, nzf = az / are;
+        const size_t jb = 3 * ((size_t)n + b);
+        const double dtn = (gr[jb] * nxf + gr[jb + 1] * nyf) + gr[jb + 2] * nzf;
+        s = s + dw * dtn;
       }
     }
   }
   const int dg = diag[c];
   a[dg] = ad;
+  if (cn) {
+    for (int k = ia[c]; k < ia[c + 1]; ++k) a[k] = 0.5 * a[k];
+    for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+      const int f = cf[k];
+      if (f >= Fi) continue;
+      if (c == owner[f]) {
+        const int sl = slot_pn[f];
+        if (sl >= 0) s = s - a[sl] * phi_o[neighbour[f]];
+      } else {
+        const int sl = slot_np[f];
+        if (sl >= 0) s = s - a[sl] * phi_o[owner[f]];
+      }
+    }
+    const double apotime = den[c] * g.vol[c] / dt;
+    double rh = 0.0;
+    for (int k = ia[c]; k < ia[c + 1]; ++k) rh = rh + a[k];
+    s = s + (apotime + (rh - a[dg])) * phi_o[c];
+    spc = spc + apotime;
+  }
   // off_diagonal_terms = sum(a(ia(i):ia(i+1)-1)) - a(diag(i)) (:359-360)
   double rs = 0.0;
   for (int k = ia[c]; k < ia[c + 1]; ++k) rs = rs + a[k];
-  const double off = rs - ad;
+  const double off = rs - a[dg];
   double aii = spc - off;
   aii = aii * (1.0 / urf);
   a[dg] = aii;
@@ -275,6 +311,83 @@ __global__ void k_ddt(int n, int scheme, const double *__restrict__ den,
     su[c] = su[c] + apotime * fo[c];
     sp[c] = sp[c] + apotime;
   }
+}
+
+// Face diffusion coefficients of calculate_temperature from the cell
+// viscosity: inner faces viscos*prl + ((vis_P + (vis_N - vis_P) lambda) -
+// viscos)*prtr (temperature.f90:125), flux faces viscos*prl + (vis_P -
+// viscos)/sigt (:178), wall faces viscos/pranl + (vis_P - viscos)/sigt
+// (:203, :237); 0 on faces without a term.
+__global__ void k_transport_dcoef(int Fi, int F, const int *__restrict__ owner,
+                                  const int *__restrict__ neighbour,
+                                  const double *__restrict__ facint, const double *__restrict__ vis,
+                                  const unsigned char *__restrict__ bc_kind, double viscos,
+                                  double prl, double prtr, double pranl, double sigt,
+                                  double *__restrict__ dcoef) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  const int p = owner[f];
+  double d;
+  if (f < Fi) {
+    const int q = neighbour[f];
+    d = viscos * prl + ((vis[p] + (vis[q] - vis[p]) * facint[f]) - viscos) * prtr;
+  } else {
+    const int kind = bc_kind[f - Fi];
+    if (kind == 2)
+      d = viscos * prl + (vis[p] - viscos) / sigt;
+    else if (kind == 0 || kind == 4)
+      d = viscos / pranl + (vis[p] - viscos) / sigt;
+    else
+      d = 0.0;
+  }
+  dcoef[f] = d;
+}
+
+// Smallest and largest cell value: per-block pairs, then one block.
+__global__ __launch_bounds__(BLK) void k_minmax(int n, const double *__restrict__ x,
+                                                double *__restrict__ part) {
+  __shared__ double slo[BLK / 64], shi[BLK / 64];
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
+    lo = fmin(lo, x[i]);
+    hi = fmax(hi, x[i]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    slo[threadIdx.x >> 6] = lo;
+    shi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < BLK / 64; ++w) {
+      lo = fmin(lo, slo[w]);
+      hi = fmax(hi, shi[w]);
+    }
+    part[2 * blockIdx.x] = lo;
+    part[2 * blockIdx.x + 1] = hi;
+  }
+}
+
+__global__ void k_minmax_final(int nb, double *__restrict__ part) {
+  if (threadIdx.x != 0) return;
+  double lo = part[0], hi = part[1];
+  for (int b = 1; b < nb; ++b) {
+    lo = fmin(lo, part[2 * b]);
+    hi = fmax(hi, part[2 * b + 1]);
+  }
+  part[0] = lo;
+  part[1] = hi;
+}
+
+// calculate_temperature's clip (:381): when the smallest value is negative,
+// every cell value below `small` becomes `small`.
+__global__ void k_clip_small(int n, double *__restrict__ x, const double *__restrict__ mm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(mm[0] < 0.0)) return;
+  x[i] = fmax(x[i], FCP_SMALL);
 }
 
 GeomP geom_of(fcp_ctx *ctx) {
@@ -326,13 +439,14 @@ int fcp_launch_update_boundary(fcp_ctx *ctx, double *phi) {
 
 int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
                         const double *grad, double gam, int scheme, const unsigned char *bc_kind,
-                        double urf, double *su, double *sp) {
+                        double urf, double *su, double *sp, const int *srd_map, int cn,
+                        const double *den, double dt, const double *phi_o) {
   FCP_TRY(fcp_coarse_join(ctx));
   ProfScope ps(ctx, "convdiff");
   k_convdiff<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, geom_of(ctx),
       ctx->srdb.p, flmass, dcoef, phi, grad, gam, scheme, bc_kind, urf, ctx->slot_pn.p,
-      ctx->slot_np.p, ctx->ia.p, ctx->diag.p, ctx->a.p, su, sp);
+      ctx->slot_np.p, ctx->ia.p, ctx->diag.p, ctx->a.p, su, sp, srd_map, cn, den, dt, phi_o);
   FCP_HIP(ctx, hipGetLastError());
   ctx->a_version++;
   return 0;
@@ -342,6 +456,25 @@ int fcp_launch_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt, const
                    const double *foo, double *su, double *sp) {
   k_ddt<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(ctx->n, scheme, den, ctx->vol.p, dt, fo, foo,
                                                         su, sp);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+int fcp_launch_transport_dcoef(fcp_ctx *ctx, const double *vis, const unsigned char *bc_kind,
+                               double viscos, double pranl, double sigt, double *dcoef) {
+  const double prl = 1.0 / pranl, prtr = 1.0 / sigt;  // temperature.f90:66-68
+  k_transport_dcoef<<<grid_for(ctx->F, BLK), BLK, 0, ctx->stream>>>(
+      ctx->Fi, ctx->F, ctx->owner.p, ctx->neighbour.p, ctx->facint.p, vis, bc_kind, viscos, prl,
+      prtr, pranl, sigt, dcoef);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+int fcp_launch_minmax_clip(fcp_ctx *ctx, double *x, int clip, double *mm) {
+  const int nb = grid_for(ctx->n, BLK) < 1024 ? grid_for(ctx->n, BLK) : 1024;
+  k_minmax<<<nb, BLK, 0, ctx->stream>>>(ctx->n, x, mm);
+  k_minmax_final<<<1, 64, 0, ctx->stream>>>(nb, mm);
+  if (clip) k_clip_small<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(ctx->n, x, mm);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }

@@ -263,6 +263,12 @@ struct fcp_ctx {
   DBuf<double> lsq_dmat;  // Dmat(9,n) or D(3,6,n)
   int lsq_method = 0, lsq_compat = 1;
 
+  // -------- scalar transport step (calculate_temperature) --------
+  DBuf<unsigned char> tr_kind;  // [Fb] boundary treatment per face
+  DBuf<int> tr_srd;             // [Fb] face whose |S|/dn a wall term reads
+  std::vector<double> tr_wf;    // wall fractions the two above were built for
+  DBuf<double> tr_grad, tr_dcoef, tr_su, tr_sp, tr_mm;
+
   // -------- batched solves (interleaved vectors, NRP values per row) --------
   std::map<long long, std::unique_ptr<MultiSched>> msched;  // keyed by rows per block (lo, up)
   DBuf<char> tpin;  // batched sweep inputs on the tile schedules
@@ -380,7 +386,13 @@ int fcp_launch_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi);
 int fcp_launch_update_boundary(fcp_ctx *ctx, double *phi);
 int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
                         const double *grad, double gam, int scheme, const unsigned char *bc_kind,
-                        double urf, double *su, double *sp);
+                        double urf, double *su, double *sp, const int *srd_map = nullptr,
+                        int cn = 0, const double *den = nullptr, double dt = 1.0,
+                        const double *phi_o = nullptr);
+int fcp_launch_transport_dcoef(fcp_ctx *ctx, const double *vis, const unsigned char *bc_kind,
+                               double viscos, double pranl, double sigt, double *dcoef);
+// mm[2*1024] partials; mm[0], mm[1] receive min and max of x[0..n)
+int fcp_launch_minmax_clip(fcp_ctx *ctx, double *x, int clip, double *mm);
 int fcp_launch_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt, const double *fo,
                    const double *foo, double *su, double *sp);
 // fcp_adjoint.hip

@@ -22,6 +22,10 @@ FCP_PREC_FP64 = 0
 FCP_PREC_MIXED = 1
 FCP_CRIT_REF_L1 = 0
 FCP_CRIT_TRUE_L2 = 1
+FCP_GRAD_GAUSS = 0
+FCP_GRAD_LSQ = 1
+FCP_GRAD_LSQ_DM = 2
+FCP_GRAD_LSQ_QR = 3
 FCP_TUNE_SWEEP_WAVES = 1
 FCP_TUNE_SWEEP_SLEEP = 2
 FCP_TUNE_SWEEP_KIND = 3
@@ -190,6 +194,38 @@ class Context:
         rc = self.lib.fcp_ddt(self.h, int(scheme), _ptr(_f64(den))[0], float(dt),
                               _ptr(_f64(phi_o))[0], _ptr(_f64(phi_oo))[0], _ptr(su)[0], _ptr(sp)[0])
         self._check(rc, "fcp_ddt")
+
+    TIME_SCHEMES = {"steady": 0, "bdf": 1, "bdf2": 2, "cn": 3}
+
+    def transport(self, phi, vis, flmass, wall_frac, viscos, pranl, sigt, urf, gds, solver,
+                  itr_max, tol_abs, tol_rel, time_scheme="steady", den=None, dt=1.0, phi_o=None,
+                  phi_oo=None, conv_scheme="linearUpwind", grad_method=FCP_GRAD_GAUSS, clip=True,
+                  grad_out=None, a_out=None, su_out=None, sp_out=None, precision=FCP_PREC_FP64,
+                  criterion=FCP_CRIT_REF_L1):
+        """calculate_temperature for one scalar field: phi[numTotal] updated in
+        place; grad_out, a_out, su_out, sp_out (optional) receive the gradient
+        and the assembled system.  Returns (status, info) with the solver's
+        resor (csrsolve's res0), the cell range before the clip and the solve
+        statistics."""
+        opts = _lib.SolveOpts()
+        self.lib.fcp_solve_opts_default(C.byref(opts))
+        opts.precision = precision
+        opts.criterion = criterion
+        resor, lo, hi = C.c_double(), C.c_double(), C.c_double()
+        sb = solver.encode()
+        wf = np.ascontiguousarray(wall_frac, dtype=np.float64)
+        rc = self.lib.fcp_transport(
+            self.h, _ptr(phi)[0], _ptr(_f64(phi_o))[0], _ptr(_f64(phi_oo))[0], _ptr(_f64(den))[0],
+            _ptr(_f64(vis))[0], _ptr(_f64(flmass))[0], _ptr(wf)[0], float(viscos), float(pranl),
+            float(sigt), float(urf), float(gds), float(dt), self.TIME_SCHEMES[time_scheme],
+            {"cds": 0, "linearUpwind": 1}[conv_scheme], int(grad_method), sb, len(sb),
+            int(itr_max), float(tol_abs), float(tol_rel), int(bool(clip)), _ptr(grad_out)[0],
+            _ptr(a_out)[0], _ptr(su_out)[0], _ptr(sp_out)[0], C.byref(resor), C.byref(lo),
+            C.byref(hi), C.byref(opts))
+        self._check(rc, "fcp_transport")
+        info = {k: getattr(opts, k) for k, _ in _lib.SolveOpts._fields_}
+        info.update(status=rc, resor=resor.value, fimin=lo.value, fimax=hi.value)
+        return rc, info
 
     def grad_gauss(self, u, out=None):
         if out is None:

@@ -49,6 +49,7 @@ extern "C" {
 #define FCP_BC_PRESSURE 5
 #define FCP_BC_EMPTY 6
 #define FCP_BC_PERIODIC 7
+#define FCP_BC_WALL_QFLUX 8 /* 'wallQFlux': prescribed heat-flux wall (temperature.f90:231) */
 
 /* Precision of a solve. */
 #define FCP_PREC_FP64 0  /* reference arithmetic, FP64 throughout            */
@@ -151,7 +152,9 @@ int fcp_nonorth_correction(fcp_ctx *ctx, const double *gamma_f,
  * 1 = 'linearUpwind' (interpolation.f90:116-157, 318-356); per boundary face
  * bc_kind 0 = Dirichlet wall (:206-210), 1 = zero-gradient wall (phi_b =
  * phi_P, :224), 2 = inlet/outlet/pressure flux face (facefluxsc_boundary
- * :256-337, :182-190); then a_ii = sp - (row sum - a_ii) and under-relaxation
+ * :256-337, :182-190), 3 = no term (symmetry, empty), 4 = prescribed normal
+ * gradient (wallQFlux, :231-262: su += dcoef |S|/dn (grad_b . n), grad_b the
+ * face's entry of grad); then a_ii = sp - (row sum - a_ii) and under-relaxation
  * a_ii /= urf, su += (1-urf) a_ii phi (:350-366).  flmass/dcoef [nFaces],
  * phi [numTotal] (boundary values of zero-gradient faces updated), grad
  * [3*numTotal]; su, sp [nCells] accumulate (pass them cleared, or holding
@@ -165,6 +168,37 @@ int fcp_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double
  * Crank-Nicolson), 2 BDF2; apotime = den*vol/dt; su/sp updated in place. */
 int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt,
             const double *phi_o, const double *phi_oo, double *su, double *sp);
+
+/* calculate_temperature (temperature.f90:35-385) as one call, for any scalar
+ * transported like the temperature: grad(phi) with grad_method (FCP_GRAD_*;
+ * least-squares matrices created beforehand), the face diffusion
+ * coefficients from the cell viscosity vis (viscos*prl + ((vis_P + (vis_N -
+ * vis_P) lambda) - viscos)*prtr inside, prl = 1/pranl, prtr = 1/sigt, and
+ * the boundary forms of :178,203,237), the time terms (time_scheme 0 steady,
+ * 1 BDF1, 2 BDF2, 3 Crank-Nicolson with its coefficient block :325-347),
+ * facefluxsc with conv_scheme (0 'cds', 1 'linearUpwind') and deferred
+ * correction gds, the boundary loop by patch type (inlet/outlet/pressure flux
+ * faces; 'wall' patches isothermal when wall_frac[ib] = bcDFraction(7,ib) is
+ * 1, adiabatic when 0; wallQFlux; symmetry/empty without a term; periodic
+ * patches are FCP_ERR_UNSUPPORTED), the diagonal with under-relaxation urf,
+ * csrsolve(solver) from the current cell values, updateBoundary, and -- when
+ * clip is 1 -- the clip of negative fields to `small` (:381).  Wall terms read
+ * |S|/dn through the reference's own wall counter (an isothermal wall that
+ * follows an adiabatic wall takes the earlier wall's faces' values).
+ * phi[numTotal] in/out; phi_o, phi_oo, den, vis [numCells] (phi_o, den for
+ * transient schemes, phi_oo for BDF2); flmass[numFaces]; wall_frac
+ * [nBoundaries]; grad (optional, [3*numTotal]) receives dphi/dxi and a, su,
+ * sp (optional; [nnz], [numCells], [numCells]) the assembled system, which
+ * the reference leaves in its module globals; resor the solver's resor
+ * (csrsolve's res0); fimin/fimax the cell range before the clip.  The
+ * assembled matrix stays resident.  Returns the solve's code. */
+int fcp_transport(fcp_ctx *ctx, double *phi, const double *phi_o, const double *phi_oo,
+                  const double *den, const double *vis, const double *flmass,
+                  const double *wall_frac, double viscos, double pranl, double sigt, double urf,
+                  double gds, double dt, int time_scheme, int conv_scheme, int grad_method,
+                  const char *solver, int solver_len, int itr_max, double tol_abs, double tol_rel,
+                  int clip, double *grad, double *a, double *su, double *sp, double *resor,
+                  double *fimin, double *fimax, fcp_solve_opts *opts);
 
 /* grad_gauss (gradients.f90:1607-1715). dudxi[3*numTotal], boundary rows 0. */
 int fcp_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi);

@@ -1029,15 +1029,30 @@ static double face_value(int scheme, int ijp, int ijn, double xf, double yf, dou
   return u[ijp] + gradfidr;
 }
 
-void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
-                  const int *slot_pn, const int *slot_np, const double *flmass,
-                  const double *dcoef, double *phi, const double *grad, double gam,
-                  int scheme, const unsigned char *bc_kind, double urf, double *a,
-                  double *su, double *sp) {
+int orc_transport_assemble(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                           const int *slot_pn, const int *slot_np, const double *flmass,
+                           const double *dcoef, double *phi, const double *grad, double gam,
+                           int scheme, const unsigned char *bc_kind, const int *srd_map,
+                           double urf, int time_scheme, const double *den, double dt,
+                           const double *phi_o, const double *phi_oo, double *a, double *su,
+                           double *sp) {
   const int n = m->n_cells, Fi = m->n_inner, F = m->n_faces;
   memset(a, 0, sizeof(double) * c->nnz);
   memset(su, 0, sizeof(double) * n);
   memset(sp, 0, sizeof(double) * n);
+  if (time_scheme < 0 || time_scheme > 3) return -1;
+  if (time_scheme != 0) { /* temperature.f90:82-102 */
+    for (int i = 0; i < n; ++i) {
+      const double apotime = den[i] * g->vol[i] / dt;
+      if (time_scheme == 2) {
+        su[i] = su[i] + apotime * (2 * phi_o[i] - 0.5 * phi_oo[i]);
+        sp[i] = sp[i] + 1.5 * apotime;
+      } else {
+        su[i] = su[i] + apotime * phi_o[i];
+        sp[i] = sp[i] + apotime;
+      }
+    }
+  }
   for (int f = 0; f < Fi; ++f) { /* temperature.f90:118-157 */
     const int p = m->owner[f], q = m->neighbour[f];
     const double fxn = g->facint[f], fxp = 1.0 - g->facint[f];
@@ -1075,6 +1090,9 @@ void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
     const int p = m->owner[f];
     const int ijb = n + (f - Fi);
     const int kind = bc_kind[f - Fi];
+    /* the reference indexes srdw by a counter of the faces it treats as
+       walls (temperature.f90:201-202,235-236) */
+    const int srd = srd_map ? srd_map[f - Fi] : f - Fi;
     if (kind == 2) { /* facefluxsc_boundary scalar_fluxes.f90:256-337 */
       const double xpn = g->xf[f] - g->xc[p], ypn = g->yf[f] - g->yc[p], zpn = g->zf[f] - g->zc[p];
       const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
@@ -1092,11 +1110,34 @@ void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
       sp[p] = sp[p] - can;
       su[p] = su[p] - can * phi[ijb] + suadd;
     } else if (kind == 0) { /* Dirichlet wall :206-210 */
-      const double dw = dcoef[f] * g->srdb[f - Fi];
+      const double dw = dcoef[f] * g->srdb[srd];
       sp[p] = sp[p] + dw;
       su[p] = su[p] + dw * phi[ijb];
-    } else { /* zero-gradient wall :224 */
+    } else if (kind == 1) { /* zero-gradient wall :224 */
       phi[ijb] = phi[p];
+    } else if (kind == 4) { /* prescribed normal gradient :231-262 */
+      const double dw = dcoef[f] * g->srdb[srd];
+      const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
+      const double are = sqrt((ax * ax + ay * ay) + az * az);
+      const double nxf = ax / are, nyf = ay / are, nzf = az / are;
+      const double dtn = (grad[3 * ijb] * nxf + grad[3 * ijb + 1] * nyf) + grad[3 * ijb + 2] * nzf;
+      su[p] = su[p] + dw * dtn;
+    } /* kind 3: no term (symmetry, empty) */
+  }
+  if (time_scheme == 3) { /* Crank-Nicolson :325-347 */
+    for (int k = 0; k < c->nnz; ++k) a[k] = 0.5 * a[k];
+    for (int f = 0; f < Fi; ++f) {
+      const int p = m->owner[f], q = m->neighbour[f];
+      su[p] = su[p] - a[slot_pn[f]] * phi_o[q];
+      su[q] = su[q] - a[slot_np[f]] * phi_o[p];
+    }
+    for (int i = 0; i < n; ++i) {
+      const double apotime = den[i] * g->vol[i] / dt;
+      double s = 0.0;
+      for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) s = s + a[k];
+      const double off = s - a[c->diag[i]];
+      su[i] = su[i] + (apotime + off) * phi_o[i];
+      sp[i] = sp[i] + apotime;
     }
   }
   const double urfr = 1.0 / urf, urfm = 1.0 - urf; /* :350-366 */
@@ -1107,6 +1148,41 @@ void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
     a[c->diag[i]] = sp[i] - off;
     a[c->diag[i]] = a[c->diag[i]] * urfr;
     su[i] = su[i] + urfm * a[c->diag[i]] * phi[i];
+  }
+  return 0;
+}
+
+void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                  const int *slot_pn, const int *slot_np, const double *flmass,
+                  const double *dcoef, double *phi, const double *grad, double gam,
+                  int scheme, const unsigned char *bc_kind, double urf, double *a,
+                  double *su, double *sp) {
+  orc_transport_assemble(m, g, c, slot_pn, slot_np, flmass, dcoef, phi, grad, gam, scheme,
+                         bc_kind, NULL, urf, 0, NULL, 1.0, NULL, NULL, a, su, sp);
+}
+
+/* Face diffusion coefficients of calculate_temperature: inner faces
+ * viscos*prl + ((vis_P + (vis_N - vis_P) lambda) - viscos)*prtr (:125),
+ * inlet/outlet/pressure faces viscos*prl + (vis_P - viscos)/sigt (:178),
+ * wall faces viscos/pranl + (vis_P - viscos)/sigt (:203, :237); other
+ * boundary faces 0.  prl = 1/pranl, prtr = 1/sigt (:66-68). */
+void orc_transport_dcoef(const orc_mesh *m, const orc_geom *g, const double *vis,
+                         const unsigned char *bc_kind, double viscos, double pranl, double sigt,
+                         double *dcoef) {
+  const double prl = 1.0 / pranl, prtr = 1.0 / sigt;
+  for (int f = 0; f < m->n_inner; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    dcoef[f] = viscos * prl + ((vis[p] + (vis[q] - vis[p]) * g->facint[f]) - viscos) * prtr;
+  }
+  for (int f = m->n_inner; f < m->n_faces; ++f) {
+    const int p = m->owner[f];
+    const int kind = bc_kind[f - m->n_inner];
+    if (kind == 2)
+      dcoef[f] = viscos * prl + (vis[p] - viscos) / sigt;
+    else if (kind == 0 || kind == 4)
+      dcoef[f] = viscos / pranl + (vis[p] - viscos) / sigt;
+    else
+      dcoef[f] = 0.0;
   }
 }
 

@@ -159,6 +159,26 @@ int orc_rcm(const orc_csr *c, int pp_iters, int *new_of_old);
  * 2 = inlet/outlet/pressure flux boundary (:182-190).  urf: under-relaxation
  * (1 = none).  Outputs a[nnz] (zeroed first), su[n], sp[n] (zeroed first).
  * phi boundary entries of zero-gradient faces are updated in place. */
+/* The assembly of calculate_temperature (temperature.f90:75-366): time terms
+ * (time_scheme 0 steady, 1 BDF1, 2 BDF2, 3 Crank-Nicolson), facefluxsc over
+ * the inner faces, the boundary loop by bc_kind (0 Dirichlet wall, 1
+ * zero-gradient wall, 2 inlet/outlet/pressure flux face, 3 no term, 4
+ * prescribed normal gradient), the Crank-Nicolson coefficient block, then
+ * the diagonal and under-relaxation.  srd_map (NULL: identity) gives, per
+ * boundary face, the boundary face whose |S|/dn the reference reads through
+ * its wall counter.  a, su, sp are overwritten.  Returns 0 or -1. */
+int orc_transport_assemble(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                           const int *slot_pn, const int *slot_np, const double *flmass,
+                           const double *dcoef, double *phi, const double *grad, double gam,
+                           int scheme, const unsigned char *bc_kind, const int *srd_map,
+                           double urf, int time_scheme, const double *den, double dt,
+                           const double *phi_o, const double *phi_oo, double *a, double *su,
+                           double *sp);
+/* Face diffusion coefficients of calculate_temperature from the cell
+ * viscosity (temperature.f90:125,178,203,237). */
+void orc_transport_dcoef(const orc_mesh *m, const orc_geom *g, const double *vis,
+                         const unsigned char *bc_kind, double viscos, double pranl, double sigt,
+                         double *dcoef);
 void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
                   const int *slot_pn, const int *slot_np,
                   const double *flmass, const double *dcoef,

@@ -179,6 +179,66 @@ class Oracle:
                            _p(a), _p(su), _p(sp))
         return a, su, sp, phi
 
+    def transport_dcoef(self, vis, bc_kind, viscos, pranl, sigt):
+        d = np.zeros(self.mesh.n_faces)
+        lib().orc_transport_dcoef(C.byref(self.cm), C.byref(self.cg),
+                                  _p(np.ascontiguousarray(vis, np.float64)),
+                                  _p(np.ascontiguousarray(bc_kind, np.uint8)), C.c_double(viscos),
+                                  C.c_double(pranl), C.c_double(sigt), _p(d))
+        return d
+
+    def transport_assemble(self, flmass, dcoef, phi, grad, gam, scheme, bc_kind, srd_map=None,
+                           urf=1.0, time_scheme=0, den=None, dt=1.0, phi_o=None, phi_oo=None):
+        m = self.mesh
+        n = m.n_cells
+        a = np.zeros(self.nnz)
+        su = np.zeros(n)
+        sp = np.zeros(n)
+        phi = np.array(phi, dtype=np.float64, copy=True)
+        opt = lambda v: None if v is None else _p(np.ascontiguousarray(v, np.float64))  # noqa: E731
+        keep = [None if v is None else np.ascontiguousarray(v, np.float64)
+                for v in (den, phi_o, phi_oo)]
+        smap = None if srd_map is None else np.ascontiguousarray(srd_map, np.int32)
+        rc = lib().orc_transport_assemble(
+            C.byref(self.cm), C.byref(self.cg), C.byref(self.cc), _p(self.slot_pn),
+            _p(self.slot_np), _p(np.ascontiguousarray(flmass, np.float64)),
+            _p(np.ascontiguousarray(dcoef, np.float64)), _p(phi),
+            _p(np.ascontiguousarray(grad, np.float64).reshape(-1)), C.c_double(gam), int(scheme),
+            _p(np.ascontiguousarray(bc_kind, np.uint8)), None if smap is None else _p(smap),
+            C.c_double(urf), int(time_scheme), opt(keep[0]), C.c_double(dt), opt(keep[1]),
+            opt(keep[2]), _p(a), _p(su), _p(sp))
+        assert rc == 0, rc
+        return a, su, sp, phi
+
+    def temperature(self, T, To, Too, den, vis, flmass, wall_frac, viscos, pranl, sigt, urf,
+                    gds, dt, tscheme, scheme, solver, itr_max, tol_abs, tol_rel,
+                    grad_method="gauss"):
+        """calculate_temperature (temperature.f90:35-385) restated: gradient,
+        face coefficients, assembly, csrsolve from the current field,
+        updateBoundary, and the clip of negative values to `small`.  Returns
+        the new field and (a, su, sp, grad, resor: the value csrsolve returns
+        in its res0 argument, i.e. the solver's resor)."""
+        m = self.mesh
+        n = m.n_cells
+        T = np.array(T, dtype=np.float64, copy=True)
+        if grad_method == "gauss":
+            gr = self.grad_gauss(T)
+        else:
+            gr = self.grad_lsq({"lsq": 0, "lsq_dm": 1, "lsq_qr": 2}[grad_method], T, compat=1)
+        kinds, smap = transport_bc(m, wall_frac)
+        dcoef = self.transport_dcoef(vis, kinds, viscos, pranl, sigt)
+        ts = TIME_SCHEMES[tscheme]
+        code = {"cds": 0, "linearUpwind": 1}[scheme]
+        a, su, sp, T = self.transport_assemble(flmass, dcoef, T, gr, gds, code, kinds, smap, urf,
+                                               ts, den, dt, To, Too)
+        out = self.solve(solver, a, su, fi0=T[:n], itr_max=itr_max, tol_abs=tol_abs,
+                         tol_rel=tol_rel)
+        T[:n] = out["fi"]
+        T = self.update_boundary(T)
+        if T[:n].min() < 0.0:
+            T[:n] = np.maximum(T[:n], orc_small())
+        return T, dict(a=a, su=su, sp=sp, grad=gr, resor=out["resor"])
+
     # -- linear algebra ----------------------------------------------------
     def spmv(self, a, x):
         y = np.zeros(self.mesh.n_cells)
@@ -219,6 +279,47 @@ class Oracle:
               tol_rel=1e-10, history=False, mr=4):
         return solve_csr(self.cc, solver, a, rhs, fi0, itr_max, tol_abs, tol_rel, history,
                          self.mesh.n_cells, mr)
+
+
+TIME_SCHEMES = {"steady": 0, "bdf": 1, "bdf2": 2, "cn": 3}
+
+
+def orc_small():
+    lib().orc_small.restype = C.c_double
+    return lib().orc_small()
+
+
+def transport_bc(mesh, wall_frac):
+    """Boundary treatment of calculate_temperature (temperature.f90:168-320)
+    per boundary face: kind 2 inlet/outlet/pressure, 0 isothermal wall
+    (bcDFraction(7,ib) = 1), 1 adiabatic wall (0), 4 prescribed heat flux
+    (wallQFlux), 3 no term (symmetry, empty, walls with another fraction);
+    and, for kinds 0 and 4, the boundary face whose |S|/dn the reference
+    reads: srdw(iWall), iWall counting the kind-0/4 faces in patch order
+    while srdw lists the faces of the 'wall' patches."""
+    nb, nin = mesh.n_bnd, mesh.n_inner
+    kinds = np.full(nb, 3, np.uint8)
+    smap = np.arange(nb, dtype=np.int32)
+    rng = [(int(mesh.start_face[i]) - nin, int(mesh.start_face[i]) - nin + int(mesh.nfaces[i]))
+           for i in range(mesh.n_patches)]
+    walls = np.concatenate([np.arange(*rng[i]) for i, t in enumerate(mesh.types) if t == "wall"]
+                           + [np.zeros(0, np.int64)]).astype(np.int32)
+    iwall = 0
+    for i, t in enumerate(mesh.types):
+        s, e = rng[i]
+        if t in ("inlet", "outlet", "pressure"):
+            kinds[s:e] = 2
+        elif (t == "wall" and wall_frac[i] == 1.0) or t == "wallQFlux":
+            kinds[s:e] = 0 if t == "wall" else 4
+            if iwall + (e - s) > walls.size:
+                raise ValueError("the reference's wall counter runs past its wall faces")
+            smap[s:e] = walls[iwall:iwall + (e - s)]
+            iwall += e - s
+        elif t == "wall" and wall_frac[i] == 0.0:
+            kinds[s:e] = 1
+        elif t == "periodic":
+            raise NotImplementedError("periodic patches")
+    return kinds, smap
 
 
 _SOLVERS = {"iccg": "orc_iccg", "dpcg": "orc_dpcg", "bicgstab": "orc_bicgstab",

@@ -15,6 +15,12 @@
 !   nonorth    gamma grad su0 su_out         (facefluxsc with zero mass flux:
 !              the explicit non-orthogonal part only, scalar_fluxes.f90:99-109,139)
 !   update_boundary phi phi_out               (updateBoundary.f90)
+!   temperature T To Too den vis flmass wallfrac viscos pranl sigt urf gds dt
+!               tscheme scheme solver itr_max tol_abs tol_rel gradmethod
+!               T_out a_out su_out sp_out grad_out info_out
+!              (calculate_temperature, temperature.f90:35-385, on the module
+!               globals set from the files; tscheme steady|bdf|bdf2|cn,
+!               gradmethod gauss|lsq|lsq_dm|lsq_qr, wallfrac = bcDFraction(7,:))
 !   solve_explicit solver n nnz itr_max tol_abs tol_rel ia ja diag a fi0 rhs fi_out info_out
 !              (explicit-array entry points iccg/dpcg/bicgstab/GaussSeidel on a
 !               CSR system of its own, 1-based int32 ia/ja/diag files)
@@ -71,6 +77,8 @@ program ref_driver
       call job_nonorth(ju)
     case ('update_boundary')
       call job_update_boundary(ju)
+    case ('temperature')
+      call job_temperature(ju)
     case ('solve_explicit')
       call job_solve_explicit(ju)
     case ('ilu')
@@ -386,6 +394,70 @@ contains
     call updateBoundary(phi)
     call wr8(fout, phi, numTotal)
     deallocate(phi)
+  end subroutine
+
+  ! One call of the reference's calculate_temperature: every module global it
+  ! reads is set from the job line, then the assembled system (a, su, sp),
+  ! the updated field, its gradient and the solver's res0 are written.
+  subroutine job_temperature(ju)
+    use variables, only: T, To, Too, dTdxi, den, vis, flmass
+    use temperature
+    integer, intent(in) :: ju
+    character(len=64) :: c, fT, fTo, fToo, fden, fvis, ffm, fwf, fTout, fa, fsu, fsp, fgr, finfo
+    character(len=30) :: tsch, schm, slv, gmeth
+    real(dp) :: vsc, prn, sgt, urf, gds, dtt, tabs, trel
+    integer :: itmax, uu
+    real(dp), allocatable :: wf(:)
+    read(ju, *) c, fT, fTo, fToo, fden, fvis, ffm, fwf, vsc, prn, sgt, urf, gds, dtt, tsch, schm, &
+                slv, itmax, tabs, trel, gmeth, fTout, fa, fsu, fsp, fgr, finfo
+    if (allocated(T)) deallocate(T, To, Too, dTdxi, den, vis, flmass)
+    allocate(T(numTotal), To(numTotal), Too(numTotal), dTdxi(3,numTotal), den(numTotal), &
+             vis(numTotal), flmass(numFaces), wf(numBoundaries))
+    To = 0.0_dp; Too = 0.0_dp; den = 0.0_dp; vis = 0.0_dp; dTdxi = 0.0_dp
+    call rd8(fT, T, numTotal)
+    call rd8(fTo, To, numCells)
+    call rd8(fToo, Too, numCells)
+    call rd8(fden, den, numCells)
+    call rd8(fvis, vis, numCells)
+    call rd8(ffm, flmass, numFaces)
+    call rd8(fwf, wf, numBoundaries)
+    bcDFraction(7, 1:numBoundaries) = wf
+    viscos = vsc
+    pranl = prn
+    sigt = sgt
+    urfT = urf
+    gdsT = gds
+    timestep = dtt
+    ltransient = trim(tsch) /= 'steady'
+    bdf = trim(tsch) == 'bdf'
+    bdf2 = trim(tsch) == 'bdf2'
+    bdf3 = .false.
+    cn = trim(tsch) == 'cn'
+    lturb = .false.
+    cSchemeT = schm
+    lSolverT = slv
+    maxiterT = itmax
+    tolAbsT = tabs
+    tolRelT = trel
+    lstsq = trim(gmeth) == 'lsq'
+    lstsq_dm = trim(gmeth) == 'lsq_dm'
+    lstsq_qr = trim(gmeth) == 'lsq_qr'
+    gauss = trim(gmeth) == 'gauss'
+    node_gauss = .false.
+    limiter = 'none'
+    if (lstsq .or. lstsq_dm .or. lstsq_qr) call create_lsq_grad_matrix
+    res = 0.0_dp; reso = 0.0_dp; pk = 0.0_dp; zk = 0.0_dp; d = 0.0_dp; uk = 0.0_dp; vk = 0.0_dp
+    resor = 0.0_dp
+    call calculate_temperature
+    call wr8(fTout, T, numTotal)
+    call wr8(fa, a, nnz)
+    call wr8(fsu, su, numCells)
+    call wr8(fsp, sp, numCells)
+    call wr8(fgr, dTdxi, 3*numTotal)
+    open(newunit=uu, file=trim(finfo), status='replace')
+    write(uu,'(es25.17)') resor(7)
+    close(uu)
+    deallocate(wf)
   end subroutine
 
 end program ref_driver

@@ -121,6 +121,66 @@ def bc_case():
     print("wrote hexbc")
 
 
+TRANSPORT_MESHES = {
+    # adiabatic wall before an isothermal one: the reference's srdw counter
+    # then reads the adiabatic faces' |S|/dn for the isothermal ones
+    "A": (("inlet", "outlet", "wall", "wall", "symmetry", "pressure"), (0, 0, 0, 1, 0, 0)),
+    "B": (("wall", "wall", "wallQFlux", "outlet", "inlet", "empty"), (1, 0, 0, 0, 0, 0)),
+}
+# (mesh, time scheme, convection scheme, solver, itr_max, tol_rel, gradient, urf, gds)
+TRANSPORT_CASES = [
+    ("A", "bdf2", "linearUpwind", "bicgstab", 10, 0.02, "gauss", 0.9, 1.0),
+    ("A", "cn", "cds", "bicgstab", 20, 1e-6, "gauss", 0.7, 0.5),
+    ("A", "steady", "linearUpwind", "gauss-seidel", 5, 1e-6, "lsq", 0.8, 1.0),
+    ("A", "bdf", "linearUpwind", "bicgstab", 10, 1e-4, "lsq_dm", 1.0, 1.0),
+    ("B", "cn", "linearUpwind", "bicgstab", 10, 1e-8, "gauss", 0.9, 1.0),
+]
+
+
+def fd(x: float) -> str:
+    """A REAL(8) literal of x for list-directed input."""
+    return f"{x:.17e}".replace("e", "d")
+
+
+def transport_case():
+    """calculate_temperature (temperature.f90:35-385) itself, on jittered 5^3
+    hexes with every non-periodic boundary treatment it has: the reference's
+    module globals are set from the inputs, the routine runs once, and the
+    assembled system, the updated field, its gradient and the solver's resor
+    (what csrsolve returns in res0) are kept."""
+    out = {}
+    for key, (types, wf) in TRANSPORT_MESHES.items():
+        mesh = hex_mesh(5, jitter=0.2, seed=9, types=types)
+        rc = RefCase(mesh)
+        rc.dump()
+        n, nt, F = mesh.n_cells, mesh.n_total, mesh.n_faces
+        T = uniform(7, 1, nt, 0.0, 1.0)
+        To = uniform(7, 2, n, 0.0, 1.0)
+        Too = uniform(7, 3, n, 0.0, 1.0)
+        den = 1.0 + 0.1 * uniform(7, 4, n, 0.0, 1.0)
+        vis = 1e-3 * (1.0 + uniform(7, 5, n, 0.0, 1.0))
+        flm = 1e-2 * uniform(7, 6, F, -1.0, 1.0)
+        ins = dict(T=T, To=To, Too=Too, den=den, vis=vis, flm=flm, wf=np.array(wf, np.float64))
+        for k, v in ins.items():
+            rc.put(k + ".bin", v)
+            out[f"{key}_in_{k}"] = v
+        for ic, (mk, ts, sch, slv, itm, trel, gm, urf, gds) in enumerate(TRANSPORT_CASES):
+            if mk != key:
+                continue
+            rc.run([f"temperature T.bin To.bin Too.bin den.bin vis.bin flm.bin wf.bin 1.0d-3 0.7d0 "
+                    f"1.0d0 {fd(urf)} {fd(gds)} 1.0d-2 {ts} {sch} {slv} {itm} 1.0d-13 {fd(trel)} "
+                    f"{gm} Tn.bin a.bin su.bin sp.bin gr.bin info.txt"])
+            out[f"c{ic}_T"] = rc.get("Tn.bin")
+            out[f"c{ic}_a"] = rc.get("a.bin")
+            out[f"c{ic}_su"] = rc.get("su.bin")
+            out[f"c{ic}_sp"] = rc.get("sp.bin")
+            out[f"c{ic}_grad"] = rc.get("gr.bin").reshape(-1, 3)
+            out[f"c{ic}_resor"] = np.array(float(open(os.path.join(rc.dir, "info.txt")).read()))
+        rc.close()
+    np.savez_compressed(os.path.join(HERE, "transport.npz"), **out)
+    print("wrote transport")
+
+
 def cavity_case():
     """The reference's own test mesh (test/testFieldOperations/polyMesh).
     The reference's OpenFOAM reader aborts under the flang runtime
@@ -306,6 +366,7 @@ CASES = {
         hex_mesh(5, jitter=0.2, seed=2,
                  types=("wall", "symmetry", "outlet", "wall", "inlet", "pressure")), "hex5j"),
     "hexbc": bc_case,
+    "transport": transport_case,
     "cavity": cavity_case,
     "kat5x5": kat_5x5_case,
     "poisson32": poisson32_case,

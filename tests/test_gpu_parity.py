@@ -377,14 +377,22 @@ def test_convdiff_urf_ddt_vs_oracle(api):
     a = np.zeros(csr["ja"].size)
     ph = phi.copy()
     ctx.convdiff(flm, dco, ph, gr, 0.8, 1, bk, su, sp, urf=0.7, a_out=a)
-    # oracle composition: faces into cleared su/sp, then add the time terms
-    # before the diagonal -- reproduce by running the oracle with su0/sp0 folded
-    a_o, su_o, sp_o, ph_o = o.convdiff(flm, dco, phi, gr, 0.8, 1, bk, urf=1.0)
+    # the oracle runs calculate_temperature's order (temperature.f90:82-366):
+    # time terms, faces, boundary faces, then the under-relaxed diagonal
+    # a_ii = (sp - sum_off)/urf and su += (1 - urf) a_ii phi_i
+    a_o, su_o, sp_o, ph_o = o.transport_assemble(flm, dco, phi, gr, 0.8, 1, bk, urf=0.7,
+                                                 time_scheme=2, den=den, dt=0.01, phi_o=fo,
+                                                 phi_oo=foo)
     assert np.array_equal(ph, ph_o)
-    off = np.ones(a.size, bool)
-    off[csr["diag"]] = False
-    assert np.array_equal(a[off], a_o[off])
-    assert np.allclose(sp, sp_o + sp0, rtol=1e-14)
+    assert np.array_equal(a, a_o)
+    assert np.array_equal(su, su_o)
+    assert np.array_equal(sp, sp_o)
+    # the URF terms are really there: with urf = 1 the diagonal and su differ
+    a_1, su_1, _, _ = o.transport_assemble(flm, dco, phi, gr, 0.8, 1, bk, urf=1.0, time_scheme=2,
+                                           den=den, dt=0.01, phi_o=fo, phi_oo=foo)
+    dg = csr["diag"]
+    assert np.allclose(a[dg], a_1[dg] / 0.7, rtol=1e-14)
+    assert not np.allclose(su, su_1, rtol=1e-6)
     ctx.close()
 
 
