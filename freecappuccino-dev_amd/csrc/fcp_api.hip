@@ -266,6 +266,10 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
         if (ctx->have_csr) return fcp_build_tile_schedules(ctx);
       }
       return 0;
+    case FCP_TUNE_SPMV:
+      if (value < 0 || value > 3) return fcp_fail(ctx, FCP_ERR_ARG, "SpMV variant is 0..3");
+      ctx->tune_spmv = value;
+      return 0;
     case FCP_TUNE_GRAPH:
       if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "graph replay is 0 or 1");
       ctx->tune_graph = value;

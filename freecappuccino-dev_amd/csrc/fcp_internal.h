@@ -287,6 +287,7 @@ struct fcp_ctx {
   int tune_sweep_tile = 0;     // rows per geometric tile (0 = auto, 1 = no tiles)
   int tune_inner_tile = 0;     // rows per tile of the tile-local preconditioner (0 = auto)
   int tune_graph = 0;          // inner iterations replayed from a captured HIP graph
+  int tune_spmv = 0;           // standalone SELL SpMV variant (FCP_TUNE_SPMV)
   // captured pair of tile-order inner iterations and the launch arguments it holds
   hipGraphExec_t tin_graph = nullptr;
   std::vector<const void *> tin_graph_key;

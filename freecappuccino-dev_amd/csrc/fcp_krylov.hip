@@ -542,6 +542,61 @@ __global__ void k_spmv_row(int n, Sell<T> m, const T *__restrict__ x, T *__restr
   y[i] = sell_row<T>(m, i, x, (T)0, false);
 }
 
+// The same product with R rows per thread (rows blk*R*BLK + r*BLK + tid, r <
+// R, so every load stays coalesced) and, with NT, non-temporal loads of the
+// matrix: its values and columns are read once and should not evict the
+// gathered x lines from L2 and the Infinity Cache.  Per-row sums are those of
+// sell_row (same entries, same order).
+template <typename T, bool NT>
+__device__ __forceinline__ T sell_ld(const T *p) {
+  if constexpr (NT)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+template <typename T, bool NT, int R>
+__global__ __launch_bounds__(BLK) void k_spmv_rows(int n, Sell<T> m, const T *__restrict__ x,
+                                                   T *__restrict__ y) {
+  const int b = (int)blockIdx.x, per = (int)gridDim.x >> 3;
+  const int blk = (b & 7) * per + (b >> 3);
+  int row[R], p[R], len[R], L[R];
+  T av[R][SELL_KU], xv[R][SELL_KU];
+  int cv[R][SELL_KU];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    row[r] = (blk * R + r) * BLK + (int)threadIdx.x;
+    const int i = row[r] < n ? row[r] : n - 1;
+    const int sl = i >> 6, b0 = m.base[sl];
+    p[r] = b0 + (i & 63);
+    L[r] = (m.base[sl + 1] - b0) >> 6;
+    len[r] = m.ia[i + 1] - m.ia[i];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k = 0; k < SELL_KU; ++k)
+      if (k < L[r]) {
+        av[r][k] = sell_ld<T, NT>(m.a + p[r] + 64 * k);
+        cv[r][k] = sell_ld<int, NT>(m.col + p[r] + 64 * k);
+      }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int k = 0; k < SELL_KU; ++k)
+      if (k < len[r]) xv[r][k] = x[cv[r][k]];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    T acc = (T)0;
+#pragma unroll
+    for (int k = 0; k < SELL_KU; ++k)
+      if (k < len[r]) acc = acc + av[r][k] * xv[r][k];
+    for (int k = SELL_KU; k < len[r]; ++k)
+      acc = acc + sell_ld<T, NT>(m.a + p[r] + 64 * k) * x[sell_ld<int, NT>(m.col + p[r] + 64 * k)];
+    if (row[r] < n) y[row[r]] = acc;
+  }
+}
+
 // res = b - A x (row order as :417-422) and the L1 / squared L2 norms of res
 // and the squared L2 norm of b.  The last block stores res0, l2, l2b and
 // initialises the Krylov scalars.
@@ -1018,7 +1073,20 @@ int fcp_launch_spmv64(fcp_ctx *ctx, const double *, const double *x, double *y) 
   Sell<double> m;
   FCP_TRY(sell_view<double>(ctx, &m));
   ProfScope ps(ctx, "spmv_sell_f64");
-  k_spmv_row<double><<<(grid_for(ctx->n, BLK) + 7) / 8 * 8, BLK, 0, ctx->stream>>>(ctx->n, m, x, y);
+  const int n = ctx->n;
+  switch (ctx->tune_spmv) {
+    case 1:
+      k_spmv_rows<double, true, 1><<<(grid_for(n, BLK) + 7) / 8 * 8, BLK, 0, ctx->stream>>>(n, m, x, y);
+      break;
+    case 2:
+      k_spmv_rows<double, true, 2><<<(grid_for(n, 2 * BLK) + 7) / 8 * 8, BLK, 0, ctx->stream>>>(n, m, x, y);
+      break;
+    case 3:
+      k_spmv_rows<double, false, 2><<<(grid_for(n, 2 * BLK) + 7) / 8 * 8, BLK, 0, ctx->stream>>>(n, m, x, y);
+      break;
+    default:
+      k_spmv_row<double><<<(grid_for(n, BLK) + 7) / 8 * 8, BLK, 0, ctx->stream>>>(n, m, x, y);
+  }
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }

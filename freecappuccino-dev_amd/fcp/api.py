@@ -35,6 +35,7 @@ FCP_TUNE_INNER_PRECOND = 6
 FCP_TUNE_INNER_TILE = 7
 FCP_TUNE_GRAPH = 8
 FCP_TUNE_INNER_COARSE = 9
+FCP_TUNE_SPMV = 10
 
 
 class FcpError(RuntimeError):
@@ -113,6 +114,23 @@ class Context:
         rc = self.lib.fcp_mesh_upload(self.h, mesh.n_nodes, mesh.n_cells, mesh.n_inner, mesh.n_bnd,
                                       mesh.n_patches, *ptrs)
         self._check(rc, "fcp_mesh_upload")
+
+    def read_mesh_file(self, case_dir: str, fmt: str = "native"):
+        """read_mesh from <case_dir>/polyMesh in the native or OpenFOAM format
+        (parsed by the library).  Returns (numNodes, numCells, numInnerFaces,
+        numBoundaryFaces, numBoundaries) and the patches (nfaces, startFace,
+        bctype codes)."""
+        sizes = np.zeros(5, np.int32)
+        b = case_dir.encode()
+        rc = self.lib.fcp_read_polymesh(self.h, b, len(b), {"native": 0, "foam": 1}[fmt],
+                                    _ptr(sizes)[0])
+        self._check(rc, "fcp_read_polymesh")
+        nb = int(sizes[4])
+        nf, sf, bt = (np.zeros(max(nb, 1), np.int32) for _ in range(3))
+        self._check(self.lib.fcp_mesh_patches(self.h, _ptr(nf)[0], _ptr(sf)[0], _ptr(bt)[0]),
+                    "fcp_mesh_patches")
+        self.mesh = None
+        return tuple(int(v) for v in sizes), (nf[:nb], sf[:nb], bt[:nb])
 
     def geometry(self):
         m = self.mesh

@@ -10,10 +10,10 @@
 program drop_in_poisson
   use fcp_ref
   implicit none
-  integer :: N, M, nNodes, nCells, nInner, nBnd, nFaces, i, j, k, c, f, p, u
+  integer :: N, M, nNodes, nCells, nInner, nBnd, nFacesAll, i, j, k, c, f, p, u
   real(dp), allocatable :: xyz(:,:), mu(:), phi(:), fi(:), fi2(:), rhs(:), g(:,:)
   integer, allocatable :: fnn(:), node(:,:), owner(:), neighbour(:)
-  integer :: pnf(6), startFace(6), bctype(6)
+  integer :: pnf(6), pStart(6), bctype(6)
   character(len=256) :: arg, outfile
   real(dp) :: res0, resor
 
@@ -24,8 +24,8 @@ program drop_in_poisson
   nCells = N**3
   nInner = 3*N*N*(N-1)
   nBnd = 6*N*N
-  nFaces = nInner + nBnd
-  allocate(xyz(3,nNodes), fnn(nFaces), node(FCP_NOMAX,nFaces), owner(nFaces), neighbour(nInner))
+  nFacesAll = nInner + nBnd
+  allocate(xyz(3,nNodes), fnn(nFacesAll), node(FCP_NOMAX,nFacesAll), owner(nFacesAll), neighbour(nInner))
   do k = 0, N
     do j = 0, N
       do i = 0, N
@@ -52,7 +52,7 @@ program drop_in_poisson
     endif
   end do
   do p = 1, 6                                ! boundary patches, faces by owner
-    startFace(p) = f
+    pStart(p) = f
     pnf(p) = N*N
     bctype(p) = 1
     do c = 0, nCells - 1
@@ -76,7 +76,7 @@ program drop_in_poisson
   end do
 
   call fcp_read_mesh(nNodes, nCells, nInner, nBnd, 6, xyz, fnn, node, owner, neighbour, &
-                     pnf, startFace, bctype)
+                     pnf, pStart, bctype)
   call create_CSR_matrix
 
   ! poisson.f90:62-75: source, zero Dirichlet values, mu = -1

@@ -13,6 +13,10 @@ module fcp_capi
   integer(c_int), parameter :: FCP_NOMAX = 6
   integer(c_int), parameter :: FCP_PREC_FP64 = 0, FCP_PREC_MIXED = 1
   integer(c_int), parameter :: FCP_CRIT_REF_L1 = 0, FCP_CRIT_TRUE_L2 = 1
+  ! gradient methods (FCP_GRAD_* of the C header; Fortran names are case-blind,
+  ! so FCP_GRAD_GAUSS would clash with fcp_grad_gauss)
+  integer(c_int), parameter :: FCP_GM_GAUSS = 0, FCP_GM_LSQ = 1, FCP_GM_LSQ_DM = 2, &
+                               FCP_GM_LSQ_QR = 3
 
   type, bind(C) :: fcp_solve_opts
     integer(c_int) :: precision
@@ -105,6 +109,59 @@ module fcp_capi
       real(c_double) :: gamma_f(*), grad(*), su(*)
     end function
 
+    integer(c_int) function fcp_read_polymesh(ctx, case_dir, case_len, format, sizes) &
+        bind(C, name='fcp_read_polymesh')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: case_dir(*)
+      integer(c_int), value :: case_len, format
+      integer(c_int) :: sizes(5)
+    end function
+
+    integer(c_int) function fcp_mesh_patches(ctx, nfaces, startFace, bctype) &
+        bind(C, name='fcp_mesh_patches')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: nfaces(*), startFace(*), bctype(*)
+    end function
+
+    integer(c_int) function fcp_convdiff(ctx, flmass, dcoef, phi, grad, gam, scheme, bc_kind, urf, &
+        a, su, sp) bind(C, name='fcp_convdiff')
+      import :: c_int, c_double, c_ptr, c_signed_char
+      type(c_ptr), value :: ctx
+      real(c_double) :: flmass(*), dcoef(*), phi(*), grad(*), su(*), sp(*)
+      real(c_double), value :: gam, urf
+      integer(c_int), value :: scheme
+      integer(c_signed_char) :: bc_kind(*)
+      type(c_ptr), value :: a  ! c_null_ptr or c_loc of a(nnz)
+    end function
+
+    integer(c_int) function fcp_ddt(ctx, scheme, den, dt, phi_o, phi_oo, su, sp) &
+        bind(C, name='fcp_ddt')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: scheme
+      real(c_double) :: den(*), phi_o(*), su(*), sp(*)
+      real(c_double), value :: dt
+      type(c_ptr), value :: phi_oo  ! c_null_ptr unless scheme 2
+    end function
+
+    integer(c_int) function fcp_transport(ctx, phi, phi_o, phi_oo, den, vis, flmass, wall_frac, &
+        viscos, pranl, sigt, urf, gds, dt, time_scheme, conv_scheme, grad_method, solver, &
+        solver_len, itr_max, tol_abs, tol_rel, clip, grad, a, su, sp, resor, fimin, fimax, opts) &
+        bind(C, name='fcp_transport')
+      import :: c_int, c_double, c_ptr, c_char, fcp_solve_opts
+      type(c_ptr), value :: ctx
+      real(c_double) :: phi(*), vis(*), flmass(*), wall_frac(*)
+      type(c_ptr), value :: phi_o, phi_oo, den  ! c_null_ptr when not needed
+      real(c_double), value :: viscos, pranl, sigt, urf, gds, dt, tol_abs, tol_rel
+      integer(c_int), value :: time_scheme, conv_scheme, grad_method, solver_len, itr_max, clip
+      character(kind=c_char) :: solver(*)
+      type(c_ptr), value :: grad, a, su, sp  ! optional outputs
+      real(c_double) :: resor, fimin, fimax
+      type(fcp_solve_opts) :: opts
+    end function
+
     integer(c_int) function fcp_matrix_transpose(ctx) bind(C, name='fcp_matrix_transpose')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
@@ -148,10 +205,10 @@ module fcp_capi
 
     integer(c_int) function fcp_create_lsq_grad_matrix(ctx, method, compat, dmat) &
         bind(C, name='fcp_create_lsq_grad_matrix')
-      import :: c_int, c_double, c_ptr
+      import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: method, compat
-      real(c_double) :: dmat(*)
+      type(c_ptr), value :: dmat  ! c_null_ptr or c_loc of a real(c_double) array
     end function
 
     integer(c_int) function fcp_grad(ctx, method, phi, dphidxi) bind(C, name='fcp_grad')
@@ -283,6 +340,16 @@ module fcp_capi
       integer(c_int), value :: n, nnz, itr_max, mr, verbose
       integer(c_int) :: ia(*), ja(*), diag(*)
       real(c_double) :: a(*), fi(*), rhs(*), resor
+      real(c_double), value :: tol_abs, tol_rel
+    end function
+
+    ! device arrays only (c_loc of device memory, e.g. from hipMalloc)
+    integer(c_int) function fcp_dpcg_noalloc(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, &
+        tol_abs, tol_rel, stream) bind(C, name='fcp_dpcg_noalloc')
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: n, nnz, itr_max
+      type(c_ptr), value :: ia, ja, a, diag, fi, rhs, stream
+      real(c_double) :: resor
       real(c_double), value :: tol_abs, tol_rel
     end function
 

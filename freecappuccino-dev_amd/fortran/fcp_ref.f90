@@ -12,9 +12,20 @@
 !                          tol_abs, tol_rel, chvar, verbose)  linear_solvers.f90:364,206,548,96
 !   pmgmres_ilu(n, nz_num, ia, ja, a, ua, x, rhs, resor, itr_max, mr, tol_abs, tol_rel,
 !               chvar, verbose)                               linear_solvers.f90:789
-! The mesh is handed over once with fcp_read_mesh (the arrays read_mesh
-! produces, geometry.f90:118-406); geometry is computed on the GPU and
-! returned in the module arrays below (geometry.f90:19-83 names).
+!   spsolve(solver, fi, rhs, res0, itr_max, tol_abs, tol_rel, chvar)
+!                              test/test_linear_solvers_spsolve.f90:103 (csrsolve's body on
+!                              the module arrays; also 'bicgstab_ilu' and 'pmgmres_ilu')
+!   read_mesh([case_dir])      geometry.f90:90 (mesh_format 'nativeMesh' or 'foamMesh')
+!   grad(phi, dPhidxi) / grad(U, V, W, dUdxi, dVdxi, dWdxi)   gradients.f90:106-214
+!   create_lsq_grad_matrix, grad_lsq, grad_lsq_dm, grad_lsq_qr gradients.f90:72,782,1334,1063
+!   updateBoundary(phi)        updateBoundary.f90:1
+!   fvm_laplacian / fvm_div / fvm_ddt   face-Gamma Laplacian, facefluxsc assembly
+!                              (temperature.f90:118-366), time terms (:82-102)
+!   calculate_scalar(...)      calculate_temperature (temperature.f90:35-385) for a
+!                              scalar with explicit parameters
+! The mesh is handed over with read_mesh (from disk) or fcp_read_mesh (the
+! arrays read_mesh produces, geometry.f90:118-406); geometry is computed on
+! the GPU and returned in the module arrays below (geometry.f90:19-83 names).
 module fcp_ref
   use iso_c_binding
   use fcp_capi
@@ -30,11 +41,28 @@ module fcp_ref
   real(dp), allocatable :: xc(:), yc(:), zc(:), vol(:)
   real(dp), allocatable :: arx(:), ary(:), arz(:), xf(:), yf(:), zf(:), facint(:), Df(:)
 
+  ! patches of the resident mesh (geometry.f90:64-80; types as FCP_BC_* codes)
+  integer :: numBoundaries = 0
+  integer, allocatable :: nfaces(:), startFace(:), bctype_code(:)
+  character(len=10), save :: mesh_format = 'nativeMesh'  ! or 'foamMesh' (geometry.f90:92)
+
   ! sparse_matrix module globals (sparse_matrix.f90:10-26)
   integer :: nnz = 0
   integer, allocatable :: ia(:), ja(:), diag(:)
   integer, allocatable :: icell_jcell_csr_index(:), jcell_icell_csr_index(:)
-  real(dp), allocatable :: a(:), su(:)
+  real(dp), allocatable, target :: a(:), su(:), sp(:)
+
+  ! gradients module globals (gradients.f90:9-10); lsq_compat keeps the
+  ! reference's least-squares arithmetic, defects included
+  logical, save :: lstsq = .false., lstsq_qr = .false., lstsq_dm = .false., gauss = .true.
+  logical, save :: node_gauss = .false.
+  character(len=20), save :: limiter = 'none'
+  logical, save :: lsq_compat = .true.
+
+  interface grad
+    module procedure grad_scalar_field
+    module procedure grad_vector_field
+  end interface
 
 contains
 
@@ -81,17 +109,51 @@ contains
     deallocate(dnb, srdb)
   end subroutine
 
+  ! read_mesh (geometry.f90:90-116): the case's polyMesh files, parsed and
+  ! uploaded by the library; geometry into the module arrays.
+  subroutine read_mesh(case_dir, device)
+    character(len=*), intent(in), optional :: case_dir
+    integer, intent(in), optional :: device
+    character(len=1024) :: cdir
+    integer(c_int) :: sizes(5), fmt, dev
+    real(dp), allocatable :: dnb(:), srdb(:)
+    cdir = '.'
+    if (present(case_dir)) cdir = case_dir
+    dev = 0
+    if (present(device)) dev = device
+    fmt = 0
+    if (trim(mesh_format) == 'foamMesh') fmt = 1
+    if (.not. c_associated(ctx)) call fcp_check(fcp_init(dev, ctx), 'fcp_init')
+    call fcp_check(fcp_read_polymesh(ctx, trim(cdir), len_trim(cdir), fmt, sizes), 'fcp_read_polymesh')
+    numCells = sizes(2); numInnerFaces = sizes(3); numBoundaryFaces = sizes(4)
+    numFaces = numInnerFaces + numBoundaryFaces; numTotal = numCells + numBoundaryFaces
+    numBoundaries = sizes(5)
+    if (allocated(nfaces)) deallocate(nfaces, startFace, bctype_code)
+    allocate(nfaces(max(numBoundaries,1)), startFace(max(numBoundaries,1)), &
+             bctype_code(max(numBoundaries,1)))
+    call fcp_check(fcp_mesh_patches(ctx, nfaces, startFace, bctype_code), 'fcp_mesh_patches')
+    if (allocated(xc)) deallocate(xc, yc, zc, vol, arx, ary, arz, xf, yf, zf, facint, Df)
+    allocate(xc(numCells), yc(numCells), zc(numCells), vol(numCells))
+    allocate(arx(numFaces), ary(numFaces), arz(numFaces), xf(numFaces), yf(numFaces), zf(numFaces))
+    allocate(facint(max(numInnerFaces,1)), Df(max(numInnerFaces,1)), dnb(max(numBoundaryFaces,1)), &
+             srdb(max(numBoundaryFaces,1)))
+    call fcp_check(fcp_geometry(ctx, xc, yc, zc, vol, arx, ary, arz, xf, yf, zf, facint, Df, &
+                                dnb, srdb), 'fcp_geometry')
+    deallocate(dnb, srdb)
+  end subroutine
+
   subroutine create_CSR_matrix
     integer(c_int) :: n
-    if (allocated(ia)) deallocate(ia, ja, diag, icell_jcell_csr_index, jcell_icell_csr_index, a, su)
+    if (allocated(ia)) deallocate(ia, ja, diag, icell_jcell_csr_index, jcell_icell_csr_index, a, su, sp)
     nnz = 2*numInnerFaces + numCells                      ! sparse_matrix.f90:110
-    allocate(ia(numCells+1), ja(nnz), diag(numCells), a(nnz), su(numCells))
+    allocate(ia(numCells+1), ja(nnz), diag(numCells), a(nnz), su(numCells), sp(numCells))
     allocate(icell_jcell_csr_index(max(numInnerFaces,1)), jcell_icell_csr_index(max(numInnerFaces,1)))
     call fcp_check(fcp_csr_build(ctx, n, ia, ja, diag, icell_jcell_csr_index, jcell_icell_csr_index), &
                    'fcp_csr_build')
     nnz = n
     a = 0.0_dp
     su = 0.0_dp
+    sp = 0.0_dp
   end subroutine
 
   subroutine laplacian(mu, phi)
@@ -104,6 +166,144 @@ contains
     real(dp), intent(inout) :: dudxi(3, numTotal)
     call fcp_check(fcp_grad_gauss(ctx, u, dudxi), 'fcp_grad_gauss')
   end subroutine
+
+  ! --- gradients (gradients.f90:72-214) ----------------------------------
+  integer(c_int) function grad_method()
+    grad_method = FCP_GM_GAUSS
+    if (lstsq) then
+      grad_method = FCP_GM_LSQ
+    elseif (lstsq_qr) then
+      grad_method = FCP_GM_LSQ_QR
+    elseif (lstsq_dm) then
+      grad_method = FCP_GM_LSQ_DM
+    endif
+  end function
+
+  subroutine create_lsq_grad_matrix
+    integer(c_int) :: m, cm
+    m = grad_method()
+    if (m == FCP_GM_GAUSS) return
+    cm = 0
+    if (lsq_compat) cm = 1
+    call fcp_check(fcp_create_lsq_grad_matrix(ctx, m, cm, c_null_ptr), 'fcp_create_lsq_grad_matrix')
+  end subroutine
+
+  subroutine grad_scalar_field(phi, dPhidxi)
+    real(dp), intent(in) :: phi(numTotal)
+    real(dp), intent(inout) :: dPhidxi(3, numTotal)
+    if (node_gauss .or. (trim(limiter) /= 'none' .and. trim(limiter) /= 'no-limit')) then
+      write(*,'(a)') '  grad: node-based Gauss and gradient limiters are not provided'
+      stop 1
+    endif
+    call fcp_check(fcp_grad(ctx, grad_method(), phi, dPhidxi), 'fcp_grad')
+  end subroutine
+
+  subroutine grad_vector_field(U, V, W, dUdxi, dVdxi, dWdxi)
+    real(dp), intent(in) :: U(numTotal), V(numTotal), W(numTotal)
+    real(dp), intent(inout) :: dUdxi(3, numTotal), dVdxi(3, numTotal), dWdxi(3, numTotal)
+    call grad_scalar_field(U, dUdxi)
+    call grad_scalar_field(V, dVdxi)
+    call grad_scalar_field(W, dWdxi)
+  end subroutine
+
+  subroutine grad_lsq(Phi, dPhidxi)
+    real(dp), intent(in) :: Phi(numTotal)
+    real(dp), intent(inout) :: dPhidxi(3, numTotal)
+    call fcp_check(fcp_grad(ctx, FCP_GM_LSQ, Phi, dPhidxi), 'fcp_grad')
+  end subroutine
+
+  subroutine grad_lsq_dm(Phi, dPhidxi)
+    real(dp), intent(in) :: Phi(numTotal)
+    real(dp), intent(inout) :: dPhidxi(3, numTotal)
+    call fcp_check(fcp_grad(ctx, FCP_GM_LSQ_DM, Phi, dPhidxi), 'fcp_grad')
+  end subroutine
+
+  subroutine grad_lsq_qr(Phi, dPhidxi)
+    real(dp), intent(in) :: Phi(numTotal)
+    real(dp), intent(inout) :: dPhidxi(3, numTotal)
+    call fcp_check(fcp_grad(ctx, FCP_GM_LSQ_QR, Phi, dPhidxi), 'fcp_grad')
+  end subroutine
+
+  subroutine updateBoundary(phi)
+    real(dp), intent(inout) :: phi(numTotal)
+    call fcp_check(fcp_update_boundary(ctx, phi), 'fcp_update_boundary')
+  end subroutine
+
+  ! --- assembly ------------------------------------------------------------
+  ! Face-diffusivity Laplacian: bc_kind 0 Dirichlet, 1 zero-gradient per
+  ! boundary face; writes a and adds the boundary terms to su.
+  subroutine fvm_laplacian(mu_f, bc_kind, phi)
+    real(dp), intent(in) :: mu_f(numFaces), phi(numTotal)
+    integer(c_signed_char), intent(in) :: bc_kind(numBoundaryFaces)
+    call fcp_check(fcp_laplacian_face(ctx, mu_f, bc_kind, phi, a, su), 'fcp_laplacian_face')
+  end subroutine
+
+  ! facefluxsc over every face with the boundary loop by bc_kind and the
+  ! under-relaxed diagonal (temperature.f90:118-366); su, sp accumulate.
+  subroutine fvm_div(flmass, dcoef, phi, dphidxi, gam, cScheme, bc_kind, urf)
+    real(dp), intent(in) :: flmass(numFaces), dcoef(numFaces), dphidxi(3, numTotal), gam, urf
+    real(dp), intent(inout) :: phi(numTotal)
+    character(len=*), intent(in) :: cScheme
+    integer(c_signed_char), intent(in) :: bc_kind(numBoundaryFaces)
+    integer(c_int) :: sch
+    sch = 1
+    if (trim(cScheme) == 'cds') sch = 0
+    call fcp_check(fcp_convdiff(ctx, flmass, dcoef, phi, dphidxi, gam, sch, bc_kind, urf, &
+                                c_loc(a), su, sp), 'fcp_convdiff')
+  end subroutine
+
+  ! Time terms of temperature.f90:82-102 into su, sp (scheme 1 BDF1 / the
+  ! volume part of Crank-Nicolson, 2 BDF2).
+  subroutine fvm_ddt(scheme, den, dt, phi_o, phi_oo)
+    integer, intent(in) :: scheme
+    real(dp), intent(in) :: den(numCells), dt, phi_o(numCells)
+    real(dp), intent(in), target, optional :: phi_oo(numCells)
+    type(c_ptr) :: poo
+    poo = c_null_ptr
+    if (present(phi_oo)) poo = c_loc(phi_oo)
+    call fcp_check(fcp_ddt(ctx, scheme, den, dt, phi_o, poo, su, sp), 'fcp_ddt')
+  end subroutine
+
+  ! calculate_temperature (temperature.f90:35-385) for a scalar phi with the
+  ! parameters the reference takes from its modules: tscheme 'steady' |
+  ! 'bdf' | 'bdf2' | 'cn', cScheme 'cds' | 'linearUpwind', wall_frac =
+  ! bcDFraction(field,:) per patch; the gradient follows the module flags.
+  ! Leaves the system in a, su, sp and prints the range line as the
+  ! reference does.
+  subroutine calculate_scalar(phi, phi_o, phi_oo, den, vis, flmass, wall_frac, viscos, pranl, &
+                              sigt, urf, gds, dt, tscheme, cScheme, solver, itr_max, tol_abs, &
+                              tol_rel, resor, chvar)
+    real(dp), intent(inout) :: phi(numTotal)
+    real(dp), intent(in), target :: phi_o(numCells), phi_oo(numCells), den(numCells)
+    real(dp), intent(in) :: vis(numCells), flmass(numFaces), wall_frac(numBoundaries)
+    real(dp), intent(in) :: viscos, pranl, sigt, urf, gds, dt, tol_abs, tol_rel
+    character(len=*), intent(in) :: tscheme, cScheme, solver, chvar
+    integer, intent(in) :: itr_max
+    real(dp), intent(out) :: resor
+    type(fcp_solve_opts) :: opts
+    integer(c_int) :: ts, sch, rc
+    real(dp) :: fimin, fimax
+    select case (trim(tscheme))
+    case ('bdf')
+      ts = 1
+    case ('bdf2')
+      ts = 2
+    case ('cn')
+      ts = 3
+    case default
+      ts = 0
+    end select
+    sch = 1
+    if (trim(cScheme) == 'cds') sch = 0
+    call fcp_solve_opts_default(opts)
+    rc = fcp_transport(ctx, phi, c_loc(phi_o), c_loc(phi_oo), c_loc(den), vis, flmass, wall_frac, &
+                       viscos, pranl, sigt, urf, gds, dt, ts, sch, grad_method(), trim(solver), &
+                       len_trim(solver), itr_max, tol_abs, tol_rel, 1, c_null_ptr, c_loc(a), &
+                       c_loc(su), c_loc(sp), resor, fimin, fimax, opts)
+    call fcp_check(rc, 'fcp_transport')
+    write(6,'(2x,es11.4,3a,es11.4)') fimin, ' <= ', trim(chvar), ' <= ', fimax
+  end subroutine
+
 
   subroutine csrsolve(solver, fi, rhs, res0, itr_max, tol_abs, tol_rel, chvar)
     character(len=*), intent(in) :: solver, chvar
@@ -212,6 +412,39 @@ contains
       stop 1
     endif
   end subroutine
+
+  ! The body of csrsolve (linear_solvers.f90:62-88) on the module arrays
+  ! numCells, nnz, ia, ja, a, diag, under the name the reference's solver
+  ! test calls it by; 'bicgstab_ilu' and 'pmgmres_ilu' name the same solvers.
+  subroutine spsolve(solver, fi, rhs, res0, itr_max, tol_abs, tol_rel, chvar)
+    character(len=*), intent(in) :: solver, chvar
+    real(dp), intent(inout) :: fi(*)
+    real(dp), intent(in) :: rhs(numCells)
+    real(dp), intent(out) :: res0
+    integer, intent(in) :: itr_max
+    real(dp), intent(in) :: tol_abs, tol_rel
+    select case (trim(solver))
+    case ('gauss-seidel')
+      call GaussSeidel(numCells, nnz, ia, ja, a, diag, fi(1:numCells), rhs, res0, itr_max, &
+                       tol_abs, tol_rel, chvar, ltest)
+    case ('dpcg')
+      call dpcg(numCells, nnz, ia, ja, a, diag, fi(1:numCells), rhs, res0, itr_max, tol_abs, &
+                tol_rel, chvar, ltest)
+    case ('iccg')
+      call iccg(numCells, nnz, ia, ja, a, diag, fi(1:numCells), rhs, res0, itr_max, tol_abs, &
+                tol_rel, chvar, ltest)
+    case ('bicgstab', 'bicgstab_ilu')
+      call bicgstab(numCells, nnz, ia, ja, a, diag, fi(1:numCells), rhs, res0, itr_max, tol_abs, &
+                    tol_rel, chvar, ltest)
+    case ('pmgmres', 'pmgmres_ilu')
+      call pmgmres_ilu(numCells, nnz, ia, ja, a, diag, fi(1:numCells), rhs, res0, itr_max, 4, &
+                       tol_abs, tol_rel, chvar, ltest)
+    case default
+      write(*,'(3a)') '  spsolve: unknown solver "', trim(solver), '"'
+      stop 1
+    end select
+  end subroutine
+
 
   subroutine fcp_ref_finalize
     if (c_associated(ctx)) call fcp_finalize(ctx)

@@ -99,6 +99,21 @@ int fcp_mesh_upload(fcp_ctx *ctx, int nNodes, int nCells, int nInnerFaces,
                     const int *neighbour, const int *nfaces, const int *startFace,
                     const int *bctype);
 
+/* read_mesh (geometry.f90:90-116) from disk: reads <case_dir>/polyMesh in the
+ * native format (format 0, read_mesh_native :118-406: 1-based text files
+ * size, boundary, points, faces, owner, neighbour) or the OpenFOAM format
+ * (format 1, read_mesh_openfoam :846-1300: 0-based lists under FoamFile
+ * headers, numCells = largest owner/neighbour + 1), both with the boundary
+ * table "name type nFaces startFace" after a header line, and uploads the
+ * mesh as fcp_mesh_upload does.  Boundary types other than the FCP_BC_ names
+ * get code 0 (no treatment).  sizes (optional) receives numNodes, numCells,
+ * numInnerFaces, numBoundaryFaces, numBoundaries. */
+int fcp_read_polymesh(fcp_ctx *ctx, const char *case_dir, int len, int format, int *sizes);
+
+/* Patches of the resident mesh: nfaces, startFace, bctype (FCP_BC_* codes),
+ * each [numBoundaries] and optional. */
+int fcp_mesh_patches(fcp_ctx *ctx, int *nfaces, int *startFace, int *bctype);
+
 /* FP64 geometry on the GPU, same arithmetic as geometry.f90:416-754
  * (interpolation_coeff_variant = 2).  Any output pointer may be NULL.
  * dnb/srdb: normal distance and |S|/dn for every boundary face (the
@@ -413,6 +428,11 @@ int fcp_dpcg_noalloc(int n, int nnz, const int *ia, const int *ja, const double 
                                     the tile indicator; up to 2048 tiles),
                                     which cuts the inner iterations several
                                     times on large meshes; 0: tile-local only */
+#define FCP_TUNE_SPMV 10         /* variant of the standalone FP64 SELL SpMV
+                                    (fcp_spmv): 0 one row per thread, 1 the
+                                    same with non-temporal matrix loads, 2 two
+                                    rows per thread with non-temporal matrix
+                                    loads, 3 two rows per thread             */
 /* Sets a launch-shape parameter of the context; results do not depend on it
  * (FCP_TUNE_INNER_PRECOND and FCP_TUNE_INNER_COARSE change the path of a
  * mixed-precision solve, not its stopping test). */

@@ -820,3 +820,36 @@ def test_coarse_correction_off_for_singular_operator(api):
                             precision=api.FCP_PREC_MIXED, criterion=api.FCP_CRIT_TRUE_L2)
     assert rc == 0 and info["rel_res_l2"] <= 1e-8, info
     ctx.close()
+
+
+@pytest.mark.parametrize("fmt", ["native", "foam"])
+def test_read_mesh_files(api, golden_dir, tmp_path, fmt):
+    """fcp_read_mesh parses polyMesh files (native: a jittered hex with every
+    boundary type written by write_native; OpenFOAM: the reference's cavity
+    test mesh) into the same resident mesh as the Python upload: geometry,
+    CSR and patches bit-equal."""
+    from fcp.mesh import write_native
+    print(f"[case] read_mesh {fmt}", flush=True)
+    if fmt == "native":
+        mesh = hex_mesh(6, jitter=0.2, seed=3,
+                        types=("inlet", "outlet", "wall", "wallQFlux", "symmetry", "empty"))
+        write_native(mesh, str(tmp_path))
+        case = str(tmp_path)
+    else:
+        case = os.path.join(golden_dir, "cavity")
+        mesh = read_foam(case)
+    ref, geo_ref, csr_ref = _setup(api, mesh)
+    ref.close()
+    ctx = api.Context(0)
+    sizes, (nf, sf, bt) = ctx.read_mesh_file(case, fmt)
+    assert sizes == (mesh.n_nodes, mesh.n_cells, mesh.n_inner, mesh.n_bnd, mesh.n_patches)
+    assert np.array_equal(nf, mesh.nfaces) and np.array_equal(sf, mesh.start_face)
+    assert np.array_equal(bt, mesh.bctype_codes)
+    ctx.mesh = mesh
+    geo = ctx.geometry()
+    csr = ctx.create_CSR_matrix()
+    for k in GEOM:
+        assert np.array_equal(geo[k], geo_ref[k]), k
+    for k in ("ia", "ja", "diag", "slot_pn", "slot_np"):
+        assert np.array_equal(csr[k], csr_ref[k]), k
+    ctx.close()
