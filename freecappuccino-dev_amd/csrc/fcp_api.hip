@@ -267,8 +267,16 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
       }
       return 0;
     case FCP_TUNE_SPMV:
-      if (value < 0 || value > 3) return fcp_fail(ctx, FCP_ERR_ARG, "SpMV variant is 0..3");
+      if (value < -1 || value > 4) return fcp_fail(ctx, FCP_ERR_ARG, "SpMV variant is -1..4");
       ctx->tune_spmv = value;
+      return 0;
+    case FCP_TUNE_TILE_PACK:
+      if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "tile packing is 0 or 1");
+      if (value != ctx->tune_tile_pack) {
+        FCP_TRY(fcp_coarse_wait(ctx));
+        ctx->tune_tile_pack = value;
+        ctx->tin_have = false;  // operands rebuilt in the chosen layout
+      }
       return 0;
     case FCP_TUNE_GRAPH:
       if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "graph replay is 0 or 1");
@@ -808,7 +816,7 @@ int fcp_spmv(fcp_ctx *ctx, const double *x, double *y) {
 
 int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d) {
   FCP_TRY(need(ctx, true, false, true));
-  if (!r || !z || kind < 0 || kind > 7)
+  if (!r || !z || kind < 0 || kind > 9)
     return fcp_fail(ctx, FCP_ERR_ARG, "fcp_precond_apply: bad argument");
   if (kind == 2) d = nullptr;  // ILU(0) has no pivot vector to return
   (void)hipSetDevice(ctx->device);
@@ -818,6 +826,8 @@ int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double
   if (d) FCP_TRY(arg_out(ctx, d, ctx->n, dout));
   if (kind == 2)
     FCP_TRY(fcp_ilu_apply_device(ctx, ri.dev, zo.dev));
+  else if (kind >= 8)  // FP64 tile-order kernel: DIC (8), plus the coarse correction (9)
+    FCP_TRY(fcp_bj32_device(ctx, kind == 8 ? 0 : 2, ri.dev, zo.dev, dout.dev, true));
   else if (kind >= 5)
     FCP_TRY(fcp_bj32_device(ctx, kind - 5, ri.dev, zo.dev, dout.dev));
   else if (kind >= 3)

@@ -284,15 +284,16 @@ int pack_block32(fcp_ctx *ctx, BlockSchedule &b);
 // Pivots and packed FP32 operands of the tile-local preconditioner (schedules
 // jlower/jupper, pivots dbj); *ok false when it does not fit this mesh.
 int bj_begin(fcp_ctx *ctx, int kind, bool *ok);
-// Inner FP32 PCG in tile order (fcp_inner.hip).
-bool tin_usable(fcp_ctx *ctx, int nrp);
+// Inner PCG in tile order (fcp_inner.hip): FP32, or FP64 with fp64 set
+// (values per position of sizeof(double) in LDS).
+bool tin_usable(fcp_ctx *ctx, int nrp, bool fp64 = false);
 int tin_pcg(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScalars *so,
             SolveScalars *si, double tol_rel, double inner_tol, int budget, int check_every,
-            int verbose, SolveScalars *h);
+            int verbose, SolveScalars *h, bool fp64 = false);
 int tin_bicgstab(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScalars *so,
                  SolveScalars *si, double tol_rel, double inner_tol, int budget, int check_every,
                  int verbose, SolveScalars *h);
-int tin_apply(fcp_ctx *ctx, const double *r, double *z, bool coarse);
+int tin_apply(fcp_ctx *ctx, const double *r, double *z, bool coarse, bool fp64 = false);
 int tin_coarse_begin(fcp_ctx *ctx);
 int tin_coarse_join(fcp_ctx *ctx);
 int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z);

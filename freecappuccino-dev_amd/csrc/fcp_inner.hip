@@ -26,6 +26,9 @@
 // NR right-hand sides are interleaved (NRP values per position); every
 // column's arithmetic is independent of NR, so a batched solve's columns
 // equal single-column solves bit for bit.
+// The PCG kernels take the value type T of the operands, vectors and sweeps:
+// float for the mixed-precision inner solves, double for FP64 iterations with
+// the same two-level preconditioner (FCP_PREC_FP64_TILE).
 #include <algorithm>
 #include <type_traits>
 
@@ -46,6 +49,26 @@ constexpr int TPD = FCP_TPD;  // sweep steps of operand prefetch
 #define FCP_TK_NT 256
 #endif
 constexpr int TK_NT = FCP_TK_NT;  // threads of a tk_pre workgroup (BLK / TK_NT positions each)
+
+// Four values of type T in one load (the sweep coefficients of a position).
+template <class T>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  using type = float4;
+  __device__ static float4 make(float a, float b, float c, float d) {
+    return make_float4(a, b, c, d);
+  }
+};
+template <>
+struct Vec4<double> {
+  using type = double4;
+  __device__ static double4 make(double a, double b, double c, double d) {
+    return make_double4(a, b, c, d);
+  }
+};
+template <class T>
+using V4 = typename Vec4<T>::type;
 
 template <int NR>
 __device__ __forceinline__ bool tk_all_done(const SolveScalars *s) {
@@ -155,22 +178,65 @@ __global__ void tk_upper_src(int n, int nb, const int *__restrict__ perm,
   for (; m < FCP_PMAXD; ++m) usrc[(size_t)FCP_PMAXD * p + m] = -1;
 }
 
-// FP32 sweep operands of the current matrix in tile order: lower and upper
-// in-tile coefficients (psrc / usrc name their CSR entries) and the pivots.
+// Distances of both sweeps' dependencies for the packed layout, 3 x 10 bits
+// (0: the zero slot `dummy`); err set where a position has a fourth
+// dependency or a distance beyond 1023 (the layout then does not apply).
+__global__ void tk_pack_offsets(int n, int nb, int dummy, const int *__restrict__ bpos,
+                                const int2 *__restrict__ dl, const int2 *__restrict__ du,
+                                unsigned *__restrict__ offl, unsigned *__restrict__ offu,
+                                int *__restrict__ err) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int i = p - bpos[tk_block_of(bpos, nb, p)];
+  bool bad = false;
+#pragma unroll
+  for (int up = 0; up < 2; ++up) {
+    const int2 d = up ? du[p] : dl[p];
+    const int sl[FCP_PMAXD] = {(int)((unsigned)d.x & 0xFFFF), (int)((unsigned)d.x >> 16),
+                               (int)((unsigned)d.y & 0xFFFF), (int)((unsigned)d.y >> 16)};
+    unsigned w = 0;
+    if (sl[3] != dummy) bad = true;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      if (sl[m] == dummy) continue;
+      const int off = up ? sl[m] - i : i - sl[m];
+      if (off < 1 || off > 1023)
+        bad = true;
+      else
+        w |= (unsigned)off << (10 * m);
+    }
+    (up ? offu : offl)[p] = w;
+  }
+  if (bad) atomicOr(err, 1);
+}
+
+template <class T>
+__device__ __forceinline__ T tk_from_bits(unsigned w) {
+  if constexpr (sizeof(T) == 4)
+    return __uint_as_float(w);
+  else
+    return __longlong_as_double((long long)w);
+}
+
+// Sweep operands of the current matrix in tile order, in precision T: lower
+// and upper in-tile coefficients (psrc / usrc name their CSR entries) and the
+// pivots; with offl / offu (packed layout) the fourth component carries the
+// dependency distances instead of a coefficient.
+template <class T>
 __global__ void tk_sweep_vals(int n, const int *__restrict__ perm, const int *__restrict__ psrc,
                               const int *__restrict__ usrc, const double *__restrict__ a,
-                              const double *__restrict__ dbj, float4 *__restrict__ al,
-                              float4 *__restrict__ au, float *__restrict__ dv) {
+                              const double *__restrict__ dbj, const unsigned *__restrict__ offl,
+                              const unsigned *__restrict__ offu, V4<T> *__restrict__ al,
+                              V4<T> *__restrict__ au, T *__restrict__ dv) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int4 ls = reinterpret_cast<const int4 *>(psrc)[p];
   const int4 us = reinterpret_cast<const int4 *>(usrc)[p];
   const int row = perm[p];
-  al[p] = make_float4(ls.x >= 0 ? (float)a[ls.x] : 0.0f, ls.y >= 0 ? (float)a[ls.y] : 0.0f,
-                      ls.z >= 0 ? (float)a[ls.z] : 0.0f, ls.w >= 0 ? (float)a[ls.w] : 0.0f);
-  au[p] = make_float4(us.x >= 0 ? (float)a[us.x] : 0.0f, us.y >= 0 ? (float)a[us.y] : 0.0f,
-                      us.z >= 0 ? (float)a[us.z] : 0.0f, us.w >= 0 ? (float)a[us.w] : 0.0f);
-  dv[p] = (float)dbj[row];
+  auto v = [&](int k) { return k >= 0 ? (T)a[k] : (T)0; };
+  al[p] = Vec4<T>::make(v(ls.x), v(ls.y), v(ls.z), offl ? tk_from_bits<T>(offl[p]) : v(ls.w));
+  au[p] = Vec4<T>::make(v(us.x), v(us.y), v(us.z), offu ? tk_from_bits<T>(offu[p]) : v(us.w));
+  dv[p] = (T)dbj[row];
 }
 // SELL-64 copy of P A P^T: slice widths, columns, values.  A row keeps its
 // CSR entry order; padding entries have value 0 and the row's own column.
@@ -213,17 +279,18 @@ __global__ void tk_row_src(int n, const int *__restrict__ perm, const int *__res
   const int row = perm[p];
   rs[p] = make_int2(ia[row], ia[row + 1] - ia[row]);
 }
-// Values of the SELL-64 copy: thread (position, blockIdx.y) fills entries
-// k = blockIdx.y, blockIdx.y + gridDim.y, ... of its row.
+// Values of the SELL-64 copy in precision T: thread (position, blockIdx.y)
+// fills entries k = blockIdx.y, blockIdx.y + gridDim.y, ... of its row.
+template <class T>
 __global__ void tk_sell_val(int n, int nslices, const int2 *__restrict__ rs,
                             const double *__restrict__ a, const int *__restrict__ base,
-                            float *__restrict__ val) {
+                            T *__restrict__ val) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nslices * 64) return;
   const int s = p >> 6, q = base[s] + (p & 63), L = (base[s + 1] - base[s]) >> 6;
   const int2 r = p < n ? rs[p] : make_int2(0, 0);
   for (int k = blockIdx.y; k < L; k += gridDim.y)
-    val[q + 64 * k] = k < r.y ? (float)a[r.x + k] : 0.0f;
+    val[q + 64 * k] = k < r.y ? (T)a[r.x + k] : (T)0;
 }
 
 // ---------------------------------------------------------------------------
@@ -231,30 +298,30 @@ __global__ void tk_sell_val(int n, int nslices, const int2 *__restrict__ rs,
 // ---------------------------------------------------------------------------
 // Operands of the PP positions (PP = BLK / NT) a thread of an NT-thread
 // workgroup handles in one step: position sb.x + t + q NT for q < PP.
-template <int PP>
+template <int PP, class T>
 struct TSlot {
   int ok[PP], i[PP];  // position valid; its LDS slot
   int2 d[PP];         // 4 x 16-bit dependency slots
-  float4 a[PP];
-  float dv[PP];
+  V4<T> a[PP];
+  T dv[PP];
 };
 
 // Operands for this thread of the step with position bounds sb (valid: the
 // step exists).  Lanes without a position load the tile's first one: a
 // branch around the loads would make the compiler drain every outstanding
 // prefetch where the paths join.
-template <int PP, int NT, bool DV = true>
-__device__ __forceinline__ void tk_load(TSlot<PP> &S, int2 sb, bool valid, int lo,
+template <int PP, int NT, bool DV = true, bool PACK = false, class T>
+__device__ __forceinline__ void tk_load(TSlot<PP, T> &S, int2 sb, bool valid, int lo,
                                         const int2 *__restrict__ dd,
-                                        const float4 *__restrict__ aa,
-                                        const float *__restrict__ dv, int t = (int)threadIdx.x) {
+                                        const V4<T> *__restrict__ aa,
+                                        const T *__restrict__ dv, int t = (int)threadIdx.x) {
 #pragma unroll
   for (int q = 0; q < PP; ++q) {
     const int i = sb.x + t + q * NT;
     S.ok[q] = (valid && i < sb.y) ? 1 : 0;
     const int ii = S.ok[q] ? i : lo;
     S.i[q] = ii - lo;
-    S.d[q] = dd[ii];
+    if (!PACK) S.d[q] = dd[ii];  // packed: the distances ride in the coefficients' w
     S.a[q] = aa[ii];
     if (DV) S.dv[q] = dv[ii];  // the backward sweep takes the pivot from LDS
   }
@@ -267,16 +334,43 @@ __device__ __forceinline__ void tk_load(TSlot<PP> &S, int2 sb, bool valid, int l
 // value (signed zeros, infinities and NaN included) unchanged, so the result
 // is the one of the sum over the used slots only.  A step's positions are
 // independent, so a thread's PP chains overlap.
-template <int NR, int NRP, int MODE, int PP>
-__device__ __forceinline__ void tk_step(const TSlot<PP> &S, float *V, float *D) {
+// Packed layout (PACK, every position with at most 3 in-tile dependencies per
+// sweep): coefficients in x, y, z and, in the bits of w, three 10-bit
+// distances from the position to its dependency (0: unused, the zero slot
+// dslot), which drops the separate 8-byte slot word per position and sweep.
+// The fourth slot it lacks is always unused, so the results are those of the
+// 4-slot layout bit for bit.
+template <class T>
+__device__ __forceinline__ unsigned tk_bits(T w) {
+  if constexpr (sizeof(T) == 4)
+    return __float_as_uint(w);
+  else
+    return (unsigned)__double_as_longlong(w);
+}
+template <int NR, int NRP, int MODE, int PP, bool PACK, class T>
+__device__ __forceinline__ void tk_step(const TSlot<PP, T> &S, T *V, T *D, int dslot) {
+  constexpr int ND = PACK ? 3 : FCP_PMAXD;
 #pragma unroll
   for (int q = 0; q < PP; ++q) {
     if (!S.ok[q]) continue;
-    const unsigned dx = (unsigned)S.d[q].x, dy = (unsigned)S.d[q].y;
-    const int dep[FCP_PMAXD] = {(int)(dx & 0xFFFF), (int)(dx >> 16), (int)(dy & 0xFFFF),
-                                (int)(dy >> 16)};
-    const float av[FCP_PMAXD] = {S.a[q].x, S.a[q].y, S.a[q].z, S.a[q].w};
-    float dvq;
+    int dep[FCP_PMAXD];
+    if (PACK) {
+      const unsigned w = tk_bits(S.a[q].w);
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        const int off = (int)((w >> (10 * m)) & 1023u);
+        dep[m] = off ? (MODE == 0 ? S.i[q] - off : S.i[q] + off) : dslot;
+      }
+      dep[3] = dslot;
+    } else {
+      const unsigned dx = (unsigned)S.d[q].x, dy = (unsigned)S.d[q].y;
+      dep[0] = (int)(dx & 0xFFFF);
+      dep[1] = (int)(dx >> 16);
+      dep[2] = (int)(dy & 0xFFFF);
+      dep[3] = (int)(dy >> 16);
+    }
+    const T av[FCP_PMAXD] = {S.a[q].x, S.a[q].y, S.a[q].z, PACK ? (T)0 : S.a[q].w};
+    T dvq;
     if (MODE == 0) {
       dvq = S.dv[q];
       D[S.i[q]] = dvq;  // kept for the backward sweep
@@ -285,10 +379,10 @@ __device__ __forceinline__ void tk_step(const TSlot<PP> &S, float *V, float *D) 
     }
 #pragma unroll
     for (int c = 0; c < NR; ++c) {
-      float acc = V[S.i[q] * NRP + c];
-      if (MODE == 1) acc = acc / (dvq + (float)FCP_SMALL);
+      T acc = V[S.i[q] * NRP + c];
+      if (MODE == 1) acc = acc / (dvq + (T)FCP_SMALL);
 #pragma unroll
-      for (int m = 0; m < FCP_PMAXD; ++m) acc = acc - av[m] * V[dep[m] * NRP + c];
+      for (int m = 0; m < ND; ++m) acc = acc - av[m] * V[dep[m] * NRP + c];
       V[S.i[q] * NRP + c] = acc * dvq;
     }
   }
@@ -299,42 +393,42 @@ __device__ __forceinline__ void tk_step(const TSlot<PP> &S, float *V, float *D) 
 // steps ahead; the bounds of the step to prefetch are read one step early
 // (always from a valid slot), so a step's critical path holds only its own
 // LDS reads and write.
-template <int NR, int NRP, int NT = BLK>
-__device__ __forceinline__ void tk_sweeps(float *V, float *D, const int2 *sb, int ns, int lo,
+template <int NR, int NRP, int NT = BLK, bool PACK = false, class T>
+__device__ __forceinline__ void tk_sweeps(T *V, T *D, const int2 *sb, int ns, int lo, int dslot,
                                           const int2 *__restrict__ dl,
-                                          const float4 *__restrict__ al,
+                                          const V4<T> *__restrict__ al,
                                           const int2 *__restrict__ du,
-                                          const float4 *__restrict__ au,
-                                          const float *__restrict__ dv) {
+                                          const V4<T> *__restrict__ au,
+                                          const T *__restrict__ dv) {
   constexpr int PP = BLK / NT;
-  TSlot<PP> S[TPD];
+  TSlot<PP, T> S[TPD];
 #pragma unroll
   for (int j = 0; j < TPD; ++j)
-    tk_load<PP, NT>(S[j], sb[j < ns ? j : 0], j < ns, lo, dl, al, dv);
+    tk_load<PP, NT, true, PACK>(S[j], sb[j < ns ? j : 0], j < ns, lo, dl, al, dv);
   int2 nxt = sb[TPD < ns ? TPD : 0];
   for (int base = 0; base < ns; base += TPD) {
 #pragma unroll
     for (int j = 0; j < TPD; ++j) {
       const int nx = base + j + TPD;
       const int2 nn = sb[nx + 1 < ns ? nx + 1 : 0];
-      tk_step<NR, NRP, 0, PP>(S[j], V, D);
+      tk_step<NR, NRP, 0, PP, PACK>(S[j], V, D, dslot);
       __syncthreads();
-      tk_load<PP, NT>(S[j], nxt, nx < ns, lo, dl, al, dv);
+      tk_load<PP, NT, true, PACK>(S[j], nxt, nx < ns, lo, dl, al, dv);
       nxt = nn;
     }
   }
 #pragma unroll
   for (int j = 0; j < TPD; ++j)
-    tk_load<PP, NT, false>(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv);
+    tk_load<PP, NT, false, PACK>(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv);
   nxt = sb[TPD < ns ? ns - 1 - TPD : 0];
   for (int base = 0; base < ns; base += TPD) {
 #pragma unroll
     for (int j = 0; j < TPD; ++j) {
       const int nx = base + j + TPD;
       const int2 nn = sb[nx + 1 < ns ? ns - 2 - nx : 0];
-      tk_step<NR, NRP, 1, PP>(S[j], V, D);
+      tk_step<NR, NRP, 1, PP, PACK>(S[j], V, D, dslot);
       __syncthreads();
-      tk_load<PP, NT, false>(S[j], nxt, nx < ns, lo, du, au, dv);
+      tk_load<PP, NT, false, PACK>(S[j], nxt, nx < ns, lo, du, au, dv);
       nxt = nn;
     }
   }
@@ -350,42 +444,43 @@ constexpr int TPDW = FCP_TPDW;  // steps of operand prefetch of a one-wave tile 
 // barrier separates the steps; a wave barrier only keeps the compiler from
 // moving LDS accesses across a step.  Steps are short, so operands are
 // prefetched TPDW steps ahead.
-template <int NR, int NRP>
-__device__ __forceinline__ void tk_sweeps_w(float *V, float *D, const int2 *sb, int ns, int lo,
-                                            int lane,
+template <int NR, int NRP, bool PACK, class T>
+__device__ __forceinline__ void tk_sweeps_w(T *V, T *D, const int2 *sb, int ns, int lo,
+                                            int dslot, int lane,
                                             const int2 *__restrict__ dl,
-                                            const float4 *__restrict__ al,
+                                            const V4<T> *__restrict__ al,
                                             const int2 *__restrict__ du,
-                                            const float4 *__restrict__ au,
-                                            const float *__restrict__ dv) {
-  TSlot<1> S[TPDW];
+                                            const V4<T> *__restrict__ au,
+                                            const T *__restrict__ dv) {
+  TSlot<1, T> S[TPDW];
 #pragma unroll
   for (int j = 0; j < TPDW; ++j)
-    tk_load<1, 64>(S[j], sb[j < ns ? j : 0], j < ns, lo, dl, al, dv, lane);
+    tk_load<1, 64, true, PACK>(S[j], sb[j < ns ? j : 0], j < ns, lo, dl, al, dv, lane);
   int2 nxt = sb[TPDW < ns ? TPDW : 0];
   for (int base = 0; base < ns; base += TPDW) {
 #pragma unroll
     for (int j = 0; j < TPDW; ++j) {
       const int nx = base + j + TPDW;
       const int2 nn = sb[nx + 1 < ns ? nx + 1 : 0];
-      tk_step<NR, NRP, 0, 1>(S[j], V, D);
+      tk_step<NR, NRP, 0, 1, PACK>(S[j], V, D, dslot);
       __builtin_amdgcn_wave_barrier();
-      tk_load<1, 64>(S[j], nxt, nx < ns, lo, dl, al, dv, lane);
+      tk_load<1, 64, true, PACK>(S[j], nxt, nx < ns, lo, dl, al, dv, lane);
       nxt = nn;
     }
   }
 #pragma unroll
   for (int j = 0; j < TPDW; ++j)
-    tk_load<1, 64, false>(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv, lane);
+    tk_load<1, 64, false, PACK>(S[j], sb[j < ns ? ns - 1 - j : 0], j < ns, lo, du, au, dv,
+                                lane);
   nxt = sb[TPDW < ns ? ns - 1 - TPDW : 0];
   for (int base = 0; base < ns; base += TPDW) {
 #pragma unroll
     for (int j = 0; j < TPDW; ++j) {
       const int nx = base + j + TPDW;
       const int2 nn = sb[nx + 1 < ns ? ns - 2 - nx : 0];
-      tk_step<NR, NRP, 1, 1>(S[j], V, D);
+      tk_step<NR, NRP, 1, 1, PACK>(S[j], V, D, dslot);
       __builtin_amdgcn_wave_barrier();
-      tk_load<1, 64, false>(S[j], nxt, nx < ns, lo, du, au, dv, lane);
+      tk_load<1, 64, false, PACK>(S[j], nxt, nx < ns, lo, du, au, dv, lane);
       nxt = nn;
     }
   }
@@ -400,26 +495,27 @@ __device__ __forceinline__ void tk_sweeps_w(float *V, float *D, const int2 *sb, 
 // s->imax iterations); then
 // z = M^-1 r and sk = r.z, bet = sk / s0.  Columns already done are left
 // untouched.
-template <int NR, int NRP, int NT>
+template <int NR, int NRP, int NT, bool PACK, class T>
 __global__ __launch_bounds__(NT) void tk_pre(
     int nb, int vslots, int dslot, const int *__restrict__ bpos, const int *__restrict__ blk_step,
-    const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const float4 *__restrict__ al,
-    const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
-    float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
-    const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
+    const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const V4<T> *__restrict__ al,
+    const int2 *__restrict__ du, const V4<T> *__restrict__ au, const T *__restrict__ dv,
+    T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+    const T *__restrict__ q, T *__restrict__ z, double *__restrict__ partials,
     int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs,
     double *__restrict__ r0, int ncp) {
   // [vslots] values (slot dslot = 0), the tile's pivots, then the step bounds
-  extern __shared__ float V[];
-  float *D = V + vslots;
+  extern __shared__ __align__(16) unsigned char tk_smem[];
+  T *V = reinterpret_cast<T *>(tk_smem);
+  T *D = V + vslots;
   int2 *sb = reinterpret_cast<int2 *>(D + ((dslot + 1) & ~1));
   if (tk_all_done<NR>(s)) return;
   bool act[NR];
-  float alf[NR];
+  T alf[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) {
     act[c] = !solve_done(s + c);
-    alf[c] = (float)s[c].alf;
+    alf[c] = (T)s[c].alf;
   }
   double part[3 * NR];
 #pragma unroll
@@ -433,7 +529,7 @@ __global__ __launch_bounds__(NT) void tk_pre(
 #pragma unroll
     for (int c = 0; c < NR; ++c) rs[c] = 0.0;
     for (int i0 = lo + (int)threadIdx.x; i0 < hi; i0 += GS_U * NT) {
-      float xv[GS_U][NR], pv[GS_U][NR], rv[GS_U][NR], qv[GS_U][NR];
+      T xv[GS_U][NR], pv[GS_U][NR], rv[GS_U][NR], qv[GS_U][NR];
 #pragma unroll
       for (int u = 0; u < GS_U; ++u) {
         const int i = i0 + u * NT;
@@ -457,7 +553,7 @@ __global__ __launch_bounds__(NT) void tk_pre(
 #pragma unroll
           for (int c = 0; c < NR; ++c) {
             const size_t o = (size_t)i * NRP + c;
-            float ri = rv[u][c];
+            T ri = rv[u][c];
             if (upd && act[c]) {
               x[o] = xv[u][c] + alf[c] * pv[u][c];
               ri = ri - alf[c] * qv[u][c];
@@ -471,7 +567,7 @@ __global__ __launch_bounds__(NT) void tk_pre(
         }
       }
     }
-    if (threadIdx.x < NRP) V[dslot * NRP + threadIdx.x] = 0.0f;  // the unused-dependency slot
+    if (threadIdx.x < NRP) V[dslot * NRP + threadIdx.x] = (T)0;  // the unused-dependency slot
     if (r0) {  // uniform
       double ts[NR];
       block_sum<NR, NT>(rs, ts);
@@ -480,13 +576,13 @@ __global__ __launch_bounds__(NT) void tk_pre(
         for (int c = 0; c < NR; ++c) r0[(size_t)c * ncp + b] = ts[c];
     }
     __syncthreads();
-    if (ns > 0) tk_sweeps<NR, NRP, NT>(V, D, sb, ns, lo, dl, al, du, au, dv);
+    if (ns > 0) tk_sweeps<NR, NRP, NT, PACK>(V, D, sb, ns, lo, dslot, dl, al, du, au, dv);
     // z out; r.z with the residual this thread wrote (or read) above
     for (int i = lo + (int)threadIdx.x; i < hi; i += NT) {
 #pragma unroll
       for (int c = 0; c < NR; ++c) {
         const size_t o = (size_t)i * NRP + c;
-        const float zi = V[(i - lo) * NRP + c];
+        const T zi = V[(i - lo) * NRP + c];
         z[o] = zi;
         part[3 * c + 2] += (double)r[o] * (double)zi;
       }
@@ -530,29 +626,29 @@ __global__ __launch_bounds__(NT) void tk_pre(
 // tk_pre with one wave per tile: four tiles per workgroup, no workgroup
 // barrier inside a tile (tiles of one schedule built with steps of at most
 // 64 positions).
-template <int NR, int NRP>
+template <int NR, int NRP, bool PACK, class T>
 __global__ __launch_bounds__(BLK) void tk_pre_w(
     int nb, int vstride, int vslots, int dslot, const int *__restrict__ bpos, const int *__restrict__ blk_step,
-    const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const float4 *__restrict__ al,
-    const int2 *__restrict__ du, const float4 *__restrict__ au, const float *__restrict__ dv,
-    float *__restrict__ x, float *__restrict__ r, const float *__restrict__ p,
-    const float *__restrict__ q, float *__restrict__ z, double *__restrict__ partials,
+    const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const V4<T> *__restrict__ al,
+    const int2 *__restrict__ du, const V4<T> *__restrict__ au, const T *__restrict__ dv,
+    T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+    const T *__restrict__ q, T *__restrict__ z, double *__restrict__ partials,
     int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs,
     double *__restrict__ r0, int ncp) {
-  // one region of vstride floats per wave: [vslots] values (slot dslot = 0),
+  // one region of vstride values per wave: [vslots] values (slot dslot = 0),
   // the tile's pivots, then the step bounds
-  extern __shared__ float VW[];
+  extern __shared__ __align__(16) unsigned char tk_smem[];
   const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
-  float *V = VW + (size_t)wv * vstride;
-  float *D = V + vslots;
+  T *V = reinterpret_cast<T *>(tk_smem) + (size_t)wv * vstride;
+  T *D = V + vslots;
   int2 *sb = reinterpret_cast<int2 *>(D + ((dslot + 1) & ~1));
   if (tk_all_done<NR>(s)) return;
   bool act[NR];
-  float alf[NR];
+  T alf[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) {
     act[c] = !solve_done(s + c);
-    alf[c] = (float)s[c].alf;
+    alf[c] = (T)s[c].alf;
   }
   double part[3 * NR];
 #pragma unroll
@@ -566,7 +662,7 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
 #pragma unroll
     for (int c = 0; c < NR; ++c) rs[c] = 0.0;
     for (int i0 = lo + lane; i0 < hi; i0 += GS_U * 64) {
-      float xv[GS_U][NR], pv[GS_U][NR], rv[GS_U][NR], qv[GS_U][NR];
+      T xv[GS_U][NR], pv[GS_U][NR], rv[GS_U][NR], qv[GS_U][NR];
 #pragma unroll
       for (int u = 0; u < GS_U; ++u) {
         const int i = i0 + u * 64;
@@ -590,7 +686,7 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
 #pragma unroll
           for (int c = 0; c < NR; ++c) {
             const size_t o = (size_t)i * NRP + c;
-            float ri = rv[u][c];
+            T ri = rv[u][c];
             if (upd && act[c]) {
               x[o] = xv[u][c] + alf[c] * pv[u][c];
               ri = ri - alf[c] * qv[u][c];
@@ -604,7 +700,7 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
         }
       }
     }
-    if (lane < NRP) V[dslot * NRP + lane] = 0.0f;  // the unused-dependency slot
+    if (lane < NRP) V[dslot * NRP + lane] = (T)0;  // the unused-dependency slot
     if (r0) {  // uniform
 #pragma unroll
       for (int c = 0; c < NR; ++c) {
@@ -613,14 +709,14 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
       }
     }
     __builtin_amdgcn_wave_barrier();
-    if (ns > 0) tk_sweeps_w<NR, NRP>(V, D, sb, ns, lo, lane, dl, al, du, au, dv);
+    if (ns > 0) tk_sweeps_w<NR, NRP, PACK>(V, D, sb, ns, lo, dslot, lane, dl, al, du, au, dv);
     __builtin_amdgcn_wave_barrier();
     // z out; r.z with the residual this thread wrote (or read) above
     for (int i = lo + lane; i < hi; i += 64) {
 #pragma unroll
       for (int c = 0; c < NR; ++c) {
         const size_t o = (size_t)i * NRP + c;
-        const float zi = V[(i - lo) * NRP + c];
+        const T zi = V[(i - lo) * NRP + c];
         z[o] = zi;
         part[3 * c + 2] += (double)r[o] * (double)zi;
       }
@@ -663,23 +759,23 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
 
 // p_new = z + bet p_old (every operand computed where it is used), q = A p_new
 // over all slice entries (padding adds 0 * p_i), pkapk = p.q, alf = sk/pkapk.
-template <int NR, int NRP>
+template <int NR, int NRP, class T>
 __global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ base,
                                                const int *__restrict__ col,
-                                               const float *__restrict__ val,
-                                               const float *__restrict__ z,
-                                               const float *__restrict__ pold,
-                                               float *__restrict__ pnew, float *__restrict__ q,
+                                               const T *__restrict__ val,
+                                               const T *__restrict__ z,
+                                               const T *__restrict__ pold,
+                                               T *__restrict__ pnew, T *__restrict__ q,
                                                double *__restrict__ partials,
                                                int *__restrict__ counter,
                                                SolveScalars *__restrict__ s) {
   if (tk_all_done<NR>(s)) return;
   bool act[NR];
-  float bet[NR];
+  T bet[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) {
     act[c] = !solve_done(s + c);
-    bet[c] = (float)s[c].bet;
+    bet[c] = (T)s[c].bet;
   }
   double part[NR];
 #pragma unroll
@@ -690,7 +786,7 @@ __global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ ba
     const int b0 = base[sl];
     const int pp = b0 + (i & 63);
     const int L = (base[sl + 1] - b0) >> 6;
-    float av[SELL_KU];
+    T av[SELL_KU];
     int cv[SELL_KU];
 #pragma unroll
     for (int k = 0; k < SELL_KU; ++k)
@@ -698,26 +794,26 @@ __global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ ba
         av[k] = val[pp + 64 * k];
         cv[k] = col[pp + 64 * k];
       }
-    float acc[NR];
+    T acc[NR];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) acc[c] = 0.0f;
+    for (int c = 0; c < NR; ++c) acc[c] = (T)0;
 #pragma unroll
     for (int k = 0; k < SELL_KU; ++k)
       if (k < L) {
 #pragma unroll
         for (int c = 0; c < NR; ++c) {
           const size_t o = (size_t)cv[k] * NRP + c;
-          const float pj = z[o] + bet[c] * pold[o];
+          const T pj = z[o] + bet[c] * pold[o];
           acc[c] = acc[c] + av[k] * pj;
         }
       }
     for (int k = SELL_KU; k < L; ++k) {
-      const float a = val[pp + 64 * k];
+      const T a = val[pp + 64 * k];
       const int j = col[pp + 64 * k];
 #pragma unroll
       for (int c = 0; c < NR; ++c) {
         const size_t o = (size_t)j * NRP + c;
-        const float pj = z[o] + bet[c] * pold[o];
+        const T pj = z[o] + bet[c] * pold[o];
         acc[c] = acc[c] + a * pj;
       }
     }
@@ -725,7 +821,7 @@ __global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ ba
     for (int c = 0; c < NR; ++c) {
       if (!act[c]) continue;
       const size_t o = (size_t)i * NRP + c;
-      const float pi = z[o] + bet[c] * pold[o];
+      const T pi = z[o] + bet[c] * pold[o];
       pnew[o] = pi;
       q[o] = acc[c];
       part[c] += (double)pi * (double)acc[c];
@@ -743,21 +839,21 @@ __global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ ba
   }
 }
 
-// r~ = P (res / |res|_2) in FP32, x~ = 0, p~ = 0 (BiCGStab: reso = r~, u = 0),
+// r~ = P (res / |res|_2) in precision T, x~ = 0, p~ = 0 (BiCGStab: reso = r~, u = 0),
 // and the inner scalars of
 // every column (tolerance from the outer residual as for the row-order inner
 // solves; columns whose outer residual already meets tol_rel start done).
-template <int NR, int NRP>
+template <int NR, int NRP, class T>
 __global__ __launch_bounds__(BLK) void tk_init(int n, const int *__restrict__ perm,
                                                const double *__restrict__ res,
                                                const SolveScalars *__restrict__ so,
-                                               float *__restrict__ r, float *__restrict__ x,
-                                               float *__restrict__ p0,
+                                               T *__restrict__ r, T *__restrict__ x,
+                                               T *__restrict__ p0,
                                                double *__restrict__ partials,
                                                int *__restrict__ counter,
                                                SolveScalars *__restrict__ s, double tol_rel,
                                                double inner_tol, int imax,
-                                               float *__restrict__ reso, float *__restrict__ u0) {
+                                               T *__restrict__ reso, T *__restrict__ u0) {
   double inv[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) inv[c] = so[c].l2 > 0.0 ? 1.0 / so[c].l2 : 0.0;
@@ -769,12 +865,12 @@ __global__ __launch_bounds__(BLK) void tk_init(int n, const int *__restrict__ pe
 #pragma unroll
     for (int c = 0; c < NR; ++c) {
       const size_t o = (size_t)i * NRP + c;
-      const float v = (float)(res[row * NRP + c] * inv[c]);
+      const T v = (T)(res[row * NRP + c] * inv[c]);
       r[o] = v;
-      x[o] = 0.0f;
-      p0[o] = 0.0f;
+      x[o] = (T)0;
+      p0[o] = (T)0;
       if (reso) reso[o] = v;
-      if (u0) u0[o] = 0.0f;
+      if (u0) u0[o] = (T)0;
       part[2 * c] += fabs((double)v);
       part[2 * c + 1] += (double)v * (double)v;
     }
@@ -810,8 +906,8 @@ __global__ __launch_bounds__(BLK) void tk_init(int n, const int *__restrict__ pe
 
 // x += |res|_2 P^T x~ (FP64), in natural row order (x read and written
 // contiguously, x~ gathered through the inverse permutation)
-template <int NR, int NRP>
-__global__ void tk_ir_update(int n, const int *__restrict__ iperm, const float *__restrict__ e,
+template <int NR, int NRP, class T>
+__global__ void tk_ir_update(int n, const int *__restrict__ iperm, const T *__restrict__ e,
                              const SolveScalars *__restrict__ so, double *__restrict__ x) {
   double sc[NR];
 #pragma unroll
@@ -835,7 +931,7 @@ __global__ void tk_ir_update(int n, const int *__restrict__ iperm, const float *
 //   tb_spmv<1> (rows): v = A z, alf = v.r / (v.v + small) (:737-748)
 // Each kernel keeps the per-row arithmetic of the row-order kernels.
 // ---------------------------------------------------------------------------
-template <int NR, int NRP, int MODE>
+template <int NR, int NRP, int MODE, bool PACK>
 __global__ __launch_bounds__(BLK) void tb_tile(
     int nb, int vslots, int dslot, const int *__restrict__ bpos, const int *__restrict__ blk_step,
     const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const float4 *__restrict__ al,
@@ -902,7 +998,7 @@ __global__ __launch_bounds__(BLK) void tb_tile(
     }
     if (threadIdx.x < NRP) V[dslot * NRP + threadIdx.x] = 0.0f;  // the unused-dependency slot
     __syncthreads();
-    if (ns > 0) tk_sweeps<NR, NRP>(V, D, sb, ns, lo, dl, al, du, au, dv);
+    if (ns > 0) tk_sweeps<NR, NRP, BLK, PACK>(V, D, sb, ns, lo, dslot, dl, al, du, au, dv);
     for (int i = lo + (int)threadIdx.x; i < hi; i += BLK) {
 #pragma unroll
       for (int c = 0; c < NR; ++c) z[(size_t)i * NRP + c] = V[(i - lo) * NRP + c];
@@ -1489,12 +1585,12 @@ __global__ __launch_bounds__(256) void tc_gram(int ncp, const double *__restrict
 // tile w / parts; every workgroup of a tile forms its entry of e0 with the
 // same summation order (thread t sums the m = t mod BLK, then the waves in
 // order).
-template <int NR, int NRP>
+template <int NR, int NRP, class VT>
 __global__ __launch_bounds__(BLK) void tc_zc(int parts, int ncp, const int *__restrict__ bpos,
                                              const int *__restrict__ failed,
                                              const double *__restrict__ G,
                                              const double *__restrict__ r0,
-                                             float *__restrict__ z, double *__restrict__ partials,
+                                             VT *__restrict__ z, double *__restrict__ partials,
                                              int *__restrict__ counter,
                                              SolveScalars *__restrict__ s) {
   __shared__ double S[NR][BLK / 64];
@@ -1508,7 +1604,7 @@ __global__ __launch_bounds__(BLK) void tc_zc(int parts, int ncp, const int *__re
   const int chunk = (len + parts - 1) / parts;
   const int i0 = lo + part * chunk, i1 = min(lo + len, i0 + chunk);
   // the first positions' z loaded while the coarse row is summed
-  float zv[GS_U][NR];
+  VT zv[GS_U][NR];
 #pragma unroll
   for (int u = 0; u < GS_U; ++u) {
     const int i = i0 + t + u * BLK;
@@ -1527,14 +1623,14 @@ __global__ __launch_bounds__(BLK) void tc_zc(int parts, int ncp, const int *__re
   __syncthreads();
   const bool off = *failed != 0;  // no factor: tile-local only
   double E[NR];
-  float e[NR];
+  VT e[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) {
     double v = 0.0;
 #pragma unroll
     for (int w = 0; w < BLK / 64; ++w) v += S[c][w];
     E[c] = off ? 0.0 : v;
-    e[c] = (float)E[c];
+    e[c] = (VT)E[c];
   }
 #pragma unroll
   for (int u = 0; u < GS_U; ++u) {
@@ -1565,28 +1661,32 @@ __global__ __launch_bounds__(BLK) void tc_zc(int parts, int ncp, const int *__re
   }
 }
 
+template <class T>
 __global__ void tk_gather_in(int n, const int *__restrict__ perm, const double *__restrict__ r,
-                             float *__restrict__ v) {
+                             T *__restrict__ v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) v[i] = (float)r[perm[i]];
+  if (i < n) v[i] = (T)r[perm[i]];
 }
-__global__ void tk_scatter_out(int n, const int *__restrict__ perm, const float *__restrict__ v,
+template <class T>
+__global__ void tk_scatter_out(int n, const int *__restrict__ perm, const T *__restrict__ v,
                                double *__restrict__ z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) z[perm[i]] = (double)v[i];
 }
 
-// LDS of one wave's tile region, in floats (values, then the step bounds).
-int tk_wave_stride(const fcp_ctx *ctx, int nrp) {
+// LDS of one wave's tile region, in values of ts bytes (values, pivots, then
+// the step bounds).
+int tk_wave_stride(const fcp_ctx *ctx, int nrp, int ts = sizeof(float)) {
   const BlockSchedule &b = ctx->jlower;
-  return (((b.R + 1) * nrp + 1) & ~1) + ((b.R + 1) & ~1) + 2 * (b.max_steps + 1);
+  return (((b.R + 1) * nrp + 1) & ~1) + ((b.R + 1) & ~1) +
+         (b.max_steps + 1) * (int)(sizeof(int2) / ts);
 }
 
-size_t tk_lds(const fcp_ctx *ctx, int nrp) {
+size_t tk_lds(const fcp_ctx *ctx, int nrp, int ts = sizeof(float)) {
   const BlockSchedule &b = ctx->jlower;
-  if (ctx->bj_wave) return (size_t)(BLK / 64) * tk_wave_stride(ctx, nrp) * sizeof(float);
+  if (ctx->bj_wave) return (size_t)(BLK / 64) * tk_wave_stride(ctx, nrp, ts) * ts;
   // values (R slots and the zero slot), pivots, then 8-byte aligned step bounds
-  return (size_t)((((b.R + 1) * nrp + 1) & ~1) + ((b.R + 1) & ~1)) * sizeof(float) +
+  return (size_t)((((b.R + 1) * nrp + 1) & ~1) + ((b.R + 1) & ~1)) * ts +
          (size_t)(b.max_steps + 1) * sizeof(int2);
 }
 
@@ -1605,8 +1705,9 @@ int tk_grid(const fcp_ctx *ctx) {
   return nb < MAXBLK ? nb : MAXBLK;
 }
 
-// Pattern (once per tile-local schedule) and values (per pivot epoch).
-int tk_prepare(fcp_ctx *ctx) {
+// Pattern (once per tile-local schedule) and values in FP32, or in FP64 with
+// fp64 set (per pivot epoch).
+int tk_prepare(fcp_ctx *ctx, bool fp64 = false) {
   const int n = ctx->n;
   const BlockSchedule &b = ctx->jlower;
   const int g = grid_for(n, BLK);
@@ -1647,32 +1748,98 @@ int tk_prepare(fcp_ctx *ctx) {
         n, nsl, b.rows.p, ctx->tin_iperm.p, ctx->ia.p, ctx->ja.p, ctx->tin_base.p,
         ctx->tin_col.p);
     FCP_HIP(ctx, hipGetLastError());
+    ctx->tin_pack = false;
+    if (ctx->tune_tile_pack) {
+      FCP_HIP(ctx, ctx->tin_offl.alloc(n));
+      FCP_HIP(ctx, ctx->tin_offu.alloc(n));
+      FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
+      tk_pack_offsets<<<g, BLK, 0, ctx->stream>>>(
+          n, b.nblocks, b.R, b.blk_pos.p, reinterpret_cast<const int2 *>(ctx->tin_dl.p),
+          reinterpret_cast<const int2 *>(ctx->tin_du.p), ctx->tin_offl.p, ctx->tin_offu.p,
+          ctx->errflag.p);
+      FCP_HIP(ctx, hipGetLastError());
+      FCP_HIP(ctx, hipMemcpyAsync(&err, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+      FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      ctx->tin_pack = err == 0;
+    }
     ctx->tin_have = true;
     ctx->tin_epoch = ~0ull;
+    ctx->tin_epoch64 = ~0ull;
   }
-  if (ctx->tin_epoch != ctx->bj_epoch) {
-    FCP_HIP(ctx, ctx->tin_al.alloc(4 * (size_t)n));
-    FCP_HIP(ctx, ctx->tin_au.alloc(4 * (size_t)n));
-    FCP_HIP(ctx, ctx->tin_dv.alloc(n));
-    tk_sweep_vals<<<g, BLK, 0, ctx->stream>>>(
+  auto values = [&](auto *al, auto *au, auto *dv, auto *val, uint64_t &epoch) -> int {
+    using T = std::remove_pointer_t<decltype(dv)>;
+    if (epoch == ctx->bj_epoch) return 0;
+    tk_sweep_vals<T><<<g, BLK, 0, ctx->stream>>>(
         n, b.rows.p, b.psrc.p, ctx->tin_usrc.p, ctx->a.p, ctx->dbj.p,
-        reinterpret_cast<float4 *>(ctx->tin_al.p), reinterpret_cast<float4 *>(ctx->tin_au.p),
-        ctx->tin_dv.p);
+        ctx->tin_pack ? ctx->tin_offl.p : nullptr, ctx->tin_pack ? ctx->tin_offu.p : nullptr,
+        reinterpret_cast<V4<T> *>(al), reinterpret_cast<V4<T> *>(au), dv);
     const int nsl = (n + 63) / 64;
-    tk_sell_val<<<dim3(grid_for((size_t)nsl * 64, BLK), SELL_KU), BLK, 0, ctx->stream>>>(
-        n, nsl, reinterpret_cast<const int2 *>(ctx->tin_rs.p), ctx->a.p, ctx->tin_base.p,
-        ctx->tin_val.p);
+    tk_sell_val<T><<<dim3(grid_for((size_t)nsl * 64, BLK), SELL_KU), BLK, 0, ctx->stream>>>(
+        n, nsl, reinterpret_cast<const int2 *>(ctx->tin_rs.p), ctx->a.p, ctx->tin_base.p, val);
     FCP_HIP(ctx, hipGetLastError());
-    ctx->tin_epoch = ctx->bj_epoch;
+    epoch = ctx->bj_epoch;
+    return 0;
+  };
+  if (fp64) {
+    FCP_HIP(ctx, ctx->tin_al64.alloc(4 * (size_t)n));
+    FCP_HIP(ctx, ctx->tin_au64.alloc(4 * (size_t)n));
+    FCP_HIP(ctx, ctx->tin_dv64.alloc(n));
+    FCP_HIP(ctx, ctx->tin_val64.alloc(ctx->tin_col.n));
+    return values(ctx->tin_al64.p, ctx->tin_au64.p, ctx->tin_dv64.p, ctx->tin_val64.p,
+                  ctx->tin_epoch64);
   }
-  return 0;
+  FCP_HIP(ctx, ctx->tin_al.alloc(4 * (size_t)n));
+  FCP_HIP(ctx, ctx->tin_au.alloc(4 * (size_t)n));
+  FCP_HIP(ctx, ctx->tin_dv.alloc(n));
+  return values(ctx->tin_al.p, ctx->tin_au.p, ctx->tin_dv.p, ctx->tin_val.p, ctx->tin_epoch);
 }
 
+// Vectors, operands and coarse buffers of a tile-order PCG in precision T.
+template <class T>
 struct TkBufs {
-  float *x, *r, *z, *q, *p[2];
+  T *x, *r, *z, *q, *p[2];
   double *r0 = nullptr;  // tile sums of r (null: no coarse correction)
   int ncp = 0;
+  const int2 *dl = nullptr, *du = nullptr;
+  const V4<T> *al = nullptr, *au = nullptr;
+  const T *dv = nullptr, *val = nullptr;
 };
+
+// Vectors (nr columns) and packed operands of the current values in
+// precision T (tk_prepare done for T).
+int tk_bufs(fcp_ctx *ctx, size_t m, TkBufs<float> &B) {
+  for (int k = 0; k < 6; ++k) FCP_HIP(ctx, ctx->tin_vec[k].alloc(m));
+  B.x = ctx->tin_vec[0].p;
+  B.r = ctx->tin_vec[1].p;
+  B.z = ctx->tin_vec[2].p;
+  B.q = ctx->tin_vec[3].p;
+  B.p[0] = ctx->tin_vec[4].p;
+  B.p[1] = ctx->tin_vec[5].p;
+  B.al = reinterpret_cast<const float4 *>(ctx->tin_al.p);
+  B.au = reinterpret_cast<const float4 *>(ctx->tin_au.p);
+  B.dv = ctx->tin_dv.p;
+  B.val = ctx->tin_val.p;
+  B.dl = reinterpret_cast<const int2 *>(ctx->tin_dl.p);
+  B.du = reinterpret_cast<const int2 *>(ctx->tin_du.p);
+  return 0;
+}
+int tk_bufs(fcp_ctx *ctx, size_t m, TkBufs<double> &B) {
+  for (int k = 0; k < 6; ++k) FCP_HIP(ctx, ctx->tin_vec64[k].alloc(m));
+  B.x = ctx->tin_vec64[0].p;
+  B.r = ctx->tin_vec64[1].p;
+  B.z = ctx->tin_vec64[2].p;
+  B.q = ctx->tin_vec64[3].p;
+  B.p[0] = ctx->tin_vec64[4].p;
+  B.p[1] = ctx->tin_vec64[5].p;
+  B.al = reinterpret_cast<const double4 *>(ctx->tin_al64.p);
+  B.au = reinterpret_cast<const double4 *>(ctx->tin_au64.p);
+  B.dv = ctx->tin_dv64.p;
+  B.val = ctx->tin_val64.p;
+  B.dl = reinterpret_cast<const int2 *>(ctx->tin_dl.p);
+  B.du = reinterpret_cast<const int2 *>(ctx->tin_du.p);
+  return 0;
+}
 
 // A0 of the current matrix, its factor and A0^-1 (over the factor), on
 // stream st; the factor's failure goes to tc_flag on the device.
@@ -1811,7 +1978,8 @@ int tc_begin(fcp_ctx *ctx) {
 
 // Coarse buffers of an inner solve of nr columns (r0 cleared: entries past
 // the last tile stay zero), or none where the correction does not apply.
-int tc_bufs(fcp_ctx *ctx, int nr, TkBufs &B) {
+template <class T>
+int tc_bufs(fcp_ctx *ctx, int nr, TkBufs<T> &B) {
   if (!ctx->inner_coarse) return 0;
   FCP_TRY(tc_prepare(ctx));
   if (ctx->tc_nc == 0) return 0;
@@ -1823,30 +1991,29 @@ int tc_bufs(fcp_ctx *ctx, int nr, TkBufs &B) {
   return 0;
 }
 
-template <int NR, int NRP>
-int tk_launch_pre(fcp_ctx *ctx, const TkBufs &B, const float *p, SolveScalars *si, int upd) {
+template <int NR, int NRP, class T>
+int tk_launch_pre(fcp_ctx *ctx, const TkBufs<T> &B, const T *p, SolveScalars *si, int upd) {
   const BlockSchedule &b = ctx->jlower;
-  ProfScope ps(ctx, "tin_pre");
-  if (ctx->bj_wave) {
-    int g = (b.nblocks + BLK / 64 - 1) / (BLK / 64);
-    g = g < MAXBLK ? g : MAXBLK;
-    tk_pre_w<NR, NRP><<<g, BLK, tk_lds(ctx, NRP), ctx->stream>>>(
-        b.nblocks, tk_wave_stride(ctx, NRP), ((b.R + 1) * NRP + 1) & ~1, b.R, b.blk_pos.p,
-        b.blk_step.p, b.step_ptr.p, reinterpret_cast<const int2 *>(ctx->tin_dl.p),
-        reinterpret_cast<const float4 *>(ctx->tin_al.p),
-        reinterpret_cast<const int2 *>(ctx->tin_du.p),
-        reinterpret_cast<const float4 *>(ctx->tin_au.p), ctx->tin_dv.p, B.x, B.r, p, B.q, B.z,
-        ctx->partials.p, redctr(ctx), si, upd, 0.0, B.r0, B.ncp);
+  constexpr int TS = sizeof(T);
+  ProfScope ps(ctx, TS == 8 ? "tin_pre64" : "tin_pre");
+  auto go = [&](auto pack) {
+    constexpr bool PK = decltype(pack)::value;
+    if (ctx->bj_wave) {
+      int g = (b.nblocks + BLK / 64 - 1) / (BLK / 64);
+      g = g < MAXBLK ? g : MAXBLK;
+      tk_pre_w<NR, NRP, PK, T><<<g, BLK, tk_lds(ctx, NRP, TS), ctx->stream>>>(
+          b.nblocks, tk_wave_stride(ctx, NRP, TS), ((b.R + 1) * NRP + 1) & ~1, b.R, b.blk_pos.p,
+          b.blk_step.p, b.step_ptr.p, B.dl, B.al, B.du, B.au, B.dv, B.x, B.r, p, B.q, B.z,
+          ctx->partials.p, redctr(ctx), si, upd, 0.0, B.r0, B.ncp);
+    } else {
+      tk_pre<NR, NRP, TK_NT, PK, T><<<tk_grid(ctx), TK_NT, tk_lds(ctx, NRP, TS), ctx->stream>>>(
+          b.nblocks, ((b.R + 1) * NRP + 1) & ~1, b.R, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
+          B.dl, B.al, B.du, B.au, B.dv, B.x, B.r, p, B.q, B.z, ctx->partials.p, redctr(ctx), si,
+          upd, 0.0, B.r0, B.ncp);
+    }
     return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
-  }
-  tk_pre<NR, NRP, TK_NT><<<tk_grid(ctx), TK_NT, tk_lds(ctx, NRP), ctx->stream>>>(
-      b.nblocks, ((b.R + 1) * NRP + 1) & ~1, b.R, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
-      reinterpret_cast<const int2 *>(ctx->tin_dl.p),
-      reinterpret_cast<const float4 *>(ctx->tin_al.p),
-      reinterpret_cast<const int2 *>(ctx->tin_du.p),
-      reinterpret_cast<const float4 *>(ctx->tin_au.p), ctx->tin_dv.p, B.x, B.r, p, B.q, B.z,
-      ctx->partials.p, redctr(ctx), si, upd, 0.0, B.r0, B.ncp);
-  return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
+  };
+  return ctx->tin_pack ? go(std::true_type()) : go(std::false_type());
 }
 
 // Inner iteration k: p_new = z + bet p_old, q = A p_new (tk_spmv), then the
@@ -1860,38 +2027,39 @@ int tc_parts(const fcp_ctx *ctx) {
   return p < 1 ? 1 : (p > 8 ? 8 : p);
 }
 
-template <int NR, int NRP>
-int tc_launch_zc(fcp_ctx *ctx, const TkBufs &B, SolveScalars *si) {
+template <int NR, int NRP, class T>
+int tc_launch_zc(fcp_ctx *ctx, const TkBufs<T> &B, SolveScalars *si) {
   ProfScope ps(ctx, "tin_coarse");
   const int parts = tc_parts(ctx);
-  tc_zc<NR, NRP><<<ctx->jlower.nblocks * parts, BLK, 0, ctx->stream>>>(
+  tc_zc<NR, NRP, T><<<ctx->jlower.nblocks * parts, BLK, 0, ctx->stream>>>(
       parts, B.ncp, ctx->jlower.blk_pos.p, ctx->tc_flag.p, ctx->tc_L.p, B.r0, B.z,
       ctx->partials.p, redctr(ctx), si);
   return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
 }
 
-template <int NR, int NRP>
-int tk_iteration(fcp_ctx *ctx, const TkBufs &B, SolveScalars *si, int k) {
-  if (B.r0) FCP_TRY((tc_launch_zc<NR, NRP>(ctx, B, si)));
-  float *pold = B.p[k & 1], *pnew = B.p[(k + 1) & 1];
+template <int NR, int NRP, class T>
+int tk_iteration(fcp_ctx *ctx, const TkBufs<T> &B, SolveScalars *si, int k) {
+  if (B.r0) FCP_TRY((tc_launch_zc<NR, NRP, T>(ctx, B, si)));
+  T *pold = B.p[k & 1], *pnew = B.p[(k + 1) & 1];
   {
-    ProfScope ps(ctx, "tin_spmv");
-    tk_spmv<NR, NRP><<<tk_spmv_grid(ctx), BLK, 0, ctx->stream>>>(
-        ctx->n, ctx->tin_base.p, ctx->tin_col.p, ctx->tin_val.p, B.z, pold, pnew, B.q,
-        ctx->partials.p, redctr(ctx), si);
+    ProfScope ps(ctx, sizeof(T) == 8 ? "tin_spmv64" : "tin_spmv");
+    tk_spmv<NR, NRP, T><<<tk_spmv_grid(ctx), BLK, 0, ctx->stream>>>(
+        ctx->n, ctx->tin_base.p, ctx->tin_col.p, B.val, B.z, pold, pnew, B.q, ctx->partials.p,
+        redctr(ctx), si);
   }
-  return tk_launch_pre<NR, NRP>(ctx, B, pnew, si, 1);
+  return tk_launch_pre<NR, NRP, T>(ctx, B, pnew, si, 1);
 }
 
 // Executable graph of two inner iterations (k even, then odd) for the current
 // buffers, captured once and reused while every pointer it holds is unchanged.
-template <int NR, int NRP>
-int tk_graph(fcp_ctx *ctx, const TkBufs &B, SolveScalars *si, hipGraphExec_t *out) {
+template <int NR, int NRP, class T>
+int tk_graph(fcp_ctx *ctx, const TkBufs<T> &B, SolveScalars *si, hipGraphExec_t *out) {
   const BlockSchedule &b = ctx->jlower;
   const std::vector<const void *> key = {
-      (const void *)(size_t)NR, B.x, B.r, B.z, B.q, B.p[0], B.p[1], si, ctx->partials.p,
-      ctx->counters.p, ctx->tin_dl.p, ctx->tin_du.p, ctx->tin_al.p, ctx->tin_au.p, ctx->tin_dv.p,
-      ctx->tin_base.p, ctx->tin_col.p, ctx->tin_val.p, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
+      (const void *)(size_t)NR, (const void *)sizeof(T), (const void *)(size_t)ctx->tin_pack,
+      B.x, B.r, B.z, B.q, B.p[0], B.p[1], si,
+      ctx->partials.p, ctx->counters.p, B.dl, B.du, B.al, B.au, B.dv,
+      ctx->tin_base.p, ctx->tin_col.p, B.val, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
       (const void *)(size_t)b.nblocks, (const void *)(size_t)b.R, (const void *)(size_t)b.max_steps,
       (const void *)(size_t)ctx->n, B.r0, (const void *)(size_t)B.ncp, ctx->tc_L.p};
   if (ctx->tin_graph && ctx->tin_graph_key == key) {
@@ -1904,8 +2072,8 @@ int tk_graph(fcp_ctx *ctx, const TkBufs &B, SolveScalars *si, hipGraphExec_t *ou
   }
   hipGraph_t g = nullptr;
   FCP_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  int rc = tk_iteration<NR, NRP>(ctx, B, si, 0);
-  if (rc == 0) rc = tk_iteration<NR, NRP>(ctx, B, si, 1);
+  int rc = tk_iteration<NR, NRP, T>(ctx, B, si, 0);
+  if (rc == 0) rc = tk_iteration<NR, NRP, T>(ctx, B, si, 1);
   const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
   FCP_TRY(rc);
   FCP_HIP(ctx, ec);
@@ -1917,32 +2085,30 @@ int tk_graph(fcp_ctx *ctx, const TkBufs &B, SolveScalars *si, hipGraphExec_t *ou
   return 0;
 }
 
-template <int NR, int NRP>
+template <int NR, int NRP, class T>
 int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, SolveScalars *si,
            double tol_rel, double inner_tol, int budget, int check_every, int verbose,
            SolveScalars *h) {
   const int n = ctx->n;
-  const size_t m = (size_t)n * NRP;
-  for (int k = 0; k < 6; ++k) FCP_HIP(ctx, ctx->tin_vec[k].alloc(m));
-  TkBufs B{ctx->tin_vec[0].p, ctx->tin_vec[1].p, ctx->tin_vec[2].p, ctx->tin_vec[3].p,
-           {ctx->tin_vec[4].p, ctx->tin_vec[5].p}};
+  TkBufs<T> B;
+  FCP_TRY(tk_bufs(ctx, (size_t)n * NRP, B));
   FCP_TRY(tc_bufs(ctx, NR, B));
   const int g = red_grid(ctx, n);
-  tk_init<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, res, so, B.r, B.x, B.p[0],
-                                               ctx->partials.p, redctr(ctx), si, tol_rel,
-                                               inner_tol, budget, nullptr, nullptr);
-  FCP_TRY((tk_launch_pre<NR, NRP>(ctx, B, B.p[0], si, 0)));
+  tk_init<NR, NRP, T><<<g, BLK, 0, ctx->stream>>>(
+      n, ctx->jlower.rows.p, res, so, B.r, B.x, B.p[0], ctx->partials.p, redctr(ctx), si,
+      tol_rel, inner_tol, budget, (T *)nullptr, (T *)nullptr);
+  FCP_TRY((tk_launch_pre<NR, NRP, T>(ctx, B, B.p[0], si, 0)));
   BatchSizer bs(check_every, BATCH_INNER, 0.0);
   int next = bs.cap;
   hipGraphExec_t ge = nullptr;
-  if (ctx->tune_graph && !ctx->prof) FCP_TRY((tk_graph<NR, NRP>(ctx, B, si, &ge)));
+  if (ctx->tune_graph && !ctx->prof) FCP_TRY((tk_graph<NR, NRP, T>(ctx, B, si, &ge)));
   int launched = 0, k = 0;
   while (launched < budget) {
     const int batch = (budget - launched) < next ? (budget - launched) : next;
     int j = 0;
     if (ge && (k & 1) == 0)  // the graph starts from an even iteration
       for (; j + 2 <= batch; j += 2, k += 2) FCP_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
-    for (; j < batch; ++j, ++k) FCP_TRY((tk_iteration<NR, NRP>(ctx, B, si, k)));
+    for (; j < batch; ++j, ++k) FCP_TRY((tk_iteration<NR, NRP, T>(ctx, B, si, k)));
     launched += batch;
     FCP_HIP(ctx, hipMemcpyAsync(h, si, NR * sizeof(SolveScalars), hipMemcpyDeviceToHost,
                                 ctx->stream));
@@ -1951,14 +2117,14 @@ int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, S
     bool all = true;
     for (int c = 0; c < NR; ++c) {
       if (verbose >= 2)
-        printf("                   PCG32[%d] iter = %4d resl = %10.3E rsm = %10.3E\n", c,
-               h[c].iter, h[c].resl, h[c].rsm);
+        printf("                   PCG%d[%d] iter = %4d resl = %10.3E rsm = %10.3E\n",
+               (int)(8 * sizeof(T)), c, h[c].iter, h[c].resl, h[c].rsm);
       all = all && h[c].done;
     }
     if (all) break;
     next = bs.next(h, NR);
   }
-  tk_ir_update<NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->tin_iperm.p, B.x, so, x);
+  tk_ir_update<NR, NRP, T><<<g, BLK, 0, ctx->stream>>>(n, ctx->tin_iperm.p, B.x, so, x);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }
@@ -1993,9 +2159,15 @@ int tb_bicgstab(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *
   auto tile = [&](auto mode) {
     constexpr int MODE = decltype(mode)::value;
     ProfScope ps(ctx, MODE == 1 ? "tinb_pz" : "tinb_xz");
-    tb_tile<NR, NRP, MODE><<<tk_grid(ctx), BLK, lds, ctx->stream>>>(
-        b.nblocks, vslots, b.R, b.blk_pos.p, b.blk_step.p, b.step_ptr.p, dl, al, du, au,
-        ctx->tin_dv.p, X, R, P, U, Z, si);
+    auto go = [&](auto pack) {
+      tb_tile<NR, NRP, MODE, decltype(pack)::value><<<tk_grid(ctx), BLK, lds, ctx->stream>>>(
+          b.nblocks, vslots, b.R, b.blk_pos.p, b.blk_step.p, b.step_ptr.p, dl, al, du, au,
+          ctx->tin_dv.p, X, R, P, U, Z, si);
+    };
+    if (ctx->tin_pack)
+      go(std::true_type());
+    else
+      go(std::false_type());
   };
   auto spmv = [&](auto mode) {
     constexpr int MODE = decltype(mode)::value;
@@ -2044,21 +2216,23 @@ namespace fcpk {
 // The tile-order inner PCG applies: tile-local preconditioner built, at most
 // FCP_PMAXD dependencies per row in either sweep, slots addressable in 16
 // bits, one tile's values and step table in LDS.
-bool tin_usable(fcp_ctx *ctx, int nrp) {
+bool tin_usable(fcp_ctx *ctx, int nrp, bool fp64) {
   const BlockSchedule &b = ctx->jlower;
+  const int ts = fp64 ? (int)sizeof(double) : (int)sizeof(float);
   return ctx->have_bj && b.nblocks > 0 && b.maxd <= FCP_PMAXD && ctx->jupper.maxd <= FCP_PMAXD &&
-         b.R < TK_SLOTS && tk_lds(ctx, nrp) + 256 <= ctx->lds_cap;
+         b.R < TK_SLOTS && tk_lds(ctx, nrp, ts) + 256 <= ctx->lds_cap;
 }
 
-// Inner FP32 PCG of nr interleaved columns (NRP = 1, 2, 4) on the FP64
-// residual res (scaled by so[c].l2), adding the correction to x; the
-// tile-local pivots must be current (bj_begin).  h receives the final inner
-// scalars.  FCP_ERR_UNSUPPORTED: the mesh does not fit (nothing launched).
+// Inner PCG (FP32, or FP64 with fp64 set) of nr interleaved columns
+// (NRP = 1, 2, 4) on the FP64 residual res (scaled by so[c].l2), adding the
+// correction to x; the tile-local pivots must be current (bj_begin).  h
+// receives the final inner scalars.  FCP_ERR_UNSUPPORTED: the mesh does not
+// fit (nothing launched).
 int tin_pcg(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScalars *so,
             SolveScalars *si, double tol_rel, double inner_tol, int budget, int check_every,
-            int verbose, SolveScalars *h) {
+            int verbose, SolveScalars *h, bool fp64) {
   const int nrp = nr <= 1 ? 1 : (nr <= 2 ? 2 : 4);
-  if (!tin_usable(ctx, nrp)) return FCP_ERR_UNSUPPORTED;
+  if (!tin_usable(ctx, nrp, fp64)) return FCP_ERR_UNSUPPORTED;
   // the coarse factorisation overlaps the tile operands' set-up and the first
   // preconditioner application; joined before its first use or on return
   FCP_TRY(tc_begin(ctx));
@@ -2066,25 +2240,30 @@ int tin_pcg(fcp_ctx *ctx, int nr, const double *res, double *x, const SolveScala
     fcp_ctx *c;
     ~Join() { (void)tc_join(c); }
   } join{ctx};
-  const int rc = tk_prepare(ctx);
+  const int rc = tk_prepare(ctx, fp64);
   if (rc == FCP_ERR_UNSUPPORTED) {
     ctx->tin_have = false;
     return rc;
   }
   FCP_TRY(rc);
+  if (fp64) {
+    if (nr != 1) return FCP_ERR_ARG;  // FP64 tile-order iterations: one column
+    return tk_pcg<1, 1, double>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every,
+                                verbose, h);
+  }
   switch (nr) {
     case 1:
-      return tk_pcg<1, 1>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every, verbose,
-                          h);
+      return tk_pcg<1, 1, float>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every,
+                                 verbose, h);
     case 2:
-      return tk_pcg<2, 2>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every, verbose,
-                          h);
+      return tk_pcg<2, 2, float>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every,
+                                 verbose, h);
     case 3:
-      return tk_pcg<3, 4>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every, verbose,
-                          h);
+      return tk_pcg<3, 4, float>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every,
+                                 verbose, h);
     case 4:
-      return tk_pcg<4, 4>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every, verbose,
-                          h);
+      return tk_pcg<4, 4, float>(ctx, res, x, so, si, tol_rel, inner_tol, budget, check_every,
+                                 verbose, h);
     default:
       return FCP_ERR_ARG;
   }
@@ -2127,38 +2306,42 @@ int tin_bicgstab(fcp_ctx *ctx, int nr, const double *res, double *x, const Solve
   }
 }
 
-// z = M^-1 r with the tile-local preconditioner in FP32 exactly as the inner
-// solves apply it (one tile-order application, pivots current); coarse: plus
-// the coarse correction (FCP_ERR_UNSUPPORTED where it does not apply).
-int tin_apply(fcp_ctx *ctx, const double *r, double *z, bool coarse) {
-  if (!tin_usable(ctx, 1)) return FCP_ERR_UNSUPPORTED;
-  FCP_TRY(tk_prepare(ctx));
-  const int n = ctx->n;
-  for (int k = 0; k < 6; ++k) FCP_HIP(ctx, ctx->tin_vec[k].alloc(n));
-  TkBufs B{ctx->tin_vec[0].p, ctx->tin_vec[1].p, ctx->tin_vec[2].p, ctx->tin_vec[3].p,
-           {ctx->tin_vec[4].p, ctx->tin_vec[5].p}};
-  if (coarse) {
-    const int on = ctx->inner_coarse;
-    ctx->inner_coarse = 1;
-    const int rc = tc_bufs(ctx, 1, B);
-    ctx->inner_coarse = on;
-    FCP_TRY(rc);
-    if (!B.r0) return FCP_ERR_UNSUPPORTED;
-    int failed = 0;
-    FCP_HIP(ctx, hipMemcpyAsync(&failed, ctx->tc_flag.p, sizeof(int), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (failed) return FCP_ERR_UNSUPPORTED;
-  }
-  // scalars in the spare slot: not done, unit tolerance, no update
-  SolveScalars *si = ctx->scal.p + 3;
-  FCP_HIP(ctx, hipMemsetAsync(si, 0, sizeof(SolveScalars), ctx->stream));
-  tk_gather_in<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, r, B.r);
-  FCP_TRY((tk_launch_pre<1, 1>(ctx, B, B.p[0], si, 0)));
-  if (coarse) FCP_TRY((tc_launch_zc<1, 1>(ctx, B, si)));
-  tk_scatter_out<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, B.z, z);
-  FCP_HIP(ctx, hipGetLastError());
-  return 0;
+// z = M^-1 r with the tile-local preconditioner in FP32 (FP64 with fp64 set)
+// exactly as the inner solves apply it (one tile-order application, pivots
+// current); coarse: plus the coarse correction (FCP_ERR_UNSUPPORTED where it
+// does not apply).
+int tin_apply(fcp_ctx *ctx, const double *r, double *z, bool coarse, bool fp64) {
+  if (!tin_usable(ctx, 1, fp64)) return FCP_ERR_UNSUPPORTED;
+  FCP_TRY(tk_prepare(ctx, fp64));
+  auto run = [&](auto tag) -> int {
+    using T = decltype(tag);
+    const int n = ctx->n;
+    TkBufs<T> B;
+    FCP_TRY(tk_bufs(ctx, n, B));
+    if (coarse) {
+      const int on = ctx->inner_coarse;
+      ctx->inner_coarse = 1;
+      const int rc = tc_bufs(ctx, 1, B);
+      ctx->inner_coarse = on;
+      FCP_TRY(rc);
+      if (!B.r0) return FCP_ERR_UNSUPPORTED;
+      int failed = 0;
+      FCP_HIP(ctx, hipMemcpyAsync(&failed, ctx->tc_flag.p, sizeof(int), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+      FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      if (failed) return FCP_ERR_UNSUPPORTED;
+    }
+    // scalars in the spare slot: not done, unit tolerance, no update
+    SolveScalars *si = ctx->scal.p + 3;
+    FCP_HIP(ctx, hipMemsetAsync(si, 0, sizeof(SolveScalars), ctx->stream));
+    tk_gather_in<T><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, r, B.r);
+    FCP_TRY((tk_launch_pre<1, 1, T>(ctx, B, B.p[0], si, 0)));
+    if (coarse) FCP_TRY((tc_launch_zc<1, 1, T>(ctx, B, si)));
+    tk_scatter_out<T><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->jlower.rows.p, B.z, z);
+    FCP_HIP(ctx, hipGetLastError());
+    return 0;
+  };
+  return fp64 ? run(0.0) : run(0.0f);
 }
 }  // namespace fcpk
 

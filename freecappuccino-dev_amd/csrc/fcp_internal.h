@@ -207,6 +207,12 @@ struct fcp_ctx {
   DBuf<float> tin_al, tin_au, tin_dv;  // [4n] coefficients, [n] pivots
   DBuf<int> tin_base, tin_col;       // SELL-64 copy of P A P^T
   DBuf<float> tin_val;
+  // packed sweep layout: three 10-bit dependency distances per position and
+  // sweep in the bits of the coefficients' fourth component (every position
+  // with at most 3 in-tile dependencies per sweep, distances below 1024)
+  bool tin_pack = false;
+  DBuf<unsigned> tin_offl, tin_offu;
+  int tune_tile_pack = 1;            // FCP_TUNE_TILE_PACK: use the packed layout where it applies
   // coarse correction of the inner PCG (one unknown per tile): tile of every
   // position, tile adjacency, the coarse operator (then its factor, then its
   // inverse; ncp x ncp row-major) and the inverse factor
@@ -223,6 +229,10 @@ struct fcp_ctx {
   bool tc_pending = false;           // a factorisation on tc_stream not yet joined
   DBuf<double> tc_L, tc_W, tc_r0;
   DBuf<float> tin_vec[7];            // PCG: x, r, z, q, p (two); BiCGStab: x, r, z, p, u, v, reso
+  // FP64 copies of the tile-order operands and PCG vectors (FCP_PREC_FP64_TILE)
+  uint64_t tin_epoch64 = ~0ull;      // pivot epoch of the FP64 values
+  DBuf<double> tin_al64, tin_au64, tin_dv64, tin_val64;
+  DBuf<double> tin_vec64[6];
   bool bj_active = false;            // FP32 sweeps use the tile-local preconditioner
   DBuf<int> tslot;      // [nnz] position of the transposed entry (DILU)
   bool sym_pattern = true;  // every entry has its transposed entry
@@ -287,7 +297,7 @@ struct fcp_ctx {
   int tune_sweep_tile = 0;     // rows per geometric tile (0 = auto, 1 = no tiles)
   int tune_inner_tile = 0;     // rows per tile of the tile-local preconditioner (0 = auto)
   int tune_graph = 0;          // inner iterations replayed from a captured HIP graph
-  int tune_spmv = 0;           // standalone SELL SpMV variant (FCP_TUNE_SPMV)
+  int tune_spmv = -1;          // standalone SELL SpMV variant (FCP_TUNE_SPMV; -1 automatic)
   // captured pair of tile-order inner iterations and the launch arguments it holds
   hipGraphExec_t tin_graph = nullptr;
   std::vector<const void *> tin_graph_key;
@@ -419,7 +429,8 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
 int fcp_ensure_workspace(fcp_ctx *ctx);
 int fcp_precond_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
 int fcp_bj_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
-int fcp_bj32_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out);
+int fcp_bj32_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out,
+                    bool fp64 = false);
 // fcp_inner.hip: a pending coarse factorisation of the inner PCG (side
 // stream) joined before the matrix changes / started after a Laplacian
 // assembly

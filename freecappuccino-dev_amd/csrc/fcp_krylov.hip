@@ -555,7 +555,7 @@ __device__ __forceinline__ T sell_ld(const T *p) {
     return *p;
 }
 
-template <typename T, bool NT, int R>
+template <typename T, bool NT, int R, bool NTY = false>
 __global__ __launch_bounds__(BLK) void k_spmv_rows(int n, Sell<T> m, const T *__restrict__ x,
                                                    T *__restrict__ y) {
   const int b = (int)blockIdx.x, per = (int)gridDim.x >> 3;
@@ -570,7 +570,7 @@ __global__ __launch_bounds__(BLK) void k_spmv_rows(int n, Sell<T> m, const T *__
     const int sl = i >> 6, b0 = m.base[sl];
     p[r] = b0 + (i & 63);
     L[r] = (m.base[sl + 1] - b0) >> 6;
-    len[r] = m.ia[i + 1] - m.ia[i];
+    len[r] = sell_ld<int, NTY>(m.ia + i + 1) - sell_ld<int, NTY>(m.ia + i);
   }
 #pragma unroll
   for (int r = 0; r < R; ++r)
@@ -593,7 +593,12 @@ __global__ __launch_bounds__(BLK) void k_spmv_rows(int n, Sell<T> m, const T *__
       if (k < len[r]) acc = acc + av[r][k] * xv[r][k];
     for (int k = SELL_KU; k < len[r]; ++k)
       acc = acc + sell_ld<T, NT>(m.a + p[r] + 64 * k) * x[sell_ld<int, NT>(m.col + p[r] + 64 * k)];
-    if (row[r] < n) y[row[r]] = acc;
+    if (row[r] < n) {
+      if constexpr (NTY)
+        __builtin_nontemporal_store(acc, y + row[r]);
+      else
+        y[row[r]] = acc;
+    }
   }
 }
 
@@ -1074,7 +1079,15 @@ int fcp_launch_spmv64(fcp_ctx *ctx, const double *, const double *x, double *y) 
   FCP_TRY(sell_view<double>(ctx, &m));
   ProfScope ps(ctx, "spmv_sell_f64");
   const int n = ctx->n;
-  switch (ctx->tune_spmv) {
+  // automatic: non-temporal matrix loads once the operator (values and
+  // columns) exceeds half of the 256 MB Infinity Cache, so that streaming it
+  // does not evict the gathered x lines; cached loads below that
+  int kind = ctx->tune_spmv;
+  if (kind < 0) kind = (size_t)ctx->nnz * 12 > ((size_t)128 << 20) ? 1 : 0;
+  switch (kind) {
+    case 4:
+      k_spmv_rows<double, true, 1, true><<<(grid_for(n, BLK) + 7) / 8 * 8, BLK, 0, ctx->stream>>>(n, m, x, y);
+      break;
     case 1:
       k_spmv_rows<double, true, 1><<<(grid_for(n, BLK) + 7) / 8 * 8, BLK, 0, ctx->stream>>>(n, m, x, y);
       break;
@@ -1557,12 +1570,17 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
   const int n = ctx->n;
   const int g = red_grid(ctx, n);
   SolveScalars *so = ctx->scal.p;      // outer (FP64)
-  SolveScalars *si = ctx->scal.p + 1;  // inner (FP32)
+  SolveScalars *si = ctx->scal.p + 1;  // inner (FP32; FP64 for FCP_PREC_FP64_TILE)
+  // FP64 inner iterations with the inner solves' two-level tile
+  // preconditioner (PCG only), under the same refinement driver
+  const bool tile64 = opts && opts->precision == FCP_PREC_FP64_TILE;
+  if (tile64 && solver != 0)
+    return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "FCP_PREC_FP64_TILE applies to iccg only");
   FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, CTR_INTS * sizeof(int), ctx->stream));
   FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
   // FP32 operator and preconditioner (diagonal computed in FP64 then rounded)
   FCP_HIP(ctx, ctx->a32.alloc(ctx->nnz));
-  if (ctx->a32_version != ctx->a_version) {
+  if (!tile64 && ctx->a32_version != ctx->a_version) {
     k_cast_a<<<red_grid(ctx, ctx->nnz), BLK, 0, ctx->stream>>>(ctx->nnz, ctx->a.p, ctx->a32.p);
     ctx->a32_version = ctx->a_version;
   }
@@ -1590,9 +1608,12 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
     fcp_ctx *c;
     ~CoarseJoin() { (void)tin_coarse_join(c); }
   } coarse_join{ctx};
-  if (solver == 0 && ctx->inner_precond == 1) FCP_TRY(tin_coarse_begin(ctx));
+  if (solver == 0 && (ctx->inner_precond == 1 || tile64)) FCP_TRY(tin_coarse_begin(ctx));
   bool use_bj = false;
-  if (solver != 1 && ctx->inner_precond >= 1) FCP_TRY(bj_prepare(ctx, kind, &use_bj));
+  if (solver != 1 && (ctx->inner_precond >= 1 || tile64)) FCP_TRY(bj_prepare(ctx, kind, &use_bj));
+  if (tile64 && !(use_bj && tin_usable(ctx, 1, true)))
+    return fcp_fail(ctx, FCP_ERR_UNSUPPORTED,
+                    "FCP_PREC_FP64_TILE: the tile preconditioner does not apply to this mesh");
   if (!use_bj) FCP_TRY(global_pivots());
   struct BjScope {
     fcp_ctx *c;
@@ -1629,6 +1650,15 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
       continue;
     }
     const int budget = itr_max - total_inner;
+    if (tile64) {
+      // FP64 inner PCG in tile order, reduced as far as this refinement step
+      // needs (no floor: FP64 reaches it)
+      FCP_TRY(tin_pcg(ctx, 1, ctx->res.p, x, so, si, tol_rel, 0.0, budget, check_every, verbose,
+                      &ctx->h_scal[1], true));
+      total_inner += ctx->h_scal[1].iter;
+      FCP_TRY(true_residual(ctx, x, b, so));
+      continue;
+    }
     // inner FP32 PCG / BiCGStab in tile order (tile-local preconditioner)
     if ((solver == 0 || solver == 2) && use_bj && ctx->inner_precond == 1 && tin_usable(ctx, 1)) {
       if (solver == 0)
@@ -1723,17 +1753,18 @@ int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z) {
 }  // namespace fcpk
 
 // Tile-local DIC (kind 0) / DILU (kind 1) / DIC with the coarse correction
-// (kind 2) applied in FP32 by the tile-order kernels of the inner solves
-// (input rounded to FP32, output widened); d_out
+// (kind 2) applied in FP32 (FP64 with fp64 set) by the tile-order kernels of
+// the inner solves (input rounded to FP32, output widened); d_out
 // (optional) receives the FP64 pivots.
-int fcp_bj32_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out) {
+int fcp_bj32_device(fcp_ctx *ctx, int kind, const double *r, double *z, double *d_out,
+                    bool fp64) {
   FCP_TRY(fcp_ensure_workspace(ctx));
   FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
   FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, CTR_INTS * sizeof(int), ctx->stream));
   bool ok = false;
   const bool coarse = kind == 2;  // tile-local DIC plus the coarse correction
   FCP_TRY(bj_prepare(ctx, coarse ? 0 : kind, &ok));
-  const int rc = ok ? tin_apply(ctx, r, z, coarse) : FCP_ERR_UNSUPPORTED;
+  const int rc = ok ? tin_apply(ctx, r, z, coarse, fp64) : FCP_ERR_UNSUPPORTED;
   if (rc == FCP_ERR_UNSUPPORTED)
     return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "tile-order preconditioner does not fit this mesh");
   FCP_TRY(rc);
@@ -1788,7 +1819,11 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
                      double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts) {
   FCP_TRY(fcp_ensure_workspace(ctx));
   const int prec = opts ? opts->precision : FCP_PREC_FP64;
-  if (solver == 3 && prec == FCP_PREC_MIXED)
+  if (prec != FCP_PREC_FP64 && prec != FCP_PREC_MIXED && prec != FCP_PREC_FP64_TILE)
+    return fcp_fail(ctx, FCP_ERR_ARG, "unknown precision %d", prec);
+  // FCP_PREC_MIXED and FCP_PREC_FP64_TILE run the refinement driver
+  const bool refine = prec != FCP_PREC_FP64;
+  if (solver == 3 && refine)
     return fcp_fail(ctx, FCP_ERR_UNSUPPORTED,
                     "gauss-seidel runs in FP64 only (no refinement variant)");
   const int crit = opts ? opts->criterion : FCP_CRIT_REF_L1;
@@ -1820,7 +1855,7 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
     (void)hipEventDestroy(e1);
     return rc;
   }
-  if (prec == FCP_PREC_MIXED)
+  if (refine)
     rc = solve_mixed(ctx, solver, x, b, itr_max, tol_abs, tol_rel, ce, verbose, opts);
   else
     rc = solve_fp64(ctx, solver, x, b, itr_max, tol_abs, tol_rel, crit, ce, verbose);
@@ -1842,9 +1877,9 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
   (void)hipEventDestroy(e1);
   const SolveScalars &S = ctx->h_scal[0];
   const SolveScalars &F = ctx->h_scal[2];
-  int iters = (prec == FCP_PREC_MIXED) ? (opts ? opts->iterations : 0) : S.iter;
+  int iters = refine ? (opts ? opts->iterations : 0) : S.iter;
   if (opts) {
-    if (prec != FCP_PREC_MIXED) {
+    if (!refine) {
       opts->iterations = S.iter;
       opts->outer_iterations = 1;
       opts->res0 = S.res0;
@@ -1857,9 +1892,9 @@ int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int i
     opts->rel_res_l2 = F.l2 / (F.l2b > 0 ? F.l2b : 1.0);
     opts->time_ms = ms;
   }
-  if (prec != FCP_PREC_MIXED && S.status < 0) return fcp_fail(ctx, S.status, "non-finite residual");
-  if (prec != FCP_PREC_MIXED && !S.conv && S.res0 >= tol_abs) return iters > 0 ? iters : 1;
-  if (prec == FCP_PREC_MIXED && F.l2 > tol_rel * F.l2b && crit == FCP_CRIT_TRUE_L2)
+  if (!refine && S.status < 0) return fcp_fail(ctx, S.status, "non-finite residual");
+  if (!refine && !S.conv && S.res0 >= tol_abs) return iters > 0 ? iters : 1;
+  if (refine && F.l2 > tol_rel * F.l2b && crit == FCP_CRIT_TRUE_L2)
     return iters > 0 ? iters : 1;
   return 0;
 }

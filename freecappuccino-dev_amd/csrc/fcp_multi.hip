@@ -1248,6 +1248,8 @@ int fcp_solve_multi_device(fcp_ctx *ctx, int solver, int nrhs, double *fi, int l
     return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "batched solves support iccg and bicgstab");
   const int n = ctx->n;
   const int prec = opts ? opts->precision : FCP_PREC_FP64;
+  if (prec != FCP_PREC_FP64 && prec != FCP_PREC_MIXED)
+    return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "batched solves run in FP64 or mixed precision");
   const int crit = opts ? opts->criterion : FCP_CRIT_REF_L1;
   const int ce = (opts && opts->check_every > 0) ? opts->check_every : 8;
   const double inner_tol = (opts && opts->inner_tol > 0) ? opts->inner_tol : 1e-4;

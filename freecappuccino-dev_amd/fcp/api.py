@@ -20,6 +20,7 @@ from .mesh import Mesh
 
 FCP_PREC_FP64 = 0
 FCP_PREC_MIXED = 1
+FCP_PREC_FP64_TILE = 2  # FP64 iterations, the mixed path's two-level tile preconditioner
 FCP_CRIT_REF_L1 = 0
 FCP_CRIT_TRUE_L2 = 1
 FCP_GRAD_GAUSS = 0
@@ -36,6 +37,7 @@ FCP_TUNE_INNER_TILE = 7
 FCP_TUNE_GRAPH = 8
 FCP_TUNE_INNER_COARSE = 9
 FCP_TUNE_SPMV = 10
+FCP_TUNE_TILE_PACK = 11
 
 
 class FcpError(RuntimeError):
@@ -345,8 +347,9 @@ class Context:
         """z = M^{-1} r: kind 0 DIC, 1 DILU (returns (z, inverted pivots)),
         2 ILU(0) of pmgmres_ilu (returns (z, None)), 3 / 4 tile-local DIC /
         DILU of the FP32 inner solves in FP64, 5 / 6 the same in FP32 by the
-        inner solves' kernel, 7 kind 5 plus the coarse correction (returns
-        (z, pivots))."""
+        inner solves' kernel, 7 kind 5 plus the coarse correction, 8 / 9
+        kinds 5 / 7 by the same kernel in FP64 (FCP_PREC_FP64_TILE's
+        preconditioner) (returns (z, pivots))."""
         n = self.mesh.n_cells
         z = np.zeros(n)
         d = np.zeros(n) if kind != 2 else None

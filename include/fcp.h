@@ -54,6 +54,14 @@ extern "C" {
 /* Precision of a solve. */
 #define FCP_PREC_FP64 0  /* reference arithmetic, FP64 throughout            */
 #define FCP_PREC_MIXED 1 /* FP32 inner Krylov + FP64 iterative refinement    */
+#define FCP_PREC_FP64_TILE 2 /* iccg only: the refinement driver of
+                                FCP_PREC_MIXED with FP64 inner iterations
+                                and the same two-level tile preconditioner
+                                (tile-local DIC + coarse correction), the
+                                like-for-like FP64 baseline of the mixed
+                                path; FCP_ERR_UNSUPPORTED where that
+                                preconditioner does not apply (no reference
+                                counterpart) */
 
 /* Stopping criterion of a solve. */
 #define FCP_CRIT_REF_L1 0  /* reference test: ||r||_1/||r0||_1 < tol_rel or
@@ -311,7 +319,8 @@ int fcp_spmv_transpose(fcp_ctx *ctx, const double *x, double *y);
  * Kinds 5 and 6 apply the same in FP32 with the tile-order kernel of the
  * inner solves (input rounded to FP32); kind 7 is kind 5 plus the coarse
  * correction P0 A0^-1 P0^T r of FCP_TUNE_INNER_COARSE (FCP_ERR_UNSUPPORTED
- * where it does not apply).  r, z of length nCells. */
+ * where it does not apply).  Kinds 8 and 9 are kinds 5 and 7 in FP64 (the
+ * tile-order kernel of FCP_PREC_FP64_TILE).  r, z of length nCells. */
 int fcp_precond_apply(fcp_ctx *ctx, int kind, const double *r, double *z, double *d);
 
 /* Tile (0-based block number) of every cell in the tile-local
@@ -429,10 +438,21 @@ int fcp_dpcg_noalloc(int n, int nnz, const int *ia, const int *ja, const double 
                                     which cuts the inner iterations several
                                     times on large meshes; 0: tile-local only */
 #define FCP_TUNE_SPMV 10         /* variant of the standalone FP64 SELL SpMV
-                                    (fcp_spmv): 0 one row per thread, 1 the
-                                    same with non-temporal matrix loads, 2 two
-                                    rows per thread with non-temporal matrix
-                                    loads, 3 two rows per thread             */
+                                    (fcp_spmv): -1 (default) automatic: 1 when
+                                    the operator exceeds 128 MB, else 0; 0 one
+                                    row per thread, 1 the same with
+                                    non-temporal matrix loads, 2 two rows per
+                                    thread with non-temporal matrix loads, 3
+                                    two rows per thread, 4 as 1 with
+                                    non-temporal row-length loads and y
+                                    stores                                    */
+#define FCP_TUNE_TILE_PACK 11    /* tile-order sweeps: 1 (default) the packed
+                                    operand layout (three 10-bit dependency
+                                    distances in each coefficient vector's
+                                    fourth component) where every position
+                                    has at most 3 in-tile dependencies per
+                                    sweep; 0 the 4-slot layout.  Both give
+                                    the same results bit for bit          */
 /* Sets a launch-shape parameter of the context; results do not depend on it
  * (FCP_TUNE_INNER_PRECOND and FCP_TUNE_INNER_COARSE change the path of a
  * mixed-precision solve, not its stopping test). */

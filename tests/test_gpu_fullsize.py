@@ -1,9 +1,10 @@
 """The 128^3 jittered-hex pressure-correction Poisson at full size (2,097,152
 cells, the benchmark workload), through size-independent properties:
 
-- the mixed-precision and the FP64 ICCG reach the requested true relative
-  residual, and the mixed solution's error against a tight FP64 solve stays
-  within three times the FP64 solution's own error at the same tolerance;
+- the mixed-precision, the FP64 and the FP64-with-tile-preconditioner ICCG
+  reach the requested true relative residual, and the mixed and FP64-tile
+  solutions' errors against a tight FP64 solve stay within three times the
+  FP64 solution's own error at the same tolerance;
 - the tile-local DIC of the FP32 inner solves is a symmetric operator
   (r2 . M^-1 r1 = r1 . M^-1 r2) in FP64 and, within FP32 rounding, in the
   FP32 tile-order kernel;
@@ -60,11 +61,13 @@ def test_fullsize_mixed_error_within_fp64_error(system):
     nr = np.linalg.norm(xr)
     err = {}
     for tol in (1e-8, 1e-12):
-        for prec in (api.FCP_PREC_MIXED, api.FCP_PREC_FP64):
+        for prec in (api.FCP_PREC_MIXED, api.FCP_PREC_FP64, api.FCP_PREC_FP64_TILE):
             err[prec, tol] = np.linalg.norm(solve(prec, tol) - xr) / nr
         print(f"[case] tol {tol:g}: err mixed {err[api.FCP_PREC_MIXED, tol]:.3e} "
-              f"fp64 {err[api.FCP_PREC_FP64, tol]:.3e}", flush=True)
+              f"fp64 {err[api.FCP_PREC_FP64, tol]:.3e} "
+              f"fp64-tile {err[api.FCP_PREC_FP64_TILE, tol]:.3e}", flush=True)
         assert err[api.FCP_PREC_MIXED, tol] <= 3.0 * err[api.FCP_PREC_FP64, tol], (tol, err)
+        assert err[api.FCP_PREC_FP64_TILE, tol] <= 3.0 * err[api.FCP_PREC_FP64, tol], (tol, err)
     assert err[api.FCP_PREC_MIXED, 1e-12] <= 1e-9, err
 
 

@@ -240,12 +240,12 @@ def test_explicit_entry_points_5x5(api, golden_dir):
         assert np.allclose(x, g[key], rtol=rtol, atol=1e-12), key
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_jittered_poisson_step_true_l2(api, precision):
     """Pressure-correction-style step at 24^3: face diffusivity, Dirichlet on
     one patch, non-orthogonal correction; solve to ||b-Ax||/||b|| <= 1e-8 in
-    FP64 and in FP32-inner/FP64-refinement, compared with a tight oracle
-    solve."""
+    FP64, in FP32-inner/FP64-refinement and in FP64 with the mixed path's
+    tile preconditioner, compared with a tight oracle solve."""
     print(f"[case] jittered step 24^3 precision={precision}", flush=True)
     mesh = workloads.poisson_jittered_mesh(24)
     ctx, geo, csr = _setup(api, mesh)
@@ -577,6 +577,43 @@ def test_tile_order_preconditioner_fp32_bitexact(api, kind, tile):
     ctx.close()
 
 
+@pytest.mark.parametrize("tile", [0, 512])
+def test_tile_order_preconditioner_fp64_bitexact(api, tile):
+    """The tile-order kernel instantiated in FP64 (kind 8, the preconditioner
+    of FCP_PREC_FP64_TILE) applies the tile-local DIC bit for bit as the
+    oracle's DIC of the block-diagonal matrix; with the coarse correction
+    (kind 9) the difference of the two applications is the dense solve of
+    the coarse system (taken with numpy from the CSR matrix) to FP64
+    rounding."""
+    print(f"[case] tile-order precond fp64 tile {tile} 32^3", flush=True)
+    mesh = hex_mesh(32, jitter=0.2, seed=4)
+    ctx, geo, csr = _setup(api, mesh)
+    ctx.tune(api.FCP_TUNE_INNER_TILE, tile)
+    o = Oracle(mesh)
+    mu = -(10.0 ** np.sin(np.arange(mesh.n_cells)))
+    a, _ = o.laplacian(mu, np.zeros(mesh.n_total))
+    ctx.matrix_set(a)
+    tiles = ctx.sweep_tiles()
+    assert tiles.max() >= 3
+    rows = np.repeat(np.arange(mesh.n_cells), np.diff(o.ia))
+    ab = np.where(tiles[rows] == tiles[o.ja], a, 0.0)
+    r = np.sin(np.arange(mesh.n_cells) * 0.1)
+    z8, d = ctx.precond_apply(r, kind=8)
+    d_o = o.dic_diag(ab)
+    assert np.array_equal(d, d_o)
+    assert np.array_equal(z8, o.dic_apply(ab, d_o, r))
+    import scipy.sparse as sp
+    nn, nc = mesh.n_cells, int(tiles.max()) + 1
+    A = sp.csr_matrix((a, o.ja, o.ia), shape=(nn, nn))
+    P0 = sp.csr_matrix((np.ones(nn), (np.arange(nn), tiles)), shape=(nn, nc))
+    A0 = (P0.T @ A @ P0).toarray()
+    z9, _ = ctx.precond_apply(r, kind=9)
+    e0 = np.linalg.solve(A0, P0.T @ r)
+    scale = np.abs(z9).max() + np.abs(e0).max()
+    assert np.abs((z9 - z8) - e0[tiles]).max() <= 1e-11 * scale
+    ctx.close()
+
+
 @pytest.mark.parametrize("n,tile", [(32, 0), (32, 512), (48, 256), (48, 64)])
 def test_coarse_correction_matches_dense_solve(api, n, tile):
     """Kind 7 adds the coarse correction P0 A0^-1 P0^T r (A0 = P0^T A P0, P0
@@ -693,6 +730,46 @@ def test_inner_graph_replay_bitequal_to_launches(api):
     for x, X in out[1]:
         assert np.array_equal(x, out[0][0][0])
         assert np.array_equal(X, out[0][0][1])
+    ctx.close()
+
+
+@pytest.mark.parametrize("tile", [0, 512])
+def test_tile_pack_layout_bitequal(api, tile):
+    """The packed tile-order operand layout (FCP_TUNE_TILE_PACK 1, the
+    default) and the 4-slot layout give the same mixed-precision ICCG and
+    BiCGStab solves and the same FP64-tile ICCG solve, bit for bit and in
+    iterations, with one workgroup per tile and with one wave per tile."""
+    print(f"[case] tile pack layout tile {tile} 24^3", flush=True)
+    mesh = workloads.poisson_jittered_mesh(24)
+    ctx, geo, csr = _setup(api, mesh)
+    ctx.tune(api.FCP_TUNE_INNER_TILE, tile)
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    o = Oracle(mesh)
+    F = mesh.n_faces
+    bk = np.full(mesh.n_bnd, 2, np.uint8)
+    a_cd, su_cd, _, _ = o.convdiff(np.cos(np.arange(F)) * 1e-2, 10.0 ** np.sin(np.arange(F)),
+                                   np.zeros(mesh.n_total), np.zeros((mesh.n_total, 3)), 1.0, 0,
+                                   bk)
+    out = {}
+    for pack in (1, 0):
+        ctx.tune(api.FCP_TUNE_TILE_PACK, pack)
+        res = []
+        for solver, prec in (("iccg", api.FCP_PREC_MIXED), ("iccg", api.FCP_PREC_FP64_TILE),
+                             ("bicgstab", api.FCP_PREC_MIXED)):
+            if solver == "iccg":
+                su = inp["source"] * geo["vol"]
+                ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su)
+            else:
+                ctx.matrix_set(a_cd)
+                su = su_cd.copy()
+            x = np.zeros(mesh.n_cells)
+            rc, info = ctx.csrsolve(solver, x, su, itr_max=5000, tol_abs=0.0, tol_rel=1e-10,
+                                    precision=prec, criterion=api.FCP_CRIT_TRUE_L2)
+            assert rc == 0 and info["rel_res_l2"] <= 1e-10, (pack, solver, prec, info)
+            res.append((x, info["iterations"]))
+        out[pack] = res
+    for (x1, i1), (x0, i0) in zip(out[1], out[0]):
+        assert i1 == i0 and np.array_equal(x1, x0)
     ctx.close()
 
 
