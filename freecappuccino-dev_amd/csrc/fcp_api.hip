@@ -278,6 +278,10 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
         ctx->tin_have = false;  // operands rebuilt in the chosen layout
       }
       return 0;
+    case FCP_TUNE_COARSE_FUSE:
+      if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "coarse fusion is 0 or 1");
+      ctx->tune_coarse_fuse = value;
+      return 0;
     case FCP_TUNE_GRAPH:
       if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "graph replay is 0 or 1");
       ctx->tune_graph = value;

@@ -489,13 +489,57 @@ __device__ __forceinline__ void tk_sweeps_w(T *V, T *D, const int2 *sb, int ns, 
 // ---------------------------------------------------------------------------
 // iteration kernels
 // ---------------------------------------------------------------------------
+// Operands of the coarse correction when tk_pre applies it itself.
+struct CoarseFuse {
+  const double *G;       // ncp x ncp rows of A0^-1 (row T: tile T's coarse row)
+  const int *failed;     // the coarse factor failed: no correction
+  unsigned *bar;         // grid barrier: arrivals at [0], generation at [32]
+  int *err;              // FCP_ERR_SWEEP when the barrier wait exceeds its bound
+};
+
+// Every workgroup of the grid arrives before any leaves.  The caller's
+// agent-scope stores are complete before its arrival is counted (as in
+// last_block_reduce); the wait is bounded by wall time like the sweep waits
+// (FCP_ERR_SWEEP recorded and the wait abandoned), so the kernel always
+// drains.  The generation is read before arriving, so a wave cannot miss the
+// release of the barrier it waits for.
+__device__ __forceinline__ void grid_barrier(unsigned *bar, int *err) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned gen = __hip_atomic_load(bar + 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // generation read, this block's stores complete
+    if (__hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+        gridDim.x - 1) {
+      __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_s_waitcnt(0);
+      __hip_atomic_fetch_add(bar + 32, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      const unsigned long long t0 = wall_clock64();
+      unsigned spins = 0;
+      while (__hip_atomic_load(bar + 32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        __builtin_amdgcn_s_sleep(2);
+        if ((++spins & 63u) == 0 && wall_clock64() - t0 > FCP_WAIT_TICKS) {
+          __hip_atomic_store(err, FCP_ERR_SWEEP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // One workgroup per tile (tiles nb..: loop).  upd: apply the previous
 // iteration's update x += alf p, r -= alf q with |r|_1, |r|_2 and its
 // stopping test (relative L2 against the column's tolerance, at most
 // s->imax iterations); then
 // z = M^-1 r and sk = r.z, bet = sk / s0.  Columns already done are left
 // untouched.
-template <int NR, int NRP, int NT, bool PACK, class T>
+// CF (coarse fused; one workgroup per tile, every workgroup resident): after
+// the sweeps every tile's residual sum is published, a grid barrier waits for
+// all of them, and each workgroup adds its tile's entry of e0 = A0^-1 r0 to z
+// while z is still in LDS and contributes e0[T] r0[T] to sk -- the work of
+// tc_zc, with the same e0 arithmetic, without its kernel and its pass over z.
+template <int NR, int NRP, int NT, bool PACK, class T, bool CF = false>
 __global__ __launch_bounds__(NT) void tk_pre(
     int nb, int vslots, int dslot, const int *__restrict__ bpos, const int *__restrict__ blk_step,
     const int *__restrict__ step_ptr, const int2 *__restrict__ dl, const V4<T> *__restrict__ al,
@@ -503,7 +547,7 @@ __global__ __launch_bounds__(NT) void tk_pre(
     T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
     const T *__restrict__ q, T *__restrict__ z, double *__restrict__ partials,
     int *__restrict__ counter, SolveScalars *__restrict__ s, int upd, double tol_abs,
-    double *__restrict__ r0, int ncp) {
+    double *__restrict__ r0, int ncp, CoarseFuse cf = {}) {
   // [vslots] values (slot dslot = 0), the tile's pivots, then the step bounds
   extern __shared__ __align__(16) unsigned char tk_smem[];
   T *V = reinterpret_cast<T *>(tk_smem);
@@ -517,9 +561,11 @@ __global__ __launch_bounds__(NT) void tk_pre(
     act[c] = !solve_done(s + c);
     alf[c] = (T)s[c].alf;
   }
-  double part[3 * NR];
+  // |r|_1, |r|_2, r.z per column, then (CF) e0[T] r0[T]
+  constexpr int NV = CF ? 4 * NR : 3 * NR;
+  double part[NV];
 #pragma unroll
-  for (int k = 0; k < 3 * NR; ++k) part[k] = 0.0;
+  for (int k = 0; k < NV; ++k) part[k] = 0.0;
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
     const int lo = bpos[b], hi = bpos[b + 1];
     const int st0 = blk_step[b], ns = blk_step[b + 1] - st0;
@@ -568,30 +614,71 @@ __global__ __launch_bounds__(NT) void tk_pre(
       }
     }
     if (threadIdx.x < NRP) V[dslot * NRP + threadIdx.x] = (T)0;  // the unused-dependency slot
+    double ts[NR];  // the tile's residual sums (thread 0)
+#pragma unroll
+    for (int c = 0; c < NR; ++c) ts[c] = 0.0;
     if (r0) {  // uniform
-      double ts[NR];
       block_sum<NR, NT>(rs, ts);
       if (threadIdx.x == 0)
 #pragma unroll
-        for (int c = 0; c < NR; ++c) r0[(size_t)c * ncp + b] = ts[c];
+        for (int c = 0; c < NR; ++c) {
+          if (CF)  // read by every workgroup after the grid barrier
+            __hip_atomic_store(reinterpret_cast<unsigned long long *>(&r0[(size_t)c * ncp + b]),
+                               (unsigned long long)__double_as_longlong(ts[c]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          else
+            r0[(size_t)c * ncp + b] = ts[c];
+        }
     }
     __syncthreads();
     if (ns > 0) tk_sweeps<NR, NRP, NT, PACK>(V, D, sb, ns, lo, dslot, dl, al, du, au, dv);
-    // z out; r.z with the residual this thread wrote (or read) above
+    T e[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) e[c] = (T)0;
+    if (CF) {
+      // e0[b] = (row b of A0^-1) . r0, summed as tc_zc sums it: thread t over
+      // m = t mod NT, then the waves in order
+      __shared__ double SC[NR][NT / 64];
+      grid_barrier(cf.bar, cf.err);
+      const int t = (int)threadIdx.x, lane = t & 63, wv = t >> 6;
+      const double *g = cf.G + (size_t)b * ncp;
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        double sum = 0.0;
+        for (int m = t; m < ncp; m += NT)
+          sum += g[m] * __hip_atomic_load(&r0[(size_t)c * ncp + m], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        sum = wave_sum(sum);
+        if (lane == 0) SC[c][wv] = sum;
+      }
+      __syncthreads();
+      const bool off = *cf.failed != 0;
+#pragma unroll
+      for (int c = 0; c < NR; ++c) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) v += SC[c][w];
+        const double E = off ? 0.0 : v;
+        e[c] = (T)E;
+        if (t == 0) part[3 * NR + c] = E * ts[c];
+      }
+    }
+    // z out (plus the coarse correction); r.z with the residual this thread
+    // wrote (or read) above and the tile-local z
     for (int i = lo + (int)threadIdx.x; i < hi; i += NT) {
 #pragma unroll
       for (int c = 0; c < NR; ++c) {
         const size_t o = (size_t)i * NRP + c;
         const T zi = V[(i - lo) * NRP + c];
-        z[o] = zi;
+        z[o] = (CF && act[c]) ? zi + e[c] : zi;
         part[3 * c + 2] += (double)r[o] * (double)zi;
       }
     }
     __syncthreads();  // V and sb are reused by the next tile
   }
-  double blk[3 * NR], tot[3 * NR];
-  block_sum<3 * NR, NT>(part, blk);
-  if (last_block_reduce<3 * NR, NT>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
+  double blk[NV], tot[NV];
+  block_sum<NV, NT>(part, blk);
+  if (last_block_reduce<NV, NT>(blk, partials, 0, counter, tot) && threadIdx.x == 0) {
 #pragma unroll
     for (int c = 0; c < NR; ++c) {
       if (!act[c]) continue;
@@ -617,6 +704,7 @@ __global__ __launch_bounds__(NT) void tk_pre(
       }
       if (!fin) {
         S.sk = tot[3 * c + 2];
+        if (CF) S.sk = S.sk + tot[3 * NR + c];  // the coarse part, as tc_zc adds it
         S.bet = S.sk / S.s0;
       }
     }
@@ -1801,6 +1889,7 @@ struct TkBufs {
   T *x, *r, *z, *q, *p[2];
   double *r0 = nullptr;  // tile sums of r (null: no coarse correction)
   int ncp = 0;
+  bool fuse = false;     // tk_pre applies the coarse correction itself
   const int2 *dl = nullptr, *du = nullptr;
   const V4<T> *al = nullptr, *au = nullptr;
   const T *dv = nullptr, *val = nullptr;
@@ -1998,7 +2087,13 @@ int tk_launch_pre(fcp_ctx *ctx, const TkBufs<T> &B, const T *p, SolveScalars *si
   ProfScope ps(ctx, TS == 8 ? "tin_pre64" : "tin_pre");
   auto go = [&](auto pack) {
     constexpr bool PK = decltype(pack)::value;
-    if (ctx->bj_wave) {
+    if (B.fuse) {
+      const CoarseFuse cf{ctx->tc_L.p, ctx->tc_flag.p, ctx->tin_bar.p, ctx->errflag.p};
+      tk_pre<NR, NRP, TK_NT, PK, T, true><<<b.nblocks, TK_NT, tk_lds(ctx, NRP, TS), ctx->stream>>>(
+          b.nblocks, ((b.R + 1) * NRP + 1) & ~1, b.R, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
+          B.dl, B.al, B.du, B.au, B.dv, B.x, B.r, p, B.q, B.z, ctx->partials.p, redctr(ctx), si,
+          upd, 0.0, B.r0, B.ncp, cf);
+    } else if (ctx->bj_wave) {
       int g = (b.nblocks + BLK / 64 - 1) / (BLK / 64);
       g = g < MAXBLK ? g : MAXBLK;
       tk_pre_w<NR, NRP, PK, T><<<g, BLK, tk_lds(ctx, NRP, TS), ctx->stream>>>(
@@ -2037,9 +2132,27 @@ int tc_launch_zc(fcp_ctx *ctx, const TkBufs<T> &B, SolveScalars *si) {
   return hipGetLastError() == hipSuccess ? 0 : FCP_ERR_HIP;
 }
 
+// Whether tk_pre can apply the coarse correction itself: one workgroup per
+// tile (not the one-wave tiles), every workgroup resident at once (the grid
+// barrier), the reduction's values within the partials' slots.
+template <int NR, int NRP, class T>
+bool tk_can_fuse(fcp_ctx *ctx) {
+  const BlockSchedule &b = ctx->jlower;
+  if (!ctx->tune_coarse_fuse || ctx->bj_wave || 4 * NR > 4 * NSLOT || b.nblocks > MAXBLK)
+    return false;
+  const size_t lds = tk_lds(ctx, NRP, sizeof(T));
+  int per_cu = 0;
+  const void *k = ctx->tin_pack
+                      ? reinterpret_cast<const void *>(tk_pre<NR, NRP, TK_NT, true, T, true>)
+                      : reinterpret_cast<const void *>(tk_pre<NR, NRP, TK_NT, false, T, true>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, TK_NT, lds) != hipSuccess)
+    return false;
+  return (long long)per_cu * ctx->ncu >= b.nblocks;
+}
+
 template <int NR, int NRP, class T>
 int tk_iteration(fcp_ctx *ctx, const TkBufs<T> &B, SolveScalars *si, int k) {
-  if (B.r0) FCP_TRY((tc_launch_zc<NR, NRP, T>(ctx, B, si)));
+  if (B.r0 && !B.fuse) FCP_TRY((tc_launch_zc<NR, NRP, T>(ctx, B, si)));
   T *pold = B.p[k & 1], *pnew = B.p[(k + 1) & 1];
   {
     ProfScope ps(ctx, sizeof(T) == 8 ? "tin_spmv64" : "tin_spmv");
@@ -2057,6 +2170,7 @@ int tk_graph(fcp_ctx *ctx, const TkBufs<T> &B, SolveScalars *si, hipGraphExec_t 
   const BlockSchedule &b = ctx->jlower;
   const std::vector<const void *> key = {
       (const void *)(size_t)NR, (const void *)sizeof(T), (const void *)(size_t)ctx->tin_pack,
+      (const void *)(size_t)B.fuse, ctx->tin_bar.p, ctx->errflag.p,
       B.x, B.r, B.z, B.q, B.p[0], B.p[1], si,
       ctx->partials.p, ctx->counters.p, B.dl, B.du, B.al, B.au, B.dv,
       ctx->tin_base.p, ctx->tin_col.p, B.val, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
@@ -2093,6 +2207,11 @@ int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, S
   TkBufs<T> B;
   FCP_TRY(tk_bufs(ctx, (size_t)n * NRP, B));
   FCP_TRY(tc_bufs(ctx, NR, B));
+  if (B.r0 && tk_can_fuse<NR, NRP, T>(ctx)) {
+    FCP_HIP(ctx, ctx->tin_bar.alloc(64));
+    FCP_HIP(ctx, hipMemsetAsync(ctx->tin_bar.p, 0, 64 * sizeof(unsigned), ctx->stream));
+    B.fuse = true;
+  }
   const int g = red_grid(ctx, n);
   tk_init<NR, NRP, T><<<g, BLK, 0, ctx->stream>>>(
       n, ctx->jlower.rows.p, res, so, B.r, B.x, B.p[0], ctx->partials.p, redctr(ctx), si,

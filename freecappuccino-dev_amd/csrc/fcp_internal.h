@@ -213,6 +213,8 @@ struct fcp_ctx {
   bool tin_pack = false;
   DBuf<unsigned> tin_offl, tin_offu;
   int tune_tile_pack = 1;            // FCP_TUNE_TILE_PACK: use the packed layout where it applies
+  int tune_coarse_fuse = 1;          // FCP_TUNE_COARSE_FUSE: coarse correction inside tk_pre
+  DBuf<unsigned> tin_bar;            // grid barrier of the fused tk_pre (two 128-byte lines)
   // coarse correction of the inner PCG (one unknown per tile): tile of every
   // position, tile adjacency, the coarse operator (then its factor, then its
   // inverse; ncp x ncp row-major) and the inverse factor

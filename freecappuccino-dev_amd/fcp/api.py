@@ -38,6 +38,7 @@ FCP_TUNE_GRAPH = 8
 FCP_TUNE_INNER_COARSE = 9
 FCP_TUNE_SPMV = 10
 FCP_TUNE_TILE_PACK = 11
+FCP_TUNE_COARSE_FUSE = 12
 
 
 class FcpError(RuntimeError):

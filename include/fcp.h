@@ -453,6 +453,13 @@ int fcp_dpcg_noalloc(int n, int nnz, const int *ia, const int *ja, const double 
                                     has at most 3 in-tile dependencies per
                                     sweep; 0 the 4-slot layout.  Both give
                                     the same results bit for bit          */
+#define FCP_TUNE_COARSE_FUSE 12  /* inner PCG with the coarse correction: 1
+                                    (default) the tile kernel adds it itself
+                                    after a grid barrier where every tile's
+                                    workgroup fits on the device at once; 0
+                                    a separate kernel.  The two differ only
+                                    in the summation order of one dot
+                                    product                                */
 /* Sets a launch-shape parameter of the context; results do not depend on it
  * (FCP_TUNE_INNER_PRECOND and FCP_TUNE_INNER_COARSE change the path of a
  * mixed-precision solve, not its stopping test). */

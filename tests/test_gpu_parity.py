@@ -773,6 +773,37 @@ def test_tile_pack_layout_bitequal(api, tile):
     ctx.close()
 
 
+@pytest.mark.parametrize("prec", [1, 2])
+def test_coarse_fused_matches_separate_kernel(api, prec):
+    """The coarse correction applied inside the tile kernel after its grid
+    barrier (FCP_TUNE_COARSE_FUSE 1, the default) and by the separate kernel
+    (0) compute the same e0 and z; only the summation order of r.z's coarse
+    part differs.  Both reach the tolerance in the same number of iterations
+    (within one) with solutions that agree to well below it, and the fused
+    solve is bit-for-bit repeatable."""
+    print(f"[case] coarse fused vs separate precision {prec} 40^3", flush=True)
+    mesh = workloads.poisson_jittered_mesh(40)
+    ctx, geo, csr = _setup(api, mesh)
+    inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
+    out = []
+    for fuse in (1, 1, 0):
+        ctx.tune(api.FCP_TUNE_COARSE_FUSE, fuse)
+        su = inp["source"] * geo["vol"]
+        ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su)
+        x = np.zeros(mesh.n_cells)
+        rc, info = ctx.csrsolve("iccg", x, su, itr_max=5000, tol_abs=0.0, tol_rel=1e-10,
+                                precision=prec, criterion=api.FCP_CRIT_TRUE_L2)
+        assert rc == 0 and info["rel_res_l2"] <= 1e-10, (fuse, info)
+        out.append((x, info["iterations"]))
+    assert np.array_equal(out[0][0], out[1][0]) and out[0][1] == out[1][1]
+    assert abs(out[0][1] - out[2][1]) <= 1, (out[0][1], out[2][1])
+    nx = np.linalg.norm(out[2][0])
+    assert np.linalg.norm(out[0][0] - out[2][0]) <= 1e-7 * nx
+    tiles = ctx.sweep_tiles()
+    assert tiles.max() >= 7  # the coarse correction applies
+    ctx.close()
+
+
 def test_autotune_picks_and_caches(api):
     """fcp_autotune picks one of the candidate tile sizes for the mesh; a
     second call, and a new context on the same mesh, take it from the cache;
