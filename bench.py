@@ -544,8 +544,10 @@ def run_adjoint(args):
 
 def run_probe(args):
     """A short fixed kernel sequence for hardware-counter runs: a calibration
-    copy of known size (256 MiB read + 256 MiB written), then 10 mixed and 10
-    FP64 PCG iterations and 5 SELL SpMVs on the benchmark system."""
+    copy of known size (256 MiB read + 256 MiB written), one assembly of the
+    benchmark step (face Laplacian, Gauss gradient, non-orthogonal
+    correction), then 10 mixed and 10 FP64 PCG iterations and 5 SELL SpMVs
+    on the benchmark system."""
     import torch
 
     from fcp import workloads
@@ -559,14 +561,17 @@ def run_probe(args):
     ctx.create_CSR_matrix(want_arrays=False)
     inp = workloads.poisson_jittered_inputs(mesh, geo["vol"], geo["xc"], geo["yc"], geo["zc"])
     dev = torch.device("cuda:0")
-    su = torch.from_numpy(inp["source"] * geo["vol"]).to(dev)
-    ctx.laplacian_face(torch.from_numpy(inp["mu_f"]).to(dev),
-                       torch.from_numpy(inp["bc_kind"]).to(dev), torch.zeros(mesh.n_total,
-                       dtype=torch.float64, device=dev), su)
+    T = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
+    su = T(inp["source"] * geo["vol"])
+    grad = torch.empty((mesh.n_total, 3), dtype=torch.float64, device=dev)
     src = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     dst.copy_(src)
+    torch.cuda.synchronize()
+    ctx.laplacian_face(T(inp["mu_f"]), T(inp["bc_kind"], torch.uint8), T(inp["phi_bc"]), su)
+    ctx.grad_gauss(T(inp["phi_prev"]), grad)
+    ctx.nonorth_correction(T(inp["gamma"]), grad, su)
     torch.cuda.synchronize()
     x = torch.zeros(mesh.n_cells, dtype=torch.float64, device=dev)
     y = torch.empty_like(x)

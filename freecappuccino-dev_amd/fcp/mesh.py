@@ -61,6 +61,10 @@ class Mesh:
     types: list = field(default_factory=list)
     nfaces: np.ndarray = None
     start_face: np.ndarray = None
+    # start face of the twin of every 'periodic' patch, in patch order (the
+    # fifth column of a periodic patch's line in polyMesh/boundary,
+    # geometry.f90:251-258; 0-based like start_face)
+    twin_start: list = field(default_factory=list)
 
     @property
     def n_nodes(self) -> int:
@@ -90,6 +94,10 @@ class Mesh:
     def bctype_codes(self) -> np.ndarray:
         return np.array([BCTYPE_CODES[t] for t in self.types], dtype=np.int32)
 
+    @property
+    def periodic_patches(self) -> list:
+        return [ib for ib, t in enumerate(self.types) if t == "periodic"]
+
     def patch_of_bface(self) -> np.ndarray:
         """Patch index of every boundary face (length nBnd)."""
         out = np.empty(self.n_bnd, dtype=np.int32)
@@ -107,11 +115,19 @@ class Mesh:
         if self.n_inner:
             assert self.neighbour.min() >= 0 and self.neighbour.max() < self.n_cells
             assert np.all(self.neighbour != self.owner[: self.n_inner])
+        per = self.periodic_patches
+        assert len(self.twin_start) == len(per)
+        for ib, t in zip(per, self.twin_start):  # the twin is a whole patch of equal size
+            tw = [jb for jb in range(self.n_patches) if int(self.start_face[jb]) == int(t)]
+            assert len(tw) == 1 and int(self.nfaces[tw[0]]) == int(self.nfaces[ib]), (ib, t)
 
 
 def hex_mesh(N: int, jitter: float = 0.0, seed: int = 2,
              types=("wall",) * 6) -> Mesh:
-    """Unit cube, N^3 hexahedra, optional interior-node jitter (fraction of h)."""
+    """Unit cube, N^3 hexahedra, optional interior-node jitter (fraction of h).
+    A 'periodic' patch takes the other patch of its axis as its twin (the
+    reference's convention: the twin is listed with its own type, usually
+    'empty')."""
     N = int(N)
     M = N + 1
     h = 1.0 / N
@@ -204,9 +220,11 @@ def hex_mesh(N: int, jitter: float = 0.0, seed: int = 2,
     node[:, :4] = allq
     nnodes = np.full(nf, 4, dtype=np.int32)
     owner = np.concatenate([own] + bown).astype(np.int32)
+    twins = [int(start[p ^ 1]) for p in range(6) if types[p] == "periodic"]
     m = Mesh(points=pts, nnodes=nnodes, node=node, owner=owner,
              neighbour=nb.astype(np.int32), n_cells=ncell, names=names,
-             types=list(types), nfaces=nb_faces, start_face=start.astype(np.int32))
+             types=list(types), nfaces=nb_faces, start_face=start.astype(np.int32),
+             twin_start=twins)
     m.check()
     return m
 
@@ -236,9 +254,12 @@ def write_native(mesh: Mesh, case_dir: str) -> None:
     np.savetxt(os.path.join(d, "owner"), mesh.owner + 1, fmt="%d")
     np.savetxt(os.path.join(d, "neighbour"), mesh.neighbour + 1, fmt="%d")
     with open(os.path.join(d, "boundary"), "w") as f:
-        f.write("# bcname bctype nFaces startFace\n")
+        f.write("# bcname bctype nFaces startFace [startFaceTwin]\n")
+        twin = dict(zip(mesh.periodic_patches, mesh.twin_start))
         for ib in range(mesh.n_patches):
-            f.write(f"{mesh.names[ib]} {mesh.types[ib]} {int(mesh.nfaces[ib])} {int(mesh.start_face[ib])}\n")
+            extra = f" {int(twin[ib])}" if ib in twin else ""
+            f.write(f"{mesh.names[ib]} {mesh.types[ib]} {int(mesh.nfaces[ib])} "
+                    f"{int(mesh.start_face[ib])}{extra}\n")
     with open(os.path.join(d, "size"), "w") as f:
         for v in (mesh.n_nodes, mesh.n_cells, mesh.n_inner, mesh.n_bnd, mesh.n_faces):
             f.write(f"{v}\n")
@@ -258,19 +279,28 @@ def read_native(case_dir: str) -> Mesh:
             node[k, : v[0]] = np.array(v[1: 1 + v[0]]) - 1
     owner = np.loadtxt(os.path.join(d, "owner"), dtype=np.int64).reshape(-1).astype(np.int32) - 1
     nb = np.loadtxt(os.path.join(d, "neighbour"), dtype=np.int64).reshape(-1).astype(np.int32) - 1
-    names, types, nf, sf = [], [], [], []
-    with open(os.path.join(d, "boundary")) as f:
+    names, types, nf, sf, tw = _read_boundary(os.path.join(d, "boundary"))
+    m = Mesh(points=pts, nnodes=nnodes, node=node, owner=owner, neighbour=nb,
+             n_cells=n_cells, names=names, types=types,
+             nfaces=np.array(nf, np.int32), start_face=np.array(sf, np.int32), twin_start=tw)
+    m.check()
+    return m
+
+
+def _read_boundary(path):
+    """polyMesh/boundary table: name type nFaces startFace, plus startFaceTwin
+    on 'periodic' lines (geometry.f90:234-258); '#' lines are comments."""
+    names, types, nf, sf, tw = [], [], [], [], []
+    with open(path) as f:
         for line in f:
             s = line.strip()
             if not s or s.startswith("#"):
                 continue
             t = s.split()
             names.append(t[0]); types.append(t[1]); nf.append(int(t[2])); sf.append(int(t[3]))
-    m = Mesh(points=pts, nnodes=nnodes, node=node, owner=owner, neighbour=nb,
-             n_cells=n_cells, names=names, types=types,
-             nfaces=np.array(nf, np.int32), start_face=np.array(sf, np.int32))
-    m.check()
-    return m
+            if t[1] == "periodic":
+                tw.append(int(t[4]))
+    return names, types, nf, sf, tw
 
 
 # --------------------------------------------------------------------------
@@ -315,18 +345,11 @@ def read_foam(case_dir: str) -> Mesh:
     owner = np.array(body.split(")")[0].split(), dtype=np.int64)[:no].astype(np.int32)
     nn, body = _foam_list(os.path.join(d, "neighbour"))
     nb = np.array(body.split(")")[0].split(), dtype=np.int64)[:nn].astype(np.int32)
-    names, types, nfs, sfs = [], [], [], []
-    with open(os.path.join(d, "boundary")) as f:
-        for line in f:
-            s = line.strip()
-            if not s or s.startswith("#"):
-                continue
-            t = s.split()
-            names.append(t[0]); types.append(t[1]); nfs.append(int(t[2])); sfs.append(int(t[3]))
+    names, types, nfs, sfs, tw = _read_boundary(os.path.join(d, "boundary"))
     n_cells = int(max(owner.max(), nb.max() if nb.size else 0)) + 1
     m = Mesh(points=pts, nnodes=nnodes, node=node, owner=owner, neighbour=nb,
              n_cells=n_cells, names=names, types=types,
-             nfaces=np.array(nfs, np.int32), start_face=np.array(sfs, np.int32))
+             nfaces=np.array(nfs, np.int32), start_face=np.array(sfs, np.int32), twin_start=tw)
     m.check()
     return m
 
@@ -372,7 +395,8 @@ def permute_cells(mesh: Mesh, new_of_old: np.ndarray) -> Mesh:
              node=np.concatenate(bnodes).astype(np.int32),
              owner=np.concatenate(own).astype(np.int32), neighbour=hi[order].astype(np.int32),
              n_cells=mesh.n_cells, names=list(mesh.names), types=list(mesh.types),
-             nfaces=mesh.nfaces.copy(), start_face=mesh.start_face.copy())
+             nfaces=mesh.nfaces.copy(), start_face=mesh.start_face.copy(),
+             twin_start=list(mesh.twin_start))
     m.check()
     return m
 

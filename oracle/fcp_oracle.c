@@ -131,30 +131,108 @@ int orc_geometry(const orc_mesh *m, orc_geom *g) {
 /* ------------------------------------------------------------------------ */
 /* CSR: sparse_matrix.f90:86-296, utils.f90:96-189                          */
 /* ------------------------------------------------------------------------ */
-static int has_periodic(const orc_mesh *m) {
+int orc_n_periodic(const orc_mesh *m) {
+  int np = 0;
   for (int ib = 0; ib < m->n_patches; ++ib)
-    if (m->bctype[ib] == 7) return 1;
+    if (m->bctype[ib] == 7) np += m->nfaces[ib];
+  return np;
+}
+
+/* Periodic pair k (patch order): the owner of periodic face i and of twin
+ * face i (sparse_matrix.f90:141-171); returns the number of pairs. */
+static int periodic_pairs(const orc_mesh *m, int *pp, int *pn) {
+  int k = 0, iper = 0;
+  for (int ib = 0; ib < m->n_patches; ++ib) {
+    if (m->bctype[ib] != 7) continue;
+    const int t0 = m->twin_start[iper++];
+    for (int i = 0; i < m->nfaces[ib]; ++i, ++k) {
+      if (pp) pp[k] = m->owner[m->start_face[ib] + i];
+      if (pn) pn[k] = m->owner[t0 + i];
+    }
+  }
+  return k;
+}
+
+/* Boundary face f on a periodic patch: its index in the patch, the twin's
+ * start face and its periodic ordinal (patch order); 0 when f is elsewhere. */
+static int periodic_face(const orc_mesh *m, int f, int *i, int *t0, int *k) {
+  int base = 0, iper = 0;
+  for (int ib = 0; ib < m->n_patches; ++ib) {
+    if (m->bctype[ib] != 7) continue;
+    const int s0 = m->start_face[ib];
+    if (f >= s0 && f < s0 + m->nfaces[ib]) {
+      *i = f - s0;
+      *t0 = m->twin_start[iper];
+      *k = base + (f - s0);
+      return 1;
+    }
+    base += m->nfaces[ib];
+    ++iper;
+  }
   return 0;
 }
 
+void orc_face_mapping(const orc_mesh *m, orc_geom *g, int *owner) {
+  memcpy(owner, m->owner, sizeof(int) * (size_t)m->n_faces);
+  int iper = 0;
+  for (int ib = 0; ib < m->n_patches; ++ib) {
+    if (m->bctype[ib] != 7) continue;
+    const int nf = m->nfaces[ib], s0 = m->start_face[ib], t0 = m->twin_start[iper++];
+    int *tmp = (int *)malloc(sizeof(int) * (size_t)(nf > 0 ? nf : 1));
+    int *itmp = (int *)malloc(sizeof(int) * (size_t)(nf > 0 ? nf : 1));
+    double *v = (double *)malloc(sizeof(double) * (size_t)(nf > 0 ? nf : 1));
+    for (int i = 0; i < nf; ++i) { /* geometry.f90:1910-1935 */
+      const int iface = s0 + i;
+      double mindist = (double)1e30f; /* mindist = 1e+30, a default-real literal */
+      tmp[i] = owner[t0 + i];
+      itmp[i] = i;
+      for (int it = 0; it < nf; ++it) {
+        const int ijn = owner[t0 + it];
+        const double dist = (fabs(g->xf[iface] - g->xc[ijn]) + fabs(g->yf[iface] - g->yc[ijn])) +
+                            fabs(g->zf[iface] - g->zc[ijn]);
+        if (dist < mindist) {
+          mindist = dist;
+          tmp[i] = ijn;
+          itmp[i] = it;
+        }
+      }
+    }
+    for (int i = 0; i < nf; ++i) owner[t0 + i] = tmp[i]; /* :1939-1941 */
+    double *arrs[6] = {g->xf, g->yf, g->zf, g->arx, g->ary, g->arz}; /* :1943-1985 */
+    for (int a = 0; a < 6; ++a) {
+      for (int i = 0; i < nf; ++i) v[i] = arrs[a][t0 + itmp[i]];
+      for (int i = 0; i < nf; ++i) arrs[a][t0 + i] = v[i];
+    }
+    free(tmp);
+    free(itmp);
+    free(v);
+  }
+}
+
 int orc_csr_nnz(const orc_mesh *m) {
-  if (has_periodic(m)) return -2;
-  return 2 * m->n_inner + m->n_cells; /* sparse_matrix.f90:110 */
+  return 2 * (m->n_inner + orc_n_periodic(m)) + m->n_cells; /* sparse_matrix.f90:110 */
 }
 
 int orc_csr_build(const orc_mesh *m, int *ia, int *ja, int *diag, int *slot_pn,
                   int *slot_np) {
-  const int n = m->n_cells, Fi = m->n_inner;
-  if (has_periodic(m)) return -2;
-  /* The reference sorts the COO list (diagonal + (P,N) + (N,P)) lexically
-   * (utils.f90:443) -- the sorted multiset is unique, so per-row counting
-   * followed by an in-row sort yields the same arrays. */
+  const int n = m->n_cells, Fi = m->n_inner, np = orc_n_periodic(m);
+  /* The reference sorts the COO list (diagonal + (P,N) + (N,P) + the
+   * periodic pairs both ways) lexically (utils.f90:443) -- the sorted
+   * multiset is unique, so per-row counting followed by an in-row sort
+   * yields the same arrays. */
   int *cnt = (int *)calloc((size_t)n + 1, sizeof(int));
-  if (!cnt) return -1;
+  int *pp = (int *)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+  int *pn = (int *)malloc(sizeof(int) * (size_t)(np > 0 ? np : 1));
+  if (!cnt || !pp || !pn) return -1;
+  periodic_pairs(m, pp, pn);
   for (int i = 0; i < n; ++i) cnt[i] = 1;
   for (int f = 0; f < Fi; ++f) {
     cnt[m->owner[f]]++;
     cnt[m->neighbour[f]]++;
+  }
+  for (int k = 0; k < np; ++k) {
+    cnt[pp[k]]++;
+    cnt[pn[k]]++;
   }
   ia[0] = 0;
   for (int i = 0; i < n; ++i) ia[i + 1] = ia[i] + cnt[i];
@@ -167,6 +245,10 @@ int orc_csr_build(const orc_mesh *m, int *ia, int *ja, int *diag, int *slot_pn,
     ja[cnt[p]++] = q;
     ja[cnt[q]++] = p;
   }
+  for (int k = 0; k < np; ++k) {
+    ja[cnt[pp[k]]++] = pn[k];
+    ja[cnt[pn[k]]++] = pp[k];
+  }
   for (int i = 0; i < n; ++i) { /* insertion sort each row */
     for (int k = ia[i] + 1; k < ia[i + 1]; ++k) {
       const int v = ja[k];
@@ -178,6 +260,14 @@ int orc_csr_build(const orc_mesh *m, int *ia, int *ja, int *diag, int *slot_pn,
       ja[l + 1] = v;
     }
   }
+  for (int i = 0; i < n; ++i) /* a repeated entry: not supported */
+    for (int k = ia[i] + 1; k < ia[i + 1]; ++k)
+      if (ja[k] == ja[k - 1]) {
+        free(cnt);
+        free(pp);
+        free(pn);
+        return -2;
+      }
   /* find_main_diag_element_positions utils.f90:149-168: the id-th entry with
    * row==col in sorted order is the diagonal of row id. */
   for (int i = 0; i < n; ++i) {
@@ -198,7 +288,19 @@ int orc_csr_build(const orc_mesh *m, int *ia, int *ja, int *diag, int *slot_pn,
     slot_pn[f] = kpn;
     slot_np[f] = knp;
   }
+  for (int k = 0; k < np; ++k) { /* sparse_matrix.f90:262-293 */
+    const int p = pp[k], q = pn[k];
+    int kpn = -1, knp = -1;
+    for (int j = ia[p]; j < ia[p + 1]; ++j)
+      if (ja[j] == q) { kpn = j; break; }
+    for (int j = ia[q]; j < ia[q + 1]; ++j)
+      if (ja[j] == p) { knp = j; break; }
+    slot_pn[Fi + k] = kpn;
+    slot_np[Fi + k] = knp;
+  }
   free(cnt);
+  free(pp);
+  free(pn);
   return ia[n];
 }
 
@@ -517,12 +619,21 @@ int orc_grad_lsq(const orc_mesh *m, const orc_geom *g, int variant, int compat,
 
 void orc_update_boundary(const orc_mesh *m, double *phi) {
   const int n = m->n_cells, Fi = m->n_inner;
+  int iper = 0;
   for (int ib = 0; ib < m->n_patches; ++ib) {
     const int t = m->bctype[ib];
     if (t == 4 || t == 2 || t == 5 || t == 6) {
       for (int i = 0; i < m->nfaces[ib]; ++i) {
         const int f = m->start_face[ib] + i;
         phi[n + (f - Fi)] = phi[m->owner[f]];
+      }
+    } else if (t == 7) { /* updateBoundary.f90:46-68 */
+      const int t0 = m->twin_start[iper++];
+      for (int i = 0; i < m->nfaces[ib]; ++i) {
+        const int f = m->start_face[ib] + i;
+        const int ijb = n + (f - Fi);
+        phi[ijb] = 0.5 * (phi[m->owner[f]] + phi[m->owner[t0 + i]]);
+        phi[n + (t0 - Fi) + i] = phi[ijb]; /* the twin takes the same value */
       }
     }
   }
@@ -1115,6 +1226,40 @@ int orc_transport_assemble(const orc_mesh *m, const orc_geom *g, const orc_csr *
       su[p] = su[p] + dw * phi[ijb];
     } else if (kind == 1) { /* zero-gradient wall :224 */
       phi[ijb] = phi[p];
+    } else if (kind == 5) { /* periodic face :262-313, facefluxsc_periodic
+                               scalar_fluxes.f90:147-251; the reference
+                               takes Df and facint of inner face i */
+      int i, t0, k;
+      if (!periodic_face(m, f, &i, &t0, &k)) return -1;
+      const int q = m->owner[t0 + i];
+      const double fxn = 0.5, fxp = fxn;
+      const double xpn = 2 * (g->xf[f] - g->xc[p]), ypn = 2 * (g->yf[f] - g->yc[p]),
+                   zpn = 2 * (g->zf[f] - g->zc[p]);
+      const double dc = dcoef[f], fm = flmass[f];
+      const double de = dc * g->Df[i];
+      const double ce = fm < 0.0 ? fm : 0.0;
+      const double cp = fm > 0.0 ? fm : 0.0;
+      const double can = -de + ce;
+      const double cap = -de - cp;
+      double dfixi = grad[3 * p] * fxp + grad[3 * q] * fxn;
+      double dfiyi = grad[3 * p + 1] * fxp + grad[3 * q + 1] * fxn;
+      double dfizi = grad[3 * p + 2] * fxp + grad[3 * q + 2] * fxn;
+      dfixi = dfixi * (g->arx[f] - g->Df[i] * xpn);
+      dfiyi = dfiyi * (g->ary[f] - g->Df[i] * ypn);
+      dfizi = dfizi * (g->arz[f] - g->Df[i] * zpn);
+      const double fdfie = dc * ((dfixi + dfiyi) + dfizi);
+      const double fii = fm >= 0.0 ? face_value(0, p, q, 0, 0, 0, fxp, phi, grad, g)
+                                   : face_value(0, q, p, 0, 0, 0, fxn, phi, grad, g);
+      double fcfie = fm * fii;
+      const double fcfii = ce * phi[q] + cp * phi[p];
+      fcfie = gam * (fcfie - fcfii);
+      const double suadd = -fcfie + fdfie;
+      a[slot_pn[Fi + k]] = can;
+      a[slot_np[Fi + k]] = cap;
+      a[c->diag[p]] = a[c->diag[p]] - can;
+      a[c->diag[q]] = a[c->diag[q]] - cap;
+      su[p] = su[p] + suadd;
+      su[q] = su[q] - suadd;
     } else if (kind == 4) { /* prescribed normal gradient :231-262 */
       const double dw = dcoef[f] * g->srdb[srd];
       const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
@@ -1177,11 +1322,16 @@ void orc_transport_dcoef(const orc_mesh *m, const orc_geom *g, const double *vis
   for (int f = m->n_inner; f < m->n_faces; ++f) {
     const int p = m->owner[f];
     const int kind = bc_kind[f - m->n_inner];
+    int i, t0, k;
     if (kind == 2)
       dcoef[f] = viscos * prl + (vis[p] - viscos) / sigt;
     else if (kind == 0 || kind == 4)
       dcoef[f] = viscos / pranl + (vis[p] - viscos) / sigt;
-    else
+    else if (kind == 5 && periodic_face(m, f, &i, &t0, &k)) {
+      /* temperature.f90:276: the interpolation factor of inner face i */
+      const int q = m->owner[t0 + i];
+      dcoef[f] = viscos * prl + ((vis[p] + (vis[q] - vis[p]) * g->facint[i]) - viscos) * prtr;
+    } else
       dcoef[f] = 0.0;
   }
 }

@@ -30,7 +30,12 @@ typedef struct {
   const int *nfaces;        /* [n_patches]                                */
   const int *start_face;    /* [n_patches] id of the patch's first face    */
   const int *bctype;        /* [n_patches] codes: 1 wall 2 symmetry 3 inlet
-                               4 outlet 5 pressure 6 empty 7 periodic      */
+                               4 outlet 5 pressure 6 empty 7 periodic
+                               8 wallQFlux                                 */
+  const int *twin_start;    /* [periodic patches] start face of each
+                               periodic patch's twin, in patch order
+                               (startFaceTwin, geometry.f90:257); NULL
+                               without periodic patches                    */
 } orc_mesh;
 
 typedef struct {
@@ -45,9 +50,23 @@ typedef struct {
 /* geometry.f90:416-754 (native reader, interpolation variant 2). */
 int orc_geometry(const orc_mesh *m, orc_geom *g);
 
-/* sparse_matrix.f90:86-296: CSR pattern, diagonal and face->slot maps.
- * ia[n+1] (ia[0]=0), ja[nnz], diag[n], slot_pn[F_i], slot_np[F_i].
- * Returns nnz (>0) or <0 on error (periodic patches are not supported). */
+/* face_mapping (geometry.f90:1848-1997), which read_mesh runs after the
+ * geometry when there are periodic patches: for face i of the k-th periodic
+ * patch, the twin face whose owner centre is nearest to the face centre (L1
+ * distance, first minimum) becomes twin face i -- its owner, centre and area
+ * vector move there.  owner[n_faces] receives the mapped owners (m->owner
+ * holds the mesh's own); g's face arrays are permuted in place. */
+void orc_face_mapping(const orc_mesh *m, orc_geom *g, int *owner);
+
+/* Faces on periodic patches (numPeriodic, geometry.f90:253). */
+int orc_n_periodic(const orc_mesh *m);
+
+/* sparse_matrix.f90:86-296: CSR pattern, diagonal and face->slot maps with
+ * the periodic pairs (owner of periodic face i, owner of twin face i; m's
+ * owners after face mapping).  ia[n+1] (ia[0]=0), ja[nnz], diag[n],
+ * slot_pn / slot_np [F_i + numPeriodic] (inner faces, then periodic faces
+ * in patch order).  Returns nnz (>0), or -2 where a periodic pair repeats a
+ * matrix entry (a pair inside one cell or one already coupled). */
 int orc_csr_nnz(const orc_mesh *m);
 int orc_csr_build(const orc_mesh *m, int *ia, int *ja, int *diag,
                   int *slot_pn, int *slot_np);
@@ -88,8 +107,9 @@ int orc_lsq_matrix(const orc_mesh *m, const orc_geom *g, int variant, int compat
 int orc_grad_lsq(const orc_mesh *m, const orc_geom *g, int variant, int compat,
                  const double *dmat, const double *phi, double *dphidxi);
 
-/* updateBoundary.f90:1-74 zero-gradient copy for outlet/symmetry/pressure/empty
- * patches (periodic not supported). */
+/* updateBoundary.f90:1-74: zero-gradient copy for outlet/symmetry/pressure/
+ * empty patches and, for periodic ones, the average of the owner and twin
+ * owner values written to both faces, patch by patch in order. */
 void orc_update_boundary(const orc_mesh *m, double *phi);
 
 /* Linear solvers on CSR (linear_solvers.f90).  fi[n] in/out, rhs[n].
@@ -163,7 +183,8 @@ int orc_rcm(const orc_csr *c, int pp_iters, int *new_of_old);
  * (time_scheme 0 steady, 1 BDF1, 2 BDF2, 3 Crank-Nicolson), facefluxsc over
  * the inner faces, the boundary loop by bc_kind (0 Dirichlet wall, 1
  * zero-gradient wall, 2 inlet/outlet/pressure flux face, 3 no term, 4
- * prescribed normal gradient), the Crank-Nicolson coefficient block, then
+ * prescribed normal gradient, 5 periodic face: facefluxsc_periodic with the
+ * twin face's owner, temperature.f90:262-313), the Crank-Nicolson coefficient block, then
  * the diagonal and under-relaxation.  srd_map (NULL: identity) gives, per
  * boundary face, the boundary face whose |S|/dn the reference reads through
  * its wall counter.  a, su, sp are overwritten.  Returns 0 or -1. */

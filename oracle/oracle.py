@@ -25,7 +25,7 @@ class _Mesh(C.Structure):
                 ("n_faces", C.c_int), ("n_patches", C.c_int),
                 ("points", C.c_void_p), ("nnodes", C.c_void_p), ("node", C.c_void_p),
                 ("owner", C.c_void_p), ("neighbour", C.c_void_p), ("nfaces", C.c_void_p),
-                ("start_face", C.c_void_p), ("bctype", C.c_void_p)]
+                ("start_face", C.c_void_p), ("bctype", C.c_void_p), ("twin_start", C.c_void_p)]
 
 
 class _Geom(C.Structure):
@@ -81,11 +81,14 @@ class Oracle:
             neighbour=np.ascontiguousarray(mesh.neighbour, dtype=np.int32),
             nfaces=np.ascontiguousarray(mesh.nfaces, dtype=np.int32),
             start_face=np.ascontiguousarray(mesh.start_face, dtype=np.int32),
-            bctype=np.ascontiguousarray(mesh.bctype_codes, dtype=np.int32))
+            bctype=np.ascontiguousarray(mesh.bctype_codes, dtype=np.int32),
+            twin_start=np.ascontiguousarray(list(mesh.twin_start) or [0], dtype=np.int32))
         k = self._keep
+        twin = _p(k["twin_start"]) if mesh.twin_start else None
         self.cm = _Mesh(mesh.n_nodes, mesh.n_cells, mesh.n_inner, mesh.n_faces, mesh.n_patches,
                         _p(k["points"]), _p(k["nnodes"]), _p(k["node"]), _p(k["owner"]),
-                        _p(k["neighbour"]), _p(k["nfaces"]), _p(k["start_face"]), _p(k["bctype"]))
+                        _p(k["neighbour"]), _p(k["nfaces"]), _p(k["start_face"]), _p(k["bctype"]),
+                        twin)
         n, F, Fi = mesh.n_cells, mesh.n_faces, mesh.n_inner
         sizes = dict(xc=n, yc=n, zc=n, vol=n, arx=F, ary=F, arz=F, xf=F, yf=F, zf=F,
                      facint=Fi, Df=Fi, dnb=F - Fi, srdb=F - Fi)
@@ -93,13 +96,20 @@ class Oracle:
         self.cg = _Geom(*[_p(self.geom[key]) for key in self.GEOM_KEYS])
         rc = L.orc_geometry(C.byref(self.cm), C.byref(self.cg))
         assert rc == 0, rc
+        # owners after face_mapping (periodic twins), used from here on
+        self.owner = k["owner"].copy()
+        if mesh.twin_start:
+            L.orc_face_mapping(C.byref(self.cm), C.byref(self.cg), _p(self.owner))
+            k["owner_mapped"] = self.owner
+            self.cm.owner = _p(self.owner)
+        self.n_periodic = L.orc_n_periodic(C.byref(self.cm))
         nnz = L.orc_csr_nnz(C.byref(self.cm))
         assert nnz > 0, nnz
         self.ia = np.zeros(n + 1, np.int32)
         self.ja = np.zeros(nnz, np.int32)
         self.diag = np.zeros(n, np.int32)
-        self.slot_pn = np.zeros(Fi, np.int32)
-        self.slot_np = np.zeros(Fi, np.int32)
+        self.slot_pn = np.zeros(Fi + self.n_periodic, np.int32)
+        self.slot_np = np.zeros(Fi + self.n_periodic, np.int32)
         rc = L.orc_csr_build(C.byref(self.cm), _p(self.ia), _p(self.ja), _p(self.diag),
                              _p(self.slot_pn), _p(self.slot_np))
         assert rc == nnz, rc
@@ -293,7 +303,8 @@ def transport_bc(mesh, wall_frac):
     """Boundary treatment of calculate_temperature (temperature.f90:168-320)
     per boundary face: kind 2 inlet/outlet/pressure, 0 isothermal wall
     (bcDFraction(7,ib) = 1), 1 adiabatic wall (0), 4 prescribed heat flux
-    (wallQFlux), 3 no term (symmetry, empty, walls with another fraction);
+    (wallQFlux), 5 periodic face (its twin patch, usually 'empty', gets no
+    term), 3 no term (symmetry, empty, walls with another fraction);
     and, for kinds 0 and 4, the boundary face whose |S|/dn the reference
     reads: srdw(iWall), iWall counting the kind-0/4 faces in patch order
     while srdw lists the faces of the 'wall' patches."""
@@ -318,7 +329,7 @@ def transport_bc(mesh, wall_frac):
         elif t == "wall" and wall_frac[i] == 0.0:
             kinds[s:e] = 1
         elif t == "periodic":
-            raise NotImplementedError("periodic patches")
+            kinds[s:e] = 5
     return kinds, smap
 
 

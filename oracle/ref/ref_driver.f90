@@ -7,6 +7,8 @@
 ! Every file name in the job file is relative to the case directory.
 ! Job lines (list-directed; names without '/'):
 !   dump                                      -> geometry and CSR arrays
+!   dump_periodic                             -> owner after face_mapping and the
+!              slot maps over inner + periodic faces (sparse_matrix.f90:241-296)
 !   laplacian  mu phi su0 a_out su_out        (laplacian.f90)
 !   grad_gauss phi out                        (gradients.f90:1607)
 !   grad_lsq   variant phi out                variant: lsq | lsq_dm | lsq_qr
@@ -63,6 +65,9 @@ program ref_driver
     case ('dump')
       read(ju, *) cmd
       call job_dump()
+    case ('dump_periodic')
+      read(ju, *) cmd
+      call job_dump_periodic()
     case ('laplacian')
       call job_laplacian(ju)
     case ('grad_gauss')
@@ -152,6 +157,16 @@ contains
     call wi4('c_slot_np.bin', jcell_icell_csr_index, numInnerFaces)
     open(newunit=u, file='sizes.txt', status='replace')
     write(u,*) numNodes, numCells, numInnerFaces, numBoundaryFaces, numFaces, nnz, nwal, nsym
+    close(u)
+  end subroutine
+
+  subroutine job_dump_periodic()
+    integer :: u
+    call wi4('p_owner.bin', owner, numFaces)
+    call wi4('p_slot_pn.bin', icell_jcell_csr_index, numInnerFaces+numPeriodic)
+    call wi4('p_slot_np.bin', jcell_icell_csr_index, numInnerFaces+numPeriodic)
+    open(newunit=u, file='psizes.txt', status='replace')
+    write(u,*) numPeriodic, nper
     close(u)
   end subroutine
 

@@ -181,6 +181,93 @@ def transport_case():
     print("wrote transport")
 
 
+PERIODIC_TYPES = ("empty", "periodic", "wall", "inlet", "empty", "periodic")
+PERIODIC_WF = (0, 0, 1, 0, 0, 0)
+# (time scheme, convection scheme, solver, itr_max, tol_rel, gradient, urf, gds)
+PERIODIC_TCASES = [
+    ("bdf2", "linearUpwind", "bicgstab", 10, 1e-6, "gauss", 0.9, 1.0),
+    ("cn", "cds", "bicgstab", 20, 1e-8, "gauss", 0.8, 0.5),
+]
+
+
+def periodic_mesh():
+    """hex_mesh(6) with the faces of each twin patch in a seeded shuffled
+    order, so that face_mapping has a permutation to find."""
+    from fcp.mesh import fisher_yates
+    mesh = hex_mesh(6, jitter=0.2, seed=11, types=PERIODIC_TYPES)
+    for k, t in enumerate(mesh.twin_start):
+        nf = int(mesh.nfaces[list(mesh.start_face).index(t)])
+        perm = fisher_yates(nf, 17 + k)
+        sl = slice(int(t), int(t) + nf)
+        mesh.node[sl] = mesh.node[sl][perm]
+        mesh.nnodes[sl] = mesh.nnodes[sl][perm]
+        mesh.owner[sl] = mesh.owner[sl][perm]
+    mesh.check()
+    return mesh
+
+
+def periodic_case():
+    """Periodic boundaries on a jittered 6^3 hex: the x+ and z+ patches are
+    'periodic' with the x- and z- patches (typed 'empty', as in the
+    reference's examples) as twins.  The reference's read_mesh maps the twin
+    faces (face_mapping, geometry.f90:1848-1997); kept are the geometry and
+    owners after the mapping, the CSR pattern with the periodic pairs and the
+    slot maps over inner + periodic faces (sparse_matrix.f90:141-171,
+    262-293), the Laplacian and Gauss gradient on the mapped owners,
+    updateBoundary's periodic average (updateBoundary.f90:46-68) and
+    calculate_temperature with its periodic branch (temperature.f90:262-313)."""
+    mesh = periodic_mesh()
+    rc = RefCase(mesh)
+    g, c, sizes = rc.dump()
+    rc.run(["dump_periodic"])
+    n, nt, F = mesh.n_cells, mesh.n_total, mesh.n_faces
+    out = {f"g_{k}": v for k, v in g.items()}
+    out.update({f"c_{k}": v for k, v in c.items()})
+    out["sizes"] = np.array(sizes, np.int64)
+    out["p_owner"] = rc.get("p_owner.bin", np.int32) - 1
+    out["p_slot_pn"] = rc.get("p_slot_pn.bin", np.int32) - 1
+    out["p_slot_np"] = rc.get("p_slot_np.bin", np.int32) - 1
+    with open(os.path.join(rc.dir, "psizes.txt")) as f:
+        out["psizes"] = np.array([int(t) for t in f.read().split()], np.int64)
+    mu = -(10.0 ** uniform(13, 1, n, -1.0, 1.0))
+    phi = uniform(13, 2, nt, -1.0, 1.0)
+    su0 = uniform(13, 3, n, -1.0, 1.0)
+    u = uniform(13, 4, nt, -1.0, 1.0)
+    out.update(in_mu=mu, in_phi=phi, in_su0=su0, in_u=u)
+    for k, v in (("mu", mu), ("phi", phi), ("su0", su0), ("u", u)):
+        rc.put(k + ".bin", v)
+    rc.run(["laplacian mu.bin phi.bin su0.bin a.bin su.bin", "grad_gauss u.bin gg.bin",
+            "update_boundary phi.bin ub.bin"])
+    out["lap_a"] = rc.get("a.bin")
+    out["lap_su"] = rc.get("su.bin")
+    out["grad_gauss"] = rc.get("gg.bin").reshape(-1, 3)
+    out["update_boundary"] = rc.get("ub.bin")
+    T = uniform(14, 1, nt, 0.0, 1.0)
+    To = uniform(14, 2, n, 0.0, 1.0)
+    Too = uniform(14, 3, n, 0.0, 1.0)
+    den = 1.0 + 0.1 * uniform(14, 4, n, 0.0, 1.0)
+    vis = 1e-3 * (1.0 + uniform(14, 5, n, 0.0, 1.0))
+    flm = 1e-2 * uniform(14, 6, F, -1.0, 1.0)
+    ins = dict(T=T, To=To, Too=Too, den=den, vis=vis, flm=flm,
+               wf=np.array(PERIODIC_WF, np.float64))
+    for k, v in ins.items():
+        rc.put(k + ".bin", v)
+        out[f"in_{k}"] = v
+    for ic, (ts, sch, slv, itm, trel, gm, urf, gds) in enumerate(PERIODIC_TCASES):
+        rc.run([f"temperature T.bin To.bin Too.bin den.bin vis.bin flm.bin wf.bin 1.0d-3 0.7d0 "
+                f"1.0d0 {fd(urf)} {fd(gds)} 1.0d-2 {ts} {sch} {slv} {itm} 1.0d-13 {fd(trel)} "
+                f"{gm} Tn.bin ta.bin tsu.bin tsp.bin tgr.bin info.txt"])
+        out[f"t{ic}_T"] = rc.get("Tn.bin")
+        out[f"t{ic}_a"] = rc.get("ta.bin")
+        out[f"t{ic}_su"] = rc.get("tsu.bin")
+        out[f"t{ic}_sp"] = rc.get("tsp.bin")
+        out[f"t{ic}_grad"] = rc.get("tgr.bin").reshape(-1, 3)
+        out[f"t{ic}_resor"] = np.array(float(open(os.path.join(rc.dir, "info.txt")).read()))
+    rc.close()
+    np.savez_compressed(os.path.join(HERE, "periodic.npz"), **out)
+    print("wrote periodic")
+
+
 def cavity_case():
     """The reference's own test mesh (test/testFieldOperations/polyMesh).
     The reference's OpenFOAM reader aborts under the flang runtime
@@ -366,6 +453,7 @@ CASES = {
         hex_mesh(5, jitter=0.2, seed=2,
                  types=("wall", "symmetry", "outlet", "wall", "inlet", "pressure")), "hex5j"),
     "hexbc": bc_case,
+    "periodic": periodic_case,
     "transport": transport_case,
     "cavity": cavity_case,
     "kat5x5": kat_5x5_case,

@@ -518,3 +518,82 @@ def test_transport_step_bitexact_vs_calculate_temperature(golden_dir, ic):
         off = np.ones(d["a"].size, bool)
         off[o.diag] = False
         assert np.allclose(d["a"][off], 0.5 * d1["a"][off], rtol=1e-15, atol=0)
+
+
+def _periodic(golden_dir):
+    import sys
+    if golden_dir not in sys.path:
+        sys.path.insert(0, golden_dir)
+    import make_golden as mg
+    return mg, np.load(os.path.join(golden_dir, "periodic.npz"))
+
+
+def test_periodic_mesh_csr_bitexact_vs_reference(golden_dir):
+    """Periodic x and z pairs with shuffled twin faces: the reference's
+    face_mapping (geometry.f90:1848-1997, run by its read_mesh) restores the
+    twin order, and the oracle's mapping gives the same owners and permuted
+    face geometry; the CSR pattern with the periodic pairs and the slot maps
+    over inner + periodic faces (sparse_matrix.f90:141-171,262-293) are the
+    reference's.  Independently of the fixture: every mapped twin cell is the
+    translate of its periodic face's owner across the cube."""
+    mg, g = _periodic(golden_dir)
+    mesh = mg.periodic_mesh()
+    o = Oracle(mesh)
+    assert np.count_nonzero(o.owner != mesh.owner) > 0  # the mapping did something
+    assert np.array_equal(o.owner, g["p_owner"])
+    for k in ("xc", "yc", "zc", "vol", "arx", "ary", "arz", "xf", "yf", "zf", "facint", "Df"):
+        assert np.array_equal(o.geom[k], g[f"g_{k}"]), k
+    for k in ("ia", "ja", "diag"):
+        assert np.array_equal(getattr(o, k), g[f"c_{k}"]), k
+    assert np.array_equal(o.slot_pn, g["p_slot_pn"]) and np.array_equal(o.slot_np, g["p_slot_np"])
+    assert o.n_periodic == int(g["psizes"][0]) == 72
+    N = 6
+    for ib, t0 in zip(mesh.periodic_patches, mesh.twin_start):
+        s0, nf = int(mesh.start_face[ib]), int(mesh.nfaces[ib])
+        p, q = o.owner[s0:s0 + nf], o.owner[t0:t0 + nf]
+        axis = ib // 2
+        ijk = lambda c: np.stack([c % N, (c // N) % N, c // (N * N)], 1)  # noqa: E731
+        d = ijk(p) - ijk(q)
+        assert np.all(d[:, axis] == N - 1) and np.all(np.delete(d, axis, 1) == 0)
+
+
+def test_periodic_operators_bitexact_vs_reference(golden_dir):
+    """On the mapped periodic mesh: the Laplacian and the Gauss gradient (the
+    twin faces now belong to the mapped owners) and updateBoundary's periodic
+    average written to both faces (updateBoundary.f90:46-68)."""
+    mg, g = _periodic(golden_dir)
+    o = Oracle(mg.periodic_mesh())
+    a, su = o.laplacian(g["in_mu"], g["in_phi"], g["in_su0"])
+    assert np.array_equal(a, g["lap_a"]) and np.array_equal(su, g["lap_su"])
+    assert np.array_equal(o.grad_gauss(g["in_u"]), g["grad_gauss"])
+    ub = o.update_boundary(g["in_phi"])
+    assert np.array_equal(ub, g["update_boundary"])
+    m, n, Fi = o.mesh, o.mesh.n_cells, o.mesh.n_inner
+    for ib, t0 in zip(m.periodic_patches, m.twin_start):
+        s0, nf = int(m.start_face[ib]), int(m.nfaces[ib])
+        pv = ub[n + s0 - Fi:n + s0 - Fi + nf]
+        assert np.array_equal(pv, ub[n + t0 - Fi:n + t0 - Fi + nf])
+        assert np.array_equal(pv, 0.5 * (g["in_phi"][o.owner[s0:s0 + nf]] +
+                                          g["in_phi"][o.owner[t0:t0 + nf]]))
+
+
+@pytest.mark.parametrize("ic", [0, 1])
+def test_periodic_transport_bitexact_vs_calculate_temperature(golden_dir, ic):
+    """calculate_temperature with its periodic branch (facefluxsc_periodic,
+    temperature.f90:262-313, scalar_fluxes.f90:147-251, which reads Df and
+    facint of inner face i for periodic face i) under BDF2 and
+    Crank-Nicolson: the assembled system, the gradient, the solved and
+    updated field and resor equal the reference's."""
+    mg, g = _periodic(golden_dir)
+    o = Oracle(mg.periodic_mesh())
+    ts, sch, slv, itm, trel, gm, urf, gds = mg.PERIODIC_TCASES[ic]
+    I = {k: g[f"in_{k}"] for k in ("T", "To", "Too", "den", "vis", "flm", "wf")}
+    T, d = o.temperature(I["T"], I["To"], I["Too"], I["den"], I["vis"], I["flm"], I["wf"], 1e-3,
+                         0.7, 1.0, urf, gds, 1e-2, ts, sch, slv, itm, 1e-13, trel, gm)
+    for k in ("a", "su", "sp", "grad"):
+        assert np.array_equal(d[k], g[f"t{ic}_{k}"]), k
+    assert np.array_equal(T, g[f"t{ic}_T"])
+    assert d["resor"] == float(g[f"t{ic}_resor"])
+    # the periodic pairs' entries are set (not left at zero)
+    Fi = o.mesh.n_inner
+    assert np.all(d["a"][o.slot_pn[Fi:]] != 0.0) and np.all(d["a"][o.slot_np[Fi:]] != 0.0)
