@@ -403,6 +403,8 @@ int fcp_mesh_upload(fcp_ctx *ctx, int nNodes, int nCells, int nInnerFaces, int n
   ctx->h_bctype.assign(bctype, bctype + nBoundaries);
   ctx->tr_kind.release();  // the transport step's boundary table belongs to the old mesh
   ctx->tr_wf.clear();
+  ctx->h_twin.clear();  // periodic twins come with fcp_mesh_periodic
+  ctx->nper = 0;
   long long tot = 0;
   for (int ib = 0; ib < nBoundaries; ++ib) {
     if (startFace[ib] < nInnerFaces || startFace[ib] + nfaces[ib] > F || nfaces[ib] < 0)
@@ -474,6 +476,43 @@ int fcp_mesh_upload(fcp_ctx *ctx, int nNodes, int nCells, int nInnerFaces, int n
   return 0;
 }
 
+int fcp_mesh_periodic(fcp_ctx *ctx, int nPeriodic, const int *startFaceTwin) {
+  FCP_TRY(need(ctx, false, false, false));
+  if (!ctx->have_mesh) return fcp_fail(ctx, FCP_ERR_STATE, "fcp_mesh_periodic: no mesh");
+  std::vector<int> per;
+  for (int ib = 0; ib < ctx->nPatches; ++ib)
+    if (ctx->h_bctype[ib] == FCP_BC_PERIODIC) per.push_back(ib);
+  if (nPeriodic != (int)per.size() || (nPeriodic > 0 && !startFaceTwin))
+    return fcp_fail(ctx, FCP_ERR_ARG,
+                    "fcp_mesh_periodic: %d twins given for %d periodic patches", nPeriodic,
+                    (int)per.size());
+  std::vector<int> used(ctx->nPatches, 0);
+  for (int k = 0; k < nPeriodic; ++k) {
+    const int ib = per[k];
+    int tb = -1;
+    for (int jb = 0; jb < ctx->nPatches; ++jb)
+      if (ctx->h_start[jb] == startFaceTwin[k]) tb = jb;
+    // the twin is a whole patch of the same size, not periodic itself and
+    // the twin of no other patch (then the reference's per-patch mappings
+    // touch disjoint faces)
+    if (tb < 0 || tb == ib || ctx->h_nfaces[tb] != ctx->h_nfaces[ib] ||
+        ctx->h_bctype[tb] == FCP_BC_PERIODIC || used[tb]++)
+      return fcp_fail(ctx, FCP_ERR_ARG, "fcp_mesh_periodic: bad twin start face %d for patch %d",
+                      startFaceTwin[k], ib);
+  }
+  (void)hipSetDevice(ctx->device);
+  FCP_TRY(fcp_coarse_wait(ctx));
+  ctx->h_twin.assign(startFaceTwin, startFaceTwin + nPeriodic);
+  FCP_HIP(ctx, ctx->owner0.alloc(ctx->F));
+  FCP_HIP(ctx, hipMemcpyAsync(ctx->owner0.p, ctx->owner.p, sizeof(int) * ctx->F,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->have_geom = ctx->have_csr = false;  // face mapping runs with the geometry
+  ctx->tr_kind.release();
+  ctx->tr_wf.clear();
+  return 0;
+}
+
 int fcp_geometry(fcp_ctx *ctx, double *xc, double *yc, double *zc, double *vol, double *arx,
                  double *ary, double *arz, double *xf, double *yf, double *zf, double *facint,
                  double *Df, double *dnb, double *srdb) {
@@ -507,8 +546,8 @@ int fcp_csr_build(fcp_ctx *ctx, int *nnz, int *ia, int *ja, int *diag, int *slot
   FCP_TRY(out_index_plus1(ctx, ctx->ia.p, ia, (size_t)ctx->n + 1));
   FCP_TRY(out_index_plus1(ctx, ctx->ja.p, ja, ctx->nnz));
   FCP_TRY(out_index_plus1(ctx, ctx->diag.p, diag, ctx->n));
-  FCP_TRY(out_index_plus1(ctx, ctx->slot_pn.p, slot_pn, ctx->Fi));
-  FCP_TRY(out_index_plus1(ctx, ctx->slot_np.p, slot_np, ctx->Fi));
+  FCP_TRY(out_index_plus1(ctx, ctx->slot_pn.p, slot_pn, (size_t)ctx->Fi + ctx->nper));
+  FCP_TRY(out_index_plus1(ctx, ctx->slot_np.p, slot_np, (size_t)ctx->Fi + ctx->nper));
   return 0;
 }
 
@@ -968,10 +1007,9 @@ int fcp_transport(fcp_ctx *ctx, double *phi, const double *phi_o, const double *
     for (int ib = 0; ib < np; ++ib) {
       const int t = ctx->h_bctype[ib], s0 = ctx->h_start[ib] - ctx->Fi, nf = ctx->h_nfaces[ib];
       for (int i = 0; i < nf; ++i) srd[s0 + i] = s0 + i;
-      if (t == FCP_BC_PERIODIC)
-        return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "fcp_transport: periodic patches");
       unsigned char k = 3;
-      if (t == FCP_BC_INLET || t == FCP_BC_OUTLET || t == FCP_BC_PRESSURE) k = 2;
+      if (t == FCP_BC_PERIODIC) k = 5;
+      else if (t == FCP_BC_INLET || t == FCP_BC_OUTLET || t == FCP_BC_PRESSURE) k = 2;
       else if ((t == FCP_BC_WALL && wf[ib] == 1.0) || t == FCP_BC_WALL_QFLUX) k = t == FCP_BC_WALL ? 0 : 4;
       else if (t == FCP_BC_WALL && wf[ib] == 0.0) k = 1;
       if (k == 0 || k == 4) {

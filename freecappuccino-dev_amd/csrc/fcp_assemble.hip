@@ -132,15 +132,37 @@ __global__ void k_grad_gauss(int n, int Fi, int ntot, const int *__restrict__ cf
   dudxi[3 * (size_t)c + 2] = gz * volr;
 }
 
-// updateBoundary.f90:31-44 zero-gradient copy.
+// updateBoundary.f90: zero-gradient copy (:31-44); a periodic face and its
+// twin take half the sum of their owners' values (:46-68) -- the twin unless
+// its own patch, coming later in the patch loop, copies its owner's value
+// (per.twb).  Only cell values are read, so every face's final value is
+// computed independently.
 __global__ void k_update_boundary(int n, int Fi, int Fb, const int *__restrict__ owner,
                                   const int *__restrict__ bpatch, const int *__restrict__ bctype,
-                                  double *__restrict__ phi) {
+                                  PerP per, double *__restrict__ phi) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= Fb) return;
+  if (per.np > 0) {
+    const int k = per.kof[b] >= 0 ? per.kof[b] : per.twb[b];
+    if (k >= 0) {
+      phi[n + b] = 0.5 * (phi[owner[per.fp[k]]] + phi[owner[per.ft[k]]]);
+      return;
+    }
+  }
   const int t = bctype[bpatch[b]];
   if (t == FCP_BC_OUTLET || t == FCP_BC_SYMMETRY || t == FCP_BC_PRESSURE || t == FCP_BC_EMPTY)
     phi[n + b] = phi[owner[Fi + b]];
+}
+
+// The periodic pairs' entries of a matrix assembled without them (laplacian
+// clears the matrix, laplacian.f90:35, and sets inner and diagonal entries).
+__global__ void k_zero_periodic(int Fi, int np, const int *__restrict__ slot_pn,
+                                const int *__restrict__ slot_np, double *__restrict__ a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const int s1 = slot_pn[Fi + k], s2 = slot_np[Fi + k];
+  if (s1 >= 0) a[s1] = 0.0;
+  if (s2 >= 0) a[s2] = 0.0;
 }
 
 // Convection-diffusion assembly as calculate_temperature composes it
@@ -152,6 +174,41 @@ __global__ void k_update_boundary(int n, int Fi, int Fb, const int *__restrict__
 struct FaceFlux {
   double can, cap, suadd;
 };
+
+// facefluxsc_periodic (scalar_fluxes.f90:147-251) for periodic face fp with
+// owner p and twin owner q: half weights, the distance vector twice the
+// owner-to-face vector, CDS for the deferred correction, and -- as the
+// reference calls it with the face's index i in its patch -- Df of inner
+// face i.
+__device__ __forceinline__ FaceFlux periodic_flux(int fp, int i, int p, int q, const GeomP &g,
+                                                  double fm, double dc,
+                                                  const double *__restrict__ phi,
+                                                  const double *__restrict__ gr, double gam) {
+  const double fxn = 0.5, fxp = fxn;
+  const double xpn = 2 * (g.xf[fp] - g.xc[p]), ypn = 2 * (g.yf[fp] - g.yc[p]),
+               zpn = 2 * (g.zf[fp] - g.zc[p]);
+  const double Dfi = g.Df[i];
+  const double de = dc * Dfi;
+  const double ce = fm < 0.0 ? fm : 0.0;
+  const double cp = fm > 0.0 ? fm : 0.0;
+  FaceFlux r;
+  r.can = -de + ce;
+  r.cap = -de - cp;
+  double dfixi = gr[3 * p] * fxp + gr[3 * q] * fxn;
+  double dfiyi = gr[3 * p + 1] * fxp + gr[3 * q + 1] * fxn;
+  double dfizi = gr[3 * p + 2] * fxp + gr[3 * q + 2] * fxn;
+  dfixi = dfixi * (g.arx[fp] - Dfi * xpn);
+  dfiyi = dfiyi * (g.ary[fp] - Dfi * ypn);
+  dfizi = dfizi * (g.arz[fp] - Dfi * zpn);
+  const double fdfie = dc * ((dfixi + dfiyi) + dfizi);
+  const int u = fm >= 0.0 ? p : q, d = fm >= 0.0 ? q : p;
+  const double fii = phi[u] + (phi[d] - phi[u]) * fxp;
+  double fcfie = fm * fii;
+  const double fcfii = ce * phi[q] + cp * phi[p];
+  fcfie = gam * (fcfie - fcfii);
+  r.suadd = -fcfie + fdfie;
+  return r;
+}
 
 __device__ __forceinline__ FaceFlux inner_flux(int f, int p, int q, const GeomP &g, double fm,
                                                double dc, const double *__restrict__ phi,
@@ -193,8 +250,11 @@ __device__ __forceinline__ FaceFlux inner_flux(int f, int p, int q, const GeomP 
 
 // Boundary kinds: 0 Dirichlet wall, 1 zero-gradient wall, 2 inlet/outlet/
 // pressure flux face, 3 no term, 4 prescribed normal gradient (wallQFlux,
-// :231-262).  srd_map (may be null) names, per boundary face, the face whose
-// |S|/dn a wall term reads.  With cn the Crank-Nicolson block (:325-347)
+// :231-262), 5 periodic face (:262-313): the pair's coefficients enter the
+// rows of both owners at the periodic face's place in the boundary loop
+// (the twin owner's row merges its pairs, per.tw_*, into its face order).
+// srd_map (may be null) names, per boundary face, the face whose |S|/dn a
+// wall term reads.  With cn the Crank-Nicolson block (:325-347)
 // runs between the boundary terms and the diagonal: the row is halved, the
 // old neighbour values enter su through the halved coefficients, and
 // (den V/dt + halved off-diagonal sum) phi_o and den V/dt join su and sp.
@@ -209,12 +269,28 @@ __global__ void k_convdiff(int n, int Fi, const int *__restrict__ cf_ptr, const 
                            double *__restrict__ a, double *__restrict__ su,
                            double *__restrict__ sp, const int *__restrict__ srd_map, int cn,
                            const double *__restrict__ den, double dt,
-                           const double *__restrict__ phi_o) {
+                           const double *__restrict__ phi_o, PerP per) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   double ad = 0.0, s = su[c], spc = sp[c];
+  // periodic pairs whose twin face c owns, in the order of their faces
+  int tw = per.np > 0 ? per.tw_ptr[c] : 0;
+  const int twe = per.np > 0 ? per.tw_ptr[c + 1] : 0;
+  auto twin_side = [&](int before) {
+    for (; tw < twe && per.fp[per.tw_k[tw]] < before; ++tw) {
+      const int kp = per.tw_k[tw], fp = per.fp[kp];
+      if (bc_kind[fp - Fi] != 5) continue;
+      const FaceFlux ff = periodic_flux(fp, per.ii[kp], owner[fp], c, g, flmass[fp], dcoef[fp],
+                                        phi, gr, gam);
+      const int sl = slot_np[Fi + kp];
+      if (sl >= 0) a[sl] = ff.cap;
+      ad = ad - ff.cap;
+      s = s - ff.suadd;
+    }
+  };
   for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
     const int f = cf[k];
+    if (f >= Fi) twin_side(f);
     if (f < Fi) {
       const int p = owner[f], q = neighbour[f];
       const FaceFlux ff = inner_flux(f, p, q, g, flmass[f], dcoef[f], phi, gr, gam, scheme);
@@ -260,9 +336,18 @@ __global__ void k_convdiff(int n, int Fi, const int *__restrict__ cf_ptr, const 
         const size_t jb = 3 * ((size_t)n + b);
         const double dtn = (gr[jb] * nxf + gr[jb + 1] * nyf) + gr[jb + 2] * nzf;
         s = s + dw * dtn;
+      } else if (kind == 5 && per.np > 0 && per.kof[b] >= 0) {  // periodic, owner side
+        const int kp = per.kof[b];
+        const FaceFlux ff = periodic_flux(f, per.ii[kp], c, owner[per.ft[kp]], g, flmass[f],
+                                          dcoef[f], phi, gr, gam);
+        const int sl = slot_pn[Fi + kp];
+        if (sl >= 0) a[sl] = ff.can;
+        ad = ad - ff.can;
+        s = s + ff.suadd;
       }
     }
   }
+  twin_side(0x7fffffff);
   const int dg = diag[c];
   a[dg] = ad;
   if (cn) {
@@ -322,7 +407,7 @@ __global__ void k_transport_dcoef(int Fi, int F, const int *__restrict__ owner,
                                   const int *__restrict__ neighbour,
                                   const double *__restrict__ facint, const double *__restrict__ vis,
                                   const unsigned char *__restrict__ bc_kind, double viscos,
-                                  double prl, double prtr, double pranl, double sigt,
+                                  double prl, double prtr, double pranl, double sigt, PerP per,
                                   double *__restrict__ dcoef) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= F) return;
@@ -337,7 +422,11 @@ __global__ void k_transport_dcoef(int Fi, int F, const int *__restrict__ owner,
       d = viscos * prl + (vis[p] - viscos) / sigt;
     else if (kind == 0 || kind == 4)
       d = viscos / pranl + (vis[p] - viscos) / sigt;
-    else
+    else if (kind == 5 && per.np > 0 && per.kof[f - Fi] >= 0) {
+      // :276, with the interpolation factor of inner face i
+      const int k = per.kof[f - Fi], q = owner[per.ft[k]];
+      d = viscos * prl + ((vis[p] + (vis[q] - vis[p]) * facint[per.ii[k]]) - viscos) * prtr;
+    } else
       d = 0.0;
   }
   dcoef[f] = d;
@@ -404,6 +493,9 @@ int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
   k_laplacian<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, geom_of(ctx),
       mu, mu_f, bc_kind, phi, ctx->slot_pn.p, ctx->slot_np.p, ctx->diag.p, ctx->a.p, su);
+  if (ctx->nper > 0)
+    k_zero_periodic<<<grid_for(ctx->nper, BLK), BLK, 0, ctx->stream>>>(
+        ctx->Fi, ctx->nper, ctx->slot_pn.p, ctx->slot_np.p, ctx->a.p);
   FCP_HIP(ctx, hipGetLastError());
   ctx->a_version++;
   return fcp_coarse_begin(ctx);
@@ -432,7 +524,8 @@ int fcp_launch_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi) {
 int fcp_launch_update_boundary(fcp_ctx *ctx, double *phi) {
   if (ctx->Fb == 0) return 0;
   k_update_boundary<<<grid_for(ctx->Fb, BLK), BLK, 0, ctx->stream>>>(
-      ctx->n, ctx->Fi, ctx->Fb, ctx->owner.p, ctx->bpatch.p, ctx->bctype.p, phi);
+      ctx->n, ctx->Fi, ctx->Fb, ctx->owner.p, ctx->bpatch.p, ctx->bctype.p,
+      fcp_periodic_view(ctx), phi);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }
@@ -446,7 +539,8 @@ int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef,
   k_convdiff<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, geom_of(ctx),
       ctx->srdb.p, flmass, dcoef, phi, grad, gam, scheme, bc_kind, urf, ctx->slot_pn.p,
-      ctx->slot_np.p, ctx->ia.p, ctx->diag.p, ctx->a.p, su, sp, srd_map, cn, den, dt, phi_o);
+      ctx->slot_np.p, ctx->ia.p, ctx->diag.p, ctx->a.p, su, sp, srd_map, cn, den, dt, phi_o,
+      fcp_periodic_view(ctx));
   FCP_HIP(ctx, hipGetLastError());
   ctx->a_version++;
   return 0;
@@ -465,7 +559,7 @@ int fcp_launch_transport_dcoef(fcp_ctx *ctx, const double *vis, const unsigned c
   const double prl = 1.0 / pranl, prtr = 1.0 / sigt;  // temperature.f90:66-68
   k_transport_dcoef<<<grid_for(ctx->F, BLK), BLK, 0, ctx->stream>>>(
       ctx->Fi, ctx->F, ctx->owner.p, ctx->neighbour.p, ctx->facint.p, vis, bc_kind, viscos, prl,
-      prtr, pranl, sigt, dcoef);
+      prtr, pranl, sigt, fcp_periodic_view(ctx), dcoef);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }

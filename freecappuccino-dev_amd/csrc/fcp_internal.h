@@ -171,6 +171,16 @@ struct fcp_ctx {
   DBuf<int> bctype;     // [nPatches]
   DBuf<int> cf_ptr;     // [n+1] cell -> faces (ascending face id)
   DBuf<int> cf_face;    // [2*Fi+Fb]
+  // periodic patches (geometry.f90:251-258, 1848-1997): the twin start face
+  // of every periodic patch, the owners as uploaded (face mapping rewrites
+  // owner and the twin faces' geometry), and per periodic pair k (patch
+  // order): its face, its twin face, the face's index in its patch; per
+  // boundary face the pair it belongs to (kof) or whose average it takes
+  // last (twb); per cell the pairs whose twin face it owns (tw_ptr / tw_k)
+  std::vector<int> h_twin;
+  DBuf<int> owner0;
+  int nper = 0;
+  DBuf<int> per_kof, per_twb, per_fp, per_ft, per_ii, per_twptr, per_twk;
 
   // -------- geometry --------
   bool have_geom = false;
@@ -359,6 +369,12 @@ const char *noalloc_message();  // fcp_noalloc.hip
 // fcp_mesh.hip
 int fcp_build_cell_faces(fcp_ctx *ctx);
 int fcp_compute_geometry(fcp_ctx *ctx);
+// Device view of the periodic pairs (np = 0 and null arrays without them).
+struct PerP {
+  int np;
+  const int *kof, *twb, *fp, *ft, *ii, *tw_ptr, *tw_k;
+};
+PerP fcp_periodic_view(const fcp_ctx *ctx);
 int fcp_build_csr(fcp_ctx *ctx);
 int fcp_build_schedules(fcp_ctx *ctx);
 int fcp_build_block_schedules(fcp_ctx *ctx);

@@ -76,9 +76,10 @@ int bctype_code(const std::string &t) {
 
 // The table "name type nFaces startFace [startFaceTwin]" after a header
 // line; lines starting with '#' are comments (file_row_count,
-// utils.f90:281-360).
+// utils.f90:281-360).  A 'periodic' line carries its twin's start face
+// (geometry.f90:251-258), collected in patch order.
 bool read_boundary(const std::string &path, std::vector<int> &nf, std::vector<int> &sf,
-                   std::vector<int> &type) {
+                   std::vector<int> &type, std::vector<int> &twin) {
   std::string txt;
   if (!slurp(path, txt)) return false;
   size_t pos = 0;
@@ -90,11 +91,16 @@ bool read_boundary(const std::string &path, std::vector<int> &nf, std::vector<in
     size_t a = line.find_first_not_of(" \t\r");
     if (a == std::string::npos || line[a] == '#') continue;
     char name[256], tname[256];
-    long n = 0, s = 0;
-    if (sscanf(line.c_str() + a, "%255s %255s %ld %ld", name, tname, &n, &s) != 4) return false;
+    long n = 0, s = 0, t = 0;
+    const int got = sscanf(line.c_str() + a, "%255s %255s %ld %ld %ld", name, tname, &n, &s, &t);
+    if (got < 4) return false;
     nf.push_back((int)n);
     sf.push_back((int)s);
     type.push_back(bctype_code(tname));
+    if (type.back() == FCP_BC_PERIODIC) {
+      if (got < 5) return false;
+      twin.push_back((int)t);
+    }
   }
   return true;
 }
@@ -147,9 +153,9 @@ int fcp_read_polymesh(fcp_ctx *ctx, const char *case_dir, int len, int format, i
   while (!dir.empty() && dir.back() == ' ') dir.pop_back();
   const std::string d = (dir.empty() ? std::string(".") : dir) + "/polyMesh/";
   std::vector<double> xyz;
-  std::vector<int> nnodes, node, owner, neighbour, nf, sf, type;
+  std::vector<int> nnodes, node, owner, neighbour, nf, sf, type, twin;
   long nNodes = 0, nCells = 0, nInner = 0, nBnd = 0, nFaces = 0;
-  if (!read_boundary(d + "boundary", nf, sf, type))
+  if (!read_boundary(d + "boundary", nf, sf, type, twin))
     return fcp_fail(ctx, FCP_ERR_ARG, "fcp_read_polymesh: cannot read %sboundary", d.c_str());
   auto bad = [&](const char *what) {
     return fcp_fail(ctx, FCP_ERR_ARG, "fcp_read_polymesh: malformed %s%s", d.c_str(), what);
@@ -269,6 +275,7 @@ int fcp_read_polymesh(fcp_ctx *ctx, const char *case_dir, int len, int format, i
                                  nInner > 0 ? neighbour.data() : nullptr, nf.data(), sf.data(),
                                  type.data());
   if (rc < 0) return rc;
+  if (!twin.empty()) FCP_TRY(fcp_mesh_periodic(ctx, (int)twin.size(), twin.data()));
   if (sizes) {
     sizes[0] = (int)nNodes;
     sizes[1] = (int)nCells;

@@ -173,9 +173,73 @@ __global__ void k_face_geom2(const int *__restrict__ owner, const int *__restric
   }
 }
 
+// ---- periodic face mapping (geometry.f90:1848-1997) -------------------------
+// Face i of a periodic patch (faces s0..s0+nf-1) takes the twin face (t0..)
+// whose owner centre is nearest to its centre: L1 distance summed x, y, z,
+// first minimum below the reference's start value 1e+30 (a default-real
+// literal).  The search reads the twin owners before any is moved.
+__global__ void k_face_map_search(int nf, int s0, int t0, const int *__restrict__ owner,
+                                  const double *__restrict__ xf, const double *__restrict__ yf,
+                                  const double *__restrict__ zf, const double *__restrict__ xc,
+                                  const double *__restrict__ yc, const double *__restrict__ zc,
+                                  int *__restrict__ tmp, int *__restrict__ itmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const double fx = xf[s0 + i], fy = yf[s0 + i], fz = zf[s0 + i];
+  double mindist = (double)1e30f;
+  int bt = owner[t0 + i], bi = i;
+  for (int it = 0; it < nf; ++it) {
+    const int ijn = owner[t0 + it];
+    const double dist = (fabs(fx - xc[ijn]) + fabs(fy - yc[ijn])) + fabs(fz - zc[ijn]);
+    if (dist < mindist) {
+      mindist = dist;
+      bt = ijn;
+      bi = it;
+    }
+  }
+  tmp[i] = bt;
+  itmp[i] = bi;
+}
+
+// The twin faces' centres and area vectors, copied aside (6 x nf)...
+__global__ void k_face_map_save(int nf, int t0, const double *__restrict__ xf,
+                                const double *__restrict__ yf, const double *__restrict__ zf,
+                                const double *__restrict__ arx, const double *__restrict__ ary,
+                                const double *__restrict__ arz, double *__restrict__ save) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  save[i] = xf[t0 + i];
+  save[nf + i] = yf[t0 + i];
+  save[2 * nf + i] = zf[t0 + i];
+  save[3 * nf + i] = arx[t0 + i];
+  save[4 * nf + i] = ary[t0 + i];
+  save[5 * nf + i] = arz[t0 + i];
+}
+// ...then twin face i takes the owner and the geometry of twin face itmp[i].
+__global__ void k_face_map_apply(int nf, int t0, const int *__restrict__ tmp,
+                                 const int *__restrict__ itmp, const double *__restrict__ save,
+                                 int *__restrict__ owner, double *__restrict__ xf,
+                                 double *__restrict__ yf, double *__restrict__ zf,
+                                 double *__restrict__ arx, double *__restrict__ ary,
+                                 double *__restrict__ arz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const int j = itmp[i];
+  owner[t0 + i] = tmp[i];
+  xf[t0 + i] = save[j];
+  yf[t0 + i] = save[nf + j];
+  zf[t0 + i] = save[2 * nf + j];
+  arx[t0 + i] = save[3 * nf + j];
+  ary[t0 + i] = save[4 * nf + j];
+  arz[t0 + i] = save[5 * nf + j];
+}
+
 // ---- CSR ------------------------------------------------------------------
+// Entries of row c: the diagonal, the other cell of each inner face and the
+// partner of each periodic pair it belongs to (tw: pairs whose twin face it
+// owns; its periodic faces are found through kof).
 __global__ void k_row_count(const int *__restrict__ cf_ptr, const int *__restrict__ cf, int n,
-                            int Fi, int *__restrict__ cnt) {
+                            int Fi, PerP per, int *__restrict__ cnt) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c > n) return;
   if (c == n) {
@@ -183,16 +247,22 @@ __global__ void k_row_count(const int *__restrict__ cf_ptr, const int *__restric
     return;
   }
   int m = 1;
-  for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) m += (cf[k] < Fi);
+  for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+    const int f = cf[k];
+    m += (f < Fi) || (per.np > 0 && per.kof[f - Fi] >= 0);
+  }
+  if (per.np > 0) m += per.tw_ptr[c + 1] - per.tw_ptr[c];
   cnt[c] = m;
 }
 
-// Row c holds {c} and the other cell of each inner face, sorted ascending
-// (the unique result of the reference's lexical COO sort, utils.f90:443).
+// Row c holds {c} and the other cell of each inner face and periodic pair,
+// sorted ascending (the unique result of the reference's lexical COO sort,
+// utils.f90:443); a repeated column (a periodic pair doubling an entry)
+// raises err.
 __global__ void k_row_fill(const int *__restrict__ cf_ptr, const int *__restrict__ cf,
                            const int *__restrict__ owner, const int *__restrict__ neighbour,
-                           int n, int Fi, const int *__restrict__ ia, int *__restrict__ ja,
-                           int *__restrict__ diag) {
+                           int n, int Fi, PerP per, const int *__restrict__ ia,
+                           int *__restrict__ ja, int *__restrict__ diag, int *__restrict__ err) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const int b = ia[c];
@@ -200,9 +270,15 @@ __global__ void k_row_fill(const int *__restrict__ cf_ptr, const int *__restrict
   ja[e++] = c;
   for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
     const int f = cf[k];
-    if (f >= Fi) continue;
+    if (f >= Fi) {
+      const int kp = per.np > 0 ? per.kof[f - Fi] : -1;
+      if (kp >= 0) ja[e++] = owner[per.ft[kp]];
+      continue;
+    }
     ja[e++] = (owner[f] == c) ? neighbour[f] : owner[f];
   }
+  if (per.np > 0)
+    for (int k = per.tw_ptr[c]; k < per.tw_ptr[c + 1]; ++k) ja[e++] = owner[per.fp[per.tw_k[k]]];
   for (int k = b + 1; k < e; ++k) {
     int v = ja[k];
     int l = k - 1;
@@ -212,6 +288,8 @@ __global__ void k_row_fill(const int *__restrict__ cf_ptr, const int *__restrict
     }
     ja[l + 1] = v;
   }
+  for (int k = b + 1; k < e; ++k)
+    if (ja[k] == ja[k - 1]) atomicOr(err, 1);
   int dpos = -1;
   for (int k = b; k < e; ++k)
     if (ja[k] == c) {
@@ -244,6 +322,20 @@ __global__ void k_slots(const int *__restrict__ owner, const int *__restrict__ n
   slot_pn[f] = (k < ia[p + 1] && ja[k] == q) ? k : -1;
   k = first_in_row(ja, ia[q], ia[q + 1], p);
   slot_np[f] = (k < ia[q + 1] && ja[k] == p) ? k : -1;
+}
+
+// The same for periodic pair k: (owner of its face, owner of its twin face)
+// (sparse_matrix.f90:262-293), stored after the inner faces' slots.
+__global__ void k_slots_periodic(PerP per, const int *__restrict__ owner, int Fi,
+                                 const int *__restrict__ ia, const int *__restrict__ ja,
+                                 int *__restrict__ slot_pn, int *__restrict__ slot_np) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= per.np) return;
+  const int p = owner[per.fp[k]], q = owner[per.ft[k]];
+  int j = first_in_row(ja, ia[p], ia[p + 1], q);
+  slot_pn[Fi + k] = (j < ia[p + 1] && ja[j] == q) ? j : -1;
+  j = first_in_row(ja, ia[q], ia[q + 1], p);
+  slot_np[Fi + k] = (j < ia[q + 1] && ja[j] == p) ? j : -1;
 }
 
 __global__ void k_row_of_tslot(const int *__restrict__ ia, const int *__restrict__ ja, int n,
@@ -316,8 +408,102 @@ int fcp_build_cell_faces(fcp_ctx *ctx) {
   return 0;
 }
 
+PerP fcp_periodic_view(const fcp_ctx *ctx) {
+  if (ctx->nper == 0) return PerP{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  return PerP{ctx->nper,        ctx->per_kof.p, ctx->per_twb.p,   ctx->per_fp.p,
+              ctx->per_ft.p,    ctx->per_ii.p,  ctx->per_twptr.p, ctx->per_twk.p};
+}
+
+// The periodic pairs after face mapping (host tables uploaded once per mesh).
+static int build_periodic(fcp_ctx *ctx) {
+  const int n = ctx->n, F = ctx->F, Fi = ctx->Fi, Fb = ctx->Fb;
+  std::vector<int> own(F);
+  FCP_HIP(ctx, hipMemcpy(own.data(), ctx->owner.p, sizeof(int) * F, hipMemcpyDeviceToHost));
+  std::vector<int> kof(Fb > 0 ? Fb : 1, -1), twb(Fb > 0 ? Fb : 1, -1), fp, ft, ii, cnt(n + 1, 0);
+  std::vector<int> patch_of(Fb > 0 ? Fb : 1, -1);
+  for (int ib = 0; ib < ctx->nPatches; ++ib)
+    for (int i = 0; i < ctx->h_nfaces[ib]; ++i) patch_of[ctx->h_start[ib] - Fi + i] = ib;
+  int iper = 0;
+  for (int ib = 0; ib < ctx->nPatches; ++ib) {
+    if (ctx->h_bctype[ib] != FCP_BC_PERIODIC) continue;
+    const int s0 = ctx->h_start[ib], t0 = ctx->h_twin[iper++], nf = ctx->h_nfaces[ib];
+    if (nf > Fi)  // the reference reads facint / Df of inner face i for periodic face i
+      return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "periodic patch with more faces than inner faces");
+    const int tb = patch_of[t0 - Fi], tt = ctx->h_bctype[tb];
+    // the twin's own zero-gradient copy (updateBoundary.f90:31-44) wins when
+    // its patch comes after this one
+    const bool own_last = tb > ib && (tt == FCP_BC_OUTLET || tt == FCP_BC_SYMMETRY ||
+                                      tt == FCP_BC_PRESSURE || tt == FCP_BC_EMPTY);
+    for (int i = 0; i < nf; ++i) {
+      const int k = (int)fp.size();
+      if (own[s0 + i] == own[t0 + i])
+        return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "periodic pair inside one cell");
+      kof[s0 - Fi + i] = k;
+      if (!own_last) twb[t0 - Fi + i] = k;
+      fp.push_back(s0 + i);
+      ft.push_back(t0 + i);
+      ii.push_back(i);
+      cnt[own[t0 + i] + 1]++;
+    }
+  }
+  const int np = (int)fp.size();
+  std::vector<int> twptr(n + 1, 0), twk(np > 0 ? np : 1);
+  for (int c = 0; c < n; ++c) twptr[c + 1] = twptr[c] + cnt[c + 1];
+  std::vector<int> fill(twptr.begin(), twptr.end() - 1);
+  for (int k = 0; k < np; ++k) twk[fill[own[ft[k]]]++] = k;  // ascending k = ascending face
+  ctx->nper = np;
+  auto up = [&](DBuf<int> &d, const std::vector<int> &h) -> int {
+    FCP_HIP(ctx, d.alloc(h.size()));
+    FCP_HIP(ctx, hipMemcpy(d.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+    return 0;
+  };
+  FCP_TRY(up(ctx->per_kof, kof));
+  FCP_TRY(up(ctx->per_twb, twb));
+  FCP_TRY(up(ctx->per_fp, fp));
+  FCP_TRY(up(ctx->per_ft, ft));
+  FCP_TRY(up(ctx->per_ii, ii));
+  FCP_TRY(up(ctx->per_twptr, twptr));
+  FCP_TRY(up(ctx->per_twk, twk));
+  return 0;
+}
+
+// face_mapping after the geometry (read_mesh, geometry.f90:110-112): patch
+// by patch in order, as the reference maps them, then the cell -> face
+// lists of the mapped owners.
+static int map_periodic_faces(fcp_ctx *ctx) {
+  int iper = 0;
+  for (int ib = 0; ib < ctx->nPatches; ++ib) {
+    if (ctx->h_bctype[ib] != FCP_BC_PERIODIC) continue;
+    const int s0 = ctx->h_start[ib], t0 = ctx->h_twin[iper++], nf = ctx->h_nfaces[ib];
+    if (nf == 0) continue;
+    DBuf<int> tmp, itmp;
+    DBuf<double> save;
+    FCP_HIP(ctx, tmp.alloc(nf));
+    FCP_HIP(ctx, itmp.alloc(nf));
+    FCP_HIP(ctx, save.alloc((size_t)6 * nf));
+    k_face_map_search<<<grid_for(nf, BLK), BLK, 0, ctx->stream>>>(
+        nf, s0, t0, ctx->owner.p, ctx->xf.p, ctx->yf.p, ctx->zf.p, ctx->xc.p, ctx->yc.p,
+        ctx->zc.p, tmp.p, itmp.p);
+    k_face_map_save<<<grid_for(nf, BLK), BLK, 0, ctx->stream>>>(
+        nf, t0, ctx->xf.p, ctx->yf.p, ctx->zf.p, ctx->arx.p, ctx->ary.p, ctx->arz.p, save.p);
+    k_face_map_apply<<<grid_for(nf, BLK), BLK, 0, ctx->stream>>>(
+        nf, t0, tmp.p, itmp.p, save.p, ctx->owner.p, ctx->xf.p, ctx->yf.p, ctx->zf.p, ctx->arx.p,
+        ctx->ary.p, ctx->arz.p);
+    FCP_HIP(ctx, hipGetLastError());
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));  // tmp, itmp, save freed on return
+  }
+  FCP_TRY(fcp_build_cell_faces(ctx));
+  return build_periodic(ctx);
+}
+
 int fcp_compute_geometry(fcp_ctx *ctx) {
   const int n = ctx->n, F = ctx->F, Fi = ctx->Fi, Fb = ctx->Fb;
+  ctx->nper = 0;
+  if (!ctx->h_twin.empty()) {  // geometry from the owners as uploaded
+    FCP_HIP(ctx, hipMemcpyAsync(ctx->owner.p, ctx->owner0.p, sizeof(int) * F,
+                                hipMemcpyDeviceToDevice, ctx->stream));
+    FCP_TRY(fcp_build_cell_faces(ctx));
+  }
   FCP_HIP(ctx, ctx->xc.alloc(n));
   FCP_HIP(ctx, ctx->yc.alloc(n));
   FCP_HIP(ctx, ctx->zc.alloc(n));
@@ -353,6 +539,7 @@ int fcp_compute_geometry(fcp_ctx *ctx) {
   }
   FCP_HIP(ctx, hipGetLastError());
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (!ctx->h_twin.empty()) FCP_TRY(map_periodic_faces(ctx));
   ctx->have_geom = true;
   return 0;
 }
@@ -360,11 +547,18 @@ int fcp_compute_geometry(fcp_ctx *ctx) {
 int fcp_build_csr(fcp_ctx *ctx) {
   FCP_TRY(fcp_coarse_wait(ctx));
   const int n = ctx->n, Fi = ctx->Fi;
-  ctx->sym_pattern = true;  // (owner, neighbour) and (neighbour, owner) for every inner face
-  for (int ib = 0; ib < ctx->nPatches; ++ib)
-    if (ctx->h_bctype[ib] == FCP_BC_PERIODIC)
-      return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "periodic patches are not supported");
-  const long long nnz = 2LL * Fi + n;  // sparse_matrix.f90:110
+  // (owner, neighbour) and (neighbour, owner) for every inner face and
+  // periodic pair
+  ctx->sym_pattern = true;
+  int nperp = 0;
+  for (int ib = 0; ib < ctx->nPatches; ++ib) nperp += ctx->h_bctype[ib] == FCP_BC_PERIODIC;
+  if ((int)ctx->h_twin.size() != nperp)
+    return fcp_fail(ctx, FCP_ERR_STATE, "periodic patches without their twins (fcp_mesh_periodic)");
+  if (nperp > 0 && !ctx->have_geom)
+    return fcp_fail(ctx, FCP_ERR_STATE, "periodic patches: compute the geometry (face mapping) first");
+  const PerP per = fcp_periodic_view(ctx);
+  const int np = per.np;
+  const long long nnz = 2LL * (Fi + np) + n;  // sparse_matrix.f90:110
   if (nnz > 0x7fffffffLL) return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "nnz exceeds int32");
   ctx->nnz = (int)nnz;
   DBuf<int> cnt;
@@ -372,18 +566,25 @@ int fcp_build_csr(fcp_ctx *ctx) {
   FCP_HIP(ctx, ctx->ia.alloc(n + 1));
   FCP_HIP(ctx, ctx->ja.alloc(nnz));
   FCP_HIP(ctx, ctx->diag.alloc(n));
-  FCP_HIP(ctx, ctx->slot_pn.alloc(Fi > 0 ? Fi : 1));
-  FCP_HIP(ctx, ctx->slot_np.alloc(Fi > 0 ? Fi : 1));
+  FCP_HIP(ctx, ctx->slot_pn.alloc(Fi + np > 0 ? Fi + np : 1));
+  FCP_HIP(ctx, ctx->slot_np.alloc(Fi + np > 0 ? Fi + np : 1));
   FCP_HIP(ctx, ctx->row_of.alloc(nnz));
   FCP_HIP(ctx, ctx->tslot.alloc(nnz));
-  k_row_count<<<grid_for(n + 1, BLK), BLK, 0, ctx->stream>>>(ctx->cf_ptr.p, ctx->cf_face.p, n, Fi, cnt.p);
+  DBuf<int> dup;
+  FCP_HIP(ctx, dup.alloc(1));
+  FCP_HIP(ctx, hipMemsetAsync(dup.p, 0, sizeof(int), ctx->stream));
+  k_row_count<<<grid_for(n + 1, BLK), BLK, 0, ctx->stream>>>(ctx->cf_ptr.p, ctx->cf_face.p, n, Fi,
+                                                               per, cnt.p);
   FCP_TRY(fcp_exclusive_scan_int(ctx, cnt.p, ctx->ia.p, n + 1));
   k_row_fill<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p,
-                                                          ctx->neighbour.p, n, Fi, ctx->ia.p,
-                                                          ctx->ja.p, ctx->diag.p);
+                                                          ctx->neighbour.p, n, Fi, per, ctx->ia.p,
+                                                          ctx->ja.p, ctx->diag.p, dup.p);
   if (Fi > 0)
     k_slots<<<grid_for(Fi, BLK), BLK, 0, ctx->stream>>>(ctx->owner.p, ctx->neighbour.p, Fi, ctx->ia.p,
                                                           ctx->ja.p, ctx->slot_pn.p, ctx->slot_np.p);
+  if (np > 0)
+    k_slots_periodic<<<grid_for(np, BLK), BLK, 0, ctx->stream>>>(
+        per, ctx->owner.p, Fi, ctx->ia.p, ctx->ja.p, ctx->slot_pn.p, ctx->slot_np.p);
   k_row_of_tslot<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(ctx->ia.p, ctx->ja.p, n, ctx->row_of.p,
                                                               ctx->tslot.p);
   FCP_HIP(ctx, hipGetLastError());
@@ -391,6 +592,11 @@ int fcp_build_csr(fcp_ctx *ctx) {
   FCP_HIP(ctx, hipMemcpyAsync(&last, ctx->ia.p + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (last != ctx->nnz) return fcp_fail(ctx, FCP_ERR_STATE, "CSR size mismatch %d vs %d", last, ctx->nnz);
+  int hdup = 0;
+  FCP_HIP(ctx, hipMemcpy(&hdup, dup.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (hdup)
+    return fcp_fail(ctx, FCP_ERR_UNSUPPORTED,
+                    "a periodic pair repeats a matrix entry (cells already coupled)");
   FCP_HIP(ctx, ctx->a.alloc(nnz));
   FCP_HIP(ctx, hipMemsetAsync(ctx->a.p, 0, sizeof(double) * nnz, ctx->stream));
   ctx->a_version++;

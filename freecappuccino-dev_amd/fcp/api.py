@@ -117,6 +117,10 @@ class Context:
         rc = self.lib.fcp_mesh_upload(self.h, mesh.n_nodes, mesh.n_cells, mesh.n_inner, mesh.n_bnd,
                                       mesh.n_patches, *ptrs)
         self._check(rc, "fcp_mesh_upload")
+        if mesh.twin_start:  # periodic patches (geometry.f90:251-258)
+            tw = np.ascontiguousarray(mesh.twin_start, dtype=np.int32)
+            self._check(self.lib.fcp_mesh_periodic(self.h, len(tw), _ptr(tw)[0]),
+                        "fcp_mesh_periodic")
 
     def read_mesh_file(self, case_dir: str, fmt: str = "native"):
         """read_mesh from <case_dir>/polyMesh in the native or OpenFOAM format
@@ -158,17 +162,18 @@ class Context:
                         "fcp_csr_build")
             self.nnz = nnz.value
             return None
-        nn = 2 * m.n_inner + m.n_cells
+        npr = int(sum(int(m.nfaces[ib]) for ib in m.periodic_patches))
+        nn = 2 * (m.n_inner + npr) + m.n_cells
         ia = np.zeros(m.n_cells + 1, np.int32)
         ja = np.zeros(nn, np.int32)
         dg = np.zeros(m.n_cells, np.int32)
-        spn = np.zeros(max(m.n_inner, 1), np.int32)
-        snp = np.zeros(max(m.n_inner, 1), np.int32)
+        spn = np.zeros(max(m.n_inner + npr, 1), np.int32)
+        snp = np.zeros(max(m.n_inner + npr, 1), np.int32)
         rc = self.lib.fcp_csr_build(self.h, C.byref(nnz), *[_ptr(a)[0] for a in (ia, ja, dg, spn, snp)])
         self._check(rc, "fcp_csr_build")
         self.nnz = nnz.value
-        return dict(ia=ia - 1, ja=ja - 1, diag=dg - 1, slot_pn=spn[: m.n_inner] - 1,
-                    slot_np=snp[: m.n_inner] - 1)
+        return dict(ia=ia - 1, ja=ja - 1, diag=dg - 1, slot_pn=spn[: m.n_inner + npr] - 1,
+                    slot_np=snp[: m.n_inner + npr] - 1)
 
     def levels(self, upper: bool = False):
         lev = np.zeros(self.mesh.n_cells, np.int32)

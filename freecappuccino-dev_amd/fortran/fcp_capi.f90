@@ -65,6 +65,14 @@ module fcp_capi
       integer(c_int) :: face_nnodes(*), node(*), owner(*), neighbour(*), nfaces(*), startFace(*), bctype(*)
     end function
 
+    integer(c_int) function fcp_mesh_periodic(ctx, nPeriodic, startFaceTwin) &
+        bind(C, name='fcp_mesh_periodic')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nPeriodic
+      integer(c_int) :: startFaceTwin(*)
+    end function
+
     integer(c_int) function fcp_geometry(ctx, xc, yc, zc, vol, arx, ary, arz, xf, yf, zf, &
         facint, Df, dnb, srdb) bind(C, name='fcp_geometry')
       import :: c_int, c_double, c_ptr

@@ -107,6 +107,21 @@ int fcp_mesh_upload(fcp_ctx *ctx, int nNodes, int nCells, int nInnerFaces,
                     const int *neighbour, const int *nfaces, const int *startFace,
                     const int *bctype);
 
+/* The twin of every FCP_BC_PERIODIC patch, in patch order: startFaceTwin as
+ * the periodic line of polyMesh/boundary gives it (geometry.f90:251-258);
+ * the twin is a whole patch of the same size (typically FCP_BC_EMPTY), not
+ * periodic and the twin of no other patch.  Call after fcp_mesh_upload;
+ * fcp_geometry then maps the twin faces as read_mesh does (face_mapping,
+ * geometry.f90:1848-1997: owners and twin-face centres and area vectors
+ * permuted), fcp_csr_build adds the periodic pairs (sparse_matrix.f90:
+ * 141-171) and their slots after the inner faces' (:262-293), the
+ * Laplacian leaves their entries zero, fcp_update_boundary averages them
+ * (updateBoundary.f90:46-68) and fcp_convdiff / fcp_transport couple them
+ * (boundary kind 5, facefluxsc_periodic, scalar_fluxes.f90:147-251).  A
+ * pair inside one cell or doubling an existing entry is FCP_ERR_UNSUPPORTED
+ * at fcp_csr_build. */
+int fcp_mesh_periodic(fcp_ctx *ctx, int nPeriodic, const int *startFaceTwin);
+
 /* read_mesh (geometry.f90:90-116) from disk: reads <case_dir>/polyMesh in the
  * native format (format 0, read_mesh_native :118-406: 1-based text files
  * size, boundary, points, faces, owner, neighbour) or the OpenFOAM format
@@ -135,7 +150,9 @@ int fcp_geometry(fcp_ctx *ctx, double *xc, double *yc, double *zc, double *vol,
  * positions and face->slot maps icell_jcell_csr_index /
  * jcell_icell_csr_index, built on the GPU and kept resident together with
  * the level schedules of the strict-lower/upper parts.  Outputs (1-based,
- * optional): ia[n+1], ja[nnz], diag[n], slot_pn[Fi], slot_np[Fi]. */
+ * optional): ia[n+1], ja[nnz], diag[n], slot_pn[Fi + numPeriodic],
+ * slot_np[Fi + numPeriodic] (the periodic pairs' slots after the inner
+ * faces', numPeriodic = faces on periodic patches; see fcp_mesh_periodic). */
 int fcp_csr_build(fcp_ctx *ctx, int *nnz, int *ia, int *ja, int *diag,
                   int *slot_pn, int *slot_np);
 
@@ -177,7 +194,9 @@ int fcp_nonorth_correction(fcp_ctx *ctx, const double *gamma_f,
  * phi_P, :224), 2 = inlet/outlet/pressure flux face (facefluxsc_boundary
  * :256-337, :182-190), 3 = no term (symmetry, empty), 4 = prescribed normal
  * gradient (wallQFlux, :231-262: su += dcoef |S|/dn (grad_b . n), grad_b the
- * face's entry of grad); then a_ii = sp - (row sum - a_ii) and under-relaxation
+ * face's entry of grad), 5 = periodic face (:262-313, facefluxsc_periodic
+ * with the twin face's owner; dcoef of the periodic face; its twin takes
+ * no term of its own when given 3); then a_ii = sp - (row sum - a_ii) and under-relaxation
  * a_ii /= urf, su += (1-urf) a_ii phi (:350-366).  flmass/dcoef [nFaces],
  * phi [numTotal] (boundary values of zero-gradient faces updated), grad
  * [3*numTotal]; su, sp [nCells] accumulate (pass them cleared, or holding
