@@ -1,6 +1,7 @@
 // fcp_api.hip -- extern "C" entry points of libfcp (see include/fcp.h).
 #include <hip/hip_runtime.h>
 
+#include <cctype>
 #include <cstdarg>
 #include <cstdio>
 #include <algorithm>
@@ -961,6 +962,168 @@ int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi, c
     printf("  %s:  Solving for %s, Initial residual = %10.3E, Final residual = %10.3E, No Iterations %d\n",
            names[code], var.c_str(), o->resor, o->resl / factor, o->iterations);
   }
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// LIS replacement (LIS_linear_solvers.f90:38-200): the LIS option syntax
+// mapped onto the library's solvers; the residual report of lis_spsolve.
+// ---------------------------------------------------------------------------
+namespace {
+
+bool lis_word(const std::string &v, const char *name, int code) {
+  return v == name || v == std::to_string(code);
+}
+
+int lis_parse(fcp_ctx *ctx, const std::string &text, fcp_ctx::LisSolver &L) {
+  std::vector<std::string> tok;
+  size_t i = 0;
+  while (i < text.size()) {
+    while (i < text.size() && isspace((unsigned char)text[i])) ++i;
+    size_t j = i;
+    while (j < text.size() && !isspace((unsigned char)text[j])) ++j;
+    if (j > i) tok.push_back(text.substr(i, j - i));
+    i = j;
+  }
+  std::string solver = "cg", precond = "none";  // LIS defaults
+  L = fcp_ctx::LisSolver();
+  for (size_t k = 0; k < tok.size(); ++k) {
+    const std::string &o = tok[k];
+    if (o == "-print" || o == "-scale" || o == "-omp_num_threads" || o == "-storage" ||
+        o == "-storage_block" || o == "-adds" || o == "-adds_iter") {
+      // output / threading / storage choices of LIS: no effect on the result here
+      if (o != "-print" || (k + 1 < tok.size() && tok[k + 1][0] != '-')) ++k;
+      continue;
+    }
+    if (k + 1 >= tok.size())
+      return fcp_fail(ctx, FCP_ERR_ARG, "LIS options: %s needs a value", o.c_str());
+    const std::string &v = tok[++k];
+    char *end = nullptr;
+    if (o == "-i") {
+      solver = v;
+    } else if (o == "-p") {
+      precond = v;
+    } else if (o == "-tol") {
+      L.tol = strtod(v.c_str(), &end);
+      if (!end || *end || !(L.tol > 0.0)) return fcp_fail(ctx, FCP_ERR_ARG, "LIS options: bad -tol");
+    } else if (o == "-maxiter") {
+      L.maxiter = (int)strtol(v.c_str(), &end, 10);
+      if (!end || *end || L.maxiter < 1) return fcp_fail(ctx, FCP_ERR_ARG, "LIS options: bad -maxiter");
+    } else if (o == "-restart") {
+      L.restart = (int)strtol(v.c_str(), &end, 10);
+      if (!end || *end || L.restart < 1) return fcp_fail(ctx, FCP_ERR_ARG, "LIS options: bad -restart");
+    } else if (o == "-initx_zeros") {
+      if (v == "true" || v == "1") L.initx_zeros = true;
+      else if (v == "false" || v == "0") L.initx_zeros = false;
+      else return fcp_fail(ctx, FCP_ERR_ARG, "LIS options: bad -initx_zeros");
+    } else if (o == "-f") {  // working precision: double only
+      if (v != "double" && v != "0")
+        return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "LIS options: -f %s (double only)", v.c_str());
+    } else if (o == "-conv_cond") {
+      if (v != "0" && v != "1") return fcp_fail(ctx, FCP_ERR_ARG, "LIS options: -conv_cond 0 or 1");
+      L.conv_cond = v[0] - '0';
+    } else {
+      return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "LIS options: %s is not supported", o.c_str());
+    }
+  }
+  // (LIS solver, LIS preconditioner) -> the library's solver of the same
+  // Krylov method and preconditioner family
+  const bool ilu = lis_word(precond, "ilu", 2), jac = lis_word(precond, "jacobi", 1),
+             none = lis_word(precond, "none", 0);
+  if (lis_word(solver, "cg", 1) && ilu) L.code = 0;               // IC(0)-type PCG (iccg)
+  else if (lis_word(solver, "cg", 1) && jac) L.code = 1;          // Jacobi PCG (dpcg)
+  else if (lis_word(solver, "bicgstab", 4) && ilu) L.code = 2;    // BiCGStab, DILU
+  else if (lis_word(solver, "gs", 11) && none) L.code = 3;        // Gauss-Seidel
+  else if (lis_word(solver, "gmres", 9) && ilu) L.code = 4;       // GMRES(m), ILU(0)
+  else
+    return fcp_fail(ctx, FCP_ERR_UNSUPPORTED,
+                    "LIS options: -i %s -p %s has no counterpart (cg+ilu, cg+jacobi, "
+                    "bicgstab+ilu, gs+none, gmres+ilu)",
+                    solver.c_str(), precond.c_str());
+  L.options = text;
+  L.have = true;
+  return 0;
+}
+
+}  // namespace
+
+int fcp_lis_create_solver(fcp_ctx *ctx, const char *options, int len) {
+  if (!ctx) return FCP_ERR_ARG;
+  if (!options) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_lis_create_solver: null options");
+  std::string text(options, options + (len > 0 ? len : strlen(options)));
+  while (!text.empty() && text.back() == ' ') text.pop_back();
+  fcp_ctx::LisSolver L;
+  FCP_TRY(lis_parse(ctx, text, L));
+  ctx->lis = L;
+  return 0;
+}
+
+int fcp_lis_destroy_solver(fcp_ctx *ctx) {
+  if (!ctx) return FCP_ERR_ARG;
+  ctx->lis = fcp_ctx::LisSolver();
+  return 0;
+}
+
+int fcp_lis_spsolve(fcp_ctx *ctx, double *fi, const double *rhs, const char *chvar,
+                    int chvar_len, double *res0_out, double *resid_out, int *iter_out) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!ctx->lis.have) return fcp_fail(ctx, FCP_ERR_STATE, "fcp_lis_spsolve: no solver created");
+  if (!fi || !rhs) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_lis_spsolve: null argument");
+  (void)hipSetDevice(ctx->device);
+  const fcp_ctx::LisSolver &L = ctx->lis;
+  const int n = ctx->n;
+  Arg<double> x, b;
+  FCP_TRY(arg_in(ctx, (const double *)fi, n, x));
+  FCP_TRY(arg_in(ctx, rhs, n, b));
+  // initial residual of the field passed in, L1 as lis_spsolve sums it
+  // (:135-143), and its L2 norm with ||b||_2 for the convergence condition
+  DBuf<double> ax;
+  FCP_HIP(ctx, ax.alloc(n));
+  FCP_TRY(fcp_launch_spmv64(ctx, ctx->a.p, x.dev, ax.p));
+  std::vector<double> hax(n), hb(n), hx(n);
+  FCP_HIP(ctx, hipMemcpyAsync(hax.data(), ax.p, sizeof(double) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(hb.data(), b.dev, sizeof(double) * n, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double res0 = 0.0, r2 = 0.0, b2 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double r = hb[i] - hax[i];
+    res0 = res0 + fabs(r);
+    r2 += r * r;
+    b2 += hb[i] * hb[i];
+  }
+  r2 = sqrt(r2);
+  b2 = sqrt(b2);
+  // LIS starts from zero unless -initx_zeros false; convergence condition 0
+  // is ||r||_2 <= tol ||r_0||_2, 1 is ||r||_2 <= tol ||b||_2
+  if (L.initx_zeros) FCP_HIP(ctx, hipMemsetAsync(x.dev, 0, sizeof(double) * n, ctx->stream));
+  const double r0n = L.initx_zeros ? b2 : r2;
+  const double rel = (L.conv_cond == 0 && !L.initx_zeros && b2 > 0.0) ? L.tol * r2 / b2 : L.tol;
+  fcp_solve_opts o;
+  fcp_solve_opts_default(&o);
+  o.criterion = FCP_CRIT_TRUE_L2;
+  int itmax = L.maxiter;
+  if (L.code == 4) itmax = (L.maxiter + L.restart - 1) / L.restart;  // GMRES counts restart cycles
+  const int keep = ctx->gm_restart;
+  ctx->gm_restart = L.restart;
+  const int rc = fcp_solve_device(ctx, L.code, x.dev, b.dev, itmax, 0.0, rel, 0, &o);
+  ctx->gm_restart = keep;
+  if (rc < 0) return rc;
+  FCP_TRY(arg_back(ctx, x));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const double denom = L.conv_cond == 0 ? r0n : b2;
+  const double resid = denom > 0.0 ? o.rel_res_l2 * b2 / denom : 0.0;
+  const int iters = o.iterations;
+  std::string var = chvar ? std::string(chvar, chvar + (chvar_len > 0 ? chvar_len : strlen(chvar)))
+                          : std::string("x");
+  while (!var.empty() && var.back() == ' ') var.pop_back();
+  printf("  LIS[%s]:  Solving for %s, Initial residual = %10.3E, Final residual = %10.3E, "
+         "No Iterations %d\n",
+         L.options.c_str(), var.c_str(), res0, res0 * resid, iters);
+  if (res0_out) *res0_out = res0;
+  if (resid_out) *resid_out = resid;
+  if (iter_out) *iter_out = iters;
   return rc;
 }
 

@@ -280,6 +280,16 @@ struct fcp_ctx {
   DBuf<double> gm_v;      // [(mr+1)*n] Krylov basis, column-major
   DBuf<char> gm_state;    // device GMRES scalars and Hessenberg matrix
   int gm_restart = 4;     // restart length of solver "pmgmres" (csrsolve :85)
+  // the LIS-replacement solver (LIS_linear_solvers.f90:38-200): the options
+  // string and what it selects (solver code of fcp_csrsolve, GMRES restart,
+  // tolerance, iteration limit, zero initial guess, convergence condition)
+  struct LisSolver {
+    bool have = false;
+    std::string options;
+    int code = 0, restart = 40, maxiter = 1000, conv_cond = 0;
+    double tol = 1e-12;
+    bool initx_zeros = true;
+  } lis;
 
   // -------- least-squares gradient matrices --------
   DBuf<double> lsq_dmat;  // Dmat(9,n) or D(3,6,n)

@@ -30,6 +30,9 @@ SIGNATURES = {
     "fcp_geometry": (_i, [_p] + [_p] * 14),
     "fcp_read_polymesh": (_i, [_p, C.c_char_p, _i, _i, _p]),
     "fcp_mesh_periodic": (_i, [_p, _i, _p]),
+    "fcp_lis_create_solver": (_i, [_p, C.c_char_p, _i]),
+    "fcp_lis_spsolve": (_i, [_p, _p, _p, C.c_char_p, _i, _p, _p, _p]),
+    "fcp_lis_destroy_solver": (_i, [_p]),
     "fcp_mesh_patches": (_i, [_p, _p, _p, _p]),
     "fcp_csr_build": (_i, [_p, C.POINTER(_i), _p, _p, _p, _p, _p]),
     "fcp_levels": (_i, [_p, _i, _p, C.POINTER(_i)]),

@@ -312,6 +312,24 @@ class Context:
         self._check(rc, "fcp_grad")
         return out
 
+    # -- LIS interface (LIS_linear_solvers.f90:38-200) ----------------------
+    def lis_create_solver(self, options: str):
+        b = options.encode()
+        self._check(self.lib.fcp_lis_create_solver(self.h, b, len(b)), "fcp_lis_create_solver")
+
+    def lis_spsolve(self, fi, rhs, chvar: str = "x"):
+        """lis_spsolve on the resident matrix: fi (current field in, solution
+        out).  Returns (status, dict(res0, resid, iter))."""
+        res0, resid, it = C.c_double(), C.c_double(), C.c_int()
+        b = chvar.encode()
+        rc = self.lib.fcp_lis_spsolve(self.h, _ptr(fi)[0], _ptr(_f64(rhs))[0], b, len(b),
+                                      C.byref(res0), C.byref(resid), C.byref(it))
+        self._check(rc, "fcp_lis_spsolve")
+        return rc, dict(res0=res0.value, resid=resid.value, iter=it.value)
+
+    def lis_destroy_solver(self):
+        self._check(self.lib.fcp_lis_destroy_solver(self.h), "fcp_lis_destroy_solver")
+
     def update_boundary(self, phi):
         self._check(self.lib.fcp_update_boundary(self.h, _ptr(phi)[0]), "fcp_update_boundary")
         return phi

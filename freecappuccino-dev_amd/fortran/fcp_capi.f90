@@ -295,6 +295,29 @@ module fcp_capi
       type(fcp_solve_opts) :: opts
     end function
 
+    integer(c_int) function fcp_lis_create_solver(ctx, options, options_len) &
+        bind(C, name='fcp_lis_create_solver')
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: options(*)
+      integer(c_int), value :: options_len
+    end function
+
+    integer(c_int) function fcp_lis_spsolve(ctx, fi, rhs, chvar, chvar_len, res0, resid, iter) &
+        bind(C, name='fcp_lis_spsolve')
+      import :: c_int, c_double, c_ptr, c_char
+      type(c_ptr), value :: ctx
+      real(c_double) :: fi(*), rhs(*), res0, resid
+      character(kind=c_char) :: chvar(*)
+      integer(c_int), value :: chvar_len
+      integer(c_int) :: iter
+    end function
+
+    integer(c_int) function fcp_lis_destroy_solver(ctx) bind(C, name='fcp_lis_destroy_solver')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
     integer(c_int) function fcp_csrsolve_multi(ctx, solver, solver_len, nrhs, fi, ldfi, rhs, res0, &
         itr_max, tol_abs, tol_rel, verbose, opts) bind(C, name='fcp_csrsolve_multi')
       import :: c_int, c_double, c_ptr, c_char, fcp_solve_opts

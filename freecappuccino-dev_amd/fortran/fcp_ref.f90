@@ -321,6 +321,30 @@ contains
     call fcp_check(rc, 'fcp_csrsolve')
   end subroutine
 
+  ! LIS_linear_solvers.f90:38-200 under the reference's names: the LIS
+  ! option string selects the library's solver; lis_spsolve solves the module
+  ! matrix (numCells, ia, ja, a) for fi and prints LIS's report line.
+  subroutine lis_create_solver(solver_options)
+    character(len=*), intent(in) :: solver_options
+    call fcp_check(fcp_lis_create_solver(ctx, solver_options, len_trim(solver_options)), &
+                   'fcp_lis_create_solver')
+  end subroutine
+
+  subroutine lis_spsolve(fi, rhs, chvar)
+    real(dp), intent(inout) :: fi(numCells)
+    real(dp), intent(in) :: rhs(numCells)
+    character(len=*), intent(in) :: chvar
+    real(dp) :: res0, resid
+    integer(c_int) :: iter
+    call fcp_check(fcp_matrix_set(ctx, a), 'fcp_matrix_set')
+    call fcp_check(fcp_lis_spsolve(ctx, fi, rhs, chvar, len_trim(chvar), res0, resid, iter), &
+                   'fcp_lis_spsolve')
+  end subroutine
+
+  subroutine lis_destroy_solver
+    call fcp_check(fcp_lis_destroy_solver(ctx), 'fcp_lis_destroy_solver')
+  end subroutine
+
   subroutine iccg(n, nz, ia_, ja_, a_, diag_, fi, rhs, resor, itr_max, tol_abs, tol_rel, chvar, verbose)
     integer, intent(in) :: n, nz, itr_max
     integer, intent(in) :: ia_(n+1), ja_(nz), diag_(n)

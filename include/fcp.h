@@ -369,6 +369,30 @@ int fcp_csrsolve(fcp_ctx *ctx, const char *solver, int solver_len, double *fi,
                  double tol_rel, const char *chvar, int chvar_len, int verbose,
                  fcp_solve_opts *opts);
 
+/* The LIS interface (LIS_linear_solvers.f90:38-200) without the LIS library:
+ * fcp_lis_create_solver (lis_create_solver :38-90) takes the LIS option
+ * string -- -i cg|bicgstab|gmres|gs (or LIS's numbers 1, 4, 9, 11), -p
+ * none|jacobi|ilu (0, 1, 2), -tol (1e-12), -maxiter (1000), -restart (40),
+ * -initx_zeros true|false (true: LIS starts from zero), -conv_cond 0|1
+ * (||r||_2 <= tol ||r_0||_2 | <= tol ||b||_2), -f double; -print, -scale,
+ * -storage*, -adds*, -omp_num_threads are accepted and have no effect --
+ * and maps the method to this library's solver of the same Krylov method
+ * and preconditioner family: cg+ilu -> iccg (DIC), cg+jacobi -> dpcg,
+ * bicgstab+ilu -> bicgstab (DILU), gs+none -> gauss-seidel, gmres+ilu ->
+ * GMRES(restart) with ILU(0); other pairs and options are
+ * FCP_ERR_UNSUPPORTED.  fcp_lis_spsolve (lis_spsolve :92-173) solves the
+ * resident system for fi[n] (in: the current field, out: the solution),
+ * prints its report line (initial residual = L1 of rhs - A fi, final =
+ * that times the relative residual norm), and returns 0 when converged,
+ * the iteration count when not, < 0 on error; res0 (L1 initial residual),
+ * resid (relative residual norm of the convergence condition) and iter are
+ * optional outputs.  No LIS counterpart of the iteration count exists:
+ * it is the iteration count of the mapped solver. */
+int fcp_lis_create_solver(fcp_ctx *ctx, const char *options, int len);
+int fcp_lis_spsolve(fcp_ctx *ctx, double *fi, const double *rhs, const char *chvar,
+                    int chvar_len, double *res0, double *resid, int *iter);
+int fcp_lis_destroy_solver(fcp_ctx *ctx);
+
 /* Batched csrsolve: nrhs (1..4) systems sharing the resident matrix, as the
  * three momentum components of velocity.f90:638-748 share one operator
  * (there each is a separate csrsolve call).  fi is fi(ldfi, nrhs) and rhs is

@@ -747,9 +747,9 @@ def test_tile_pack_layout_bitequal(api, tile):
     o = Oracle(mesh)
     F = mesh.n_faces
     bk = np.full(mesh.n_bnd, 2, np.uint8)
-    a_cd, su_cd, _, _ = o.convdiff(np.cos(np.arange(F)) * 1e-2, 10.0 ** np.sin(np.arange(F)),
-                                   np.zeros(mesh.n_total), np.zeros((mesh.n_total, 3)), 1.0, 0,
-                                   bk)
+    a_cd, _, _, _ = o.convdiff(np.cos(np.arange(F)) * 1e-2, 10.0 ** np.sin(np.arange(F)),
+                               np.zeros(mesh.n_total), np.zeros((mesh.n_total, 3)), 1.0, 0, bk)
+    su_cd = np.sin(np.arange(mesh.n_cells) * 0.7)  # a right-hand side with content
     out = {}
     for pack in (1, 0):
         ctx.tune(api.FCP_TUNE_TILE_PACK, pack)
