@@ -1400,6 +1400,40 @@ int fcp_csrsolve_multi(fcp_ctx *ctx, const char *solver, int solver_len, int nrh
   return rc;
 }
 
+int fcp_csrsolve_multi_diag(fcp_ctx *ctx, const char *solver, int solver_len, int nrhs,
+                            double *fi, int ldfi, const double *rhs, const double *diagv,
+                            double *res0, int itr_max, double tol_abs, double tol_rel,
+                            int verbose, fcp_solve_opts *opts) {
+  FCP_TRY(need(ctx, true, false, true));
+  if (!solver || !fi || !rhs || !diagv)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_csrsolve_multi_diag: null argument");
+  if (nrhs < 1 || nrhs > 4)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_csrsolve_multi_diag: nrhs %d", nrhs);
+  if (ldfi < ctx->n) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_csrsolve_multi_diag: ldfi < numCells");
+  const int code = solver_code(solver, solver_len);
+  if (code != 0 && code != 2)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_csrsolve_multi_diag: solver must be iccg or bicgstab");
+  (void)hipSetDevice(ctx->device);
+  Arg<double> x, b, d;
+  FCP_TRY(arg_in(ctx, (const double *)fi, (size_t)ldfi * nrhs, x));
+  FCP_TRY(arg_in(ctx, rhs, (size_t)ctx->n * nrhs, b));
+  FCP_TRY(arg_in(ctx, diagv, (size_t)ctx->n * nrhs, d));
+  fcp_solve_opts local;
+  fcp_solve_opts_default(&local);
+  fcp_solve_opts *o = opts ? opts : &local;
+  const int rc = fcp_solve_multi_device(ctx, code, nrhs, x.dev, ldfi, b.dev, res0, itr_max,
+                                        tol_abs, tol_rel, verbose, o, d.dev);
+  if (rc < 0) return rc;
+  FCP_TRY(arg_back(ctx, x));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (verbose >= 1)
+    printf("  %s x%d (own diagonals):  Initial residual = %10.3E, worst |b-Ax|/|b| = %10.3E, "
+           "No Iterations %d\n",
+           code == 0 ? "PCG(IC0)" : "BiCGStab(ILU(0))", nrhs, o->res0, o->rel_res_l2,
+           o->iterations);
+  return rc;
+}
+
 // Explicit-array entry points on a library-owned context (device 0).
 static std::mutex g_explicit_mu;
 static fcp_ctx *g_explicit_ctx = nullptr;

@@ -223,6 +223,9 @@ template <typename T>
 struct Sell {
   const int *base, *col, *ia;
   const T *a;
+  // batched products with a diagonal of their own per column: [n*NRP]
+  // interleaved, used in place of the diagonal entry (null: the matrix's)
+  const T *dcol = nullptr;
 };
 
 // Entries of a row whose loads are issued together (rows longer than this
@@ -297,6 +300,9 @@ int tin_apply(fcp_ctx *ctx, const double *r, double *z, bool coarse, bool fp64 =
 int tin_coarse_begin(fcp_ctx *ctx);
 int tin_coarse_join(fcp_ctx *ctx);
 int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z);
+// DIC (kind 0) / DILU (kind 1) pivots of the matrix values a (the resident
+// pattern) into d, as the solves compute them.
+int precond_pivots64(fcp_ctx *ctx, int kind, const double *a, double *d);
 int precond_apply32(fcp_ctx *ctx, const float *r, float *y, float *z);
 
 }  // namespace fcpk

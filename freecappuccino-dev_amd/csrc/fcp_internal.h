@@ -306,7 +306,15 @@ struct fcp_ctx {
   DBuf<char> tpin;  // batched sweep inputs on the tile schedules
   DBuf<char> jpin;  // batched sweep inputs on the tile-local schedules
   DBuf<double> mv64[9];
-  DBuf<float> mv32[8];
+  DBuf<float> mv32[9];
+  // per-column diagonals of a batched solve (velocity.f90:611-632: the
+  // momentum components share the off-diagonals): the diagonals and the
+  // pivots interleaved by row, the matrix with one column's diagonal (pivot
+  // set-up), and the pivots in each sweep schedule's processing order
+  bool mdc_active = false;
+  DBuf<double> mdcol, mpiv, mdiag_a;
+  DBuf<double> mpdc[2];
+  const void *mpdc_of[2] = {nullptr, nullptr};
 
   // -------- staging --------
   DBuf<char> stage[4];
@@ -475,7 +483,8 @@ int fcp_atx_device(fcp_ctx *ctx, const double *x, double *y);
 // fcp_multi.hip: batched solve of nrhs columns (column-major, device memory)
 int fcp_solve_multi_device(fcp_ctx *ctx, int solver, int nrhs, double *fi, int ldfi,
                            const double *rhs, double *res0_out, int itr_max, double tol_abs,
-                           double tol_rel, int verbose, fcp_solve_opts *opts);
+                           double tol_rel, int verbose, fcp_solve_opts *opts,
+                           const double *diagv = nullptr);
 
 // rocPRIM-based primitives (fcp_prim.hip)
 int fcp_exclusive_scan_int(fcp_ctx *ctx, const int *in, int *out, int n);

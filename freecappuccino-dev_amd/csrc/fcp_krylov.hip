@@ -1553,6 +1553,14 @@ int solve_fp64(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max
   return 0;
 }
 
+}  // namespace
+namespace fcpk {
+int precond_pivots64(fcp_ctx *ctx, int kind, const double *a, double *d) {
+  return precond_setup<double>(ctx, kind, a, d);
+}
+}  // namespace fcpk
+namespace {
+
 // True FP64 residual r = b - A x with norms into scalars slot.
 int true_residual(fcp_ctx *ctx, const double *x, const double *b, SolveScalars *s) {
   const int n = ctx->n;

@@ -63,19 +63,30 @@ __device__ __forceinline__ void msell_row(const Sell<T> &m, int i, const T *__re
       av[k] = m.a[p + 64 * k];
       cv[k] = m.col[p + 64 * k];
     }
+  // per-column diagonal: column r's own value at the diagonal entry, in the
+  // entry's place of the sum
+  const T *dc = m.dcol ? m.dcol + (size_t)i * NRP : nullptr;
 #pragma unroll
   for (int k = 0; k < SELL_KU; ++k)
     if (k < len) {
       const T *xc = x + (size_t)cv[k] * NRP;
+      const bool dg = dc && cv[k] == i;
 #pragma unroll
-      for (int r = 0; r < NR; ++r)
-        acc[r] = subtract ? acc[r] - av[k] * xc[r] : acc[r] + av[k] * xc[r];
+      for (int r = 0; r < NR; ++r) {
+        const T ak = dg ? dc[r] : av[k];
+        acc[r] = subtract ? acc[r] - ak * xc[r] : acc[r] + ak * xc[r];
+      }
     }
   for (int k = SELL_KU; k < len; ++k) {
     const T a = m.a[p + 64 * k];
-    const T *xc = x + (size_t)m.col[p + 64 * k] * NRP;
+    const int j = m.col[p + 64 * k];
+    const T *xc = x + (size_t)j * NRP;
+    const bool dg = dc && j == i;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = subtract ? acc[r] - a * xc[r] : acc[r] + a * xc[r];
+    for (int r = 0; r < NR; ++r) {
+      const T ak = dg ? dc[r] : a;
+      acc[r] = subtract ? acc[r] - ak * xc[r] : acc[r] + ak * xc[r];
+    }
   }
 }
 
@@ -239,6 +250,13 @@ __global__ void mk_deinterleave(int n, const double *__restrict__ il, double *__
   }
 }
 
+// a copy of the matrix with the diagonal of one column
+__global__ void mk_set_diag(int n, const int *__restrict__ diag, const double *__restrict__ dv,
+                            double *__restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[diag[i]] = dv[i];
+}
+
 // one column of an interleaved vector <-> a contiguous vector
 template <typename T, int NRP>
 __global__ void mk_column(int n, const T *__restrict__ src, int sstride, int soff,
@@ -331,7 +349,7 @@ __global__ __launch_bounds__(BLK) void mk_xr_update(
     int n, const int *__restrict__ diag, const T *__restrict__ a, T *__restrict__ x,
     T *__restrict__ r, const T *__restrict__ p, const T *__restrict__ q,
     double *__restrict__ partials, int *__restrict__ counter, SolveScalars *__restrict__ s,
-    int crit, double tol_abs, double tol_rel, int itr_max) {
+    int crit, double tol_abs, double tol_rel, int itr_max, const T *__restrict__ dcol) {
   if (all_done<NR>(s)) return;
   bool dn[NR], first[NR];
   T alf[NR];
@@ -345,11 +363,12 @@ __global__ __launch_bounds__(BLK) void mk_xr_update(
 #pragma unroll
   for (int c = 0; c < 3 * NR; ++c) part[c] = 0.0;
   for (int i = blockIdx.x * BLK + threadIdx.x; i < n; i += gridDim.x * BLK) {
-    const double ad = (double)a[diag[i]];
+    const double ad0 = dcol ? 0.0 : (double)a[diag[i]];
 #pragma unroll
     for (int c = 0; c < NR; ++c) {
       const size_t o = (size_t)i * NRP + c;
       if (dn[c]) continue;
+      const double ad = dcol ? (double)dcol[o] : ad0;
       const T xi = x[o] + alf[c] * p[o];
       x[o] = xi;
       const T ri = r[o] - alf[c] * q[o];
@@ -606,7 +625,7 @@ __device__ __forceinline__ void row_store(__amdgpu_buffer_rsrc_t rs, int off, co
 template <typename T, int NR, int NRP>
 struct MSlot {
   int row, cnt, ok, pos;
-  T dv;
+  T dv[NRP];  // the row's pivot, per column
   T xin[NRP];
   int dep[FCP_PMAXD];
   T av[FCP_PMAXD];
@@ -618,12 +637,15 @@ struct MArgs {
   const int *prc, *pdep;
   const T *pa, *pdx, *pin;
   int n;
+  // per-column pivots in processing order ([n*NRP]; null: the shared ones)
+  const T *pdc = nullptr;
 };
 
 template <typename T, int NRP>
 __device__ __forceinline__ void m_fetch(const MArgs<T> &P, __amdgpu_buffer_rsrc_t rin, int idx,
-                                        int &row, int &cnt, T &dv, T (&xin)[NRP], int *dep,
-                                        T *av) {
+                                        int &row, int &cnt, T (&dvs)[NRP], T (&xin)[NRP],
+                                        int *dep, T *av) {
+  T dv;
   const int2 rc = reinterpret_cast<const int2 *>(P.prc)[idx];
   row = rc.x;
   cnt = rc.y & 0xFFFF;
@@ -648,6 +670,8 @@ __device__ __forceinline__ void m_fetch(const MArgs<T> &P, __amdgpu_buffer_rsrc_
     av[3] = (T)a1.y;
     dv = (T)reinterpret_cast<const double2 *>(P.pdx)[idx].x;
   }
+#pragma unroll
+  for (int r = 0; r < NRP; ++r) dvs[r] = P.pdc ? P.pdc[(size_t)idx * NRP + r] : dv;
   row_load<T, NRP, 0>(rin, idx * NRP * (int)sizeof(T), xin);
 }
 
@@ -675,13 +699,14 @@ __device__ __forceinline__ void m_load2(MSlot<T, NR, NRP> &S, __amdgpu_buffer_rs
 }
 
 template <typename T, int NR, int NRP, int MODE>
-__device__ __forceinline__ void m_row_value(int cnt, T dv, const T (&xin)[NRP], const int *dep,
+__device__ __forceinline__ void m_row_value(int cnt, const T (&dv)[NRP], const T (&xin)[NRP],
+                                            const int *dep,
                                             const T *av, const T (*rv)[NRP], const T *zl, T *out,
                                             int *err, int slp, T (&val)[NRP]) {
   using B = Bits<T>;
   T acc[NR];
 #pragma unroll
-  for (int r = 0; r < NR; ++r) acc[r] = (MODE == 0) ? xin[r] : xin[r] / (dv + (T)FCP_SMALL);
+  for (int r = 0; r < NR; ++r) acc[r] = (MODE == 0) ? xin[r] : xin[r] / (dv[r] + (T)FCP_SMALL);
 #pragma unroll
   for (int m = 0; m < FCP_PMAXD; ++m) {
     if (m < cnt) {
@@ -700,7 +725,7 @@ __device__ __forceinline__ void m_row_value(int cnt, T dv, const T (&xin)[NRP], 
   }
 #pragma unroll
   for (int r = 0; r < NRP; ++r) {
-    T o = r < NR ? acc[r] * dv : (T)0;
+    T o = r < NR ? acc[r] * dv[r] : (T)0;
     if (B::to(o) == B::sent) o = B::from(B::qnan);
     val[r] = o;
   }
@@ -722,7 +747,7 @@ __device__ __forceinline__ void m_compute(const MSlot<T, NR, NRP> &S, const MArg
   const int e = sptr[step + 1];
   for (int idx = sptr[step] + (int)threadIdx.x + BLK; idx < e; idx += BLK) {
     int row, cnt, dep[FCP_PMAXD];
-    T dv, xin[NRP], av[FCP_PMAXD];
+    T dv[NRP], xin[NRP], av[FCP_PMAXD];
     m_fetch<T, NRP>(P, rin, idx, row, cnt, dv, xin, dep, av);
     m_row_value<T, NR, NRP, MODE>(cnt, dv, xin, dep, av, nullptr, zl, out, err, slp, val);
 #pragma unroll
@@ -902,6 +927,7 @@ int mprecond_apply(fcp_ctx *ctx, const T *r, T *y, T *z, T *tmp3, const SolveSca
   } else {
     FCP_TRY((msched_for<T, NRP>(ctx, &blo, &bup, &pbuf)));
   }
+  const bool pcol = sizeof(T) == 8 && ctx->mdc_active;  // pivots of their own per column
   if (blo && (size_t)n * NRP * sizeof(T) < ((size_t)1 << 31)) {
     const size_t m = (size_t)n * NRP;
     FCP_HIP(ctx, pbuf->alloc(m * sizeof(T)));
@@ -917,6 +943,17 @@ int mprecond_apply(fcp_ctx *ctx, const T *r, T *y, T *z, T *tmp3, const SolveSca
       T *out = u ? z : y;
       mk_gather<T, NR, NRP><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, b.prc.p, in, pin, s);
       MArgs<T> P{b.prc.p, b.pdep.p, pa_of<T>(b), pdx_of<T>(b), pin, n};
+      if constexpr (sizeof(T) == 8) {
+        if (pcol) {  // the columns' pivots in this schedule's processing order
+          if (ctx->mpdc_of[u] != &b) {
+            FCP_HIP(ctx, ctx->mpdc[u].alloc(m));
+            mk_gather<T, NR, NRP><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(
+                n, b.prc.p, ctx->mpiv.p, ctx->mpdc[u].p, nullptr);
+            ctx->mpdc_of[u] = &b;
+          }
+          P.pdc = ctx->mpdc[u].p;
+        }
+      }
       const int sptr_off = (int)(((size_t)b.R * NRP * sizeof(T) + 15) & ~(size_t)15);
       const size_t lds = (size_t)sptr_off + (size_t)(b.max_steps + 2) * sizeof(int);
       ProfScope ps(ctx, u ? (sizeof(T) == 8 ? "msweep_bwd_f64" : "msweep_bwd_f32")
@@ -933,6 +970,9 @@ int mprecond_apply(fcp_ctx *ctx, const T *r, T *y, T *z, T *tmp3, const SolveSca
     }
     return 0;
   }
+  if (pcol)
+    return fcp_fail(ctx, FCP_ERR_UNSUPPORTED,
+                    "per-column diagonals need the batched sweeps (schedule does not fit)");
   // one column at a time: tmp3 holds 3 contiguous vectors of n
   T *cr = tmp3, *cy = tmp3 + n, *cz = tmp3 + 2 * (size_t)n;
   for (int c = 0; c < NR; ++c) {
@@ -951,7 +991,9 @@ template <typename T>
 int mview(fcp_ctx *ctx, Sell<T> *m);
 template <>
 int mview<double>(fcp_ctx *ctx, Sell<double> *m) {
-  return sell_view64(ctx, m);
+  FCP_TRY(sell_view64(ctx, m));
+  if (ctx->mdc_active) m->dcol = ctx->mdcol.p;  // the columns' own diagonals
+  return 0;
 }
 template <>
 int mview<float>(fcp_ctx *ctx, Sell<float> *m) {
@@ -963,6 +1005,14 @@ template <typename T>
 struct MVecs {
   T *x, *r, *reso, *p, *z, *y, *u, *v, *tmp3;
 };
+
+template <typename T>
+const T *mdcol_of(fcp_ctx *ctx) {
+  if constexpr (sizeof(T) == 8)
+    return ctx->mdc_active ? ctx->mdcol.p : nullptr;
+  else
+    return nullptr;
+}
 
 template <typename T, int NR, int NRP>
 int mpcg_iteration(fcp_ctx *ctx, int precond, const T *a, MVecs<T> &V, SolveScalars *s, int crit,
@@ -978,7 +1028,7 @@ int mpcg_iteration(fcp_ctx *ctx, int precond, const T *a, MVecs<T> &V, SolveScal
                                                       s);
   mk_xr_update<T, NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->diag.p, a, V.x, V.r, V.p, V.u,
                                                         ctx->partials.p, redctr(ctx), s, crit,
-                                                        tol_abs, tol_rel, itr_max);
+                                                        tol_abs, tol_rel, itr_max, mdcol_of<T>(ctx));
   (void)precond;
   FCP_HIP(ctx, hipGetLastError());
   return 0;
@@ -1003,7 +1053,7 @@ int mbicgstab_iteration(fcp_ctx *ctx, const T *a, MVecs<T> &V, SolveScalars *s, 
                                                      redctr(ctx), s);
   mk_xr_update<T, NR, NRP><<<g, BLK, 0, ctx->stream>>>(n, ctx->diag.p, a, V.x, V.r, V.z, V.v,
                                                         ctx->partials.p, redctr(ctx), s, crit,
-                                                        tol_abs, tol_rel, itr_max);
+                                                        tol_abs, tol_rel, itr_max, mdcol_of<T>(ctx));
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }
@@ -1066,7 +1116,7 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
   MVecs<double> V64{x, ctx->mv64[0].p, ctx->mv64[1].p, ctx->mv64[2].p, ctx->mv64[3].p,
                     ctx->mv64[4].p, ctx->mv64[5].p, ctx->mv64[6].p, ctx->mv64[7].p};
   Sell<double> m64;
-  FCP_TRY(sell_view64(ctx, &m64));
+  FCP_TRY(mview<double>(ctx, &m64));
   for (int c = 0; c < MAXNR; ++c) {
     res.iters[c] = 0;
     res.conv[c] = 1;  // FP64 solves record their stopping test below
@@ -1240,7 +1290,8 @@ int msolve(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max, do
 // ---------------------------------------------------------------------------
 int fcp_solve_multi_device(fcp_ctx *ctx, int solver, int nrhs, double *fi, int ldfi,
                            const double *rhs, double *res0_out, int itr_max, double tol_abs,
-                           double tol_rel, int verbose, fcp_solve_opts *opts) {
+                           double tol_rel, int verbose, fcp_solve_opts *opts,
+                           const double *diagv) {
   FCP_TRY(fcp_ensure_workspace(ctx));
   if (nrhs < 1 || nrhs > MAXNR)
     return fcp_fail(ctx, FCP_ERR_ARG, "nrhs must be in [1, %d]", MAXNR);
@@ -1259,6 +1310,42 @@ int fcp_solve_multi_device(fcp_ctx *ctx, int solver, int nrhs, double *fi, int l
   FCP_HIP(ctx, ctx->mv64[8].alloc(2 * m));
   double *xi = ctx->mv64[8].p, *bi = ctx->mv64[8].p + m;
   const int g = red_grid(ctx, n);
+  // per-column diagonals (velocity.f90:611-632): interleaved, and each
+  // column's DIC / DILU pivots from the matrix with its diagonal
+  struct DcScope {
+    fcp_ctx *c;
+    ~DcScope() { c->mdc_active = false; }
+  } dc_scope{ctx};
+  ctx->mdc_active = false;
+  if (diagv) {
+    if (prec != FCP_PREC_FP64)
+      return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "per-column diagonals: FP64 solves only");
+    FCP_TRY(fcp_ensure_workspace(ctx));
+    FCP_HIP(ctx, ctx->mdcol.alloc(m));
+    FCP_HIP(ctx, ctx->mpiv.alloc(m));
+    FCP_HIP(ctx, ctx->mdiag_a.alloc(ctx->nnz));
+    DBuf<double> piv;
+    FCP_HIP(ctx, piv.alloc(n));
+    FCP_HIP(ctx, hipMemsetAsync(ctx->mdcol.p, 0, m * sizeof(double), ctx->stream));
+    FCP_HIP(ctx, hipMemsetAsync(ctx->mpiv.p, 0, m * sizeof(double), ctx->stream));
+    const int kind = solver == 2 ? 1 : 0;
+    for (int c = 0; c < nrhs; ++c) {
+      const double *dvc = diagv + (size_t)c * n;
+      mk_column<double, 1><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, dvc, 1, 0, ctx->mdcol.p,
+                                                                       NRP, c);
+      FCP_HIP(ctx, hipMemcpyAsync(ctx->mdiag_a.p, ctx->a.p, sizeof(double) * ctx->nnz,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+      mk_set_diag<<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, ctx->diag.p, dvc,
+                                                            ctx->mdiag_a.p);
+      FCP_TRY(precond_pivots64(ctx, kind, ctx->mdiag_a.p, piv.p));
+      mk_column<double, 1><<<grid_for(n, BLK), BLK, 0, ctx->stream>>>(n, piv.p, 1, 0, ctx->mpiv.p,
+                                                                       NRP, c);
+    }
+    FCP_HIP(ctx, hipGetLastError());
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));  // piv freed on return
+    ctx->mpdc_of[0] = ctx->mpdc_of[1] = nullptr;
+    ctx->mdc_active = true;
+  }
   hipEvent_t e0, e1;
   FCP_HIP(ctx, hipEventCreate(&e0));
   FCP_HIP(ctx, hipEventCreate(&e1));
@@ -1291,7 +1378,7 @@ int fcp_solve_multi_device(fcp_ctx *ctx, int solver, int nrhs, double *fi, int l
   SolveScalars *sf = ctx->scal.p + 4;
   SolveScalars h[MAXNR];
   Sell<double> m64;
-  FCP_TRY(sell_view64(ctx, &m64));
+  FCP_TRY(mview<double>(ctx, &m64));
   switch (nrhs) {
 #define FCP_MRES(K)                                                                              \
   case K:                                                                                       \

@@ -64,6 +64,8 @@ SIGNATURES = {
     "fcp_grad": (_i, [_p, _i, _p, _p]),
     "fcp_csrsolve_multi": (_i, [_p, C.c_char_p, _i, _i, _p, _i, _p, _p, _i, _d, _d, _i,
                                 C.POINTER(SolveOpts)]),
+    "fcp_csrsolve_multi_diag": (_i, [_p, C.c_char_p, _i, _i, _p, _i, _p, _p, _p, _i, _d, _d, _i,
+                                     C.POINTER(SolveOpts)]),
     "fcp_iccg": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _d, _d, _i]),
     "fcp_dpcg": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _d, _d, _i]),
     "fcp_bicgstab": (_i, [_i, _i, _p, _p, _p, _p, _p, _p, C.POINTER(_d), _i, _d, _d, _i]),

@@ -413,10 +413,12 @@ class Context:
 
     def csrsolve_multi(self, solver: str, fi, rhs, itr_max=1000, tol_abs=0.0, tol_rel=1e-10,
                        verbose=0, precision=FCP_PREC_FP64, criterion=FCP_CRIT_REF_L1,
-                       inner_tol=1e-4, max_outer=30, check_every=8):
+                       inner_tol=1e-4, max_outer=30, check_every=8, diag=None):
         """Batched csrsolve of nrhs columns sharing the resident matrix.  fi and
         rhs are (nrhs, n) row-major arrays (numpy or torch), i.e. Fortran
-        fi(n, nrhs); fi is updated in place.  Returns (status, info, res0[nrhs])."""
+        fi(n, nrhs); fi is updated in place.  diag (nrhs, n), optional: each
+        column's own diagonal (fcp_csrsolve_multi_diag, the momentum
+        components' solves).  Returns (status, info, res0[nrhs])."""
         nrhs, n = rhs.shape
         opts = _lib.SolveOpts()
         self.lib.fcp_solve_opts_default(C.byref(opts))
@@ -427,11 +429,19 @@ class Context:
         opts.check_every = check_every
         res0 = (C.c_double * nrhs)()
         sb = solver.encode()
-        rc = self.lib.fcp_csrsolve_multi(self.h, sb, len(sb), int(nrhs), _ptr(fi)[0],
-                                         int(fi.shape[1]), _ptr(_f64(rhs))[0], res0,
-                                         int(itr_max), float(tol_abs), float(tol_rel),
-                                         int(verbose), C.byref(opts))
-        self._check(rc, "fcp_csrsolve_multi")
+        if diag is None:
+            rc = self.lib.fcp_csrsolve_multi(self.h, sb, len(sb), int(nrhs), _ptr(fi)[0],
+                                             int(fi.shape[1]), _ptr(_f64(rhs))[0], res0,
+                                             int(itr_max), float(tol_abs), float(tol_rel),
+                                             int(verbose), C.byref(opts))
+            self._check(rc, "fcp_csrsolve_multi")
+        else:
+            rc = self.lib.fcp_csrsolve_multi_diag(self.h, sb, len(sb), int(nrhs), _ptr(fi)[0],
+                                                  int(fi.shape[1]), _ptr(_f64(rhs))[0],
+                                                  _ptr(_f64(diag))[0], res0, int(itr_max),
+                                                  float(tol_abs), float(tol_rel), int(verbose),
+                                                  C.byref(opts))
+            self._check(rc, "fcp_csrsolve_multi_diag")
         info = {k: getattr(opts, k) for k, _ in _lib.SolveOpts._fields_}
         info["status"] = rc
         return rc, info, np.array(res0[:])

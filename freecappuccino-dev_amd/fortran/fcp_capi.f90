@@ -329,6 +329,17 @@ module fcp_capi
       type(fcp_solve_opts) :: opts
     end function
 
+    integer(c_int) function fcp_csrsolve_multi_diag(ctx, solver, solver_len, nrhs, fi, ldfi, rhs, &
+        diagv, res0, itr_max, tol_abs, tol_rel, verbose, opts) bind(C, name='fcp_csrsolve_multi_diag')
+      import :: c_int, c_double, c_ptr, c_char, fcp_solve_opts
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: solver(*)
+      integer(c_int), value :: solver_len, nrhs, ldfi, itr_max, verbose
+      real(c_double) :: fi(*), rhs(*), diagv(*), res0(*)
+      real(c_double), value :: tol_abs, tol_rel
+      type(fcp_solve_opts) :: opts
+    end function
+
     integer(c_int) function fcp_iccg(n, nnz, ia, ja, a, diag, fi, rhs, resor, itr_max, &
         tol_abs, tol_rel, verbose) bind(C, name='fcp_iccg')
       import :: c_int, c_double

@@ -408,6 +408,21 @@ int fcp_csrsolve_multi(fcp_ctx *ctx, const char *solver, int solver_len, int nrh
                        int itr_max, double tol_abs, double tol_rel, int verbose,
                        fcp_solve_opts *opts);
 
+/* fcp_csrsolve_multi with a diagonal of its own per column and the resident
+ * matrix's off-diagonals shared, as the momentum components of
+ * velocity.f90:611-748 share theirs: diagv(numCells, nrhs), column-major,
+ * holds each column's diagonal entries (a_ii after under-relaxation, :617-
+ * 627, :673-681, :729-737); the resident matrix's own diagonal is not used.
+ * Each column is bit-equal to fcp_csrsolve on the resident matrix with its
+ * diagonal replaced by that column's (DIC / DILU pivots per column, set up
+ * from that matrix); the batched SpMV and sweeps read the off-diagonals
+ * once for all columns.  FP64 only (FCP_ERR_UNSUPPORTED for other
+ * precisions, and where the batched sweeps do not fit the mesh). */
+int fcp_csrsolve_multi_diag(fcp_ctx *ctx, const char *solver, int solver_len, int nrhs,
+                            double *fi, int ldfi, const double *rhs, const double *diagv,
+                            double *res0, int itr_max, double tol_abs, double tol_rel,
+                            int verbose, fcp_solve_opts *opts);
+
 /* Explicit-array entry points with the reference signatures
  * iccg/dpcg/bicgstab/GaussSeidel(n,nnz,ia,ja,a,diag,fi,rhs,resor,itr_max,
  * tol_abs,tol_rel,chvar,verbose) (linear_solvers.f90:364,206,548,96).
