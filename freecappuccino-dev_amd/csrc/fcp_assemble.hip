@@ -11,6 +11,20 @@ namespace {
 
 constexpr int BLK = 256;
 
+// Cell of this thread, XCD-chunked: workgroups are dispatched to the eight
+// XCDs round-robin (workgroup b on XCD b % 8), each XCD with its own L2.
+// XCD x takes the x-th eighth of the cells, so the face and cell data that
+// neighbouring cells share -- those one mesh plane away included -- are
+// fetched into one L2 rather than into two.  Launch xcd_grid(m) workgroups.
+__device__ __forceinline__ int xcd_cell(int m) {
+  const int chunk = ((m + 8 * BLK - 1) / (8 * BLK)) * BLK;
+  return (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3) * BLK + (int)threadIdx.x;
+}
+int xcd_grid(int m) {
+  const int chunk = ((m + 8 * BLK - 1) / (8 * BLK)) * BLK;
+  return m > 0 ? 8 * (chunk / BLK) : 1;
+}
+
 struct GeomP {
   const double *xc, *yc, *zc, *vol, *arx, *ary, *arz, *xf, *yf, *zf, *facint, *Df;
 };
@@ -25,7 +39,7 @@ __global__ void k_laplacian(int n, int Fi, const int *__restrict__ cf_ptr,
                             const double *__restrict__ phi, const int *__restrict__ slot_pn,
                             const int *__restrict__ slot_np, const int *__restrict__ diag,
                             double *__restrict__ a, double *__restrict__ su) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_cell(n);
   if (c >= n) return;
   double ad = 0.0;  // a(diag) starts from the cleared matrix (:35)
   double s = su[c];
@@ -68,7 +82,7 @@ __global__ void k_nonorth(int n, int Fi, const int *__restrict__ cf_ptr, const i
                           const int *__restrict__ owner, const int *__restrict__ neighbour, GeomP g,
                           const double *__restrict__ gam, const double *__restrict__ gr,
                           double *__restrict__ su) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_cell(n);
   if (c >= n) return;
   double s = su[c];
   for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
@@ -95,7 +109,7 @@ __global__ void k_grad_gauss(int n, int Fi, int ntot, const int *__restrict__ cf
                              const int *__restrict__ cf, const int *__restrict__ owner,
                              const int *__restrict__ neighbour, GeomP g,
                              const double *__restrict__ u, double *__restrict__ dudxi) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_cell(ntot);
   if (c >= ntot) return;
   if (c >= n) {
     dudxi[3 * (size_t)c] = 0.0;
@@ -270,7 +284,7 @@ __global__ void k_convdiff(int n, int Fi, const int *__restrict__ cf_ptr, const 
                            double *__restrict__ sp, const int *__restrict__ srd_map, int cn,
                            const double *__restrict__ den, double dt,
                            const double *__restrict__ phi_o, PerP per) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = xcd_cell(n);
   if (c >= n) return;
   double ad = 0.0, s = su[c], spc = sp[c];
   // periodic pairs whose twin face c owns, in the order of their faces
@@ -490,7 +504,7 @@ int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
                          const unsigned char *bc_kind, const double *phi, double *su) {
   FCP_TRY(fcp_coarse_join(ctx));
   ProfScope ps(ctx, mu_f ? "laplacian_face" : "laplacian");
-  k_laplacian<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(
+  k_laplacian<<<xcd_grid(ctx->n), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, geom_of(ctx),
       mu, mu_f, bc_kind, phi, ctx->slot_pn.p, ctx->slot_np.p, ctx->diag.p, ctx->a.p, su);
   if (ctx->nper > 0)
@@ -503,7 +517,7 @@ int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
 
 int fcp_launch_nonorth(fcp_ctx *ctx, const double *gamma_f, const double *grad, double *su) {
   ProfScope ps(ctx, "nonorth");
-  k_nonorth<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(ctx->n, ctx->Fi, ctx->cf_ptr.p,
+  k_nonorth<<<xcd_grid(ctx->n), BLK, 0, ctx->stream>>>(ctx->n, ctx->Fi, ctx->cf_ptr.p,
                                                               ctx->cf_face.p, ctx->owner.p,
                                                               ctx->neighbour.p, geom_of(ctx),
                                                               gamma_f, grad, su);
@@ -514,7 +528,7 @@ int fcp_launch_nonorth(fcp_ctx *ctx, const double *gamma_f, const double *grad, 
 int fcp_launch_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi) {
   ProfScope ps(ctx, "grad_gauss");
   const int ntot = ctx->n + ctx->Fb;
-  k_grad_gauss<<<grid_for(ntot, BLK), BLK, 0, ctx->stream>>>(
+  k_grad_gauss<<<xcd_grid(ntot), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ntot, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p,
       geom_of(ctx), u, dudxi);
   FCP_HIP(ctx, hipGetLastError());
@@ -536,7 +550,7 @@ int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef,
                         const double *den, double dt, const double *phi_o) {
   FCP_TRY(fcp_coarse_join(ctx));
   ProfScope ps(ctx, "convdiff");
-  k_convdiff<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(
+  k_convdiff<<<xcd_grid(ctx->n), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, geom_of(ctx),
       ctx->srdb.p, flmass, dcoef, phi, grad, gam, scheme, bc_kind, urf, ctx->slot_pn.p,
       ctx->slot_np.p, ctx->ia.p, ctx->diag.p, ctx->a.p, su, sp, srd_map, cn, den, dt, phi_o,
