@@ -806,6 +806,26 @@ int fcp_grad(fcp_ctx *ctx, int method, const double *phi, double *dphidxi) {
   FCP_TRY(need(ctx, true, true, false));
   if (!phi || !dphidxi) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_grad: null argument");
   if (method == FCP_GRAD_GAUSS) return fcp_grad_gauss(ctx, phi, dphidxi);
+  if (method == FCP_GRAD_GAUSS_FVX || method == FCP_GRAD_GAUSS_CORRECTED) {
+    (void)hipSetDevice(ctx->device);
+    const size_t ntot = (size_t)ctx->n + ctx->Fb;
+    FCP_HIP(ctx, ctx->fvx_old.alloc(3 * ntot));
+    Arg<double> in, io;
+    FCP_TRY(arg_in(ctx, phi, ntot, in));
+    const double *dfo = nullptr;
+    if (method == FCP_GRAD_GAUSS_CORRECTED) {
+      FCP_TRY(arg_in(ctx, (const double *)dphidxi, 3 * ntot, io));
+      FCP_HIP(ctx, hipMemcpyAsync(ctx->fvx_old.p, io.dev, sizeof(double) * 3 * ctx->n,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+      dfo = ctx->fvx_old.p;
+    } else {
+      FCP_TRY(arg_out(ctx, dphidxi, 3 * ntot, io));
+    }
+    FCP_TRY(fcp_launch_grad_fvx(ctx, in.dev, dfo, ctx->fvx_old.p, io.dev));
+    FCP_TRY(arg_back(ctx, io));
+    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return 0;
+  }
   if (method != ctx->lsq_method)
     return fcp_fail(ctx, FCP_ERR_STATE, "fcp_grad: least-squares matrices of method %d not created",
                     method);

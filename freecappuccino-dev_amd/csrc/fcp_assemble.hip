@@ -146,6 +146,65 @@ __global__ void k_grad_gauss(int n, int Fi, int ntot, const int *__restrict__ cf
   dudxi[3 * (size_t)c + 2] = gz * volr;
 }
 
+// One sweep of fvxGradient.f90's Gauss gradient: the inner-face value is
+// corrected by the old gradient dfo (gradco :1761-1817; dfo == nullptr is
+// the zero gradient of grad_gauss's first sweep, :1589-1591), boundary faces
+// by gradbc (:1819-1838).  Each cell gathers its faces in ascending order,
+// which is the order of the reference's face loops for that cell.
+__global__ void k_grad_fvx(int n, int Fi, int ntot, const int *__restrict__ cf_ptr,
+                           const int *__restrict__ cf, const int *__restrict__ owner,
+                           const int *__restrict__ neighbour, GeomP g,
+                           const double *__restrict__ u, const double *__restrict__ dfo,
+                           double *__restrict__ dudxi) {
+  const int c = xcd_cell(ntot);
+  if (c >= ntot) return;
+  if (c >= n) {
+    dudxi[3 * (size_t)c] = 0.0;
+    dudxi[3 * (size_t)c + 1] = 0.0;
+    dudxi[3 * (size_t)c + 2] = 0.0;
+    return;
+  }
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+    const int f = cf[k];
+    if (f < Fi) {
+      const int p = owner[f], q = neighbour[f];
+      const double fxn = g.facint[f];
+      const double fxp = 1.0 - fxn;
+      const double xi = g.xc[p] * fxp + g.xc[q] * fxn;
+      const double yi = g.yc[p] * fxp + g.yc[q] * fxn;
+      const double zi = g.zc[p] * fxp + g.zc[q] * fxn;
+      double dfxi = 0.0, dfyi = 0.0, dfzi = 0.0;
+      if (dfo) {
+        dfxi = dfo[3 * (size_t)p] * fxp + dfo[3 * (size_t)q] * fxn;
+        dfyi = dfo[3 * (size_t)p + 1] * fxp + dfo[3 * (size_t)q + 1] * fxn;
+        dfzi = dfo[3 * (size_t)p + 2] * fxp + dfo[3 * (size_t)q + 2] * fxn;
+      }
+      const double fie = (((u[p] * fxp + u[q] * fxn) + dfxi * (g.xf[f] - xi)) +
+                          dfyi * (g.yf[f] - yi)) + dfzi * (g.zf[f] - zi);
+      const double dfxe = fie * g.arx[f], dfye = fie * g.ary[f], dfze = fie * g.arz[f];
+      if (c == p) {
+        gx = gx + dfxe;
+        gy = gy + dfye;
+        gz = gz + dfze;
+      } else {
+        gx = gx - dfxe;
+        gy = gy - dfye;
+        gz = gz - dfze;
+      }
+    } else {
+      const double fi = u[n + (f - Fi)];
+      gx = gx + fi * g.arx[f];
+      gy = gy + fi * g.ary[f];
+      gz = gz + fi * g.arz[f];
+    }
+  }
+  const double volr = 1.0 / g.vol[c];
+  dudxi[3 * (size_t)c] = gx * volr;
+  dudxi[3 * (size_t)c + 1] = gy * volr;
+  dudxi[3 * (size_t)c + 2] = gz * volr;
+}
+
 // updateBoundary.f90: zero-gradient copy (:31-44); a periodic face and its
 // twin take half the sum of their owners' values (:46-68) -- the twin unless
 // its own patch, coming later in the patch loop, copies its owner's value
@@ -531,6 +590,25 @@ int fcp_launch_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi) {
   k_grad_gauss<<<xcd_grid(ntot), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ntot, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p,
       geom_of(ctx), u, dudxi);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+int fcp_launch_grad_fvx(fcp_ctx *ctx, const double *u, const double *dfo, double *tmp,
+                        double *dudxi) {
+  ProfScope ps(ctx, "grad_fvx");
+  const int ntot = ctx->n + ctx->Fb;
+  auto sweep = [&](const double *old, double *out) {
+    k_grad_fvx<<<xcd_grid(ntot), BLK, 0, ctx->stream>>>(
+        ctx->n, ctx->Fi, ntot, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p,
+        geom_of(ctx), u, old, out);
+  };
+  if (dfo) {
+    sweep(dfo, dudxi);
+  } else {
+    sweep(nullptr, tmp);
+    sweep(tmp, dudxi);
+  }
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }

@@ -293,6 +293,7 @@ struct fcp_ctx {
 
   // -------- least-squares gradient matrices --------
   DBuf<double> lsq_dmat;  // Dmat(9,n) or D(3,6,n)
+  DBuf<double> fvx_old;   // [3*(n+Fb)] old gradient of the fvx Gauss sweeps
   int lsq_method = 0, lsq_compat = 1;
 
   // -------- scalar transport step (calculate_temperature) --------
@@ -430,6 +431,11 @@ int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
                          const unsigned char *bc_kind, const double *phi, double *su);
 int fcp_launch_nonorth(fcp_ctx *ctx, const double *gamma_f, const double *grad, double *su);
 int fcp_launch_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi);
+// fvxGradient.f90 Gauss gradient: dfo == nullptr runs grad_gauss's two
+// sweeps (the first into tmp[3*(n+Fb)]), else grad_gauss_corrected's one
+// sweep from dfo[3*n] (which must not alias dudxi).
+int fcp_launch_grad_fvx(fcp_ctx *ctx, const double *u, const double *dfo, double *tmp,
+                        double *dudxi);
 int fcp_launch_update_boundary(fcp_ctx *ctx, double *phi);
 int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
                         const double *grad, double gam, int scheme, const unsigned char *bc_kind,

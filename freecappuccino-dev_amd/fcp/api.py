@@ -27,6 +27,8 @@ FCP_GRAD_GAUSS = 0
 FCP_GRAD_LSQ = 1
 FCP_GRAD_LSQ_DM = 2
 FCP_GRAD_LSQ_QR = 3
+FCP_GRAD_GAUSS_FVX = 4
+FCP_GRAD_GAUSS_CORRECTED = 5
 FCP_TUNE_SWEEP_WAVES = 1
 FCP_TUNE_SWEEP_SLEEP = 2
 FCP_TUNE_SWEEP_KIND = 3
@@ -304,10 +306,17 @@ class Context:
         self._check(rc, "fcp_create_lsq_grad_matrix")
         return out
 
-    def grad(self, method: int, phi, out=None):
-        """grad(phi, dPhidxi): method 0 Gauss, 1-3 least squares."""
+    def grad(self, method: int, phi, out=None, gold=None):
+        """grad(phi, dPhidxi): method 0 Gauss, 1-3 least squares, 4 the
+        two-sweep Gauss of fvxGradient.f90 (:1549-1645), 5 its
+        grad_gauss_corrected (:1664-1758) started from gold (n, 3) -- or, with
+        gold None, from the cell rows of out on entry."""
         if out is None:
             out = np.zeros((self.mesh.n_total, 3))
+        if gold is not None:
+            n = self.mesh.n_cells
+            out[:n] = np.asarray(gold, np.float64).reshape(-1, 3) if isinstance(out, np.ndarray) \
+                else gold.reshape(-1, 3)
         rc = self.lib.fcp_grad(self.h, int(method), _ptr(_f64(phi))[0], _ptr(out)[0])
         self._check(rc, "fcp_grad")
         return out

@@ -16,7 +16,8 @@ module fcp_capi
   ! gradient methods (FCP_GRAD_* of the C header; Fortran names are case-blind,
   ! so FCP_GRAD_GAUSS would clash with fcp_grad_gauss)
   integer(c_int), parameter :: FCP_GM_GAUSS = 0, FCP_GM_LSQ = 1, FCP_GM_LSQ_DM = 2, &
-                               FCP_GM_LSQ_QR = 3
+                               FCP_GM_LSQ_QR = 3, FCP_GM_GAUSS_FVX = 4, &
+                               FCP_GM_GAUSS_CORRECTED = 5
 
   type, bind(C) :: fcp_solve_opts
     integer(c_int) :: precision

@@ -224,6 +224,39 @@ contains
     call fcp_check(fcp_grad(ctx, FCP_GM_LSQ_QR, Phi, dPhidxi), 'fcp_grad')
   end subroutine
 
+  ! fvxGradient.f90's grad_gauss (:1549-1645) and grad_gauss_corrected
+  ! (:1664-1758), with their separate component arrays of numCells; a caller
+  ! of that module renames them on use (use fcp_ref, only: grad_gauss =>
+  ! fvx_grad_gauss).  The corrected form starts from dUdx, dUdy, dUdz on entry.
+  subroutine fvx_grad_gauss(u, dUdx, dUdy, dUdz)
+    real(dp), intent(in) :: u(numTotal)
+    real(dp), intent(inout) :: dUdx(numCells), dUdy(numCells), dUdz(numCells)
+    call fvx_grad(FCP_GM_GAUSS_FVX, u, dUdx, dUdy, dUdz)
+  end subroutine
+
+  subroutine fvx_grad_gauss_corrected(u, dUdx, dUdy, dUdz)
+    real(dp), intent(in) :: u(numTotal)
+    real(dp), intent(inout) :: dUdx(numCells), dUdy(numCells), dUdz(numCells)
+    call fvx_grad(FCP_GM_GAUSS_CORRECTED, u, dUdx, dUdy, dUdz)
+  end subroutine
+
+  subroutine fvx_grad(method, u, dUdx, dUdy, dUdz)
+    integer(c_int), intent(in) :: method
+    real(dp), intent(in) :: u(numTotal)
+    real(dp), intent(inout) :: dUdx(numCells), dUdy(numCells), dUdz(numCells)
+    real(dp), allocatable :: g(:, :)
+    allocate(g(3, numTotal))
+    g = 0.0_dp
+    g(1, 1:numCells) = dUdx
+    g(2, 1:numCells) = dUdy
+    g(3, 1:numCells) = dUdz
+    call fcp_check(fcp_grad(ctx, method, u, g), 'fcp_grad')
+    dUdx = g(1, 1:numCells)
+    dUdy = g(2, 1:numCells)
+    dUdz = g(3, 1:numCells)
+    deallocate(g)
+  end subroutine
+
   subroutine updateBoundary(phi)
     real(dp), intent(inout) :: phi(numTotal)
     call fcp_check(fcp_update_boundary(ctx, phi), 'fcp_update_boundary')

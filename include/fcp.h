@@ -295,6 +295,10 @@ int fcp_rcm(fcp_ctx *ctx, int pp_iters, int *perm);
 #define FCP_GRAD_LSQ 1     /* lstsq    : grad_lsq    (:782-893)   */
 #define FCP_GRAD_LSQ_DM 2  /* lstsq_dm : grad_lsq_dm (:1334-1486) */
 #define FCP_GRAD_LSQ_QR 3  /* lstsq_qr : grad_lsq_qr (:1063-1152) */
+/* fvxGradient.f90's Gauss gradients, face values corrected by an old
+ * gradient (gradco :1761-1817): */
+#define FCP_GRAD_GAUSS_FVX 4        /* grad_gauss, two sweeps (:1549-1645)   */
+#define FCP_GRAD_GAUSS_CORRECTED 5  /* grad_gauss_corrected (:1664-1758)     */
 
 /* create_lsq_grad_matrix (gradients.f90:72-101): builds and keeps the
  * per-cell least-squares matrices of `method` (Dmat(9,numCells) for lsq and
@@ -305,8 +309,11 @@ int fcp_rcm(fcp_ctx *ctx, int pp_iters, int *perm);
 int fcp_create_lsq_grad_matrix(fcp_ctx *ctx, int method, int compat, double *dmat);
 
 /* grad(phi, dPhidxi) for a scalar field (gradients.f90:106-163, no limiter):
- * FCP_GRAD_GAUSS, or the least-squares method whose matrices were created.
- * dPhidxi[3*numTotal], boundary rows 0. */
+ * FCP_GRAD_GAUSS, or the least-squares method whose matrices were created;
+ * or fvxGradient.f90's FCP_GRAD_GAUSS_FVX or FCP_GRAD_GAUSS_CORRECTED, for
+ * which dPhidxi is intent(inout) as in the reference: its cell rows on entry
+ * are the old gradient the one corrected sweep starts from.
+ * dPhidxi[3*numTotal], boundary rows 0 on return. */
 int fcp_grad(fcp_ctx *ctx, int method, const double *phi, double *dphidxi);
 
 /* updateBoundary (updateBoundary.f90:1-74): zero-gradient copy for

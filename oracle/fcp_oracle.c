@@ -403,6 +403,68 @@ void orc_grad_gauss(const orc_mesh *m, const orc_geom *g, const double *u,
   }
 }
 
+/* One Gauss sweep of fvxGradient.f90 with face values corrected by the old
+ * gradient dfo[3*n] (gradco :1761-1817), boundary faces by gradbc
+ * (:1819-1838), then the 1/vol scaling (:1623-1631 / :1746-1754). */
+static void fvx_sweep(const orc_mesh *m, const orc_geom *g, const double *u,
+                      const double *dfo, double *out) {
+  const int n = m->n_cells, Fi = m->n_inner, F = m->n_faces;
+  memset(out, 0, sizeof(double) * 3 * (size_t)(n + F - Fi));
+  for (int f = 0; f < Fi; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    const double fxn = g->facint[f];
+    const double fxp = 1.0 - fxn;
+    const double xi = g->xc[p] * fxp + g->xc[q] * fxn;
+    const double yi = g->yc[p] * fxp + g->yc[q] * fxn;
+    const double zi = g->zc[p] * fxp + g->zc[q] * fxn;
+    const double dfxi = dfo[3 * p] * fxp + dfo[3 * q] * fxn;
+    const double dfyi = dfo[3 * p + 1] * fxp + dfo[3 * q + 1] * fxn;
+    const double dfzi = dfo[3 * p + 2] * fxp + dfo[3 * q + 2] * fxn;
+    const double fie = (((u[p] * fxp + u[q] * fxn) + dfxi * (g->xf[f] - xi)) +
+                        dfyi * (g->yf[f] - yi)) + dfzi * (g->zf[f] - zi);
+    const double dfxe = fie * g->arx[f], dfye = fie * g->ary[f], dfze = fie * g->arz[f];
+    out[3 * p] = out[3 * p] + dfxe;
+    out[3 * p + 1] = out[3 * p + 1] + dfye;
+    out[3 * p + 2] = out[3 * p + 2] + dfze;
+    out[3 * q] = out[3 * q] - dfxe;
+    out[3 * q + 1] = out[3 * q + 1] - dfye;
+    out[3 * q + 2] = out[3 * q + 2] - dfze;
+  }
+  for (int f = Fi; f < F; ++f) {
+    const int p = m->owner[f];
+    const double fi = u[n + (f - Fi)];
+    out[3 * p] = out[3 * p] + fi * g->arx[f];
+    out[3 * p + 1] = out[3 * p + 1] + fi * g->ary[f];
+    out[3 * p + 2] = out[3 * p + 2] + fi * g->arz[f];
+  }
+  for (int c = 0; c < n; ++c) {
+    const double volr = 1.0 / g->vol[c];
+    out[3 * c] = out[3 * c] * volr;
+    out[3 * c + 1] = out[3 * c + 1] * volr;
+    out[3 * c + 2] = out[3 * c + 2] * volr;
+  }
+}
+
+int orc_grad_gauss_fvx(const orc_mesh *m, const orc_geom *g, const double *u,
+                       const double *gold, double *dudxi) {
+  const int n = m->n_cells;
+  double *dfo = (double *)calloc(3 * (size_t)n + 3, sizeof(double));
+  if (!dfo) return -1;
+  if (gold) {
+    /* grad_gauss_corrected :1700-1703: old gradient = the given one */
+    memcpy(dfo, gold, sizeof(double) * 3 * (size_t)n);
+    fvx_sweep(m, g, u, dfo, dudxi);
+  } else {
+    /* grad_gauss :1593-1641: lc = 1 from a zero old gradient, lc = 2 from
+     * the first sweep's result */
+    fvx_sweep(m, g, u, dfo, dudxi);
+    memcpy(dfo, dudxi, sizeof(double) * 3 * (size_t)n);
+    fvx_sweep(m, g, u, dfo, dudxi);
+  }
+  free(dfo);
+  return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Least-squares gradients                                                  */
 /* ------------------------------------------------------------------------ */

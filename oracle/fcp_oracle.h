@@ -96,6 +96,13 @@ void orc_laplacian_face(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
 void orc_grad_gauss(const orc_mesh *m, const orc_geom *g, const double *u,
                     double *dudxi);
 
+/* fvxGradient.f90 Gauss gradients with face values corrected by an old
+ * gradient (gradco :1761-1817).  gold == NULL: grad_gauss (:1549-1645), two
+ * sweeps from a zero old gradient; else grad_gauss_corrected (:1664-1758),
+ * one sweep from gold[3*n].  dudxi[3*numTotal], boundary rows 0. */
+int orc_grad_gauss_fvx(const orc_mesh *m, const orc_geom *g, const double *u,
+                       const double *gold, double *dudxi);
+
 /* LSQ gradients: variant 0 = lsq (gradients.f90:660-893), 1 = lsq_dm
  * (:1157-1486), 2 = lsq_qr with modified Gram-Schmidt (:900-1152).
  * compat=1 reproduces the reference arithmetic including its defects

@@ -145,6 +145,18 @@ class Oracle:
                              _p(np.ascontiguousarray(u, np.float64)), _p(out))
         return out.reshape(-1, 3)
 
+    def grad_gauss_fvx(self, u, gold=None):
+        """fvxGradient.f90 grad_gauss (:1549-1645, two sweeps) when gold is
+        None, else grad_gauss_corrected (:1664-1758) from gold (n, 3)."""
+        m = self.mesh
+        out = np.zeros(3 * m.n_total)
+        g = None if gold is None else np.ascontiguousarray(gold, np.float64).reshape(-1)
+        rc = lib().orc_grad_gauss_fvx(C.byref(self.cm), C.byref(self.cg),
+                                      _p(np.ascontiguousarray(u, np.float64)),
+                                      None if g is None else _p(g), _p(out))
+        assert rc == 0, rc
+        return out.reshape(-1, 3)
+
     def lsq_matrix(self, variant: int, compat: int = 1):
         n = self.mesh.n_cells
         dmat = np.zeros(n * (18 if variant == 2 else 9))

@@ -12,6 +12,10 @@
 !   laplacian  mu phi su0 a_out su_out        (laplacian.f90)
 !   grad_gauss phi out                        (gradients.f90:1607)
 !   grad_lsq   variant phi out                variant: lsq | lsq_dm | lsq_qr
+!   grad_fvx   phi gin out2 outc              fvxGradient.f90: the two-sweep
+!              corrected Gauss grad_gauss (:1549-1645) into out2, and
+!              grad_gauss_corrected (:1664-1758) started from gin into outc
+!              (gradients AoS (3,numCells))
 !   solve      solver itr_max tol_abs tol_rel a fi0 rhs fi_out info_out
 !   convdiff   flmass dcoef phi grad gam scheme bckind a_out su_out sp_out
 !   nonorth    gamma grad su0 su_out         (facefluxsc with zero mass flux:
@@ -33,6 +37,7 @@ program ref_driver
   use geometry
   use sparse_matrix
   use gradients
+  use fvxGradient, only: fvx_gauss => grad_gauss, fvx_gauss_corrected => grad_gauss_corrected
   use linear_solvers
   use scalar_fluxes
   implicit none
@@ -74,6 +79,8 @@ program ref_driver
       call job_grad_gauss(ju)
     case ('grad_lsq')
       call job_grad_lsq(ju)
+    case ('grad_fvx')
+      call job_grad_fvx(ju)
     case ('solve')
       call job_solve(ju)
     case ('convdiff')
@@ -168,6 +175,37 @@ contains
     open(newunit=u, file='psizes.txt', status='replace')
     write(u,*) numPeriodic, nper
     close(u)
+  end subroutine
+
+  subroutine job_grad_fvx(ju)
+    integer, intent(in) :: ju
+    character(len=64) :: c, fphi, fgin, fo2, foc
+    real(dp), allocatable :: phi(:), g(:), dx(:), dy(:), dz(:)
+    integer :: i
+    read(ju, *) c, fphi, fgin, fo2, foc
+    allocate(phi(numTotal), g(3*numCells), dx(numCells), dy(numCells), dz(numCells))
+    call rd8(fphi, phi, numTotal)
+    call fvx_gauss(phi, dx, dy, dz)
+    do i = 1, numCells
+      g(3*i-2) = dx(i)
+      g(3*i-1) = dy(i)
+      g(3*i) = dz(i)
+    end do
+    call wr8(fo2, g, 3*numCells)
+    call rd8(fgin, g, 3*numCells)
+    do i = 1, numCells
+      dx(i) = g(3*i-2)
+      dy(i) = g(3*i-1)
+      dz(i) = g(3*i)
+    end do
+    call fvx_gauss_corrected(phi, dx, dy, dz)
+    do i = 1, numCells
+      g(3*i-2) = dx(i)
+      g(3*i-1) = dy(i)
+      g(3*i) = dz(i)
+    end do
+    call wr8(foc, g, 3*numCells)
+    deallocate(phi, g, dx, dy, dz)
   end subroutine
 
   subroutine job_laplacian(ju)

@@ -181,6 +181,34 @@ def transport_case():
     print("wrote transport")
 
 
+def grad_fvx_case():
+    """fvxGradient.f90's Gauss gradients on a jittered 5^3 hex and a jittered
+    3^3 tet mesh: the two-sweep corrected grad_gauss (:1549-1645, face
+    values through gradco :1761-1817 with the previous sweep's gradient) and
+    grad_gauss_corrected (:1664-1758) started from a given gradient."""
+    from fcp.mesh import tet_mesh
+    out = {}
+    meshes = {"hex": hex_mesh(5, jitter=0.2, seed=7,
+                              types=("inlet", "outlet", "wall", "symmetry", "pressure", "empty")),
+              "tet": tet_mesh(3, jitter=0.2, seed=3)}
+    for key, mesh in meshes.items():
+        rc = RefCase(mesh)
+        rc.dump()
+        n, nt = mesh.n_cells, mesh.n_total
+        phi = uniform(15, 1, nt, -1.0, 1.0)
+        gin = uniform(15, 2, 3 * n, -1.0, 1.0)
+        rc.put("phi.bin", phi)
+        rc.put("gin.bin", gin)
+        rc.run(["grad_fvx phi.bin gin.bin g2.bin gc.bin"])
+        out[f"{key}_in_phi"] = phi
+        out[f"{key}_in_gin"] = gin.reshape(-1, 3)
+        out[f"{key}_gauss2"] = rc.get("g2.bin").reshape(-1, 3)
+        out[f"{key}_corrected"] = rc.get("gc.bin").reshape(-1, 3)
+        rc.close()
+    np.savez_compressed(os.path.join(HERE, "gradfvx.npz"), **out)
+    print("wrote gradfvx")
+
+
 PERIODIC_TYPES = ("empty", "periodic", "wall", "inlet", "empty", "periodic")
 PERIODIC_WF = (0, 0, 1, 0, 0, 0)
 # (time scheme, convection scheme, solver, itr_max, tol_rel, gradient, urf, gds)
@@ -454,6 +482,7 @@ CASES = {
                  types=("wall", "symmetry", "outlet", "wall", "inlet", "pressure")), "hex5j"),
     "hexbc": bc_case,
     "periodic": periodic_case,
+    "gradfvx": grad_fvx_case,
     "transport": transport_case,
     "cavity": cavity_case,
     "kat5x5": kat_5x5_case,
