@@ -642,6 +642,20 @@ int fcp_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double
   return 0;
 }
 
+int fcp_convdiff_cell(fcp_ctx *ctx, const double *flmass, const double *diffcoef, double *phi,
+                      const double *grad, double gam, int scheme, const unsigned char *bc_kind,
+                      double urf, double *a, double *su, double *sp) {
+  FCP_TRY(need(ctx, true, true, true));
+  if (!diffcoef) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_convdiff_cell: bad argument");
+  (void)hipSetDevice(ctx->device);
+  Arg<double> dc;
+  FCP_TRY(arg_in(ctx, diffcoef, ctx->n, dc));
+  FCP_HIP(ctx, ctx->cd_gamf.alloc(ctx->F > 0 ? ctx->F : 1));
+  FCP_TRY(fcp_launch_face_gamma_cells(ctx, dc.dev, ctx->cd_gamf.p));
+  return fcp_convdiff(ctx, flmass, ctx->cd_gamf.p, phi, grad, gam, scheme, bc_kind, urf, a, su,
+                      sp);
+}
+
 int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt, const double *phi_o,
             const double *phi_oo, double *su, double *sp) {
   FCP_TRY(need(ctx, true, true, false));

@@ -205,6 +205,24 @@ __global__ void k_grad_fvx(int n, int Fi, int ntot, const int *__restrict__ cf_p
   dudxi[3 * (size_t)c + 2] = gz * volr;
 }
 
+// facefluxscalar's face diffusivity (scalar_fluxes.f90:384-392): an inner
+// face interpolates the cell array, game = dc(P)*(1-lambda) + dc(N)*lambda;
+// a boundary face takes its owner's value.
+__global__ void k_face_gamma_cells(int Fi, int F, const int *__restrict__ owner,
+                                   const int *__restrict__ neighbour,
+                                   const double *__restrict__ facint,
+                                   const double *__restrict__ dc, double *__restrict__ gf) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  if (f < Fi) {
+    const double fxn = facint[f];
+    const double fxp = 1.0 - fxn;
+    gf[f] = dc[owner[f]] * fxp + dc[neighbour[f]] * fxn;
+  } else {
+    gf[f] = dc[owner[f]];
+  }
+}
+
 // updateBoundary.f90: zero-gradient copy (:31-44); a periodic face and its
 // twin take half the sum of their owners' values (:46-68) -- the twin unless
 // its own patch, coming later in the patch loop, copies its owner's value
@@ -609,6 +627,15 @@ int fcp_launch_grad_fvx(fcp_ctx *ctx, const double *u, const double *dfo, double
     sweep(nullptr, tmp);
     sweep(tmp, dudxi);
   }
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+int fcp_launch_face_gamma_cells(fcp_ctx *ctx, const double *dc, double *gf) {
+  const int F = ctx->F;
+  if (F == 0) return 0;
+  k_face_gamma_cells<<<grid_for(F, BLK), BLK, 0, ctx->stream>>>(
+      ctx->Fi, F, ctx->owner.p, ctx->neighbour.p, ctx->facint.p, dc, gf);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }

@@ -294,6 +294,7 @@ struct fcp_ctx {
   // -------- least-squares gradient matrices --------
   DBuf<double> lsq_dmat;  // Dmat(9,n) or D(3,6,n)
   DBuf<double> fvx_old;   // [3*(n+Fb)] old gradient of the fvx Gauss sweeps
+  DBuf<double> cd_gamf;   // [F] face diffusivity of fcp_convdiff_cell
   int lsq_method = 0, lsq_compat = 1;
 
   // -------- scalar transport step (calculate_temperature) --------
@@ -437,6 +438,8 @@ int fcp_launch_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi);
 int fcp_launch_grad_fvx(fcp_ctx *ctx, const double *u, const double *dfo, double *tmp,
                         double *dudxi);
 int fcp_launch_update_boundary(fcp_ctx *ctx, double *phi);
+// facefluxscalar's face diffusivity gf[F] from the cell array dc[n].
+int fcp_launch_face_gamma_cells(fcp_ctx *ctx, const double *dc, double *gf);
 int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
                         const double *grad, double gam, int scheme, const unsigned char *bc_kind,
                         double urf, double *su, double *sp, const int *srd_map = nullptr,

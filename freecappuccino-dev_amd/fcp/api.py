@@ -218,6 +218,20 @@ class Context:
         self._check(rc, "fcp_convdiff")
         return a_out
 
+    def convdiff_cell(self, flmass, diffcoef, phi, grad, gam, scheme, bc_kind, su, sp, urf=1.0,
+                      a_out=None):
+        """convdiff with facefluxscalar (scalar_fluxes.f90:344-464): the
+        diffusivity per cell, interpolated to each inner face; a boundary face
+        takes its owner's."""
+        if isinstance(bc_kind, np.ndarray):
+            bc_kind = np.ascontiguousarray(bc_kind, dtype=np.uint8)
+        rc = self.lib.fcp_convdiff_cell(self.h, _ptr(_f64(flmass))[0], _ptr(_f64(diffcoef))[0],
+                                        _ptr(phi)[0], _ptr(_f64(grad))[0], float(gam), int(scheme),
+                                        _ptr(bc_kind)[0], float(urf), _ptr(a_out)[0],
+                                        _ptr(su)[0], _ptr(sp)[0])
+        self._check(rc, "fcp_convdiff_cell")
+        return a_out
+
     def ddt(self, scheme, den, dt, phi_o, su, sp, phi_oo=None):
         rc = self.lib.fcp_ddt(self.h, int(scheme), _ptr(_f64(den))[0], float(dt),
                               _ptr(_f64(phi_o))[0], _ptr(_f64(phi_oo))[0], _ptr(su)[0], _ptr(sp)[0])

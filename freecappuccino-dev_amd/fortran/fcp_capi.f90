@@ -145,6 +145,17 @@ module fcp_capi
       type(c_ptr), value :: a  ! c_null_ptr or c_loc of a(nnz)
     end function
 
+    integer(c_int) function fcp_convdiff_cell(ctx, flmass, diffcoef, phi, grad, gam, scheme, &
+        bc_kind, urf, a, su, sp) bind(C, name='fcp_convdiff_cell')
+      import :: c_int, c_double, c_ptr, c_signed_char
+      type(c_ptr), value :: ctx
+      real(c_double) :: flmass(*), diffcoef(*), phi(*), grad(*), su(*), sp(*)
+      real(c_double), value :: gam, urf
+      integer(c_int), value :: scheme
+      integer(c_signed_char) :: bc_kind(*)
+      type(c_ptr), value :: a  ! c_null_ptr or c_loc of a(nnz)
+    end function
+
     integer(c_int) function fcp_ddt(ctx, scheme, den, dt, phi_o, phi_oo, su, sp) &
         bind(C, name='fcp_ddt')
       import :: c_int, c_double, c_ptr

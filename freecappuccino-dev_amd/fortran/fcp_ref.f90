@@ -285,6 +285,20 @@ contains
                                 c_loc(a), su, sp), 'fcp_convdiff')
   end subroutine
 
+  ! fvm_div with facefluxscalar on the inner faces (scalar_fluxes.f90:
+  ! 344-464): the diffusivity per cell; boundary faces take the owner's.
+  subroutine fvm_div_cell(flmass, diffcoef, phi, dphidxi, gam, cScheme, bc_kind, urf)
+    real(dp), intent(in) :: flmass(numFaces), diffcoef(numCells), dphidxi(3, numTotal), gam, urf
+    real(dp), intent(inout) :: phi(numTotal)
+    character(len=*), intent(in) :: cScheme
+    integer(c_signed_char), intent(in) :: bc_kind(numBoundaryFaces)
+    integer(c_int) :: sch
+    sch = 1
+    if (trim(cScheme) == 'cds') sch = 0
+    call fcp_check(fcp_convdiff_cell(ctx, flmass, diffcoef, phi, dphidxi, gam, sch, bc_kind, &
+                                     urf, c_loc(a), su, sp), 'fcp_convdiff_cell')
+  end subroutine
+
   ! Time terms of temperature.f90:82-102 into su, sp (scheme 1 BDF1 / the
   ! volume part of Crank-Nicolson, 2 BDF2).
   subroutine fvm_ddt(scheme, den, dt, phi_o, phi_oo)

@@ -206,6 +206,17 @@ int fcp_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double
                  const unsigned char *bc_kind, double urf, double *a, double *su,
                  double *sp);
 
+/* fcp_convdiff with facefluxscalar (scalar_fluxes.f90:344-464) on the inner
+ * faces: the diffusivity diffcoef[numCells] is given per cell and each inner
+ * face takes game = diffcoef(P)*(1-lambda) + diffcoef(N)*lambda (:384-392),
+ * used as facefluxsc uses its dcoef.  The reference has no boundary form of
+ * facefluxscalar; here a boundary face (periodic ones included) takes
+ * diffcoef of its owner.  Other arguments and outputs as fcp_convdiff. */
+int fcp_convdiff_cell(fcp_ctx *ctx, const double *flmass, const double *diffcoef, double *phi,
+                      const double *grad, double gam, int scheme,
+                      const unsigned char *bc_kind, double urf, double *a, double *su,
+                      double *sp);
+
 /* Time terms (temperature.f90:82-102): scheme 1 BDF1 (and the volume part of
  * Crank-Nicolson), 2 BDF2; apotime = den*vol/dt; su/sp updated in place. */
 int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt,

@@ -1359,6 +1359,17 @@ int orc_transport_assemble(const orc_mesh *m, const orc_geom *g, const orc_csr *
   return 0;
 }
 
+void orc_face_gamma_cells(const orc_mesh *m, const orc_geom *g, const double *diffcoef,
+                          double *dcoef) {
+  const int Fi = m->n_inner, F = m->n_faces;
+  for (int f = 0; f < Fi; ++f) {
+    const double fxn = g->facint[f];
+    const double fxp = 1.0 - fxn;
+    dcoef[f] = diffcoef[m->owner[f]] * fxp + diffcoef[m->neighbour[f]] * fxn;
+  }
+  for (int f = Fi; f < F; ++f) dcoef[f] = diffcoef[m->owner[f]];
+}
+
 void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
                   const int *slot_pn, const int *slot_np, const double *flmass,
                   const double *dcoef, double *phi, const double *grad, double gam,

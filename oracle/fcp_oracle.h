@@ -207,6 +207,12 @@ int orc_transport_assemble(const orc_mesh *m, const orc_geom *g, const orc_csr *
 void orc_transport_dcoef(const orc_mesh *m, const orc_geom *g, const double *vis,
                          const unsigned char *bc_kind, double viscos, double pranl, double sigt,
                          double *dcoef);
+/* facefluxscalar's face diffusivity (scalar_fluxes.f90:384-392): inner face
+ * game = diffcoef(P)*(1-lambda) + diffcoef(N)*lambda; a boundary face takes
+ * diffcoef of its owner.  With it, orc_convdiff is facefluxscalar's
+ * assembly: facefluxscalar is facefluxsc with dcoef = game. */
+void orc_face_gamma_cells(const orc_mesh *m, const orc_geom *g, const double *diffcoef,
+                          double *dcoef);
 void orc_convdiff(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
                   const int *slot_pn, const int *slot_np,
                   const double *flmass, const double *dcoef,

@@ -201,6 +201,19 @@ class Oracle:
                            _p(a), _p(su), _p(sp))
         return a, su, sp, phi
 
+    def face_gamma_cells(self, diffcoef):
+        """facefluxscalar's face diffusivity from a cell array
+        (scalar_fluxes.f90:384-392); boundary faces take the owner's."""
+        d = np.zeros(self.mesh.n_faces)
+        lib().orc_face_gamma_cells(C.byref(self.cm), C.byref(self.cg),
+                                   _p(np.ascontiguousarray(diffcoef, np.float64)), _p(d))
+        return d
+
+    def convdiff_cell(self, flmass, diffcoef, phi, grad, gam, scheme, bc_kind, urf=1.0):
+        """The convdiff assembly with facefluxscalar on the inner faces."""
+        return self.convdiff(flmass, self.face_gamma_cells(diffcoef), phi, grad, gam, scheme,
+                             bc_kind, urf)
+
     def transport_dcoef(self, vis, bc_kind, viscos, pranl, sigt):
         d = np.zeros(self.mesh.n_faces)
         lib().orc_transport_dcoef(C.byref(self.cm), C.byref(self.cg),

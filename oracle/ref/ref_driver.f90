@@ -18,6 +18,10 @@
 !              (gradients AoS (3,numCells))
 !   solve      solver itr_max tol_abs tol_rel a fi0 rhs fi_out info_out
 !   convdiff   flmass dcoef phi grad gam scheme bckind a_out su_out sp_out
+!   convdiff_cell flmass vis phi grad gam scheme bckind a_out su_out sp_out
+!              (as convdiff with facefluxscalar, scalar_fluxes.f90:344-464, on
+!               the inner faces: the cell array vis(numCells) interpolated to
+!               the face; boundary faces take vis of their owner)
 !   nonorth    gamma grad su0 su_out         (facefluxsc with zero mass flux:
 !              the explicit non-orthogonal part only, scalar_fluxes.f90:99-109,139)
 !   update_boundary phi phi_out               (updateBoundary.f90)
@@ -84,7 +88,9 @@ program ref_driver
     case ('solve')
       call job_solve(ju)
     case ('convdiff')
-      call job_convdiff(ju)
+      call job_convdiff(ju, .false.)
+    case ('convdiff_cell')
+      call job_convdiff(ju, .true.)
     case ('nonorth')
       call job_nonorth(ju)
     case ('update_boundary')
@@ -350,18 +356,27 @@ contains
   ! Steady convection-diffusion assembly with the reference flux routines,
   ! composed as calculate_temperature does (temperature.f90:118-366).
   ! bckind per boundary face: 0 Dirichlet wall, 1 zero-gradient, 2 flux bc.
-  subroutine job_convdiff(ju)
+  subroutine job_convdiff(ju, cellgam)
     integer, intent(in) :: ju
+    logical, intent(in) :: cellgam
     character(len=64) :: c, ffm, fdc, fphi, fgr, fbk, fa, fsu, fsp
     character(len=30) :: scheme
     real(dp) :: gam, cap, can, suadd, dw, off
-    real(dp), allocatable :: fm(:), dc(:), phi(:), gr(:,:), bk(:), spl(:)
+    real(dp), allocatable :: fm(:), dc(:), phi(:), gr(:,:), bk(:), spl(:), vis(:)
     integer :: i, ijp, ijn, k, iface, ijb, inp
     read(ju, *) c, ffm, fdc, fphi, fgr, gam, scheme, fbk, fa, fsu, fsp
     allocate(fm(numFaces), dc(numFaces), phi(numTotal), gr(3,numTotal), bk(numBoundaryFaces), &
-             spl(numCells))
+             spl(numCells), vis(numCells))
     call rd8(ffm, fm, numFaces)
-    call rd8(fdc, dc, numFaces)
+    if (cellgam) then
+      call rd8(fdc, vis, numCells)
+      dc = 0.0_dp
+      do iface = numInnerFaces + 1, numFaces
+        dc(iface) = vis(owner(iface))
+      enddo
+    else
+      call rd8(fdc, dc, numFaces)
+    endif
     call rd8(fphi, phi, numTotal)
     call rd8(fgr, gr, 3*numTotal)
     call rd8(fbk, bk, numBoundaryFaces)
@@ -369,8 +384,13 @@ contains
     do i = 1, numInnerFaces
       ijp = owner(i)
       ijn = neighbour(i)
-      call facefluxsc(i, ijp, ijn, xf(i), yf(i), zf(i), arx(i), ary(i), arz(i), &
-                      fm(i), facint(i), gam, scheme, phi, gr, dc(i), cap, can, suadd)
+      if (cellgam) then
+        call facefluxscalar(i, ijp, ijn, xf(i), yf(i), zf(i), arx(i), ary(i), arz(i), &
+                            fm(i), facint(i), gam, scheme, phi, gr, vis, cap, can, suadd)
+      else
+        call facefluxsc(i, ijp, ijn, xf(i), yf(i), zf(i), arx(i), ary(i), arz(i), &
+                        fm(i), facint(i), gam, scheme, phi, gr, dc(i), cap, can, suadd)
+      endif
       k = icell_jcell_csr_index(i); a(k) = can
       k = jcell_icell_csr_index(i); a(k) = cap
       k = diag(ijp); a(k) = a(k) - can
@@ -405,7 +425,7 @@ contains
     call wr8(fa, a, nnz)
     call wr8(fsu, su, numCells)
     call wr8(fsp, spl, numCells)
-    deallocate(fm, dc, phi, gr, bk, spl)
+    deallocate(fm, dc, phi, gr, bk, spl, vis)
   end subroutine
 
   ! The explicit non-orthogonal correction of the inner faces, taken from

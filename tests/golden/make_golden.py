@@ -209,6 +209,40 @@ def grad_fvx_case():
     print("wrote gradfvx")
 
 
+def flux_scalar_case():
+    """facefluxscalar (scalar_fluxes.f90:344-464): the convection-diffusion
+    assembly of the convdiff job with the diffusivity given per cell and
+    interpolated to each inner face, on a jittered 5^3 hex (every boundary
+    kind) and a jittered 3^3 tet mesh, under both face-value schemes."""
+    from fcp.mesh import tet_mesh
+    out = {}
+    meshes = {"hex": hex_mesh(5, jitter=0.2, seed=7,
+                              types=("inlet", "outlet", "wall", "symmetry", "pressure", "empty")),
+              "tet": tet_mesh(3, jitter=0.2, seed=3)}
+    for key, mesh in meshes.items():
+        rc = RefCase(mesh)
+        rc.dump()
+        n, nt, F, Fi = mesh.n_cells, mesh.n_total, mesh.n_faces, mesh.n_inner
+        flm = uniform(16, 1, F, -1.0, 1.0)
+        vis = 10.0 ** uniform(16, 2, n, -1.0, 1.0)
+        phi = uniform(16, 3, nt, -1.0, 1.0)
+        gr = uniform(16, 4, 3 * nt, -1.0, 1.0)
+        bk = (np.arange(F - Fi) % 3).astype(np.float64)
+        for k, v in (("flm", flm), ("vis", vis), ("phi", phi), ("gr", gr), ("bk", bk)):
+            rc.put(k + ".bin", v)
+        out.update({f"{key}_in_flmass": flm, f"{key}_in_vis": vis, f"{key}_in_phi": phi,
+                    f"{key}_in_grad": gr.reshape(-1, 3), f"{key}_in_bckind": bk.astype(np.uint8)})
+        for scheme in ("cds", "linearUpwind"):
+            rc.run([f"convdiff_cell flm.bin vis.bin phi.bin gr.bin 0.7d0 {scheme} bk.bin "
+                    "ca.bin csu.bin csp.bin"])
+            out[f"{key}_{scheme}_a"] = rc.get("ca.bin")
+            out[f"{key}_{scheme}_su"] = rc.get("csu.bin")
+            out[f"{key}_{scheme}_sp"] = rc.get("csp.bin")
+        rc.close()
+    np.savez_compressed(os.path.join(HERE, "fluxscalar.npz"), **out)
+    print("wrote fluxscalar")
+
+
 PERIODIC_TYPES = ("empty", "periodic", "wall", "inlet", "empty", "periodic")
 PERIODIC_WF = (0, 0, 1, 0, 0, 0)
 # (time scheme, convection scheme, solver, itr_max, tol_rel, gradient, urf, gds)
@@ -483,6 +517,7 @@ CASES = {
     "hexbc": bc_case,
     "periodic": periodic_case,
     "gradfvx": grad_fvx_case,
+    "fluxscalar": flux_scalar_case,
     "transport": transport_case,
     "cavity": cavity_case,
     "kat5x5": kat_5x5_case,
