@@ -1534,3 +1534,237 @@ int orc_rcm(const orc_csr *c, int pp_iters, int *new_of_old) {
   free(kids);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Pressure correction of SIMPLE: calcp_simple.f90:69-419 (incompressible,  */
+/* no pressure patches, one corrector, pscheme 'linear')                    */
+/* ------------------------------------------------------------------------ */
+/* Rhie-Chow mass flux and pressure-correction coefficient of an inner face:
+ * facefluxmass2, faceflux_mass.f90:175-252. */
+static double rc_inner(const orc_geom *g, int f, int p, int q, const double *u, const double *v,
+                       const double *w, const double *pr, const double *dp, const double *den,
+                       const double *apu, double *fm) {
+  const double lam = g->facint[f];
+  const double fxn = lam, fxp = 1.0 - lam;
+  const double xpn = g->xc[q] - g->xc[p], ypn = g->yc[q] - g->yc[p], zpn = g->zc[q] - g->zc[p];
+  const double dene = den[p] * fxp + den[q] * fxn;
+  const double Kj = g->vol[p] * apu[p] * fxp + g->vol[q] * apu[q] * fxn;
+  const double cap = -(dene * Kj * g->Df[f]);
+  const double ui = u[p] + (u[q] - u[p]) * lam; /* face_value_cds, interpolation.f90:116-157 */
+  const double vi = v[p] + (v[q] - v[p]) * lam;
+  const double wi = w[p] + (w[q] - w[p]) * lam;
+  const double dpxi = (dp[3 * q] * fxp + dp[3 * p] * fxn) * xpn;
+  const double dpyi = (dp[3 * q + 1] * fxp + dp[3 * p + 1] * fxn) * ypn;
+  const double dpzi = (dp[3 * q + 2] * fxp + dp[3 * p + 2] * fxn) * zpn;
+  *fm = dene * ((ui * g->arx[f] + vi * g->ary[f]) + wi * g->arz[f]) +
+        cap * ((((pr[q] - pr[p]) - dpxi) - dpyi) - dpzi);
+  return cap;
+}
+
+/* facefluxmass2_periodic, faceflux_mass.f90:313-372: face f of a periodic
+ * patch (patch-local index i, whose Df(i) -- an inner face's -- it reads),
+ * p its owner, q the twin face's owner, lambda = 1/2, velocities weighted by
+ * the other cell's 1/a_P. */
+static double rc_periodic(const orc_geom *g, int f, int i, int p, int q, const double *u,
+                          const double *v, const double *w, const double *pr, const double *dp,
+                          const double *den, const double *apu, const double *apv,
+                          const double *apw, double *fm) {
+  const double fxn = 0.5, fxp = 1.0 - 0.5;
+  const double xpn = 2.0 * (g->xf[f] - g->xc[p]), ypn = 2.0 * (g->yf[f] - g->yc[p]),
+               zpn = 2.0 * (g->zf[f] - g->zc[p]);
+  const double dene = den[p] * fxp + den[q] * fxn;
+  double Kj = g->vol[p] * apu[p] * fxp + g->vol[q] * apu[q] * fxn;
+  const double cap = -(dene * Kj * g->Df[i]);
+  Kj = (apu[p] + apu[q]) + SMALL;
+  const double ui = (u[p] * apu[q] + u[q] * apu[p]) / Kj;
+  const double vi = (v[p] * apv[q] + v[q] * apv[p]) / Kj;
+  const double wi = (w[p] * apw[q] + w[q] * apw[p]) / Kj;
+  const double dpxi = (dp[3 * q] * fxp + dp[3 * p] * fxn) * xpn;
+  const double dpyi = (dp[3 * q + 1] * fxp + dp[3 * p + 1] * fxn) * ypn;
+  const double dpzi = (dp[3 * q + 2] * fxp + dp[3 * p + 2] * fxn) * zpn;
+  *fm = dene * ((ui * g->arx[f] + vi * g->ary[f]) + wi * g->arz[f]) +
+        cap * ((((pr[q] - pr[p]) - dpxi) - dpyi) - dpzi);
+  return cap;
+}
+
+int orc_pcorr_assemble(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                       const int *slot_pn, const int *slot_np, double *u, double *v, double *w,
+                       const double *pr, const double *dpdxi, const double *den,
+                       const double *apu, const double *apv, const double *apw, double *flmass,
+                       int const_mflux, double flomas, double *a, double *su) {
+  const int n = m->n_cells, Fi = m->n_inner;
+  for (int ib = 0; ib < m->n_patches; ++ib) {
+    if (m->bctype[ib] == 5) return -1; /* pressure patches: not restated */
+    if (m->bctype[ib] == 7 && m->nfaces[ib] > Fi) return -1;
+  }
+  memset(a, 0, sizeof(double) * (size_t)c->nnz);
+  memset(su, 0, sizeof(double) * (size_t)n);
+  for (int f = 0; f < Fi; ++f) { /* :78-119 */
+    const int p = m->owner[f], q = m->neighbour[f];
+    const double cap = rc_inner(g, f, p, q, u, v, w, pr, dpdxi, den, apu, &flmass[f]);
+    a[slot_pn[f]] = cap;
+    a[slot_np[f]] = cap;
+    a[c->diag[p]] = a[c->diag[p]] - cap;
+    a[c->diag[q]] = a[c->diag[q]] - cap;
+    su[p] = su[p] - flmass[f];
+    su[q] = su[q] + flmass[f];
+  }
+  if (!const_mflux) { /* adjustMassFlow, faceflux_mass.f90:833-912 */
+    double flowo = 0.0;
+    for (int ib = 0; ib < m->n_patches; ++ib) {
+      if (m->bctype[ib] != 4) continue;
+      for (int i = 0; i < m->nfaces[ib]; ++i) {
+        const int f = m->start_face[ib] + i, p = m->owner[f], b = n + (f - Fi);
+        u[b] = u[p];
+        v[b] = v[p];
+        w[b] = w[p];
+        flmass[f] = den[p] * ((u[b] * g->arx[f] + v[b] * g->ary[f]) + w[b] * g->arz[f]);
+        flowo = flowo + flmass[f];
+      }
+    }
+    const double fac = flomas / (flowo + SMALL);
+    for (int ib = 0; ib < m->n_patches; ++ib) {
+      if (m->bctype[ib] != 4) continue;
+      for (int i = 0; i < m->nfaces[ib]; ++i) {
+        const int f = m->start_face[ib] + i, b = n + (f - Fi);
+        flmass[f] = flmass[f] * fac;
+        u[b] = u[b] * fac;
+        v[b] = v[b] * fac;
+        w[b] = w[b] * fac;
+      }
+    }
+  }
+  int iper = 0, l = Fi;
+  for (int ib = 0; ib < m->n_patches; ++ib) { /* :129-228 */
+    const int t = m->bctype[ib];
+    if (t == 3 || t == 4) {
+      for (int i = 0; i < m->nfaces[ib]; ++i) {
+        const int f = m->start_face[ib] + i, p = m->owner[f];
+        su[p] = su[p] - flmass[f];
+      }
+    } else if (t == 7) {
+      const int t0 = m->twin_start[iper++];
+      for (int i = 0; i < m->nfaces[ib]; ++i, ++l) {
+        const int f = m->start_face[ib] + i, p = m->owner[f], q = m->owner[t0 + i];
+        const double cap =
+            rc_periodic(g, f, i, p, q, u, v, w, pr, dpdxi, den, apu, apv, apw, &flmass[f]);
+        a[slot_pn[l]] = cap;
+        a[slot_np[l]] = cap;
+        a[c->diag[p]] = a[c->diag[p]] - cap;
+        a[c->diag[q]] = a[c->diag[q]] - cap;
+        su[p] = su[p] - flmass[f];
+        su[q] = su[q] + flmass[f];
+      }
+    }
+  }
+  return 0;
+}
+
+/* bpres (bpres.f90): stage 1 copies the owner's value to every non-pressure
+ * boundary face, stage 2 extrapolates with the gradient to wall faces. */
+static void orc_bpres(const orc_mesh *m, const orc_geom *g, double *pp, const double *dp,
+                      int stage) {
+  const int n = m->n_cells, Fi = m->n_inner;
+  for (int ib = 0; ib < m->n_patches; ++ib) {
+    const int t = m->bctype[ib];
+    if (stage == 1 ? t == 5 : t != 1) continue;
+    for (int i = 0; i < m->nfaces[ib]; ++i) {
+      const int f = m->start_face[ib] + i, p = m->owner[f], b = n + (f - Fi);
+      if (stage == 1) {
+        pp[b] = pp[p];
+      } else {
+        const double xpb = g->xf[f] - g->xc[p], ypb = g->yf[f] - g->yc[p],
+                     zpb = g->zf[f] - g->zc[p];
+        pp[b] = ((pp[p] + dp[3 * p] * xpb) + dp[3 * p + 1] * ypb) + dp[3 * p + 2] * zpb;
+      }
+    }
+  }
+}
+
+void orc_pcorr_correct(const orc_mesh *m, const orc_geom *g, const int *slot_pn,
+                       const double *a, double *pp, double *flmass, double *u, double *v,
+                       double *w, double *pr, const double *apu, const double *apv,
+                       const double *apw, double urfp, int pref, double *dpdxi, double *su,
+                       double *sv, double *sw) {
+  const int n = m->n_cells, Fi = m->n_inner, F = m->n_faces;
+  for (int f = 0; f < Fi; ++f) { /* :333-345 */
+    const int p = m->owner[f], q = m->neighbour[f];
+    flmass[f] = flmass[f] + a[slot_pn[f]] * (pp[q] - pp[p]);
+  }
+  int iper = 0, l = Fi;
+  for (int ib = 0; ib < m->n_patches; ++ib) { /* :348-376 */
+    if (m->bctype[ib] != 7) continue;
+    const int t0 = m->twin_start[iper++];
+    for (int i = 0; i < m->nfaces[ib]; ++i, ++l) {
+      const int f = m->start_face[ib] + i, p = m->owner[f], q = m->owner[t0 + i];
+      flmass[f] = flmass[f] + a[slot_pn[l]] * (pp[q] - pp[p]);
+      flmass[t0 + i] = flmass[f];
+    }
+  }
+  const double ppref = pp[pref]; /* :395 */
+  /* gradp_and_sources(pp), nablap.f90:19-209 with pscheme 'linear' */
+  memset(su, 0, sizeof(double) * (size_t)n);
+  memset(sv, 0, sizeof(double) * (size_t)n);
+  memset(sw, 0, sizeof(double) * (size_t)n);
+  for (int f = 0; f < Fi; ++f) {
+    const int p = m->owner[f], q = m->neighbour[f];
+    const double pf = pp[p] + (pp[q] - pp[p]) * g->facint[f];
+    const double dfxe = pf * g->arx[f], dfye = pf * g->ary[f], dfze = pf * g->arz[f];
+    su[p] = su[p] - dfxe;
+    sv[p] = sv[p] - dfye;
+    sw[p] = sw[p] - dfze;
+    su[q] = su[q] + dfxe;
+    sv[q] = sv[q] + dfye;
+    sw[q] = sw[q] + dfze;
+  }
+  for (int stage = 1; stage <= 2; ++stage) {
+    orc_bpres(m, g, pp, dpdxi, stage);
+    for (int c = 0; c < n; ++c) {
+      dpdxi[3 * c] = -su[c];
+      dpdxi[3 * c + 1] = -sv[c];
+      dpdxi[3 * c + 2] = -sw[c];
+    }
+    for (int f = Fi; f < F; ++f) {
+      const int p = m->owner[f], b = n + (f - Fi);
+      dpdxi[3 * p] = dpdxi[3 * p] + pp[b] * g->arx[f];
+      dpdxi[3 * p + 1] = dpdxi[3 * p + 1] + pp[b] * g->ary[f];
+      dpdxi[3 * p + 2] = dpdxi[3 * p + 2] + pp[b] * g->arz[f];
+    }
+    for (int c = 0; c < n; ++c) {
+      const double volr = 1.0 / g->vol[c];
+      dpdxi[3 * c] = dpdxi[3 * c] * volr;
+      dpdxi[3 * c + 1] = dpdxi[3 * c + 1] * volr;
+      dpdxi[3 * c + 2] = dpdxi[3 * c + 2] * volr;
+    }
+  }
+  for (int f = Fi; f < F; ++f) {
+    const int p = m->owner[f], b = n + (f - Fi);
+    su[p] = su[p] - pp[b] * g->arx[f];
+    sv[p] = sv[p] - pp[b] * g->ary[f];
+    sw[p] = sw[p] - pp[b] * g->arz[f];
+  }
+  for (int c = 0; c < n; ++c) { /* :401-404 */
+    u[c] = u[c] + su[c] * apu[c];
+    v[c] = v[c] + sv[c] * apv[c];
+    w[c] = w[c] + sw[c] * apw[c];
+    pr[c] = pr[c] + urfp * (pp[c] - ppref);
+  }
+  /* updateVelocityAtBoundary, velocity.f90:1184-1262 */
+  for (int ib = 0; ib < m->n_patches; ++ib) {
+    const int t = m->bctype[ib];
+    if (t != 6 && t != 7 && t != 2) continue;
+    for (int i = 0; i < m->nfaces[ib]; ++i) {
+      const int f = m->start_face[ib] + i, p = m->owner[f], b = n + (f - Fi);
+      if (t == 2) {
+        const double Unmag = (u[p] * g->arx[f] + v[p] * g->ary[f]) + w[p] * g->arz[f];
+        u[b] = u[p] - Unmag * g->arx[f];
+        v[b] = v[p] - Unmag * g->ary[f];
+        w[b] = w[p] - Unmag * g->arz[f];
+      } else {
+        u[b] = u[p];
+        v[b] = v[p];
+        w[b] = w[p];
+      }
+    }
+  }
+}

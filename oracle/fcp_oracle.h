@@ -259,6 +259,29 @@ int orc_pmgmres_ilu(const orc_csr *c, const double *a, double *x, const double *
 double orc_small(void);
 double orc_s0_init(void);
 
+/* Pressure-correction step of SIMPLE, calcp_simple.f90:69-419, for an
+ * incompressible case without pressure patches (pscheme 'linear', one
+ * corrector).  orc_pcorr_assemble: facefluxmass2 on the inner faces
+ * (faceflux_mass.f90:175-252), adjustMassFlow unless const_mflux (:833-912,
+ * outlet values of u, v, w and outlet fluxes rescaled), inlet/outlet sources
+ * and facefluxmass2_periodic (:313-372); a[nnz] and su[n] overwritten,
+ * flmass[F] updated.  Returns -1 for a pressure patch.
+ * orc_pcorr_correct: from the solved correction pp[numTotal], the mass-flux
+ * correction (:333-376), ppref = pp[pref] (0-based), gradp_and_sources
+ * (nablap.f90:19-209 with bpres) into su/sv/sw[n] and dpdxi's cell rows,
+ * u,v,w += s*ap, p += urfp (pp - ppref), then updateVelocityAtBoundary
+ * (velocity.f90:1184-1262). */
+int orc_pcorr_assemble(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
+                       const int *slot_pn, const int *slot_np, double *u, double *v, double *w,
+                       const double *pr, const double *dpdxi, const double *den,
+                       const double *apu, const double *apv, const double *apw, double *flmass,
+                       int const_mflux, double flomas, double *a, double *su);
+void orc_pcorr_correct(const orc_mesh *m, const orc_geom *g, const int *slot_pn,
+                       const double *a, double *pp, double *flmass, double *u, double *v,
+                       double *w, double *pr, const double *apu, const double *apv,
+                       const double *apw, double urfp, int pref, double *dpdxi, double *su,
+                       double *sv, double *sw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -214,6 +214,37 @@ class Oracle:
         return self.convdiff(flmass, self.face_gamma_cells(diffcoef), phi, grad, gam, scheme,
                              bc_kind, urf)
 
+    def pcorr_assemble(self, u, v, w, p, dpdxi, den, apu, apv, apw, flmass, const_mflux, flomas):
+        """calcp_simple.f90:69-228: returns (a, su, flmass, u, v, w)."""
+        m = self.mesh
+        f64 = lambda x: np.array(x, dtype=np.float64, copy=True).reshape(-1)  # noqa: E731
+        u, v, w, flmass = f64(u), f64(v), f64(w), f64(flmass)
+        keep = [f64(x) for x in (p, dpdxi, den, apu, apv, apw)]
+        a = np.zeros(self.nnz)
+        su = np.zeros(m.n_cells)
+        rc = lib().orc_pcorr_assemble(C.byref(self.cm), C.byref(self.cg), C.byref(self.cc),
+                                      _p(self.slot_pn), _p(self.slot_np), _p(u), _p(v), _p(w),
+                                      *[_p(x) for x in keep], _p(flmass), int(const_mflux),
+                                      C.c_double(flomas), _p(a), _p(su))
+        assert rc == 0, rc
+        return a, su, flmass, u, v, w
+
+    def pcorr_correct(self, a, pp, flmass, u, v, w, p, apu, apv, apw, urfp, pref, dpdxi=None):
+        """calcp_simple.f90:333-419 from the solved pp; pref 0-based.
+        Returns dict of pp, flmass, u, v, w, p, dpdxi, su, sv, sw."""
+        m = self.mesh
+        f64 = lambda x: np.array(x, dtype=np.float64, copy=True).reshape(-1)  # noqa: E731
+        o = dict(pp=f64(pp), flmass=f64(flmass), u=f64(u), v=f64(v), w=f64(w), p=f64(p),
+                 dpdxi=np.zeros(3 * m.n_total) if dpdxi is None else f64(dpdxi),
+                 su=np.zeros(m.n_cells), sv=np.zeros(m.n_cells), sw=np.zeros(m.n_cells))
+        keep = [f64(x) for x in (a, apu, apv, apw)]
+        lib().orc_pcorr_correct(C.byref(self.cm), C.byref(self.cg), _p(self.slot_pn),
+                                _p(keep[0]), _p(o["pp"]), _p(o["flmass"]), _p(o["u"]), _p(o["v"]),
+                                _p(o["w"]), _p(o["p"]), _p(keep[1]), _p(keep[2]), _p(keep[3]),
+                                C.c_double(urfp), int(pref), _p(o["dpdxi"]), _p(o["su"]),
+                                _p(o["sv"]), _p(o["sw"]))
+        return o
+
     def transport_dcoef(self, vis, bc_kind, viscos, pranl, sigt):
         d = np.zeros(self.mesh.n_faces)
         lib().orc_transport_dcoef(C.byref(self.cm), C.byref(self.cg),

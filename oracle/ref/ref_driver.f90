@@ -34,6 +34,23 @@
 !   solve_explicit solver n nnz itr_max tol_abs tol_rel ia ja diag a fi0 rhs fi_out info_out
 !              (explicit-array entry points iccg/dpcg/bicgstab/GaussSeidel on a
 !               CSR system of its own, 1-based int32 ia/ja/diag files)
+!   pcorr_assemble u v w p dpdxi den apu apv apw flmass const_mflux flomas
+!                  a_out su_out flmass_out u_out v_out w_out
+!              (calcp_simple.f90:69-228 for an incompressible case without
+!               pressure patches: facefluxmass2 on the inner faces,
+!               adjustMassFlow unless const_mflux, the inlet/outlet sources
+!               and facefluxmass2_periodic; faceflux_mass.f90:175-252,
+!               313-372, 833-912.  calcp_simple.f90 itself does not compile
+!               at this revision -- lSolverP, maxiterP, tolAbsP, tolRelP and
+!               urfp are undeclared at :324,419 -- so its steps are composed
+!               here from the compiled routines, in its statement order)
+!   pcorr_correct pp flmass a u v w p apu apv apw urfp prefcell
+!                 pp_out flmass_out u_out v_out w_out p_out dpdxi_out su_out sv_out sw_out
+!              (calcp_simple.f90:333-419 for one corrector: flux correction
+!               on inner and periodic faces, ppref = pp(pRefCell),
+!               gradp_and_sources (nablap.f90:19-209, pscheme 'linear', bpres
+!               stages), velocity and pressure correction, then
+!               updateVelocityAtBoundary velocity.f90:1184-1262)
 ! Binary files are raw little-endian stream files (real(8) / integer(4)).
 program ref_driver
   use types
@@ -101,6 +118,10 @@ program ref_driver
       call job_solve_explicit(ju)
     case ('ilu')
       call job_ilu(ju)
+    case ('pcorr_assemble')
+      call job_pcorr_assemble(ju)
+    case ('pcorr_correct')
+      call job_pcorr_correct(ju)
     case default
       read(ju, *) cmd
       write(*,*) 'ref_driver: unknown job ', trim(cmd)
@@ -531,6 +552,153 @@ contains
     write(uu,'(es25.17)') resor(7)
     close(uu)
     deallocate(wf)
+  end subroutine
+
+  subroutine pcorr_alloc()
+    use variables, only: u, v, w, p, pp, den, dPdxi, flmass
+    if (allocated(u)) deallocate(u, v, w, p, pp, den, dPdxi, flmass)
+    allocate(u(numTotal), v(numTotal), w(numTotal), p(numTotal), pp(numTotal), den(numTotal), &
+             dPdxi(3,numTotal), flmass(numFaces))
+    u = 0.0_dp; v = 0.0_dp; w = 0.0_dp; p = 0.0_dp; pp = 0.0_dp; den = 0.0_dp
+    dPdxi = 0.0_dp; flmass = 0.0_dp
+  end subroutine
+
+  subroutine job_pcorr_assemble(ju)
+    use variables, only: u, v, w, p, den, dPdxi, flmass
+    use faceflux_mass
+    integer, intent(in) :: ju
+    character(len=64) :: c, fu, fv, fw, fp, fdp, fden, fapu, fapv, fapw, ffm, fa, fsu, ffo, &
+                         fuo, fvo, fwo
+    integer :: cmf, i, k, ib, iface, ijp, ijn, iPer, l, ifc, iftwin
+    real(dp) :: flo, cap, can
+    read(ju, *) c, fu, fv, fw, fp, fdp, fden, fapu, fapv, fapw, ffm, cmf, flo, fa, fsu, ffo, &
+                fuo, fvo, fwo
+    call pcorr_alloc()
+    call rd8(fu, u, numTotal)
+    call rd8(fv, v, numTotal)
+    call rd8(fw, w, numTotal)
+    call rd8(fp, p, numTotal)
+    call rd8(fdp, dPdxi, 3*numTotal)
+    call rd8(fden, den, numTotal)
+    call rd8(fapu, apu, numCells)
+    call rd8(fapv, apv, numCells)
+    call rd8(fapw, apw, numCells)
+    call rd8(ffm, flmass, numFaces)
+    const_mflux = cmf /= 0
+    flomas = flo
+    a = 0.0_dp
+    su = 0.0_dp
+    do i = 1, numInnerFaces
+      ijp = owner(i)
+      ijn = neighbour(i)
+      call facefluxmass2(i, ijp, ijn, arx(i), ary(i), arz(i), facint(i), cap, can, flmass(i))
+      k = icell_jcell_csr_index(i); a(k) = cap
+      k = jcell_icell_csr_index(i); a(k) = cap
+      k = diag(ijp); a(k) = a(k) - cap
+      k = diag(ijn); a(k) = a(k) - cap
+      su(ijp) = su(ijp) - flmass(i)
+      su(ijn) = su(ijn) + flmass(i)
+    enddo
+    if (.not. const_mflux) call adjustMassFlow
+    iPer = 0
+    l = numInnerFaces
+    do ib = 1, numBoundaries
+      if (bctype(ib) == 'inlet' .or. bctype(ib) == 'outlet') then
+        do i = 1, nfaces(ib)
+          iface = startFace(ib) + i
+          ijp = owner(iface)
+          su(ijp) = su(ijp) - flmass(iface)
+        enddo
+      elseif (bctype(ib) == 'periodic') then
+        iPer = iPer + 1
+        do i = 1, nfaces(ib)
+          ifc = startFace(ib) + i
+          ijp = owner(ifc)
+          iftwin = startFaceTwin(iPer) + i
+          ijn = owner(iftwin)
+          call facefluxmass2_periodic(i, ijp, ijn, xf(ifc), yf(ifc), zf(ifc), arx(ifc), ary(ifc), &
+                                      arz(ifc), half, cap, can, flmass(ifc))
+          l = l + 1
+          k = icell_jcell_csr_index(l); a(k) = cap
+          k = jcell_icell_csr_index(l); a(k) = cap
+          k = diag(ijp); a(k) = a(k) - cap
+          k = diag(ijn); a(k) = a(k) - cap
+          su(ijp) = su(ijp) - flmass(ifc)
+          su(ijn) = su(ijn) + flmass(ifc)
+        enddo
+      endif
+    enddo
+    call wr8(fa, a, nnz)
+    call wr8(fsu, su, numCells)
+    call wr8(ffo, flmass, numFaces)
+    call wr8(fuo, u, numTotal)
+    call wr8(fvo, v, numTotal)
+    call wr8(fwo, w, numTotal)
+  end subroutine
+
+  subroutine job_pcorr_correct(ju)
+    use variables, only: u, v, w, p, pp, dPdxi, flmass
+    use nablap
+    use velocity, only: updateVelocityAtBoundary
+    integer, intent(in) :: ju
+    character(len=64) :: c, fpp, ffm, fa, fu, fv, fw, fp, fapu, fapv, fapw, fppo, ffmo, fuo, fvo, &
+                         fwo, fpo, fdpo, fsu, fsv, fsw
+    integer :: iref, ib, i, iface, ijp, ijn, iPer, l, iftwin, k
+    real(dp) :: urfpp, ppref
+    read(ju, *) c, fpp, ffm, fa, fu, fv, fw, fp, fapu, fapv, fapw, urfpp, iref, fppo, ffmo, fuo, &
+                fvo, fwo, fpo, fdpo, fsu, fsv, fsw
+    call pcorr_alloc()
+    call rd8(fpp, pp, numTotal)
+    call rd8(ffm, flmass, numFaces)
+    call rd8(fa, a, nnz)
+    call rd8(fu, u, numTotal)
+    call rd8(fv, v, numTotal)
+    call rd8(fw, w, numTotal)
+    call rd8(fp, p, numTotal)
+    call rd8(fapu, apu, numCells)
+    call rd8(fapv, apv, numCells)
+    call rd8(fapw, apw, numCells)
+    pRefCell = iref
+    do iface = 1, numInnerFaces
+      ijp = owner(iface)
+      ijn = neighbour(iface)
+      k = icell_jcell_csr_index(iface)
+      flmass(iface) = flmass(iface) + a(k) * (pp(ijn)-pp(ijp))
+    enddo
+    iPer = 0
+    l = numInnerFaces
+    do ib = 1, numBoundaries
+      if (bctype(ib) == 'periodic') then
+        iPer = iPer + 1
+        do i = 1, nfaces(ib)
+          iface = startFace(ib) + i
+          ijp = owner(iface)
+          iftwin = startFaceTwin(iPer) + i
+          ijn = owner(iftwin)
+          l = l + 1
+          k = icell_jcell_csr_index(l)
+          flmass(iface) = flmass(iface) + a(k) * (pp(ijn)-pp(ijp))
+          flmass(iftwin) = flmass(iface)
+        enddo
+      endif
+    enddo
+    ppref = pp(pRefCell)
+    call gradp_and_sources(pp)
+    u(1:numCells) = u(1:numCells) + su*apu
+    v(1:numCells) = v(1:numCells) + sv*apv
+    w(1:numCells) = w(1:numCells) + sw*apw
+    p(1:numCells) = p(1:numCells) + urfpp*(pp(1:numCells)-ppref)
+    call updateVelocityAtBoundary
+    call wr8(fppo, pp, numTotal)
+    call wr8(ffmo, flmass, numFaces)
+    call wr8(fuo, u, numTotal)
+    call wr8(fvo, v, numTotal)
+    call wr8(fwo, w, numTotal)
+    call wr8(fpo, p, numTotal)
+    call wr8(fdpo, dPdxi, 3*numTotal)
+    call wr8(fsu, su, numCells)
+    call wr8(fsv, sv, numCells)
+    call wr8(fsw, sw, numCells)
   end subroutine
 
 end program ref_driver

@@ -243,6 +243,68 @@ def flux_scalar_case():
     print("wrote fluxscalar")
 
 
+PCORR_TYPES = ("inlet", "outlet", "wall", "symmetry", "empty", "wall")
+
+
+def pcorr_inputs(mesh, seed):
+    """Inputs of the pressure-correction fixture: velocities, pressure and its
+    gradient, density, 1/a_P of the momentum rows and a mass flux whose
+    inlet part is what bcin would prescribe (den u.S of the inlet values)."""
+    n, nt, F = mesh.n_cells, mesh.n_total, mesh.n_faces
+    I = dict(u=uniform(seed, 1, nt, -1.0, 1.0), v=uniform(seed, 2, nt, -1.0, 1.0),
+             w=uniform(seed, 3, nt, -1.0, 1.0), p=uniform(seed, 4, nt, -1.0, 1.0),
+             dpdxi=uniform(seed, 5, 3 * nt, -1.0, 1.0),
+             den=1.0 + 0.2 * uniform(seed, 6, nt, -1.0, 1.0),
+             apu=10.0 ** uniform(seed, 7, n, -2.0, 0.0), apv=10.0 ** uniform(seed, 8, n, -2.0, 0.0),
+             apw=10.0 ** uniform(seed, 9, n, -2.0, 0.0), flmass=uniform(seed, 10, F, -1.0, 1.0),
+             pp=uniform(seed, 11, nt, -1.0, 1.0))
+    return I
+
+
+def pcorr_case():
+    """The pressure-correction step of SIMPLE (calcp_simple.f90:69-419,
+    composed from the reference's facefluxmass2, facefluxmass2_periodic,
+    adjustMassFlow, gradp_and_sources, bpres and updateVelocityAtBoundary):
+    the assembly on a jittered 6^3 hex with inlet/outlet/wall/symmetry/empty
+    patches (with and without adjustMassFlow) and on the periodic 6^3 mesh,
+    and the correction from a given pressure correction on both."""
+    out = {}
+    cases = {"hex": (hex_mesh(6, jitter=0.2, seed=13, types=PCORR_TYPES), 17),
+             "per": (periodic_mesh(), 18)}
+    for key, (mesh, seed) in cases.items():
+        rc = RefCase(mesh)
+        rc.dump()
+        I = pcorr_inputs(mesh, seed)
+        for k, v in I.items():
+            rc.put(k + ".bin", v)
+            out[f"{key}_in_{k}"] = v
+        Fi = mesh.n_inner
+        inlet = [i for i, pt in enumerate(mesh.patch_of_bface()) if mesh.types[pt] == "inlet"]
+        # the inlet mass flow, to six decimals so that its text reads back
+        # as the same double on both sides
+        flomas = float(f"{np.sum(I['flmass'][Fi + np.array(inlet, int)]):.6f}") if inlet else 0.0
+        out[f"{key}_flomas"] = np.array(flomas)
+        for cmf in ((0, 1) if key == "hex" else (1,)):
+            rc.run([f"pcorr_assemble u.bin v.bin w.bin p.bin dpdxi.bin den.bin apu.bin apv.bin "
+                    f"apw.bin flmass.bin {cmf} {flomas!r} a.bin su.bin fm.bin uo.bin vo.bin wo.bin"])
+            t = f"{key}_c{cmf}"
+            for k, f in (("a", "a.bin"), ("su", "su.bin"), ("flmass", "fm.bin"), ("u", "uo.bin"),
+                         ("v", "vo.bin"), ("w", "wo.bin")):
+                out[f"{t}_{k}"] = rc.get(f)
+        cmf = 1
+        t = f"{key}_c{cmf}"
+        rc.run([f"pcorr_correct pp.bin fm.bin a.bin uo.bin vo.bin wo.bin p.bin apu.bin apv.bin "
+                f"apw.bin 0.3d0 7 ppo.bin fmo.bin uo2.bin vo2.bin wo2.bin po.bin dpo.bin "
+                f"csu.bin csv.bin csw.bin"])
+        for k, f in (("pp", "ppo.bin"), ("flmass", "fmo.bin"), ("u", "uo2.bin"), ("v", "vo2.bin"),
+                     ("w", "wo2.bin"), ("p", "po.bin"), ("dpdxi", "dpo.bin"), ("su", "csu.bin"),
+                     ("sv", "csv.bin"), ("sw", "csw.bin")):
+            out[f"{key}_corr_{k}"] = rc.get(f)
+        rc.close()
+    np.savez_compressed(os.path.join(HERE, "pcorr.npz"), **out)
+    print("wrote pcorr")
+
+
 PERIODIC_TYPES = ("empty", "periodic", "wall", "inlet", "empty", "periodic")
 PERIODIC_WF = (0, 0, 1, 0, 0, 0)
 # (time scheme, convection scheme, solver, itr_max, tol_rel, gradient, urf, gds)
@@ -518,6 +580,7 @@ CASES = {
     "periodic": periodic_case,
     "gradfvx": grad_fvx_case,
     "fluxscalar": flux_scalar_case,
+    "pcorr": pcorr_case,
     "transport": transport_case,
     "cavity": cavity_case,
     "kat5x5": kat_5x5_case,

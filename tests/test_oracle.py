@@ -682,3 +682,88 @@ def test_facefluxscalar_bitexact_vs_reference(golden_dir, key, scheme, code):
     p, q = mesh.owner[:Fi], mesh.neighbour[:Fi]
     assert np.allclose(gf[:Fi], I["vis"][p] + (I["vis"][q] - I["vis"][p]) * lam, rtol=1e-14)
     assert np.array_equal(gf[Fi:], I["vis"][mesh.owner[Fi:]])
+
+
+def _pcorr(golden_dir):
+    import sys
+    if golden_dir not in sys.path:
+        sys.path.insert(0, golden_dir)
+    import make_golden as mg
+    g = np.load(os.path.join(golden_dir, "pcorr.npz"))
+    meshes = {"hex": hex_mesh(6, jitter=0.2, seed=13, types=mg.PCORR_TYPES),
+              "per": mg.periodic_mesh()}
+    return mg, g, meshes
+
+
+@pytest.mark.parametrize("key,cmf", [("hex", 0), ("hex", 1), ("per", 1)])
+def test_pcorr_assemble_bitexact_vs_reference(golden_dir, key, cmf):
+    """The pressure-correction assembly of calcp_simple.f90:69-228, as the
+    reference's own facefluxmass2 / facefluxmass2_periodic / adjustMassFlow
+    compose it (pcorr fixture): matrix, source, mass fluxes and the outlet
+    velocities adjustMassFlow rescales, bit for bit.  Independently: the
+    matrix is symmetric with zero row sums (a pure Neumann operator), and
+    su is minus the discrete divergence of the new mass fluxes."""
+    mg, g, meshes = _pcorr(golden_dir)
+    mesh = meshes[key]
+    o = Oracle(mesh)
+    I = {k: g[f"{key}_in_{k}"] for k in ("u", "v", "w", "p", "dpdxi", "den", "apu", "apv", "apw",
+                                        "flmass")}
+    a, su, fm, u, v, w = o.pcorr_assemble(I["u"], I["v"], I["w"], I["p"], I["dpdxi"], I["den"],
+                                          I["apu"], I["apv"], I["apw"], I["flmass"], cmf,
+                                          float(g[f"{key}_flomas"]))
+    t = f"{key}_c{cmf}"
+    for k, x in (("a", a), ("su", su), ("flmass", fm), ("u", u), ("v", v), ("w", w)):
+        assert np.array_equal(x, g[f"{t}_{k}"]), k
+    import scipy.sparse as sps
+    n = mesh.n_cells
+    A = sps.csr_matrix((a, o.ja, o.ia), shape=(n, n))
+    assert abs(A - A.T).max() == 0.0
+    assert np.allclose(np.asarray(A.sum(axis=1)).ravel(), 0.0, atol=1e-15)
+    assert (A.diagonal() > 0).all()
+    div = np.zeros(n)
+    Fi = mesh.n_inner
+    np.add.at(div, o.owner[:Fi], fm[:Fi])
+    np.subtract.at(div, mesh.neighbour, fm[:Fi])
+    types = [mesh.types[pt] for pt in mesh.patch_of_bface()]
+    for b, t_ in enumerate(types):
+        f = Fi + b
+        if t_ in ("inlet", "outlet"):
+            div[o.owner[f]] += fm[f]
+    for ib, t0 in zip(mesh.periodic_patches, mesh.twin_start):
+        s0, nf = int(mesh.start_face[ib]), int(mesh.nfaces[ib])
+        np.add.at(div, o.owner[s0:s0 + nf], fm[s0:s0 + nf])
+        np.subtract.at(div, o.owner[t0:t0 + nf], fm[s0:s0 + nf])
+    assert np.allclose(su, -div, rtol=0, atol=1e-13)
+    if cmf == 0:  # global mass conservation: outlet flow = inlet flow
+        out_f = [Fi + b for b, t_ in enumerate(types) if t_ == "outlet"]
+        assert np.isclose(fm[out_f].sum(), float(g[f"{key}_flomas"]), rtol=1e-12)
+
+
+@pytest.mark.parametrize("key", ["hex", "per"])
+def test_pcorr_correct_bitexact_vs_reference(golden_dir, key):
+    """The correction step of calcp_simple.f90:333-419 (flux correction,
+    gradp_and_sources with bpres, velocity and pressure update,
+    updateVelocityAtBoundary) from a given pressure correction: every output
+    bit for bit.  Independently: the flux correction is A pp's face split --
+    the divergence of the flux change is A pp (to rounding)."""
+    mg, g, meshes = _pcorr(golden_dir)
+    mesh = meshes[key]
+    o = Oracle(mesh)
+    c1 = f"{key}_c1"
+    I = {k: g[f"{key}_in_{k}"] for k in ("p", "apu", "apv", "apw", "pp")}
+    r = o.pcorr_correct(g[f"{c1}_a"], I["pp"], g[f"{c1}_flmass"], g[f"{c1}_u"], g[f"{c1}_v"],
+                        g[f"{c1}_w"], I["p"], I["apu"], I["apv"], I["apw"], 0.3, 6)
+    for k in ("pp", "flmass", "u", "v", "w", "p", "dpdxi", "su", "sv", "sw"):
+        assert np.array_equal(r[k], g[f"{key}_corr_{k}"]), k
+    import scipy.sparse as sps
+    n, Fi = mesh.n_cells, mesh.n_inner
+    A = sps.csr_matrix((g[f"{c1}_a"], o.ja, o.ia), shape=(n, n))
+    dfm = r["flmass"] - g[f"{c1}_flmass"]
+    div = np.zeros(n)
+    np.add.at(div, o.owner[:Fi], dfm[:Fi])
+    np.subtract.at(div, mesh.neighbour, dfm[:Fi])
+    for ib, t0 in zip(mesh.periodic_patches, mesh.twin_start):
+        s0, nf = int(mesh.start_face[ib]), int(mesh.nfaces[ib])
+        np.add.at(div, o.owner[s0:s0 + nf], dfm[s0:s0 + nf])
+        np.subtract.at(div, o.owner[t0:t0 + nf], dfm[s0:s0 + nf])
+    assert np.allclose(div, A @ I["pp"][:n], rtol=0, atol=1e-14)
