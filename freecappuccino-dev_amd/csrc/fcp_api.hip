@@ -656,6 +656,94 @@ int fcp_convdiff_cell(fcp_ctx *ctx, const double *flmass, const double *diffcoef
                       sp);
 }
 
+static int pcorr_check(fcp_ctx *ctx, const char *who) {
+  for (int ib = 0; ib < ctx->nPatches; ++ib)
+    if (ctx->h_bctype[ib] == FCP_BC_PRESSURE)
+      return fcp_fail(ctx, FCP_ERR_UNSUPPORTED, "%s: pressure patches are not supported", who);
+  return 0;
+}
+
+int fcp_pressure_assemble(fcp_ctx *ctx, double *u, double *v, double *w, const double *p,
+                          const double *dpdxi, const double *den, const double *apu,
+                          const double *apv, const double *apw, double *flmass, int const_mflux,
+                          double flomas, double *a, double *su) {
+  FCP_TRY(need(ctx, true, true, true));
+  if (!u || !v || !w || !p || !dpdxi || !den || !apu || !apv || !apw || !flmass || !su)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_pressure_assemble: null argument");
+  FCP_TRY(pcorr_check(ctx, "fcp_pressure_assemble"));
+  (void)hipSetDevice(ctx->device);
+  const size_t n = ctx->n, nt = n + ctx->Fb;
+  Arg<double> U, V, W, P, D, R, AU, AV, AW, FM, S;
+  FCP_TRY(arg_in(ctx, (const double *)u, nt, U));
+  FCP_TRY(arg_in(ctx, (const double *)v, nt, V));
+  FCP_TRY(arg_in(ctx, (const double *)w, nt, W));
+  FCP_TRY(arg_in(ctx, p, nt, P));
+  FCP_TRY(arg_in(ctx, dpdxi, 3 * nt, D));
+  FCP_TRY(arg_in(ctx, den, nt, R));
+  FCP_TRY(arg_in(ctx, apu, n, AU));
+  FCP_TRY(arg_in(ctx, apv, n, AV));
+  FCP_TRY(arg_in(ctx, apw, n, AW));
+  FCP_TRY(arg_in(ctx, (const double *)flmass, ctx->F, FM));
+  FCP_TRY(arg_out(ctx, su, n, S));
+  FCP_TRY(fcp_launch_pcorr_assemble(ctx, U.dev, V.dev, W.dev, P.dev, D.dev, R.dev, AU.dev, AV.dev,
+                                    AW.dev, FM.dev, const_mflux, flomas, S.dev));
+  if (!const_mflux) {
+    FCP_TRY(arg_back(ctx, U));
+    FCP_TRY(arg_back(ctx, V));
+    FCP_TRY(arg_back(ctx, W));
+  }
+  FCP_TRY(arg_back(ctx, FM));
+  FCP_TRY(arg_back(ctx, S));
+  FCP_TRY(out_copy(ctx, ctx->a.p, a, ctx->nnz));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int fcp_pressure_correct(fcp_ctx *ctx, double *pp, double *flmass, double *u, double *v,
+                         double *w, double *p, double *dpdxi, const double *apu,
+                         const double *apv, const double *apw, double urfp, int pref_cell,
+                         double *su, double *sv, double *sw) {
+  FCP_TRY(need(ctx, true, true, true));
+  if (!pp || !flmass || !u || !v || !w || !p || !dpdxi || !apu || !apv || !apw)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_pressure_correct: null argument");
+  if (pref_cell < 1 || pref_cell > ctx->n)
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_pressure_correct: pRefCell %d outside 1..%d",
+                    pref_cell, ctx->n);
+  FCP_TRY(pcorr_check(ctx, "fcp_pressure_correct"));
+  (void)hipSetDevice(ctx->device);
+  const size_t n = ctx->n, nt = n + ctx->Fb;
+  Arg<double> PP, FM, U, V, W, P, D, AU, AV, AW, SU, SV, SW;
+  FCP_TRY(arg_in(ctx, (const double *)pp, nt, PP));
+  FCP_TRY(arg_in(ctx, (const double *)flmass, ctx->F, FM));
+  FCP_TRY(arg_in(ctx, (const double *)u, nt, U));
+  FCP_TRY(arg_in(ctx, (const double *)v, nt, V));
+  FCP_TRY(arg_in(ctx, (const double *)w, nt, W));
+  FCP_TRY(arg_in(ctx, (const double *)p, nt, P));
+  FCP_TRY(arg_in(ctx, (const double *)dpdxi, 3 * nt, D));
+  FCP_TRY(arg_in(ctx, apu, n, AU));
+  FCP_TRY(arg_in(ctx, apv, n, AV));
+  FCP_TRY(arg_in(ctx, apw, n, AW));
+  // su, sv, sw: the caller's arrays or scratch of the context
+  FCP_HIP(ctx, ctx->pc_s.alloc(3 * n > 0 ? 3 * n : 1));
+  double *s3[3] = {su, sv, sw};
+  Arg<double> SA[3];
+  for (int k = 0; k < 3; ++k) {
+    if (s3[k]) {
+      FCP_TRY(arg_out(ctx, s3[k], n, SA[k]));
+    } else {
+      SA[k].dev = ctx->pc_s.p + k * n;
+    }
+  }
+  FCP_TRY(fcp_launch_pcorr_correct(ctx, PP.dev, FM.dev, U.dev, V.dev, W.dev, P.dev, D.dev,
+                                   AU.dev, AV.dev, AW.dev, urfp, pref_cell - 1, SA[0].dev,
+                                   SA[1].dev, SA[2].dev));
+  for (Arg<double> *x : {&PP, &FM, &U, &V, &W, &P, &D}) FCP_TRY(arg_back(ctx, *x));
+  for (int k = 0; k < 3; ++k)
+    if (s3[k]) FCP_TRY(arg_back(ctx, SA[k]));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
 int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt, const double *phi_o,
             const double *phi_oo, double *su, double *sp) {
   FCP_TRY(need(ctx, true, true, false));

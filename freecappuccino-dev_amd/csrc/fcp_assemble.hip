@@ -570,6 +570,321 @@ __global__ void k_clip_small(int n, double *__restrict__ x, const double *__rest
   x[i] = fmax(x[i], FCP_SMALL);
 }
 
+// ---------------------------------------------------------------------------
+// Pressure correction of SIMPLE (calcp_simple.f90:69-419): incompressible,
+// no pressure patches, pscheme 'linear', one corrector.
+// ---------------------------------------------------------------------------
+struct RcIn {
+  const double *u, *v, *w, *p, *dp, *den, *apu, *apv, *apw;
+};
+
+// facefluxmass2 (faceflux_mass.f90:175-252): pressure-correction coefficient
+// and Rhie-Chow mass flux of inner face f.
+__device__ __forceinline__ double rc_inner(const GeomP &g, int f, int p, int q, const RcIn &in,
+                                           double &fm) {
+  const double lam = g.facint[f];
+  const double fxn = lam, fxp = 1.0 - lam;
+  const double xpn = g.xc[q] - g.xc[p], ypn = g.yc[q] - g.yc[p], zpn = g.zc[q] - g.zc[p];
+  const double dene = in.den[p] * fxp + in.den[q] * fxn;
+  const double Kj = g.vol[p] * in.apu[p] * fxp + g.vol[q] * in.apu[q] * fxn;
+  const double cap = -(dene * Kj * g.Df[f]);
+  const double ui = in.u[p] + (in.u[q] - in.u[p]) * lam;  // face_value_cds
+  const double vi = in.v[p] + (in.v[q] - in.v[p]) * lam;
+  const double wi = in.w[p] + (in.w[q] - in.w[p]) * lam;
+  const double dpxi = (in.dp[3 * (size_t)q] * fxp + in.dp[3 * (size_t)p] * fxn) * xpn;
+  const double dpyi = (in.dp[3 * (size_t)q + 1] * fxp + in.dp[3 * (size_t)p + 1] * fxn) * ypn;
+  const double dpzi = (in.dp[3 * (size_t)q + 2] * fxp + in.dp[3 * (size_t)p + 2] * fxn) * zpn;
+  fm = dene * ((ui * g.arx[f] + vi * g.ary[f]) + wi * g.arz[f]) +
+       cap * ((((in.p[q] - in.p[p]) - dpxi) - dpyi) - dpzi);
+  return cap;
+}
+
+// facefluxmass2_periodic (:313-372): periodic face f (index i in its patch,
+// whose Df(i) -- an inner face's -- the reference reads), p its owner, q the
+// twin face's owner, lambda = 1/2, velocities weighted by the other cell's
+// 1/a_P.
+__device__ __forceinline__ double rc_periodic(const GeomP &g, int f, int i, int p, int q,
+                                              const RcIn &in, double &fm) {
+  const double fxn = 0.5, fxp = 1.0 - 0.5;
+  const double xpn = 2.0 * (g.xf[f] - g.xc[p]), ypn = 2.0 * (g.yf[f] - g.yc[p]),
+               zpn = 2.0 * (g.zf[f] - g.zc[p]);
+  const double dene = in.den[p] * fxp + in.den[q] * fxn;
+  const double Kj0 = g.vol[p] * in.apu[p] * fxp + g.vol[q] * in.apu[q] * fxn;
+  const double cap = -(dene * Kj0 * g.Df[i]);
+  const double Kj = (in.apu[p] + in.apu[q]) + FCP_SMALL;
+  const double ui = (in.u[p] * in.apu[q] + in.u[q] * in.apu[p]) / Kj;
+  const double vi = (in.v[p] * in.apv[q] + in.v[q] * in.apv[p]) / Kj;
+  const double wi = (in.w[p] * in.apw[q] + in.w[q] * in.apw[p]) / Kj;
+  const double dpxi = (in.dp[3 * (size_t)q] * fxp + in.dp[3 * (size_t)p] * fxn) * xpn;
+  const double dpyi = (in.dp[3 * (size_t)q + 1] * fxp + in.dp[3 * (size_t)p + 1] * fxn) * ypn;
+  const double dpzi = (in.dp[3 * (size_t)q + 2] * fxp + in.dp[3 * (size_t)p + 2] * fxn) * zpn;
+  fm = dene * ((ui * g.arx[f] + vi * g.ary[f]) + wi * g.arz[f]) +
+       cap * ((((in.p[q] - in.p[p]) - dpxi) - dpyi) - dpzi);
+  return cap;
+}
+
+__device__ __forceinline__ int btype(const int *bpatch, const int *bctype, int b) {
+  return bctype[bpatch[b]];
+}
+
+// adjustMassFlow (:833-912), first pass: outlet values of u, v, w copied
+// from the owner and the outlet mass fluxes from them.
+__global__ void k_pc_outlet(int n, int Fi, int Fb, const int *__restrict__ owner,
+                            const int *__restrict__ bpatch, const int *__restrict__ bctype,
+                            GeomP g, const double *__restrict__ den, double *__restrict__ u,
+                            double *__restrict__ v, double *__restrict__ w,
+                            double *__restrict__ flmass) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= Fb || btype(bpatch, bctype, b) != FCP_BC_OUTLET) return;
+  const int f = Fi + b, p = owner[f];
+  const size_t jb = (size_t)n + b;
+  const double ub = u[p], vb = v[p], wb = w[p];
+  u[jb] = ub;
+  v[jb] = vb;
+  w[jb] = wb;
+  flmass[f] = den[p] * ((ub * g.arx[f] + vb * g.ary[f]) + wb * g.arz[f]);
+}
+
+// The outlet mass flow summed in the reference's order (patches in order,
+// faces in order: ascending boundary face), then fac = flomas/(flowo+small).
+// One workgroup: the values of 256 faces staged in LDS, summed by one lane.
+__global__ void __launch_bounds__(256) k_pc_flowo(int Fi, int Fb, const int *__restrict__ bpatch,
+                                                  const int *__restrict__ bctype,
+                                                  const double *__restrict__ flmass,
+                                                  double flomas, double *__restrict__ fac) {
+  __shared__ double val[256];
+  __shared__ int use[256];
+  double flowo = 0.0;
+  for (int base = 0; base < Fb; base += 256) {
+    const int b = base + (int)threadIdx.x;
+    const bool o = b < Fb && btype(bpatch, bctype, b) == FCP_BC_OUTLET;
+    use[threadIdx.x] = o ? 1 : 0;
+    val[threadIdx.x] = o ? flmass[Fi + b] : 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int j = 0; j < 256; ++j)
+        if (use[j]) flowo = flowo + val[j];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) fac[0] = flomas / (flowo + FCP_SMALL);
+}
+
+__global__ void k_pc_scale(int n, int Fi, int Fb, const int *__restrict__ bpatch,
+                           const int *__restrict__ bctype, const double *__restrict__ fac,
+                           double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+                           double *__restrict__ flmass) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= Fb || btype(bpatch, bctype, b) != FCP_BC_OUTLET) return;
+  const double fc = fac[0];
+  const size_t jb = (size_t)n + b;
+  flmass[Fi + b] = flmass[Fi + b] * fc;
+  u[jb] = u[jb] * fc;
+  v[jb] = v[jb] * fc;
+  w[jb] = w[jb] * fc;
+}
+
+// The pressure-correction rows (calcp_simple.f90:78-228): every row gathers
+// its faces in ascending order -- inner faces, then its boundary faces, with
+// the periodic pairs whose twin face it owns merged in at their periodic
+// face's place.  The owner of a face writes the face's new mass flux.
+__global__ void k_pc_assemble(int n, int Fi, const int *__restrict__ cf_ptr,
+                              const int *__restrict__ cf, const int *__restrict__ owner,
+                              const int *__restrict__ neighbour, const int *__restrict__ bpatch,
+                              const int *__restrict__ bctype, GeomP g, RcIn in,
+                              const int *__restrict__ slot_pn, const int *__restrict__ slot_np,
+                              const int *__restrict__ diag, double *__restrict__ flmass,
+                              double *__restrict__ a, double *__restrict__ su, PerP per) {
+  const int c = xcd_cell(n);
+  if (c >= n) return;
+  double ad = 0.0, s = 0.0;
+  int tw = per.np > 0 ? per.tw_ptr[c] : 0;
+  const int twe = per.np > 0 ? per.tw_ptr[c + 1] : 0;
+  auto twin_side = [&](int before) {
+    for (; tw < twe && per.fp[per.tw_k[tw]] < before; ++tw) {
+      const int kp = per.tw_k[tw], fp = per.fp[kp];
+      double fm;
+      const double cap = rc_periodic(g, fp, per.ii[kp], owner[fp], c, in, fm);
+      const int sl = slot_np[Fi + kp];
+      if (sl >= 0) a[sl] = cap;
+      ad = ad - cap;
+      s = s + fm;
+    }
+  };
+  for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+    const int f = cf[k];
+    if (f < Fi) {
+      const int p = owner[f], q = neighbour[f];
+      double fm;
+      const double cap = rc_inner(g, f, p, q, in, fm);
+      const int sl = (c == p) ? slot_pn[f] : slot_np[f];
+      if (sl >= 0) a[sl] = cap;
+      if (c == p) {
+        flmass[f] = fm;
+        s = s - fm;
+      } else {
+        s = s + fm;
+      }
+      ad = ad - cap;
+    } else {
+      twin_side(f);
+      const int b = f - Fi, t = btype(bpatch, bctype, b);
+      if (t == FCP_BC_INLET || t == FCP_BC_OUTLET) {
+        s = s - flmass[f];
+      } else if (t == FCP_BC_PERIODIC && per.np > 0 && per.kof[b] >= 0) {
+        const int kp = per.kof[b];
+        double fm;
+        const double cap = rc_periodic(g, f, per.ii[kp], c, owner[per.ft[kp]], in, fm);
+        const int sl = slot_pn[Fi + kp];
+        if (sl >= 0) a[sl] = cap;
+        flmass[f] = fm;
+        ad = ad - cap;
+        s = s - fm;
+      }
+    }
+  }
+  twin_side(0x7fffffff);
+  a[diag[c]] = ad;
+  su[c] = s;
+}
+
+// Mass-flux correction (:333-376): inner faces, and periodic faces whose twin
+// takes the same corrected flux.
+__global__ void k_pc_fluxcorr(int Fi, int F, const int *__restrict__ owner,
+                              const int *__restrict__ neighbour, const int *__restrict__ slot_pn,
+                              const double *__restrict__ a, const double *__restrict__ pp,
+                              double *__restrict__ flmass, PerP per) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= F) return;
+  if (f < Fi) {
+    const int p = owner[f], q = neighbour[f], sl = slot_pn[f];
+    if (sl >= 0) flmass[f] = flmass[f] + a[sl] * (pp[q] - pp[p]);
+  } else if (per.np > 0 && per.kof[f - Fi] >= 0) {
+    const int kp = per.kof[f - Fi], ft = per.ft[kp], sl = slot_pn[Fi + kp];
+    const int p = owner[f], q = owner[ft];
+    if (sl < 0) return;
+    const double fm = flmass[f] + a[sl] * (pp[q] - pp[p]);
+    flmass[f] = fm;
+    flmass[ft] = fm;
+  }
+}
+
+// bpres (bpres.f90): stage 1 copies the owner's value to every boundary face
+// (no pressure patches here); stage 2 extrapolates to wall faces with the
+// stage-1 gradient.
+__global__ void k_pc_bpres(int n, int Fi, int Fb, int stage, const int *__restrict__ owner,
+                           const int *__restrict__ bpatch, const int *__restrict__ bctype,
+                           GeomP g, const double *__restrict__ dp, double *__restrict__ pp) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= Fb) return;
+  const int f = Fi + b, p = owner[f];
+  if (stage == 1) {
+    pp[(size_t)n + b] = pp[p];
+  } else if (btype(bpatch, bctype, b) == FCP_BC_WALL) {
+    const double xpb = g.xf[f] - g.xc[p], ypb = g.yf[f] - g.yc[p], zpb = g.zf[f] - g.zc[p];
+    pp[(size_t)n + b] = ((pp[p] + dp[3 * (size_t)p] * xpb) + dp[3 * (size_t)p + 1] * ypb) +
+                        dp[3 * (size_t)p + 2] * zpb;
+  }
+}
+
+// gradp_and_sources (nablap.f90:19-209), one bpres stage per launch: the
+// inner-face sums -(sum pf S) (first launch: computed and kept in su/sv/sw;
+// later: read back), the gradient (-s + sum_b pb S) / vol, and with `last`
+// the boundary terms s - sum_b pb S, u += su apu, v += sv apv, w += sw apw
+// and p += urfp (pp - pp[pref]) (calcp_simple.f90:398-404).
+__global__ void k_pc_grad(int n, int Fi, const int *__restrict__ cf_ptr,
+                          const int *__restrict__ cf, const int *__restrict__ owner,
+                          const int *__restrict__ neighbour, GeomP g, const double *__restrict__ pp,
+                          int first, int last, double *__restrict__ dp, double *__restrict__ su,
+                          double *__restrict__ sv, double *__restrict__ sw,
+                          const double *__restrict__ apu, const double *__restrict__ apv,
+                          const double *__restrict__ apw, double urfp, int pref,
+                          double *__restrict__ u, double *__restrict__ v, double *__restrict__ w,
+                          double *__restrict__ pr) {
+  const int c = xcd_cell(n);
+  if (c >= n) return;
+  double sx, sy, sz;
+  const int k0 = cf_ptr[c], k1 = cf_ptr[c + 1];
+  if (first) {
+    sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int k = k0; k < k1; ++k) {
+      const int f = cf[k];
+      if (f >= Fi) break;
+      const int p = owner[f], q = neighbour[f];
+      const double pf = pp[p] + (pp[q] - pp[p]) * g.facint[f];
+      const double dfxe = pf * g.arx[f], dfye = pf * g.ary[f], dfze = pf * g.arz[f];
+      if (c == p) {
+        sx = sx - dfxe;
+        sy = sy - dfye;
+        sz = sz - dfze;
+      } else {
+        sx = sx + dfxe;
+        sy = sy + dfye;
+        sz = sz + dfze;
+      }
+    }
+    su[c] = sx;
+    sv[c] = sy;
+    sw[c] = sz;
+  } else {
+    sx = su[c];
+    sy = sv[c];
+    sz = sw[c];
+  }
+  double dx = -sx, dy = -sy, dz = -sz;
+  for (int k = k0; k < k1; ++k) {
+    const int f = cf[k];
+    if (f < Fi) continue;
+    const double pb = pp[(size_t)n + (f - Fi)];
+    dx = dx + pb * g.arx[f];
+    dy = dy + pb * g.ary[f];
+    dz = dz + pb * g.arz[f];
+  }
+  const double volr = 1.0 / g.vol[c];
+  dp[3 * (size_t)c] = dx * volr;
+  dp[3 * (size_t)c + 1] = dy * volr;
+  dp[3 * (size_t)c + 2] = dz * volr;
+  if (!last) return;
+  for (int k = k0; k < k1; ++k) {
+    const int f = cf[k];
+    if (f < Fi) continue;
+    const double pb = pp[(size_t)n + (f - Fi)];
+    sx = sx - pb * g.arx[f];
+    sy = sy - pb * g.ary[f];
+    sz = sz - pb * g.arz[f];
+  }
+  su[c] = sx;
+  sv[c] = sy;
+  sw[c] = sz;
+  u[c] = u[c] + sx * apu[c];
+  v[c] = v[c] + sy * apv[c];
+  w[c] = w[c] + sz * apw[c];
+  pr[c] = pr[c] + urfp * (pp[c] - pp[pref]);
+}
+
+// updateVelocityAtBoundary (velocity.f90:1184-1262): empty and periodic faces
+// take the owner's velocity, symmetry faces its reflection with the
+// (unnormalised) area vector.
+__global__ void k_pc_velbnd(int n, int Fi, int Fb, const int *__restrict__ owner,
+                            const int *__restrict__ bpatch, const int *__restrict__ bctype,
+                            GeomP g, double *__restrict__ u, double *__restrict__ v,
+                            double *__restrict__ w) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= Fb) return;
+  const int t = btype(bpatch, bctype, b);
+  const int f = Fi + b, p = owner[f];
+  const size_t jb = (size_t)n + b;
+  if (t == FCP_BC_EMPTY || t == FCP_BC_PERIODIC) {
+    u[jb] = u[p];
+    v[jb] = v[p];
+    w[jb] = w[p];
+  } else if (t == FCP_BC_SYMMETRY) {
+    const double Unmag = (u[p] * g.arx[f] + v[p] * g.ary[f]) + w[p] * g.arz[f];
+    u[jb] = u[p] - Unmag * g.arx[f];
+    v[jb] = v[p] - Unmag * g.ary[f];
+    w[jb] = w[p] - Unmag * g.arz[f];
+  }
+}
+
 GeomP geom_of(fcp_ctx *ctx) {
   return GeomP{ctx->xc.p, ctx->yc.p, ctx->zc.p, ctx->vol.p, ctx->arx.p, ctx->ary.p, ctx->arz.p,
                ctx->xf.p, ctx->yf.p, ctx->zf.p, ctx->facint.p, ctx->Df.p};
@@ -688,6 +1003,61 @@ int fcp_launch_minmax_clip(fcp_ctx *ctx, double *x, int clip, double *mm) {
   k_minmax<<<nb, BLK, 0, ctx->stream>>>(ctx->n, x, mm);
   k_minmax_final<<<1, 64, 0, ctx->stream>>>(nb, mm);
   if (clip) k_clip_small<<<grid_for(ctx->n, BLK), BLK, 0, ctx->stream>>>(ctx->n, x, mm);
+  FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+int fcp_launch_pcorr_assemble(fcp_ctx *ctx, double *u, double *v, double *w, const double *p,
+                              const double *dpdxi, const double *den, const double *apu,
+                              const double *apv, const double *apw, double *flmass,
+                              int const_mflux, double flomas, double *su) {
+  FCP_TRY(fcp_coarse_join(ctx));
+  ProfScope ps(ctx, "pcorr_assemble");
+  const int n = ctx->n, Fi = ctx->Fi, Fb = ctx->Fb;
+  const GeomP g = geom_of(ctx);
+  if (!const_mflux && Fb > 0) {  // adjustMassFlow
+    FCP_HIP(ctx, ctx->pc_fac.alloc(1));
+    k_pc_outlet<<<grid_for(Fb, BLK), BLK, 0, ctx->stream>>>(n, Fi, Fb, ctx->owner.p, ctx->bpatch.p,
+                                                            ctx->bctype.p, g, den, u, v, w, flmass);
+    k_pc_flowo<<<1, 256, 0, ctx->stream>>>(Fi, Fb, ctx->bpatch.p, ctx->bctype.p, flmass, flomas,
+                                           ctx->pc_fac.p);
+    k_pc_scale<<<grid_for(Fb, BLK), BLK, 0, ctx->stream>>>(n, Fi, Fb, ctx->bpatch.p, ctx->bctype.p,
+                                                           ctx->pc_fac.p, u, v, w, flmass);
+  }
+  const RcIn in{u, v, w, p, dpdxi, den, apu, apv, apw};
+  k_pc_assemble<<<xcd_grid(n), BLK, 0, ctx->stream>>>(
+      n, Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, ctx->bpatch.p,
+      ctx->bctype.p, g, in, ctx->slot_pn.p, ctx->slot_np.p, ctx->diag.p, flmass, ctx->a.p, su,
+      fcp_periodic_view(ctx));
+  FCP_HIP(ctx, hipGetLastError());
+  ctx->a_version++;
+  return 0;
+}
+
+int fcp_launch_pcorr_correct(fcp_ctx *ctx, double *pp, double *flmass, double *u, double *v,
+                             double *w, double *p, double *dpdxi, const double *apu,
+                             const double *apv, const double *apw, double urfp, int pref,
+                             double *su, double *sv, double *sw) {
+  ProfScope ps(ctx, "pcorr_correct");
+  const int n = ctx->n, Fi = ctx->Fi, Fb = ctx->Fb, F = ctx->F;
+  const GeomP g = geom_of(ctx);
+  if (F > 0)
+    k_pc_fluxcorr<<<grid_for(F, BLK), BLK, 0, ctx->stream>>>(Fi, F, ctx->owner.p,
+                                                             ctx->neighbour.p, ctx->slot_pn.p,
+                                                             ctx->a.p, pp, flmass,
+                                                             fcp_periodic_view(ctx));
+  for (int stage = 1; stage <= 2; ++stage) {
+    if (Fb > 0)
+      k_pc_bpres<<<grid_for(Fb, BLK), BLK, 0, ctx->stream>>>(n, Fi, Fb, stage, ctx->owner.p,
+                                                             ctx->bpatch.p, ctx->bctype.p, g,
+                                                             dpdxi, pp);
+    k_pc_grad<<<xcd_grid(n), BLK, 0, ctx->stream>>>(
+        n, Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, g, pp, stage == 1,
+        stage == 2, dpdxi, su, sv, sw, apu, apv, apw, urfp, pref, u, v, w, p);
+  }
+  if (Fb > 0)
+    k_pc_velbnd<<<grid_for(Fb, BLK), BLK, 0, ctx->stream>>>(n, Fi, Fb, ctx->owner.p, ctx->bpatch.p,
+                                                            ctx->bctype.p, g, u, v, w);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
 }

@@ -295,6 +295,8 @@ struct fcp_ctx {
   DBuf<double> lsq_dmat;  // Dmat(9,n) or D(3,6,n)
   DBuf<double> fvx_old;   // [3*(n+Fb)] old gradient of the fvx Gauss sweeps
   DBuf<double> cd_gamf;   // [F] face diffusivity of fcp_convdiff_cell
+  DBuf<double> pc_fac;    // [1] adjustMassFlow's outlet scaling factor
+  DBuf<double> pc_s;      // [3n] su, sv, sw of fcp_pressure_correct when not asked for
   int lsq_method = 0, lsq_compat = 1;
 
   // -------- scalar transport step (calculate_temperature) --------
@@ -440,6 +442,17 @@ int fcp_launch_grad_fvx(fcp_ctx *ctx, const double *u, const double *dfo, double
 int fcp_launch_update_boundary(fcp_ctx *ctx, double *phi);
 // facefluxscalar's face diffusivity gf[F] from the cell array dc[n].
 int fcp_launch_face_gamma_cells(fcp_ctx *ctx, const double *dc, double *gf);
+// Pressure correction of SIMPLE (calcp_simple.f90:69-419) on the resident
+// matrix: the assembly (a, su; flmass and, with adjustMassFlow, the outlet
+// values of u, v, w updated) and the correction from the solved pp.
+int fcp_launch_pcorr_assemble(fcp_ctx *ctx, double *u, double *v, double *w, const double *p,
+                              const double *dpdxi, const double *den, const double *apu,
+                              const double *apv, const double *apw, double *flmass,
+                              int const_mflux, double flomas, double *su);
+int fcp_launch_pcorr_correct(fcp_ctx *ctx, double *pp, double *flmass, double *u, double *v,
+                             double *w, double *p, double *dpdxi, const double *apu,
+                             const double *apv, const double *apw, double urfp, int pref,
+                             double *su, double *sv, double *sw);
 int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
                         const double *grad, double gam, int scheme, const unsigned char *bc_kind,
                         double urf, double *su, double *sp, const int *srd_map = nullptr,

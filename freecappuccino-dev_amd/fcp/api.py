@@ -232,6 +232,29 @@ class Context:
         self._check(rc, "fcp_convdiff_cell")
         return a_out
 
+    def pressure_assemble(self, u, v, w, p, dpdxi, den, apu, apv, apw, flmass, su,
+                          const_mflux=True, flomas=0.0, a_out=None):
+        """calcp_simple's pressure-correction assembly (calcp_simple.f90:
+        69-228); flmass (and, without const_mflux, the outlet values of u, v,
+        w) updated in place, su overwritten; the matrix stays resident."""
+        rc = self.lib.fcp_pressure_assemble(
+            self.h, _ptr(u)[0], _ptr(v)[0], _ptr(w)[0], _ptr(_f64(p))[0], _ptr(_f64(dpdxi))[0],
+            _ptr(_f64(den))[0], _ptr(_f64(apu))[0], _ptr(_f64(apv))[0], _ptr(_f64(apw))[0],
+            _ptr(flmass)[0], 1 if const_mflux else 0,
+            float(flomas), _ptr(a_out)[0], _ptr(su)[0])
+        self._check(rc, "fcp_pressure_assemble")
+        return a_out
+
+    def pressure_correct(self, pp, flmass, u, v, w, p, dpdxi, apu, apv, apw, urfp, pref_cell,
+                         su=None, sv=None, sw=None):
+        """calcp_simple's correction from the solved pp (calcp_simple.f90:
+        333-419); pref_cell 1-based.  pp, flmass, u, v, w, p, dpdxi in/out."""
+        rc = self.lib.fcp_pressure_correct(
+            self.h, _ptr(pp)[0], _ptr(flmass)[0], _ptr(u)[0], _ptr(v)[0], _ptr(w)[0], _ptr(p)[0],
+            _ptr(dpdxi)[0], _ptr(_f64(apu))[0], _ptr(_f64(apv))[0], _ptr(_f64(apw))[0],
+            float(urfp), int(pref_cell), _ptr(su)[0], _ptr(sv)[0], _ptr(sw)[0])
+        self._check(rc, "fcp_pressure_correct")
+
     def ddt(self, scheme, den, dt, phi_o, su, sp, phi_oo=None):
         rc = self.lib.fcp_ddt(self.h, int(scheme), _ptr(_f64(den))[0], float(dt),
                               _ptr(_f64(phi_o))[0], _ptr(_f64(phi_oo))[0], _ptr(su)[0], _ptr(sp)[0])

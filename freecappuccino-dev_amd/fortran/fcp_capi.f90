@@ -156,6 +156,27 @@ module fcp_capi
       type(c_ptr), value :: a  ! c_null_ptr or c_loc of a(nnz)
     end function
 
+    integer(c_int) function fcp_pressure_assemble(ctx, u, v, w, p, dpdxi, den, apu, apv, apw, &
+        flmass, const_mflux, flomas, a, su) bind(C, name='fcp_pressure_assemble')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: u(*), v(*), w(*), p(*), dpdxi(*), den(*), apu(*), apv(*), apw(*)
+      real(c_double) :: flmass(*), su(*)
+      integer(c_int), value :: const_mflux
+      real(c_double), value :: flomas
+      type(c_ptr), value :: a  ! c_null_ptr or c_loc of a(nnz)
+    end function
+
+    integer(c_int) function fcp_pressure_correct(ctx, pp, flmass, u, v, w, p, dpdxi, apu, apv, &
+        apw, urfp, pref_cell, su, sv, sw) bind(C, name='fcp_pressure_correct')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: pp(*), flmass(*), u(*), v(*), w(*), p(*), dpdxi(*), apu(*), apv(*), apw(*)
+      real(c_double), value :: urfp
+      integer(c_int), value :: pref_cell
+      type(c_ptr), value :: su, sv, sw  ! c_null_ptr or c_loc of arrays (numCells)
+    end function
+
     integer(c_int) function fcp_ddt(ctx, scheme, den, dt, phi_o, phi_oo, su, sp) &
         bind(C, name='fcp_ddt')
       import :: c_int, c_double, c_ptr

@@ -23,6 +23,10 @@
 !                              (temperature.f90:118-366), time terms (:82-102)
 !   calculate_scalar(...)      calculate_temperature (temperature.f90:35-385) for a
 !                              scalar with explicit parameters
+!   calcp_simple(...)          calcp_simple.f90:69-419 (incompressible, no pressure
+!                              patches, one corrector) with explicit arguments
+!   fvx_grad_gauss / fvx_grad_gauss_corrected   fvxGradient.f90:1549,1664
+!   fvm_div_cell               facefluxscalar assembly (scalar_fluxes.f90:344-464)
 ! The mesh is handed over with read_mesh (from disk) or fcp_read_mesh (the
 ! arrays read_mesh produces, geometry.f90:118-406); geometry is computed on
 ! the GPU and returned in the module arrays below (geometry.f90:19-83 names).
@@ -297,6 +301,33 @@ contains
     if (trim(cScheme) == 'cds') sch = 0
     call fcp_check(fcp_convdiff_cell(ctx, flmass, diffcoef, phi, dphidxi, gam, sch, bc_kind, &
                                      urf, c_loc(a), su, sp), 'fcp_convdiff_cell')
+  end subroutine
+
+  ! calcp_simple (calcp_simple.f90:69-419) for an incompressible case
+  ! without pressure patches and one corrector, with the reference's module
+  ! globals passed explicitly: the pressure-correction assembly into a, su
+  ! (facefluxmass2, adjustMassFlow unless const_mflux), csrsolve(solver) for
+  ! pp, then the flux, velocity and pressure correction (gradp_and_sources,
+  ! updateVelocityAtBoundary).  resor receives csrsolve's res0.
+  subroutine calcp_simple(u, v, w, p, pp, dPdxi, den, apu, apv, apw, flmass, const_mflux, &
+                          flomas, urfp, pRefCell, solver, itr_max, tol_abs, tol_rel, resor)
+    real(dp), intent(inout) :: u(numTotal), v(numTotal), w(numTotal), p(numTotal), pp(numTotal)
+    real(dp), intent(inout) :: dPdxi(3, numTotal), flmass(numFaces)
+    real(dp), intent(in) :: den(numTotal), apu(numCells), apv(numCells), apw(numCells)
+    logical, intent(in) :: const_mflux
+    real(dp), intent(in) :: flomas, urfp, tol_abs, tol_rel
+    integer, intent(in) :: pRefCell, itr_max
+    character(len=*), intent(in) :: solver
+    real(dp), intent(out) :: resor
+    integer(c_int) :: cm
+    cm = 0
+    if (const_mflux) cm = 1
+    call fcp_check(fcp_pressure_assemble(ctx, u, v, w, p, dPdxi, den, apu, apv, apw, flmass, cm, &
+                                         flomas, c_loc(a), su), 'fcp_pressure_assemble')
+    call csrsolve(solver, pp, su, resor, itr_max, tol_abs, tol_rel, 'p')
+    call fcp_check(fcp_pressure_correct(ctx, pp, flmass, u, v, w, p, dPdxi, apu, apv, apw, urfp, &
+                                        pRefCell, c_loc(su), c_null_ptr, c_null_ptr), &
+                   'fcp_pressure_correct')
   end subroutine
 
   ! Time terms of temperature.f90:82-102 into su, sp (scheme 1 BDF1 / the

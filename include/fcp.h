@@ -217,6 +217,44 @@ int fcp_convdiff_cell(fcp_ctx *ctx, const double *flmass, const double *diffcoef
                       const unsigned char *bc_kind, double urf, double *a, double *su,
                       double *sp);
 
+/* The pressure-correction step of SIMPLE (calcp_simple.f90:69-419) for an
+ * incompressible flow on a mesh without 'pressure' patches
+ * (FCP_ERR_UNSUPPORTED), pressure interpolated 'linear', one corrector
+ * (npcor = 1); the caller solves between the two calls (fcp_csrsolve on the
+ * resident matrix, or lis_spsolve).
+ *
+ * fcp_pressure_assemble (:69-228): per inner face facefluxmass2
+ * (faceflux_mass.f90:175-252) -- coefficient -den_f (V/a_P)_f D_f into both
+ * off-diagonals and subtracted from both diagonals, Rhie-Chow mass flux into
+ * flmass and -/+ flmass into su; unless const_mflux, adjustMassFlow (:833-912:
+ * outlet u, v, w copied from the owner, outlet fluxes den u.S summed in
+ * face order and scaled with u, v, w by flomas/(sum + small)); su -= flmass
+ * on inlet and outlet faces; periodic pairs by facefluxmass2_periodic
+ * (:313-372, which reads Df of inner face i for periodic face i).
+ * u, v, w, p, den [numTotal]; dpdxi [3*numTotal]; apu, apv, apw [numCells]
+ * (1/a_P of the momentum rows); flmass [numFaces] in/out; a (optional) [nnz]
+ * and su [numCells] overwritten.  The matrix stays resident.
+ *
+ * fcp_pressure_correct (:333-419): flmass += a_PN (pp_N - pp_P) on inner and
+ * periodic faces (the twin face takes the periodic face's flux); ppref =
+ * pp(pref_cell) (1-based); gradp_and_sources(pp) (nablap.f90:19-209:
+ * bpres stage 1 copies pp to the boundary faces, stage 2 extrapolates it to
+ * wall faces with the stage-1 gradient; dpdxi's cell rows receive the
+ * gradient of pp, su/sv/sw = -sum_f pp_f S_f); u += su apu, v += sv apv,
+ * w += sw apw, p += urfp (pp - ppref) on the cells; then
+ * updateVelocityAtBoundary (velocity.f90:1184-1262: empty and periodic
+ * faces take the owner's velocity, symmetry faces u_P - (u_P.S) S).  pp
+ * (boundary values updated), flmass, u, v, w, p, dpdxi in/out; su, sv, sw
+ * [numCells] optional outputs.  Uses the resident matrix of the assembly. */
+int fcp_pressure_assemble(fcp_ctx *ctx, double *u, double *v, double *w, const double *p,
+                          const double *dpdxi, const double *den, const double *apu,
+                          const double *apv, const double *apw, double *flmass, int const_mflux,
+                          double flomas, double *a, double *su);
+int fcp_pressure_correct(fcp_ctx *ctx, double *pp, double *flmass, double *u, double *v,
+                         double *w, double *p, double *dpdxi, const double *apu,
+                         const double *apv, const double *apw, double urfp, int pref_cell,
+                         double *su, double *sv, double *sw);
+
 /* Time terms (temperature.f90:82-102): scheme 1 BDF1 (and the volume part of
  * Crank-Nicolson), 2 BDF2; apotime = den*vol/dt; su/sp updated in place. */
 int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt,

@@ -280,9 +280,10 @@ def pcorr_case():
             out[f"{key}_in_{k}"] = v
         Fi = mesh.n_inner
         inlet = [i for i, pt in enumerate(mesh.patch_of_bface()) if mesh.types[pt] == "inlet"]
-        # the inlet mass flow, to six decimals so that its text reads back
-        # as the same double on both sides
-        flomas = float(f"{np.sum(I['flmass'][Fi + np.array(inlet, int)]):.6f}") if inlet else 0.0
+        # the mass flow entering through the inlets (inlet fluxes are
+        # negative: S points out of the domain), to six decimals so that its
+        # text reads back as the same double on both sides
+        flomas = float(f"{-np.sum(I['flmass'][Fi + np.array(inlet, int)]):.6f}") if inlet else 0.0
         out[f"{key}_flomas"] = np.array(flomas)
         for cmf in ((0, 1) if key == "hex" else (1,)):
             rc.run([f"pcorr_assemble u.bin v.bin w.bin p.bin dpdxi.bin den.bin apu.bin apv.bin "
