@@ -635,7 +635,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--size", type=int, default=None,
+                    help="cells per edge (default: 128, tets 96, convdiff 192)")
     ap.add_argument("--cpu-size", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=1)
@@ -652,6 +653,8 @@ def main():
     ap.add_argument("--spmv-size", type=int, default=192,
                     help="mesh size of the SpMV measured beyond the 256 MB Infinity Cache (0: skip)")
     args = ap.parse_args()
+    if args.size is None:
+        args.size = {"tets": 96, "convdiff": 192}.get(args.workload, 128)
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus})", file=sys.stderr)

@@ -1,0 +1,129 @@
+"""The secondary workloads at their full sizes, through size-independent
+properties (the oracle finishes these sizes in seconds where it is used):
+
+- 192^3 momentum-like convection-diffusion (7.1M cells): the projected mass
+  fluxes are face-divergence-free, three columns sharing one matrix reach
+  ||b - A x||_2 / ||b||_2 <= 1e-8 in the batched mixed DILU-BiCGStab (checked
+  with a host SpMV of the assembled matrix), and the batched column equals
+  its single-column solve bit for bit;
+- 96^3 x 6 tetrahedra with permuted cells (5.3M cells): GPU reverse
+  Cuthill-McKee bit-exact with the CPU rule, the renumbered mesh's geometry,
+  CSR, Gauss and least-squares gradients bit-exact with the oracle, and the
+  FP64 Poisson solve (diffusivity 10^U(-2,2)) to 1e-8 checked on the host.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from fcp import workloads
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def api():
+    from fcp import api as _api
+    return _api
+
+
+def _log(t0, msg):
+    print(f"[case] {msg} ({time.time() - t0:.1f}s)", flush=True)
+
+
+def _relres(csr, a, x, b):
+    import scipy.sparse as sps
+    n = b.size
+    A = sps.csr_matrix((a, csr["ja"], csr["ia"]), shape=(n, n))
+    return np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+
+
+def test_convdiff_192_three_columns(api):
+    t0 = time.time()
+    N = 192
+    mesh = workloads.convdiff_mesh(N)
+    ctx = api.Context(0)
+    ctx.read_mesh(mesh)
+    geo = ctx.geometry()
+    csr = ctx.create_CSR_matrix()
+    _log(t0, f"convdiff {N}^3 mesh, geometry, CSR n={mesh.n_cells}")
+    flux, div = workloads.convdiff_flux(mesh, geo, csr, ctx)
+    inp = workloads.convdiff_inputs(mesh, geo, flux)
+    assert np.abs(div).max() <= 1e-10 * np.abs(flux).max(), np.abs(div).max()
+    _log(t0, f"projected flux max|div| {np.abs(div).max():.1e}")
+    n, nt = mesh.n_cells, mesh.n_total
+    rhs = np.empty((3, n))
+    a = np.zeros(csr["ja"].size)
+    for r in range(3):
+        phi = np.array(inp["phi"][r])
+        grad = ctx.grad_gauss(phi)
+        sp = np.zeros(n)
+        rhs[r] = inp["source"][r] * geo["vol"]
+        ctx.convdiff(flux, inp["gamma"], phi, grad, 1.0, 1, inp["bc_kind"], rhs[r], sp,
+                     urf=inp["urf"], a_out=a)
+    _log(t0, "assembled three columns")
+    X = np.zeros((3, n))
+    rc, info, _ = ctx.csrsolve_multi("bicgstab", X, rhs, itr_max=20000, tol_rel=1e-8,
+                                     precision=api.FCP_PREC_MIXED,
+                                     criterion=api.FCP_CRIT_TRUE_L2)
+    assert rc == 0, info
+    _log(t0, f"batched mixed solve iters={info['iterations']}")
+    for r in range(3):
+        rr = _relres(csr, a, X[r], rhs[r])
+        assert rr <= 1e-8, (r, rr)
+    x0 = np.zeros(n)
+    rc, info1 = ctx.csrsolve("bicgstab", x0, rhs[0], itr_max=20000, tol_rel=1e-8,
+                             precision=api.FCP_PREC_MIXED, criterion=api.FCP_CRIT_TRUE_L2)
+    assert rc == 0 and np.array_equal(x0, X[0])
+    _log(t0, "single column equals batched column 0")
+    ctx.close()
+
+
+def test_tets_96_rcm_geometry_gradients_poisson(api):
+    from fcp.mesh import permute_cells
+    t0 = time.time()
+    mesh0 = workloads.tet_mesh_workload(96)
+    _log(t0, f"tets 96^3x6 mesh n={mesh0.n_cells}")
+    ctx0 = api.Context(0)
+    ctx0.read_mesh(mesh0)
+    ctx0.geometry()
+    ctx0.create_CSR_matrix()
+    perm = ctx0.rcm(4)
+    ctx0.close()
+    o0 = Oracle(mesh0)
+    assert np.array_equal(perm, o0.rcm(4))
+    del o0
+    _log(t0, "GPU RCM equals the CPU rule")
+    mesh = permute_cells(mesh0, perm)
+    del mesh0
+    ctx = api.Context(0)
+    ctx.read_mesh(mesh)
+    geo = ctx.geometry()
+    csr = ctx.create_CSR_matrix()
+    o = Oracle(mesh)
+    for k in ("xc", "yc", "zc", "vol", "arx", "ary", "arz", "xf", "yf", "zf", "facint", "Df"):
+        assert np.array_equal(geo[k], o.geom[k]), k
+    for k in ("ia", "ja", "diag", "slot_pn", "slot_np"):
+        assert np.array_equal(csr[k], getattr(o, k)), k
+    _log(t0, "renumbered geometry and CSR equal the oracle's")
+    inp = workloads.tet_inputs(mesh, geo)
+    f = inp["f"]
+    assert np.array_equal(ctx.grad_gauss(f), o.grad_gauss(f))
+    for method in (1, 2):
+        ctx.create_lsq_grad_matrix(method, compat=False)
+        assert np.array_equal(ctx.grad(method, f), o.grad_lsq(method - 1, f, compat=0)), method
+    _log(t0, "Gauss and least-squares gradients equal the oracle's")
+    n = mesh.n_cells
+    su = inp["source"] * geo["vol"]
+    a = np.zeros(csr["ja"].size)
+    ctx.laplacian_face(inp["mu_f"], inp["bc_kind"], inp["phi_bc"], su, a_out=a)
+    b = su.copy()
+    x = np.zeros(n)
+    rc, info = ctx.csrsolve("iccg", x, su, itr_max=20000, tol_rel=1e-8,
+                            precision=api.FCP_PREC_FP64, criterion=api.FCP_CRIT_TRUE_L2)
+    assert rc == 0, info
+    rr = _relres(csr, a, x, b)
+    assert rr <= 1e-8, rr
+    _log(t0, f"FP64 ICCG iters={info['iterations']} host rel. residual {rr:.2e}")
+    ctx.close()
