@@ -1681,28 +1681,12 @@ static void orc_bpres(const orc_mesh *m, const orc_geom *g, double *pp, const do
   }
 }
 
-void orc_pcorr_correct(const orc_mesh *m, const orc_geom *g, const int *slot_pn,
-                       const double *a, double *pp, double *flmass, double *u, double *v,
-                       double *w, double *pr, const double *apu, const double *apv,
-                       const double *apw, double urfp, int pref, double *dpdxi, double *su,
-                       double *sv, double *sw) {
+/* gradp_and_sources(p), nablap.f90:19-209 with pscheme 'linear': su/sv/sw =
+ * -sum_f p_f S_f, p's boundary values by bpres's two stages, dp's cell rows
+ * the gradient of p. */
+static void gradp_sources(const orc_mesh *m, const orc_geom *g, double *pp, double *dpdxi,
+                          double *su, double *sv, double *sw) {
   const int n = m->n_cells, Fi = m->n_inner, F = m->n_faces;
-  for (int f = 0; f < Fi; ++f) { /* :333-345 */
-    const int p = m->owner[f], q = m->neighbour[f];
-    flmass[f] = flmass[f] + a[slot_pn[f]] * (pp[q] - pp[p]);
-  }
-  int iper = 0, l = Fi;
-  for (int ib = 0; ib < m->n_patches; ++ib) { /* :348-376 */
-    if (m->bctype[ib] != 7) continue;
-    const int t0 = m->twin_start[iper++];
-    for (int i = 0; i < m->nfaces[ib]; ++i, ++l) {
-      const int f = m->start_face[ib] + i, p = m->owner[f], q = m->owner[t0 + i];
-      flmass[f] = flmass[f] + a[slot_pn[l]] * (pp[q] - pp[p]);
-      flmass[t0 + i] = flmass[f];
-    }
-  }
-  const double ppref = pp[pref]; /* :395 */
-  /* gradp_and_sources(pp), nablap.f90:19-209 with pscheme 'linear' */
   memset(su, 0, sizeof(double) * (size_t)n);
   memset(sv, 0, sizeof(double) * (size_t)n);
   memset(sw, 0, sizeof(double) * (size_t)n);
@@ -1743,13 +1727,11 @@ void orc_pcorr_correct(const orc_mesh *m, const orc_geom *g, const int *slot_pn,
     sv[p] = sv[p] - pp[b] * g->ary[f];
     sw[p] = sw[p] - pp[b] * g->arz[f];
   }
-  for (int c = 0; c < n; ++c) { /* :401-404 */
-    u[c] = u[c] + su[c] * apu[c];
-    v[c] = v[c] + sv[c] * apv[c];
-    w[c] = w[c] + sw[c] * apw[c];
-    pr[c] = pr[c] + urfp * (pp[c] - ppref);
-  }
-  /* updateVelocityAtBoundary, velocity.f90:1184-1262 */
+}
+
+/* updateVelocityAtBoundary, velocity.f90:1184-1262 */
+static void vel_bnd(const orc_mesh *m, const orc_geom *g, double *u, double *v, double *w) {
+  const int n = m->n_cells, Fi = m->n_inner;
   for (int ib = 0; ib < m->n_patches; ++ib) {
     const int t = m->bctype[ib];
     if (t != 6 && t != 7 && t != 2) continue;
@@ -1767,4 +1749,293 @@ void orc_pcorr_correct(const orc_mesh *m, const orc_geom *g, const int *slot_pn,
       }
     }
   }
+}
+
+void orc_pcorr_correct(const orc_mesh *m, const orc_geom *g, const int *slot_pn,
+                       const double *a, double *pp, double *flmass, double *u, double *v,
+                       double *w, double *pr, const double *apu, const double *apv,
+                       const double *apw, double urfp, int pref, double *dpdxi, double *su,
+                       double *sv, double *sw) {
+  const int n = m->n_cells, Fi = m->n_inner;
+  for (int f = 0; f < Fi; ++f) { /* :333-345 */
+    const int p = m->owner[f], q = m->neighbour[f];
+    flmass[f] = flmass[f] + a[slot_pn[f]] * (pp[q] - pp[p]);
+  }
+  int iper = 0, l = Fi;
+  for (int ib = 0; ib < m->n_patches; ++ib) { /* :348-376 */
+    if (m->bctype[ib] != 7) continue;
+    const int t0 = m->twin_start[iper++];
+    for (int i = 0; i < m->nfaces[ib]; ++i, ++l) {
+      const int f = m->start_face[ib] + i, p = m->owner[f], q = m->owner[t0 + i];
+      flmass[f] = flmass[f] + a[slot_pn[l]] * (pp[q] - pp[p]);
+      flmass[t0 + i] = flmass[f];
+    }
+  }
+  const double ppref = pp[pref]; /* :395 */
+  gradp_sources(m, g, pp, dpdxi, su, sv, sw);
+  for (int c = 0; c < n; ++c) { /* :401-404 */
+    u[c] = u[c] + su[c] * apu[c];
+    v[c] = v[c] + sv[c] * apv[c];
+    w[c] = w[c] + sw[c] * apw[c];
+    pr[c] = pr[c] + urfp * (pp[c] - ppref);
+  }
+  vel_bnd(m, g, u, v, w);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Momentum assembly of calcuvw: velocity.f90:50-750                        */
+/* ------------------------------------------------------------------------ */
+/* sngrad (gradients.f90:1720-1781): the face-interpolated gradient, its
+ * corrected form d_ii and the non-orthogonal part d_i (S - D_f d). */
+static void orc_sngrad(const orc_geom *g, int f, int p, int q, const double *phi,
+                       const double *gr, double *di, double *dii) {
+  const double lam = g->facint[f];
+  const double fxn = lam, fxp = 1.0 - lam;
+  const double xpn = g->xc[q] - g->xc[p], ypn = g->yc[q] - g->yc[p], zpn = g->zc[q] - g->zc[p];
+  const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
+  const double vole = (xpn * ax + ypn * ay) + zpn * az;
+  double dx = gr[3 * p] * fxp + gr[3 * q] * fxn;
+  double dy = gr[3 * p + 1] * fxp + gr[3 * q + 1] * fxn;
+  double dz = gr[3 * p + 2] * fxp + gr[3 * q + 2] * fxn;
+  const double e = (((phi[q] - phi[p]) - dx * xpn) - dy * ypn) - dz * zpn;
+  dii[0] = dx + ax / vole * e;
+  dii[1] = dy + ay / vole * e;
+  dii[2] = dz + az / vole * e;
+  di[0] = dx * (ax - g->Df[f] * xpn);
+  di[1] = dy * (ay - g->Df[f] * ypn);
+  di[2] = dz * (az - g->Df[f] * zpn);
+}
+
+/* The corrected boundary-face gradient of facefluxuvw_bnd (velocity.f90:
+ * 916-1037): owner gradient, corrected by the face value. */
+static void orc_snbnd(const double *gr, int p, int b, const double *phi, double xpn, double ypn,
+                      double zpn, double ax, double ay, double az, double vole, double Dfi,
+                      double *di, double *dii) {
+  const double dx = gr[3 * p], dy = gr[3 * p + 1], dz = gr[3 * p + 2];
+  const double e = (((phi[b] - phi[p]) - dx * xpn) - dy * ypn) - dz * zpn;
+  dii[0] = dx + ax / vole * e;
+  dii[1] = dy + ay / vole * e;
+  dii[2] = dz + az / vole * e;
+  di[0] = dx * (ax - Dfi * xpn);
+  di[1] = dy * (ay - Dfi * ypn);
+  di[2] = dz * (az - Dfi * zpn);
+}
+
+int orc_momentum(const orc_mesh *m, const orc_geom *g, const orc_csr *c, const int *slot_pn,
+                 const int *slot_np, double *u, double *v, double *w, const double *uo,
+                 const double *vo, const double *wo, const double *uoo, const double *voo,
+                 const double *woo, double *pr, const double *den, const double *vis,
+                 const double *visw, const double *flmass, double viscos, double dt,
+                 int time_scheme, int scheme, double gds, const double *urf, double *a,
+                 double *dpdxi, double *dudxi, double *dvdxi, double *dwdxi, double *rhs,
+                 double *diag3, double *ap3, double *sp3, double *sv_out, double *sw_out) {
+  const int n = m->n_cells, Fi = m->n_inner;
+  for (int ib = 0; ib < m->n_patches; ++ib)
+    if (m->bctype[ib] == 7 && m->nfaces[ib] > Fi) return -1;
+  double *su = (double *)calloc(3 * (size_t)n, sizeof(double));
+  if (!su) return -1;
+  double *sv = su + n, *sw = su + 2 * (size_t)n;
+  double *spu = sp3, *spv = sp3 + n, *sp = sp3 + 2 * (size_t)n;
+  memset(sp3, 0, sizeof(double) * 3 * (size_t)n);
+  const double *U[3] = {u, v, w}, *G[3] = {dudxi, dvdxi, dwdxi};
+  vel_bnd(m, g, u, v, w); /* :153 */
+  orc_grad_gauss(m, g, u, dudxi);
+  orc_grad_gauss(m, g, v, dvdxi);
+  orc_grad_gauss(m, g, w, dwdxi);
+  gradp_sources(m, g, pr, dpdxi, su, sv, sw); /* :160 */
+  if (time_scheme > 0) /* :162-214 */
+    for (int i = 0; i < n; ++i) {
+      const double apotime = den[i] * g->vol[i] / dt;
+      if (time_scheme == 1 || time_scheme == 3) {
+        su[i] = su[i] + apotime * uo[i];
+        sv[i] = sv[i] + apotime * vo[i];
+        sw[i] = sw[i] + apotime * wo[i];
+        spu[i] = spu[i] + apotime;
+        spv[i] = spv[i] + apotime;
+        sp[i] = sp[i] + apotime;
+      } else {
+        su[i] = su[i] + apotime * (2.0 * uo[i] - 0.5 * uoo[i]);
+        sv[i] = sv[i] + apotime * (2.0 * vo[i] - 0.5 * voo[i]);
+        sw[i] = sw[i] + apotime * (2.0 * wo[i] - 0.5 * woo[i]);
+        spu[i] = spu[i] + 1.5 * apotime;
+        spv[i] = spv[i] + 1.5 * apotime;
+        sp[i] = sp[i] + 1.5 * apotime;
+      }
+    }
+  for (int f = 0; f < Fi; ++f) { /* facefluxuvw :754-878, scatter :216-240 */
+    const int p = m->owner[f], q = m->neighbour[f];
+    const double lam = g->facint[f], fxn = lam, fxp = 1.0 - lam;
+    const double fm = flmass[f];
+    const double game = vis[p] + (vis[q] - vis[p]) * lam;
+    const double de = game * g->Df[f];
+    const double ce = fm < 0.0 ? fm : 0.0, cp = fm > 0.0 ? fm : 0.0;
+    const double can = -de + ce, cap = -de - cp;
+    double di[3][3], dii[3][3];
+    for (int r = 0; r < 3; ++r) orc_sngrad(g, f, p, q, U[r], G[r], di[r], dii[r]);
+    const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
+    double sadd[3];
+    for (int r = 0; r < 3; ++r) {
+      /* fdue = game (du_ii_x ax + dv_ii_x ay + dw_ii_x az), transposed */
+      double fde = game * ((dii[0][r] * ax + dii[1][r] * ay) + dii[2][r] * az);
+      const double fdi = game * ((di[r][0] + di[r][1]) + di[r][2]);
+      fde = fde + fdi;
+      const double fuds = cp * U[r][p] + ce * U[r][q];
+      const double ue = fm >= 0.0 ? face_value(scheme, p, q, g->xf[f], g->yf[f], g->zf[f], fxp,
+                                               U[r], G[r], g)
+                                  : face_value(scheme, q, p, g->xf[f], g->yf[f], g->zf[f], fxn,
+                                               U[r], G[r], g);
+      const double fhigh = fm * ue;
+      sadd[r] = -(gds * (fhigh - fuds)) + fde;
+    }
+    a[slot_pn[f]] = can;
+    a[slot_np[f]] = cap;
+    su[p] = su[p] + sadd[0];
+    sv[p] = sv[p] + sadd[1];
+    sw[p] = sw[p] + sadd[2];
+    su[q] = su[q] - sadd[0];
+    sv[q] = sv[q] - sadd[1];
+    sw[q] = sw[q] - sadd[2];
+  }
+  int iper = 0, iwall = 0, l = Fi;
+  for (int ib = 0; ib < m->n_patches; ++ib) { /* :242-368 */
+    const int t = m->bctype[ib];
+    for (int i = 0; i < m->nfaces[ib]; ++i) {
+      const int f = m->start_face[ib] + i, p = m->owner[f], b = n + (f - Fi);
+      const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
+      if (t == 3 || t == 4 || t == 5) { /* facefluxuvw_bnd :882-1037 */
+        const double xpn = g->xf[f] - g->xc[p], ypn = g->yf[f] - g->yc[p],
+                     zpn = g->zf[f] - g->zc[p];
+        const double vole = (xpn * ax + ypn * ay) + zpn * az;
+        const double Dfi = ((ax * ax + ay * ay) + az * az) / vole;
+        const double game = vis[b];
+        const double de = game * Dfi;
+        const double fm = flmass[f];
+        const double cb = -de + (fm < 0.0 ? fm : 0.0);
+        double di[3][3], dii[3][3];
+        for (int r = 0; r < 3; ++r)
+          orc_snbnd(G[r], p, b, U[r], xpn, ypn, zpn, ax, ay, az, vole, Dfi, di[r], dii[r]);
+        double sadd[3];
+        for (int r = 0; r < 3; ++r) {
+          const double fde = game * ((dii[0][r] * ax + dii[1][r] * ay) + dii[2][r] * az);
+          const double fdi = game * ((di[r][0] + di[r][1]) + di[r][2]);
+          sadd[r] = fde + fdi;
+        }
+        spu[p] = spu[p] - cb;
+        spv[p] = spv[p] - cb;
+        sp[p] = sp[p] - cb;
+        su[p] = su[p] - cb * u[b] + sadd[0];
+        sv[p] = sv[p] - cb * v[b] + sadd[1];
+        sw[p] = sw[p] - cb * w[b] + sadd[2];
+      } else if (t == 2) { /* symmetry :281-306 (vis(inp) with inp = numCells+1) */
+        const double are = sqrt((ax * ax + ay * ay) + az * az);
+        const double arer = 1.0 / are;
+        const double nxf = ax * arer, nyf = ay * arer, nzf = az * arer;
+        const double dpb = ((g->xf[f] - g->xc[p]) * nxf + (g->yf[f] - g->yc[p]) * nyf) +
+                           (g->zf[f] - g->zc[p]) * nzf;
+        const double cf = 2.0 * vis[n] * are / dpb;
+        su[p] = su[p] - cf * nxf * (nyf * v[p] + nzf * w[p]);
+        sv[p] = sv[p] - cf * nyf * (nxf * u[p] + nzf * w[p]);
+        sw[p] = sw[p] - cf * nzf * (nxf * u[p] + nyf * v[p]);
+        spu[p] = spu[p] + cf * (nxf * nxf);
+        spv[p] = spv[p] + cf * (nyf * nyf);
+        sp[p] = sp[p] + cf * (nzf * nzf);
+      } else if (t == 7) { /* facefluxuvw_periodic :1038-1180 */
+        const int q = m->owner[m->twin_start[iper] + i];
+        const double fxn = 0.5, fxp = fxn;
+        const double xpn = 2.0 * (g->xf[f] - g->xc[p]), ypn = 2.0 * (g->yf[f] - g->yc[p]),
+                     zpn = 2.0 * (g->zf[f] - g->zc[p]);
+        const double dpn = sqrt((xpn * xpn + ypn * ypn) + zpn * zpn);
+        const double are = sqrt((ax * ax + ay * ay) + az * az);
+        const double game = vis[p] * fxp + vis[q] * fxn;
+        const double de = game * ((ax * ax + ay * ay) + az * az) / ((xpn * ax + ypn * ay) + zpn * az);
+        const double fm = flmass[f];
+        const double ce = fm < 0.0 ? fm : 0.0, cp = fm > 0.0 ? fm : 0.0;
+        const double can = -de + ce, cap = -de - cp;
+        double d[3][3];
+        for (int r = 0; r < 3; ++r)
+          for (int k = 0; k < 3; ++k) d[r][k] = G[r][3 * p + k] * fxp + G[r][3 * q + k] * fxn;
+        /* d[0] = (duxi, duyi, duzi), d[1] = dv.., d[2] = dw.. */
+        const double fde[3] = {
+            game * (((d[0][0] + d[0][0]) * ax + (d[0][1] + d[1][0]) * ay) + (d[0][2] + d[2][0]) * az),
+            game * (((d[0][1] + d[1][0]) * ax + (d[1][1] + d[1][1]) * ay) + (d[1][2] + d[2][1]) * az),
+            game * (((d[0][2] + d[2][0]) * ax + (d[2][1] + d[1][2]) * ay) + (d[2][2] + d[2][2]) * az)};
+        double sadd[3];
+        for (int r = 0; r < 3; ++r) {
+          const double fdi = game * are / dpn * ((d[r][0] * xpn + d[r][1] * ypn) + d[r][2] * zpn);
+          const double fuds = cp * U[r][p] + ce * U[r][q];
+          const double ue = fm >= 0.0 ? U[r][p] + (U[r][q] - U[r][p]) * fxp
+                                      : U[r][q] + (U[r][p] - U[r][q]) * fxn;
+          const double fhigh = fm * ue;
+          sadd[r] = (-(gds * (fhigh - fuds)) + fde[r]) - fdi;
+        }
+        a[slot_pn[l]] = can;
+        a[slot_np[l]] = cap;
+        ++l;
+        su[p] = su[p] + sadd[0];
+        sv[p] = sv[p] + sadd[1];
+        sw[p] = sw[p] + sadd[2];
+        su[q] = su[q] - sadd[0];
+        sv[q] = sv[q] - sadd[1];
+        sw[q] = sw[q] - sadd[2];
+      } else if (t == 1) { /* wall :330-360 */
+        const double viss = viscos > visw[iwall] ? viscos : visw[iwall];
+        ++iwall;
+        const double are = sqrt((ax * ax + ay * ay) + az * az);
+        const double arer = 1.0 / are;
+        const double nxf = ax * arer, nyf = ay * arer, nzf = az * arer;
+        const double dpb = ((g->xf[f] - g->xc[p]) * nxf + (g->yf[f] - g->yc[p]) * nyf) +
+                           (g->zf[f] - g->zc[p]) * nzf;
+        const double vsol = viss * are / dpb;
+        const double upb = u[p] - u[b], vpb = v[p] - v[b], wpb = w[p] - w[b];
+        spu[p] = spu[p] + vsol * (1.0 - nxf * nxf);
+        spv[p] = spv[p] + vsol * (1.0 - nyf * nyf);
+        sp[p] = sp[p] + vsol * (1.0 - nzf * nzf);
+        su[p] = su[p] + vsol * ((u[b] * (1.0 - nxf * nxf) + vpb * nyf * nxf) + wpb * nzf * nxf);
+        sv[p] = sv[p] + vsol * ((upb * nxf * nyf + v[b] * (1.0 - nyf * nyf)) + wpb * nzf * nyf);
+        sw[p] = sw[p] + vsol * ((upb * nxf * nzf + vpb * nyf * nzf) + w[b] * (1.0 - nzf * nzf));
+      }
+    }
+    if (t == 7) ++iper;
+  }
+  /* CN (:375-390) and the three diagonals (:392-450), component by
+   * component; the row sums read whatever the diagonal entry holds */
+  const double *O[3] = {uo, vo, wo};
+  double *S[3] = {su, sv, sw}, *SP[3] = {spu, spv, sp};
+  if (time_scheme == 3)
+    for (int k = 0; k < c->nnz; ++k) a[k] = 0.5 * a[k];
+  for (int r = 0; r < 3; ++r) {
+    if (time_scheme == 3) {
+      for (int f = 0; f < Fi; ++f) {
+        const int p = m->owner[f], q = m->neighbour[f];
+        S[r][p] = S[r][p] - a[slot_pn[f]] * O[r][q];
+        S[r][q] = S[r][q] - a[slot_np[f]] * O[r][p];
+      }
+      for (int i = 0; i < n; ++i) {
+        const double apotime = den[i] * g->vol[i] / dt;
+        double s = 0.0;
+        for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) s = s + a[k];
+        const double off = s - a[c->diag[i]];
+        S[r][i] = S[r][i] + (apotime + off) * O[r][i];
+        SP[r][i] = SP[r][i] + apotime;
+      }
+    }
+    if (r > 0)
+      for (int i = 0; i < n; ++i) a[c->diag[i]] = 0.0;
+    const double urfr = 1.0 / urf[r], urfm = 1.0 - urf[r];
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int k = c->ia[i]; k < c->ia[i + 1]; ++k) s = s + a[k];
+      const double off = s - a[c->diag[i]];
+      double ad = SP[r][i] - off;
+      ap3[r * (size_t)n + i] = 1.0 / (ad + SMALL);
+      ad = ad * urfr;
+      a[c->diag[i]] = ad;
+      diag3[r * (size_t)n + i] = ad;
+      rhs[r * (size_t)n + i] = S[r][i] + urfm * ad * U[r][i];
+    }
+  }
+  memcpy(sv_out, sv, sizeof(double) * (size_t)n);
+  memcpy(sw_out, sw, sizeof(double) * (size_t)n);
+  free(su);
+  return 0;
 }

@@ -282,6 +282,29 @@ void orc_pcorr_correct(const orc_mesh *m, const orc_geom *g, const int *slot_pn,
                        const double *apw, double urfp, int pref, double *dpdxi, double *su,
                        double *sv, double *sw);
 
+/* The assembly of calcuvw (velocity.f90:50-750) before its three solves,
+ * steady / BDF1 / BDF2 / Crank-Nicolson (time_scheme 0/1/2/3), no
+ * buoyancy, Lorentz force, PISO or constant mass flux, Gauss gradients:
+ * updateVelocityAtBoundary, grad(u, v, w), gradp_and_sources(p), time terms,
+ * facefluxuvw on the inner faces, the boundary loop (inlet/outlet/pressure
+ * facefluxuvw_bnd, symmetry, periodic facefluxuvw_periodic, wall with
+ * visw[wall face] in wall order), CN and the diagonal of every component
+ * with its URF.  a[nnz] on entry holds the matrix's previous content (the
+ * reference never clears it, and the u row sums read its diagonal); on
+ * return the shared off-diagonals and w's diagonal.  rhs, diag3, ap3, sp3
+ * [3n] (u, v, w blocks): the right-hand sides, URF-scaled diagonals, 1/a_P
+ * and the diagonal sources; sv_out/sw_out the v and w sources before their
+ * URF terms.  u, v, w, pr: boundary values updated; dpdxi, dudxi, dvdxi,
+ * dwdxi [3*numTotal]. */
+int orc_momentum(const orc_mesh *m, const orc_geom *g, const orc_csr *c, const int *slot_pn,
+                 const int *slot_np, double *u, double *v, double *w, const double *uo,
+                 const double *vo, const double *wo, const double *uoo, const double *voo,
+                 const double *woo, double *pr, const double *den, const double *vis,
+                 const double *visw, const double *flmass, double viscos, double dt,
+                 int time_scheme, int scheme, double gds, const double *urf, double *a,
+                 double *dpdxi, double *dudxi, double *dvdxi, double *dwdxi, double *rhs,
+                 double *diag3, double *ap3, double *sp3, double *sv_out, double *sw_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -245,6 +245,41 @@ class Oracle:
                                 _p(o["sv"]), _p(o["sw"]))
         return o
 
+    TIME_SCHEMES = {"steady": 0, "bdf": 1, "bdf2": 2, "cn": 3}
+
+    def momentum(self, I, astale, viscos, dt, tscheme, scheme, gds, urf):
+        """calcuvw's assembly (velocity.f90:50-750) before its solves.
+        I: dict of u, v, w (numTotal), uo, vo, wo, uoo, voo, woo (cells), p, den,
+        vis (numTotal), visw (wall faces), flmass.  Returns a dict."""
+        m = self.mesh
+        n, nt = m.n_cells, m.n_total
+        f64 = lambda x: np.array(x, dtype=np.float64, copy=True).reshape(-1)  # noqa: E731
+        o = {k: f64(I[k]) for k in ("u", "v", "w", "p")}
+        keep = {k: f64(I[k]) for k in ("uo", "vo", "wo", "uoo", "voo", "woo", "den", "vis", "visw",
+                                        "flmass")}
+        o["a"] = f64(astale)
+        for k in ("dpdxi", "dudxi", "dvdxi", "dwdxi"):
+            o[k] = np.zeros(3 * nt)
+        for k in ("rhs", "diag", "ap", "sp"):
+            o[k] = np.zeros(3 * n)
+        o["sv"], o["sw"] = np.zeros(n), np.zeros(n)
+        sch = {"cds": 0, "linearUpwind": 1}[scheme] if isinstance(scheme, str) else int(scheme)
+        ts = self.TIME_SCHEMES[tscheme] if isinstance(tscheme, str) else int(tscheme)
+        urf = np.ascontiguousarray(urf, np.float64)
+        rc = lib().orc_momentum(C.byref(self.cm), C.byref(self.cg), C.byref(self.cc),
+                                _p(self.slot_pn), _p(self.slot_np), _p(o["u"]), _p(o["v"]),
+                                _p(o["w"]), *[_p(keep[k]) for k in ("uo", "vo", "wo", "uoo", "voo",
+                                                                     "woo")],
+                                _p(o["p"]), _p(keep["den"]), _p(keep["vis"]), _p(keep["visw"]),
+                                _p(keep["flmass"]), C.c_double(viscos), C.c_double(dt), ts, sch,
+                                C.c_double(gds), _p(urf), _p(o["a"]), _p(o["dpdxi"]),
+                                _p(o["dudxi"]), _p(o["dvdxi"]), _p(o["dwdxi"]), _p(o["rhs"]),
+                                _p(o["diag"]), _p(o["ap"]), _p(o["sp"]), _p(o["sv"]), _p(o["sw"]))
+        assert rc == 0, rc
+        for k in ("rhs", "diag", "ap", "sp"):
+            o[k] = o[k].reshape(3, n)
+        return o
+
     def transport_dcoef(self, vis, bc_kind, viscos, pranl, sigt):
         d = np.zeros(self.mesh.n_faces)
         lib().orc_transport_dcoef(C.byref(self.cm), C.byref(self.cg),

@@ -51,6 +51,15 @@
 !               gradp_and_sources (nablap.f90:19-209, pscheme 'linear', bpres
 !               stages), velocity and pressure correction, then
 !               updateVelocityAtBoundary velocity.f90:1184-1262)
+!   momentum u v w uo vo wo uoo voo p den vis visw flmass astale viscos dt tscheme scheme gds
+!            urf1 urf2 urf3 maxiter tolrel
+!            a_out su_out sv_out sw_out spu_out spv_out sp_out apu_out apv_out apw_out
+!            u_out v_out w_out p_out dpdxi_out dudxi_out
+!              (calcuvw, velocity.f90:50-750, on module globals set from the files:
+!               woo = uoo/2; a holds astale before the call -- calcuvw never clears it and
+!               the u row sums read its diagonal; tscheme steady|bdf|bdf2|cn;
+!               bicgstab with maxiter iterations, 0 leaves u, v, w unsolved;
+!               no buoyancy, Lorentz force, PISO or constant mass flux)
 ! Binary files are raw little-endian stream files (real(8) / integer(4)).
 program ref_driver
   use types
@@ -122,6 +131,8 @@ program ref_driver
       call job_pcorr_assemble(ju)
     case ('pcorr_correct')
       call job_pcorr_correct(ju)
+    case ('momentum')
+      call job_momentum(ju)
     case default
       read(ju, *) cmd
       write(*,*) 'ref_driver: unknown job ', trim(cmd)
@@ -699,6 +710,91 @@ contains
     call wr8(fsu, su, numCells)
     call wr8(fsv, sv, numCells)
     call wr8(fsw, sw, numCells)
+  end subroutine
+
+  subroutine job_momentum(ju)
+    use variables, only: u, v, w, uo, vo, wo, uoo, voo, woo, p, den, vis, visw, flmass, &
+                         dUdxi, dVdxi, dWdxi, dPdxi
+    use velocity, only: calcuvw, urfU, gdsU, cSchemeU, lSolverU, maxiterU, tolAbsU, tolRelU
+    use mhd, only: calcEpot
+    integer, intent(in) :: ju
+    character(len=64) :: c, fu, fv, fw, fuo, fvo, fwo, fuoo, fvoo, fp, fden, fvis, fvisw, ffm, fas
+    character(len=64) :: fa, fsu, fsv, fsw, fspu, fspv, fsp, fapu, fapv, fapw, fuo2, fvo2, fwo2
+    character(len=64) :: fpo, fdpo, fduo
+    character(len=30) :: tsch, schm
+    real(dp) :: vsc, dtt, gds, u1, u2, u3, trel
+    integer :: itmax
+    read(ju, *) c, fu, fv, fw, fuo, fvo, fwo, fuoo, fvoo, fp, fden, fvis, fvisw, ffm, fas, vsc, dtt, &
+                tsch, schm, gds, u1, u2, u3, itmax, trel, fa, fsu, fsv, fsw, fspu, fspv, fsp, fapu, &
+                fapv, fapw, fuo2, fvo2, fwo2, fpo, fdpo, fduo
+    if (allocated(u)) deallocate(u, v, w, p, den, flmass)
+    if (allocated(dPdxi)) deallocate(dPdxi)
+    if (allocated(uo)) deallocate(uo, vo, wo, uoo, voo, woo)
+    if (allocated(vis)) deallocate(vis)
+    if (allocated(visw)) deallocate(visw)
+    if (allocated(dUdxi)) deallocate(dUdxi, dVdxi, dWdxi)
+    allocate(u(numTotal), v(numTotal), w(numTotal), p(numTotal), den(numTotal), flmass(numFaces), &
+             dPdxi(3,numTotal), uo(numTotal), vo(numTotal), wo(numTotal), uoo(numTotal), &
+             voo(numTotal), woo(numTotal), vis(numTotal), visw(max(nwal,1)), dUdxi(3,numTotal), &
+             dVdxi(3,numTotal), dWdxi(3,numTotal))
+    uo = 0.0_dp; vo = 0.0_dp; wo = 0.0_dp; uoo = 0.0_dp; voo = 0.0_dp; woo = 0.0_dp
+    dPdxi = 0.0_dp; dUdxi = 0.0_dp; dVdxi = 0.0_dp; dWdxi = 0.0_dp; visw = 0.0_dp
+    call rd8(fu, u, numTotal)
+    call rd8(fv, v, numTotal)
+    call rd8(fw, w, numTotal)
+    call rd8(fuo, uo, numCells)
+    call rd8(fvo, vo, numCells)
+    call rd8(fwo, wo, numCells)
+    call rd8(fuoo, uoo, numCells)
+    call rd8(fvoo, voo, numCells)
+    woo = 0.5_dp*uoo
+    call rd8(fp, p, numTotal)
+    call rd8(fden, den, numTotal)
+    call rd8(fvis, vis, numTotal)
+    if (nwal > 0) call rd8(fvisw, visw, nwal)
+    call rd8(ffm, flmass, numFaces)
+    call rd8(fas, a, nnz)
+    viscos = vsc
+    timestep = dtt
+    ltransient = trim(tsch) /= 'steady'
+    bdf = trim(tsch) == 'bdf'
+    bdf2 = trim(tsch) == 'bdf2'
+    bdf3 = .false.
+    cn = trim(tsch) == 'cn'
+    piso = .false.
+    lbuoy = .false.
+    const_mflux = .false.
+    calcEpot = .false.
+    lstsq = .false.; lstsq_dm = .false.; lstsq_qr = .false.; gauss = .true.
+    node_gauss = .false.
+    limiter = 'none'
+    urfU = (/ u1, u2, u3 /)
+    gdsU = gds
+    cSchemeU = schm
+    lSolverU = 'bicgstab'
+    maxiterU = itmax
+    tolAbsU = 0.0_dp
+    tolRelU = trel
+    su = 0.0_dp; sv = 0.0_dp; sw = 0.0_dp
+    res = 0.0_dp; reso = 0.0_dp; pk = 0.0_dp; zk = 0.0_dp; d = 0.0_dp; uk = 0.0_dp; vk = 0.0_dp
+    resor = 0.0_dp
+    call calcuvw
+    call wr8(fa, a, nnz)
+    call wr8(fsu, su, numCells)
+    call wr8(fsv, sv, numCells)
+    call wr8(fsw, sw, numCells)
+    call wr8(fspu, spu, numCells)
+    call wr8(fspv, spv, numCells)
+    call wr8(fsp, sp, numCells)
+    call wr8(fapu, apu, numCells)
+    call wr8(fapv, apv, numCells)
+    call wr8(fapw, apw, numCells)
+    call wr8(fuo2, u, numTotal)
+    call wr8(fvo2, v, numTotal)
+    call wr8(fwo2, w, numTotal)
+    call wr8(fpo, p, numTotal)
+    call wr8(fdpo, dPdxi, 3*numTotal)
+    call wr8(fduo, dUdxi, 3*numTotal)
   end subroutine
 
 end program ref_driver

@@ -306,6 +306,69 @@ def pcorr_case():
     print("wrote pcorr")
 
 
+# (time scheme, convection scheme, gds, urf u/v/w, bicgstab iterations)
+MOMENTUM_CASES = [
+    ("steady", "linearUpwind", 1.0, (0.7, 0.6, 0.5), 0),
+    ("bdf2", "cds", 0.8, (0.8, 0.8, 0.8), 0),
+    ("cn", "linearUpwind", 1.0, (0.7, 0.7, 0.7), 0),
+    ("bdf", "linearUpwind", 1.0, (0.7, 0.7, 0.7), 5),
+]
+
+
+def momentum_inputs(mesh, seed):
+    """Inputs of the momentum fixture: current, old and older velocities,
+    pressure, density, viscosity (cells and boundary faces), the wall faces'
+    effective viscosity, mass fluxes and the matrix's previous content."""
+    n, nt, F = mesh.n_cells, mesh.n_total, mesh.n_faces
+    nwal = int(sum(int(mesh.nfaces[ib]) for ib, t in enumerate(mesh.types) if t == "wall"))
+    return dict(u=uniform(seed, 1, nt, -1.0, 1.0), v=uniform(seed, 2, nt, -1.0, 1.0),
+                w=uniform(seed, 3, nt, -1.0, 1.0), uo=uniform(seed, 4, n, -1.0, 1.0),
+                vo=uniform(seed, 5, n, -1.0, 1.0), wo=uniform(seed, 6, n, -1.0, 1.0),
+                uoo=uniform(seed, 7, n, -1.0, 1.0), voo=uniform(seed, 8, n, -1.0, 1.0),
+                p=uniform(seed, 9, nt, -1.0, 1.0), den=1.0 + 0.2 * uniform(seed, 10, nt, -1.0, 1.0),
+                vis=10.0 ** uniform(seed, 11, nt, -3.0, -1.0),
+                visw=10.0 ** uniform(seed, 12, max(nwal, 1), -3.0, -1.0),
+                flmass=uniform(seed, 13, F, -1.0, 1.0) * 1e-2,
+                astale=uniform(seed, 14, 7 * n + 2 * F, -1.0, 1.0))
+
+
+def momentum_case():
+    """calcuvw (velocity.f90:50-750) itself on a jittered 6^3 hex with inlet,
+    outlet, wall, symmetry and empty patches and on the periodic 6^3 mesh:
+    with zero solver iterations the assembled systems (a with w's diagonal,
+    su = w's right-hand side, sv, sw, the diagonal sources and 1/a_P of all
+    three components, gradients and pressure boundary values); with five
+    BiCGStab iterations the solved velocities."""
+    out = {}
+    cases = {"hex": (hex_mesh(6, jitter=0.2, seed=19, types=PCORR_TYPES), 25),
+             "per": (periodic_mesh(), 26)}
+    for key, (mesh, seed) in cases.items():
+        rc = RefCase(mesh)
+        g, c, sizes = rc.dump()
+        nnz = int(c["ja"].size)
+        I = momentum_inputs(mesh, seed)
+        I["astale"] = I["astale"][:nnz]
+        for k, v in I.items():
+            rc.put(k + ".bin", v)
+            out[f"{key}_in_{k}"] = v
+        for ic, (ts, sch, gds, urf, its) in enumerate(MOMENTUM_CASES):
+            rc.run([f"momentum u.bin v.bin w.bin uo.bin vo.bin wo.bin uoo.bin voo.bin p.bin den.bin "
+                    f"vis.bin visw.bin flmass.bin astale.bin 1.0d-3 0.05d0 {ts} {sch} {gds!r} "
+                    f"{urf[0]!r} {urf[1]!r} {urf[2]!r} {its} 1.0d-3 a.bin su.bin sv.bin sw.bin "
+                    f"spu.bin spv.bin sp.bin apu.bin apv.bin apw.bin uo2.bin vo2.bin wo2.bin "
+                    f"po.bin dpo.bin duo.bin"])
+            t = f"{key}_m{ic}"
+            for k in ("a", "su", "sv", "sw", "spu", "spv", "sp", "apu", "apv", "apw"):
+                out[f"{t}_{k}"] = rc.get(k + ".bin")
+            for k, f in (("u", "uo2"), ("v", "vo2"), ("w", "wo2"), ("p", "po")):
+                out[f"{t}_{k}"] = rc.get(f + ".bin")
+            out[f"{t}_dpdxi"] = rc.get("dpo.bin").reshape(-1, 3)
+            out[f"{t}_dudxi"] = rc.get("duo.bin").reshape(-1, 3)
+        rc.close()
+    np.savez_compressed(os.path.join(HERE, "momentum.npz"), **out)
+    print("wrote momentum")
+
+
 PERIODIC_TYPES = ("empty", "periodic", "wall", "inlet", "empty", "periodic")
 PERIODIC_WF = (0, 0, 1, 0, 0, 0)
 # (time scheme, convection scheme, solver, itr_max, tol_rel, gradient, urf, gds)
@@ -582,6 +645,7 @@ CASES = {
     "gradfvx": grad_fvx_case,
     "fluxscalar": flux_scalar_case,
     "pcorr": pcorr_case,
+    "momentum": momentum_case,
     "transport": transport_case,
     "cavity": cavity_case,
     "kat5x5": kat_5x5_case,
