@@ -744,6 +744,80 @@ int fcp_pressure_correct(fcp_ctx *ctx, double *pp, double *flmass, double *u, do
   return 0;
 }
 
+int fcp_momentum_assemble(fcp_ctx *ctx, double *u, double *v, double *w, const double *uo,
+                          const double *vo, const double *wo, const double *uoo,
+                          const double *voo, const double *woo, double *p, const double *den,
+                          const double *vis, const double *visw, const double *flmass,
+                          double viscos, double dt, int time_scheme, int conv_scheme, double gds,
+                          const double *urf, double *dpdxi, double *grad, double *rhs,
+                          double *diagv, double *ap, double *spv, double *src, double *a) {
+  FCP_TRY(need(ctx, true, true, true));
+  if (!u || !v || !w || !p || !den || !vis || !flmass || !urf || !rhs || !diagv ||
+      (time_scheme > 0 && (!uo || !vo || !wo)) || (time_scheme == 2 && (!uoo || !voo || !woo)))
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_momentum_assemble: null argument");
+  if (time_scheme < 0 || time_scheme > 3 || (conv_scheme != 0 && conv_scheme != 1) ||
+      (time_scheme > 0 && !(dt > 0.0)) || !(urf[0] > 0.0) || !(urf[1] > 0.0) || !(urf[2] > 0.0))
+    return fcp_fail(ctx, FCP_ERR_ARG, "fcp_momentum_assemble: bad argument");
+  (void)hipSetDevice(ctx->device);
+  const size_t n = ctx->n, nt = n + ctx->Fb;
+  // wall index of every boundary face, in patch order (the reference's iWall)
+  std::vector<int> wall(ctx->Fb > 0 ? ctx->Fb : 1, -1);
+  int nwal = 0;
+  for (int ib = 0; ib < ctx->nPatches; ++ib)
+    if (ctx->h_bctype[ib] == FCP_BC_WALL)
+      for (int i = 0; i < ctx->h_nfaces[ib]; ++i) wall[ctx->h_start[ib] - ctx->Fi + i] = nwal++;
+  if (nwal > 0 && !visw) return fcp_fail(ctx, FCP_ERR_ARG, "fcp_momentum_assemble: visw needed");
+  FCP_HIP(ctx, ctx->mom_wall.alloc(wall.size()));
+  FCP_HIP(ctx, hipMemcpyAsync(ctx->mom_wall.p, wall.data(), sizeof(int) * wall.size(),
+                              hipMemcpyHostToDevice, ctx->stream));
+  FCP_HIP(ctx, ctx->mom_grad.alloc(9 * nt));
+  FCP_HIP(ctx, ctx->mom_s3.alloc(3 * n > 0 ? 3 * n : 1));
+  FCP_HIP(ctx, ctx->mom_dp.alloc(3 * nt));
+  for (auto &b : ctx->mom_w3) FCP_HIP(ctx, b.alloc(3 * n > 0 ? 3 * n : 1));
+  Arg<double> U, V, W, UO, VO, WO, UOO, VOO, WOO, P, R, VI, VW, FM, D, GR, RH, DG, AP, SPV, SRC;
+  FCP_TRY(arg_in(ctx, (const double *)u, nt, U));
+  FCP_TRY(arg_in(ctx, (const double *)v, nt, V));
+  FCP_TRY(arg_in(ctx, (const double *)w, nt, W));
+  FCP_TRY(arg_in(ctx, uo, uo ? n : 0, UO));
+  FCP_TRY(arg_in(ctx, vo, vo ? n : 0, VO));
+  FCP_TRY(arg_in(ctx, wo, wo ? n : 0, WO));
+  FCP_TRY(arg_in(ctx, uoo, uoo ? n : 0, UOO));
+  FCP_TRY(arg_in(ctx, voo, voo ? n : 0, VOO));
+  FCP_TRY(arg_in(ctx, woo, woo ? n : 0, WOO));
+  FCP_TRY(arg_in(ctx, (const double *)p, nt, P));
+  FCP_TRY(arg_in(ctx, den, nt, R));
+  FCP_TRY(arg_in(ctx, vis, nt, VI));
+  FCP_TRY(arg_in(ctx, visw, visw ? (size_t)(nwal > 0 ? nwal : 1) : 0, VW));
+  FCP_TRY(arg_in(ctx, flmass, ctx->F, FM));
+  // optional outputs go to scratch of the context when not asked for
+  auto out_or = [&](double *user, size_t cnt, Arg<double> &A, double *scratch) -> int {
+    if (user) return arg_out(ctx, user, cnt, A);
+    A.dev = scratch;
+    return 0;
+  };
+  if (dpdxi) {
+    FCP_TRY(arg_in(ctx, (const double *)dpdxi, 3 * nt, D));
+  } else {
+    D.dev = ctx->mom_dp.p;
+  }
+  FCP_TRY(out_or(grad, 9 * nt, GR, ctx->mom_grad.p));
+  FCP_TRY(arg_out(ctx, rhs, 3 * n, RH));
+  FCP_TRY(arg_out(ctx, diagv, 3 * n, DG));
+  FCP_TRY(out_or(ap, 3 * n, AP, ctx->mom_w3[0].p));
+  FCP_TRY(out_or(spv, 3 * n, SPV, ctx->mom_w3[1].p));
+  FCP_TRY(out_or(src, 3 * n, SRC, ctx->mom_w3[2].p));
+  FCP_TRY(fcp_launch_momentum(ctx, U.dev, V.dev, W.dev, UO.dev, VO.dev, WO.dev, UOO.dev, VOO.dev,
+                              WOO.dev, P.dev, R.dev, VI.dev, VW.dev ? VW.dev : ctx->mom_s3.p,
+                              ctx->mom_wall.p, FM.dev, viscos, dt, time_scheme, conv_scheme, gds,
+                              urf, D.dev, GR.dev, ctx->mom_s3.p, RH.dev, DG.dev, AP.dev, SPV.dev,
+                              SRC.dev));
+  for (Arg<double> *x : {&U, &V, &W, &P, &D, &GR, &RH, &DG, &AP, &SPV, &SRC})
+    FCP_TRY(arg_back(ctx, *x));
+  FCP_TRY(out_copy(ctx, ctx->a.p, a, ctx->nnz));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
 int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt, const double *phi_o,
             const double *phi_oo, double *su, double *sp) {
   FCP_TRY(need(ctx, true, true, false));
@@ -1522,6 +1596,14 @@ int fcp_csrsolve_multi(fcp_ctx *ctx, const char *solver, int solver_len, int nrh
   return rc;
 }
 
+namespace {
+__global__ void k_set_diag(int n, const int *__restrict__ diag, const double *__restrict__ dv,
+                           double *__restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[diag[i]] = dv[i];
+}
+}  // namespace
+
 int fcp_csrsolve_multi_diag(fcp_ctx *ctx, const char *solver, int solver_len, int nrhs,
                             double *fi, int ldfi, const double *rhs, const double *diagv,
                             double *res0, int itr_max, double tol_abs, double tol_rel,
@@ -1543,8 +1625,35 @@ int fcp_csrsolve_multi_diag(fcp_ctx *ctx, const char *solver, int solver_len, in
   fcp_solve_opts local;
   fcp_solve_opts_default(&local);
   fcp_solve_opts *o = opts ? opts : &local;
-  const int rc = fcp_solve_multi_device(ctx, code, nrhs, x.dev, ldfi, b.dev, res0, itr_max,
-                                        tol_abs, tol_rel, verbose, o, d.dev);
+  int rc = fcp_solve_multi_device(ctx, code, nrhs, x.dev, ldfi, b.dev, res0, itr_max, tol_abs,
+                                  tol_rel, verbose, o, d.dev);
+  if (rc == FCP_ERR_UNSUPPORTED && o->precision == FCP_PREC_FP64) {
+    // the batched sweeps do not apply to this pattern (e.g. their schedule
+    // does not fit): one column at a time on the matrix with that column's
+    // diagonal, as the reference's separate csrsolve calls run; the resident
+    // matrix keeps the last column's diagonal
+    ctx->mdc_active = false;
+    int worst = 0;
+    double rel = 0.0;
+    int its = 0;
+    for (int c = 0; c < nrhs; ++c) {
+      k_set_diag<<<grid_for(ctx->n, 256), 256, 0, ctx->stream>>>(
+          ctx->n, ctx->diag.p, d.dev + (size_t)c * ctx->n, ctx->a.p);
+      FCP_HIP(ctx, hipGetLastError());
+      ctx->a_version++;
+      const int rc1 = fcp_solve_device(ctx, code, x.dev + (size_t)c * ldfi,
+                                       b.dev + (size_t)c * ctx->n, itr_max, tol_abs, tol_rel,
+                                       verbose, o);
+      if (rc1 < 0) return rc1;
+      if (res0) res0[c] = o->res0;
+      worst = rc1 > worst ? rc1 : worst;
+      rel = o->rel_res_l2 > rel ? o->rel_res_l2 : rel;
+      its = o->iterations > its ? o->iterations : its;
+    }
+    o->rel_res_l2 = rel;
+    o->iterations = its;
+    rc = worst;
+  }
   if (rc < 0) return rc;
   FCP_TRY(arg_back(ctx, x));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));

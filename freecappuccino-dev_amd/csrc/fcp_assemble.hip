@@ -855,6 +855,7 @@ __global__ void k_pc_grad(int n, int Fi, const int *__restrict__ cf_ptr,
   su[c] = sx;
   sv[c] = sy;
   sw[c] = sz;
+  if (!apu) return;  // gradp_and_sources alone (the momentum sources)
   u[c] = u[c] + sx * apu[c];
   v[c] = v[c] + sy * apv[c];
   w[c] = w[c] + sz * apw[c];
@@ -882,6 +883,292 @@ __global__ void k_pc_velbnd(int n, int Fi, int Fb, const int *__restrict__ owner
     u[jb] = u[p] - Unmag * g.arx[f];
     v[jb] = v[p] - Unmag * g.ary[f];
     w[jb] = w[p] - Unmag * g.arz[f];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Momentum assembly of calcuvw (velocity.f90:50-750) before its three solves.
+// ---------------------------------------------------------------------------
+struct MomIn {
+  const double *U[3], *G[3], *O[3], *OO[3];
+  const double *den, *vis, *visw, *flmass;
+  const int *wall_of;  // [Fb] index of the face among the wall faces in patch order, or -1
+  double viscos, dt, gds, urf[3];
+  int ts, scheme;
+};
+
+// sngrad (gradients.f90:1720-1781) of component phi at inner face f.
+__device__ __forceinline__ void mom_sngrad(const GeomP &g, int f, int p, int q,
+                                           const double *__restrict__ phi,
+                                           const double *__restrict__ gr, double di[3],
+                                           double dii[3]) {
+  const double lam = g.facint[f];
+  const double fxn = lam, fxp = 1.0 - lam;
+  const double xpn = g.xc[q] - g.xc[p], ypn = g.yc[q] - g.yc[p], zpn = g.zc[q] - g.zc[p];
+  const double ax = g.arx[f], ay = g.ary[f], az = g.arz[f];
+  const double vole = (xpn * ax + ypn * ay) + zpn * az;
+  const double dx = gr[3 * (size_t)p] * fxp + gr[3 * (size_t)q] * fxn;
+  const double dy = gr[3 * (size_t)p + 1] * fxp + gr[3 * (size_t)q + 1] * fxn;
+  const double dz = gr[3 * (size_t)p + 2] * fxp + gr[3 * (size_t)q + 2] * fxn;
+  const double e = (((phi[q] - phi[p]) - dx * xpn) - dy * ypn) - dz * zpn;
+  dii[0] = dx + ax / vole * e;
+  dii[1] = dy + ay / vole * e;
+  dii[2] = dz + az / vole * e;
+  const double Dff = g.Df[f];
+  di[0] = dx * (ax - Dff * xpn);
+  di[1] = dy * (ay - Dff * ypn);
+  di[2] = dz * (az - Dff * zpn);
+}
+
+// facefluxuvw (velocity.f90:754-878): coefficients and the three sources.
+__device__ __forceinline__ void mom_inner(const GeomP &g, int f, int p, int q, const MomIn &in,
+                                          double &can, double &cap, double sadd[3]) {
+  const double lam = g.facint[f], fxn = lam, fxp = 1.0 - lam;
+  const double fm = in.flmass[f];
+  const double game = in.vis[p] + (in.vis[q] - in.vis[p]) * lam;
+  const double de = game * g.Df[f];
+  const double ce = fm < 0.0 ? fm : 0.0, cp = fm > 0.0 ? fm : 0.0;
+  can = -de + ce;
+  cap = -de - cp;
+  double di[3][3], dii[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) mom_sngrad(g, f, p, q, in.U[r], in.G[r], di[r], dii[r]);
+  const double ax = g.arx[f], ay = g.ary[f], az = g.arz[f];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    double fde = game * ((dii[0][r] * ax + dii[1][r] * ay) + dii[2][r] * az);
+    const double fdi = game * ((di[r][0] + di[r][1]) + di[r][2]);
+    fde = fde + fdi;
+    const double *phi = in.U[r], *gr = in.G[r];
+    const double fuds = cp * phi[p] + ce * phi[q];
+    double ue;
+    const int up = fm >= 0.0 ? p : q, dn = fm >= 0.0 ? q : p;
+    if (in.scheme == 0) {  // face_value_cds with the weight the caller passes
+      ue = phi[up] + (phi[dn] - phi[up]) * (fm >= 0.0 ? fxp : fxn);
+    } else {  // face_value_2nd_upwind
+      const double gfr = (gr[3 * (size_t)up] * (g.xf[f] - g.xc[up]) +
+                          gr[3 * (size_t)up + 1] * (g.yf[f] - g.yc[up])) +
+                         gr[3 * (size_t)up + 2] * (g.zf[f] - g.zc[up]);
+      ue = phi[up] + gfr;
+    }
+    const double fhigh = fm * ue;
+    sadd[r] = -(in.gds * (fhigh - fuds)) + fde;
+  }
+}
+
+// facefluxuvw_periodic (velocity.f90:1038-1180): periodic face f, owner p,
+// twin owner q.
+__device__ __forceinline__ void mom_periodic(const GeomP &g, int f, int p, int q,
+                                             const MomIn &in, double &can, double &cap,
+                                             double sadd[3]) {
+  const double fxn = 0.5, fxp = fxn;
+  const double ax = g.arx[f], ay = g.ary[f], az = g.arz[f];
+  const double xpn = 2.0 * (g.xf[f] - g.xc[p]), ypn = 2.0 * (g.yf[f] - g.yc[p]),
+               zpn = 2.0 * (g.zf[f] - g.zc[p]);
+  const double dpn = sqrt((xpn * xpn + ypn * ypn) + zpn * zpn);
+  const double are = sqrt((ax * ax + ay * ay) + az * az);
+  const double game = in.vis[p] * fxp + in.vis[q] * fxn;
+  const double de = game * ((ax * ax + ay * ay) + az * az) / ((xpn * ax + ypn * ay) + zpn * az);
+  const double fm = in.flmass[f];
+  const double ce = fm < 0.0 ? fm : 0.0, cp = fm > 0.0 ? fm : 0.0;
+  can = -de + ce;
+  cap = -de - cp;
+  double d[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      d[r][k] = in.G[r][3 * (size_t)p + k] * fxp + in.G[r][3 * (size_t)q + k] * fxn;
+  const double fde[3] = {
+      game * (((d[0][0] + d[0][0]) * ax + (d[0][1] + d[1][0]) * ay) + (d[0][2] + d[2][0]) * az),
+      game * (((d[0][1] + d[1][0]) * ax + (d[1][1] + d[1][1]) * ay) + (d[1][2] + d[2][1]) * az),
+      game * (((d[0][2] + d[2][0]) * ax + (d[2][1] + d[1][2]) * ay) + (d[2][2] + d[2][2]) * az)};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double *phi = in.U[r];
+    const double fdi = game * are / dpn * ((d[r][0] * xpn + d[r][1] * ypn) + d[r][2] * zpn);
+    const double fuds = cp * phi[p] + ce * phi[q];
+    const double ue = fm >= 0.0 ? phi[p] + (phi[q] - phi[p]) * fxp
+                                : phi[q] + (phi[p] - phi[q]) * fxn;
+    const double fhigh = fm * ue;
+    sadd[r] = (-(in.gds * (fhigh - fuds)) + fde[r]) - fdi;
+  }
+}
+
+// One thread per cell: the pressure sources from gradp_and_sources in s3,
+// then the time terms, the cell's faces in ascending order (inner faces,
+// boundary faces by type, the periodic pairs whose twin face it owns at their
+// periodic face's place), the Crank-Nicolson block and the three diagonals.
+// Only this thread reads or writes the entries of row c.
+__global__ void k_mom_cells(int n, int Fi, const int *__restrict__ cf_ptr,
+                            const int *__restrict__ cf, const int *__restrict__ owner,
+                            const int *__restrict__ neighbour, const int *__restrict__ bpatch,
+                            const int *__restrict__ bctype, GeomP g, MomIn in,
+                            const int *__restrict__ slot_pn, const int *__restrict__ slot_np,
+                            const int *__restrict__ ia, const int *__restrict__ diag,
+                            double *__restrict__ a, const double *__restrict__ s3,
+                            double *__restrict__ rhs3, double *__restrict__ diag3,
+                            double *__restrict__ ap3, double *__restrict__ sp3,
+                            double *__restrict__ src3, PerP per) {
+  const int c = xcd_cell(n);
+  if (c >= n) return;
+  double S[3], SP[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int r = 0; r < 3; ++r) S[r] = s3[r * (size_t)n + c];
+  if (in.ts > 0) {  // :162-214
+    const double apotime = in.den[c] * g.vol[c] / in.dt;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      if (in.ts == 1 || in.ts == 3) {
+        S[r] = S[r] + apotime * in.O[r][c];
+        SP[r] = SP[r] + apotime;
+      } else {
+        S[r] = S[r] + apotime * (2.0 * in.O[r][c] - 0.5 * in.OO[r][c]);
+        SP[r] = SP[r] + 1.5 * apotime;
+      }
+    }
+  }
+  int tw = per.np > 0 ? per.tw_ptr[c] : 0;
+  const int twe = per.np > 0 ? per.tw_ptr[c + 1] : 0;
+  auto twin_side = [&](int before) {
+    for (; tw < twe && per.fp[per.tw_k[tw]] < before; ++tw) {
+      const int kp = per.tw_k[tw], fp = per.fp[kp];
+      double can, cap, sadd[3];
+      mom_periodic(g, fp, owner[fp], c, in, can, cap, sadd);
+      const int sl = slot_np[Fi + kp];
+      if (sl >= 0) a[sl] = cap;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) S[r] = S[r] - sadd[r];
+    }
+  };
+  const int k0 = cf_ptr[c], k1 = cf_ptr[c + 1];
+  for (int k = k0; k < k1; ++k) {
+    const int f = cf[k];
+    if (f < Fi) {
+      const int p = owner[f], q = neighbour[f];
+      double can, cap, sadd[3];
+      mom_inner(g, f, p, q, in, can, cap, sadd);
+      if (c == p) {
+        const int sl = slot_pn[f];
+        if (sl >= 0) a[sl] = can;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) S[r] = S[r] + sadd[r];
+      } else {
+        const int sl = slot_np[f];
+        if (sl >= 0) a[sl] = cap;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) S[r] = S[r] - sadd[r];
+      }
+      continue;
+    }
+    twin_side(f);
+    const int b = f - Fi, t = bctype[bpatch[b]];
+    const size_t jb = (size_t)n + b;
+    const double ax = g.arx[f], ay = g.ary[f], az = g.arz[f];
+    if (t == FCP_BC_INLET || t == FCP_BC_OUTLET || t == FCP_BC_PRESSURE) {
+      // facefluxuvw_bnd (velocity.f90:882-1037), scatter :262-279
+      const double xpn = g.xf[f] - g.xc[c], ypn = g.yf[f] - g.yc[c], zpn = g.zf[f] - g.zc[c];
+      const double vole = (xpn * ax + ypn * ay) + zpn * az;
+      const double Dfi = ((ax * ax + ay * ay) + az * az) / vole;
+      const double game = in.vis[jb];
+      const double de = game * Dfi;
+      const double fm = in.flmass[f];
+      const double cb = -de + (fm < 0.0 ? fm : 0.0);
+      double di[3][3], dii[3][3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double *gr = in.G[r], *phi = in.U[r];
+        const double dx = gr[3 * (size_t)c], dy = gr[3 * (size_t)c + 1], dz = gr[3 * (size_t)c + 2];
+        const double e = (((phi[jb] - phi[c]) - dx * xpn) - dy * ypn) - dz * zpn;
+        dii[r][0] = dx + ax / vole * e;
+        dii[r][1] = dy + ay / vole * e;
+        dii[r][2] = dz + az / vole * e;
+        di[r][0] = dx * (ax - Dfi * xpn);
+        di[r][1] = dy * (ay - Dfi * ypn);
+        di[r][2] = dz * (az - Dfi * zpn);
+      }
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double fde = game * ((dii[0][r] * ax + dii[1][r] * ay) + dii[2][r] * az);
+        const double fdi = game * ((di[r][0] + di[r][1]) + di[r][2]);
+        SP[r] = SP[r] - cb;
+        S[r] = S[r] - cb * in.U[r][jb] + (fde + fdi);
+      }
+    } else if (t == FCP_BC_SYMMETRY) {  // :281-306, vis(inp) with inp = numCells+1
+      const double are = sqrt((ax * ax + ay * ay) + az * az);
+      const double arer = 1.0 / are;
+      const double nf[3] = {ax * arer, ay * arer, az * arer};
+      const double dpb = ((g.xf[f] - g.xc[c]) * nf[0] + (g.yf[f] - g.yc[c]) * nf[1]) +
+                         (g.zf[f] - g.zc[c]) * nf[2];
+      const double cf_ = 2.0 * in.vis[n] * are / dpb;
+      const double uc = in.U[0][c], vc = in.U[1][c], wc = in.U[2][c];
+      S[0] = S[0] - cf_ * nf[0] * (nf[1] * vc + nf[2] * wc);
+      S[1] = S[1] - cf_ * nf[1] * (nf[0] * uc + nf[2] * wc);
+      S[2] = S[2] - cf_ * nf[2] * (nf[0] * uc + nf[1] * vc);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) SP[r] = SP[r] + cf_ * (nf[r] * nf[r]);
+    } else if (t == FCP_BC_PERIODIC && per.np > 0 && per.kof[b] >= 0) {
+      const int kp = per.kof[b];
+      double can, cap, sadd[3];
+      mom_periodic(g, f, c, owner[per.ft[kp]], in, can, cap, sadd);
+      const int sl = slot_pn[Fi + kp];
+      if (sl >= 0) a[sl] = can;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) S[r] = S[r] + sadd[r];
+    } else if (t == FCP_BC_WALL) {  // :330-360
+      const double vw = in.visw[in.wall_of[b]];
+      const double viss = in.viscos > vw ? in.viscos : vw;
+      const double are = sqrt((ax * ax + ay * ay) + az * az);
+      const double arer = 1.0 / are;
+      const double nxf = ax * arer, nyf = ay * arer, nzf = az * arer;
+      const double dpb = ((g.xf[f] - g.xc[c]) * nxf + (g.yf[f] - g.yc[c]) * nyf) +
+                         (g.zf[f] - g.zc[c]) * nzf;
+      const double vsol = viss * are / dpb;
+      const double ub = in.U[0][jb], vb = in.U[1][jb], wb = in.U[2][jb];
+      const double upb = in.U[0][c] - ub, vpb = in.U[1][c] - vb, wpb = in.U[2][c] - wb;
+      SP[0] = SP[0] + vsol * (1.0 - nxf * nxf);
+      SP[1] = SP[1] + vsol * (1.0 - nyf * nyf);
+      SP[2] = SP[2] + vsol * (1.0 - nzf * nzf);
+      S[0] = S[0] + vsol * ((ub * (1.0 - nxf * nxf) + vpb * nyf * nxf) + wpb * nzf * nxf);
+      S[1] = S[1] + vsol * ((upb * nxf * nyf + vb * (1.0 - nyf * nyf)) + wpb * nzf * nyf);
+      S[2] = S[2] + vsol * ((upb * nxf * nzf + vpb * nyf * nzf) + wb * (1.0 - nzf * nzf));
+    }
+  }
+  twin_side(0x7fffffff);
+  // Crank-Nicolson (:375-390) and the diagonals (:392-450), component by
+  // component; the row sums read whatever the diagonal entry holds
+  const int r0 = ia[c], r1 = ia[c + 1], dg = diag[c];
+  if (in.ts == 3)
+    for (int k = r0; k < r1; ++k) a[k] = 0.5 * a[k];
+  for (int r = 0; r < 3; ++r) {
+    if (in.ts == 3) {
+      for (int k = k0; k < k1; ++k) {
+        const int f = cf[k];
+        if (f >= Fi) break;
+        const int p = owner[f], q = neighbour[f];
+        if (c == p) S[r] = S[r] - a[slot_pn[f]] * in.O[r][q];
+        else S[r] = S[r] - a[slot_np[f]] * in.O[r][p];
+      }
+      const double apotime = in.den[c] * g.vol[c] / in.dt;
+      double sum = 0.0;
+      for (int k = r0; k < r1; ++k) sum = sum + a[k];
+      const double off = sum - a[dg];
+      S[r] = S[r] + (apotime + off) * in.O[r][c];
+      SP[r] = SP[r] + apotime;
+    }
+    if (r > 0) a[dg] = 0.0;
+    const double urfr = 1.0 / in.urf[r], urfm = 1.0 - in.urf[r];
+    double sum = 0.0;
+    for (int k = r0; k < r1; ++k) sum = sum + a[k];
+    const double off = sum - a[dg];
+    double ad = SP[r] - off;
+    ap3[r * (size_t)n + c] = 1.0 / (ad + FCP_SMALL);
+    ad = ad * urfr;
+    a[dg] = ad;
+    diag3[r * (size_t)n + c] = ad;
+    rhs3[r * (size_t)n + c] = S[r] + urfm * ad * in.U[r][c];
+    sp3[r * (size_t)n + c] = SP[r];
+    src3[r * (size_t)n + c] = S[r];
   }
 }
 
@@ -1059,5 +1346,52 @@ int fcp_launch_pcorr_correct(fcp_ctx *ctx, double *pp, double *flmass, double *u
     k_pc_velbnd<<<grid_for(Fb, BLK), BLK, 0, ctx->stream>>>(n, Fi, Fb, ctx->owner.p, ctx->bpatch.p,
                                                             ctx->bctype.p, g, u, v, w);
   FCP_HIP(ctx, hipGetLastError());
+  return 0;
+}
+
+int fcp_launch_momentum(fcp_ctx *ctx, double *u, double *v, double *w, const double *uo,
+                        const double *vo, const double *wo, const double *uoo,
+                        const double *voo, const double *woo, double *p, const double *den,
+                        const double *vis, const double *visw, const int *wall_of,
+                        const double *flmass, double viscos, double dt, int ts, int scheme,
+                        double gds, const double *urf, double *dpdxi, double *grad9,
+                        double *s3, double *rhs3, double *diag3, double *ap3, double *sp3,
+                        double *src3) {
+  FCP_TRY(fcp_coarse_join(ctx));
+  ProfScope ps(ctx, "momentum");
+  const int n = ctx->n, Fi = ctx->Fi, Fb = ctx->Fb;
+  const size_t nt = (size_t)n + Fb;
+  const GeomP g = geom_of(ctx);
+  if (Fb > 0)  // updateVelocityAtBoundary (:153)
+    k_pc_velbnd<<<grid_for(Fb, BLK), BLK, 0, ctx->stream>>>(n, Fi, Fb, ctx->owner.p, ctx->bpatch.p,
+                                                            ctx->bctype.p, g, u, v, w);
+  double *U[3] = {u, v, w};
+  for (int r = 0; r < 3; ++r) FCP_TRY(fcp_launch_grad_gauss(ctx, U[r], grad9 + 3 * nt * r));
+  for (int stage = 1; stage <= 2; ++stage) {  // gradp_and_sources(p) (:160)
+    if (Fb > 0)
+      k_pc_bpres<<<grid_for(Fb, BLK), BLK, 0, ctx->stream>>>(n, Fi, Fb, stage, ctx->owner.p,
+                                                             ctx->bpatch.p, ctx->bctype.p, g,
+                                                             dpdxi, p);
+    k_pc_grad<<<xcd_grid(n), BLK, 0, ctx->stream>>>(
+        n, Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, g, p, stage == 1,
+        stage == 2, dpdxi, s3, s3 + n, s3 + 2 * (size_t)n, nullptr, nullptr, nullptr, 0.0, 0,
+        nullptr, nullptr, nullptr, nullptr);
+  }
+  MomIn in;
+  for (int r = 0; r < 3; ++r) {
+    in.U[r] = U[r];
+    in.G[r] = grad9 + 3 * nt * r;
+    in.urf[r] = urf[r];
+  }
+  in.O[0] = uo, in.O[1] = vo, in.O[2] = wo;
+  in.OO[0] = uoo, in.OO[1] = voo, in.OO[2] = woo;
+  in.den = den, in.vis = vis, in.visw = visw, in.flmass = flmass, in.wall_of = wall_of;
+  in.viscos = viscos, in.dt = dt, in.gds = gds, in.ts = ts, in.scheme = scheme;
+  k_mom_cells<<<xcd_grid(n), BLK, 0, ctx->stream>>>(
+      n, Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, ctx->bpatch.p,
+      ctx->bctype.p, g, in, ctx->slot_pn.p, ctx->slot_np.p, ctx->ia.p, ctx->diag.p, ctx->a.p, s3,
+      rhs3, diag3, ap3, sp3, src3, fcp_periodic_view(ctx));
+  FCP_HIP(ctx, hipGetLastError());
+  ctx->a_version++;
   return 0;
 }

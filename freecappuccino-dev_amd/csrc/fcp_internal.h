@@ -297,6 +297,8 @@ struct fcp_ctx {
   DBuf<double> cd_gamf;   // [F] face diffusivity of fcp_convdiff_cell
   DBuf<double> pc_fac;    // [1] adjustMassFlow's outlet scaling factor
   DBuf<double> pc_s;      // [3n] su, sv, sw of fcp_pressure_correct when not asked for
+  DBuf<int> mom_wall;     // [Fb] wall index of every boundary face (-1: not a wall)
+  DBuf<double> mom_grad, mom_s3, mom_dp, mom_w3[5];  // momentum assembly scratch
   int lsq_method = 0, lsq_compat = 1;
 
   // -------- scalar transport step (calculate_temperature) --------
@@ -453,6 +455,17 @@ int fcp_launch_pcorr_correct(fcp_ctx *ctx, double *pp, double *flmass, double *u
                              double *w, double *p, double *dpdxi, const double *apu,
                              const double *apv, const double *apw, double urfp, int pref,
                              double *su, double *sv, double *sw);
+// calcuvw's assembly (velocity.f90:50-750) before its solves: u, v, w and p
+// boundary values updated, dpdxi [3 nt], grad9 [9 nt] (u, v, w gradients),
+// s3 [3n] scratch; rhs3, diag3, ap3, sp3, src3 [3n] per component.
+int fcp_launch_momentum(fcp_ctx *ctx, double *u, double *v, double *w, const double *uo,
+                        const double *vo, const double *wo, const double *uoo,
+                        const double *voo, const double *woo, double *p, const double *den,
+                        const double *vis, const double *visw, const int *wall_of,
+                        const double *flmass, double viscos, double dt, int ts, int scheme,
+                        double gds, const double *urf, double *dpdxi, double *grad9,
+                        double *s3, double *rhs3, double *diag3, double *ap3, double *sp3,
+                        double *src3);
 int fcp_launch_convdiff(fcp_ctx *ctx, const double *flmass, const double *dcoef, double *phi,
                         const double *grad, double gam, int scheme, const unsigned char *bc_kind,
                         double urf, double *su, double *sp, const int *srd_map = nullptr,

@@ -42,6 +42,8 @@ SIGNATURES = {
     "fcp_grad_gauss": (_i, [_p, _p, _p]),
     "fcp_convdiff": (_i, [_p, _p, _p, _p, _p, _d, _i, _p, _d, _p, _p, _p]),
     "fcp_convdiff_cell": (_i, [_p, _p, _p, _p, _p, _d, _i, _p, _d, _p, _p, _p]),
+    "fcp_momentum_assemble": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p,
+                                   _d, _d, _i, _i, _d, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "fcp_pressure_assemble": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i, _d, _p, _p]),
     "fcp_pressure_correct": (_i, [_p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _d, _i, _p, _p,
                                   _p]),

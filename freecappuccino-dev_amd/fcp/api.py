@@ -232,6 +232,31 @@ class Context:
         self._check(rc, "fcp_convdiff_cell")
         return a_out
 
+    def momentum_assemble(self, u, v, w, p, den, vis, flmass, viscos, urf, uo=None, vo=None,
+                          wo=None, uoo=None, voo=None, woo=None, visw=None, dt=1.0,
+                          time_scheme="steady", conv_scheme="linearUpwind", gds=1.0, dpdxi=None,
+                          grad=None, rhs=None, diag=None, ap=None, spv=None, src=None, a_out=None):
+        """calcuvw's assembly (velocity.f90:50-750) before its solves.  u, v, w,
+        p updated in place (boundary values); returns (rhs, diag), each (3, n),
+        for csrsolve_multi(..., diag=diag)."""
+        n = self.mesh.n_cells
+        if rhs is None:
+            rhs = np.zeros((3, n))
+        if diag is None:
+            diag = np.zeros((3, n))
+        ts = self.TIME_SCHEMES[time_scheme] if isinstance(time_scheme, str) else int(time_scheme)
+        cs = {"cds": 0, "linearUpwind": 1}[conv_scheme] if isinstance(conv_scheme, str) \
+            else int(conv_scheme)
+        urf = np.ascontiguousarray(urf, np.float64)
+        args = [u, v, w] + [_f64(x) if x is not None else None for x in (uo, vo, wo, uoo, voo, woo)]
+        rc = self.lib.fcp_momentum_assemble(
+            self.h, *[_ptr(x)[0] for x in args], _ptr(p)[0], _ptr(_f64(den))[0],
+            _ptr(_f64(vis))[0], _ptr(_f64(visw))[0], _ptr(_f64(flmass))[0], float(viscos),
+            float(dt), ts, cs, float(gds), _ptr(urf)[0], _ptr(dpdxi)[0], _ptr(grad)[0],
+            _ptr(rhs)[0], _ptr(diag)[0], _ptr(ap)[0], _ptr(spv)[0], _ptr(src)[0], _ptr(a_out)[0])
+        self._check(rc, "fcp_momentum_assemble")
+        return rhs, diag
+
     def pressure_assemble(self, u, v, w, p, dpdxi, den, apu, apv, apw, flmass, su,
                           const_mflux=True, flomas=0.0, a_out=None):
         """calcp_simple's pressure-correction assembly (calcp_simple.f90:

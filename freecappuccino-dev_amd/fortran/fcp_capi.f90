@@ -156,6 +156,18 @@ module fcp_capi
       type(c_ptr), value :: a  ! c_null_ptr or c_loc of a(nnz)
     end function
 
+    integer(c_int) function fcp_momentum_assemble(ctx, u, v, w, uo, vo, wo, uoo, voo, woo, p, &
+        den, vis, visw, flmass, viscos, dt, time_scheme, conv_scheme, gds, urf, dpdxi, grad, &
+        rhs, diagv, ap, spv, src, a) bind(C, name='fcp_momentum_assemble')
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: u(*), v(*), w(*), uo(*), vo(*), wo(*), uoo(*), voo(*), woo(*), p(*)
+      real(c_double) :: den(*), vis(*), visw(*), flmass(*), urf(*), rhs(*), diagv(*)
+      real(c_double), value :: viscos, dt, gds
+      integer(c_int), value :: time_scheme, conv_scheme
+      type(c_ptr), value :: dpdxi, grad, ap, spv, src, a  ! c_null_ptr or c_loc of arrays
+    end function
+
     integer(c_int) function fcp_pressure_assemble(ctx, u, v, w, p, dpdxi, den, apu, apv, apw, &
         flmass, const_mflux, flomas, a, su) bind(C, name='fcp_pressure_assemble')
       import :: c_int, c_double, c_ptr

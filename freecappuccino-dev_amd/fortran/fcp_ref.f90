@@ -23,6 +23,8 @@
 !                              (temperature.f90:118-366), time terms (:82-102)
 !   calculate_scalar(...)      calculate_temperature (temperature.f90:35-385) for a
 !                              scalar with explicit parameters
+!   calcuvw(...)               velocity.f90:50-750 (assembly + the three solves with a
+!                              diagonal per component) with explicit arguments
 !   calcp_simple(...)          calcp_simple.f90:69-419 (incompressible, no pressure
 !                              patches, one corrector) with explicit arguments
 !   fvx_grad_gauss / fvx_grad_gauss_corrected   fvxGradient.f90:1549,1664
@@ -301,6 +303,61 @@ contains
     if (trim(cScheme) == 'cds') sch = 0
     call fcp_check(fcp_convdiff_cell(ctx, flmass, diffcoef, phi, dphidxi, gam, sch, bc_kind, &
                                      urf, c_loc(a), su, sp), 'fcp_convdiff_cell')
+  end subroutine
+
+  ! calcuvw (velocity.f90:50-750) with the reference's module globals passed
+  ! explicitly: the momentum assembly (fcp_momentum_assemble) on the resident
+  ! matrix, then the three components solved together with their own
+  ! diagonals (fcp_csrsolve_multi_diag); apu, apv, apw receive 1/a_P for
+  ! calcp_simple, resor(3) each component's res0.  uo..woo are read for the
+  ! transient schemes only.
+  subroutine calcuvw(u, v, w, uo, vo, wo, uoo, voo, woo, p, dPdxi, den, vis, visw, flmass, &
+                     viscos, timestep, tscheme, cSchemeU, gdsU, urfU, apu, apv, apw, lSolverU, &
+                     maxiterU, tolAbsU, tolRelU, resor)
+    real(dp), intent(inout) :: u(numTotal), v(numTotal), w(numTotal), p(numTotal)
+    real(dp), intent(inout), target :: dPdxi(3, numTotal)
+    real(dp), intent(in) :: uo(numCells), vo(numCells), wo(numCells)
+    real(dp), intent(in) :: uoo(numCells), voo(numCells), woo(numCells)
+    real(dp), intent(in) :: den(numTotal), vis(numTotal), visw(*), flmass(numFaces)
+    real(dp), intent(in) :: viscos, timestep, gdsU, urfU(3), tolAbsU, tolRelU
+    character(len=*), intent(in) :: tscheme, cSchemeU, lSolverU
+    integer, intent(in) :: maxiterU
+    real(dp), intent(out), target :: apu(numCells), apv(numCells), apw(numCells)
+    real(dp), intent(out) :: resor(3)
+    real(dp), allocatable, target :: rhs(:, :), dg(:, :), ap3(:, :), fi(:, :)
+    type(fcp_solve_opts) :: opts
+    integer(c_int) :: ts, sch, rc
+    select case (trim(tscheme))
+    case ('bdf')
+      ts = 1
+    case ('bdf2')
+      ts = 2
+    case ('cn')
+      ts = 3
+    case default
+      ts = 0
+    end select
+    sch = 1
+    if (trim(cSchemeU) == 'cds') sch = 0
+    allocate(rhs(numCells, 3), dg(numCells, 3), ap3(numCells, 3), fi(numCells, 3))
+    call fcp_check(fcp_momentum_assemble(ctx, u, v, w, uo, vo, wo, uoo, voo, woo, p, den, vis, &
+                                         visw, flmass, viscos, timestep, ts, sch, gdsU, urfU, &
+                                         c_loc(dPdxi), c_null_ptr, rhs, dg, c_loc(ap3), &
+                                         c_null_ptr, c_null_ptr, c_loc(a)), 'fcp_momentum_assemble')
+    apu = ap3(:, 1)
+    apv = ap3(:, 2)
+    apw = ap3(:, 3)
+    fi(:, 1) = u(1:numCells)
+    fi(:, 2) = v(1:numCells)
+    fi(:, 3) = w(1:numCells)
+    call fcp_solve_opts_default(opts)
+    rc = fcp_csrsolve_multi_diag(ctx, lSolverU, len_trim(lSolverU), 3, fi, numCells, rhs, dg, &
+                                 resor, maxiterU, tolAbsU, tolRelU, 0, opts)
+    if (rc < 0) call fcp_check(rc, 'fcp_csrsolve_multi_diag')
+    u(1:numCells) = fi(:, 1)
+    v(1:numCells) = fi(:, 2)
+    w(1:numCells) = fi(:, 3)
+    deallocate(rhs, dg, ap3, fi)
   end subroutine
 
   ! calcp_simple (calcp_simple.f90:69-419) for an incompressible case

@@ -255,6 +255,38 @@ int fcp_pressure_correct(fcp_ctx *ctx, double *pp, double *flmass, double *u, do
                          const double *apv, const double *apw, double urfp, int pref_cell,
                          double *su, double *sv, double *sw);
 
+/* The momentum assembly of calcuvw (velocity.f90:50-750) before its three
+ * solves, for steady, BDF1, BDF2 and Crank-Nicolson time schemes
+ * (time_scheme 0/1/2/3), conv_scheme 0 'cds' / 1 'linearUpwind', Gauss
+ * gradients, no buoyancy, Lorentz force, PISO or constant mass flux:
+ * updateVelocityAtBoundary, the gradients of u, v, w, gradp_and_sources(p),
+ * the time terms, facefluxuvw per inner face (with sngrad's transposed
+ * stress terms, :754-878), per boundary face by patch type inlet / outlet /
+ * pressure (facefluxuvw_bnd :882-1037), symmetry (:281-306, which reads the
+ * viscosity at index numCells+1), periodic (facefluxuvw_periodic
+ * :1038-1180) and wall (visw of the face's place among the wall faces, in
+ * patch order), the Crank-Nicolson block and, per component, the diagonal
+ * a_P = sp - (sum(row) - a_row_diag) with its relaxation factor urf[3].  As in
+ * the reference the row sums of u read whatever the resident matrix held on
+ * its diagonal before the call, and v's and w's a zeroed diagonal.
+ * u, v, w, p [numTotal] (boundary values updated); uo, vo, wo (transient),
+ * uoo, voo, woo (BDF2) [numCells]; den, vis [numTotal]; visw [wall faces];
+ * flmass [numFaces].  Outputs: the resident matrix's off-diagonals (shared)
+ * and w's diagonal, as the reference leaves a (a optional copy);
+ * rhs and diagv [3*numCells]: the right-hand side and relaxed diagonal of
+ * u, v and w (column-major), ready for fcp_csrsolve_multi_diag; optional
+ * dpdxi [3*numTotal] (cell rows: grad p), grad [9*numTotal] (grad u, v, w),
+ * ap [3*numCells] (1/a_P: apu, apv, apw), spv [3*numCells] (the diagonal
+ * sources spu, spv, sp) and src [3*numCells] (su, sv, sw before the
+ * relaxation terms). */
+int fcp_momentum_assemble(fcp_ctx *ctx, double *u, double *v, double *w, const double *uo,
+                          const double *vo, const double *wo, const double *uoo,
+                          const double *voo, const double *woo, double *p, const double *den,
+                          const double *vis, const double *visw, const double *flmass,
+                          double viscos, double dt, int time_scheme, int conv_scheme, double gds,
+                          const double *urf, double *dpdxi, double *grad, double *rhs,
+                          double *diagv, double *ap, double *spv, double *src, double *a);
+
 /* Time terms (temperature.f90:82-102): scheme 1 BDF1 (and the volume part of
  * Crank-Nicolson), 2 BDF2; apotime = den*vol/dt; su/sp updated in place. */
 int fcp_ddt(fcp_ctx *ctx, int scheme, const double *den, double dt,
