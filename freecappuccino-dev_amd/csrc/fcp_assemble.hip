@@ -862,7 +862,7 @@ __global__ void k_pc_grad(int n, int Fi, const int *__restrict__ cf_ptr,
   pr[c] = pr[c] + urfp * (pp[c] - pp[pref]);
 }
 
-// updateVelocityAtBoundary (velocity.f90:1184-1262): empty and periodic faces
+// updateVelocityAtBoundary (velocity.f90:1184-1279): empty and periodic faces
 // take the owner's velocity, symmetry faces its reflection with the
 // (unnormalised) area vector.
 __global__ void k_pc_velbnd(int n, int Fi, int Fb, const int *__restrict__ owner,
@@ -956,7 +956,7 @@ __device__ __forceinline__ void mom_inner(const GeomP &g, int f, int p, int q, c
   }
 }
 
-// facefluxuvw_periodic (velocity.f90:1038-1180): periodic face f, owner p,
+// facefluxuvw_periodic (velocity.f90:1038-1179): periodic face f, owner p,
 // twin owner q.
 __device__ __forceinline__ void mom_periodic(const GeomP &g, int f, int p, int q,
                                              const MomIn &in, double &can, double &cap,
@@ -1015,7 +1015,7 @@ __global__ void k_mom_cells(int n, int Fi, const int *__restrict__ cf_ptr,
   double S[3], SP[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int r = 0; r < 3; ++r) S[r] = s3[r * (size_t)n + c];
-  if (in.ts > 0) {  // :162-214
+  if (in.ts > 0) {  // :226-279
     const double apotime = in.den[c] * g.vol[c] / in.dt;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -1066,7 +1066,7 @@ __global__ void k_mom_cells(int n, int Fi, const int *__restrict__ cf_ptr,
     const size_t jb = (size_t)n + b;
     const double ax = g.arx[f], ay = g.ary[f], az = g.arz[f];
     if (t == FCP_BC_INLET || t == FCP_BC_OUTLET || t == FCP_BC_PRESSURE) {
-      // facefluxuvw_bnd (velocity.f90:882-1037), scatter :262-279
+      // facefluxuvw_bnd (velocity.f90:882-1034), scatter :328-352
       const double xpn = g.xf[f] - g.xc[c], ypn = g.yf[f] - g.yc[c], zpn = g.zf[f] - g.zc[c];
       const double vole = (xpn * ax + ypn * ay) + zpn * az;
       const double Dfi = ((ax * ax + ay * ay) + az * az) / vole;
@@ -1094,7 +1094,7 @@ __global__ void k_mom_cells(int n, int Fi, const int *__restrict__ cf_ptr,
         SP[r] = SP[r] - cb;
         S[r] = S[r] - cb * in.U[r][jb] + (fde + fdi);
       }
-    } else if (t == FCP_BC_SYMMETRY) {  // :281-306, vis(inp) with inp = numCells+1
+    } else if (t == FCP_BC_SYMMETRY) {  // :354-391, vis(inp) with inp = numCells+1
       const double are = sqrt((ax * ax + ay * ay) + az * az);
       const double arer = 1.0 / are;
       const double nf[3] = {ax * arer, ay * arer, az * arer};
@@ -1115,7 +1115,7 @@ __global__ void k_mom_cells(int n, int Fi, const int *__restrict__ cf_ptr,
       if (sl >= 0) a[sl] = can;
 #pragma unroll
       for (int r = 0; r < 3; ++r) S[r] = S[r] + sadd[r];
-    } else if (t == FCP_BC_WALL) {  // :330-360
+    } else if (t == FCP_BC_WALL) {  // :437-482
       const double vw = in.visw[in.wall_of[b]];
       const double viss = in.viscos > vw ? in.viscos : vw;
       const double are = sqrt((ax * ax + ay * ay) + az * az);
@@ -1135,7 +1135,7 @@ __global__ void k_mom_cells(int n, int Fi, const int *__restrict__ cf_ptr,
     }
   }
   twin_side(0x7fffffff);
-  // Crank-Nicolson (:375-390) and the diagonals (:392-450), component by
+  // Crank-Nicolson (:582-606) and the diagonals (:608-746), component by
   // component; the row sums read whatever the diagonal entry holds
   const int r0 = ia[c], r1 = ia[c + 1], dg = diag[c];
   if (in.ts == 3)
@@ -1362,12 +1362,12 @@ int fcp_launch_momentum(fcp_ctx *ctx, double *u, double *v, double *w, const dou
   const int n = ctx->n, Fi = ctx->Fi, Fb = ctx->Fb;
   const size_t nt = (size_t)n + Fb;
   const GeomP g = geom_of(ctx);
-  if (Fb > 0)  // updateVelocityAtBoundary (:153)
+  if (Fb > 0)  // updateVelocityAtBoundary (:166)
     k_pc_velbnd<<<grid_for(Fb, BLK), BLK, 0, ctx->stream>>>(n, Fi, Fb, ctx->owner.p, ctx->bpatch.p,
                                                             ctx->bctype.p, g, u, v, w);
   double *U[3] = {u, v, w};
   for (int r = 0; r < 3; ++r) FCP_TRY(fcp_launch_grad_gauss(ctx, U[r], grad9 + 3 * nt * r));
-  for (int stage = 1; stage <= 2; ++stage) {  // gradp_and_sources(p) (:160)
+  for (int stage = 1; stage <= 2; ++stage) {  // gradp_and_sources(p) (:173)
     if (Fb > 0)
       k_pc_bpres<<<grid_for(Fb, BLK), BLK, 0, ctx->stream>>>(n, Fi, Fb, stage, ctx->owner.p,
                                                              ctx->bpatch.p, ctx->bctype.p, g,

@@ -242,7 +242,7 @@ int fcp_convdiff_cell(fcp_ctx *ctx, const double *flmass, const double *diffcoef
  * wall faces with the stage-1 gradient; dpdxi's cell rows receive the
  * gradient of pp, su/sv/sw = -sum_f pp_f S_f); u += su apu, v += sv apv,
  * w += sw apw, p += urfp (pp - ppref) on the cells; then
- * updateVelocityAtBoundary (velocity.f90:1184-1262: empty and periodic
+ * updateVelocityAtBoundary (velocity.f90:1184-1279: empty and periodic
  * faces take the owner's velocity, symmetry faces u_P - (u_P.S) S).  pp
  * (boundary values updated), flmass, u, v, w, p, dpdxi in/out; su, sv, sw
  * [numCells] optional outputs.  Uses the resident matrix of the assembly. */
@@ -262,9 +262,9 @@ int fcp_pressure_correct(fcp_ctx *ctx, double *pp, double *flmass, double *u, do
  * updateVelocityAtBoundary, the gradients of u, v, w, gradp_and_sources(p),
  * the time terms, facefluxuvw per inner face (with sngrad's transposed
  * stress terms, :754-878), per boundary face by patch type inlet / outlet /
- * pressure (facefluxuvw_bnd :882-1037), symmetry (:281-306, which reads the
+ * pressure (facefluxuvw_bnd :882-1034), symmetry (:354-391, which reads the
  * viscosity at index numCells+1), periodic (facefluxuvw_periodic
- * :1038-1180) and wall (visw of the face's place among the wall faces, in
+ * :1038-1179) and wall (visw of the face's place among the wall faces, in
  * patch order), the Crank-Nicolson block and, per component, the diagonal
  * a_P = sp - (sum(row) - a_row_diag) with its relaxation factor urf[3].  As in
  * the reference the row sums of u read whatever the resident matrix held on

@@ -1729,7 +1729,7 @@ static void gradp_sources(const orc_mesh *m, const orc_geom *g, double *pp, doub
   }
 }
 
-/* updateVelocityAtBoundary, velocity.f90:1184-1262 */
+/* updateVelocityAtBoundary, velocity.f90:1184-1279 */
 static void vel_bnd(const orc_mesh *m, const orc_geom *g, double *u, double *v, double *w) {
   const int n = m->n_cells, Fi = m->n_inner;
   for (int ib = 0; ib < m->n_patches; ++ib) {
@@ -1838,12 +1838,12 @@ int orc_momentum(const orc_mesh *m, const orc_geom *g, const orc_csr *c, const i
   double *spu = sp3, *spv = sp3 + n, *sp = sp3 + 2 * (size_t)n;
   memset(sp3, 0, sizeof(double) * 3 * (size_t)n);
   const double *U[3] = {u, v, w}, *G[3] = {dudxi, dvdxi, dwdxi};
-  vel_bnd(m, g, u, v, w); /* :153 */
+  vel_bnd(m, g, u, v, w); /* :166 */
   orc_grad_gauss(m, g, u, dudxi);
   orc_grad_gauss(m, g, v, dvdxi);
   orc_grad_gauss(m, g, w, dwdxi);
-  gradp_sources(m, g, pr, dpdxi, su, sv, sw); /* :160 */
-  if (time_scheme > 0) /* :162-214 */
+  gradp_sources(m, g, pr, dpdxi, su, sv, sw); /* :173 */
+  if (time_scheme > 0) /* :226-279 */
     for (int i = 0; i < n; ++i) {
       const double apotime = den[i] * g->vol[i] / dt;
       if (time_scheme == 1 || time_scheme == 3) {
@@ -1862,7 +1862,7 @@ int orc_momentum(const orc_mesh *m, const orc_geom *g, const orc_csr *c, const i
         sp[i] = sp[i] + 1.5 * apotime;
       }
     }
-  for (int f = 0; f < Fi; ++f) { /* facefluxuvw :754-878, scatter :216-240 */
+  for (int f = 0; f < Fi; ++f) { /* facefluxuvw :754-878, scatter :286-314 */
     const int p = m->owner[f], q = m->neighbour[f];
     const double lam = g->facint[f], fxn = lam, fxp = 1.0 - lam;
     const double fm = flmass[f];
@@ -1897,12 +1897,12 @@ int orc_momentum(const orc_mesh *m, const orc_geom *g, const orc_csr *c, const i
     sw[q] = sw[q] - sadd[2];
   }
   int iper = 0, iwall = 0, l = Fi;
-  for (int ib = 0; ib < m->n_patches; ++ib) { /* :242-368 */
+  for (int ib = 0; ib < m->n_patches; ++ib) { /* :326-482 */
     const int t = m->bctype[ib];
     for (int i = 0; i < m->nfaces[ib]; ++i) {
       const int f = m->start_face[ib] + i, p = m->owner[f], b = n + (f - Fi);
       const double ax = g->arx[f], ay = g->ary[f], az = g->arz[f];
-      if (t == 3 || t == 4 || t == 5) { /* facefluxuvw_bnd :882-1037 */
+      if (t == 3 || t == 4 || t == 5) { /* facefluxuvw_bnd :882-1034 */
         const double xpn = g->xf[f] - g->xc[p], ypn = g->yf[f] - g->yc[p],
                      zpn = g->zf[f] - g->zc[p];
         const double vole = (xpn * ax + ypn * ay) + zpn * az;
@@ -1926,7 +1926,7 @@ int orc_momentum(const orc_mesh *m, const orc_geom *g, const orc_csr *c, const i
         su[p] = su[p] - cb * u[b] + sadd[0];
         sv[p] = sv[p] - cb * v[b] + sadd[1];
         sw[p] = sw[p] - cb * w[b] + sadd[2];
-      } else if (t == 2) { /* symmetry :281-306 (vis(inp) with inp = numCells+1) */
+      } else if (t == 2) { /* symmetry :354-391 (vis(inp) with inp = numCells+1) */
         const double are = sqrt((ax * ax + ay * ay) + az * az);
         const double arer = 1.0 / are;
         const double nxf = ax * arer, nyf = ay * arer, nzf = az * arer;
@@ -1939,7 +1939,7 @@ int orc_momentum(const orc_mesh *m, const orc_geom *g, const orc_csr *c, const i
         spu[p] = spu[p] + cf * (nxf * nxf);
         spv[p] = spv[p] + cf * (nyf * nyf);
         sp[p] = sp[p] + cf * (nzf * nzf);
-      } else if (t == 7) { /* facefluxuvw_periodic :1038-1180 */
+      } else if (t == 7) { /* facefluxuvw_periodic :1038-1179 */
         const int q = m->owner[m->twin_start[iper] + i];
         const double fxn = 0.5, fxp = fxn;
         const double xpn = 2.0 * (g->xf[f] - g->xc[p]), ypn = 2.0 * (g->yf[f] - g->yc[p]),
@@ -1977,7 +1977,7 @@ int orc_momentum(const orc_mesh *m, const orc_geom *g, const orc_csr *c, const i
         su[q] = su[q] - sadd[0];
         sv[q] = sv[q] - sadd[1];
         sw[q] = sw[q] - sadd[2];
-      } else if (t == 1) { /* wall :330-360 */
+      } else if (t == 1) { /* wall :437-482 */
         const double viss = viscos > visw[iwall] ? viscos : visw[iwall];
         ++iwall;
         const double are = sqrt((ax * ax + ay * ay) + az * az);
@@ -1997,7 +1997,7 @@ int orc_momentum(const orc_mesh *m, const orc_geom *g, const orc_csr *c, const i
     }
     if (t == 7) ++iper;
   }
-  /* CN (:375-390) and the three diagonals (:392-450), component by
+  /* CN (:582-606) and the three diagonals (:608-746), component by
    * component; the row sums read whatever the diagonal entry holds */
   const double *O[3] = {uo, vo, wo};
   double *S[3] = {su, sv, sw}, *SP[3] = {spu, spv, sp};

@@ -270,7 +270,7 @@ double orc_s0_init(void);
  * correction (:333-376), ppref = pp[pref] (0-based), gradp_and_sources
  * (nablap.f90:19-209 with bpres) into su/sv/sw[n] and dpdxi's cell rows,
  * u,v,w += s*ap, p += urfp (pp - ppref), then updateVelocityAtBoundary
- * (velocity.f90:1184-1262). */
+ * (velocity.f90:1184-1279). */
 int orc_pcorr_assemble(const orc_mesh *m, const orc_geom *g, const orc_csr *c,
                        const int *slot_pn, const int *slot_np, double *u, double *v, double *w,
                        const double *pr, const double *dpdxi, const double *den,

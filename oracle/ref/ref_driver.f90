@@ -50,7 +50,7 @@
 !               on inner and periodic faces, ppref = pp(pRefCell),
 !               gradp_and_sources (nablap.f90:19-209, pscheme 'linear', bpres
 !               stages), velocity and pressure correction, then
-!               updateVelocityAtBoundary velocity.f90:1184-1262)
+!               updateVelocityAtBoundary velocity.f90:1184-1279)
 !   momentum u v w uo vo wo uoo voo p den vis visw flmass astale viscos dt tscheme scheme gds
 !            urf1 urf2 urf3 maxiter tolrel
 !            a_out su_out sv_out sw_out spu_out spv_out sp_out apu_out apv_out apw_out
