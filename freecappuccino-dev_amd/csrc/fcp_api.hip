@@ -279,6 +279,10 @@ int fcp_tune_set(fcp_ctx *ctx, int param, int value) {
         ctx->tin_have = false;  // operands rebuilt in the chosen layout
       }
       return 0;
+    case FCP_TUNE_ASM_BLOCKS:
+      if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "assembly blocks is 0 or 1");
+      ctx->tune_asm_blk = value;
+      return 0;
     case FCP_TUNE_COARSE_FUSE:
       if (value != 0 && value != 1) return fcp_fail(ctx, FCP_ERR_ARG, "coarse fusion is 0 or 1");
       ctx->tune_coarse_fuse = value;
@@ -1732,6 +1736,7 @@ static int explicit_solve(int code, int n, int nnz, const int *ia, const int *ja
   FCP_TRY(fcp_build_schedules(ctx));
   FCP_TRY(fcp_build_sell(ctx));
   ctx->have_csr = true;
+  ctx->asm_state = 0;
   FCP_TRY(fcp_matrix_set(ctx, a));
   fcp_solve_opts o;
   fcp_solve_opts_default(&o);

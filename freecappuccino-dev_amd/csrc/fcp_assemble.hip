@@ -7,6 +7,8 @@
 // every row is produced without atomics and bit-equal to the reference.
 #include "fcp_internal.h"
 
+#include <algorithm>
+
 namespace {
 
 constexpr int BLK = 256;
@@ -1172,6 +1174,196 @@ __global__ void k_mom_cells(int n, int Fi, const int *__restrict__ cf_ptr,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Block forms of the face Laplacian, Gauss gradient and non-orthogonal
+// correction: a workgroup takes 256 consecutive cells; first every inner face
+// its cells own and every face reaching them from another block is computed
+// once into LDS (coalesced runs of faces), then each cell sums its faces'
+// terms from LDS in ascending face order, exactly as the per-cell kernels do.
+// ---------------------------------------------------------------------------
+constexpr int ASM_B = 256;        // cells per block
+constexpr int ASM_MAXF = 1536;    // faces staged per block
+constexpr int ASM_MAXE = 2560;    // CSR entries of a block's rows staged for the Laplacian
+
+struct AsmBlk {
+  const int *fo, *incp, *inc, *slot;
+  int nb;
+};
+// Per cell-face incidence: bit 31 a boundary face (read the face id from the
+// cell's face list), bit 30 the cell is the face's neighbour, bits 0-10 the
+// face's LDS slot, bits 11-22 the row entry's position in the block's CSR
+// range (4095: none).
+constexpr int ASM_BND = (int)0x80000000u, ASM_NB = 0x40000000;
+__device__ __forceinline__ int asm_j(int sl) { return sl & 0x7FF; }
+__device__ __forceinline__ int asm_e(int sl) { return (sl >> 11) & 0xFFF; }
+
+__device__ __forceinline__ int asm_block(int nb) {
+  const int chunk = (nb + 7) / 8;
+  return (int)(blockIdx.x & 7) * chunk + (int)(blockIdx.x >> 3);
+}
+int asm_grid(int nb) { return nb > 0 ? 8 * ((nb + 7) / 8) : 1; }
+
+// face of staged position j of block b
+__device__ __forceinline__ int asm_face(const AsmBlk &B, int b, int j, int nown) {
+  return j < nown ? B.fo[b] + j : B.inc[B.incp[b] + (j - nown)];
+}
+
+__global__ void __launch_bounds__(ASM_B)
+    k_grad_blk(int n, int Fi, AsmBlk B, const int *__restrict__ cf_ptr, const int *__restrict__ cf,
+               const int *__restrict__ owner, const int *__restrict__ neighbour, GeomP g,
+               const double *__restrict__ u, double *__restrict__ dudxi) {
+  __shared__ double cont[3 * ASM_MAXF];
+  const int b = asm_block(B.nb);
+  if (b >= B.nb) return;
+  const int nown = B.fo[b + 1] - B.fo[b], nf = nown + (B.incp[b + 1] - B.incp[b]);
+  for (int j = threadIdx.x; j < nf; j += ASM_B) {
+    const int f = asm_face(B, b, j, nown);
+    const int p = owner[f], q = neighbour[f];
+    const double fie = u[p] + (u[q] - u[p]) * g.facint[f];
+    cont[3 * j] = fie * g.arx[f];
+    cont[3 * j + 1] = fie * g.ary[f];
+    cont[3 * j + 2] = fie * g.arz[f];
+  }
+  __syncthreads();
+  const int c = b * ASM_B + (int)threadIdx.x;
+  if (c >= n) return;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+    const int sl = B.slot[k];
+    if (sl >= 0) {
+      const int j = asm_j(sl);
+      if (!(sl & ASM_NB)) {
+        gx = gx + cont[3 * j];
+        gy = gy + cont[3 * j + 1];
+        gz = gz + cont[3 * j + 2];
+      } else {
+        gx = gx - cont[3 * j];
+        gy = gy - cont[3 * j + 1];
+        gz = gz - cont[3 * j + 2];
+      }
+    } else {  // gradbc :1696-1715
+      const int f = cf[k];
+      const double fi = u[n + (f - Fi)];
+      gx = gx + fi * g.arx[f];
+      gy = gy + fi * g.ary[f];
+      gz = gz + fi * g.arz[f];
+    }
+  }
+  const double volr = 1.0 / g.vol[c];
+  dudxi[3 * (size_t)c] = gx * volr;
+  dudxi[3 * (size_t)c + 1] = gy * volr;
+  dudxi[3 * (size_t)c + 2] = gz * volr;
+}
+
+__global__ void k_zero_rows3(int n, int ntot, double *__restrict__ x) {
+  const int i = n + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntot) return;
+  x[3 * (size_t)i] = 0.0;
+  x[3 * (size_t)i + 1] = 0.0;
+  x[3 * (size_t)i + 2] = 0.0;
+}
+
+__global__ void __launch_bounds__(ASM_B)
+    k_nonorth_blk(int n, int Fi, AsmBlk B, const int *__restrict__ cf_ptr,
+                  const int *__restrict__ cf, const int *__restrict__ owner,
+                  const int *__restrict__ neighbour, GeomP g, const double *__restrict__ gam,
+                  const double *__restrict__ gr, double *__restrict__ su) {
+  __shared__ double cont[ASM_MAXF];
+  const int b = asm_block(B.nb);
+  if (b >= B.nb) return;
+  const int nown = B.fo[b + 1] - B.fo[b], nf = nown + (B.incp[b + 1] - B.incp[b]);
+  for (int j = threadIdx.x; j < nf; j += ASM_B) {
+    const int f = asm_face(B, b, j, nown);
+    const int p = owner[f], q = neighbour[f];
+    const double fxn = g.facint[f], fxp = 1.0 - g.facint[f];
+    const double xpn = g.xc[q] - g.xc[p], ypn = g.yc[q] - g.yc[p], zpn = g.zc[q] - g.zc[p];
+    const double Dff = g.Df[f];
+    double dfixi = gr[3 * p] * fxp + gr[3 * q] * fxn;
+    double dfiyi = gr[3 * p + 1] * fxp + gr[3 * q + 1] * fxn;
+    double dfizi = gr[3 * p + 2] * fxp + gr[3 * q + 2] * fxn;
+    dfixi = dfixi * (g.arx[f] - Dff * xpn);
+    dfiyi = dfiyi * (g.ary[f] - Dff * ypn);
+    dfizi = dfizi * (g.arz[f] - Dff * zpn);
+    cont[j] = gam[f] * ((dfixi + dfiyi) + dfizi);
+  }
+  __syncthreads();
+  const int c = b * ASM_B + (int)threadIdx.x;
+  if (c >= n) return;
+  double s = su[c];
+  for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+    const int sl = B.slot[k];
+    if (sl < 0) continue;
+    const int j = asm_j(sl);
+    s = (sl & ASM_NB) ? s - cont[j] : s + cont[j];
+  }
+  su[c] = s;
+}
+
+// Laplacian rows of a block, written to a from LDS in one coalesced pass
+// (entries no face writes -- the periodic pairs' -- are written as 0).
+__global__ void __launch_bounds__(ASM_B)
+    k_lap_blk(int n, int Fi, AsmBlk B, const int *__restrict__ cf_ptr, const int *__restrict__ cf,
+              const int *__restrict__ owner, const int *__restrict__ neighbour, GeomP g,
+              const double *__restrict__ mu, const double *__restrict__ mu_f,
+              const unsigned char *__restrict__ bc_kind, const double *__restrict__ phi,
+              const int *__restrict__ slot_pn, const int *__restrict__ slot_np,
+              const int *__restrict__ ia, const int *__restrict__ diag, double *__restrict__ a,
+              double *__restrict__ su) {
+  __shared__ double cont[ASM_MAXF];
+  __shared__ double rb[ASM_MAXE];
+  const int b = asm_block(B.nb);
+  if (b >= B.nb) return;
+  const int c0 = b * ASM_B, c1 = min(n, c0 + ASM_B);
+  const int e0 = ia[c0], ne = ia[c1] - e0;
+  for (int e = threadIdx.x; e < ne; e += ASM_B) rb[e] = 0.0;
+  const int nown = B.fo[b + 1] - B.fo[b], nf = nown + (B.incp[b + 1] - B.incp[b]);
+  for (int j = threadIdx.x; j < nf; j += ASM_B) {
+    const int f = asm_face(B, b, j, nown);
+    double cap;
+    if (mu_f) {
+      cap = mu_f[f] * g.Df[f];
+    } else {  // faceFluxLaplacian :113-151
+      const int p = owner[f], q = neighbour[f];
+      const double fxn = g.facint[f], fxp = 1.0 - g.facint[f];
+      const double xpn = g.xc[q] - g.xc[p], ypn = g.yc[q] - g.yc[p], zpn = g.zc[q] - g.zc[p];
+      const double ax = g.arx[f], ay = g.ary[f], az = g.arz[f];
+      const double smdpn = ((ax * ax + ay * ay) + az * az) / ((ax * xpn + ay * ypn) + az * zpn);
+      cap = (fxp * mu[p] + fxn * mu[q]) * smdpn;
+    }
+    cont[j] = cap;
+  }
+  __syncthreads();
+  const int c = c0 + (int)threadIdx.x;
+  if (c < n) {
+    double ad = 0.0;  // a(diag) starts from the cleared matrix (:35)
+    double s = su[c];
+    for (int k = cf_ptr[c]; k < cf_ptr[c + 1]; ++k) {
+      const int sl = B.slot[k];
+      if (sl >= 0) {
+        const double cap = cont[asm_j(sl)];
+        const int e = asm_e(sl);
+        if (e != 0xFFF) rb[e] = cap;
+        ad = ad - cap;
+      } else {
+        const int f = cf[k];
+        if (bc_kind && bc_kind[f - Fi] != 0) continue;
+        const double ax = g.arx[f], ay = g.ary[f], az = g.arz[f];
+        const double are = sqrt((ax * ax + ay * ay) + az * az);
+        const double nxf = ax / are, nyf = ay / are, nzf = az / are;
+        const double dfn = ((g.xf[f] - g.xc[c]) * nxf + (g.yf[f] - g.yc[c]) * nyf) +
+                           (g.zf[f] - g.zc[c]) * nzf;
+        const double dcoef = (mu_f ? mu_f[f] : mu[c]) * are / dfn;
+        ad = ad - dcoef;
+        s = s - dcoef * phi[n + (f - Fi)];
+      }
+    }
+    rb[diag[c] - e0] = ad;
+    su[c] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ne; e += ASM_B) a[e0 + e] = rb[e];
+}
+
 GeomP geom_of(fcp_ctx *ctx) {
   return GeomP{ctx->xc.p, ctx->yc.p, ctx->zc.p, ctx->vol.p, ctx->arx.p, ctx->ary.p, ctx->arz.p,
                ctx->xf.p, ctx->yf.p, ctx->zf.p, ctx->facint.p, ctx->Df.p};
@@ -1179,13 +1371,107 @@ GeomP geom_of(fcp_ctx *ctx) {
 
 }  // namespace
 
+// Cell blocks of the per-cell assembly kernels (once per CSR pattern): needs
+// the inner faces sorted by owner and every block's staged faces within LDS.
+static int asm_prepare(fcp_ctx *ctx) {
+  if (ctx->asm_state != 0 || !ctx->have_csr) return 0;
+  ctx->asm_state = -1;
+  const int n = ctx->n, Fi = ctx->Fi;
+  if (n == 0 || Fi == 0) return 0;
+  std::vector<int> own(Fi), nbr(Fi), cfp(n + 1), ia(n + 1), spn(Fi), snp(Fi);
+  FCP_HIP(ctx, hipMemcpy(own.data(), ctx->owner.p, sizeof(int) * Fi, hipMemcpyDeviceToHost));
+  FCP_HIP(ctx, hipMemcpy(nbr.data(), ctx->neighbour.p, sizeof(int) * Fi, hipMemcpyDeviceToHost));
+  FCP_HIP(ctx, hipMemcpy(cfp.data(), ctx->cf_ptr.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  FCP_HIP(ctx, hipMemcpy(ia.data(), ctx->ia.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+  FCP_HIP(ctx, hipMemcpy(spn.data(), ctx->slot_pn.p, sizeof(int) * Fi, hipMemcpyDeviceToHost));
+  FCP_HIP(ctx, hipMemcpy(snp.data(), ctx->slot_np.p, sizeof(int) * Fi, hipMemcpyDeviceToHost));
+  for (int f = 1; f < Fi; ++f)
+    if (own[f] < own[f - 1]) return 0;
+  const int nb = (n + ASM_B - 1) / ASM_B;
+  std::vector<int> fo(nb + 1), incp(nb + 1, 0);
+  for (int b = 0; b <= nb; ++b)
+    fo[b] = (int)(std::lower_bound(own.begin(), own.end(), b * ASM_B) - own.begin());
+  for (int f = 0; f < Fi; ++f)
+    if (nbr[f] / ASM_B != own[f] / ASM_B) incp[nbr[f] / ASM_B + 1]++;
+  for (int b = 0; b < nb; ++b) incp[b + 1] += incp[b];
+  std::vector<int> inc(incp[nb] > 0 ? incp[nb] : 1), fill(incp.begin(), incp.end() - 1);
+  for (int f = 0; f < Fi; ++f)
+    if (nbr[f] / ASM_B != own[f] / ASM_B) inc[fill[nbr[f] / ASM_B]++] = f;
+  bool rows = true;
+  for (int b = 0; b < nb; ++b) {
+    if ((fo[b + 1] - fo[b]) + (incp[b + 1] - incp[b]) > ASM_MAXF) return 0;
+    const int c0 = b * ASM_B, c1 = std::min(n, c0 + ASM_B);
+    if (ia[c1] - ia[c0] > ASM_MAXE) rows = false;
+  }
+  const int ncf = cfp[n];
+  std::vector<int> cf(ncf > 0 ? ncf : 1), slot(ncf > 0 ? ncf : 1, ASM_BND);
+  FCP_HIP(ctx, hipMemcpy(cf.data(), ctx->cf_face.p, sizeof(int) * ncf, hipMemcpyDeviceToHost));
+  for (int c = 0; c < n; ++c) {
+    const int b = c / ASM_B, nown = fo[b + 1] - fo[b], e0 = ia[b * ASM_B];
+    for (int k = cfp[c]; k < cfp[c + 1]; ++k) {
+      const int f = cf[k];
+      if (f >= Fi) continue;  // boundary: ASM_BND
+      int j;
+      if (f >= fo[b] && f < fo[b + 1]) {
+        j = f - fo[b];
+      } else {
+        const int *lo = inc.data() + incp[b], *hi = inc.data() + incp[b + 1];
+        const int *it = std::lower_bound(lo, hi, f);
+        if (it == hi || *it != f) return 0;
+        j = nown + (int)(it - lo);
+      }
+      const int rs = own[f] != c ? snp[f] : spn[f];
+      int e = rs >= 0 ? rs - e0 : 0xFFF;
+      if (e < 0 || e > 0xFFF) e = 0xFFF;
+      if (rs >= 0 && e == 0xFFF) rows = false;  // outside the staged range
+      slot[k] = j | (e << 11) | (own[f] != c ? ASM_NB : 0);
+    }
+  }
+  FCP_HIP(ctx, ctx->asm_fo.alloc(nb + 1));
+  FCP_HIP(ctx, ctx->asm_incp.alloc(nb + 1));
+  FCP_HIP(ctx, ctx->asm_inc.alloc(inc.size()));
+  FCP_HIP(ctx, ctx->asm_slot.alloc(slot.size()));
+  FCP_HIP(ctx, hipMemcpy(ctx->asm_fo.p, fo.data(), sizeof(int) * fo.size(), hipMemcpyHostToDevice));
+  FCP_HIP(ctx, hipMemcpy(ctx->asm_incp.p, incp.data(), sizeof(int) * incp.size(),
+                         hipMemcpyHostToDevice));
+  FCP_HIP(ctx, hipMemcpy(ctx->asm_inc.p, inc.data(), sizeof(int) * inc.size(),
+                         hipMemcpyHostToDevice));
+  FCP_HIP(ctx, hipMemcpy(ctx->asm_slot.p, slot.data(), sizeof(int) * slot.size(),
+                         hipMemcpyHostToDevice));
+  ctx->asm_nb = nb;
+  ctx->asm_rows = rows;
+  ctx->asm_state = 1;
+  return 0;
+}
+
+static bool asm_on(fcp_ctx *ctx) {
+  if (!ctx->tune_asm_blk) return false;
+  if (asm_prepare(ctx) != 0) {
+    ctx->asm_state = -1;
+    return false;
+  }
+  return ctx->asm_state == 1;
+}
+
+static AsmBlk asm_view(const fcp_ctx *ctx) {
+  return AsmBlk{ctx->asm_fo.p, ctx->asm_incp.p, ctx->asm_inc.p, ctx->asm_slot.p, ctx->asm_nb};
+}
+
 int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
                          const unsigned char *bc_kind, const double *phi, double *su) {
   FCP_TRY(fcp_coarse_join(ctx));
+  const bool blk = asm_on(ctx) && ctx->asm_rows;
   ProfScope ps(ctx, mu_f ? "laplacian_face" : "laplacian");
-  k_laplacian<<<xcd_grid(ctx->n), BLK, 0, ctx->stream>>>(
-      ctx->n, ctx->Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p, geom_of(ctx),
-      mu, mu_f, bc_kind, phi, ctx->slot_pn.p, ctx->slot_np.p, ctx->diag.p, ctx->a.p, su);
+  if (blk)
+    k_lap_blk<<<asm_grid(ctx->asm_nb), ASM_B, 0, ctx->stream>>>(
+        ctx->n, ctx->Fi, asm_view(ctx), ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p,
+        ctx->neighbour.p, geom_of(ctx), mu, mu_f, bc_kind, phi, ctx->slot_pn.p, ctx->slot_np.p,
+        ctx->ia.p, ctx->diag.p, ctx->a.p, su);
+  else
+    k_laplacian<<<xcd_grid(ctx->n), BLK, 0, ctx->stream>>>(
+        ctx->n, ctx->Fi, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p,
+        geom_of(ctx), mu, mu_f, bc_kind, phi, ctx->slot_pn.p, ctx->slot_np.p, ctx->diag.p,
+        ctx->a.p, su);
   if (ctx->nper > 0)
     k_zero_periodic<<<grid_for(ctx->nper, BLK), BLK, 0, ctx->stream>>>(
         ctx->Fi, ctx->nper, ctx->slot_pn.p, ctx->slot_np.p, ctx->a.p);
@@ -1195,6 +1481,14 @@ int fcp_launch_laplacian(fcp_ctx *ctx, const double *mu, const double *mu_f,
 }
 
 int fcp_launch_nonorth(fcp_ctx *ctx, const double *gamma_f, const double *grad, double *su) {
+  if (asm_on(ctx)) {
+    ProfScope ps(ctx, "nonorth");
+    k_nonorth_blk<<<asm_grid(ctx->asm_nb), ASM_B, 0, ctx->stream>>>(
+        ctx->n, ctx->Fi, asm_view(ctx), ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p,
+        ctx->neighbour.p, geom_of(ctx), gamma_f, grad, su);
+    FCP_HIP(ctx, hipGetLastError());
+    return 0;
+  }
   ProfScope ps(ctx, "nonorth");
   k_nonorth<<<xcd_grid(ctx->n), BLK, 0, ctx->stream>>>(ctx->n, ctx->Fi, ctx->cf_ptr.p,
                                                               ctx->cf_face.p, ctx->owner.p,
@@ -1205,8 +1499,18 @@ int fcp_launch_nonorth(fcp_ctx *ctx, const double *gamma_f, const double *grad, 
 }
 
 int fcp_launch_grad_gauss(fcp_ctx *ctx, const double *u, double *dudxi) {
-  ProfScope ps(ctx, "grad_gauss");
   const int ntot = ctx->n + ctx->Fb;
+  if (asm_on(ctx)) {
+    ProfScope ps(ctx, "grad_gauss");
+    k_grad_blk<<<asm_grid(ctx->asm_nb), ASM_B, 0, ctx->stream>>>(
+        ctx->n, ctx->Fi, asm_view(ctx), ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p,
+        ctx->neighbour.p, geom_of(ctx), u, dudxi);
+    if (ctx->Fb > 0)
+      k_zero_rows3<<<grid_for(ctx->Fb, BLK), BLK, 0, ctx->stream>>>(ctx->n, ntot, dudxi);
+    FCP_HIP(ctx, hipGetLastError());
+    return 0;
+  }
+  ProfScope ps(ctx, "grad_gauss");
   k_grad_gauss<<<xcd_grid(ntot), BLK, 0, ctx->stream>>>(
       ctx->n, ctx->Fi, ntot, ctx->cf_ptr.p, ctx->cf_face.p, ctx->owner.p, ctx->neighbour.p,
       geom_of(ctx), u, dudxi);

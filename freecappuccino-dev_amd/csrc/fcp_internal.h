@@ -298,6 +298,17 @@ struct fcp_ctx {
   DBuf<double> pc_fac;    // [1] adjustMassFlow's outlet scaling factor
   DBuf<double> pc_s;      // [3n] su, sv, sw of fcp_pressure_correct when not asked for
   DBuf<int> mom_wall;     // [Fb] wall index of every boundary face (-1: not a wall)
+  // cell blocks of the per-cell assembly kernels (256 cells each): the inner
+  // faces a block's cells own ([asm_fo[b], asm_fo[b+1]), inner faces sorted
+  // by owner) and the faces reaching it from other blocks (asm_inc), staged in
+  // LDS; per cell-face incidence the face's LDS slot (bit 30: the cell is the
+  // face's neighbour).  asm_state 0 not built for this CSR, 1 ready, -1 not
+  // applicable (the per-cell kernels read faces from HBM instead).
+  int asm_state = 0;
+  bool asm_rows = false;             // block rows fit the LDS row buffer
+  int asm_nb = 0;
+  DBuf<int> asm_fo, asm_incp, asm_inc, asm_slot;
+  int tune_asm_blk = 1;              // FCP_TUNE_ASM_BLOCKS
   DBuf<double> mom_grad, mom_s3, mom_dp, mom_w3[5];  // momentum assembly scratch
   int lsq_method = 0, lsq_compat = 1;
 

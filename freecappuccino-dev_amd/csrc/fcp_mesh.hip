@@ -603,6 +603,7 @@ int fcp_build_csr(fcp_ctx *ctx) {
   FCP_TRY(fcp_build_schedules(ctx));
   FCP_TRY(fcp_build_sell(ctx));
   ctx->have_csr = true;
+  ctx->asm_state = 0;
   return 0;
 }
 

@@ -41,6 +41,7 @@ FCP_TUNE_INNER_COARSE = 9
 FCP_TUNE_SPMV = 10
 FCP_TUNE_TILE_PACK = 11
 FCP_TUNE_COARSE_FUSE = 12
+FCP_TUNE_ASM_BLOCKS = 13
 
 
 class FcpError(RuntimeError):

@@ -606,6 +606,16 @@ int fcp_dpcg_noalloc(int n, int nnz, const int *ia, const int *ja, const double 
                                     a separate kernel.  The two differ only
                                     in the summation order of one dot
                                     product                                */
+#define FCP_TUNE_ASM_BLOCKS 13  /* face Laplacian, Gauss gradient and
+                                    non-orthogonal correction: 1 (default)
+                                    blocks of 256 cells compute each face's
+                                    term once into LDS (the faces their
+                                    cells own, and those reaching them from
+                                    other blocks) and write their rows of a
+                                    from LDS, where the inner faces are
+                                    sorted by owner and a block's faces fit;
+                                    0 every cell reads its faces from HBM.
+                                    Same results bit for bit               */
 /* Sets a launch-shape parameter of the context; results do not depend on it
  * (FCP_TUNE_INNER_PRECOND and FCP_TUNE_INNER_COARSE change the path of a
  * mixed-precision solve, not its stopping test). */

@@ -773,6 +773,56 @@ def test_tile_pack_layout_bitequal(api, tile):
     ctx.close()
 
 
+@pytest.mark.parametrize("which", ["hex", "tets", "periodic"])
+def test_assembly_blocks_bitequal(api, golden_dir, which):
+    """The block forms of the face Laplacian (face and cell diffusivity), the
+    Gauss gradient and the non-orthogonal correction (FCP_TUNE_ASM_BLOCKS 1,
+    the default: each block's faces computed once into LDS, rows written from
+    LDS) give the per-cell kernels' results bit for bit, and the oracle's."""
+    import sys
+    from fcp.mesh import tet_mesh
+    print(f"[case] assembly blocks {which}", flush=True)
+    if which == "hex":
+        mesh = workloads.poisson_jittered_mesh(40)
+    elif which == "tets":
+        mesh = tet_mesh(8, jitter=0.15, seed=3)
+    else:
+        if golden_dir not in sys.path:
+            sys.path.insert(0, golden_dir)
+        import make_golden as mg
+        mesh = mg.periodic_mesh()
+    ctx, geo, csr = _setup(api, mesh)
+    o = Oracle(mesh)
+    n, nt, F = mesh.n_cells, mesh.n_total, mesh.n_faces
+    rng = np.random.default_rng(5)
+    mu_f = 10.0 ** rng.uniform(-1, 1, F)
+    mu = 10.0 ** rng.uniform(-1, 1, n)
+    phi = rng.uniform(-1, 1, nt)
+    bk = (rng.uniform(0, 1, mesh.n_bnd) < 0.5).astype(np.uint8)
+    su0 = rng.uniform(-1, 1, n)
+    out = {}
+    for blk in (1, 0):
+        ctx.tune(api.FCP_TUNE_ASM_BLOCKS, blk)
+        r = {}
+        a = np.zeros(csr["ja"].size)
+        r["su"] = su0.copy()
+        ctx.laplacian_face(mu_f, bk, phi, r["su"], a_out=a)
+        r["a"] = a.copy()
+        r["grad"] = ctx.grad_gauss(phi)
+        ctx.nonorth_correction(mu_f, r["grad"], r["su"])
+        a2 = np.zeros(csr["ja"].size)
+        r["su2"] = su0.copy()
+        ctx.laplacian(mu, phi, r["su2"], a_out=a2)
+        r["a2"] = a2
+        out[blk] = r
+    for k in out[0]:
+        assert np.array_equal(out[1][k], out[0][k]), k
+    a_o, su_o = o.laplacian_face(mu_f, bk, phi, su0)
+    assert np.array_equal(out[1]["a"], a_o)
+    assert np.array_equal(out[1]["grad"], o.grad_gauss(phi))
+    ctx.close()
+
+
 @pytest.mark.parametrize("prec", [1, 2])
 def test_coarse_fused_matches_separate_kernel(api, prec):
     """The coarse correction applied inside the tile kernel after its grid
