@@ -1289,7 +1289,7 @@ __global__ __launch_bounds__(BLK) void tb_bet(int n, float *__restrict__ x, floa
 // (deterministic results).
 // ---------------------------------------------------------------------------
 constexpr int TCB = 64;         // block size of the coarse factorisation
-constexpr int TC_MAXN = 2048;   // most tiles with a coarse correction
+constexpr int TC_MAXN = FCP_COARSE_MAXN;  // most tiles with a coarse correction
 constexpr int TC_MAXDEG = 32;   // widest tile adjacency (the tile itself included)
 
 __global__ void tc_tile_of_pos(int n, int nb, const int *__restrict__ bpos,

@@ -81,6 +81,7 @@ struct BlockSchedule {
   uint64_t pver64 = ~0ull, pver32 = ~0ull;
 };
 #define FCP_PMAXD 4  // dependencies per row handled by the pipelined sweep
+#define FCP_COARSE_MAXN 2048  // most tiles that get the inner PCG's coarse correction
 
 // Row-block schedules of a batched (several right-hand sides) sweep, whose
 // block size depends on the bytes per row kept in LDS.

@@ -950,15 +950,22 @@ int fcp_build_tile_schedules(fcp_ctx *ctx) {
 }
 
 // Blocks of the tile-local preconditioner: geometric tiles of about 4096
-// cells when the mesh has geometry (any numbering: blocks never wait on one
-// another), else runs of 4096 contiguous rows.
+// cells (more above 2048 such tiles) when the mesh has geometry (any
+// numbering: blocks never wait on one another), else runs of contiguous rows
+// of the same size.
 int fcp_build_bj_schedules(fcp_ctx *ctx) {
   FCP_TRY(fcp_coarse_wait(ctx));  // a pending coarse factorisation reads the schedules
   ctx->have_bj = false;
   ctx->tin_have = false;
   ctx->tc_have = false;
   const int n = ctx->n;
-  const int target = ctx->tune_inner_tile >= 64 ? ctx->tune_inner_tile : 4096;
+  int target = ctx->tune_inner_tile >= 64 ? ctx->tune_inner_tile : 4096;
+  // by default, tiles large enough that their count -- rounded to the tile
+  // grid, so with a factor 2 of margin -- stays within the coarse
+  // correction's dense limit (192^3 gets 8192-cell tiles, 256^3 16384)
+  if (ctx->tune_inner_tile < 64)
+    while ((long long)n > (long long)target * (FCP_COARSE_MAXN / 2) && target < 65536)
+      target *= 2;
   std::vector<int> rank;
   int m = 0;
   if (ctx->have_geom && ctx->tune_sweep_tile != 1 && n >= 4 * target) {
