@@ -585,7 +585,8 @@ int fcp_dpcg_noalloc(int n, int nnz, const int *ia, const int *ja, const double 
                                     times on large meshes; 0: tile-local only */
 #define FCP_TUNE_SPMV 10         /* variant of the standalone FP64 SELL SpMV
                                     (fcp_spmv): -1 (default) automatic: 1 when
-                                    the operator exceeds 128 MB, else 0; 0 one
+                                    the operator and two vectors exceed the
+                                    256 MB Infinity Cache, else 0; 0 one
                                     row per thread, 1 the same with
                                     non-temporal matrix loads, 2 two rows per
                                     thread with non-temporal matrix loads, 3
