@@ -79,7 +79,7 @@ def alg_bytes(name, n, nnz, Fi, Fb, local=None):
         "tin_coarse": 2 * n * 4,
         "bj_sweep_fwd_f32": local * 8 + 4 * n + 3 * n * 4 if local is not None else None,
         "bj_sweep_bwd_f32": local * 8 + 4 * n + 3 * n * 4 if local is not None else None,
-        "residual_f64": nnz * 12 + (n + 1) * 4 + 3 * n * 8,
+        "residual_f64": nnz * 12 + 3 * n * 8,
         "dot_rz": 2 * n * 8,
         "pupdate": 3 * n * 8,
         "xr_update": 6 * n * 8,

@@ -219,6 +219,11 @@ struct RawVec<16> {
 // ---------------------------------------------------------------------------
 // SELL-64 copy of the operator
 // ---------------------------------------------------------------------------
+// Slice s holds rows 64s..64s+63; entry k of its row r sits at
+// base[s] + 64k + (r & 63), for k below the slice width (base[s+1]-base[s])/64.
+// A row's entries fill k = 0.. in CSR order; the slots after them hold the
+// column -1 and the value 0, so a product finds the row's end from the
+// columns it loads anyway.
 template <typename T>
 struct Sell {
   const int *base, *col, *ia;
@@ -249,25 +254,28 @@ __device__ __forceinline__ T sell_row(const Sell<T> &m, int i, const T *__restri
   const int b0 = m.base[sl];
   const int p = b0 + (i & 63);
   const int L = (m.base[sl + 1] - b0) >> 6;  // slice width: k < L exists for all its rows
-  const int len = m.ia[i + 1] - m.ia[i];
   T av[SELL_KU], xv[SELL_KU];
   int cv[SELL_KU];
 #pragma unroll
-  for (int k = 0; k < SELL_KU; ++k)
+  for (int k = 0; k < SELL_KU; ++k) {
+    cv[k] = -1;
     if (k < L) {
       av[k] = m.a[p + 64 * k];
       cv[k] = m.col[p + 64 * k];
     }
+  }
 #pragma unroll
   for (int k = 0; k < SELL_KU; ++k)
-    if (k < len) xv[k] = x[cv[k]];
+    if (cv[k] >= 0) xv[k] = x[cv[k]];
   T acc = init;
 #pragma unroll
   for (int k = 0; k < SELL_KU; ++k)
-    if (k < len) acc = subtract ? acc - av[k] * xv[k] : acc + av[k] * xv[k];
-  for (int k = SELL_KU; k < len; ++k)
-    acc = subtract ? acc - m.a[p + 64 * k] * x[m.col[p + 64 * k]]
-                   : acc + m.a[p + 64 * k] * x[m.col[p + 64 * k]];
+    if (cv[k] >= 0) acc = subtract ? acc - av[k] * xv[k] : acc + av[k] * xv[k];
+  for (int k = SELL_KU; k < L; ++k) {
+    const int j = m.col[p + 64 * k];
+    if (j < 0) break;
+    acc = subtract ? acc - m.a[p + 64 * k] * x[j] : acc + m.a[p + 64 * k] * x[j];
+  }
   return acc;
 }
 

@@ -514,7 +514,7 @@ __global__ void k_sell_col(const int *__restrict__ ia, const int *__restrict__ j
   if (r >= nslices * 64) return;
   const int s = r >> 6, p = base[s] + (r & 63), L = (base[s + 1] - base[s]) >> 6;
   const int len = r < n ? ia[r + 1] - ia[r] : 0;
-  for (int k = 0; k < L; ++k) col[p + 64 * k] = (k < len) ? ja[ia[r] + k] : (r < n ? r : 0);
+  for (int k = 0; k < L; ++k) col[p + 64 * k] = (k < len) ? ja[ia[r] + k] : -1;
 }
 
 template <typename T>
@@ -560,7 +560,7 @@ __global__ __launch_bounds__(BLK) void k_spmv_rows(int n, Sell<T> m, const T *__
                                                    T *__restrict__ y) {
   const int b = (int)blockIdx.x, per = (int)gridDim.x >> 3;
   const int blk = (b & 7) * per + (b >> 3);
-  int row[R], p[R], len[R], L[R];
+  int row[R], p[R], L[R];
   T av[R][SELL_KU], xv[R][SELL_KU];
   int cv[R][SELL_KU];
 #pragma unroll
@@ -570,29 +570,33 @@ __global__ __launch_bounds__(BLK) void k_spmv_rows(int n, Sell<T> m, const T *__
     const int sl = i >> 6, b0 = m.base[sl];
     p[r] = b0 + (i & 63);
     L[r] = (m.base[sl + 1] - b0) >> 6;
-    len[r] = sell_ld<int, NTY>(m.ia + i + 1) - sell_ld<int, NTY>(m.ia + i);
   }
 #pragma unroll
   for (int r = 0; r < R; ++r)
 #pragma unroll
-    for (int k = 0; k < SELL_KU; ++k)
+    for (int k = 0; k < SELL_KU; ++k) {
+      cv[r][k] = -1;
       if (k < L[r]) {
         av[r][k] = sell_ld<T, NT>(m.a + p[r] + 64 * k);
         cv[r][k] = sell_ld<int, NT>(m.col + p[r] + 64 * k);
       }
+    }
 #pragma unroll
   for (int r = 0; r < R; ++r)
 #pragma unroll
     for (int k = 0; k < SELL_KU; ++k)
-      if (k < len[r]) xv[r][k] = x[cv[r][k]];
+      if (cv[r][k] >= 0) xv[r][k] = x[cv[r][k]];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     T acc = (T)0;
 #pragma unroll
     for (int k = 0; k < SELL_KU; ++k)
-      if (k < len[r]) acc = acc + av[r][k] * xv[r][k];
-    for (int k = SELL_KU; k < len[r]; ++k)
-      acc = acc + sell_ld<T, NT>(m.a + p[r] + 64 * k) * x[sell_ld<int, NT>(m.col + p[r] + 64 * k)];
+      if (cv[r][k] >= 0) acc = acc + av[r][k] * xv[r][k];
+    for (int k = SELL_KU; k < L[r]; ++k) {
+      const int j = sell_ld<int, NT>(m.col + p[r] + 64 * k);
+      if (j < 0) break;
+      acc = acc + sell_ld<T, NT>(m.a + p[r] + 64 * k) * x[j];
+    }
     if (row[r] < n) {
       if constexpr (NTY)
         __builtin_nontemporal_store(acc, y + row[r]);
@@ -1079,13 +1083,13 @@ int fcp_launch_spmv64(fcp_ctx *ctx, const double *, const double *x, double *y) 
   FCP_TRY(sell_view<double>(ctx, &m));
   ProfScope ps(ctx, "spmv_sell_f64");
   const int n = ctx->n;
-  // automatic: non-temporal matrix loads once the operator (values and
-  // columns) and the two vectors exceed the 256 MB Infinity Cache, so that
-  // streaming it does not evict the gathered x lines; cached loads while it
-  // all fits (128^3: 37.7 against 39.2 us per SpMV, 192^3: 131.6 against
-  // 157.6 us)
+  // automatic: non-temporal matrix loads and y stores once the operator
+  // (values and columns) and the two vectors exceed the 256 MB Infinity
+  // Cache, so that streaming them does not evict the gathered x lines; cached
+  // loads while it all fits (128^3: 31.8 against 38.6 us per SpMV, 192^3:
+  // 126.1 against 142.5 us)
   int kind = ctx->tune_spmv;
-  if (kind < 0) kind = (size_t)ctx->nnz * 12 + (size_t)n * 16 > ((size_t)256 << 20) ? 1 : 0;
+  if (kind < 0) kind = (size_t)ctx->nnz * 12 + (size_t)n * 16 > ((size_t)256 << 20) ? 4 : 0;
   switch (kind) {
     case 4:
       k_spmv_rows<double, true, 1, true><<<(grid_for(n, BLK) + 7) / 8 * 8, BLK, 0, ctx->stream>>>(n, m, x, y);

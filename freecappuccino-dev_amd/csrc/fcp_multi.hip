@@ -54,21 +54,22 @@ __device__ __forceinline__ void msell_row(const Sell<T> &m, int i, const T *__re
   const int b0 = m.base[sl];
   const int p = b0 + (i & 63);
   const int L = (m.base[sl + 1] - b0) >> 6;
-  const int len = m.ia[i + 1] - m.ia[i];
   T av[SELL_KU];
   int cv[SELL_KU];
 #pragma unroll
-  for (int k = 0; k < SELL_KU; ++k)
+  for (int k = 0; k < SELL_KU; ++k) {
+    cv[k] = -1;  // past the row's end (padding slots hold -1 too)
     if (k < L) {
       av[k] = m.a[p + 64 * k];
       cv[k] = m.col[p + 64 * k];
     }
+  }
   // per-column diagonal: column r's own value at the diagonal entry, in the
   // entry's place of the sum
   const T *dc = m.dcol ? m.dcol + (size_t)i * NRP : nullptr;
 #pragma unroll
   for (int k = 0; k < SELL_KU; ++k)
-    if (k < len) {
+    if (cv[k] >= 0) {
       const T *xc = x + (size_t)cv[k] * NRP;
       const bool dg = dc && cv[k] == i;
 #pragma unroll
@@ -77,9 +78,10 @@ __device__ __forceinline__ void msell_row(const Sell<T> &m, int i, const T *__re
         acc[r] = subtract ? acc[r] - ak * xc[r] : acc[r] + ak * xc[r];
       }
     }
-  for (int k = SELL_KU; k < len; ++k) {
-    const T a = m.a[p + 64 * k];
+  for (int k = SELL_KU; k < L; ++k) {
     const int j = m.col[p + 64 * k];
+    if (j < 0) break;
+    const T a = m.a[p + 64 * k];
     const T *xc = x + (size_t)j * NRP;
     const bool dg = dc && j == i;
 #pragma unroll

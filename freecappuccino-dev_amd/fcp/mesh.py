@@ -401,6 +401,70 @@ def permute_cells(mesh: Mesh, new_of_old: np.ndarray) -> Mesh:
     return m
 
 
+def merge_cells(mesh: Mesh, group) -> Mesh:
+    """Polyhedral mesh whose cell c is the union of the cells i of ``mesh``
+    with ``group[i] == c`` (no reference counterpart: a generator of test
+    meshes with more faces per cell than the synthetic hexes and tets have).
+
+    Faces between cells of one group are removed; every other face is kept
+    with its vertices.  Inner faces are re-oriented so that the owner is the
+    lower new cell (vertex order reversed where the pair swaps) and re-sorted
+    by owner, then neighbour, as the reference's meshes are; boundary faces
+    keep their patches and order.  Two cells must share at most one face
+    (the reference's CSR keeps one entry per face, ``sparse_matrix.f90:108-
+    175``), which is checked.  Periodic patches are not supported.
+    """
+    assert not mesh.periodic_patches
+    group = np.asarray(group, dtype=np.int64)
+    assert group.shape == (mesh.n_cells,)
+    nc = int(group.max()) + 1
+    assert np.array_equal(np.unique(group), np.arange(nc))
+    ni = mesh.n_inner
+    o = group[mesh.owner[:ni]]
+    nb = group[mesh.neighbour]
+    keep = np.nonzero(o != nb)[0]
+    fo, fn = np.minimum(o, nb)[keep], np.maximum(o, nb)[keep]
+    order = np.lexsort((fn, fo))
+    pair = fo[order] * nc + fn[order]
+    assert np.all(np.diff(pair) > 0), "two cells share more than one face"
+    inner = keep[order]
+    node_in = mesh.node[inner].copy()
+    nn_in = mesh.nnodes[inner]
+    for r in np.nonzero(o[inner] > nb[inner])[0]:
+        m = int(nn_in[r])
+        node_in[r, :m] = node_in[r, :m][::-1]
+    nf_new = len(inner) + mesh.n_bnd
+    owner = np.empty(nf_new, dtype=np.int32)
+    owner[:len(inner)] = fo[order]
+    owner[len(inner):] = group[mesh.owner[ni:]]
+    out = Mesh(points=mesh.points.copy(),
+               nnodes=np.concatenate([nn_in, mesh.nnodes[ni:]]).astype(np.int32),
+               node=np.concatenate([node_in, mesh.node[ni:]]).astype(np.int32),
+               owner=owner, neighbour=fn[order].astype(np.int32), n_cells=nc,
+               names=list(mesh.names), types=list(mesh.types), nfaces=mesh.nfaces.copy(),
+               start_face=(np.asarray(mesh.start_face) - (ni - len(inner))).astype(np.int32),
+               twin_start=[])
+    out.check()
+    return out
+
+
+def staggered_pairs(N: int) -> np.ndarray:
+    """Cell groups for ``merge_cells`` on ``hex_mesh(N)``: along x, cells
+    i, i+1 merge into one polyhedron for i = s, s+2, ... with s = (j+k) mod 2,
+    so a merged cell's y and z neighbours are two different cells on each
+    side (ten faces, ten neighbours, rows of eleven CSR entries); cells left
+    over at the ends of a row stay single."""
+    i, j, k = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    i, j, k = (a.transpose(2, 1, 0).ravel() for a in (i, j, k))  # cell i + N j + N^2 k
+    s = (j + k) % 2
+    lead = (i - s) % 2 == 0
+    paired = (i >= s) & np.where(lead, i + 1 < N, i - 1 >= s)
+    first = np.where(paired & ~lead, i - 1, i)  # the x-lower cell of the group
+    key = first + N * j + N * N * k
+    _, group = np.unique(key, return_inverse=True)
+    return group.astype(np.int64)
+
+
 def fisher_yates(n: int, seed: int, stream: int = 200) -> np.ndarray:
     """Permutation of range(n) by Fisher-Yates with SplitMix64 draws:
     for i = n-1 .. 1, j = floor(u_i (i+1)), swap(a[i], a[j])."""

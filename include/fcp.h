@@ -584,15 +584,16 @@ int fcp_dpcg_noalloc(int n, int nnz, const int *ia, const int *ja, const double 
                                     which cuts the inner iterations several
                                     times on large meshes; 0: tile-local only */
 #define FCP_TUNE_SPMV 10         /* variant of the standalone FP64 SELL SpMV
-                                    (fcp_spmv): -1 (default) automatic: 1 when
+                                    (fcp_spmv): -1 (default) automatic: 4 when
                                     the operator and two vectors exceed the
                                     256 MB Infinity Cache, else 0; 0 one
                                     row per thread, 1 the same with
                                     non-temporal matrix loads, 2 two rows per
                                     thread with non-temporal matrix loads, 3
                                     two rows per thread, 4 as 1 with
-                                    non-temporal row-length loads and y
-                                    stores                                    */
+                                    non-temporal y stores; each row ends at
+                                    its first padding slot (column -1), so
+                                    no variant reads the row pointers      */
 #define FCP_TUNE_TILE_PACK 11    /* tile-order sweeps: 1 (default) the packed
                                     operand layout (three 10-bit dependency
                                     distances in each coefficient vector's
