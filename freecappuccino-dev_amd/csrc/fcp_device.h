@@ -223,7 +223,9 @@ struct RawVec<16> {
 // base[s] + 64k + (r & 63), for k below the slice width (base[s+1]-base[s])/64.
 // A row's entries fill k = 0.. in CSR order; the slots after them hold the
 // column -1 and the value 0, so a product finds the row's end from the
-// columns it loads anyway.
+// columns it loads anyway.  Where padding every slice to the widest costs
+// little, all slices have one width W (base[s] = 64 W s) and the products
+// compute a row's place instead of loading it.
 template <typename T>
 struct Sell {
   const int *base, *col, *ia;
@@ -231,7 +233,16 @@ struct Sell {
   // batched products with a diagonal of their own per column: [n*NRP]
   // interleaved, used in place of the diagonal entry (null: the matrix's)
   const T *dcol = nullptr;
+  int width = 0;  // the uniform slice width, 0: widths differ (read base)
 };
+
+// Entry 0 of row i of a SELL-64 array (.x) and its slice width (.y).
+__device__ __forceinline__ int2 sell_at(const int *__restrict__ base, int W, int i) {
+  const int sl = i >> 6;
+  if (W > 0) return make_int2(sl * 64 * W + (i & 63), W);
+  const int b0 = base[sl];
+  return make_int2(b0 + (i & 63), (base[sl + 1] - b0) >> 6);
+}
 
 // Entries of a row whose loads are issued together (rows longer than this
 // finish in a plain loop).
@@ -250,10 +261,8 @@ constexpr int GS_U = 4;
 template <typename T>
 __device__ __forceinline__ T sell_row(const Sell<T> &m, int i, const T *__restrict__ x, T init,
                                       bool subtract) {
-  const int sl = i >> 6;
-  const int b0 = m.base[sl];
-  const int p = b0 + (i & 63);
-  const int L = (m.base[sl + 1] - b0) >> 6;  // slice width: k < L exists for all its rows
+  const int2 at = sell_at(m.base, m.width, i);
+  const int p = at.x, L = at.y;  // slice width: k < L exists for all its rows
   T av[SELL_KU], xv[SELL_KU];
   int cv[SELL_KU];
 #pragma unroll

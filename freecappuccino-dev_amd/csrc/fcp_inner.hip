@@ -848,7 +848,7 @@ __global__ __launch_bounds__(BLK) void tk_pre_w(
 // p_new = z + bet p_old (every operand computed where it is used), q = A p_new
 // over all slice entries (padding adds 0 * p_i), pkapk = p.q, alf = sk/pkapk.
 template <int NR, int NRP, class T>
-__global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ base,
+__global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ base, int W,
                                                const int *__restrict__ col,
                                                const T *__restrict__ val,
                                                const T *__restrict__ z,
@@ -870,10 +870,8 @@ __global__ __launch_bounds__(BLK) void tk_spmv(int n, const int *__restrict__ ba
   for (int c = 0; c < NR; ++c) part[c] = 0.0;
   for (XRows X = xcd_rows(n); X.i < X.end; X.i += X.step) {
     const int i = X.i;
-    const int sl = i >> 6;
-    const int b0 = base[sl];
-    const int pp = b0 + (i & 63);
-    const int L = (base[sl + 1] - b0) >> 6;
+    const int2 at = sell_at(base, W, i);
+    const int pp = at.x, L = at.y;
     T av[SELL_KU];
     int cv[SELL_KU];
 #pragma unroll
@@ -1099,7 +1097,7 @@ __global__ __launch_bounds__(BLK) void tb_tile(
 // ukreso = out.w, gam = bet / ukreso (w = reso); MODE 1: vr = out.w,
 // vv = out.out, alf = vr / (vv + small) (w = r).
 template <int NR, int NRP, int MODE>
-__global__ __launch_bounds__(BLK) void tb_spmv(int n, const int *__restrict__ base,
+__global__ __launch_bounds__(BLK) void tb_spmv(int n, const int *__restrict__ base, int W,
                                                const int *__restrict__ col,
                                                const float *__restrict__ val,
                                                const float *__restrict__ z,
@@ -1117,10 +1115,8 @@ __global__ __launch_bounds__(BLK) void tb_spmv(int n, const int *__restrict__ ba
   for (int k = 0; k < NV; ++k) part[k] = 0.0;
   for (XRows X = xcd_rows(n); X.i < X.end; X.i += X.step) {
     const int i = X.i;
-    const int sl = i >> 6;
-    const int b0 = base[sl];
-    const int pp = b0 + (i & 63);
-    const int L = (base[sl + 1] - b0) >> 6;
+    const int2 at = sell_at(base, W, i);
+    const int pp = at.x, L = at.y;
     float av[SELL_KU];
     int cv[SELL_KU];
 #pragma unroll
@@ -1815,6 +1811,7 @@ int tk_prepare(fcp_ctx *ctx, bool fp64 = false) {
     FCP_HIP(ctx, ctx->tin_base.alloc(nsl + 1));
     tk_sell_len<<<grid_for(nsl + 1, BLK), BLK, 0, ctx->stream>>>(n, nsl, b.rows.p, ctx->ia.p,
                                                                  len.p);
+    FCP_TRY(sell_uniform(ctx, len.p, nsl, &ctx->tin_width));
     FCP_TRY(fcp_exclusive_scan_int(ctx, len.p, ctx->tin_base.p, nsl + 1));
     int npad = 0;
     FCP_HIP(ctx, hipMemcpyAsync(&npad, ctx->tin_base.p + nsl, sizeof(int), hipMemcpyDeviceToHost,
@@ -2157,7 +2154,8 @@ int tk_iteration(fcp_ctx *ctx, const TkBufs<T> &B, SolveScalars *si, int k) {
   {
     ProfScope ps(ctx, sizeof(T) == 8 ? "tin_spmv64" : "tin_spmv");
     tk_spmv<NR, NRP, T><<<tk_spmv_grid(ctx), BLK, 0, ctx->stream>>>(
-        ctx->n, ctx->tin_base.p, ctx->tin_col.p, B.val, B.z, pold, pnew, B.q, ctx->partials.p,
+        ctx->n, ctx->tin_base.p, ctx->tin_width, ctx->tin_col.p, B.val, B.z, pold, pnew, B.q,
+        ctx->partials.p,
         redctr(ctx), si);
   }
   return tk_launch_pre<NR, NRP, T>(ctx, B, pnew, si, 1);
@@ -2173,7 +2171,8 @@ int tk_graph(fcp_ctx *ctx, const TkBufs<T> &B, SolveScalars *si, hipGraphExec_t 
       (const void *)(size_t)B.fuse, ctx->tin_bar.p, ctx->errflag.p,
       B.x, B.r, B.z, B.q, B.p[0], B.p[1], si,
       ctx->partials.p, ctx->counters.p, B.dl, B.du, B.al, B.au, B.dv,
-      ctx->tin_base.p, ctx->tin_col.p, B.val, b.blk_pos.p, b.blk_step.p, b.step_ptr.p,
+      ctx->tin_base.p, (const void *)(size_t)ctx->tin_width, ctx->tin_col.p, B.val, b.blk_pos.p,
+      b.blk_step.p, b.step_ptr.p,
       (const void *)(size_t)b.nblocks, (const void *)(size_t)b.R, (const void *)(size_t)b.max_steps,
       (const void *)(size_t)ctx->n, B.r0, (const void *)(size_t)B.ncp, ctx->tc_L.p};
   if (ctx->tin_graph && ctx->tin_graph_key == key) {
@@ -2292,7 +2291,7 @@ int tb_bicgstab(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *
     constexpr int MODE = decltype(mode)::value;
     ProfScope ps(ctx, MODE == 0 ? "tinb_u" : "tinb_v");
     tb_spmv<NR, NRP, MODE><<<tk_spmv_grid(ctx), BLK, 0, ctx->stream>>>(
-        n, ctx->tin_base.p, ctx->tin_col.p, ctx->tin_val.p, Z, MODE == 0 ? Reso : R,
+        n, ctx->tin_base.p, ctx->tin_width, ctx->tin_col.p, ctx->tin_val.p, Z, MODE == 0 ? Reso : R,
         MODE == 0 ? U : Vv, ctx->partials.p, redctr(ctx), si);
   };
   bet(0);

@@ -217,6 +217,7 @@ struct fcp_ctx {
   DBuf<int> tin_dl, tin_du;          // [2n] 4 x 16-bit LDS slots of both sweeps
   DBuf<float> tin_al, tin_au, tin_dv;  // [4n] coefficients, [n] pivots
   DBuf<int> tin_base, tin_col;       // SELL-64 copy of P A P^T
+  int tin_width = 0;                 // its uniform slice width (0: widths differ)
   DBuf<float> tin_val;
   // packed sweep layout: three 10-bit dependency distances per position and
   // sweep in the bits of the coefficients' fourth component (every position
@@ -255,6 +256,7 @@ struct fcp_ctx {
   int nslices = 0;
   DBuf<int> sell_base;  // [nslices+1] offset of each slice (multiple of 64)
   DBuf<int> sell_col;   // [sell_base[nslices]]
+  int sell_width = 0;   // uniform slice width (0: widths differ, read sell_base)
   DBuf<double> sell_a64;
   DBuf<float> sell_a32;
   uint64_t sell64_version = ~0ull, sell32_version = ~0ull;
@@ -506,6 +508,7 @@ int fcp_launch_lsq_matrix(fcp_ctx *ctx, int method, int compat);
 int fcp_launch_grad_lsq(fcp_ctx *ctx, const double *phi, double *dphi);
 // fcp_krylov.hip
 int fcp_build_sell(fcp_ctx *ctx);
+int sell_uniform(fcp_ctx *ctx, int *len, int nslices, int *width);
 int fcp_launch_spmv64(fcp_ctx *ctx, const double *a, const double *x, double *y);
 int fcp_solve_device(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_max,
                      double tol_abs, double tol_rel, int verbose, fcp_solve_opts *opts);

@@ -23,6 +23,7 @@
 //   iterative-refinement loop (true residual, update and norms in FP64).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -567,9 +568,9 @@ __global__ __launch_bounds__(BLK) void k_spmv_rows(int n, Sell<T> m, const T *__
   for (int r = 0; r < R; ++r) {
     row[r] = (blk * R + r) * BLK + (int)threadIdx.x;
     const int i = row[r] < n ? row[r] : n - 1;
-    const int sl = i >> 6, b0 = m.base[sl];
-    p[r] = b0 + (i & 63);
-    L[r] = (m.base[sl + 1] - b0) >> 6;
+    const int2 at = sell_at(m.base, m.width, i);
+    p[r] = at.x;
+    L[r] = at.y;
   }
 #pragma unroll
   for (int r = 0; r < R; ++r)
@@ -1013,6 +1014,32 @@ int fcp_ensure_workspace(fcp_ctx *ctx) {
   return 0;
 }
 
+// Slice lengths (64 x width) of a SELL-64 copy: where padding every slice to
+// the widest adds at most 2% slots, every length becomes 64 W and *width = W
+// (the products then compute a row's place); else *width = 0.
+int sell_uniform(fcp_ctx *ctx, int *len, int nslices, int *width) {
+  *width = 0;
+  if (nslices <= 0) return 0;
+  std::vector<int> h(nslices);
+  FCP_HIP(ctx, hipMemcpyAsync(h.data(), len, sizeof(int) * nslices, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  long long tot = 0;
+  int wmax = 0;
+  for (int v : h) {
+    tot += v;
+    wmax = v > wmax ? v : wmax;
+  }
+  const long long uni = (long long)wmax * nslices;
+  if (wmax == 0 || uni * 50 > tot * 51 || uni > (1LL << 30)) return 0;
+  std::fill(h.begin(), h.end(), wmax);
+  FCP_HIP(ctx, hipMemcpyAsync(len, h.data(), sizeof(int) * nslices, hipMemcpyHostToDevice,
+                              ctx->stream));
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *width = wmax / 64;
+  return 0;
+}
+
 int fcp_build_sell(fcp_ctx *ctx) {
   const int n = ctx->n;
   ctx->nslices = (n + 63) / 64;
@@ -1021,6 +1048,7 @@ int fcp_build_sell(fcp_ctx *ctx) {
   FCP_HIP(ctx, ctx->sell_base.alloc(ctx->nslices + 1));
   k_sell_len<<<grid_for(ctx->nslices + 1, BLK), BLK, 0, ctx->stream>>>(ctx->ia.p, n, ctx->nslices,
                                                                        len.p);
+  FCP_TRY(sell_uniform(ctx, len.p, ctx->nslices, &ctx->sell_width));
   FCP_TRY(fcp_exclusive_scan_int(ctx, len.p, ctx->sell_base.p, ctx->nslices + 1));
   int npad = 0;
   FCP_HIP(ctx, hipMemcpyAsync(&npad, ctx->sell_base.p + ctx->nslices, sizeof(int),
@@ -1049,6 +1077,7 @@ int sell_view<double>(fcp_ctx *ctx, Sell<double> *out) {
     ctx->sell64_version = ctx->a_version;
   }
   *out = Sell<double>{ctx->sell_base.p, ctx->sell_col.p, ctx->ia.p, ctx->sell_a64.p};
+  out->width = ctx->sell_width;
   return 0;
 }
 template <>
@@ -1062,6 +1091,7 @@ int sell_view<float>(fcp_ctx *ctx, Sell<float> *out) {
     ctx->sell32_version = ctx->a_version;
   }
   *out = Sell<float>{ctx->sell_base.p, ctx->sell_col.p, ctx->ia.p, ctx->sell_a32.p};
+  out->width = ctx->sell_width;
   return 0;
 }
 }  // namespace

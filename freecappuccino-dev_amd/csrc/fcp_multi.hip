@@ -50,10 +50,8 @@ __device__ __forceinline__ void done_mask(const SolveScalars *s, bool (&d)[NR]) 
 template <typename T, int NR, int NRP>
 __device__ __forceinline__ void msell_row(const Sell<T> &m, int i, const T *__restrict__ x,
                                           T (&acc)[NR], bool subtract) {
-  const int sl = i >> 6;
-  const int b0 = m.base[sl];
-  const int p = b0 + (i & 63);
-  const int L = (m.base[sl + 1] - b0) >> 6;
+  const int2 at = sell_at(m.base, m.width, i);
+  const int p = at.x, L = at.y;
   T av[SELL_KU];
   int cv[SELL_KU];
 #pragma unroll
