@@ -362,6 +362,11 @@ void fcp_finalize(fcp_ctx *ctx) {
   }
   for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
   if (ctx->h_scal) (void)hipHostFree(ctx->h_scal);
+  if (ctx->h_err) (void)hipHostFree(ctx->h_err);
+  if (ctx->h_poll) (void)hipHostFree(ctx->h_poll);
+  if (ctx->h_perr) (void)hipHostFree(ctx->h_perr);
+  for (auto e : ctx->poll_ev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->tin_graph) (void)hipGraphExecDestroy(ctx->tin_graph);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -297,13 +297,18 @@ int sell_view32(fcp_ctx *ctx, Sell<float> *out);
 inline int *tickets(fcp_ctx *ctx) { return ctx->counters.p; }
 inline int *redctr(fcp_ctx *ctx) { return ctx->counters.p + 32; }
 int check_err(fcp_ctx *ctx);
+// Copies bytes of device scalars d to host memory h and the sweep error flag
+// to its pinned mirror, waits once for both, and fails on a recorded error.
+int poll_device(fcp_ctx *ctx, void *h, const void *d, size_t bytes);
 int prepare_precond(fcp_ctx *ctx, int kind, bool fp32, bool pivots = true);
 bool sched_fits(fcp_ctx *ctx, const BlockSchedule &lo, const BlockSchedule &up, size_t row_bytes);
 int pack_block64(fcp_ctx *ctx, BlockSchedule &b);
 int pack_block32(fcp_ctx *ctx, BlockSchedule &b);
-// Pivots and packed FP32 operands of the tile-local preconditioner (schedules
+// Pivots of the tile-local preconditioner (schedules
 // jlower/jupper, pivots dbj); *ok false when it does not fit this mesh.
 int bj_begin(fcp_ctx *ctx, int kind, bool *ok);
+// Packs the row-gather tile sweeps' FP32 operands if the tile pivots changed.
+int bj_pack32(fcp_ctx *ctx);
 // Inner PCG in tile order (fcp_inner.hip): FP32, or FP64 with fp64 set
 // (values per position of sizeof(double) in LDS).
 bool tin_usable(fcp_ctx *ctx, int nrp, bool fp64 = false);

@@ -538,9 +538,7 @@ int fcp_gmres_device(fcp_ctx *ctx, double *x, const double *b, int itr_max, int 
   GmState h;
   for (int cycles = 0;;) {
     for (int c = 0; c < per_poll && cycles < max_cycles; ++c, ++cycles) FCP_TRY(cycle());
-    FCP_HIP(ctx, hipMemcpyAsync(&h, st, sizeof(GmState), hipMemcpyDeviceToHost, ctx->stream));
-    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    FCP_TRY(check_err(ctx));
+    FCP_TRY(poll_device(ctx, &h, st, sizeof(GmState)));
     if (verbose >= 2)
       printf("  ITR = %4d  Residual = %14.6G  K = %6d  Residual = %14.6G\n", h.itr, h.rho,
              h.itr_used, h.resl);

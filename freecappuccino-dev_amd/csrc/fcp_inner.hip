@@ -279,8 +279,9 @@ __global__ void tk_row_src(int n, const int *__restrict__ perm, const int *__res
   const int row = perm[p];
   rs[p] = make_int2(ia[row], ia[row + 1] - ia[row]);
 }
-// Values of the SELL-64 copy in precision T: thread (position, blockIdx.y)
-// fills entries k = blockIdx.y, blockIdx.y + gridDim.y, ... of its row.
+// Values of the SELL-64 copy in precision T, one thread per position: the
+// row's CSR entries (contiguous, so each line of a is fetched once) are all
+// loaded before the coalesced stores of the slice's columns.
 template <class T>
 __global__ void tk_sell_val(int n, int nslices, const int2 *__restrict__ rs,
                             const double *__restrict__ a, const int *__restrict__ base,
@@ -289,8 +290,13 @@ __global__ void tk_sell_val(int n, int nslices, const int2 *__restrict__ rs,
   if (p >= nslices * 64) return;
   const int s = p >> 6, q = base[s] + (p & 63), L = (base[s + 1] - base[s]) >> 6;
   const int2 r = p < n ? rs[p] : make_int2(0, 0);
-  for (int k = blockIdx.y; k < L; k += gridDim.y)
-    val[q + 64 * k] = k < r.y ? (T)a[r.x + k] : (T)0;
+  T v[SELL_KU];
+#pragma unroll
+  for (int k = 0; k < SELL_KU; ++k) v[k] = k < r.y ? (T)a[r.x + k] : (T)0;
+#pragma unroll
+  for (int k = 0; k < SELL_KU; ++k)
+    if (k < L) val[q + 64 * k] = v[k];
+  for (int k = SELL_KU; k < L; ++k) val[q + 64 * k] = k < r.y ? (T)a[r.x + k] : (T)0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1860,7 +1866,7 @@ int tk_prepare(fcp_ctx *ctx, bool fp64 = false) {
         ctx->tin_pack ? ctx->tin_offl.p : nullptr, ctx->tin_pack ? ctx->tin_offu.p : nullptr,
         reinterpret_cast<V4<T> *>(al), reinterpret_cast<V4<T> *>(au), dv);
     const int nsl = (n + 63) / 64;
-    tk_sell_val<T><<<dim3(grid_for((size_t)nsl * 64, BLK), SELL_KU), BLK, 0, ctx->stream>>>(
+    tk_sell_val<T><<<grid_for((size_t)nsl * 64, BLK), BLK, 0, ctx->stream>>>(
         n, nsl, reinterpret_cast<const int2 *>(ctx->tin_rs.p), ctx->a.p, ctx->tin_base.p, val);
     FCP_HIP(ctx, hipGetLastError());
     epoch = ctx->bj_epoch;
@@ -2198,6 +2204,15 @@ int tk_graph(fcp_ctx *ctx, const TkBufs<T> &B, SolveScalars *si, hipGraphExec_t 
   return 0;
 }
 
+// The pinned slots and events of the pipelined polls (once per context).
+int poll_slots(fcp_ctx *ctx) {
+  if (!ctx->h_poll) FCP_HIP(ctx, hipHostMalloc((void **)&ctx->h_poll, 8 * sizeof(SolveScalars)));
+  if (!ctx->h_perr) FCP_HIP(ctx, hipHostMalloc((void **)&ctx->h_perr, 2 * sizeof(int)));
+  for (auto &e : ctx->poll_ev)
+    if (!e) FCP_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return 0;
+}
+
 template <int NR, int NRP, class T>
 int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, SolveScalars *si,
            double tol_rel, double inner_tol, int budget, int check_every, int verbose,
@@ -2217,31 +2232,77 @@ int tk_pcg(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *so, S
       tol_rel, inner_tol, budget, (T *)nullptr, (T *)nullptr);
   FCP_TRY((tk_launch_pre<NR, NRP, T>(ctx, B, B.p[0], si, 0)));
   BatchSizer bs(check_every, BATCH_INNER, 0.0);
-  int next = bs.cap;
   hipGraphExec_t ge = nullptr;
   if (ctx->tune_graph && !ctx->prof) FCP_TRY((tk_graph<NR, NRP, T>(ctx, B, si, &ge)));
   int launched = 0, k = 0;
-  while (launched < budget) {
-    const int batch = (budget - launched) < next ? (budget - launched) : next;
+  auto run = [&](int batch) -> int {
     int j = 0;
     if (ge && (k & 1) == 0)  // the graph starts from an even iteration
       for (; j + 2 <= batch; j += 2, k += 2) FCP_HIP(ctx, hipGraphLaunch(ge, ctx->stream));
     for (; j < batch; ++j, ++k) FCP_TRY((tk_iteration<NR, NRP, T>(ctx, B, si, k)));
     launched += batch;
-    FCP_HIP(ctx, hipMemcpyAsync(h, si, NR * sizeof(SolveScalars), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    FCP_TRY(check_err(ctx));
+    return 0;
+  };
+  // Pipelined polls: the scalars after every batch are copied to a pinned
+  // slot, and the next batch is queued before the host waits for the
+  // previous slot, so the device does not idle while the host reads it.
+  // Batches queued after the device-side stopping test exit at their first
+  // instruction and leave the scalars unchanged.  A batch is queued ahead
+  // only while the convergence rate seen so far predicts that the work
+  // already queued does not reach the tolerance.
+  FCP_TRY(poll_slots(ctx));
+  auto post = [&](int slot) -> int {
+    FCP_HIP(ctx, hipMemcpyAsync(ctx->h_poll + 4 * slot, si, NR * sizeof(SolveScalars),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    FCP_HIP(ctx, hipMemcpyAsync(ctx->h_perr + slot, ctx->errflag.p, sizeof(int),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    FCP_HIP(ctx, hipEventRecord(ctx->poll_ev[slot], ctx->stream));
+    return 0;
+  };
+  const int cap = bs.cap;
+  FCP_TRY(run(budget < cap ? budget : cap));
+  FCP_TRY(post(0));
+  int slot = 0;     // the slot of the oldest poll not yet read
+  int ahead = cap;  // iterations predicted to be needed after those queued
+  const SolveScalars *hs = nullptr;
+  for (;;) {
+    int spec = 0;
+    if (ahead > 0 && launched < budget) {
+      spec = ahead < cap ? ahead : cap;
+      if (spec > budget - launched) spec = budget - launched;
+      FCP_TRY(run(spec));
+      FCP_TRY(post(1 - slot));
+    }
+    FCP_HIP(ctx, hipEventSynchronize(ctx->poll_ev[slot]));
+    if (ctx->h_perr[slot])
+      return fcp_fail(ctx, ctx->h_perr[slot], "sweep dependency wait exceeded its bound");
+    hs = ctx->h_poll + 4 * slot;
     bool all = true;
     for (int c = 0; c < NR; ++c) {
       if (verbose >= 2)
         printf("                   PCG%d[%d] iter = %4d resl = %10.3E rsm = %10.3E\n",
-               (int)(8 * sizeof(T)), c, h[c].iter, h[c].resl, h[c].rsm);
-      all = all && h[c].done;
+               (int)(8 * sizeof(T)), c, hs[c].iter, hs[c].resl, hs[c].rsm);
+      all = all && hs[c].done;
     }
     if (all) break;
-    next = bs.next(h, NR);
+    bs.next(hs, NR);
+    const int need = bs.last_need;  // iterations predicted past this poll
+    if (spec > 0) {
+      ahead = need - spec;  // the batch queued since is part of them
+      slot = 1 - slot;
+      continue;
+    }
+    if (launched >= budget) break;  // not converged within the budget
+    int b = need < cap ? need : cap;
+    if (b > budget - launched) b = budget - launched;
+    FCP_TRY(run(b));
+    FCP_TRY(post(slot));
+    ahead = need - b;
   }
+  // the scalars are final once the stopping test has fired (or the budget
+  // is spent); wait for any batch still queued before they are returned
+  FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int c = 0; c < NR; ++c) h[c] = hs[c];
   tk_ir_update<NR, NRP, T><<<g, BLK, 0, ctx->stream>>>(n, ctx->tin_iperm.p, B.x, so, x);
   FCP_HIP(ctx, hipGetLastError());
   return 0;
@@ -2309,10 +2370,7 @@ int tb_bicgstab(fcp_ctx *ctx, const double *res, double *x, const SolveScalars *
     }
     FCP_HIP(ctx, hipGetLastError());
     launched += batch;
-    FCP_HIP(ctx, hipMemcpyAsync(h, si, NR * sizeof(SolveScalars), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    FCP_TRY(check_err(ctx));
+    FCP_TRY(poll_device(ctx, h, si, NR * sizeof(SolveScalars)));
     bool all = true;
     for (int c = 0; c < NR; ++c) {
       if (verbose >= 2)

@@ -117,6 +117,7 @@ enum { BATCH_L1 = 0, BATCH_L2 = 1, BATCH_INNER = 2 };
 struct BatchSizer {
   static constexpr int MAXC = 8;
   int cap, mode;
+  int last_need = 0;  // the last next()'s estimate before the cap (cap while no rate is known)
   double tol;
   double rho[MAXC];
   int it[MAXC];
@@ -138,12 +139,13 @@ struct BatchSizer {
       if (it[c] >= 0 && S.iter > it[c] && ratio > 0.0 && ratio < rho[c] && target > 0.0) {
         const double q = log(ratio / rho[c]) / (double)(S.iter - it[c]);
         const double need = target < ratio ? log(target / ratio) / q : 0.0;
-        est = need < (double)cap ? (int)ceil(need) + 1 : cap;
+        est = need < 1e6 ? (int)ceil(need) + 1 : 1000000;
       }
       rho[c] = ratio;
       it[c] = S.iter;
       if (est > out) out = est;
     }
+    last_need = out;
     return out > cap ? cap : out;
   }
 };
@@ -275,6 +277,12 @@ struct fcp_ctx {
   DBuf<int> counters;        // [16] block-completion counters / tickets
   DBuf<SolveScalars> scal;   // device scalars
   SolveScalars *h_scal = nullptr;  // pinned mirror
+  int *h_err = nullptr;            // pinned mirror of errflag
+  // the tile-order PCG's pipelined polls: two pinned slots of 4 scalars and
+  // an error flag each, and the event that marks each slot's copy complete
+  SolveScalars *h_poll = nullptr;
+  int *h_perr = nullptr;
+  hipEvent_t poll_ev[2] = {nullptr, nullptr};
   DBuf<int> errflag;         // [1]
 
   // -------- restarted GMRES with ILU(0) --------

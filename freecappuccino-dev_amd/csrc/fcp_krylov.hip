@@ -524,8 +524,14 @@ __global__ void k_sell_val(const int *__restrict__ ia, const double *__restrict_
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nslices * 64) return;
   const int s = r >> 6, p = base[s] + (r & 63), L = (base[s + 1] - base[s]) >> 6;
-  const int len = r < n ? ia[r + 1] - ia[r] : 0;
-  for (int k = 0; k < L; ++k) out[p + 64 * k] = (k < len) ? (T)a[ia[r] + k] : (T)0;
+  const int r0 = r < n ? ia[r] : 0, len = r < n ? ia[r + 1] - r0 : 0;
+  T v[SELL_KU];  // the row's entries loaded together, then stored slot by slot
+#pragma unroll
+  for (int k = 0; k < SELL_KU; ++k) v[k] = k < len ? (T)a[r0 + k] : (T)0;
+#pragma unroll
+  for (int k = 0; k < SELL_KU; ++k)
+    if (k < L) out[p + 64 * k] = v[k];
+  for (int k = SELL_KU; k < L; ++k) out[p + 64 * k] = (k < len) ? (T)a[r0 + k] : (T)0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1011,6 +1017,7 @@ int fcp_ensure_workspace(fcp_ctx *ctx) {
   FCP_HIP(ctx, ctx->scal.alloc(4));
   FCP_HIP(ctx, ctx->errflag.alloc(1));
   if (!ctx->h_scal) FCP_HIP(ctx, hipHostMalloc((void **)&ctx->h_scal, 4 * sizeof(SolveScalars)));
+  if (!ctx->h_err) FCP_HIP(ctx, hipHostMalloc((void **)&ctx->h_err, sizeof(int)));
   return 0;
 }
 
@@ -1097,10 +1104,14 @@ int sell_view<float>(fcp_ctx *ctx, Sell<float> *out) {
 }  // namespace
 
 namespace fcpk {
-int check_err(fcp_ctx *ctx) {
-  int e = 0;
-  FCP_HIP(ctx, hipMemcpyAsync(&e, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+int check_err(fcp_ctx *ctx) { return poll_device(ctx, nullptr, nullptr, 0); }
+int poll_device(fcp_ctx *ctx, void *h, const void *d, size_t bytes) {
+  if (!ctx->h_err) FCP_HIP(ctx, hipHostMalloc((void **)&ctx->h_err, sizeof(int)));
+  if (bytes) FCP_HIP(ctx, hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  FCP_HIP(ctx, hipMemcpyAsync(ctx->h_err, ctx->errflag.p, sizeof(int), hipMemcpyDeviceToHost,
+                              ctx->stream));
   FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int e = *ctx->h_err;
   if (e) return fcp_fail(ctx, e, "sweep dependency wait exceeded its bound");
   return 0;
 }
@@ -1370,15 +1381,23 @@ int bj_prepare(fcp_ctx *ctx, int kind, bool *ok) {
     ctx->bj_kind = kind;
     ctx->bj_epoch++;
   }
+  *ok = true;
+  return 0;
+}
+
+// FP32 operands of the row-gather tile-local sweeps, packed with the tile
+// pivots on first use after the pivots change (the tile-order kernels read
+// their own layout and never need them).
+int bj_pack32_now(fcp_ctx *ctx) {
   for (BlockSchedule *b : {&ctx->jlower, &ctx->jupper})
     FCP_TRY(pack_block<float>(ctx, *b, ctx->dbj.p, ctx->bj_epoch | (1ull << 63)));
-  *ok = true;
   return 0;
 }
 
 // z = M^{-1} r with the tile-local sweeps (forward into y, backward into z);
 // no dependency crosses a block, so no sentinel fill is needed.
 int bj_apply32(fcp_ctx *ctx, const float *r, float *y, float *z, const SolveScalars *s) {
+  FCP_TRY(bj_pack32_now(ctx));
   k_reset_tickets<<<1, 1, 0, ctx->stream>>>(tickets(ctx));
   {
     ProfScope ps(ctx, "bj_sweep_fwd_f32");
@@ -1505,10 +1524,7 @@ int iterate_until_done(fcp_ctx *ctx, SolveScalars *s_dev, int slot, int itr_max,
     const int batch = (itr_max - launched) < next ? (itr_max - launched) : next;
     for (int k = 0; k < batch; ++k) FCP_TRY(one_iteration());
     launched += batch;
-    FCP_HIP(ctx, hipMemcpyAsync(&ctx->h_scal[slot], s_dev, sizeof(SolveScalars),
-                                hipMemcpyDeviceToHost, ctx->stream));
-    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    FCP_TRY(check_err(ctx));
+    FCP_TRY(poll_device(ctx, &ctx->h_scal[slot], s_dev, sizeof(SolveScalars)));
     if (verbose >= 2)
       printf("                   %s iter = %4d resl = %10.3E rsm = %10.3E\n", tag,
              ctx->h_scal[slot].iter, ctx->h_scal[slot].resl, ctx->h_scal[slot].rsm);
@@ -1786,6 +1802,7 @@ int prepare_precond(fcp_ctx *ctx, int kind, bool fp32, bool pivots) {
 int pack_block64(fcp_ctx *ctx, BlockSchedule &b) { return pack_block<double>(ctx, b); }
 int pack_block32(fcp_ctx *ctx, BlockSchedule &b) { return pack_block<float>(ctx, b); }
 int bj_begin(fcp_ctx *ctx, int kind, bool *ok) { return bj_prepare(ctx, kind, ok); }
+int bj_pack32(fcp_ctx *ctx) { return bj_pack32_now(ctx); }
 
 // z = M^{-1} r with the single-vector sweeps (current pivots).
 int precond_apply64(fcp_ctx *ctx, const double *r, double *y, double *z) {

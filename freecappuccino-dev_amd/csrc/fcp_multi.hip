@@ -918,6 +918,7 @@ int mprecond_apply(fcp_ctx *ctx, const T *r, T *y, T *z, T *tmp3, const SolveSca
   // its pivots, and no dependency crosses a block (no sentinel fill needed)
   const bool bj = sizeof(T) == 4 && ctx->bj_active;
   if (bj) {
+    FCP_TRY(bj_pack32(ctx));
     if ((long long)n * NRP * (long long)sizeof(T) < FCP_OOB &&
         sched_fits(ctx, ctx->jlower, ctx->jupper, NRP * sizeof(T))) {
       blo = &ctx->jlower;
@@ -1069,10 +1070,7 @@ int miterate(fcp_ctx *ctx, SolveScalars *s_dev, SolveScalars *h, int itr_max, in
     const int batch = (itr_max - launched) < next ? (itr_max - launched) : next;
     for (int k = 0; k < batch; ++k) FCP_TRY(one());
     launched += batch;
-    FCP_HIP(ctx, hipMemcpyAsync(h, s_dev, NR * sizeof(SolveScalars), hipMemcpyDeviceToHost,
-                                ctx->stream));
-    FCP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    FCP_TRY(check_err(ctx));
+    FCP_TRY(poll_device(ctx, h, s_dev, NR * sizeof(SolveScalars)));
     bool all = true;
     for (int c = 0; c < NR; ++c) {
       if (verbose >= 2)
