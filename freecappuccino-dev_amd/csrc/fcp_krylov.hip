@@ -1639,11 +1639,9 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
   FCP_HIP(ctx, hipMemsetAsync(ctx->counters.p, 0, CTR_INTS * sizeof(int), ctx->stream));
   FCP_HIP(ctx, hipMemsetAsync(ctx->errflag.p, 0, sizeof(int), ctx->stream));
   // FP32 operator and preconditioner (diagonal computed in FP64 then rounded)
+  // (the CSR-order FP32 copy of A is cast when the row-gather inner
+  // iterations first need it; the tile-order kernels have their own copy)
   FCP_HIP(ctx, ctx->a32.alloc(ctx->nnz));
-  if (!tile64 && ctx->a32_version != ctx->a_version) {
-    k_cast_a<<<red_grid(ctx, ctx->nnz), BLK, 0, ctx->stream>>>(ctx->nnz, ctx->a.p, ctx->a32.p);
-    ctx->a32_version = ctx->a_version;
-  }
   const int kind = solver == 2 ? 1 : 0;
   // global DIC / DILU pivots (FP64 and their FP32 rounding), computed when a
   // sweep needs them
@@ -1732,6 +1730,10 @@ int solve_mixed(fcp_ctx *ctx, int solver, double *x, const double *b, int itr_ma
       continue;
     }
     // inner FP32 solve of A e = r/|r|
+    if (ctx->a32_version != ctx->a_version) {
+      k_cast_a<<<red_grid(ctx, ctx->nnz), BLK, 0, ctx->stream>>>(ctx->nnz, ctx->a.p, ctx->a32.p);
+      ctx->a32_version = ctx->a_version;
+    }
     k_scale_to32<<<g, BLK, 0, ctx->stream>>>(n, ctx->res.p, so, ctx->r32.p, ctx->e32.p);
     k_zero_f32<<<g, BLK, 0, ctx->stream>>>(n, ctx->p32.p, solver == 2 ? ctx->u32.p : ctx->q32.p);
     k_init32<<<g, BLK, 0, ctx->stream>>>(n, ctx->r32.p, solver == 2 ? ctx->reso32.p : nullptr,
